@@ -1,0 +1,260 @@
+/*
+ * pxg.h -- C ABI of the MI355X rigid-body contact step.
+ *
+ * This is the drop-in boundary for physxcoopa's `PhysicsWorld::step_fixed()` hot path:
+ * broadphase pair finding, narrowphase (box/sphere/capsule), contact-graph colouring, the
+ * colour-by-colour sequential-impulse solver, integration, islands and sleep. Host code keeps
+ * the reference's body/shape/material model; everything here is plain C: opaque handles,
+ * plain structs, pointers and sizes. Every function returns an int status (PXG_OK == 0) and
+ * never throws; pxg_last_error() returns the message of the last failure on this thread.
+ *
+ * Conventions (same as the reference):
+ *   - Z-up, metres, seconds, kg. Gravity default (0, 0, -9.81).
+ *   - Quaternions are stored x, y, z, w (glm's storage order).
+ *   - Matrices are 3x3 column-major (glm::mat3 layout): m[c*3 + r].
+ *   - Body type order Static, Kinematic, Dynamic  (physxcoopa/dynamics/body.h:19-23).
+ *   - Shape type order Sphere, Box, Capsule        (physxcoopa/collision/shape.h:37-43).
+ *   - Combine mode order Average, Minimum, Maximum, Multiply
+ *                                                  (physxcoopa/dynamics/physics_material.h:28-33).
+ *   - Body index == position in the body table; shape slot == position in the shape table.
+ *
+ * One host thread per handle. Each world handle owns one HIP stream on the current device.
+ */
+#ifndef PXG_H
+#define PXG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXG_OK 0
+#define PXG_ERR_INVALID 1   /* bad argument or handle */
+#define PXG_ERR_CAPACITY 2  /* a fixed-capacity buffer would overflow */
+#define PXG_ERR_HIP 3       /* a HIP runtime call failed */
+#define PXG_ERR_STATE 4     /* call not valid in the current state */
+#define PXG_ERR_DEVICE 5    /* a kernel reported an internal error (bounded wait expired, ...) */
+
+enum pxg_body_type { PXG_STATIC = 0, PXG_KINEMATIC = 1, PXG_DYNAMIC = 2 };
+enum pxg_shape_type { PXG_SPHERE = 0, PXG_BOX = 1, PXG_CAPSULE = 2 };
+enum pxg_combine { PXG_AVERAGE = 0, PXG_MINIMUM = 1, PXG_MAXIMUM = 2, PXG_MULTIPLY = 3 };
+enum pxg_phase { PXG_PHASE_FORCES_COLLIDE = 0, PXG_PHASE_SOLVE_INTEGRATE = 1 };
+
+/* Body constraint bits (physxcoopa/dynamics/body.h:37-47). */
+#define PXG_FREEZE_POS_X (1u << 0)
+#define PXG_FREEZE_POS_Y (1u << 1)
+#define PXG_FREEZE_POS_Z (1u << 2)
+#define PXG_FREEZE_ROT_X (1u << 3)
+#define PXG_FREEZE_ROT_Y (1u << 4)
+#define PXG_FREEZE_ROT_Z (1u << 5)
+
+/*
+ * Solver / broadphase tunables. Replaces util::PhysicsConfig (physxcoopa/util/config.h:22-145)
+ * plus the world's gravity (world.h:2099) and the GPU capacities.
+ */
+typedef struct pxg_config {
+    uint32_t velocity_iterations;     /* 16 */
+    uint32_t relax_iterations;        /* 2  */
+    uint32_t position_iterations;     /* 4  */
+    float position_correction;        /* 0.2 */
+    float linear_slop;                /* 0.005 */
+    float max_linear_correction;      /* 0.2 */
+    float restitution_threshold;      /* 0.5 m/s */
+    float sleep_linear;               /* 0.01 m/s */
+    float sleep_angular;              /* 0.02 rad/s */
+    float sleep_time;                 /* 0.5 s */
+    float aabb_margin;                /* 0.1 m */
+    uint32_t max_substeps;            /* 8 */
+    float fixed_dt;                   /* 1/60 s */
+    float max_linear_velocity;        /* 200 m/s */
+    uint32_t use_exact_quaternion_integration; /* 0 */
+    float gravity[3];                 /* (0, 0, -9.81) */
+    /* Residual mode: when residual_mode != 0 the velocity pass repeats until the velocity
+     * residual (max over active contact rows, m/s) is <= residual_tolerance or
+     * residual_max_iterations passes have run; relax and position passes are unchanged. */
+    uint32_t residual_mode;           /* 0 */
+    float residual_tolerance;         /* 1e-4 */
+    uint32_t residual_max_iterations; /* 256 */
+    /* Broadphase: shapes whose fat AABB is wider than this in x or y are paired by a
+     * brute-force path instead of the x-y grid (a ground slab, say). */
+    float large_extent;               /* 16 m */
+    /* Capacities of the device buffers. 0 = derive from the first upload. */
+    uint32_t max_bodies;
+    uint32_t max_shapes;
+    uint32_t max_pairs;
+    uint32_t max_manifolds;
+    /* Learned warm start: 1 = predict initial impulses for contacts with no cached impulse. */
+    uint32_t learned_warm_start;      /* 0 */
+    uint32_t reserved[7];
+} pxg_config;
+
+/* Full dynamic state of one body (physxcoopa/dynamics/body.h:85-216). */
+typedef struct pxg_body {
+    uint32_t type;          /* pxg_body_type */
+    uint32_t alive;         /* 0 = free slot (ignored everywhere) */
+    uint32_t awake;
+    uint32_t use_gravity;
+    uint32_t constraints;   /* PXG_FREEZE_* bits */
+    uint32_t generation;    /* carried, not interpreted */
+    uint32_t reserved0;
+    uint32_t reserved1;
+    float position[3];
+    float sleep_timer;
+    float orientation[4];   /* x, y, z, w */
+    float linear_velocity[3];
+    float mass;
+    float angular_velocity[3];
+    float inv_mass;
+    float force[3];
+    float linear_drag;
+    float torque[3];
+    float angular_drag;
+    float inv_inertia_local[3];
+    float reserved2;
+    float inv_inertia_world[9]; /* column-major; recomputed by the library */
+    float prev_position[3];
+    float reserved3;
+    float prev_orientation[4];
+} pxg_body;
+
+/* One collision shape slot (physxcoopa/collision/shape.h:54-157) with its material
+ * (physxcoopa/dynamics/physics_material.h:40-58) and layer. */
+typedef struct pxg_shape {
+    uint32_t owner;          /* body index */
+    uint32_t type;           /* pxg_shape_type */
+    uint32_t enabled;        /* 0 = no broadphase proxy (body without collider) */
+    uint32_t is_trigger;
+    uint32_t layer;          /* 0..31 */
+    int32_t capsule_axis;    /* 0 = X, 1 = Y, 2 = Z */
+    uint32_t friction_combine;
+    uint32_t restitution_combine;
+    float local_center[3];
+    float radius;            /* sphere */
+    float local_rotation[4]; /* x, y, z, w */
+    float half_extents[3];   /* box */
+    float capsule_radius;
+    float capsule_half_height;
+    float dynamic_friction;  /* 0.6 default material */
+    float static_friction;   /* carried, unused by the solver (as in the reference) */
+    float restitution;       /* 0.0 default material */
+} pxg_shape;
+
+/* One contact point (physxcoopa/collision/manifold.h:27-52). */
+typedef struct pxg_contact {
+    float position[3];
+    float penetration;
+    float separation;
+    float normal_impulse;
+    float tangent_impulse[2];
+    uint32_t feature_id;
+    float restitution_bias;
+    float reserved[2];
+} pxg_contact;
+
+/* One contact manifold (physxcoopa/collision/manifold.h:64-97). */
+typedef struct pxg_manifold {
+    uint32_t a, b;            /* body indices; normal points a -> b */
+    uint32_t shape_a, shape_b;/* shape slots (the broadphase pair) */
+    float normal[3];
+    float friction;
+    float restitution;
+    uint32_t count;
+    uint32_t is_trigger;
+    uint32_t valid;
+    pxg_contact points[4];
+} pxg_manifold;
+
+/* Per-step counters. */
+typedef struct pxg_stats {
+    uint32_t pairs;              /* broadphase pairs */
+    uint32_t manifolds;          /* valid manifolds, triggers included */
+    uint32_t points;             /* contact points in valid manifolds */
+    uint32_t colours;            /* colours used by the solver this step */
+    uint32_t active_manifolds;   /* manifolds the solver ran (non-trigger, an awake dynamic side) */
+    uint32_t velocity_passes;    /* velocity passes run (== velocity_iterations unless residual mode) */
+    uint32_t awake_bodies;
+    uint32_t learned_predictions;/* warm-start misses filled by the learned predictor */
+    float residual;              /* velocity residual after the last velocity pass (m/s) */
+    float reserved0;
+    uint64_t steps;              /* fixed steps run since creation */
+} pxg_stats;
+
+typedef struct pxg_world pxg_world;
+
+/* --- configuration and lifetime --- */
+void pxg_config_default(pxg_config* cfg);
+int pxg_create(const pxg_config* cfg, pxg_world** out);          /* PhysicsWorld(config), world.h:81 */
+int pxg_destroy(pxg_world* w);
+int pxg_get_config(pxg_world* w, pxg_config* out);                /* config(), world.h:1207 */
+int pxg_set_config(pxg_world* w, const pxg_config* cfg);          /* capacities must not change */
+int pxg_set_gravity(pxg_world* w, const float g[3]);              /* set_gravity(), world.h:1144 */
+
+/* --- state upload / download (caller-owned host buffers are copied) --- */
+/* Replaces the body table (add_body/remove_body, world.h:95-143). World inverse inertia is
+ * recomputed from orientation exactly as add_body() does. */
+int pxg_set_bodies(pxg_world* w, const pxg_body* bodies, uint32_t n);
+int pxg_get_bodies(pxg_world* w, pxg_body* out, uint32_t n);
+/* Replaces the shape table and creates each enabled shape's broadphase proxy with
+ * fat = tight.expand(margin) at its owner's current pose (create_shape_proxy_, world.h:1369-1382). */
+int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n);
+/* 32 x 32 layer matrix rows (broadphase/layer_matrix.h:21-47). */
+int pxg_set_layer_masks(pxg_world* w, const uint32_t masks[32]);
+
+/* --- stepping --- */
+int pxg_step_fixed(pxg_world* w, float h);                        /* step_fixed(), world.h:1278 */
+/* Split step for the on_substep seam (world.h:1299): FORCES_COLLIDE runs gravity, force
+ * integration, broadphase and narrowphase; SOLVE_INTEGRATE runs the solve, integration and sleep.
+ * Bodies may be re-uploaded between the two. */
+int pxg_step_phase(pxg_world* w, uint32_t phase, float h);
+int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run);     /* step(), world.h:1229 */
+int pxg_synchronize(pxg_world* w);
+
+/* --- results --- */
+int pxg_get_stats(pxg_world* w, pxg_stats* out);
+/* Broadphase pairs of the last step (shape slots, a < b), in discovery order. */
+int pxg_get_pairs(pxg_world* w, uint32_t* ab, uint32_t cap, uint32_t* n);
+/* Valid manifolds of the last step (manifolds(), world.h:731) in discovery order, with the
+ * solver's final impulses. */
+int pxg_get_manifolds(pxg_world* w, pxg_manifold* out, uint32_t cap, uint32_t* n);
+/* The solver's order of the last step: manifold indices (into pxg_get_manifolds order), colour
+ * by colour, followed by the non-trigger manifolds the solver skipped (no awake dynamic side).
+ * colour_offsets[c] .. colour_offsets[c+1] delimit colour c; n_colours colours. */
+int pxg_get_solve_order(pxg_world* w, uint32_t* order, uint32_t cap, uint32_t* n,
+                        uint32_t* colour_offsets, uint32_t colour_cap, uint32_t* n_colours);
+/* Per-shape fat AABBs (min xyz, max xyz) -- persistent broadphase state. */
+int pxg_get_fat_aabbs(pxg_world* w, float* out6, uint32_t n);
+int pxg_set_fat_aabbs(pxg_world* w, const float* in6, uint32_t n);
+/* Warm-start cache (dynamics/solver.h:63-69): keys and (normal, tangent1, tangent2). */
+int pxg_get_warm_cache(pxg_world* w, uint64_t* keys, float* impulses3, uint32_t cap, uint32_t* n);
+int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const float* impulses3, uint32_t n);
+/* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
+int pxg_state_hash(pxg_world* w, uint64_t* out);
+
+/* Learned warm start: BF16 weights of the 24-64-64-3 MLP (row-major W[out][in]) and FP32
+ * biases plus the 24 input scales. No counterpart in the reference (new hook at its warm-start
+ * miss branch, dynamics/solver.h:166-170). */
+int pxg_set_mlp(pxg_world* w, const uint16_t* w1, const float* b1, const uint16_t* w2,
+                const float* b2, const uint16_t* w3, const float* b3, const float* in_scale);
+
+/* Profiling of the dominant kernel, the velocity pass: with profiling on, every velocity-pass
+ * launch is bracketed by HIP events on the world's stream and counts its algorithmic bytes
+ * (192 per manifold + 40 per contact point). pxg_get_profile returns and resets the totals. No
+ * counterpart in the reference (it has no per-stage timers). */
+typedef struct pxg_profile {
+    double velocity_ms;          /* summed event time of velocity-pass launches */
+    double velocity_bytes;       /* summed algorithmic bytes of those launches */
+    uint64_t velocity_launches;
+    uint64_t reserved[5];
+} pxg_profile;
+int pxg_set_profiling(pxg_world* w, uint32_t on);
+int pxg_get_profile(pxg_world* w, pxg_profile* out);
+
+const char* pxg_last_error(void);
+const char* pxg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PXG_H */

@@ -1,0 +1,239 @@
+"""ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke() and bench.py's cpu_baseline).
+
+ctypes wrapper over oracle/_build/libpxo.so, the CPU restatement of physxcoopa's rigid-body
+step (see oracle/pxo_step.hpp). It takes the same ABI arrays as the GPU library.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from physxcoopa_amd._abi import BODY_DTYPE, MANIFOLD_DTYPE, ptr
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libpxo.so")
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, u32, f32, i32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int, ctypes.c_uint64
+        sig = {
+            "pxo_create": ([vp, i32], vp), "pxo_destroy": ([vp], None), "pxo_set_config": ([vp, vp], None),
+            "pxo_set_bodies": ([vp, vp, u32], None), "pxo_get_bodies": ([vp, vp, u32], None),
+            "pxo_set_shapes": ([vp, vp, u32], None), "pxo_set_layer_masks": ([vp, vp], None),
+            "pxo_set_brute_force": ([vp, u32], None), "pxo_set_order": ([vp, vp, u32], None),
+            "pxo_set_miss_table": ([vp, vp, vp, u32], None), "pxo_step_fixed": ([vp, f32], None),
+            "pxo_step_phase": ([vp, u32, f32], None), "pxo_run": ([vp, u32, f32], ctypes.c_double),
+            "pxo_get_pairs": ([vp, vp, u32], u32), "pxo_get_manifolds": ([vp, vp, u32], u32),
+            "pxo_get_fat": ([vp, vp, u32], None), "pxo_set_fat": ([vp, vp, u32], None),
+            "pxo_get_cache": ([vp, vp, vp, u32], u32), "pxo_set_cache": ([vp, vp, vp, u32], None),
+            "pxo_state_hash": ([vp], u64), "pxo_stats": ([vp, vp], None),
+            "pxo_residual_trace": ([vp, vp, u32], u32),
+            "pxo_generate_contacts": ([vp, vp, vp, vp, vp, vp, i32, vp], i32),
+            "pxo_box_box": ([vp, vp, vp, vp, vp, vp, i32, vp], i32),
+            "pxo_sat": ([vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp], i32),
+            "pxo_seg_seg": ([vp] * 6, None), "pxo_world_bounds": ([vp, vp, vp, vp], None),
+            "pxo_combine": ([f32, f32, i32, i32], f32), "pxo_orthonormal_basis": ([vp, vp, vp], None),
+            "pxo_quat_rotate": ([vp, vp, vp], None), "pxo_mat3_cast": ([vp, vp], None),
+            "pxo_pack_key": ([u32, u32, u32], u64),
+        }
+        for name, (args, res) in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def f32(x, n=None):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    if n is not None:
+        assert a.size == n
+    return a
+
+
+class OracleWorld:
+    """The CPU reference step over pxg ABI arrays. dbl=True runs the same algorithm in float64."""
+
+    def __init__(self, config, bodies, shapes, dbl=False, brute_force=False, layer_masks=None):
+        L = lib()
+        self._cfg = config
+        self.h = L.pxo_create(ctypes.byref(config), 1 if dbl else 0)
+        self.n_bodies = len(bodies)
+        self.n_shapes = len(shapes)
+        b = np.ascontiguousarray(bodies)
+        s = np.ascontiguousarray(shapes)
+        L.pxo_set_bodies(self.h, ptr(b), self.n_bodies)
+        if layer_masks is not None:
+            m = np.ascontiguousarray(np.asarray(layer_masks, np.uint32))
+            L.pxo_set_layer_masks(self.h, ptr(m))
+        L.pxo_set_shapes(self.h, ptr(s), self.n_shapes)
+        if brute_force:
+            L.pxo_set_brute_force(self.h, 1)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().pxo_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_config(self, config):
+        self._cfg = config
+        lib().pxo_set_config(self.h, ctypes.byref(config))
+
+    def step_fixed(self, h=None):
+        lib().pxo_step_fixed(self.h, np.float32(self._cfg.fixed_dt if h is None else h))
+
+    def step_phase(self, phase, h=None):
+        lib().pxo_step_phase(self.h, phase, np.float32(self._cfg.fixed_dt if h is None else h))
+
+    def run(self, steps, h=None):
+        return lib().pxo_run(self.h, steps, np.float32(self._cfg.fixed_dt if h is None else h))
+
+    def set_order(self, pairs_ab):
+        a = np.ascontiguousarray(np.asarray(pairs_ab, np.uint32).reshape(-1, 2))
+        lib().pxo_set_order(self.h, ptr(a), len(a))
+
+    def set_miss_table(self, keys, impulses):
+        k = np.ascontiguousarray(np.asarray(keys, np.uint64))
+        i = np.ascontiguousarray(np.asarray(impulses, np.float32).reshape(-1, 3))
+        lib().pxo_set_miss_table(self.h, ptr(k), ptr(i), len(k))
+
+    def bodies(self):
+        out = np.zeros(self.n_bodies, BODY_DTYPE)
+        lib().pxo_get_bodies(self.h, ptr(out), self.n_bodies)
+        return out
+
+    def set_bodies(self, bodies):
+        b = np.ascontiguousarray(bodies)
+        lib().pxo_set_bodies(self.h, ptr(b), len(b))
+
+    def pairs(self):
+        n = lib().pxo_get_pairs(self.h, None, 0)
+        out = np.zeros((max(n, 1), 2), np.uint32)
+        lib().pxo_get_pairs(self.h, ptr(out), n)
+        return out[:n]
+
+    def manifolds(self):
+        n = lib().pxo_get_manifolds(self.h, None, 0)
+        out = np.zeros(max(n, 1), MANIFOLD_DTYPE)
+        lib().pxo_get_manifolds(self.h, ptr(out), n)
+        return out[:n]
+
+    def fat_aabbs(self):
+        out = np.zeros((self.n_shapes, 6), np.float32)
+        lib().pxo_get_fat(self.h, ptr(out), self.n_shapes)
+        return out
+
+    def set_fat_aabbs(self, boxes):
+        b = np.ascontiguousarray(np.asarray(boxes, np.float32).reshape(-1, 6))
+        lib().pxo_set_fat(self.h, ptr(b), len(b))
+
+    def warm_cache(self):
+        n = lib().pxo_get_cache(self.h, None, None, 0)
+        keys = np.zeros(max(n, 1), np.uint64)
+        imp = np.zeros((max(n, 1), 3), np.float32)
+        lib().pxo_get_cache(self.h, ptr(keys), ptr(imp), n)
+        return keys[:n], imp[:n]
+
+    def set_warm_cache(self, keys, imp):
+        k = np.ascontiguousarray(np.asarray(keys, np.uint64))
+        i = np.ascontiguousarray(np.asarray(imp, np.float32).reshape(-1, 3))
+        lib().pxo_set_cache(self.h, ptr(k), ptr(i), len(k))
+
+    def state_hash(self):
+        return int(lib().pxo_state_hash(self.h))
+
+    def stats(self):
+        out = np.zeros(4, np.float64)
+        lib().pxo_stats(self.h, ptr(out))
+        return dict(velocity_passes=int(out[0]), residual=float(out[1]), predicted=int(out[2]), steps=int(out[3]))
+
+    def residual_trace(self):
+        n = lib().pxo_residual_trace(self.h, None, 0)
+        out = np.zeros(max(n, 1), np.float64)
+        lib().pxo_residual_trace(self.h, ptr(out), n)
+        return out[:n]
+
+
+# ---- single-function helpers for known-answer tests ----
+
+def generate_contacts(shape_a, pos_a, rot_a, shape_b, pos_b, rot_b, speculative=False):
+    out = np.zeros(1, MANIFOLD_DTYPE)
+    sa, sb = np.ascontiguousarray(shape_a).reshape(1), np.ascontiguousarray(shape_b).reshape(1)
+    keep = [f32(pos_a, 3), f32(rot_a, 4), f32(pos_b, 3), f32(rot_b, 4)]
+    hit = lib().pxo_generate_contacts(ptr(sa), ptr(keep[0]), ptr(keep[1]), ptr(sb), ptr(keep[2]), ptr(keep[3]),
+                                      1 if speculative else 0, ptr(out))
+    return bool(hit), out[0]
+
+
+def box_box(ca, hea, qa, cb, heb, qb, speculative=False):
+    out = np.zeros(1, MANIFOLD_DTYPE)
+    k = [f32(ca, 3), f32(hea, 3), f32(qa, 4), f32(cb, 3), f32(heb, 3), f32(qb, 4)]
+    hit = lib().pxo_box_box(*[ptr(a) for a in k], 1 if speculative else 0, ptr(out))
+    return bool(hit), out[0]
+
+
+def sat(ca, hea, qa, cb, heb, qb, speculative=False):
+    axis, sep = ctypes.c_int(), ctypes.c_int()
+    depth = np.zeros(1, np.float32)
+    normal = np.zeros(3, np.float32)
+    k = [f32(ca, 3), f32(hea, 3), f32(qa, 4), f32(cb, 3), f32(heb, 3), f32(qb, 4)]
+    hit = lib().pxo_sat(*[ptr(a) for a in k], 1 if speculative else 0, ctypes.byref(axis), ptr(depth), ptr(normal),
+                        ctypes.byref(sep))
+    return bool(hit), axis.value, float(depth[0]), normal, bool(sep.value)
+
+
+def seg_seg(p1, q1, p2, q2):
+    c1, c2 = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    k = [f32(p1, 3), f32(q1, 3), f32(p2, 3), f32(q2, 3)]
+    lib().pxo_seg_seg(*[ptr(a) for a in k], ptr(c1), ptr(c2))
+    return c1, c2
+
+
+def world_bounds(shape, pos, rot):
+    out = np.zeros(6, np.float32)
+    s = np.ascontiguousarray(shape).reshape(1)
+    k = [f32(pos, 3), f32(rot, 4)]
+    lib().pxo_world_bounds(ptr(s), ptr(k[0]), ptr(k[1]), ptr(out))
+    return out
+
+
+def combine(a, b, ma, mb):
+    return float(lib().pxo_combine(np.float32(a), np.float32(b), ma, mb))
+
+
+def orthonormal_basis(n):
+    t1, t2 = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    k = f32(n, 3)
+    lib().pxo_orthonormal_basis(ptr(k), ptr(t1), ptr(t2))
+    return t1, t2
+
+
+def quat_rotate(q, v):
+    out = np.zeros(3, np.float32)
+    k = [f32(q, 4), f32(v, 3)]
+    lib().pxo_quat_rotate(ptr(k[0]), ptr(k[1]), ptr(out))
+    return out
+
+
+def mat3_cast(q):
+    out = np.zeros(9, np.float32)
+    k = f32(q, 4)
+    lib().pxo_mat3_cast(ptr(k), ptr(out))
+    return out.reshape(3, 3)  # [col][row]
+
+
+def pack_key(a, b, fid):
+    return int(lib().pxo_pack_key(a, b, fid))

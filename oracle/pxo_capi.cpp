@@ -1,0 +1,303 @@
+// ORACLE -- test infrastructure only. Nothing in the product may include, link or run this.
+//
+// extern "C" surface of the CPU oracle, loaded by tests/ (ctypes) and by bench.py's CPU
+// baseline leg. Mirrors the pxg_* C ABI so a test can drive both sides with the same buffers.
+#include "pxo_step.hpp"
+
+#include <chrono>
+#include <map>
+
+using namespace pxo;
+
+namespace {
+
+template <class T> void cfg_from(Config<T>& c, const pxg_config* in) {
+    c.vel_iters = in->velocity_iterations; c.relax_iters = in->relax_iterations; c.pos_iters = in->position_iterations;
+    c.pos_corr = T(in->position_correction); c.slop = T(in->linear_slop); c.max_corr = T(in->max_linear_correction);
+    c.rest_thresh = T(in->restitution_threshold); c.sleep_lin = T(in->sleep_linear); c.sleep_ang = T(in->sleep_angular);
+    c.sleep_time = T(in->sleep_time); c.margin = T(in->aabb_margin); c.max_substeps = in->max_substeps;
+    c.fixed_dt = T(in->fixed_dt); c.max_lin_vel = T(in->max_linear_velocity); c.exact_quat = in->use_exact_quaternion_integration != 0;
+    c.gravity = V3<T>(T(in->gravity[0]), T(in->gravity[1]), T(in->gravity[2]));
+    c.residual_mode = in->residual_mode != 0; c.residual_tol = T(in->residual_tolerance); c.residual_max = in->residual_max_iterations;
+}
+template <class T> V3<T> v3(const float* f) { return V3<T>(T(f[0]), T(f[1]), T(f[2])); }
+template <class T> Q<T> q4(const float* f) { Q<T> q; q.x = T(f[0]); q.y = T(f[1]); q.z = T(f[2]); q.w = T(f[3]); return q; }
+template <class T> void put3(float* f, const V3<T>& v) { f[0] = float(v.x); f[1] = float(v.y); f[2] = float(v.z); }
+template <class T> void put4(float* f, const Q<T>& q) { f[0] = float(q.x); f[1] = float(q.y); f[2] = float(q.z); f[3] = float(q.w); }
+
+template <class T> Body<T> body_from(const pxg_body& d) {
+    Body<T> b;
+    b.type = int(d.type); b.alive = d.alive != 0; b.awake = d.awake != 0; b.use_gravity = d.use_gravity != 0;
+    b.constraints = d.constraints; b.generation = d.generation;
+    b.pos = v3<T>(d.position); b.sleep_timer = T(d.sleep_timer); b.q = q4<T>(d.orientation);
+    b.v = v3<T>(d.linear_velocity); b.mass = T(d.mass); b.w = v3<T>(d.angular_velocity); b.inv_mass = T(d.inv_mass);
+    b.force = v3<T>(d.force); b.ldrag = T(d.linear_drag); b.torque = v3<T>(d.torque); b.adrag = T(d.angular_drag);
+    b.iil = v3<T>(d.inv_inertia_local);
+    b.prev_pos = v3<T>(d.prev_position); b.prev_q = q4<T>(d.prev_orientation);
+    update_world_inertia(b);  // add_body(), world.h:113
+    return b;
+}
+template <class T> void body_to(const Body<T>& b, pxg_body& d) {
+    std::memset(&d, 0, sizeof(d));
+    d.type = uint32_t(b.type); d.alive = b.alive; d.awake = b.awake; d.use_gravity = b.use_gravity;
+    d.constraints = b.constraints; d.generation = b.generation;
+    put3(d.position, b.pos); d.sleep_timer = float(b.sleep_timer); put4(d.orientation, b.q);
+    put3(d.linear_velocity, b.v); d.mass = float(b.mass); put3(d.angular_velocity, b.w); d.inv_mass = float(b.inv_mass);
+    put3(d.force, b.force); d.linear_drag = float(b.ldrag); put3(d.torque, b.torque); d.angular_drag = float(b.adrag);
+    put3(d.inv_inertia_local, b.iil);
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) d.inv_inertia_world[c * 3 + r] = float(b.iiw.c[c][r]);
+    put3(d.prev_position, b.prev_pos); put4(d.prev_orientation, b.prev_q);
+}
+template <class T> Shape<T> shape_from(const pxg_shape& d) {
+    Shape<T> s;
+    s.owner = d.owner; s.type = int(d.type); s.enabled = d.enabled != 0; s.trigger = d.is_trigger != 0;
+    s.layer = d.layer; s.caxis = d.capsule_axis; s.fcomb = int(d.friction_combine); s.rcomb = int(d.restitution_combine);
+    s.lc = v3<T>(d.local_center); s.radius = T(d.radius); s.lr = q4<T>(d.local_rotation); s.he = v3<T>(d.half_extents);
+    s.cr = T(d.capsule_radius); s.chh = T(d.capsule_half_height); s.friction = T(d.dynamic_friction); s.restitution = T(d.restitution);
+    return s;
+}
+template <class T> void manifold_to(const Manifold<T>& m, pxg_manifold& d) {
+    std::memset(&d, 0, sizeof(d));
+    d.a = m.a; d.b = m.b; d.shape_a = m.sa; d.shape_b = m.sb;
+    put3(d.normal, m.n); d.friction = float(m.friction); d.restitution = float(m.restitution);
+    d.count = uint32_t(m.count); d.is_trigger = m.trigger; d.valid = m.valid;
+    for (int i = 0; i < 4; ++i) {
+        const Contact<T>& c = m.p[i];
+        pxg_contact& o = d.points[i];
+        put3(o.position, c.pos); o.penetration = float(c.pen); o.separation = float(c.sep);
+        o.normal_impulse = float(c.ni); o.tangent_impulse[0] = float(c.ti[0]); o.tangent_impulse[1] = float(c.ti[1]);
+        o.feature_id = c.fid; o.restitution_bias = float(c.rbias);
+    }
+}
+
+struct Handle {
+    int dbl = 0;
+    World<float> f;
+    World<double> d;
+};
+
+template <class T> World<T>& W(Handle* h);
+template <> World<float>& W<float>(Handle* h) { return h->f; }
+template <> World<double>& W<double>(Handle* h) { return h->d; }
+
+#define DISPATCH(h, call) ((h)->dbl ? call<double>(W<double>(h)) : call<float>(W<float>(h)))
+
+}  // namespace
+
+extern "C" {
+
+void* pxo_create(const pxg_config* cfg, int dbl) {
+    Handle* h = new Handle();
+    h->dbl = dbl;
+    cfg_from(h->f.cfg, cfg);
+    cfg_from(h->d.cfg, cfg);
+    return h;
+}
+void pxo_destroy(void* p) { delete static_cast<Handle*>(p); }
+
+void pxo_set_config(void* p, const pxg_config* cfg) {
+    Handle* h = static_cast<Handle*>(p);
+    cfg_from(h->f.cfg, cfg);
+    cfg_from(h->d.cfg, cfg);
+}
+
+void pxo_set_bodies(void* p, const pxg_body* in, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        w.bodies.resize(n);
+        for (uint32_t i = 0; i < n; ++i) w.bodies[i] = body_from<T>(in[i]);
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+void pxo_get_bodies(void* p, pxg_body* out, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) { for (uint32_t i = 0; i < n && i < w.bodies.size(); ++i) body_to(w.bodies[i], out[i]); };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+void pxo_set_shapes(void* p, const pxg_shape* in, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        w.shapes.resize(n);
+        for (uint32_t i = 0; i < n; ++i) w.shapes[i] = shape_from<T>(in[i]);
+        w.create_proxies();
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+void pxo_set_layer_masks(void* p, const uint32_t* m) {
+    Handle* h = static_cast<Handle*>(p);
+    for (int i = 0; i < 32; ++i) { h->f.layer_mask[i] = m[i]; h->d.layer_mask[i] = m[i]; }
+}
+void pxo_set_brute_force(void* p, uint32_t on) {
+    Handle* h = static_cast<Handle*>(p);
+    h->f.brute_force = on; h->d.brute_force = on;
+}
+// Solve order for the following steps: n shape pairs (a, b).  n == 0 clears it (pair order).
+void pxo_set_order(void* p, const uint32_t* ab, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        w.order_hint.clear();
+        for (uint32_t i = 0; i < n; ++i) w.order_hint.push_back({ab[2 * i], ab[2 * i + 1]});
+        w.use_order_hint = n > 0;
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+void pxo_set_miss_table(void* p, const uint64_t* keys, const float* imp, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        w.miss_table.clear();
+        for (uint32_t i = 0; i < n; ++i) { Cached c; c.n = imp[3 * i]; c.t1 = imp[3 * i + 1]; c.t2 = imp[3 * i + 2]; w.miss_table[keys[i]] = c; }
+        w.use_miss_table = n > 0;
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+void pxo_step_fixed(void* p, float hstep) {
+    Handle* h = static_cast<Handle*>(p);
+    if (h->dbl) h->d.step_fixed(double(hstep)); else h->f.step_fixed(hstep);
+}
+void pxo_step_phase(void* p, uint32_t phase, float hstep) {
+    Handle* h = static_cast<Handle*>(p);
+    if (h->dbl) { if (phase == 0) h->d.step_phase_collide(double(hstep)); else h->d.step_phase_solve(double(hstep)); }
+    else { if (phase == 0) h->f.step_phase_collide(hstep); else h->f.step_phase_solve(hstep); }
+}
+// Runs `steps` fixed steps and returns the wall time of the loop in seconds (steady_clock).
+double pxo_run(void* p, uint32_t steps, float hstep) {
+    Handle* h = static_cast<Handle*>(p);
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0; i < steps; ++i) pxo_step_fixed(p, hstep);
+    auto t1 = std::chrono::steady_clock::now();
+    (void)h;
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+uint32_t pxo_get_pairs(void* p, uint32_t* ab, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) -> uint32_t {
+        uint32_t n = uint32_t(w.pairs.size());
+        for (uint32_t i = 0; i < n && i < cap; ++i) { ab[2 * i] = w.pairs[i].first; ab[2 * i + 1] = w.pairs[i].second; }
+        return n;
+    };
+    return h->dbl ? go(h->d) : go(h->f);
+}
+uint32_t pxo_get_manifolds(void* p, pxg_manifold* out, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) -> uint32_t {
+        uint32_t n = uint32_t(w.manifolds.size());
+        for (uint32_t i = 0; i < n && i < cap; ++i) manifold_to(w.manifolds[i], out[i]);
+        return n;
+    };
+    return h->dbl ? go(h->d) : go(h->f);
+}
+void pxo_get_fat(void* p, float* out6, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        for (uint32_t i = 0; i < n && i < w.shapes.size(); ++i) {
+            if (w.proxy[i] < 0) { for (int k = 0; k < 6; ++k) out6[6 * i + k] = 0; continue; }
+            auto b = w.fat(i);
+            put3(out6 + 6 * i, b.mn); put3(out6 + 6 * i + 3, b.mx);
+        }
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+void pxo_set_fat(void* p, const float* in6, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        std::vector<AABB<T>> boxes(w.shapes.size());
+        for (uint32_t i = 0; i < n && i < w.shapes.size(); ++i) { boxes[i].mn = v3<T>(in6 + 6 * i); boxes[i].mx = v3<T>(in6 + 6 * i + 3); }
+        w.set_fat(boxes);
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+// Cache entries sorted by key.
+uint32_t pxo_get_cache(void* p, uint64_t* keys, float* imp, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) -> uint32_t {
+        std::map<uint64_t, Cached> sorted(w.cache.begin(), w.cache.end());
+        uint32_t i = 0;
+        for (auto& kv : sorted) {
+            if (i < cap) { keys[i] = kv.first; imp[3 * i] = float(kv.second.n); imp[3 * i + 1] = float(kv.second.t1); imp[3 * i + 2] = float(kv.second.t2); }
+            ++i;
+        }
+        return uint32_t(sorted.size());
+    };
+    return h->dbl ? go(h->d) : go(h->f);
+}
+void pxo_set_cache(void* p, const uint64_t* keys, const float* imp, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        w.cache.clear();
+        for (uint32_t i = 0; i < n; ++i) { Cached c; c.n = imp[3 * i]; c.t1 = imp[3 * i + 1]; c.t2 = imp[3 * i + 2]; w.cache[keys[i]] = c; }
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+uint64_t pxo_state_hash(void* p) {
+    Handle* h = static_cast<Handle*>(p);
+    return h->dbl ? h->d.state_hash() : h->f.state_hash();
+}
+// out: [vel_passes, residual, predicted, steps]
+void pxo_stats(void* p, double* out) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) { out[0] = w.vel_passes; out[1] = double(w.last_residual); out[2] = w.last_predicted; out[3] = double(w.steps); };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+uint32_t pxo_residual_trace(void* p, double* out, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) -> uint32_t {
+        for (uint32_t i = 0; i < w.residual_trace.size() && i < cap; ++i) out[i] = double(w.residual_trace[i]);
+        return uint32_t(w.residual_trace.size());
+    };
+    return h->dbl ? go(h->d) : go(h->f);
+}
+
+// ---- single-function entry points for known-answer tests (FP32) ----
+int pxo_generate_contacts(const pxg_shape* sa, const float* pa, const float* ra, const pxg_shape* sb, const float* pb,
+                          const float* rb, int spec, pxg_manifold* out) {
+    Manifold<float> m;
+    bool hit = generate_contacts(0u, shape_from<float>(*sa), v3<float>(pa), q4<float>(ra), 1u, shape_from<float>(*sb),
+                                 v3<float>(pb), q4<float>(rb), m, spec != 0);
+    manifold_to(m, *out);
+    return hit ? 1 : 0;
+}
+int pxo_box_box(const float* ca, const float* hea, const float* qa, const float* cb, const float* heb, const float* qb, int spec,
+                pxg_manifold* out) {
+    OBB<float> A{v3<float>(ca), v3<float>(hea), q4<float>(qa)};
+    OBB<float> B{v3<float>(cb), v3<float>(heb), q4<float>(qb)};
+    Manifold<float> m;
+    bool hit = box_box(A, B, m, spec != 0);
+    manifold_to(m, *out);
+    return hit ? 1 : 0;
+}
+// SAT only: returns hit, writes axis, penetration/separation, normal
+int pxo_sat(const float* ca, const float* hea, const float* qa, const float* cb, const float* heb, const float* qb, int spec,
+            int* axis, float* depth, float* normal, int* separated) {
+    OBB<float> A{v3<float>(ca), v3<float>(hea), q4<float>(qa)};
+    OBB<float> B{v3<float>(cb), v3<float>(heb), q4<float>(qb)};
+    SatResult<float> r;
+    bool hit = sat_obb_obb(A, B, r, spec != 0);
+    *axis = r.axis; *depth = r.separated ? r.sep : r.pen; put3(normal, r.n); *separated = r.separated;
+    return hit ? 1 : 0;
+}
+void pxo_seg_seg(const float* p1, const float* q1, const float* p2, const float* q2, float* c1, float* c2) {
+    V3<float> a, b;
+    closest_points_seg_seg(v3<float>(p1), v3<float>(q1), v3<float>(p2), v3<float>(q2), a, b);
+    put3(c1, a); put3(c2, b);
+}
+void pxo_world_bounds(const pxg_shape* s, const float* pos, const float* rot, float* out6) {
+    AABB<float> b = world_bounds(shape_from<float>(*s), v3<float>(pos), q4<float>(rot));
+    put3(out6, b.mn); put3(out6 + 3, b.mx);
+}
+float pxo_combine(float a, float b, int ma, int mb) { return combine(a, b, ma, mb); }
+void pxo_orthonormal_basis(const float* n, float* t1, float* t2) {
+    V3<float> a, b;
+    orthonormal_basis(v3<float>(n), a, b);
+    put3(t1, a); put3(t2, b);
+}
+void pxo_quat_rotate(const float* q, const float* v, float* out) { put3(out, qrot(q4<float>(q), v3<float>(v))); }
+void pxo_mat3_cast(const float* q, float* out9) {
+    M3<float> m = mat3_cast(q4<float>(q));
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) out9[c * 3 + r] = m.c[c][r];
+}
+uint64_t pxo_pack_key(uint32_t a, uint32_t b, uint32_t fid) { return pack_key(a, b, fid); }
+
+}  // extern "C"

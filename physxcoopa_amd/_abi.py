@@ -1,0 +1,128 @@
+"""Data layouts of the pxg C ABI (include/pxg.h) as numpy structured dtypes and ctypes structs.
+
+The same layouts are used to feed the GPU library and the CPU oracle, so one scene buffer can
+drive both. Field order and sizes must match include/pxg.h exactly; tests/test_abi.py checks the
+sizes against the compiled library.
+"""
+import ctypes
+
+import numpy as np
+
+PXG_OK = 0
+PXG_STATIC, PXG_KINEMATIC, PXG_DYNAMIC = 0, 1, 2
+PXG_SPHERE, PXG_BOX, PXG_CAPSULE = 0, 1, 2
+PXG_AVERAGE, PXG_MINIMUM, PXG_MAXIMUM, PXG_MULTIPLY = 0, 1, 2, 3
+PXG_PHASE_FORCES_COLLIDE, PXG_PHASE_SOLVE_INTEGRATE = 0, 1
+
+BODY_DTYPE = np.dtype([
+    ("type", "<u4"), ("alive", "<u4"), ("awake", "<u4"), ("use_gravity", "<u4"),
+    ("constraints", "<u4"), ("generation", "<u4"), ("reserved0", "<u4"), ("reserved1", "<u4"),
+    ("position", "<f4", 3), ("sleep_timer", "<f4"),
+    ("orientation", "<f4", 4),
+    ("linear_velocity", "<f4", 3), ("mass", "<f4"),
+    ("angular_velocity", "<f4", 3), ("inv_mass", "<f4"),
+    ("force", "<f4", 3), ("linear_drag", "<f4"),
+    ("torque", "<f4", 3), ("angular_drag", "<f4"),
+    ("inv_inertia_local", "<f4", 3), ("reserved2", "<f4"),
+    ("inv_inertia_world", "<f4", 9),
+    ("prev_position", "<f4", 3), ("reserved3", "<f4"),
+    ("prev_orientation", "<f4", 4),
+])
+assert BODY_DTYPE.itemsize == 212
+
+SHAPE_DTYPE = np.dtype([
+    ("owner", "<u4"), ("type", "<u4"), ("enabled", "<u4"), ("is_trigger", "<u4"),
+    ("layer", "<u4"), ("capsule_axis", "<i4"), ("friction_combine", "<u4"), ("restitution_combine", "<u4"),
+    ("local_center", "<f4", 3), ("radius", "<f4"),
+    ("local_rotation", "<f4", 4),
+    ("half_extents", "<f4", 3), ("capsule_radius", "<f4"),
+    ("capsule_half_height", "<f4"), ("dynamic_friction", "<f4"), ("static_friction", "<f4"), ("restitution", "<f4"),
+])
+assert SHAPE_DTYPE.itemsize == 96
+
+CONTACT_DTYPE = np.dtype([
+    ("position", "<f4", 3), ("penetration", "<f4"), ("separation", "<f4"), ("normal_impulse", "<f4"),
+    ("tangent_impulse", "<f4", 2), ("feature_id", "<u4"), ("restitution_bias", "<f4"), ("reserved", "<f4", 2),
+])
+assert CONTACT_DTYPE.itemsize == 48
+
+MANIFOLD_DTYPE = np.dtype([
+    ("a", "<u4"), ("b", "<u4"), ("shape_a", "<u4"), ("shape_b", "<u4"),
+    ("normal", "<f4", 3), ("friction", "<f4"),
+    ("restitution", "<f4"), ("count", "<u4"), ("is_trigger", "<u4"), ("valid", "<u4"),
+    ("points", CONTACT_DTYPE, 4),
+])
+assert MANIFOLD_DTYPE.itemsize == 240
+
+
+class Config(ctypes.Structure):
+    """pxg_config (include/pxg.h)."""
+    _fields_ = [
+        ("velocity_iterations", ctypes.c_uint32), ("relax_iterations", ctypes.c_uint32),
+        ("position_iterations", ctypes.c_uint32), ("position_correction", ctypes.c_float),
+        ("linear_slop", ctypes.c_float), ("max_linear_correction", ctypes.c_float),
+        ("restitution_threshold", ctypes.c_float), ("sleep_linear", ctypes.c_float),
+        ("sleep_angular", ctypes.c_float), ("sleep_time", ctypes.c_float), ("aabb_margin", ctypes.c_float),
+        ("max_substeps", ctypes.c_uint32), ("fixed_dt", ctypes.c_float), ("max_linear_velocity", ctypes.c_float),
+        ("use_exact_quaternion_integration", ctypes.c_uint32), ("gravity", ctypes.c_float * 3),
+        ("residual_mode", ctypes.c_uint32), ("residual_tolerance", ctypes.c_float),
+        ("residual_max_iterations", ctypes.c_uint32), ("large_extent", ctypes.c_float),
+        ("max_bodies", ctypes.c_uint32), ("max_shapes", ctypes.c_uint32), ("max_pairs", ctypes.c_uint32),
+        ("max_manifolds", ctypes.c_uint32), ("learned_warm_start", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 7),
+    ]
+
+
+class Stats(ctypes.Structure):
+    """pxg_stats (include/pxg.h)."""
+    _fields_ = [
+        ("pairs", ctypes.c_uint32), ("manifolds", ctypes.c_uint32), ("points", ctypes.c_uint32),
+        ("colours", ctypes.c_uint32), ("active_manifolds", ctypes.c_uint32), ("velocity_passes", ctypes.c_uint32),
+        ("awake_bodies", ctypes.c_uint32), ("learned_predictions", ctypes.c_uint32),
+        ("residual", ctypes.c_float), ("reserved0", ctypes.c_float), ("steps", ctypes.c_uint64),
+    ]
+
+
+class Profile(ctypes.Structure):
+    """pxg_profile (include/pxg.h)."""
+    _fields_ = [("velocity_ms", ctypes.c_double), ("velocity_bytes", ctypes.c_double),
+                ("velocity_launches", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 5)]
+
+
+def default_config(**overrides):
+    """util::PhysicsConfig defaults (physxcoopa/util/config.h:22-145) plus the GPU extras."""
+    c = Config()
+    c.velocity_iterations = 16
+    c.relax_iterations = 2
+    c.position_iterations = 4
+    c.position_correction = 0.2
+    c.linear_slop = 0.005
+    c.max_linear_correction = 0.2
+    c.restitution_threshold = 0.5
+    c.sleep_linear = 0.01
+    c.sleep_angular = 0.02
+    c.sleep_time = 0.5
+    c.aabb_margin = 0.1
+    c.max_substeps = 8
+    c.fixed_dt = np.float32(1.0) / np.float32(60.0)
+    c.max_linear_velocity = 200.0
+    c.use_exact_quaternion_integration = 0
+    c.gravity[0], c.gravity[1], c.gravity[2] = 0.0, 0.0, -9.81
+    c.residual_mode = 0
+    c.residual_tolerance = 1e-4
+    c.residual_max_iterations = 256
+    c.large_extent = 16.0
+    c.learned_warm_start = 0
+    for k, v in overrides.items():
+        if k == "gravity":
+            for i in range(3):
+                c.gravity[i] = v[i]
+        else:
+            setattr(c, k, v)
+    return c
+
+
+def ptr(a):
+    """ctypes void pointer to a contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)

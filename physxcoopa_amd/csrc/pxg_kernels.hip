@@ -1,0 +1,1199 @@
+// Step kernels of one world on gfx950 (CDNA4): integration, broadphase (x-y grid over fat
+// AABBs plus a brute-force path for large shapes), narrowphase, manifold compaction,
+// contact-graph colouring, the colour-by-colour solver passes, the warm-start hash table,
+// islands/sleep and velocity integration. Launch shapes: 256-thread blocks (4 wave64s), one
+// work item per thread, grid-stride where the item count lives on the device.
+#include "pxg_state.h"
+#include "pxg_device.h"
+
+using namespace pxd;
+
+namespace pxg {
+
+namespace {
+
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t o) {
+    uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t hash64(unsigned long long k) {
+    return hash32(uint32_t(k) ^ hash32(uint32_t(k >> 32) + 0x9e3779b9u));
+}
+
+__device__ __forceinline__ bool body_active(uint32_t f) {  // dynamic and awake (solver.h:86-95)
+    return (f & BF_TYPE) == 2u && (f & BF_AWAKE) && (f & BF_ALIVE);
+}
+
+__device__ __forceinline__ ShapeGeom load_shape(const DevState& d, uint32_t i, uint4 info) {
+    ShapeGeom s;
+    s.type = int(info.y & 15u);
+    s.caxis = int((info.y >> 8) & 3u);
+    float4 lc = d.s_lc[i];
+    s.lc = mk(lc.x, lc.y, lc.z);
+    s.radius = lc.w;
+    s.lr = ldq(d.s_lr[i]);
+    float4 dim = d.s_dim[i];
+    s.he = mk(dim.x, dim.y, dim.z);
+    s.cr = dim.w;
+    s.chh = d.s_mat[i].x;
+    return s;
+}
+__device__ __forceinline__ bool shape_live(const DevState& d, uint4 info) {
+    return (info.y & SF_ENABLED) && (d.b_flags[info.x] & BF_ALIVE);
+}
+
+__device__ __forceinline__ m3 load_iiw(const float4* p, uint32_t i) {
+    float4 a = p[3 * i], b = p[3 * i + 1], c = p[3 * i + 2];
+    m3 m;
+    m.c0 = mk(a.x, a.y, a.z);
+    m.c1 = mk(a.w, b.x, b.y);
+    m.c2 = mk(b.z, b.w, c.x);
+    return m;
+}
+__device__ __forceinline__ void store_iiw(float4* p, uint32_t i, const m3& m, float tail0 = 0.0f, float tail1 = 0.0f,
+                                          float tail2 = 0.0f) {
+    p[3 * i] = make_float4(m.c0.x, m.c0.y, m.c0.z, m.c1.x);
+    p[3 * i + 1] = make_float4(m.c1.y, m.c1.z, m.c2.x, m.c2.y);
+    p[3 * i + 2] = make_float4(m.c2.z, tail0, tail1, tail2);
+}
+
+__device__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[kBlock / 64];
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < wid) off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + x - v;
+}
+
+// Wave-wide reductions; every lane of the wave must reach the call.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { uint32_t u = __shfl_xor(v, o, 64); v = u < v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { uint32_t u = __shfl_xor(v, o, 64); v = u > v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ bool lane0() { return (threadIdx.x & 63) == 0; }
+
+// ------------------------------------------------------------------------------------------
+// generic helpers
+// ------------------------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long* p, unsigned long long v, size_t n) {
+    for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+// 3-phase exclusive scan: each block covers 1024 elements (4 per thread).
+__global__ void k_scan_reduce(const uint32_t* in, uint32_t n, uint32_t* bsum) {
+    uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t s = 0;
+    for (int k = 0; k < 4; ++k)
+        if (base + k < n) s += in[base + k];
+    uint32_t tot;
+    block_exclusive_scan(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+__global__ void k_scan_bsums(uint32_t* bsum, uint32_t nb, uint32_t* total) {
+    // single block of 256 threads, up to 1024 block sums (4 per thread)
+    uint32_t base = threadIdx.x * 4u;
+    uint32_t v[4], s = 0;
+    for (int k = 0; k < 4; ++k) { v[k] = (base + k < nb) ? bsum[base + k] : 0u; s += v[k]; }
+    uint32_t tot;
+    uint32_t off = block_exclusive_scan(s, &tot);
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < nb) bsum[base + k] = off;
+        off += v[k];
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+__global__ void k_scan_down(const uint32_t* in, uint32_t* out, uint32_t n, const uint32_t* bsum) {
+    uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t v[4], s = 0;
+    for (int k = 0; k < 4; ++k) { v[k] = (base + k < n) ? in[base + k] : 0u; s += v[k]; }
+    uint32_t off = block_exclusive_scan(s, nullptr) + bsum[blockIdx.x];
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) out[base + k] = off;
+        off += v[k];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// bodies
+// ------------------------------------------------------------------------------------------
+// update_world_inertia for every body (integrator.h:57-68), as add_body() does (world.h:113)
+__global__ void k_init_bodies(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    uint32_t f = d.b_flags[i];
+    m3 m;
+    if ((f & BF_TYPE) != 2u) { m.c0 = splat(0.0f); m.c1 = splat(0.0f); m.c2 = splat(0.0f); }
+    else m = world_inertia(ldq(d.b_rot[i]), ld3(d.b_iil[i]));
+    store_iiw(d.b_iiw, i, m);
+}
+
+// apply_gravity + integrate_forces (world.h:1283-1295, integrator.h:31-35, 78-94)
+__global__ void k_integrate_forces(DevState d, StepParams p) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {  // reset this step's counters
+        Counters* c = d.cnt;
+        c->pairs = 0; c->manifolds = 0; c->points = 0; c->active = 0; c->ordered = 0; c->remaining = 0;
+        c->colours = 0; c->n_large = 0; c->awake = 0; c->predicted = 0; c->residual_bits = 0;
+        c->red[0] = 0xFFFFFFFFu; c->red[1] = 0xFFFFFFFFu; c->red[2] = 0u; c->red[3] = 0u; c->red[4] = 0u; c->red[5] = 0u;
+    }
+    if (i >= d.nb) return;
+    uint32_t f = d.b_flags[i];
+    if (!(f & BF_ALIVE)) return;
+    bool dyn = (f & BF_TYPE) == 2u, awake = (f & BF_AWAKE) != 0;
+    float4 fo = d.b_force[i], to = d.b_torque[i];
+    if (!dyn || !awake) {
+        d.b_force[i] = make_float4(0.0f, 0.0f, 0.0f, fo.w);
+        d.b_torque[i] = make_float4(0.0f, 0.0f, 0.0f, to.w);
+        return;
+    }
+    float4 vm = d.b_vel[i], wm = d.b_angv[i];
+    v3 force = ld3(fo), torque = ld3(to);
+    if (f & BF_GRAVITY) force = force + mk(p.gx, p.gy, p.gz) * wm.w;  // force_accum += g * mass
+    v3 v = ld3(vm), w = ld3(wm);
+    v = v + (force * vm.w) * p.h;
+    m3 I = load_iiw(d.b_iiw, i);
+    w = w + mul(I, torque) * p.h;
+    v = v * (1.0f / (1.0f + p.h * fo.w));
+    w = w * (1.0f / (1.0f + p.h * to.w));
+    uint32_t cons = f >> BF_CONSTRAINT_SHIFT;
+    if (cons & 1u) v.x = 0.0f;
+    if (cons & 2u) v.y = 0.0f;
+    if (cons & 4u) v.z = 0.0f;
+    if (cons & 8u) w.x = 0.0f;
+    if (cons & 16u) w.y = 0.0f;
+    if (cons & 32u) w.z = 0.0f;
+    d.b_vel[i] = st3(v, vm.w);
+    d.b_angv[i] = st3(w, wm.w);
+    d.b_force[i] = make_float4(0.0f, 0.0f, 0.0f, fo.w);
+    d.b_torque[i] = make_float4(0.0f, 0.0f, 0.0f, to.w);
+}
+
+// ------------------------------------------------------------------------------------------
+// broadphase
+// ------------------------------------------------------------------------------------------
+// create_shape_proxy_ (world.h:1369-1382): fat = tight.expand(margin) at the owner's pose
+__global__ void k_init_proxies(DevState d, StepParams p) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ns) return;
+    uint4 info = d.s_info[i];
+    if (!shape_live(d, info)) {
+        d.s_fmin[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        d.s_fmax[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    ShapeGeom s = load_shape(d, i, info);
+    v3 mn, mx;
+    world_bounds(s, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]), mn, mx);
+    d.s_fmin[i] = st3(mn - splat(p.margin), 0.0f);
+    d.s_fmax[i] = st3(mx + splat(p.margin), 0.0f);
+}
+
+// compute_broadphase_bounds_ + move_proxy (world.h:1489-1518, aabb_tree.h:76-85); large-shape
+// list; reductions for the grid over small shapes' fat AABBs.
+__global__ void k_bounds(DevState d, StepParams p) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t r0 = 0xFFFFFFFFu, r1 = 0xFFFFFFFFu, r2 = 0u, r3 = 0u, r4 = 0u, r5 = 0u;
+    if (i < d.ns) {
+    uint4 info = d.s_info[i];
+    if (!shape_live(d, info)) {
+        d.s_tmin[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+    } else {
+    ShapeGeom s = load_shape(d, i, info);
+    v3 mn, mx;
+    uint32_t ob = info.x;
+    world_bounds(s, ld3(d.b_pos[ob]), ldq(d.b_rot[ob]), mn, mx);
+    v3 fmn = ld3(d.s_fmin[i]), fmx = ld3(d.s_fmax[i]);
+    if (!(info.y & SF_STATIC)) {
+        v3 disp = ld3(d.b_vel[ob]) * p.h;
+        v3 smn = vmin(mn, mn + disp), smx = vmax(mx, mx + disp);
+        bool contains = smn.x >= fmn.x && smn.x <= fmx.x && smn.y >= fmn.y && smn.y <= fmx.y && smn.z >= fmn.z &&
+                        smn.z <= fmx.z && smx.x >= fmn.x && smx.x <= fmx.x && smx.y >= fmn.y && smx.y <= fmx.y &&
+                        smx.z >= fmn.z && smx.z <= fmx.z;
+        if (!contains) {
+            fmn = smn - splat(p.margin);
+            fmx = smx + splat(p.margin);
+            d.s_fmin[i] = st3(fmn, 0.0f);
+            d.s_fmax[i] = st3(fmx, 0.0f);
+        }
+    }
+    float ex = fmx.x - fmn.x, ey = fmx.y - fmn.y;
+    bool large = ex > p.large_extent || ey > p.large_extent;
+    d.s_tmin[i] = st3(mn, large ? 1.0f : 0.0f);
+    d.s_tmax[i] = st3(mx, 0.0f);
+    Counters* c = d.cnt;
+    if (large) {
+        uint32_t k = atomicAdd(&c->n_large, 1u);
+        if (k < uint32_t(kMaxLarge)) d.large_list[k] = i;
+        else atomicOr(&c->err, ERR_LARGE);
+    } else {
+        r0 = f2o(fmn.x); r1 = f2o(fmn.y); r2 = r0; r3 = r1; r4 = f2o(ex); r5 = f2o(ey);
+    }
+    }
+    }
+    r0 = wave_min(r0); r1 = wave_min(r1); r2 = wave_max(r2); r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
+    if (lane0() && r2 != 0u) {
+        Counters* c = d.cnt;
+        atomicMin(&c->red[0], r0); atomicMin(&c->red[1], r1);
+        atomicMax(&c->red[2], r2); atomicMax(&c->red[3], r3);
+        atomicMax(&c->red[4], r4); atomicMax(&c->red[5], r5);
+    }
+}
+
+__global__ void k_grid_setup(DevState d) {
+    Counters* c = d.cnt;
+    float ox = 0.0f, oy = 0.0f, cell = 1.0f, exty = 0.0f;
+    int nx = 1, ny = 1;
+    if (c->red[0] != 0xFFFFFFFFu) {
+        ox = o2f(c->red[0]);
+        oy = o2f(c->red[1]);
+        float mxx = o2f(c->red[2]), mxy = o2f(c->red[3]);
+        float ex = o2f(c->red[4]);
+        exty = o2f(c->red[5]);
+        cell = fmaxf(fmaxf(ex, exty), 1e-3f);
+        for (int it = 0; it < 64; ++it) {
+            float fx = floorf((mxx - ox) / cell) + 1.0f, fy = floorf((mxy - oy) / cell) + 1.0f;
+            if (fx * fy <= float(kMaxCells)) { nx = int(fx); ny = int(fy); break; }
+            cell *= 2.0f;
+        }
+    }
+    c->grid[0] = ox; c->grid[1] = oy; c->grid[2] = 1.0f / cell; c->grid[3] = exty;
+    c->grid[4] = float(nx); c->grid[5] = float(ny);
+}
+
+struct Grid {
+    float ox, oy, inv, exty;
+    int nx, ny;
+};
+__device__ __forceinline__ Grid load_grid(const Counters* c) {
+    Grid g;
+    g.ox = c->grid[0]; g.oy = c->grid[1]; g.inv = c->grid[2]; g.exty = c->grid[3];
+    g.nx = int(c->grid[4]); g.ny = int(c->grid[5]);
+    return g;
+}
+__device__ __forceinline__ int cellx(const Grid& g, float x) {
+    float f = floorf((x - g.ox) * g.inv);
+    return f < 0.0f ? 0 : (f > float(g.nx - 1) ? g.nx - 1 : int(f));
+}
+__device__ __forceinline__ int celly(const Grid& g, float y) {
+    float f = floorf((y - g.oy) * g.inv);
+    return f < 0.0f ? 0 : (f > float(g.ny - 1) ? g.ny - 1 : int(f));
+}
+
+__global__ void k_grid_clear(DevState d) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kMaxCells; i += gridDim.x * blockDim.x) {
+        d.cell_count[i] = 0u;
+        d.cell_cursor[i] = 0u;
+    }
+}
+__global__ void k_grid_count(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ns) return;
+    float4 t = d.s_tmin[i];
+    if (t.w != 0.0f) return;  // inactive (-1) or large (1)
+    Grid g = load_grid(d.cnt);
+    float4 f = d.s_fmin[i];
+    atomicAdd(&d.cell_count[celly(g, f.y) * g.nx + cellx(g, f.x)], 1u);
+}
+__global__ void k_grid_scatter(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ns) return;
+    float4 t = d.s_tmin[i];
+    if (t.w != 0.0f) return;
+    Grid g = load_grid(d.cnt);
+    float4 f = d.s_fmin[i];
+    int c = celly(g, f.y) * g.nx + cellx(g, f.x);
+    d.cell_items[d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1u)] = i;
+}
+
+// The reference pair rule (world.h:1545-1577 with aabb.h:35-39): owners differ, not both static,
+// and tight(i) overlaps fat(j) or tight(j) overlaps fat(i).
+__device__ __forceinline__ bool ovl(v3 amn, v3 amx, v3 bmn, v3 bmx) {
+    return amn.x <= bmx.x && amx.x >= bmn.x && amn.y <= bmx.y && amx.y >= bmn.y && amn.z <= bmx.z && amx.z >= bmn.z;
+}
+
+// One emitter per pair: small-small pairs by the shape with the smaller (fat min x, index),
+// small-large pairs by the small shape, large-large pairs by the smaller index. mode 0 counts,
+// mode 1 writes the pairs and sorts the emitter's segment by (a, b).
+template <int MODE>
+__global__ void k_pairs(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ns) return;
+    float4 ti4 = d.s_tmin[i];
+    if (ti4.w < 0.0f) {
+        if (MODE == 0) d.pair_count[i] = 0u;
+        return;
+    }
+    uint4 info = d.s_info[i];
+    uint32_t own_i = info.x;
+    bool st_i = (info.y & SF_STATIC) != 0;
+    bool large_i = ti4.w > 0.0f;
+    v3 tmn = ld3(ti4), tmx = ld3(d.s_tmax[i]);
+    v3 fmn = ld3(d.s_fmin[i]), fmx = ld3(d.s_fmax[i]);
+    uint32_t n = 0, off = 0, cap = d.cap_pairs;
+    if (MODE == 1) off = d.pair_off[i];
+    const Counters* c = d.cnt;
+
+    auto test = [&](uint32_t j) -> bool {
+        uint4 ij = d.s_info[j];
+        if (ij.x == own_i) return false;
+        if (st_i && (ij.y & SF_STATIC)) return false;
+        v3 jfmn = ld3(d.s_fmin[j]), jfmx = ld3(d.s_fmax[j]);
+        if (ovl(tmn, tmx, jfmn, jfmx)) return true;
+        v3 jtmn = ld3(d.s_tmin[j]), jtmx = ld3(d.s_tmax[j]);
+        return ovl(jtmn, jtmx, fmn, fmx);
+    };
+    auto emit = [&](uint32_t j) {
+        if (MODE == 1) {
+            uint32_t k = off + n;
+            if (k < cap) d.pairs[k] = make_uint2(i < j ? i : j, i < j ? j : i);
+        }
+        ++n;
+    };
+
+    if (!large_i) {
+        Grid g = load_grid(c);
+        int cx0 = cellx(g, fmn.x), cx1 = cellx(g, fmx.x);
+        int cy0 = celly(g, fmn.y - g.exty), cy1 = celly(g, fmx.y);
+        for (int cy = cy0; cy <= cy1; ++cy) {
+            for (int cx = cx0; cx <= cx1; ++cx) {
+                int cc = cy * g.nx + cx;
+                uint32_t s0 = d.cell_start[cc], s1 = s0 + d.cell_count[cc];
+                for (uint32_t sl = s0; sl < s1; ++sl) {
+                    uint32_t j = d.cell_items[sl];
+                    if (j == i) continue;
+                    float jx = d.s_fmin[j].x;
+                    if (!(jx > fmn.x || (jx == fmn.x && j > i))) continue;
+                    if (jx > fmx.x) continue;
+                    if (test(j)) emit(j);
+                }
+            }
+        }
+    }
+    uint32_t nl = c->n_large < uint32_t(kMaxLarge) ? c->n_large : uint32_t(kMaxLarge);
+    for (uint32_t k = 0; k < nl; ++k) {
+        uint32_t j = d.large_list[k];
+        if (j == i) continue;
+        if (large_i && j < i) continue;
+        if (test(j)) emit(j);
+    }
+    if (MODE == 0) {
+        d.pair_count[i] = n;
+    } else if (n > 1 && off + n <= cap) {
+        // insertion sort of this emitter's segment by (a, b)
+        uint2* seg = d.pairs + off;
+        for (uint32_t a = 1; a < n; ++a) {
+            uint2 v = seg[a];
+            unsigned long long kv = (static_cast<unsigned long long>(v.x) << 32) | v.y;
+            uint32_t b = a;
+            while (b > 0) {
+                uint2 u = seg[b - 1];
+                unsigned long long ku = (static_cast<unsigned long long>(u.x) << 32) | u.y;
+                if (ku <= kv) break;
+                seg[b] = u;
+                --b;
+            }
+            seg[b] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// narrowphase (world.h:1598-1654, collision/narrowphase.h:175-335)
+// ------------------------------------------------------------------------------------------
+__global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_pairs) return;
+    uint2 pr = d.pairs[k];
+    uint32_t i = pr.x, j = pr.y;
+    uint4 ii = d.s_info[i], ij = d.s_info[j];
+    uint32_t bi = ii.x, bj = ij.x;
+    uint32_t fi = d.b_flags[bi], fj = d.b_flags[bj];
+    uint32_t valid = 0;
+    Man m;
+    m.count = 0;
+    float4 mi = d.s_mat[i], mj = d.s_mat[j];
+    if ((fi & BF_ALIVE) && (fj & BF_ALIVE) && ((fi & BF_TYPE) == 2u || (fj & BF_TYPE) == 2u) &&
+        (d.layer_mask[ii.z & 31u] & (1u << (ij.z & 31u)))) {
+        ShapeGeom si = load_shape(d, i, ii), sj = load_shape(d, j, ij);
+        float4 vi = d.b_vel[bi], vj = d.b_vel[bj];
+        bool spec = (len(ld3(vi)) * prm.h > min_extent(si)) || (len(ld3(vj)) * prm.h > min_extent(sj));
+        if (generate_contacts(si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), spec, m))
+            valid = 1u;
+    }
+    d.pm_valid[k] = valid;
+    if (!valid) return;
+    float fr = combine(mi.y, mj.y, int(ii.w & 255u), int(ij.w & 255u));
+    float re = combine(mi.z, mj.z, int(ii.w >> 8), int(ij.w >> 8));
+    uint32_t trig = ((ii.y | ij.y) & SF_TRIGGER) ? 1u : 0u;
+    d.pm_ab[k] = make_uint2(bi, bj);
+    d.pm_nf[k] = st3(m.n, fr);
+    d.pm_meta[k] = make_uint4(uint32_t(m.count) | (trig << 8), __float_as_uint(re), 0u, 0u);
+    for (int t = 0; t < 4; ++t) {
+        if (t < m.count) {
+            d.pm_p[4 * k + t] = st3(m.p[t].pos, m.p[t].pen);
+            d.pm_q[4 * k + t] = make_float4(m.p[t].sep, __uint_as_float(m.p[t].fid), 0.0f, 0.0f);
+        } else {
+            d.pm_p[4 * k + t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            d.pm_q[4 * k + t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+}
+
+// serial compaction in pair order (world.h:1647-1653)
+__global__ void k_compact(DevState d, uint32_t n_pairs) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t pts = 0;
+    if (k < n_pairs) {
+    if (k == n_pairs - 1) d.cnt->manifolds = d.pm_off[k] + d.pm_valid[k];
+    uint32_t m = d.pm_off[k];
+    if (d.pm_valid[k] && m >= d.cap_manifolds) atomicOr(&d.cnt->err, ERR_MANIFOLDS);
+    else if (d.pm_valid[k]) {
+    d.m_ab[m] = d.pm_ab[k];
+    d.m_shapes[m] = d.pairs[k];
+    d.m_nf[m] = d.pm_nf[k];
+    uint4 meta = d.pm_meta[k];
+    d.m_meta[m] = meta;
+    for (int t = 0; t < 4; ++t) {
+        d.m_p[4 * m + t] = d.pm_p[4 * k + t];
+        d.m_q[4 * m + t] = d.pm_q[4 * k + t];
+    }
+    pts = meta.x & 255u;
+    }
+    }
+    pts = wave_sum(pts);
+    if (lane0() && pts) atomicAdd(&d.cnt->points, pts);
+}
+
+// ------------------------------------------------------------------------------------------
+// colouring: Jones-Plassmann rounds with hashed priorities, first-fit colour from per-body
+// 64-bit masks. Only awake dynamic bodies constrain (apply_impulse_pair writes only those).
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t kUncoloured = 0xFFFFFFFFu, kSkipped = 0xFFFFFFFEu, kTrigger = 0xFFFFFFFDu;
+
+__global__ void k_colour_init(DevState d, uint32_t n_bound) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n_man = d.cnt->manifolds;
+    uint32_t add = 0;
+    if (m < n_man && m < n_bound) {
+        uint2 ab = d.m_ab[m];
+        uint4 meta = d.m_meta[m];
+        bool trig = (meta.x >> 8) & 1u;
+        bool act = body_active(d.b_flags[ab.x]) || body_active(d.b_flags[ab.y]);
+        uint32_t c = trig ? kTrigger : (act ? kUncoloured : kSkipped);
+        d.m_colour[m] = c;
+        add = c == kUncoloured ? 1u : 0u;
+    }
+    add = wave_sum(add);
+    if (lane0() && add) atomicAdd(&d.cnt->remaining, add);
+}
+
+__device__ __forceinline__ unsigned long long colour_key(uint32_t round, uint32_t m, uint2 ab, uint2 sh) {
+    uint32_t pr = hash32(ab.x * 0x9E3779B1u ^ hash32(ab.y + 0x85EBCA77u) ^ (sh.x * 0xC2B2AE3Du)) & 0xFFFu;
+    return (static_cast<unsigned long long>(round) << 44) | (static_cast<unsigned long long>(pr) << 32) | m;
+}
+
+__global__ void k_colour_propose(DevState d, uint32_t n_bound, uint32_t round) {
+    if (d.cnt->remaining == 0u) return;
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_bound || m >= d.cnt->manifolds || d.m_colour[m] != kUncoloured) return;
+    uint2 ab = d.m_ab[m];
+    unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+    if (body_active(d.b_flags[ab.x])) atomicMax(&d.col_best[ab.x], key);
+    if (body_active(d.b_flags[ab.y])) atomicMax(&d.col_best[ab.y], key);
+}
+
+__global__ void k_colour_select(DevState d, uint32_t n_bound, uint32_t round) {
+    if (d.cnt->remaining == 0u) return;
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t took = 0, top = 0;
+    if (m < n_bound && m < d.cnt->manifolds && d.m_colour[m] == kUncoloured) {
+        uint2 ab = d.m_ab[m];
+        unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+        bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
+        bool mine = (!aa || d.col_best[ab.x] == key) && (!ba || d.col_best[ab.y] == key);
+        if (mine) {
+            unsigned long long used = (aa ? d.col_mask[ab.x] : 0ull) | (ba ? d.col_mask[ab.y] : 0ull);
+            unsigned long long freeb = ~used;
+            if (freeb == 0ull) {
+                atomicOr(&d.cnt->err, ERR_COLOURS);
+            } else {
+                uint32_t c = uint32_t(__ffsll(static_cast<long long>(freeb)) - 1);
+                d.m_colour[m] = c;
+                unsigned long long bit = 1ull << c;
+                if (aa) atomicOr(&d.col_mask[ab.x], bit);
+                if (ba) atomicOr(&d.col_mask[ab.y], bit);
+                took = 1u;
+                top = c + 1u;
+            }
+        }
+    }
+    took = wave_sum(took);
+    top = wave_max(top);
+    if (lane0() && took) {
+        atomicSub(&d.cnt->remaining, took);
+        atomicMax(&d.cnt->colours, top);
+    }
+}
+
+// stable counting sort of manifolds by bucket (colour 0..63, 64 = skipped by the solver)
+__device__ __forceinline__ int bucket_of(uint32_t c) { return c < uint32_t(kMaxColours) ? int(c) : (c == kSkipped ? kMaxColours : -1); }
+
+__global__ void k_colour_hist(DevState d, uint32_t n_man, uint32_t nblk) {
+    __shared__ uint32_t hist[kBuckets];
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) hist[b] = 0u;
+    __syncthreads();
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n_man && m < d.cnt->manifolds) {
+        int b = bucket_of(d.m_colour[m]);
+        if (b >= 0) atomicAdd(&hist[b], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) d.col_table[b * nblk + blockIdx.x] = hist[b];
+}
+__global__ void k_colour_scatter(DevState d, uint32_t n_man, uint32_t nblk) {
+    __shared__ int bk[kBlock];
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = (m < n_man && m < d.cnt->manifolds) ? bucket_of(d.m_colour[m]) : -1;
+    bk[threadIdx.x] = b;
+    __syncthreads();
+    if (b < 0) return;
+    uint32_t rank = 0;
+    for (uint32_t t = 0; t < threadIdx.x; ++t) rank += (bk[t] == b) ? 1u : 0u;
+    d.order[d.col_table_off[b * nblk + blockIdx.x] + rank] = m;
+}
+__global__ void k_colour_offsets(DevState d, uint32_t nblk, const uint32_t* total) {
+    int b = threadIdx.x;
+    if (b <= kBuckets) {
+        uint32_t v = b < kBuckets ? d.col_table_off[b * nblk] : *total;
+        d.cnt->colour_off[b] = v;
+    }
+    if (b == 0) {
+        d.cnt->active = d.col_table_off[kMaxColours * nblk];
+        d.cnt->ordered = *total;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// solver (dynamics/solver.h)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ bool cache_lookup(const DevState& d, int table, unsigned long long key, float4& val) {
+    const unsigned long long* keys = d.ck_keys[table];
+    uint32_t mask = d.cache_mask;
+    uint32_t idx = hash64(key) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        unsigned long long k = keys[idx];
+        if (k == key) { val = d.ck_val[table][idx]; return true; }
+        if (k == kEmptyKey) return false;
+        idx = (idx + 1u) & mask;
+    }
+    return false;
+}
+
+// Gathers manifolds into solve order, captures restitution targets (solver.h:121-137) and
+// seeds impulses from the previous step's cache (solver.h:144-175); misses start at zero.
+__global__ void k_solver_setup(DevState d, StepParams prm, uint32_t n_ordered) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_ordered) return;
+    uint32_t m = d.order[s];
+    uint2 ab = d.m_ab[m];
+    float4 nf = d.m_nf[m];
+    uint4 meta = d.m_meta[m];
+    uint32_t cnt = meta.x & 255u;
+    float rest = __uint_as_float(meta.y);
+    bool active = s < d.cnt->active;
+    d.sm_ab[s] = ab;
+    d.sm_nf[s] = nf;
+    d.sm_cnt[s] = cnt;
+    v3 n = ld3(nf);
+    v3 pa = ld3(d.b_pos[ab.x]), pb = ld3(d.b_pos[ab.y]);
+    v3 va = ld3(d.b_vel[ab.x]), vb = ld3(d.b_vel[ab.y]);
+    v3 wa = ld3(d.b_angv[ab.x]), wb = ld3(d.b_angv[ab.y]);
+    int prev = d.cur ^ 1;
+    for (uint32_t t = 0; t < 4; ++t) {
+        float4 pp = d.m_p[4 * m + t], qq = d.m_q[4 * m + t];
+        float4 aux = make_float4(qq.x, 0.0f, qq.y, 0.0f);
+        float4 imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (t < cnt) {
+            v3 pos = ld3(pp);
+            v3 ra = pos - pa, rb = pos - pb;
+            v3 vpa = va + cross(wa, ra), vpb = vb + cross(wb, rb);
+            float vap = dot(vpb - vpa, n);
+            aux.y = (-vap > prm.rest_thresh) ? -rest * vap : 0.0f;
+            if (active) {
+                float4 cv;
+                if (cache_lookup(d, prev, pack_key(ab.x, ab.y, __float_as_uint(qq.y)), cv)) imp = cv;
+                else aux.w = 1.0f;  // warm-start miss
+            }
+        }
+        d.sp_pos[4 * s + t] = pp;
+        d.sp_aux[4 * s + t] = aux;
+        d.sp_imp[4 * s + t] = imp;
+    }
+}
+
+// compact solver bodies: effective inverse mass / inertia are zero unless dynamic and awake
+__global__ void k_solver_bodies(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    uint32_t f = d.b_flags[i];
+    bool act = body_active(f);
+    float4 v = d.b_vel[i], w = d.b_angv[i], p = d.b_pos[i];
+    d.sb_v[i] = make_float4(v.x, v.y, v.z, act ? v.w : 0.0f);
+    d.sb_w[i] = make_float4(w.x, w.y, w.z, act ? 1.0f : 0.0f);
+    m3 I;
+    if (act) I = load_iiw(d.b_iiw, i);
+    else { I.c0 = splat(0.0f); I.c1 = splat(0.0f); I.c2 = splat(0.0f); }
+    store_iiw(d.sb_I, i, I, p.x, p.y, p.z);
+}
+
+struct SBody {
+    v3 v, w, x;
+    float im;
+    bool act;
+    m3 I;
+};
+__device__ __forceinline__ SBody load_sbody(const DevState& d, uint32_t i) {
+    SBody b;
+    float4 v = d.sb_v[i], w = d.sb_w[i];
+    float4 a = d.sb_I[3 * i], c = d.sb_I[3 * i + 1], e = d.sb_I[3 * i + 2];
+    b.v = ld3(v); b.im = v.w;
+    b.w = ld3(w); b.act = w.w != 0.0f;
+    b.I.c0 = mk(a.x, a.y, a.z); b.I.c1 = mk(a.w, c.x, c.y); b.I.c2 = mk(c.z, c.w, e.x);
+    b.x = mk(e.y, e.z, e.w);
+    return b;
+}
+__device__ __forceinline__ void store_sbody(const DevState& d, uint32_t i, const SBody& b) {
+    if (!b.act) return;
+    d.sb_v[i] = st3(b.v, b.im);
+    d.sb_w[i] = st3(b.w, 1.0f);
+}
+// apply_impulse_pair (solver.h:86-95)
+__device__ __forceinline__ void apply_pair(SBody& a, SBody& b, v3 p, v3 J) {
+    if (a.act) { a.v = a.v - J * a.im; a.w = a.w - mul(a.I, cross(p - a.x, J)); }
+    if (b.act) { b.v = b.v + J * b.im; b.w = b.w + mul(b.I, cross(p - b.x, J)); }
+}
+
+// warm_start's impulse application (solver.h:164-173), one colour
+__global__ void k_warm_apply(DevState d, uint32_t begin, uint32_t end) {
+    uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= end) return;
+    uint2 ab = d.sm_ab[s];
+    SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
+    v3 n = ld3(d.sm_nf[s]);
+    uint32_t cnt = d.sm_cnt[s];
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+    for (uint32_t i = 0; i < cnt; ++i) {
+        float4 imp = d.sp_imp[4 * s + i];
+        v3 J = n * imp.x + t1 * imp.y + t2 * imp.z;
+        if (dot(J, J) > 0.0f) apply_pair(A, B, ld3(d.sp_pos[4 * s + i]), J);
+    }
+    store_sbody(d, ab.x, A);
+    store_sbody(d, ab.y, B);
+}
+
+// solve_velocity_pass (solver.h:212-281), one colour
+__device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset);
+
+__global__ void k_velocity(DevState d, StepParams prm, uint32_t begin, uint32_t end, int use_rest, uint32_t offset) {
+    uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (prm.profile) {
+        // algorithmic bytes of one velocity pass: 192 per manifold + 40 per point
+        uint32_t bytes = s < end ? 192u + 40u * d.sm_cnt[s] : 0u;
+        bytes = wave_sum(bytes);
+        if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+    }
+    if (s >= end) return;
+    velocity_manifold(d, prm, s, use_rest, offset);
+}
+
+__device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset) {
+    uint2 ab = d.sm_ab[s];
+    SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
+    float4 nf = d.sm_nf[s];
+    v3 n = ld3(nf);
+    float mu = nf.w;
+    uint32_t cnt = d.sm_cnt[s];
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+    for (uint32_t k = 0; k < cnt; ++k) {
+        uint32_t i = (k + offset) % cnt;
+        float4 pp = d.sp_pos[4 * s + i], aux = d.sp_aux[4 * s + i], imp = d.sp_imp[4 * s + i];
+        v3 pos = ld3(pp);
+        v3 ra = pos - A.x, rb = pos - B.x;
+        {
+            v3 ran = cross(ra, n), rbn = cross(rb, n);
+            float kn = A.im + B.im + dot(ran, mul(A.I, ran)) + dot(rbn, mul(B.I, rbn));
+            if (kn > kEps) {
+                v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
+                float vn = dot(vb - va, n);
+                float target = aux.x > 0.0f ? -aux.x / prm.h : (use_rest ? aux.y : 0.0f);
+                float lambda = -(vn - target) / kn;
+                float ni = gmax(imp.x + lambda, 0.0f);
+                float delta = ni - imp.x;
+                imp.x = ni;
+                apply_pair(A, B, pos, n * delta);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            v3 dir = t == 0 ? t1 : t2;
+            v3 rat = cross(ra, dir), rbt = cross(rb, dir);
+            float kt = A.im + B.im + dot(rat, mul(A.I, rat)) + dot(rbt, mul(B.I, rbt));
+            if (kt <= kEps) continue;
+            v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
+            float vt = dot(vb - va, dir);
+            float lt = -vt / kt;
+            float maxf = mu * imp.x;
+            float old = t == 0 ? imp.y : imp.z;
+            float nt = sclamp(old + lt, -maxf, maxf);
+            float delta = nt - old;
+            if (t == 0) imp.y = nt; else imp.z = nt;
+            apply_pair(A, B, pos, dir * delta);
+        }
+        d.sp_imp[4 * s + i] = imp;
+    }
+    store_sbody(d, ab.x, A);
+    store_sbody(d, ab.y, B);
+}
+
+// velocity residual (m/s) over the solver's active rows, max-reduced into cnt->residual_bits
+__global__ void k_residual(DevState d, StepParams prm, uint32_t n_active, int use_rest) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    float r = 0.0f;
+    if (s < n_active) {
+        uint2 ab = d.sm_ab[s];
+        SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
+        float4 nf = d.sm_nf[s];
+        v3 n = ld3(nf);
+        v3 t1, t2;
+        orthonormal_basis(n, t1, t2);
+        uint32_t cnt = d.sm_cnt[s];
+        for (uint32_t i = 0; i < cnt; ++i) {
+            float4 pp = d.sp_pos[4 * s + i], aux = d.sp_aux[4 * s + i], imp = d.sp_imp[4 * s + i];
+            v3 pos = ld3(pp);
+            v3 ra = pos - A.x, rb = pos - B.x;
+            v3 dv = (B.v + cross(B.w, rb)) - (A.v + cross(A.w, ra));
+            float vn = dot(dv, n);
+            float target = aux.x > 0.0f ? -aux.x / prm.h : (use_rest ? aux.y : 0.0f);
+            float rn = imp.x > 0.0f ? fabsf(vn - target) : gmax(0.0f, target - vn);
+            r = gmax(r, rn);
+            float maxf = nf.w * imp.x;
+            for (int t = 0; t < 2; ++t) {
+                float vt = dot(dv, t == 0 ? t1 : t2);
+                float lt = t == 0 ? imp.y : imp.z;
+                float rt = 0.0f;
+                if (maxf > 0.0f) rt = fabsf(lt) < maxf ? fabsf(vt) : gmax(0.0f, lt < 0.0f ? -vt : vt);
+                r = gmax(r, rt);
+            }
+        }
+    }
+    // wave max then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+    if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(&d.cnt->residual_bits, __float_as_uint(r));
+}
+__global__ void k_residual_reset(DevState d) { d.cnt->residual_bits = 0u; }
+
+// solve_position setup: positions and reference snapshot (solver.h:295-296)
+__global__ void k_position_setup(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    float4 p = d.b_pos[i];
+    d.sb_x[i] = make_float4(p.x, p.y, p.z, d.sb_v[i].w);
+    d.sb_ref[i] = p;
+}
+// solve_position, one colour (solver.h:298-327)
+__global__ void k_position(DevState d, StepParams prm, uint32_t begin, uint32_t end) {
+    uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= end) return;
+    uint2 ab = d.sm_ab[s];
+    float4 xa4 = d.sb_x[ab.x], xb4 = d.sb_x[ab.y];
+    bool aa = d.sb_w[ab.x].w != 0.0f, ba = d.sb_w[ab.y].w != 0.0f;
+    float ima = xa4.w, imb = xb4.w;
+    float k = ima + imb;
+    if (k <= kEps) return;
+    v3 xa = ld3(xa4), xb = ld3(xb4);
+    v3 ra = ld3(d.sb_ref[ab.x]), rb = ld3(d.sb_ref[ab.y]);
+    v3 n = ld3(d.sm_nf[s]);
+    v3 da = xa - ra, db = xb - rb;
+    uint32_t cnt = d.sm_cnt[s];
+    for (uint32_t i = 0; i < cnt; ++i) {
+        float pen = d.sp_pos[4 * s + i].w;
+        float cur = pen - dot(db - da, n);
+        float mag = gmax(0.0f, cur - prm.slop) * prm.pos_corr;
+        mag = gmin(mag, prm.max_corr);
+        if (mag <= 0.0f) continue;
+        v3 corr = n * (mag / k);
+        if (aa) { xa = xa - corr * ima; da = xa - ra; }
+        if (ba) { xb = xb + corr * imb; db = xb - rb; }
+    }
+    if (aa) d.sb_x[ab.x] = st3(xa, ima);
+    if (ba) d.sb_x[ab.y] = st3(xb, imb);
+}
+
+// write solver bodies back; rebuild the warm-start cache (solver.h:767-781): insert in solve
+// order, last writer (highest order) wins on key collisions
+__global__ void k_bodies_back(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    if (d.sb_w[i].w == 0.0f) return;
+    float4 v = d.sb_v[i], w = d.sb_w[i], x = d.sb_x[i];
+    float4 ov = d.b_vel[i], ow = d.b_angv[i], op = d.b_pos[i];
+    d.b_vel[i] = make_float4(v.x, v.y, v.z, ov.w);
+    d.b_angv[i] = make_float4(w.x, w.y, w.z, ow.w);
+    d.b_pos[i] = make_float4(x.x, x.y, x.z, op.w);
+}
+__global__ void k_cache_insert(DevState d, uint32_t n_ordered) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_ordered) return;
+    uint2 ab = d.sm_ab[s];
+    uint32_t cnt = d.sm_cnt[s];
+    unsigned long long* keys = d.ck_keys[d.cur];
+    uint32_t* ord = d.ck_ord[d.cur];
+    uint32_t mask = d.cache_mask;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        unsigned long long key = pack_key(ab.x, ab.y, __float_as_uint(d.sp_aux[4 * s + i].z));
+        uint32_t idx = hash64(key) & mask;
+        bool done = false;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            unsigned long long old = atomicCAS(&keys[idx], kEmptyKey, key);
+            if (old == kEmptyKey || old == key) {
+                atomicMax(&ord[idx], 4u * s + i + 1u);
+                done = true;
+                break;
+            }
+            idx = (idx + 1u) & mask;
+        }
+        if (!done) atomicOr(&d.cnt->err, ERR_CACHE);
+    }
+}
+__global__ void k_cache_write(DevState d, uint32_t n_ordered) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_ordered) return;
+    uint2 ab = d.sm_ab[s];
+    uint32_t cnt = d.sm_cnt[s];
+    const unsigned long long* keys = d.ck_keys[d.cur];
+    const uint32_t* ord = d.ck_ord[d.cur];
+    uint32_t mask = d.cache_mask;
+    for (uint32_t i = 0; i < cnt; ++i) {
+        unsigned long long key = pack_key(ab.x, ab.y, __float_as_uint(d.sp_aux[4 * s + i].z));
+        uint32_t idx = hash64(key) & mask;
+        for (uint32_t probe = 0; probe <= mask; ++probe) {
+            if (keys[idx] == key) {
+                if (ord[idx] == 4u * s + i + 1u) d.ck_val[d.cur][idx] = d.sp_imp[4 * s + i];
+                break;
+            }
+            if (keys[idx] == kEmptyKey) break;
+            idx = (idx + 1u) & mask;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// islands and sleep (solver.h:808-876, island.h:25-49)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
+    while (true) {
+        uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == x) return x;
+        uint32_t g = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (g != p) __hip_atomic_store(&parent[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = p;
+    }
+}
+__global__ void k_uf_init(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    d.uf_parent[i] = i;
+    d.isl_min[i] = 0x7F800000u;
+}
+// unite dynamic-dynamic manifolds (triggers included); roots hook onto the smaller index so the
+// final root is the component's smallest body index
+__global__ void k_uf_union(DevState d, uint32_t n_man) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_man) return;
+    uint2 ab = d.m_ab[m];
+    if ((d.b_flags[ab.x] & BF_TYPE) != 2u || (d.b_flags[ab.y] & BF_TYPE) != 2u) return;
+    uint32_t a = ab.x, b = ab.y;
+    for (int guard = 0; guard < (1 << 24); ++guard) {
+        a = uf_find(d.uf_parent, a);
+        b = uf_find(d.uf_parent, b);
+        if (a == b) return;
+        if (a > b) { uint32_t t = a; a = b; b = t; }
+        if (atomicCAS(&d.uf_parent[b], b, a) == b) return;
+    }
+}
+__global__ void k_uf_compress(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    uint32_t r = uf_find(d.uf_parent, i);
+    __hip_atomic_store(&d.uf_parent[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__global__ void k_sleep_timers(DevState d, StepParams p) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    uint32_t f = d.b_flags[i];
+    if (!body_active(f)) return;
+    v3 v = ld3(d.b_vel[i]), w = ld3(d.b_angv[i]);
+    float4 pt = d.b_pos[i];
+    float l2 = dot(v, v), a2 = dot(w, w);
+    if (l2 < p.sleep_lin * p.sleep_lin && a2 < p.sleep_ang * p.sleep_ang) pt.w = pt.w + p.h;
+    else pt.w = 0.0f;
+    d.b_pos[i] = pt;
+}
+// wake_if_kinematic_moving both ways per valid manifold (solver.h:787-795, 849-853)
+__global__ void k_kin_wake(DevState d, StepParams p, uint32_t n_man) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_man) return;
+    uint2 ab = d.m_ab[m];
+    for (int side = 0; side < 2; ++side) {
+        uint32_t kin = side == 0 ? ab.x : ab.y, dyn = side == 0 ? ab.y : ab.x;
+        uint32_t fk = d.b_flags[kin], fd = d.b_flags[dyn];
+        if ((fk & BF_TYPE) != 1u || (fd & BF_TYPE) != 2u || (fd & BF_AWAKE)) continue;
+        v3 kv = ld3(d.b_vel[kin]), kw = ld3(d.b_angv[kin]);
+        bool lm = dot(kv, kv) > p.sleep_lin * p.sleep_lin;
+        bool am = dot(kw, kw) > p.sleep_ang * p.sleep_ang;
+        if (lm || am) {
+            atomicOr(&d.b_flags[dyn], BF_AWAKE);
+            float4 pt = d.b_pos[dyn];
+            pt.w = 0.0f;
+            d.b_pos[dyn] = pt;
+        }
+    }
+}
+__global__ void k_island_min(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    if (!body_active(d.b_flags[i])) return;
+    atomicMin(&d.isl_min[d.uf_parent[i]], __float_as_uint(d.b_pos[i].w));
+}
+__global__ void k_sleep(DevState d, StepParams p) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    uint32_t f = d.b_flags[i];
+    if (!body_active(f)) return;
+    if (__uint_as_float(d.isl_min[d.uf_parent[i]]) >= p.sleep_time) {
+        d.b_flags[i] = f & ~BF_AWAKE;
+        float4 v = d.b_vel[i], w = d.b_angv[i];
+        d.b_vel[i] = make_float4(0.0f, 0.0f, 0.0f, v.w);
+        d.b_angv[i] = make_float4(0.0f, 0.0f, 0.0f, w.w);
+    }
+}
+
+// prev pose, integrate_velocities, update_world_inertia (world.h:1303-1309, integrator.h:109-133)
+__global__ void k_integrate_velocities(DevState d, StepParams p) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t f = i < d.nb ? d.b_flags[i] : 0u;
+    uint32_t aw = ((f & BF_ALIVE) && (f & BF_TYPE) == 2u && (f & BF_AWAKE)) ? 1u : 0u;
+    aw = wave_sum(aw);
+    if (lane0() && aw) atomicAdd(&d.cnt->awake, aw);
+    if (i >= d.nb) return;
+    if (!(f & BF_ALIVE)) return;
+    float4 pos = d.b_pos[i], rot = d.b_rot[i];
+    d.b_prev_pos[i] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+    d.b_prev_rot[i] = rot;
+    bool dyn = (f & BF_TYPE) == 2u;
+    if (dyn && (f & BF_AWAKE)) {
+        float4 vm = d.b_vel[i];
+        v3 v = ld3(vm);
+        float s2 = dot(v, v);
+        float m2 = p.max_lin_vel * p.max_lin_vel;
+        if (s2 > m2 && s2 > kEps) {
+            v = v * (p.max_lin_vel / sqrtf(s2));
+            d.b_vel[i] = st3(v, vm.w);
+        }
+        v3 x = ld3(pos) + v * p.h;
+        d.b_pos[i] = st3(x, pos.w);
+        quat q = ldq(rot);
+        v3 w = ld3(d.b_angv[i]);
+        if (p.exact_quat) {
+            float angle = len(w) * p.h;
+            if (angle > kEps) {
+                v3 ax = w / len(w);
+                float half = angle * 0.5f;
+                float sn = sinf(half);
+                quat dq;
+                dq.w = cosf(half); dq.x = ax.x * sn; dq.y = ax.y * sn; dq.z = ax.z * sn;
+                q = qnormalize(qmul(dq, q));
+            }
+        } else {
+            quat wq;
+            wq.w = 0.0f; wq.x = w.x; wq.y = w.y; wq.z = w.z;
+            quat dq = qmul(wq, q);
+            float hh = 0.5f * p.h;
+            quat s;
+            s.w = q.w + dq.w * hh; s.x = q.x + dq.x * hh; s.y = q.y + dq.y * hh; s.z = q.z + dq.z * hh;
+            q = qnormalize(s);
+        }
+        d.b_rot[i] = stq(q);
+        store_iiw(d.b_iiw, i, world_inertia(q, ld3(d.b_iil[i])));
+    } else if (!dyn) {
+        m3 z;
+        z.c0 = splat(0.0f); z.c1 = splat(0.0f); z.c2 = splat(0.0f);
+        store_iiw(d.b_iiw, i, z);
+    }
+}
+
+inline uint32_t blocks(uint32_t n) { return (n + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st) {
+    if (!n) return;
+    uint32_t g = uint32_t(std::min<size_t>((n + kBlock - 1) / kBlock, 4096));
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, st, p, v, n);
+}
+void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st) {
+    if (!n) return;
+    uint32_t g = uint32_t(std::min<size_t>((n + kBlock - 1) / kBlock, 4096));
+    hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(kBlock), 0, st, p, v, n);
+}
+// exclusive scan of n <= 1M elements; scratch holds >= 1024 words
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* scratch, hipStream_t st) {
+    uint32_t nb = (n + 1023) / 1024;
+    if (nb == 0) nb = 1;
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, st, in, n, scratch);
+    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kBlock), 0, st, scratch, nb, total);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, st, in, out, n, scratch);
+}
+
+void launch_init_bodies(const DevState& d, hipStream_t st) {
+    if (d.nb) hipLaunchKernelGGL(k_init_bodies, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+}
+void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st) {
+    if (d.ns) hipLaunchKernelGGL(k_init_proxies, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d, p);
+}
+void launch_integrate_forces(const DevState& d, const StepParams& p, hipStream_t st) {
+    hipLaunchKernelGGL(k_integrate_forces, dim3(blocks(d.nb ? d.nb : 1)), dim3(kBlock), 0, st, d, p);
+}
+// bounds, grid build, per-emitter pair counts and their scan (pair_off; total in cnt->pairs)
+void launch_broadphase_count(const DevState& d, const StepParams& p, hipStream_t st) {
+    if (!d.ns) return;
+    hipLaunchKernelGGL(k_bounds, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d, p);
+    hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(1), 0, st, d);
+    hipLaunchKernelGGL(k_grid_clear, dim3(256), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
+    launch_scan(d.cell_count, d.cell_start, kMaxCells, nullptr, d.scan_tmp, st);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_pairs<0>, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
+    launch_scan(d.pair_count, d.pair_off, d.ns, &d.cnt->pairs, d.scan_tmp, st);
+}
+void launch_broadphase_write(const DevState& d, const StepParams& p, hipStream_t st) {
+    (void)p;
+    if (d.ns) hipLaunchKernelGGL(k_pairs<1>, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
+}
+void launch_narrowphase(const DevState& d, const StepParams& p, uint32_t n_pairs, hipStream_t st) {
+    if (!n_pairs) return;
+    hipLaunchKernelGGL(k_narrowphase, dim3(blocks(n_pairs)), dim3(kBlock), 0, st, d, p, n_pairs);
+    launch_scan(d.pm_valid, d.pm_off, n_pairs, nullptr, d.scan_tmp, st);
+    hipLaunchKernelGGL(k_compact, dim3(blocks(n_pairs)), dim3(kBlock), 0, st, d, n_pairs);
+}
+void launch_colour_init(const DevState& d, uint32_t n_man, hipStream_t st) {
+    launch_fill_u64(d.col_best, 0ull, d.nb, st);
+    launch_fill_u64(d.col_mask, 0ull, d.nb, st);
+    if (n_man) hipLaunchKernelGGL(k_colour_init, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
+}
+void launch_colour_rounds(const DevState& d, uint32_t n_man, uint32_t first_round, uint32_t rounds, hipStream_t st) {
+    if (!n_man) return;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL(k_colour_propose, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man, first_round + r);
+        hipLaunchKernelGGL(k_colour_select, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man, first_round + r);
+    }
+}
+void launch_build_order(const DevState& d, uint32_t n_man, hipStream_t st) {
+    uint32_t nblk = blocks(n_man ? n_man : 1);
+    if (n_man) hipLaunchKernelGGL(k_colour_hist, dim3(nblk), dim3(kBlock), 0, st, d, n_man, nblk);
+    else launch_fill_u32(d.col_table, 0u, size_t(kBuckets) * nblk, st);
+    // col_table_off doubles as scan scratch elsewhere: scan into it, block sums in col_table's tail
+    uint32_t n = kBuckets * nblk;
+    uint32_t* scratch = d.col_table + n;
+    uint32_t* total = scratch + 1024;
+    launch_scan(d.col_table, d.col_table_off, n, total, scratch, st);
+    if (n_man) hipLaunchKernelGGL(k_colour_scatter, dim3(nblk), dim3(kBlock), 0, st, d, n_man, nblk);
+    hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(128), 0, st, d, nblk, total);
+}
+void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t n_ordered, hipStream_t st) {
+    if (n_ordered) hipLaunchKernelGGL(k_solver_setup, dim3(blocks(n_ordered)), dim3(kBlock), 0, st, d, p, n_ordered);
+    if (d.nb) hipLaunchKernelGGL(k_solver_bodies, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+}
+void launch_warm_apply(const DevState& d, uint32_t begin, uint32_t end, hipStream_t st) {
+    if (end > begin) hipLaunchKernelGGL(k_warm_apply, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, begin, end);
+}
+void launch_velocity(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, int use_rest, uint32_t offset,
+                     hipStream_t st) {
+    if (end > begin)
+        hipLaunchKernelGGL(k_velocity, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, p, begin, end, use_rest, offset);
+}
+void launch_residual(const DevState& d, const StepParams& p, uint32_t n_active, int use_rest, hipStream_t st) {
+    hipLaunchKernelGGL(k_residual_reset, dim3(1), dim3(1), 0, st, d);
+    if (n_active) hipLaunchKernelGGL(k_residual, dim3(blocks(n_active)), dim3(kBlock), 0, st, d, p, n_active, use_rest);
+}
+void launch_position_setup(const DevState& d, hipStream_t st) {
+    if (d.nb) hipLaunchKernelGGL(k_position_setup, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+}
+void launch_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st) {
+    if (end > begin) hipLaunchKernelGGL(k_position, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, p, begin, end);
+}
+void launch_solver_finish(const DevState& d, uint32_t n_ordered, hipStream_t st) {
+    if (d.nb) hipLaunchKernelGGL(k_bodies_back, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+    size_t cap = size_t(d.cache_mask) + 1;
+    launch_fill_u64(d.ck_keys[d.cur], kEmptyKey, cap, st);
+    launch_fill_u32(d.ck_ord[d.cur], 0u, cap, st);
+    if (n_ordered) {
+        hipLaunchKernelGGL(k_cache_insert, dim3(blocks(n_ordered)), dim3(kBlock), 0, st, d, n_ordered);
+        hipLaunchKernelGGL(k_cache_write, dim3(blocks(n_ordered)), dim3(kBlock), 0, st, d, n_ordered);
+    }
+}
+void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t n_man, hipStream_t st) {
+    if (!d.nb) return;
+    hipLaunchKernelGGL(k_uf_init, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+    if (n_man) hipLaunchKernelGGL(k_uf_union, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
+    hipLaunchKernelGGL(k_uf_compress, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_sleep_timers, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
+    if (n_man) hipLaunchKernelGGL(k_kin_wake, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, p, n_man);
+    hipLaunchKernelGGL(k_island_min, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_sleep, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
+}
+void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st) {
+    if (d.nb) hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
+}
+
+}  // namespace pxg

@@ -1,0 +1,155 @@
+// Device memory layout of one world and the launchers of its step kernels.
+//
+// Bodies and shapes are stored structure-of-arrays with 16-byte records so that a wavefront's
+// loads of one field coalesce; the solver works on compact per-step copies laid out in solve
+// order (colour by colour), so each colour's manifolds are one contiguous range.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pxg {
+
+constexpr int kBlock = 256;
+constexpr int kMaxColours = 64;      // colours per step (64-bit per-body colour masks)
+constexpr int kBuckets = kMaxColours + 1;  // colours + the "skipped by the solver" group
+constexpr int kMaxLarge = 256;       // shapes on the brute-force broadphase path
+constexpr uint32_t kMaxCells = 1u << 18;
+
+// body flag bits
+constexpr uint32_t BF_TYPE = 3u, BF_ALIVE = 1u << 2, BF_AWAKE = 1u << 3, BF_GRAVITY = 1u << 4;
+constexpr int BF_CONSTRAINT_SHIFT = 8;
+// shape info bits (s_info.y)
+constexpr uint32_t SF_ENABLED = 1u << 4, SF_TRIGGER = 1u << 5, SF_STATIC = 1u << 6;
+
+// error bits reported through DevState::err
+constexpr uint32_t ERR_PAIRS = 1u, ERR_LARGE = 2u, ERR_COLOURS = 4u, ERR_MANIFOLDS = 8u, ERR_CACHE = 16u,
+                   ERR_CELLS = 32u;
+
+struct Counters {  // device-side counters, read back by the host at the step's sync points
+    uint32_t pairs;
+    uint32_t manifolds;
+    uint32_t points;
+    uint32_t active;        // manifolds the solver runs
+    uint32_t ordered;       // active + skipped non-trigger manifolds (cache rebuild list)
+    uint32_t remaining;     // colouring: active manifolds not yet coloured
+    uint32_t colours;
+    uint32_t n_large;
+    uint32_t err;
+    uint32_t awake;
+    uint32_t predicted;
+    uint32_t residual_bits; // float bits of the last velocity residual (max, >= 0)
+    uint32_t red[8];        // broadphase reductions (sortable-float encoded)
+    uint32_t colour_off[kBuckets + 1];
+    float grid[8];          // ox, oy, inv_cell, ext_y, nx, ny (as floats)
+    unsigned long long prof_vel_bytes;  // algorithmic bytes of profiled velocity passes
+};
+
+struct DevState {
+    uint32_t nb, ns;              // bodies, shapes in use
+    uint32_t cap_pairs, cap_manifolds, cache_mask;
+    // bodies
+    float4 *b_pos;      // position, sleep_timer
+    float4 *b_rot;      // x, y, z, w
+    float4 *b_vel;      // linear velocity, inv_mass
+    float4 *b_angv;     // angular velocity, mass
+    float4 *b_force;    // force accumulator, linear drag
+    float4 *b_torque;   // torque accumulator, angular drag
+    float4 *b_iil;      // inverse inertia (local diagonal), -
+    float4 *b_iiw;      // 3 records per body: world inverse inertia, column-major 9 floats + pad
+    float4 *b_prev_pos, *b_prev_rot;
+    uint32_t *b_flags;
+    // shapes
+    uint4 *s_info;      // owner, type | flags | caxis << 8, layer, fcomb | rcomb << 8
+    float4 *s_lc;       // local centre, sphere radius
+    float4 *s_lr;       // local rotation
+    float4 *s_dim;      // box half extents, capsule radius
+    float4 *s_mat;      // capsule half height, dynamic friction, restitution, static friction
+    float4 *s_fmin, *s_fmax;   // fat AABB (persistent broadphase state)
+    float4 *s_tmin, *s_tmax;   // tight AABB of this step; w of s_tmin: 1 = large shape
+    uint32_t *layer_mask;      // 32 rows
+    // broadphase
+    uint32_t *cell_count, *cell_start, *cell_cursor, *cell_items;
+    uint32_t *large_list;
+    uint32_t *pair_count, *pair_off;   // per shape (emitter)
+    uint2 *pairs;                      // (shape a, shape b), a < b
+    // pair-indexed narrowphase output
+    uint32_t *pm_valid, *pm_off;
+    uint2 *pm_ab;
+    float4 *pm_nf;
+    uint4 *pm_meta;
+    float4 *pm_p, *pm_q;
+    uint32_t *scan_tmp;   // >= 2048 words of scan scratch
+    // manifolds, discovery order (dense)
+    uint2 *m_ab;        // body a, body b
+    uint2 *m_shapes;    // shape a, shape b
+    float4 *m_nf;       // normal, friction
+    uint4 *m_meta;      // count | trigger << 8, restitution bits, active, -
+    float4 *m_p;        // 4 per manifold: position, penetration
+    float4 *m_q;        // 4 per manifold: separation, feature id bits, -, -
+    // colouring
+    uint32_t *m_colour;
+    unsigned long long *col_best;   // per body
+    unsigned long long *col_mask;   // per body
+    uint32_t *col_table, *col_table_off;
+    uint32_t *order;                // solve position -> manifold index
+    // solver arrays, solve order
+    uint2 *sm_ab;
+    float4 *sm_nf;
+    uint32_t *sm_cnt;
+    float4 *sp_pos;     // position, penetration
+    float4 *sp_aux;     // separation, restitution bias, feature id bits, miss flag
+    float4 *sp_imp;     // normal, tangent1, tangent2 impulses
+    // solver bodies
+    float4 *sb_v;       // linear velocity, effective inverse mass
+    float4 *sb_w;       // angular velocity, active (1/0)
+    float4 *sb_I;       // 3 per body: effective world inverse inertia (9) + position (3)
+    float4 *sb_x;       // position pass: position, effective inverse mass
+    float4 *sb_ref;     // position pass reference positions
+    // warm-start cache: two open-addressing tables, swapped every step
+    unsigned long long *ck_keys[2];
+    uint32_t *ck_ord[2];
+    float4 *ck_val[2];
+    int cur;            // table written this step (the other is read)
+    // islands
+    uint32_t *uf_parent;
+    uint32_t *isl_min;
+    Counters *cnt;
+    // learned warm start
+    const void *mlp;    // packed weights (nullptr = off)
+};
+
+struct StepParams {
+    float h;
+    float gx, gy, gz;
+    float margin, large_extent;
+    float pos_corr, slop, max_corr, rest_thresh;
+    float sleep_lin, sleep_ang, sleep_time, max_lin_vel;
+    uint32_t exact_quat;
+    uint32_t profile;   // 1: velocity passes count their algorithmic bytes
+};
+
+// launchers (pxg_kernels.hip); all asynchronous on `st`
+void launch_integrate_forces(const DevState& d, const StepParams& p, hipStream_t st);
+void launch_broadphase_count(const DevState& d, const StepParams& p, hipStream_t st);
+void launch_broadphase_write(const DevState& d, const StepParams& p, hipStream_t st);
+void launch_narrowphase(const DevState& d, const StepParams& p, uint32_t n_pairs, hipStream_t st);
+void launch_colour_init(const DevState& d, uint32_t n_manifolds, hipStream_t st);
+void launch_colour_rounds(const DevState& d, uint32_t n_manifolds, uint32_t first_round, uint32_t rounds, hipStream_t st);
+void launch_build_order(const DevState& d, uint32_t n_manifolds, hipStream_t st);
+void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t n_ordered, hipStream_t st);
+void launch_warm_apply(const DevState& d, uint32_t begin, uint32_t end, hipStream_t st);
+void launch_velocity(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, int use_rest, uint32_t offset,
+                     hipStream_t st);
+void launch_residual(const DevState& d, const StepParams& p, uint32_t n_active, int use_rest, hipStream_t st);
+void launch_position_setup(const DevState& d, hipStream_t st);
+void launch_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st);
+void launch_solver_finish(const DevState& d, uint32_t n_ordered, hipStream_t st);
+void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t n_manifolds, hipStream_t st);
+void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st);
+void launch_init_bodies(const DevState& d, hipStream_t st);
+void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st);
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
+void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
+void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* scratch, hipStream_t st);
+
+}  // namespace pxg

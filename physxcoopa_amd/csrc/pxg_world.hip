@@ -1,0 +1,814 @@
+// C ABI of the MI355X contact step (include/pxg.h): world handles, device buffers, uploads,
+// downloads and the host-side orchestration of one fixed step. The step runs on the handle's own
+// HIP stream; the host waits on the device twice per step: once for the pair count (sizes the
+// narrowphase) and once for the colour table (sizes the per-colour solver launches).
+#include "../../include/pxg.h"
+#include "pxg_state.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace pxg;
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+    g_last_error = "[pxg] " + msg;
+    return code;
+}
+#define HIPCHK(call)                                                                                     \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) return fail(PXG_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct pxg_world {
+    pxg_config cfg;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevState d{};
+    Counters* h_cnt = nullptr;  // pinned
+    std::vector<void*> allocs;
+    uint32_t cap_b = 0, cap_s = 0, cap_p = 0, cap_m = 0;
+    uint32_t n_bodies = 0, n_shapes = 0;
+    std::vector<uint32_t> body_types;
+    uint32_t layer_masks[32];
+    float accumulator = 0.0f;
+    int phase = 0;              // 0: next call collides, 1: collision done, solve pending
+    uint32_t last_pairs = 0, last_manifolds = 0, last_ordered = 0, last_active = 0, last_colours = 0;
+    uint32_t colour_off[kBuckets + 1];
+    pxg_stats stats{};
+    uint32_t colour_round = 0;
+    bool shapes_ready = false;
+    // profiling: HIP event pairs around every velocity-pass launch
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+    double prof_vel_ms = 0.0;
+    uint64_t prof_vel_launches = 0;
+};
+
+extern "C" void pxg_config_default(pxg_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->velocity_iterations = 16;
+    c->relax_iterations = 2;
+    c->position_iterations = 4;
+    c->position_correction = 0.2f;
+    c->linear_slop = 0.005f;
+    c->max_linear_correction = 0.2f;
+    c->restitution_threshold = 0.5f;
+    c->sleep_linear = 0.01f;
+    c->sleep_angular = 0.02f;
+    c->sleep_time = 0.5f;
+    c->aabb_margin = 0.1f;
+    c->max_substeps = 8;
+    c->fixed_dt = 1.0f / 60.0f;
+    c->max_linear_velocity = 200.0f;
+    c->gravity[2] = -9.81f;
+    c->residual_tolerance = 1e-4f;
+    c->residual_max_iterations = 256;
+    c->large_extent = 16.0f;
+}
+
+extern "C" const char* pxg_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char* pxg_version(void) { return "pxg 0.1.0 gfx950"; }
+
+template <class T>
+static int dalloc(pxg_world* w, T*& p, size_t count) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(PXG_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    w->allocs.push_back(q);
+    p = static_cast<T*>(q);
+    return PXG_OK;
+}
+#define DALLOC(ptr, n)                        \
+    do {                                      \
+        int r_ = dalloc(w, (ptr), (n));       \
+        if (r_ != PXG_OK) return r_;          \
+    } while (0)
+
+static void free_all(pxg_world* w) {
+    for (void* p : w->allocs) (void)hipFree(p);
+    w->allocs.clear();
+}
+
+static StepParams params(const pxg_world* w, float h) {
+    StepParams p;
+    const pxg_config& c = w->cfg;
+    p.h = h;
+    p.gx = c.gravity[0]; p.gy = c.gravity[1]; p.gz = c.gravity[2];
+    p.margin = c.aabb_margin;
+    p.large_extent = c.large_extent > 0.0f ? c.large_extent : 16.0f;
+    p.pos_corr = c.position_correction;
+    p.slop = c.linear_slop;
+    p.max_corr = c.max_linear_correction;
+    p.rest_thresh = c.restitution_threshold;
+    p.sleep_lin = c.sleep_linear;
+    p.sleep_ang = c.sleep_angular;
+    p.sleep_time = c.sleep_time;
+    p.max_lin_vel = c.max_linear_velocity;
+    p.exact_quat = c.use_exact_quaternion_integration;
+    p.profile = w->profiling ? 1u : 0u;
+    return p;
+}
+
+static hipEvent_t take_event(pxg_world* w) {
+    if (!w->ev_pool.empty()) {
+        hipEvent_t e = w->ev_pool.back();
+        w->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// one velocity-pass launch (one colour), bracketed by events when profiling
+static void velocity_launch(pxg_world* w, const StepParams& p, uint32_t b, uint32_t e, int rest, uint32_t off) {
+    if (e <= b) return;
+    if (!w->profiling) {
+        launch_velocity(w->d, p, b, e, rest, off, w->stream);
+        return;
+    }
+    hipEvent_t e0 = take_event(w), e1 = take_event(w);
+    (void)hipEventRecord(e0, w->stream);
+    launch_velocity(w->d, p, b, e, rest, off, w->stream);
+    (void)hipEventRecord(e1, w->stream);
+    w->ev_pending.push_back({e0, e1});
+}
+
+extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
+    if (!out) return fail(PXG_ERR_INVALID, "pxg_create: null output");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PXG_ERR_HIP, "pxg_create: no HIP device");
+    pxg_world* w = new pxg_world();
+    if (cfg) w->cfg = *cfg;
+    else pxg_config_default(&w->cfg);
+    for (int i = 0; i < 32; ++i) w->layer_masks[i] = 0xFFFFFFFFu;
+    (void)hipGetDevice(&w->device);
+    if (hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete w;
+        return fail(PXG_ERR_HIP, "pxg_create: hipStreamCreate failed");
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&w->h_cnt), sizeof(Counters), hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(w->stream);
+        delete w;
+        return fail(PXG_ERR_HIP, "pxg_create: hipHostMalloc failed");
+    }
+    std::memset(w->h_cnt, 0, sizeof(Counters));
+    *out = w;
+    return PXG_OK;
+}
+
+extern "C" int pxg_destroy(pxg_world* w) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_destroy: null handle");
+    (void)hipStreamSynchronize(w->stream);
+    for (auto& pr : w->ev_pending) { w->ev_pool.push_back(pr.first); w->ev_pool.push_back(pr.second); }
+    for (hipEvent_t e : w->ev_pool) (void)hipEventDestroy(e);
+    free_all(w);
+    if (w->h_cnt) (void)hipHostFree(w->h_cnt);
+    (void)hipStreamDestroy(w->stream);
+    delete w;
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_config(pxg_world* w, pxg_config* out) {
+    if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_get_config: null argument");
+    *out = w->cfg;
+    return PXG_OK;
+}
+extern "C" int pxg_set_config(pxg_world* w, const pxg_config* cfg) {
+    if (!w || !cfg) return fail(PXG_ERR_INVALID, "pxg_set_config: null argument");
+    pxg_config c = *cfg;
+    c.max_bodies = w->cfg.max_bodies;
+    c.max_shapes = w->cfg.max_shapes;
+    c.max_pairs = w->cfg.max_pairs;
+    c.max_manifolds = w->cfg.max_manifolds;
+    w->cfg = c;
+    return PXG_OK;
+}
+extern "C" int pxg_set_gravity(pxg_world* w, const float g[3]) {
+    if (!w || !g) return fail(PXG_ERR_INVALID, "pxg_set_gravity: null argument");
+    for (int i = 0; i < 3; ++i) w->cfg.gravity[i] = g[i];
+    return PXG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// allocation
+// ---------------------------------------------------------------------------------------------
+static int alloc_bodies(pxg_world* w, uint32_t n) {
+    uint32_t cap = std::max(n, w->cfg.max_bodies);
+    if (w->cap_b) {
+        if (n > w->cap_b) return fail(PXG_ERR_CAPACITY, "pxg_set_bodies: more bodies than the world's capacity");
+        return PXG_OK;
+    }
+    w->cap_b = cap;
+    DevState& d = w->d;
+    DALLOC(d.b_pos, cap); DALLOC(d.b_rot, cap); DALLOC(d.b_vel, cap); DALLOC(d.b_angv, cap);
+    DALLOC(d.b_force, cap); DALLOC(d.b_torque, cap); DALLOC(d.b_iil, cap); DALLOC(d.b_iiw, 3 * size_t(cap));
+    DALLOC(d.b_prev_pos, cap); DALLOC(d.b_prev_rot, cap); DALLOC(d.b_flags, cap);
+    DALLOC(d.col_best, cap); DALLOC(d.col_mask, cap);
+    DALLOC(d.sb_v, cap); DALLOC(d.sb_w, cap); DALLOC(d.sb_I, 3 * size_t(cap)); DALLOC(d.sb_x, cap); DALLOC(d.sb_ref, cap);
+    DALLOC(d.uf_parent, cap); DALLOC(d.isl_min, cap);
+    DALLOC(d.cnt, 1);
+    DALLOC(d.layer_mask, 32);
+    DALLOC(d.scan_tmp, 4096);
+    HIPCHK(hipMemsetAsync(d.cnt, 0, sizeof(Counters), w->stream));
+    return PXG_OK;
+}
+
+static uint32_t next_pow2(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return uint32_t(p);
+}
+
+static int alloc_shapes(pxg_world* w, uint32_t n) {
+    if (w->cap_s) {
+        if (n > w->cap_s) return fail(PXG_ERR_CAPACITY, "pxg_set_shapes: more shapes than the world's capacity");
+        return PXG_OK;
+    }
+    uint32_t cap = std::max(n, w->cfg.max_shapes);
+    w->cap_s = cap;
+    w->cap_p = w->cfg.max_pairs ? w->cfg.max_pairs : 24u * cap + 4096u;
+    w->cap_m = w->cfg.max_manifolds ? w->cfg.max_manifolds : std::min(w->cap_p, 12u * cap + 4096u);
+    if (w->cap_p > (1u << 20)) w->cap_p = 1u << 20;  // scan limit
+    if (w->cap_m > w->cap_p) w->cap_m = w->cap_p;
+    DevState& d = w->d;
+    d.cap_pairs = w->cap_p;
+    d.cap_manifolds = w->cap_m;
+    DALLOC(d.s_info, cap); DALLOC(d.s_lc, cap); DALLOC(d.s_lr, cap); DALLOC(d.s_dim, cap); DALLOC(d.s_mat, cap);
+    DALLOC(d.s_fmin, cap); DALLOC(d.s_fmax, cap); DALLOC(d.s_tmin, cap); DALLOC(d.s_tmax, cap);
+    DALLOC(d.cell_count, kMaxCells); DALLOC(d.cell_start, kMaxCells); DALLOC(d.cell_cursor, kMaxCells);
+    DALLOC(d.cell_items, cap); DALLOC(d.large_list, kMaxLarge);
+    DALLOC(d.pair_count, cap); DALLOC(d.pair_off, cap);
+    size_t P = w->cap_p, M = w->cap_m;
+    DALLOC(d.pairs, P); DALLOC(d.pm_valid, P); DALLOC(d.pm_off, P);
+    DALLOC(d.pm_ab, P); DALLOC(d.pm_nf, P); DALLOC(d.pm_meta, P); DALLOC(d.pm_p, 4 * P); DALLOC(d.pm_q, 4 * P);
+    DALLOC(d.m_ab, M); DALLOC(d.m_shapes, M); DALLOC(d.m_nf, M); DALLOC(d.m_meta, M); DALLOC(d.m_p, 4 * M);
+    DALLOC(d.m_q, 4 * M); DALLOC(d.m_colour, M);
+    size_t nblk = (P + kBlock - 1) / kBlock;
+    DALLOC(d.col_table, size_t(kBuckets) * nblk + 2048); DALLOC(d.col_table_off, size_t(kBuckets) * nblk + 16);
+    DALLOC(d.order, M);
+    DALLOC(d.sm_ab, M); DALLOC(d.sm_nf, M); DALLOC(d.sm_cnt, M);
+    DALLOC(d.sp_pos, 4 * M); DALLOC(d.sp_aux, 4 * M); DALLOC(d.sp_imp, 4 * M);
+    uint32_t cache_cap = next_pow2(8ull * M);
+    d.cache_mask = cache_cap - 1;
+    for (int t = 0; t < 2; ++t) {
+        DALLOC(d.ck_keys[t], cache_cap); DALLOC(d.ck_ord[t], cache_cap); DALLOC(d.ck_val[t], cache_cap);
+        launch_fill_u64(d.ck_keys[t], ~0ull, cache_cap, w->stream);
+        launch_fill_u32(d.ck_ord[t], 0u, cache_cap, w->stream);
+    }
+    d.cur = 0;
+    return PXG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// uploads and downloads
+// ---------------------------------------------------------------------------------------------
+extern "C" int pxg_set_bodies(pxg_world* w, const pxg_body* bodies, uint32_t n) {
+    if (!w || (!bodies && n)) return fail(PXG_ERR_INVALID, "pxg_set_bodies: null argument");
+    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_bodies: not allowed between the two step phases");
+    int r = alloc_bodies(w, n);
+    if (r != PXG_OK) return r;
+    std::vector<float4> pos(n), rot(n), vel(n), angv(n), force(n), torque(n), iil(n), ppos(n), prot(n);
+    std::vector<uint32_t> flags(n);
+    w->body_types.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const pxg_body& b = bodies[i];
+        pos[i] = make_float4(b.position[0], b.position[1], b.position[2], b.sleep_timer);
+        rot[i] = make_float4(b.orientation[0], b.orientation[1], b.orientation[2], b.orientation[3]);
+        vel[i] = make_float4(b.linear_velocity[0], b.linear_velocity[1], b.linear_velocity[2], b.inv_mass);
+        angv[i] = make_float4(b.angular_velocity[0], b.angular_velocity[1], b.angular_velocity[2], b.mass);
+        force[i] = make_float4(b.force[0], b.force[1], b.force[2], b.linear_drag);
+        torque[i] = make_float4(b.torque[0], b.torque[1], b.torque[2], b.angular_drag);
+        iil[i] = make_float4(b.inv_inertia_local[0], b.inv_inertia_local[1], b.inv_inertia_local[2], 0.0f);
+        ppos[i] = make_float4(b.prev_position[0], b.prev_position[1], b.prev_position[2], 0.0f);
+        prot[i] = make_float4(b.prev_orientation[0], b.prev_orientation[1], b.prev_orientation[2], b.prev_orientation[3]);
+        if (b.type > 2) return fail(PXG_ERR_INVALID, "pxg_set_bodies: bad body type");
+        flags[i] = (b.type & BF_TYPE) | (b.alive ? BF_ALIVE : 0u) | (b.awake ? BF_AWAKE : 0u) |
+                   (b.use_gravity ? BF_GRAVITY : 0u) | ((b.constraints & 63u) << BF_CONSTRAINT_SHIFT);
+        w->body_types[i] = b.type;
+    }
+    DevState& d = w->d;
+    hipStream_t s = w->stream;
+    if (n) {
+        HIPCHK(hipMemcpyAsync(d.b_pos, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_rot, rot.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_vel, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_angv, angv.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_force, force.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_torque, torque.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_iil, iil.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_prev_pos, ppos.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_prev_rot, prot.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_flags, flags.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    }
+    d.nb = n;
+    w->n_bodies = n;
+    launch_init_bodies(d, s);
+    HIPCHK(hipStreamSynchronize(s));  // host staging buffers go out of scope
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_bodies(pxg_world* w, pxg_body* out, uint32_t n) {
+    if (!w || (!out && n)) return fail(PXG_ERR_INVALID, "pxg_get_bodies: null argument");
+    if (n > w->n_bodies) return fail(PXG_ERR_INVALID, "pxg_get_bodies: n exceeds the body count");
+    if (!n) return PXG_OK;
+    std::vector<float4> pos(n), rot(n), vel(n), angv(n), force(n), torque(n), iil(n), iiw(3 * n), ppos(n), prot(n);
+    std::vector<uint32_t> flags(n);
+    DevState& d = w->d;
+    hipStream_t s = w->stream;
+    HIPCHK(hipMemcpyAsync(pos.data(), d.b_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(rot.data(), d.b_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(vel.data(), d.b_vel, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(angv.data(), d.b_angv, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(force.data(), d.b_force, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(torque.data(), d.b_torque, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(iil.data(), d.b_iil, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(iiw.data(), d.b_iiw, 3 * n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ppos.data(), d.b_prev_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(prot.data(), d.b_prev_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(flags.data(), d.b_flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < n; ++i) {
+        pxg_body& b = out[i];
+        std::memset(&b, 0, sizeof(b));
+        uint32_t f = flags[i];
+        b.type = f & BF_TYPE;
+        b.alive = (f & BF_ALIVE) ? 1u : 0u;
+        b.awake = (f & BF_AWAKE) ? 1u : 0u;
+        b.use_gravity = (f & BF_GRAVITY) ? 1u : 0u;
+        b.constraints = (f >> BF_CONSTRAINT_SHIFT) & 63u;
+        b.position[0] = pos[i].x; b.position[1] = pos[i].y; b.position[2] = pos[i].z; b.sleep_timer = pos[i].w;
+        b.orientation[0] = rot[i].x; b.orientation[1] = rot[i].y; b.orientation[2] = rot[i].z; b.orientation[3] = rot[i].w;
+        b.linear_velocity[0] = vel[i].x; b.linear_velocity[1] = vel[i].y; b.linear_velocity[2] = vel[i].z; b.inv_mass = vel[i].w;
+        b.angular_velocity[0] = angv[i].x; b.angular_velocity[1] = angv[i].y; b.angular_velocity[2] = angv[i].z; b.mass = angv[i].w;
+        b.force[0] = force[i].x; b.force[1] = force[i].y; b.force[2] = force[i].z; b.linear_drag = force[i].w;
+        b.torque[0] = torque[i].x; b.torque[1] = torque[i].y; b.torque[2] = torque[i].z; b.angular_drag = torque[i].w;
+        b.inv_inertia_local[0] = iil[i].x; b.inv_inertia_local[1] = iil[i].y; b.inv_inertia_local[2] = iil[i].z;
+        const float* m = reinterpret_cast<const float*>(&iiw[3 * i]);
+        for (int k = 0; k < 9; ++k) b.inv_inertia_world[k] = m[k];
+        b.prev_position[0] = ppos[i].x; b.prev_position[1] = ppos[i].y; b.prev_position[2] = ppos[i].z;
+        b.prev_orientation[0] = prot[i].x; b.prev_orientation[1] = prot[i].y; b.prev_orientation[2] = prot[i].z;
+        b.prev_orientation[3] = prot[i].w;
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n) {
+    if (!w || (!shapes && n)) return fail(PXG_ERR_INVALID, "pxg_set_shapes: null argument");
+    if (!w->cap_b) return fail(PXG_ERR_STATE, "pxg_set_shapes: upload bodies first");
+    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_shapes: not allowed between the two step phases");
+    int r = alloc_shapes(w, n);
+    if (r != PXG_OK) return r;
+    std::vector<uint4> info(n);
+    std::vector<float4> lc(n), lr(n), dim(n), mat(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const pxg_shape& s = shapes[i];
+        if (s.owner >= w->n_bodies) return fail(PXG_ERR_INVALID, "pxg_set_shapes: owner out of range");
+        if (s.type > 2) return fail(PXG_ERR_INVALID, "pxg_set_shapes: only sphere, box and capsule are supported");
+        if (s.capsule_axis < 0 || s.capsule_axis > 2) return fail(PXG_ERR_INVALID, "pxg_set_shapes: bad capsule axis");
+        uint32_t flags = s.type | (s.enabled ? SF_ENABLED : 0u) | (s.is_trigger ? SF_TRIGGER : 0u) |
+                         (w->body_types[s.owner] == PXG_STATIC ? SF_STATIC : 0u) | (uint32_t(s.capsule_axis) << 8);
+        info[i] = make_uint4(s.owner, flags, s.layer & 31u, (s.friction_combine & 255u) | ((s.restitution_combine & 255u) << 8));
+        lc[i] = make_float4(s.local_center[0], s.local_center[1], s.local_center[2], s.radius);
+        lr[i] = make_float4(s.local_rotation[0], s.local_rotation[1], s.local_rotation[2], s.local_rotation[3]);
+        dim[i] = make_float4(s.half_extents[0], s.half_extents[1], s.half_extents[2], s.capsule_radius);
+        mat[i] = make_float4(s.capsule_half_height, s.dynamic_friction, s.restitution, s.static_friction);
+    }
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    if (n) {
+        HIPCHK(hipMemcpyAsync(d.s_info, info.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_lc, lc.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_lr, lr.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_dim, dim.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_mat, mat.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipMemcpyAsync(d.layer_mask, w->layer_masks, sizeof(w->layer_masks), hipMemcpyHostToDevice, st));
+    d.ns = n;
+    w->n_shapes = n;
+    launch_init_proxies(d, params(w, w->cfg.fixed_dt), st);
+    HIPCHK(hipStreamSynchronize(st));
+    w->shapes_ready = true;
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_layer_masks(pxg_world* w, const uint32_t masks[32]) {
+    if (!w || !masks) return fail(PXG_ERR_INVALID, "pxg_set_layer_masks: null argument");
+    std::memcpy(w->layer_masks, masks, sizeof(w->layer_masks));
+    if (w->d.layer_mask) {
+        HIPCHK(hipMemcpyAsync(w->d.layer_mask, w->layer_masks, sizeof(w->layer_masks), hipMemcpyHostToDevice, w->stream));
+        HIPCHK(hipStreamSynchronize(w->stream));
+    }
+    return PXG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// stepping
+// ---------------------------------------------------------------------------------------------
+static int read_counters(pxg_world* w) {
+    HIPCHK(hipMemcpyAsync(w->h_cnt, w->d.cnt, sizeof(Counters), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    uint32_t err = w->h_cnt->err;
+    if (err) {
+        uint32_t zero = 0;
+        HIPCHK(hipMemcpy(&w->d.cnt->err, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        std::string what;
+        if (err & ERR_PAIRS) what += " pair buffer full;";
+        if (err & ERR_LARGE) what += " too many large shapes;";
+        if (err & ERR_COLOURS) what += " more than 64 colours;";
+        if (err & ERR_MANIFOLDS) what += " manifold buffer full;";
+        if (err & ERR_CACHE) what += " warm-start table full;";
+        return fail(PXG_ERR_CAPACITY, "device reported:" + what);
+    }
+    return PXG_OK;
+}
+
+// gravity, force integration, broadphase, narrowphase (world.h:1278-1297)
+static int phase_collide(pxg_world* w, float h) {
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    StepParams p = params(w, h);
+    launch_integrate_forces(d, p, st);
+    launch_broadphase_count(d, p, st);
+    int r = read_counters(w);  // sync 1: pair count
+    if (r != PXG_OK) return r;
+    uint32_t P = w->h_cnt->pairs;
+    if (P > w->cap_p) return fail(PXG_ERR_CAPACITY, "pair buffer full (" + std::to_string(P) + " pairs)");
+    launch_broadphase_write(d, p, st);
+    launch_narrowphase(d, p, P, st);
+    if (P == 0) {
+        uint32_t zero = 0;
+        HIPCHK(hipMemcpyAsync(&d.cnt->manifolds, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    w->last_pairs = P;
+    return PXG_OK;
+}
+
+// colouring, solve (dynamics/solver.h:914-958), islands/sleep, integration (world.h:1301-1309)
+static int phase_solve(pxg_world* w, float h) {
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    StepParams p = params(w, h);
+    const pxg_config& c = w->cfg;
+    uint32_t P = w->last_pairs;
+    uint32_t bound = P ? P : 1;
+    d.cur ^= 1;
+    launch_colour_init(d, bound, st);
+    uint32_t round = 1;
+    launch_colour_rounds(d, bound, round, 24, st);
+    round += 24;
+    launch_build_order(d, bound, st);
+    int r = read_counters(w);  // sync 2: colour table
+    if (r != PXG_OK) return r;
+    for (int extra = 0; w->h_cnt->remaining != 0 && extra < 64; ++extra) {
+        launch_colour_rounds(d, bound, round, 16, st);
+        round += 16;
+        launch_build_order(d, bound, st);
+        r = read_counters(w);
+        if (r != PXG_OK) return r;
+    }
+    if (w->h_cnt->remaining != 0) return fail(PXG_ERR_DEVICE, "colouring did not converge");
+    const Counters& hc = *w->h_cnt;
+    uint32_t M = hc.manifolds, C = hc.colours, active = hc.active, ordered = hc.ordered;
+    std::memcpy(w->colour_off, hc.colour_off, sizeof(w->colour_off));
+    const uint32_t* co = w->colour_off;
+
+    launch_solver_setup(d, p, ordered, st);
+    for (uint32_t k = 0; k < C; ++k) launch_warm_apply(d, co[k], co[k + 1], st);
+    uint32_t passes = 0;
+    float residual = 0.0f;
+    if (c.residual_mode) {
+        uint32_t maxit = c.residual_max_iterations ? c.residual_max_iterations : 256;
+        while (passes < maxit) {
+            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, passes);
+            ++passes;
+            launch_residual(d, p, active, 0, st);
+            HIPCHK(hipMemcpyAsync(&w->h_cnt->residual_bits, &d.cnt->residual_bits, 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            uint32_t bits = w->h_cnt->residual_bits;
+            std::memcpy(&residual, &bits, 4);
+            if (residual <= c.residual_tolerance) break;
+        }
+    } else {
+        for (uint32_t it = 0; it < c.velocity_iterations; ++it)
+            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, it);
+        passes = c.velocity_iterations;
+    }
+    for (uint32_t it = 0; it < c.relax_iterations; ++it)
+        for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 1, it);
+    launch_position_setup(d, st);
+    for (uint32_t it = 0; it < c.position_iterations; ++it)
+        for (uint32_t k = 0; k < C; ++k) launch_position(d, p, co[k], co[k + 1], st);
+    launch_solver_finish(d, ordered, st);
+    launch_islands_sleep(d, p, M, st);
+    launch_integrate_velocities(d, p, st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(PXG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    w->last_manifolds = M;
+    w->last_ordered = ordered;
+    w->last_active = active;
+    w->last_colours = C;
+    pxg_stats& s = w->stats;
+    s.pairs = P;
+    s.manifolds = M;
+    s.points = hc.points;
+    s.colours = C;
+    s.active_manifolds = active;
+    s.velocity_passes = passes;
+    s.residual = residual;
+    s.steps += 1;
+    return PXG_OK;
+}
+
+extern "C" int pxg_step_phase(pxg_world* w, uint32_t phase, float h) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_step_phase: null handle");
+    if (!w->shapes_ready) return fail(PXG_ERR_STATE, "pxg_step_phase: upload bodies and shapes first");
+    if (!(h > 0.0f)) return fail(PXG_ERR_INVALID, "pxg_step_phase: h must be positive");
+    if (phase == PXG_PHASE_FORCES_COLLIDE) {
+        if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_step_phase: collision phase already run");
+        int r = phase_collide(w, h);
+        if (r != PXG_OK) return r;
+        w->phase = 1;
+        return PXG_OK;
+    }
+    if (phase == PXG_PHASE_SOLVE_INTEGRATE) {
+        if (w->phase != 1) return fail(PXG_ERR_STATE, "pxg_step_phase: run the collision phase first");
+        w->phase = 0;
+        return phase_solve(w, h);
+    }
+    return fail(PXG_ERR_INVALID, "pxg_step_phase: unknown phase");
+}
+
+extern "C" int pxg_step_fixed(pxg_world* w, float h) {
+    int r = pxg_step_phase(w, PXG_PHASE_FORCES_COLLIDE, h);
+    if (r != PXG_OK) return r;
+    return pxg_step_phase(w, PXG_PHASE_SOLVE_INTEGRATE, h);
+}
+
+// step(): accumulate the clamped frame time and run fixed substeps (world.h:1229-1240)
+extern "C" int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_step: null handle");
+    float frame = std::min(dt, 0.25f);
+    w->accumulator += frame;
+    float h = w->cfg.fixed_dt;
+    uint32_t n = 0;
+    while (w->accumulator >= h && n < w->cfg.max_substeps) {
+        int r = pxg_step_fixed(w, h);
+        if (r != PXG_OK) return r;
+        w->accumulator -= h;
+        ++n;
+    }
+    if (substeps_run) *substeps_run = n;
+    return PXG_OK;
+}
+
+extern "C" int pxg_synchronize(pxg_world* w) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_synchronize: null handle");
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_stats(pxg_world* w, pxg_stats* out) {
+    if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_get_stats: null argument");
+    HIPCHK(hipMemcpyAsync(w->h_cnt, w->d.cnt, sizeof(Counters), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    w->stats.awake_bodies = w->h_cnt->awake;
+    w->stats.learned_predictions = w->h_cnt->predicted;
+    *out = w->stats;
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_pairs(pxg_world* w, uint32_t* ab, uint32_t cap, uint32_t* n) {
+    if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_pairs: null argument");
+    *n = w->last_pairs;
+    uint32_t k = std::min(cap, w->last_pairs);
+    if (k && ab) {
+        HIPCHK(hipMemcpyAsync(ab, w->d.pairs, k * sizeof(uint2), hipMemcpyDeviceToHost, w->stream));
+        HIPCHK(hipStreamSynchronize(w->stream));
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_solve_order(pxg_world* w, uint32_t* order, uint32_t cap, uint32_t* n, uint32_t* colour_offsets,
+                                   uint32_t colour_cap, uint32_t* n_colours) {
+    if (!w || !n || !n_colours) return fail(PXG_ERR_INVALID, "pxg_get_solve_order: null argument");
+    *n = w->last_ordered;
+    *n_colours = w->last_colours;
+    uint32_t k = std::min(cap, w->last_ordered);
+    if (k && order) {
+        HIPCHK(hipMemcpyAsync(order, w->d.order, k * sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream));
+        HIPCHK(hipStreamSynchronize(w->stream));
+    }
+    if (colour_offsets) {
+        for (uint32_t c = 0; c <= w->last_colours && c < colour_cap; ++c) colour_offsets[c] = w->colour_off[c];
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_manifolds(pxg_world* w, pxg_manifold* out, uint32_t cap, uint32_t* n) {
+    if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_manifolds: null argument");
+    uint32_t M = w->last_manifolds, O = w->last_ordered;
+    *n = M;
+    if (!out || !cap || !M) return PXG_OK;
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    std::vector<uint2> ab(M), sh(M);
+    std::vector<float4> nf(M), p(4 * size_t(M)), q(4 * size_t(M)), aux(4 * size_t(O)), imp(4 * size_t(O));
+    std::vector<uint4> meta(M);
+    std::vector<uint32_t> order(O);
+    HIPCHK(hipMemcpyAsync(ab.data(), d.m_ab, M * sizeof(uint2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sh.data(), d.m_shapes, M * sizeof(uint2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nf.data(), d.m_nf, M * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(meta.data(), d.m_meta, M * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(p.data(), d.m_p, 4 * M * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(q.data(), d.m_q, 4 * M * sizeof(float4), hipMemcpyDeviceToHost, st));
+    if (O) {
+        HIPCHK(hipMemcpyAsync(order.data(), d.order, O * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(aux.data(), d.sp_aux, 4 * O * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(imp.data(), d.sp_imp, 4 * O * sizeof(float4), hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<int64_t> slot(M, -1);
+    for (uint32_t s = 0; s < O; ++s) slot[order[s]] = s;
+    uint32_t k = std::min(cap, M);
+    for (uint32_t m = 0; m < k; ++m) {
+        pxg_manifold& o = out[m];
+        std::memset(&o, 0, sizeof(o));
+        o.a = ab[m].x; o.b = ab[m].y; o.shape_a = sh[m].x; o.shape_b = sh[m].y;
+        o.normal[0] = nf[m].x; o.normal[1] = nf[m].y; o.normal[2] = nf[m].z; o.friction = nf[m].w;
+        std::memcpy(&o.restitution, &meta[m].y, 4);
+        o.count = meta[m].x & 255u;
+        o.is_trigger = (meta[m].x >> 8) & 1u;
+        o.valid = 1;
+        for (int t = 0; t < 4; ++t) {
+            pxg_contact& c = o.points[t];
+            const float4& pp = p[4 * m + t];
+            const float4& qq = q[4 * m + t];
+            c.position[0] = pp.x; c.position[1] = pp.y; c.position[2] = pp.z; c.penetration = pp.w;
+            c.separation = qq.x;
+            std::memcpy(&c.feature_id, &qq.y, 4);
+            if (slot[m] >= 0) {
+                const float4& a = aux[4 * slot[m] + t];
+                const float4& im = imp[4 * slot[m] + t];
+                c.restitution_bias = a.y;
+                c.normal_impulse = im.x; c.tangent_impulse[0] = im.y; c.tangent_impulse[1] = im.z;
+            }
+        }
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_fat_aabbs(pxg_world* w, float* out6, uint32_t n) {
+    if (!w || (!out6 && n)) return fail(PXG_ERR_INVALID, "pxg_get_fat_aabbs: null argument");
+    if (n > w->n_shapes) return fail(PXG_ERR_INVALID, "pxg_get_fat_aabbs: n exceeds the shape count");
+    if (!n) return PXG_OK;
+    std::vector<float4> mn(n), mx(n);
+    HIPCHK(hipMemcpyAsync(mn.data(), w->d.s_fmin, n * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipMemcpyAsync(mx.data(), w->d.s_fmax, n * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        out6[6 * i] = mn[i].x; out6[6 * i + 1] = mn[i].y; out6[6 * i + 2] = mn[i].z;
+        out6[6 * i + 3] = mx[i].x; out6[6 * i + 4] = mx[i].y; out6[6 * i + 5] = mx[i].z;
+    }
+    return PXG_OK;
+}
+extern "C" int pxg_set_fat_aabbs(pxg_world* w, const float* in6, uint32_t n) {
+    if (!w || (!in6 && n)) return fail(PXG_ERR_INVALID, "pxg_set_fat_aabbs: null argument");
+    if (n > w->n_shapes) return fail(PXG_ERR_INVALID, "pxg_set_fat_aabbs: n exceeds the shape count");
+    if (!n) return PXG_OK;
+    std::vector<float4> mn(n), mx(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        mn[i] = make_float4(in6[6 * i], in6[6 * i + 1], in6[6 * i + 2], 0.0f);
+        mx[i] = make_float4(in6[6 * i + 3], in6[6 * i + 4], in6[6 * i + 5], 0.0f);
+    }
+    HIPCHK(hipMemcpyAsync(w->d.s_fmin, mn.data(), n * sizeof(float4), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipMemcpyAsync(w->d.s_fmax, mx.data(), n * sizeof(float4), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_warm_cache(pxg_world* w, uint64_t* keys, float* imp3, uint32_t cap, uint32_t* n) {
+    if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_warm_cache: null argument");
+    *n = 0;
+    if (!w->d.ck_keys[0]) return PXG_OK;
+    size_t tc = size_t(w->d.cache_mask) + 1;
+    std::vector<unsigned long long> k(tc);
+    std::vector<float4> v(tc);
+    int t = w->d.cur;
+    HIPCHK(hipMemcpyAsync(k.data(), w->d.ck_keys[t], tc * 8, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipMemcpyAsync(v.data(), w->d.ck_val[t], tc * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    std::vector<std::pair<uint64_t, float4>> e;
+    for (size_t i = 0; i < tc; ++i)
+        if (k[i] != ~0ull) e.push_back({k[i], v[i]});
+    std::sort(e.begin(), e.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+    *n = uint32_t(e.size());
+    for (uint32_t i = 0; i < cap && i < e.size(); ++i) {
+        if (keys) keys[i] = e[i].first;
+        if (imp3) { imp3[3 * i] = e[i].second.x; imp3[3 * i + 1] = e[i].second.y; imp3[3 * i + 2] = e[i].second.z; }
+    }
+    return PXG_OK;
+}
+
+static uint32_t host_hash32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+static uint32_t host_hash64(uint64_t k) { return host_hash32(uint32_t(k) ^ host_hash32(uint32_t(k >> 32) + 0x9e3779b9u)); }
+
+extern "C" int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const float* imp3, uint32_t n) {
+    if (!w || ((!keys || !imp3) && n)) return fail(PXG_ERR_INVALID, "pxg_set_warm_cache: null argument");
+    if (!w->d.ck_keys[0]) return fail(PXG_ERR_STATE, "pxg_set_warm_cache: upload shapes first");
+    size_t tc = size_t(w->d.cache_mask) + 1;
+    if (n > tc / 2) return fail(PXG_ERR_CAPACITY, "pxg_set_warm_cache: too many entries");
+    std::vector<unsigned long long> k(tc, ~0ull);
+    std::vector<uint32_t> o(tc, 0u);
+    std::vector<float4> v(tc, make_float4(0, 0, 0, 0));
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t idx = host_hash64(keys[i]) & w->d.cache_mask;
+        while (k[idx] != ~0ull && k[idx] != keys[i]) idx = (idx + 1) & w->d.cache_mask;
+        k[idx] = keys[i];
+        o[idx] = 1;
+        v[idx] = make_float4(imp3[3 * i], imp3[3 * i + 1], imp3[3 * i + 2], 0.0f);
+    }
+    int t = w->d.cur;
+    HIPCHK(hipMemcpyAsync(w->d.ck_keys[t], k.data(), tc * 8, hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipMemcpyAsync(w->d.ck_ord[t], o.data(), tc * 4, hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipMemcpyAsync(w->d.ck_val[t], v.data(), tc * sizeof(float4), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+// FNV-1a over alive bodies' position, orientation and velocities (world.h:1334-1359)
+extern "C" int pxg_state_hash(pxg_world* w, uint64_t* out) {
+    if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_state_hash: null argument");
+    uint32_t n = w->n_bodies;
+    std::vector<pxg_body> b(n);
+    int r = pxg_get_bodies(w, b.data(), n);
+    if (r != PXG_OK) return r;
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void* p, size_t len) {
+        const uint8_t* c = static_cast<const uint8_t*>(p);
+        for (size_t i = 0; i < len; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+    };
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!b[i].alive) continue;
+        mix(b[i].position, 12);
+        mix(b[i].orientation, 16);
+        mix(b[i].linear_velocity, 12);
+        mix(b[i].angular_velocity, 12);
+    }
+    *out = h;
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_mlp(pxg_world* w, const uint16_t*, const float*, const uint16_t*, const float*, const uint16_t*,
+                           const float*, const float*) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_set_mlp: null handle");
+    return fail(PXG_ERR_STATE, "pxg_set_mlp: learned warm start not built into this library");
+}
+
+// Profiling of the dominant kernel (the velocity pass): HIP events on the world's stream around
+// every launch. No counterpart in the reference.
+extern "C" int pxg_set_profiling(pxg_world* w, uint32_t on) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_set_profiling: null handle");
+    w->profiling = on != 0;
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_profile(pxg_world* w, pxg_profile* out) {
+    if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_get_profile: null argument");
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (auto& pr : w->ev_pending) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        w->prof_vel_ms += ms;
+        w->prof_vel_launches += 1;
+        w->ev_pool.push_back(pr.first);
+        w->ev_pool.push_back(pr.second);
+    }
+    w->ev_pending.clear();
+    HIPCHK(hipMemcpy(w->h_cnt, w->d.cnt, sizeof(Counters), hipMemcpyDeviceToHost));
+    std::memset(out, 0, sizeof(*out));
+    out->velocity_ms = w->prof_vel_ms;
+    out->velocity_launches = w->prof_vel_launches;
+    out->velocity_bytes = double(w->h_cnt->prof_vel_bytes);
+    w->prof_vel_ms = 0.0;
+    w->prof_vel_launches = 0;
+    unsigned long long zero = 0;
+    HIPCHK(hipMemcpy(&w->d.cnt->prof_vel_bytes, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    return PXG_OK;
+}

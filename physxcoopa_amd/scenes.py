@@ -1,0 +1,385 @@
+"""Deterministic scene generators for the rigid-body contact step.
+
+Every scene is a pair of numpy structured arrays (bodies, shapes) in the pxg ABI layouts, plus the
+solver settings it is quoted with. Values are float32 exactly as the GPU and the CPU oracle see
+them. Inertia helpers restate physxcoopa/dynamics/inertia.h:36-107 in float32 arithmetic.
+
+Scenes:
+  * config1_scene: 64 unit boxes on a 4x4x4 grid over a 50x50x1 m ground (the reference's CPU case).
+  * config2_scene: the 16,384-body mixed pile (40% boxes, 40% spheres, 20% Z capsules).
+  * config3_scenes: thousands of independent small scenes (box pyramids and random drops).
+  * small fixtures mirroring physxcoopa/tests/support/physics_fixtures.h.
+"""
+import math
+
+import numpy as np
+
+from ._abi import (BODY_DTYPE, SHAPE_DTYPE, PXG_STATIC, PXG_DYNAMIC, PXG_SPHERE, PXG_BOX, PXG_CAPSULE,
+                   PXG_AVERAGE)
+
+F = np.float32
+PI_F = F(3.14159265358979323846)
+EPS_F = F(1e-6)
+
+
+def _fmax(a, b):  # std::max(a, b) = (a < b) ? b : a
+    return b if a < b else a
+
+
+def box_inverse_inertia(half_extents, mass):
+    """physxcoopa/dynamics/inertia.h:36-47 in float32."""
+    mass = F(mass)
+    if mass <= F(0):
+        return (F(0), F(0), F(0))
+    e = [F(2.0) * F(h) for h in half_extents]
+    e2 = [x * x for x in e]
+    ix = (mass * (e2[1] + e2[2])) / F(12.0)
+    iy = (mass * (e2[0] + e2[2])) / F(12.0)
+    iz = (mass * (e2[0] + e2[1])) / F(12.0)
+    return (F(1.0) / _fmax(ix, EPS_F), F(1.0) / _fmax(iy, EPS_F), F(1.0) / _fmax(iz, EPS_F))
+
+
+def sphere_inverse_inertia(radius, mass):
+    """physxcoopa/dynamics/inertia.h:56-61 in float32."""
+    mass, r = F(mass), F(radius)
+    if mass <= F(0):
+        return (F(0), F(0), F(0))
+    i = ((F(0.4) * mass) * r) * r
+    inv = F(1.0) / _fmax(i, EPS_F)
+    return (inv, inv, inv)
+
+
+def capsule_inverse_inertia(radius, half_height, mass):
+    """physxcoopa/dynamics/inertia.h:87-107 in float32: (perp, perp, axial) with the long axis Z."""
+    mass, r, hh = F(mass), F(radius), F(half_height)
+    if mass <= F(0):
+        return (F(0), F(0), F(0))
+    vol_cyl = ((PI_F * r) * r) * (F(2.0) * hh)
+    vol_hemi = (((F(4.0 / 3.0) * PI_F) * r) * r) * r
+    vol_total = vol_cyl + vol_hemi
+    if vol_total < EPS_F:
+        return sphere_inverse_inertia(r, mass)
+    m_cyl = mass * (vol_cyl / vol_total)
+    m_hemi = mass * (vol_hemi / vol_total)
+    i_axial = ((F(0.5) * m_cyl) * r) * r + ((F(0.4) * m_hemi) * r) * r
+    i_perp = ((m_cyl * ((F(3.0) * r) * r + (F(4.0) * hh) * hh)) / F(12.0) + ((F(0.4) * m_hemi) * r) * r) \
+        + (m_hemi * hh) * hh
+    return (F(1.0) / _fmax(i_perp, EPS_F), F(1.0) / _fmax(i_perp, EPS_F), F(1.0) / _fmax(i_axial, EPS_F))
+
+
+def quat_axis_angle(axis, angle):
+    """(x, y, z, w) of a rotation of `angle` radians about unit `axis`."""
+    s = math.sin(angle * 0.5)
+    return (axis[0] * s, axis[1] * s, axis[2] * s, math.cos(angle * 0.5))
+
+
+def random_quaternion(rng):
+    """Uniform random unit quaternion (Shoemake's method), (x, y, z, w)."""
+    u1, u2, u3 = rng.random(3)
+    a, b = math.sqrt(1.0 - u1), math.sqrt(u1)
+    q = (a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2),
+         b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3))
+    n = math.sqrt(sum(c * c for c in q))
+    return tuple(c / n for c in q)
+
+
+class SceneBuilder:
+    """Collects bodies and one shape per body (or none) into pxg ABI arrays."""
+
+    def __init__(self):
+        self._bodies = []
+        self._shapes = []
+
+    def add_body(self, btype=PXG_DYNAMIC, position=(0, 0, 0), orientation=(0, 0, 0, 1), mass=1.0,
+                 inv_mass=None, inv_inertia=(1, 1, 1), velocity=(0, 0, 0), angular_velocity=(0, 0, 0),
+                 linear_drag=0.0, angular_drag=0.05, use_gravity=True, awake=True, constraints=0):
+        b = np.zeros((), dtype=BODY_DTYPE)
+        b["type"] = btype
+        b["alive"] = 1
+        b["awake"] = 1 if awake else 0
+        b["use_gravity"] = 1 if use_gravity else 0
+        b["constraints"] = constraints
+        b["position"] = position
+        b["orientation"] = orientation
+        b["linear_velocity"] = velocity
+        b["angular_velocity"] = angular_velocity
+        b["mass"] = mass
+        b["inv_mass"] = (F(1.0) / F(mass) if F(mass) > 0 else F(0)) if inv_mass is None else inv_mass
+        if btype != PXG_DYNAMIC and inv_mass is None:
+            b["inv_mass"] = 1.0  # the reference's Body default; never read for non-dynamic bodies
+        b["inv_inertia_local"] = inv_inertia
+        b["linear_drag"] = linear_drag
+        b["angular_drag"] = angular_drag
+        b["prev_position"] = position
+        b["prev_orientation"] = orientation
+        self._bodies.append(b)
+        return len(self._bodies) - 1
+
+    def add_shape(self, owner, stype, radius=0.5, half_extents=(0.5, 0.5, 0.5), capsule_radius=0.5,
+                  capsule_half_height=0.5, capsule_axis=2, friction=0.6, restitution=0.0,
+                  friction_combine=PXG_AVERAGE, restitution_combine=PXG_AVERAGE, is_trigger=False, layer=0,
+                  local_center=(0, 0, 0), local_rotation=(0, 0, 0, 1)):
+        s = np.zeros((), dtype=SHAPE_DTYPE)
+        s["owner"] = owner
+        s["type"] = stype
+        s["enabled"] = 1
+        s["is_trigger"] = 1 if is_trigger else 0
+        s["layer"] = layer
+        s["capsule_axis"] = capsule_axis
+        s["friction_combine"] = friction_combine
+        s["restitution_combine"] = restitution_combine
+        s["local_center"] = local_center
+        s["radius"] = radius
+        s["local_rotation"] = local_rotation
+        s["half_extents"] = half_extents
+        s["capsule_radius"] = capsule_radius
+        s["capsule_half_height"] = capsule_half_height
+        s["dynamic_friction"] = friction
+        s["static_friction"] = friction
+        s["restitution"] = restitution
+        self._shapes.append(s)
+        return len(self._shapes) - 1
+
+    # -- convenience (physxcoopa/tests/support/physics_fixtures.h:37-118) --
+    def static_box(self, position, half_extents, orientation=(0, 0, 0, 1), **mat):
+        i = self.add_body(PXG_STATIC, position, orientation, mass=1.0)
+        self.add_shape(i, PXG_BOX, half_extents=half_extents, **mat)
+        return i
+
+    def dynamic_box(self, position, half_extents, orientation=(0, 0, 0, 1), mass=1.0, **kw):
+        mat = {k: kw.pop(k) for k in list(kw) if k in ("friction", "restitution", "friction_combine",
+                                                       "restitution_combine", "is_trigger", "layer")}
+        i = self.add_body(PXG_DYNAMIC, position, orientation, mass=mass,
+                          inv_inertia=box_inverse_inertia(half_extents, mass), **kw)
+        self.add_shape(i, PXG_BOX, half_extents=half_extents, **mat)
+        return i
+
+    def dynamic_sphere(self, position, radius, mass=1.0, **kw):
+        mat = {k: kw.pop(k) for k in list(kw) if k in ("friction", "restitution", "friction_combine",
+                                                       "restitution_combine", "is_trigger", "layer")}
+        i = self.add_body(PXG_DYNAMIC, position, mass=mass, inv_inertia=sphere_inverse_inertia(radius, mass), **kw)
+        self.add_shape(i, PXG_SPHERE, radius=radius, **mat)
+        return i
+
+    def dynamic_capsule(self, position, radius, half_height, axis=2, orientation=(0, 0, 0, 1), mass=1.0, **kw):
+        mat = {k: kw.pop(k) for k in list(kw) if k in ("friction", "restitution", "friction_combine",
+                                                       "restitution_combine", "is_trigger", "layer")}
+        raw = capsule_inverse_inertia(radius, half_height, mass)
+        if axis == 0:
+            local = (raw[2], raw[1], raw[0])
+        elif axis == 1:
+            local = (raw[0], raw[2], raw[1])
+        else:
+            local = raw
+        i = self.add_body(PXG_DYNAMIC, position, orientation, mass=mass, inv_inertia=local, **kw)
+        self.add_shape(i, PXG_CAPSULE, capsule_radius=radius, capsule_half_height=half_height, capsule_axis=axis, **mat)
+        return i
+
+    def arrays(self):
+        bodies = np.array(self._bodies, dtype=BODY_DTYPE) if self._bodies else np.zeros(0, BODY_DTYPE)
+        shapes = np.array(self._shapes, dtype=SHAPE_DTYPE) if self._shapes else np.zeros(0, SHAPE_DTYPE)
+        return np.ascontiguousarray(bodies), np.ascontiguousarray(shapes)
+
+
+class Scene:
+    """A generated scene: ABI arrays plus the solver settings it is quoted with."""
+
+    def __init__(self, name, bodies, shapes, config=None, steps=300):
+        self.name = name
+        self.bodies = bodies
+        self.shapes = shapes
+        self.config = dict(config or {})
+        self.steps = steps
+
+    @property
+    def n_dynamic(self):
+        return int(np.count_nonzero(self.bodies["type"] == PXG_DYNAMIC))
+
+
+def config1_scene(seed=1):
+    """64 dynamic unit boxes (half 0.5 m, 1 kg) on a 4x4x4 grid, 1.2 m spacing, bottom faces 1 m
+    above a static 50x50x1 m ground; jitter U(+-0.05 m) in position and U(+-5 deg) in yaw.
+    Friction 0.5, restitution 0, 8 velocity and 3 position iterations, 300 steps."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    sb.static_box((0, 0, -0.5), (25, 25, 0.5), friction=0.5)
+    for k in range(4):
+        for j in range(4):
+            for i in range(4):
+                jit = rng.uniform(-0.05, 0.05, 3)
+                yaw = math.radians(rng.uniform(-5.0, 5.0))
+                pos = ((i - 1.5) * 1.2 + jit[0], (j - 1.5) * 1.2 + jit[1], 1.5 + k * 1.2 + jit[2])
+                sb.dynamic_box(pos, (0.5, 0.5, 0.5), quat_axis_angle((0, 0, 1), yaw), mass=1.0, friction=0.5)
+    b, s = sb.arrays()
+    return Scene("config1_64_boxes", b, s, dict(velocity_iterations=8, position_iterations=3), steps=300)
+
+
+def _pile_shape(rng):
+    """One body of the 40% box / 40% sphere / 20% capsule mix: (kind, dims, bounding radius)."""
+    u = rng.random()
+    if u < 0.4:
+        he = tuple(float(F(x)) for x in rng.uniform(0.2, 0.6, 3))
+        return PXG_BOX, he, math.sqrt(sum(h * h for h in he))
+    if u < 0.8:
+        r = float(F(rng.uniform(0.2, 0.5)))
+        return PXG_SPHERE, r, r
+    return PXG_CAPSULE, (0.2, 0.4), 0.6
+
+
+def _pile_mass(kind, dims, density=1000.0):
+    rho = F(density)
+    if kind == PXG_BOX:
+        hx, hy, hz = (F(d) for d in dims)
+        return (((F(8.0) * hx) * hy) * hz) * rho
+    if kind == PXG_SPHERE:
+        r = F(dims)
+        return ((((F(4.0 / 3.0) * PI_F) * r) * r) * r) * rho
+    r, hh = F(dims[0]), F(dims[1])
+    return (((PI_F * r) * r) * (F(2.0) * hh) + (((F(4.0 / 3.0) * PI_F) * r) * r) * r) * rho
+
+
+def _add_pile_body(sb, kind, dims, pos, quat, friction, restitution):
+    m = _pile_mass(kind, dims)
+    if kind == PXG_BOX:
+        return sb.dynamic_box(pos, dims, quat, mass=m, friction=friction, restitution=restitution)
+    if kind == PXG_SPHERE:
+        i = sb.dynamic_sphere(pos, dims, mass=m, friction=friction, restitution=restitution)
+        sb._bodies[i]["orientation"] = quat
+        sb._bodies[i]["prev_orientation"] = quat
+        return i
+    return sb.dynamic_capsule(pos, dims[0], dims[1], 2, quat, mass=m, friction=friction, restitution=restitution)
+
+
+def config2_scene(seed=2, n=16384, lattice=(64, 64, 40)):
+    """The large mixed pile: n dynamic bodies (40% boxes half U[0.2,0.6]^3, 40% spheres r
+    U[0.2,0.5], 20% Z capsules r 0.2 / half-height 0.4), density 1000 kg/m^3, uniform random
+    orientation, one body per 1 m cell of a 64x64x40 lattice whose floor is 1 m above a static
+    200x200x1 m ground. Friction U[0.3,0.8], restitution U[0,0.2], Average combine (the ground
+    samples its own pair once). 8 velocity / 3 position iterations, 600 steps.
+
+    No initial overlap: boxes up to 0.6 m half-extent and 1.2 m capsules are larger than a 1 m
+    cell, so a body whose bounding sphere would overlap an already-placed neighbour's is moved
+    to another free cell (seeded rejection sampling)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = lattice
+    ncell = nx * ny * nz
+    assert n <= ncell
+    sb = SceneBuilder()
+    gf, gr = float(F(rng.uniform(0.3, 0.8))), float(F(rng.uniform(0.0, 0.2)))
+    sb.static_box((0, 0, -0.5), (100, 100, 0.5), friction=gf, restitution=gr)
+    occupied = {}
+    perm = rng.permutation(ncell)
+    cursor = 0
+    for _ in range(n):
+        kind, dims, brad = _pile_shape(rng)
+        quat = random_quaternion(rng)
+        fr, re = float(F(rng.uniform(0.3, 0.8))), float(F(rng.uniform(0.0, 0.2)))
+        while True:
+            c = int(perm[cursor])
+            cursor += 1
+            ix, iy, iz = c % nx, (c // nx) % ny, c // (nx * ny)
+            center = (ix - nx / 2 + 0.5, iy - ny / 2 + 0.5, 1.0 + iz + 0.5)
+            ok = True
+            for dx in (-2, -1, 0, 1, 2):
+                for dy in (-2, -1, 0, 1, 2):
+                    for dz in (-2, -1, 0, 1, 2):
+                        o = occupied.get((ix + dx, iy + dy, iz + dz))
+                        if o is None:
+                            continue
+                        d2 = sum((center[t] - o[0][t]) ** 2 for t in range(3))
+                        if d2 < (brad + o[1]) ** 2:
+                            ok = False
+            if ok:
+                break
+        occupied[(ix, iy, iz)] = (center, brad)
+        _add_pile_body(sb, kind, dims, center, quat, fr, re)
+    b, s = sb.arrays()
+    return Scene("config2_pile_%d" % n, b, s, dict(velocity_iterations=8, position_iterations=3), steps=600)
+
+
+def pyramid_scene(n_bodies, friction=0.5):
+    """A 2-D box pyramid of n_bodies unit boxes (base width ceil(sqrt(2N))) on a 20x20x1 m ground."""
+    sb = SceneBuilder()
+    sb.static_box((0, 0, -0.5), (10, 10, 0.5), friction=friction)
+    base = int(math.ceil(math.sqrt(2 * n_bodies)))
+    placed = 0
+    row = 0
+    while placed < n_bodies and base - row > 0:
+        width = base - row
+        for i in range(width):
+            if placed >= n_bodies:
+                break
+            x = (i - (width - 1) / 2.0) * 1.0
+            sb.dynamic_box((x, 0.0, 0.5 + row * 1.0), (0.5, 0.5, 0.5), mass=1.0, friction=friction)
+            placed += 1
+        row += 1
+    return sb.arrays()
+
+
+def drop_scene(n_bodies, rng):
+    """n_bodies of the pile shape mix dropped in a 10x10x20 m column over a 20x20x1 m ground."""
+    sb = SceneBuilder()
+    sb.static_box((0, 0, -0.5), (10, 10, 0.5), friction=0.5)
+    # 1.25 m cells in an 8x8 column: no initial overlap for the largest body (bounding radius 1.04)
+    cells = rng.permutation(8 * 8 * 16)[:n_bodies]
+    for c in cells:
+        ix, iy, iz = c % 8, (c // 8) % 8, c // 64
+        kind, dims, _ = _pile_shape(rng)
+        quat = random_quaternion(rng)
+        pos = ((ix - 3.5) * 1.25, (iy - 3.5) * 1.25, 1.2 + iz * 1.25)
+        _add_pile_body(sb, kind, dims, pos, quat, 0.5, 0.0)
+    return sb.arrays()
+
+
+def config3_scenes(seed=3, n_scenes=4096, n_min=8, n_max=256):
+    """Independent scenes with N ~ U{8..256} dynamic bodies: even scenes are box pyramids, odd
+    scenes random drops of the pile shape mix. 8 velocity / 3 position iterations, 300 steps.
+    Returns (bodies, shapes, body_offsets, shape_offsets, settings)."""
+    rng = np.random.default_rng(seed)
+    bl, sl = [], []
+    for k in range(n_scenes):
+        nb = int(rng.integers(n_min, n_max + 1))
+        b, s = pyramid_scene(nb) if k % 2 == 0 else drop_scene(nb, rng)
+        bl.append(b)
+        sl.append(s)
+    body_off = np.zeros(n_scenes + 1, np.uint32)
+    shape_off = np.zeros(n_scenes + 1, np.uint32)
+    body_off[1:] = np.cumsum([len(b) for b in bl])
+    shape_off[1:] = np.cumsum([len(s) for s in sl])
+    bodies = np.ascontiguousarray(np.concatenate(bl))
+    shapes = np.ascontiguousarray(np.concatenate(sl))
+    return bodies, shapes, body_off, shape_off, dict(velocity_iterations=8, position_iterations=3)
+
+
+# ---- small fixtures (physxcoopa/tests/support/physics_fixtures.h) ----
+
+def ground(sb, material=None):
+    """Static 8 x 8 m slab whose top face is at z = 0 (physics_fixtures.h:37-44)."""
+    return sb.static_box((0, 0, -0.1), (4, 4, 0.1), **(material or {}))
+
+
+def sphere_rain_scene(n=80, seed=1234, radius=0.4):
+    """80 unit spheres dropped over the fixture ground (physics_world_test.cpp:177-207 layout)."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    ground(sb)
+    for _ in range(n):
+        p = (rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(2, 8))
+        sb.dynamic_sphere(p, radius, mass=1.0)
+    return sb.arrays()
+
+
+def mixed_pile_scene(n=200, seed=7, spread=4.0):
+    """A small mixed pile (boxes, spheres, capsules, random orientations) over the fixture ground."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    sb.static_box((0, 0, -0.5), (10, 10, 0.5), friction=0.6)
+    side = int(math.ceil(math.sqrt(n / 4.0)))
+    for k in range(n):
+        ix, iy, iz = k % side, (k // side) % side, k // (side * side)
+        kind, dims, _ = _pile_shape(rng)
+        quat = random_quaternion(rng)
+        pos = ((ix - side / 2) * 1.3 * spread / 4.0, (iy - side / 2) * 1.3 * spread / 4.0, 1.2 + iz * 1.3)
+        fr, re = float(F(rng.uniform(0.3, 0.8))), float(F(rng.uniform(0.0, 0.2)))
+        _add_pile_body(sb, kind, dims, pos, quat, fr, re)
+    return sb.arrays()

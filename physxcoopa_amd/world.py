@@ -1,0 +1,330 @@
+"""Python mirror of physxcoopa's PhysicsWorld over the pxg C ABI.
+
+`World` is the thin handle wrapper (scene arrays in, arrays out). `PhysicsWorld` mirrors the
+reference's interface (physxcoopa/world.h): add_body/remove_body/get_body with generational
+BodyIds, step(dt)/step_fixed(h), manifolds(), config(), layers(), set_gravity(), on_substep
+callbacks between the collision and solve phases, and world_state_hash(). All compute runs in
+libpxg.so on the GPU; there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import BODY_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, Config, Stats, Profile, ptr
+from ._lib import lib, check
+
+
+class World:
+    """One GPU world handle built from ABI arrays (bodies, shapes)."""
+
+    def __init__(self, config, bodies, shapes, layer_masks=None):
+        self._cfg = config
+        h = ctypes.c_void_p()
+        check(lib().pxg_create(ctypes.byref(config), ctypes.byref(h)), "pxg_create")
+        self.h = h
+        self.n_bodies = len(bodies)
+        self.n_shapes = len(shapes)
+        b = np.ascontiguousarray(bodies, dtype=BODY_DTYPE)
+        s = np.ascontiguousarray(shapes, dtype=SHAPE_DTYPE)
+        check(lib().pxg_set_bodies(self.h, ptr(b), self.n_bodies), "pxg_set_bodies")
+        if layer_masks is not None:
+            m = np.ascontiguousarray(np.asarray(layer_masks, np.uint32))
+            check(lib().pxg_set_layer_masks(self.h, ptr(m)), "pxg_set_layer_masks")
+        check(lib().pxg_set_shapes(self.h, ptr(s), self.n_shapes), "pxg_set_shapes")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pxg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def config(self):
+        return self._cfg
+
+    def set_config(self, config):
+        self._cfg = config
+        check(lib().pxg_set_config(self.h, ctypes.byref(config)), "pxg_set_config")
+
+    def set_layer_masks(self, masks):
+        m = np.ascontiguousarray(np.asarray(masks, np.uint32))
+        check(lib().pxg_set_layer_masks(self.h, ptr(m)), "pxg_set_layer_masks")
+
+    def step_fixed(self, h=None):
+        check(lib().pxg_step_fixed(self.h, np.float32(self._cfg.fixed_dt if h is None else h)), "pxg_step_fixed")
+
+    def step_phase(self, phase, h=None):
+        check(lib().pxg_step_phase(self.h, phase, np.float32(self._cfg.fixed_dt if h is None else h)), "pxg_step_phase")
+
+    def step(self, dt):
+        n = ctypes.c_uint32()
+        check(lib().pxg_step(self.h, np.float32(dt), ctypes.byref(n)), "pxg_step")
+        return n.value
+
+    def synchronize(self):
+        check(lib().pxg_synchronize(self.h), "pxg_synchronize")
+
+    def stats(self):
+        s = Stats()
+        check(lib().pxg_get_stats(self.h, ctypes.byref(s)), "pxg_get_stats")
+        return {f: getattr(s, f) for f, _ in Stats._fields_ if not f.startswith("reserved")}
+
+    def bodies(self):
+        out = np.zeros(self.n_bodies, BODY_DTYPE)
+        check(lib().pxg_get_bodies(self.h, ptr(out), self.n_bodies), "pxg_get_bodies")
+        return out
+
+    def set_bodies(self, bodies):
+        b = np.ascontiguousarray(bodies, dtype=BODY_DTYPE)
+        check(lib().pxg_set_bodies(self.h, ptr(b), len(b)), "pxg_set_bodies")
+        self.n_bodies = len(b)
+
+    def pairs(self):
+        n = ctypes.c_uint32()
+        check(lib().pxg_get_pairs(self.h, None, 0, ctypes.byref(n)), "pxg_get_pairs")
+        out = np.zeros((max(n.value, 1), 2), np.uint32)
+        check(lib().pxg_get_pairs(self.h, ptr(out), n.value, ctypes.byref(n)), "pxg_get_pairs")
+        return out[:n.value]
+
+    def manifolds(self):
+        n = ctypes.c_uint32()
+        check(lib().pxg_get_manifolds(self.h, None, 0, ctypes.byref(n)), "pxg_get_manifolds")
+        out = np.zeros(max(n.value, 1), MANIFOLD_DTYPE)
+        check(lib().pxg_get_manifolds(self.h, ptr(out), n.value, ctypes.byref(n)), "pxg_get_manifolds")
+        return out[:n.value]
+
+    def solve_order(self):
+        """(order, colour_offsets): manifold indices in solve order and the colour boundaries."""
+        n, nc = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().pxg_get_solve_order(self.h, None, 0, ctypes.byref(n), None, 0, ctypes.byref(nc)), "order")
+        order = np.zeros(max(n.value, 1), np.uint32)
+        offs = np.zeros(nc.value + 1, np.uint32)
+        check(lib().pxg_get_solve_order(self.h, ptr(order), n.value, ctypes.byref(n), ptr(offs), nc.value + 1,
+                                        ctypes.byref(nc)), "pxg_get_solve_order")
+        return order[:n.value], offs
+
+    def fat_aabbs(self):
+        out = np.zeros((self.n_shapes, 6), np.float32)
+        check(lib().pxg_get_fat_aabbs(self.h, ptr(out), self.n_shapes), "pxg_get_fat_aabbs")
+        return out
+
+    def set_fat_aabbs(self, boxes):
+        b = np.ascontiguousarray(np.asarray(boxes, np.float32).reshape(-1, 6))
+        check(lib().pxg_set_fat_aabbs(self.h, ptr(b), len(b)), "pxg_set_fat_aabbs")
+
+    def warm_cache(self):
+        n = ctypes.c_uint32()
+        check(lib().pxg_get_warm_cache(self.h, None, None, 0, ctypes.byref(n)), "pxg_get_warm_cache")
+        keys = np.zeros(max(n.value, 1), np.uint64)
+        imp = np.zeros((max(n.value, 1), 3), np.float32)
+        check(lib().pxg_get_warm_cache(self.h, ptr(keys), ptr(imp), n.value, ctypes.byref(n)), "pxg_get_warm_cache")
+        return keys[:n.value], imp[:n.value]
+
+    def set_warm_cache(self, keys, imp):
+        k = np.ascontiguousarray(np.asarray(keys, np.uint64))
+        i = np.ascontiguousarray(np.asarray(imp, np.float32).reshape(-1, 3))
+        check(lib().pxg_set_warm_cache(self.h, ptr(k), ptr(i), len(k)), "pxg_set_warm_cache")
+
+    def state_hash(self):
+        out = ctypes.c_uint64()
+        check(lib().pxg_state_hash(self.h, ctypes.byref(out)), "pxg_state_hash")
+        return out.value
+
+    def set_profiling(self, on):
+        check(lib().pxg_set_profiling(self.h, 1 if on else 0), "pxg_set_profiling")
+
+    def profile(self):
+        p = Profile()
+        check(lib().pxg_get_profile(self.h, ctypes.byref(p)), "pxg_get_profile")
+        return dict(velocity_ms=p.velocity_ms, velocity_bytes=p.velocity_bytes, velocity_launches=p.velocity_launches)
+
+
+class BodyId:
+    """Generational handle (physxcoopa/dynamics/body.h:61-77)."""
+    INVALID = 0xFFFFFFFF
+
+    def __init__(self, index=INVALID, generation=0):
+        self.index, self.generation = index, generation
+
+    def is_valid(self):
+        return self.index != BodyId.INVALID
+
+    def __eq__(self, o):
+        return isinstance(o, BodyId) and (self.index, self.generation) == (o.index, o.generation)
+
+    def __hash__(self):
+        return hash((self.index, self.generation))
+
+    def __repr__(self):
+        return "BodyId(%d, %d)" % (self.index, self.generation)
+
+
+class LayerMatrix:
+    """32x32 symmetric collision matrix (physxcoopa/broadphase/layer_matrix.h:21-47)."""
+
+    def __init__(self):
+        self.mask = np.full(32, 0xFFFFFFFF, np.uint32)
+
+    def should_collide(self, a, b):
+        return bool(self.mask[a & 31] & np.uint32(1 << (b & 31)))
+
+    def set_layer_collision(self, a, b, collide):
+        a, b = a & 31, b & 31
+        if collide:
+            self.mask[a] |= np.uint32(1 << b)
+            self.mask[b] |= np.uint32(1 << a)
+        else:
+            self.mask[a] &= np.uint32(~(1 << b) & 0xFFFFFFFF)
+            self.mask[b] &= np.uint32(~(1 << a) & 0xFFFFFFFF)
+
+
+class PhysicsWorld:
+    """The reference's PhysicsWorld interface (physxcoopa/world.h) over the GPU step.
+
+    Bodies are ABI records (numpy scalars of BODY_DTYPE); a shape is a SHAPE_DTYPE record or
+    None ("Rigidbody with no Collider"). Structural edits are staged on the host and uploaded
+    before the next step; body state lives on the GPU between steps.
+    """
+
+    def __init__(self, config=None):
+        self._cfg = config if config is not None else _abi.default_config()
+        self._bodies = np.zeros(0, BODY_DTYPE)
+        self._shapes = np.zeros(0, SHAPE_DTYPE)
+        self._generations = []
+        self._free = []
+        self._layers = LayerMatrix()
+        self._world = None
+        self._dirty = True
+        self._state_on_device = False
+        self.on_substep = []
+        self._accumulator = 0.0
+
+    # -- lifecycle (world.h:95-143) --
+    def add_body(self, body, shape=None):
+        self._pull()
+        b = np.array(body, dtype=BODY_DTYPE).reshape(())
+        b["alive"] = 1
+        b["prev_position"] = b["position"]
+        b["prev_orientation"] = b["orientation"]
+        if self._free:
+            idx = self._free.pop()
+            self._bodies[idx] = b
+        else:
+            idx = len(self._bodies)
+            self._bodies = np.concatenate([self._bodies, b.reshape(1)])
+            self._generations.append(0)
+        s = np.zeros((), SHAPE_DTYPE) if shape is None else np.array(shape, dtype=SHAPE_DTYPE).reshape(())
+        s["owner"] = idx
+        if idx < len(self._shapes):
+            self._shapes[idx] = s
+        else:
+            self._shapes = np.concatenate([self._shapes, s.reshape(1)])
+        self._dirty = True
+        return BodyId(idx, self._generations[idx])
+
+    def remove_body(self, bid):
+        if not self.is_valid(bid):
+            return
+        self._pull()
+        self._bodies[bid.index]["alive"] = 0
+        self._shapes[bid.index]["enabled"] = 0
+        self._generations[bid.index] += 1
+        self._free.append(bid.index)
+        self._dirty = True
+
+    def is_valid(self, bid):
+        return (bid.is_valid() and bid.index < len(self._bodies) and bool(self._bodies[bid.index]["alive"])
+                and self._generations[bid.index] == bid.generation)
+
+    def get_body(self, bid):
+        """A copy of the body's current state, or None for a stale handle (world.h:347)."""
+        if not self.is_valid(bid):
+            return None
+        self._pull()
+        return self._bodies[bid.index].copy()
+
+    def set_body(self, bid, body):
+        if not self.is_valid(bid):
+            return
+        self._pull()
+        self._bodies[bid.index] = body
+        self._dirty = True
+
+    def config(self):
+        return self._cfg
+
+    def layers(self):
+        self._dirty = True
+        return self._layers
+
+    def set_gravity(self, g):
+        for i in range(3):
+            self._cfg.gravity[i] = g[i]
+        if self._world is not None:
+            self._world.set_config(self._cfg)
+
+    # -- stepping (world.h:1229-1317) --
+    def _sync(self):
+        if self._world is None or self._dirty:
+            self._pull()
+            if self._world is not None:
+                fat = self._world.fat_aabbs() if self._world.n_shapes == len(self._shapes) else None
+                keys, imp = self._world.warm_cache()
+                self._world.close()
+            else:
+                fat, keys, imp = None, None, None
+            self._world = World(self._cfg, self._bodies, self._shapes, self._layers.mask)
+            if fat is not None:
+                self._world.set_fat_aabbs(fat)
+            if keys is not None and len(keys):
+                self._world.set_warm_cache(keys, imp)
+            self._dirty = False
+        self._state_on_device = True
+
+    def _pull(self):
+        if self._world is not None and self._state_on_device:
+            self._bodies = self._world.bodies()
+            self._state_on_device = False
+
+    def step_fixed(self, h=None):
+        h = self._cfg.fixed_dt if h is None else h
+        self._sync()
+        if self.on_substep:
+            self._world.step_phase(_abi.PXG_PHASE_FORCES_COLLIDE, h)
+            self._bodies = self._world.bodies()
+            before = self._bodies.copy()
+            for cb in self.on_substep:
+                cb(self, h)
+            if not np.array_equal(before.view(np.uint8), self._bodies.view(np.uint8)):
+                self._world.set_bodies(self._bodies)
+            self._world.step_phase(_abi.PXG_PHASE_SOLVE_INTEGRATE, h)
+        else:
+            self._world.step_fixed(h)
+        self._state_on_device = True
+
+    def step(self, dt):
+        frame = min(dt, 0.25)
+        self._accumulator += frame
+        h = self._cfg.fixed_dt
+        n = 0
+        while self._accumulator >= h and n < self._cfg.max_substeps:
+            self.step_fixed(h)
+            self._accumulator -= h
+            n += 1
+        return n
+
+    def manifolds(self):
+        return self._world.manifolds() if self._world is not None else np.zeros(0, MANIFOLD_DTYPE)
+
+    def world_state_hash(self):
+        self._sync()
+        return self._world.state_hash()
+
+    def stats(self):
+        return self._world.stats() if self._world is not None else {}

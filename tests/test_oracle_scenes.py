@@ -1,0 +1,335 @@
+"""The CPU oracle re-runs the reference's own end-to-end tests (physxcoopa/tests/contact_solver_test.cpp,
+sleep_test.cpp, physics_world_test.cpp, broadphase_test.cpp) with the same scenes and bars."""
+import math
+
+import numpy as np
+import pytest
+
+from physxcoopa_amd import scenes, _abi
+from physxcoopa_amd._abi import PXG_DYNAMIC, PXG_STATIC, PXG_KINEMATIC, PXG_BOX, PXG_SPHERE
+
+H = np.float32(1.0) / np.float32(60.0)
+
+
+def world(oracle, sb, **cfg):
+    b, s = sb.arrays()
+    return oracle.OracleWorld(_abi.default_config(**cfg), b, s)
+
+
+def best_up_alignment(q):
+    x, y, z, w = [float(c) for c in q]
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    return np.abs(R[2, :]).max()
+
+
+def test_free_fall_matches_semi_implicit_euler(oracle):
+    # physics_world_test.cpp:26-58
+    sb = scenes.SceneBuilder()
+    sb.add_body(PXG_DYNAMIC, (0, 0, 0), mass=1.0)
+    sb.add_body(PXG_KINEMATIC, (1, 2, 3))
+    sb.add_body(PXG_STATIC, (4, 5, 6))
+    w = world(oracle, sb)
+    for _ in range(60):
+        w.step_fixed()
+    b = w.bodies()
+    n = 60.0
+    expected = -9.81 * float(H) ** 2 * n * (n + 1) * 0.5
+    assert abs(b["position"][0][2] - expected) < 0.01
+    assert abs(b["position"][0][2] + 0.5 * 9.81) < 0.2
+    np.testing.assert_allclose(b["position"][1], [1, 2, 3], atol=1e-5)
+    np.testing.assert_allclose(b["position"][2], [4, 5, 6], atol=1e-5)
+
+
+def test_static_and_kinematic_pairs_generate_no_manifolds(oracle):
+    # broadphase_test.cpp:87-110
+    sb = scenes.SceneBuilder()
+    for p in ((0, 0, 0), (0, 0, 0)):
+        i = sb.add_body(PXG_STATIC, p)
+        sb.add_shape(i, PXG_SPHERE, radius=1.0)
+    for p in ((5, 0, 0), (5, 0, 0)):
+        i = sb.add_body(PXG_KINEMATIC, p)
+        sb.add_shape(i, PXG_SPHERE, radius=1.0)
+    w = world(oracle, sb)
+    w.step_fixed()
+    assert len(w.manifolds()) == 0
+
+
+def test_layer_matrix_blocks_pair(oracle):
+    # broadphase_test.cpp:112-134
+    sb = scenes.SceneBuilder()
+    i = sb.add_body(PXG_STATIC, (0, 0, 0))
+    sb.add_shape(i, PXG_SPHERE, radius=1.0, layer=0)
+    sb.dynamic_sphere((0.5, 0, 0), 1.0, layer=1)
+    b, s = sb.arrays()
+    masks = np.full(32, 0xFFFFFFFF, np.uint32)
+    masks[0] &= ~np.uint32(1 << 1)
+    masks[1] &= ~np.uint32(1 << 0)
+    w = oracle.OracleWorld(_abi.default_config(), b, s, layer_masks=masks)
+    w.step_fixed()
+    assert len(w.manifolds()) == 0
+    w2 = oracle.OracleWorld(_abi.default_config(), b, s)
+    w2.step_fixed()
+    assert len(w2.manifolds()) > 0
+
+
+def test_tree_pairs_equal_brute_force_rule(oracle):
+    # broadphase_test.cpp:18-85: the tree finds exactly the brute-force pair set (here, the
+    # reference's asymmetric tight-vs-fat rule) before and after motion
+    b, s = scenes.mixed_pile_scene(n=250, seed=4)
+    c = _abi.default_config()
+    t = oracle.OracleWorld(c, b, s)
+    f = oracle.OracleWorld(c, b, s, brute_force=True)
+    for k in range(90):
+        t.step_fixed()
+        f.step_fixed()
+        pt, pf = t.pairs(), f.pairs()
+        assert np.array_equal(pt, pf), k
+    assert t.state_hash() == f.state_hash()
+
+
+def test_determinism_two_runs(oracle):
+    # physics_world_test.cpp:177-207
+    b, s = scenes.sphere_rain_scene()
+    hs = []
+    for _ in range(2):
+        w = oracle.OracleWorld(_abi.default_config(), b, s)
+        for _ in range(180):
+            w.step_fixed()
+        hs.append(w.state_hash())
+    assert hs[0] == hs[1]
+
+
+def test_restitution_one_bounces_high_zero_does_not(oracle):
+    # contact_solver_test.cpp:19-58
+    mat = dict(restitution=1.0)
+    sb = scenes.SceneBuilder()
+    scenes.ground(sb, mat)
+    sb.dynamic_sphere((0, 0, 2.0), 0.5, **mat)
+    w = world(oracle, sb)
+    bounced, maxz = False, 0.0
+    for _ in range(300):
+        w.step_fixed()
+        b = w.bodies()[1]
+        if b["linear_velocity"][2] > 0:
+            bounced = True
+            maxz = max(maxz, float(b["position"][2]))
+    assert bounced and maxz > 0.5 + 0.6 * 1.5
+    sb = scenes.SceneBuilder()
+    scenes.ground(sb)
+    sb.dynamic_sphere((0, 0, 2.0), 0.5)
+    w = world(oracle, sb)
+    contacted, up = False, 0.0
+    for _ in range(300):
+        w.step_fixed()
+        b = w.bodies()[1]
+        contacted |= b["position"][2] < 0.6
+        if contacted:
+            up = max(up, float(b["linear_velocity"][2]))
+    assert contacted and up < 1.0
+
+
+def test_kinematic_platform_pushes_resting_body(oracle):
+    # contact_solver_test.cpp:60-77
+    sb = scenes.SceneBuilder()
+    i = sb.add_body(PXG_KINEMATIC, (0, 0, 0), velocity=(0, 0, 2.0))
+    sb.add_shape(i, PXG_BOX, half_extents=(2, 2, 0.1))
+    sb.dynamic_sphere((0, 0, 0.6), 0.5)
+    w = world(oracle, sb, gravity=(0, 0, 0))
+    for _ in range(10):
+        w.step_fixed()
+    s = w.bodies()[1]
+    assert s["position"][2] > 0.55 and s["linear_velocity"][2] > 0.5
+
+
+def test_tumbled_box_settles_flat(oracle):
+    # contact_solver_test.cpp:85-98
+    sb = scenes.SceneBuilder()
+    scenes.ground(sb)
+    ax = np.array([1.0, 0.3, 0.0])
+    ax /= np.linalg.norm(ax)
+    sb.dynamic_box((0, 0, 2.0), (0.5, 0.5, 0.5), scenes.quat_axis_angle(ax, math.radians(25)))
+    w = world(oracle, sb)
+    for _ in range(600):
+        w.step_fixed()
+    b = w.bodies()[1]
+    assert (not b["awake"]) or (np.linalg.norm(b["linear_velocity"]) < 0.02 and np.linalg.norm(b["angular_velocity"]) < 0.02)
+    assert best_up_alignment(b["orientation"]) > 0.999
+
+
+def test_fast_box_does_not_tunnel_through_thin_wall(oracle):
+    # contact_solver_test.cpp:117-141 (speculative contacts)
+    sb = scenes.SceneBuilder()
+    sb.static_box((5, 0, 0), (0.05, 2, 2))
+    sb.dynamic_box((4.5, 0, 0), (0.1, 0.1, 0.1), velocity=(50, 0, 0))
+    w = world(oracle, sb, gravity=(0, 0, 0))
+    w.step_fixed()
+    b = w.bodies()[1]
+    assert b["position"][0] < 5.0 and b["linear_velocity"][0] < 50.0
+    for _ in range(300):
+        w.step_fixed()
+    b = w.bodies()[1]
+    assert b["position"][0] < 4.951 and np.linalg.norm(b["linear_velocity"]) < 0.05
+
+
+def test_ten_box_stack_settles_flat_and_sleeps(oracle):
+    # contact_solver_test.cpp:145-162
+    sb = scenes.SceneBuilder()
+    sb.static_box((0, 0, -0.5), (5, 5, 0.5))
+    for i in range(10):
+        sb.dynamic_box((0, 0, 0.5 * (2 * i + 1)), (0.5, 0.5, 0.5))
+    w = world(oracle, sb)
+    for _ in range(1000):
+        w.step_fixed()
+    b = w.bodies()[1:]
+    assert np.all(np.abs(b["position"][:, 0]) < 0.01) and np.all(np.abs(b["position"][:, 1]) < 0.01)
+    assert all(best_up_alignment(q) > 0.999 for q in b["orientation"])
+    assert not np.any(b["awake"])
+
+
+@pytest.mark.parametrize("mu,holds", [(0.8, True), (0.2, False)])
+def test_friction_on_slope(oracle, mu, holds):
+    # contact_solver_test.cpp:164-189
+    rot = scenes.quat_axis_angle((0, 1, 0), math.radians(30))
+    x, y, z, ww = rot
+    n = np.array([2 * (x * z + ww * y), 2 * (y * z - ww * x), 1 - 2 * (x * x + y * y)])
+    tangent = np.array([n[2], 0.0, -n[0]])
+    tangent /= np.linalg.norm(tangent)
+    mat = dict(friction=mu, friction_combine=_abi.PXG_MINIMUM)
+    sb = scenes.SceneBuilder()
+    sb.static_box((0, 0, 0), (5, 5, 0.5), rot, **mat)
+    start = n * 0.75
+    sb.dynamic_box(tuple(start), (0.25, 0.25, 0.25), rot, **mat)
+    w = world(oracle, sb)
+    for _ in range(300):
+        w.step_fixed()
+    slid = float(np.dot(w.bodies()[1]["position"] - start, tangent))
+    if holds:
+        assert abs(slid) < 0.05
+    else:
+        assert abs(slid) > 0.3
+
+
+def test_mass_ratio_100_to_1_stays_stable(oracle):
+    # contact_solver_test.cpp:191-228
+    sb = scenes.SceneBuilder()
+    sb.static_box((0, 0, -0.5), (5, 5, 0.5))
+    sb.dynamic_box((0, 0, 0.5), (0.5, 0.5, 0.5), mass=1.0)
+    sb.dynamic_box((0, 0, 1.5), (0.5, 0.5, 0.5), mass=100.0)
+    w = world(oracle, sb)
+    for _ in range(500):
+        w.step_fixed()
+        b = w.bodies()
+        assert np.all(np.isfinite(b["position"]))
+        assert np.all(np.linalg.norm(b["linear_velocity"], axis=1) < 50)
+    b = w.bodies()
+    assert 0.3 < b["position"][1][2] < 0.7 and 1.3 < b["position"][2][2] < 1.7
+
+
+def test_capsules_rest_at_radius_and_pile(oracle):
+    # contact_solver_test.cpp:232-265
+    sb = scenes.SceneBuilder()
+    sb.static_box((0, 0, -0.5), (5, 5, 0.5))
+    sb.dynamic_capsule((3, 0, 1.5), 0.3, 0.5, axis=0)
+    for i in range(5):
+        sb.dynamic_capsule((0.1 * i, 0, 1.0 + 0.5 * i), 0.25, 0.4, axis=0)
+    w = world(oracle, sb)
+    for _ in range(500):
+        w.step_fixed()
+        b = w.bodies()[2:]
+        assert np.all(np.isfinite(b["position"]))
+        assert np.all(np.linalg.norm(b["linear_velocity"], axis=1) < 50)
+        assert np.all(b["position"][:, 2] > -1.0)
+    lone = w.bodies()[1]
+    assert abs(lone["position"][2] - 0.3) < 0.05
+    assert np.linalg.norm(lone["linear_velocity"]) < 0.1
+
+
+def test_resting_sphere_sleeps_at_rest_height(oracle):
+    # sleep_test.cpp:19-36
+    sb = scenes.SceneBuilder()
+    scenes.ground(sb)
+    sb.dynamic_sphere((0, 0, 2.0), 0.5)
+    w = world(oracle, sb)
+    slept = False
+    for _ in range(400):
+        w.step_fixed()
+        if not w.bodies()[1]["awake"]:
+            slept = True
+            break
+    assert slept
+    b = w.bodies()[1]
+    assert abs(b["position"][2] - 0.5) < 0.05
+    assert np.linalg.norm(b["linear_velocity"]) < 0.01 + 1e-4
+
+
+def test_moving_kinematic_platform_wakes_sleeping_body(oracle):
+    # sleep_test.cpp:47-66
+    sb = scenes.SceneBuilder()
+    i = sb.add_body(PXG_KINEMATIC, (0, 0, -0.5))
+    sb.add_shape(i, PXG_BOX, half_extents=(2, 2, 0.5))
+    sb.dynamic_sphere((0, 0, 1.0), 0.5)
+    w = world(oracle, sb)
+    for _ in range(120):
+        w.step_fixed()
+    b = w.bodies()
+    assert not b[1]["awake"]
+    b[0]["linear_velocity"] = (3.0, 0, 0)
+    w.set_bodies(b)
+    w.step_fixed()
+    assert w.bodies()[1]["awake"]
+
+
+def test_sleeping_body_ignores_awake_neighbour(oracle):
+    # sleep_test.cpp:103-124
+    sb = scenes.SceneBuilder()
+    sb.dynamic_sphere((0, 0, 0), 0.5, awake=False)
+    sb.dynamic_sphere((0.9, 0, 0), 0.5, velocity=(-2, 0, 0))
+    w = world(oracle, sb, gravity=(0, 0, 0))
+    w.step_fixed()
+    s = w.bodies()[0]
+    assert not s["awake"]
+    np.testing.assert_allclose(s["linear_velocity"], 0, atol=1e-6)
+    np.testing.assert_allclose(s["angular_velocity"], 0, atol=1e-6)
+
+
+def test_resting_sphere_impulse_is_m_g_h_and_residual_zero(oracle):
+    # closed form: a resting unit sphere's normal impulse per step cancels gravity, m*g*h
+    sb = scenes.SceneBuilder()
+    scenes.ground(sb)
+    sb.dynamic_sphere((0, 0, 0.5), 0.5)
+    w = world(oracle, sb, sleep_time=1e9)
+    for _ in range(60):
+        w.step_fixed()
+    m = w.manifolds()
+    assert len(m) == 1
+    assert abs(m[0]["points"][0]["normal_impulse"] - 9.81 / 60.0) < 1e-4
+    # residual mode: from the warm start a resting contact converges in one pass
+    c = _abi.default_config(residual_mode=1, sleep_time=1e9)
+    w.set_config(c)
+    w.step_fixed()
+    st = w.stats()
+    assert st["velocity_passes"] == 1 and st["residual"] < 1e-6
+
+
+def test_double_precision_run_tracks_float(oracle):
+    """The FP64 oracle on config 1 matches FP32 through the free fall (before any contact) and
+    keeps the 300-step aggregates close: the error reference for FP32 runs."""
+    sc = scenes.config1_scene(seed=1)
+    c = _abi.default_config(**sc.config)
+    f = oracle.OracleWorld(c, sc.bodies, sc.shapes)
+    d = oracle.OracleWorld(c, sc.bodies, sc.shapes, dbl=True)
+    for _ in range(20):
+        f.step_fixed()
+        d.step_fixed()
+    diff = np.abs(f.bodies()["position"].astype(np.float64) - d.bodies()["position"]).max()
+    assert diff < 1e-5
+    for _ in range(280):
+        f.step_fixed()
+        d.step_fixed()
+    bf, bd = f.bodies(), d.bodies()
+    # settled: every box rests in a 4-high column; mean height and max speed agree
+    assert abs(bf["position"][1:, 2].mean() - bd["position"][1:, 2].mean()) < 0.05
+    assert np.linalg.norm(bf["linear_velocity"], axis=1).max() < 0.1
+    assert np.linalg.norm(bd["linear_velocity"], axis=1).max() < 0.1
