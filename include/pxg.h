@@ -253,6 +253,36 @@ int pxg_get_profile(pxg_world* w, pxg_profile* out);
 const char* pxg_last_error(void);
 const char* pxg_version(void);
 
+/* ---- batched stepping of many independent scenes ----
+ * Scenes are concatenated: scene k owns bodies body_offsets[k] .. body_offsets[k+1] and shapes
+ * shape_offsets[k] .. shape_offsets[k+1]; a shape's owner is a scene-local body index. Each
+ * scene behaves exactly like its own PhysicsWorld (world.h) stepped with step_fixed(h); one
+ * pxg_batch_step call advances every scene `steps` times in one persistent launch. At most 288
+ * bodies and 288 shapes per scene. No counterpart in the reference (it steps one world). */
+typedef struct pxg_batch pxg_batch;
+typedef struct pxg_batch_scene_stats {
+    uint32_t pairs, manifolds, points, colours;
+    uint32_t active_manifolds, velocity_passes, error, ordered;
+    float residual;
+    uint32_t awake_bodies;
+    uint32_t reserved[2];
+} pxg_batch_scene_stats;
+int pxg_batch_create(const pxg_config* cfg, uint32_t n_scenes, const uint32_t* body_offsets,
+                     const uint32_t* shape_offsets, const pxg_body* bodies, const pxg_shape* shapes,
+                     pxg_batch** out);
+int pxg_batch_destroy(pxg_batch* b);
+/* block_size 64/128/256/512 threads per scene; longest_first = take scenes by decreasing size */
+int pxg_batch_set_tuning(pxg_batch* b, uint32_t block_size, uint32_t longest_first);
+int pxg_batch_set_layer_masks(pxg_batch* b, const uint32_t masks[32]);
+int pxg_batch_step(pxg_batch* b, float h, uint32_t steps);
+int pxg_batch_synchronize(pxg_batch* b);
+int pxg_batch_get_bodies(pxg_batch* b, pxg_body* out, uint32_t n);
+int pxg_batch_get_stats(pxg_batch* b, pxg_batch_scene_stats* out, uint32_t n);
+int pxg_batch_get_manifolds(pxg_batch* b, uint32_t scene, pxg_manifold* out, uint32_t cap, uint32_t* n);
+int pxg_batch_get_solve_order(pxg_batch* b, uint32_t scene, uint32_t* order, uint32_t cap, uint32_t* n,
+                              uint32_t* colour_offsets, uint32_t colour_cap, uint32_t* n_colours);
+const char* pxg_batch_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,14 @@ MANIFOLD_DTYPE = np.dtype([
 assert MANIFOLD_DTYPE.itemsize == 240
 
 
+BATCH_STATS_DTYPE = np.dtype([
+    ("pairs", "<u4"), ("manifolds", "<u4"), ("points", "<u4"), ("colours", "<u4"),
+    ("active_manifolds", "<u4"), ("velocity_passes", "<u4"), ("error", "<u4"), ("ordered", "<u4"),
+    ("residual", "<f4"), ("awake_bodies", "<u4"), ("reserved", "<u4", 2),
+])
+assert BATCH_STATS_DTYPE.itemsize == 48
+
+
 class Config(ctypes.Structure):
     """pxg_config (include/pxg.h)."""
     _fields_ = [

@@ -36,6 +36,20 @@ _SIGNATURES = {
     "pxg_set_mlp": ([ctypes.c_void_p] * 8, ctypes.c_int),
     "pxg_set_profiling": ([ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_profile": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_create": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                          ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_set_tuning": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
+    "pxg_batch_set_layer_masks": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_step": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_uint32], ctypes.c_int),
+    "pxg_batch_synchronize": ([ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_get_bodies": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_batch_get_stats": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_batch_get_manifolds": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p],
+                                ctypes.c_int),
+    "pxg_batch_get_solve_order": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_last_error": ([], ctypes.c_char_p),
     "pxg_last_error": ([], ctypes.c_char_p),
     "pxg_version": ([], ctypes.c_char_p),
 }
@@ -64,4 +78,10 @@ def lib():
 def check(rc, what=""):
     if rc != 0:
         msg = lib().pxg_last_error().decode()
+        raise PxgError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def check_batch(rc, what=""):
+    if rc != 0:
+        msg = lib().pxg_batch_last_error().decode()
         raise PxgError("%s failed (%d): %s" % (what, rc, msg))

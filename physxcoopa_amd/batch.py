@@ -1,0 +1,84 @@
+"""Batched stepping of many independent scenes (pxg_batch_* in include/pxg.h).
+
+Each scene behaves like its own PhysicsWorld stepped with step_fixed(h); one `step(k)` call
+advances every scene k times in one persistent, graph-captured launch.
+"""
+import ctypes
+
+import numpy as np
+
+from ._abi import BODY_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, BATCH_STATS_DTYPE, ptr
+from ._lib import lib, check_batch
+
+
+class Batch:
+    def __init__(self, config, bodies, shapes, body_offsets, shape_offsets, layer_masks=None):
+        self._cfg = config
+        self.body_offsets = np.ascontiguousarray(np.asarray(body_offsets, np.uint32))
+        self.shape_offsets = np.ascontiguousarray(np.asarray(shape_offsets, np.uint32))
+        self.n_scenes = len(self.body_offsets) - 1
+        b = np.ascontiguousarray(bodies, dtype=BODY_DTYPE)
+        s = np.ascontiguousarray(shapes, dtype=SHAPE_DTYPE)
+        self.n_bodies = len(b)
+        h = ctypes.c_void_p()
+        check_batch(lib().pxg_batch_create(ctypes.byref(config), self.n_scenes, ptr(self.body_offsets),
+                                           ptr(self.shape_offsets), ptr(b), ptr(s), ctypes.byref(h)), "pxg_batch_create")
+        self.h = h
+        if layer_masks is not None:
+            m = np.ascontiguousarray(np.asarray(layer_masks, np.uint32))
+            check_batch(lib().pxg_batch_set_layer_masks(self.h, ptr(m)), "pxg_batch_set_layer_masks")
+        self.tuning = (256, True)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pxg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, block_size, longest_first=True):
+        check_batch(lib().pxg_batch_set_tuning(self.h, block_size, 1 if longest_first else 0), "pxg_batch_set_tuning")
+        self.tuning = (block_size, bool(longest_first))
+
+    def step(self, steps=1, h=None):
+        check_batch(lib().pxg_batch_step(self.h, np.float32(self._cfg.fixed_dt if h is None else h), steps),
+                    "pxg_batch_step")
+
+    def synchronize(self):
+        check_batch(lib().pxg_batch_synchronize(self.h), "pxg_batch_synchronize")
+
+    def bodies(self):
+        out = np.zeros(self.n_bodies, BODY_DTYPE)
+        check_batch(lib().pxg_batch_get_bodies(self.h, ptr(out), self.n_bodies), "pxg_batch_get_bodies")
+        return out
+
+    def scene_bodies(self, k, all_bodies=None):
+        b = self.bodies() if all_bodies is None else all_bodies
+        return b[self.body_offsets[k]:self.body_offsets[k + 1]]
+
+    def stats(self):
+        out = np.zeros(self.n_scenes, BATCH_STATS_DTYPE)
+        check_batch(lib().pxg_batch_get_stats(self.h, ptr(out), self.n_scenes), "pxg_batch_get_stats")
+        return out
+
+    def manifolds(self, scene):
+        n = ctypes.c_uint32()
+        check_batch(lib().pxg_batch_get_manifolds(self.h, scene, None, 0, ctypes.byref(n)), "pxg_batch_get_manifolds")
+        out = np.zeros(max(n.value, 1), MANIFOLD_DTYPE)
+        check_batch(lib().pxg_batch_get_manifolds(self.h, scene, ptr(out), n.value, ctypes.byref(n)),
+                    "pxg_batch_get_manifolds")
+        return out[:n.value]
+
+    def solve_order(self, scene):
+        n, nc = ctypes.c_uint32(), ctypes.c_uint32()
+        check_batch(lib().pxg_batch_get_solve_order(self.h, scene, None, 0, ctypes.byref(n), None, 0,
+                                                    ctypes.byref(nc)), "order")
+        order = np.zeros(max(n.value, 1), np.uint32)
+        offs = np.zeros(nc.value + 1, np.uint32)
+        check_batch(lib().pxg_batch_get_solve_order(self.h, scene, ptr(order), n.value, ctypes.byref(n), ptr(offs),
+                                                    nc.value + 1, ctypes.byref(nc)), "pxg_batch_get_solve_order")
+        return order[:n.value], offs

@@ -172,7 +172,9 @@ __global__ void k_integrate_forces(DevState d, StepParams p) {
         Counters* c = d.cnt;
         c->pairs = 0; c->manifolds = 0; c->points = 0; c->active = 0; c->ordered = 0; c->remaining = 0;
         c->colours = 0; c->n_large = 0; c->awake = 0; c->predicted = 0; c->residual_bits = 0;
-        c->red[0] = 0xFFFFFFFFu; c->red[1] = 0xFFFFFFFFu; c->red[2] = 0u; c->red[3] = 0u; c->red[4] = 0u; c->red[5] = 0u;
+        for (int k = 0; k < 3; ++k) c->red[k] = 0xFFFFFFFFu;
+        for (int k = 3; k < 12; ++k) c->red[k] = 0u;
+        c->passes = 0; c->res_slot[0] = 0u; c->res_slot[1] = 0u;
     }
     if (i >= d.nb) return;
     uint32_t f = d.b_flags[i];
@@ -230,91 +232,110 @@ __global__ void k_init_proxies(DevState d, StepParams p) {
 // list; reductions for the grid over small shapes' fat AABBs.
 __global__ void k_bounds(DevState d, StepParams p) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t r0 = 0xFFFFFFFFu, r1 = 0xFFFFFFFFu, r2 = 0u, r3 = 0u, r4 = 0u, r5 = 0u;
+    // min x, y, z and max x, y, z of fat min corners; max extents x, y, z
+    uint32_t rmn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, rmx[3] = {0u, 0u, 0u}, rex[3] = {0u, 0u, 0u};
     if (i < d.ns) {
-    uint4 info = d.s_info[i];
-    if (!shape_live(d, info)) {
-        d.s_tmin[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-    } else {
-    ShapeGeom s = load_shape(d, i, info);
-    v3 mn, mx;
-    uint32_t ob = info.x;
-    world_bounds(s, ld3(d.b_pos[ob]), ldq(d.b_rot[ob]), mn, mx);
-    v3 fmn = ld3(d.s_fmin[i]), fmx = ld3(d.s_fmax[i]);
-    if (!(info.y & SF_STATIC)) {
-        v3 disp = ld3(d.b_vel[ob]) * p.h;
-        v3 smn = vmin(mn, mn + disp), smx = vmax(mx, mx + disp);
-        bool contains = smn.x >= fmn.x && smn.x <= fmx.x && smn.y >= fmn.y && smn.y <= fmx.y && smn.z >= fmn.z &&
-                        smn.z <= fmx.z && smx.x >= fmn.x && smx.x <= fmx.x && smx.y >= fmn.y && smx.y <= fmx.y &&
-                        smx.z >= fmn.z && smx.z <= fmx.z;
-        if (!contains) {
-            fmn = smn - splat(p.margin);
-            fmx = smx + splat(p.margin);
-            d.s_fmin[i] = st3(fmn, 0.0f);
-            d.s_fmax[i] = st3(fmx, 0.0f);
+        uint4 info = d.s_info[i];
+        if (!shape_live(d, info)) {
+            d.s_tmin[i] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        } else {
+            ShapeGeom s = load_shape(d, i, info);
+            v3 mn, mx;
+            uint32_t ob = info.x;
+            world_bounds(s, ld3(d.b_pos[ob]), ldq(d.b_rot[ob]), mn, mx);
+            v3 fmn = ld3(d.s_fmin[i]), fmx = ld3(d.s_fmax[i]);
+            if (!(info.y & SF_STATIC)) {
+                v3 disp = ld3(d.b_vel[ob]) * p.h;
+                v3 smn = vmin(mn, mn + disp), smx = vmax(mx, mx + disp);
+                bool contains = smn.x >= fmn.x && smn.x <= fmx.x && smn.y >= fmn.y && smn.y <= fmx.y &&
+                                smn.z >= fmn.z && smn.z <= fmx.z && smx.x >= fmn.x && smx.x <= fmx.x &&
+                                smx.y >= fmn.y && smx.y <= fmx.y && smx.z >= fmn.z && smx.z <= fmx.z;
+                if (!contains) {
+                    fmn = smn - splat(p.margin);
+                    fmx = smx + splat(p.margin);
+                    d.s_fmin[i] = st3(fmn, 0.0f);
+                    d.s_fmax[i] = st3(fmx, 0.0f);
+                }
+            }
+            float ex = fmx.x - fmn.x, ey = fmx.y - fmn.y, ez = fmx.z - fmn.z;
+            bool large = ex > p.large_extent || ey > p.large_extent || ez > p.large_extent;
+            d.s_tmin[i] = st3(mn, large ? 1.0f : 0.0f);
+            d.s_tmax[i] = st3(mx, 0.0f);
+            if (large) {
+                uint32_t k = atomicAdd(&d.cnt->n_large, 1u);
+                if (k < uint32_t(kMaxLarge)) d.large_list[k] = i;
+                else atomicOr(&d.cnt->err, ERR_LARGE);
+            } else {
+                rmn[0] = rmx[0] = f2o(fmn.x);
+                rmn[1] = rmx[1] = f2o(fmn.y);
+                rmn[2] = rmx[2] = f2o(fmn.z);
+                rex[0] = f2o(ex); rex[1] = f2o(ey); rex[2] = f2o(ez);
+            }
         }
     }
-    float ex = fmx.x - fmn.x, ey = fmx.y - fmn.y;
-    bool large = ex > p.large_extent || ey > p.large_extent;
-    d.s_tmin[i] = st3(mn, large ? 1.0f : 0.0f);
-    d.s_tmax[i] = st3(mx, 0.0f);
-    Counters* c = d.cnt;
-    if (large) {
-        uint32_t k = atomicAdd(&c->n_large, 1u);
-        if (k < uint32_t(kMaxLarge)) d.large_list[k] = i;
-        else atomicOr(&c->err, ERR_LARGE);
-    } else {
-        r0 = f2o(fmn.x); r1 = f2o(fmn.y); r2 = r0; r3 = r1; r4 = f2o(ex); r5 = f2o(ey);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        rmn[k] = wave_min(rmn[k]);
+        rmx[k] = wave_max(rmx[k]);
+        rex[k] = wave_max(rex[k]);
     }
-    }
-    }
-    r0 = wave_min(r0); r1 = wave_min(r1); r2 = wave_max(r2); r3 = wave_max(r3); r4 = wave_max(r4); r5 = wave_max(r5);
-    if (lane0() && r2 != 0u) {
+    if (lane0() && rmx[0] != 0u) {
         Counters* c = d.cnt;
-        atomicMin(&c->red[0], r0); atomicMin(&c->red[1], r1);
-        atomicMax(&c->red[2], r2); atomicMax(&c->red[3], r3);
-        atomicMax(&c->red[4], r4); atomicMax(&c->red[5], r5);
+        for (int k = 0; k < 3; ++k) {
+            atomicMin(&c->red[k], rmn[k]);
+            atomicMax(&c->red[3 + k], rmx[k]);
+            atomicMax(&c->red[6 + k], rex[k]);
+        }
     }
 }
 
+// 3-D uniform grid over the small shapes' fat-min corners. The cell is at least the largest fat
+// extent, so every candidate partner's min corner lies in a bounded block of cells.
 __global__ void k_grid_setup(DevState d) {
     Counters* c = d.cnt;
-    float ox = 0.0f, oy = 0.0f, cell = 1.0f, exty = 0.0f;
-    int nx = 1, ny = 1;
+    float o[3] = {0.0f, 0.0f, 0.0f}, ext[3] = {0.0f, 0.0f, 0.0f};
+    int n[3] = {1, 1, 1};
+    float cell = 1.0f;
     if (c->red[0] != 0xFFFFFFFFu) {
-        ox = o2f(c->red[0]);
-        oy = o2f(c->red[1]);
-        float mxx = o2f(c->red[2]), mxy = o2f(c->red[3]);
-        float ex = o2f(c->red[4]);
-        exty = o2f(c->red[5]);
-        cell = fmaxf(fmaxf(ex, exty), 1e-3f);
+        float mx[3];
+        for (int k = 0; k < 3; ++k) {
+            o[k] = o2f(c->red[k]);
+            mx[k] = o2f(c->red[3 + k]);
+            ext[k] = o2f(c->red[6 + k]);
+        }
+        cell = fmaxf(fmaxf(fmaxf(ext[0], ext[1]), ext[2]), 1e-3f);
         for (int it = 0; it < 64; ++it) {
-            float fx = floorf((mxx - ox) / cell) + 1.0f, fy = floorf((mxy - oy) / cell) + 1.0f;
-            if (fx * fy <= float(kMaxCells)) { nx = int(fx); ny = int(fy); break; }
+            float f[3];
+            for (int k = 0; k < 3; ++k) f[k] = floorf((mx[k] - o[k]) / cell) + 1.0f;
+            if (f[0] * f[1] * f[2] <= float(kMaxCells)) {
+                for (int k = 0; k < 3; ++k) n[k] = int(f[k]);
+                break;
+            }
             cell *= 2.0f;
         }
     }
-    c->grid[0] = ox; c->grid[1] = oy; c->grid[2] = 1.0f / cell; c->grid[3] = exty;
-    c->grid[4] = float(nx); c->grid[5] = float(ny);
+    c->grid[0] = o[0]; c->grid[1] = o[1]; c->grid[2] = o[2]; c->grid[3] = 1.0f / cell;
+    c->grid[4] = ext[1]; c->grid[5] = ext[2];
+    c->grid[6] = float(n[0]); c->grid[7] = float(n[1]); c->grid[8] = float(n[2]);
 }
 
 struct Grid {
-    float ox, oy, inv, exty;
-    int nx, ny;
+    float o[3], inv, exty, extz;
+    int n[3];
 };
 __device__ __forceinline__ Grid load_grid(const Counters* c) {
     Grid g;
-    g.ox = c->grid[0]; g.oy = c->grid[1]; g.inv = c->grid[2]; g.exty = c->grid[3];
-    g.nx = int(c->grid[4]); g.ny = int(c->grid[5]);
+    g.o[0] = c->grid[0]; g.o[1] = c->grid[1]; g.o[2] = c->grid[2]; g.inv = c->grid[3];
+    g.exty = c->grid[4]; g.extz = c->grid[5];
+    g.n[0] = int(c->grid[6]); g.n[1] = int(c->grid[7]); g.n[2] = int(c->grid[8]);
     return g;
 }
-__device__ __forceinline__ int cellx(const Grid& g, float x) {
-    float f = floorf((x - g.ox) * g.inv);
-    return f < 0.0f ? 0 : (f > float(g.nx - 1) ? g.nx - 1 : int(f));
+__device__ __forceinline__ int cell_of(const Grid& g, int k, float x) {
+    float f = floorf((x - g.o[k]) * g.inv);
+    return f < 0.0f ? 0 : (f > float(g.n[k] - 1) ? g.n[k] - 1 : int(f));
 }
-__device__ __forceinline__ int celly(const Grid& g, float y) {
-    float f = floorf((y - g.oy) * g.inv);
-    return f < 0.0f ? 0 : (f > float(g.ny - 1) ? g.ny - 1 : int(f));
+__device__ __forceinline__ int cell_index(const Grid& g, v3 p) {
+    return (cell_of(g, 2, p.z) * g.n[1] + cell_of(g, 1, p.y)) * g.n[0] + cell_of(g, 0, p.x);
 }
 
 __global__ void k_grid_clear(DevState d) {
@@ -326,11 +347,9 @@ __global__ void k_grid_clear(DevState d) {
 __global__ void k_grid_count(DevState d) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.ns) return;
-    float4 t = d.s_tmin[i];
-    if (t.w != 0.0f) return;  // inactive (-1) or large (1)
+    if (d.s_tmin[i].w != 0.0f) return;  // inactive (-1) or large (1)
     Grid g = load_grid(d.cnt);
-    float4 f = d.s_fmin[i];
-    atomicAdd(&d.cell_count[celly(g, f.y) * g.nx + cellx(g, f.x)], 1u);
+    atomicAdd(&d.cell_count[cell_index(g, ld3(d.s_fmin[i]))], 1u);
 }
 __global__ void k_grid_scatter(DevState d) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -339,8 +358,14 @@ __global__ void k_grid_scatter(DevState d) {
     if (t.w != 0.0f) return;
     Grid g = load_grid(d.cnt);
     float4 f = d.s_fmin[i];
-    int c = celly(g, f.y) * g.nx + cellx(g, f.x);
-    d.cell_items[d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1u)] = i;
+    int c = cell_index(g, ld3(f));
+    uint32_t slot = d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1u);
+    uint4 info = d.s_info[i];
+    d.ci_fmin[slot] = make_float4(f.x, f.y, f.z, __uint_as_float(i));
+    float4 fx = d.s_fmax[i];
+    d.ci_fmax[slot] = make_float4(fx.x, fx.y, fx.z, __uint_as_float(info.x));
+    d.ci_tmin[slot] = make_float4(t.x, t.y, t.z, (info.y & SF_STATIC) ? 1.0f : 0.0f);
+    d.ci_tmax[slot] = d.s_tmax[i];
 }
 
 // The reference pair rule (world.h:1545-1577 with aabb.h:35-39): owners differ, not both static,
@@ -371,15 +396,6 @@ __global__ void k_pairs(DevState d) {
     if (MODE == 1) off = d.pair_off[i];
     const Counters* c = d.cnt;
 
-    auto test = [&](uint32_t j) -> bool {
-        uint4 ij = d.s_info[j];
-        if (ij.x == own_i) return false;
-        if (st_i && (ij.y & SF_STATIC)) return false;
-        v3 jfmn = ld3(d.s_fmin[j]), jfmx = ld3(d.s_fmax[j]);
-        if (ovl(tmn, tmx, jfmn, jfmx)) return true;
-        v3 jtmn = ld3(d.s_tmin[j]), jtmx = ld3(d.s_tmax[j]);
-        return ovl(jtmn, jtmx, fmn, fmx);
-    };
     auto emit = [&](uint32_t j) {
         if (MODE == 1) {
             uint32_t k = off + n;
@@ -390,19 +406,26 @@ __global__ void k_pairs(DevState d) {
 
     if (!large_i) {
         Grid g = load_grid(c);
-        int cx0 = cellx(g, fmn.x), cx1 = cellx(g, fmx.x);
-        int cy0 = celly(g, fmn.y - g.exty), cy1 = celly(g, fmx.y);
-        for (int cy = cy0; cy <= cy1; ++cy) {
-            for (int cx = cx0; cx <= cx1; ++cx) {
-                int cc = cy * g.nx + cx;
-                uint32_t s0 = d.cell_start[cc], s1 = s0 + d.cell_count[cc];
+        int x0 = cell_of(g, 0, fmn.x), x1 = cell_of(g, 0, fmx.x);
+        int y0 = cell_of(g, 1, fmn.y - g.exty), y1 = cell_of(g, 1, fmx.y);
+        int z0 = cell_of(g, 2, fmn.z - g.extz), z1 = cell_of(g, 2, fmx.z);
+        for (int cz = z0; cz <= z1; ++cz) {
+            for (int cy = y0; cy <= y1; ++cy) {
+                int row = (cz * g.n[1] + cy) * g.n[0];
+                // cells x0..x1 of one row are contiguous in the cell-ordered arrays
+                uint32_t s0 = d.cell_start[row + x0];
+                uint32_t s1 = d.cell_start[row + x1] + d.cell_count[row + x1];
                 for (uint32_t sl = s0; sl < s1; ++sl) {
-                    uint32_t j = d.cell_items[sl];
-                    if (j == i) continue;
-                    float jx = d.s_fmin[j].x;
-                    if (!(jx > fmn.x || (jx == fmn.x && j > i))) continue;
-                    if (jx > fmx.x) continue;
-                    if (test(j)) emit(j);
+                    float4 jf = d.ci_fmin[sl];
+                    uint32_t j = __float_as_uint(jf.w);
+                    if (!(jf.x > fmn.x || (jf.x == fmn.x && j > i))) continue;
+                    if (jf.x > fmx.x) continue;
+                    float4 jF = d.ci_fmax[sl];
+                    if (__float_as_uint(jF.w) == own_i) continue;
+                    float4 jt = d.ci_tmin[sl];
+                    if (st_i && jt.w != 0.0f) continue;
+                    float4 jT = d.ci_tmax[sl];
+                    if (ovl(tmn, tmx, ld3(jf), ld3(jF)) || ovl(ld3(jt), ld3(jT), fmn, fmx)) emit(j);
                 }
             }
         }
@@ -412,7 +435,10 @@ __global__ void k_pairs(DevState d) {
         uint32_t j = d.large_list[k];
         if (j == i) continue;
         if (large_i && j < i) continue;
-        if (test(j)) emit(j);
+        uint4 ij = d.s_info[j];
+        if (ij.x == own_i) continue;
+        if (st_i && (ij.y & SF_STATIC)) continue;
+        if (ovl(tmn, tmx, ld3(d.s_fmin[j]), ld3(d.s_fmax[j])) || ovl(ld3(d.s_tmin[j]), ld3(d.s_tmax[j]), fmn, fmx)) emit(j);
     }
     if (MODE == 0) {
         d.pair_count[i] = n;
@@ -711,10 +737,8 @@ __device__ __forceinline__ void apply_pair(SBody& a, SBody& b, v3 p, v3 J) {
     if (b.act) { b.v = b.v + J * b.im; b.w = b.w + mul(b.I, cross(p - b.x, J)); }
 }
 
-// warm_start's impulse application (solver.h:164-173), one colour
-__global__ void k_warm_apply(DevState d, uint32_t begin, uint32_t end) {
-    uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= end) return;
+// warm_start's impulse application (solver.h:164-173) for solver manifold s
+__device__ void warm_manifold(const DevState& d, uint32_t s) {
     uint2 ab = d.sm_ab[s];
     SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
     v3 n = ld3(d.sm_nf[s]);
@@ -728,6 +752,10 @@ __global__ void k_warm_apply(DevState d, uint32_t begin, uint32_t end) {
     }
     store_sbody(d, ab.x, A);
     store_sbody(d, ab.y, B);
+}
+__global__ void k_warm_apply(DevState d, uint32_t begin, uint32_t end) {
+    uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < end) warm_manifold(d, s);
 }
 
 // solve_velocity_pass (solver.h:212-281), one colour
@@ -795,11 +823,12 @@ __device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint
     store_sbody(d, ab.y, B);
 }
 
-// velocity residual (m/s) over the solver's active rows, max-reduced into cnt->residual_bits
-__global__ void k_residual(DevState d, StepParams prm, uint32_t n_active, int use_rest) {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+// Velocity residual (m/s) of solver manifold s: normal rows measure the complementarity
+// violation of (lambda_n, v_n - target); friction rows the violation of stick (|lambda_t| <
+// mu lambda_n needs v_t = 0) or slip (the friction impulse must oppose v_t).
+__device__ float residual_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest) {
     float r = 0.0f;
-    if (s < n_active) {
+    {
         uint2 ab = d.sm_ab[s];
         SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
         float4 nf = d.sm_nf[s];
@@ -826,7 +855,12 @@ __global__ void k_residual(DevState d, StepParams prm, uint32_t n_active, int us
             }
         }
     }
-    // wave max then one atomic per wave
+    return r;
+}
+// max-reduces the residual of the active solver manifolds into cnt->residual_bits
+__global__ void k_residual(DevState d, StepParams prm, uint32_t n_active, int use_rest) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    float r = s < n_active ? residual_manifold(d, prm, s, use_rest) : 0.0f;
     for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
     if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(&d.cnt->residual_bits, __float_as_uint(r));
 }
@@ -840,10 +874,8 @@ __global__ void k_position_setup(DevState d) {
     d.sb_x[i] = make_float4(p.x, p.y, p.z, d.sb_v[i].w);
     d.sb_ref[i] = p;
 }
-// solve_position, one colour (solver.h:298-327)
-__global__ void k_position(DevState d, StepParams prm, uint32_t begin, uint32_t end) {
-    uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= end) return;
+// solve_position for solver manifold s (solver.h:298-327)
+__device__ void position_manifold(const DevState& d, const StepParams& prm, uint32_t s) {
     uint2 ab = d.sm_ab[s];
     float4 xa4 = d.sb_x[ab.x], xb4 = d.sb_x[ab.y];
     bool aa = d.sb_w[ab.x].w != 0.0f, ba = d.sb_w[ab.y].w != 0.0f;
@@ -867,6 +899,10 @@ __global__ void k_position(DevState d, StepParams prm, uint32_t begin, uint32_t 
     }
     if (aa) d.sb_x[ab.x] = st3(xa, ima);
     if (ba) d.sb_x[ab.y] = st3(xb, imb);
+}
+__global__ void k_position(DevState d, StepParams prm, uint32_t begin, uint32_t end) {
+    uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < end) position_manifold(d, prm, s);
 }
 
 // write solver bodies back; rebuild the warm-start cache (solver.h:767-781): insert in solve
@@ -999,11 +1035,23 @@ __global__ void k_kin_wake(DevState d, StepParams p, uint32_t n_man) {
         }
     }
 }
+// per-island minimum sleep timer; lanes of a wave that share a root (one big island is common)
+// combine their minimum before a single atomic
 __global__ void k_island_min(DevState d) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= d.nb) return;
-    if (!body_active(d.b_flags[i])) return;
-    atomicMin(&d.isl_min[d.uf_parent[i]], __float_as_uint(d.b_pos[i].w));
+    bool act = i < d.nb && body_active(d.b_flags[i]);
+    uint32_t root = act ? d.uf_parent[i] : 0xFFFFFFFFu;
+    uint32_t val = act ? __float_as_uint(d.b_pos[i].w) : 0xFFFFFFFFu;
+    unsigned long long pending = __ballot(act);
+    int lane = threadIdx.x & 63;
+    while (pending) {
+        int leader = __ffsll(static_cast<long long>(pending)) - 1;
+        uint32_t r = __shfl(root, leader, 64);
+        bool mine = act && root == r;
+        uint32_t m = wave_min(mine ? val : 0xFFFFFFFFu);
+        if (lane == leader) atomicMin(&d.isl_min[r], m);
+        pending &= ~__ballot(mine);
+    }
 }
 __global__ void k_sleep(DevState d, StepParams p) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1069,6 +1117,205 @@ __global__ void k_integrate_velocities(DevState d, StepParams p) {
         m3 z;
         z.c0 = splat(0.0f); z.c1 = splat(0.0f); z.c2 = splat(0.0f);
         store_iiw(d.b_iiw, i, z);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// persistent kernels: one cooperative launch per solve (and per colouring), grid-wide barriers
+// between colours. Every wait is bounded: a barrier that does not complete within the spin
+// budget raises ERR_BARRIER and releases all waiters, so the kernel always terminates.
+// ------------------------------------------------------------------------------------------
+constexpr uint32_t ERR_BARRIER = 64u;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+struct GridBarrier {
+    uint32_t* bar;     // [0] arrivals, [1] generation
+    uint32_t nblocks;
+    uint32_t gen;      // generation this block expects next (thread 0's copy)
+};
+__device__ __forceinline__ void grid_sync(GridBarrier& g, uint32_t* err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        uint32_t arrived = __hip_atomic_fetch_add(&g.bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == g.nblocks - 1) {
+            __hip_atomic_store(&g.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&g.bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&g.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g.gen) {
+                if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    atomicOr(err, ERR_BARRIER);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        g.gen += 1;
+        __threadfence();
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ GridBarrier grid_barrier_init(uint32_t* bar) {
+    GridBarrier g;
+    g.bar = bar;
+    g.nblocks = gridDim.x;
+    g.gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return g;
+}
+
+__device__ __forceinline__ uint32_t n_active_all(const DevState& d) { return d.cnt->active; }
+
+struct SolveCfg {
+    uint32_t vel_iters, relax_iters, pos_iters;
+    uint32_t residual_mode, residual_max;
+    float residual_tol;
+};
+
+// The whole solve of one step (dynamics/solver.h:914-958 after the restitution capture):
+// warm start, velocity passes (fixed count or until the residual <= tolerance), relax passes,
+// position passes, colour by colour with a grid barrier between colours.
+__global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepParams prm, SolveCfg cfg, uint32_t* bar) {
+    GridBarrier g = grid_barrier_init(bar);
+    uint32_t* err = &d.cnt->err;
+    const uint32_t C = d.cnt->colours;
+    const uint32_t* co = d.cnt->colour_off;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
+    for (uint32_t c = 0; c < C; ++c) {
+        for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) warm_manifold(d, s);
+        grid_sync(g, err);
+    }
+    if (prm.profile) {
+        // algorithmic bytes of the solve: warm start 192 + 40n per manifold (bodies r/w and the
+        // points' positions and impulses), each position pass 92 + 4n; the velocity and relax
+        // passes add 192 + 40n each below
+        uint32_t bytes = 0;
+        for (uint32_t s = tid; s < n_active_all(d); s += nthreads) {
+            uint32_t n = d.sm_cnt[s];
+            bytes += 192u + 40u * n + cfg.pos_iters * (92u + 4u * n);
+        }
+        bytes = wave_sum(bytes);
+        if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+    }
+    uint32_t passes = 0;
+    __shared__ float s_red[kBlock / 64];
+    const uint32_t n_active = d.cnt->active;
+    uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : cfg.vel_iters;
+    for (uint32_t it = 0; it < max_pass; ++it) {
+        for (uint32_t c = 0; c < C; ++c) {
+            uint32_t b = co[c], e = co[c + 1];
+            if (prm.profile) {
+                uint32_t bytes = 0;
+                for (uint32_t s = b + tid; s < e; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
+                bytes = wave_sum(bytes);
+                if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+            }
+            for (uint32_t s = b + tid; s < e; s += nthreads) velocity_manifold(d, prm, s, 0, it);
+            grid_sync(g, err);
+        }
+        ++passes;
+        if (cfg.residual_mode) {
+            float r = 0.0f;
+            for (uint32_t s = tid; s < n_active; s += nthreads) r = gmax(r, residual_manifold(d, prm, s, 0));
+            for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+            if (lane0()) s_red[threadIdx.x >> 6] = r;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                float m = 0.0f;
+                for (int w = 0; w < kBlock / 64; ++w) m = fmaxf(m, s_red[w]);
+                if (m > 0.0f) atomicMax(&d.cnt->res_slot[it & 1], __float_as_uint(m));
+            }
+            grid_sync(g, err);
+            float rr = __uint_as_float(__hip_atomic_load(&d.cnt->res_slot[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (tid == 0) {
+                d.cnt->residual_bits = __float_as_uint(rr);
+                d.cnt->res_slot[(it + 1) & 1] = 0u;  // the slot the next pass reduces into
+            }
+            grid_sync(g, err);
+            if (rr <= cfg.residual_tol) break;
+        }
+    }
+    if (tid == 0) d.cnt->passes = passes;
+    for (uint32_t it = 0; it < cfg.relax_iters; ++it) {
+        for (uint32_t c = 0; c < C; ++c) {
+            if (prm.profile) {
+                uint32_t bytes = 0;
+                for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
+                bytes = wave_sum(bytes);
+                if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+            }
+            for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, 1, it);
+            grid_sync(g, err);
+        }
+    }
+    // solve_position: snapshot, then position passes
+    for (uint32_t i = tid; i < d.nb; i += nthreads) {
+        float4 p = d.b_pos[i];
+        d.sb_x[i] = make_float4(p.x, p.y, p.z, d.sb_v[i].w);
+        d.sb_ref[i] = p;
+    }
+    grid_sync(g, err);
+    for (uint32_t it = 0; it < cfg.pos_iters; ++it) {
+        for (uint32_t c = 0; c < C; ++c) {
+            for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) position_manifold(d, prm, s);
+            grid_sync(g, err);
+        }
+    }
+}
+
+// Colouring rounds in one cooperative launch: propose / select until every active manifold
+// has a colour (bounded by max_rounds).
+__global__ void __launch_bounds__(kBlock) k_colour_persistent(DevState d, uint32_t first_round, uint32_t max_rounds,
+                                                               uint32_t* bar) {
+    GridBarrier g = grid_barrier_init(bar);
+    uint32_t* err = &d.cnt->err;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
+    const uint32_t n_man = d.cnt->manifolds;
+    for (uint32_t r = 0; r < max_rounds; ++r) {
+        if (__hip_atomic_load(&d.cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) break;
+        uint32_t round = first_round + r;
+        for (uint32_t m = tid; m < n_man; m += nthreads) {
+            if (d.m_colour[m] != kUncoloured) continue;
+            uint2 ab = d.m_ab[m];
+            unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+            if (body_active(d.b_flags[ab.x])) atomicMax(&d.col_best[ab.x], key);
+            if (body_active(d.b_flags[ab.y])) atomicMax(&d.col_best[ab.y], key);
+        }
+        grid_sync(g, err);
+        uint32_t base = tid - (tid & 63u);
+        for (uint32_t m0 = base; m0 < n_man; m0 += nthreads) {
+            uint32_t m = m0 + (tid & 63u);
+            uint32_t took = 0, top = 0;
+            if (m < n_man && d.m_colour[m] == kUncoloured) {
+                uint2 ab = d.m_ab[m];
+                unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+                bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
+                bool mine = (!aa || d.col_best[ab.x] == key) && (!ba || d.col_best[ab.y] == key);
+                if (mine) {
+                    unsigned long long used = (aa ? d.col_mask[ab.x] : 0ull) | (ba ? d.col_mask[ab.y] : 0ull);
+                    unsigned long long freeb = ~used;
+                    if (freeb == 0ull) {
+                        atomicOr(err, ERR_COLOURS);
+                    } else {
+                        uint32_t c = uint32_t(__ffsll(static_cast<long long>(freeb)) - 1);
+                        d.m_colour[m] = c;
+                        unsigned long long bit = 1ull << c;
+                        if (aa) atomicOr(&d.col_mask[ab.x], bit);
+                        if (ba) atomicOr(&d.col_mask[ab.y], bit);
+                        took = 1u;
+                        top = c + 1u;
+                    }
+                }
+            }
+            took = wave_sum(took);
+            top = wave_max(top);
+            if (lane0() && took) {
+                atomicSub(&d.cnt->remaining, took);
+                atomicMax(&d.cnt->colours, top);
+            }
+        }
+        grid_sync(g, err);
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER) break;
     }
 }
 
@@ -1192,6 +1439,36 @@ void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t n_man
     hipLaunchKernelGGL(k_island_min, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_sleep, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
 }
+// cooperative launches; returns the HIP error of the launch
+hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[6], float tol,
+                                   uint32_t nblocks, uint32_t* bar, hipStream_t st) {
+    SolveCfg c;
+    c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
+    c.residual_mode = cfgv[3]; c.residual_max = cfgv[4]; c.residual_tol = tol;
+    DevState dd = d;
+    StepParams pp = p;
+    void* args[] = {&dd, &pp, &c, &bar};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks), dim3(kBlock),
+                                      args, 0, st);
+}
+hipError_t launch_colour_persistent(const DevState& d, uint32_t first_round, uint32_t max_rounds, uint32_t nblocks,
+                                    uint32_t* bar, hipStream_t st) {
+    DevState dd = d;
+    void* args[] = {&dd, &first_round, &max_rounds, &bar};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_colour_persistent), dim3(nblocks), dim3(kBlock),
+                                      args, 0, st);
+}
+int max_coresident_blocks(int device) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve_persistent), kBlock, 0) != hipSuccess)
+        return 0;
+    int per_cu2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, reinterpret_cast<const void*>(&k_colour_persistent), kBlock, 0) != hipSuccess)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    return std::min(per_cu, per_cu2) * cus;
+}
+
 void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st) {
     if (d.nb) hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
 }

@@ -38,10 +38,12 @@ struct Counters {  // device-side counters, read back by the host at the step's 
     uint32_t awake;
     uint32_t predicted;
     uint32_t residual_bits; // float bits of the last velocity residual (max, >= 0)
-    uint32_t red[8];        // broadphase reductions (sortable-float encoded)
+    uint32_t red[12];       // broadphase reductions (sortable-float encoded)
     uint32_t colour_off[kBuckets + 1];
-    float grid[8];          // ox, oy, inv_cell, ext_y, nx, ny (as floats)
+    float grid[12];         // ox, oy, oz, inv_cell, ext_y, ext_z, nx, ny, nz (as floats)
     unsigned long long prof_vel_bytes;  // algorithmic bytes of profiled velocity passes
+    uint32_t passes;        // velocity passes run by the last solve
+    uint32_t res_slot[2];   // residual max-reduction slots (float bits), alternating per pass
 };
 
 struct DevState {
@@ -68,7 +70,9 @@ struct DevState {
     float4 *s_tmin, *s_tmax;   // tight AABB of this step; w of s_tmin: 1 = large shape
     uint32_t *layer_mask;      // 32 rows
     // broadphase
-    uint32_t *cell_count, *cell_start, *cell_cursor, *cell_items;
+    uint32_t *cell_count, *cell_start, *cell_cursor;
+    float4 *ci_fmin, *ci_fmax, *ci_tmin, *ci_tmax;  // cell-ordered copies: fat/tight boxes, w = index /
+                                                    // owner / static flag bits
     uint32_t *large_list;
     uint32_t *pair_count, *pair_off;   // per shape (emitter)
     uint2 *pairs;                      // (shape a, shape b), a < b
@@ -113,6 +117,7 @@ struct DevState {
     // islands
     uint32_t *uf_parent;
     uint32_t *isl_min;
+    uint32_t *bar;      // grid barrier state of the persistent kernels: arrivals, generation
     Counters *cnt;
     // learned warm start
     const void *mlp;    // packed weights (nullptr = off)
@@ -151,5 +156,13 @@ void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st)
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* scratch, hipStream_t st);
+// persistent cooperative kernels (grid barriers in bar[2]); cfgv = vel, relax, pos iterations,
+// residual mode, residual max passes
+hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[6], float tol,
+                                   uint32_t nblocks, uint32_t* bar, hipStream_t st);
+hipError_t launch_colour_persistent(const DevState& d, uint32_t first_round, uint32_t max_rounds, uint32_t nblocks,
+                                    uint32_t* bar, hipStream_t st);
+int max_coresident_blocks(int device);
+constexpr uint32_t ERR_BARRIER_BIT = 64u;
 
 }  // namespace pxg

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -52,6 +53,8 @@ struct pxg_world {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
     double prof_vel_ms = 0.0;
     uint64_t prof_vel_launches = 0;
+    int coop_blocks = 0;        // co-resident blocks for the persistent kernels (0: not available)
+    bool eager = false;         // PXG_EAGER=1: one launch per colour instead of persistent kernels
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -165,6 +168,11 @@ extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
         return fail(PXG_ERR_HIP, "pxg_create: hipHostMalloc failed");
     }
     std::memset(w->h_cnt, 0, sizeof(Counters));
+    int coop = 0;
+    (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, w->device);
+    w->coop_blocks = coop ? max_coresident_blocks(w->device) : 0;
+    const char* eager = std::getenv("PXG_EAGER");
+    w->eager = eager && eager[0] == '1';
     *out = w;
     return PXG_OK;
 }
@@ -222,6 +230,8 @@ static int alloc_bodies(pxg_world* w, uint32_t n) {
     DALLOC(d.cnt, 1);
     DALLOC(d.layer_mask, 32);
     DALLOC(d.scan_tmp, 4096);
+    DALLOC(d.bar, 2);
+    HIPCHK(hipMemsetAsync(d.bar, 0, 2 * sizeof(uint32_t), w->stream));
     HIPCHK(hipMemsetAsync(d.cnt, 0, sizeof(Counters), w->stream));
     return PXG_OK;
 }
@@ -249,7 +259,8 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
     DALLOC(d.s_info, cap); DALLOC(d.s_lc, cap); DALLOC(d.s_lr, cap); DALLOC(d.s_dim, cap); DALLOC(d.s_mat, cap);
     DALLOC(d.s_fmin, cap); DALLOC(d.s_fmax, cap); DALLOC(d.s_tmin, cap); DALLOC(d.s_tmax, cap);
     DALLOC(d.cell_count, kMaxCells); DALLOC(d.cell_start, kMaxCells); DALLOC(d.cell_cursor, kMaxCells);
-    DALLOC(d.cell_items, cap); DALLOC(d.large_list, kMaxLarge);
+    DALLOC(d.ci_fmin, cap); DALLOC(d.ci_fmax, cap); DALLOC(d.ci_tmin, cap); DALLOC(d.ci_tmax, cap);
+    DALLOC(d.large_list, kMaxLarge);
     DALLOC(d.pair_count, cap); DALLOC(d.pair_off, cap);
     size_t P = w->cap_p, M = w->cap_m;
     DALLOC(d.pairs, P); DALLOC(d.pm_valid, P); DALLOC(d.pm_off, P);
@@ -430,6 +441,10 @@ static int read_counters(pxg_world* w) {
         if (err & ERR_COLOURS) what += " more than 64 colours;";
         if (err & ERR_MANIFOLDS) what += " manifold buffer full;";
         if (err & ERR_CACHE) what += " warm-start table full;";
+        if (err & ERR_BARRIER_BIT) {
+            what += " grid barrier timed out;";
+            HIPCHK(hipMemset(w->d.bar, 0, 2 * sizeof(uint32_t)));
+        }
         return fail(PXG_ERR_CAPACITY, "device reported:" + what);
     }
     return PXG_OK;
@@ -468,8 +483,15 @@ static int phase_solve(pxg_world* w, float h) {
     d.cur ^= 1;
     launch_colour_init(d, bound, st);
     uint32_t round = 1;
-    launch_colour_rounds(d, bound, round, 24, st);
-    round += 24;
+    const bool persistent = w->coop_blocks > 0 && !w->eager;
+    if (persistent) {
+        uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kBlock - 1) / kBlock));
+        HIPCHK(launch_colour_persistent(d, round, 64, nb, d.bar, st));
+        round += 64;
+    } else {
+        launch_colour_rounds(d, bound, round, 24, st);
+        round += 24;
+    }
     launch_build_order(d, bound, st);
     int r = read_counters(w);  // sync 2: colour table
     if (r != PXG_OK) return r;
@@ -485,51 +507,79 @@ static int phase_solve(pxg_world* w, float h) {
     uint32_t M = hc.manifolds, C = hc.colours, active = hc.active, ordered = hc.ordered;
     std::memcpy(w->colour_off, hc.colour_off, sizeof(w->colour_off));
     const uint32_t* co = w->colour_off;
+    uint32_t widest = 0;
+    for (uint32_t k = 0; k < C; ++k) widest = std::max(widest, co[k + 1] - co[k]);
 
     launch_solver_setup(d, p, ordered, st);
-    for (uint32_t k = 0; k < C; ++k) launch_warm_apply(d, co[k], co[k + 1], st);
     uint32_t passes = 0;
     float residual = 0.0f;
-    if (c.residual_mode) {
-        uint32_t maxit = c.residual_max_iterations ? c.residual_max_iterations : 256;
-        while (passes < maxit) {
-            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, passes);
-            ++passes;
-            launch_residual(d, p, active, 0, st);
-            HIPCHK(hipMemcpyAsync(&w->h_cnt->residual_bits, &d.cnt->residual_bits, 4, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            uint32_t bits = w->h_cnt->residual_bits;
-            std::memcpy(&residual, &bits, 4);
-            if (residual <= c.residual_tolerance) break;
+    if (persistent) {
+        uint32_t cfgv[6] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
+                            c.residual_max_iterations ? c.residual_max_iterations : 256u, 0u};
+        uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (widest + kBlock - 1) / kBlock));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (w->profiling) {
+            e0 = take_event(w);
+            e1 = take_event(w);
+            (void)hipEventRecord(e0, st);
         }
+        HIPCHK(launch_solve_persistent(d, p, cfgv, c.residual_tolerance, nb, d.bar, st));
+        if (w->profiling) {
+            (void)hipEventRecord(e1, st);
+            w->ev_pending.push_back({e0, e1});
+        }
+        passes = c.residual_mode ? 0u : c.velocity_iterations;  // residual mode: read back later
     } else {
-        for (uint32_t it = 0; it < c.velocity_iterations; ++it)
-            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, it);
-        passes = c.velocity_iterations;
+        for (uint32_t k = 0; k < C; ++k) launch_warm_apply(d, co[k], co[k + 1], st);
+        if (c.residual_mode) {
+            uint32_t maxit = c.residual_max_iterations ? c.residual_max_iterations : 256;
+            while (passes < maxit) {
+                for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, passes);
+                ++passes;
+                launch_residual(d, p, active, 0, st);
+                HIPCHK(hipMemcpyAsync(&w->h_cnt->residual_bits, &d.cnt->residual_bits, 4, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                uint32_t bits = w->h_cnt->residual_bits;
+                std::memcpy(&residual, &bits, 4);
+                if (residual <= c.residual_tolerance) break;
+            }
+        } else {
+            for (uint32_t it = 0; it < c.velocity_iterations; ++it)
+                for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, it);
+            passes = c.velocity_iterations;
+        }
+        for (uint32_t it = 0; it < c.relax_iterations; ++it)
+            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 1, it);
+        launch_position_setup(d, st);
+        for (uint32_t it = 0; it < c.position_iterations; ++it)
+            for (uint32_t k = 0; k < C; ++k) launch_position(d, p, co[k], co[k + 1], st);
     }
-    for (uint32_t it = 0; it < c.relax_iterations; ++it)
-        for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 1, it);
-    launch_position_setup(d, st);
-    for (uint32_t it = 0; it < c.position_iterations; ++it)
-        for (uint32_t k = 0; k < C; ++k) launch_position(d, p, co[k], co[k + 1], st);
     launch_solver_finish(d, ordered, st);
     launch_islands_sleep(d, p, M, st);
     launch_integrate_velocities(d, p, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(PXG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    if (persistent && c.residual_mode) {
+        // residual mode needs the pass count of this step for the statistics
+        HIPCHK(hipMemcpyAsync(w->h_cnt, d.cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        passes = w->h_cnt->passes;
+        std::memcpy(&residual, &w->h_cnt->residual_bits, 4);
+        if (w->h_cnt->err) return read_counters(w);
+    }
     w->last_manifolds = M;
     w->last_ordered = ordered;
     w->last_active = active;
     w->last_colours = C;
-    pxg_stats& s = w->stats;
-    s.pairs = P;
-    s.manifolds = M;
-    s.points = hc.points;
-    s.colours = C;
-    s.active_manifolds = active;
-    s.velocity_passes = passes;
-    s.residual = residual;
-    s.steps += 1;
+    pxg_stats& sst = w->stats;
+    sst.pairs = P;
+    sst.manifolds = M;
+    sst.points = hc.points;
+    sst.colours = C;
+    sst.active_manifolds = active;
+    sst.velocity_passes = passes;
+    sst.residual = residual;
+    sst.steps += 1;
     return PXG_OK;
 }
 

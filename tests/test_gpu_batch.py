@@ -1,0 +1,113 @@
+"""Batched stepping (pxg_batch_*) against the CPU oracle, scene by scene.
+
+Every checked scene is also run as its own oracle world; each step the oracle is given the batch's
+solve order for that scene and the two must agree (same bar as tests/parity.py)."""
+import numpy as np
+import pytest
+
+from physxcoopa_amd import scenes, _abi
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_batch():
+    b, s, bo, so, cfg = scenes.config3_scenes(seed=5, n_scenes=24, n_min=8, n_max=96)
+    return b, s, bo, so, cfg
+
+
+def compare_scene(batch, k, ow, step_report):
+    gm = batch.manifolds(k)
+    om = ow.manifolds()
+    assert len(gm) == len(om), "scene %d: manifold count %d vs %d" % (k, len(gm), len(om))
+    gk = {(int(m["shape_a"]), int(m["shape_b"])): m for m in gm}
+    for o in om:
+        g = gk[(int(o["shape_a"]), int(o["shape_b"]))]
+        n = int(o["count"])
+        assert g["count"] == o["count"]
+        assert np.array_equal(g["points"]["feature_id"][:n], o["points"]["feature_id"][:n])
+        assert np.abs(g["points"]["position"][:n] - o["points"]["position"][:n]).max(initial=0) <= 1e-5
+        assert np.abs(g["normal"] - o["normal"]).max() <= 1e-5
+        if not o["is_trigger"]:
+            for f in ("normal_impulse", "tangent_impulse"):
+                gv, ov = g["points"][f][:n].astype(np.float64), o["points"][f][:n].astype(np.float64)
+                assert np.all(np.abs(gv - ov) <= 1e-4 * np.abs(ov) + 1e-6), (k, f)
+    gb = batch.scene_bodies(k)
+    ob = ow.bodies()
+    for f in ("position", "orientation", "linear_velocity", "angular_velocity"):
+        d = np.abs(gb[f].astype(np.float64) - ob[f]).max()
+        assert d <= 1e-5, "scene %d body %s differs by %g" % (k, f, d)
+    assert np.array_equal(gb["awake"], ob["awake"])
+    step_report["bitexact"] += int(np.array_equal(gb[["position", "orientation", "linear_velocity",
+                                                      "angular_velocity"]].tobytes(),
+                                                  ob[["position", "orientation", "linear_velocity",
+                                                      "angular_velocity"]].tobytes()))
+    step_report["checks"] += 1
+
+
+def test_batch_matches_oracle_per_scene(small_batch, oracle):
+    from physxcoopa_amd.batch import Batch
+    b, s, bo, so, settings = small_batch
+    cfg = _abi.default_config(**settings)
+    batch = Batch(cfg, b, s, bo, so)
+    checked = [0, 1, 2, 3, 7, 12, 23]
+    ows = {k: oracle.OracleWorld(cfg, b[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]]) for k in checked}
+    rep = dict(checks=0, bitexact=0)
+    for step in range(120):
+        batch.step(1)
+        for k in checked:
+            gm = batch.manifolds(k)
+            order, _ = batch.solve_order(k)
+            pairs = np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1) if len(order) else np.zeros((0, 2))
+            ows[k].set_order(pairs)
+            ows[k].step_fixed()
+            compare_scene(batch, k, ows[k], rep)
+    print("batch parity:", rep, flush=True)
+    st = batch.stats()
+    assert np.all(st["error"] == 0)
+    assert st["manifolds"].sum() > 0
+
+
+def test_batch_multi_step_launch_equals_single_steps(small_batch):
+    """K steps in one persistent launch give the same state as K launches of one step."""
+    from physxcoopa_amd.batch import Batch
+    b, s, bo, so, settings = small_batch
+    cfg = _abi.default_config(**settings)
+    a = Batch(cfg, b, s, bo, so)
+    c = Batch(cfg, b, s, bo, so)
+    for _ in range(30):
+        a.step(1)
+    c.step(30)
+    ba, bc = a.bodies(), c.bodies()
+    assert ba.tobytes() == bc.tobytes()
+
+
+@pytest.mark.parametrize("block,lpt", [(64, True), (128, True), (512, True), (256, False)])
+def test_batch_tunings_give_identical_results(small_batch, block, lpt):
+    from physxcoopa_amd.batch import Batch
+    b, s, bo, so, settings = small_batch
+    cfg = _abi.default_config(**settings)
+    ref = Batch(cfg, b, s, bo, so)
+    ref.step(40)
+    t = Batch(cfg, b, s, bo, so)
+    t.set_tuning(block, lpt)
+    t.step(40)
+    assert ref.bodies().tobytes() == t.bodies().tobytes()
+
+
+def test_batch_residual_mode_matches_oracle(small_batch, oracle):
+    from physxcoopa_amd.batch import Batch
+    b, s, bo, so, settings = small_batch
+    cfg = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=48, **settings)
+    batch = Batch(cfg, b, s, bo, so)
+    k = 2
+    ow = oracle.OracleWorld(cfg, b[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]])
+    rep = dict(checks=0, bitexact=0)
+    for step in range(60):
+        batch.step(1)
+        gm = batch.manifolds(k)
+        order, _ = batch.solve_order(k)
+        ow.set_order(np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1) if len(order) else np.zeros((0, 2)))
+        ow.step_fixed()
+        compare_scene(batch, k, ow, rep)
+        assert batch.stats()[k]["velocity_passes"] == ow.stats()["velocity_passes"]

@@ -68,3 +68,22 @@ def test_gpu_deterministic_across_runs(sides):
         hashes.append(w.state_hash())
         w.close()
     assert hashes[0] == hashes[1]
+
+
+def test_parity_eager_launch_path(sides, monkeypatch):
+    """The one-launch-per-colour fallback gives the same results as the persistent kernels."""
+    monkeypatch.setenv("PXG_EAGER", "1")
+    _, _, rep = _run(sides, scenes.mixed_pile_scene(n=200, seed=11), 60, velocity_iterations=8, position_iterations=3)
+    assert rep["manifolds"] > 0
+
+
+def test_parity_residual_mode(sides):
+    """Residual mode: velocity passes repeat until the residual <= 1e-4 (cap 64 here); the GPU
+    must run the same number of passes as the oracle and reach the same state."""
+    World, OracleWorld = sides
+    sc = scenes.config1_scene(seed=1)
+    c = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=64, **sc.config)
+    gw, ow, rep = lockstep(World, OracleWorld, c, sc.bodies, sc.shapes, 80, resync=True)
+    print(rep, gw.stats(), ow.stats(), flush=True)
+    assert gw.stats()["velocity_passes"] == ow.stats()["velocity_passes"]
+    assert abs(gw.stats()["residual"] - ow.stats()["residual"]) <= 1e-6 + 1e-4 * abs(ow.stats()["residual"])
