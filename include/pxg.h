@@ -231,11 +231,48 @@ int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const float* impulses
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);
 
-/* Learned warm start: BF16 weights of the 24-64-64-3 MLP (row-major W[out][in]) and FP32
- * biases plus the 24 input scales. No counterpart in the reference (new hook at its warm-start
- * miss branch, dynamics/solver.h:166-170). */
-int pxg_set_mlp(pxg_world* w, const uint16_t* w1, const float* b1, const uint16_t* w2,
-                const float* b2, const uint16_t* w3, const float* b3, const float* in_scale);
+/* ---- learned warm start ----
+ * A 24-64-64-3 ReLU network predicts the initial (normal, tangent1, tangent2) impulses of contact
+ * points that have no cached impulse (the reference starts them at zero, dynamics/solver.h:166-170).
+ * Predictions are clamped to lambda_n >= 0 and |lambda_t| <= mu lambda_n; the solver then runs as
+ * usual, so the residual still judges convergence. Weights are FP32 here and BF16 on the device
+ * (inference on the matrix cores with FP32 accumulation). Input i is (x_i - in_shift_i) *
+ * in_scale_i; output k is y_k * out_scale_k. No counterpart in the reference.
+ * Features, per contact point, from the pre-solve state:
+ *   0-2 normal; 3-5 relative velocity (b - a) at the point in the contact frame (n, t1, t2);
+ *   6 penetration; 7 separation; 8-9 effective inverse masses of a and b; 10-12 / 13-15 lever
+ *   arms ra / rb in the contact frame; 16-18 / 19-21 Ia^-1 (ra x n) / Ib^-1 (rb x n) in the
+ *   contact frame; 22 friction; 23 restitution. */
+typedef struct pxg_mlp {
+    float w1[64 * 24]; /* [out][in] */
+    float b1[64];
+    float w2[64 * 64];
+    float b2[64];
+    float w3[3 * 64];
+    float b3[3];
+    float in_shift[24];
+    float in_scale[24];
+    float out_scale[3];
+} pxg_mlp;
+typedef struct pxg_mlp_train_config {
+    uint32_t epochs, batch, seed, reserved0;
+    float lr, lr_decay, beta1, beta2, eps, weight_decay;
+} pxg_mlp_train_config;
+/* NULL turns the learned warm start off. */
+int pxg_set_mlp(pxg_world* w, const pxg_mlp* net);
+/* Records of the last step: features of every active contact point (24 floats), its final
+ * impulses (3 floats) and whether it missed the warm-start cache. Requires pxg_set_recording(1)
+ * before the step. */
+int pxg_set_recording(pxg_world* w, uint32_t on);
+int pxg_get_records(pxg_world* w, float* features, float* impulses, uint32_t* missed, uint32_t cap, uint32_t* n);
+/* The learned predictions the last step applied: warm-start keys and clamped impulses. */
+int pxg_get_predictions(pxg_world* w, uint64_t* keys, float* impulses3, uint32_t cap, uint32_t* n);
+/* Standalone inference / gradient / training on the current device. */
+int pxg_mlp_infer(const pxg_mlp* net, const float* features, uint32_t n, float* out3);
+int pxg_mlp_gradients(const pxg_mlp* net, const float* X, const float* Y, uint32_t n, pxg_mlp* grad, float* loss);
+int pxg_mlp_train(pxg_mlp* net, const float* X, const float* Y, uint32_t n, const pxg_mlp_train_config* cfg,
+                  float* loss_per_epoch);
+const char* pxg_mlp_last_error(void);
 
 /* Profiling of the dominant kernel, the velocity pass: with profiling on, every velocity-pass
  * launch is bracketed by HIP events on the world's stream and counts its algorithmic bytes

@@ -33,7 +33,18 @@ _SIGNATURES = {
                            ctypes.c_int),
     "pxg_set_warm_cache": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_state_hash": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
-    "pxg_set_mlp": ([ctypes.c_void_p] * 8, ctypes.c_int),
+    "pxg_set_mlp": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_set_recording": ([ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_get_records": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                         ctypes.c_void_p], ctypes.c_int),
+    "pxg_get_predictions": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p],
+                            ctypes.c_int),
+    "pxg_mlp_infer": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
+    "pxg_mlp_gradients": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                           ctypes.c_void_p], ctypes.c_int),
+    "pxg_mlp_train": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                       ctypes.c_void_p], ctypes.c_int),
+    "pxg_mlp_last_error": ([], ctypes.c_char_p),
     "pxg_set_profiling": ([ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_profile": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_batch_create": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
@@ -78,6 +89,12 @@ def lib():
 def check(rc, what=""):
     if rc != 0:
         msg = lib().pxg_last_error().decode()
+        raise PxgError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def check_mlp(rc, what=""):
+    if rc != 0:
+        msg = lib().pxg_mlp_last_error().decode()
         raise PxgError("%s failed (%d): %s" % (what, rc, msg))
 
 

@@ -5,6 +5,7 @@
 // work item per thread, grid-stride where the item count lives on the device.
 #include "pxg_state.h"
 #include "pxg_device.h"
+#include "pxg_mlp.h"
 
 using namespace pxd;
 
@@ -654,7 +655,22 @@ __device__ __forceinline__ bool cache_lookup(const DevState& d, int table, unsig
 }
 
 // Gathers manifolds into solve order, captures restitution targets (solver.h:121-137) and
-// seeds impulses from the previous step's cache (solver.h:144-175); misses start at zero.
+// seeds impulses from the previous step's cache (solver.h:144-175); misses start at zero, or, with
+// the learned warm start on, queue their features for the network.
+__device__ __forceinline__ FeatBody feat_body(const DevState& d, uint32_t i) {
+    FeatBody b;
+    uint32_t f = d.b_flags[i];
+    bool act = body_active(f);
+    b.x = ld3(d.b_pos[i]);
+    float4 v = d.b_vel[i];
+    b.v = ld3(v);
+    b.w = ld3(d.b_angv[i]);
+    b.im = act ? v.w : 0.0f;
+    if (act) b.I = load_iiw(d.b_iiw, i);
+    else { b.I.c0 = splat(0.0f); b.I.c1 = splat(0.0f); b.I.c2 = splat(0.0f); }
+    return b;
+}
+
 __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t n_ordered) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_ordered) return;
@@ -673,6 +689,8 @@ __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t n_ordered) {
     v3 va = ld3(d.b_vel[ab.x]), vb = ld3(d.b_vel[ab.y]);
     v3 wa = ld3(d.b_angv[ab.x]), wb = ld3(d.b_angv[ab.y]);
     int prev = d.cur ^ 1;
+    bool fb_loaded = false;
+    FeatBody FA, FB;
     for (uint32_t t = 0; t < 4; ++t) {
         float4 pp = d.m_p[4 * m + t], qq = d.m_q[4 * m + t];
         float4 aux = make_float4(qq.x, 0.0f, qq.y, 0.0f);
@@ -685,14 +703,46 @@ __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t n_ordered) {
             aux.y = (-vap > prm.rest_thresh) ? -rest * vap : 0.0f;
             if (active) {
                 float4 cv;
-                if (cache_lookup(d, prev, pack_key(ab.x, ab.y, __float_as_uint(qq.y)), cv)) imp = cv;
+                bool hit = cache_lookup(d, prev, pack_key(ab.x, ab.y, __float_as_uint(qq.y)), cv);
+                if (hit) imp = cv;
                 else aux.w = 1.0f;  // warm-start miss
+                if ((!hit && d.mlp) || d.record) {
+                    if (!fb_loaded) { FA = feat_body(d, ab.x); FB = feat_body(d, ab.y); fb_loaded = true; }
+                    float f[kMlpIn];
+                    contact_features(FA, FB, n, pos, pp.w, qq.x, nf.w, rest, f);
+                    float* row;
+                    if (d.record) {
+                        row = d.feat + size_t(4 * s + t) * kMlpIn;
+                    } else {
+                        uint32_t k = atomicAdd(&d.cnt->predicted, 1u);
+                        d.miss_idx[k] = 4 * s + t;
+                        row = d.feat + size_t(k) * kMlpIn;
+                    }
+                    for (int q = 0; q < kMlpIn; ++q) row[q] = f[q];
+                }
             }
         }
         d.sp_pos[4 * s + t] = pp;
         d.sp_aux[4 * s + t] = aux;
         d.sp_imp[4 * s + t] = imp;
     }
+}
+
+// clamps each prediction to the friction cone (lambda_n >= 0, |lambda_t| <= mu lambda_n) and makes
+// it the point's initial impulse
+__global__ void k_apply_predictions(DevState d, uint32_t n_bound) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_bound || j >= d.cnt->predicted) return;
+    uint32_t p = d.miss_idx[j];
+    float mu = d.sm_nf[p / 4].w;
+    float ln = fmaxf(d.pred[3 * j], 0.0f);
+    float lim = mu * ln;
+    float t1 = fminf(fmaxf(d.pred[3 * j + 1], -lim), lim);
+    float t2 = fminf(fmaxf(d.pred[3 * j + 2], -lim), lim);
+    d.pred[3 * j] = ln;
+    d.pred[3 * j + 1] = t1;
+    d.pred[3 * j + 2] = t2;
+    d.sp_imp[p] = make_float4(ln, t1, t2, 0.0f);
 }
 
 // compact solver bodies: effective inverse mass / inertia are zero unless dynamic and awake
@@ -1400,6 +1450,11 @@ void launch_build_order(const DevState& d, uint32_t n_man, hipStream_t st) {
 void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t n_ordered, hipStream_t st) {
     if (n_ordered) hipLaunchKernelGGL(k_solver_setup, dim3(blocks(n_ordered)), dim3(kBlock), 0, st, d, p, n_ordered);
     if (d.nb) hipLaunchKernelGGL(k_solver_bodies, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+}
+void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st) {
+    if (!n_bound || !d.mlp) return;
+    launch_mlp_infer_dev(d.mlp, d.feat, d.pred, &d.cnt->predicted, n_bound, st);
+    hipLaunchKernelGGL(k_apply_predictions, dim3(blocks(n_bound)), dim3(kBlock), 0, st, d, n_bound);
 }
 void launch_warm_apply(const DevState& d, uint32_t begin, uint32_t end, hipStream_t st) {
     if (end > begin) hipLaunchKernelGGL(k_warm_apply, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, begin, end);

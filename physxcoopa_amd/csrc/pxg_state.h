@@ -9,6 +9,7 @@
 
 namespace pxg {
 
+struct MlpDev;
 constexpr int kBlock = 256;
 constexpr int kMaxColours = 64;      // colours per step (64-bit per-body colour masks)
 constexpr int kBuckets = kMaxColours + 1;  // colours + the "skipped by the solver" group
@@ -120,7 +121,11 @@ struct DevState {
     uint32_t *bar;      // grid barrier state of the persistent kernels: arrivals, generation
     Counters *cnt;
     // learned warm start
-    const void *mlp;    // packed weights (nullptr = off)
+    const struct MlpDev *mlp;  // BF16 network (nullptr = off)
+    float *feat;        // learned: compact feature rows of misses; recording: rows per point (4 s + i)
+    uint32_t *miss_idx; // learned: point index of each compact row
+    float *pred;        // learned: clamped predictions, 3 per compact row
+    uint32_t record;    // 1: write every active point's features (dataset recording)
 };
 
 struct StepParams {
@@ -152,6 +157,7 @@ void launch_solver_finish(const DevState& d, uint32_t n_ordered, hipStream_t st)
 void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t n_manifolds, hipStream_t st);
 void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st);
 void launch_init_bodies(const DevState& d, hipStream_t st);
+void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st);
 void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);

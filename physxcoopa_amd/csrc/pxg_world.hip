@@ -4,6 +4,7 @@
 // narrowphase) and once for the colour table (sizes the per-colour solver launches).
 #include "../../include/pxg.h"
 #include "pxg_state.h"
+#include "pxg_mlp.h"
 
 #include <hip/hip_runtime.h>
 
@@ -54,6 +55,7 @@ struct pxg_world {
     double prof_vel_ms = 0.0;
     uint64_t prof_vel_launches = 0;
     int coop_blocks = 0;        // co-resident blocks for the persistent kernels (0: not available)
+    MlpDev* d_mlp = nullptr;
     bool eager = false;         // PXG_EAGER=1: one launch per colour instead of persistent kernels
 };
 
@@ -511,6 +513,7 @@ static int phase_solve(pxg_world* w, float h) {
     for (uint32_t k = 0; k < C; ++k) widest = std::max(widest, co[k + 1] - co[k]);
 
     launch_solver_setup(d, p, ordered, st);
+    if (d.mlp && !d.record) launch_apply_predictions(d, 4 * active, st);
     uint32_t passes = 0;
     float residual = 0.0f;
     if (persistent) {
@@ -636,7 +639,7 @@ extern "C" int pxg_get_stats(pxg_world* w, pxg_stats* out) {
     HIPCHK(hipMemcpyAsync(w->h_cnt, w->d.cnt, sizeof(Counters), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     w->stats.awake_bodies = w->h_cnt->awake;
-    w->stats.learned_predictions = w->h_cnt->predicted;
+    w->stats.learned_predictions = (w->d.mlp && !w->d.record) ? w->h_cnt->predicted : 0u;
     *out = w->stats;
     return PXG_OK;
 }
@@ -825,10 +828,114 @@ extern "C" int pxg_state_hash(pxg_world* w, uint64_t* out) {
     return PXG_OK;
 }
 
-extern "C" int pxg_set_mlp(pxg_world* w, const uint16_t*, const float*, const uint16_t*, const float*, const uint16_t*,
-                           const float*, const float*) {
+namespace pxg { void mlp_to_device_image(const pxg_mlp* net, MlpDev* img); }
+
+static int ensure_learn_buffers(pxg_world* w) {
+    if (w->d.feat) return PXG_OK;
+    if (!w->cap_m) return fail(PXG_ERR_STATE, "upload shapes before enabling the learned warm start");
+    DevState& d = w->d;
+    DALLOC(d.feat, size_t(4) * w->cap_m * kMlpIn);
+    DALLOC(d.miss_idx, size_t(4) * w->cap_m);
+    DALLOC(d.pred, size_t(4) * w->cap_m * kMlpOut);
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_mlp(pxg_world* w, const pxg_mlp* net) {
     if (!w) return fail(PXG_ERR_INVALID, "pxg_set_mlp: null handle");
-    return fail(PXG_ERR_STATE, "pxg_set_mlp: learned warm start not built into this library");
+    if (!net) {
+        w->d.mlp = nullptr;
+        return PXG_OK;
+    }
+    int r = ensure_learn_buffers(w);
+    if (r != PXG_OK) return r;
+    if (!w->d_mlp) {
+        MlpDev* p = nullptr;
+        DALLOC(p, 1);
+        w->d_mlp = p;
+    }
+    MlpDev img;
+    mlp_to_device_image(net, &img);
+    HIPCHK(hipMemcpyAsync(w->d_mlp, &img, sizeof(img), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    w->d.mlp = w->d_mlp;
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_recording(pxg_world* w, uint32_t on) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_set_recording: null handle");
+    if (on) {
+        int r = ensure_learn_buffers(w);
+        if (r != PXG_OK) return r;
+    }
+    w->d.record = on ? 1u : 0u;
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_records(pxg_world* w, float* features, float* impulses, uint32_t* missed, uint32_t cap, uint32_t* n) {
+    if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_records: null argument");
+    *n = 0;
+    if (!w->d.record || !w->d.feat) return fail(PXG_ERR_STATE, "pxg_get_records: recording is off");
+    uint32_t A = w->last_active;
+    if (!A) return PXG_OK;
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    std::vector<uint32_t> cnt(A);
+    std::vector<float> f(size_t(4) * A * kMlpIn);
+    std::vector<float4> imp(4 * size_t(A)), aux(4 * size_t(A));
+    HIPCHK(hipMemcpyAsync(cnt.data(), d.sm_cnt, A * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(f.data(), d.feat, f.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(imp.data(), d.sp_imp, imp.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(aux.data(), d.sp_aux, aux.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint32_t k = 0;
+    for (uint32_t s = 0; s < A; ++s) {
+        for (uint32_t t = 0; t < cnt[s]; ++t, ++k) {
+            if (k >= cap) continue;
+            size_t pidx = 4 * size_t(s) + t;
+            if (features) std::memcpy(features + size_t(k) * kMlpIn, &f[pidx * kMlpIn], kMlpIn * 4);
+            if (impulses) { impulses[3 * k] = imp[pidx].x; impulses[3 * k + 1] = imp[pidx].y; impulses[3 * k + 2] = imp[pidx].z; }
+            if (missed) missed[k] = aux[pidx].w != 0.0f ? 1u : 0u;
+        }
+    }
+    *n = k;
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_predictions(pxg_world* w, uint64_t* keys, float* imp3, uint32_t cap, uint32_t* n) {
+    if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_predictions: null argument");
+    *n = 0;
+    if (!w->d.mlp || w->d.record) return PXG_OK;
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    HIPCHK(hipMemcpyAsync(w->h_cnt, d.cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    uint32_t P = w->h_cnt->predicted, A = w->last_active;
+    if (!P) return PXG_OK;
+    std::vector<uint32_t> mi(P);
+    std::vector<float> pr(3 * size_t(P));
+    std::vector<uint2> ab(A);
+    std::vector<float4> aux(4 * size_t(A));
+    HIPCHK(hipMemcpyAsync(mi.data(), d.miss_idx, P * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(pr.data(), d.pred, pr.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ab.data(), d.sm_ab, A * sizeof(uint2), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(aux.data(), d.sp_aux, aux.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    std::vector<std::pair<uint64_t, uint32_t>> e;
+    for (uint32_t j = 0; j < P; ++j) {
+        uint32_t p = mi[j], s = p / 4;
+        uint32_t fid;
+        std::memcpy(&fid, &aux[p].z, 4);
+        uint32_t lo = std::min(ab[s].x, ab[s].y), hi = std::max(ab[s].x, ab[s].y);
+        uint64_t key = (uint64_t(lo) << 40) ^ (uint64_t(hi) << 12) ^ uint64_t(fid);
+        e.push_back({key, j});
+    }
+    std::sort(e.begin(), e.end());
+    *n = P;
+    for (uint32_t i = 0; i < P && i < cap; ++i) {
+        if (keys) keys[i] = e[i].first;
+        if (imp3) for (int k = 0; k < 3; ++k) imp3[3 * i + k] = pr[3 * size_t(e[i].second) + k];
+    }
+    return PXG_OK;
 }
 
 // Profiling of the dominant kernel (the velocity pass): HIP events on the world's stream around

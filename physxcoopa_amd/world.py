@@ -136,6 +136,33 @@ class World:
         check(lib().pxg_state_hash(self.h, ctypes.byref(out)), "pxg_state_hash")
         return out.value
 
+    def set_mlp(self, net):
+        """Turns the learned warm start on with `net` (a learned.MlpParams) or off with None."""
+        check(lib().pxg_set_mlp(self.h, ctypes.byref(net) if net is not None else None), "pxg_set_mlp")
+
+    def set_recording(self, on):
+        check(lib().pxg_set_recording(self.h, 1 if on else 0), "pxg_set_recording")
+
+    def records(self):
+        """(features [n, 24], final impulses [n, 3], missed [n]) of the last step's active points."""
+        n = ctypes.c_uint32()
+        st = self.stats()
+        cap = 4 * max(1, st["active_manifolds"])
+        f = np.zeros((cap, 24), np.float32)
+        y = np.zeros((cap, 3), np.float32)
+        m = np.zeros(cap, np.uint32)
+        check(lib().pxg_get_records(self.h, ptr(f), ptr(y), ptr(m), cap, ctypes.byref(n)), "pxg_get_records")
+        return f[:n.value], y[:n.value], m[:n.value]
+
+    def predictions(self):
+        """(keys, clamped impulses [n, 3]) the learned warm start applied in the last step."""
+        n = ctypes.c_uint32()
+        check(lib().pxg_get_predictions(self.h, None, None, 0, ctypes.byref(n)), "pxg_get_predictions")
+        k = np.zeros(max(n.value, 1), np.uint64)
+        i = np.zeros((max(n.value, 1), 3), np.float32)
+        check(lib().pxg_get_predictions(self.h, ptr(k), ptr(i), n.value, ctypes.byref(n)), "pxg_get_predictions")
+        return k[:n.value], i[:n.value]
+
     def set_profiling(self, on):
         check(lib().pxg_set_profiling(self.h, 1 if on else 0), "pxg_set_profiling")
 

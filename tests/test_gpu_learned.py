@@ -1,0 +1,178 @@
+"""Learned warm start on the GPU: training gradients and MFMA inference against the float64 oracle,
+feature parity, lockstep parity with predictions, and the gain over the plain warm start."""
+import numpy as np
+import pytest
+
+from physxcoopa_amd import scenes, _abi
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_net(learned, rng):
+    X = rng.normal(0, 1, (512, 24)).astype(np.float32)
+    Y = rng.normal(0, 1, (512, 3)).astype(np.float32)
+    return learned.init_params(X, Y, seed=3), X, Y
+
+
+def oracle_params(net):
+    p = {k: v.astype(np.float64) for k, v in net.arrays().items()}
+    p["w1"] = p["w1"].reshape(64, 24)
+    p["w2"] = p["w2"].reshape(64, 64)
+    p["w3"] = p["w3"].reshape(3, 64)
+    return p
+
+
+def test_gpu_gradients_match_cpu_oracle():
+    from physxcoopa_amd import learned
+    import pxo_mlp
+    rng = np.random.default_rng(1)
+    net, X, Y = rand_net(learned, rng)
+    X = rng.normal(0, 2, (300, 24)).astype(np.float32)
+    Y = rng.normal(0, 1, (300, 3)).astype(np.float32)
+    g, loss = learned.gradients(net, X, Y)
+    p = oracle_params(net)
+    ref_loss, ref = pxo_mlp.loss_and_grads(p, X, Y)
+    assert abs(loss - ref_loss) <= 1e-4 * abs(ref_loss)
+    ga = g.arrays()
+    for k in ("w1", "b1", "w2", "b2", "w3", "b3"):
+        a, b = ga[k].astype(np.float64).ravel(), ref[k].ravel()
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-12)
+        print(k, "max rel err", err)
+        assert err < 2e-4, (k, err)
+
+
+def test_gpu_mfma_inference_matches_bf16_oracle():
+    from physxcoopa_amd import learned
+    import pxo_mlp
+    rng = np.random.default_rng(2)
+    net, _, _ = rand_net(learned, rng)
+    X = rng.normal(0, 1.5, (1000, 24)).astype(np.float32)   # 1000 rows: a partial last tile too
+    y = learned.infer(net, X)
+    p = oracle_params(net)
+    ref = pxo_mlp.predict_bf16(p, X)
+    ref64 = pxo_mlp.predict(p, X)
+    scale = np.abs(ref).max()
+    err = np.abs(y - ref).max() / scale
+    err64 = np.abs(y - ref64).max() / scale
+    print("mfma vs bf16 oracle", err, "vs fp64", err64)
+    assert err < 2e-3
+    assert err64 < 5e-2
+
+
+def _features_np(pre, m):
+    """Reference features (float64) from the pre-solve bodies and one manifold's point."""
+    out = []
+    n = m["normal"].astype(np.float64)
+    sign = 1.0 if n[2] >= 0 else -1.0
+    a = -1.0 / (sign + n[2])
+    b = n[0] * n[1] * a
+    t1 = np.array([1 + sign * n[0] * n[0] * a, sign * b, -sign * n[0]])
+    t2 = np.array([b, sign + n[1] * n[1] * a, -n[1]])
+    A, B = pre[m["a"]], pre[m["b"]]
+
+    def eff(body):
+        act = body["type"] == 2 and body["awake"]
+        I = body["inv_inertia_world"].astype(np.float64).reshape(3, 3).T if act else np.zeros((3, 3))
+        return (float(body["inv_mass"]) if act else 0.0), I
+    ima, Ia = eff(A)
+    imb, Ib = eff(B)
+    for i in range(int(m["count"])):
+        pt = m["points"][i]
+        pos = pt["position"].astype(np.float64)
+        ra = pos - A["position"]
+        rb = pos - B["position"]
+        dv = (B["linear_velocity"] + np.cross(B["angular_velocity"], rb)) - (A["linear_velocity"] + np.cross(A["angular_velocity"], ra))
+        ia = Ia @ np.cross(ra, n)
+        ib = Ib @ np.cross(rb, n)
+        f = np.concatenate([n, [dv @ n, dv @ t1, dv @ t2], [pt["penetration"], pt["separation"]], [ima, imb],
+                            [ra @ n, ra @ t1, ra @ t2], [rb @ n, rb @ t1, rb @ t2], [ia @ n, ia @ t1, ia @ t2],
+                            [ib @ n, ib @ t1, ib @ t2], [m["friction"], m["restitution"]]])
+        out.append(f)
+    return out
+
+
+def test_recorded_features_match_reference():
+    from physxcoopa_amd.world import World
+    sc = scenes.config1_scene(seed=4)
+    w = World(_abi.default_config(**sc.config), sc.bodies, sc.shapes)
+    for _ in range(40):
+        w.step_fixed()
+    w.set_recording(True)
+    # the pre-solve state: bodies after this step's force integration (re-derived by stepping a copy
+    # through the collision phase)
+    w.step_phase(_abi.PXG_PHASE_FORCES_COLLIDE)
+    pre = w.bodies()
+    w.step_phase(_abi.PXG_PHASE_SOLVE_INTEGRATE)
+    f, y, missed = w.records()
+    gm = w.manifolds()
+    order, offs = w.solve_order()
+    active = offs[-1]
+    ref = []
+    for s in range(active):
+        ref.extend(_features_np(pre, gm[order[s]]))
+    ref = np.array(ref)
+    assert f.shape == ref.shape and len(f) > 0
+    assert np.abs(f - ref).max() < 2e-5
+
+
+def _lockstep_learned(net, steps, oracle):
+    from physxcoopa_amd.world import World
+    sc = scenes.config1_scene(seed=9)
+    cfg = _abi.default_config(**sc.config)
+    gw = World(cfg, sc.bodies, sc.shapes)
+    gw.set_mlp(net)
+    ow = oracle.OracleWorld(cfg, sc.bodies, sc.shapes)
+    predicted = 0
+    for _ in range(steps):
+        gw.step_fixed()
+        keys, imp = gw.predictions()
+        predicted += len(keys)
+        ow.set_miss_table(keys, imp)
+        gm = gw.manifolds()
+        order, _ = gw.solve_order()
+        ow.set_order(np.stack([gm["shape_a"][order], gm["shape_b"][order]], 1) if len(order) else np.zeros((0, 2)))
+        ow.step_fixed()
+        gb, ob = gw.bodies(), ow.bodies()
+        for f in ("position", "linear_velocity", "angular_velocity", "orientation"):
+            assert np.abs(gb[f].astype(np.float64) - ob[f]).max() <= 1e-5
+    return predicted
+
+
+@pytest.fixture(scope="module")
+def trained_net():
+    """A network trained on the GPU from config-1 piles solved at 64 velocity iterations (a small
+    version of the full recipe: 8 piles x 200 steps)."""
+    from physxcoopa_amd import learned
+    X, Y, M = learned.record_dataset(list(range(100, 108)), steps=200, velocity_iterations=64)
+    X, Y, M = learned.subsample(X, Y, M, 60000)
+    net = learned.init_params(X, Y, seed=0)
+    net, losses = learned.train(net, X, Y, epochs=20, batch=1024, lr=2e-3)
+    print("records", len(X), "misses", int(M.sum()), "loss", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.5 * losses[0]
+    return net
+
+
+def test_learned_lockstep_parity(trained_net, oracle):
+    predicted = _lockstep_learned(trained_net, 120, oracle)
+    assert predicted > 0
+
+
+def test_learned_warm_start_reduces_passes_to_residual(trained_net):
+    """Config-1 pile (a seed not trained on), residual mode 1e-4 (cap 256): velocity passes with the
+    learned warm start vs the plain warm start."""
+    from physxcoopa_amd.world import World
+    sc = scenes.config1_scene(seed=77)
+    cfg = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=256, **sc.config)
+    totals = {}
+    for name, net in (("plain", None), ("learned", trained_net)):
+        w = World(cfg, sc.bodies, sc.shapes)
+        w.set_mlp(net)
+        passes = 0
+        for _ in range(300):
+            w.step_fixed()
+            st = w.stats()
+            passes += st["velocity_passes"]
+        totals[name] = passes
+        w.close()
+    print("velocity passes to residual 1e-4 over 300 steps:", totals)
+    assert totals["learned"] <= totals["plain"]
