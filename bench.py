@@ -101,7 +101,8 @@ def main():
     achieved = prof["velocity_bytes"] / vel_s / 1e9 if vel_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_velocity", "launches": int(prof["velocity_launches"]),
+                "kernel": "k_velocity" if os.environ.get("PXG_EAGER") == "1" else "k_solve_persistent",
+                "launches": int(prof["velocity_launches"]),
                 "avg_launch_us": round(1e6 * vel_s / max(1, prof["velocity_launches"]), 3),
                 "bytes_per_launch": round(prof["velocity_bytes"] / max(1, prof["velocity_launches"]), 1)}
 

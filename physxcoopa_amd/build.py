@@ -47,7 +47,27 @@ def build(verbose=False, jobs=4):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode())
     os.replace(out_tmp, LIB)
+    build_facade_test(verbose)
     return LIB
+
+
+FACADE_TEST = os.path.join(HERE, "pxg_world_test")
+
+
+def build_facade_test(verbose=False):
+    """The reference's PhysicsWorld tests over the C++ facade (tests/cpp), linked to libpxg.so."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "physics_world_test.cpp")
+    if not os.path.exists(src):
+        return None
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(root, "include"),
+           src, "-L", HERE, "-lpxg", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", FACADE_TEST]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("facade test build failed:\n" + r.stdout.decode())
+    return FACADE_TEST
 
 
 def _wait(item):

@@ -216,6 +216,13 @@ __device__ __forceinline__ ShapeGeom shape_geom(const BatchDev& bd, uint32_t gi)
     s.chh = bd.s_mat[gi].x;
     return s;
 }
+// Contact generation is called out of line: the narrowphase's working set (clipping polygons,
+// the manifold under construction) then lives only in this call's frame instead of adding to the
+// register and scratch budget of the whole scene kernel.
+__device__ __noinline__ bool scene_contacts(const ShapeGeom& si, v3 pi, quat qi, const ShapeGeom& sj, v3 pj, quat qj,
+                                            bool spec, Man& m) {
+    return generate_contacts(si, pi, qi, sj, pj, qj, spec, m);
+}
 __device__ __forceinline__ bool ovl(float4 amn, float4 amx, float4 bmn, float4 bmx) {
     return amn.x <= bmx.x && amx.x >= bmn.x && amn.y <= bmx.y && amx.y >= bmn.y && amn.z <= bmx.z && amx.z >= bmn.z;
 }
@@ -326,7 +333,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
             (layer_mask[ii.z & 31u] & (1u << (ij.z & 31u)))) {
             ShapeGeom si = shape_geom(bd, S.shape_off + i), sj = shape_geom(bd, S.shape_off + j);
             bool spec = (len(ld3(L.vel[bi])) * cfg.h > min_extent(si)) || (len(ld3(L.vel[bj])) * cfg.h > min_extent(sj));
-            valid = generate_contacts(si, ld3(L.pos[bi]), ldq(L.rot[bi]), sj, ld3(L.pos[bj]), ldq(L.rot[bj]), spec, m);
+            valid = scene_contacts(si, ld3(L.pos[bi]), ldq(L.rot[bi]), sj, ld3(L.pos[bj]), ldq(L.rot[bj]), spec, m);
         }
         bd.m_valid[gm] = valid ? 1u : 0u;
         L.colour[k] = kInvalid;

@@ -504,26 +504,31 @@ __global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
     }
 }
 
-// serial compaction in pair order (world.h:1647-1653)
+// compaction in pair order (world.h:1647-1653)
 __global__ void k_compact(DevState d, uint32_t n_pairs) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t pts = 0;
     if (k < n_pairs) {
-    if (k == n_pairs - 1) d.cnt->manifolds = d.pm_off[k] + d.pm_valid[k];
-    uint32_t m = d.pm_off[k];
-    if (d.pm_valid[k] && m >= d.cap_manifolds) atomicOr(&d.cnt->err, ERR_MANIFOLDS);
-    else if (d.pm_valid[k]) {
-    d.m_ab[m] = d.pm_ab[k];
-    d.m_shapes[m] = d.pairs[k];
-    d.m_nf[m] = d.pm_nf[k];
-    uint4 meta = d.pm_meta[k];
-    d.m_meta[m] = meta;
-    for (int t = 0; t < 4; ++t) {
-        d.m_p[4 * m + t] = d.pm_p[4 * k + t];
-        d.m_q[4 * m + t] = d.pm_q[4 * k + t];
-    }
-    pts = meta.x & 255u;
-    }
+        if (k == n_pairs - 1) {
+            // clamp to the buffer so later kernels stay in bounds; the overflow is reported
+            uint32_t total = d.pm_off[k] + d.pm_valid[k];
+            d.cnt->manifolds = total < d.cap_manifolds ? total : d.cap_manifolds;
+        }
+        uint32_t m = d.pm_off[k];
+        if (d.pm_valid[k] && m >= d.cap_manifolds) {
+            atomicOr(&d.cnt->err, ERR_MANIFOLDS);
+        } else if (d.pm_valid[k]) {
+            d.m_ab[m] = d.pm_ab[k];
+            d.m_shapes[m] = d.pairs[k];
+            d.m_nf[m] = d.pm_nf[k];
+            uint4 meta = d.pm_meta[k];
+            d.m_meta[m] = meta;
+            for (int t = 0; t < 4; ++t) {
+                d.m_p[4 * m + t] = d.pm_p[4 * k + t];
+                d.m_q[4 * m + t] = d.pm_q[4 * k + t];
+            }
+            pts = meta.x & 255u;
+        }
     }
     pts = wave_sum(pts);
     if (lane0() && pts) atomicAdd(&d.cnt->points, pts);
@@ -671,9 +676,9 @@ __device__ __forceinline__ FeatBody feat_body(const DevState& d, uint32_t i) {
     return b;
 }
 
-__global__ void k_solver_setup(DevState d, StepParams prm, uint32_t n_ordered) {
+__global__ void k_solver_setup(DevState d, StepParams prm, uint32_t bound) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_ordered) return;
+    if (s >= bound || s >= d.cnt->ordered) return;
     uint32_t m = d.order[s];
     uint2 ab = d.m_ab[m];
     float4 nf = d.m_nf[m];
@@ -704,7 +709,7 @@ __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t n_ordered) {
             if (active) {
                 float4 cv;
                 bool hit = cache_lookup(d, prev, pack_key(ab.x, ab.y, __float_as_uint(qq.y)), cv);
-                if (hit) imp = cv;
+                if (hit) imp = make_float4(cv.x, cv.y, cv.z, 0.0f);
                 else aux.w = 1.0f;  // warm-start miss
                 if ((!hit && d.mlp) || d.record) {
                     if (!fb_loaded) { FA = feat_body(d, ab.x); FB = feat_body(d, ab.y); fb_loaded = true; }
@@ -787,18 +792,33 @@ __device__ __forceinline__ void apply_pair(SBody& a, SBody& b, v3 p, v3 J) {
     if (b.act) { b.v = b.v + J * b.im; b.w = b.w + mul(b.I, cross(p - b.x, J)); }
 }
 
+// Solver manifolds load all their data in one go: every load of a slot is independent of the
+// others, so they are all in flight together (a point loop that loads as it goes waits on each
+// point's loads in turn, since its stores may alias the next point's loads). Slots are padded
+// to four points, so the loads need no count. The four points are separate SSA values and a
+// point chosen at run time is a select (sel4), never an indexed access that would put the
+// slot in scratch memory.
+__device__ __forceinline__ float4 sel4(uint32_t i, float4 a, float4 b, float4 c, float4 e) {
+    return i == 0u ? a : i == 1u ? b : i == 2u ? c : e;
+}
+
 // warm_start's impulse application (solver.h:164-173) for solver manifold s
 __device__ void warm_manifold(const DevState& d, uint32_t s) {
-    uint2 ab = d.sm_ab[s];
+    const uint2 ab = d.sm_ab[s];
+    const float4 nf = d.sm_nf[s];
+    const uint32_t cnt = d.sm_cnt[s];
+    const float4 p0 = d.sp_pos[4 * s], p1 = d.sp_pos[4 * s + 1], p2 = d.sp_pos[4 * s + 2], p3 = d.sp_pos[4 * s + 3];
+    const float4 m0 = d.sp_imp[4 * s], m1 = d.sp_imp[4 * s + 1], m2 = d.sp_imp[4 * s + 2], m3v = d.sp_imp[4 * s + 3];
     SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
-    v3 n = ld3(d.sm_nf[s]);
-    uint32_t cnt = d.sm_cnt[s];
+    v3 n = ld3(nf);
     v3 t1, t2;
     orthonormal_basis(n, t1, t2);
-    for (uint32_t i = 0; i < cnt; ++i) {
-        float4 imp = d.sp_imp[4 * s + i];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        if (i >= cnt) break;
+        float4 imp = sel4(i, m0, m1, m2, m3v);
         v3 J = n * imp.x + t1 * imp.y + t2 * imp.z;
-        if (dot(J, J) > 0.0f) apply_pair(A, B, ld3(d.sp_pos[4 * s + i]), J);
+        if (dot(J, J) > 0.0f) apply_pair(A, B, ld3(sel4(i, p0, p1, p2, p3)), J);
     }
     store_sbody(d, ab.x, A);
     store_sbody(d, ab.y, B);
@@ -811,7 +831,7 @@ __global__ void k_warm_apply(DevState d, uint32_t begin, uint32_t end) {
 // solve_velocity_pass (solver.h:212-281), one colour
 __device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset);
 
-__global__ void k_velocity(DevState d, StepParams prm, uint32_t begin, uint32_t end, int use_rest, uint32_t offset) {
+__global__ void __launch_bounds__(kBlock) k_velocity(DevState d, StepParams prm, uint32_t begin, uint32_t end, int use_rest, uint32_t offset) {
     uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (prm.profile) {
         // algorithmic bytes of one velocity pass: 192 per manifold + 40 per point
@@ -823,52 +843,74 @@ __global__ void k_velocity(DevState d, StepParams prm, uint32_t begin, uint32_t 
     velocity_manifold(d, prm, s, use_rest, offset);
 }
 
+// One contact point's three rows (solver.h:226-277). Returns the updated impulses.
+__device__ __forceinline__ float4 velocity_point(SBody& A, SBody& B, v3 n, v3 t1, v3 t2, float mu, float4 pp,
+                                                 float4 aux, float4 imp, int use_rest, float h) {
+    v3 pos = ld3(pp);
+    v3 ra = pos - A.x, rb = pos - B.x;
+    {
+        v3 ran = cross(ra, n), rbn = cross(rb, n);
+        float kn = A.im + B.im + dot(ran, mul(A.I, ran)) + dot(rbn, mul(B.I, rbn));
+        if (kn > kEps) {
+            v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
+            float vn = dot(vb - va, n);
+            float target = aux.x > 0.0f ? -aux.x / h : (use_rest ? aux.y : 0.0f);
+            float lambda = -(vn - target) / kn;
+            float ni = gmax(imp.x + lambda, 0.0f);
+            float delta = ni - imp.x;
+            imp.x = ni;
+            apply_pair(A, B, pos, n * delta);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        v3 dir = t == 0 ? t1 : t2;
+        v3 rat = cross(ra, dir), rbt = cross(rb, dir);
+        float kt = A.im + B.im + dot(rat, mul(A.I, rat)) + dot(rbt, mul(B.I, rbt));
+        if (kt <= kEps) continue;
+        v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
+        float vt = dot(vb - va, dir);
+        float lt = -vt / kt;
+        float maxf = mu * imp.x;
+        float old = t == 0 ? imp.y : imp.z;
+        float nt = sclamp(old + lt, -maxf, maxf);
+        float delta = nt - old;
+        if (t == 0) imp.y = nt; else imp.z = nt;
+        apply_pair(A, B, pos, dir * delta);
+    }
+    return imp;
+}
+
+// Points run in the order (k + offset) % count (solver.h:221-224). The k-th point's slot index
+// is computed first, so the loads of all points are issued together from global memory and the
+// loop itself works on fixed registers.
 __device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset) {
-    uint2 ab = d.sm_ab[s];
+    const uint2 ab = d.sm_ab[s];
+    const float4 nf = d.sm_nf[s];
+    const uint32_t cnt = d.sm_cnt[s];
+    const uint32_t r = cnt ? offset % cnt : 0u;
+    uint32_t idx[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) idx[k] = 4u * s + (k < cnt ? (k + r) % cnt : k);
+    float4 pp[4], aux[4], imp[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        pp[k] = d.sp_pos[idx[k]];
+        aux[k] = d.sp_aux[idx[k]];
+        imp[k] = d.sp_imp[idx[k]];
+    }
     SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
-    float4 nf = d.sm_nf[s];
     v3 n = ld3(nf);
-    float mu = nf.w;
-    uint32_t cnt = d.sm_cnt[s];
     v3 t1, t2;
     orthonormal_basis(n, t1, t2);
-    for (uint32_t k = 0; k < cnt; ++k) {
-        uint32_t i = (k + offset) % cnt;
-        float4 pp = d.sp_pos[4 * s + i], aux = d.sp_aux[4 * s + i], imp = d.sp_imp[4 * s + i];
-        v3 pos = ld3(pp);
-        v3 ra = pos - A.x, rb = pos - B.x;
-        {
-            v3 ran = cross(ra, n), rbn = cross(rb, n);
-            float kn = A.im + B.im + dot(ran, mul(A.I, ran)) + dot(rbn, mul(B.I, rbn));
-            if (kn > kEps) {
-                v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
-                float vn = dot(vb - va, n);
-                float target = aux.x > 0.0f ? -aux.x / prm.h : (use_rest ? aux.y : 0.0f);
-                float lambda = -(vn - target) / kn;
-                float ni = gmax(imp.x + lambda, 0.0f);
-                float delta = ni - imp.x;
-                imp.x = ni;
-                apply_pair(A, B, pos, n * delta);
-            }
-        }
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            v3 dir = t == 0 ? t1 : t2;
-            v3 rat = cross(ra, dir), rbt = cross(rb, dir);
-            float kt = A.im + B.im + dot(rat, mul(A.I, rat)) + dot(rbt, mul(B.I, rbt));
-            if (kt <= kEps) continue;
-            v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
-            float vt = dot(vb - va, dir);
-            float lt = -vt / kt;
-            float maxf = mu * imp.x;
-            float old = t == 0 ? imp.y : imp.z;
-            float nt = sclamp(old + lt, -maxf, maxf);
-            float delta = nt - old;
-            if (t == 0) imp.y = nt; else imp.z = nt;
-            apply_pair(A, B, pos, dir * delta);
-        }
-        d.sp_imp[4 * s + i] = imp;
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= cnt) break;
+        imp[k] = velocity_point(A, B, n, t1, t2, nf.w, pp[k], aux[k], imp[k], use_rest, prm.h);
     }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+        if (k < cnt) d.sp_imp[idx[k]] = imp[k];
     store_sbody(d, ab.x, A);
     store_sbody(d, ab.y, B);
 }
@@ -915,6 +957,7 @@ __global__ void k_residual(DevState d, StepParams prm, uint32_t n_active, int us
     if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(&d.cnt->residual_bits, __float_as_uint(r));
 }
 __global__ void k_residual_reset(DevState d) { d.cnt->residual_bits = 0u; }
+__global__ void k_set_passes(DevState d, uint32_t n) { d.cnt->passes = n; }
 
 // solve_position setup: positions and reference snapshot (solver.h:295-296)
 __global__ void k_position_setup(DevState d) {
@@ -927,18 +970,23 @@ __global__ void k_position_setup(DevState d) {
 // solve_position for solver manifold s (solver.h:298-327)
 __device__ void position_manifold(const DevState& d, const StepParams& prm, uint32_t s) {
     uint2 ab = d.sm_ab[s];
+    v3 n = ld3(d.sm_nf[s]);
+    uint32_t cnt = d.sm_cnt[s];
+    float pen4[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) pen4[t] = d.sp_pos[4 * s + t].w;
     float4 xa4 = d.sb_x[ab.x], xb4 = d.sb_x[ab.y];
     bool aa = d.sb_w[ab.x].w != 0.0f, ba = d.sb_w[ab.y].w != 0.0f;
+    v3 ra = ld3(d.sb_ref[ab.x]), rb = ld3(d.sb_ref[ab.y]);
     float ima = xa4.w, imb = xb4.w;
     float k = ima + imb;
     if (k <= kEps) return;
     v3 xa = ld3(xa4), xb = ld3(xb4);
-    v3 ra = ld3(d.sb_ref[ab.x]), rb = ld3(d.sb_ref[ab.y]);
-    v3 n = ld3(d.sm_nf[s]);
     v3 da = xa - ra, db = xb - rb;
-    uint32_t cnt = d.sm_cnt[s];
-    for (uint32_t i = 0; i < cnt; ++i) {
-        float pen = d.sp_pos[4 * s + i].w;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        if (i >= cnt) break;
+        float pen = pen4[i];
         float cur = pen - dot(db - da, n);
         float mag = gmax(0.0f, cur - prm.slop) * prm.pos_corr;
         mag = gmin(mag, prm.max_corr);
@@ -967,9 +1015,9 @@ __global__ void k_bodies_back(DevState d) {
     d.b_angv[i] = make_float4(w.x, w.y, w.z, ow.w);
     d.b_pos[i] = make_float4(x.x, x.y, x.z, op.w);
 }
-__global__ void k_cache_insert(DevState d, uint32_t n_ordered) {
+__global__ void k_cache_insert(DevState d, uint32_t bound) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_ordered) return;
+    if (s >= bound || s >= d.cnt->ordered) return;
     uint2 ab = d.sm_ab[s];
     uint32_t cnt = d.sm_cnt[s];
     unsigned long long* keys = d.ck_keys[d.cur];
@@ -991,9 +1039,11 @@ __global__ void k_cache_insert(DevState d, uint32_t n_ordered) {
         if (!done) atomicOr(&d.cnt->err, ERR_CACHE);
     }
 }
-__global__ void k_cache_write(DevState d, uint32_t n_ordered) {
+// the value's w keeps the manifold's colour: the next step's colouring starts from it
+__global__ void k_cache_write(DevState d, uint32_t bound) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_ordered) return;
+    if (s >= bound || s >= d.cnt->ordered) return;
+    const float colour = __uint_as_float(d.m_colour[d.order[s]]);
     uint2 ab = d.sm_ab[s];
     uint32_t cnt = d.sm_cnt[s];
     const unsigned long long* keys = d.ck_keys[d.cur];
@@ -1004,7 +1054,10 @@ __global__ void k_cache_write(DevState d, uint32_t n_ordered) {
         uint32_t idx = hash64(key) & mask;
         for (uint32_t probe = 0; probe <= mask; ++probe) {
             if (keys[idx] == key) {
-                if (ord[idx] == 4u * s + i + 1u) d.ck_val[d.cur][idx] = d.sp_imp[4 * s + i];
+                if (ord[idx] == 4u * s + i + 1u) {
+                    float4 v = d.sp_imp[4 * s + i];
+                    d.ck_val[d.cur][idx] = make_float4(v.x, v.y, v.z, colour);
+                }
                 break;
             }
             if (keys[idx] == kEmptyKey) break;
@@ -1033,9 +1086,9 @@ __global__ void k_uf_init(DevState d) {
 }
 // unite dynamic-dynamic manifolds (triggers included); roots hook onto the smaller index so the
 // final root is the component's smallest body index
-__global__ void k_uf_union(DevState d, uint32_t n_man) {
+__global__ void k_uf_union(DevState d, uint32_t bound) {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n_man) return;
+    if (m >= bound || m >= d.cnt->manifolds) return;
     uint2 ab = d.m_ab[m];
     if ((d.b_flags[ab.x] & BF_TYPE) != 2u || (d.b_flags[ab.y] & BF_TYPE) != 2u) return;
     uint32_t a = ab.x, b = ab.y;
@@ -1066,9 +1119,9 @@ __global__ void k_sleep_timers(DevState d, StepParams p) {
     d.b_pos[i] = pt;
 }
 // wake_if_kinematic_moving both ways per valid manifold (solver.h:787-795, 849-853)
-__global__ void k_kin_wake(DevState d, StepParams p, uint32_t n_man) {
+__global__ void k_kin_wake(DevState d, StepParams p, uint32_t bound) {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n_man) return;
+    if (m >= bound || m >= d.cnt->manifolds) return;
     uint2 ab = d.m_ab[m];
     for (int side = 0; side < 2; ++side) {
         uint32_t kin = side == 0 ? ab.x : ab.y, dyn = side == 0 ? ab.y : ab.x;
@@ -1178,39 +1231,47 @@ __global__ void k_integrate_velocities(DevState d, StepParams p) {
 constexpr uint32_t ERR_BARRIER = 64u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
+// Counting barrier: bar[0] is zeroed before each launch and counts arrivals; barrier k of the
+// launch is complete when k * nblocks blocks have arrived. One release add per block, relaxed
+// polling, one acquire fence after the wait (an acquiring poll would invalidate caches on every
+// spin). Returns true when the launch must end: a wait expired (ERR_BARRIER is then set, and
+// every other block leaves at its next wait).
 struct GridBarrier {
-    uint32_t* bar;     // [0] arrivals, [1] generation
+    uint32_t* count;
     uint32_t nblocks;
-    uint32_t gen;      // generation this block expects next (thread 0's copy)
+    uint32_t epoch;  // barriers passed (thread 0's copy)
 };
-__device__ __forceinline__ void grid_sync(GridBarrier& g, uint32_t* err) {
+__device__ __forceinline__ bool grid_sync(GridBarrier& g, uint32_t* err) {
+    __shared__ uint32_t s_abort;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t arrived = __hip_atomic_fetch_add(&g.bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == g.nblocks - 1) {
-            __hip_atomic_store(&g.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(&g.bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&g.bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g.gen) {
-                if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    atomicOr(err, ERR_BARRIER);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+        g.epoch += 1;
+        __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t target = g.epoch * g.nblocks;
+        uint32_t spins = 0, abort = 0;
+        while (__hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > kSpinLimit) {
+                atomicOr(err, ERR_BARRIER);
+                abort = 1;
+                break;
             }
+            if ((spins & 255u) == 0u && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)) {
+                abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
-        g.gen += 1;
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_abort = abort;
     }
     __syncthreads();
+    return s_abort != 0u;
 }
 __device__ __forceinline__ GridBarrier grid_barrier_init(uint32_t* bar) {
     GridBarrier g;
-    g.bar = bar;
+    g.count = bar;
     g.nblocks = gridDim.x;
-    g.gen = __hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    g.epoch = 0;
     return g;
 }
 
@@ -1225,7 +1286,7 @@ struct SolveCfg {
 // The whole solve of one step (dynamics/solver.h:914-958 after the restitution capture):
 // warm start, velocity passes (fixed count or until the residual <= tolerance), relax passes,
 // position passes, colour by colour with a grid barrier between colours.
-__global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepParams prm, SolveCfg cfg, uint32_t* bar) {
+__global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, StepParams prm, SolveCfg cfg, uint32_t* bar) {
     GridBarrier g = grid_barrier_init(bar);
     uint32_t* err = &d.cnt->err;
     const uint32_t C = d.cnt->colours;
@@ -1233,7 +1294,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepPar
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     for (uint32_t c = 0; c < C; ++c) {
         for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) warm_manifold(d, s);
-        grid_sync(g, err);
+        if (grid_sync(g, err)) return;
     }
     if (prm.profile) {
         // algorithmic bytes of the solve: warm start 192 + 40n per manifold (bodies r/w and the
@@ -1248,7 +1309,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepPar
         if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
     }
     uint32_t passes = 0;
-    __shared__ float s_red[kBlock / 64];
+    __shared__ float s_red[kPersistBlock / 64];
     const uint32_t n_active = d.cnt->active;
     uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : cfg.vel_iters;
     for (uint32_t it = 0; it < max_pass; ++it) {
@@ -1261,7 +1322,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepPar
                 if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
             }
             for (uint32_t s = b + tid; s < e; s += nthreads) velocity_manifold(d, prm, s, 0, it);
-            grid_sync(g, err);
+            if (grid_sync(g, err)) return;
         }
         ++passes;
         if (cfg.residual_mode) {
@@ -1272,16 +1333,16 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepPar
             __syncthreads();
             if (threadIdx.x == 0) {
                 float m = 0.0f;
-                for (int w = 0; w < kBlock / 64; ++w) m = fmaxf(m, s_red[w]);
+                for (int w = 0; w < kPersistBlock / 64; ++w) m = fmaxf(m, s_red[w]);
                 if (m > 0.0f) atomicMax(&d.cnt->res_slot[it & 1], __float_as_uint(m));
             }
-            grid_sync(g, err);
+            if (grid_sync(g, err)) return;
             float rr = __uint_as_float(__hip_atomic_load(&d.cnt->res_slot[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
             if (tid == 0) {
                 d.cnt->residual_bits = __float_as_uint(rr);
                 d.cnt->res_slot[(it + 1) & 1] = 0u;  // the slot the next pass reduces into
             }
-            grid_sync(g, err);
+            if (grid_sync(g, err)) return;
             if (rr <= cfg.residual_tol) break;
         }
     }
@@ -1295,7 +1356,7 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepPar
                 if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
             }
             for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, 1, it);
-            grid_sync(g, err);
+            if (grid_sync(g, err)) return;
         }
     }
     // solve_position: snapshot, then position passes
@@ -1304,26 +1365,153 @@ __global__ void __launch_bounds__(kBlock) k_solve_persistent(DevState d, StepPar
         d.sb_x[i] = make_float4(p.x, p.y, p.z, d.sb_v[i].w);
         d.sb_ref[i] = p;
     }
-    grid_sync(g, err);
+    if (grid_sync(g, err)) return;
     for (uint32_t it = 0; it < cfg.pos_iters; ++it) {
         for (uint32_t c = 0; c < C; ++c) {
             for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) position_manifold(d, prm, s);
-            grid_sync(g, err);
+            if (grid_sync(g, err)) return;
         }
     }
 }
 
-// Colouring rounds in one cooperative launch: propose / select until every active manifold
-// has a colour (bounded by max_rounds).
-__global__ void __launch_bounds__(kBlock) k_colour_persistent(DevState d, uint32_t first_round, uint32_t max_rounds,
-                                                               uint32_t* bar) {
+// The solve without grid barriers: every body carries a version counter (zeroed per step) that
+// counts the solver updates applied to it. The updates of one body happen in solve order, pass
+// after pass, so the k-th manifold (in colour order) touching body a in pass p is update
+// p * deg(a) + k of a, where deg(a) = popcount(colours at a) and k = popcount(colours at a below
+// this manifold's colour) -- both read from the colouring's per-body colour mask. A manifold
+// waits until both of its awake dynamic bodies reach its update numbers, applies the update and
+// advances them. This is the same sequential order as the colour-by-colour solve (identical
+// results) while each manifold waits only for the manifolds it depends on.
+//
+// Progress: a thread takes its manifolds in (pass, colour) order and every wait is on a manifold
+// of an earlier (pass, colour); manifolds of one colour never share a written body, so lanes of a
+// wave never wait on each other. All blocks are co-resident (cooperative launch), so the earliest
+// unfinished manifold can always run. Every wait is bounded (ERR_BARRIER, then all waits end).
+__device__ __forceinline__ bool wait_version(const uint32_t* ver, uint32_t b, uint32_t want, uint32_t* err) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(&ver[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
+        if (++spins > kSpinLimit) {
+            atomicOr(err, ERR_BARRIER);
+            return false;
+        }
+        if ((spins & 255u) == 0u && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER))
+            return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(kPersistBlock) k_solve_dataflow(DevState d, StepParams prm, SolveCfg cfg) {
+    uint32_t* err = &d.cnt->err;
+    uint32_t* ver = d.ver;
+    const uint32_t C = d.cnt->colours;
+    const uint32_t* co = d.cnt->colour_off;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
+    if (prm.profile) {
+        // algorithmic bytes: warm start and each velocity/relax pass 192 + 40n per manifold, each
+        // position pass 92 + 4n
+        uint32_t bytes = 0;
+        const uint32_t vp = 1u + cfg.vel_iters + cfg.relax_iters;
+        for (uint32_t s = tid; s < n_active_all(d); s += nthreads) {
+            uint32_t n = d.sm_cnt[s];
+            bytes += vp * (192u + 40u * n) + cfg.pos_iters * (92u + 4u * n);
+        }
+        bytes = wave_sum(bytes);
+        if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+    }
+    const uint32_t passes = 1u + cfg.vel_iters + cfg.relax_iters + cfg.pos_iters;
+    for (uint32_t p = 0; p < passes; ++p) {
+        for (uint32_t c = 0; c < C; ++c) {
+            const unsigned long long below = (1ull << c) - 1ull;
+            for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) {
+                uint2 ab = d.sm_ab[s];
+                bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
+                uint32_t ea = 0, eb = 0;
+                if (aa) {
+                    unsigned long long m = d.col_mask[ab.x];
+                    ea = p * uint32_t(__popcll(m)) + uint32_t(__popcll(m & below));
+                    if (!wait_version(ver, ab.x, ea, err)) return;
+                }
+                if (ba) {
+                    unsigned long long m = d.col_mask[ab.y];
+                    eb = p * uint32_t(__popcll(m)) + uint32_t(__popcll(m & below));
+                    if (!wait_version(ver, ab.y, eb, err)) return;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (p == 0) warm_manifold(d, s);
+                else if (p <= cfg.vel_iters) velocity_manifold(d, prm, s, 0, p - 1u);
+                else if (p <= cfg.vel_iters + cfg.relax_iters) velocity_manifold(d, prm, s, 1, p - 1u - cfg.vel_iters);
+                else position_manifold(d, prm, s);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (aa) __hip_atomic_store(&ver[ab.x], ea + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (ba) __hip_atomic_store(&ver[ab.y], eb + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    if (tid == 0) d.cnt->passes = cfg.vel_iters;
+}
+
+// The whole colouring in one cooperative launch. Contacts persist from step to step, so each
+// active manifold first tries the colour its first contact point had last step (kept in the
+// warm-start entry). A hinted colour is kept unless another hinted manifold claims the same colour
+// at the same awake dynamic body, in which case every claimant drops it; the outcome does not
+// depend on timing. Jones-Plassmann rounds (first-fit over the per-body masks) colour the rest.
+__global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d, uint32_t max_rounds, uint32_t* bar) {
     GridBarrier g = grid_barrier_init(bar);
     uint32_t* err = &d.cnt->err;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     const uint32_t n_man = d.cnt->manifolds;
-    for (uint32_t r = 0; r < max_rounds; ++r) {
+    const int prev = d.cur ^ 1;
+    constexpr uint32_t kHinted = 0x80000000u;
+    // A: claim last step's colour
+    for (uint32_t m = tid; m < n_man; m += nthreads) {
+        if (d.m_colour[m] != kUncoloured) continue;
+        uint2 ab = d.m_ab[m];
+        float4 cv;
+        if (!cache_lookup(d, prev, pack_key(ab.x, ab.y, __float_as_uint(d.m_q[4 * m].y)), cv)) continue;
+        uint32_t c = __float_as_uint(cv.w);
+        if (c >= uint32_t(kMaxColours)) continue;
+        unsigned long long bit = 1ull << c;
+        d.m_colour[m] = kHinted | c;
+        if (body_active(d.b_flags[ab.x]) && (atomicOr(&d.col_mask[ab.x], bit) & bit)) atomicOr(&d.col_dup[ab.x], bit);
+        if (body_active(d.b_flags[ab.y]) && (atomicOr(&d.col_mask[ab.y], bit) & bit)) atomicOr(&d.col_dup[ab.y], bit);
+    }
+    if (grid_sync(g, err)) return;
+    // B: keep the uncontested claims; a dropped claim clears its bit (no kept manifold shares it)
+    {
+        uint32_t base = tid - (tid & 63u);
+        for (uint32_t m0 = base; m0 < n_man; m0 += nthreads) {
+            uint32_t m = m0 + (tid & 63u);
+            uint32_t took = 0, top = 0;
+            uint32_t col = m < n_man ? d.m_colour[m] : 0u;
+            if (m < n_man && col != kUncoloured && (col & kHinted) && col < (kHinted | uint32_t(kMaxColours))) {
+                uint32_t c = col & ~kHinted;
+                unsigned long long bit = 1ull << c;
+                uint2 ab = d.m_ab[m];
+                bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
+                bool lost = (aa && (d.col_dup[ab.x] & bit)) || (ba && (d.col_dup[ab.y] & bit));
+                if (lost) {
+                    d.m_colour[m] = kUncoloured;
+                    if (aa) atomicAnd(&d.col_mask[ab.x], ~bit);
+                    if (ba) atomicAnd(&d.col_mask[ab.y], ~bit);
+                } else {
+                    d.m_colour[m] = c;
+                    took = 1u;
+                    top = c + 1u;
+                }
+            }
+            took = wave_sum(took);
+            top = wave_max(top);
+            if (lane0() && took) {
+                atomicSub(&d.cnt->remaining, took);
+                atomicMax(&d.cnt->colours, top);
+            }
+        }
+    }
+    if (grid_sync(g, err)) return;
+    // C: Jones-Plassmann rounds for the rest
+    for (uint32_t round = 1; round <= max_rounds; ++round) {
         if (__hip_atomic_load(&d.cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) break;
-        uint32_t round = first_round + r;
         for (uint32_t m = tid; m < n_man; m += nthreads) {
             if (d.m_colour[m] != kUncoloured) continue;
             uint2 ab = d.m_ab[m];
@@ -1331,7 +1519,7 @@ __global__ void __launch_bounds__(kBlock) k_colour_persistent(DevState d, uint32
             if (body_active(d.b_flags[ab.x])) atomicMax(&d.col_best[ab.x], key);
             if (body_active(d.b_flags[ab.y])) atomicMax(&d.col_best[ab.y], key);
         }
-        grid_sync(g, err);
+        if (grid_sync(g, err)) return;
         uint32_t base = tid - (tid & 63u);
         for (uint32_t m0 = base; m0 < n_man; m0 += nthreads) {
             uint32_t m = m0 + (tid & 63u);
@@ -1364,9 +1552,12 @@ __global__ void __launch_bounds__(kBlock) k_colour_persistent(DevState d, uint32
                 atomicMax(&d.cnt->colours, top);
             }
         }
-        grid_sync(g, err);
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER) break;
+        if (grid_sync(g, err)) return;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_COLOURS) break;
     }
+    // an uncoloured active manifold would be left out of the solve: report it
+    if (tid == 0 && __hip_atomic_load(&d.cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+        atomicOr(err, ERR_COLOURS);
 }
 
 inline uint32_t blocks(uint32_t n) { return (n + kBlock - 1) / kBlock; }
@@ -1426,6 +1617,7 @@ void launch_narrowphase(const DevState& d, const StepParams& p, uint32_t n_pairs
 void launch_colour_init(const DevState& d, uint32_t n_man, hipStream_t st) {
     launch_fill_u64(d.col_best, 0ull, d.nb, st);
     launch_fill_u64(d.col_mask, 0ull, d.nb, st);
+    launch_fill_u64(d.col_dup, 0ull, d.nb, st);
     if (n_man) hipLaunchKernelGGL(k_colour_init, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
 }
 void launch_colour_rounds(const DevState& d, uint32_t n_man, uint32_t first_round, uint32_t rounds, hipStream_t st) {
@@ -1447,8 +1639,8 @@ void launch_build_order(const DevState& d, uint32_t n_man, hipStream_t st) {
     if (n_man) hipLaunchKernelGGL(k_colour_scatter, dim3(nblk), dim3(kBlock), 0, st, d, n_man, nblk);
     hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(128), 0, st, d, nblk, total);
 }
-void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t n_ordered, hipStream_t st) {
-    if (n_ordered) hipLaunchKernelGGL(k_solver_setup, dim3(blocks(n_ordered)), dim3(kBlock), 0, st, d, p, n_ordered);
+void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st) {
+    if (bound) hipLaunchKernelGGL(k_solver_setup, dim3(blocks(bound)), dim3(kBlock), 0, st, d, p, bound);
     if (d.nb) hipLaunchKernelGGL(k_solver_bodies, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
 }
 void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st) {
@@ -1468,60 +1660,85 @@ void launch_residual(const DevState& d, const StepParams& p, uint32_t n_active, 
     hipLaunchKernelGGL(k_residual_reset, dim3(1), dim3(1), 0, st, d);
     if (n_active) hipLaunchKernelGGL(k_residual, dim3(blocks(n_active)), dim3(kBlock), 0, st, d, p, n_active, use_rest);
 }
+void launch_set_passes(const DevState& d, uint32_t passes, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_passes, dim3(1), dim3(1), 0, st, d, passes);
+}
 void launch_position_setup(const DevState& d, hipStream_t st) {
     if (d.nb) hipLaunchKernelGGL(k_position_setup, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
 }
 void launch_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st) {
     if (end > begin) hipLaunchKernelGGL(k_position, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, p, begin, end);
 }
-void launch_solver_finish(const DevState& d, uint32_t n_ordered, hipStream_t st) {
+void launch_solver_finish(const DevState& d, uint32_t bound, hipStream_t st) {
     if (d.nb) hipLaunchKernelGGL(k_bodies_back, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     size_t cap = size_t(d.cache_mask) + 1;
     launch_fill_u64(d.ck_keys[d.cur], kEmptyKey, cap, st);
     launch_fill_u32(d.ck_ord[d.cur], 0u, cap, st);
-    if (n_ordered) {
-        hipLaunchKernelGGL(k_cache_insert, dim3(blocks(n_ordered)), dim3(kBlock), 0, st, d, n_ordered);
-        hipLaunchKernelGGL(k_cache_write, dim3(blocks(n_ordered)), dim3(kBlock), 0, st, d, n_ordered);
+    if (bound) {
+        hipLaunchKernelGGL(k_cache_insert, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
+        hipLaunchKernelGGL(k_cache_write, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
     }
 }
-void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t n_man, hipStream_t st) {
+void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st) {
     if (!d.nb) return;
     hipLaunchKernelGGL(k_uf_init, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
-    if (n_man) hipLaunchKernelGGL(k_uf_union, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
+    if (bound) hipLaunchKernelGGL(k_uf_union, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
     hipLaunchKernelGGL(k_uf_compress, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_sleep_timers, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
-    if (n_man) hipLaunchKernelGGL(k_kin_wake, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, p, n_man);
+    if (bound) hipLaunchKernelGGL(k_kin_wake, dim3(blocks(bound)), dim3(kBlock), 0, st, d, p, bound);
     hipLaunchKernelGGL(k_island_min, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_sleep, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
 }
-// cooperative launches; returns the HIP error of the launch
+// cooperative launches; returns the HIP error of the launch. The barrier count is zeroed by a
+// kernel (not a memset) so the launch sequence may be captured in a graph.
 hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[6], float tol,
-                                   uint32_t nblocks, uint32_t* bar, hipStream_t st) {
+                                   uint32_t nblocks, hipStream_t st) {
     SolveCfg c;
     c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
     c.residual_mode = cfgv[3]; c.residual_max = cfgv[4]; c.residual_tol = tol;
+    launch_fill_u32(d.bar, 0u, 1, st);
     DevState dd = d;
     StepParams pp = p;
+    uint32_t* bar = d.bar;
     void* args[] = {&dd, &pp, &c, &bar};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks), dim3(kBlock),
-                                      args, 0, st);
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks),
+                                      dim3(kPersistBlock), args, 0, st);
 }
-hipError_t launch_colour_persistent(const DevState& d, uint32_t first_round, uint32_t max_rounds, uint32_t nblocks,
-                                    uint32_t* bar, hipStream_t st) {
+hipError_t launch_solve_dataflow(const DevState& d, const StepParams& p, const uint32_t cfgv[6], uint32_t nblocks,
+                                 hipStream_t st) {
+    SolveCfg c;
+    c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
+    c.residual_mode = 0; c.residual_max = 0; c.residual_tol = 0.0f;
+    launch_fill_u32(d.ver, 0u, d.nb ? d.nb : 1, st);
+    if (d.nb) hipLaunchKernelGGL(k_position_setup, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     DevState dd = d;
-    void* args[] = {&dd, &first_round, &max_rounds, &bar};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_colour_persistent), dim3(nblocks), dim3(kBlock),
-                                      args, 0, st);
+    StepParams pp = p;
+    void* args[] = {&dd, &pp, &c};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_dataflow), dim3(nblocks),
+                                      dim3(kPersistBlock), args, 0, st);
+}
+hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, hipStream_t st) {
+    launch_fill_u32(d.bar, 0u, 1, st);
+    DevState dd = d;
+    uint32_t* bar = d.bar;
+    void* args[] = {&dd, &max_rounds, &bar};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_colour_persistent), dim3(nblocks),
+                                      dim3(kPersistBlock), args, 0, st);
 }
 int max_coresident_blocks(int device) {
-    int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve_persistent), kBlock, 0) != hipSuccess)
+    int per_cu = 0, per_cu2 = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_solve_persistent),
+                                                     kPersistBlock, 0) != hipSuccess)
         return 0;
-    int per_cu2 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, reinterpret_cast<const void*>(&k_colour_persistent), kBlock, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, reinterpret_cast<const void*>(&k_colour_persistent),
+                                                     kPersistBlock, 0) != hipSuccess)
+        return 0;
+    int per_cu3 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu3, reinterpret_cast<const void*>(&k_solve_dataflow),
+                                                     kPersistBlock, 0) != hipSuccess)
         return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    return std::min(per_cu, per_cu2) * cus;
+    return std::min(std::min(per_cu, per_cu2), per_cu3) * cus;
 }
 
 void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st) {

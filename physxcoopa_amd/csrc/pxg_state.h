@@ -11,6 +11,7 @@ namespace pxg {
 
 struct MlpDev;
 constexpr int kBlock = 256;
+constexpr int kPersistBlock = 512;  // persistent cooperative kernels: 8 waves per block (up to 256 VGPRs each)
 constexpr int kMaxColours = 64;      // colours per step (64-bit per-body colour masks)
 constexpr int kBuckets = kMaxColours + 1;  // colours + the "skipped by the solver" group
 constexpr int kMaxLarge = 256;       // shapes on the brute-force broadphase path
@@ -95,6 +96,7 @@ struct DevState {
     uint32_t *m_colour;
     unsigned long long *col_best;   // per body
     unsigned long long *col_mask;   // per body
+    unsigned long long *col_dup;    // per body: colours claimed twice by last step's colours
     uint32_t *col_table, *col_table_off;
     uint32_t *order;                // solve position -> manifold index
     // solver arrays, solve order
@@ -118,7 +120,8 @@ struct DevState {
     // islands
     uint32_t *uf_parent;
     uint32_t *isl_min;
-    uint32_t *bar;      // grid barrier state of the persistent kernels: arrivals, generation
+    uint32_t *bar;      // grid barrier of the persistent kernels: arrival count (zeroed per launch)
+    uint32_t *ver;      // per body: solver updates applied this step (dataflow solve)
     Counters *cnt;
     // learned warm start
     const struct MlpDev *mlp;  // BF16 network (nullptr = off)
@@ -138,7 +141,8 @@ struct StepParams {
     uint32_t profile;   // 1: velocity passes count their algorithmic bytes
 };
 
-// launchers (pxg_kernels.hip); all asynchronous on `st`
+// launchers (pxg_kernels.hip); all asynchronous on `st`. Where a launcher takes a count bound,
+// the grid covers the bound and the kernels read the step's actual count from Counters.
 void launch_integrate_forces(const DevState& d, const StepParams& p, hipStream_t st);
 void launch_broadphase_count(const DevState& d, const StepParams& p, hipStream_t st);
 void launch_broadphase_write(const DevState& d, const StepParams& p, hipStream_t st);
@@ -146,15 +150,16 @@ void launch_narrowphase(const DevState& d, const StepParams& p, uint32_t n_pairs
 void launch_colour_init(const DevState& d, uint32_t n_manifolds, hipStream_t st);
 void launch_colour_rounds(const DevState& d, uint32_t n_manifolds, uint32_t first_round, uint32_t rounds, hipStream_t st);
 void launch_build_order(const DevState& d, uint32_t n_manifolds, hipStream_t st);
-void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t n_ordered, hipStream_t st);
+void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st);
 void launch_warm_apply(const DevState& d, uint32_t begin, uint32_t end, hipStream_t st);
 void launch_velocity(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, int use_rest, uint32_t offset,
                      hipStream_t st);
 void launch_residual(const DevState& d, const StepParams& p, uint32_t n_active, int use_rest, hipStream_t st);
 void launch_position_setup(const DevState& d, hipStream_t st);
+void launch_set_passes(const DevState& d, uint32_t passes, hipStream_t st);  // residual mode, eager path
 void launch_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st);
-void launch_solver_finish(const DevState& d, uint32_t n_ordered, hipStream_t st);
-void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t n_manifolds, hipStream_t st);
+void launch_solver_finish(const DevState& d, uint32_t bound, hipStream_t st);
+void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st);
 void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st);
 void launch_init_bodies(const DevState& d, hipStream_t st);
 void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st);
@@ -162,12 +167,16 @@ void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st)
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* scratch, hipStream_t st);
-// persistent cooperative kernels (grid barriers in bar[2]); cfgv = vel, relax, pos iterations,
-// residual mode, residual max passes
+// persistent cooperative kernels (kPersistBlock threads per block, grid barrier on d.bar);
+// cfgv = vel, relax, pos iterations, residual mode, residual max passes
 hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[6], float tol,
-                                   uint32_t nblocks, uint32_t* bar, hipStream_t st);
-hipError_t launch_colour_persistent(const DevState& d, uint32_t first_round, uint32_t max_rounds, uint32_t nblocks,
-                                    uint32_t* bar, hipStream_t st);
+                                   uint32_t nblocks, hipStream_t st);
+// the same solve in fixed-iteration mode with per-body version counters instead of grid barriers
+hipError_t launch_solve_dataflow(const DevState& d, const StepParams& p, const uint32_t cfgv[6], uint32_t nblocks,
+                                 hipStream_t st);
+// colours every active manifold: last step's colours where still valid, then Jones-Plassmann
+// rounds (at most max_rounds) for the rest
+hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, hipStream_t st);
 int max_coresident_blocks(int device);
 constexpr uint32_t ERR_BARRIER_BIT = 64u;
 

@@ -4,6 +4,7 @@
 // narrowphase) and once for the colour table (sizes the per-colour solver launches).
 #include "../../include/pxg.h"
 #include "pxg_state.h"
+#include "pxg_hostio.h"
 #include "pxg_mlp.h"
 
 #include <hip/hip_runtime.h>
@@ -44,6 +45,8 @@ struct pxg_world {
     float accumulator = 0.0f;
     int phase = 0;              // 0: next call collides, 1: collision done, solve pending
     uint32_t last_pairs = 0, last_manifolds = 0, last_ordered = 0, last_active = 0, last_colours = 0;
+    bool results_pending = false;  // the last step's counters are still on the device
+    uint32_t colour_blocks = 64, solve_blocks = 32;  // caps on the persistent kernels' grids
     uint32_t colour_off[kBuckets + 1];
     pxg_stats stats{};
     uint32_t colour_round = 0;
@@ -57,6 +60,8 @@ struct pxg_world {
     int coop_blocks = 0;        // co-resident blocks for the persistent kernels (0: not available)
     MlpDev* d_mlp = nullptr;
     bool eager = false;         // PXG_EAGER=1: one launch per colour instead of persistent kernels
+    std::string coop_note;      // why the persistent kernels were dropped, if they were
+    bool dataflow = true;       // fixed-iteration solve without grid barriers (PXG_SOLVE=barrier: off)
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -175,6 +180,10 @@ extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
     w->coop_blocks = coop ? max_coresident_blocks(w->device) : 0;
     const char* eager = std::getenv("PXG_EAGER");
     w->eager = eager && eager[0] == '1';
+    // grid caps of the persistent kernels: fewer blocks make each grid barrier cheaper
+    if (const char* v = std::getenv("PXG_SOLVE_BLOCKS")) w->solve_blocks = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("PXG_COLOUR_BLOCKS")) w->colour_blocks = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("PXG_SOLVE")) w->dataflow = std::strcmp(v, "barrier") != 0;
     *out = w;
     return PXG_OK;
 }
@@ -226,9 +235,9 @@ static int alloc_bodies(pxg_world* w, uint32_t n) {
     DALLOC(d.b_pos, cap); DALLOC(d.b_rot, cap); DALLOC(d.b_vel, cap); DALLOC(d.b_angv, cap);
     DALLOC(d.b_force, cap); DALLOC(d.b_torque, cap); DALLOC(d.b_iil, cap); DALLOC(d.b_iiw, 3 * size_t(cap));
     DALLOC(d.b_prev_pos, cap); DALLOC(d.b_prev_rot, cap); DALLOC(d.b_flags, cap);
-    DALLOC(d.col_best, cap); DALLOC(d.col_mask, cap);
+    DALLOC(d.col_best, cap); DALLOC(d.col_mask, cap); DALLOC(d.col_dup, cap);
     DALLOC(d.sb_v, cap); DALLOC(d.sb_w, cap); DALLOC(d.sb_I, 3 * size_t(cap)); DALLOC(d.sb_x, cap); DALLOC(d.sb_ref, cap);
-    DALLOC(d.uf_parent, cap); DALLOC(d.isl_min, cap);
+    DALLOC(d.uf_parent, cap); DALLOC(d.isl_min, cap); DALLOC(d.ver, cap);
     DALLOC(d.cnt, 1);
     DALLOC(d.layer_mask, 32);
     DALLOC(d.scan_tmp, 4096);
@@ -288,47 +297,44 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
 // ---------------------------------------------------------------------------------------------
 // uploads and downloads
 // ---------------------------------------------------------------------------------------------
+// Between the two step phases (the on_substep seam, world.h:1299) only the state of the same
+// bodies may change: the count and every body's type must stay, and the world inertia is kept.
 extern "C" int pxg_set_bodies(pxg_world* w, const pxg_body* bodies, uint32_t n) {
     if (!w || (!bodies && n)) return fail(PXG_ERR_INVALID, "pxg_set_bodies: null argument");
-    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_bodies: not allowed between the two step phases");
-    int r = alloc_bodies(w, n);
-    if (r != PXG_OK) return r;
-    std::vector<float4> pos(n), rot(n), vel(n), angv(n), force(n), torque(n), iil(n), ppos(n), prot(n);
-    std::vector<uint32_t> flags(n);
-    w->body_types.resize(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const pxg_body& b = bodies[i];
-        pos[i] = make_float4(b.position[0], b.position[1], b.position[2], b.sleep_timer);
-        rot[i] = make_float4(b.orientation[0], b.orientation[1], b.orientation[2], b.orientation[3]);
-        vel[i] = make_float4(b.linear_velocity[0], b.linear_velocity[1], b.linear_velocity[2], b.inv_mass);
-        angv[i] = make_float4(b.angular_velocity[0], b.angular_velocity[1], b.angular_velocity[2], b.mass);
-        force[i] = make_float4(b.force[0], b.force[1], b.force[2], b.linear_drag);
-        torque[i] = make_float4(b.torque[0], b.torque[1], b.torque[2], b.angular_drag);
-        iil[i] = make_float4(b.inv_inertia_local[0], b.inv_inertia_local[1], b.inv_inertia_local[2], 0.0f);
-        ppos[i] = make_float4(b.prev_position[0], b.prev_position[1], b.prev_position[2], 0.0f);
-        prot[i] = make_float4(b.prev_orientation[0], b.prev_orientation[1], b.prev_orientation[2], b.prev_orientation[3]);
-        if (b.type > 2) return fail(PXG_ERR_INVALID, "pxg_set_bodies: bad body type");
-        flags[i] = (b.type & BF_TYPE) | (b.alive ? BF_ALIVE : 0u) | (b.awake ? BF_AWAKE : 0u) |
-                   (b.use_gravity ? BF_GRAVITY : 0u) | ((b.constraints & 63u) << BF_CONSTRAINT_SHIFT);
-        w->body_types[i] = b.type;
+    const bool seam = w->phase != 0;
+    if (seam) {
+        if (n != w->n_bodies) return fail(PXG_ERR_STATE, "pxg_set_bodies: the body count cannot change between the two step phases");
+        for (uint32_t i = 0; i < n; ++i)
+            if (bodies[i].type != w->body_types[i])
+                return fail(PXG_ERR_STATE, "pxg_set_bodies: body types cannot change between the two step phases");
+    }
+    BodySoA soa;
+    if (!pack_bodies(bodies, n, soa)) return fail(PXG_ERR_INVALID, "pxg_set_bodies: bad body type");
+    if (!seam) {
+        int r = alloc_bodies(w, n);
+        if (r != PXG_OK) return r;
+        w->body_types.resize(n);
+        for (uint32_t i = 0; i < n; ++i) w->body_types[i] = bodies[i].type;
     }
     DevState& d = w->d;
     hipStream_t s = w->stream;
     if (n) {
-        HIPCHK(hipMemcpyAsync(d.b_pos, pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_rot, rot.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_vel, vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_angv, angv.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_force, force.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_torque, torque.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_iil, iil.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_prev_pos, ppos.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_prev_rot, prot.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d.b_flags, flags.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_pos, soa.pos.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_rot, soa.rot.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_vel, soa.vel.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_angv, soa.angv.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_force, soa.force.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_torque, soa.torque.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_iil, soa.iil.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_prev_pos, soa.ppos.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_prev_rot, soa.prot.data(), n * sizeof(float4), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d.b_flags, soa.flags.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     }
-    d.nb = n;
-    w->n_bodies = n;
-    launch_init_bodies(d, s);
+    if (!seam) {
+        d.nb = n;
+        w->n_bodies = n;
+        launch_init_bodies(d, s);
+    }
     HIPCHK(hipStreamSynchronize(s));  // host staging buffers go out of scope
     return PXG_OK;
 }
@@ -337,44 +343,24 @@ extern "C" int pxg_get_bodies(pxg_world* w, pxg_body* out, uint32_t n) {
     if (!w || (!out && n)) return fail(PXG_ERR_INVALID, "pxg_get_bodies: null argument");
     if (n > w->n_bodies) return fail(PXG_ERR_INVALID, "pxg_get_bodies: n exceeds the body count");
     if (!n) return PXG_OK;
-    std::vector<float4> pos(n), rot(n), vel(n), angv(n), force(n), torque(n), iil(n), iiw(3 * n), ppos(n), prot(n);
-    std::vector<uint32_t> flags(n);
+    BodySoA soa;
+    soa.resize(n);
+    std::vector<float4> iiw(3 * size_t(n));
     DevState& d = w->d;
     hipStream_t s = w->stream;
-    HIPCHK(hipMemcpyAsync(pos.data(), d.b_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(rot.data(), d.b_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(vel.data(), d.b_vel, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(angv.data(), d.b_angv, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(force.data(), d.b_force, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(torque.data(), d.b_torque, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(iil.data(), d.b_iil, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.pos.data(), d.b_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.rot.data(), d.b_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.vel.data(), d.b_vel, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.angv.data(), d.b_angv, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.force.data(), d.b_force, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.torque.data(), d.b_torque, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.iil.data(), d.b_iil, n * sizeof(float4), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(iiw.data(), d.b_iiw, 3 * n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(ppos.data(), d.b_prev_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(prot.data(), d.b_prev_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(flags.data(), d.b_flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.ppos.data(), d.b_prev_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.prot.data(), d.b_prev_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(soa.flags.data(), d.b_flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    for (uint32_t i = 0; i < n; ++i) {
-        pxg_body& b = out[i];
-        std::memset(&b, 0, sizeof(b));
-        uint32_t f = flags[i];
-        b.type = f & BF_TYPE;
-        b.alive = (f & BF_ALIVE) ? 1u : 0u;
-        b.awake = (f & BF_AWAKE) ? 1u : 0u;
-        b.use_gravity = (f & BF_GRAVITY) ? 1u : 0u;
-        b.constraints = (f >> BF_CONSTRAINT_SHIFT) & 63u;
-        b.position[0] = pos[i].x; b.position[1] = pos[i].y; b.position[2] = pos[i].z; b.sleep_timer = pos[i].w;
-        b.orientation[0] = rot[i].x; b.orientation[1] = rot[i].y; b.orientation[2] = rot[i].z; b.orientation[3] = rot[i].w;
-        b.linear_velocity[0] = vel[i].x; b.linear_velocity[1] = vel[i].y; b.linear_velocity[2] = vel[i].z; b.inv_mass = vel[i].w;
-        b.angular_velocity[0] = angv[i].x; b.angular_velocity[1] = angv[i].y; b.angular_velocity[2] = angv[i].z; b.mass = angv[i].w;
-        b.force[0] = force[i].x; b.force[1] = force[i].y; b.force[2] = force[i].z; b.linear_drag = force[i].w;
-        b.torque[0] = torque[i].x; b.torque[1] = torque[i].y; b.torque[2] = torque[i].z; b.angular_drag = torque[i].w;
-        b.inv_inertia_local[0] = iil[i].x; b.inv_inertia_local[1] = iil[i].y; b.inv_inertia_local[2] = iil[i].z;
-        const float* m = reinterpret_cast<const float*>(&iiw[3 * i]);
-        for (int k = 0; k < 9; ++k) b.inv_inertia_world[k] = m[k];
-        b.prev_position[0] = ppos[i].x; b.prev_position[1] = ppos[i].y; b.prev_position[2] = ppos[i].z;
-        b.prev_orientation[0] = prot[i].x; b.prev_orientation[1] = prot[i].y; b.prev_orientation[2] = prot[i].z;
-        b.prev_orientation[3] = prot[i].w;
-    }
+    unpack_bodies(soa, iiw, n, out);
     return PXG_OK;
 }
 
@@ -384,29 +370,17 @@ extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n)
     if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_shapes: not allowed between the two step phases");
     int r = alloc_shapes(w, n);
     if (r != PXG_OK) return r;
-    std::vector<uint4> info(n);
-    std::vector<float4> lc(n), lr(n), dim(n), mat(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const pxg_shape& s = shapes[i];
-        if (s.owner >= w->n_bodies) return fail(PXG_ERR_INVALID, "pxg_set_shapes: owner out of range");
-        if (s.type > 2) return fail(PXG_ERR_INVALID, "pxg_set_shapes: only sphere, box and capsule are supported");
-        if (s.capsule_axis < 0 || s.capsule_axis > 2) return fail(PXG_ERR_INVALID, "pxg_set_shapes: bad capsule axis");
-        uint32_t flags = s.type | (s.enabled ? SF_ENABLED : 0u) | (s.is_trigger ? SF_TRIGGER : 0u) |
-                         (w->body_types[s.owner] == PXG_STATIC ? SF_STATIC : 0u) | (uint32_t(s.capsule_axis) << 8);
-        info[i] = make_uint4(s.owner, flags, s.layer & 31u, (s.friction_combine & 255u) | ((s.restitution_combine & 255u) << 8));
-        lc[i] = make_float4(s.local_center[0], s.local_center[1], s.local_center[2], s.radius);
-        lr[i] = make_float4(s.local_rotation[0], s.local_rotation[1], s.local_rotation[2], s.local_rotation[3]);
-        dim[i] = make_float4(s.half_extents[0], s.half_extents[1], s.half_extents[2], s.capsule_radius);
-        mat[i] = make_float4(s.capsule_half_height, s.dynamic_friction, s.restitution, s.static_friction);
-    }
+    ShapeSoA soa;
+    const char* err = pack_shapes(shapes, n, w->n_bodies, [&](size_t i) { return w->body_types[shapes[i].owner] == PXG_STATIC; }, soa);
+    if (err) return fail(PXG_ERR_INVALID, std::string("pxg_set_shapes: ") + err);
     DevState& d = w->d;
     hipStream_t st = w->stream;
     if (n) {
-        HIPCHK(hipMemcpyAsync(d.s_info, info.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(d.s_lc, lc.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(d.s_lr, lr.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(d.s_dim, dim.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
-        HIPCHK(hipMemcpyAsync(d.s_mat, mat.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_info, soa.info.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_lc, soa.lc.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_lr, soa.lr.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_dim, soa.dim.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.s_mat, soa.mat.data(), n * sizeof(float4), hipMemcpyHostToDevice, st));
     }
     HIPCHK(hipMemcpyAsync(d.layer_mask, w->layer_masks, sizeof(w->layer_masks), hipMemcpyHostToDevice, st));
     d.ns = n;
@@ -457,9 +431,10 @@ static int phase_collide(pxg_world* w, float h) {
     DevState& d = w->d;
     hipStream_t st = w->stream;
     StepParams p = params(w, h);
+    w->results_pending = false;  // a new step starts: the last step's counters are reset
     launch_integrate_forces(d, p, st);
     launch_broadphase_count(d, p, st);
-    int r = read_counters(w);  // sync 1: pair count
+    int r = read_counters(w);  // sync 1: pair count (also reports the last step's device errors)
     if (r != PXG_OK) return r;
     uint32_t P = w->h_cnt->pairs;
     if (P > w->cap_p) return fail(PXG_ERR_CAPACITY, "pair buffer full (" + std::to_string(P) + " pairs)");
@@ -474,28 +449,51 @@ static int phase_collide(pxg_world* w, float h) {
     return PXG_OK;
 }
 
-// colouring, solve (dynamics/solver.h:914-958), islands/sleep, integration (world.h:1301-1309)
-static int phase_solve(pxg_world* w, float h) {
+// Copies the last step's counters into the host-side statistics (the persistent path leaves
+// them on the device until a caller asks); reports device errors.
+static int finish_results(pxg_world* w) {
+    if (!w->results_pending) return PXG_OK;
+    w->results_pending = false;
+    int r = read_counters(w);
+    if (r != PXG_OK) return r;
+    const Counters& hc = *w->h_cnt;
+    std::memcpy(w->colour_off, hc.colour_off, sizeof(w->colour_off));
+    w->last_manifolds = hc.manifolds;
+    w->last_ordered = hc.ordered;
+    w->last_active = hc.active;
+    w->last_colours = hc.colours;
+    pxg_stats& sst = w->stats;
+    sst.manifolds = hc.manifolds;
+    sst.points = hc.points;
+    sst.colours = hc.colours;
+    sst.active_manifolds = hc.active;
+    sst.velocity_passes = w->cfg.residual_mode ? hc.passes : w->cfg.velocity_iterations;
+    float residual = 0.0f;
+    std::memcpy(&residual, &hc.residual_bits, 4);
+    sst.residual = residual;
+    return PXG_OK;
+}
+
+// A cooperative launch can be refused while another process holds the device's cooperative
+// resources; the step then continues on the eager path (same results, one launch per colour).
+static bool cooperative_refused(hipError_t e) {
+    return e == hipErrorIllegalState || e == hipErrorNotSupported || e == hipErrorCooperativeLaunchTooLarge;
+}
+static void drop_persistent(pxg_world* w, hipError_t e) {
+    (void)hipGetLastError();
+    w->coop_blocks = 0;
+    w->coop_note = std::string("persistent kernels unavailable (") + hipGetErrorString(e) + "), using per-colour launches";
+}
+
+// Eager colouring: rounds launched from the host until every active manifold has a colour; the
+// colour table is read back because it sizes the per-colour launches.
+static int colour_eager(pxg_world* w, uint32_t bound) {
     DevState& d = w->d;
     hipStream_t st = w->stream;
-    StepParams p = params(w, h);
-    const pxg_config& c = w->cfg;
-    uint32_t P = w->last_pairs;
-    uint32_t bound = P ? P : 1;
-    d.cur ^= 1;
-    launch_colour_init(d, bound, st);
-    uint32_t round = 1;
-    const bool persistent = w->coop_blocks > 0 && !w->eager;
-    if (persistent) {
-        uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kBlock - 1) / kBlock));
-        HIPCHK(launch_colour_persistent(d, round, 64, nb, d.bar, st));
-        round += 64;
-    } else {
-        launch_colour_rounds(d, bound, round, 24, st);
-        round += 24;
-    }
+    launch_colour_rounds(d, bound, 1, 24, st);
+    uint32_t round = 25;
     launch_build_order(d, bound, st);
-    int r = read_counters(w);  // sync 2: colour table
+    int r = read_counters(w);
     if (r != PXG_OK) return r;
     for (int extra = 0; w->h_cnt->remaining != 0 && extra < 64; ++extra) {
         launch_colour_rounds(d, bound, round, 16, st);
@@ -505,84 +503,126 @@ static int phase_solve(pxg_world* w, float h) {
         if (r != PXG_OK) return r;
     }
     if (w->h_cnt->remaining != 0) return fail(PXG_ERR_DEVICE, "colouring did not converge");
+    return PXG_OK;
+}
+
+// Eager solve over the colour table in w->h_cnt: one launch per colour per pass. `setup_done`:
+// the solver arrays (and learned predictions) were already written this step.
+static int solve_eager(pxg_world* w, const StepParams& p, uint32_t bound, bool setup_done) {
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    const pxg_config& c = w->cfg;
     const Counters& hc = *w->h_cnt;
-    uint32_t M = hc.manifolds, C = hc.colours, active = hc.active, ordered = hc.ordered;
+    uint32_t C = hc.colours, active = hc.active;
     std::memcpy(w->colour_off, hc.colour_off, sizeof(w->colour_off));
     const uint32_t* co = w->colour_off;
-    uint32_t widest = 0;
-    for (uint32_t k = 0; k < C; ++k) widest = std::max(widest, co[k + 1] - co[k]);
-
-    launch_solver_setup(d, p, ordered, st);
-    if (d.mlp && !d.record) launch_apply_predictions(d, 4 * active, st);
+    if (!setup_done) {
+        launch_solver_setup(d, p, bound, st);
+        if (d.mlp && !d.record) launch_apply_predictions(d, 4 * active, st);
+    }
     uint32_t passes = 0;
-    float residual = 0.0f;
+    for (uint32_t k = 0; k < C; ++k) launch_warm_apply(d, co[k], co[k + 1], st);
+    if (c.residual_mode) {
+        uint32_t maxit = c.residual_max_iterations ? c.residual_max_iterations : 256;
+        while (passes < maxit) {
+            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, passes);
+            ++passes;
+            launch_residual(d, p, active, 0, st);
+            HIPCHK(hipMemcpyAsync(&w->h_cnt->residual_bits, &d.cnt->residual_bits, 4, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            float residual = 0.0f;
+            std::memcpy(&residual, &w->h_cnt->residual_bits, 4);
+            if (residual <= c.residual_tolerance) break;
+        }
+        launch_set_passes(d, passes, st);
+    } else {
+        for (uint32_t it = 0; it < c.velocity_iterations; ++it)
+            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, it);
+    }
+    for (uint32_t it = 0; it < c.relax_iterations; ++it)
+        for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 1, it);
+    launch_position_setup(d, st);
+    for (uint32_t it = 0; it < c.position_iterations; ++it)
+        for (uint32_t k = 0; k < C; ++k) launch_position(d, p, co[k], co[k + 1], st);
+    return PXG_OK;
+}
+
+// colouring, solve (dynamics/solver.h:914-958), islands/sleep, integration (world.h:1301-1309).
+// The persistent path enqueues the whole phase without waiting on the device; the eager path
+// (one launch per colour) reads the colour table back to size its launches.
+static int phase_solve(pxg_world* w, float h) {
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    StepParams p = params(w, h);
+    const pxg_config& c = w->cfg;
+    uint32_t P = w->last_pairs;
+    uint32_t bound = P ? P : 1;
+    d.cur ^= 1;
+    launch_colour_init(d, bound, st);
+    w->stats.pairs = P;
+    w->stats.steps += 1;
+    bool persistent = w->coop_blocks > 0 && !w->eager;
     if (persistent) {
+        uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kPersistBlock - 1) / kPersistBlock));
+        nb = std::min(nb, w->colour_blocks);
+        hipError_t e = launch_colour_persistent(d, 1024, nb, st);
+        if (e != hipSuccess) {
+            if (!cooperative_refused(e)) return fail(PXG_ERR_HIP, std::string("colouring launch: ") + hipGetErrorString(e));
+            drop_persistent(w, e);
+            persistent = false;
+        } else {
+            launch_build_order(d, bound, st);
+        }
+    }
+    if (!persistent) {
+        int r = colour_eager(w, bound);
+        if (r != PXG_OK) return r;
+    }
+    bool solved = false;
+    if (persistent) {
+        launch_solver_setup(d, p, bound, st);
+        if (d.mlp && !d.record) launch_apply_predictions(d, 4 * bound, st);
         uint32_t cfgv[6] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
                             c.residual_max_iterations ? c.residual_max_iterations : 256u, 0u};
-        uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (widest + kBlock - 1) / kBlock));
+        // the widest colour is at most the manifold count; threads loop when a colour is wider
+        uint32_t snb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kPersistBlock - 1) / kPersistBlock));
+        snb = std::min(snb, w->solve_blocks);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (w->profiling) {
             e0 = take_event(w);
             e1 = take_event(w);
             (void)hipEventRecord(e0, st);
         }
-        HIPCHK(launch_solve_persistent(d, p, cfgv, c.residual_tolerance, nb, d.bar, st));
-        if (w->profiling) {
-            (void)hipEventRecord(e1, st);
-            w->ev_pending.push_back({e0, e1});
-        }
-        passes = c.residual_mode ? 0u : c.velocity_iterations;  // residual mode: read back later
-    } else {
-        for (uint32_t k = 0; k < C; ++k) launch_warm_apply(d, co[k], co[k + 1], st);
-        if (c.residual_mode) {
-            uint32_t maxit = c.residual_max_iterations ? c.residual_max_iterations : 256;
-            while (passes < maxit) {
-                for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, passes);
-                ++passes;
-                launch_residual(d, p, active, 0, st);
-                HIPCHK(hipMemcpyAsync(&w->h_cnt->residual_bits, &d.cnt->residual_bits, 4, hipMemcpyDeviceToHost, st));
-                HIPCHK(hipStreamSynchronize(st));
-                uint32_t bits = w->h_cnt->residual_bits;
-                std::memcpy(&residual, &bits, 4);
-                if (residual <= c.residual_tolerance) break;
+        // fixed iteration counts: the barrier-free dataflow solve; residual mode needs a grid-wide
+        // reduction after every pass and uses the barrier solve
+        hipError_t e = (!c.residual_mode && w->dataflow) ? launch_solve_dataflow(d, p, cfgv, snb, st)
+                                                          : launch_solve_persistent(d, p, cfgv, c.residual_tolerance, snb, st);
+        if (e == hipSuccess) {
+            solved = true;
+            if (w->profiling) {
+                (void)hipEventRecord(e1, st);
+                w->ev_pending.push_back({e0, e1});
             }
         } else {
-            for (uint32_t it = 0; it < c.velocity_iterations; ++it)
-                for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, it);
-            passes = c.velocity_iterations;
+            if (!cooperative_refused(e)) return fail(PXG_ERR_HIP, std::string("solve launch: ") + hipGetErrorString(e));
+            drop_persistent(w, e);
+            int r = read_counters(w);  // the colour table for the per-colour launches
+            if (r != PXG_OK) return r;
+            r = solve_eager(w, p, bound, true);
+            if (r != PXG_OK) return r;
+            solved = true;
         }
-        for (uint32_t it = 0; it < c.relax_iterations; ++it)
-            for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 1, it);
-        launch_position_setup(d, st);
-        for (uint32_t it = 0; it < c.position_iterations; ++it)
-            for (uint32_t k = 0; k < C; ++k) launch_position(d, p, co[k], co[k + 1], st);
     }
-    launch_solver_finish(d, ordered, st);
-    launch_islands_sleep(d, p, M, st);
+    if (!solved) {
+        int r = solve_eager(w, p, bound, false);
+        if (r != PXG_OK) return r;
+    }
+    launch_solver_finish(d, bound, st);
+    launch_islands_sleep(d, p, bound, st);
     launch_integrate_velocities(d, p, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(PXG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
-    if (persistent && c.residual_mode) {
-        // residual mode needs the pass count of this step for the statistics
-        HIPCHK(hipMemcpyAsync(w->h_cnt, d.cnt, sizeof(Counters), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
-        passes = w->h_cnt->passes;
-        std::memcpy(&residual, &w->h_cnt->residual_bits, 4);
-        if (w->h_cnt->err) return read_counters(w);
-    }
-    w->last_manifolds = M;
-    w->last_ordered = ordered;
-    w->last_active = active;
-    w->last_colours = C;
-    pxg_stats& sst = w->stats;
-    sst.pairs = P;
-    sst.manifolds = M;
-    sst.points = hc.points;
-    sst.colours = C;
-    sst.active_manifolds = active;
-    sst.velocity_passes = passes;
-    sst.residual = residual;
-    sst.steps += 1;
+    w->results_pending = true;
     return PXG_OK;
 }
 
@@ -631,11 +671,13 @@ extern "C" int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run) {
 extern "C" int pxg_synchronize(pxg_world* w) {
     if (!w) return fail(PXG_ERR_INVALID, "pxg_synchronize: null handle");
     HIPCHK(hipStreamSynchronize(w->stream));
-    return PXG_OK;
+    return finish_results(w);
 }
 
 extern "C" int pxg_get_stats(pxg_world* w, pxg_stats* out) {
     if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_get_stats: null argument");
+    int r = finish_results(w);
+    if (r != PXG_OK) return r;
     HIPCHK(hipMemcpyAsync(w->h_cnt, w->d.cnt, sizeof(Counters), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     w->stats.awake_bodies = w->h_cnt->awake;
@@ -658,6 +700,10 @@ extern "C" int pxg_get_pairs(pxg_world* w, uint32_t* ab, uint32_t cap, uint32_t*
 extern "C" int pxg_get_solve_order(pxg_world* w, uint32_t* order, uint32_t cap, uint32_t* n, uint32_t* colour_offsets,
                                    uint32_t colour_cap, uint32_t* n_colours) {
     if (!w || !n || !n_colours) return fail(PXG_ERR_INVALID, "pxg_get_solve_order: null argument");
+    {
+        int fr = finish_results(w);
+        if (fr != PXG_OK) return fr;
+    }
     *n = w->last_ordered;
     *n_colours = w->last_colours;
     uint32_t k = std::min(cap, w->last_ordered);
@@ -673,6 +719,10 @@ extern "C" int pxg_get_solve_order(pxg_world* w, uint32_t* order, uint32_t cap, 
 
 extern "C" int pxg_get_manifolds(pxg_world* w, pxg_manifold* out, uint32_t cap, uint32_t* n) {
     if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_manifolds: null argument");
+    {
+        int fr = finish_results(w);
+        if (fr != PXG_OK) return fr;
+    }
     uint32_t M = w->last_manifolds, O = w->last_ordered;
     *n = M;
     if (!out || !cap || !M) return PXG_OK;
@@ -795,7 +845,11 @@ extern "C" int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const floa
         while (k[idx] != ~0ull && k[idx] != keys[i]) idx = (idx + 1) & w->d.cache_mask;
         k[idx] = keys[i];
         o[idx] = 1;
-        v[idx] = make_float4(imp3[3 * i], imp3[3 * i + 1], imp3[3 * i + 2], 0.0f);
+        // w carries the manifold's colour of the step that wrote the entry: none for uploads
+        const uint32_t no_colour = 0xFFFFFFFFu;
+        float nc;
+        std::memcpy(&nc, &no_colour, 4);
+        v[idx] = make_float4(imp3[3 * i], imp3[3 * i + 1], imp3[3 * i + 2], nc);
     }
     int t = w->d.cur;
     HIPCHK(hipMemcpyAsync(w->d.ck_keys[t], k.data(), tc * 8, hipMemcpyHostToDevice, w->stream));
@@ -873,6 +927,10 @@ extern "C" int pxg_set_recording(pxg_world* w, uint32_t on) {
 
 extern "C" int pxg_get_records(pxg_world* w, float* features, float* impulses, uint32_t* missed, uint32_t cap, uint32_t* n) {
     if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_records: null argument");
+    {
+        int fr = finish_results(w);
+        if (fr != PXG_OK) return fr;
+    }
     *n = 0;
     if (!w->d.record || !w->d.feat) return fail(PXG_ERR_STATE, "pxg_get_records: recording is off");
     uint32_t A = w->last_active;
@@ -903,6 +961,10 @@ extern "C" int pxg_get_records(pxg_world* w, float* features, float* impulses, u
 
 extern "C" int pxg_get_predictions(pxg_world* w, uint64_t* keys, float* imp3, uint32_t cap, uint32_t* n) {
     if (!w || !n) return fail(PXG_ERR_INVALID, "pxg_get_predictions: null argument");
+    {
+        int fr = finish_results(w);
+        if (fr != PXG_OK) return fr;
+    }
     *n = 0;
     if (!w->d.mlp || w->d.record) return PXG_OK;
     DevState& d = w->d;

@@ -90,3 +90,12 @@ def test_no_gpu_create_fails_cleanly():
     rc = _lib.lib().pxg_create(None, ctypes.byref(h))
     assert rc != 0
     assert b"device" in _lib.lib().pxg_last_error()
+
+
+def test_cpp_facade_header_compiles(tmp_path):
+    """include/pxg_world.hpp (the PhysicsWorld facade) is self-contained C++17 over pxg.h."""
+    src = tmp_path / "facade.cpp"
+    src.write_text('#include "pxg_world.hpp"\nint main() { pxg::BodyId id; return id.is_valid() ? 1 : 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    assert r.returncode == 0, r.stdout.decode()

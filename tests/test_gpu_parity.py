@@ -77,6 +77,30 @@ def test_parity_eager_launch_path(sides, monkeypatch):
     assert rep["manifolds"] > 0
 
 
+def test_parity_barrier_solve_path(sides, monkeypatch):
+    """The solve with a grid barrier between colours (the residual-mode kernel, here in fixed
+    iteration mode) gives the same results as the default barrier-free dataflow solve."""
+    monkeypatch.setenv("PXG_SOLVE", "barrier")
+    _, _, rep = _run(sides, scenes.mixed_pile_scene(n=200, seed=12), 60, velocity_iterations=8, position_iterations=3)
+    assert rep["manifolds"] > 0
+
+
+def test_dataflow_and_barrier_solves_agree_bit_for_bit(monkeypatch):
+    """Same scene, same steps: the two persistent solves must produce identical bodies."""
+    from physxcoopa_amd.world import World
+    sc = scenes.config2_scene(seed=3, n=2048, lattice=(32, 32, 8))
+    cfg = _abi.default_config(**sc.config)
+    out = []
+    for mode in ("dataflow", "barrier"):
+        monkeypatch.setenv("PXG_SOLVE", mode)
+        w = World(cfg, sc.bodies, sc.shapes)
+        for _ in range(150):
+            w.step_fixed()
+        out.append(w.bodies())
+        w.close()
+    assert out[0].tobytes() == out[1].tobytes()
+
+
 def test_parity_residual_mode(sides):
     """Residual mode: velocity passes repeat until the residual <= 1e-4 (cap 64 here); the GPU
     must run the same number of passes as the oracle and reach the same state."""
