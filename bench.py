@@ -27,6 +27,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def calibrate_copy():
+    """A known-size copy kernel: 10 copies of 256 MiB (read 256 MiB, write 256 MiB each)."""
+    import torch
+    n = 256 << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    a.fill_(1)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(10):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": n, "copy_bytes_written_per_launch": n, "launches": 10}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -36,7 +50,15 @@ def main():
     ap.add_argument("--residual", action="store_true", help="velocity passes until residual <= 1e-4 (cap 256)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--order", choices=["reference", "coloured"], default="reference",
+                    help="solve order: the reference's manifold order (default) or the greedy colouring")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain device-to-device copy of 256 MiB ten times (the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the contact step has no multi-GPU path); "
@@ -52,6 +74,7 @@ def main():
     sc = scenes.config2_scene(seed=2, n=args.bodies, lattice=lattice or (64, 64, max(1, (args.bodies + 4095) // 4096 * 10)))
     log("scene %s built in %.1fs" % (sc.name, time.time() - t0))
     cfg = _abi.default_config(**sc.config)
+    cfg.solve_order = _abi.PXG_ORDER_REFERENCE if args.order == "reference" else _abi.PXG_ORDER_COLOURED
     if args.residual:
         cfg.residual_mode = 1
         cfg.residual_tolerance = 1e-4
@@ -136,7 +159,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": sc.name, "dynamic_bodies": n_dyn, "solver": "plain coloured GPU",
+        "config": {"workload": sc.name, "dynamic_bodies": n_dyn, "solver": "plain GPU, %s order" % args.order,
                    "velocity_iterations": int(cfg.velocity_iterations), "relax_iterations": int(cfg.relax_iterations),
                    "position_iterations": int(cfg.position_iterations), "residual_mode": bool(args.residual),
                    "residual_tolerance": 1e-4 if args.residual else None},

@@ -86,8 +86,16 @@ typedef struct pxg_config {
     uint32_t max_manifolds;
     /* Learned warm start: 1 = predict initial impulses for contacts with no cached impulse. */
     uint32_t learned_warm_start;      /* 0 */
-    uint32_t reserved[7];
+    /* Solve order. PXG_ORDER_REFERENCE: manifolds that share an awake dynamic body are solved in
+     * the reference's manifold order (pair order, broadphase/pair_cache.h:23-52), so the result
+     * is the reference's sequential Gauss-Seidel result; manifolds that share no such body run in
+     * parallel (colours = longest chains of that order). PXG_ORDER_COLOURED: greedy colouring,
+     * fewer colours (faster), a different but equally valid sequential order. */
+    uint32_t solve_order;             /* PXG_ORDER_REFERENCE */
+    uint32_t reserved[6];
 } pxg_config;
+
+enum pxg_solve_order { PXG_ORDER_REFERENCE = 0, PXG_ORDER_COLOURED = 1 };
 
 /* Full dynamic state of one body (physxcoopa/dynamics/body.h:85-216). */
 typedef struct pxg_body {

@@ -111,6 +111,9 @@ public:
     Signal<PhysicsWorld&, float> on_substep;
 
     explicit PhysicsWorld(const pxg_config& config = default_config()) : cfg_(config) {
+        // device buffers are sized once per handle: reserve room to add bodies later
+        if (cfg_.max_bodies == 0) cfg_.max_bodies = 256;
+        if (cfg_.max_shapes == 0) cfg_.max_shapes = cfg_.max_bodies;
         check(pxg_create(&cfg_, &w_), "pxg_create");
         applied_cfg_ = cfg_;
     }
@@ -284,6 +287,32 @@ public:
     pxg_world* handle() { return w_; }
 
 private:
+    // A new handle with room for at least n bodies; the warm-start cache moves over (the caller
+    // restores the fat AABBs).
+    void grow(uint32_t n) {
+        uint32_t cap = cfg_.max_bodies;
+        while (cap < n) cap *= 2;
+        uint32_t nk = 0;
+        check(pxg_get_warm_cache(w_, nullptr, nullptr, 0, &nk), "warm cache");
+        std::vector<uint64_t> keys(nk);
+        std::vector<float> imp(size_t(nk) * 3);
+        if (nk) check(pxg_get_warm_cache(w_, keys.data(), imp.data(), nk, &nk), "warm cache");
+        pxg_config c = cfg_;
+        c.max_bodies = c.max_shapes = cap;
+        pxg_world* nw = nullptr;
+        check(pxg_create(&c, &nw), "pxg_create");
+        pxg_destroy(w_);
+        w_ = nw;
+        cfg_ = c;
+        applied_cfg_ = c;
+        check(pxg_set_layer_masks(w_, layers_.rows()), "layers");
+        pending_cache_keys_ = std::move(keys);
+        pending_cache_imp_ = std::move(imp);
+    }
+
+public:
+
+private:
     bool alive_on_host(uint32_t i) const {
         // aliveness changes only on the host (add/remove), so the mirror's flag is current
         return bodies_[i].alive != 0;
@@ -312,6 +341,7 @@ private:
             // the broadphase keeps each surviving shape's fat AABB; new shapes get tight + margin
             std::vector<float> fat(size_t(uploaded_shapes_) * 6);
             if (uploaded_shapes_) check(pxg_get_fat_aabbs(w_, fat.data(), uploaded_shapes_), "fat AABBs");
+            if (n > cfg_.max_bodies || n > cfg_.max_shapes) grow(n);
             check(pxg_set_bodies(w_, bodies_.data(), n), "bodies");
             check(pxg_set_shapes(w_, shapes_.data(), n), "shapes");
             std::vector<float> now(size_t(n) * 6);
@@ -321,6 +351,12 @@ private:
             for (uint32_t i = 0; i < uploaded_shapes_ && i < n; ++i)
                 if (!fresh[i]) std::memcpy(&now[size_t(i) * 6], &fat[size_t(i) * 6], 6 * sizeof(float));
             check(pxg_set_fat_aabbs(w_, now.data(), n), "fat AABBs");
+            if (!pending_cache_keys_.empty()) {
+                check(pxg_set_warm_cache(w_, pending_cache_keys_.data(), pending_cache_imp_.data(),
+                                         static_cast<uint32_t>(pending_cache_keys_.size())), "warm cache");
+                pending_cache_keys_.clear();
+                pending_cache_imp_.clear();
+            }
             uploaded_shapes_ = n;
             new_shapes_.clear();
             structure_dirty_ = false;
@@ -345,6 +381,8 @@ private:
     std::vector<pxg_body> bodies_;
     std::vector<pxg_shape> shapes_;  // shape slot i belongs to body i
     std::vector<uint32_t> generations_, free_, new_shapes_;
+    std::vector<uint64_t> pending_cache_keys_;
+    std::vector<float> pending_cache_imp_;
     uint32_t uploaded_shapes_ = 0;
     bool structure_dirty_ = false, bodies_dirty_ = false, state_on_device_ = false, in_substep_ = false;
     float accumulator_ = 0.0f;

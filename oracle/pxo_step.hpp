@@ -369,10 +369,11 @@ template <class T> struct World {
         }
     }
 
-    void rebuild_cache(std::vector<Manifold<T>*>& ms) {  // solver.h:767-781
+    // solver.h:767-781. The reference inserts in its manifold order (pair order); with a solve
+    // order hint the insertion still follows pair order, so a key collision resolves the same way.
+    void rebuild_cache() {
         cache.clear();
-        for (Manifold<T>* mp : ms) {
-            Manifold<T>& m = *mp;
+        for (Manifold<T>& m : manifolds) {
             if (!m.valid || m.trigger) continue;
             for (int i = 0; i < m.count; ++i) {
                 Cached c;
@@ -481,7 +482,7 @@ template <class T> struct World {
         for (uint32_t i = 0; i < cfg.relax_iters; ++i) velocity_pass(ms, true, i, h);
         if (!cfg.residual_mode) last_residual = residual(ms, cfg.relax_iters > 0, h);
         solve_position(ms);
-        rebuild_cache(ms);
+        rebuild_cache();
         islands_and_sleep(h);
     }
 

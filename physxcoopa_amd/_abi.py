@@ -13,6 +13,7 @@ PXG_STATIC, PXG_KINEMATIC, PXG_DYNAMIC = 0, 1, 2
 PXG_SPHERE, PXG_BOX, PXG_CAPSULE = 0, 1, 2
 PXG_AVERAGE, PXG_MINIMUM, PXG_MAXIMUM, PXG_MULTIPLY = 0, 1, 2, 3
 PXG_PHASE_FORCES_COLLIDE, PXG_PHASE_SOLVE_INTEGRATE = 0, 1
+PXG_ORDER_REFERENCE, PXG_ORDER_COLOURED = 0, 1
 
 BODY_DTYPE = np.dtype([
     ("type", "<u4"), ("alive", "<u4"), ("awake", "<u4"), ("use_gravity", "<u4"),
@@ -77,7 +78,7 @@ class Config(ctypes.Structure):
         ("residual_max_iterations", ctypes.c_uint32), ("large_extent", ctypes.c_float),
         ("max_bodies", ctypes.c_uint32), ("max_shapes", ctypes.c_uint32), ("max_pairs", ctypes.c_uint32),
         ("max_manifolds", ctypes.c_uint32), ("learned_warm_start", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 7),
+        ("solve_order", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6),
     ]
 
 
@@ -121,6 +122,7 @@ def default_config(**overrides):
     c.residual_max_iterations = 256
     c.large_extent = 16.0
     c.learned_warm_start = 0
+    c.solve_order = PXG_ORDER_REFERENCE
     for k, v in overrides.items():
         if k == "gravity":
             for i in range(3):

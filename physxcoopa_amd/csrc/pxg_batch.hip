@@ -613,7 +613,8 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
             __syncthreads();
         }
     }
-    // 11. warm-start cache rebuild (solver.h:767-781): last writer in solve order wins
+    // 11. warm-start cache rebuild (solver.h:767-781): on a key collision the last writer in the
+    // reference's manifold order (pair order, which is the pair index here) wins
     for (uint32_t s = t; s < ordered; s += TB) {
         uint32_t gm = S.man_off + L.order[s];
         uint2 ab = bd.m_ab[gm];
@@ -621,7 +622,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
         for (uint32_t q = 0; q < cnt; ++q) {
             unsigned long long key = pack_key(ab.x, ab.y, __float_as_uint(bd.m_q[4 * gm + q].y)) |
                                      (static_cast<unsigned long long>(gen & 0x3FFFu) << 50);
-            unsigned long long ord = (static_cast<unsigned long long>(gen) << 32) | (4u * s + q + 1u);
+            unsigned long long ord = (static_cast<unsigned long long>(gen) << 32) | (4u * L.order[s] + q + 1u);
             uint32_t idx = hash64(key) & cmask;
             bool done = false;
             for (uint32_t probe = 0; probe <= cmask && !done; ++probe) {
@@ -647,7 +648,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
         for (uint32_t q = 0; q < cnt; ++q) {
             unsigned long long key = pack_key(ab.x, ab.y, __float_as_uint(bd.m_q[4 * gm + q].y)) |
                                      (static_cast<unsigned long long>(gen & 0x3FFFu) << 50);
-            unsigned long long ord = (static_cast<unsigned long long>(gen) << 32) | (4u * s + q + 1u);
+            unsigned long long ord = (static_cast<unsigned long long>(gen) << 32) | (4u * L.order[s] + q + 1u);
             uint32_t idx = hash64(key) & cmask;
             for (uint32_t probe = 0; probe <= cmask; ++probe) {
                 unsigned long long v = bd.c_key[S.cache_off + idx];

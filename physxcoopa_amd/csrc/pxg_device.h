@@ -347,14 +347,14 @@ PXD bool sat_obb_obb(const OBB& a, const OBB& b, Sat& res, bool allow_sep) {
         float score = overlap + bias;
         if (score < best_score) { best_score = score; best_overlap = overlap; best_axis = id; best_n = n; }
     }
-    if (separated) {
-        if (!allow_sep || best_sep_axis < 0) return false;
-        res.pen = 0.0f; res.sep = -best_sep_overlap; res.axis = best_sep_axis; res.n = best_sep_n; res.separated = true;
-        return true;
-    }
-    if (best_axis < 0) return false;
-    res.pen = best_overlap; res.sep = 0.0f; res.axis = best_axis; res.n = best_n; res.separated = false;
-    return true;
+    // one exit, every field written by selects on `separated`
+    const bool ok = separated ? (allow_sep && best_sep_axis >= 0) : (best_axis >= 0);
+    res.separated = separated;
+    res.pen = separated ? 0.0f : best_overlap;
+    res.sep = separated ? -best_sep_overlap : 0.0f;
+    res.axis = separated ? best_sep_axis : best_axis;
+    res.n = separated ? best_sep_n : best_n;
+    return ok;
 }
 
 // Sutherland-Hodgman clip against one plane (collision/clip.h:40-64); at most 9 vertices.

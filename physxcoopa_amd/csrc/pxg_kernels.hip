@@ -376,15 +376,20 @@ __device__ __forceinline__ bool ovl(v3 amn, v3 amx, v3 bmn, v3 bmx) {
 }
 
 // One emitter per pair: small-small pairs by the shape with the smaller (fat min x, index),
-// small-large pairs by the small shape, large-large pairs by the smaller index. mode 0 counts,
-// mode 1 writes the pairs and sorts the emitter's segment by (a, b).
+// small-large pairs by the small shape, large-large pairs by the smaller index. Each emitter's
+// candidate cells are rows of contiguous cells (one row per (z, y) cell pair of its fat box);
+// kPairLanes threads share an emitter, thread r taking rows r, r + kPairLanes, ... (thread 0 also
+// takes the large-shape list), so the scan is spread over many more lanes than emitters. Mode 0
+// counts per (emitter, lane); mode 1 writes at the scanned offsets; the emitter's segment is then
+// sorted by (a, b) (k_pairs_sort), so the pair list does not depend on the lane split.
 template <int MODE>
 __global__ void k_pairs(DevState d) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = gid / kPairLanes, r = gid % kPairLanes;
     if (i >= d.ns) return;
     float4 ti4 = d.s_tmin[i];
     if (ti4.w < 0.0f) {
-        if (MODE == 0) d.pair_count[i] = 0u;
+        if (MODE == 0) d.pair_count[gid] = 0u;
         return;
     }
     uint4 info = d.s_info[i];
@@ -394,7 +399,7 @@ __global__ void k_pairs(DevState d) {
     v3 tmn = ld3(ti4), tmx = ld3(d.s_tmax[i]);
     v3 fmn = ld3(d.s_fmin[i]), fmx = ld3(d.s_fmax[i]);
     uint32_t n = 0, off = 0, cap = d.cap_pairs;
-    if (MODE == 1) off = d.pair_off[i];
+    if (MODE == 1) off = d.pair_off[gid];
     const Counters* c = d.cnt;
 
     auto emit = [&](uint32_t j) {
@@ -410,55 +415,62 @@ __global__ void k_pairs(DevState d) {
         int x0 = cell_of(g, 0, fmn.x), x1 = cell_of(g, 0, fmx.x);
         int y0 = cell_of(g, 1, fmn.y - g.exty), y1 = cell_of(g, 1, fmx.y);
         int z0 = cell_of(g, 2, fmn.z - g.extz), z1 = cell_of(g, 2, fmx.z);
-        for (int cz = z0; cz <= z1; ++cz) {
-            for (int cy = y0; cy <= y1; ++cy) {
-                int row = (cz * g.n[1] + cy) * g.n[0];
-                // cells x0..x1 of one row are contiguous in the cell-ordered arrays
-                uint32_t s0 = d.cell_start[row + x0];
-                uint32_t s1 = d.cell_start[row + x1] + d.cell_count[row + x1];
-                for (uint32_t sl = s0; sl < s1; ++sl) {
-                    float4 jf = d.ci_fmin[sl];
-                    uint32_t j = __float_as_uint(jf.w);
-                    if (!(jf.x > fmn.x || (jf.x == fmn.x && j > i))) continue;
-                    if (jf.x > fmx.x) continue;
-                    float4 jF = d.ci_fmax[sl];
-                    if (__float_as_uint(jF.w) == own_i) continue;
-                    float4 jt = d.ci_tmin[sl];
-                    if (st_i && jt.w != 0.0f) continue;
-                    float4 jT = d.ci_tmax[sl];
-                    if (ovl(tmn, tmx, ld3(jf), ld3(jF)) || ovl(ld3(jt), ld3(jT), fmn, fmx)) emit(j);
-                }
+        const uint32_t ny = uint32_t(y1 - y0 + 1), rows = uint32_t(z1 - z0 + 1) * ny;
+        for (uint32_t q = r; q < rows; q += kPairLanes) {
+            int cz = z0 + int(q / ny), cy = y0 + int(q % ny);
+            int row = (cz * g.n[1] + cy) * g.n[0];
+            // cells x0..x1 of one row are contiguous in the cell-ordered arrays
+            uint32_t s0 = d.cell_start[row + x0];
+            uint32_t s1 = d.cell_start[row + x1] + d.cell_count[row + x1];
+            for (uint32_t sl = s0; sl < s1; ++sl) {
+                float4 jf = d.ci_fmin[sl];
+                uint32_t j = __float_as_uint(jf.w);
+                if (!(jf.x > fmn.x || (jf.x == fmn.x && j > i))) continue;
+                if (jf.x > fmx.x) continue;
+                float4 jF = d.ci_fmax[sl];
+                if (__float_as_uint(jF.w) == own_i) continue;
+                float4 jt = d.ci_tmin[sl];
+                if (st_i && jt.w != 0.0f) continue;
+                float4 jT = d.ci_tmax[sl];
+                if (ovl(tmn, tmx, ld3(jf), ld3(jF)) || ovl(ld3(jt), ld3(jT), fmn, fmx)) emit(j);
             }
         }
     }
-    uint32_t nl = c->n_large < uint32_t(kMaxLarge) ? c->n_large : uint32_t(kMaxLarge);
-    for (uint32_t k = 0; k < nl; ++k) {
-        uint32_t j = d.large_list[k];
-        if (j == i) continue;
-        if (large_i && j < i) continue;
-        uint4 ij = d.s_info[j];
-        if (ij.x == own_i) continue;
-        if (st_i && (ij.y & SF_STATIC)) continue;
-        if (ovl(tmn, tmx, ld3(d.s_fmin[j]), ld3(d.s_fmax[j])) || ovl(ld3(d.s_tmin[j]), ld3(d.s_tmax[j]), fmn, fmx)) emit(j);
-    }
-    if (MODE == 0) {
-        d.pair_count[i] = n;
-    } else if (n > 1 && off + n <= cap) {
-        // insertion sort of this emitter's segment by (a, b)
-        uint2* seg = d.pairs + off;
-        for (uint32_t a = 1; a < n; ++a) {
-            uint2 v = seg[a];
-            unsigned long long kv = (static_cast<unsigned long long>(v.x) << 32) | v.y;
-            uint32_t b = a;
-            while (b > 0) {
-                uint2 u = seg[b - 1];
-                unsigned long long ku = (static_cast<unsigned long long>(u.x) << 32) | u.y;
-                if (ku <= kv) break;
-                seg[b] = u;
-                --b;
-            }
-            seg[b] = v;
+    if (r == 0) {
+        uint32_t nl = c->n_large < uint32_t(kMaxLarge) ? c->n_large : uint32_t(kMaxLarge);
+        for (uint32_t k = 0; k < nl; ++k) {
+            uint32_t j = d.large_list[k];
+            if (j == i) continue;
+            if (large_i && j < i) continue;
+            uint4 ij = d.s_info[j];
+            if (ij.x == own_i) continue;
+            if (st_i && (ij.y & SF_STATIC)) continue;
+            if (ovl(tmn, tmx, ld3(d.s_fmin[j]), ld3(d.s_fmax[j])) || ovl(ld3(d.s_tmin[j]), ld3(d.s_tmax[j]), fmn, fmx)) emit(j);
         }
+    }
+    if (MODE == 0) d.pair_count[gid] = n;
+}
+// insertion sort of each emitter's segment by (a, b)
+__global__ void k_pairs_sort(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ns) return;
+    uint32_t b0 = d.pair_off[i * kPairLanes];
+    uint32_t b1 = (i + 1 < d.ns) ? d.pair_off[(i + 1) * kPairLanes] : d.cnt->pairs;
+    if (b1 > d.cap_pairs || b1 - b0 < 2u) return;
+    uint2* seg = d.pairs + b0;
+    uint32_t n = b1 - b0;
+    for (uint32_t a = 1; a < n; ++a) {
+        uint2 v = seg[a];
+        unsigned long long kv = (static_cast<unsigned long long>(v.x) << 32) | v.y;
+        uint32_t b = a;
+        while (b > 0) {
+            uint2 u = seg[b - 1];
+            unsigned long long ku = (static_cast<unsigned long long>(u.x) << 32) | u.y;
+            if (ku <= kv) break;
+            seg[b] = u;
+            --b;
+        }
+        seg[b] = v;
     }
 }
 
@@ -631,6 +643,7 @@ __global__ void k_colour_scatter(DevState d, uint32_t n_man, uint32_t nblk) {
     for (uint32_t t = 0; t < threadIdx.x; ++t) rank += (bk[t] == b) ? 1u : 0u;
     d.order[d.col_table_off[b * nblk + blockIdx.x] + rank] = m;
 }
+static_assert(kBuckets + 1 <= kBlock, "k_colour_offsets: one thread per bucket offset");
 __global__ void k_colour_offsets(DevState d, uint32_t nblk, const uint32_t* total) {
     int b = threadIdx.x;
     if (b <= kBuckets) {
@@ -1015,13 +1028,19 @@ __global__ void k_bodies_back(DevState d) {
     d.b_angv[i] = make_float4(w.x, w.y, w.z, ow.w);
     d.b_pos[i] = make_float4(x.x, x.y, x.z, op.w);
 }
+// position of point i of the manifold of shapes sh in the reference's insertion order (pair
+// order, then point index), plus one; 0 marks an empty slot
+__device__ __forceinline__ unsigned long long ref_order(uint2 sh, uint32_t i) {
+    return ((static_cast<unsigned long long>(sh.x) << 40) | (static_cast<unsigned long long>(sh.y) << 4) | i) + 1ull;
+}
 __global__ void k_cache_insert(DevState d, uint32_t bound) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= bound || s >= d.cnt->ordered) return;
     uint2 ab = d.sm_ab[s];
     uint32_t cnt = d.sm_cnt[s];
     unsigned long long* keys = d.ck_keys[d.cur];
-    uint32_t* ord = d.ck_ord[d.cur];
+    unsigned long long* ord = d.ck_ord[d.cur];
+    const uint2 sh = d.m_shapes[d.order[s]];
     uint32_t mask = d.cache_mask;
     for (uint32_t i = 0; i < cnt; ++i) {
         unsigned long long key = pack_key(ab.x, ab.y, __float_as_uint(d.sp_aux[4 * s + i].z));
@@ -1030,7 +1049,7 @@ __global__ void k_cache_insert(DevState d, uint32_t bound) {
         for (uint32_t probe = 0; probe <= mask; ++probe) {
             unsigned long long old = atomicCAS(&keys[idx], kEmptyKey, key);
             if (old == kEmptyKey || old == key) {
-                atomicMax(&ord[idx], 4u * s + i + 1u);
+                atomicMax(&ord[idx], ref_order(sh, i));
                 done = true;
                 break;
             }
@@ -1047,14 +1066,15 @@ __global__ void k_cache_write(DevState d, uint32_t bound) {
     uint2 ab = d.sm_ab[s];
     uint32_t cnt = d.sm_cnt[s];
     const unsigned long long* keys = d.ck_keys[d.cur];
-    const uint32_t* ord = d.ck_ord[d.cur];
+    const unsigned long long* ord = d.ck_ord[d.cur];
+    const uint2 sh = d.m_shapes[d.order[s]];
     uint32_t mask = d.cache_mask;
     for (uint32_t i = 0; i < cnt; ++i) {
         unsigned long long key = pack_key(ab.x, ab.y, __float_as_uint(d.sp_aux[4 * s + i].z));
         uint32_t idx = hash64(key) & mask;
         for (uint32_t probe = 0; probe <= mask; ++probe) {
             if (keys[idx] == key) {
-                if (ord[idx] == 4u * s + i + 1u) {
+                if (ord[idx] == ref_order(sh, i)) {
                     float4 v = d.sp_imp[4 * s + i];
                     d.ck_val[d.cur][idx] = make_float4(v.x, v.y, v.z, colour);
                 }
@@ -1374,81 +1394,128 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     }
 }
 
-// The solve without grid barriers: every body carries a version counter (zeroed per step) that
-// counts the solver updates applied to it. The updates of one body happen in solve order, pass
-// after pass, so the k-th manifold (in colour order) touching body a in pass p is update
-// p * deg(a) + k of a, where deg(a) = popcount(colours at a) and k = popcount(colours at a below
-// this manifold's colour) -- both read from the colouring's per-body colour mask. A manifold
-// waits until both of its awake dynamic bodies reach its update numbers, applies the update and
-// advances them. This is the same sequential order as the colour-by-colour solve (identical
-// results) while each manifold waits only for the manifolds it depends on.
-//
-// Progress: a thread takes its manifolds in (pass, colour) order and every wait is on a manifold
-// of an earlier (pass, colour); manifolds of one colour never share a written body, so lanes of a
-// wave never wait on each other. All blocks are co-resident (cooperative launch), so the earliest
-// unfinished manifold can always run. Every wait is bounded (ERR_BARRIER, then all waits end).
-__device__ __forceinline__ bool wait_version(const uint32_t* ver, uint32_t b, uint32_t want, uint32_t* err) {
-    uint32_t spins = 0;
-    while (__hip_atomic_load(&ver[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != want) {
-        if (++spins > kSpinLimit) {
-            atomicOr(err, ERR_BARRIER);
-            return false;
-        }
-        if ((spins & 255u) == 0u && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER))
-            return false;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return true;
+// ---- reference solve order ----
+// The reference solves manifolds one after the other in pair order (broadphase/pair_cache.h:23-52,
+// dynamics/solver.h:212-281). Two manifolds that write no common body commute, so any order that
+// keeps every awake dynamic body's manifolds in pair order gives the same result. Each such body's
+// manifolds form a chain in pair order; a manifold's colour is one more than the larger colour
+// of its two predecessors (its chain layer), so colour-major order keeps every chain in order.
+__device__ __forceinline__ unsigned long long pair_key(const DevState& d, uint32_t m) {
+    uint2 sh = d.m_shapes[m];
+    return (static_cast<unsigned long long>(sh.x) << 32) | sh.y;
 }
-
-__global__ void __launch_bounds__(kPersistBlock) k_solve_dataflow(DevState d, StepParams prm, SolveCfg cfg) {
-    uint32_t* err = &d.cnt->err;
-    uint32_t* ver = d.ver;
-    const uint32_t C = d.cnt->colours;
-    const uint32_t* co = d.cnt->colour_off;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
-    if (prm.profile) {
-        // algorithmic bytes: warm start and each velocity/relax pass 192 + 40n per manifold, each
-        // position pass 92 + 4n
-        uint32_t bytes = 0;
-        const uint32_t vp = 1u + cfg.vel_iters + cfg.relax_iters;
-        for (uint32_t s = tid; s < n_active_all(d); s += nthreads) {
-            uint32_t n = d.sm_cnt[s];
-            bytes += vp * (192u + 40u * n) + cfg.pos_iters * (92u + 4u * n);
+__device__ __forceinline__ void ref_count(const DevState& d, uint32_t m) {
+    d.m_pred[2 * m] = kNoPred;
+    d.m_pred[2 * m + 1] = kNoPred;
+    if (d.m_colour[m] != kUncoloured) return;
+    uint2 ab = d.m_ab[m];
+    if (body_active(d.b_flags[ab.x])) atomicAdd(&d.adj_cnt[ab.x], 1u);
+    if (body_active(d.b_flags[ab.y])) atomicAdd(&d.adj_cnt[ab.y], 1u);
+}
+__device__ __forceinline__ void ref_scatter(const DevState& d, uint32_t m) {
+    if (d.m_colour[m] != kUncoloured) return;
+    uint2 ab = d.m_ab[m];
+    if (body_active(d.b_flags[ab.x])) d.adj[d.adj_start[ab.x] + atomicAdd(&d.adj_cur[ab.x], 1u)] = m;
+    if (body_active(d.b_flags[ab.y])) d.adj[d.adj_start[ab.y] + atomicAdd(&d.adj_cur[ab.y], 1u)] = m;
+}
+// sorts body x's chain into pair order and links each manifold to its predecessor on x
+__device__ __forceinline__ void ref_chain(const DevState& d, uint32_t x) {
+    const uint32_t n = d.adj_cnt[x];
+    if (n == 0u) return;
+    uint32_t* seg = d.adj + d.adj_start[x];
+    for (uint32_t a = 1; a < n; ++a) {
+        uint32_t v = seg[a];
+        unsigned long long kv = pair_key(d, v);
+        uint32_t b = a;
+        while (b > 0 && pair_key(d, seg[b - 1]) > kv) {
+            seg[b] = seg[b - 1];
+            --b;
         }
-        bytes = wave_sum(bytes);
-        if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+        seg[b] = v;
     }
-    const uint32_t passes = 1u + cfg.vel_iters + cfg.relax_iters + cfg.pos_iters;
-    for (uint32_t p = 0; p < passes; ++p) {
-        for (uint32_t c = 0; c < C; ++c) {
-            const unsigned long long below = (1ull << c) - 1ull;
-            for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) {
-                uint2 ab = d.sm_ab[s];
-                bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
-                uint32_t ea = 0, eb = 0;
-                if (aa) {
-                    unsigned long long m = d.col_mask[ab.x];
-                    ea = p * uint32_t(__popcll(m)) + uint32_t(__popcll(m & below));
-                    if (!wait_version(ver, ab.x, ea, err)) return;
-                }
-                if (ba) {
-                    unsigned long long m = d.col_mask[ab.y];
-                    eb = p * uint32_t(__popcll(m)) + uint32_t(__popcll(m & below));
-                    if (!wait_version(ver, ab.y, eb, err)) return;
-                }
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                if (p == 0) warm_manifold(d, s);
-                else if (p <= cfg.vel_iters) velocity_manifold(d, prm, s, 0, p - 1u);
-                else if (p <= cfg.vel_iters + cfg.relax_iters) velocity_manifold(d, prm, s, 1, p - 1u - cfg.vel_iters);
-                else position_manifold(d, prm, s);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-                if (aa) __hip_atomic_store(&ver[ab.x], ea + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (ba) __hip_atomic_store(&ver[ab.y], eb + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t m = seg[k];
+        uint32_t side = d.m_ab[m].x == x ? 0u : 1u;
+        d.m_pred[2 * m + side] = k ? seg[k - 1] : kNoPred;
+    }
+}
+// one layering round for manifold m: coloured once both predecessors are; returns the colour + 1
+// (0: not coloured this round)
+__device__ __forceinline__ uint32_t ref_layer(const DevState& d, uint32_t m, uint32_t* err) {
+    if (d.m_colour[m] != kUncoloured) return 0u;
+    const uint32_t pa = d.m_pred[2 * m], pb = d.m_pred[2 * m + 1];
+    uint32_t c = 0u;
+    if (pa != kNoPred) {
+        uint32_t ca = __hip_atomic_load(&d.m_colour[pa], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (ca >= uint32_t(kMaxColours)) return 0u;
+        c = ca + 1u;
+    }
+    if (pb != kNoPred) {
+        uint32_t cb = __hip_atomic_load(&d.m_colour[pb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cb >= uint32_t(kMaxColours)) return 0u;
+        c = c > cb + 1u ? c : cb + 1u;
+    }
+    if (c >= uint32_t(kMaxColours)) {
+        atomicOr(err, ERR_COLOURS);
+        return 0u;
+    }
+    __hip_atomic_store(&d.m_colour[m], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return c + 1u;
+}
+// wave-aggregated bookkeeping of a colouring step: `top` = colour + 1 of a manifold coloured by
+// this lane (0: none)
+__device__ __forceinline__ void colour_taken(const DevState& d, uint32_t top) {
+    uint32_t took = wave_sum(top ? 1u : 0u);
+    top = wave_max(top);
+    if (lane0() && took) {
+        atomicSub(&d.cnt->remaining, took);
+        atomicMax(&d.cnt->colours, top);
+    }
+}
+__global__ void k_ref_count(DevState d, uint32_t bound) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < bound && m < d.cnt->manifolds) ref_count(d, m);
+}
+__global__ void k_ref_scatter(DevState d, uint32_t bound) {
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < bound && m < d.cnt->manifolds) ref_scatter(d, m);
+}
+__global__ void k_ref_chain(DevState d) {
+    uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < d.nb) ref_chain(d, x);
+}
+__global__ void k_ref_layer(DevState d, uint32_t bound) {
+    if (d.cnt->remaining == 0u) return;
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t top = (m < bound && m < d.cnt->manifolds) ? ref_layer(d, m, &d.cnt->err) : 0u;
+    colour_taken(d, top);
+}
+// exclusive scan of n counts by one block (any block size up to 1024)
+__device__ void block_scan_range(const uint32_t* in, uint32_t* out, uint32_t n) {
+    __shared__ uint32_t wtot[16];
+    __shared__ uint32_t carry;
+    const uint32_t t = threadIdx.x, nt = blockDim.x, lane = t & 63u, wid = t >> 6, nw = (nt + 63u) >> 6;
+    if (t == 0) carry = 0u;
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += nt) {
+        uint32_t i = base + t;
+        uint32_t v = i < n ? in[i] : 0u, x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= uint32_t(o)) x += y;
         }
+        if (lane == 63u) wtot[wid] = x;
+        __syncthreads();
+        uint32_t off = carry, tot = 0u;
+        for (uint32_t w = 0; w < nw; ++w) {
+            if (w < wid) off += wtot[w];
+            tot += wtot[w];
+        }
+        if (i < n) out[i] = off + x - v;
+        __syncthreads();
+        if (t == 0) carry += tot;
+        __syncthreads();
     }
-    if (tid == 0) d.cnt->passes = cfg.vel_iters;
 }
 
 // The whole colouring in one cooperative launch. Contacts persist from step to step, so each
@@ -1456,11 +1523,35 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_dataflow(DevState d, St
 // warm-start entry). A hinted colour is kept unless another hinted manifold claims the same colour
 // at the same awake dynamic body, in which case every claimant drops it; the outcome does not
 // depend on timing. Jones-Plassmann rounds (first-fit over the per-body masks) colour the rest.
-__global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d, uint32_t max_rounds, uint32_t* bar) {
+__global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d, uint32_t max_rounds, uint32_t reference,
+                                                                   uint32_t* bar) {
     GridBarrier g = grid_barrier_init(bar);
     uint32_t* err = &d.cnt->err;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     const uint32_t n_man = d.cnt->manifolds;
+    if (reference) {
+        for (uint32_t m = tid; m < n_man; m += nthreads) ref_count(d, m);
+        if (grid_sync(g, err)) return;
+        if (blockIdx.x == 0) block_scan_range(d.adj_cnt, d.adj_start, d.nb);
+        if (grid_sync(g, err)) return;
+        for (uint32_t m = tid; m < n_man; m += nthreads) ref_scatter(d, m);
+        if (grid_sync(g, err)) return;
+        for (uint32_t x = tid; x < d.nb; x += nthreads) ref_chain(d, x);
+        if (grid_sync(g, err)) return;
+        for (uint32_t round = 0; round < max_rounds; ++round) {
+            if (__hip_atomic_load(&d.cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) break;
+            const uint32_t base = tid - (tid & 63u);
+            for (uint32_t m0 = base; m0 < n_man; m0 += nthreads) {
+                const uint32_t m = m0 + (tid & 63u);
+                colour_taken(d, m < n_man ? ref_layer(d, m, err) : 0u);
+            }
+            if (grid_sync(g, err)) return;
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_COLOURS) break;
+        }
+        if (tid == 0 && __hip_atomic_load(&d.cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+            atomicOr(err, ERR_COLOURS);
+        return;
+    }
     const int prev = d.cur ^ 1;
     constexpr uint32_t kHinted = 0x80000000u;
     // A: claim last step's colour
@@ -1470,7 +1561,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
         float4 cv;
         if (!cache_lookup(d, prev, pack_key(ab.x, ab.y, __float_as_uint(d.m_q[4 * m].y)), cv)) continue;
         uint32_t c = __float_as_uint(cv.w);
-        if (c >= uint32_t(kMaxColours)) continue;
+        if (c >= uint32_t(kMaskColours)) continue;
         unsigned long long bit = 1ull << c;
         d.m_colour[m] = kHinted | c;
         if (body_active(d.b_flags[ab.x]) && (atomicOr(&d.col_mask[ab.x], bit) & bit)) atomicOr(&d.col_dup[ab.x], bit);
@@ -1484,7 +1575,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
             uint32_t m = m0 + (tid & 63u);
             uint32_t took = 0, top = 0;
             uint32_t col = m < n_man ? d.m_colour[m] : 0u;
-            if (m < n_man && col != kUncoloured && (col & kHinted) && col < (kHinted | uint32_t(kMaxColours))) {
+            if (m < n_man && col != kUncoloured && (col & kHinted) && col < (kHinted | uint32_t(kMaskColours))) {
                 uint32_t c = col & ~kHinted;
                 unsigned long long bit = 1ull << c;
                 uint2 ab = d.m_ab[m];
@@ -1601,12 +1692,16 @@ void launch_broadphase_count(const DevState& d, const StepParams& p, hipStream_t
     hipLaunchKernelGGL(k_grid_count, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
     launch_scan(d.cell_count, d.cell_start, kMaxCells, nullptr, d.scan_tmp, st);
     hipLaunchKernelGGL(k_grid_scatter, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
-    hipLaunchKernelGGL(k_pairs<0>, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
-    launch_scan(d.pair_count, d.pair_off, d.ns, &d.cnt->pairs, d.scan_tmp, st);
+    const uint32_t lanes = d.ns * uint32_t(kPairLanes);
+    hipLaunchKernelGGL(k_pairs<0>, dim3(blocks(lanes)), dim3(kBlock), 0, st, d);
+    launch_scan(d.pair_count, d.pair_off, lanes, &d.cnt->pairs, d.scan_tmp, st);
 }
 void launch_broadphase_write(const DevState& d, const StepParams& p, hipStream_t st) {
     (void)p;
-    if (d.ns) hipLaunchKernelGGL(k_pairs<1>, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
+    if (d.ns) {
+        hipLaunchKernelGGL(k_pairs<1>, dim3(blocks(d.ns * uint32_t(kPairLanes))), dim3(kBlock), 0, st, d);
+        hipLaunchKernelGGL(k_pairs_sort, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d);
+    }
 }
 void launch_narrowphase(const DevState& d, const StepParams& p, uint32_t n_pairs, hipStream_t st) {
     if (!n_pairs) return;
@@ -1618,7 +1713,21 @@ void launch_colour_init(const DevState& d, uint32_t n_man, hipStream_t st) {
     launch_fill_u64(d.col_best, 0ull, d.nb, st);
     launch_fill_u64(d.col_mask, 0ull, d.nb, st);
     launch_fill_u64(d.col_dup, 0ull, d.nb, st);
+    launch_fill_u32(d.adj_cnt, 0u, d.nb, st);
+    launch_fill_u32(d.adj_cur, 0u, d.nb, st);
     if (n_man) hipLaunchKernelGGL(k_colour_init, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
+}
+void launch_reference_chains(const DevState& d, uint32_t n_man, hipStream_t st) {
+    if (!n_man || !d.nb) return;
+    hipLaunchKernelGGL(k_ref_count, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
+    launch_scan(d.adj_cnt, d.adj_start, d.nb, nullptr, d.scan_tmp, st);
+    hipLaunchKernelGGL(k_ref_scatter, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
+    hipLaunchKernelGGL(k_ref_chain, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+}
+void launch_reference_layers(const DevState& d, uint32_t n_man, uint32_t rounds, hipStream_t st) {
+    if (!n_man) return;
+    for (uint32_t r = 0; r < rounds; ++r)
+        hipLaunchKernelGGL(k_ref_layer, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man);
 }
 void launch_colour_rounds(const DevState& d, uint32_t n_man, uint32_t first_round, uint32_t rounds, hipStream_t st) {
     if (!n_man) return;
@@ -1637,7 +1746,7 @@ void launch_build_order(const DevState& d, uint32_t n_man, hipStream_t st) {
     uint32_t* total = scratch + 1024;
     launch_scan(d.col_table, d.col_table_off, n, total, scratch, st);
     if (n_man) hipLaunchKernelGGL(k_colour_scatter, dim3(nblk), dim3(kBlock), 0, st, d, n_man, nblk);
-    hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(128), 0, st, d, nblk, total);
+    hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(kBlock), 0, st, d, nblk, total);
 }
 void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st) {
     if (bound) hipLaunchKernelGGL(k_solver_setup, dim3(blocks(bound)), dim3(kBlock), 0, st, d, p, bound);
@@ -1673,7 +1782,7 @@ void launch_solver_finish(const DevState& d, uint32_t bound, hipStream_t st) {
     if (d.nb) hipLaunchKernelGGL(k_bodies_back, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     size_t cap = size_t(d.cache_mask) + 1;
     launch_fill_u64(d.ck_keys[d.cur], kEmptyKey, cap, st);
-    launch_fill_u32(d.ck_ord[d.cur], 0u, cap, st);
+    launch_fill_u64(d.ck_ord[d.cur], 0ull, cap, st);
     if (bound) {
         hipLaunchKernelGGL(k_cache_insert, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
         hipLaunchKernelGGL(k_cache_write, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
@@ -1704,24 +1813,12 @@ hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks),
                                       dim3(kPersistBlock), args, 0, st);
 }
-hipError_t launch_solve_dataflow(const DevState& d, const StepParams& p, const uint32_t cfgv[6], uint32_t nblocks,
-                                 hipStream_t st) {
-    SolveCfg c;
-    c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
-    c.residual_mode = 0; c.residual_max = 0; c.residual_tol = 0.0f;
-    launch_fill_u32(d.ver, 0u, d.nb ? d.nb : 1, st);
-    if (d.nb) hipLaunchKernelGGL(k_position_setup, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
-    DevState dd = d;
-    StepParams pp = p;
-    void* args[] = {&dd, &pp, &c};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_dataflow), dim3(nblocks),
-                                      dim3(kPersistBlock), args, 0, st);
-}
-hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, hipStream_t st) {
+hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, uint32_t reference,
+                                    hipStream_t st) {
     launch_fill_u32(d.bar, 0u, 1, st);
     DevState dd = d;
     uint32_t* bar = d.bar;
-    void* args[] = {&dd, &max_rounds, &bar};
+    void* args[] = {&dd, &max_rounds, &reference, &bar};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_colour_persistent), dim3(nblocks),
                                       dim3(kPersistBlock), args, 0, st);
 }
@@ -1733,12 +1830,8 @@ int max_coresident_blocks(int device) {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, reinterpret_cast<const void*>(&k_colour_persistent),
                                                      kPersistBlock, 0) != hipSuccess)
         return 0;
-    int per_cu3 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu3, reinterpret_cast<const void*>(&k_solve_dataflow),
-                                                     kPersistBlock, 0) != hipSuccess)
-        return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
-    return std::min(std::min(per_cu, per_cu2), per_cu3) * cus;
+    return std::min(per_cu, per_cu2) * cus;
 }
 
 void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st) {

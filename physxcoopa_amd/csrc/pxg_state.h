@@ -12,10 +12,13 @@ namespace pxg {
 struct MlpDev;
 constexpr int kBlock = 256;
 constexpr int kPersistBlock = 512;  // persistent cooperative kernels: 8 waves per block (up to 256 VGPRs each)
-constexpr int kMaxColours = 64;      // colours per step (64-bit per-body colour masks)
+constexpr int kMaxColours = 254;     // colours per step (reference order: chain layers)
+constexpr int kMaskColours = 64;     // greedy colouring: colours tracked in 64-bit per-body masks
 constexpr int kBuckets = kMaxColours + 1;  // colours + the "skipped by the solver" group
 constexpr int kMaxLarge = 256;       // shapes on the brute-force broadphase path
 constexpr uint32_t kMaxCells = 1u << 18;
+constexpr int kPairLanes = 9;        // broadphase threads per emitter shape
+constexpr uint32_t kMaxShapes = (1u << 20) / kPairLanes;  // the pair-count scan covers 2^20 entries
 
 // body flag bits
 constexpr uint32_t BF_TYPE = 3u, BF_ALIVE = 1u << 2, BF_AWAKE = 1u << 3, BF_GRAVITY = 1u << 4;
@@ -76,7 +79,7 @@ struct DevState {
     float4 *ci_fmin, *ci_fmax, *ci_tmin, *ci_tmax;  // cell-ordered copies: fat/tight boxes, w = index /
                                                     // owner / static flag bits
     uint32_t *large_list;
-    uint32_t *pair_count, *pair_off;   // per shape (emitter)
+    uint32_t *pair_count, *pair_off;   // per (emitter shape, lane): kPairLanes entries per shape
     uint2 *pairs;                      // (shape a, shape b), a < b
     // pair-indexed narrowphase output
     uint32_t *pm_valid, *pm_off;
@@ -97,6 +100,10 @@ struct DevState {
     unsigned long long *col_best;   // per body
     unsigned long long *col_mask;   // per body
     unsigned long long *col_dup;    // per body: colours claimed twice by last step's colours
+    // reference solve order: each awake dynamic body's manifolds in pair order
+    uint32_t *adj_cnt, *adj_start, *adj_cur;  // per body
+    uint32_t *adj;                  // 2 per manifold: the bodies' chains, concatenated
+    uint32_t *m_pred;               // 2 per manifold: predecessor on body a / body b (kNoPred: none)
     uint32_t *col_table, *col_table_off;
     uint32_t *order;                // solve position -> manifold index
     // solver arrays, solve order
@@ -114,14 +121,13 @@ struct DevState {
     float4 *sb_ref;     // position pass reference positions
     // warm-start cache: two open-addressing tables, swapped every step
     unsigned long long *ck_keys[2];
-    uint32_t *ck_ord[2];
+    unsigned long long *ck_ord[2];  // last writer in the reference's manifold order
     float4 *ck_val[2];
     int cur;            // table written this step (the other is read)
     // islands
     uint32_t *uf_parent;
     uint32_t *isl_min;
     uint32_t *bar;      // grid barrier of the persistent kernels: arrival count (zeroed per launch)
-    uint32_t *ver;      // per body: solver updates applied this step (dataflow solve)
     Counters *cnt;
     // learned warm start
     const struct MlpDev *mlp;  // BF16 network (nullptr = off)
@@ -130,6 +136,8 @@ struct DevState {
     float *pred;        // learned: clamped predictions, 3 per compact row
     uint32_t record;    // 1: write every active point's features (dataset recording)
 };
+
+constexpr uint32_t kNoPred = 0xFFFFFFFFu;
 
 struct StepParams {
     float h;
@@ -171,12 +179,13 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
 // cfgv = vel, relax, pos iterations, residual mode, residual max passes
 hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[6], float tol,
                                    uint32_t nblocks, hipStream_t st);
-// the same solve in fixed-iteration mode with per-body version counters instead of grid barriers
-hipError_t launch_solve_dataflow(const DevState& d, const StepParams& p, const uint32_t cfgv[6], uint32_t nblocks,
-                                 hipStream_t st);
 // colours every active manifold: last step's colours where still valid, then Jones-Plassmann
 // rounds (at most max_rounds) for the rest
-hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, hipStream_t st);
+hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, uint32_t reference,
+                                    hipStream_t st);
+// the reference-order colouring with one launch per stage (rounds launched by the host)
+void launch_reference_chains(const DevState& d, uint32_t n_manifolds, hipStream_t st);
+void launch_reference_layers(const DevState& d, uint32_t n_manifolds, uint32_t rounds, hipStream_t st);
 int max_coresident_blocks(int device);
 constexpr uint32_t ERR_BARRIER_BIT = 64u;
 

@@ -61,7 +61,6 @@ struct pxg_world {
     MlpDev* d_mlp = nullptr;
     bool eager = false;         // PXG_EAGER=1: one launch per colour instead of persistent kernels
     std::string coop_note;      // why the persistent kernels were dropped, if they were
-    bool dataflow = true;       // fixed-iteration solve without grid barriers (PXG_SOLVE=barrier: off)
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -183,7 +182,6 @@ extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
     // grid caps of the persistent kernels: fewer blocks make each grid barrier cheaper
     if (const char* v = std::getenv("PXG_SOLVE_BLOCKS")) w->solve_blocks = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("PXG_COLOUR_BLOCKS")) w->colour_blocks = std::max(1, std::atoi(v));
-    if (const char* v = std::getenv("PXG_SOLVE")) w->dataflow = std::strcmp(v, "barrier") != 0;
     *out = w;
     return PXG_OK;
 }
@@ -236,8 +234,9 @@ static int alloc_bodies(pxg_world* w, uint32_t n) {
     DALLOC(d.b_force, cap); DALLOC(d.b_torque, cap); DALLOC(d.b_iil, cap); DALLOC(d.b_iiw, 3 * size_t(cap));
     DALLOC(d.b_prev_pos, cap); DALLOC(d.b_prev_rot, cap); DALLOC(d.b_flags, cap);
     DALLOC(d.col_best, cap); DALLOC(d.col_mask, cap); DALLOC(d.col_dup, cap);
+    DALLOC(d.adj_cnt, cap); DALLOC(d.adj_start, cap); DALLOC(d.adj_cur, cap);
     DALLOC(d.sb_v, cap); DALLOC(d.sb_w, cap); DALLOC(d.sb_I, 3 * size_t(cap)); DALLOC(d.sb_x, cap); DALLOC(d.sb_ref, cap);
-    DALLOC(d.uf_parent, cap); DALLOC(d.isl_min, cap); DALLOC(d.ver, cap);
+    DALLOC(d.uf_parent, cap); DALLOC(d.isl_min, cap);
     DALLOC(d.cnt, 1);
     DALLOC(d.layer_mask, 32);
     DALLOC(d.scan_tmp, 4096);
@@ -259,6 +258,7 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
         return PXG_OK;
     }
     uint32_t cap = std::max(n, w->cfg.max_shapes);
+    if (cap > kMaxShapes) return fail(PXG_ERR_CAPACITY, "pxg_set_shapes: at most " + std::to_string(kMaxShapes) + " shapes per world");
     w->cap_s = cap;
     w->cap_p = w->cfg.max_pairs ? w->cfg.max_pairs : 24u * cap + 4096u;
     w->cap_m = w->cfg.max_manifolds ? w->cfg.max_manifolds : std::min(w->cap_p, 12u * cap + 4096u);
@@ -272,12 +272,13 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
     DALLOC(d.cell_count, kMaxCells); DALLOC(d.cell_start, kMaxCells); DALLOC(d.cell_cursor, kMaxCells);
     DALLOC(d.ci_fmin, cap); DALLOC(d.ci_fmax, cap); DALLOC(d.ci_tmin, cap); DALLOC(d.ci_tmax, cap);
     DALLOC(d.large_list, kMaxLarge);
-    DALLOC(d.pair_count, cap); DALLOC(d.pair_off, cap);
+    DALLOC(d.pair_count, size_t(kPairLanes) * cap); DALLOC(d.pair_off, size_t(kPairLanes) * cap);
     size_t P = w->cap_p, M = w->cap_m;
     DALLOC(d.pairs, P); DALLOC(d.pm_valid, P); DALLOC(d.pm_off, P);
     DALLOC(d.pm_ab, P); DALLOC(d.pm_nf, P); DALLOC(d.pm_meta, P); DALLOC(d.pm_p, 4 * P); DALLOC(d.pm_q, 4 * P);
     DALLOC(d.m_ab, M); DALLOC(d.m_shapes, M); DALLOC(d.m_nf, M); DALLOC(d.m_meta, M); DALLOC(d.m_p, 4 * M);
     DALLOC(d.m_q, 4 * M); DALLOC(d.m_colour, M);
+    DALLOC(d.adj, 2 * M); DALLOC(d.m_pred, 2 * M);
     size_t nblk = (P + kBlock - 1) / kBlock;
     DALLOC(d.col_table, size_t(kBuckets) * nblk + 2048); DALLOC(d.col_table_off, size_t(kBuckets) * nblk + 16);
     DALLOC(d.order, M);
@@ -288,7 +289,7 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
     for (int t = 0; t < 2; ++t) {
         DALLOC(d.ck_keys[t], cache_cap); DALLOC(d.ck_ord[t], cache_cap); DALLOC(d.ck_val[t], cache_cap);
         launch_fill_u64(d.ck_keys[t], ~0ull, cache_cap, w->stream);
-        launch_fill_u32(d.ck_ord[t], 0u, cache_cap, w->stream);
+        launch_fill_u64(d.ck_ord[t], 0ull, cache_cap, w->stream);
     }
     d.cur = 0;
     return PXG_OK;
@@ -490,13 +491,20 @@ static void drop_persistent(pxg_world* w, hipError_t e) {
 static int colour_eager(pxg_world* w, uint32_t bound) {
     DevState& d = w->d;
     hipStream_t st = w->stream;
-    launch_colour_rounds(d, bound, 1, 24, st);
+    const bool reference = w->cfg.solve_order == PXG_ORDER_REFERENCE;
+    if (reference) {
+        launch_reference_chains(d, bound, st);
+        launch_reference_layers(d, bound, 24, st);
+    } else {
+        launch_colour_rounds(d, bound, 1, 24, st);
+    }
     uint32_t round = 25;
     launch_build_order(d, bound, st);
     int r = read_counters(w);
     if (r != PXG_OK) return r;
     for (int extra = 0; w->h_cnt->remaining != 0 && extra < 64; ++extra) {
-        launch_colour_rounds(d, bound, round, 16, st);
+        if (reference) launch_reference_layers(d, bound, 16, st);
+        else launch_colour_rounds(d, bound, round, 16, st);
         round += 16;
         launch_build_order(d, bound, st);
         r = read_counters(w);
@@ -565,7 +573,7 @@ static int phase_solve(pxg_world* w, float h) {
     if (persistent) {
         uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kPersistBlock - 1) / kPersistBlock));
         nb = std::min(nb, w->colour_blocks);
-        hipError_t e = launch_colour_persistent(d, 1024, nb, st);
+        hipError_t e = launch_colour_persistent(d, 1024, nb, c.solve_order == PXG_ORDER_REFERENCE ? 1u : 0u, st);
         if (e != hipSuccess) {
             if (!cooperative_refused(e)) return fail(PXG_ERR_HIP, std::string("colouring launch: ") + hipGetErrorString(e));
             drop_persistent(w, e);
@@ -593,10 +601,7 @@ static int phase_solve(pxg_world* w, float h) {
             e1 = take_event(w);
             (void)hipEventRecord(e0, st);
         }
-        // fixed iteration counts: the barrier-free dataflow solve; residual mode needs a grid-wide
-        // reduction after every pass and uses the barrier solve
-        hipError_t e = (!c.residual_mode && w->dataflow) ? launch_solve_dataflow(d, p, cfgv, snb, st)
-                                                          : launch_solve_persistent(d, p, cfgv, c.residual_tolerance, snb, st);
+        hipError_t e = launch_solve_persistent(d, p, cfgv, c.residual_tolerance, snb, st);
         if (e == hipSuccess) {
             solved = true;
             if (w->profiling) {
@@ -838,13 +843,13 @@ extern "C" int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const floa
     size_t tc = size_t(w->d.cache_mask) + 1;
     if (n > tc / 2) return fail(PXG_ERR_CAPACITY, "pxg_set_warm_cache: too many entries");
     std::vector<unsigned long long> k(tc, ~0ull);
-    std::vector<uint32_t> o(tc, 0u);
+    std::vector<unsigned long long> o(tc, 0ull);
     std::vector<float4> v(tc, make_float4(0, 0, 0, 0));
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t idx = host_hash64(keys[i]) & w->d.cache_mask;
         while (k[idx] != ~0ull && k[idx] != keys[i]) idx = (idx + 1) & w->d.cache_mask;
         k[idx] = keys[i];
-        o[idx] = 1;
+        o[idx] = 1ull;
         // w carries the manifold's colour of the step that wrote the entry: none for uploads
         const uint32_t no_colour = 0xFFFFFFFFu;
         float nc;
@@ -853,7 +858,7 @@ extern "C" int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const floa
     }
     int t = w->d.cur;
     HIPCHK(hipMemcpyAsync(w->d.ck_keys[t], k.data(), tc * 8, hipMemcpyHostToDevice, w->stream));
-    HIPCHK(hipMemcpyAsync(w->d.ck_ord[t], o.data(), tc * 4, hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipMemcpyAsync(w->d.ck_ord[t], o.data(), tc * 8, hipMemcpyHostToDevice, w->stream));
     HIPCHK(hipMemcpyAsync(w->d.ck_val[t], v.data(), tc * sizeof(float4), hipMemcpyHostToDevice, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     return PXG_OK;

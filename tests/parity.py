@@ -80,7 +80,9 @@ def compare_step(gw, ow, pre_bodies, report):
     report["colours"] = max(report["colours"], len(offs) - 1)
 
 
-def lockstep(World, OracleWorld, cfg, bodies, shapes, steps, resync=True):
+def lockstep(World, OracleWorld, cfg, bodies, shapes, steps, resync=True, feed_order=True):
+    """feed_order: give the oracle the GPU's solve order each step; without it the oracle solves in
+    its own (the reference's) manifold order."""
     gw = World(cfg, bodies, shapes)
     ow = OracleWorld(cfg, bodies, shapes)
     report = dict(steps=0, manifolds=0, bitexact_manifolds=0, bitexact_bodies=0, max_body_diff=0.0, colours=0)
@@ -93,8 +95,9 @@ def lockstep(World, OracleWorld, cfg, bodies, shapes, steps, resync=True):
         pre = gw.bodies()
         gw.step_fixed()
         gm = gw.manifolds()
-        order, _ = gw.solve_order()
-        ow.set_order(np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1) if len(order) else np.zeros((0, 2)))
+        if feed_order:
+            order, _ = gw.solve_order()
+            ow.set_order(np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1) if len(order) else np.zeros((0, 2)))
         ow.step_fixed()
         compare_step(gw, ow, pre, report)
     return gw, ow, report

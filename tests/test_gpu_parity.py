@@ -16,12 +16,12 @@ def sides(oracle):
     return World, oracle.OracleWorld
 
 
-def _run(sides, scene_arrays, steps, resync=True, **cfg):
+def _run(sides, scene_arrays, steps, resync=True, feed_order=True, **cfg):
     World, OracleWorld = sides
     b, s = scene_arrays
     c = _abi.default_config(**cfg)
     print("lockstep: %d bodies, %d shapes, %d steps" % (len(b), len(s), steps), flush=True)
-    gw, ow, rep = lockstep(World, OracleWorld, c, b, s, steps, resync=resync)
+    gw, ow, rep = lockstep(World, OracleWorld, c, b, s, steps, resync=resync, feed_order=feed_order)
     print(rep, flush=True)
     return gw, ow, rep
 
@@ -56,6 +56,42 @@ def test_free_running_hash_matches_oracle_config1(sides):
     assert gw.state_hash() == ow.state_hash()
 
 
+# In the default (reference) solve order the GPU's colour-by-colour solve is the reference's own
+# sequential order up to commuting manifolds: the oracle solving in its own order, with no hint
+# from the GPU, must give the same bits.
+@pytest.mark.parametrize("name", ["config1", "mixed_pile", "sphere_rain", "pile_2048"])
+def test_reference_order_equals_oracle_natural_order(sides, name):
+    if name == "config1":
+        sc = scenes.config1_scene(seed=5)
+        arrays, cfg, steps = (sc.bodies, sc.shapes), sc.config, 200
+    elif name == "mixed_pile":
+        arrays, cfg, steps = scenes.mixed_pile_scene(n=300, seed=21), dict(velocity_iterations=8, position_iterations=3), 150
+    elif name == "sphere_rain":
+        arrays, cfg, steps = scenes.sphere_rain_scene(seed=99), {}, 200
+    else:
+        sc = scenes.config2_scene(seed=4, n=2048, lattice=(32, 32, 8))
+        arrays, cfg, steps = (sc.bodies, sc.shapes), sc.config, 80
+    _, _, rep = _run(sides, arrays, steps, feed_order=False, **cfg)
+    assert rep["manifolds"] > 0
+    assert rep["bitexact_bodies"] == rep["steps"]
+
+
+def test_reference_order_free_running_equals_oracle(sides):
+    """No resync and no order hint: 300 free-running steps of config 1 (the reference's scene) on
+    the GPU end in the same world_state_hash as the oracle's own sequential solve."""
+    sc = scenes.config1_scene(seed=1)
+    gw, ow, rep = _run(sides, (sc.bodies, sc.shapes), 300, resync=False, feed_order=False, **sc.config)
+    assert gw.state_hash() == ow.state_hash()
+
+
+def test_coloured_order_parity(sides):
+    """The faster greedy colouring: a different sequential order, checked against the oracle fed
+    that order."""
+    _, _, rep = _run(sides, scenes.mixed_pile_scene(n=300, seed=8), 100, velocity_iterations=8, position_iterations=3,
+                     solve_order=_abi.PXG_ORDER_COLOURED)
+    assert rep["manifolds"] > 0 and rep["bitexact_bodies"] == rep["steps"]
+
+
 def test_gpu_deterministic_across_runs(sides):
     World, _ = sides
     b, s = scenes.mixed_pile_scene(n=300, seed=3)
@@ -75,30 +111,6 @@ def test_parity_eager_launch_path(sides, monkeypatch):
     monkeypatch.setenv("PXG_EAGER", "1")
     _, _, rep = _run(sides, scenes.mixed_pile_scene(n=200, seed=11), 60, velocity_iterations=8, position_iterations=3)
     assert rep["manifolds"] > 0
-
-
-def test_parity_barrier_solve_path(sides, monkeypatch):
-    """The solve with a grid barrier between colours (the residual-mode kernel, here in fixed
-    iteration mode) gives the same results as the default barrier-free dataflow solve."""
-    monkeypatch.setenv("PXG_SOLVE", "barrier")
-    _, _, rep = _run(sides, scenes.mixed_pile_scene(n=200, seed=12), 60, velocity_iterations=8, position_iterations=3)
-    assert rep["manifolds"] > 0
-
-
-def test_dataflow_and_barrier_solves_agree_bit_for_bit(monkeypatch):
-    """Same scene, same steps: the two persistent solves must produce identical bodies."""
-    from physxcoopa_amd.world import World
-    sc = scenes.config2_scene(seed=3, n=2048, lattice=(32, 32, 8))
-    cfg = _abi.default_config(**sc.config)
-    out = []
-    for mode in ("dataflow", "barrier"):
-        monkeypatch.setenv("PXG_SOLVE", mode)
-        w = World(cfg, sc.bodies, sc.shapes)
-        for _ in range(150):
-            w.step_fixed()
-        out.append(w.bodies())
-        w.close()
-    assert out[0].tobytes() == out[1].tobytes()
 
 
 def test_parity_residual_mode(sides):
