@@ -41,6 +41,75 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read_per_launch": n, "copy_bytes_written_per_launch": n, "launches": 10}), flush=True)
 
 
+def bench_batch(args):
+    """4096 independent scenes (half box pyramids, half random drops of the pile's shape mix) of
+    8-256 bodies each, stepped K at a time in one graph-captured persistent launch."""
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only; got --gpus %d" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    from physxcoopa_amd import scenes, _abi, autotune
+    from physxcoopa_amd.batch import Batch
+    t0 = time.time()
+    b, s, bo, so, settings = scenes.config3_scenes(seed=3, n_scenes=4096, n_min=8, n_max=256)
+    cfg = _abi.default_config(**settings)
+    n_dyn = int(np.sum(b["type"] == 2))
+    log("batch built in %.1fs: %d scenes, %d dynamic bodies" % (time.time() - t0, len(bo) - 1, n_dyn))
+    (block, lpt), timings = autotune.tune(cfg, b, s, bo, so, steps=8, use_cache=False, log=log)
+    bt = Batch(cfg, b, s, bo, so)
+    bt.set_tuning(block, lpt)
+    chunk = 30
+    done = 0
+    while done < args.warmup:
+        k = min(chunk, args.warmup - done)
+        bt.step(k)
+        done += k
+    bt.synchronize()
+    log("warm-up %d steps in %.1fs" % (args.warmup, time.time() - t0))
+    t_start = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(chunk, args.steps - done)
+        bt.step(k)
+        done += k
+    bt.synchronize()
+    elapsed = time.perf_counter() - t_start
+    st = bt.stats()
+    assert np.all(st["error"] == 0), "batch reported errors"
+    value = n_dyn * args.steps / elapsed
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pxo
+        bodies = bt.bodies()
+        rng = np.random.default_rng(0)
+        t_cpu, bs_cpu = 0.0, 0
+        for k in rng.permutation(len(bo) - 1):
+            sb, ss = bodies[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]]
+            ow = pxo.OracleWorld(cfg, sb, ss)
+            t_cpu += ow.run(10)
+            bs_cpu += int(np.sum(sb["type"] == 2)) * 10
+            if t_cpu > args.cpu_seconds:
+                break
+        cpu = {"value": round(bs_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": 1, "kind": "port",
+               "sample": "randomly chosen scenes of the batch, 10 steps each from the GPU state, %d body-steps "
+                         "(C++ oracle, -O2, one thread)" % bs_cpu}
+    out = {
+        "metric": "body·steps/s at solver residual <=1e-4 (plain vs batched+learned); HBM GB/s",
+        "value": round(value, 1), "unit": "body·steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "config3_batch_4096", "scenes": len(bo) - 1, "dynamic_bodies": n_dyn,
+                   "velocity_iterations": int(cfg.velocity_iterations), "position_iterations": int(cfg.position_iterations),
+                   "block": block, "longest_first": lpt, "steps_per_launch": chunk},
+        "autotune_seconds": timings,
+        "manifolds": int(st["manifolds"].sum()), "points": int(st["points"].sum()),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -52,12 +121,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", choices=["reference", "coloured"], default="reference",
                     help="solve order: the reference's manifold order (default) or the greedy colouring")
+    ap.add_argument("--workload", choices=["pile", "batch"], default="pile",
+                    help="pile: the 16k-body pile (one world); batch: 4096 independent scenes of 8-256 bodies")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB ten times (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.workload == "batch":
+        bench_batch(args)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -1089,12 +1089,16 @@ __global__ void k_cache_write(DevState d, uint32_t bound) {
 // ------------------------------------------------------------------------------------------
 // islands and sleep (solver.h:808-876, island.h:25-49)
 // ------------------------------------------------------------------------------------------
+// Parent pointers only ever move to smaller indices (a root is hooked under the smaller root),
+// so a stale read is an older ancestor: following it still ends in the same set. Reads are plain
+// loads; the only atomic is the hook, whose returned value continues the search when another
+// thread hooked the root first.
 __device__ __forceinline__ uint32_t uf_find(uint32_t* parent, uint32_t x) {
     while (true) {
-        uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t p = parent[x];
         if (p == x) return x;
-        uint32_t g = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (g != p) __hip_atomic_store(&parent[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t g = parent[p];
+        if (g != p) parent[x] = g;  // path halving (a benign race: g is an ancestor of x)
         x = p;
     }
 }
@@ -1111,13 +1115,14 @@ __global__ void k_uf_union(DevState d, uint32_t bound) {
     if (m >= bound || m >= d.cnt->manifolds) return;
     uint2 ab = d.m_ab[m];
     if ((d.b_flags[ab.x] & BF_TYPE) != 2u || (d.b_flags[ab.y] & BF_TYPE) != 2u) return;
-    uint32_t a = ab.x, b = ab.y;
-    for (int guard = 0; guard < (1 << 24); ++guard) {
-        a = uf_find(d.uf_parent, a);
-        b = uf_find(d.uf_parent, b);
-        if (a == b) return;
+    uint32_t a = uf_find(d.uf_parent, ab.x), b = uf_find(d.uf_parent, ab.y);
+    for (int guard = 0; guard < (1 << 24) && a != b; ++guard) {
         if (a > b) { uint32_t t = a; a = b; b = t; }
-        if (atomicCAS(&d.uf_parent[b], b, a) == b) return;
+        uint32_t old = atomicCAS(&d.uf_parent[b], b, a);
+        if (old == b) return;
+        // b was hooked meanwhile: continue from its current ancestors
+        b = uf_find(d.uf_parent, old);
+        a = uf_find(d.uf_parent, a);
     }
 }
 __global__ void k_uf_compress(DevState d) {
@@ -1287,6 +1292,39 @@ __device__ __forceinline__ bool grid_sync(GridBarrier& g, uint32_t* err) {
     __syncthreads();
     return s_abort != 0u;
 }
+// The same barrier split in two: grid_arrive publishes this block's writes and counts it in;
+// grid_wait blocks until every block has arrived. Work between the two (loads of data no other
+// block writes in this phase) overlaps the barrier's latency.
+__device__ __forceinline__ void grid_arrive(GridBarrier& g) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        g.epoch += 1;
+        __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+__device__ __forceinline__ bool grid_wait(GridBarrier& g, uint32_t* err) {
+    __shared__ uint32_t s_abort_w;
+    if (threadIdx.x == 0) {
+        const uint32_t target = g.epoch * g.nblocks;
+        uint32_t spins = 0, abort = 0;
+        while (__hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (++spins > kSpinLimit) {
+                atomicOr(err, ERR_BARRIER);
+                abort = 1;
+                break;
+            }
+            if ((spins & 255u) == 0u && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)) {
+                abort = 1;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_abort_w = abort;
+    }
+    __syncthreads();
+    return s_abort_w != 0u;
+}
 __device__ __forceinline__ GridBarrier grid_barrier_init(uint32_t* bar) {
     GridBarrier g;
     g.count = bar;
@@ -1302,6 +1340,65 @@ struct SolveCfg {
     uint32_t residual_mode, residual_max;
     float residual_tol;
 };
+
+// A velocity-pass manifold with everything that does not change during the pass loaded ahead:
+// its slot (points in the pass's point order) and both bodies' inverse inertia and position. Only
+// the two bodies' velocities are read after the barrier that orders this colour after the last.
+struct PreSlot {
+    uint32_t s;  // kNoSlot: none
+    uint2 ab;
+    float4 nf;
+    uint32_t cnt;
+    uint32_t idx[4];
+    float4 pp[4], aux[4], imp[4];
+    float4 Ia[3], Ib[3];
+};
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+__device__ __forceinline__ void prefetch_slot(const DevState& d, uint32_t s, uint32_t offset, PreSlot& x) {
+    x.s = s;
+    x.ab = d.sm_ab[s];
+    x.nf = d.sm_nf[s];
+    x.cnt = d.sm_cnt[s];
+    const uint32_t r = x.cnt ? offset % x.cnt : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) x.idx[k] = 4u * s + (k < x.cnt ? (k + r) % x.cnt : k);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        x.pp[k] = d.sp_pos[x.idx[k]];
+        x.aux[k] = d.sp_aux[x.idx[k]];
+        x.imp[k] = d.sp_imp[x.idx[k]];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        x.Ia[j] = d.sb_I[3 * x.ab.x + j];
+        x.Ib[j] = d.sb_I[3 * x.ab.y + j];
+    }
+}
+__device__ __forceinline__ SBody sbody_from(float4 v, float4 w, const float4 (&I)[3]) {
+    SBody b;
+    b.v = ld3(v); b.im = v.w;
+    b.w = ld3(w); b.act = w.w != 0.0f;
+    b.I.c0 = mk(I[0].x, I[0].y, I[0].z); b.I.c1 = mk(I[0].w, I[1].x, I[1].y); b.I.c2 = mk(I[1].z, I[1].w, I[2].x);
+    b.x = mk(I[2].y, I[2].z, I[2].w);
+    return b;
+}
+// velocity_manifold for a prefetched slot (same arithmetic, same order)
+__device__ void velocity_prefetched(const DevState& d, const StepParams& prm, PreSlot& x, int use_rest) {
+    SBody A = sbody_from(d.sb_v[x.ab.x], d.sb_w[x.ab.x], x.Ia), B = sbody_from(d.sb_v[x.ab.y], d.sb_w[x.ab.y], x.Ib);
+    v3 n = ld3(x.nf);
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= x.cnt) break;
+        x.imp[k] = velocity_point(A, B, n, t1, t2, x.nf.w, x.pp[k], x.aux[k], x.imp[k], use_rest, prm.h);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+        if (k < x.cnt) d.sp_imp[x.idx[k]] = x.imp[k];
+    store_sbody(d, x.ab.x, A);
+    store_sbody(d, x.ab.y, B);
+}
 
 // The whole solve of one step (dynamics/solver.h:914-958 after the restitution capture):
 // warm start, velocity passes (fixed count or until the residual <= tolerance), relax passes,
@@ -1331,7 +1428,37 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     uint32_t passes = 0;
     __shared__ float s_red[kPersistBlock / 64];
     const uint32_t n_active = d.cnt->active;
-    uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : cfg.vel_iters;
+    if (!cfg.residual_mode) {
+        // velocity then relax passes, colour by colour; the next colour's first slot of this
+        // thread is loaded between arriving at the barrier and waiting on it
+        const uint32_t nph = (cfg.vel_iters + cfg.relax_iters) * C;
+        PreSlot x;
+        x.s = kNoSlot;
+        if (nph && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, x);
+        for (uint32_t ph = 0; ph < nph; ++ph) {
+            const uint32_t pass = ph / C, c = ph % C;
+            const int use_rest = pass >= cfg.vel_iters ? 1 : 0;
+            const uint32_t it = use_rest ? pass - cfg.vel_iters : pass;
+            if (prm.profile) {
+                uint32_t bytes = 0;
+                for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
+                bytes = wave_sum(bytes);
+                if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+            }
+            if (x.s != kNoSlot) velocity_prefetched(d, prm, x, use_rest);
+            for (uint32_t s = co[c] + tid + nthreads; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, use_rest, it);
+            grid_arrive(g);
+            x.s = kNoSlot;
+            if (ph + 1 < nph) {
+                const uint32_t np = (ph + 1) / C, nc = (ph + 1) % C;
+                const uint32_t nit = np >= cfg.vel_iters ? np - cfg.vel_iters : np;
+                if (co[nc] + tid < co[nc + 1]) prefetch_slot(d, co[nc] + tid, nit, x);
+            }
+            if (grid_wait(g, err)) return;
+        }
+        if (tid == 0) d.cnt->passes = cfg.vel_iters;
+    }
+    uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : 0u;
     for (uint32_t it = 0; it < max_pass; ++it) {
         for (uint32_t c = 0; c < C; ++c) {
             uint32_t b = co[c], e = co[c + 1];
@@ -1366,8 +1493,8 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (rr <= cfg.residual_tol) break;
         }
     }
-    if (tid == 0) d.cnt->passes = passes;
-    for (uint32_t it = 0; it < cfg.relax_iters; ++it) {
+    if (cfg.residual_mode && tid == 0) d.cnt->passes = passes;
+    for (uint32_t it = 0; cfg.residual_mode && it < cfg.relax_iters; ++it) {
         for (uint32_t c = 0; c < C; ++c) {
             if (prm.profile) {
                 uint32_t bytes = 0;
