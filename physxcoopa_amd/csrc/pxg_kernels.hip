@@ -1125,11 +1125,15 @@ __global__ void k_uf_union(DevState d, uint32_t bound) {
         a = uf_find(d.uf_parent, a);
     }
 }
-__global__ void k_uf_compress(DevState d) {
+// Each body's root, into a separate array: the parent array is only read here. (A find that
+// halves paths while other threads store their final roots could overwrite a stored root with an
+// intermediate ancestor, splitting one island's key in two.)
+__global__ void k_uf_roots(DevState d) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nb) return;
-    uint32_t r = uf_find(d.uf_parent, i);
-    __hip_atomic_store(&d.uf_parent[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t x = i;
+    for (uint32_t p = d.uf_parent[x]; p != x; p = d.uf_parent[x]) x = p;
+    d.uf_root[i] = x;
 }
 __global__ void k_sleep_timers(DevState d, StepParams p) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1168,7 +1172,7 @@ __global__ void k_kin_wake(DevState d, StepParams p, uint32_t bound) {
 __global__ void k_island_min(DevState d) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool act = i < d.nb && body_active(d.b_flags[i]);
-    uint32_t root = act ? d.uf_parent[i] : 0xFFFFFFFFu;
+    uint32_t root = act ? d.uf_root[i] : 0xFFFFFFFFu;
     uint32_t val = act ? __float_as_uint(d.b_pos[i].w) : 0xFFFFFFFFu;
     unsigned long long pending = __ballot(act);
     int lane = threadIdx.x & 63;
@@ -1186,7 +1190,7 @@ __global__ void k_sleep(DevState d, StepParams p) {
     if (i >= d.nb) return;
     uint32_t f = d.b_flags[i];
     if (!body_active(f)) return;
-    if (__uint_as_float(d.isl_min[d.uf_parent[i]]) >= p.sleep_time) {
+    if (__uint_as_float(d.isl_min[d.uf_root[i]]) >= p.sleep_time) {
         d.b_flags[i] = f & ~BF_AWAKE;
         float4 v = d.b_vel[i], w = d.b_angv[i];
         d.b_vel[i] = make_float4(0.0f, 0.0f, 0.0f, v.w);
@@ -1919,7 +1923,7 @@ void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t bound
     if (!d.nb) return;
     hipLaunchKernelGGL(k_uf_init, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     if (bound) hipLaunchKernelGGL(k_uf_union, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
-    hipLaunchKernelGGL(k_uf_compress, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_uf_roots, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_sleep_timers, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
     if (bound) hipLaunchKernelGGL(k_kin_wake, dim3(blocks(bound)), dim3(kBlock), 0, st, d, p, bound);
     hipLaunchKernelGGL(k_island_min, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);

@@ -126,6 +126,7 @@ struct DevState {
     int cur;            // table written this step (the other is read)
     // islands
     uint32_t *uf_parent;
+    uint32_t *uf_root;  // each body's island root, after the unions
     uint32_t *isl_min;
     uint32_t *bar;      // grid barrier of the persistent kernels: arrival count (zeroed per launch)
     Counters *cnt;

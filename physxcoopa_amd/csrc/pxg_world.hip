@@ -236,7 +236,7 @@ static int alloc_bodies(pxg_world* w, uint32_t n) {
     DALLOC(d.col_best, cap); DALLOC(d.col_mask, cap); DALLOC(d.col_dup, cap);
     DALLOC(d.adj_cnt, cap); DALLOC(d.adj_start, cap); DALLOC(d.adj_cur, cap);
     DALLOC(d.sb_v, cap); DALLOC(d.sb_w, cap); DALLOC(d.sb_I, 3 * size_t(cap)); DALLOC(d.sb_x, cap); DALLOC(d.sb_ref, cap);
-    DALLOC(d.uf_parent, cap); DALLOC(d.isl_min, cap);
+    DALLOC(d.uf_parent, cap); DALLOC(d.uf_root, cap); DALLOC(d.isl_min, cap);
     DALLOC(d.cnt, 1);
     DALLOC(d.layer_mask, 32);
     DALLOC(d.scan_tmp, 4096);

@@ -123,3 +123,26 @@ def test_parity_residual_mode(sides):
     print(rep, gw.stats(), ow.stats(), flush=True)
     assert gw.stats()["velocity_passes"] == ow.stats()["velocity_passes"]
     assert abs(gw.stats()["residual"] - ow.stats()["residual"]) <= 1e-6 + 1e-4 * abs(ow.stats()["residual"])
+
+
+def test_full_pile_free_running_equals_oracle(sides):
+    """The 16384-body pile (64x64x40 lattice) at full size, free-running for 300 steps with no
+    resync: every step's world_state_hash and awake count equal the oracle's. Islands start to
+    fall asleep after about 100 steps, so this covers the union-find and sleep kernels at the
+    concurrency of the full workload."""
+    World, OracleWorld = sides
+    sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
+    c = _abi.default_config(**sc.config)
+    gw = World(c, sc.bodies, sc.shapes)
+    ow = OracleWorld(c, sc.bodies, sc.shapes)
+    asleep = 0
+    for i in range(300):
+        gw.step_fixed()
+        ow.step_fixed()
+        if gw.state_hash() != ow.state_hash():
+            gb, ob = gw.bodies(), ow.bodies()
+            raise AssertionError("step %d: hashes differ (awake %d vs %d)" % (i + 1, gb["awake"].sum(), ob["awake"].sum()))
+        if i % 50 == 49:
+            asleep = int(len(sc.bodies) - gw.bodies()["awake"].sum())
+            print("step", i + 1, "asleep", asleep, flush=True)
+    assert asleep > 0
