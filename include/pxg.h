@@ -188,6 +188,42 @@ typedef struct pxg_stats {
     uint64_t steps;              /* fixed steps run since creation */
 } pxg_stats;
 
+/* Joint types (physxcoopa/dynamics/joint.h:57-61). */
+enum pxg_joint_type { PXG_HINGE = 0, PXG_BALL = 1, PXG_CONE_TWIST = 2 };
+
+/* One joint (physxcoopa/dynamics/joint.h:74-135). Anchors and axes are in each body's own frame.
+ * A hinge locks two rotational DOFs (optional angle limit [min_angle, max_angle]); a ball joint
+ * holds only the anchors together; a cone-twist adds a swing cone (swing_limit >= 0) and a twist
+ * range (twist_min <= twist_max). Limits are measured from rest_relative_rotation =
+ * inverse(qa) * qb at creation (world.h:415). The impulse accumulators are solver state: the
+ * point and hinge-axis ones carry over between steps (the joint's warm start). */
+typedef struct pxg_joint {
+    uint32_t type;               /* pxg_joint_type */
+    uint32_t body_a, body_b;     /* body indices */
+    uint32_t valid;              /* 0: ignored by every pass (removed joint, or a removed body) */
+    uint32_t enabled;
+    uint32_t use_limits;         /* hinge angle limit on */
+    uint32_t collide_connected;  /* 0: body_a and body_b generate no contacts with each other */
+    uint32_t reserved0;
+    float local_anchor_a[3];
+    float min_angle;             /* radians */
+    float local_anchor_b[3];
+    float max_angle;
+    float local_axis_a[3];       /* hinge axis / twist axis (normalised where read) */
+    float swing_limit;           /* cone half-angle; negative: swing free */
+    float local_axis_b[3];
+    float twist_min;             /* twist_min > twist_max: twist free */
+    float rest_relative_rotation[4];  /* x, y, z, w */
+    float twist_max;
+    float limit_impulse;         /* hinge limit / cone twist limit accumulator */
+    float swing_impulse;
+    float reserved1;
+    float point_impulse[3];
+    float reserved2;
+    float axis_impulse[2];
+    float reserved3[2];
+} pxg_joint;
+
 typedef struct pxg_world pxg_world;
 
 /* --- configuration and lifetime --- */
@@ -208,6 +244,16 @@ int pxg_get_bodies(pxg_world* w, pxg_body* out, uint32_t n);
 int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n);
 /* 32 x 32 layer matrix rows (broadphase/layer_matrix.h:21-47). */
 int pxg_set_layer_masks(pxg_world* w, const uint32_t masks[32]);
+/* Replaces the joint table: the joints_ array behind add_hinge_joint / add_ball_joint /
+ * add_cone_twist_joint / add_joint / remove_joint / get_joint (world.h:400-518). Joints are
+ * solved in table order (joint passes alternate direction per velocity iteration, solver.h:939-946);
+ * body indices must be < the body count. n == 0 removes every joint. */
+int pxg_set_joints(pxg_world* w, const pxg_joint* joints, uint32_t n);
+/* Downloads the joint table with the solver's accumulators (get_joint, world.h:514). */
+int pxg_get_joints(pxg_world* w, pxg_joint* out, uint32_t n);
+/* Body pairs whose contacts are switched off regardless of layers: set_pair_collision(a, b,
+ * false), world.h:527-532. ab holds n (a, b) body index pairs; n == 0 clears the set. */
+int pxg_set_ignored_pairs(pxg_world* w, const uint32_t* ab, uint32_t n);
 
 /* --- stepping --- */
 int pxg_step_fixed(pxg_world* w, float h);                        /* step_fixed(), world.h:1278 */

@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from physxcoopa_amd._abi import BODY_DTYPE, MANIFOLD_DTYPE, ptr
+from physxcoopa_amd._abi import BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, ptr
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libpxo.so")
@@ -46,6 +46,12 @@ def lib():
             "pxo_combine": ([f32, f32, i32, i32], f32), "pxo_orthonormal_basis": ([vp, vp, vp], None),
             "pxo_quat_rotate": ([vp, vp, vp], None), "pxo_mat3_cast": ([vp, vp], None),
             "pxo_pack_key": ([u32, u32, u32], u64),
+            "pxo_set_joints": ([vp, vp, u32], None), "pxo_get_joints": ([vp, vp, u32], u32),
+            "pxo_set_ignored_pairs": ([vp, vp, u32], None), "pxo_set_atan2_mode": ([vp, u32], None),
+            "pxo_m3_inverse": ([vp, vp], None), "pxo_m3_det": ([vp], f32),
+            "pxo_point_block_mass": ([vp, vp, f32, f32, vp, vp, vp], i32),
+            "pxo_hinge_angle": ([vp, vp, vp, vp, u32], f32), "pxo_cone_swing": ([vp, vp, vp, u32], f32),
+            "pxo_atan2": ([f32, f32, u32], f32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -100,6 +106,24 @@ class OracleWorld:
 
     def run(self, steps, h=None):
         return lib().pxo_run(self.h, steps, np.float32(self._cfg.fixed_dt if h is None else h))
+
+    def set_joints(self, joints):
+        j = np.ascontiguousarray(joints, dtype=JOINT_DTYPE)
+        self.n_joints = len(j)
+        lib().pxo_set_joints(self.h, ptr(j), len(j))
+
+    def joints(self):
+        out = np.zeros(getattr(self, "n_joints", 0), JOINT_DTYPE)
+        lib().pxo_get_joints(self.h, ptr(out), len(out))
+        return out
+
+    def set_ignored_pairs(self, ab):
+        a = np.ascontiguousarray(np.asarray(ab, np.uint32).reshape(-1, 2))
+        lib().pxo_set_ignored_pairs(self.h, ptr(a), len(a))
+
+    def set_atan2_mode(self, rounded):
+        """False: the reference's std::atan2 on floats; True: atan2 in double rounded once."""
+        lib().pxo_set_atan2_mode(self.h, 1 if rounded else 0)
 
     def set_order(self, pairs_ab):
         a = np.ascontiguousarray(np.asarray(pairs_ab, np.uint32).reshape(-1, 2))

@@ -70,6 +70,35 @@ template <class T> void manifold_to(const Manifold<T>& m, pxg_manifold& d) {
     }
 }
 
+template <class T> Joint<T> joint_from(const pxg_joint& d) {
+    Joint<T> j;
+    j.type = int(d.type); j.a = d.body_a; j.b = d.body_b;
+    j.valid = d.valid != 0; j.enabled = d.enabled != 0; j.use_limits = d.use_limits != 0;
+    j.collide_connected = d.collide_connected != 0;
+    j.anchor_a = v3<T>(d.local_anchor_a); j.anchor_b = v3<T>(d.local_anchor_b);
+    j.axis_a = v3<T>(d.local_axis_a); j.axis_b = v3<T>(d.local_axis_b);
+    j.min_angle = T(d.min_angle); j.max_angle = T(d.max_angle);
+    j.swing_limit = T(d.swing_limit); j.twist_min = T(d.twist_min); j.twist_max = T(d.twist_max);
+    j.rest = q4<T>(d.rest_relative_rotation);
+    j.point_impulse = v3<T>(d.point_impulse);
+    j.axis_impulse[0] = T(d.axis_impulse[0]); j.axis_impulse[1] = T(d.axis_impulse[1]);
+    j.limit_impulse = T(d.limit_impulse); j.swing_impulse = T(d.swing_impulse);
+    return j;
+}
+template <class T> void joint_to(const Joint<T>& j, pxg_joint& d) {
+    std::memset(&d, 0, sizeof(d));
+    d.type = uint32_t(j.type); d.body_a = j.a; d.body_b = j.b;
+    d.valid = j.valid; d.enabled = j.enabled; d.use_limits = j.use_limits; d.collide_connected = j.collide_connected;
+    put3(d.local_anchor_a, j.anchor_a); put3(d.local_anchor_b, j.anchor_b);
+    put3(d.local_axis_a, j.axis_a); put3(d.local_axis_b, j.axis_b);
+    d.min_angle = float(j.min_angle); d.max_angle = float(j.max_angle);
+    d.swing_limit = float(j.swing_limit); d.twist_min = float(j.twist_min); d.twist_max = float(j.twist_max);
+    put4(d.rest_relative_rotation, j.rest);
+    put3(d.point_impulse, j.point_impulse);
+    d.axis_impulse[0] = float(j.axis_impulse[0]); d.axis_impulse[1] = float(j.axis_impulse[1]);
+    d.limit_impulse = float(j.limit_impulse); d.swing_impulse = float(j.swing_impulse);
+}
+
 struct Handle {
     int dbl = 0;
     World<float> f;
@@ -128,6 +157,37 @@ void pxo_set_shapes(void* p, const pxg_shape* in, uint32_t n) {
 void pxo_set_layer_masks(void* p, const uint32_t* m) {
     Handle* h = static_cast<Handle*>(p);
     for (int i = 0; i < 32; ++i) { h->f.layer_mask[i] = m[i]; h->d.layer_mask[i] = m[i]; }
+}
+void pxo_set_joints(void* p, const pxg_joint* in, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        w.joints.resize(n);
+        for (uint32_t i = 0; i < n; ++i) w.joints[i] = joint_from<T>(in[i]);
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+uint32_t pxo_get_joints(void* p, pxg_joint* out, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) -> uint32_t {
+        for (uint32_t i = 0; i < cap && i < w.joints.size(); ++i) joint_to(w.joints[i], out[i]);
+        return uint32_t(w.joints.size());
+    };
+    return h->dbl ? go(h->d) : go(h->f);
+}
+void pxo_set_ignored_pairs(void* p, const uint32_t* ab, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        w.ignored.clear();
+        for (uint32_t i = 0; i < n; ++i) w.ignored.insert(w.body_pair_key(ab[2 * i], ab[2 * i + 1]));
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+// 0: std::atan2 on the scalar type (the reference's call); 1: atan2 in double, rounded once
+void pxo_set_atan2_mode(void* p, uint32_t mode) {
+    Handle* h = static_cast<Handle*>(p);
+    h->f.atan2_mode = mode ? Atan2Mode::Rounded : Atan2Mode::Libm;
+    h->d.atan2_mode = h->f.atan2_mode;
 }
 void pxo_set_brute_force(void* p, uint32_t on) {
     Handle* h = static_cast<Handle*>(p);
@@ -299,5 +359,34 @@ void pxo_mat3_cast(const float* q, float* out9) {
     for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) out9[c * 3 + r] = m.c[c][r];
 }
 uint64_t pxo_pack_key(uint32_t a, uint32_t b, uint32_t fid) { return pack_key(a, b, fid); }
+// joint helpers (FP32, libm atan2 unless mode == 1)
+void pxo_m3_inverse(const float* in9, float* out9) {
+    M3<float> m;
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) m.c[c][r] = in9[c * 3 + r];
+    M3<float> i = m3_inverse(m);
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) out9[c * 3 + r] = i.c[c][r];
+}
+float pxo_m3_det(const float* in9) {
+    M3<float> m;
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) m.c[c][r] = in9[c * 3 + r];
+    return m3_det(m);
+}
+int pxo_point_block_mass(const float* ra, const float* rb, float ima, float imb, const float* Ia9, const float* Ib9, float* out9) {
+    M3<float> A, B, K;
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) { A.c[c][r] = Ia9[c * 3 + r]; B.c[c][r] = Ib9[c * 3 + r]; }
+    int ok = point_block_mass(v3<float>(ra), v3<float>(rb), ima, imb, A, B, K) ? 1 : 0;
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) out9[c * 3 + r] = K.c[c][r];
+    return ok;
+}
+float pxo_hinge_angle(const float* qa, const float* qb, const float* axis, const float* rest, uint32_t mode) {
+    return hinge_angle(q4<float>(qa), q4<float>(qb), v3<float>(axis), q4<float>(rest), mode ? Atan2Mode::Rounded : Atan2Mode::Libm);
+}
+float pxo_cone_swing(const float* ta, const float* tb, float* axis, uint32_t mode) {
+    V3<float> s;
+    float a = cone_swing(v3<float>(ta), v3<float>(tb), s, mode ? Atan2Mode::Rounded : Atan2Mode::Libm);
+    put3(axis, s);
+    return a;
+}
+float pxo_atan2(float y, float x, uint32_t mode) { return atan2_of(y, x, mode ? Atan2Mode::Rounded : Atan2Mode::Libm); }
 
 }  // extern "C"

@@ -4,6 +4,9 @@
 // physxcoopa/dynamics/solver.h:46-958 and the integrator in physxcoopa/dynamics/integrator.h.
 #pragma once
 #include "pxo_world.hpp"
+#include "pxo_joints.hpp"
+
+#include <set>
 
 namespace pxo {
 
@@ -69,6 +72,11 @@ template <class T> struct World {
     std::unordered_map<uint64_t, Cached> miss_table;
     bool use_miss_table = false;
     uint32_t brute_force = 0;
+    // joints (world.h:2125) and the bodies whose contacts are switched off (world.h:2127, keys
+    // lo << 32 | hi of the two body indices)
+    std::vector<Joint<T>> joints;
+    std::set<uint64_t> ignored;
+    Atan2Mode atan2_mode = Atan2Mode::Libm;
     // diagnostics of the last step
     uint32_t vel_passes = 0;
     T last_residual = 0;
@@ -175,6 +183,18 @@ template <class T> struct World {
         // PairCache::add drops x == y (pair_cache.h:43-46); owners differ so shapes differ too
     }
 
+    static uint64_t body_pair_key(uint32_t a, uint32_t b) {  // world.h:1657-1660
+        return (uint64_t(std::min(a, b)) << 32) | std::max(a, b);
+    }
+    bool joint_blocks(uint32_t i, uint32_t j) const {  // world.h:1686-1694
+        for (const auto& jt : joints) {
+            if (!jt.valid || jt.collide_connected) continue;
+            if ((jt.a == i && jt.b == j) || (jt.a == j && jt.b == i)) return true;
+        }
+        return false;
+    }
+    JointSolver<T, Body<T>> joint_solver() { return JointSolver<T, Body<T>>{bodies, joints, atan2_mode}; }
+
     // world.h:1598-1654
     void narrowphase(T h) {
         manifolds.clear();
@@ -189,6 +209,8 @@ template <class T> struct World {
             bool either_dyn = bi.type == PXG_DYNAMIC || bj.type == PXG_DYNAMIC;
             if (!either_dyn) continue;
             if (!should_collide(si.layer, sj.layer)) continue;
+            if (joint_blocks(ii, ij)) continue;
+            if (!ignored.empty() && ignored.count(body_pair_key(ii, ij))) continue;
             T ei = min_extent(si), ej = min_extent(sj);
             bool spec = (length(bi.v) * h > ei) || (length(bj.v) * h > ej);
             Manifold<T> m;
@@ -404,6 +426,13 @@ template <class T> struct World {
                 if (ra != rb) parent[ra] = rb;
             }
         }
+        for (auto& j : joints) {
+            if (!j.valid || !j.enabled) continue;
+            if (bodies[j.a].type == PXG_DYNAMIC && bodies[j.b].type == PXG_DYNAMIC) {
+                uint32_t ra = find(j.a), rb = find(j.b);
+                if (ra != rb) parent[ra] = rb;
+            }
+        }
         for (auto& b : bodies) {
             if (!b.alive || b.type != PXG_DYNAMIC || !b.awake) continue;
             T l2 = dot(b.v, b.v), a2 = dot(b.w, b.w);
@@ -414,6 +443,11 @@ template <class T> struct World {
             if (!m.valid) continue;
             wake_if_kin_moving(bodies[m.a], bodies[m.b]);
             wake_if_kin_moving(bodies[m.b], bodies[m.a]);
+        }
+        for (auto& j : joints) {
+            if (!j.valid || !j.enabled) continue;
+            wake_if_kin_moving(bodies[j.a], bodies[j.b]);
+            wake_if_kin_moving(bodies[j.b], bodies[j.a]);
         }
         std::unordered_map<uint32_t, T> mint;
         for (uint32_t i = 0; i < bodies.size(); ++i) {
@@ -454,12 +488,15 @@ template <class T> struct World {
 
     uint32_t last_predicted = 0;
 
-    // dynamics/solver.h:914-958 (no joints)
+    // dynamics/solver.h:914-958
     void solve(T h) {
+        JointSolver<T, Body<T>> js = joint_solver();
+        js.wake_jointed();
         std::vector<Manifold<T>*> ms = solve_order();
         capture_restitution(ms);
         last_predicted = 0;
         warm_start(ms, last_predicted);
+        js.warm_start();
         residual_trace.clear();
         if (cfg.residual_mode) {
             uint32_t i = 0;
@@ -467,6 +504,7 @@ template <class T> struct World {
             while (i < cfg.residual_max) {
                 if (i >= 1 && r <= cfg.residual_tol) break;
                 velocity_pass(ms, false, i, h);
+                js.velocity_pass(h, (i & 1u) != 0);
                 ++i;
                 r = residual(ms, false, h);
                 residual_trace.push_back(r);
@@ -476,12 +514,17 @@ template <class T> struct World {
         } else {
             for (uint32_t i = 0; i < cfg.vel_iters; ++i) {
                 velocity_pass(ms, false, i, h);
+                js.velocity_pass(h, (i & 1u) != 0);
             }
             vel_passes = cfg.vel_iters;
         }
-        for (uint32_t i = 0; i < cfg.relax_iters; ++i) velocity_pass(ms, true, i, h);
+        for (uint32_t i = 0; i < cfg.relax_iters; ++i) {
+            velocity_pass(ms, true, i, h);
+            js.velocity_pass(h, (i & 1u) != 0);
+        }
         if (!cfg.residual_mode) last_residual = residual(ms, cfg.relax_iters > 0, h);
         solve_position(ms);
+        js.position_pass(cfg);
         rebuild_cache();
         islands_and_sleep(h);
     }
@@ -541,6 +584,11 @@ template <class T> struct World {
     void step_phase_solve(T h) {
         solve(h);
         integrate_velocities_all(h);
+        if (!joints.empty()) {  // world.h:1310-1314, solver.h:755-761
+            JointSolver<T, Body<T>> js = joint_solver();
+            for (uint32_t it = 0; it < std::max(1u, cfg.pos_iters); ++it) js.position_pass(cfg);
+            for (auto& b : bodies) if (b.alive) update_world_inertia(b);
+        }
         ++steps;
     }
     void step_fixed(T h) { step_phase_collide(h); step_phase_solve(h); }  // world.h:1278-1317

@@ -55,6 +55,22 @@ MANIFOLD_DTYPE = np.dtype([
 ])
 assert MANIFOLD_DTYPE.itemsize == 240
 
+PXG_HINGE, PXG_BALL, PXG_CONE_TWIST = 0, 1, 2
+
+JOINT_DTYPE = np.dtype([
+    ("type", "<u4"), ("body_a", "<u4"), ("body_b", "<u4"), ("valid", "<u4"),
+    ("enabled", "<u4"), ("use_limits", "<u4"), ("collide_connected", "<u4"), ("reserved0", "<u4"),
+    ("local_anchor_a", "<f4", 3), ("min_angle", "<f4"),
+    ("local_anchor_b", "<f4", 3), ("max_angle", "<f4"),
+    ("local_axis_a", "<f4", 3), ("swing_limit", "<f4"),
+    ("local_axis_b", "<f4", 3), ("twist_min", "<f4"),
+    ("rest_relative_rotation", "<f4", 4),
+    ("twist_max", "<f4"), ("limit_impulse", "<f4"), ("swing_impulse", "<f4"), ("reserved1", "<f4"),
+    ("point_impulse", "<f4", 3), ("reserved2", "<f4"),
+    ("axis_impulse", "<f4", 2), ("reserved3", "<f4", 2),
+])
+assert JOINT_DTYPE.itemsize == 160
+
 
 BATCH_STATS_DTYPE = np.dtype([
     ("pairs", "<u4"), ("manifolds", "<u4"), ("points", "<u4"), ("colours", "<u4"),

@@ -5,6 +5,7 @@
 // work item per thread, grid-stride where the item count lives on the device.
 #include "pxg_state.h"
 #include "pxg_device.h"
+#include "pxg_joints.h"
 #include "pxg_mlp.h"
 
 using namespace pxd;
@@ -477,6 +478,19 @@ __global__ void k_pairs_sort(DevState d) {
 // ------------------------------------------------------------------------------------------
 // narrowphase (world.h:1598-1654, collision/narrowphase.h:175-335)
 // ------------------------------------------------------------------------------------------
+// joint_blocks_collision_ and ignored_pairs_ (world.h:1619-1620): binary search of the sorted
+// body-pair keys
+__device__ __forceinline__ bool contacts_off(const DevState& d, uint32_t a, uint32_t b) {
+    if (d.n_nc == 0u) return false;
+    const unsigned long long key = (static_cast<unsigned long long>(a < b ? a : b) << 32) | (a < b ? b : a);
+    uint32_t lo = 0, hi = d.n_nc;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (d.nc_keys[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < d.n_nc && d.nc_keys[lo] == key;
+}
 __global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
     uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_pairs) return;
@@ -490,7 +504,7 @@ __global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
     m.count = 0;
     float4 mi = d.s_mat[i], mj = d.s_mat[j];
     if ((fi & BF_ALIVE) && (fj & BF_ALIVE) && ((fi & BF_TYPE) == 2u || (fj & BF_TYPE) == 2u) &&
-        (d.layer_mask[ii.z & 31u] & (1u << (ij.z & 31u)))) {
+        (d.layer_mask[ii.z & 31u] & (1u << (ij.z & 31u))) && !contacts_off(d, bi, bj)) {
         ShapeGeom si = load_shape(d, i, ii), sj = load_shape(d, j, ij);
         float4 vi = d.b_vel[bi], vj = d.b_vel[bj];
         bool spec = (len(ld3(vi)) * prm.h > min_extent(si)) || (len(ld3(vj)) * prm.h > min_extent(sj));
@@ -1135,6 +1149,22 @@ __global__ void k_uf_roots(DevState d) {
     for (uint32_t p = d.uf_parent[x]; p != x; p = d.uf_parent[x]) x = p;
     d.uf_root[i] = x;
 }
+// joint pairs unite islands too (solver.h:820-825)
+__global__ void k_uf_union_joints(DevState d) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.nj) return;
+    uint4 m = d.j_meta[j];
+    if (!m.w) return;
+    if ((d.b_flags[m.y] & BF_TYPE) != 2u || (d.b_flags[m.z] & BF_TYPE) != 2u) return;
+    uint32_t a = uf_find(d.uf_parent, m.y), b = uf_find(d.uf_parent, m.z);
+    for (int guard = 0; guard < (1 << 24) && a != b; ++guard) {
+        if (a > b) { uint32_t t = a; a = b; b = t; }
+        uint32_t old = atomicCAS(&d.uf_parent[b], b, a);
+        if (old == b) return;
+        b = uf_find(d.uf_parent, old);
+        a = uf_find(d.uf_parent, a);
+    }
+}
 __global__ void k_sleep_timers(DevState d, StepParams p) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.nb) return;
@@ -1154,6 +1184,27 @@ __global__ void k_kin_wake(DevState d, StepParams p, uint32_t bound) {
     uint2 ab = d.m_ab[m];
     for (int side = 0; side < 2; ++side) {
         uint32_t kin = side == 0 ? ab.x : ab.y, dyn = side == 0 ? ab.y : ab.x;
+        uint32_t fk = d.b_flags[kin], fd = d.b_flags[dyn];
+        if ((fk & BF_TYPE) != 1u || (fd & BF_TYPE) != 2u || (fd & BF_AWAKE)) continue;
+        v3 kv = ld3(d.b_vel[kin]), kw = ld3(d.b_angv[kin]);
+        bool lm = dot(kv, kv) > p.sleep_lin * p.sleep_lin;
+        bool am = dot(kw, kw) > p.sleep_ang * p.sleep_ang;
+        if (lm || am) {
+            atomicOr(&d.b_flags[dyn], BF_AWAKE);
+            float4 pt = d.b_pos[dyn];
+            pt.w = 0.0f;
+            d.b_pos[dyn] = pt;
+        }
+    }
+}
+// wake_if_kinematic_moving both ways per live joint (solver.h:854-858)
+__global__ void k_kin_wake_joints(DevState d, StepParams p) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.nj) return;
+    uint4 m = d.j_meta[j];
+    if (!m.w) return;
+    for (int side = 0; side < 2; ++side) {
+        uint32_t kin = side == 0 ? m.y : m.z, dyn = side == 0 ? m.z : m.y;
         uint32_t fk = d.b_flags[kin], fd = d.b_flags[dyn];
         if ((fk & BF_TYPE) != 1u || (fd & BF_TYPE) != 2u || (fd & BF_AWAKE)) continue;
         v3 kv = ld3(d.b_vel[kin]), kw = ld3(d.b_angv[kin]);
@@ -1250,6 +1301,170 @@ __global__ void k_integrate_velocities(DevState d, StepParams p) {
         z.c0 = splat(0.0f); z.c1 = splat(0.0f); z.c2 = splat(0.0f);
         store_iiw(d.b_iiw, i, z);
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// joints (dynamics/solver.h:330-761, world.h:1310-1314). Each pass visits the joints layer by
+// layer; a layer holds joints that share no dynamic body, in table order, so the per-body
+// update order is the reference's sequential one.
+// ------------------------------------------------------------------------------------------
+// bodies as the velocity passes and the warm start see them: the solver bodies
+__device__ __forceinline__ JBody jb_solver(const DevState& d, uint32_t i) {
+    JBody b;
+    float4 v = d.sb_v[i], w = d.sb_w[i];
+    float4 a = d.sb_I[3 * i], c = d.sb_I[3 * i + 1], e = d.sb_I[3 * i + 2];
+    b.v = ld3(v); b.im = v.w;
+    b.w = ld3(w); b.act = w.w != 0.0f;
+    b.I.c0 = mk(a.x, a.y, a.z); b.I.c1 = mk(a.w, c.x, c.y); b.I.c2 = mk(c.z, c.w, e.x);
+    b.x = mk(e.y, e.z, e.w);
+    b.q = ldq(d.b_rot[i]);
+    return b;
+}
+__device__ __forceinline__ void jb_solver_store(const DevState& d, uint32_t i, const JBody& b) {
+    if (!b.act) return;
+    d.sb_v[i] = st3(b.v, b.im);
+    d.sb_w[i] = st3(b.w, 1.0f);
+}
+// the position pass of the solve: positions from the position-pass state
+__device__ __forceinline__ JBody jb_position(const DevState& d, uint32_t i) {
+    JBody b = jb_solver(d, i);
+    b.x = ld3(d.sb_x[i]);
+    return b;
+}
+__device__ __forceinline__ void jb_position_store(const DevState& d, uint32_t i, const JBody& b) {
+    if (!b.act) return;
+    d.sb_x[i] = st3(b.x, b.im);
+    d.b_rot[i] = stq(b.q);
+}
+// after integration: the body arrays
+__device__ __forceinline__ JBody jb_world(const DevState& d, uint32_t i) {
+    JBody b;
+    uint32_t f = d.b_flags[i];
+    b.act = body_active(f);
+    b.x = ld3(d.b_pos[i]);
+    b.q = ldq(d.b_rot[i]);
+    b.v = ld3(d.b_vel[i]);
+    b.w = ld3(d.b_angv[i]);
+    b.im = b.act ? d.b_vel[i].w : 0.0f;
+    b.I = b.act ? load_iiw(d.b_iiw, i) : zero3();
+    return b;
+}
+__device__ __forceinline__ void jb_world_store(const DevState& d, uint32_t i, const JBody& b) {
+    if (!b.act) return;
+    float4 p = d.b_pos[i];
+    d.b_pos[i] = st3(b.x, p.w);
+    d.b_rot[i] = stq(b.q);
+}
+__device__ __forceinline__ float inv_step(float h) { return h > 0.0f ? 1.0f / h : 0.0f; }
+
+__device__ void joint_warm_one(const DevState& d, uint32_t j) {
+    uint4 m = d.j_meta[j];
+    if (!m.w) return;
+    JDef def = load_jdef(d.j_def, j);
+    JAcc acc = load_jacc(d.j_acc, j);
+    acc.limit = 0.0f;  // limits start every step cold (solver.h:487-488)
+    acc.swing = 0.0f;
+    JBody a = jb_solver(d, m.y), b = jb_solver(d, m.z);
+    joint_warm(def, acc, a, b);
+    store_jacc(d.j_acc, j, acc);
+    jb_solver_store(d, m.y, a);
+    jb_solver_store(d, m.z, b);
+}
+__device__ void joint_velocity_one(const DevState& d, uint32_t j, float inv_h) {
+    uint4 m = d.j_meta[j];
+    if (!m.w) return;
+    JDef def = load_jdef(d.j_def, j);
+    JAcc acc = load_jacc(d.j_acc, j);
+    JBody a = jb_solver(d, m.y), b = jb_solver(d, m.z);
+    if (!a.act && !b.act) return;
+    joint_velocity(def, acc, a, b, inv_h);
+    store_jacc(d.j_acc, j, acc);
+    jb_solver_store(d, m.y, a);
+    jb_solver_store(d, m.z, b);
+}
+__device__ void joint_position_one(const DevState& d, const StepParams& p, uint32_t j) {
+    uint4 m = d.j_meta[j];
+    if (!m.w) return;
+    JDef def = load_jdef(d.j_def, j);
+    JBody a = jb_position(d, m.y), b = jb_position(d, m.z);
+    if (!a.act && !b.act) return;
+    joint_position(def, a, b, p.slop, p.pos_corr, p.max_corr);
+    jb_position_store(d, m.y, a);
+    jb_position_store(d, m.z, b);
+}
+__device__ void joint_post_one(const DevState& d, const StepParams& p, uint32_t j) {
+    uint4 m = d.j_meta[j];
+    if (!m.w) return;
+    JDef def = load_jdef(d.j_def, j);
+    JBody a = jb_world(d, m.y), b = jb_world(d, m.z);
+    if (!a.act && !b.act) return;
+    joint_position(def, a, b, p.slop, p.pos_corr, p.max_corr);
+    jb_world_store(d, m.y, a);
+    jb_world_store(d, m.z, b);
+}
+// the joints of layer l (forward or reverse order), strided over `nthreads` threads
+__device__ __forceinline__ void joint_layer_range(const DevState& d, uint32_t l, bool rev, uint32_t& begin, uint32_t& end,
+                                                  const uint32_t*& order) {
+    const uint32_t* off = d.j_layer_off + (rev ? d.nj_layers + 1u : 0u);
+    order = d.j_order + (rev ? d.nj : 0u);
+    begin = off[l];
+    end = off[l + 1];
+}
+
+// wake_jointed_islands (solver.h:887-901): sweeps until nothing changes. One block; the fixed
+// point is the same whatever order the joints are visited in (every dynamic body jointed,
+// through dynamic bodies, to an awake one wakes).
+__global__ void __launch_bounds__(1024) k_wake_jointed(DevState d) {
+    for (uint32_t pass = 0; pass <= d.nj; ++pass) {
+        int changed = 0;
+        for (uint32_t j = threadIdx.x; j < d.nj; j += blockDim.x) {
+            uint4 m = d.j_meta[j];
+            if (!m.w) continue;
+            uint32_t fa = d.b_flags[m.y], fb = d.b_flags[m.z];
+            if ((fa & BF_TYPE) != 2u || (fb & BF_TYPE) != 2u) continue;
+            bool aa = (fa & BF_AWAKE) != 0u, ba = (fb & BF_AWAKE) != 0u;
+            if (aa == ba) continue;
+            uint32_t s = aa ? m.z : m.y;
+            d.b_flags[s] = (aa ? fb : fa) | BF_AWAKE;
+            float4 pt = d.b_pos[s];
+            pt.w = 0.0f;
+            d.b_pos[s] = pt;
+            changed = 1;
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
+}
+__global__ void k_joint_warm(DevState d, uint32_t begin, uint32_t end) {
+    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < end) joint_warm_one(d, d.j_order[t]);
+}
+__global__ void k_joint_velocity(DevState d, StepParams p, int rev, uint32_t begin, uint32_t end) {
+    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < end) joint_velocity_one(d, d.j_order[(rev ? d.nj : 0u) + t], inv_step(p.h));
+}
+__global__ void k_joint_position(DevState d, StepParams p, uint32_t begin, uint32_t end) {
+    uint32_t t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < end) joint_position_one(d, p, d.j_order[t]);
+}
+// solve_joints_post_integrate (solver.h:755-761): max(1, position_iterations) forward passes on
+// the integrated poses, one block, layers separated by block barriers
+__global__ void __launch_bounds__(1024) k_joints_post(DevState d, StepParams p, uint32_t iters) {
+    for (uint32_t it = 0; it < iters; ++it) {
+        for (uint32_t l = 0; l < d.nj_layers; ++l) {
+            const uint32_t b = d.j_layer_off[l], e = d.j_layer_off[l + 1];
+            for (uint32_t t = b + threadIdx.x; t < e; t += blockDim.x) joint_post_one(d, p, d.j_order[t]);
+            __syncthreads();
+        }
+    }
+}
+// update_world_inertia for every alive body (world.h:1313)
+__global__ void k_update_inertia(DevState d) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    uint32_t f = d.b_flags[i];
+    if (!(f & BF_ALIVE)) return;
+    if ((f & BF_TYPE) == 2u) store_iiw(d.b_iiw, i, world_inertia(ldq(d.b_rot[i]), ld3(d.b_iil[i])));
+    else store_iiw(d.b_iiw, i, zero3());
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1417,6 +1632,16 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
         for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) warm_manifold(d, s);
         if (grid_sync(g, err)) return;
     }
+    // joints: warm start, forward layers (solver.h:917)
+    const uint32_t L = d.nj_layers;
+    const float inv_h = inv_step(prm.h);
+    for (uint32_t l = 0; l < L; ++l) {
+        uint32_t jb, je;
+        const uint32_t* jo;
+        joint_layer_range(d, l, false, jb, je, jo);
+        for (uint32_t t = jb + tid; t < je; t += nthreads) joint_warm_one(d, jo[t]);
+        if (grid_sync(g, err)) return;
+    }
     if (prm.profile) {
         // algorithmic bytes of the solve: warm start 192 + 40n per manifold (bodies r/w and the
         // points' positions and impulses), each position pass 92 + 4n; the velocity and relax
@@ -1433,30 +1658,40 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     __shared__ float s_red[kPersistBlock / 64];
     const uint32_t n_active = d.cnt->active;
     if (!cfg.residual_mode) {
-        // velocity then relax passes, colour by colour; the next colour's first slot of this
-        // thread is loaded between arriving at the barrier and waiting on it
-        const uint32_t nph = (cfg.vel_iters + cfg.relax_iters) * C;
+        // velocity then relax passes: each pass is the contact colours, then the joint layers
+        // (solver.h:939-946, joints alternating direction); the next contact colour's first
+        // slot of this thread is loaded between arriving at the barrier and waiting on it
+        const uint32_t per = C + L;
+        const uint32_t nph = (cfg.vel_iters + cfg.relax_iters) * per;
         PreSlot x;
         x.s = kNoSlot;
-        if (nph && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, x);
+        if (nph && C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, x);
         for (uint32_t ph = 0; ph < nph; ++ph) {
-            const uint32_t pass = ph / C, c = ph % C;
+            const uint32_t pass = ph / per, k = ph % per;
             const int use_rest = pass >= cfg.vel_iters ? 1 : 0;
             const uint32_t it = use_rest ? pass - cfg.vel_iters : pass;
-            if (prm.profile) {
-                uint32_t bytes = 0;
-                for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
-                bytes = wave_sum(bytes);
-                if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+            if (k < C) {
+                const uint32_t c = k;
+                if (prm.profile) {
+                    uint32_t bytes = 0;
+                    for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
+                    bytes = wave_sum(bytes);
+                    if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
+                }
+                if (x.s != kNoSlot) velocity_prefetched(d, prm, x, use_rest);
+                for (uint32_t s = co[c] + tid + nthreads; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, use_rest, it);
+            } else {
+                uint32_t jb, je;
+                const uint32_t* jo;
+                joint_layer_range(d, k - C, (it & 1u) != 0u, jb, je, jo);
+                for (uint32_t t = jb + tid; t < je; t += nthreads) joint_velocity_one(d, jo[t], inv_h);
             }
-            if (x.s != kNoSlot) velocity_prefetched(d, prm, x, use_rest);
-            for (uint32_t s = co[c] + tid + nthreads; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, use_rest, it);
             grid_arrive(g);
             x.s = kNoSlot;
             if (ph + 1 < nph) {
-                const uint32_t np = (ph + 1) / C, nc = (ph + 1) % C;
+                const uint32_t np = (ph + 1) / per, nk = (ph + 1) % per;
                 const uint32_t nit = np >= cfg.vel_iters ? np - cfg.vel_iters : np;
-                if (co[nc] + tid < co[nc + 1]) prefetch_slot(d, co[nc] + tid, nit, x);
+                if (nk < C && co[nk] + tid < co[nk + 1]) prefetch_slot(d, co[nk] + tid, nit, x);
             }
             if (grid_wait(g, err)) return;
         }
@@ -1473,6 +1708,13 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
                 if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
             }
             for (uint32_t s = b + tid; s < e; s += nthreads) velocity_manifold(d, prm, s, 0, it);
+            if (grid_sync(g, err)) return;
+        }
+        for (uint32_t l = 0; l < L; ++l) {
+            uint32_t jb, je;
+            const uint32_t* jo;
+            joint_layer_range(d, l, (it & 1u) != 0u, jb, je, jo);
+            for (uint32_t t = jb + tid; t < je; t += nthreads) joint_velocity_one(d, jo[t], inv_h);
             if (grid_sync(g, err)) return;
         }
         ++passes;
@@ -1509,6 +1751,13 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, 1, it);
             if (grid_sync(g, err)) return;
         }
+        for (uint32_t l = 0; l < L; ++l) {
+            uint32_t jb, je;
+            const uint32_t* jo;
+            joint_layer_range(d, l, (it & 1u) != 0u, jb, je, jo);
+            for (uint32_t t = jb + tid; t < je; t += nthreads) joint_velocity_one(d, jo[t], inv_h);
+            if (grid_sync(g, err)) return;
+        }
     }
     // solve_position: snapshot, then position passes
     for (uint32_t i = tid; i < d.nb; i += nthreads) {
@@ -1522,6 +1771,14 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) position_manifold(d, prm, s);
             if (grid_sync(g, err)) return;
         }
+    }
+    // one joint position pass after the contacts' (solver.h:954)
+    for (uint32_t l = 0; l < L; ++l) {
+        uint32_t jb, je;
+        const uint32_t* jo;
+        joint_layer_range(d, l, false, jb, je, jo);
+        for (uint32_t t = jb + tid; t < je; t += nthreads) joint_position_one(d, prm, jo[t]);
+        if (grid_sync(g, err)) return;
     }
 }
 
@@ -1923,9 +2180,11 @@ void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t bound
     if (!d.nb) return;
     hipLaunchKernelGGL(k_uf_init, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     if (bound) hipLaunchKernelGGL(k_uf_union, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
+    if (d.nj) hipLaunchKernelGGL(k_uf_union_joints, dim3(blocks(d.nj)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_uf_roots, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_sleep_timers, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
     if (bound) hipLaunchKernelGGL(k_kin_wake, dim3(blocks(bound)), dim3(kBlock), 0, st, d, p, bound);
+    if (d.nj) hipLaunchKernelGGL(k_kin_wake_joints, dim3(blocks(d.nj)), dim3(kBlock), 0, st, d, p);
     hipLaunchKernelGGL(k_island_min, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
     hipLaunchKernelGGL(k_sleep, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
 }
@@ -1967,6 +2226,24 @@ int max_coresident_blocks(int device) {
 
 void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st) {
     if (d.nb) hipLaunchKernelGGL(k_integrate_velocities, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d, p);
+}
+void launch_wake_jointed(const DevState& d, hipStream_t st) {
+    if (d.nj) hipLaunchKernelGGL(k_wake_jointed, dim3(1), dim3(1024), 0, st, d);
+}
+void launch_joint_warm(const DevState& d, uint32_t begin, uint32_t end, hipStream_t st) {
+    if (end > begin) hipLaunchKernelGGL(k_joint_warm, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, begin, end);
+}
+void launch_joint_velocity(const DevState& d, const StepParams& p, int rev, uint32_t begin, uint32_t end, hipStream_t st) {
+    if (end > begin)
+        hipLaunchKernelGGL(k_joint_velocity, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, p, rev, begin, end);
+}
+void launch_joint_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st) {
+    if (end > begin) hipLaunchKernelGGL(k_joint_position, dim3(blocks(end - begin)), dim3(kBlock), 0, st, d, p, begin, end);
+}
+void launch_joints_post(const DevState& d, const StepParams& p, uint32_t pos_iters, hipStream_t st) {
+    if (!d.nj) return;
+    hipLaunchKernelGGL(k_joints_post, dim3(1), dim3(1024), 0, st, d, p, pos_iters > 1u ? pos_iters : 1u);
+    if (d.nb) hipLaunchKernelGGL(k_update_inertia, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
 }
 
 }  // namespace pxg

@@ -128,6 +128,18 @@ struct DevState {
     uint32_t *uf_parent;
     uint32_t *uf_root;  // each body's island root, after the unions
     uint32_t *isl_min;
+    // joints (dynamics/joint.h): definitions and accumulators (pxg_joints.h layout), the solve
+    // order as layers (no two joints of a layer share a dynamic body), forward then reverse
+    uint32_t nj, nj_layers;
+    uint4 *j_meta;        // type, body a, body b, live (valid and enabled)
+    float4 *j_def;        // 6 per joint
+    float4 *j_acc;        // 2 per joint
+    uint32_t *j_order;    // 2 * nj joint indices: forward layering, then reverse layering
+    uint32_t *j_layer_off;// 2 * (nj_layers + 1) offsets into j_order's two halves
+    // body pairs whose contacts are off (jointed without collide_connected, or ignored),
+    // sorted keys lo << 32 | hi
+    unsigned long long *nc_keys;
+    uint32_t n_nc;
     uint32_t *bar;      // grid barrier of the persistent kernels: arrival count (zeroed per launch)
     Counters *cnt;
     // learned warm start
@@ -170,6 +182,14 @@ void launch_position(const DevState& d, const StepParams& p, uint32_t begin, uin
 void launch_solver_finish(const DevState& d, uint32_t bound, hipStream_t st);
 void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st);
 void launch_integrate_velocities(const DevState& d, const StepParams& p, hipStream_t st);
+// joints: wake_jointed_islands before the solve; the eager path's per-layer launches (layer
+// [begin, end) of the forward order, or of the reverse order when rev); the post-integrate
+// position passes and world inertia update (world.h:1310-1314)
+void launch_wake_jointed(const DevState& d, hipStream_t st);
+void launch_joint_warm(const DevState& d, uint32_t begin, uint32_t end, hipStream_t st);
+void launch_joint_velocity(const DevState& d, const StepParams& p, int rev, uint32_t begin, uint32_t end, hipStream_t st);
+void launch_joint_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st);
+void launch_joints_post(const DevState& d, const StepParams& p, uint32_t pos_iters, hipStream_t st);
 void launch_init_bodies(const DevState& d, hipStream_t st);
 void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st);
 void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st);

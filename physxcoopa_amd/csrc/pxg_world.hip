@@ -61,6 +61,12 @@ struct pxg_world {
     MlpDev* d_mlp = nullptr;
     bool eager = false;         // PXG_EAGER=1: one launch per colour instead of persistent kernels
     std::string coop_note;      // why the persistent kernels were dropped, if they were
+    // joints: the uploaded table (accumulators refreshed on download), the layer offsets of the
+    // forward and reverse orders, and the ignored body pairs (sorted keys)
+    std::vector<pxg_joint> joints;
+    std::vector<uint32_t> jl_fwd, jl_rev;
+    std::vector<uint64_t> ignored;
+    uint32_t cap_j = 0, cap_nc = 0;
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -108,6 +114,15 @@ static void free_all(pxg_world* w) {
     for (void* p : w->allocs) (void)hipFree(p);
     w->allocs.clear();
 }
+// frees one buffer of the world (for buffers that grow)
+static void dfree(pxg_world* w, void* p) {
+    if (!p) return;
+    auto it = std::find(w->allocs.begin(), w->allocs.end(), p);
+    if (it != w->allocs.end()) w->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+static int rebuild_joints(pxg_world* w);
 
 static StepParams params(const pxg_world* w, float h) {
     StepParams p;
@@ -337,6 +352,7 @@ extern "C" int pxg_set_bodies(pxg_world* w, const pxg_body* bodies, uint32_t n) 
         launch_init_bodies(d, s);
     }
     HIPCHK(hipStreamSynchronize(s));  // host staging buffers go out of scope
+    if (!seam && (!w->joints.empty() || !w->ignored.empty())) return rebuild_joints(w);  // body types order the joint layers
     return PXG_OK;
 }
 
@@ -390,6 +406,174 @@ extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n)
     HIPCHK(hipStreamSynchronize(st));
     w->shapes_ready = true;
     return PXG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// joints and ignored pairs
+// ---------------------------------------------------------------------------------------------
+// Layers of one joint pass visiting the table forward (or reversed): a joint goes one layer after
+// the last earlier joint that shares a dynamic body with it, so every body sees its joints in the
+// pass's order. Non-dynamic bodies are never written by a joint row and do not order anything.
+static uint32_t layer_joints(const pxg_world* w, const std::vector<uint8_t>& live, bool rev, std::vector<uint32_t>& order,
+                             std::vector<uint32_t>& off) {
+    const size_t n = w->joints.size();
+    std::vector<uint32_t> last(w->n_bodies, 0), layer(n, 0);
+    uint32_t L = 0;
+    auto dyn = [&](uint32_t b) { return w->body_types[b] == PXG_DYNAMIC; };
+    for (size_t t = 0; t < n; ++t) {
+        size_t j = rev ? n - 1 - t : t;
+        if (!live[j]) continue;
+        uint32_t a = w->joints[j].body_a, b = w->joints[j].body_b, l = 0;
+        if (dyn(a)) l = std::max(l, last[a]);
+        if (dyn(b)) l = std::max(l, last[b]);
+        layer[j] = l;
+        if (dyn(a)) last[a] = l + 1;
+        if (dyn(b)) last[b] = l + 1;
+        L = std::max(L, l + 1);
+    }
+    off.assign(L + 2, 0);
+    for (size_t j = 0; j < n; ++j) if (live[j]) off[layer[j] + 1]++;
+    for (uint32_t l = 0; l < L; ++l) off[l + 1] += off[l];
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    order.assign(n, 0);
+    for (size_t t = 0; t < n; ++t) {
+        size_t j = rev ? n - 1 - t : t;
+        if (live[j]) order[cur[layer[j]]++] = uint32_t(j);
+    }
+    off.resize(L + 1);
+    return L;
+}
+
+// Re-derives everything that depends on the joint table and the body table: live flags, the
+// two layerings and the no-contact pairs; uploads them.
+static int rebuild_joints(pxg_world* w) {
+    DevState& d = w->d;
+    hipStream_t st = w->stream;
+    const uint32_t n = uint32_t(w->joints.size());
+    std::vector<uint8_t> live(n, 0);
+    std::vector<uint64_t> keys(w->ignored);
+    std::vector<uint4> meta(n);
+    for (uint32_t j = 0; j < n; ++j) {
+        const pxg_joint& jt = w->joints[j];
+        bool in_range = jt.body_a < w->n_bodies && jt.body_b < w->n_bodies;
+        live[j] = (jt.valid && jt.enabled && in_range) ? 1 : 0;
+        meta[j] = make_uint4(jt.type, in_range ? jt.body_a : 0u, in_range ? jt.body_b : 0u, live[j]);
+        if (jt.valid && !jt.collide_connected && in_range) {  // joint_blocks_collision_, world.h:1686-1694
+            uint32_t lo = std::min(jt.body_a, jt.body_b), hi = std::max(jt.body_a, jt.body_b);
+            keys.push_back((uint64_t(lo) << 32) | hi);
+        }
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    std::vector<uint32_t> of, orv;
+    uint32_t Lf = layer_joints(w, live, false, of, w->jl_fwd);
+    uint32_t Lr = layer_joints(w, live, true, orv, w->jl_rev);
+    uint32_t L = std::max(Lf, Lr);
+    w->jl_fwd.resize(L + 1, w->jl_fwd.back());
+    w->jl_rev.resize(L + 1, w->jl_rev.back());
+    if (n > w->cap_j) {
+        dfree(w, d.j_meta); dfree(w, d.j_def); dfree(w, d.j_acc); dfree(w, d.j_order); dfree(w, d.j_layer_off);
+        d.j_meta = nullptr; d.j_def = nullptr; d.j_acc = nullptr; d.j_order = nullptr; d.j_layer_off = nullptr;
+        uint32_t cap = std::max(n, 2 * w->cap_j);
+        DALLOC(d.j_meta, cap); DALLOC(d.j_def, 6 * size_t(cap)); DALLOC(d.j_acc, 2 * size_t(cap));
+        DALLOC(d.j_order, 2 * size_t(cap)); DALLOC(d.j_layer_off, 2 * (size_t(cap) + 1));
+        w->cap_j = cap;
+    }
+    if (keys.size() > w->cap_nc) {
+        dfree(w, d.nc_keys);
+        d.nc_keys = nullptr;
+        uint32_t cap = std::max<uint32_t>(uint32_t(keys.size()), 2 * w->cap_nc);
+        DALLOC(d.nc_keys, cap);
+        w->cap_nc = cap;
+    }
+    std::vector<uint32_t> ord(2 * size_t(n)), offs(2 * (size_t(L) + 1));
+    std::copy(of.begin(), of.end(), ord.begin());
+    std::copy(orv.begin(), orv.end(), ord.begin() + n);
+    std::copy(w->jl_fwd.begin(), w->jl_fwd.end(), offs.begin());
+    std::copy(w->jl_rev.begin(), w->jl_rev.end(), offs.begin() + L + 1);
+    if (n) {
+        HIPCHK(hipMemcpyAsync(d.j_meta, meta.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.j_order, ord.data(), ord.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(d.j_layer_off, offs.data(), offs.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
+    if (!keys.empty())
+        HIPCHK(hipMemcpyAsync(d.nc_keys, keys.data(), keys.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    d.nj = n;
+    d.nj_layers = L;
+    d.n_nc = uint32_t(keys.size());
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_joints(pxg_world* w, const pxg_joint* joints, uint32_t n) {
+    if (!w || (!joints && n)) return fail(PXG_ERR_INVALID, "pxg_set_joints: null argument");
+    for (uint32_t j = 0; j < n; ++j) {
+        const pxg_joint& jt = joints[j];
+        if (jt.type > PXG_CONE_TWIST) return fail(PXG_ERR_INVALID, "pxg_set_joints: bad joint type");
+        if (jt.body_a >= w->n_bodies || jt.body_b >= w->n_bodies)
+            return fail(PXG_ERR_INVALID, "pxg_set_joints: body index beyond the body table");
+        if (jt.body_a == jt.body_b) return fail(PXG_ERR_INVALID, "pxg_set_joints: a joint needs two different bodies");
+    }
+    w->joints.assign(joints, joints + n);
+    int r = rebuild_joints(w);
+    if (r != PXG_OK) return r;
+    if (!n) return PXG_OK;
+    std::vector<float4> def(6 * size_t(n)), acc(2 * size_t(n));
+    for (uint32_t j = 0; j < n; ++j) {
+        const pxg_joint& jt = joints[j];
+        uint32_t flags = 0;
+        if (jt.type == PXG_HINGE && jt.use_limits) flags |= 1u;                      // JF_LIMITS
+        if (jt.type == PXG_CONE_TWIST && jt.swing_limit >= 0.0f) flags |= 2u;        // has_swing_limit()
+        if (jt.type == PXG_CONE_TWIST && jt.twist_min <= jt.twist_max) flags |= 4u;  // has_twist_limit()
+        float tf, ff;
+        uint32_t ty = jt.type;
+        std::memcpy(&tf, &ty, 4);
+        std::memcpy(&ff, &flags, 4);
+        const float* aa = jt.local_anchor_a; const float* ab = jt.local_anchor_b;
+        const float* xa = jt.local_axis_a; const float* xb = jt.local_axis_b; const float* q = jt.rest_relative_rotation;
+        def[6 * j + 0] = make_float4(aa[0], aa[1], aa[2], jt.min_angle);
+        def[6 * j + 1] = make_float4(ab[0], ab[1], ab[2], jt.max_angle);
+        def[6 * j + 2] = make_float4(xa[0], xa[1], xa[2], jt.swing_limit);
+        def[6 * j + 3] = make_float4(xb[0], xb[1], xb[2], jt.twist_min);
+        def[6 * j + 4] = make_float4(q[0], q[1], q[2], q[3]);
+        def[6 * j + 5] = make_float4(jt.twist_max, tf, ff, 0.0f);
+        acc[2 * j + 0] = make_float4(jt.point_impulse[0], jt.point_impulse[1], jt.point_impulse[2], jt.limit_impulse);
+        acc[2 * j + 1] = make_float4(jt.axis_impulse[0], jt.axis_impulse[1], jt.swing_impulse, 0.0f);
+    }
+    DevState& d = w->d;
+    HIPCHK(hipMemcpyAsync(d.j_def, def.data(), def.size() * sizeof(float4), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipMemcpyAsync(d.j_acc, acc.data(), acc.size() * sizeof(float4), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_joints(pxg_world* w, pxg_joint* out, uint32_t n) {
+    if (!w || (!out && n)) return fail(PXG_ERR_INVALID, "pxg_get_joints: null argument");
+    if (n > w->joints.size()) return fail(PXG_ERR_INVALID, "pxg_get_joints: more joints requested than the table holds");
+    if (!n) return PXG_OK;
+    std::vector<float4> acc(2 * size_t(n));
+    HIPCHK(hipMemcpyAsync(acc.data(), w->d.j_acc, acc.size() * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (uint32_t j = 0; j < n; ++j) {
+        pxg_joint& jt = w->joints[j];
+        jt.point_impulse[0] = acc[2 * j].x; jt.point_impulse[1] = acc[2 * j].y; jt.point_impulse[2] = acc[2 * j].z;
+        jt.limit_impulse = acc[2 * j].w;
+        jt.axis_impulse[0] = acc[2 * j + 1].x; jt.axis_impulse[1] = acc[2 * j + 1].y;
+        jt.swing_impulse = acc[2 * j + 1].z;
+        out[j] = jt;
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_ignored_pairs(pxg_world* w, const uint32_t* ab, uint32_t n) {
+    if (!w || (!ab && n)) return fail(PXG_ERR_INVALID, "pxg_set_ignored_pairs: null argument");
+    w->ignored.clear();
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t a = ab[2 * i], b = ab[2 * i + 1];
+        if (a == b) continue;  // set_pair_collision ignores a == b (world.h:528)
+        w->ignored.push_back((uint64_t(std::min(a, b)) << 32) | std::max(a, b));
+    }
+    return rebuild_joints(w);
 }
 
 extern "C" int pxg_set_layer_masks(pxg_world* w, const uint32_t masks[32]) {
@@ -529,11 +713,19 @@ static int solve_eager(pxg_world* w, const StepParams& p, uint32_t bound, bool s
         if (d.mlp && !d.record) launch_apply_predictions(d, 4 * active, st);
     }
     uint32_t passes = 0;
+    const uint32_t L = d.nj_layers;
+    auto joint_pass = [&](uint32_t it) {
+        const bool rev = (it & 1u) != 0u;
+        const std::vector<uint32_t>& off = rev ? w->jl_rev : w->jl_fwd;
+        for (uint32_t l = 0; l < L; ++l) launch_joint_velocity(d, p, rev ? 1 : 0, off[l], off[l + 1], st);
+    };
     for (uint32_t k = 0; k < C; ++k) launch_warm_apply(d, co[k], co[k + 1], st);
+    for (uint32_t l = 0; l < L; ++l) launch_joint_warm(d, w->jl_fwd[l], w->jl_fwd[l + 1], st);
     if (c.residual_mode) {
         uint32_t maxit = c.residual_max_iterations ? c.residual_max_iterations : 256;
         while (passes < maxit) {
             for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, passes);
+            joint_pass(passes);
             ++passes;
             launch_residual(d, p, active, 0, st);
             HIPCHK(hipMemcpyAsync(&w->h_cnt->residual_bits, &d.cnt->residual_bits, 4, hipMemcpyDeviceToHost, st));
@@ -544,14 +736,19 @@ static int solve_eager(pxg_world* w, const StepParams& p, uint32_t bound, bool s
         }
         launch_set_passes(d, passes, st);
     } else {
-        for (uint32_t it = 0; it < c.velocity_iterations; ++it)
+        for (uint32_t it = 0; it < c.velocity_iterations; ++it) {
             for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, it);
+            joint_pass(it);
+        }
     }
-    for (uint32_t it = 0; it < c.relax_iterations; ++it)
+    for (uint32_t it = 0; it < c.relax_iterations; ++it) {
         for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 1, it);
+        joint_pass(it);
+    }
     launch_position_setup(d, st);
     for (uint32_t it = 0; it < c.position_iterations; ++it)
         for (uint32_t k = 0; k < C; ++k) launch_position(d, p, co[k], co[k + 1], st);
+    for (uint32_t l = 0; l < L; ++l) launch_joint_position(d, p, w->jl_fwd[l], w->jl_fwd[l + 1], st);
     return PXG_OK;
 }
 
@@ -566,6 +763,7 @@ static int phase_solve(pxg_world* w, float h) {
     uint32_t P = w->last_pairs;
     uint32_t bound = P ? P : 1;
     d.cur ^= 1;
+    launch_wake_jointed(d, st);  // solve() starts with wake_jointed_islands (solver.h:916)
     launch_colour_init(d, bound, st);
     w->stats.pairs = P;
     w->stats.steps += 1;
@@ -625,6 +823,7 @@ static int phase_solve(pxg_world* w, float h) {
     launch_solver_finish(d, bound, st);
     launch_islands_sleep(d, p, bound, st);
     launch_integrate_velocities(d, p, st);
+    launch_joints_post(d, p, c.position_iterations, st);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(PXG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     w->results_pending = true;

@@ -383,3 +383,135 @@ def mixed_pile_scene(n=200, seed=7, spread=4.0):
         fr, re = float(F(rng.uniform(0.3, 0.8))), float(F(rng.uniform(0.0, 0.2)))
         _add_pile_body(sb, kind, dims, pos, quat, fr, re)
     return sb.arrays()
+
+
+# ---- ragdolls (joints) ----
+# A 15-bone capsule ragdoll standing along +Z, pelvis centre at the origin of its own frame.
+# (name, capsule axis, radius, half height, centre, mass)
+RAGDOLL_BONES = [
+    ("pelvis", 0, 0.12, 0.08, (0.0, 0.0, 0.0), 9.0),
+    ("abdomen", 2, 0.11, 0.04, (0.0, 0.0, 0.22), 8.0),
+    ("chest", 0, 0.13, 0.08, (0.0, 0.0, 0.46), 12.0),
+    ("neck", 2, 0.08, 0.02, (0.0, 0.0, 0.66), 1.5),
+    ("head", 2, 0.10, 0.04, (0.0, 0.0, 0.86), 6.5),
+    ("upper_arm_l", 0, 0.08, 0.11, (0.42, 0.0, 0.46), 2.5),
+    ("forearm_l", 0, 0.08, 0.10, (0.76, 0.0, 0.46), 1.8),
+    ("upper_arm_r", 0, 0.08, 0.11, (-0.42, 0.0, 0.46), 2.5),
+    ("forearm_r", 0, 0.08, 0.10, (-0.76, 0.0, 0.46), 1.8),
+    ("thigh_l", 2, 0.10, 0.14, (0.14, 0.0, -0.36), 7.0),
+    ("shin_l", 2, 0.08, 0.15, (0.14, 0.0, -0.82), 4.0),
+    ("foot_l", 1, 0.08, 0.04, (0.14, 0.06, -1.10), 1.2),
+    ("thigh_r", 2, 0.10, 0.14, (-0.14, 0.0, -0.36), 7.0),
+    ("shin_r", 2, 0.08, 0.15, (-0.14, 0.0, -0.82), 4.0),
+    ("foot_r", 1, 0.08, 0.04, (-0.14, 0.06, -1.10), 1.2),
+]
+# (kind, parent, child, junction point, axis (hinge axis / twist axis), limits)
+#   cone: (swing, twist_min, twist_max) in degrees; hinge: (min, max) in degrees
+RAGDOLL_JOINTS = [
+    ("cone", "pelvis", "abdomen", (0.0, 0.0, 0.11), (0, 0, 1), (30.0, -20.0, 20.0)),
+    ("cone", "abdomen", "chest", (0.0, 0.0, 0.33), (0, 0, 1), (25.0, -15.0, 15.0)),
+    ("cone", "chest", "neck", (0.0, 0.0, 0.58), (0, 0, 1), (40.0, -40.0, 40.0)),
+    ("hinge", "neck", "head", (0.0, 0.0, 0.74), (1, 0, 0), (-30.0, 40.0)),
+    ("cone", "chest", "upper_arm_l", (0.25, 0.0, 0.46), (1, 0, 0), (70.0, -45.0, 45.0)),
+    ("hinge", "upper_arm_l", "forearm_l", (0.59, 0.0, 0.46), (0, 0, 1), (0.0, 140.0)),
+    ("cone", "chest", "upper_arm_r", (-0.25, 0.0, 0.46), (-1, 0, 0), (70.0, -45.0, 45.0)),
+    ("hinge", "upper_arm_r", "forearm_r", (-0.59, 0.0, 0.46), (0, 0, 1), (-140.0, 0.0)),
+    ("cone", "pelvis", "thigh_l", (0.14, 0.0, -0.14), (0, 0, -1), (60.0, -30.0, 30.0)),
+    ("hinge", "thigh_l", "shin_l", (0.14, 0.0, -0.59), (1, 0, 0), (-140.0, 0.0)),
+    ("hinge", "shin_l", "foot_l", (0.14, 0.0, -1.03), (1, 0, 0), (-30.0, 30.0)),
+    ("cone", "pelvis", "thigh_r", (-0.14, 0.0, -0.14), (0, 0, -1), (60.0, -30.0, 30.0)),
+    ("hinge", "thigh_r", "shin_r", (-0.14, 0.0, -0.59), (1, 0, 0), (-140.0, 0.0)),
+    ("hinge", "shin_r", "foot_r", (-0.14, 0.0, -1.03), (1, 0, 0), (-30.0, 30.0)),
+]
+
+
+def _seg_dist(p1, q1, p2, q2):
+    """Distance between segments p1q1 and p2q2 (float64, for the rest-pose overlap test)."""
+    d1, d2, r = q1 - p1, q2 - p2, p1 - p2
+    a, e, f = d1 @ d1, d2 @ d2, d2 @ r
+    c = d1 @ r
+    b = d1 @ d2
+    den = a * e - b * b
+    s = np.clip((b * f - c * e) / den, 0.0, 1.0) if den > 1e-12 else 0.0
+    t = (b * s + f) / e if e > 1e-12 else 0.0
+    if t < 0.0:
+        t, s = 0.0, np.clip(-c / a, 0.0, 1.0)
+    elif t > 1.0:
+        t, s = 1.0, np.clip((b - c) / a, 0.0, 1.0)
+    return float(np.linalg.norm((p1 + d1 * s) - (p2 + d2 * t)))
+
+
+def add_ragdoll(sb, origin, orientation, rng=None, friction=0.6):
+    """Adds a 15-bone ragdoll (70 kg) at `origin` rotated by `orientation` (x, y, z, w). Hinge limits
+    are jittered by up to 10 degrees with `rng`. Returns (joint records, ignored body pairs): every
+    joint has collide_connected off, and bones that overlap at rest without sharing a joint are
+    returned as pairs for set_pair_collision(a, b, false) (world.h:527-532)."""
+    from . import joints as J
+    q = np.array(orientation, np.float64)
+    idx = {}
+    segs = {}
+    for name, axis, r, hh, c, m in RAGDOLL_BONES:
+        c = np.array(c, np.float64)
+        pos = np.asarray(origin, np.float64) + J.quat_rotate(q, c).astype(np.float64)
+        idx[name] = sb.dynamic_capsule(tuple(float(F(x)) for x in pos), r, hh, axis=axis,
+                                       orientation=tuple(float(F(x)) for x in q), mass=m, friction=friction)
+        e = np.zeros(3)
+        e[axis] = hh
+        segs[name] = (c - e, c + e, r)
+    b, _ = sb.arrays()
+    recs = []
+    centre = {name: np.array(c, np.float64) for name, _, _, _, c, _ in RAGDOLL_BONES}
+    jitter = (lambda: float(rng.uniform(-10.0, 10.0))) if rng is not None else (lambda: 0.0)
+    for kind, pa, ch, jp, ax, lim in RAGDOLL_JOINTS:
+        jp = np.array(jp, np.float64)
+        la = tuple(float(F(x)) for x in jp - centre[pa])
+        lb = tuple(float(F(x)) for x in jp - centre[ch])
+        if kind == "cone":
+            sw, tmin, tmax = lim
+            recs.append(J.cone_twist(b, idx[pa], idx[ch], la, lb, ax, F(math.radians(sw)), F(math.radians(tmin)),
+                                     F(math.radians(tmax))))
+        else:
+            lo, hi = lim
+            j = jitter()
+            recs.append(J.hinge(b, idx[pa], idx[ch], la, lb, ax, ax, True, F(math.radians(lo + min(j, 0.0))),
+                                F(math.radians(hi + max(j, 0.0)))))
+    jointed = {(min(idx[p], idx[c]), max(idx[p], idx[c])) for _, p, c, _, _, _ in RAGDOLL_JOINTS}
+    ignored = []
+    names = [n for n, *_ in RAGDOLL_BONES]
+    for i, na in enumerate(names):
+        for nb in names[i + 1:]:
+            key = (min(idx[na], idx[nb]), max(idx[na], idx[nb]))
+            if key in jointed:
+                continue
+            (p1, q1, r1), (p2, q2, r2) = segs[na], segs[nb]
+            if _seg_dist(p1, q1, p2, q2) < r1 + r2 + 0.05:
+                ignored.append(key)
+    return recs, ignored
+
+
+def ragdoll_scene(seed=5, n_ragdolls=1, n_boxes=32, spread=3.0):
+    """The ragdoll scene: `n_ragdolls` 15-bone capsule ragdolls (70 kg each; cone-twist at the
+    spine, neck, shoulders and hips, hinges at the head, elbows, knees and ankles with jittered
+    limits) dropped from 2 m with random orientations onto `n_boxes` random boxes (half extents
+    U[0.1, 0.4] m) over a 10 x 10 x 1 m ground box. dt 1/120, 12 velocity iterations.
+    Returns (Scene, joint records, ignored pairs)."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    sb.static_box((0, 0, -0.5), (5, 5, 0.5), friction=0.6)
+    for _ in range(n_boxes):
+        he = tuple(float(F(x)) for x in rng.uniform(0.1, 0.4, 3))
+        pos = (float(F(rng.uniform(-4, 4))), float(F(rng.uniform(-4, 4))), float(F(he[2] + 0.001)))
+        yaw = quat_axis_angle((0.0, 0.0, 1.0), float(rng.uniform(0, 2 * math.pi)))
+        mass = float(F(8.0 * he[0] * he[1] * he[2] * 500.0))
+        sb.dynamic_box(pos, he, orientation=tuple(float(F(x)) for x in yaw), mass=mass, friction=0.6)
+    recs, ignored = [], []
+    for k in range(n_ragdolls):
+        ang = 2 * math.pi * k / max(1, n_ragdolls)
+        rad = 0.0 if n_ragdolls == 1 else spread
+        origin = (rad * math.cos(ang), rad * math.sin(ang), 2.0 + 1.2)
+        r, ig = add_ragdoll(sb, origin, random_quaternion(rng), rng)
+        recs.extend(r)
+        ignored.extend(ig)
+    b, s = sb.arrays()
+    cfg = dict(fixed_dt=float(F(1.0) / F(120.0)), velocity_iterations=12)
+    return Scene("ragdolls_%d" % n_ragdolls, b, s, cfg, steps=600), recs, ignored

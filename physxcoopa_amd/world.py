@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, Config, Stats, Profile, ptr
+from ._abi import BODY_DTYPE, JOINT_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, Config, Stats, Profile, ptr
 from ._lib import lib, check
 
 
@@ -55,6 +55,21 @@ class World:
     def set_layer_masks(self, masks):
         m = np.ascontiguousarray(np.asarray(masks, np.uint32))
         check(lib().pxg_set_layer_masks(self.h, ptr(m)), "pxg_set_layer_masks")
+
+    def set_joints(self, joints):
+        """Replaces the joint table (pxg_joint records, see physxcoopa_amd.joints)."""
+        j = np.ascontiguousarray(joints, dtype=JOINT_DTYPE)
+        self.n_joints = len(j)
+        check(lib().pxg_set_joints(self.h, ptr(j), len(j)), "pxg_set_joints")
+
+    def joints(self):
+        out = np.zeros(getattr(self, "n_joints", 0), JOINT_DTYPE)
+        check(lib().pxg_get_joints(self.h, ptr(out), len(out)), "pxg_get_joints")
+        return out
+
+    def set_ignored_pairs(self, ab):
+        a = np.ascontiguousarray(np.asarray(ab, np.uint32).reshape(-1, 2))
+        check(lib().pxg_set_ignored_pairs(self.h, ptr(a), len(a)), "pxg_set_ignored_pairs")
 
     def step_fixed(self, h=None):
         check(lib().pxg_step_fixed(self.h, np.float32(self._cfg.fixed_dt if h is None else h)), "pxg_step_fixed")

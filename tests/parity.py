@@ -76,22 +76,41 @@ def compare_step(gw, ow, pre_bodies, report):
         report["max_body_diff"] = max(report["max_body_diff"], float(d))
     assert np.array_equal(gb["awake"], ob["awake"]), "sleep flags differ"
     report["bitexact_bodies"] += int(np.array_equal(gb.view(np.uint8), ob.view(np.uint8)))
+    if getattr(gw, "n_joints", 0):
+        gj, oj = gw.joints(), ow.joints()
+        for f in ("point_impulse", "axis_impulse", "limit_impulse", "swing_impulse"):
+            gv, ov = gj[f].astype(np.float64), oj[f].astype(np.float64)
+            assert np.all(np.abs(gv - ov) <= TOL_IMP_REL * np.abs(ov) + TOL_IMP_ABS * 10), "joint %s differs" % f
+        report["bitexact_joints"] += int(np.array_equal(gj.view(np.uint8), oj.view(np.uint8)))
     report["steps"] += 1
     report["colours"] = max(report["colours"], len(offs) - 1)
 
 
-def lockstep(World, OracleWorld, cfg, bodies, shapes, steps, resync=True, feed_order=True):
+def lockstep(World, OracleWorld, cfg, bodies, shapes, steps, resync=True, feed_order=True, joints=None, ignored=None,
+             rounded_atan2=True):
     """feed_order: give the oracle the GPU's solve order each step; without it the oracle solves in
-    its own (the reference's) manifold order."""
+    its own (the reference's) manifold order. joints / ignored: a joint table and ignored body
+    pairs for both sides. rounded_atan2: the oracle evaluates atan2 in double and rounds once, as
+    the GPU does (False: the reference's float std::atan2, one ulp apart at most)."""
     gw = World(cfg, bodies, shapes)
     ow = OracleWorld(cfg, bodies, shapes)
-    report = dict(steps=0, manifolds=0, bitexact_manifolds=0, bitexact_bodies=0, max_body_diff=0.0, colours=0)
+    ow.set_atan2_mode(rounded_atan2)
+    if joints is not None:
+        gw.set_joints(joints)
+        ow.set_joints(joints)
+    if ignored is not None:
+        gw.set_ignored_pairs(ignored)
+        ow.set_ignored_pairs(ignored)
+    report = dict(steps=0, manifolds=0, bitexact_manifolds=0, bitexact_bodies=0, bitexact_joints=0, max_body_diff=0.0,
+                  colours=0)
     for _ in range(steps):
         if resync:
             ow.set_bodies(gw.bodies())
             ow.set_fat_aabbs(gw.fat_aabbs())
             k, i = gw.warm_cache()
             ow.set_warm_cache(k, i)
+            if joints is not None:
+                ow.set_joints(gw.joints())
         pre = gw.bodies()
         gw.step_fixed()
         gm = gw.manifolds()

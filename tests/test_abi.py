@@ -46,8 +46,8 @@ def test_struct_sizes_match_the_header():
 #include <stddef.h>
 #include "pxg.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(pxg_body), sizeof(pxg_shape), sizeof(pxg_contact),
-         sizeof(pxg_manifold), sizeof(pxg_config), sizeof(pxg_stats), sizeof(pxg_profile));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(pxg_body), sizeof(pxg_shape), sizeof(pxg_contact),
+         sizeof(pxg_manifold), sizeof(pxg_config), sizeof(pxg_stats), sizeof(pxg_profile), sizeof(pxg_joint));
   return 0;
 }
 """
@@ -60,7 +60,7 @@ int main(void) {
     sizes = [int(x) for x in subprocess.check_output([exe]).split()]
     assert sizes == [_abi.BODY_DTYPE.itemsize, _abi.SHAPE_DTYPE.itemsize, _abi.CONTACT_DTYPE.itemsize,
                      _abi.MANIFOLD_DTYPE.itemsize, ctypes.sizeof(_abi.Config), ctypes.sizeof(_abi.Stats),
-                     ctypes.sizeof(_abi.Profile)]
+                     ctypes.sizeof(_abi.Profile), _abi.JOINT_DTYPE.itemsize]
 
 
 def test_config_default_matches_reference_defaults():
