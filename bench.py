@@ -42,8 +42,9 @@ def calibrate_copy():
 
 
 def bench_batch(args):
-    """4096 independent scenes (half box pyramids, half random drops of the pile's shape mix) of
-    8-256 bodies each, stepped K at a time in one graph-captured persistent launch."""
+    """Batched scenes stepped K at a time in one graph-captured persistent launch. batch: 4096
+    independent scenes (half box pyramids, half random drops of the pile's shape mix) of 8-256
+    bodies; ragdolls: 2048 scenes of one 15-bone ragdoll over 32 boxes (joints)."""
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
@@ -51,12 +52,24 @@ def bench_batch(args):
     from physxcoopa_amd import scenes, _abi, autotune
     from physxcoopa_amd.batch import Batch
     t0 = time.time()
-    b, s, bo, so, settings = scenes.config3_scenes(seed=3, n_scenes=4096, n_min=8, n_max=256)
+    jt = jo = ig = io = None
+    if args.workload == "ragdolls":
+        b, s, bo, so, jt, jo, ig, io, settings = scenes.config5_scenes(seed=5, n_scenes=2048)
+        name = "config5_ragdolls_2048"
+    else:
+        b, s, bo, so, settings = scenes.config3_scenes(seed=3, n_scenes=4096, n_min=8, n_max=256)
+        name = "config3_batch_4096"
     cfg = _abi.default_config(**settings)
     n_dyn = int(np.sum(b["type"] == 2))
     log("batch built in %.1fs: %d scenes, %d dynamic bodies" % (time.time() - t0, len(bo) - 1, n_dyn))
-    (block, lpt), timings = autotune.tune(cfg, b, s, bo, so, steps=8, use_cache=False, log=log)
+
+    def setup(batch):
+        if jt is not None:
+            batch.set_joints(jo, jt)
+            batch.set_ignored_pairs(io, ig)
+    (block, lpt), timings = autotune.tune(cfg, b, s, bo, so, steps=8, use_cache=False, log=log, setup=setup)
     bt = Batch(cfg, b, s, bo, so)
+    setup(bt)
     bt.set_tuning(block, lpt)
     chunk = 30
     done = 0
@@ -84,9 +97,13 @@ def bench_batch(args):
         bodies = bt.bodies()
         rng = np.random.default_rng(0)
         t_cpu, bs_cpu = 0.0, 0
+        gj = bt.joints() if jt is not None else None
         for k in rng.permutation(len(bo) - 1):
             sb, ss = bodies[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]]
             ow = pxo.OracleWorld(cfg, sb, ss)
+            if gj is not None:
+                ow.set_joints(gj[jo[k]:jo[k + 1]])
+                ow.set_ignored_pairs(ig[io[k]:io[k + 1]])
             t_cpu += ow.run(10)
             bs_cpu += int(np.sum(sb["type"] == 2)) * 10
             if t_cpu > args.cpu_seconds:
@@ -99,7 +116,7 @@ def bench_batch(args):
         "value": round(value, 1), "unit": "body·steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1000.0 * elapsed / args.steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "config3_batch_4096", "scenes": len(bo) - 1, "dynamic_bodies": n_dyn,
+        "config": {"workload": name, "scenes": len(bo) - 1, "dynamic_bodies": n_dyn, "joints": 0 if jt is None else len(jt),
                    "velocity_iterations": int(cfg.velocity_iterations), "position_iterations": int(cfg.position_iterations),
                    "block": block, "longest_first": lpt, "steps_per_launch": chunk},
         "autotune_seconds": timings,
@@ -121,8 +138,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", choices=["reference", "coloured"], default="reference",
                     help="solve order: the reference's manifold order (default) or the greedy colouring")
-    ap.add_argument("--workload", choices=["pile", "batch"], default="pile",
-                    help="pile: the 16k-body pile (one world); batch: 4096 independent scenes of 8-256 bodies")
+    ap.add_argument("--workload", choices=["pile", "batch", "ragdolls"], default="pile",
+                    help="pile: the 16k-body pile (one world); batch: 4096 independent scenes of 8-256 bodies; "
+                         "ragdolls: 2048 scenes of a jointed ragdoll over 32 boxes")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB ten times (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -130,7 +148,7 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.workload == "batch":
+    if args.workload in ("batch", "ragdolls"):
         bench_batch(args)
         return
 

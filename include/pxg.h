@@ -372,6 +372,13 @@ int pxg_batch_get_stats(pxg_batch* b, pxg_batch_scene_stats* out, uint32_t n);
 int pxg_batch_get_manifolds(pxg_batch* b, uint32_t scene, pxg_manifold* out, uint32_t cap, uint32_t* n);
 int pxg_batch_get_solve_order(pxg_batch* b, uint32_t scene, uint32_t* order, uint32_t cap, uint32_t* n,
                               uint32_t* colour_offsets, uint32_t colour_cap, uint32_t* n_colours);
+/* Joint tables of every scene: scene k owns joints[joint_offsets[k] .. joint_offsets[k+1]) with
+ * scene-local body indices (as pxg_set_joints for one world, world.h:400-518). */
+int pxg_batch_set_joints(pxg_batch* b, const uint32_t* joint_offsets, const pxg_joint* joints);
+int pxg_batch_get_joints(pxg_batch* b, pxg_joint* out, uint32_t n);
+/* Ignored body pairs of every scene (set_pair_collision(a, b, false), world.h:527-532): scene k
+ * owns the pairs ab[2 pair_offsets[k] .. 2 pair_offsets[k+1]), scene-local. */
+int pxg_batch_set_ignored_pairs(pxg_batch* b, const uint32_t* pair_offsets, const uint32_t* ab);
 const char* pxg_batch_last_error(void);
 
 #ifdef __cplusplus

@@ -21,6 +21,9 @@ _SIGNATURES = {
     "pxg_set_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_ignored_pairs": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_batch_set_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_get_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_batch_set_ignored_pairs": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_step_fixed": ([ctypes.c_void_p, ctypes.c_float], ctypes.c_int),
     "pxg_step_phase": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float], ctypes.c_int),
     "pxg_step": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p], ctypes.c_int),

@@ -34,9 +34,10 @@ def _save(d):
         pass
 
 
-def tune(config, bodies, shapes, body_offsets, shape_offsets, steps=8, use_cache=True, log=None):
-    """Returns ((block_size, longest_first), {candidate: seconds}) for this batch shape."""
-    key = shape_key(len(body_offsets) - 1, len(bodies), config)
+def tune(config, bodies, shapes, body_offsets, shape_offsets, steps=8, use_cache=True, log=None, setup=None):
+    """Returns ((block_size, longest_first), {candidate: seconds}) for this batch shape. `setup`,
+    if given, is called on each candidate batch before timing (joints, ignored pairs, network)."""
+    key = shape_key(len(body_offsets) - 1, len(bodies), config) + (":setup" if setup else "")
     cache = _load() if use_cache else {}
     if key in cache:
         c = cache[key]
@@ -44,6 +45,8 @@ def tune(config, bodies, shapes, body_offsets, shape_offsets, steps=8, use_cache
     timings = {}
     for block, lpt in CANDIDATES:
         b = Batch(config, bodies, shapes, body_offsets, shape_offsets)
+        if setup:
+            setup(b)
         b.set_tuning(block, lpt)
         b.step(1)          # graph capture + first touch
         b.synchronize()

@@ -7,7 +7,7 @@ import ctypes
 
 import numpy as np
 
-from ._abi import BODY_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, BATCH_STATS_DTYPE, ptr
+from ._abi import BODY_DTYPE, JOINT_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, BATCH_STATS_DTYPE, ptr
 from ._lib import lib, check_batch
 
 
@@ -39,6 +39,23 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def set_joints(self, joint_offsets, joints):
+        """Scene k owns joints[joint_offsets[k]:joint_offsets[k+1]] (scene-local body indices)."""
+        self.joint_offsets = np.ascontiguousarray(np.asarray(joint_offsets, np.uint32))
+        j = np.ascontiguousarray(joints, dtype=JOINT_DTYPE)
+        self.n_joints = len(j)
+        check_batch(lib().pxg_batch_set_joints(self.h, ptr(self.joint_offsets), ptr(j)), "pxg_batch_set_joints")
+
+    def joints(self):
+        out = np.zeros(getattr(self, "n_joints", 0), JOINT_DTYPE)
+        check_batch(lib().pxg_batch_get_joints(self.h, ptr(out), len(out)), "pxg_batch_get_joints")
+        return out
+
+    def set_ignored_pairs(self, pair_offsets, ab):
+        o = np.ascontiguousarray(np.asarray(pair_offsets, np.uint32))
+        a = np.ascontiguousarray(np.asarray(ab, np.uint32).reshape(-1, 2))
+        check_batch(lib().pxg_batch_set_ignored_pairs(self.h, ptr(o), ptr(a)), "pxg_batch_set_ignored_pairs")
 
     def set_tuning(self, block_size, longest_first=True):
         check_batch(lib().pxg_batch_set_tuning(self.h, block_size, 1 if longest_first else 0), "pxg_batch_set_tuning")

@@ -14,6 +14,7 @@
 // cache rebuild, islands and sleep, integration.
 #include "../../include/pxg.h"
 #include "pxg_device.h"
+#include "pxg_joints.h"
 #include "pxg_state.h"
 
 #include <hip/hip_runtime.h>
@@ -40,6 +41,9 @@ struct Scene {
     uint32_t body_off, n_bodies, shape_off, n_shapes;
     uint32_t man_off, man_cap;     // pair-indexed manifold region
     uint32_t cache_off, cache_mask;
+    uint32_t j_off, n_j;           // joints [j_off, j_off + n_j) (scene-local body indices)
+    uint32_t j_layers, jl_off;     // layers of one joint pass; offsets at j_loff[jl_off .. + 2 (L + 1))
+    uint32_t nc_off, n_nc;         // sorted body pairs without contacts, at nc_keys[nc_off ..]
 };
 
 struct SceneStats {
@@ -79,6 +83,14 @@ struct BatchDev {
     float4* c_val;
     const uint32_t* gen0;         // generation of the first step of this launch (device)
     const uint32_t* layer_mask;   // 32 rows
+    // joints (pxg_joints.h records; type, a, b, live in j_meta); each scene's solve order is its
+    // joints in layers, forward (j_order[2 j_off ..]) and reverse (j_order[2 j_off + n_j ..])
+    const uint4* j_meta;
+    const float4* j_def;
+    float4* j_acc;
+    const uint16_t* j_order;      // scene-local joint indices
+    const uint32_t* j_loff;
+    const uint32_t* nc_keys;      // lo << 16 | hi
 };
 
 struct Cfg {
@@ -169,6 +181,83 @@ __device__ __forceinline__ void lb_store(Lds& L, uint32_t i, const LB& b) {
 __device__ __forceinline__ void lapply(LB& a, LB& b, v3 p, v3 J) {
     if (a.act) { a.v = a.v - J * a.im; a.w = a.w - mul(a.I, cross(p - a.x, J)); }
     if (b.act) { b.v = b.v + J * b.im; b.w = b.w + mul(b.I, cross(p - b.x, J)); }
+}
+
+// joint rows on the scene's LDS bodies
+__device__ __forceinline__ JBody jb_lds(const Lds& L, uint32_t i) {
+    JBody b;
+    b.act = active_flag(L.flags[i]);
+    float4 v = L.vel[i], w = L.angv[i];
+    b.v = ld3(v);
+    b.w = ld3(w);
+    b.im = b.act ? v.w : 0.0f;
+    b.I = b.act ? iiw_of(L, i) : zero3();
+    b.x = ld3(L.pos[i]);
+    b.q = ldq(L.rot[i]);
+    return b;
+}
+__device__ __forceinline__ void jb_store_vel(Lds& L, uint32_t i, const JBody& b) {
+    if (!b.act) return;
+    L.vel[i] = st3(b.v, L.vel[i].w);
+    L.angv[i] = st3(b.w, L.angv[i].w);
+}
+__device__ __forceinline__ void jb_store_pose(Lds& L, uint32_t i, const JBody& b) {
+    if (!b.act) return;
+    L.pos[i] = st3(b.x, L.pos[i].w);
+    L.rot[i] = stq(b.q);
+}
+// body pairs without contacts (world.h:1619-1620): binary search of the scene's sorted keys
+__device__ __forceinline__ bool scene_contacts_off(const BatchDev& bd, const Scene& S, uint32_t a, uint32_t b) {
+    if (!S.n_nc) return false;
+    const uint32_t key = ((a < b ? a : b) << 16) | (a < b ? b : a);
+    const uint32_t* k = bd.nc_keys + S.nc_off;
+    uint32_t lo = 0, hi = S.n_nc;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (k[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < S.n_nc && k[lo] == key;
+}
+// one joint pass over the scene's layers (mode 0 warm start, 1 velocity, 2 position), block
+// barriers between layers
+template <int TB>
+__device__ __forceinline__ void scene_joint_pass(const BatchDev& bd, const Cfg& cfg, Lds& L, const Scene& S, int mode,
+                                                 bool rev) {
+    const uint32_t* off = bd.j_loff + S.jl_off + (rev ? S.j_layers + 1 : 0u);
+    const uint16_t* ord = bd.j_order + 2u * S.j_off + (rev ? S.n_j : 0u);
+    const float inv_h = cfg.h > 0.0f ? 1.0f / cfg.h : 0.0f;
+    for (uint32_t l = 0; l < S.j_layers; ++l) {
+        for (uint32_t t = off[l] + threadIdx.x; t < off[l + 1]; t += TB) {
+            const uint32_t j = S.j_off + ord[t];
+            const uint4 m = bd.j_meta[j];
+            if (!m.w) continue;
+            JDef def = load_jdef(bd.j_def, j);
+            JBody a = jb_lds(L, m.y), b = jb_lds(L, m.z);
+            if (mode == 0) {
+                JAcc acc = load_jacc(bd.j_acc, j);
+                acc.limit = 0.0f;
+                acc.swing = 0.0f;
+                joint_warm(def, acc, a, b);
+                store_jacc(bd.j_acc, j, acc);
+                jb_store_vel(L, m.y, a);
+                jb_store_vel(L, m.z, b);
+            } else if (mode == 1) {
+                if (!a.act && !b.act) continue;
+                JAcc acc = load_jacc(bd.j_acc, j);
+                joint_velocity(def, acc, a, b, inv_h);
+                store_jacc(bd.j_acc, j, acc);
+                jb_store_vel(L, m.y, a);
+                jb_store_vel(L, m.z, b);
+            } else {
+                if (!a.act && !b.act) continue;
+                joint_position(def, a, b, cfg.slop, cfg.pos_corr, cfg.max_corr);
+                jb_store_pose(L, m.y, a);
+                jb_store_pose(L, m.z, b);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 __device__ __forceinline__ uint32_t wsum(uint32_t v) {
@@ -318,6 +407,24 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
     }
     if (t == 0) L.n_pairs = np;
     __syncthreads();
+    // wake_jointed_islands (solver.h:887-901), the first act of solve(): before the manifolds'
+    // activity is classified below. Sweeps until nothing changes.
+    for (uint32_t pass = 0; S.n_j && pass <= S.n_j; ++pass) {
+        int changed = 0;
+        for (uint32_t j = t; j < S.n_j; j += TB) {
+            const uint4 m = bd.j_meta[S.j_off + j];
+            if (!m.w) continue;
+            uint32_t fa = L.flags[m.y], fb = L.flags[m.z];
+            if ((fa & BF_TYPE) != 2u || (fb & BF_TYPE) != 2u) continue;
+            bool aw = (fa & BF_AWAKE) != 0u, bw = (fb & BF_AWAKE) != 0u;
+            if (aw == bw) continue;
+            uint32_t sl = aw ? m.z : m.y;
+            atomicOr(&L.flags[sl], BF_AWAKE);
+            L.pos[sl].w = 0.0f;
+            changed = 1;
+        }
+        if (!__syncthreads_or(changed)) break;
+    }
     // 4. narrowphase (world.h:1598-1654), pair-indexed manifolds
     uint32_t pts = 0;
     for (uint32_t k = t; k < np; k += TB) {
@@ -330,7 +437,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
         m.count = 0;
         bool valid = false;
         if ((fi & BF_ALIVE) && (fj & BF_ALIVE) && ((fi & BF_TYPE) == 2u || (fj & BF_TYPE) == 2u) &&
-            (layer_mask[ii.z & 31u] & (1u << (ij.z & 31u)))) {
+            (layer_mask[ii.z & 31u] & (1u << (ij.z & 31u))) && !scene_contacts_off(bd, S, bi, bj)) {
             ShapeGeom si = shape_geom(bd, S.shape_off + i), sj = shape_geom(bd, S.shape_off + j);
             bool spec = (len(ld3(L.vel[bi])) * cfg.h > min_extent(si)) || (len(ld3(L.vel[bj])) * cfg.h > min_extent(sj));
             valid = scene_contacts(si, ld3(L.pos[bi]), ldq(L.rot[bi]), sj, ld3(L.pos[bj]), ldq(L.rot[bj]), spec, m);
@@ -480,7 +587,9 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
         }
         __syncthreads();
     }
-    // 9. velocity and relax passes (solver.h:212-281)
+    if (S.n_j) scene_joint_pass<TB>(bd, cfg, L, S, 0, false);  // warm_start_joints (solver.h:917)
+    // 9. velocity and relax passes (solver.h:212-281), each followed by a joint pass alternating
+    // direction (solver.h:939-946)
     auto vpass = [&](int use_rest, uint32_t offset) {
         for (uint32_t c = 0; c < C; ++c) {
             for (uint32_t s = L.colour_off[c] + t; s < L.colour_off[c + 1]; s += TB) {
@@ -533,6 +642,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
             }
             __syncthreads();
         }
+        if (S.n_j) scene_joint_pass<TB>(bd, cfg, L, S, 1, (offset & 1u) != 0u);
     };
     uint32_t passes = 0;
     float resid = 0.0f;
@@ -613,6 +723,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
             __syncthreads();
         }
     }
+    if (S.n_j) scene_joint_pass<TB>(bd, cfg, L, S, 2, false);  // solve_joint_position (solver.h:954)
     // 11. warm-start cache rebuild (solver.h:767-781): on a key collision the last writer in the
     // reference's manifold order (pair order, which is the pair index here) wins
     for (uint32_t s = t; s < ordered; s += TB) {
@@ -683,6 +794,23 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
             if (atomicCAS(word, old, nw) == old) break;
         }
     }
+    for (uint32_t jj = t; jj < S.n_j; jj += TB) {  // joints unite islands too (solver.h:820-825)
+        const uint4 m = bd.j_meta[S.j_off + jj];
+        if (!m.w || (L.flags[m.y] & BF_TYPE) != 2u || (L.flags[m.z] & BF_TYPE) != 2u) continue;
+        uint32_t a = m.y, b = m.z;
+        for (int guard = 0; guard < 100000; ++guard) {
+            while (L.parent[a] != a) a = L.parent[a];
+            while (L.parent[b] != b) b = L.parent[b];
+            if (a == b) break;
+            if (a > b) { uint32_t x = a; a = b; b = x; }
+            uint32_t* word = reinterpret_cast<uint32_t*>(&L.parent[b & ~1u]);
+            uint32_t sh = (b & 1u) * 16u;
+            uint32_t old = *word;
+            if (((old >> sh) & 0xFFFFu) != b) continue;
+            uint32_t nw = (old & ~(0xFFFFu << sh)) | (a << sh);
+            if (atomicCAS(word, old, nw) == old) break;
+        }
+    }
     __syncthreads();
     for (uint32_t i = t; i < nb; i += TB) {
         uint32_t f = L.flags[i];
@@ -699,6 +827,20 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
         uint2 ab = bd.m_ab[S.man_off + k];
         for (int side = 0; side < 2; ++side) {
             uint32_t kin = side == 0 ? ab.x : ab.y, dyn = side == 0 ? ab.y : ab.x;
+            uint32_t fk = L.flags[kin], fd = L.flags[dyn];
+            if ((fk & BF_TYPE) != 1u || (fd & BF_TYPE) != 2u || (fd & BF_AWAKE)) continue;
+            v3 kv = ld3(L.vel[kin]), kw = ld3(L.angv[kin]);
+            if (dot(kv, kv) > cfg.sleep_lin * cfg.sleep_lin || dot(kw, kw) > cfg.sleep_ang * cfg.sleep_ang) {
+                atomicOr(&L.flags[dyn], BF_AWAKE);
+                L.pos[dyn].w = 0.0f;
+            }
+        }
+    }
+    for (uint32_t jj = t; jj < S.n_j; jj += TB) {  // and through joints (solver.h:854-858)
+        const uint4 m = bd.j_meta[S.j_off + jj];
+        if (!m.w) continue;
+        for (int side = 0; side < 2; ++side) {
+            uint32_t kin = side == 0 ? m.y : m.z, dyn = side == 0 ? m.z : m.y;
             uint32_t fk = L.flags[kin], fd = L.flags[dyn];
             if ((fk & BF_TYPE) != 1u || (fd & BF_TYPE) != 2u || (fd & BF_AWAKE)) continue;
             v3 kv = ld3(L.vel[kin]), kw = ld3(L.angv[kin]);
@@ -775,6 +917,16 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
             m3 z;
             z.c0 = splat(0.0f); z.c1 = splat(0.0f); z.c2 = splat(0.0f);
             set_iiw(L, i, z);
+        }
+    }
+    __syncthreads();
+    if (S.n_j) {  // joint drift on the integrated poses, then world inertia (world.h:1310-1314)
+        for (uint32_t it = 0; it < (cfg.pos_iters > 1u ? cfg.pos_iters : 1u); ++it) scene_joint_pass<TB>(bd, cfg, L, S, 2, false);
+        for (uint32_t i = t; i < nb; i += TB) {
+            uint32_t f = L.flags[i];
+            if (!(f & BF_ALIVE)) continue;
+            if ((f & BF_TYPE) == 2u) set_iiw(L, i, world_inertia(ldq(L.rot[i]), ld3(bd.b_iil[S.body_off + i])));
+            else set_iiw(L, i, zero3());
         }
     }
     awake_n = wsum(awake_n);
@@ -943,6 +1095,13 @@ struct pxg_batch {
     uint32_t graph_steps = 0, graph_block = 0, graph_lpt = 2;
     float graph_h = 0.0f;
     pxg_config graph_cfg{};
+    // joints and ignored pairs (scene-local body indices), body types for the joint layering
+    std::vector<uint32_t> body_types;
+    std::vector<pxg_joint> joints;
+    std::vector<uint32_t> joint_offsets;
+    std::vector<uint32_t> ignored;          // lo << 16 | hi
+    std::vector<uint32_t> ignored_offsets;
+    std::vector<void*> joint_allocs;
 };
 
 template <class T>
@@ -967,13 +1126,143 @@ static int bupload(pxg_batch* b, T* dst, const std::vector<T>& src) {
     return PXG_OK;
 }
 
-static void batch_free(pxg_batch* b) {
+static void drop_graph(pxg_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+static void batch_free(pxg_batch* b) {
+    drop_graph(b);
     for (void* p : b->allocs) (void)hipFree(p);
     b->allocs.clear();
+    for (void* p : b->joint_allocs) (void)hipFree(p);
+    b->joint_allocs.clear();
+}
+
+// Per scene: live joints, their forward and reverse layers, and the sorted body pairs without
+// contacts (jointed without collide_connected, or ignored); uploads them with the scene table.
+static int rebuild_batch_joints(pxg_batch* b) {
+    (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);  // the captured launch holds the old buffers
+    for (void* p : b->joint_allocs) (void)hipFree(p);
+    b->joint_allocs.clear();
+    const uint32_t nj = uint32_t(b->joints.size());
+    std::vector<uint4> meta(nj);
+    std::vector<float4> def(6 * size_t(nj)), acc(2 * size_t(nj));
+    std::vector<uint16_t> order(2 * size_t(nj));
+    std::vector<uint32_t> loff, keys;
+    for (uint32_t k = 0; k < b->n_scenes; ++k) {
+        pxb::Scene& S = b->scenes[k];
+        const uint32_t j0 = b->joint_offsets[k], n = b->joint_offsets[k + 1] - j0;
+        std::vector<uint32_t> types(b->body_types.begin() + S.body_off, b->body_types.begin() + S.body_off + S.n_bodies);
+        std::vector<uint8_t> live(n, 0);
+        std::vector<uint32_t> sk(b->ignored.begin() + b->ignored_offsets[k], b->ignored.begin() + b->ignored_offsets[k + 1]);
+        for (uint32_t q = 0; q < n; ++q) {
+            const pxg_joint& jt = b->joints[j0 + q];
+            bool in_range = jt.body_a < S.n_bodies && jt.body_b < S.n_bodies;
+            live[q] = (jt.valid && jt.enabled && in_range) ? 1 : 0;
+            meta[j0 + q] = make_uint4(jt.type, in_range ? jt.body_a : 0u, in_range ? jt.body_b : 0u, live[q]);
+            pxg::pack_joint(jt, &def[6 * size_t(j0 + q)], &acc[2 * size_t(j0 + q)]);
+            if (jt.valid && !jt.collide_connected && in_range)
+                sk.push_back((std::min(jt.body_a, jt.body_b) << 16) | std::max(jt.body_a, jt.body_b));
+        }
+        std::vector<uint32_t> of, orv, lf, lr;
+        uint32_t Lf = pxg::layer_joint_list(b->joints.data() + j0, n, live, types, false, of, lf);
+        uint32_t Lr = pxg::layer_joint_list(b->joints.data() + j0, n, live, types, true, orv, lr);
+        uint32_t L = std::max(Lf, Lr);
+        lf.resize(L + 1, lf.back());
+        lr.resize(L + 1, lr.back());
+        for (uint32_t q = 0; q < n; ++q) {
+            order[2 * size_t(j0) + q] = uint16_t(of[q]);
+            order[2 * size_t(j0) + n + q] = uint16_t(orv[q]);
+        }
+        S.j_off = j0;
+        S.n_j = n;
+        S.j_layers = L;
+        S.jl_off = uint32_t(loff.size());
+        loff.insert(loff.end(), lf.begin(), lf.end());
+        loff.insert(loff.end(), lr.begin(), lr.end());
+        std::sort(sk.begin(), sk.end());
+        sk.erase(std::unique(sk.begin(), sk.end()), sk.end());
+        S.nc_off = uint32_t(keys.size());
+        S.n_nc = uint32_t(sk.size());
+        keys.insert(keys.end(), sk.begin(), sk.end());
+    }
+    auto up = [&](auto*& dst, const auto& src) -> int {
+        using T = typename std::remove_reference<decltype(src)>::type::value_type;
+        void* q = nullptr;
+        size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+        if (hipMalloc(&q, bytes) != hipSuccess) return bfail(PXG_ERR_HIP, "hipMalloc (joints)");
+        b->joint_allocs.push_back(q);
+        if (!src.empty() && hipMemcpyAsync(q, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+            return bfail(PXG_ERR_HIP, "hipMemcpyAsync (joints)");
+        dst = static_cast<typename std::remove_reference<decltype(dst)>::type>(q);
+        return PXG_OK;
+    };
+    pxb::BatchDev& d = b->d;
+    const uint4* jm; const float4* jd; float4* ja; const uint16_t* jo; const uint32_t* jl; const uint32_t* nk;
+    int r;
+    if ((r = up(jm, meta)) || (r = up(jd, def)) || (r = up(ja, acc)) || (r = up(jo, order)) || (r = up(jl, loff)) ||
+        (r = up(nk, keys)))
+        return r;
+    d.j_meta = jm; d.j_def = jd; d.j_acc = ja; d.j_order = jo; d.j_loff = jl; d.nc_keys = nk;
+    BCHK(hipMemcpyAsync(const_cast<pxb::Scene*>(d.scenes), b->scenes.data(), b->scenes.size() * sizeof(pxb::Scene),
+                        hipMemcpyHostToDevice, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_batch_set_joints(pxg_batch* b, const uint32_t* joint_offsets, const pxg_joint* joints) {
+    if (!b || !joint_offsets || (!joints && joint_offsets[b->n_scenes]))
+        return bfail(PXG_ERR_INVALID, "pxg_batch_set_joints: null argument");
+    for (uint32_t k = 0; k < b->n_scenes; ++k) {
+        if (joint_offsets[k + 1] < joint_offsets[k]) return bfail(PXG_ERR_INVALID, "pxg_batch_set_joints: offsets must be non-decreasing");
+        if (joint_offsets[k + 1] - joint_offsets[k] > 4096u)
+            return bfail(PXG_ERR_CAPACITY, "pxg_batch_set_joints: more than 4096 joints in scene " + std::to_string(k));
+        for (uint32_t j = joint_offsets[k]; j < joint_offsets[k + 1]; ++j) {
+            const pxg_joint& jt = joints[j];
+            if (jt.type > PXG_CONE_TWIST) return bfail(PXG_ERR_INVALID, "pxg_batch_set_joints: bad joint type");
+            if (jt.body_a >= b->scenes[k].n_bodies || jt.body_b >= b->scenes[k].n_bodies)
+                return bfail(PXG_ERR_INVALID, "pxg_batch_set_joints: body index beyond scene " + std::to_string(k));
+            if (jt.body_a == jt.body_b) return bfail(PXG_ERR_INVALID, "pxg_batch_set_joints: a joint needs two different bodies");
+        }
+    }
+    b->joint_offsets.assign(joint_offsets, joint_offsets + b->n_scenes + 1);
+    b->joints.assign(joints, joints + joint_offsets[b->n_scenes]);
+    return rebuild_batch_joints(b);
+}
+
+extern "C" int pxg_batch_get_joints(pxg_batch* b, pxg_joint* out, uint32_t n) {
+    if (!b || (!out && n)) return bfail(PXG_ERR_INVALID, "pxg_batch_get_joints: null argument");
+    if (n > b->joints.size()) return bfail(PXG_ERR_INVALID, "pxg_batch_get_joints: more joints requested than the batch holds");
+    if (!n) return PXG_OK;
+    std::vector<float4> acc(2 * size_t(n));
+    BCHK(hipMemcpyAsync(acc.data(), b->d.j_acc, acc.size() * sizeof(float4), hipMemcpyDeviceToHost, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    for (uint32_t j = 0; j < n; ++j) {
+        pxg::unpack_joint_acc(&acc[2 * size_t(j)], b->joints[j]);
+        out[j] = b->joints[j];
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_batch_set_ignored_pairs(pxg_batch* b, const uint32_t* pair_offsets, const uint32_t* ab) {
+    if (!b || !pair_offsets || (!ab && pair_offsets[b->n_scenes]))
+        return bfail(PXG_ERR_INVALID, "pxg_batch_set_ignored_pairs: null argument");
+    b->ignored.clear();
+    b->ignored_offsets.assign(b->n_scenes + 1, 0u);
+    for (uint32_t k = 0; k < b->n_scenes; ++k) {
+        if (pair_offsets[k + 1] < pair_offsets[k]) return bfail(PXG_ERR_INVALID, "pxg_batch_set_ignored_pairs: offsets must be non-decreasing");
+        for (uint32_t i = pair_offsets[k]; i < pair_offsets[k + 1]; ++i) {
+            uint32_t x = ab[2 * i], y = ab[2 * i + 1];
+            if (x >= b->scenes[k].n_bodies || y >= b->scenes[k].n_bodies)
+                return bfail(PXG_ERR_INVALID, "pxg_batch_set_ignored_pairs: body index beyond scene " + std::to_string(k));
+            if (x != y) b->ignored.push_back((std::min(x, y) << 16) | std::max(x, y));
+        }
+        b->ignored_offsets[k + 1] = uint32_t(b->ignored.size());
+    }
+    return rebuild_batch_joints(b);
 }
 
 extern "C" const char* pxg_batch_last_error(void) { return g_batch_error.c_str(); }
@@ -1036,6 +1325,10 @@ extern "C" int pxg_batch_create(const pxg_config* cfg, uint32_t n_scenes, const 
     // bodies and shapes
     pxg::BodySoA bs;
     if (!pxg::pack_bodies(bodies, b->nb, bs)) return bail(bfail(PXG_ERR_INVALID, "pxg_batch_create: bad body type"));
+    b->body_types.resize(b->nb);
+    for (uint32_t i = 0; i < b->nb; ++i) b->body_types[i] = bodies[i].type;
+    b->joint_offsets.assign(n_scenes + 1, 0u);
+    b->ignored_offsets.assign(n_scenes + 1, 0u);
     pxg::ShapeSoA ss;
     std::vector<uint32_t> shape_scene(b->ns);
     for (uint32_t k = 0; k < n_scenes; ++k)
@@ -1093,6 +1386,7 @@ extern "C" int pxg_batch_create(const pxg_config* cfg, uint32_t n_scenes, const 
     if (b->nb) hipLaunchKernelGGL(pxb::k_batch_prepare, dim3((b->nb + 255) / 256), dim3(256), 0, st, d, b->nb, b->ns, b->cfg.aabb_margin);
     if (n_scenes) hipLaunchKernelGGL(pxb::k_batch_proxies, dim3(n_scenes), dim3(256), 0, st, d, n_scenes, b->cfg.aabb_margin);
     if (hipStreamSynchronize(st) != hipSuccess) return bail(bfail(PXG_ERR_HIP, "pxg_batch_create: setup kernels failed"));
+    if ((r = rebuild_batch_joints(b)) != PXG_OK) return bail(r);  // empty joint table
     (void)hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->blocks_per_cu[0], reinterpret_cast<const void*>(&pxb::k_batch<64>), 64, 0);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->blocks_per_cu[1], reinterpret_cast<const void*>(&pxb::k_batch<128>), 128, 0);

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -90,6 +91,71 @@ inline const char* pack_shapes(const pxg_shape* in, size_t n, uint32_t n_owner_b
         o.mat[i] = make_float4(s.capsule_half_height, s.dynamic_friction, s.restitution, s.static_friction);
     }
     return nullptr;
+}
+
+
+// ---- joints ----
+// Layers of one joint pass visiting joints[0..n) forward (or reversed): a joint goes one layer after
+// the last earlier joint that shares a dynamic body with it, so each body sees its joints in the
+// pass's order. Non-dynamic bodies are never written by a joint row and order nothing. Joints
+// with live[j] == 0 are left out. Returns the layer count L; order lists joint indices layer by
+// layer, off holds the L + 1 layer offsets.
+inline uint32_t layer_joint_list(const pxg_joint* joints, size_t n, const std::vector<uint8_t>& live,
+                                 const std::vector<uint32_t>& body_types, bool rev, std::vector<uint32_t>& order,
+                                 std::vector<uint32_t>& off) {
+    std::vector<uint32_t> last(body_types.size(), 0), layer(n, 0);
+    uint32_t L = 0;
+    auto dyn = [&](uint32_t b) { return b < body_types.size() && body_types[b] == PXG_DYNAMIC; };
+    for (size_t t = 0; t < n; ++t) {
+        size_t j = rev ? n - 1 - t : t;
+        if (!live[j]) continue;
+        uint32_t a = joints[j].body_a, b = joints[j].body_b, l = 0;
+        if (dyn(a)) l = std::max(l, last[a]);
+        if (dyn(b)) l = std::max(l, last[b]);
+        layer[j] = l;
+        if (dyn(a)) last[a] = l + 1;
+        if (dyn(b)) last[b] = l + 1;
+        L = std::max(L, l + 1);
+    }
+    off.assign(L + 2, 0);
+    for (size_t j = 0; j < n; ++j) if (live[j]) off[layer[j] + 1]++;
+    for (uint32_t l = 0; l < L; ++l) off[l + 1] += off[l];
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    order.assign(n, 0);
+    for (size_t t = 0; t < n; ++t) {
+        size_t j = rev ? n - 1 - t : t;
+        if (live[j]) order[cur[layer[j]]++] = uint32_t(j);
+    }
+    off.resize(L + 1);
+    return L;
+}
+
+// Device records of one joint (layout in pxg_joints.h): 6 float4 of definition, 2 of accumulators.
+inline void pack_joint(const pxg_joint& jt, float4* def, float4* acc) {
+    uint32_t flags = 0;
+    if (jt.type == PXG_HINGE && jt.use_limits) flags |= 1u;                      // hinge limit row
+    if (jt.type == PXG_CONE_TWIST && jt.swing_limit >= 0.0f) flags |= 2u;        // has_swing_limit()
+    if (jt.type == PXG_CONE_TWIST && jt.twist_min <= jt.twist_max) flags |= 4u;  // has_twist_limit()
+    float tf, ff;
+    uint32_t ty = jt.type;
+    std::memcpy(&tf, &ty, 4);
+    std::memcpy(&ff, &flags, 4);
+    const float* aa = jt.local_anchor_a; const float* ab = jt.local_anchor_b;
+    const float* xa = jt.local_axis_a; const float* xb = jt.local_axis_b; const float* q = jt.rest_relative_rotation;
+    def[0] = make_float4(aa[0], aa[1], aa[2], jt.min_angle);
+    def[1] = make_float4(ab[0], ab[1], ab[2], jt.max_angle);
+    def[2] = make_float4(xa[0], xa[1], xa[2], jt.swing_limit);
+    def[3] = make_float4(xb[0], xb[1], xb[2], jt.twist_min);
+    def[4] = make_float4(q[0], q[1], q[2], q[3]);
+    def[5] = make_float4(jt.twist_max, tf, ff, 0.0f);
+    acc[0] = make_float4(jt.point_impulse[0], jt.point_impulse[1], jt.point_impulse[2], jt.limit_impulse);
+    acc[1] = make_float4(jt.axis_impulse[0], jt.axis_impulse[1], jt.swing_impulse, 0.0f);
+}
+inline void unpack_joint_acc(const float4* acc, pxg_joint& jt) {
+    jt.point_impulse[0] = acc[0].x; jt.point_impulse[1] = acc[0].y; jt.point_impulse[2] = acc[0].z;
+    jt.limit_impulse = acc[0].w;
+    jt.axis_impulse[0] = acc[1].x; jt.axis_impulse[1] = acc[1].y;
+    jt.swing_impulse = acc[1].z;
 }
 
 }  // namespace pxg

@@ -411,39 +411,6 @@ extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n)
 // ---------------------------------------------------------------------------------------------
 // joints and ignored pairs
 // ---------------------------------------------------------------------------------------------
-// Layers of one joint pass visiting the table forward (or reversed): a joint goes one layer after
-// the last earlier joint that shares a dynamic body with it, so every body sees its joints in the
-// pass's order. Non-dynamic bodies are never written by a joint row and do not order anything.
-static uint32_t layer_joints(const pxg_world* w, const std::vector<uint8_t>& live, bool rev, std::vector<uint32_t>& order,
-                             std::vector<uint32_t>& off) {
-    const size_t n = w->joints.size();
-    std::vector<uint32_t> last(w->n_bodies, 0), layer(n, 0);
-    uint32_t L = 0;
-    auto dyn = [&](uint32_t b) { return w->body_types[b] == PXG_DYNAMIC; };
-    for (size_t t = 0; t < n; ++t) {
-        size_t j = rev ? n - 1 - t : t;
-        if (!live[j]) continue;
-        uint32_t a = w->joints[j].body_a, b = w->joints[j].body_b, l = 0;
-        if (dyn(a)) l = std::max(l, last[a]);
-        if (dyn(b)) l = std::max(l, last[b]);
-        layer[j] = l;
-        if (dyn(a)) last[a] = l + 1;
-        if (dyn(b)) last[b] = l + 1;
-        L = std::max(L, l + 1);
-    }
-    off.assign(L + 2, 0);
-    for (size_t j = 0; j < n; ++j) if (live[j]) off[layer[j] + 1]++;
-    for (uint32_t l = 0; l < L; ++l) off[l + 1] += off[l];
-    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
-    order.assign(n, 0);
-    for (size_t t = 0; t < n; ++t) {
-        size_t j = rev ? n - 1 - t : t;
-        if (live[j]) order[cur[layer[j]]++] = uint32_t(j);
-    }
-    off.resize(L + 1);
-    return L;
-}
-
 // Re-derives everything that depends on the joint table and the body table: live flags, the
 // two layerings and the no-contact pairs; uploads them.
 static int rebuild_joints(pxg_world* w) {
@@ -466,8 +433,8 @@ static int rebuild_joints(pxg_world* w) {
     std::sort(keys.begin(), keys.end());
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     std::vector<uint32_t> of, orv;
-    uint32_t Lf = layer_joints(w, live, false, of, w->jl_fwd);
-    uint32_t Lr = layer_joints(w, live, true, orv, w->jl_rev);
+    uint32_t Lf = layer_joint_list(w->joints.data(), n, live, w->body_types, false, of, w->jl_fwd);
+    uint32_t Lr = layer_joint_list(w->joints.data(), n, live, w->body_types, true, orv, w->jl_rev);
     uint32_t L = std::max(Lf, Lr);
     w->jl_fwd.resize(L + 1, w->jl_fwd.back());
     w->jl_rev.resize(L + 1, w->jl_rev.back());
@@ -519,27 +486,7 @@ extern "C" int pxg_set_joints(pxg_world* w, const pxg_joint* joints, uint32_t n)
     if (r != PXG_OK) return r;
     if (!n) return PXG_OK;
     std::vector<float4> def(6 * size_t(n)), acc(2 * size_t(n));
-    for (uint32_t j = 0; j < n; ++j) {
-        const pxg_joint& jt = joints[j];
-        uint32_t flags = 0;
-        if (jt.type == PXG_HINGE && jt.use_limits) flags |= 1u;                      // JF_LIMITS
-        if (jt.type == PXG_CONE_TWIST && jt.swing_limit >= 0.0f) flags |= 2u;        // has_swing_limit()
-        if (jt.type == PXG_CONE_TWIST && jt.twist_min <= jt.twist_max) flags |= 4u;  // has_twist_limit()
-        float tf, ff;
-        uint32_t ty = jt.type;
-        std::memcpy(&tf, &ty, 4);
-        std::memcpy(&ff, &flags, 4);
-        const float* aa = jt.local_anchor_a; const float* ab = jt.local_anchor_b;
-        const float* xa = jt.local_axis_a; const float* xb = jt.local_axis_b; const float* q = jt.rest_relative_rotation;
-        def[6 * j + 0] = make_float4(aa[0], aa[1], aa[2], jt.min_angle);
-        def[6 * j + 1] = make_float4(ab[0], ab[1], ab[2], jt.max_angle);
-        def[6 * j + 2] = make_float4(xa[0], xa[1], xa[2], jt.swing_limit);
-        def[6 * j + 3] = make_float4(xb[0], xb[1], xb[2], jt.twist_min);
-        def[6 * j + 4] = make_float4(q[0], q[1], q[2], q[3]);
-        def[6 * j + 5] = make_float4(jt.twist_max, tf, ff, 0.0f);
-        acc[2 * j + 0] = make_float4(jt.point_impulse[0], jt.point_impulse[1], jt.point_impulse[2], jt.limit_impulse);
-        acc[2 * j + 1] = make_float4(jt.axis_impulse[0], jt.axis_impulse[1], jt.swing_impulse, 0.0f);
-    }
+    for (uint32_t j = 0; j < n; ++j) pack_joint(joints[j], &def[6 * size_t(j)], &acc[2 * size_t(j)]);
     DevState& d = w->d;
     HIPCHK(hipMemcpyAsync(d.j_def, def.data(), def.size() * sizeof(float4), hipMemcpyHostToDevice, w->stream));
     HIPCHK(hipMemcpyAsync(d.j_acc, acc.data(), acc.size() * sizeof(float4), hipMemcpyHostToDevice, w->stream));
@@ -555,12 +502,8 @@ extern "C" int pxg_get_joints(pxg_world* w, pxg_joint* out, uint32_t n) {
     HIPCHK(hipMemcpyAsync(acc.data(), w->d.j_acc, acc.size() * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     for (uint32_t j = 0; j < n; ++j) {
-        pxg_joint& jt = w->joints[j];
-        jt.point_impulse[0] = acc[2 * j].x; jt.point_impulse[1] = acc[2 * j].y; jt.point_impulse[2] = acc[2 * j].z;
-        jt.limit_impulse = acc[2 * j].w;
-        jt.axis_impulse[0] = acc[2 * j + 1].x; jt.axis_impulse[1] = acc[2 * j + 1].y;
-        jt.swing_impulse = acc[2 * j + 1].z;
-        out[j] = jt;
+        unpack_joint_acc(&acc[2 * size_t(j)], w->joints[j]);
+        out[j] = w->joints[j];
     }
     return PXG_OK;
 }

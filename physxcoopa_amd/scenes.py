@@ -515,3 +515,22 @@ def ragdoll_scene(seed=5, n_ragdolls=1, n_boxes=32, spread=3.0):
     b, s = sb.arrays()
     cfg = dict(fixed_dt=float(F(1.0) / F(120.0)), velocity_iterations=12)
     return Scene("ragdolls_%d" % n_ragdolls, b, s, cfg, steps=600), recs, ignored
+
+
+def config5_scenes(seed=5, n_scenes=2048, n_boxes=32):
+    """The batched ragdoll workload: `n_scenes` independent ragdoll scenes (one ragdoll, 32 boxes,
+    the 10 x 10 x 1 m ground; see ragdoll_scene). Returns (bodies, shapes, body_offsets,
+    shape_offsets, joints, joint_offsets, ignored pairs, pair_offsets, settings); joints and pairs
+    use scene-local body indices."""
+    from . import joints as J
+    bl, sl, jl, il = [], [], [], []
+    for k in range(n_scenes):
+        sc, recs, ig = ragdoll_scene(seed=seed * 100003 + k, n_ragdolls=1, n_boxes=n_boxes)
+        bl.append(sc.bodies)
+        sl.append(sc.shapes)
+        jl.append(J.table(recs))
+        il.append(np.asarray(ig, np.uint32).reshape(-1, 2))
+    off = lambda xs: np.concatenate([[0], np.cumsum([len(x) for x in xs])]).astype(np.uint32)
+    return (np.ascontiguousarray(np.concatenate(bl)), np.ascontiguousarray(np.concatenate(sl)), off(bl), off(sl),
+            np.ascontiguousarray(np.concatenate(jl)), off(jl), np.ascontiguousarray(np.concatenate(il)), off(il),
+            dict(fixed_dt=float(F(1.0) / F(120.0)), velocity_iterations=12))

@@ -111,3 +111,38 @@ def test_batch_residual_mode_matches_oracle(small_batch, oracle):
         ow.step_fixed()
         compare_scene(batch, k, ow, rep)
         assert batch.stats()[k]["velocity_passes"] == ow.stats()["velocity_passes"]
+
+
+def test_batch_ragdolls_match_oracle_per_scene(oracle):
+    """Batched ragdoll scenes (joints, ignored pairs): each checked scene equals its own oracle world
+    fed the batch's contact order, joints included, bit for bit."""
+    from physxcoopa_amd.batch import Batch
+    b, s, bo, so, jt, jo, ig, io, settings = scenes.config5_scenes(seed=7, n_scenes=12)
+    cfg = _abi.default_config(**settings)
+    batch = Batch(cfg, b, s, bo, so)
+    batch.set_joints(jo, jt)
+    batch.set_ignored_pairs(io, ig)
+    checked = [0, 3, 5, 11]
+    ows = {}
+    for k in checked:
+        ow = oracle.OracleWorld(cfg, b[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]])
+        ow.set_atan2_mode(True)
+        ow.set_joints(jt[jo[k]:jo[k + 1]])
+        ow.set_ignored_pairs(ig[io[k]:io[k + 1]])
+        ows[k] = ow
+    rep = dict(checks=0, bitexact=0)
+    for step in range(150):
+        batch.step(1)
+        for k in checked:
+            gm = batch.manifolds(k)
+            order, _ = batch.solve_order(k)
+            pairs = np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1) if len(order) else np.zeros((0, 2))
+            ows[k].set_order(pairs)
+            ows[k].step_fixed()
+            compare_scene(batch, k, ows[k], rep)
+    print("ragdoll batch parity:", rep, flush=True)
+    assert rep["bitexact"] == rep["checks"]
+    gj = batch.joints()
+    for k in checked:
+        assert np.array_equal(gj[jo[k]:jo[k + 1]].view(np.uint8), ows[k].joints().view(np.uint8))
+    assert np.all(batch.stats()["error"] == 0)
