@@ -379,6 +379,12 @@ int pxg_batch_get_joints(pxg_batch* b, pxg_joint* out, uint32_t n);
 /* Ignored body pairs of every scene (set_pair_collision(a, b, false), world.h:527-532): scene k
  * owns the pairs ab[2 pair_offsets[k] .. 2 pair_offsets[k+1]), scene-local. */
 int pxg_batch_set_ignored_pairs(pxg_batch* b, const uint32_t* pair_offsets, const uint32_t* ab);
+/* The learned warm start in every scene (as pxg_set_mlp for one world); NULL turns it off. At most
+ * 256 misses per scene and step are predicted, the rest start at zero as in the reference. */
+int pxg_batch_set_mlp(pxg_batch* b, const pxg_mlp* net);
+/* The last step's predictions of one scene, sorted by warm-start key (scene-local bodies). */
+int pxg_batch_get_predictions(pxg_batch* b, uint32_t scene, uint64_t* keys, float* impulses3, uint32_t cap,
+                              uint32_t* n);
 const char* pxg_batch_last_error(void);
 
 #ifdef __cplusplus

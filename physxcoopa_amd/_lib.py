@@ -22,6 +22,9 @@ _SIGNATURES = {
     "pxg_get_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_ignored_pairs": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_batch_set_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_set_mlp": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_batch_get_predictions": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                   ctypes.c_void_p], ctypes.c_int),
     "pxg_batch_get_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_batch_set_ignored_pairs": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_step_fixed": ([ctypes.c_void_p, ctypes.c_float], ctypes.c_int),

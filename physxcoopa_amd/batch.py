@@ -57,6 +57,20 @@ class Batch:
         a = np.ascontiguousarray(np.asarray(ab, np.uint32).reshape(-1, 2))
         check_batch(lib().pxg_batch_set_ignored_pairs(self.h, ptr(o), ptr(a)), "pxg_batch_set_ignored_pairs")
 
+    def set_mlp(self, net):
+        """The learned warm start in every scene with `net` (a learned.MlpParams), or off with None."""
+        check_batch(lib().pxg_batch_set_mlp(self.h, ctypes.byref(net) if net is not None else None), "pxg_batch_set_mlp")
+
+    def predictions(self, scene):
+        """(keys, clamped impulses [n, 3]) the learned warm start applied in `scene` in the last step."""
+        n = ctypes.c_uint32()
+        check_batch(lib().pxg_batch_get_predictions(self.h, scene, None, None, 0, ctypes.byref(n)), "predictions")
+        k = np.zeros(max(n.value, 1), np.uint64)
+        i = np.zeros((max(n.value, 1), 3), np.float32)
+        check_batch(lib().pxg_batch_get_predictions(self.h, scene, ptr(k), ptr(i), n.value, ctypes.byref(n)),
+                    "pxg_batch_get_predictions")
+        return k[:n.value], i[:n.value]
+
     def set_tuning(self, block_size, longest_first=True):
         check_batch(lib().pxg_batch_set_tuning(self.h, block_size, 1 if longest_first else 0), "pxg_batch_set_tuning")
         self.tuning = (block_size, bool(longest_first))

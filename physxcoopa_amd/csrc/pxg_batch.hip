@@ -15,6 +15,7 @@
 #include "../../include/pxg.h"
 #include "pxg_device.h"
 #include "pxg_joints.h"
+#include "pxg_mlp.h"
 #include "pxg_state.h"
 
 #include <hip/hip_runtime.h>
@@ -91,7 +92,14 @@ struct BatchDev {
     const uint16_t* j_order;      // scene-local joint indices
     const uint32_t* j_loff;
     const uint32_t* nc_keys;      // lo << 16 | hi
+    // learned warm start: network (nullptr = off) and per-scene staging of the warm-start misses of
+    // the last step (kMissRows rows per scene: features, point index, clamped prediction)
+    const pxg::MlpDev* mlp;
+    float* feat;
+    uint32_t* miss_pt;
+    float* pred;
 };
+constexpr uint32_t kMissRows = 256;  // misses predicted per scene and step; the rest start at zero
 
 struct Cfg {
     float h, gx, gy, gz, margin;
@@ -131,6 +139,8 @@ struct Lds {
     uint32_t n_pairs, remaining, colours, err, active, ordered, points, passes, awake, n_valid;
     float residual;
     uint32_t scene;
+    uint32_t n_miss;                    // learned warm start: misses queued this step
+    uint16_t mlp_h[8][16][pxg::kMlpH];  // per wave: hidden activations between the network's layers
 };
 
 }  // namespace pxb
@@ -468,7 +478,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
     }
     // 5. colouring (Jones-Plassmann rounds, first-fit colour; only awake dynamic bodies constrain)
     for (uint32_t i = t; i < nb; i += TB) { L.best[i] = 0ull; L.mask[i] = 0ull; }
-    if (t == 0) { L.remaining = 0; L.colours = 0; L.points = 0; L.n_valid = 0; }
+    if (t == 0) { L.remaining = 0; L.colours = 0; L.points = 0; L.n_valid = 0; L.n_miss = 0; }
     __syncthreads();
     uint32_t rem = 0, nval = 0;
     for (uint32_t k = t; k < np; k += TB) {
@@ -560,13 +570,52 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
                     if ((v >> 50) != ((gen - 1u) & 0x3FFFu)) break;
                     idx = (idx + 1u) & cmask;
                 }
-                if (!hit) qq.w = 1.0f;
+                if (!hit) {
+                    qq.w = 1.0f;
+                    if (bd.mlp) {  // queue the miss for the network (features of the pre-solve state)
+                        uint32_t row = atomicAdd(&L.n_miss, 1u);
+                        if (row < kMissRows) {
+                            pxg::FeatBody FA, FB;
+                            const bool aa = active_flag(L.flags[ab.x]), ba = active_flag(L.flags[ab.y]);
+                            FA.x = pa; FA.v = va; FA.w = wa; FA.im = aa ? L.vel[ab.x].w : 0.0f;
+                            FA.I = aa ? iiw_of(L, ab.x) : zero3();
+                            FB.x = pb; FB.v = vb; FB.w = wb; FB.im = ba ? L.vel[ab.y].w : 0.0f;
+                            FB.I = ba ? iiw_of(L, ab.y) : zero3();
+                            float f[pxg::kMlpIn];
+                            pxg::contact_features(FA, FB, n, pos, pp.w, qq.x, nf.w, rest, f);
+                            const size_t r0 = size_t(L.scene) * kMissRows + row;
+                            for (int c = 0; c < pxg::kMlpIn; ++c) bd.feat[r0 * pxg::kMlpIn + c] = f[c];
+                            bd.miss_pt[r0] = 4u * gm + q;
+                        }
+                    }
+                }
             }
             bd.m_q[4 * gm + q] = qq;
             bd.m_imp[4 * gm + q] = imp;
         }
     }
     __syncthreads();
+    if (bd.mlp) {
+        // the learned warm start: the network on the matrix cores, 16 misses per wave at a time,
+        // then each prediction clamped to the friction cone becomes the point's initial impulse
+        const uint32_t nm = L.n_miss < kMissRows ? L.n_miss : kMissRows;
+        const size_t base = size_t(L.scene) * kMissRows;
+        const uint32_t wave = t >> 6;
+        const pxg::MlpView W = pxg::mlp_view(bd.mlp);
+        for (uint32_t row0 = wave * 16u; row0 < nm; row0 += (TB / 64) * 16u)
+            pxg::mlp_rows16(W, bd.feat + base * pxg::kMlpIn, row0, nm, L.mlp_h[wave], bd.pred + base * pxg::kMlpOut);
+        __syncthreads();
+        for (uint32_t r = t; r < nm; r += TB) {
+            const uint32_t pt = bd.miss_pt[base + r];
+            const float mu = bd.m_nf[pt / 4u].w;
+            float* pr = bd.pred + (base + r) * pxg::kMlpOut;
+            const float ln = fmaxf(pr[0], 0.0f), lim = mu * ln;
+            const float t1 = fminf(fmaxf(pr[1], -lim), lim), t2 = fminf(fmaxf(pr[2], -lim), lim);
+            pr[0] = ln; pr[1] = t1; pr[2] = t2;
+            bd.m_imp[pt] = make_float4(ln, t1, t2, 0.0f);
+        }
+        __syncthreads();
+    }
     // 8. warm start, colour by colour
     for (uint32_t c = 0; c < C; ++c) {
         for (uint32_t s = L.colour_off[c] + t; s < L.colour_off[c + 1]; s += TB) {
@@ -997,7 +1046,9 @@ __global__ void __launch_bounds__(TB) k_batch(BatchDev bd, Cfg cfg) {
             SceneStats st;
             st.pairs = L.n_pairs; st.manifolds = L.n_valid; st.points = L.points; st.colours = L.colours;
             st.active = L.active; st.passes = L.passes; st.err = L.err; st.ordered = L.ordered;
-            st.residual = L.residual; st.awake = L.awake; st.reserved[0] = 0; st.reserved[1] = 0;
+            st.residual = L.residual; st.awake = L.awake;
+            st.reserved[0] = bd.mlp ? (L.n_miss < kMissRows ? L.n_miss : kMissRows) : 0u;
+            st.reserved[1] = 0;
             bd.stats[sc] = st;
         }
     }
@@ -1102,6 +1153,7 @@ struct pxg_batch {
     std::vector<uint32_t> ignored;          // lo << 16 | hi
     std::vector<uint32_t> ignored_offsets;
     std::vector<void*> joint_allocs;
+    pxg::MlpDev* mlp_dev = nullptr;   // the network's device image (learned warm start)
 };
 
 template <class T>
@@ -1210,6 +1262,74 @@ static int rebuild_batch_joints(pxg_batch* b) {
     BCHK(hipMemcpyAsync(const_cast<pxb::Scene*>(d.scenes), b->scenes.data(), b->scenes.size() * sizeof(pxb::Scene),
                         hipMemcpyHostToDevice, b->stream));
     BCHK(hipStreamSynchronize(b->stream));
+    return PXG_OK;
+}
+
+namespace pxg { void mlp_to_device_image(const pxg_mlp* net, MlpDev* img); }
+
+// The learned warm start for every scene (the single world's pxg_set_mlp); net == nullptr turns it off.
+extern "C" int pxg_batch_set_mlp(pxg_batch* b, const pxg_mlp* net) {
+    if (!b) return bfail(PXG_ERR_INVALID, "pxg_batch_set_mlp: null handle");
+    (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    if (!net) {
+        b->d.mlp = nullptr;
+        return PXG_OK;
+    }
+    if (!b->d.feat) {
+        const size_t rows = size_t(b->n_scenes) * pxb::kMissRows;
+        pxg::MlpDev* img_dev = nullptr;
+        BALLOC(img_dev, 1);
+        BALLOC(b->d.feat, rows * pxg::kMlpIn);
+        BALLOC(b->d.miss_pt, rows);
+        BALLOC(b->d.pred, rows * pxg::kMlpOut);
+        b->mlp_dev = img_dev;
+    }
+    pxg::MlpDev img;
+    pxg::mlp_to_device_image(net, &img);
+    BCHK(hipMemcpyAsync(b->mlp_dev, &img, sizeof(img), hipMemcpyHostToDevice, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    b->d.mlp = b->mlp_dev;
+    return PXG_OK;
+}
+
+// The last step's predictions of one scene, sorted by warm-start key (pack_warm_start_key,
+// dynamics/solver.h:46-51, scene-local body indices): the clamped initial impulses of its misses.
+extern "C" int pxg_batch_get_predictions(pxg_batch* b, uint32_t scene, uint64_t* keys, float* imp3, uint32_t cap,
+                                         uint32_t* n) {
+    if (!b || !n || scene >= b->n_scenes) return bfail(PXG_ERR_INVALID, "pxg_batch_get_predictions: bad argument");
+    *n = 0;
+    if (!b->d.mlp) return PXG_OK;
+    hipStream_t st = b->stream;
+    pxb::SceneStats ss;
+    BCHK(hipMemcpyAsync(&ss, b->d.stats + scene, sizeof(ss), hipMemcpyDeviceToHost, st));
+    BCHK(hipStreamSynchronize(st));
+    const uint32_t P = ss.reserved[0];
+    if (!P) return PXG_OK;
+    const size_t base = size_t(scene) * pxb::kMissRows;
+    std::vector<uint32_t> pt(P);
+    std::vector<float> pr(3 * size_t(P));
+    BCHK(hipMemcpyAsync(pt.data(), b->d.miss_pt + base, P * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    BCHK(hipMemcpyAsync(pr.data(), b->d.pred + base * 3, pr.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    BCHK(hipStreamSynchronize(st));
+    std::vector<std::pair<uint64_t, uint32_t>> e;
+    for (uint32_t j = 0; j < P; ++j) {
+        const uint32_t gm = pt[j] / 4u, q = pt[j] % 4u;
+        uint2 ab;
+        float4 qq;
+        BCHK(hipMemcpy(&ab, b->d.m_ab + gm, sizeof(ab), hipMemcpyDeviceToHost));
+        BCHK(hipMemcpy(&qq, b->d.m_q + 4 * size_t(gm) + q, sizeof(qq), hipMemcpyDeviceToHost));
+        uint32_t fid;
+        std::memcpy(&fid, &qq.y, 4);
+        const uint32_t lo = std::min(ab.x, ab.y), hi = std::max(ab.x, ab.y);
+        e.push_back({(uint64_t(lo) << 40) ^ (uint64_t(hi) << 12) ^ uint64_t(fid), j});
+    }
+    std::sort(e.begin(), e.end());
+    *n = P;
+    for (uint32_t i = 0; i < P && i < cap; ++i) {
+        if (keys) keys[i] = e[i].first;
+        if (imp3) for (int k = 0; k < 3; ++k) imp3[3 * i + k] = pr[3 * size_t(e[i].second) + k];
+    }
     return PXG_OK;
 }
 

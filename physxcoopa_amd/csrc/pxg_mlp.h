@@ -53,6 +53,97 @@ __device__ __forceinline__ void contact_features(const FeatBody& A, const FeatBo
     f[22] = mu; f[23] = e;
 }
 
+// ---- inference on the matrix cores ----
+typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ uint16_t mlp_f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+// Where the weights are read from (LDS copies or the device image itself).
+struct MlpView {
+    const uint16_t *w1, *w2, *w3;
+    const float *b1, *b2, *b3, *shift, *scale, *oscale;
+};
+__device__ __forceinline__ MlpView mlp_view(const MlpDev* n) {
+    MlpView v;
+    v.w1 = n->w1; v.w2 = n->w2; v.w3 = n->w3; v.b1 = n->b1; v.b2 = n->b2; v.b3 = n->b3;
+    v.shift = n->in_shift; v.scale = n->in_scale; v.oscale = n->out_scale;
+    return v;
+}
+// LDS writes of this wave visible to its own later LDS reads
+__device__ __forceinline__ void mlp_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// One wave64 evaluates rows row0 .. row0+15 of `feats` (kMlpIn floats per row, n rows valid) with
+// v_mfma_f32_16x16x16_bf16: lane = 16 g + r holds row r's K values 4g..4g+3 of each 16-wide K step
+// (A operand) and weight row r of the 16-wide N tile (B operand); each accumulator lane holds 4
+// rows x 1 column, re-laid out through the wave's LDS buffer h between layers. Writes 3 outputs
+// per valid row to out. All 64 lanes must call it.
+__device__ __forceinline__ void mlp_rows16(const MlpView& W, const float* feats, uint32_t row0, uint32_t n,
+                                           uint16_t (*h)[kMlpH], float* out) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const uint32_t row = row0 + uint32_t(r);
+    mlp_bf16x4 a1[2];
+    for (int ks = 0; ks < 2; ++ks) {
+        for (int q = 0; q < 4; ++q) {
+            int k = ks * 16 + 4 * g + q;
+            float x = 0.0f;
+            if (row < n && k < kMlpIn) x = (feats[size_t(row) * kMlpIn + k] - W.shift[k]) * W.scale[k];
+            a1[ks][q] = short(mlp_f2bf(x));
+        }
+    }
+    for (int nt = 0; nt < 4; ++nt) {
+        mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < 2; ++ks) {
+            mlp_bf16x4 b;
+            const uint16_t* wp = &W.w1[(nt * 16 + r) * kMlpInPad + ks * 16 + 4 * g];
+            b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ks], b, acc, 0, 0, 0);
+        }
+        int col = nt * 16 + r;
+        for (int i = 0; i < 4; ++i) h[4 * g + i][col] = mlp_f2bf(fmaxf(acc[i] + W.b1[col], 0.0f));
+    }
+    mlp_wave_sync();
+    mlp_bf16x4 a2[4];
+    for (int ks = 0; ks < 4; ++ks)
+        for (int q = 0; q < 4; ++q) a2[ks][q] = short(h[r][ks * 16 + 4 * g + q]);
+    mlp_wave_sync();
+    for (int nt = 0; nt < 4; ++nt) {
+        mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int ks = 0; ks < 4; ++ks) {
+            mlp_bf16x4 b;
+            const uint16_t* wp = &W.w2[(nt * 16 + r) * kMlpH + ks * 16 + 4 * g];
+            b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ks], b, acc, 0, 0, 0);
+        }
+        int col = nt * 16 + r;
+        for (int i = 0; i < 4; ++i) h[4 * g + i][col] = mlp_f2bf(fmaxf(acc[i] + W.b2[col], 0.0f));
+    }
+    mlp_wave_sync();
+    mlp_bf16x4 a3[4];
+    for (int ks = 0; ks < 4; ++ks)
+        for (int q = 0; q < 4; ++q) a3[ks][q] = short(h[r][ks * 16 + 4 * g + q]);
+    mlp_wave_sync();
+    mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < 4; ++ks) {
+        mlp_bf16x4 b;
+        const uint16_t* wp = &W.w3[r * kMlpH + ks * 16 + 4 * g];
+        b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[ks], b, acc, 0, 0, 0);
+    }
+    if (r < kMlpOut) {
+        for (int i = 0; i < 4; ++i) {
+            uint32_t orow = row0 + uint32_t(4 * g + i);
+            if (orow < n) out[size_t(orow) * kMlpOut + r] = (acc[i] + W.b3[r]) * W.oscale[r];
+        }
+    }
+}
+
 // launchers (pxg_mlp.hip)
 // predictions for `n` feature rows (device pointers): out[i*3 .. +3]; rows are independent
 void launch_mlp_infer(const MlpDev* net, const float* feats, float* out, uint32_t n, hipStream_t st);

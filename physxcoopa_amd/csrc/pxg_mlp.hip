@@ -22,15 +22,6 @@ namespace pxg {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return uint16_t(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 constexpr int kWavesPerBlock = 4;  // 64 rows per block
 
@@ -51,69 +42,12 @@ __global__ void __launch_bounds__(256) k_mlp_infer(const MlpDev* net, const floa
     if (t < kMlpIn) { s_shift[t] = net->in_shift[t]; s_scale[t] = net->in_scale[t]; }
     if (t < kMlpOut) s_oscale[t] = net->out_scale[t];
     __syncthreads();
-    const int wave = t >> 6, lane = t & 63;
-    const int r = lane & 15, g = lane >> 4;
-    for (uint32_t row0 = (blockIdx.x * kWavesPerBlock + wave) * 16u; row0 < ((n + 63u) & ~63u);
-         row0 += gridDim.x * kWavesPerBlock * 16u) {
-        // ---- layer 1: A = scaled features (row r, K = 4g..4g+3 of each 16-wide K step) ----
-        uint32_t row = row0 + uint32_t(r);
-        bf16x4 a1[2];
-        for (int ks = 0; ks < 2; ++ks) {
-            for (int q = 0; q < 4; ++q) {
-                int k = ks * 16 + 4 * g + q;
-                float x = 0.0f;
-                if (row < n && k < kMlpIn) x = (feats[size_t(row) * kMlpIn + k] - s_shift[k]) * s_scale[k];
-                a1[ks][q] = short(f2bf(x));
-            }
-        }
-        for (int nt = 0; nt < 4; ++nt) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int ks = 0; ks < 2; ++ks) {
-                bf16x4 b;
-                const uint16_t* wp = &s_w1[(nt * 16 + r) * kMlpInPad + ks * 16 + 4 * g];
-                b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ks], b, acc, 0, 0, 0);
-            }
-            int col = nt * 16 + r;
-            for (int i = 0; i < 4; ++i) s_h[wave][4 * g + i][col] = f2bf(fmaxf(acc[i] + s_b1[col], 0.0f));
-        }
-        __syncthreads();
-        // ---- layer 2 ----
-        bf16x4 a2[4];
-        for (int ks = 0; ks < 4; ++ks)
-            for (int q = 0; q < 4; ++q) a2[ks][q] = short(s_h[wave][r][ks * 16 + 4 * g + q]);
-        __syncthreads();
-        for (int nt = 0; nt < 4; ++nt) {
-            f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int ks = 0; ks < 4; ++ks) {
-                bf16x4 b;
-                const uint16_t* wp = &s_w2[(nt * 16 + r) * kMlpH + ks * 16 + 4 * g];
-                b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ks], b, acc, 0, 0, 0);
-            }
-            int col = nt * 16 + r;
-            for (int i = 0; i < 4; ++i) s_h[wave][4 * g + i][col] = f2bf(fmaxf(acc[i] + s_b2[col], 0.0f));
-        }
-        __syncthreads();
-        // ---- layer 3 (3 outputs padded to 16) ----
-        bf16x4 a3[4];
-        for (int ks = 0; ks < 4; ++ks)
-            for (int q = 0; q < 4; ++q) a3[ks][q] = short(s_h[wave][r][ks * 16 + 4 * g + q]);
-        __syncthreads();
-        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int ks = 0; ks < 4; ++ks) {
-            bf16x4 b;
-            const uint16_t* wp = &s_w3[r * kMlpH + ks * 16 + 4 * g];
-            b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[ks], b, acc, 0, 0, 0);
-        }
-        if (r < kMlpOut) {
-            for (int i = 0; i < 4; ++i) {
-                uint32_t orow = row0 + uint32_t(4 * g + i);
-                if (orow < n) out[size_t(orow) * kMlpOut + r] = (acc[i] + s_b3[r]) * s_oscale[r];
-            }
-        }
-    }
+    MlpView W;
+    W.w1 = s_w1; W.w2 = s_w2; W.w3 = s_w3; W.b1 = s_b1; W.b2 = s_b2; W.b3 = s_b3;
+    W.shift = s_shift; W.scale = s_scale; W.oscale = s_oscale;
+    const int wave = t >> 6;
+    for (uint32_t row0 = (blockIdx.x * kWavesPerBlock + wave) * 16u; row0 < n; row0 += gridDim.x * kWavesPerBlock * 16u)
+        mlp_rows16(W, feats, row0, n, s_h[wave], out);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -146,3 +146,45 @@ def test_batch_ragdolls_match_oracle_per_scene(oracle):
     for k in checked:
         assert np.array_equal(gj[jo[k]:jo[k + 1]].view(np.uint8), ows[k].joints().view(np.uint8))
     assert np.all(batch.stats()["error"] == 0)
+
+
+def test_batch_learned_warm_start_matches_oracle(oracle):
+    """Batched ragdolls with the learned warm start on: the network runs inside the batch kernel on
+    the matrix cores; each checked scene equals its oracle world fed the batch's contact order and
+    the batch's (clamped) predictions for its warm-start misses."""
+    from physxcoopa_amd.batch import Batch
+    from physxcoopa_amd import learned
+    rng = np.random.default_rng(3)
+    net = learned.init_params(rng.normal(0, 1, (512, 24)).astype(np.float32),
+                              np.abs(rng.normal(0, 1, (512, 3))).astype(np.float32), seed=5)
+    b, s, bo, so, jt, jo, ig, io, settings = scenes.config5_scenes(seed=9, n_scenes=8)
+    cfg = _abi.default_config(**settings)
+    batch = Batch(cfg, b, s, bo, so)
+    batch.set_joints(jo, jt)
+    batch.set_ignored_pairs(io, ig)
+    batch.set_mlp(net)
+    checked = [0, 4, 7]
+    ows = {}
+    for k in checked:
+        ow = oracle.OracleWorld(cfg, b[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]])
+        ow.set_atan2_mode(True)
+        ow.set_joints(jt[jo[k]:jo[k + 1]])
+        ow.set_ignored_pairs(ig[io[k]:io[k + 1]])
+        ows[k] = ow
+    rep = dict(checks=0, bitexact=0)
+    predicted = 0
+    for step in range(100):
+        batch.step(1)
+        for k in checked:
+            gm = batch.manifolds(k)
+            order, _ = batch.solve_order(k)
+            pairs = np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1) if len(order) else np.zeros((0, 2))
+            ows[k].set_order(pairs)
+            keys, imp = batch.predictions(k)
+            predicted += len(keys)
+            ows[k].set_miss_table(keys, imp)
+            ows[k].step_fixed()
+            compare_scene(batch, k, ows[k], rep)
+    print("learned batch parity:", rep, "predicted", predicted, flush=True)
+    assert predicted > 0
+    assert rep["bitexact"] == rep["checks"]
