@@ -1,15 +1,23 @@
 """Benchmark of the rigid-body contact step on one MI355X.
 
-Workload (N=1): the 16,384-body mixed pile (40% boxes, 40% spheres, 20% capsules over a
-200x200x1 m ground), 8 velocity / 2 relax / 3 position iterations, dt 1/60. W untimed warm-up
-steps let the pile fall and settle, then K steps are timed with the world's inputs resident in
-HBM. A "step" is one full step_fixed: broadphase, narrowphase, colouring, warm start, velocity,
+Default workload (N=1): the 16,384-body mixed pile (40% boxes, 40% spheres, 20% capsules over a
+200x200x1 m ground), dt 1/60. W untimed warm-up steps (8 velocity, 2 relax, 3 position
+iterations) let the pile fall and settle; then each solver configuration runs K timed steps in
+residual mode (velocity passes until the residual <= 1e-4, at most 256), inputs resident in HBM.
+A "step" is one full step_fixed: broadphase, narrowphase, colouring, warm start, velocity,
 relax and position passes, cache rebuild, islands/sleep and integration.
 
-Prints ONE JSON line: value = dynamic body-steps per second (whole job), the roofline of the
-dominant kernel (the velocity pass, timed with HIP events on the world's stream, algorithmic
-bytes 192 per manifold + 40 per contact point per pass), and the CPU oracle (the reference
-algorithm restated in C++, one core) timed on a bounded sample of the same settled pile.
+Two configurations of the GPU solver are measured from the same scene: plain (the reference's
+solve order, plain warm start) and batched + learned (greedy colour batches, learned warm start);
+the headline value is the latter. Prints ONE JSON line: value = dynamic body-steps per second,
+the plain solver's figures beside it, the roofline of the dominant kernel (the persistent solve
+kernel, timed with HIP events on the world's stream; algorithmic bytes 192 per manifold + 40 per
+contact point per velocity pass), and the CPU oracle (the reference algorithm restated in C++,
+one core) timed on a bounded sample of the same settled pile with the same residual policy.
+
+Other workloads: --workload batch (4096 independent scenes), ragdolls (2048 jointed ragdoll
+scenes, learned warm start on), learned (config-1 piles, passes to the residual with and without
+the learned warm start).
 """
 import argparse
 import json
@@ -45,17 +53,15 @@ def bench_batch(args):
     """Batched scenes stepped K at a time in one graph-captured persistent launch. batch: 4096
     independent scenes (half box pyramids, half random drops of the pile's shape mix) of 8-256
     bodies; ragdolls: 2048 scenes of one 15-bone ragdoll over 32 boxes (joints)."""
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; got --gpus %d" % args.gpus, file=sys.stderr)
-        sys.exit(2)
     import numpy as np
-    from physxcoopa_amd import scenes, _abi, autotune
+    from physxcoopa_amd import scenes, _abi, autotune, learned
     from physxcoopa_amd.batch import Batch
     t0 = time.time()
-    jt = jo = ig = io = None
+    jt = jo = ig = io = net = None
     if args.workload == "ragdolls":
         b, s, bo, so, jt, jo, ig, io, settings = scenes.config5_scenes(seed=5, n_scenes=2048)
         name = "config5_ragdolls_2048"
+        net = learned.MlpParams.load(learned.DEFAULT_NET)  # the learned warm start is on in this workload
     else:
         b, s, bo, so, settings = scenes.config3_scenes(seed=3, n_scenes=4096, n_min=8, n_max=256)
         name = "config3_batch_4096"
@@ -67,6 +73,8 @@ def bench_batch(args):
         if jt is not None:
             batch.set_joints(jo, jt)
             batch.set_ignored_pairs(io, ig)
+        if net is not None:
+            batch.set_mlp(net)
     (block, lpt), timings = autotune.tune(cfg, b, s, bo, so, steps=8, use_cache=False, log=log, setup=setup)
     bt = Batch(cfg, b, s, bo, so)
     setup(bt)
@@ -119,6 +127,8 @@ def bench_batch(args):
         "config": {"workload": name, "scenes": len(bo) - 1, "dynamic_bodies": n_dyn, "joints": 0 if jt is None else len(jt),
                    "velocity_iterations": int(cfg.velocity_iterations), "position_iterations": int(cfg.position_iterations),
                    "block": block, "longest_first": lpt, "steps_per_launch": chunk},
+        "learned_warm_start": net is not None,
+        "learned_predictions": int(st["learned_predictions"].sum()) if net is not None else 0,
         "autotune_seconds": timings,
         "manifolds": int(st["manifolds"].sum()), "points": int(st["points"].sum()),
         "roofline": None,
@@ -127,20 +137,246 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_learned(args):
+    """The learned warm start on config-1 piles it was not trained on, 64 of them stepped as one batch
+    for 300 steps in residual mode (velocity passes until the residual <= 1e-4, at most 256): total
+    velocity passes and time, plain warm start against learned."""
+    import numpy as np
+    from physxcoopa_amd import scenes, _abi, learned
+    from physxcoopa_amd.batch import Batch
+    net = learned.MlpParams.load(learned.DEFAULT_NET)
+    seeds = list(range(500, 564))
+    sc_list = [scenes.config1_scene(seed=k) for k in seeds]
+    b = np.ascontiguousarray(np.concatenate([sc.bodies for sc in sc_list]))
+    s = np.ascontiguousarray(np.concatenate([sc.shapes for sc in sc_list]))
+    bo = np.concatenate([[0], np.cumsum([len(sc.bodies) for sc in sc_list])]).astype(np.uint32)
+    so = np.concatenate([[0], np.cumsum([len(sc.shapes) for sc in sc_list])]).astype(np.uint32)
+    n_dyn = int(np.sum(b["type"] == 2))
+    cfg = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=256, **sc_list[0].config)
+    steps = 300
+    res = {}
+    for name, mlp in (("plain", None), ("learned", net)):
+        bt = Batch(cfg, b, s, bo, so)
+        bt.set_mlp(mlp)
+        bt.set_tuning(256, True)
+        bt.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps // 30):
+            bt.step(30)
+        bt.synchronize()
+        el = time.perf_counter() - t0
+        st = bt.stats()
+        assert np.all(st["error"] == 0), "batch reported errors"
+        res[name] = {"seconds": el, "passes": int(st["velocity_passes_total"].sum()),
+                     "residual_mean_last": float(st["residual"].mean())}
+        bt.close()
+        log("%s: %.2f s, %d velocity passes" % (name, el, res[name]["passes"]))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pxo
+        sc = sc_list[0]
+        ow = pxo.OracleWorld(cfg, sc.bodies, sc.shapes)
+        t_cpu = ow.run(steps)
+        cpu = {"value": round(sc.n_dynamic * steps / t_cpu, 1), "unit": "body·steps/s", "cores": 1, "kind": "port",
+               "sample": "one of the piles (seed %d), 300 residual-mode steps, plain warm start (C++ oracle, -O2, "
+                         "one thread)" % seeds[0]}
+    value = n_dyn * steps / res["learned"]["seconds"]
+    out = {
+        "metric": "body·steps/s at solver residual <=1e-4 (plain vs batched+learned); HBM GB/s",
+        "value": round(value, 1), "unit": "body·steps/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(1000.0 * res["learned"]["seconds"] / steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 solver, bf16 network", "data": "synthetic",
+        "config": {"workload": "config4_learned_config1_piles_x64", "piles": len(seeds), "residual_tolerance": 1e-4,
+                   "residual_max_passes": 256, "batched": True,
+                   "network": "24-64-64-3, trained on 200k records of 32 other config-1 piles at 64 iterations"},
+        "velocity_passes": {k: v["passes"] for k, v in res.items()},
+        "passes_saved": round(1.0 - res["learned"]["passes"] / max(1, res["plain"]["passes"]), 4),
+        "seconds": {k: round(v["seconds"], 3) for k, v in res.items()},
+        "residual_mean_last_step": {k: round(v["residual_mean_last"], 7) for k, v in res.items()},
+        "plain_gpu": {"value": round(n_dyn * steps / res["plain"]["seconds"], 1)},
+        "vs_plain_gpu": round(res["plain"]["seconds"] / res["learned"]["seconds"], 3),
+        "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def pile_world(sc, order, net):
+    from physxcoopa_amd import _abi
+    from physxcoopa_amd.world import World
+    cfg = _abi.default_config(**sc.config)
+    cfg.solve_order = order
+    w = World(cfg, sc.bodies, sc.shapes)
+    w.set_mlp(net)
+    return w, cfg
+
+
+def timed_steps(w, steps, per_step_stats):
+    """Times `steps` step_fixed calls (device synchronised on both sides); returns (seconds,
+    velocity passes, contact points, max residual, profile of the solve kernel)."""
+    w.set_profiling(True)
+    w.profile()  # reset
+    w.synchronize()
+    passes = pts = 0
+    resid = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.step_fixed()
+        if per_step_stats:
+            st = w.stats()
+            passes += st["velocity_passes"]
+            pts += st["points"]
+            resid = max(resid, st["residual"])
+    w.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = w.profile()
+    w.set_profiling(False)
+    if not per_step_stats:
+        st = w.stats()
+        passes, pts, resid = st["velocity_passes"] * steps, st["points"] * steps, None
+    return elapsed, passes, pts, resid, prof
+
+
+def roofline_of(prof, kernel="k_solve_persistent", traffic=None):
+    vel_s = prof["velocity_ms"] / 1000.0
+    achieved = prof["velocity_bytes"] / vel_s / 1e9 if vel_s > 0 else 0.0
+    n = max(1, prof["velocity_launches"])
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
+            "launches": int(prof["velocity_launches"]), "avg_launch_us": round(1e6 * vel_s / n, 3),
+            "bytes_per_launch": round(prof["velocity_bytes"] / n, 1)}
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the solve kernel from the committed rocprofv3 PMC summary of the same
+    workload (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "solve_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f).get(key)
+        return None if t is None else round(float(t["hbm_bytes_per_launch"]), 1)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def bench_pile(args):
+    """The 16,384-body pile, two configurations from the same scene: the plain GPU solver (the
+    reference's solve order, plain warm start) and the batched + learned one (greedy colour
+    batches, learned warm start). Each is warmed up with the fixed iteration counts (8 velocity,
+    2 relax, 3 position), timed for 60 steps at those counts, then switched to residual mode
+    (velocity passes until the residual <= 1e-4, at most 256) and timed for K steps. The headline
+    value is the batched + learned solver in residual mode."""
+    import numpy as np
+    from physxcoopa_amd import scenes, _abi, learned
+    t0 = time.time()
+    sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
+    n_dyn = sc.n_dynamic
+    log("scene %s built in %.1fs" % (sc.name, time.time() - t0))
+    net = learned.MlpParams.load(learned.DEFAULT_NET)
+    runs = {}
+    worlds = {}
+    plan = [("plain", _abi.PXG_ORDER_REFERENCE, None), ("batched_learned", _abi.PXG_ORDER_COLOURED, net)]
+    if args.only:
+        plan = [p for p in plan if p[0] == args.only]
+    for name, order, mlp in plan:
+        w, cfg = pile_world(sc, order, mlp)
+        t1 = time.time()
+        for i in range(args.warmup):
+            w.step_fixed()
+            if (i + 1) % 180 == 0:
+                w.synchronize()
+                log("%s warm-up %d/%d %.1fs" % (name, i + 1, args.warmup, time.time() - t1))
+        w.synchronize()
+        el_f, passes_f, pts_f, _, prof_f = timed_steps(w, args.fixed_steps, False)
+        res_cfg = _abi.default_config(**sc.config)
+        res_cfg.solve_order = order
+        res_cfg.residual_mode, res_cfg.residual_tolerance, res_cfg.residual_max_iterations = 1, 1e-4, 256
+        w.set_config(res_cfg)
+        el, passes, pts, resid, prof = timed_steps(w, args.steps, True)
+        st = w.stats()
+        runs[name] = {"value": n_dyn * args.steps / el, "ms_per_step": 1000.0 * el / args.steps,
+                      "contacts_per_s": pts / el, "velocity_passes_per_step": passes / args.steps,
+                      "residual_max": resid, "fixed_iterations_value": n_dyn * args.fixed_steps / el_f,
+                      "fixed_iterations_ms_per_step": 1000.0 * el_f / args.fixed_steps,
+                      "manifolds": st["manifolds"], "points": st["points"], "colours": st["colours"],
+                      "awake_bodies": st["awake_bodies"], "learned_predictions": st["learned_predictions"],
+                      "prof": prof, "prof_fixed": prof_f, "cfg": res_cfg}
+        worlds[name] = w
+        log("%s: residual mode %.2f ms/step (%.1f passes), fixed %.2f ms/step" %
+            (name, runs[name]["ms_per_step"], runs[name]["velocity_passes_per_step"], runs[name]["fixed_iterations_ms_per_step"]))
+    head = runs.get("batched_learned") or runs["plain"]
+    cpu = None
+    if not args.no_cpu and "plain" in worlds:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pxo
+        w = worlds["plain"]
+        ow = pxo.OracleWorld(runs["plain"]["cfg"], w.bodies(), sc.shapes)
+        ow.set_fat_aabbs(w.fat_aabbs())
+        k, imp = w.warm_cache()
+        ow.set_warm_cache(k, imp)
+        n_cpu, t_cpu, p_cpu = 0, 0.0, 0
+        while t_cpu < args.cpu_seconds and n_cpu < args.steps:
+            t_cpu += ow.run(1)
+            n_cpu += 1
+            p_cpu += int(ow.stats()["velocity_passes"])
+        cpu = {"value": round(n_dyn * n_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": 1, "kind": "port",
+               "sample": "%d residual-mode steps (tolerance 1e-4, cap 256; %.1f passes/step) of the settled %d-body "
+                         "pile from the plain GPU world's state (C++ oracle, -O2, one thread)" %
+                         (n_cpu, p_cpu / n_cpu, n_dyn),
+               "ms_per_step": round(1000 * t_cpu / n_cpu, 2)}
+    key = "config2_pile_16384/%s" % ("coloured_learned_residual" if head is runs.get("batched_learned") else "reference_residual")
+    out = {
+        "metric": "body·steps/s at solver residual <=1e-4 (plain vs batched+learned); HBM GB/s",
+        "value": round(head["value"], 1),
+        "unit": "body·steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": sc.name, "dynamic_bodies": n_dyn,
+                   "solver": "batched+learned GPU (greedy colour batches, learned warm start)" if "batched_learned" in runs
+                   else "plain GPU (reference order)",
+                   "residual_tolerance": 1e-4, "residual_max_passes": 256, "warmup_iterations": "8 velocity, 2 relax, 3 position"},
+        "contacts_per_s": round(head["contacts_per_s"], 1),
+        "velocity_passes_per_step": round(head["velocity_passes_per_step"], 2),
+        "residual_max": round(float(head["residual_max"]), 7),
+        "manifolds": head["manifolds"], "points": head["points"], "colours": head["colours"],
+        "awake_bodies": head["awake_bodies"], "learned_predictions_last_step": head["learned_predictions"],
+        "fixed_iterations": {k: {"value": round(r["fixed_iterations_value"], 1),
+                                 "ms_per_step": round(r["fixed_iterations_ms_per_step"], 4)} for k, r in runs.items()},
+        "plain_gpu": None if "plain" not in runs else {
+            "value": round(runs["plain"]["value"], 1), "ms_per_step": round(runs["plain"]["ms_per_step"], 4),
+            "velocity_passes_per_step": round(runs["plain"]["velocity_passes_per_step"], 2),
+            "residual_max": round(float(runs["plain"]["residual_max"]), 7)},
+        "vs_plain_gpu": round(head["value"] / runs["plain"]["value"], 3) if "plain" in runs and "batched_learned" in runs else None,
+        "vs_cpu": round(head["value"] / cpu["value"], 2) if cpu else None,
+        "roofline": roofline_of(head["prof"], traffic=measured_traffic(key)),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps (residual mode for the pile)")
     ap.add_argument("--warmup", type=int, default=540)
-    ap.add_argument("--bodies", type=int, default=16384)
-    ap.add_argument("--residual", action="store_true", help="velocity passes until residual <= 1e-4 (cap 256)")
+    ap.add_argument("--fixed-steps", type=int, default=60, help="pile: steps timed at the fixed iteration counts")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--order", choices=["reference", "coloured"], default="reference",
-                    help="solve order: the reference's manifold order (default) or the greedy colouring")
-    ap.add_argument("--workload", choices=["pile", "batch", "ragdolls"], default="pile",
+    ap.add_argument("--only", choices=["plain", "batched_learned"], default=None,
+                    help="pile: run one of the two configurations only (for profiling)")
+    ap.add_argument("--workload", choices=["pile", "batch", "ragdolls", "learned"], default="pile",
                     help="pile: the 16k-body pile (one world); batch: 4096 independent scenes of 8-256 bodies; "
-                         "ragdolls: 2048 scenes of a jointed ragdoll over 32 boxes")
+                         "ragdolls: 2048 scenes of a jointed ragdoll over 32 boxes; learned: config-1 piles, "
+                         "passes and time to the residual with and without the learned warm start")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB ten times (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -148,122 +384,16 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    if args.workload in ("batch", "ragdolls"):
-        bench_batch(args)
-        return
-
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the contact step has no multi-GPU path); "
               "got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
-    from physxcoopa_amd import scenes, _abi
-    from physxcoopa_amd.world import World
-
-    t0 = time.time()
-    lattice = (64, 64, 40) if args.bodies == 16384 else None
-    sc = scenes.config2_scene(seed=2, n=args.bodies, lattice=lattice or (64, 64, max(1, (args.bodies + 4095) // 4096 * 10)))
-    log("scene %s built in %.1fs" % (sc.name, time.time() - t0))
-    cfg = _abi.default_config(**sc.config)
-    cfg.solve_order = _abi.PXG_ORDER_REFERENCE if args.order == "reference" else _abi.PXG_ORDER_COLOURED
-    if args.residual:
-        cfg.residual_mode = 1
-        cfg.residual_tolerance = 1e-4
-        cfg.residual_max_iterations = 256
-    w = World(cfg, sc.bodies, sc.shapes)
-    n_dyn = sc.n_dynamic
-
-    t0 = time.time()
-    for i in range(args.warmup):
-        w.step_fixed()
-        if (i + 1) % 100 == 0:
-            w.synchronize()
-            st = w.stats()
-            log("warmup %d/%d  %.1fs  manifolds %d points %d colours %d awake %d" %
-                (i + 1, args.warmup, time.time() - t0, st["manifolds"], st["points"], st["colours"], st["awake_bodies"]))
-    w.synchronize()
-
-    # timed region (the velocity-pass launches are bracketed by HIP events on the world's stream)
-    w.set_profiling(True)
-    w.profile()  # reset
-    pts = 0
-    passes = 0
-    resid = []
-    w.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        w.step_fixed()
-        if args.residual:
-            st = w.stats()
-            pts += st["points"]
-            passes += st["velocity_passes"]
-            resid.append(st["residual"])
-    w.synchronize()
-    t_end = time.perf_counter()
-    prof = w.profile()
-    w.set_profiling(False)
-    st = w.stats()
-    if not args.residual:
-        pts = st["points"] * args.steps
-        passes = st["velocity_passes"] * args.steps
-    elapsed = t_end - t_start
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = n_dyn * args.steps / elapsed
-    contacts_per_s = pts / elapsed
-
-    vel_s = prof["velocity_ms"] / 1000.0
-    achieved = prof["velocity_bytes"] / vel_s / 1e9 if vel_s > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_velocity" if os.environ.get("PXG_EAGER") == "1" else "k_solve_persistent",
-                "launches": int(prof["velocity_launches"]),
-                "avg_launch_us": round(1e6 * vel_s / max(1, prof["velocity_launches"]), 3),
-                "bytes_per_launch": round(prof["velocity_bytes"] / max(1, prof["velocity_launches"]), 1)}
-
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pxo
-        ow = pxo.OracleWorld(cfg, w.bodies(), sc.shapes)
-        ow.set_fat_aabbs(w.fat_aabbs())
-        k, imp = w.warm_cache()
-        ow.set_warm_cache(k, imp)
-        n_cpu, t_cpu = 0, 0.0
-        while t_cpu < args.cpu_seconds and n_cpu < args.steps:
-            t_cpu += ow.run(1)
-            n_cpu += 1
-        cpu = {"value": round(n_dyn * n_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": 1, "kind": "port",
-               "sample": "%d steps of the settled %d-body pile from the GPU state after %d steps "
-                         "(C++ oracle, -O2, one thread)" % (n_cpu, n_dyn, args.warmup + args.steps),
-               "ms_per_step": round(1000 * t_cpu / n_cpu, 2)}
-
-    out = {
-        "metric": "body·steps/s at solver residual <=1e-4 (plain vs batched+learned); HBM GB/s",
-        "value": round(value, 1),
-        "unit": "body·steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": sc.name, "dynamic_bodies": n_dyn, "solver": "plain GPU, %s order" % args.order,
-                   "velocity_iterations": int(cfg.velocity_iterations), "relax_iterations": int(cfg.relax_iterations),
-                   "position_iterations": int(cfg.position_iterations), "residual_mode": bool(args.residual),
-                   "residual_tolerance": 1e-4 if args.residual else None},
-        "contacts_per_s": round(contacts_per_s, 1),
-        "velocity_passes_per_step": round(passes / args.steps, 2),
-        "residual_last": (round(float(np.max(resid)), 7) if resid else round(float(st["residual"]), 7)),
-        "manifolds": st["manifolds"], "points": st["points"], "colours": st["colours"],
-        "awake_bodies": st["awake_bodies"],
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    if args.workload in ("batch", "ragdolls"):
+        bench_batch(args)
+    elif args.workload == "learned":
+        bench_learned(args)
+    else:
+        bench_pile(args)
 
 
 if __name__ == "__main__":

@@ -356,7 +356,8 @@ typedef struct pxg_batch_scene_stats {
     uint32_t active_manifolds, velocity_passes, error, ordered;
     float residual;
     uint32_t awake_bodies;
-    uint32_t reserved[2];
+    uint32_t learned_predictions;   /* warm-start misses the network predicted in the last step */
+    uint32_t velocity_passes_total; /* velocity passes of every step since the batch was created */
 } pxg_batch_scene_stats;
 int pxg_batch_create(const pxg_config* cfg, uint32_t n_scenes, const uint32_t* body_offsets,
                      const uint32_t* shape_offsets, const pxg_body* bodies, const pxg_shape* shapes,

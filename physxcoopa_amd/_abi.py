@@ -75,7 +75,7 @@ assert JOINT_DTYPE.itemsize == 160
 BATCH_STATS_DTYPE = np.dtype([
     ("pairs", "<u4"), ("manifolds", "<u4"), ("points", "<u4"), ("colours", "<u4"),
     ("active_manifolds", "<u4"), ("velocity_passes", "<u4"), ("error", "<u4"), ("ordered", "<u4"),
-    ("residual", "<f4"), ("awake_bodies", "<u4"), ("reserved", "<u4", 2),
+    ("residual", "<f4"), ("awake_bodies", "<u4"), ("learned_predictions", "<u4"), ("velocity_passes_total", "<u4"),
 ])
 assert BATCH_STATS_DTYPE.itemsize == 48
 

@@ -140,8 +140,16 @@ struct Lds {
     float residual;
     uint32_t scene;
     uint32_t n_miss;                    // learned warm start: misses queued this step
-    uint16_t mlp_h[8][16][pxg::kMlpH];  // per wave: hidden activations between the network's layers
+    uint32_t passes_total;              // velocity passes summed over the steps of this launch
 };
+// The network's per-wave hidden-activation buffers (16 rows x 64 BF16 each) reuse the pair list,
+// which is dead between the narrowphase and the next step's broadphase; at most kMlpWaves waves
+// of a workgroup run the network.
+constexpr uint32_t kMlpWaves = 4;
+static_assert(kMlpWaves * 16 * pxg::kMlpH * sizeof(uint16_t) <= kMaxPairs * sizeof(uint32_t), "MLP buffers fit the pair list");
+__device__ __forceinline__ uint16_t (*mlp_buffer(Lds& L, uint32_t wave))[pxg::kMlpH] {
+    return reinterpret_cast<uint16_t (*)[pxg::kMlpH]>(reinterpret_cast<uint16_t*>(L.pairs) + wave * 16u * pxg::kMlpH);
+}
 
 }  // namespace pxb
 
@@ -601,9 +609,11 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
         const uint32_t nm = L.n_miss < kMissRows ? L.n_miss : kMissRows;
         const size_t base = size_t(L.scene) * kMissRows;
         const uint32_t wave = t >> 6;
+        const uint32_t nwaves = (TB / 64) < kMlpWaves ? (TB / 64) : kMlpWaves;
         const pxg::MlpView W = pxg::mlp_view(bd.mlp);
-        for (uint32_t row0 = wave * 16u; row0 < nm; row0 += (TB / 64) * 16u)
-            pxg::mlp_rows16(W, bd.feat + base * pxg::kMlpIn, row0, nm, L.mlp_h[wave], bd.pred + base * pxg::kMlpOut);
+        if (wave < nwaves)
+            for (uint32_t row0 = wave * 16u; row0 < nm; row0 += nwaves * 16u)
+                pxg::mlp_rows16(W, bd.feat + base * pxg::kMlpIn, row0, nm, mlp_buffer(L, wave), bd.pred + base * pxg::kMlpOut);
         __syncthreads();
         for (uint32_t r = t; r < nm; r += TB) {
             const uint32_t pt = bd.miss_pt[base + r];
@@ -982,7 +992,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
     if (t == 0) L.awake = 0;
     __syncthreads();
     if ((t & 63) == 0) atomicAdd(&L.awake, awake_n);
-    if (t == 0) { L.passes = passes; L.residual = resid; }
+    if (t == 0) { L.passes = passes; L.residual = resid; L.passes_total += passes; }
     __syncthreads();
 }
 
@@ -998,6 +1008,7 @@ __global__ void __launch_bounds__(TB) k_batch(BatchDev bd, Cfg cfg) {
             uint32_t q = atomicAdd(bd.queue, 1u);
             L.scene = q < bd.n_scenes ? bd.queue_order[q] : 0xFFFFFFFFu;
             L.err = 0;
+            L.passes_total = 0;
         }
         __syncthreads();
         const uint32_t sc = L.scene;
@@ -1048,7 +1059,7 @@ __global__ void __launch_bounds__(TB) k_batch(BatchDev bd, Cfg cfg) {
             st.active = L.active; st.passes = L.passes; st.err = L.err; st.ordered = L.ordered;
             st.residual = L.residual; st.awake = L.awake;
             st.reserved[0] = bd.mlp ? (L.n_miss < kMissRows ? L.n_miss : kMissRows) : 0u;
-            st.reserved[1] = 0;
+            st.reserved[1] = bd.stats[sc].reserved[1] + L.passes_total;  // velocity passes since creation
             bd.stats[sc] = st;
         }
     }

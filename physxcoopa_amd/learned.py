@@ -5,6 +5,7 @@ The network predicts the converged (normal, tangent1, tangent2) impulses of a co
 plain solver then runs as usual, so the residual judges the prediction.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -100,13 +101,14 @@ def train(net, X, Y, epochs=30, batch=2048, lr=2e-3, lr_decay=0.93, seed=1, weig
     return net, losses
 
 
-def record_dataset(seeds, steps=300, velocity_iterations=64, log=None):
-    """Runs config-1 piles (one per seed) with the plain solver at `velocity_iterations` and
-    returns (features, converged impulses, missed) of every active contact point of every step."""
+def record_dataset(seeds, steps=300, velocity_iterations=64, log=None, scene=None):
+    """Runs config-1 piles (one per seed; or `scene(seed)`) with the plain solver at
+    `velocity_iterations` and returns (features, converged impulses, missed) of every active contact
+    point of every step."""
     from .world import World
     Xs, Ys, Ms = [], [], []
     for k, seed in enumerate(seeds):
-        sc = scenes.config1_scene(seed=seed)
+        sc = scenes.config1_scene(seed=seed) if scene is None else scene(seed)
         cfg = _abi.default_config(**dict(sc.config, velocity_iterations=velocity_iterations))
         w = World(cfg, sc.bodies, sc.shapes)
         w.set_recording(True)
@@ -132,3 +134,36 @@ def subsample(X, Y, M, n, seed=0):
     pick = np.concatenate([miss, rng.choice(hit, size=min(k, len(hit)), replace=False)]) if k else miss[:n]
     rng.shuffle(pick)
     return X[pick], Y[pick], M[pick]
+
+
+DEFAULT_NET = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mlp_config1.npz")
+
+
+def train_config1(n_piles=32, records=200000, steps=300, epochs=30, seed0=1000, log=None):
+    """The recipe of the learned warm start: `records` contact records (every warm-start miss, topped
+    up with hits) from `n_piles` seeded config-1 piles solved at 64 velocity iterations, trained on
+    the GPU. Returns (net, losses)."""
+    X, Y, M = record_dataset(list(range(seed0, seed0 + n_piles)), steps=steps, velocity_iterations=64, log=log)
+    X, Y, M = subsample(X, Y, M, records)
+    net = init_params(X, Y, seed=0)
+    net, losses = train(net, X, Y, epochs=epochs, batch=2048, lr=2e-3)
+    return net, losses
+
+
+def main():
+    import argparse
+    import sys
+    ap = argparse.ArgumentParser(description="train the learned warm start's network on the GPU")
+    ap.add_argument("--out", default=DEFAULT_NET)
+    ap.add_argument("--piles", type=int, default=32)
+    ap.add_argument("--records", type=int, default=200000)
+    ap.add_argument("--epochs", type=int, default=30)
+    a = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    net, losses = train_config1(a.piles, a.records, epochs=a.epochs, log=log)
+    net.save(a.out)
+    log("loss %.4f -> %.4f, saved %s" % (losses[0], losses[-1], a.out))
+
+
+if __name__ == "__main__":
+    main()
