@@ -11,6 +11,8 @@
 
 #include "pxg.h"
 
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -97,6 +99,47 @@ inline pxg_config default_config() {
     return c;
 }
 // A disabled shape: the body takes part in dynamics but never in collision (world.h:85-95).
+// Generational joint handle (dynamics/joint.h:39-52).
+struct JointId {
+    static constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+    uint32_t index = kInvalid;
+    uint32_t generation = 0;
+    bool is_valid() const { return index != kInvalid; }
+    bool operator==(const JointId& o) const { return index == o.index && generation == o.generation; }
+    bool operator!=(const JointId& o) const { return !(*this == o); }
+};
+
+struct Vec3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+};
+
+// glm quaternion arithmetic in float, glm's term order (x, y, z, w storage) -- for the joints'
+// rest rotation, inverse(qa) * qb at creation (world.h:415).
+namespace quat_ops {
+inline void mul(const float* p, const float* q, float* r) {
+    const float w = p[3] * q[3] - p[0] * q[0] - p[1] * q[1] - p[2] * q[2];
+    const float x = p[3] * q[0] + p[0] * q[3] + p[1] * q[2] - p[2] * q[1];
+    const float y = p[3] * q[1] + p[1] * q[3] + p[2] * q[0] - p[0] * q[2];
+    const float z = p[3] * q[2] + p[2] * q[3] + p[0] * q[1] - p[1] * q[0];
+    r[0] = x; r[1] = y; r[2] = z; r[3] = w;
+}
+inline void inverse(const float* q, float* r) {
+    const float d = (q[0] * q[0] + q[1] * q[1]) + (q[2] * q[2] + q[3] * q[3]);
+    r[0] = -q[0] / d; r[1] = -q[1] / d; r[2] = -q[2] / d; r[3] = q[3] / d;
+}
+inline void rotate(const float* q, const float* v, float* r) {
+    auto cross = [](const float* a, const float* b, float* c) {
+        c[0] = a[1] * b[2] - b[1] * a[2];
+        c[1] = a[2] * b[0] - b[2] * a[0];
+        c[2] = a[0] * b[1] - b[0] * a[1];
+    };
+    float uv[3], uuv[3];
+    cross(q, v, uv);
+    cross(q, uv, uuv);
+    for (int i = 0; i < 3; ++i) r[i] = v[i] + ((uv[i] * q[3]) + uuv[i]) * 2.0f;
+}
+}  // namespace quat_ops
+
 inline pxg_shape no_shape() {
     pxg_shape s;
     std::memset(&s, 0, sizeof(s));
@@ -157,6 +200,23 @@ public:
 
     void remove_body(BodyId id) {
         if (!is_valid(id)) return;
+        // joints on the body stop acting at once, and its ignored pairs are forgotten, also when
+        // the removal itself is deferred (world.h:130-142, 331-333)
+        pull_joints();
+        for (auto& j : joints_) {
+            if (j.valid && (j.body_a == id.index || j.body_b == id.index)) {
+                j.valid = 0;
+                joints_dirty_ = true;
+            }
+        }
+        for (size_t k = 0; k < ignored_.size();) {
+            if (ignored_[k].first == id.index || ignored_[k].second == id.index) {
+                ignored_.erase(ignored_.begin() + long(k));
+                ignored_dirty_ = true;
+            } else {
+                ++k;
+            }
+        }
         if (in_substep_) {
             deferred_.push_back({false, id, pxg_body{}, no_shape()});
             return;
@@ -193,6 +253,107 @@ public:
         structure_dirty_ = true;
         new_shapes_.push_back(id.index);  // its proxy is re-created
         return &shapes_[id.index];
+    }
+
+    // --- joints (world.h:378-518) ---
+    // A hinge: b's local_axis_b stays aligned with a's local_axis_a; with use_limits the twist
+    // about it stays in [min_angle, max_angle] from the relative orientation at creation.
+    JointId add_hinge_joint(BodyId a, BodyId b, Vec3 local_anchor_a, Vec3 local_anchor_b, Vec3 local_axis_a,
+                            Vec3 local_axis_b, bool use_limits = false, float min_angle = 0.0f, float max_angle = 0.0f,
+                            bool collide_connected = false) {
+        if (!is_valid(a) || !is_valid(b)) return JointId{};
+        pxg_joint j = joint_base(PXG_HINGE, a, b, local_anchor_a, local_anchor_b, collide_connected);
+        set3(j.local_axis_a, local_axis_a);
+        set3(j.local_axis_b, local_axis_b);
+        j.use_limits = use_limits ? 1u : 0u;
+        j.min_angle = min_angle;
+        j.max_angle = max_angle;
+        return insert_joint(j);
+    }
+    JointId add_ball_joint(BodyId a, BodyId b, Vec3 local_anchor_a, Vec3 local_anchor_b, bool collide_connected = false) {
+        if (!is_valid(a) || !is_valid(b)) return JointId{};
+        return insert_joint(joint_base(PXG_BALL, a, b, local_anchor_a, local_anchor_b, collide_connected));
+    }
+    // b's twist axis is a's carried through the rest pose (world.h:456-477).
+    JointId add_cone_twist_joint(BodyId a, BodyId b, Vec3 local_anchor_a, Vec3 local_anchor_b, Vec3 local_twist_axis_a,
+                                 float swing_limit, float twist_min, float twist_max, bool collide_connected = false) {
+        if (!is_valid(a) || !is_valid(b)) return JointId{};
+        pxg_joint j = joint_base(PXG_CONE_TWIST, a, b, local_anchor_a, local_anchor_b, collide_connected);
+        float ax[3] = {local_twist_axis_a.x, local_twist_axis_a.y, local_twist_axis_a.z};
+        const float d = (ax[0] * ax[0] + ax[1] * ax[1]) + ax[2] * ax[2];
+        const float inv = 1.0f / std::sqrt(d);
+        for (float& c : ax) c = c * inv;
+        std::memcpy(j.local_axis_a, ax, sizeof(ax));
+        float rinv[4];
+        quat_ops::inverse(j.rest_relative_rotation, rinv);
+        quat_ops::rotate(rinv, ax, j.local_axis_b);
+        j.swing_limit = swing_limit;
+        j.twist_min = twist_min;
+        j.twist_max = twist_max;
+        return insert_joint(j);
+    }
+    // Inserts a fully described joint; its accumulators start at zero (world.h:485-493).
+    JointId add_joint(const pxg_joint& desc) {
+        if (!is_valid(BodyId{desc.body_a, generations_at(desc.body_a)}) ||
+            !is_valid(BodyId{desc.body_b, generations_at(desc.body_b)}))
+            return JointId{};
+        pxg_joint j = desc;
+        std::memset(j.point_impulse, 0, sizeof(j.point_impulse));
+        std::memset(j.axis_impulse, 0, sizeof(j.axis_impulse));
+        j.limit_impulse = j.swing_impulse = 0.0f;
+        j.valid = 1;
+        return insert_joint(j);
+    }
+    bool is_valid(JointId id) const {
+        return id.is_valid() && id.index < joints_.size() && joint_alive_[id.index] &&
+               joint_generations_[id.index] == id.generation;
+    }
+    void remove_joint(JointId id) {  // world.h:502-510
+        if (!is_valid(id)) return;
+        pull_joints();
+        joints_[id.index].valid = 0;
+        joint_alive_[id.index] = 0;
+        ++joint_generations_[id.index];
+        free_joints_.push_back(id.index);
+        joints_dirty_ = true;
+    }
+    // The joint (with the solver's accumulators), or nullptr for a stale handle (world.h:514).
+    // Writes through the pointer reach the GPU before the next step.
+    pxg_joint* get_joint(JointId id) {
+        if (!is_valid(id)) return nullptr;
+        pull_joints();
+        joints_dirty_ = true;
+        return &joints_[id.index];
+    }
+    const pxg_joint* get_joint(JointId id) const {
+        if (!is_valid(id)) return nullptr;
+        const_cast<PhysicsWorld*>(this)->pull_joints();
+        return &joints_[id.index];
+    }
+    // Contacts between two bodies off (collide false) or back on, regardless of layers
+    // (world.h:527-536).
+    void set_pair_collision(BodyId a, BodyId b, bool collide) {
+        if (!is_valid(a) || !is_valid(b) || a == b) return;
+        const std::pair<uint32_t, uint32_t> key(std::min(a.index, b.index), std::max(a.index, b.index));
+        for (size_t k = 0; k < ignored_.size(); ++k) {
+            if (ignored_[k] == key) {
+                if (collide) {
+                    ignored_.erase(ignored_.begin() + long(k));
+                    ignored_dirty_ = true;
+                }
+                return;
+            }
+        }
+        if (!collide) {
+            ignored_.push_back(key);
+            ignored_dirty_ = true;
+        }
+    }
+    bool pair_collides(BodyId a, BodyId b) const {
+        const std::pair<uint32_t, uint32_t> key(std::min(a.index, b.index), std::max(a.index, b.index));
+        for (const auto& p : ignored_)
+            if (p == key) return false;
+        return true;
     }
 
     uint32_t body_count() const {
@@ -251,9 +412,11 @@ public:
                 check(pxg_set_bodies(w_, bodies_.data(), static_cast<uint32_t>(bodies_.size())), "on_substep upload");
                 bodies_dirty_ = false;
             }
+            push_joints();
             check(pxg_step_phase(w_, PXG_PHASE_SOLVE_INTEGRATE, h), "step_fixed (solve)");
         }
         state_on_device_ = true;
+        joints_on_device_ = !joints_.empty();
         manifolds_valid_ = false;
     }
 
@@ -306,6 +469,8 @@ private:
         cfg_ = c;
         applied_cfg_ = c;
         check(pxg_set_layer_masks(w_, layers_.rows()), "layers");
+        joints_dirty_ = !joints_.empty();
+        ignored_dirty_ = !ignored_.empty();
         pending_cache_keys_ = std::move(keys);
         pending_cache_imp_ = std::move(imp);
     }
@@ -313,6 +478,63 @@ private:
 public:
 
 private:
+    static void set3(float* d, Vec3 v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+    uint32_t generations_at(uint32_t i) const { return i < generations_.size() ? generations_[i] : 0u; }
+    pxg_joint joint_base(uint32_t type, BodyId a, BodyId b, Vec3 anchor_a, Vec3 anchor_b, bool collide_connected) {
+        pull();
+        pxg_joint j;
+        std::memset(&j, 0, sizeof(j));
+        j.type = type;
+        j.body_a = a.index;
+        j.body_b = b.index;
+        j.valid = j.enabled = 1;
+        j.collide_connected = collide_connected ? 1u : 0u;
+        set3(j.local_anchor_a, anchor_a);
+        set3(j.local_anchor_b, anchor_b);
+        j.local_axis_a[2] = j.local_axis_b[2] = 1.0f;
+        j.swing_limit = -1.0f;
+        j.twist_min = 1.0f;
+        j.twist_max = -1.0f;
+        float ia[4];
+        quat_ops::inverse(bodies_[a.index].orientation, ia);
+        quat_ops::mul(ia, bodies_[b.index].orientation, j.rest_relative_rotation);
+        return j;
+    }
+    JointId insert_joint(const pxg_joint& j) {  // world.h:1663-1679
+        pull_joints();
+        uint32_t index;
+        if (!free_joints_.empty()) {
+            index = free_joints_.back();
+            free_joints_.pop_back();
+            joints_[index] = j;
+        } else {
+            index = static_cast<uint32_t>(joints_.size());
+            joints_.push_back(j);
+            joint_generations_.push_back(0);
+            joint_alive_.push_back(0);
+        }
+        joint_alive_[index] = 1;
+        joints_dirty_ = true;
+        return JointId{index, joint_generations_[index]};
+    }
+    void pull_joints() {
+        if (!joints_on_device_ || joints_.empty() || joints_dirty_) return;
+        check(pxg_get_joints(w_, joints_.data(), static_cast<uint32_t>(joints_.size())), "get_joint");
+        joints_on_device_ = false;
+    }
+    void push_joints() {
+        if (joints_dirty_) {
+            check(pxg_set_joints(w_, joints_.data(), static_cast<uint32_t>(joints_.size())), "joints");
+            joints_dirty_ = false;
+        }
+        if (ignored_dirty_) {
+            std::vector<uint32_t> ab;
+            for (const auto& p : ignored_) { ab.push_back(p.first); ab.push_back(p.second); }
+            check(pxg_set_ignored_pairs(w_, ab.data(), static_cast<uint32_t>(ignored_.size())), "ignored pairs");
+            ignored_dirty_ = false;
+        }
+    }
+
     bool alive_on_host(uint32_t i) const {
         // aliveness changes only on the host (add/remove), so the mirror's flag is current
         return bodies_[i].alive != 0;
@@ -365,6 +587,7 @@ private:
             check(pxg_set_bodies(w_, bodies_.data(), n), "bodies");
             bodies_dirty_ = false;
         }
+        push_joints();
     }
 
     struct Deferred {
@@ -385,6 +608,11 @@ private:
     std::vector<float> pending_cache_imp_;
     uint32_t uploaded_shapes_ = 0;
     bool structure_dirty_ = false, bodies_dirty_ = false, state_on_device_ = false, in_substep_ = false;
+    std::vector<pxg_joint> joints_;
+    std::vector<uint32_t> joint_generations_, free_joints_;
+    std::vector<uint8_t> joint_alive_;
+    std::vector<std::pair<uint32_t, uint32_t>> ignored_;
+    bool joints_dirty_ = false, ignored_dirty_ = false, joints_on_device_ = false;
     float accumulator_ = 0.0f;
     mutable std::vector<pxg_manifold> manifolds_;
     mutable bool manifolds_valid_ = false;

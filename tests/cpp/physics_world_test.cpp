@@ -4,6 +4,7 @@
 // tests/test_gpu_cpp_facade.py. Prints one line per test and exits non-zero on any failure.
 #include "pxg_world.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -328,6 +329,261 @@ void on_substep_slot_changes_velocity_and_defers_creation() {
     CHECK(calls == 2);
 }
 
+
+// --- joints (joints_test.cpp, sleep_test.cpp:134-175, world.h:378-536) ---
+struct V3 {
+    float x, y, z;
+};
+V3 v3(const float* p) { return V3{p[0], p[1], p[2]}; }
+V3 sub(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V3 cross(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+float length(V3 a) { return std::sqrt(dot(a, a)); }
+V3 rotate(const float* q, const float* v) {
+    float r[3];
+    pxg::quat_ops::rotate(q, v, r);
+    return V3{r[0], r[1], r[2]};
+}
+V3 world_anchor(const pxg_body& b, const float* local) {
+    V3 r = rotate(b.orientation, local);
+    return V3{b.position[0] + r.x, b.position[1] + r.y, b.position[2] + r.z};
+}
+// Twist of b relative to a about a's local axis, measured from the rest pose
+// (hinge_current_angle, solver.h:357-368).
+float twist_angle(const pxg_body& a, const pxg_body& b, const pxg_joint& j) {
+    float ia[4], rel[4], ir[4], d[4];
+    pxg::quat_ops::inverse(a.orientation, ia);
+    pxg::quat_ops::mul(ia, b.orientation, rel);
+    pxg::quat_ops::inverse(j.rest_relative_rotation, ir);
+    pxg::quat_ops::mul(rel, ir, d);
+    V3 ax = v3(j.local_axis_a);
+    const float n = length(ax);
+    ax = V3{ax.x / n, ax.y / n, ax.z / n};
+    V3 dv{d[0], d[1], d[2]};
+    float w = d[3] < -1.0f ? -1.0f : (d[3] > 1.0f ? 1.0f : d[3]);
+    if (w < 0.0f) {
+        w = -w;
+        dv = V3{-dv.x, -dv.y, -dv.z};
+    }
+    return 2.0f * std::atan2(dot(dv, ax), w);
+}
+pxg_body capsule_body(float x, float y, float z, float r, float hh, float mass) {
+    pxg_body b = body(PXG_DYNAMIC, x, y, z);
+    b.mass = mass;
+    b.inv_mass = 1.0f / mass;
+    // inertia_for_shape for a Z capsule (dynamics/inertia.h:87-107)
+    const float pi = 3.14159265358979f;
+    const float vc = pi * r * r * (2.0f * hh), vh = (4.0f / 3.0f) * pi * r * r * r;
+    const float mc = mass * vc / (vc + vh), mh = mass * vh / (vc + vh);
+    const float axial = 0.5f * mc * r * r + 0.4f * mh * r * r;
+    const float perp = mc * (3.0f * r * r + 4.0f * hh * hh) / 12.0f + 0.4f * mh * r * r + mh * hh * hh;
+    b.inv_inertia_local[0] = b.inv_inertia_local[1] = 1.0f / perp;
+    b.inv_inertia_local[2] = 1.0f / axial;
+    return b;
+}
+pxg_shape capsule_shape(float r, float hh) {
+    pxg_shape s = box_shape(0, 0, 0);
+    s.type = PXG_CAPSULE;
+    s.capsule_radius = r;
+    s.capsule_half_height = hh;
+    return s;
+}
+const float kPi = 3.14159265358979f;
+float radians(float deg) { return deg * kPi / 180.0f; }
+
+void hinge_door_swings_and_stops_at_its_limit() {
+    pxg::PhysicsWorld world;
+    world.set_gravity(0, 0, 0);
+    pxg::BodyId frame = world.add_body(body(PXG_STATIC, 0, 0, 0));
+    pxg_body d = body(PXG_DYNAMIC, 1, 0, 0);
+    d.angular_velocity[2] = 5.0f;
+    pxg::BodyId door = world.add_body(d);
+    pxg::JointId joint = world.add_hinge_joint(frame, door, {0, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, 1}, true, 0.0f,
+                                               radians(90.0f));
+    CHECK(joint.is_valid());
+    for (int i = 0; i < 300; ++i) world.step_fixed(kH);
+    const pxg_joint* j = world.get_joint(joint);
+    CHECK(j != nullptr);
+    const pxg_body& a = *world.get_body(frame);
+    const pxg_body& b = *world.get_body(door);
+    CHECK_NEAR(length(sub(world_anchor(b, j->local_anchor_b), world_anchor(a, j->local_anchor_a))), 0.0, 0.02);
+    CHECK_NEAR(rotate(b.orientation, j->local_axis_b).z, 1.0, 0.02);
+    CHECK_NEAR(twist_angle(a, b, *j), radians(90.0f), radians(3.0f));
+    CHECK(length(v3(b.angular_velocity)) < 0.05f);
+    CHECK(j->limit_impulse != 0.0f || j->point_impulse[0] != 0.0f || j->point_impulse[1] != 0.0f);
+}
+
+void hinge_free_pendulum_swings_without_drift() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId ceiling = world.add_body(body(PXG_STATIC, 0, 0, 0));
+    pxg::BodyId pend = world.add_body(body(PXG_DYNAMIC, 1, 0, 0));
+    pxg::JointId joint = world.add_hinge_joint(ceiling, pend, {0, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, 1, 0});
+    CHECK(joint.is_valid());
+    float max_drift = 0.0f, min_align = 1.0f;
+    const float start_z = world.get_body(pend)->position[2];
+    for (int i = 0; i < 120; ++i) {
+        world.step_fixed(kH);
+        const pxg_joint& j = *world.get_joint(joint);
+        const pxg_body& a = *world.get_body(ceiling);
+        const pxg_body& b = *world.get_body(pend);
+        max_drift = std::max(max_drift, length(sub(world_anchor(b, j.local_anchor_b), world_anchor(a, j.local_anchor_a))));
+        min_align = std::min(min_align, dot(rotate(a.orientation, j.local_axis_a), rotate(b.orientation, j.local_axis_b)));
+    }
+    CHECK(max_drift < 0.05f);
+    CHECK(min_align > 0.98f);
+    CHECK(world.get_body(pend)->position[2] < start_z - 0.3f);
+}
+
+void hinge_zero_range_limit_acts_rigid() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId frame = world.add_body(body(PXG_STATIC, 0, 0, 0));
+    pxg_body p = body(PXG_DYNAMIC, 1, 0, 0);
+    p.angular_velocity[2] = 5.0f;
+    pxg::BodyId panel = world.add_body(p);
+    pxg::JointId joint = world.add_hinge_joint(frame, panel, {0, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, 1}, true, 0.0f, 0.0f);
+    CHECK(joint.is_valid());
+    for (int i = 0; i < 120; ++i) world.step_fixed(kH);
+    const pxg_joint& j = *world.get_joint(joint);
+    const pxg_body& a = *world.get_body(frame);
+    const pxg_body& b = *world.get_body(panel);
+    CHECK_NEAR(twist_angle(a, b, j), 0.0, radians(3.0f));
+    CHECK_NEAR(length(sub(world_anchor(b, j.local_anchor_b), world_anchor(a, j.local_anchor_a))), 0.0, 0.05);
+}
+
+void ball_joint_holds_anchor_and_releases_on_remove() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId ceiling = world.add_body(body(PXG_STATIC, 0, 0, 0));
+    pxg_body bb = capsule_body(0, 0, -0.5f, 0.08f, 0.42f, 2.0f);
+    bb.linear_velocity[0] = 2.0f;
+    bb.linear_velocity[1] = 1.5f;
+    pxg::BodyId bob = world.add_body(bb, capsule_shape(0.08f, 0.42f));
+    pxg::JointId joint = world.add_ball_joint(ceiling, bob, {0, 0, 0}, {0, 0, 0.5f});
+    CHECK(joint.is_valid());
+    CHECK(world.get_joint(joint)->type == PXG_BALL);
+    float max_drift = 0.0f, max_x = 0.0f, max_y = 0.0f;
+    for (int i = 0; i < 180; ++i) {
+        world.step_fixed(kH);
+        const pxg_body& b = *world.get_body(bob);
+        const pxg_joint& j = *world.get_joint(joint);
+        max_drift = std::max(max_drift, length(world_anchor(b, j.local_anchor_b)));
+        max_x = std::max(max_x, std::fabs(b.position[0]));
+        max_y = std::max(max_y, std::fabs(b.position[1]));
+    }
+    CHECK(max_drift < 0.02f);
+    CHECK(max_x > 0.15f && max_y > 0.1f);
+    CHECK_NEAR(length(v3(world.get_body(bob)->position)), 0.5, 0.03);
+    world.remove_joint(joint);
+    CHECK(!world.is_valid(joint));
+    CHECK(world.get_joint(joint) == nullptr);
+    for (int i = 0; i < 30; ++i) world.step_fixed(kH);
+    CHECK(length(v3(world.get_body(bob)->position)) > 0.8f);
+}
+
+void cone_twist_respects_swing_and_twist_limits() {
+    pxg::PhysicsWorld world;
+    world.set_gravity(0, 0, 0);
+    pxg::BodyId shoulder = world.add_body(body(PXG_STATIC, 0, 0, 0));
+    pxg_body ab = capsule_body(0, 0, -0.3f, 0.06f, 0.24f, 2.0f);
+    ab.angular_velocity[0] = 6.0f;
+    ab.angular_velocity[2] = 9.0f;
+    pxg::BodyId arm = world.add_body(ab, capsule_shape(0.06f, 0.24f));
+    const float swing_limit = radians(40.0f), twist_min = radians(-20.0f), twist_max = radians(25.0f);
+    pxg::JointId joint = world.add_cone_twist_joint(shoulder, arm, {0, 0, 0}, {0, 0, 0.3f}, {0, 0, -1}, swing_limit,
+                                                    twist_min, twist_max);
+    CHECK(joint.is_valid());
+    float max_swing = 0.0f, max_twist = -10.0f, min_twist = 10.0f, max_drift = 0.0f;
+    for (int i = 0; i < 240; ++i) {
+        world.step_fixed(kH);
+        const pxg_body& a = *world.get_body(shoulder);
+        const pxg_body& b = *world.get_body(arm);
+        const pxg_joint& j = *world.get_joint(joint);
+        V3 ta = rotate(a.orientation, j.local_axis_a), tb = rotate(b.orientation, j.local_axis_b);
+        max_swing = std::max(max_swing, std::atan2(length(cross(ta, tb)), dot(ta, tb)));
+        const float tw = twist_angle(a, b, j);
+        max_twist = std::max(max_twist, tw);
+        min_twist = std::min(min_twist, tw);
+        max_drift = std::max(max_drift, length(world_anchor(b, j.local_anchor_b)));
+    }
+    const float tol = radians(4.0f);
+    CHECK(max_swing <= swing_limit + tol);
+    CHECK(max_swing >= swing_limit - tol);
+    CHECK(min_twist >= twist_min - tol);
+    CHECK(min_twist <= twist_min + tol);
+    CHECK(max_twist <= twist_max + tol);
+    CHECK(max_drift < 0.02f);
+}
+
+void joint_connected_bodies_share_a_sleep_island() {
+    pxg::PhysicsWorld world;
+    world.set_gravity(0, 0, 0);
+    pxg::BodyId a = world.add_body(body(PXG_DYNAMIC, 0, 0, 0));
+    pxg_body bb = body(PXG_DYNAMIC, 1, 0, 0);
+    bb.angular_velocity[2] = 2.0f;
+    pxg::BodyId b = world.add_body(bb);
+    pxg::JointId joint = world.add_hinge_joint(a, b, {0, 0, 0}, {-1, 0, 0}, {0, 0, 1}, {0, 0, 1});
+    CHECK(joint.is_valid());
+    for (int i = 0; i < 90; ++i) world.step_fixed(kH);
+    CHECK(length(v3(world.get_body(b)->angular_velocity)) > 0.5f);
+    CHECK(world.get_body(a)->awake);
+}
+
+// remove_body turns every joint on the body off at once (world.h:130-139); the joint slot itself
+// lives on until remove_joint, after which its stale handle is rejected when the slot is reused
+void removing_a_body_invalidates_its_joints_and_stale_joint_handles_are_rejected() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId ceiling = world.add_body(body(PXG_STATIC, 0, 0, 0));
+    pxg::BodyId bob = world.add_body(body(PXG_DYNAMIC, 0, 0, -0.5f));
+    pxg_body ob = body(PXG_DYNAMIC, 3, 0, -0.5f);
+    ob.linear_velocity[1] = 1.0f;  // keeps swinging, so it never sleeps while held
+    pxg::BodyId other = world.add_body(ob);
+    auto radius = [&]() {
+        const float* p = world.get_body(other)->position;
+        return length(V3{p[0] - 3.0f, p[1], p[2]});
+    };
+    pxg::JointId j1 = world.add_ball_joint(ceiling, bob, {0, 0, 0}, {0, 0, 0.5f});
+    pxg::JointId j2 = world.add_ball_joint(ceiling, other, {3, 0, 0}, {0, 0, 0.5f});
+    for (int i = 0; i < 10; ++i) world.step_fixed(kH);
+    CHECK_NEAR(radius(), 0.5, 0.03);
+    world.remove_body(bob);
+    CHECK(world.is_valid(j1));
+    CHECK(world.get_joint(j1)->valid == 0);
+    CHECK(world.get_joint(j2)->valid == 1);
+    for (int i = 0; i < 30; ++i) world.step_fixed(kH);
+    CHECK_NEAR(radius(), 0.5, 0.03);  // j2 still holds
+    world.remove_joint(j1);
+    pxg::JointId j3 = world.add_ball_joint(ceiling, other, {3, 0, 0}, {0, 0, 0.5f});
+    CHECK(j3.index == j1.index);
+    CHECK(!world.is_valid(j1));
+    CHECK(world.is_valid(j3));
+    CHECK(world.get_joint(j1) == nullptr);
+    world.remove_joint(j2);
+    world.remove_joint(j3);
+    for (int i = 0; i < 30; ++i) world.step_fixed(kH);
+    CHECK(world.get_body(other)->position[2] < -0.7f);  // no joint left: it falls
+}
+
+// set_pair_collision(a, b, false) drops the pair's contacts regardless of layers; turning it back
+// on restores them; pair_collides reports the setting (world.h:527-536)
+void pair_collision_switch_lets_a_sphere_fall_through_the_ground() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId ground = add_static_ground(world);
+    pxg::BodyId kept = world.add_body(body(PXG_DYNAMIC, 1, 0, 0.5f), sphere_shape(0.5f));
+    pxg::BodyId dropped = world.add_body(body(PXG_DYNAMIC, -1, 0, 0.5f), sphere_shape(0.5f));
+    CHECK(world.pair_collides(ground, dropped));
+    world.set_pair_collision(ground, dropped, false);
+    world.set_pair_collision(dropped, ground, false);  // same pair either way round
+    CHECK(!world.pair_collides(dropped, ground));
+    for (int i = 0; i < 60; ++i) world.step_fixed(kH);
+    CHECK_NEAR(world.get_body(kept)->position[2], 0.5, 0.02);
+    CHECK(world.get_body(dropped)->position[2] < -1.0f);
+    world.set_pair_collision(ground, dropped, true);
+    CHECK(world.pair_collides(ground, dropped));
+    pxg::BodyId kept2 = world.add_body(body(PXG_DYNAMIC, 2, 2, 0.5f), sphere_shape(0.5f));
+    world.set_pair_collision(ground, kept2, false);
+    world.remove_body(kept2);  // a removed body forgets its pairs
+    CHECK(world.pair_collides(ground, kept2));
+}
+
 }  // namespace
 
 int main() {
@@ -348,6 +604,15 @@ int main() {
         {"mass_ratio_100_to_1_stack_stays_stable", mass_ratio_100_to_1_stack_stays_stable},
         {"same_scene_twice_hashes_the_same", same_scene_twice_hashes_the_same},
         {"on_substep_slot_changes_velocity_and_defers_creation", on_substep_slot_changes_velocity_and_defers_creation},
+        {"hinge_door_swings_and_stops_at_its_limit", hinge_door_swings_and_stops_at_its_limit},
+        {"hinge_free_pendulum_swings_without_drift", hinge_free_pendulum_swings_without_drift},
+        {"hinge_zero_range_limit_acts_rigid", hinge_zero_range_limit_acts_rigid},
+        {"ball_joint_holds_anchor_and_releases_on_remove", ball_joint_holds_anchor_and_releases_on_remove},
+        {"cone_twist_respects_swing_and_twist_limits", cone_twist_respects_swing_and_twist_limits},
+        {"joint_connected_bodies_share_a_sleep_island", joint_connected_bodies_share_a_sleep_island},
+        {"removing_a_body_invalidates_its_joints_and_stale_joint_handles_are_rejected",
+         removing_a_body_invalidates_its_joints_and_stale_joint_handles_are_rejected},
+        {"pair_collision_switch_lets_a_sphere_fall_through_the_ground", pair_collision_switch_lets_a_sphere_fall_through_the_ground},
     };
     for (const T& t : tests) {
         int before = g_failures;
