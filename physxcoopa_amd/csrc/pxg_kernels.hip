@@ -583,9 +583,14 @@ __global__ void k_colour_init(DevState d, uint32_t n_bound) {
     if (lane0() && add) atomicAdd(&d.cnt->remaining, add);
 }
 
-__device__ __forceinline__ unsigned long long colour_key(uint32_t round, uint32_t m, uint2 ab, uint2 sh) {
+// Jones-Plassmann priority of manifold m in a round: manifolds with more contact points first
+// (they take the low colours, so the later colours hold only one-point manifolds, whose three rows
+// are a quarter of a four-point manifold's twelve: a colour lasts as long as its longest
+// manifold), then a hash of the pair, then the index.
+__device__ __forceinline__ unsigned long long colour_key(uint32_t round, uint32_t m, uint2 ab, uint2 sh, uint32_t cnt) {
     uint32_t pr = hash32(ab.x * 0x9E3779B1u ^ hash32(ab.y + 0x85EBCA77u) ^ (sh.x * 0xC2B2AE3Du)) & 0xFFFu;
-    return (static_cast<unsigned long long>(round) << 44) | (static_cast<unsigned long long>(pr) << 32) | m;
+    const unsigned long long np = cnt < 4u ? cnt : 4u;
+    return (static_cast<unsigned long long>(round) << 47) | (np << 44) | (static_cast<unsigned long long>(pr) << 32) | m;
 }
 
 __global__ void k_colour_propose(DevState d, uint32_t n_bound, uint32_t round) {
@@ -593,7 +598,7 @@ __global__ void k_colour_propose(DevState d, uint32_t n_bound, uint32_t round) {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n_bound || m >= d.cnt->manifolds || d.m_colour[m] != kUncoloured) return;
     uint2 ab = d.m_ab[m];
-    unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+    unsigned long long key = colour_key(round, m, ab, d.m_shapes[m], d.m_meta[m].x & 255u);
     if (body_active(d.b_flags[ab.x])) atomicMax(&d.col_best[ab.x], key);
     if (body_active(d.b_flags[ab.y])) atomicMax(&d.col_best[ab.y], key);
 }
@@ -604,7 +609,7 @@ __global__ void k_colour_select(DevState d, uint32_t n_bound, uint32_t round) {
     uint32_t took = 0, top = 0;
     if (m < n_bound && m < d.cnt->manifolds && d.m_colour[m] == kUncoloured) {
         uint2 ab = d.m_ab[m];
-        unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+        unsigned long long key = colour_key(round, m, ab, d.m_shapes[m], d.m_meta[m].x & 255u);
         bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
         bool mine = (!aa || d.col_best[ab.x] == key) && (!ba || d.col_best[ab.y] == key);
         if (mine) {
@@ -631,25 +636,32 @@ __global__ void k_colour_select(DevState d, uint32_t n_bound, uint32_t round) {
     }
 }
 
-// stable counting sort of manifolds by bucket (colour 0..63, 64 = skipped by the solver)
-__device__ __forceinline__ int bucket_of(uint32_t c) { return c < uint32_t(kMaxColours) ? int(c) : (c == kSkipped ? kMaxColours : -1); }
+// Stable counting sort of manifolds by bucket: colour c's multi-point manifolds (2c), then its
+// one-point ones (2c + 1), then the group the solver skips. The manifolds of one colour write
+// disjoint bodies, so their order within the colour does not change the result; grouping them by
+// point count keeps a wave's lanes on the same number of rows (a wave runs as many point rows as
+// its longest manifold has, and most manifolds have one point).
+__device__ __forceinline__ int bucket_of(uint32_t c, uint32_t cnt) {
+    if (c < uint32_t(kMaxColours)) return int(2u * c + (cnt <= 1u ? 1u : 0u));
+    return c == kSkipped ? 2 * kMaxColours : -1;
+}
 
 __global__ void k_colour_hist(DevState d, uint32_t n_man, uint32_t nblk) {
-    __shared__ uint32_t hist[kBuckets];
-    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) hist[b] = 0u;
+    __shared__ uint32_t hist[kSortBuckets];
+    for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x) hist[b] = 0u;
     __syncthreads();
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < n_man && m < d.cnt->manifolds) {
-        int b = bucket_of(d.m_colour[m]);
+        int b = bucket_of(d.m_colour[m], d.m_meta[m].x & 255u);
         if (b >= 0) atomicAdd(&hist[b], 1u);
     }
     __syncthreads();
-    for (int b = threadIdx.x; b < kBuckets; b += blockDim.x) d.col_table[b * nblk + blockIdx.x] = hist[b];
+    for (int b = threadIdx.x; b < kSortBuckets; b += blockDim.x) d.col_table[b * nblk + blockIdx.x] = hist[b];
 }
 __global__ void k_colour_scatter(DevState d, uint32_t n_man, uint32_t nblk) {
     __shared__ int bk[kBlock];
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    int b = (m < n_man && m < d.cnt->manifolds) ? bucket_of(d.m_colour[m]) : -1;
+    int b = (m < n_man && m < d.cnt->manifolds) ? bucket_of(d.m_colour[m], d.m_meta[m].x & 255u) : -1;
     bk[threadIdx.x] = b;
     __syncthreads();
     if (b < 0) return;
@@ -657,15 +669,13 @@ __global__ void k_colour_scatter(DevState d, uint32_t n_man, uint32_t nblk) {
     for (uint32_t t = 0; t < threadIdx.x; ++t) rank += (bk[t] == b) ? 1u : 0u;
     d.order[d.col_table_off[b * nblk + blockIdx.x] + rank] = m;
 }
-static_assert(kBuckets + 1 <= kBlock, "k_colour_offsets: one thread per bucket offset");
+// colour_off[c]: first solve position of colour c (c = kMaxColours: the skipped group; one past
+// it: the total)
 __global__ void k_colour_offsets(DevState d, uint32_t nblk, const uint32_t* total) {
-    int b = threadIdx.x;
-    if (b <= kBuckets) {
-        uint32_t v = b < kBuckets ? d.col_table_off[b * nblk] : *total;
-        d.cnt->colour_off[b] = v;
-    }
-    if (b == 0) {
-        d.cnt->active = d.col_table_off[kMaxColours * nblk];
+    for (int c = threadIdx.x; c <= kBuckets; c += blockDim.x)
+        d.cnt->colour_off[c] = c < kBuckets ? d.col_table_off[size_t(2 * c) * nblk] : *total;
+    if (threadIdx.x == 0) {
+        d.cnt->active = d.col_table_off[size_t(2 * kMaxColours) * nblk];
         d.cnt->ordered = *total;
     }
 }
@@ -703,6 +713,49 @@ __device__ __forceinline__ FeatBody feat_body(const DevState& d, uint32_t i) {
     return b;
 }
 
+struct SBody {
+    v3 v, w, x;
+    float im;
+    bool act;
+    m3 I;
+};
+__device__ __forceinline__ SBody load_sbody(const DevState& d, uint32_t i) {
+    SBody b;
+    float4 v = ld_dev(d.sb_v + i), w = ld_dev(d.sb_w + i);
+    float4 a = d.sb_I[3 * i], c = d.sb_I[3 * i + 1], e = d.sb_I[3 * i + 2];
+    b.v = ld3(v); b.im = v.w;
+    b.w = ld3(w); b.act = w.w != 0.0f;
+    b.I.c0 = mk(a.x, a.y, a.z); b.I.c1 = mk(a.w, c.x, c.y); b.I.c2 = mk(c.z, c.w, e.x);
+    b.x = mk(e.y, e.z, e.w);
+    return b;
+}
+__device__ __forceinline__ void store_sbody(const DevState& d, uint32_t i, const SBody& b) {
+    if (!b.act) return;
+    d.sb_v[i] = st3(b.v, b.im);
+    d.sb_w[i] = st3(b.w, 1.0f);
+}
+// The per-step constants of a contact point's rows: normal and tangent effective masses
+// (solver.h:226-277, the same expressions the rows evaluate) and the separation bias -sep / h used
+// when the separation is positive.
+__device__ __forceinline__ float4 point_row(const SBody& A, const SBody& B, v3 n, v3 pos, float sep, float h) {
+    v3 ra = pos - A.x, rb = pos - B.x;
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+    v3 ran = cross(ra, n), rbn = cross(rb, n);
+    float kn = A.im + B.im + dot(ran, mul(A.I, ran)) + dot(rbn, mul(B.I, rbn));
+    v3 ra1 = cross(ra, t1), rb1 = cross(rb, t1);
+    float k1 = A.im + B.im + dot(ra1, mul(A.I, ra1)) + dot(rb1, mul(B.I, rb1));
+    v3 ra2 = cross(ra, t2), rb2 = cross(rb, t2);
+    float k2 = A.im + B.im + dot(ra2, mul(A.I, ra2)) + dot(rb2, mul(B.I, rb2));
+    return make_float4(kn, k1, k2, sep > 0.0f ? -sep / h : 0.0f);
+}
+// The normal row's velocity target (solver.h:236-240): the separation bias when the point is
+// separated, else the restitution target in the relax passes and zero in the others.
+__device__ __forceinline__ float4 row_for_pass(float4 row, float4 aux, int use_rest) {
+    row.w = aux.x > 0.0f ? row.w : (use_rest ? aux.y : 0.0f);
+    return row;
+}
+
 __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t bound) {
     uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= bound || s >= d.cnt->ordered) return;
@@ -723,6 +776,7 @@ __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t bound) {
     int prev = d.cur ^ 1;
     bool fb_loaded = false;
     FeatBody FA, FB;
+    const SBody SA = load_sbody(d, ab.x), SB = load_sbody(d, ab.y);
     for (uint32_t t = 0; t < 4; ++t) {
         float4 pp = d.m_p[4 * m + t], qq = d.m_q[4 * m + t];
         float4 aux = make_float4(qq.x, 0.0f, qq.y, 0.0f);
@@ -757,6 +811,7 @@ __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t bound) {
         d.sp_pos[4 * s + t] = pp;
         d.sp_aux[4 * s + t] = aux;
         d.sp_imp[4 * s + t] = imp;
+        d.sp_row[4 * s + t] = t < cnt ? point_row(SA, SB, n, ld3(pp), aux.x, prm.h) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
@@ -792,27 +847,6 @@ __global__ void k_solver_bodies(DevState d) {
     store_iiw(d.sb_I, i, I, p.x, p.y, p.z);
 }
 
-struct SBody {
-    v3 v, w, x;
-    float im;
-    bool act;
-    m3 I;
-};
-__device__ __forceinline__ SBody load_sbody(const DevState& d, uint32_t i) {
-    SBody b;
-    float4 v = d.sb_v[i], w = d.sb_w[i];
-    float4 a = d.sb_I[3 * i], c = d.sb_I[3 * i + 1], e = d.sb_I[3 * i + 2];
-    b.v = ld3(v); b.im = v.w;
-    b.w = ld3(w); b.act = w.w != 0.0f;
-    b.I.c0 = mk(a.x, a.y, a.z); b.I.c1 = mk(a.w, c.x, c.y); b.I.c2 = mk(c.z, c.w, e.x);
-    b.x = mk(e.y, e.z, e.w);
-    return b;
-}
-__device__ __forceinline__ void store_sbody(const DevState& d, uint32_t i, const SBody& b) {
-    if (!b.act) return;
-    d.sb_v[i] = st3(b.v, b.im);
-    d.sb_w[i] = st3(b.w, 1.0f);
-}
 // apply_impulse_pair (solver.h:86-95)
 __device__ __forceinline__ void apply_pair(SBody& a, SBody& b, v3 p, v3 J) {
     if (a.act) { a.v = a.v - J * a.im; a.w = a.w - mul(a.I, cross(p - a.x, J)); }
@@ -870,18 +904,19 @@ __global__ void __launch_bounds__(kBlock) k_velocity(DevState d, StepParams prm,
     velocity_manifold(d, prm, s, use_rest, offset);
 }
 
-// One contact point's three rows (solver.h:226-277). Returns the updated impulses.
-__device__ __forceinline__ float4 velocity_point(SBody& A, SBody& B, v3 n, v3 t1, v3 t2, float mu, float4 pp,
-                                                 float4 aux, float4 imp, int use_rest, float h) {
-    v3 pos = ld3(pp);
+// One contact point's three rows (solver.h:226-277). Returns the updated impulses. The effective
+// masses depend only on the bodies' inverse mass, inverse inertia and position and on the point,
+// none of which a velocity pass changes, so they are computed once per step (point_row, with the
+// same expressions the rows would evaluate); row = (kn, kt1, kt2, the normal row's target).
+__device__ __forceinline__ float4 velocity_point(SBody& A, SBody& B, v3 n, v3 t1, v3 t2, float mu, v3 pos,
+                                                 float4 row, float4 imp) {
     v3 ra = pos - A.x, rb = pos - B.x;
     {
-        v3 ran = cross(ra, n), rbn = cross(rb, n);
-        float kn = A.im + B.im + dot(ran, mul(A.I, ran)) + dot(rbn, mul(B.I, rbn));
+        const float kn = row.x;
         if (kn > kEps) {
             v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
             float vn = dot(vb - va, n);
-            float target = aux.x > 0.0f ? -aux.x / h : (use_rest ? aux.y : 0.0f);
+            float target = row.w;
             float lambda = -(vn - target) / kn;
             float ni = gmax(imp.x + lambda, 0.0f);
             float delta = ni - imp.x;
@@ -892,8 +927,7 @@ __device__ __forceinline__ float4 velocity_point(SBody& A, SBody& B, v3 n, v3 t1
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         v3 dir = t == 0 ? t1 : t2;
-        v3 rat = cross(ra, dir), rbt = cross(rb, dir);
-        float kt = A.im + B.im + dot(rat, mul(A.I, rat)) + dot(rbt, mul(B.I, rbt));
+        const float kt = t == 0 ? row.y : row.z;
         if (kt <= kEps) continue;
         v3 va = A.v + cross(A.w, ra), vb = B.v + cross(B.w, rb);
         float vt = dot(vb - va, dir);
@@ -919,12 +953,12 @@ __device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint
     uint32_t idx[4];
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) idx[k] = 4u * s + (k < cnt ? (k + r) % cnt : k);
-    float4 pp[4], aux[4], imp[4];
+    float4 pp[4], row[4], imp[4];
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         pp[k] = d.sp_pos[idx[k]];
-        aux[k] = d.sp_aux[idx[k]];
-        imp[k] = d.sp_imp[idx[k]];
+        row[k] = row_for_pass(d.sp_row[idx[k]], d.sp_aux[idx[k]], use_rest);
+        imp[k] = ld_dev(d.sp_imp + idx[k]);
     }
     SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
     v3 n = ld3(nf);
@@ -933,7 +967,7 @@ __device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         if (k >= cnt) break;
-        imp[k] = velocity_point(A, B, n, t1, t2, nf.w, pp[k], aux[k], imp[k], use_rest, prm.h);
+        imp[k] = velocity_point(A, B, n, t1, t2, nf.w, ld3(pp[k]), row[k], imp[k]);
     }
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k)
@@ -955,8 +989,18 @@ __device__ float residual_manifold(const DevState& d, const StepParams& prm, uin
         v3 t1, t2;
         orthonormal_basis(n, t1, t2);
         uint32_t cnt = d.sm_cnt[s];
-        for (uint32_t i = 0; i < cnt; ++i) {
-            float4 pp = d.sp_pos[4 * s + i], aux = d.sp_aux[4 * s + i], imp = d.sp_imp[4 * s + i];
+        // slots are padded to four points: all loads go out together
+        float4 pp4[4], aux4[4], imp4[4];
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            pp4[i] = d.sp_pos[4 * s + i];
+            aux4[i] = d.sp_aux[4 * s + i];
+            imp4[i] = ld_dev(d.sp_imp + 4 * s + i);
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            if (i >= cnt) break;
+            const float4 pp = pp4[i], aux = aux4[i], imp = imp4[i];
             v3 pos = ld3(pp);
             v3 ra = pos - A.x, rb = pos - B.x;
             v3 dv = (B.v + cross(B.w, rb)) - (A.v + cross(A.w, ra));
@@ -1002,9 +1046,9 @@ __device__ void position_manifold(const DevState& d, const StepParams& prm, uint
     float pen4[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) pen4[t] = d.sp_pos[4 * s + t].w;
-    float4 xa4 = d.sb_x[ab.x], xb4 = d.sb_x[ab.y];
-    bool aa = d.sb_w[ab.x].w != 0.0f, ba = d.sb_w[ab.y].w != 0.0f;
-    v3 ra = ld3(d.sb_ref[ab.x]), rb = ld3(d.sb_ref[ab.y]);
+    float4 xa4 = ld_dev(d.sb_x + ab.x), xb4 = ld_dev(d.sb_x + ab.y);
+    bool aa = ld_dev(d.sb_w + ab.x).w != 0.0f, ba = ld_dev(d.sb_w + ab.y).w != 0.0f;
+    v3 ra = ld3(ld_dev(d.sb_ref + ab.x)), rb = ld3(ld_dev(d.sb_ref + ab.y));
     float ima = xa4.w, imb = xb4.w;
     float k = ima + imb;
     if (k <= kEps) return;
@@ -1311,13 +1355,13 @@ __global__ void k_integrate_velocities(DevState d, StepParams p) {
 // bodies as the velocity passes and the warm start see them: the solver bodies
 __device__ __forceinline__ JBody jb_solver(const DevState& d, uint32_t i) {
     JBody b;
-    float4 v = d.sb_v[i], w = d.sb_w[i];
+    float4 v = ld_dev(d.sb_v + i), w = ld_dev(d.sb_w + i);
     float4 a = d.sb_I[3 * i], c = d.sb_I[3 * i + 1], e = d.sb_I[3 * i + 2];
     b.v = ld3(v); b.im = v.w;
     b.w = ld3(w); b.act = w.w != 0.0f;
     b.I.c0 = mk(a.x, a.y, a.z); b.I.c1 = mk(a.w, c.x, c.y); b.I.c2 = mk(c.z, c.w, e.x);
     b.x = mk(e.y, e.z, e.w);
-    b.q = ldq(d.b_rot[i]);
+    b.q = ldq(ld_dev(d.b_rot + i));
     return b;
 }
 __device__ __forceinline__ void jb_solver_store(const DevState& d, uint32_t i, const JBody& b) {
@@ -1328,7 +1372,7 @@ __device__ __forceinline__ void jb_solver_store(const DevState& d, uint32_t i, c
 // the position pass of the solve: positions from the position-pass state
 __device__ __forceinline__ JBody jb_position(const DevState& d, uint32_t i) {
     JBody b = jb_solver(d, i);
-    b.x = ld3(d.sb_x[i]);
+    b.x = ld3(ld_dev(d.sb_x + i));
     return b;
 }
 __device__ __forceinline__ void jb_position_store(const DevState& d, uint32_t i, const JBody& b) {
@@ -1476,37 +1520,59 @@ constexpr uint32_t ERR_BARRIER = 64u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 // Counting barrier: bar[0] is zeroed before each launch and counts arrivals; barrier k of the
-// launch is complete when k * nblocks blocks have arrived. One release add per block, relaxed
-// polling, one acquire fence after the wait (an acquiring poll would invalidate caches on every
-// spin). Returns true when the launch must end: a wait expired (ERR_BARRIER is then set, and
-// every other block leaves at its next wait).
+// launch is complete when k * nblocks blocks have arrived. Returns true when the launch must end:
+// a wait expired (ERR_BARRIER is then set, and every other block leaves at its next wait).
+//
+// Two forms. Far (blocks on several XCDs, each with its own L2): one release add per block
+// (writes this XCD's dirty L2 lines back), relaxed polling, one acquire fence after the wait
+// (invalidates stale L2 lines; an acquiring poll would do that on every spin). Near (every block
+// on one XCD, so one L2 serves them all): every wave waits until its stores have completed at the
+// L2 (wait_stores; __syncthreads alone does not wait for global stores), and every load of data
+// written during the launch is a device-scope load (ld_dev) that bypasses the L1, so the add and
+// the poll are relaxed and no cache is written back or invalidated. On one XCD this costs ~1.2 us per barrier where the far
+// form costs 16-21 us (32 blocks taking turns at one L2's write-back and invalidation).
 struct GridBarrier {
     uint32_t* count;
     uint32_t nblocks;
     uint32_t epoch;  // barriers passed (thread 0's copy)
+    uint32_t near;
 };
+// Waits until this wave's global stores have completed (s_waitcnt vmcnt(0); the expcnt and
+// lgkmcnt fields of the GFX9 encoding at their maxima wait for nothing). The signal fences keep
+// the compiler from moving memory accesses across the wait.
+__device__ __forceinline__ void wait_stores() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+__device__ __forceinline__ bool barrier_poll(const GridBarrier& g, uint32_t* err) {
+    const uint32_t target = g.epoch * g.nblocks;
+    uint32_t spins = 0;
+    while (__hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if (++spins > kSpinLimit) {
+            atomicOr(err, ERR_BARRIER);
+            return true;
+        }
+        if ((spins & 255u) == 0u && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER))
+            return true;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return false;
+}
+__device__ __forceinline__ void barrier_add(GridBarrier& g) {
+    g.epoch += 1;
+    if (g.near) __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ bool grid_sync(GridBarrier& g, uint32_t* err) {
     __shared__ uint32_t s_abort;
+    if (g.near) wait_stores();
     __syncthreads();
     if (threadIdx.x == 0) {
-        g.epoch += 1;
-        __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t target = g.epoch * g.nblocks;
-        uint32_t spins = 0, abort = 0;
-        while (__hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > kSpinLimit) {
-                atomicOr(err, ERR_BARRIER);
-                abort = 1;
-                break;
-            }
-            if ((spins & 255u) == 0u && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)) {
-                abort = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        s_abort = abort;
+        barrier_add(g);
+        const bool abort = barrier_poll(g, err);
+        if (!g.near) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_abort = abort ? 1u : 0u;
     }
     __syncthreads();
     return s_abort != 0u;
@@ -1515,31 +1581,16 @@ __device__ __forceinline__ bool grid_sync(GridBarrier& g, uint32_t* err) {
 // grid_wait blocks until every block has arrived. Work between the two (loads of data no other
 // block writes in this phase) overlaps the barrier's latency.
 __device__ __forceinline__ void grid_arrive(GridBarrier& g) {
+    if (g.near) wait_stores();
     __syncthreads();
-    if (threadIdx.x == 0) {
-        g.epoch += 1;
-        __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) barrier_add(g);
 }
 __device__ __forceinline__ bool grid_wait(GridBarrier& g, uint32_t* err) {
     __shared__ uint32_t s_abort_w;
     if (threadIdx.x == 0) {
-        const uint32_t target = g.epoch * g.nblocks;
-        uint32_t spins = 0, abort = 0;
-        while (__hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > kSpinLimit) {
-                atomicOr(err, ERR_BARRIER);
-                abort = 1;
-                break;
-            }
-            if ((spins & 255u) == 0u && (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)) {
-                abort = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        s_abort_w = abort;
+        const bool abort = barrier_poll(g, err);
+        if (!g.near) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_abort_w = abort ? 1u : 0u;
     }
     __syncthreads();
     return s_abort_w != 0u;
@@ -1549,7 +1600,43 @@ __device__ __forceinline__ GridBarrier grid_barrier_init(uint32_t* bar) {
     g.count = bar;
     g.nblocks = gridDim.x;
     g.epoch = 0;
+    g.near = 0;
     return g;
+}
+
+// The XCD (accelerator complex die) this wave runs on: hardware register XCC_ID, bits 3:0.
+__device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg(20 | (3 << 11)) & 15u; }
+
+// Near-barrier set-up of a launch with blocks on every XCD: each block counts itself in
+// bar[2 + its XCD] and in bar[1]; once all have, the blocks of the lowest-numbered XCD that has
+// any carry on, ranked by their arrival, and the others leave. Returns false for a block that
+// leaves (or when the wait expired, with ERR_BARRIER set).
+__device__ __forceinline__ bool near_blocks(uint32_t* bar, uint32_t* err, uint32_t& rank, uint32_t& nblocks) {
+    __shared__ uint32_t s_go, s_rank, s_n;
+    if (threadIdx.x == 0) {
+        const uint32_t x = xcc_id();
+        const uint32_t r = __hip_atomic_fetch_add(bar + 2 + x, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        GridBarrier reg;
+        reg.count = bar + 1;
+        reg.nblocks = gridDim.x;
+        reg.epoch = 0;
+        reg.near = 0;
+        barrier_add(reg);
+        const bool abort = barrier_poll(reg, err);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        uint32_t chosen = 16u, n = 0u;
+        for (uint32_t k = 0; k < 16u && !abort; ++k) {
+            const uint32_t c = __hip_atomic_load(bar + 2 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c && chosen == 16u) { chosen = k; n = c; }
+        }
+        s_go = !abort && x == chosen ? 1u : 0u;
+        s_rank = r;
+        s_n = n;
+    }
+    __syncthreads();
+    rank = s_rank;
+    nblocks = s_n;
+    return s_go != 0u;
 }
 
 __device__ __forceinline__ uint32_t n_active_all(const DevState& d) { return d.cnt->active; }
@@ -1558,22 +1645,23 @@ struct SolveCfg {
     uint32_t vel_iters, relax_iters, pos_iters;
     uint32_t residual_mode, residual_max;
     float residual_tol;
+    uint32_t one_xcd;  // launched on every XCD; the blocks of one carry on with near barriers
 };
 
-// A velocity-pass manifold with everything that does not change during the pass loaded ahead:
-// its slot (points in the pass's point order) and both bodies' inverse inertia and position. Only
-// the two bodies' velocities are read after the barrier that orders this colour after the last.
+// A velocity-pass manifold's slot loaded ahead (points in the pass's point order, rows with the
+// pass's targets): nothing in it changes during the pass. The two bodies are read after the
+// barrier that orders this colour after the last (their inertia loads go out with the velocity
+// loads, so holding them across the barrier would only cost registers).
 struct PreSlot {
     uint32_t s;  // kNoSlot: none
     uint2 ab;
     float4 nf;
     uint32_t cnt;
     uint32_t idx[4];
-    float4 pp[4], aux[4], imp[4];
-    float4 Ia[3], Ib[3];
+    float4 pp[4], row[4], imp[4];
 };
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
-__device__ __forceinline__ void prefetch_slot(const DevState& d, uint32_t s, uint32_t offset, PreSlot& x) {
+__device__ __forceinline__ void prefetch_slot(const DevState& d, uint32_t s, uint32_t offset, int use_rest, PreSlot& x) {
     x.s = s;
     x.ab = d.sm_ab[s];
     x.nf = d.sm_nf[s];
@@ -1584,33 +1672,20 @@ __device__ __forceinline__ void prefetch_slot(const DevState& d, uint32_t s, uin
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         x.pp[k] = d.sp_pos[x.idx[k]];
-        x.aux[k] = d.sp_aux[x.idx[k]];
-        x.imp[k] = d.sp_imp[x.idx[k]];
+        x.row[k] = row_for_pass(d.sp_row[x.idx[k]], d.sp_aux[x.idx[k]], use_rest);
+        x.imp[k] = ld_dev(d.sp_imp + x.idx[k]);
     }
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        x.Ia[j] = d.sb_I[3 * x.ab.x + j];
-        x.Ib[j] = d.sb_I[3 * x.ab.y + j];
-    }
-}
-__device__ __forceinline__ SBody sbody_from(float4 v, float4 w, const float4 (&I)[3]) {
-    SBody b;
-    b.v = ld3(v); b.im = v.w;
-    b.w = ld3(w); b.act = w.w != 0.0f;
-    b.I.c0 = mk(I[0].x, I[0].y, I[0].z); b.I.c1 = mk(I[0].w, I[1].x, I[1].y); b.I.c2 = mk(I[1].z, I[1].w, I[2].x);
-    b.x = mk(I[2].y, I[2].z, I[2].w);
-    return b;
 }
 // velocity_manifold for a prefetched slot (same arithmetic, same order)
-__device__ void velocity_prefetched(const DevState& d, const StepParams& prm, PreSlot& x, int use_rest) {
-    SBody A = sbody_from(d.sb_v[x.ab.x], d.sb_w[x.ab.x], x.Ia), B = sbody_from(d.sb_v[x.ab.y], d.sb_w[x.ab.y], x.Ib);
+__device__ void velocity_prefetched(const DevState& d, PreSlot& x) {
+    SBody A = load_sbody(d, x.ab.x), B = load_sbody(d, x.ab.y);
     v3 n = ld3(x.nf);
     v3 t1, t2;
     orthonormal_basis(n, t1, t2);
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
         if (k >= x.cnt) break;
-        x.imp[k] = velocity_point(A, B, n, t1, t2, x.nf.w, x.pp[k], x.aux[k], x.imp[k], use_rest, prm.h);
+        x.imp[k] = velocity_point(A, B, n, t1, t2, x.nf.w, ld3(x.pp[k]), x.row[k], x.imp[k]);
     }
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k)
@@ -1625,9 +1700,19 @@ __device__ void velocity_prefetched(const DevState& d, const StepParams& prm, Pr
 __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, StepParams prm, SolveCfg cfg, uint32_t* bar) {
     GridBarrier g = grid_barrier_init(bar);
     uint32_t* err = &d.cnt->err;
+    uint32_t rank = blockIdx.x;
+    if (cfg.one_xcd) {
+        if (!near_blocks(bar, err, rank, g.nblocks)) return;
+        g.near = 1;
+    }
     const uint32_t C = d.cnt->colours;
     const uint32_t* co = d.cnt->colour_off;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
+    // Which thread takes which manifold of a colour does not change the result (the manifolds of
+    // one colour write disjoint bodies). Consecutive 64-manifold chunks go to consecutive blocks,
+    // so a colour narrower than the grid is spread over every CU instead of filling the first
+    // blocks' waves while the others idle.
+    const uint32_t tid = ((threadIdx.x >> 6) * g.nblocks + rank) * 64u + (threadIdx.x & 63u);
+    const uint32_t nthreads = g.nblocks * blockDim.x;
     for (uint32_t c = 0; c < C; ++c) {
         for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) warm_manifold(d, s);
         if (grid_sync(g, err)) return;
@@ -1665,7 +1750,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
         const uint32_t nph = (cfg.vel_iters + cfg.relax_iters) * per;
         PreSlot x;
         x.s = kNoSlot;
-        if (nph && C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, x);
+        if (nph && C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, cfg.vel_iters == 0u ? 1 : 0, x);
         for (uint32_t ph = 0; ph < nph; ++ph) {
             const uint32_t pass = ph / per, k = ph % per;
             const int use_rest = pass >= cfg.vel_iters ? 1 : 0;
@@ -1678,7 +1763,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
                     bytes = wave_sum(bytes);
                     if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
                 }
-                if (x.s != kNoSlot) velocity_prefetched(d, prm, x, use_rest);
+                if (x.s != kNoSlot) velocity_prefetched(d, x);
                 for (uint32_t s = co[c] + tid + nthreads; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, use_rest, it);
             } else {
                 uint32_t jb, je;
@@ -1691,13 +1776,19 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (ph + 1 < nph) {
                 const uint32_t np = (ph + 1) / per, nk = (ph + 1) % per;
                 const uint32_t nit = np >= cfg.vel_iters ? np - cfg.vel_iters : np;
-                if (nk < C && co[nk] + tid < co[nk + 1]) prefetch_slot(d, co[nk] + tid, nit, x);
+                if (nk < C && co[nk] + tid < co[nk + 1]) prefetch_slot(d, co[nk] + tid, nit, np >= cfg.vel_iters ? 1 : 0, x);
             }
             if (grid_wait(g, err)) return;
         }
         if (tid == 0) d.cnt->passes = cfg.vel_iters;
     }
+    // residual mode: velocity passes until the residual <= tolerance; as above, each colour's
+    // barrier overlaps the load of this thread's first slot of the next colour (of the next pass
+    // after the residual's barrier: loads only, harmless when the loop ends there)
     uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : 0u;
+    PreSlot y;
+    y.s = kNoSlot;
+    if (max_pass && C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, 0, y);
     for (uint32_t it = 0; it < max_pass; ++it) {
         for (uint32_t c = 0; c < C; ++c) {
             uint32_t b = co[c], e = co[c + 1];
@@ -1707,8 +1798,12 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
                 bytes = wave_sum(bytes);
                 if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
             }
-            for (uint32_t s = b + tid; s < e; s += nthreads) velocity_manifold(d, prm, s, 0, it);
-            if (grid_sync(g, err)) return;
+            if (y.s != kNoSlot) velocity_prefetched(d, y);
+            for (uint32_t s = b + tid + nthreads; s < e; s += nthreads) velocity_manifold(d, prm, s, 0, it);
+            grid_arrive(g);
+            y.s = kNoSlot;
+            if (c + 1 < C && co[c + 1] + tid < co[c + 2]) prefetch_slot(d, co[c + 1] + tid, it, 0, y);
+            if (grid_wait(g, err)) return;
         }
         for (uint32_t l = 0; l < L; ++l) {
             uint32_t jb, je;
@@ -1718,26 +1813,28 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (grid_sync(g, err)) return;
         }
         ++passes;
-        if (cfg.residual_mode) {
-            float r = 0.0f;
-            for (uint32_t s = tid; s < n_active; s += nthreads) r = gmax(r, residual_manifold(d, prm, s, 0));
-            for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
-            if (lane0()) s_red[threadIdx.x >> 6] = r;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                float m = 0.0f;
-                for (int w = 0; w < kPersistBlock / 64; ++w) m = fmaxf(m, s_red[w]);
-                if (m > 0.0f) atomicMax(&d.cnt->res_slot[it & 1], __float_as_uint(m));
-            }
-            if (grid_sync(g, err)) return;
-            float rr = __uint_as_float(__hip_atomic_load(&d.cnt->res_slot[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            if (tid == 0) {
-                d.cnt->residual_bits = __float_as_uint(rr);
-                d.cnt->res_slot[(it + 1) & 1] = 0u;  // the slot the next pass reduces into
-            }
-            if (grid_sync(g, err)) return;
-            if (rr <= cfg.residual_tol) break;
+        float r = 0.0f;
+        for (uint32_t s = tid; s < n_active; s += nthreads) r = gmax(r, residual_manifold(d, prm, s, 0));
+        for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+        if (lane0()) s_red[threadIdx.x >> 6] = r;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float m = 0.0f;
+            for (int w = 0; w < kPersistBlock / 64; ++w) m = fmaxf(m, s_red[w]);
+            if (m > 0.0f) atomicMax(&d.cnt->res_slot[it & 1], __float_as_uint(m));
         }
+        grid_arrive(g);
+        if (C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, it + 1u, 0, y);
+        if (grid_wait(g, err)) return;
+        // every block's maximum is in: all threads read the same value. The other slot is
+        // cleared for the next pass: its last readers read it before this pass's first barrier,
+        // and its next writers add to it only after the next pass's colour barriers.
+        float rr = __uint_as_float(__hip_atomic_load(&d.cnt->res_slot[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (tid == 0) {
+            d.cnt->residual_bits = __float_as_uint(rr);
+            d.cnt->res_slot[(it + 1) & 1] = 0u;
+        }
+        if (rr <= cfg.residual_tol) break;
     }
     if (cfg.residual_mode && tid == 0) d.cnt->passes = passes;
     for (uint32_t it = 0; cfg.residual_mode && it < cfg.relax_iters; ++it) {
@@ -1762,7 +1859,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     // solve_position: snapshot, then position passes
     for (uint32_t i = tid; i < d.nb; i += nthreads) {
         float4 p = d.b_pos[i];
-        d.sb_x[i] = make_float4(p.x, p.y, p.z, d.sb_v[i].w);
+        d.sb_x[i] = make_float4(p.x, p.y, p.z, ld_dev(d.sb_v + i).w);
         d.sb_ref[i] = p;
     }
     if (grid_sync(g, err)) return;
@@ -1911,13 +2008,15 @@ __device__ void block_scan_range(const uint32_t* in, uint32_t* out, uint32_t n) 
 // warm-start entry). A hinted colour is kept unless another hinted manifold claims the same colour
 // at the same awake dynamic body, in which case every claimant drops it; the outcome does not
 // depend on timing. Jones-Plassmann rounds (first-fit over the per-body masks) colour the rest.
+// reference: 0 = coloured order starting from last step's colours, 1 = the reference order's
+// chain layers, 2 = coloured order from scratch (Jones-Plassmann rounds only).
 __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d, uint32_t max_rounds, uint32_t reference,
                                                                    uint32_t* bar) {
     GridBarrier g = grid_barrier_init(bar);
     uint32_t* err = &d.cnt->err;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x, nthreads = gridDim.x * blockDim.x;
     const uint32_t n_man = d.cnt->manifolds;
-    if (reference) {
+    if (reference == 1u) {
         for (uint32_t m = tid; m < n_man; m += nthreads) ref_count(d, m);
         if (grid_sync(g, err)) return;
         if (blockIdx.x == 0) block_scan_range(d.adj_cnt, d.adj_start, d.nb);
@@ -1942,8 +2041,8 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
     }
     const int prev = d.cur ^ 1;
     constexpr uint32_t kHinted = 0x80000000u;
-    // A: claim last step's colour
-    for (uint32_t m = tid; m < n_man; m += nthreads) {
+    // A: claim last step's colour (mode 2: colour afresh, skipping A and B)
+    for (uint32_t m = tid; reference == 0u && m < n_man; m += nthreads) {
         if (d.m_colour[m] != kUncoloured) continue;
         uint2 ab = d.m_ab[m];
         float4 cv;
@@ -1957,7 +2056,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
     }
     if (grid_sync(g, err)) return;
     // B: keep the uncontested claims; a dropped claim clears its bit (no kept manifold shares it)
-    {
+    if (reference == 0u) {
         uint32_t base = tid - (tid & 63u);
         for (uint32_t m0 = base; m0 < n_man; m0 += nthreads) {
             uint32_t m = m0 + (tid & 63u);
@@ -1994,7 +2093,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
         for (uint32_t m = tid; m < n_man; m += nthreads) {
             if (d.m_colour[m] != kUncoloured) continue;
             uint2 ab = d.m_ab[m];
-            unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+            unsigned long long key = colour_key(round, m, ab, d.m_shapes[m], d.m_meta[m].x & 255u);
             if (body_active(d.b_flags[ab.x])) atomicMax(&d.col_best[ab.x], key);
             if (body_active(d.b_flags[ab.y])) atomicMax(&d.col_best[ab.y], key);
         }
@@ -2005,7 +2104,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
             uint32_t took = 0, top = 0;
             if (m < n_man && d.m_colour[m] == kUncoloured) {
                 uint2 ab = d.m_ab[m];
-                unsigned long long key = colour_key(round, m, ab, d.m_shapes[m]);
+                unsigned long long key = colour_key(round, m, ab, d.m_shapes[m], d.m_meta[m].x & 255u);
                 bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
                 bool mine = (!aa || d.col_best[ab.x] == key) && (!ba || d.col_best[ab.y] == key);
                 if (mine) {
@@ -2053,13 +2152,20 @@ void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipS
     uint32_t g = uint32_t(std::min<size_t>((n + kBlock - 1) / kBlock, 4096));
     hipLaunchKernelGGL(k_fill_u64, dim3(g), dim3(kBlock), 0, st, p, v, n);
 }
-// exclusive scan of n <= 1M elements; scratch holds >= 1024 words
+// exclusive scan of n elements (in == out allowed). scratch holds the block sums: ceil(n / 1024)
+// words (>= 1), plus, when n > 2^20, what the scan of those block sums needs in turn.
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* scratch, hipStream_t st) {
     uint32_t nb = (n + 1023) / 1024;
     if (nb == 0) nb = 1;
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, st, in, n, scratch);
-    hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kBlock), 0, st, scratch, nb, total);
+    if (nb <= 1024) hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(kBlock), 0, st, scratch, nb, total);
+    else launch_scan(scratch, scratch, nb, total, scratch + nb, st);
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, st, in, out, n, scratch);
+}
+// scratch words launch_scan needs for n elements
+static size_t scan_scratch_words(size_t n) {
+    size_t nb = std::max<size_t>(1, (n + 1023) / 1024);
+    return nb <= 1024 ? nb : nb + scan_scratch_words(nb);
 }
 
 void launch_init_bodies(const DevState& d, hipStream_t st) {
@@ -2124,21 +2230,27 @@ void launch_colour_rounds(const DevState& d, uint32_t n_man, uint32_t first_roun
         hipLaunchKernelGGL(k_colour_select, dim3(blocks(n_man)), dim3(kBlock), 0, st, d, n_man, first_round + r);
     }
 }
+size_t col_table_words(size_t max_blocks) {
+    const size_t n = size_t(kSortBuckets) * max_blocks;
+    return n + scan_scratch_words(n) + 1;
+}
 void launch_build_order(const DevState& d, uint32_t n_man, hipStream_t st) {
     uint32_t nblk = blocks(n_man ? n_man : 1);
     if (n_man) hipLaunchKernelGGL(k_colour_hist, dim3(nblk), dim3(kBlock), 0, st, d, n_man, nblk);
-    else launch_fill_u32(d.col_table, 0u, size_t(kBuckets) * nblk, st);
+    else launch_fill_u32(d.col_table, 0u, size_t(kSortBuckets) * nblk, st);
     // col_table_off doubles as scan scratch elsewhere: scan into it, block sums in col_table's tail
-    uint32_t n = kBuckets * nblk;
+    // (sized by col_table_words)
+    uint32_t n = kSortBuckets * nblk;
     uint32_t* scratch = d.col_table + n;
-    uint32_t* total = scratch + 1024;
+    uint32_t* total = scratch + scan_scratch_words(n);
     launch_scan(d.col_table, d.col_table_off, n, total, scratch, st);
     if (n_man) hipLaunchKernelGGL(k_colour_scatter, dim3(nblk), dim3(kBlock), 0, st, d, n_man, nblk);
     hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(kBlock), 0, st, d, nblk, total);
 }
 void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st) {
-    if (bound) hipLaunchKernelGGL(k_solver_setup, dim3(blocks(bound)), dim3(kBlock), 0, st, d, p, bound);
+    // solver bodies first: the point rows are computed from them
     if (d.nb) hipLaunchKernelGGL(k_solver_bodies, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+    if (bound) hipLaunchKernelGGL(k_solver_setup, dim3(blocks(bound)), dim3(kBlock), 0, st, d, p, bound);
 }
 void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st) {
     if (!n_bound || !d.mlp) return;
@@ -2195,7 +2307,8 @@ hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const
     SolveCfg c;
     c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
     c.residual_mode = cfgv[3]; c.residual_max = cfgv[4]; c.residual_tol = tol;
-    launch_fill_u32(d.bar, 0u, 1, st);
+    c.one_xcd = cfgv[5];
+    launch_fill_u32(d.bar, 0u, kBarWords, st);
     DevState dd = d;
     StepParams pp = p;
     uint32_t* bar = d.bar;

@@ -11,10 +11,17 @@ namespace pxg {
 
 struct MlpDev;
 constexpr int kBlock = 256;
+// grid barrier words: [0] arrivals, [1] set-up arrivals, [2 + x] blocks on XCD x (near barriers)
+constexpr uint32_t kBarWords = 18;
+// XCDs of an MI355X: a one-XCD solve launches this many blocks per block that carries on
+constexpr uint32_t kXcds = 8;
 constexpr int kPersistBlock = 512;  // persistent cooperative kernels: 8 waves per block (up to 256 VGPRs each)
 constexpr int kMaxColours = 254;     // colours per step (reference order: chain layers)
 constexpr int kMaskColours = 64;     // greedy colouring: colours tracked in 64-bit per-body masks
 constexpr int kBuckets = kMaxColours + 1;  // colours + the "skipped by the solver" group
+// buckets of the solve-order sort: each colour split into its multi-point and its one-point
+// manifolds (in that order), then the skipped group
+constexpr int kSortBuckets = 2 * kMaxColours + 1;
 constexpr int kMaxLarge = 256;       // shapes on the brute-force broadphase path
 constexpr uint32_t kMaxCells = 1u << 18;
 constexpr int kPairLanes = 9;        // broadphase threads per emitter shape
@@ -113,6 +120,7 @@ struct DevState {
     float4 *sp_pos;     // position, penetration
     float4 *sp_aux;     // separation, restitution bias, feature id bits, miss flag
     float4 *sp_imp;     // normal, tangent1, tangent2 impulses
+    float4 *sp_row;     // normal, tangent1, tangent2 effective masses; velocity bias (-separation / h, or 0)
     // solver bodies
     float4 *sb_v;       // linear velocity, effective inverse mass
     float4 *sb_w;       // angular velocity, active (1/0)
@@ -170,6 +178,7 @@ void launch_broadphase_write(const DevState& d, const StepParams& p, hipStream_t
 void launch_narrowphase(const DevState& d, const StepParams& p, uint32_t n_pairs, hipStream_t st);
 void launch_colour_init(const DevState& d, uint32_t n_manifolds, hipStream_t st);
 void launch_colour_rounds(const DevState& d, uint32_t n_manifolds, uint32_t first_round, uint32_t rounds, hipStream_t st);
+size_t col_table_words(size_t max_blocks);  // words of col_table for up to max_blocks manifold blocks
 void launch_build_order(const DevState& d, uint32_t n_manifolds, hipStream_t st);
 void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st);
 void launch_warm_apply(const DevState& d, uint32_t begin, uint32_t end, hipStream_t st);

@@ -47,6 +47,10 @@ struct pxg_world {
     uint32_t last_pairs = 0, last_manifolds = 0, last_ordered = 0, last_active = 0, last_colours = 0;
     bool results_pending = false;  // the last step's counters are still on the device
     uint32_t colour_blocks = 64, solve_blocks = 32;  // caps on the persistent kernels' grids
+    bool solve_one_xcd = true;  // the solve runs on the blocks of one XCD with near barriers
+    // coloured order: colour every step afresh (-1: in residual mode only, where the solve dominates
+    // the step) instead of starting from last step's colours
+    int colour_fresh = -1;
     uint32_t colour_off[kBuckets + 1];
     pxg_stats stats{};
     uint32_t colour_round = 0;
@@ -197,6 +201,8 @@ extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
     // grid caps of the persistent kernels: fewer blocks make each grid barrier cheaper
     if (const char* v = std::getenv("PXG_SOLVE_BLOCKS")) w->solve_blocks = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("PXG_COLOUR_BLOCKS")) w->colour_blocks = std::max(1, std::atoi(v));
+    if (const char* v = std::getenv("PXG_SOLVE_ONE_XCD")) w->solve_one_xcd = v[0] != '0';
+    if (const char* v = std::getenv("PXG_COLOUR_FRESH")) w->colour_fresh = v[0] == '1' ? 1 : 0;
     *out = w;
     return PXG_OK;
 }
@@ -255,8 +261,8 @@ static int alloc_bodies(pxg_world* w, uint32_t n) {
     DALLOC(d.cnt, 1);
     DALLOC(d.layer_mask, 32);
     DALLOC(d.scan_tmp, 4096);
-    DALLOC(d.bar, 2);
-    HIPCHK(hipMemsetAsync(d.bar, 0, 2 * sizeof(uint32_t), w->stream));
+    DALLOC(d.bar, kBarWords);
+    HIPCHK(hipMemsetAsync(d.bar, 0, kBarWords * sizeof(uint32_t), w->stream));
     HIPCHK(hipMemsetAsync(d.cnt, 0, sizeof(Counters), w->stream));
     return PXG_OK;
 }
@@ -295,10 +301,10 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
     DALLOC(d.m_q, 4 * M); DALLOC(d.m_colour, M);
     DALLOC(d.adj, 2 * M); DALLOC(d.m_pred, 2 * M);
     size_t nblk = (P + kBlock - 1) / kBlock;
-    DALLOC(d.col_table, size_t(kBuckets) * nblk + 2048); DALLOC(d.col_table_off, size_t(kBuckets) * nblk + 16);
+    DALLOC(d.col_table, col_table_words(nblk)); DALLOC(d.col_table_off, size_t(kSortBuckets) * nblk + 16);
     DALLOC(d.order, M);
     DALLOC(d.sm_ab, M); DALLOC(d.sm_nf, M); DALLOC(d.sm_cnt, M);
-    DALLOC(d.sp_pos, 4 * M); DALLOC(d.sp_aux, 4 * M); DALLOC(d.sp_imp, 4 * M);
+    DALLOC(d.sp_pos, 4 * M); DALLOC(d.sp_aux, 4 * M); DALLOC(d.sp_imp, 4 * M); DALLOC(d.sp_row, 4 * M);
     uint32_t cache_cap = next_pow2(8ull * M);
     d.cache_mask = cache_cap - 1;
     for (int t = 0; t < 2; ++t) {
@@ -547,7 +553,7 @@ static int read_counters(pxg_world* w) {
         if (err & ERR_CACHE) what += " warm-start table full;";
         if (err & ERR_BARRIER_BIT) {
             what += " grid barrier timed out;";
-            HIPCHK(hipMemset(w->d.bar, 0, 2 * sizeof(uint32_t)));
+            HIPCHK(hipMemset(w->d.bar, 0, kBarWords * sizeof(uint32_t)));
         }
         return fail(PXG_ERR_CAPACITY, "device reported:" + what);
     }
@@ -714,7 +720,12 @@ static int phase_solve(pxg_world* w, float h) {
     if (persistent) {
         uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kPersistBlock - 1) / kPersistBlock));
         nb = std::min(nb, w->colour_blocks);
-        hipError_t e = launch_colour_persistent(d, 1024, nb, c.solve_order == PXG_ORDER_REFERENCE ? 1u : 0u, st);
+        // Fresh colouring gives the manifolds with more points the low colours (see colour_key):
+        // each later colour then holds only one-point manifolds and its phase is shorter, which
+        // outweighs the extra colouring rounds when the solve runs hundreds of passes.
+        const bool fresh = w->colour_fresh < 0 ? c.residual_mode != 0 : w->colour_fresh > 0;
+        const uint32_t mode = c.solve_order == PXG_ORDER_REFERENCE ? 1u : (fresh ? 2u : 0u);
+        hipError_t e = launch_colour_persistent(d, 1024, nb, mode, st);
         if (e != hipSuccess) {
             if (!cooperative_refused(e)) return fail(PXG_ERR_HIP, std::string("colouring launch: ") + hipGetErrorString(e));
             drop_persistent(w, e);
@@ -731,11 +742,15 @@ static int phase_solve(pxg_world* w, float h) {
     if (persistent) {
         launch_solver_setup(d, p, bound, st);
         if (d.mlp && !d.record) launch_apply_predictions(d, 4 * bound, st);
-        uint32_t cfgv[6] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
-                            c.residual_max_iterations ? c.residual_max_iterations : 256u, 0u};
         // the widest colour is at most the manifold count; threads loop when a colour is wider
         uint32_t snb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kPersistBlock - 1) / kPersistBlock));
         snb = std::min(snb, w->solve_blocks);
+        // one XCD: launch on all of them (blocks go to XCDs in turn); the blocks that land on one
+        // carry on, the rest leave at once
+        const bool one_xcd = w->solve_one_xcd && static_cast<uint32_t>(w->coop_blocks) >= kXcds * snb;
+        if (one_xcd) snb *= kXcds;
+        uint32_t cfgv[6] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
+                            c.residual_max_iterations ? c.residual_max_iterations : 256u, one_xcd ? 1u : 0u};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (w->profiling) {
             e0 = take_event(w);
