@@ -41,16 +41,15 @@ PXD v3 vmin(v3 a, v3 b) { return mk(gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.
 PXD v3 vmax(v3 a, v3 b) { return mk(gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)); }
 PXD v3 ld3(float4 f) { return mk(f.x, f.y, f.z); }
 PXD float4 st3(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
-// Load of data that another workgroup of the same launch may have written before the last grid
-// barrier: two device-scope 8-byte loads, which bypass the CU's L1 (stores write through it).
-// With every such load going through here, a barrier among workgroups that share one L2 only has
-// to wait for stores to complete; it needs no L2 write-back or invalidation.
-PXD float4 ld_dev(const float4* p) {
-    uint64_t* q = reinterpret_cast<uint64_t*>(const_cast<float4*>(p));
-    const uint64_t lo = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t hi = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return make_float4(__uint_as_float(static_cast<uint32_t>(lo)), __uint_as_float(static_cast<uint32_t>(lo >> 32)),
-                       __uint_as_float(static_cast<uint32_t>(hi)), __uint_as_float(static_cast<uint32_t>(hi >> 32)));
+// Load of element i of a float4 array that another workgroup of the same launch may have written
+// before the last grid barrier: one device-scope (sc1) 16-byte buffer load, which bypasses the
+// CU's L1 (stores write through it). With every such load going through here, a barrier among
+// workgroups that share one L2 only has to wait for stores to complete; it needs no L2 write-back
+// or invalidation. The base must be the same for the whole wave (it goes into the buffer
+// resource); word 3 of the resource is the raw 32-bit buffer format of gfx9-family GPUs.
+PXD float4 ld_dev(const float4* base, uint32_t i) {
+    __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(base), 0, 0x7FFFFFFF, 0x00020000);
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, static_cast<int>(i * 16u), 0, 16));
 }
 
 constexpr float kEps = 1e-6f;  // util::k_epsilon, physxcoopa/util/math.h:19

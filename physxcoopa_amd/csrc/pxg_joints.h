@@ -53,7 +53,7 @@ PXD JDef load_jdef(const float4* def, uint32_t j) {
 }
 PXD JAcc load_jacc(const float4* acc, uint32_t j) {
     JAcc r;
-    float4 a = ld_dev(acc + 2 * j), b = ld_dev(acc + 2 * j + 1);
+    float4 a = ld_dev(acc, 2 * j), b = ld_dev(acc, 2 * j + 1);
     r.point = ld3(a); r.limit = a.w;
     r.axis0 = b.x; r.axis1 = b.y; r.swing = b.z;
     return r;

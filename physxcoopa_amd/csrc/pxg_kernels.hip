@@ -721,7 +721,7 @@ struct SBody {
 };
 __device__ __forceinline__ SBody load_sbody(const DevState& d, uint32_t i) {
     SBody b;
-    float4 v = ld_dev(d.sb_v + i), w = ld_dev(d.sb_w + i);
+    float4 v = ld_dev(d.sb_v, i), w = ld_dev(d.sb_w, i);
     float4 a = d.sb_I[3 * i], c = d.sb_I[3 * i + 1], e = d.sb_I[3 * i + 2];
     b.v = ld3(v); b.im = v.w;
     b.w = ld3(w); b.act = w.w != 0.0f;
@@ -864,7 +864,7 @@ __device__ __forceinline__ float4 sel4(uint32_t i, float4 a, float4 b, float4 c,
 }
 
 // warm_start's impulse application (solver.h:164-173) for solver manifold s
-__device__ void warm_manifold(const DevState& d, uint32_t s) {
+__device__ __forceinline__ void warm_manifold(const DevState& d, uint32_t s) {
     const uint2 ab = d.sm_ab[s];
     const float4 nf = d.sm_nf[s];
     const uint32_t cnt = d.sm_cnt[s];
@@ -890,7 +890,7 @@ __global__ void k_warm_apply(DevState d, uint32_t begin, uint32_t end) {
 }
 
 // solve_velocity_pass (solver.h:212-281), one colour
-__device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset);
+__device__ __forceinline__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset);
 
 __global__ void __launch_bounds__(kBlock) k_velocity(DevState d, StepParams prm, uint32_t begin, uint32_t end, int use_rest, uint32_t offset) {
     uint32_t s = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -945,7 +945,7 @@ __device__ __forceinline__ float4 velocity_point(SBody& A, SBody& B, v3 n, v3 t1
 // Points run in the order (k + offset) % count (solver.h:221-224). The k-th point's slot index
 // is computed first, so the loads of all points are issued together from global memory and the
 // loop itself works on fixed registers.
-__device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset) {
+__device__ __forceinline__ void velocity_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest, uint32_t offset) {
     const uint2 ab = d.sm_ab[s];
     const float4 nf = d.sm_nf[s];
     const uint32_t cnt = d.sm_cnt[s];
@@ -958,7 +958,7 @@ __device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint
     for (uint32_t k = 0; k < 4; ++k) {
         pp[k] = d.sp_pos[idx[k]];
         row[k] = row_for_pass(d.sp_row[idx[k]], d.sp_aux[idx[k]], use_rest);
-        imp[k] = ld_dev(d.sp_imp + idx[k]);
+        imp[k] = ld_dev(d.sp_imp, idx[k]);
     }
     SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
     v3 n = ld3(nf);
@@ -979,7 +979,7 @@ __device__ void velocity_manifold(const DevState& d, const StepParams& prm, uint
 // Velocity residual (m/s) of solver manifold s: normal rows measure the complementarity
 // violation of (lambda_n, v_n - target); friction rows the violation of stick (|lambda_t| <
 // mu lambda_n needs v_t = 0) or slip (the friction impulse must oppose v_t).
-__device__ float residual_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest) {
+__device__ __forceinline__ float residual_manifold(const DevState& d, const StepParams& prm, uint32_t s, int use_rest) {
     float r = 0.0f;
     {
         uint2 ab = d.sm_ab[s];
@@ -995,7 +995,7 @@ __device__ float residual_manifold(const DevState& d, const StepParams& prm, uin
         for (uint32_t i = 0; i < 4; ++i) {
             pp4[i] = d.sp_pos[4 * s + i];
             aux4[i] = d.sp_aux[4 * s + i];
-            imp4[i] = ld_dev(d.sp_imp + 4 * s + i);
+            imp4[i] = ld_dev(d.sp_imp, 4 * s + i);
         }
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
@@ -1039,16 +1039,16 @@ __global__ void k_position_setup(DevState d) {
     d.sb_ref[i] = p;
 }
 // solve_position for solver manifold s (solver.h:298-327)
-__device__ void position_manifold(const DevState& d, const StepParams& prm, uint32_t s) {
+__device__ __forceinline__ void position_manifold(const DevState& d, const StepParams& prm, uint32_t s) {
     uint2 ab = d.sm_ab[s];
     v3 n = ld3(d.sm_nf[s]);
     uint32_t cnt = d.sm_cnt[s];
     float pen4[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) pen4[t] = d.sp_pos[4 * s + t].w;
-    float4 xa4 = ld_dev(d.sb_x + ab.x), xb4 = ld_dev(d.sb_x + ab.y);
-    bool aa = ld_dev(d.sb_w + ab.x).w != 0.0f, ba = ld_dev(d.sb_w + ab.y).w != 0.0f;
-    v3 ra = ld3(ld_dev(d.sb_ref + ab.x)), rb = ld3(ld_dev(d.sb_ref + ab.y));
+    float4 xa4 = ld_dev(d.sb_x, ab.x), xb4 = ld_dev(d.sb_x, ab.y);
+    bool aa = ld_dev(d.sb_w, ab.x).w != 0.0f, ba = ld_dev(d.sb_w, ab.y).w != 0.0f;
+    v3 ra = ld3(ld_dev(d.sb_ref, ab.x)), rb = ld3(ld_dev(d.sb_ref, ab.y));
     float ima = xa4.w, imb = xb4.w;
     float k = ima + imb;
     if (k <= kEps) return;
@@ -1355,13 +1355,13 @@ __global__ void k_integrate_velocities(DevState d, StepParams p) {
 // bodies as the velocity passes and the warm start see them: the solver bodies
 __device__ __forceinline__ JBody jb_solver(const DevState& d, uint32_t i) {
     JBody b;
-    float4 v = ld_dev(d.sb_v + i), w = ld_dev(d.sb_w + i);
+    float4 v = ld_dev(d.sb_v, i), w = ld_dev(d.sb_w, i);
     float4 a = d.sb_I[3 * i], c = d.sb_I[3 * i + 1], e = d.sb_I[3 * i + 2];
     b.v = ld3(v); b.im = v.w;
     b.w = ld3(w); b.act = w.w != 0.0f;
     b.I.c0 = mk(a.x, a.y, a.z); b.I.c1 = mk(a.w, c.x, c.y); b.I.c2 = mk(c.z, c.w, e.x);
     b.x = mk(e.y, e.z, e.w);
-    b.q = ldq(ld_dev(d.b_rot + i));
+    b.q = ldq(ld_dev(d.b_rot, i));
     return b;
 }
 __device__ __forceinline__ void jb_solver_store(const DevState& d, uint32_t i, const JBody& b) {
@@ -1372,7 +1372,7 @@ __device__ __forceinline__ void jb_solver_store(const DevState& d, uint32_t i, c
 // the position pass of the solve: positions from the position-pass state
 __device__ __forceinline__ JBody jb_position(const DevState& d, uint32_t i) {
     JBody b = jb_solver(d, i);
-    b.x = ld3(ld_dev(d.sb_x + i));
+    b.x = ld3(ld_dev(d.sb_x, i));
     return b;
 }
 __device__ __forceinline__ void jb_position_store(const DevState& d, uint32_t i, const JBody& b) {
@@ -1401,7 +1401,7 @@ __device__ __forceinline__ void jb_world_store(const DevState& d, uint32_t i, co
 }
 __device__ __forceinline__ float inv_step(float h) { return h > 0.0f ? 1.0f / h : 0.0f; }
 
-__device__ void joint_warm_one(const DevState& d, uint32_t j) {
+__device__ __forceinline__ void joint_warm_one(const DevState& d, uint32_t j) {
     uint4 m = d.j_meta[j];
     if (!m.w) return;
     JDef def = load_jdef(d.j_def, j);
@@ -1414,7 +1414,7 @@ __device__ void joint_warm_one(const DevState& d, uint32_t j) {
     jb_solver_store(d, m.y, a);
     jb_solver_store(d, m.z, b);
 }
-__device__ void joint_velocity_one(const DevState& d, uint32_t j, float inv_h) {
+__device__ __forceinline__ void joint_velocity_one(const DevState& d, uint32_t j, float inv_h) {
     uint4 m = d.j_meta[j];
     if (!m.w) return;
     JDef def = load_jdef(d.j_def, j);
@@ -1426,7 +1426,7 @@ __device__ void joint_velocity_one(const DevState& d, uint32_t j, float inv_h) {
     jb_solver_store(d, m.y, a);
     jb_solver_store(d, m.z, b);
 }
-__device__ void joint_position_one(const DevState& d, const StepParams& p, uint32_t j) {
+__device__ __forceinline__ void joint_position_one(const DevState& d, const StepParams& p, uint32_t j) {
     uint4 m = d.j_meta[j];
     if (!m.w) return;
     JDef def = load_jdef(d.j_def, j);
@@ -1673,11 +1673,11 @@ __device__ __forceinline__ void prefetch_slot(const DevState& d, uint32_t s, uin
     for (uint32_t k = 0; k < 4; ++k) {
         x.pp[k] = d.sp_pos[x.idx[k]];
         x.row[k] = row_for_pass(d.sp_row[x.idx[k]], d.sp_aux[x.idx[k]], use_rest);
-        x.imp[k] = ld_dev(d.sp_imp + x.idx[k]);
+        x.imp[k] = ld_dev(d.sp_imp, x.idx[k]);
     }
 }
 // velocity_manifold for a prefetched slot (same arithmetic, same order)
-__device__ void velocity_prefetched(const DevState& d, PreSlot& x) {
+__device__ __forceinline__ void velocity_prefetched(const DevState& d, PreSlot& x) {
     SBody A = load_sbody(d, x.ab.x), B = load_sbody(d, x.ab.y);
     v3 n = ld3(x.nf);
     v3 t1, t2;
@@ -1813,8 +1813,13 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (grid_sync(g, err)) return;
         }
         ++passes;
+        // two manifolds per iteration, the second index clamped rather than branched on, so the
+        // loads of both are in flight together (a repeated manifold does not change the maximum)
         float r = 0.0f;
-        for (uint32_t s = tid; s < n_active; s += nthreads) r = gmax(r, residual_manifold(d, prm, s, 0));
+        for (uint32_t s = tid; s < n_active; s += 2u * nthreads) {
+            const uint32_t s1 = s + nthreads < n_active ? s + nthreads : s;
+            r = gmax(r, gmax(residual_manifold(d, prm, s, 0), residual_manifold(d, prm, s1, 0)));
+        }
         for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
         if (lane0()) s_red[threadIdx.x >> 6] = r;
         __syncthreads();
@@ -1859,7 +1864,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     // solve_position: snapshot, then position passes
     for (uint32_t i = tid; i < d.nb; i += nthreads) {
         float4 p = d.b_pos[i];
-        d.sb_x[i] = make_float4(p.x, p.y, p.z, ld_dev(d.sb_v + i).w);
+        d.sb_x[i] = make_float4(p.x, p.y, p.z, ld_dev(d.sb_v, i).w);
         d.sb_ref[i] = p;
     }
     if (grid_sync(g, err)) return;
