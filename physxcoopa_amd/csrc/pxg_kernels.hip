@@ -956,9 +956,12 @@ __device__ __forceinline__ void velocity_manifold(const DevState& d, const StepP
     float4 pp[4], row[4], imp[4];
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
-        pp[k] = d.sp_pos[idx[k]];
-        row[k] = row_for_pass(d.sp_row[idx[k]], d.sp_aux[idx[k]], use_rest);
-        imp[k] = ld_dev(d.sp_imp, idx[k]);
+        pp[k] = row[k] = imp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (k < cnt) {
+            pp[k] = d.sp_pos[idx[k]];
+            row[k] = row_for_pass(d.sp_row[idx[k]], d.sp_aux[idx[k]], use_rest);
+            imp[k] = ld_dev(d.sp_imp, idx[k]);
+        }
     }
     SBody A = load_sbody(d, ab.x), B = load_sbody(d, ab.y);
     v3 n = ld3(nf);
@@ -989,13 +992,17 @@ __device__ __forceinline__ float residual_manifold(const DevState& d, const Step
         v3 t1, t2;
         orthonormal_basis(n, t1, t2);
         uint32_t cnt = d.sm_cnt[s];
-        // slots are padded to four points: all loads go out together
+        // all the points' loads go out together; a wave whose manifolds all have fewer points
+        // skips the loads of the points none of them has
         float4 pp4[4], aux4[4], imp4[4];
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
-            pp4[i] = d.sp_pos[4 * s + i];
-            aux4[i] = d.sp_aux[4 * s + i];
-            imp4[i] = ld_dev(d.sp_imp, 4 * s + i);
+            pp4[i] = aux4[i] = imp4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (i < cnt) {
+                pp4[i] = d.sp_pos[4 * s + i];
+                aux4[i] = d.sp_aux[4 * s + i];
+                imp4[i] = ld_dev(d.sp_imp, 4 * s + i);
+            }
         }
 #pragma unroll
         for (uint32_t i = 0; i < 4; ++i) {
@@ -1671,9 +1678,12 @@ __device__ __forceinline__ void prefetch_slot(const DevState& d, uint32_t s, uin
     for (uint32_t k = 0; k < 4; ++k) x.idx[k] = 4u * s + (k < x.cnt ? (k + r) % x.cnt : k);
 #pragma unroll
     for (uint32_t k = 0; k < 4; ++k) {
-        x.pp[k] = d.sp_pos[x.idx[k]];
-        x.row[k] = row_for_pass(d.sp_row[x.idx[k]], d.sp_aux[x.idx[k]], use_rest);
-        x.imp[k] = ld_dev(d.sp_imp, x.idx[k]);
+        x.pp[k] = x.row[k] = x.imp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (k < x.cnt) {
+            x.pp[k] = d.sp_pos[x.idx[k]];
+            x.row[k] = row_for_pass(d.sp_row[x.idx[k]], d.sp_aux[x.idx[k]], use_rest);
+            x.imp[k] = ld_dev(d.sp_imp, x.idx[k]);
+        }
     }
 }
 // velocity_manifold for a prefetched slot (same arithmetic, same order)
