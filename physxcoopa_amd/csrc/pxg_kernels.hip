@@ -1646,7 +1646,6 @@ __device__ __forceinline__ bool near_blocks(uint32_t* bar, uint32_t* err, uint32
     return s_go != 0u;
 }
 
-__device__ __forceinline__ uint32_t n_active_all(const DevState& d) { return d.cnt->active; }
 
 struct SolveCfg {
     uint32_t vel_iters, relax_iters, pos_iters;
@@ -1704,6 +1703,22 @@ __device__ __forceinline__ void velocity_prefetched(const DevState& d, PreSlot& 
     store_sbody(d, x.ab.y, B);
 }
 
+// Algorithmic bytes of one solve, for the roofline (profiling only; launched after the solve so
+// the solve's own time excludes it): per solver manifold with n points, the warm start and each
+// velocity or relax pass move 192 + 40 n bytes (both bodies read and written, the points'
+// positions, rows and impulses), each position pass 92 + 4 n.
+__global__ void k_solve_bytes(DevState d, uint32_t vel_passes, uint32_t relax, uint32_t pos_iters, uint32_t residual_mode) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t passes = residual_mode ? d.cnt->passes : vel_passes;
+    unsigned long long bytes = 0;
+    if (s < d.cnt->active) {
+        const unsigned long long n = d.sm_cnt[s];
+        bytes = (192ull + 40ull * n) * (1ull + passes + relax) + pos_iters * (92ull + 4ull * n);
+    }
+    for (int o = 32; o > 0; o >>= 1) bytes += __shfl_xor(bytes, o, 64);
+    if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, bytes);
+}
+
 // The whole solve of one step (dynamics/solver.h:914-958 after the restitution capture):
 // warm start, velocity passes (fixed count or until the residual <= tolerance), relax passes,
 // position passes, colour by colour with a grid barrier between colours.
@@ -1737,18 +1752,6 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
         for (uint32_t t = jb + tid; t < je; t += nthreads) joint_warm_one(d, jo[t]);
         if (grid_sync(g, err)) return;
     }
-    if (prm.profile) {
-        // algorithmic bytes of the solve: warm start 192 + 40n per manifold (bodies r/w and the
-        // points' positions and impulses), each position pass 92 + 4n; the velocity and relax
-        // passes add 192 + 40n each below
-        uint32_t bytes = 0;
-        for (uint32_t s = tid; s < n_active_all(d); s += nthreads) {
-            uint32_t n = d.sm_cnt[s];
-            bytes += 192u + 40u * n + cfg.pos_iters * (92u + 4u * n);
-        }
-        bytes = wave_sum(bytes);
-        if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
-    }
     uint32_t passes = 0;
     __shared__ float s_red[kPersistBlock / 64];
     const uint32_t n_active = d.cnt->active;
@@ -1767,12 +1770,6 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             const uint32_t it = use_rest ? pass - cfg.vel_iters : pass;
             if (k < C) {
                 const uint32_t c = k;
-                if (prm.profile) {
-                    uint32_t bytes = 0;
-                    for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
-                    bytes = wave_sum(bytes);
-                    if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
-                }
                 if (x.s != kNoSlot) velocity_prefetched(d, x);
                 for (uint32_t s = co[c] + tid + nthreads; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, use_rest, it);
             } else {
@@ -1802,12 +1799,6 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     for (uint32_t it = 0; it < max_pass; ++it) {
         for (uint32_t c = 0; c < C; ++c) {
             uint32_t b = co[c], e = co[c + 1];
-            if (prm.profile) {
-                uint32_t bytes = 0;
-                for (uint32_t s = b + tid; s < e; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
-                bytes = wave_sum(bytes);
-                if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
-            }
             if (y.s != kNoSlot) velocity_prefetched(d, y);
             for (uint32_t s = b + tid + nthreads; s < e; s += nthreads) velocity_manifold(d, prm, s, 0, it);
             grid_arrive(g);
@@ -1854,12 +1845,6 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     if (cfg.residual_mode && tid == 0) d.cnt->passes = passes;
     for (uint32_t it = 0; cfg.residual_mode && it < cfg.relax_iters; ++it) {
         for (uint32_t c = 0; c < C; ++c) {
-            if (prm.profile) {
-                uint32_t bytes = 0;
-                for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) bytes += 192u + 40u * d.sm_cnt[s];
-                bytes = wave_sum(bytes);
-                if (lane0() && bytes) atomicAdd(&d.cnt->prof_vel_bytes, static_cast<unsigned long long>(bytes));
-            }
             for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, 1, it);
             if (grid_sync(g, err)) return;
         }
@@ -2330,6 +2315,9 @@ hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const
     void* args[] = {&dd, &pp, &c, &bar};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks),
                                       dim3(kPersistBlock), args, 0, st);
+}
+void launch_solve_bytes(const DevState& d, const uint32_t cfgv[6], uint32_t bound, hipStream_t st) {
+    if (bound) hipLaunchKernelGGL(k_solve_bytes, dim3(blocks(bound)), dim3(kBlock), 0, st, d, cfgv[0], cfgv[1], cfgv[2], cfgv[3]);
 }
 hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, uint32_t reference,
                                     hipStream_t st) {

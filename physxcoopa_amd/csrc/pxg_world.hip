@@ -763,6 +763,7 @@ static int phase_solve(pxg_world* w, float h) {
             if (w->profiling) {
                 (void)hipEventRecord(e1, st);
                 w->ev_pending.push_back({e0, e1});
+                launch_solve_bytes(d, cfgv, bound, st);
             }
         } else {
             if (!cooperative_refused(e)) return fail(PXG_ERR_HIP, std::string("solve launch: ") + hipGetErrorString(e));
