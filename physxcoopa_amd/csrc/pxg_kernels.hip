@@ -1703,6 +1703,24 @@ __device__ __forceinline__ void velocity_prefetched(const DevState& d, PreSlot& 
     store_sbody(d, x.ab.y, B);
 }
 
+// The trailing colours of at most one wave each (colours [first, C)) are run by one wave, one after
+// another: the wave waits for its stores to complete before the next colour's loads, which read
+// the L2. That costs each such colour its own latency only, not a grid barrier.
+__device__ __forceinline__ uint32_t tail_start(const uint32_t* co, uint32_t C) {
+    uint32_t t = C;
+    while (t > 0 && co[t] - co[t - 1] <= 64u) --t;
+    return C - t >= 2u ? t : C;
+}
+template <class F>
+__device__ __forceinline__ void run_tail(const uint32_t* co, uint32_t first, uint32_t C, bool mine, F&& f) {
+    if (!mine) return;
+    for (uint32_t c = first; c < C; ++c) {
+        const uint32_t s = co[c] + (threadIdx.x & 63u);
+        if (s < co[c + 1]) f(s);
+        wait_stores();
+    }
+}
+
 // Algorithmic bytes of one solve, for the roofline (profiling only; launched after the solve so
 // the solve's own time excludes it): per solver manifold with n points, the warm start and each
 // velocity or relax pass move 192 + 40 n bytes (both bodies read and written, the points'
@@ -1738,6 +1756,9 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     // blocks' waves while the others idle.
     const uint32_t tid = ((threadIdx.x >> 6) * g.nblocks + rank) * 64u + (threadIdx.x & 63u);
     const uint32_t nthreads = g.nblocks * blockDim.x;
+    const uint32_t T = tail_start(co, C);           // colours [T, C): the tail, one phase
+    const uint32_t CP = T + (T < C ? 1u : 0u);      // contact phases per pass
+    const bool tail_wave = rank == 0 && threadIdx.x < 64u;
     for (uint32_t c = 0; c < C; ++c) {
         for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) warm_manifold(d, s);
         if (grid_sync(g, err)) return;
@@ -1759,23 +1780,25 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
         // velocity then relax passes: each pass is the contact colours, then the joint layers
         // (solver.h:939-946, joints alternating direction); the next contact colour's first
         // slot of this thread is loaded between arriving at the barrier and waiting on it
-        const uint32_t per = C + L;
+        const uint32_t per = CP + L;
         const uint32_t nph = (cfg.vel_iters + cfg.relax_iters) * per;
         PreSlot x;
         x.s = kNoSlot;
-        if (nph && C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, cfg.vel_iters == 0u ? 1 : 0, x);
+        if (nph && T && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, cfg.vel_iters == 0u ? 1 : 0, x);
         for (uint32_t ph = 0; ph < nph; ++ph) {
             const uint32_t pass = ph / per, k = ph % per;
             const int use_rest = pass >= cfg.vel_iters ? 1 : 0;
             const uint32_t it = use_rest ? pass - cfg.vel_iters : pass;
-            if (k < C) {
+            if (k < T) {
                 const uint32_t c = k;
                 if (x.s != kNoSlot) velocity_prefetched(d, x);
                 for (uint32_t s = co[c] + tid + nthreads; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, use_rest, it);
+            } else if (k < CP) {
+                run_tail(co, T, C, tail_wave, [&](uint32_t s) { velocity_manifold(d, prm, s, use_rest, it); });
             } else {
                 uint32_t jb, je;
                 const uint32_t* jo;
-                joint_layer_range(d, k - C, (it & 1u) != 0u, jb, je, jo);
+                joint_layer_range(d, k - CP, (it & 1u) != 0u, jb, je, jo);
                 for (uint32_t t = jb + tid; t < je; t += nthreads) joint_velocity_one(d, jo[t], inv_h);
             }
             grid_arrive(g);
@@ -1783,7 +1806,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (ph + 1 < nph) {
                 const uint32_t np = (ph + 1) / per, nk = (ph + 1) % per;
                 const uint32_t nit = np >= cfg.vel_iters ? np - cfg.vel_iters : np;
-                if (nk < C && co[nk] + tid < co[nk + 1]) prefetch_slot(d, co[nk] + tid, nit, np >= cfg.vel_iters ? 1 : 0, x);
+                if (nk < T && co[nk] + tid < co[nk + 1]) prefetch_slot(d, co[nk] + tid, nit, np >= cfg.vel_iters ? 1 : 0, x);
             }
             if (grid_wait(g, err)) return;
         }
@@ -1795,16 +1818,20 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : 0u;
     PreSlot y;
     y.s = kNoSlot;
-    if (max_pass && C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, 0, y);
+    if (max_pass && T && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, 0, y);
     for (uint32_t it = 0; it < max_pass; ++it) {
-        for (uint32_t c = 0; c < C; ++c) {
+        for (uint32_t c = 0; c < T; ++c) {
             uint32_t b = co[c], e = co[c + 1];
             if (y.s != kNoSlot) velocity_prefetched(d, y);
             for (uint32_t s = b + tid + nthreads; s < e; s += nthreads) velocity_manifold(d, prm, s, 0, it);
             grid_arrive(g);
             y.s = kNoSlot;
-            if (c + 1 < C && co[c + 1] + tid < co[c + 2]) prefetch_slot(d, co[c + 1] + tid, it, 0, y);
+            if (c + 1 < T && co[c + 1] + tid < co[c + 2]) prefetch_slot(d, co[c + 1] + tid, it, 0, y);
             if (grid_wait(g, err)) return;
+        }
+        if (T < C) {
+            run_tail(co, T, C, tail_wave, [&](uint32_t s) { velocity_manifold(d, prm, s, 0, it); });
+            if (grid_sync(g, err)) return;
         }
         for (uint32_t l = 0; l < L; ++l) {
             uint32_t jb, je;
@@ -1830,7 +1857,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (m > 0.0f) atomicMax(&d.cnt->res_slot[it & 1], __float_as_uint(m));
         }
         grid_arrive(g);
-        if (C && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, it + 1u, 0, y);
+        if (T && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, it + 1u, 0, y);
         if (grid_wait(g, err)) return;
         // every block's maximum is in: all threads read the same value. The other slot is
         // cleared for the next pass: its last readers read it before this pass's first barrier,
@@ -1844,8 +1871,12 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     }
     if (cfg.residual_mode && tid == 0) d.cnt->passes = passes;
     for (uint32_t it = 0; cfg.residual_mode && it < cfg.relax_iters; ++it) {
-        for (uint32_t c = 0; c < C; ++c) {
+        for (uint32_t c = 0; c < T; ++c) {
             for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) velocity_manifold(d, prm, s, 1, it);
+            if (grid_sync(g, err)) return;
+        }
+        if (T < C) {
+            run_tail(co, T, C, tail_wave, [&](uint32_t s) { velocity_manifold(d, prm, s, 1, it); });
             if (grid_sync(g, err)) return;
         }
         for (uint32_t l = 0; l < L; ++l) {
@@ -1864,8 +1895,12 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     }
     if (grid_sync(g, err)) return;
     for (uint32_t it = 0; it < cfg.pos_iters; ++it) {
-        for (uint32_t c = 0; c < C; ++c) {
+        for (uint32_t c = 0; c < T; ++c) {
             for (uint32_t s = co[c] + tid; s < co[c + 1]; s += nthreads) position_manifold(d, prm, s);
+            if (grid_sync(g, err)) return;
+        }
+        if (T < C) {
+            run_tail(co, T, C, tail_wave, [&](uint32_t s) { position_manifold(d, prm, s); });
             if (grid_sync(g, err)) return;
         }
     }
