@@ -815,6 +815,37 @@ __global__ void k_solver_setup(DevState d, StepParams prm, uint32_t bound) {
     }
 }
 
+// First colour of the solve's tail: the trailing colours of at most one wave each (at least two
+// of them), which one wave runs back to back; C when there is no such tail.
+__device__ __forceinline__ uint32_t tail_start(const uint32_t* co, uint32_t C) {
+    uint32_t t = C;
+    while (t > 0 && co[t] - co[t - 1] <= 64u) --t;
+    return C - t >= 2u ? t : C;
+}
+// Residual mode: a manifold's end-of-pass residual reads only its bodies' velocities. When no tail
+// colour writes either (awake dynamic) body, they are final before the tail starts, and the
+// residual is computed by the waves the tail leaves idle.
+__device__ __forceinline__ bool sb_active(const DevState& d, uint32_t i) { return d.sb_w[i].w != 0.0f; }
+__global__ void k_ready_last(DevState d, uint32_t bound) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= bound || s >= d.cnt->active) return;
+    const uint32_t c = d.m_colour[d.order[s]];
+    const uint2 ab = d.sm_ab[s];
+    if (sb_active(d, ab.x)) atomicMax(&d.body_last[ab.x], c);
+    if (sb_active(d, ab.y)) atomicMax(&d.body_last[ab.y], c);
+}
+// the order within each part depends on timing; nothing depends on it (the residual is a maximum)
+__global__ void k_ready_split(DevState d, uint32_t bound) {
+    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = d.cnt->active;
+    if (s >= bound || s >= n) return;
+    const uint32_t T = tail_start(d.cnt->colour_off, d.cnt->colours);
+    const uint2 ab = d.sm_ab[s];
+    const bool late = (sb_active(d, ab.x) && d.body_last[ab.x] >= T) || (sb_active(d, ab.y) && d.body_last[ab.y] >= T);
+    if (late) d.ready_list[n - 1u - atomicAdd(&d.ready_cnt[1], 1u)] = s;
+    else d.ready_list[atomicAdd(&d.ready_cnt[0], 1u)] = s;
+}
+
 // clamps each prediction to the friction cone (lambda_n >= 0, |lambda_t| <= mu lambda_n) and makes
 // it the point's initial impulse
 __global__ void k_apply_predictions(DevState d, uint32_t n_bound) {
@@ -1652,6 +1683,7 @@ struct SolveCfg {
     uint32_t residual_mode, residual_max;
     float residual_tol;
     uint32_t one_xcd;  // launched on every XCD; the blocks of one carry on with near barriers
+    uint32_t ready;    // residual mode: positions split by ready_list (residuals during the tail)
 };
 
 // A velocity-pass manifold's slot loaded ahead (points in the pass's point order, rows with the
@@ -1706,11 +1738,6 @@ __device__ __forceinline__ void velocity_prefetched(const DevState& d, PreSlot& 
 // The trailing colours of at most one wave each (colours [first, C)) are run by one wave, one after
 // another: the wave waits for its stores to complete before the next colour's loads, which read
 // the L2. That costs each such colour its own latency only, not a grid barrier.
-__device__ __forceinline__ uint32_t tail_start(const uint32_t* co, uint32_t C) {
-    uint32_t t = C;
-    while (t > 0 && co[t] - co[t - 1] <= 64u) --t;
-    return C - t >= 2u ? t : C;
-}
 template <class F>
 __device__ __forceinline__ void run_tail(const uint32_t* co, uint32_t first, uint32_t C, bool mine, F&& f) {
     if (!mine) return;
@@ -1719,6 +1746,21 @@ __device__ __forceinline__ void run_tail(const uint32_t* co, uint32_t first, uin
         if (s < co[c + 1]) f(s);
         wait_stores();
     }
+}
+
+// Maximum residual of the solve positions list[j] (j itself without a list) for j in [lo, hi)
+// taken by this thread: j = lo + u, lo + u + step, ...; two per iteration, the second index
+// clamped rather than branched on, so the loads of both are in flight together (a repeated
+// manifold does not change the maximum).
+__device__ __forceinline__ float residual_span(const DevState& d, const StepParams& prm, const uint32_t* list, uint32_t lo,
+                                               uint32_t hi, uint32_t u, uint32_t step) {
+    float r = 0.0f;
+    for (uint32_t j = lo + u; j < hi; j += 2u * step) {
+        const uint32_t j1 = j + step < hi ? j + step : j;
+        const uint32_t s0 = list ? list[j] : j, s1 = list ? list[j1] : j1;
+        r = gmax(r, gmax(residual_manifold(d, prm, s0, 0), residual_manifold(d, prm, s1, 0)));
+    }
+    return r;
 }
 
 // Algorithmic bytes of one solve, for the roofline (profiling only; launched after the solve so
@@ -1829,8 +1871,19 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (c + 1 < T && co[c + 1] + tid < co[c + 2]) prefetch_slot(d, co[c + 1] + tid, it, 0, y);
             if (grid_wait(g, err)) return;
         }
+        // With the split (no joints: joint layers would run after the tail), the blocks the tail
+        // leaves idle compute the residual of the manifolds whose bodies no tail colour writes
+        // (final before the tail); the rest follow the tail's barrier. The tail's own block stays
+        // out of it: residual gathers on its CU would slow the tail's chain of colours.
+        float r = 0.0f;
+        const bool split = cfg.ready != 0u && L == 0u && T < C && g.nblocks > 1u;
         if (T < C) {
-            run_tail(co, T, C, tail_wave, [&](uint32_t s) { velocity_manifold(d, prm, s, 0, it); });
+            if (tail_wave) {
+                run_tail(co, T, C, true, [&](uint32_t s) { velocity_manifold(d, prm, s, 0, it); });
+            } else if (split && rank != 0u) {
+                const uint32_t u = ((threadIdx.x >> 6) * (g.nblocks - 1u) + rank - 1u) * 64u + (threadIdx.x & 63u);
+                r = residual_span(d, prm, d.ready_list, 0u, d.ready_cnt[0], u, (g.nblocks - 1u) * blockDim.x);
+            }
             if (grid_sync(g, err)) return;
         }
         for (uint32_t l = 0; l < L; ++l) {
@@ -1841,13 +1894,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (grid_sync(g, err)) return;
         }
         ++passes;
-        // two manifolds per iteration, the second index clamped rather than branched on, so the
-        // loads of both are in flight together (a repeated manifold does not change the maximum)
-        float r = 0.0f;
-        for (uint32_t s = tid; s < n_active; s += 2u * nthreads) {
-            const uint32_t s1 = s + nthreads < n_active ? s + nthreads : s;
-            r = gmax(r, gmax(residual_manifold(d, prm, s, 0), residual_manifold(d, prm, s1, 0)));
-        }
+        r = gmax(r, residual_span(d, prm, split ? d.ready_list : nullptr, split ? d.ready_cnt[0] : 0u, n_active, tid, nthreads));
         for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
         if (lane0()) s_red[threadIdx.x >> 6] = r;
         __syncthreads();
@@ -2282,6 +2329,13 @@ void launch_build_order(const DevState& d, uint32_t n_man, hipStream_t st) {
     if (n_man) hipLaunchKernelGGL(k_colour_scatter, dim3(nblk), dim3(kBlock), 0, st, d, n_man, nblk);
     hipLaunchKernelGGL(k_colour_offsets, dim3(1), dim3(kBlock), 0, st, d, nblk, total);
 }
+void launch_ready_split(const DevState& d, uint32_t bound, hipStream_t st) {
+    if (!bound) return;
+    if (d.nb) launch_fill_u32(d.body_last, 0u, d.nb, st);
+    launch_fill_u32(d.ready_cnt, 0u, 2, st);
+    hipLaunchKernelGGL(k_ready_last, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
+    hipLaunchKernelGGL(k_ready_split, dim3(blocks(bound)), dim3(kBlock), 0, st, d, bound);
+}
 void launch_solver_setup(const DevState& d, const StepParams& p, uint32_t bound, hipStream_t st) {
     // solver bodies first: the point rows are computed from them
     if (d.nb) hipLaunchKernelGGL(k_solver_bodies, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
@@ -2337,12 +2391,13 @@ void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t bound
 }
 // cooperative launches; returns the HIP error of the launch. The barrier count is zeroed by a
 // kernel (not a memset) so the launch sequence may be captured in a graph.
-hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[6], float tol,
+hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[7], float tol,
                                    uint32_t nblocks, hipStream_t st) {
     SolveCfg c;
     c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
     c.residual_mode = cfgv[3]; c.residual_max = cfgv[4]; c.residual_tol = tol;
     c.one_xcd = cfgv[5];
+    c.ready = cfgv[6];
     launch_fill_u32(d.bar, 0u, kBarWords, st);
     DevState dd = d;
     StepParams pp = p;
@@ -2351,7 +2406,7 @@ hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks),
                                       dim3(kPersistBlock), args, 0, st);
 }
-void launch_solve_bytes(const DevState& d, const uint32_t cfgv[6], uint32_t bound, hipStream_t st) {
+void launch_solve_bytes(const DevState& d, const uint32_t cfgv[7], uint32_t bound, hipStream_t st) {
     if (bound) hipLaunchKernelGGL(k_solve_bytes, dim3(blocks(bound)), dim3(kBlock), 0, st, d, cfgv[0], cfgv[1], cfgv[2], cfgv[3]);
 }
 hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, uint32_t reference,

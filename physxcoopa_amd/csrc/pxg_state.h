@@ -113,6 +113,12 @@ struct DevState {
     uint32_t *m_pred;               // 2 per manifold: predecessor on body a / body b (kNoPred: none)
     uint32_t *col_table, *col_table_off;
     uint32_t *order;                // solve position -> manifold index
+    // residual mode: the active solve positions split by whether a tail colour (see the solve
+    // kernel) writes one of their bodies: those it does not, from the front of ready_list (their
+    // count in ready_cnt[0]), the others from the back (ready_cnt[1])
+    uint32_t *body_last;            // per body: last colour writing it
+    uint32_t *ready_cnt;            // 2 counters
+    uint32_t *ready_list;
     // solver arrays, solve order
     uint2 *sm_ab;
     float4 *sm_nf;
@@ -209,7 +215,9 @@ void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total,
 // cfgv = vel, relax, pos iterations, residual mode, residual max passes
 // algorithmic bytes of the last persistent solve into cnt->prof_vel_bytes (profiling)
 void launch_solve_bytes(const DevState& d, const uint32_t cfgv[6], uint32_t bound, hipStream_t st);
-hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[6], float tol,
+// splits the active solve positions for the residual computed during the tail colours
+void launch_ready_split(const DevState& d, uint32_t bound, hipStream_t st);
+hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[7], float tol,
                                    uint32_t nblocks, hipStream_t st);
 // colours every active manifold: last step's colours where still valid, then Jones-Plassmann
 // rounds (at most max_rounds) for the rest

@@ -257,7 +257,7 @@ static int alloc_bodies(pxg_world* w, uint32_t n) {
     DALLOC(d.col_best, cap); DALLOC(d.col_mask, cap); DALLOC(d.col_dup, cap);
     DALLOC(d.adj_cnt, cap); DALLOC(d.adj_start, cap); DALLOC(d.adj_cur, cap);
     DALLOC(d.sb_v, cap); DALLOC(d.sb_w, cap); DALLOC(d.sb_I, 3 * size_t(cap)); DALLOC(d.sb_x, cap); DALLOC(d.sb_ref, cap);
-    DALLOC(d.uf_parent, cap); DALLOC(d.uf_root, cap); DALLOC(d.isl_min, cap);
+    DALLOC(d.uf_parent, cap); DALLOC(d.uf_root, cap); DALLOC(d.isl_min, cap); DALLOC(d.body_last, cap);
     DALLOC(d.cnt, 1);
     DALLOC(d.layer_mask, 32);
     DALLOC(d.scan_tmp, 4096);
@@ -302,7 +302,7 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
     DALLOC(d.adj, 2 * M); DALLOC(d.m_pred, 2 * M);
     size_t nblk = (P + kBlock - 1) / kBlock;
     DALLOC(d.col_table, col_table_words(nblk)); DALLOC(d.col_table_off, size_t(kSortBuckets) * nblk + 16);
-    DALLOC(d.order, M);
+    DALLOC(d.order, M); DALLOC(d.ready_list, M); DALLOC(d.ready_cnt, 2);
     DALLOC(d.sm_ab, M); DALLOC(d.sm_nf, M); DALLOC(d.sm_cnt, M);
     DALLOC(d.sp_pos, 4 * M); DALLOC(d.sp_aux, 4 * M); DALLOC(d.sp_imp, 4 * M); DALLOC(d.sp_row, 4 * M);
     uint32_t cache_cap = next_pow2(8ull * M);
@@ -749,8 +749,13 @@ static int phase_solve(pxg_world* w, float h) {
         // carry on, the rest leave at once
         const bool one_xcd = w->solve_one_xcd && static_cast<uint32_t>(w->coop_blocks) >= kXcds * snb;
         if (one_xcd) snb *= kXcds;
-        uint32_t cfgv[6] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
-                            c.residual_max_iterations ? c.residual_max_iterations : 256u, one_xcd ? 1u : 0u};
+        // residual mode without joints: split the manifolds for the residual computed during the
+        // tail colours
+        const bool ready = c.residual_mode && d.nj == 0;
+        if (ready) launch_ready_split(d, bound, st);
+        uint32_t cfgv[7] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
+                            c.residual_max_iterations ? c.residual_max_iterations : 256u, one_xcd ? 1u : 0u,
+                            ready ? 1u : 0u};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (w->profiling) {
             e0 = take_event(w);
