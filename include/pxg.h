@@ -224,6 +224,16 @@ typedef struct pxg_joint {
     float reserved3[2];
 } pxg_joint;
 
+/* One contact event (physxcoopa/collision/contact_event.h:29-36): a shape pair's enter / stay /
+ * exit transition of one step, for a solid contact or a trigger overlap. a and b are the owner
+ * body indices of the pair's shapes (lower shape slot first). */
+enum pxg_event_type { PXG_EVENT_ENTER = 0, PXG_EVENT_STAY = 1, PXG_EVENT_EXIT = 2 };
+typedef struct pxg_contact_event {
+    uint32_t a, b;
+    uint32_t type;        /* pxg_event_type */
+    uint32_t is_trigger;
+} pxg_contact_event;
+
 typedef struct pxg_world pxg_world;
 
 /* --- configuration and lifetime --- */
@@ -282,6 +292,17 @@ int pxg_set_fat_aabbs(pxg_world* w, const float* in6, uint32_t n);
 /* Warm-start cache (dynamics/solver.h:63-69): keys and (normal, tangent1, tangent2). */
 int pxg_get_warm_cache(pxg_world* w, uint64_t* keys, float* impulses3, uint32_t cap, uint32_t* n);
 int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const float* impulses3, uint32_t n);
+/* Contact events queued since the last clear (events(), world.h:744): every step_fixed appends
+ * its events (emit_contact_events_, world.h:1998-2018: per cache -- solid, then trigger -- all
+ * enters, then stays, then exits, each in pair order); pxg_step clears the queue before its
+ * substeps (world.h:1230). n receives the queued count; min(cap, n) events are copied. Returns
+ * PXG_ERR_CAPACITY (events still copied) when events were dropped because the queue was full. */
+int pxg_get_events(pxg_world* w, pxg_contact_event* out, uint32_t cap, uint32_t* n);
+int pxg_clear_events(pxg_world* w);                               /* clear_events(), world.h:748 */
+/* Event generation on (default) or off, and the queue capacity in events (0 = keep; default
+ * 2 * max_manifolds + 1024). Switching it on starts from an empty pair snapshot, so the next step
+ * reports its contacts as enters. No counterpart in the reference, which always classifies. */
+int pxg_set_events(pxg_world* w, uint32_t enabled, uint32_t capacity);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);
 

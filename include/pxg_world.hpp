@@ -113,6 +113,15 @@ struct Vec3 {
     float x = 0.0f, y = 0.0f, z = 0.0f;
 };
 
+// One pair's enter / stay / exit transition of one step (collision/contact_event.h:29-36).
+struct ContactEvent {
+    enum class Type { Enter, Stay, Exit };
+    BodyId a;
+    BodyId b;
+    Type type = Type::Enter;
+    bool is_trigger = false;
+};
+
 // glm quaternion arithmetic in float, glm's term order (x, y, z, w storage) -- for the joints'
 // rest rotation, inverse(qa) * qb at creation (world.h:415).
 namespace quat_ops {
@@ -379,6 +388,7 @@ public:
     // --- stepping (world.h:1229-1317) ---
     // Accumulates dt (clamped to 0.25 s) and runs up to max_substeps fixed substeps.
     uint32_t step(float dt) {
+        clear_events();  // world.h:1230
         const float frame = dt < 0.25f ? dt : 0.25f;
         accumulator_ += frame;
         const float h = cfg_.fixed_dt;
@@ -418,6 +428,38 @@ public:
         state_on_device_ = true;
         joints_on_device_ = !joints_.empty();
         manifolds_valid_ = false;
+        events_valid_ = false;
+    }
+
+    // Contact events queued since the last clear_events() (world.h:744): every step_fixed()
+    // appends its enters, stays and exits; step() clears the queue first. A capacity growth
+    // (more bodies than the handle holds) starts a new handle, whose first step reports its
+    // contacts as enters.
+    const std::vector<ContactEvent>& events() const {
+        if (!events_valid_) {
+            uint32_t n = 0;
+            check(pxg_get_events(w_, nullptr, 0, &n), "events");
+            std::vector<pxg_contact_event> raw(n);
+            if (n) check(pxg_get_events(w_, raw.data(), n, &n), "events");
+            events_.clear();
+            for (uint32_t i = 0; i < n && i < raw.size(); ++i) {
+                ContactEvent e;
+                e.a = BodyId{raw[i].a, raw[i].a < generations_.size() ? generations_[raw[i].a] : 0u};
+                e.b = BodyId{raw[i].b, raw[i].b < generations_.size() ? generations_[raw[i].b] : 0u};
+                e.type = raw[i].type == PXG_EVENT_ENTER ? ContactEvent::Type::Enter
+                         : raw[i].type == PXG_EVENT_STAY ? ContactEvent::Type::Stay
+                                                         : ContactEvent::Type::Exit;
+                e.is_trigger = raw[i].is_trigger != 0;
+                events_.push_back(e);
+            }
+            events_valid_ = true;
+        }
+        return events_;
+    }
+    void clear_events() {  // world.h:748
+        check(pxg_clear_events(w_), "clear_events");
+        events_.clear();
+        events_valid_ = true;
     }
 
     // This step's valid manifolds with the solver's final impulses (world.h:731).
@@ -616,6 +658,8 @@ private:
     float accumulator_ = 0.0f;
     mutable std::vector<pxg_manifold> manifolds_;
     mutable bool manifolds_valid_ = false;
+    mutable std::vector<ContactEvent> events_;
+    mutable bool events_valid_ = false;
 };
 
 }  // namespace pxg

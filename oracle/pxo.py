@@ -52,6 +52,7 @@ def lib():
             "pxo_point_block_mass": ([vp, vp, f32, f32, vp, vp, vp], i32),
             "pxo_hinge_angle": ([vp, vp, vp, vp, u32], f32), "pxo_cone_swing": ([vp, vp, vp, u32], f32),
             "pxo_atan2": ([f32, f32, u32], f32),
+            "pxo_get_events": ([vp, vp, u32], u32), "pxo_clear_events": ([vp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -143,11 +144,28 @@ class OracleWorld:
         b = np.ascontiguousarray(bodies)
         lib().pxo_set_bodies(self.h, ptr(b), len(b))
 
+    def set_shapes(self, shapes):
+        """Replaces the shape table (proxies are created afresh, world.h:1369-1382)."""
+        a = np.ascontiguousarray(shapes)
+        self.n_shapes = len(a)
+        lib().pxo_set_shapes(self.h, ptr(a), len(a))
+
     def pairs(self):
         n = lib().pxo_get_pairs(self.h, None, 0)
         out = np.zeros((max(n, 1), 2), np.uint32)
         lib().pxo_get_pairs(self.h, ptr(out), n)
         return out[:n]
+
+    def events(self):
+        """Contact events queued since the last clear_events(): rows (a, b, type, is_trigger), type
+        0 enter, 1 stay, 2 exit (world.h:744, collision/contact_event.h:29-36)."""
+        n = lib().pxo_get_events(self.h, None, 0)
+        out = np.zeros((max(n, 1), 4), np.uint32)
+        lib().pxo_get_events(self.h, ptr(out), n)
+        return out[:n]
+
+    def clear_events(self):
+        lib().pxo_clear_events(self.h)
 
     def manifolds(self):
         n = lib().pxo_get_manifolds(self.h, None, 0)

@@ -109,6 +109,16 @@ template <class T> World<T>& W(Handle* h);
 template <> World<float>& W<float>(Handle* h) { return h->f; }
 template <> World<double>& W<double>(Handle* h) { return h->d; }
 
+// events(), clear_events() (world.h:744-748); out4: a, b (owner body indices), type
+// (0 enter, 1 stay, 2 exit), is_trigger per event
+template <class T> uint32_t get_events_t(World<T>& w, uint32_t* out4, uint32_t cap) {
+    uint32_t n = uint32_t(w.events.size());
+    for (uint32_t i = 0; i < n && i < cap; ++i) {
+        out4[4 * i] = w.events[i].a; out4[4 * i + 1] = w.events[i].b;
+        out4[4 * i + 2] = w.events[i].type; out4[4 * i + 3] = w.events[i].trigger;
+    }
+    return n;
+}
 #define DISPATCH(h, call) ((h)->dbl ? call<double>(W<double>(h)) : call<float>(W<float>(h)))
 
 }  // namespace
@@ -221,15 +231,30 @@ void pxo_step_phase(void* p, uint32_t phase, float hstep) {
     if (h->dbl) { if (phase == 0) h->d.step_phase_collide(double(hstep)); else h->d.step_phase_solve(double(hstep)); }
     else { if (phase == 0) h->f.step_phase_collide(hstep); else h->f.step_phase_solve(hstep); }
 }
+void pxo_clear_events(void* p);
 // Runs `steps` fixed steps and returns the wall time of the loop in seconds (steady_clock).
 double pxo_run(void* p, uint32_t steps, float hstep) {
     Handle* h = static_cast<Handle*>(p);
     auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t i = 0; i < steps; ++i) pxo_step_fixed(p, hstep);
+    for (uint32_t i = 0; i < steps; ++i) {
+        pxo_clear_events(p);  // one step() per frame clears the queue first (world.h:1230)
+        pxo_step_fixed(p, hstep);
+    }
     auto t1 = std::chrono::steady_clock::now();
     (void)h;
     return std::chrono::duration<double>(t1 - t0).count();
 }
+// events(), clear_events() (world.h:744-748)
+uint32_t pxo_get_events(void* p, uint32_t* out4, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    return h->dbl ? get_events_t(h->d, out4, cap) : get_events_t(h->f, out4, cap);
+}
+void pxo_clear_events(void* p) {
+    Handle* h = static_cast<Handle*>(p);
+    h->f.events.clear();
+    h->d.events.clear();
+}
+
 uint32_t pxo_get_pairs(void* p, uint32_t* ab, uint32_t cap) {
     Handle* h = static_cast<Handle*>(p);
     auto go = [&](auto& w) -> uint32_t {

@@ -6,6 +6,8 @@
 #include "pxo_world.hpp"
 #include "pxo_joints.hpp"
 
+#include <algorithm>
+#include <iterator>
 #include <set>
 
 namespace pxo {
@@ -77,6 +79,11 @@ template <class T> struct World {
     std::vector<Joint<T>> joints;
     std::set<uint64_t> ignored;
     Atan2Mode atan2_mode = Atan2Mode::Libm;
+    // contact events (collision/contact_event.h:29-36): last step's confirmed shape pairs,
+    // [0] solid and [1] trigger (world.h:2112-2113), and the queue callers drain (world.h:744)
+    struct Event { uint32_t a, b, type, trigger; };
+    std::vector<std::pair<uint32_t, uint32_t>> ev_prev[2];
+    std::vector<Event> events;
     // diagnostics of the last step
     uint32_t vel_passes = 0;
     T last_residual = 0;
@@ -576,10 +583,39 @@ template <class T> struct World {
         }
     }
 
+    // world.h:1964-1978 (the caches roll over and refill) and 1998-2018 (emit_contact_events_):
+    // per cache, PairCache::classify (pair_cache.h:63-82) as sorted-set differences, pushed as
+    // all enters, then all stays, then all exits; the solid cache before the trigger cache.
+    void emit_events() {
+        typedef std::pair<uint32_t, uint32_t> PP;
+        std::vector<PP> cur[2];
+        for (const auto& m : manifolds) cur[m.trigger ? 1 : 0].push_back(PP(m.sa, m.sb));
+        for (int k = 0; k < 2; ++k) {
+            std::sort(cur[k].begin(), cur[k].end());  // PairCache::finalize
+            cur[k].erase(std::unique(cur[k].begin(), cur[k].end()), cur[k].end());
+            std::vector<PP> added, stayed, removed;
+            std::set_difference(cur[k].begin(), cur[k].end(), ev_prev[k].begin(), ev_prev[k].end(), std::back_inserter(added));
+            std::set_difference(ev_prev[k].begin(), ev_prev[k].end(), cur[k].begin(), cur[k].end(), std::back_inserter(removed));
+            std::set_difference(cur[k].begin(), cur[k].end(), added.begin(), added.end(), std::back_inserter(stayed));
+            auto push = [&](const PP& p, uint32_t type) {
+                if (p.first >= shapes.size() || p.second >= shapes.size()) return;
+                if (type != 2u && (!shapes[p.first].enabled || !shapes[p.second].enabled)) return;
+                uint32_t oa = shapes[p.first].owner, ob = shapes[p.second].owner;
+                if (oa >= bodies.size() || ob >= bodies.size() || !bodies[oa].alive || !bodies[ob].alive) return;
+                events.push_back(Event{oa, ob, type, uint32_t(k)});
+            };
+            for (const PP& p : added) push(p, 0u);
+            for (const PP& p : stayed) push(p, 1u);
+            for (const PP& p : removed) push(p, 2u);
+            ev_prev[k].swap(cur[k]);
+        }
+    }
+
     void step_phase_collide(T h) {
         integrate_forces_all(h);
         broadphase(h);
         narrowphase(h);
+        emit_events();
     }
     void step_phase_solve(T h) {
         solve(h);

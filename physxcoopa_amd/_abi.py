@@ -55,6 +55,11 @@ MANIFOLD_DTYPE = np.dtype([
 ])
 assert MANIFOLD_DTYPE.itemsize == 240
 
+PXG_EVENT_ENTER, PXG_EVENT_STAY, PXG_EVENT_EXIT = 0, 1, 2
+
+EVENT_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("type", "<u4"), ("is_trigger", "<u4")])
+assert EVENT_DTYPE.itemsize == 16
+
 PXG_HINGE, PXG_BALL, PXG_CONE_TWIST = 0, 1, 2
 
 JOINT_DTYPE = np.dtype([

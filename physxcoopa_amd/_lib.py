@@ -42,6 +42,9 @@ _SIGNATURES = {
                            ctypes.c_int),
     "pxg_set_warm_cache": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_state_hash": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_get_events": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
+    "pxg_clear_events": ([ctypes.c_void_p], ctypes.c_int),
+    "pxg_set_events": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_mlp": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_set_recording": ([ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_records": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,

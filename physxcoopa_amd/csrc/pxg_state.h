@@ -56,6 +56,9 @@ struct Counters {  // device-side counters, read back by the host at the step's 
     unsigned long long prof_vel_bytes;  // algorithmic bytes of profiled velocity passes
     uint32_t passes;        // velocity passes run by the last solve
     uint32_t res_slot[2];   // residual max-reduction slots (float bits), alternating per pass
+    uint32_t ev_n;          // contact events queued (<= the queue's capacity)
+    uint32_t ev_dropped;    // events lost to a full queue since it was last cleared
+    uint32_t ev_snap_n[2];  // entries of the two pair snapshots
 };
 
 struct DevState {
@@ -162,6 +165,18 @@ struct DevState {
     uint32_t *miss_idx; // learned: point index of each compact row
     float *pred;        // learned: clamped predictions, 3 per compact row
     uint32_t record;    // 1: write every active point's features (dataset recording)
+    // contact events: each step's confirmed shape pairs (its valid manifolds) as sorted keys
+    // kind << 34 | shape a << 17 | shape b (kind 0 solid, 1 trigger), double-buffered (this
+    // step's in ev_snap[cur], last step's in the other, counts in Counters::ev_snap_n); the
+    // unsorted keys and the sort's scratch; per snapshot entry of both steps its event category,
+    // per block of 1024 entries and category the event count, then the queue position; the queue
+    unsigned long long *ev_keys, *ev_snap[2];  // cap_pairs each
+    void *ev_sort_tmp;
+    size_t ev_sort_bytes;
+    uint8_t *ev_code;   // 2 * cap_pairs: this step's entries, then last step's
+    uint32_t *ev_boff;  // 6 categories x (2 * ceil(cap_pairs / 1024) + 2) blocks
+    uint4 *ev_out;      // body a, body b, type (0 enter, 1 stay, 2 exit), is_trigger
+    uint32_t cap_events;
 };
 
 constexpr uint32_t kNoPred = 0xFFFFFFFFu;
@@ -206,6 +221,11 @@ void launch_joint_velocity(const DevState& d, const StepParams& p, int rev, uint
 void launch_joint_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st);
 void launch_joints_post(const DevState& d, const StepParams& p, uint32_t pos_iters, hipStream_t st);
 void launch_init_bodies(const DevState& d, hipStream_t st);
+// contact events of this step (world.h:1964-2018, pxg_events.hip): sorts this step's valid
+// manifolds (at most n_pairs) into the snapshot ev_snap[cur], compares it with last step's (at
+// most prev_bound entries) and appends the events to the queue
+size_t events_sort_bytes(uint32_t cap_pairs);
+hipError_t launch_events(const DevState& d, uint32_t n_pairs, uint32_t prev_bound, int cur, hipStream_t st);
 void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st);
 void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st);
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);

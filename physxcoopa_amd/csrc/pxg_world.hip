@@ -71,6 +71,12 @@ struct pxg_world {
     std::vector<uint32_t> jl_fwd, jl_rev;
     std::vector<uint64_t> ignored;
     uint32_t cap_j = 0, cap_nc = 0;
+    // contact events: on by default (the reference always classifies, world.h:1977-1978); the
+    // snapshot written this step, last step's pair count, the queue capacity (0: default)
+    bool ev_on = true;
+    int ev_cur = 0;
+    uint32_t ev_prev_n = 0;
+    uint32_t ev_cap_req = 0;
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -297,6 +303,17 @@ static int alloc_shapes(pxg_world* w, uint32_t n) {
     size_t P = w->cap_p, M = w->cap_m;
     DALLOC(d.pairs, P); DALLOC(d.pm_valid, P); DALLOC(d.pm_off, P);
     DALLOC(d.pm_ab, P); DALLOC(d.pm_nf, P); DALLOC(d.pm_meta, P); DALLOC(d.pm_p, 4 * P); DALLOC(d.pm_q, 4 * P);
+    DALLOC(d.ev_keys, P); DALLOC(d.ev_snap[0], P); DALLOC(d.ev_snap[1], P);
+    d.ev_sort_bytes = events_sort_bytes(w->cap_p);
+    if (!d.ev_sort_bytes) return fail(PXG_ERR_HIP, "pxg_set_shapes: sizing the contact-event sort failed");
+    {
+        uint8_t* tmp = nullptr;
+        DALLOC(tmp, d.ev_sort_bytes);
+        d.ev_sort_tmp = tmp;
+    }
+    DALLOC(d.ev_code, 2 * P); DALLOC(d.ev_boff, 12 * ((P + 1023) / 1024) + 12);
+    d.cap_events = w->ev_cap_req ? w->ev_cap_req : uint32_t(std::min<size_t>(2 * M + 1024, 0xFFFFFFFFull));
+    DALLOC(d.ev_out, d.cap_events);
     DALLOC(d.m_ab, M); DALLOC(d.m_shapes, M); DALLOC(d.m_nf, M); DALLOC(d.m_meta, M); DALLOC(d.m_p, 4 * M);
     DALLOC(d.m_q, 4 * M); DALLOC(d.m_colour, M);
     DALLOC(d.adj, 2 * M); DALLOC(d.m_pred, 2 * M);
@@ -574,6 +591,11 @@ static int phase_collide(pxg_world* w, float h) {
     if (P > w->cap_p) return fail(PXG_ERR_CAPACITY, "pair buffer full (" + std::to_string(P) + " pairs)");
     launch_broadphase_write(d, p, st);
     launch_narrowphase(d, p, P, st);
+    if (w->ev_on) {  // world.h:1964-1978
+        HIPCHK(launch_events(d, P, w->ev_prev_n, w->ev_cur, st));
+        w->ev_prev_n = P;
+        w->ev_cur ^= 1;
+    }
     if (P == 0) {
         uint32_t zero = 0;
         HIPCHK(hipMemcpyAsync(&d.cnt->manifolds, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
@@ -822,6 +844,8 @@ extern "C" int pxg_step_fixed(pxg_world* w, float h) {
 // step(): accumulate the clamped frame time and run fixed substeps (world.h:1229-1240)
 extern "C" int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run) {
     if (!w) return fail(PXG_ERR_INVALID, "pxg_step: null handle");
+    int rc = pxg_clear_events(w);  // world.h:1230
+    if (rc != PXG_OK) return rc;
     float frame = std::min(dt, 0.25f);
     w->accumulator += frame;
     float h = w->cfg.fixed_dt;
@@ -833,6 +857,58 @@ extern "C" int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run) {
         ++n;
     }
     if (substeps_run) *substeps_run = n;
+    return PXG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// contact events (world.h:744-748, 1998-2018)
+// ---------------------------------------------------------------------------------------------
+extern "C" int pxg_clear_events(pxg_world* w) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_clear_events: null handle");
+    if (!w->d.cnt) return PXG_OK;
+    launch_fill_u32(&w->d.cnt->ev_n, 0u, 2, w->stream);  // ev_n, ev_dropped
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_events(pxg_world* w, uint32_t enabled, uint32_t capacity) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_set_events: null handle");
+    bool on = enabled != 0;
+    if (on && !w->ev_on && w->d.cnt) {  // no snapshot was kept while events were off
+        w->ev_prev_n = 0;
+        launch_fill_u32(w->d.cnt->ev_snap_n, 0u, 2, w->stream);
+    }
+    w->ev_on = on;
+    if (capacity && capacity != w->d.cap_events && w->d.ev_out) {
+        HIPCHK(hipStreamSynchronize(w->stream));
+        dfree(w, w->d.ev_out);
+        w->d.ev_out = nullptr;
+        w->d.cap_events = 0;
+        DALLOC(w->d.ev_out, capacity);
+        w->d.cap_events = capacity;
+        int r = pxg_clear_events(w);
+        if (r != PXG_OK) return r;
+    }
+    if (capacity) w->ev_cap_req = capacity;
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_events(pxg_world* w, pxg_contact_event* out, uint32_t cap, uint32_t* n) {
+    if (!w || !n || (!out && cap)) return fail(PXG_ERR_INVALID, "pxg_get_events: null argument");
+    *n = 0;
+    if (!w->d.cnt || !w->d.ev_out) return PXG_OK;
+    uint32_t hv[2] = {0u, 0u};
+    HIPCHK(hipMemcpyAsync(hv, &w->d.cnt->ev_n, sizeof(hv), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    *n = hv[0];
+    uint32_t k = std::min(cap, hv[0]);
+    if (k) {
+        static_assert(sizeof(pxg_contact_event) == sizeof(uint4), "event record layout");
+        HIPCHK(hipMemcpy(out, w->d.ev_out, size_t(k) * sizeof(uint4), hipMemcpyDeviceToHost));
+    }
+    if (hv[1])
+        return fail(PXG_ERR_CAPACITY, "pxg_get_events: " + std::to_string(hv[1]) + " events were dropped: the queue holds " +
+                                          std::to_string(w->d.cap_events) + " (clear it with pxg_clear_events, or raise "
+                                          "the capacity with pxg_set_events)");
     return PXG_OK;
 }
 

@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, JOINT_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, Config, Stats, Profile, ptr
+from ._abi import BODY_DTYPE, JOINT_DTYPE, SHAPE_DTYPE, MANIFOLD_DTYPE, EVENT_DTYPE, Config, Stats, Profile, ptr
 from ._lib import lib, check
 
 
@@ -100,6 +100,12 @@ class World:
         check(lib().pxg_set_bodies(self.h, ptr(b), len(b)), "pxg_set_bodies")
         self.n_bodies = len(b)
 
+    def set_shapes(self, shapes):
+        """Replaces the shape table; proxies are created afresh (world.h:1369-1382)."""
+        s = np.ascontiguousarray(shapes, dtype=SHAPE_DTYPE)
+        check(lib().pxg_set_shapes(self.h, ptr(s), len(s)), "pxg_set_shapes")
+        self.n_shapes = len(s)
+
     def pairs(self):
         n = ctypes.c_uint32()
         check(lib().pxg_get_pairs(self.h, None, 0, ctypes.byref(n)), "pxg_get_pairs")
@@ -113,6 +119,22 @@ class World:
         out = np.zeros(max(n.value, 1), MANIFOLD_DTYPE)
         check(lib().pxg_get_manifolds(self.h, ptr(out), n.value, ctypes.byref(n)), "pxg_get_manifolds")
         return out[:n.value]
+
+    def events(self):
+        """Contact events queued since the last clear (world.h:744): a structured array of
+        EVENT_DTYPE (a, b owner bodies; type 0 enter, 1 stay, 2 exit; is_trigger). Raises when
+        the queue overflowed since it was last cleared."""
+        n = ctypes.c_uint32()
+        check(lib().pxg_get_events(self.h, None, 0, ctypes.byref(n)), "pxg_get_events")
+        out = np.zeros(max(n.value, 1), EVENT_DTYPE)
+        check(lib().pxg_get_events(self.h, ptr(out), n.value, ctypes.byref(n)), "pxg_get_events")
+        return out[:n.value]
+
+    def clear_events(self):
+        check(lib().pxg_clear_events(self.h), "pxg_clear_events")
+
+    def set_events(self, enabled, capacity=0):
+        check(lib().pxg_set_events(self.h, 1 if enabled else 0, capacity), "pxg_set_events")
 
     def solve_order(self):
         """(order, colour_offsets): manifold indices in solve order and the colour boundaries."""

@@ -586,6 +586,60 @@ void pair_collision_switch_lets_a_sphere_fall_through_the_ground() {
 
 }  // namespace
 
+// contact_events_test.cpp:23-61: a 0.1 m probe flies through a static trigger sphere of radius 1
+// with gravity off; one enter, one exit, a stay every step in between
+void trigger_fires_enter_and_exit_once_with_stays_between() {
+    pxg::PhysicsWorld world;
+    world.set_gravity(0.0f, 0.0f, 0.0f);
+    pxg_shape trig = sphere_shape(1.0f);
+    trig.is_trigger = 1;
+    pxg::BodyId trigger_id = world.add_body(body(PXG_STATIC, 0, 0, 0), trig);
+    pxg_body probe = body(PXG_DYNAMIC, -3.0f, 0, 0);
+    probe.linear_velocity[0] = 2.0f;
+    float inv = 1.0f / (0.4f * 1.0f * 0.1f * 0.1f);
+    probe.inv_inertia_local[0] = probe.inv_inertia_local[1] = probe.inv_inertia_local[2] = inv;
+    pxg::BodyId probe_id = world.add_body(probe, sphere_shape(0.1f));
+    int enter = 0, stay = 0, exit_ = 0;
+    for (int i = 0; i < 200; ++i) {
+        world.clear_events();
+        world.step_fixed(kH);
+        for (const pxg::ContactEvent& ev : world.events()) {
+            if (!ev.is_trigger) continue;
+            bool pair = (ev.a == trigger_id && ev.b == probe_id) || (ev.a == probe_id && ev.b == trigger_id);
+            if (!pair) continue;
+            if (ev.type == pxg::ContactEvent::Type::Enter) ++enter;
+            else if (ev.type == pxg::ContactEvent::Type::Stay) ++stay;
+            else ++exit_;
+        }
+    }
+    CHECK(enter == 1);
+    CHECK(exit_ == 1);
+    CHECK(stay > 0);
+}
+
+// the sphere bounces off the ground (restitution 0.9 and 0, Average: 0.45): solid enters and
+// exits; step() drains the queue each call (world.h:1230)
+void bouncing_sphere_enters_and_exits_and_step_clears_the_queue() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId ground = add_static_ground(world);
+    pxg::BodyId ball = add_dynamic_sphere(world, 0, 0, 3.0f, 0.5f, 0.9f);
+    int enter = 0, exit_ = 0;
+    size_t most = 0;
+    for (int i = 0; i < 300; ++i) {
+        world.step(kH);
+        most = std::max(most, world.events().size());
+        for (const pxg::ContactEvent& ev : world.events()) {
+            bool pair = (ev.a == ground && ev.b == ball) || (ev.a == ball && ev.b == ground);
+            if (!pair || ev.is_trigger) continue;
+            if (ev.type == pxg::ContactEvent::Type::Enter) ++enter;
+            if (ev.type == pxg::ContactEvent::Type::Exit) ++exit_;
+        }
+    }
+    CHECK(enter >= 1);
+    CHECK(exit_ >= 1);
+    CHECK(most <= 2);  // one substep per step(kH): at most this step's events of the one pair
+}
+
 int main() {
     struct T {
         const char* name;
@@ -613,6 +667,9 @@ int main() {
         {"removing_a_body_invalidates_its_joints_and_stale_joint_handles_are_rejected",
          removing_a_body_invalidates_its_joints_and_stale_joint_handles_are_rejected},
         {"pair_collision_switch_lets_a_sphere_fall_through_the_ground", pair_collision_switch_lets_a_sphere_fall_through_the_ground},
+        {"trigger_fires_enter_and_exit_once_with_stays_between", trigger_fires_enter_and_exit_once_with_stays_between},
+        {"bouncing_sphere_enters_and_exits_and_step_clears_the_queue",
+         bouncing_sphere_enters_and_exits_and_step_clears_the_queue},
     };
     for (const T& t : tests) {
         int before = g_failures;

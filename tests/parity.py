@@ -6,7 +6,8 @@ GPU's solve order (colour by colour), then both step and every output is compare
   * manifolds: identical key sets, counts and feature ids; points, normals, depths within 1e-5 m;
   * solver impulses within 1e-4 relative (atol 1e-6);
   * body state after integration within 1e-5;
-  * the colouring is valid (no two manifolds of one colour share an awake dynamic body).
+  * the colouring is valid (no two manifolds of one colour share an awake dynamic body);
+  * contact events (enter / stay / exit): identical lists, in the same order.
 Bit-exact agreement is also counted and reported.
 """
 import numpy as np
@@ -82,6 +83,11 @@ def compare_step(gw, ow, pre_bodies, report):
             gv, ov = gj[f].astype(np.float64), oj[f].astype(np.float64)
             assert np.all(np.abs(gv - ov) <= TOL_IMP_REL * np.abs(ov) + TOL_IMP_ABS * 10), "joint %s differs" % f
         report["bitexact_joints"] += int(np.array_equal(gj.view(np.uint8), oj.view(np.uint8)))
+    if hasattr(gw, "events"):
+        ge, oe = gw.events(), ow.events()
+        ge = np.stack([ge["a"], ge["b"], ge["type"], ge["is_trigger"]], axis=1) if len(ge) else np.zeros((0, 4), np.uint32)
+        assert np.array_equal(ge, oe), "contact events differ: gpu %d oracle %d" % (len(ge), len(oe))
+        report["events"] = report.get("events", 0) + len(ge)
     report["steps"] += 1
     report["colours"] = max(report["colours"], len(offs) - 1)
 
@@ -112,6 +118,9 @@ def lockstep(World, OracleWorld, cfg, bodies, shapes, steps, resync=True, feed_o
             if joints is not None:
                 ow.set_joints(gw.joints())
         pre = gw.bodies()
+        if hasattr(gw, "clear_events"):
+            gw.clear_events()
+            ow.clear_events()
         gw.step_fixed()
         gm = gw.manifolds()
         if feed_order:
