@@ -234,6 +234,35 @@ typedef struct pxg_contact_event {
     uint32_t is_trigger;
 } pxg_contact_event;
 
+/* Scene queries (query/queries.h:56-98, geometry/ray.h:32-35). A ray starts at origin and runs
+ * along direction (unit length) up to max_distance (FLT_MAX: unbounded). */
+typedef struct pxg_ray {
+    float origin[3];
+    float max_distance;
+    float direction[3];
+    float reserved;
+} pxg_ray;
+#define PXG_NO_BODY 0xFFFFFFFFu
+/* QueryFilter without its host-side predicate: bit shape.layer of layer_mask must be set, trigger
+ * shapes are skipped unless include_triggers, the shapes of body ignore_body are skipped. A null
+ * filter passes every shape. */
+typedef struct pxg_query_filter {
+    uint32_t layer_mask;       /* ~0u: every layer */
+    uint32_t include_triggers; /* 1 */
+    uint32_t ignore_body;      /* PXG_NO_BODY: none */
+    uint32_t reserved;
+} pxg_query_filter;
+/* RaycastHit (query/queries.h:56-76): hit = 0 when the ray hit nothing (the rest is then zero). */
+typedef struct pxg_raycast_hit {
+    float point[3];
+    float distance;
+    float normal[3];
+    uint32_t hit;
+    uint32_t body;
+    uint32_t shape_index;
+    uint32_t reserved[2];
+} pxg_raycast_hit;
+
 typedef struct pxg_world pxg_world;
 
 /* --- configuration and lifetime --- */
@@ -303,6 +332,17 @@ int pxg_clear_events(pxg_world* w);                               /* clear_event
  * 2 * max_manifolds + 1024). Switching it on starts from an empty pair snapshot, so the next step
  * reports its contacts as enters. No counterpart in the reference, which always classifies. */
 int pxg_set_events(pxg_world* w, uint32_t enabled, uint32_t capacity);
+/* Scene raycasts against the bodies' current poses, culled by the broadphase's fat AABBs as the
+ * reference's trees cull them (sphere, box and capsule shapes). pxg_raycast: the closest hit of
+ * each of n rays (raycast(), world.h:766-791; equal distances go to the lower shape slot).
+ * pxg_raycast_any: any[i] = 1 when ray i hits anything (raycast_any(), world.h:803-817).
+ * pxg_raycast_all: every hit of one ray, nearest first (raycast_all(), world.h:825-847); n
+ * receives the hit count, min(cap, n) hits are copied. Host buffers in and out; the queries run on
+ * the world's stream after its pending steps. */
+int pxg_raycast(pxg_world* w, const pxg_ray* rays, uint32_t n, const pxg_query_filter* filter, pxg_raycast_hit* hits);
+int pxg_raycast_any(pxg_world* w, const pxg_ray* rays, uint32_t n, const pxg_query_filter* filter, uint32_t* any);
+int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query_filter* filter, pxg_raycast_hit* hits,
+                    uint32_t cap, uint32_t* n);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);
 

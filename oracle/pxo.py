@@ -53,6 +53,8 @@ def lib():
             "pxo_hinge_angle": ([vp, vp, vp, vp, u32], f32), "pxo_cone_swing": ([vp, vp, vp, u32], f32),
             "pxo_atan2": ([f32, f32, u32], f32),
             "pxo_get_events": ([vp, vp, u32], u32), "pxo_clear_events": ([vp], None),
+            "pxo_raycast": ([vp, vp, u32, vp, vp], None), "pxo_raycast_all": ([vp, vp, vp, vp, u32], u32),
+            "pxo_ray_shape": ([vp, vp, vp, vp, vp, vp], i32), "pxo_ray_aabb": ([vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -167,6 +169,25 @@ class OracleWorld:
     def clear_events(self):
         lib().pxo_clear_events(self.h)
 
+    def raycast(self, rays, query_filter=None):
+        """Closest hit per ray (RAY_DTYPE in, HIT_DTYPE out; world.h:766-791)."""
+        from physxcoopa_amd._abi import HIT_DTYPE
+        r = np.ascontiguousarray(rays)
+        out = np.zeros(max(len(r), 1), HIT_DTYPE)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter))
+        lib().pxo_raycast(self.h, ptr(r), len(r), f, ptr(out))
+        return out[:len(r)]
+
+    def raycast_all(self, ray, query_filter=None):
+        """Every hit of one ray, nearest first (world.h:825-847)."""
+        from physxcoopa_amd._abi import HIT_DTYPE
+        r = np.ascontiguousarray(ray).reshape(1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter))
+        n = lib().pxo_raycast_all(self.h, ptr(r), f, None, 0)
+        out = np.zeros(max(n, 1), HIT_DTYPE)
+        lib().pxo_raycast_all(self.h, ptr(r), f, ptr(out), n)
+        return out[:n]
+
     def manifolds(self):
         n = lib().pxo_get_manifolds(self.h, None, 0)
         out = np.zeros(max(n, 1), MANIFOLD_DTYPE)
@@ -275,6 +296,21 @@ def mat3_cast(q):
     k = f32(q, 4)
     lib().pxo_mat3_cast(ptr(k), ptr(out))
     return out.reshape(3, 3)  # [col][row]
+
+
+def ray_shape(shape, pos, rot, ray):
+    """raycast_shape for one shape at a pose: (hit, t, normal)."""
+    s = np.ascontiguousarray(np.array(shape).reshape(()))
+    r = np.ascontiguousarray(np.array(ray).reshape(()))
+    t = np.zeros(1, np.float32)
+    n = np.zeros(3, np.float32)
+    hit = lib().pxo_ray_shape(ptr(s), ptr(f32(pos, 3)), ptr(f32(rot, 4)), ptr(r), ptr(t), ptr(n))
+    return bool(hit), float(t[0]), n
+
+
+def ray_aabb(ray, mn, mx):
+    r = np.ascontiguousarray(np.array(ray).reshape(()))
+    return bool(lib().pxo_ray_aabb(ptr(r), ptr(f32(mn, 3)), ptr(f32(mx, 3))))
 
 
 def pack_key(a, b, fid):

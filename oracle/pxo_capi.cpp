@@ -3,6 +3,7 @@
 // extern "C" surface of the CPU oracle, loaded by tests/ (ctypes) and by bench.py's CPU
 // baseline leg. Mirrors the pxg_* C ABI so a test can drive both sides with the same buffers.
 #include "pxo_step.hpp"
+#include "pxo_query.hpp"
 
 #include <chrono>
 #include <map>
@@ -119,6 +120,24 @@ template <class T> uint32_t get_events_t(World<T>& w, uint32_t* out4, uint32_t c
     }
     return n;
 }
+template <class T> Ray<T> ray_from(const pxg_ray& r) {
+    Ray<T> o;
+    o.o = v3<T>(r.origin);
+    o.d = v3<T>(r.direction);
+    o.max_d = T(r.max_distance);
+    return o;
+}
+inline QueryFilter filter_from(const pxg_query_filter* f) {
+    QueryFilter q;
+    if (f) { q.layer_mask = f->layer_mask; q.include_triggers = f->include_triggers != 0; q.ignore = f->ignore_body; }
+    return q;
+}
+template <class T> void hit_to(const RayHit<T>& h, pxg_raycast_hit& o) {
+    std::memset(&o, 0, sizeof(o));
+    put3(o.point, h.point); o.distance = float(h.distance); put3(o.normal, h.normal);
+    o.hit = 1; o.body = h.body; o.shape_index = h.shape;
+}
+
 #define DISPATCH(h, call) ((h)->dbl ? call<double>(W<double>(h)) : call<float>(W<float>(h)))
 
 }  // namespace
@@ -244,6 +263,46 @@ double pxo_run(void* p, uint32_t steps, float hstep) {
     (void)h;
     return std::chrono::duration<double>(t1 - t0).count();
 }
+// raycast() / raycast_any() (closest = 0: any hit, reported as the closest) for n rays;
+// raycast_all() for one ray (returns the hit count)
+void pxo_raycast(void* p, const pxg_ray* rays, uint32_t n, const pxg_query_filter* f, pxg_raycast_hit* out) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        for (uint32_t i = 0; i < n; ++i) {
+            auto hits = world_raycast(w, ray_from<T>(rays[i]), filter_from(f), true);
+            if (hits.empty()) std::memset(&out[i], 0, sizeof(out[i]));
+            else hit_to(hits[0], out[i]);
+        }
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+}
+uint32_t pxo_raycast_all(void* p, const pxg_ray* ray, const pxg_query_filter* f, pxg_raycast_hit* out, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    uint32_t n = 0;
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        auto hits = world_raycast(w, ray_from<T>(*ray), filter_from(f), false);
+        n = uint32_t(hits.size());
+        for (uint32_t i = 0; i < n && i < cap; ++i) hit_to(hits[i], out[i]);
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+    return n;
+}
+// raycast_shape for one shape at a pose (float)
+int pxo_ray_shape(const pxg_shape* s, const float* pos, const float* rot, const pxg_ray* ray, float* t, float* n3) {
+    Shape<float> sh = shape_from<float>(*s);
+    float tt = 0.0f;
+    V3<float> nn;
+    if (!raycast_shape(ray_from<float>(*ray), sh, v3<float>(pos), q4<float>(rot), tt, nn)) return 0;
+    *t = tt;
+    put3(n3, nn);
+    return 1;
+}
+int pxo_ray_aabb(const pxg_ray* ray, const float* mn, const float* mx) {
+    return ray_aabb(ray_from<float>(*ray), v3<float>(mn), v3<float>(mx)) ? 1 : 0;
+}
+
 // events(), clear_events() (world.h:744-748)
 uint32_t pxo_get_events(void* p, uint32_t* out4, uint32_t cap) {
     Handle* h = static_cast<Handle*>(p);

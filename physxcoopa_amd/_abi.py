@@ -60,6 +60,34 @@ PXG_EVENT_ENTER, PXG_EVENT_STAY, PXG_EVENT_EXIT = 0, 1, 2
 EVENT_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("type", "<u4"), ("is_trigger", "<u4")])
 assert EVENT_DTYPE.itemsize == 16
 
+RAY_DTYPE = np.dtype([("origin", "<f4", 3), ("max_distance", "<f4"), ("direction", "<f4", 3), ("reserved", "<f4")])
+assert RAY_DTYPE.itemsize == 32
+PXG_NO_BODY = 0xFFFFFFFF
+FILTER_DTYPE = np.dtype([("layer_mask", "<u4"), ("include_triggers", "<u4"), ("ignore_body", "<u4"), ("reserved", "<u4")])
+assert FILTER_DTYPE.itemsize == 16
+HIT_DTYPE = np.dtype([("point", "<f4", 3), ("distance", "<f4"), ("normal", "<f4", 3), ("hit", "<u4"), ("body", "<u4"),
+                      ("shape_index", "<u4"), ("reserved", "<u4", 2)])
+assert HIT_DTYPE.itemsize == 48
+
+
+def rays(origins, directions, max_distance=np.finfo(np.float32).max):
+    """A RAY_DTYPE array from (n, 3) origins and directions (and one or n max distances)."""
+    o = np.asarray(origins, np.float32).reshape(-1, 3)
+    r = np.zeros(len(o), RAY_DTYPE)
+    r["origin"] = o
+    r["direction"] = np.asarray(directions, np.float32).reshape(-1, 3)
+    r["max_distance"] = max_distance
+    return r
+
+
+def query_filter(layer_mask=0xFFFFFFFF, include_triggers=True, ignore_body=PXG_NO_BODY):
+    f = np.zeros(1, FILTER_DTYPE)
+    f["layer_mask"] = layer_mask
+    f["include_triggers"] = 1 if include_triggers else 0
+    f["ignore_body"] = ignore_body
+    return f
+
+
 PXG_HINGE, PXG_BALL, PXG_CONE_TWIST = 0, 1, 2
 
 JOINT_DTYPE = np.dtype([

@@ -44,6 +44,10 @@ _SIGNATURES = {
     "pxg_state_hash": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_get_events": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
     "pxg_clear_events": ([ctypes.c_void_p], ctypes.c_int),
+    "pxg_raycast": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_raycast_any": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_raycast_all": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                         ctypes.c_void_p], ctypes.c_int),
     "pxg_set_events": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_mlp": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_set_recording": ([ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),

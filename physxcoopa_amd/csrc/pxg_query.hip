@@ -1,0 +1,323 @@
+// Scene raycasts against one world (query/queries.h, geometry/ray.h, world.h:766-847).
+//
+// The reference walks its two AABB trees with the ray's slab test against the proxies' fat AABBs
+// and tests every leaf it reaches exactly. A leaf's box lies inside its ancestors' and the slab
+// test is monotonic in the box, so the set of shapes tested is exactly the live shapes whose fat
+// AABB the slab test accepts; the GPU tests that set directly. One thread owns one ray; a block's
+// 256 rays stream the shape table through LDS in tiles of 256 shapes, so each shape record is read
+// from HBM once per block. A prep pass turns every shape into its world geometry (sphere centre
+// and radius, box centre, half extents and orientation, capsule end points and radius) from the
+// bodies' current poses once per query batch. The closest hit keeps the nearest distance; shapes
+// are visited in slot order and only a strictly nearer hit replaces the best, so equal distances
+// go to the lower slot.
+#include "pxg_state.h"
+#include "pxg_device.h"
+
+using namespace pxd;
+
+namespace pxg {
+namespace {
+
+constexpr uint32_t kQueryTile = 256u;  // shapes per LDS tile = rays per block
+constexpr uint32_t QF_LIVE = 1u, QF_TRIGGER = 2u;
+
+// glm::inverse(quat): conjugate / dot(q, q)
+__device__ __forceinline__ quat qinverse_q(quat q) {
+    float d = (q.x * q.x + q.y * q.y) + (q.z * q.z + q.w * q.w);
+    quat r;
+    r.w = q.w / d; r.x = -q.x / d; r.y = -q.y / d; r.z = -q.z / d;
+    return r;
+}
+// std::max / std::min over an initializer list: the first largest / smallest element
+__device__ __forceinline__ float max4(float a, float b, float c, float d) {
+    float m = a;
+    if (m < b) m = b;
+    if (m < c) m = c;
+    if (m < d) m = d;
+    return m;
+}
+__device__ __forceinline__ float min4(float a, float b, float c, float d) {
+    float m = a;
+    if (b < m) m = b;
+    if (c < m) m = c;
+    if (d < m) m = d;
+    return m;
+}
+__device__ __forceinline__ v3 vmul(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+
+struct QRay {
+    v3 o, d;
+    float maxd;
+};
+
+// Ray::intersect (geometry/ray.h:47-60)
+__device__ __forceinline__ bool ray_aabb(const QRay& r, v3 mn, v3 mx) {
+    const float e = 1e-9f;
+    v3 inv = mk(1.0f / (r.d.x + e), 1.0f / (r.d.y + e), 1.0f / (r.d.z + e));
+    v3 t0 = vmul(mn - r.o, inv), t1 = vmul(mx - r.o, inv);
+    v3 tmin = vmin(t0, t1), tmax = vmax(t0, t1);
+    float t_near = max4(tmin.x, tmin.y, tmin.z, 0.0f);
+    float t_far = min4(tmax.x, tmax.y, tmax.z, r.maxd);
+    return !(t_far < t_near);
+}
+// ray_vs_sphere (geometry/ray.h:62-78)
+__device__ __forceinline__ bool ray_sphere(const QRay& r, v3 c, float rad, float maxd, float& t, v3& n) {
+    v3 oc = r.o - c;
+    float b = dot(oc, r.d);
+    float cc = dot(oc, oc) - rad * rad;
+    float disc = b * b - cc;
+    if (disc < 0.0f) return false;
+    float s = sqrtf(disc);
+    float t0 = -b - s, t1 = -b + s;
+    float hit = t0 >= 0.0f ? t0 : t1;
+    if (hit < 0.0f || hit > maxd) return false;
+    t = hit;
+    n = normalize((r.o + r.d * t) - c);
+    return true;
+}
+// ray_vs_obb (geometry/ray.h:101-125)
+__device__ __forceinline__ bool ray_obb(const QRay& r, v3 c, v3 he, quat q, float& t, v3& n) {
+    quat inv = qinverse_q(q);
+    v3 lo = qrot(inv, r.o - c);
+    v3 ld = qrot(inv, r.d);
+    const float e = 1e-9f;
+    v3 invd = mk(1.0f / (ld.x + e), 1.0f / (ld.y + e), 1.0f / (ld.z + e));
+    v3 t0 = vmul(-he - lo, invd), t1 = vmul(he - lo, invd);
+    v3 tmin = vmin(t0, t1), tmax = vmax(t0, t1);
+    float t_near = max4(tmin.x, tmin.y, tmin.z, 0.0f);
+    float t_far = min4(tmax.x, tmax.y, tmax.z, r.maxd);
+    if (t_far < t_near) return false;
+    int axis = 0;
+    if (tmin.y > tmin.x) axis = 1;
+    float ta = axis == 0 ? tmin.x : tmin.y;
+    if (tmin.z > ta) axis = 2;
+    float lda = axis == 0 ? ld.x : (axis == 1 ? ld.y : ld.z);
+    float sgn = lda >= 0.0f ? -1.0f : 1.0f;
+    v3 ln = mk(axis == 0 ? sgn : 0.0f, axis == 1 ? sgn : 0.0f, axis == 2 ? sgn : 0.0f);
+    t = t_near;
+    n = qrot(q, ln);
+    return true;
+}
+// ray_vs_capsule (geometry/ray.h:138-194)
+__device__ __forceinline__ bool ray_capsule(const QRay& r, v3 a, v3 b, float rad, float& t, v3& n) {
+    v3 axis = b - a;
+    float axis_len2 = dot(axis, axis);
+    bool found = false;
+    float best_t = r.maxd;
+    v3 best_n = mk(0.0f, 0.0f, 0.0f);
+    if (axis_len2 > 1e-6f) {
+        v3 axis_n = axis / sqrtf(axis_len2);
+        v3 oc = r.o - a;
+        v3 d_perp = r.d - axis_n * dot(r.d, axis_n);
+        v3 oc_perp = oc - axis_n * dot(oc, axis_n);
+        float qa = dot(d_perp, d_perp);
+        if (qa > 1e-6f) {
+            float qb = dot(d_perp, oc_perp);
+            float qc = dot(oc_perp, oc_perp) - rad * rad;
+            float disc = qb * qb - qa * qc;
+            if (disc >= 0.0f) {
+                float s = sqrtf(disc);
+                float cands[2] = {(-qb - s) / qa, (-qb + s) / qa};
+                for (int k = 0; k < 2; ++k) {
+                    float cand = cands[k];
+                    if (cand < 0.0f || cand >= best_t) continue;
+                    float along = dot((r.o + r.d * cand) - a, axis_n);
+                    if (along < 0.0f || along > sqrtf(axis_len2)) continue;
+                    v3 hp = r.o + r.d * cand;
+                    v3 cl = a + axis_n * along;
+                    best_t = cand;
+                    best_n = normalize(hp - cl);
+                    found = true;
+                    break;
+                }
+            }
+        }
+    }
+    for (int k = 0; k < 2; ++k) {
+        float ct;
+        v3 cn;
+        if (ray_sphere(r, k == 0 ? a : b, rad, best_t, ct, cn)) {
+            best_t = ct;
+            best_n = cn;
+            found = true;
+        }
+    }
+    if (!found) return false;
+    t = best_t;
+    n = best_n;
+    return true;
+}
+
+// One shape's query record: fat AABB, world geometry (g0, g1, g2 by type: sphere (c, r);
+// box (c, -), (he, -), q; capsule (a, r), (b, -)), and (owner, type, flags, layer).
+struct QShape {
+    float4 fmin, fmax, g0, g1, g2;
+    uint4 info;
+};
+
+__device__ __forceinline__ bool shape_ray(const QRay& r, const QShape& s, float& t, v3& n) {
+    if (s.info.y == SH_SPHERE) return ray_sphere(r, ld3(s.g0), s.g0.w, r.maxd, t, n);
+    if (s.info.y == SH_BOX) return ray_obb(r, ld3(s.g0), ld3(s.g1), ldq(s.g2), t, n);
+    if (s.info.y == SH_CAPSULE) return ray_capsule(r, ld3(s.g0), ld3(s.g1), s.g0.w, t, n);
+    return false;
+}
+// QueryFilter::accepts without the predicate (query/queries.h:92-98), for a live proxy
+__device__ __forceinline__ bool accepts(const uint4 info, const QueryFilterDev& f) {
+    if (!(info.z & QF_LIVE)) return false;
+    if (!((f.layer_mask >> (info.w & 31u)) & 1u)) return false;
+    if (!f.include_triggers && (info.z & QF_TRIGGER)) return false;
+    if (f.ignore_body != 0xFFFFFFFFu && info.x == f.ignore_body) return false;
+    return true;
+}
+
+// world_sphere / world_obb / world_capsule (collision/shape.h:160-188) from the current poses
+__global__ void k_query_prep(DevState d, float4* geom, uint4* qinfo) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.ns) return;
+    const uint4 info = d.s_info[i];
+    ShapeGeom s;
+    s.type = int(info.y & 15u);
+    s.caxis = int((info.y >> 8) & 3u);
+    const float4 lc = d.s_lc[i];
+    s.lc = mk(lc.x, lc.y, lc.z);
+    s.radius = lc.w;
+    s.lr = ldq(d.s_lr[i]);
+    const float4 dim = d.s_dim[i];
+    s.he = mk(dim.x, dim.y, dim.z);
+    s.cr = dim.w;
+    s.chh = d.s_mat[i].x;
+    const uint32_t owner = info.x;
+    const bool live = (info.y & SF_ENABLED) && owner < d.nb && (d.b_flags[owner] & BF_ALIVE);
+    float4 g0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g1 = g0, g2 = g0;
+    if (live) {
+        const v3 pos = ld3(d.b_pos[owner]);
+        const quat rot = ldq(d.b_rot[owner]);
+        if (s.type == SH_SPHERE) {
+            Sph o = world_sphere(s, pos, rot);
+            g0 = st3(o.c, o.r);
+        } else if (s.type == SH_BOX) {
+            OBB o = world_obb(s, pos, rot);
+            g0 = st3(o.c, 0.0f);
+            g1 = st3(o.he, 0.0f);
+            g2 = stq(o.q);
+        } else if (s.type == SH_CAPSULE) {
+            Cap o = world_capsule(s, pos, rot);
+            g0 = st3(o.a, o.r);
+            g1 = st3(o.b, 0.0f);
+        }
+    }
+    geom[3 * i] = g0;
+    geom[3 * i + 1] = g1;
+    geom[3 * i + 2] = g2;
+    qinfo[i] = make_uint4(owner, uint32_t(s.type), (live ? QF_LIVE : 0u) | ((info.y & SF_TRIGGER) ? QF_TRIGGER : 0u),
+                          info.z);
+}
+
+__device__ __forceinline__ QShape load_qshape(const DevState& d, const float4* geom, const uint4* qinfo, uint32_t j) {
+    QShape s;
+    s.fmin = d.s_fmin[j];
+    s.fmax = d.s_fmax[j];
+    s.g0 = geom[3 * j];
+    s.g1 = geom[3 * j + 1];
+    s.g2 = geom[3 * j + 2];
+    s.info = qinfo[j];
+    return s;
+}
+
+// rays: 2 float4 per ray (origin, max distance; direction, -); hits: 3 float4 per ray (point,
+// distance; normal, hit flag bits; body, shape, -, - as bits). any != 0: stop at the first hit.
+__global__ void __launch_bounds__(kQueryTile) k_raycast(DevState d, const float4* geom, const uint4* qinfo,
+                                                        const float4* rays, uint32_t n, QueryFilterDev f, uint32_t any,
+                                                        float4* hits) {
+    __shared__ QShape tile[kQueryTile];
+    const uint32_t i = blockIdx.x * kQueryTile + threadIdx.x;
+    QRay r;
+    r.o = mk(0.0f, 0.0f, 0.0f);
+    r.d = r.o;
+    r.maxd = 0.0f;
+    bool active = i < n;
+    if (active) {
+        const float4 a = rays[2 * i], b = rays[2 * i + 1];
+        r.o = ld3(a);
+        r.maxd = a.w;
+        r.d = ld3(b);
+    }
+    bool found = false;
+    float best_t = 0.0f;
+    v3 best_n = mk(0.0f, 0.0f, 0.0f);
+    uint32_t best_s = 0u;
+    for (uint32_t base = 0; base < d.ns; base += kQueryTile) {
+        __syncthreads();
+        if (base + threadIdx.x < d.ns) tile[threadIdx.x] = load_qshape(d, geom, qinfo, base + threadIdx.x);
+        __syncthreads();
+        const uint32_t m = d.ns - base < kQueryTile ? d.ns - base : kQueryTile;
+        for (uint32_t k = 0; active && k < m; ++k) {
+            const QShape& s = tile[k];
+            if (!accepts(s.info, f)) continue;
+            if (!ray_aabb(r, ld3(s.fmin), ld3(s.fmax))) continue;
+            float t;
+            v3 nn;
+            if (!shape_ray(r, s, t, nn)) continue;
+            if (!found || t < best_t) {
+                found = true;
+                best_t = t;
+                best_n = nn;
+                best_s = base + k;
+                if (any) active = false;
+            }
+        }
+    }
+    if (i >= n) return;
+    if (!found) {
+        hits[3 * i] = hits[3 * i + 1] = hits[3 * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const v3 p = r.o + r.d * best_t;
+    hits[3 * i] = st3(p, best_t);
+    hits[3 * i + 1] = st3(best_n, __uint_as_float(1u));
+    hits[3 * i + 2] = make_float4(__uint_as_float(qinfo[best_s].x), __uint_as_float(best_s), 0.0f, 0.0f);
+}
+
+// every hit of one ray, one thread per shape, in arbitrary order (the host sorts them)
+__global__ void k_raycast_all(DevState d, const float4* geom, const uint4* qinfo, const float4* ray,
+                              QueryFilterDev f, float4* hits, uint32_t cap, uint32_t* count) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.ns) return;
+    const QShape s = load_qshape(d, geom, qinfo, j);
+    if (!accepts(s.info, f)) return;
+    QRay r;
+    const float4 a = ray[0], b = ray[1];
+    r.o = ld3(a);
+    r.maxd = a.w;
+    r.d = ld3(b);
+    if (!ray_aabb(r, ld3(s.fmin), ld3(s.fmax))) return;
+    float t;
+    v3 nn;
+    if (!shape_ray(r, s, t, nn)) return;
+    const uint32_t slot = atomicAdd(count, 1u);
+    if (slot >= cap) return;
+    const v3 p = r.o + r.d * t;
+    hits[3 * slot] = st3(p, t);
+    hits[3 * slot + 1] = st3(nn, __uint_as_float(1u));
+    hits[3 * slot + 2] = make_float4(__uint_as_float(s.info.x), __uint_as_float(j), 0.0f, 0.0f);
+}
+
+}  // namespace
+
+void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, hipStream_t st) {
+    if (d.ns) hipLaunchKernelGGL(k_query_prep, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, geom, qinfo);
+}
+void launch_raycast(const DevState& d, const float4* geom, const uint4* qinfo, const float4* rays, uint32_t n,
+                    const QueryFilterDev& f, uint32_t any, float4* hits, hipStream_t st) {
+    if (n)
+        hipLaunchKernelGGL(k_raycast, dim3((n + kQueryTile - 1u) / kQueryTile), dim3(kQueryTile), 0, st, d, geom, qinfo,
+                           rays, n, f, any, hits);
+}
+void launch_raycast_all(const DevState& d, const float4* geom, const uint4* qinfo, const float4* ray,
+                        const QueryFilterDev& f, float4* hits, uint32_t cap, uint32_t* count, hipStream_t st) {
+    if (d.ns)
+        hipLaunchKernelGGL(k_raycast_all, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, geom, qinfo, ray, f, hits, cap,
+                           count);
+}
+
+}  // namespace pxg

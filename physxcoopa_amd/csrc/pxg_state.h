@@ -225,6 +225,17 @@ void launch_init_bodies(const DevState& d, hipStream_t st);
 // manifolds (at most n_pairs) into the snapshot ev_snap[cur], compares it with last step's (at
 // most prev_bound entries) and appends the events to the queue
 size_t events_sort_bytes(uint32_t cap_pairs);
+// scene raycasts (pxg_query.hip): per-shape world geometry (3 float4 per shape) and query info
+// (uint4) from the current poses, then the closest (or any) hit of n rays (2 float4 each; hits 3
+// float4 each), or every hit of one ray (unordered, at most cap, count in *count)
+struct QueryFilterDev {
+    uint32_t layer_mask, include_triggers, ignore_body;
+};
+void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, hipStream_t st);
+void launch_raycast(const DevState& d, const float4* geom, const uint4* qinfo, const float4* rays, uint32_t n,
+                    const QueryFilterDev& f, uint32_t any, float4* hits, hipStream_t st);
+void launch_raycast_all(const DevState& d, const float4* geom, const uint4* qinfo, const float4* ray,
+                        const QueryFilterDev& f, float4* hits, uint32_t cap, uint32_t* count, hipStream_t st);
 hipError_t launch_events(const DevState& d, uint32_t n_pairs, uint32_t prev_bound, int cur, hipStream_t st);
 void launch_apply_predictions(const DevState& d, uint32_t n_bound, hipStream_t st);
 void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st);

@@ -77,6 +77,13 @@ struct pxg_world {
     int ev_cur = 0;
     uint32_t ev_prev_n = 0;
     uint32_t ev_cap_req = 0;
+    // scene queries: per-shape world geometry and info, rays and hits (grown on demand)
+    float4* q_geom = nullptr;
+    uint4* q_info = nullptr;
+    float4* q_rays = nullptr;
+    float4* q_hits = nullptr;
+    uint32_t* q_count = nullptr;
+    uint32_t q_cap = 0;
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -909,6 +916,103 @@ extern "C" int pxg_get_events(pxg_world* w, pxg_contact_event* out, uint32_t cap
         return fail(PXG_ERR_CAPACITY, "pxg_get_events: " + std::to_string(hv[1]) + " events were dropped: the queue holds " +
                                           std::to_string(w->d.cap_events) + " (clear it with pxg_clear_events, or raise "
                                           "the capacity with pxg_set_events)");
+    return PXG_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// scene raycasts (world.h:766-847)
+// ---------------------------------------------------------------------------------------------
+static QueryFilterDev query_filter(const pxg_query_filter* f) {
+    QueryFilterDev q;
+    q.layer_mask = f ? f->layer_mask : ~0u;
+    q.include_triggers = f ? (f->include_triggers != 0 ? 1u : 0u) : 1u;
+    q.ignore_body = f ? f->ignore_body : PXG_NO_BODY;
+    return q;
+}
+// query buffers for n rays (n hits), and the per-shape records filled from the current poses
+static int query_setup(pxg_world* w, uint32_t n) {
+    if (!w->shapes_ready) return fail(PXG_ERR_STATE, "scene query: upload bodies and shapes first");
+    if (!w->q_geom) {
+        DALLOC(w->q_geom, 3 * size_t(w->cap_s));
+        DALLOC(w->q_info, w->cap_s);
+        DALLOC(w->q_count, 1);
+    }
+    if (n > w->q_cap) {
+        HIPCHK(hipStreamSynchronize(w->stream));
+        dfree(w, w->q_rays);
+        dfree(w, w->q_hits);
+        w->q_rays = nullptr;
+        w->q_hits = nullptr;
+        w->q_cap = 0;
+        uint32_t cap = std::max<uint32_t>(n, 1024u);
+        DALLOC(w->q_rays, 2 * size_t(cap));
+        DALLOC(w->q_hits, 3 * size_t(cap));
+        w->q_cap = cap;
+    }
+    launch_query_prep(w->d, w->q_geom, w->q_info, w->stream);
+    return PXG_OK;
+}
+static_assert(sizeof(pxg_ray) == 2 * sizeof(float4) && sizeof(pxg_raycast_hit) == 3 * sizeof(float4),
+              "query record layouts");
+
+static int raycast_batch(pxg_world* w, const pxg_ray* rays, uint32_t n, const pxg_query_filter* filter, uint32_t any,
+                         const char* what) {
+    int r = query_setup(w, n);
+    if (r != PXG_OK) return r;
+    HIPCHK(hipMemcpyAsync(w->q_rays, rays, size_t(n) * sizeof(pxg_ray), hipMemcpyHostToDevice, w->stream));
+    launch_raycast(w->d, w->q_geom, w->q_info, w->q_rays, n, query_filter(filter), any, w->q_hits, w->stream);
+    (void)what;
+    return PXG_OK;
+}
+
+extern "C" int pxg_raycast(pxg_world* w, const pxg_ray* rays, uint32_t n, const pxg_query_filter* filter,
+                           pxg_raycast_hit* hits) {
+    if (!w || (n && (!rays || !hits))) return fail(PXG_ERR_INVALID, "pxg_raycast: null argument");
+    if (!n) return PXG_OK;
+    int r = raycast_batch(w, rays, n, filter, 0u, "pxg_raycast");
+    if (r != PXG_OK) return r;
+    HIPCHK(hipMemcpyAsync(hits, w->q_hits, size_t(n) * sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_raycast_any(pxg_world* w, const pxg_ray* rays, uint32_t n, const pxg_query_filter* filter,
+                               uint32_t* any) {
+    if (!w || (n && (!rays || !any))) return fail(PXG_ERR_INVALID, "pxg_raycast_any: null argument");
+    if (!n) return PXG_OK;
+    int r = raycast_batch(w, rays, n, filter, 1u, "pxg_raycast_any");
+    if (r != PXG_OK) return r;
+    std::vector<pxg_raycast_hit> hits(n);
+    HIPCHK(hipMemcpyAsync(hits.data(), w->q_hits, size_t(n) * sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (uint32_t i = 0; i < n; ++i) any[i] = hits[i].hit;
+    return PXG_OK;
+}
+
+extern "C" int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query_filter* filter, pxg_raycast_hit* hits,
+                               uint32_t cap, uint32_t* n) {
+    if (!w || !ray || !n || (cap && !hits)) return fail(PXG_ERR_INVALID, "pxg_raycast_all: null argument");
+    *n = 0;
+    // every shape can be hit once: the hit buffer holds one record per shape
+    int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u));
+    if (r != PXG_OK) return r;
+    HIPCHK(hipMemcpyAsync(w->q_rays, ray, sizeof(pxg_ray), hipMemcpyHostToDevice, w->stream));
+    launch_fill_u32(w->q_count, 0u, 1, w->stream);
+    launch_raycast_all(w->d, w->q_geom, w->q_info, w->q_rays, query_filter(filter), w->q_hits, w->q_cap, w->q_count,
+                       w->stream);
+    uint32_t count = 0;
+    HIPCHK(hipMemcpyAsync(&count, w->q_count, sizeof(count), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    count = std::min(count, w->q_cap);
+    std::vector<pxg_raycast_hit> all(count);
+    if (count)
+        HIPCHK(hipMemcpy(all.data(), w->q_hits, size_t(count) * sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost));
+    // nearest first (world.h:840-842); equal distances by shape slot
+    std::sort(all.begin(), all.end(), [](const pxg_raycast_hit& a, const pxg_raycast_hit& b) {
+        return a.distance < b.distance || (a.distance == b.distance && a.shape_index < b.shape_index);
+    });
+    *n = count;
+    for (uint32_t i = 0; i < count && i < cap; ++i) hits[i] = all[i];
     return PXG_OK;
 }
 

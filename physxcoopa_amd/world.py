@@ -133,6 +133,32 @@ class World:
     def clear_events(self):
         check(lib().pxg_clear_events(self.h), "pxg_clear_events")
 
+    def raycast(self, rays, query_filter=None):
+        """Closest hit of each ray (RAY_DTYPE in, HIT_DTYPE out; world.h:766-791)."""
+        r = np.ascontiguousarray(rays, dtype=_abi.RAY_DTYPE).reshape(-1)
+        out = np.zeros(max(len(r), 1), _abi.HIT_DTYPE)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        check(lib().pxg_raycast(self.h, ptr(r), len(r), f, ptr(out)), "pxg_raycast")
+        return out[:len(r)]
+
+    def raycast_any(self, rays, query_filter=None):
+        """Whether each ray hits anything (world.h:803-817)."""
+        r = np.ascontiguousarray(rays, dtype=_abi.RAY_DTYPE).reshape(-1)
+        out = np.zeros(max(len(r), 1), np.uint32)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        check(lib().pxg_raycast_any(self.h, ptr(r), len(r), f, ptr(out)), "pxg_raycast_any")
+        return out[:len(r)].astype(bool)
+
+    def raycast_all(self, ray, query_filter=None):
+        """Every hit of one ray, nearest first (world.h:825-847)."""
+        r = np.ascontiguousarray(ray, dtype=_abi.RAY_DTYPE).reshape(1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        n = ctypes.c_uint32()
+        check(lib().pxg_raycast_all(self.h, ptr(r), f, None, 0, ctypes.byref(n)), "pxg_raycast_all")
+        out = np.zeros(max(n.value, 1), _abi.HIT_DTYPE)
+        check(lib().pxg_raycast_all(self.h, ptr(r), f, ptr(out), n.value, ctypes.byref(n)), "pxg_raycast_all")
+        return out[:n.value]
+
     def set_events(self, enabled, capacity=0):
         check(lib().pxg_set_events(self.h, 1 if enabled else 0, capacity), "pxg_set_events")
 

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from physxcoopa_amd import scenes, _abi
-from parity import lockstep
+from parity import lockstep, compare_step
 
 pytestmark = pytest.mark.gpu
 
@@ -123,6 +123,49 @@ def test_parity_residual_mode(sides):
     print(rep, gw.stats(), ow.stats(), flush=True)
     assert gw.stats()["velocity_passes"] == ow.stats()["velocity_passes"]
     assert abs(gw.stats()["residual"] - ow.stats()["residual"]) <= 1e-6 + 1e-4 * abs(ow.stats()["residual"])
+
+
+@pytest.mark.parametrize("n,lattice,warm,steps", [(2048, (32, 32, 8), 90, 6), (16384, (64, 64, 40), 240, 3)],
+                         ids=["pile_2048", "pile_16384"])
+def test_parity_residual_mode_coloured_piles(sides, n, lattice, warm, steps):
+    """Residual mode on piles large enough for a multi-block solve with a tail of short colours
+    (the coloured order): the residual of each manifold is computed during the colour phase after
+    the last colour writing its bodies, and the GPU must run the oracle's pass count and reach its
+    state bit for bit. The GPU state after `warm` fixed-iteration steps is handed to the oracle;
+    the oracle then solves in the GPU's order."""
+    World, OracleWorld = sides
+    sc = scenes.config2_scene(seed=2, n=n, lattice=lattice)
+    c = _abi.default_config(**sc.config)
+    c.solve_order = _abi.PXG_ORDER_COLOURED
+    gw = World(c, sc.bodies, sc.shapes)
+    for _ in range(warm):
+        gw.step_fixed()
+    rc = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=256, **sc.config)
+    rc.solve_order = _abi.PXG_ORDER_COLOURED
+    gw.set_config(rc)
+    ow = OracleWorld(rc, gw.bodies(), sc.shapes)
+    report = dict(steps=0, manifolds=0, bitexact_manifolds=0, bitexact_bodies=0, bitexact_joints=0, max_body_diff=0.0,
+                  colours=0)
+    for i in range(steps):
+        ow.set_bodies(gw.bodies())
+        ow.set_fat_aabbs(gw.fat_aabbs())
+        ow.set_warm_cache(*gw.warm_cache())
+        pre = gw.bodies()
+        gw.clear_events()
+        ow.clear_events()
+        gw.step_fixed()
+        gm = gw.manifolds()
+        order, _ = gw.solve_order()
+        ow.set_order(np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1))
+        ow.step_fixed()
+        # the oracle's first step has no previous pair set to compare events against
+        compare_step(gw, ow, pre, report, check_events=i > 0)
+        gs, os_ = gw.stats(), ow.stats()
+        print("step %d: passes %d / %d, residual %.3g / %.3g, colours %d" %
+              (i, gs["velocity_passes"], os_["velocity_passes"], gs["residual"], os_["residual"], gs["colours"]), flush=True)
+        assert gs["velocity_passes"] == os_["velocity_passes"]
+        assert gs["residual"] == os_["residual"]
+    assert report["bitexact_bodies"] == steps
 
 
 def test_full_pile_free_running_equals_oracle(sides):

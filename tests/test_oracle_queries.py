@@ -1,0 +1,235 @@
+"""Scene raycasts in the CPU oracle: the reference's own query tests
+(physxcoopa/tests/queries_test.cpp) with their known answers, the ray-vs-shape functions against
+an independent method (the first root of the shape's signed distance along the ray, found by
+sampling and bisection in float64), the inside-start rules of geometry/ray.h, the slab test the
+trees cull with, and the QueryFilter fields."""
+import numpy as np
+import pytest
+
+from physxcoopa_amd import scenes, _abi
+from physxcoopa_amd._abi import PXG_STATIC, PXG_SPHERE, PXG_BOX, PXG_CAPSULE
+
+
+def static_box_world(oracle, centres, half=0.5, **shape_kw):
+    sb = scenes.SceneBuilder()
+    ids = []
+    for c in centres:
+        i = sb.add_body(PXG_STATIC, c)
+        sb.add_shape(i, PXG_BOX, half_extents=(half, half, half), **shape_kw)
+        ids.append(i)
+    b, s = sb.arrays()
+    return oracle.OracleWorld(_abi.default_config(), b, s), ids
+
+
+def x_ray(maxd=100.0, direction=(1.0, 0.0, 0.0)):
+    return _abi.rays([(0.0, 0.0, 0.0)], [direction], maxd)
+
+
+def test_raycast_hits_box_with_exact_distance_point_and_normal(oracle):
+    # queries_test.cpp:22-37
+    w, (box,) = static_box_world(oracle, [(5.0, 0.0, 0.0)])
+    h = w.raycast(x_ray())[0]
+    assert h["hit"] == 1 and h["body"] == box
+    assert abs(h["distance"] - 4.5) < 1e-3
+    np.testing.assert_allclose(h["point"], [4.5, 0, 0], atol=1e-3)
+    np.testing.assert_allclose(h["normal"], [-1, 0, 0], atol=1e-3)
+
+
+def test_raycast_all_sorts_hits_and_any_agrees(oracle):
+    # queries_test.cpp:73-97
+    w, _ = static_box_world(oracle, [(9.0, 0, 0), (3.0, 0, 0), (6.0, 0, 0)])
+    hits = w.raycast_all(x_ray()[0])
+    assert len(hits) == 3
+    np.testing.assert_allclose(hits["distance"], [2.5, 5.5, 8.5], atol=1e-3)
+    assert w.raycast(x_ray())[0]["hit"] == 1
+    assert w.raycast(x_ray(direction=(-1.0, 0, 0)))[0]["hit"] == 0  # nothing behind the origin
+
+
+def test_query_filter_ignore_layer_and_triggers(oracle):
+    # queries_test.cpp:270-293: `ignore` skips the near body; a layer mask and include_triggers
+    # hide shapes the same way
+    sb = scenes.SceneBuilder()
+    near = sb.add_body(PXG_STATIC, (3.0, 0, 0))
+    sb.add_shape(near, PXG_BOX, half_extents=(0.5, 0.5, 0.5), layer=3, is_trigger=True)
+    far = sb.add_body(PXG_STATIC, (6.0, 0, 0))
+    sb.add_shape(far, PXG_BOX, half_extents=(0.5, 0.5, 0.5))
+    b, s = sb.arrays()
+    w = oracle.OracleWorld(_abi.default_config(), b, s)
+    r = x_ray()
+    assert w.raycast(r)[0]["body"] == near
+    for f in (_abi.query_filter(ignore_body=near), _abi.query_filter(layer_mask=~np.uint32(1 << 3) & 0xFFFFFFFF),
+              _abi.query_filter(include_triggers=False)):
+        h = w.raycast(r, f)[0]
+        assert h["body"] == far and abs(h["distance"] - 5.5) < 1e-4
+        assert len(w.raycast_all(r[0], f)) == 1
+    assert len(w.raycast_all(r[0])) == 2
+
+
+def test_inside_start_rules(oracle):
+    # geometry/ray.h: from inside a sphere the hit is where the ray leaves it (t1); from inside a
+    # box the hit is at t = 0; a ray whose max distance ends short of a shape misses it
+    sph = np.zeros((), _abi.SHAPE_DTYPE)
+    sph["type"], sph["radius"], sph["enabled"] = PXG_SPHERE, 2.0, 1
+    sph["local_rotation"] = (0, 0, 0, 1)
+    r = _abi.rays([(0.5, 0, 0)], [(1.0, 0, 0)], 100.0)[0]
+    hit, t, n = oracle.ray_shape(sph, (0, 0, 0), (0, 0, 0, 1), r)
+    assert hit and abs(t - 1.5) < 1e-6 and np.allclose(n, [1, 0, 0])
+    box = np.zeros((), _abi.SHAPE_DTYPE)
+    box["type"], box["half_extents"], box["enabled"] = PXG_BOX, (1, 1, 1), 1
+    box["local_rotation"] = (0, 0, 0, 1)
+    hit, t, n = oracle.ray_shape(box, (0, 0, 0), (0, 0, 0, 1), r)
+    assert hit and t == 0.0
+    short = _abi.rays([(-5.0, 0, 0)], [(1.0, 0, 0)], 2.9)[0]
+    assert not oracle.ray_shape(sph, (0, 0, 0), (0, 0, 0, 1), short)[0]
+    assert oracle.ray_shape(sph, (0, 0, 0), (0, 0, 0, 1), _abi.rays([(-5.0, 0, 0)], [(1.0, 0, 0)], 3.0)[0])[0]
+
+
+# --- independent method: first root of the signed distance along the ray, float64 ---
+
+def quat_matrix(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                     [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                     [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+
+
+def sdf(kind, p, geo):
+    if kind == PXG_SPHERE:
+        c, r = geo
+        return np.linalg.norm(p - c, axis=-1) - r
+    if kind == PXG_BOX:
+        c, R, he = geo
+        lp = (p - c) @ R
+        q = np.abs(lp) - he
+        return np.linalg.norm(np.maximum(q, 0.0), axis=-1) + np.minimum(np.max(q, axis=-1), 0.0)
+    a, b, r = geo
+    ab = b - a
+    t = np.clip(((p - a) @ ab) / (ab @ ab), 0.0, 1.0)
+    return np.linalg.norm(p - (a + t[..., None] * ab), axis=-1) - r
+
+
+def first_root(kind, geo, o, d, maxd, n=20001):
+    ts = np.linspace(0.0, maxd, n)
+    f = sdf(kind, o + ts[:, None] * d, geo)
+    if f[0] <= 0:
+        return None  # starts inside: covered by the inside-start test
+    idx = np.nonzero(f <= 0)[0]
+    if len(idx) == 0:
+        return np.inf
+    lo, hi = ts[idx[0] - 1], ts[idx[0]]
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        if sdf(kind, (o + mid * d)[None], geo)[0] > 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def sdf_normal(kind, geo, p, h=1e-6):
+    g = np.array([(sdf(kind, (p + h * e)[None], geo)[0] - sdf(kind, (p - h * e)[None], geo)[0]) / (2 * h)
+                  for e in np.eye(3)])
+    return g / np.linalg.norm(g)
+
+
+@pytest.mark.parametrize("kind", [PXG_SPHERE, PXG_BOX, PXG_CAPSULE])
+def test_ray_vs_shape_equals_first_root_of_signed_distance(oracle, kind):
+    rng = np.random.default_rng(100 + kind)
+    checked = hits = 0
+    while checked < 120:
+        s = np.zeros((), _abi.SHAPE_DTYPE)
+        s["type"], s["enabled"], s["capsule_axis"] = kind, 1, int(rng.integers(0, 3))
+        s["local_rotation"] = (0, 0, 0, 1)
+        s["radius"] = rng.uniform(0.2, 1.0)
+        s["half_extents"] = rng.uniform(0.2, 1.0, 3)
+        s["capsule_radius"], s["capsule_half_height"] = rng.uniform(0.1, 0.5), rng.uniform(0.1, 0.8)
+        pos = rng.uniform(-1, 1, 3)
+        q = scenes.random_quaternion(rng)
+        # aim near the shape so that many of the rays hit
+        o = pos + rng.uniform(-4, 4, 3)
+        tgt = pos + rng.uniform(-1.2, 1.2, 3) * (0.4 if kind == PXG_CAPSULE else 1.0)
+        d = (tgt - o) / np.linalg.norm(tgt - o)
+        o32, d32 = o.astype(np.float32), d.astype(np.float32)
+        o, d = o32.astype(np.float64), d32.astype(np.float64)
+        R = quat_matrix(np.asarray(q, np.float64))
+        if kind == PXG_SPHERE:
+            geo = (pos, float(s["radius"]))
+        elif kind == PXG_BOX:
+            geo = (pos, R, s["half_extents"].astype(np.float64))
+        else:
+            ax = R[:, int(s["capsule_axis"])]
+            hh = float(s["capsule_half_height"])
+            geo = (pos - ax * hh, pos + ax * hh, float(s["capsule_radius"]))
+        maxd = 12.0
+        ref = first_root(kind, geo, o, d, maxd)
+        if ref is None:
+            continue
+        ray = _abi.rays([o32], [d32], maxd)[0]
+        hit, t, n = oracle.ray_shape(s, pos.astype(np.float32), np.asarray(q, np.float32), ray)
+        checked += 1
+        if not np.isfinite(ref):
+            # grazing rays within a hair of the surface are left out
+            assert not hit or sdf(kind, (o + t * d)[None], geo)[0] < 1e-4
+            continue
+        assert hit, (kind, ref)
+        assert abs(t - ref) <= 2e-4 * max(1.0, ref), (kind, t, ref)
+        p = o + t * d
+        n_ref = sdf_normal(kind, geo, p)
+        edge = kind == PXG_BOX and np.sort(np.abs(((p - geo[0]) @ geo[1]) / geo[2]))[-2] > 0.999
+        if not edge:
+            assert np.allclose(n, n_ref, atol=2e-3), (kind, n, n_ref)
+        hits += 1
+    assert hits > 30
+
+
+def test_slab_test_agrees_with_interval_overlap(oracle):
+    """Ray::intersect (geometry/ray.h:47-60): the ray segment [0, max] meets the box iff the
+    per-axis entry/exit intervals overlap (float64 reference, boxes clearly hit or missed)."""
+    rng = np.random.default_rng(7)
+    agree = 0
+    for _ in range(400):
+        mn = rng.uniform(-2, 1, 3)
+        mx = mn + rng.uniform(0.1, 2, 3)
+        o = rng.uniform(-4, 4, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        maxd = rng.uniform(0.5, 8)
+        o32, d32 = o.astype(np.float32), d.astype(np.float32)
+        o, d = o32.astype(np.float64), d32.astype(np.float64)
+        with np.errstate(divide="ignore"):
+            t0, t1 = (mn - o) / d, (mx - o) / d
+        near = max(np.max(np.minimum(t0, t1)), 0.0)
+        far = min(np.min(np.maximum(t0, t1)), maxd)
+        if abs(far - near) < 1e-4:
+            continue
+        got = oracle.ray_aabb(_abi.rays([o32], [d32], maxd)[0], mn.astype(np.float32), mx.astype(np.float32))
+        assert got == (near <= far)
+        agree += 1
+    assert agree > 300
+
+
+def test_world_raycast_is_the_nearest_of_all_hits(oracle):
+    """On a settled pile: raycast() is raycast_all()'s first hit, every hit lies on the shape
+    surface it names, and raycast_all() is sorted."""
+    b, s = scenes.mixed_pile_scene(n=150, seed=3)
+    w = oracle.OracleWorld(_abi.default_config(), b, s)
+    for _ in range(90):
+        w.step_fixed()
+    rng = np.random.default_rng(5)
+    o = np.stack([rng.uniform(-4, 4, 64), rng.uniform(-4, 4, 64), np.full(64, 12.0)], axis=1)
+    d = np.tile([0.0, 0.0, -1.0], (64, 1)) + rng.normal(scale=0.2, size=(64, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = _abi.rays(o, d, 40.0)
+    closest = w.raycast(r)
+    n_hit = 0
+    for i in range(64):
+        allh = w.raycast_all(r[i])
+        assert np.all(np.diff(allh["distance"]) >= 0)
+        if len(allh) == 0:
+            assert closest[i]["hit"] == 0
+            continue
+        n_hit += 1
+        assert closest[i].tobytes() == allh[0].tobytes()
+        np.testing.assert_allclose(allh["point"], r[i]["origin"] + allh["distance"][:, None] * r[i]["direction"],
+                                   atol=1e-5)
+    assert n_hit > 40
