@@ -203,6 +203,66 @@ def bench_learned(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_queries(args):
+    """Scene raycasts on the 16,384-body pile after the warm-up steps: K batches of 65,536 rays
+    (from 45 m above the pile, downward with a normal jitter of 0.3, max distance 200 m), closest
+    hit per ray. The timed batches run on rays resident in device memory (pxg_raycast_device: the
+    per-shape world geometry pass and the raycast kernel); the host-buffer rate (pxg_raycast, PCIe
+    copies and one wait included) is reported beside it. No CPU baseline: the oracle tests every
+    shape per ray (the result the reference's tree walk gives), which would understate the
+    reference's own tree-culled CPU cost."""
+    import numpy as np
+    import torch
+    from physxcoopa_amd import scenes, _abi
+    from physxcoopa_amd.world import World
+    from physxcoopa_amd._lib import lib, check
+    sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
+    cfg = _abi.default_config(**sc.config)
+    w = World(cfg, sc.bodies, sc.shapes)
+    for _ in range(args.warmup):
+        w.step_fixed()
+    w.synchronize()
+    n = 65536
+    rng = np.random.default_rng(9)
+    o = np.stack([rng.uniform(-34, 34, n), rng.uniform(-34, 34, n), np.full(n, 45.0)], axis=1)
+    dvec = np.tile([0.0, 0.0, -1.0], (n, 1)) + rng.normal(scale=0.3, size=(n, 3))
+    dvec /= np.linalg.norm(dvec, axis=1, keepdims=True)
+    rays = _abi.rays(o, dvec, 200.0)
+    d_rays = torch.from_numpy(rays.view(np.uint8).copy()).to("cuda")
+    d_hits = torch.empty(n * _abi.HIT_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    h = w.h
+    check(lib().pxg_raycast_device(h, d_rays.data_ptr(), n, None, d_hits.data_ptr()), "pxg_raycast_device")
+    w.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        check(lib().pxg_raycast_device(h, d_rays.data_ptr(), n, None, d_hits.data_ptr()), "pxg_raycast_device")
+    w.synchronize()
+    el = time.perf_counter() - t0
+    dev_hits = d_hits.cpu().numpy().view(_abi.HIT_DTYPE)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        host_hits = w.raycast(rays)
+    el_host = time.perf_counter() - t1
+    assert dev_hits.tobytes() == host_hits.tobytes(), "device and host entry points differ"
+    pairs = float(n) * len(sc.shapes)
+    flop = 26.0 * pairs  # slab test against the fat AABB per (ray, shape) pair
+    out = {
+        "metric": "rays/s (closest hit, scene raycasts on the settled 16k-body pile)",
+        "value": round(n * args.steps / el, 1), "unit": "rays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1000.0 * el / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "raycasts_config2_pile_16384", "rays_per_batch": n, "shapes": int(len(sc.shapes)),
+                   "hit_fraction": round(float(dev_hits["hit"].mean()), 4)},
+        "host_buffers": {"value": round(n * args.steps / el_host, 1), "ms_per_batch": round(1000.0 * el_host / args.steps, 4)},
+        "roofline": {"bound": "fp32", "achieved": round(flop * args.steps / el / 1e12, 3), "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": round(flop * args.steps / el / 1e12 / 157.3, 5), "traffic": None,
+                     "kernel": "k_raycast (+ k_query_prep)", "flop_per_batch": flop},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def pile_world(sc, order, net):
     from physxcoopa_amd import _abi
     from physxcoopa_amd.world import World
@@ -373,10 +433,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["plain", "batched_learned"], default=None,
                     help="pile: run one of the two configurations only (for profiling)")
-    ap.add_argument("--workload", choices=["pile", "batch", "ragdolls", "learned"], default="pile",
+    ap.add_argument("--workload", choices=["pile", "batch", "ragdolls", "learned", "queries"], default="pile",
                     help="pile: the 16k-body pile (one world); batch: 4096 independent scenes of 8-256 bodies; "
                          "ragdolls: 2048 scenes of a jointed ragdoll over 32 boxes; learned: config-1 piles, "
-                         "passes and time to the residual with and without the learned warm start")
+                         "passes and time to the residual with and without the learned warm start; queries: "
+                         "batches of 65,536 raycasts against the settled pile")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device-to-device copy of 256 MiB ten times (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -392,6 +453,8 @@ def main():
         bench_batch(args)
     elif args.workload == "learned":
         bench_learned(args)
+    elif args.workload == "queries":
+        bench_queries(args)
     else:
         bench_pile(args)
 

@@ -341,6 +341,11 @@ int pxg_set_events(pxg_world* w, uint32_t enabled, uint32_t capacity);
  * the world's stream after its pending steps. */
 int pxg_raycast(pxg_world* w, const pxg_ray* rays, uint32_t n, const pxg_query_filter* filter, pxg_raycast_hit* hits);
 int pxg_raycast_any(pxg_world* w, const pxg_ray* rays, uint32_t n, const pxg_query_filter* filter, uint32_t* any);
+/* pxg_raycast for rays already in device memory (written before the call) and hits written to
+ * device memory, enqueued on the world's stream without waiting: pxg_synchronize orders the
+ * caller after it. For callers whose rays live on the GPU; no counterpart in the reference. */
+int pxg_raycast_device(pxg_world* w, const pxg_ray* d_rays, uint32_t n, const pxg_query_filter* filter,
+                       pxg_raycast_hit* d_hits);
 int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query_filter* filter, pxg_raycast_hit* hits,
                     uint32_t cap, uint32_t* n);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */

@@ -113,6 +113,26 @@ struct Vec3 {
     float x = 0.0f, y = 0.0f, z = 0.0f;
 };
 
+// geometry::Ray (geometry/ray.h:32-35), query::RaycastHit and query::QueryFilter without its
+// predicate (query/queries.h:56-98)
+struct Ray {
+    Vec3 origin;
+    Vec3 direction{0.0f, 0.0f, -1.0f};  // unit length
+    float max_distance = 3.402823466e+38f;
+};
+struct RaycastHit {
+    Vec3 point;
+    Vec3 normal{0.0f, 0.0f, 1.0f};
+    float distance = 0.0f;
+    BodyId body;
+    uint32_t shape_index = 0xFFFFFFFFu;
+};
+struct QueryFilter {
+    uint32_t layer_mask = ~0u;
+    bool include_triggers = true;
+    BodyId ignore;
+};
+
 // One pair's enter / stay / exit transition of one step (collision/contact_event.h:29-36).
 struct ContactEvent {
     enum class Type { Enter, Stay, Exit };
@@ -456,6 +476,63 @@ public:
         }
         return events_;
     }
+    // --- scene queries (world.h:766-847), against the bodies' current poses ---
+    // Closest hit along the ray (equal distances: the lower shape slot).
+    bool raycast(const Ray& ray, RaycastHit& hit, const QueryFilter& filter = QueryFilter()) {
+        std::vector<Ray> one(1, ray);
+        std::vector<RaycastHit> hits;
+        std::vector<uint8_t> found;
+        raycast_batch(one, hits, found, filter);
+        if (!found[0]) return false;
+        hit = hits[0];
+        return true;
+    }
+    bool raycast(const Ray& ray, RaycastHit& hit, uint32_t layer_mask, bool include_triggers = true) {
+        QueryFilter f;
+        f.layer_mask = layer_mask;
+        f.include_triggers = include_triggers;
+        return raycast(ray, hit, f);
+    }
+    bool raycast_any(const Ray& ray, const QueryFilter& filter = QueryFilter()) {
+        push();
+        pxg_ray r = to_ray(ray);
+        pxg_query_filter f = to_filter(filter);
+        uint32_t any = 0;
+        check(pxg_raycast_any(w_, &r, 1, &f, &any), "raycast_any");
+        return any != 0;
+    }
+    // Every hit along the ray, nearest first.
+    std::vector<RaycastHit> raycast_all(const Ray& ray, const QueryFilter& filter = QueryFilter()) {
+        push();
+        pxg_ray r = to_ray(ray);
+        pxg_query_filter f = to_filter(filter);
+        uint32_t n = 0;
+        check(pxg_raycast_all(w_, &r, &f, nullptr, 0, &n), "raycast_all");
+        std::vector<pxg_raycast_hit> raw(n);
+        if (n) check(pxg_raycast_all(w_, &r, &f, raw.data(), n, &n), "raycast_all");
+        std::vector<RaycastHit> out;
+        for (uint32_t i = 0; i < n && i < raw.size(); ++i) out.push_back(from_hit(raw[i]));
+        return out;
+    }
+    // The closest hits of many rays in one GPU pass (no counterpart in the reference, which casts
+    // one ray per call): hits[i] is valid where found[i] != 0.
+    void raycast_batch(const std::vector<Ray>& rays, std::vector<RaycastHit>& hits, std::vector<uint8_t>& found,
+                       const QueryFilter& filter = QueryFilter()) {
+        push();
+        std::vector<pxg_ray> r(rays.size());
+        for (size_t i = 0; i < rays.size(); ++i) r[i] = to_ray(rays[i]);
+        std::vector<pxg_raycast_hit> raw(rays.size());
+        pxg_query_filter f = to_filter(filter);
+        check(pxg_raycast(w_, r.data(), static_cast<uint32_t>(r.size()), &f, raw.data()), "raycast");
+        hits.assign(rays.size(), RaycastHit());
+        found.assign(rays.size(), 0);
+        for (size_t i = 0; i < raw.size(); ++i) {
+            if (!raw[i].hit) continue;
+            hits[i] = from_hit(raw[i]);
+            found[i] = 1;
+        }
+    }
+
     void clear_events() {  // world.h:748
         check(pxg_clear_events(w_), "clear_events");
         events_.clear();
@@ -580,6 +657,32 @@ private:
     bool alive_on_host(uint32_t i) const {
         // aliveness changes only on the host (add/remove), so the mirror's flag is current
         return bodies_[i].alive != 0;
+    }
+
+    static pxg_ray to_ray(const Ray& r) {
+        pxg_ray o;
+        std::memset(&o, 0, sizeof(o));
+        set3(o.origin, r.origin);
+        set3(o.direction, r.direction);
+        o.max_distance = r.max_distance;
+        return o;
+    }
+    pxg_query_filter to_filter(const QueryFilter& f) const {
+        pxg_query_filter o;
+        std::memset(&o, 0, sizeof(o));
+        o.layer_mask = f.layer_mask;
+        o.include_triggers = f.include_triggers ? 1u : 0u;
+        o.ignore_body = is_valid(f.ignore) ? f.ignore.index : PXG_NO_BODY;
+        return o;
+    }
+    RaycastHit from_hit(const pxg_raycast_hit& h) const {
+        RaycastHit o;
+        o.point = Vec3{h.point[0], h.point[1], h.point[2]};
+        o.normal = Vec3{h.normal[0], h.normal[1], h.normal[2]};
+        o.distance = h.distance;
+        o.body = BodyId{h.body, h.body < generations_.size() ? generations_[h.body] : 0u};
+        o.shape_index = h.shape_index;
+        return o;
     }
 
     // device -> host mirror

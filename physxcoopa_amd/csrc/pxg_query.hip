@@ -3,15 +3,19 @@
 // The reference walks its two AABB trees with the ray's slab test against the proxies' fat AABBs
 // and tests every leaf it reaches exactly. A leaf's box lies inside its ancestors' and the slab
 // test is monotonic in the box, so the set of shapes tested is exactly the live shapes whose fat
-// AABB the slab test accepts; the GPU tests that set directly. One thread owns one ray; a block's
-// 256 rays stream the shape table through LDS in tiles of 256 shapes, so each shape record is read
-// from HBM once per block. A prep pass turns every shape into its world geometry (sphere centre
-// and radius, box centre, half extents and orientation, capsule end points and radius) from the
-// bodies' current poses once per query batch. The closest hit keeps the nearest distance; shapes
-// are visited in slot order and only a strictly nearer hit replaces the best, so equal distances
-// go to the lower slot.
+// AABB the slab test accepts; the GPU tests that set directly, culled by a two-level layout built
+// per query batch: a prep pass turns every shape into its world geometry (sphere centre and
+// radius, box centre, half extents and orientation, capsule end points and radius) from the
+// bodies' current poses; shapes and rays are sorted by the Morton code of their fat-box centre /
+// origin; consecutive 256-shape tiles of the sorted shapes get their bounding box. One thread owns
+// one ray, a block's 256 rays are neighbours in Morton order, and a block loads a tile into LDS
+// only when the slab test accepts the tile's box for one of its rays (a shape's fat box lies in
+// its tile's box, so nothing the reference tests is skipped). The closest hit keeps the nearest
+// distance, equal distances going to the lower shape slot.
 #include "pxg_state.h"
 #include "pxg_device.h"
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 using namespace pxd;
 
@@ -44,6 +48,33 @@ __device__ __forceinline__ float min4(float a, float b, float c, float d) {
     return m;
 }
 __device__ __forceinline__ v3 vmul(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+
+// order-preserving float <-> uint encoding (for atomic min / max of floats)
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u); }
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+// Morton code of p in the box [lo, hi] (10 bits per axis, clamped)
+__device__ __forceinline__ uint32_t morton(v3 p, v3 lo, v3 hi) {
+    uint32_t q[3];
+    const float pc[3] = {p.x, p.y, p.z}, lc[3] = {lo.x, lo.y, lo.z}, hc[3] = {hi.x, hi.y, hi.z};
+    for (int k = 0; k < 3; ++k) {
+        const float ext = hc[k] - lc[k];
+        float u = ext > 0.0f ? (pc[k] - lc[k]) / ext : 0.0f;
+        u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+        q[k] = uint32_t(u * 1023.0f);
+    }
+    return spread10(q[0]) | (spread10(q[1]) << 1) | (spread10(q[2]) << 2);
+}
 
 struct QRay {
     v3 o, d;
@@ -171,7 +202,7 @@ __device__ __forceinline__ bool accepts(const uint4 info, const QueryFilterDev& 
 }
 
 // world_sphere / world_obb / world_capsule (collision/shape.h:160-188) from the current poses
-__global__ void k_query_prep(DevState d, float4* geom, uint4* qinfo) {
+__global__ void k_query_prep(DevState d, float4* geom, uint4* qinfo, uint32_t* bounds6) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= d.ns) return;
     const uint4 info = d.s_info[i];
@@ -211,6 +242,83 @@ __global__ void k_query_prep(DevState d, float4* geom, uint4* qinfo) {
     geom[3 * i + 2] = g2;
     qinfo[i] = make_uint4(owner, uint32_t(s.type), (live ? QF_LIVE : 0u) | ((info.y & SF_TRIGGER) ? QF_TRIGGER : 0u),
                           info.z);
+    if (live) {
+        const v3 c = (ld3(d.s_fmin[i]) + ld3(d.s_fmax[i])) * 0.5f;
+        atomicMin(&bounds6[0], f2o(c.x)); atomicMin(&bounds6[1], f2o(c.y)); atomicMin(&bounds6[2], f2o(c.z));
+        atomicMax(&bounds6[3], f2o(c.x)); atomicMax(&bounds6[4], f2o(c.y)); atomicMax(&bounds6[5], f2o(c.z));
+    }
+}
+__device__ __forceinline__ void load_bounds(const uint32_t* b6, v3& lo, v3& hi) {
+    lo = mk(o2f(b6[0]), o2f(b6[1]), o2f(b6[2]));
+    hi = mk(o2f(b6[3]), o2f(b6[4]), o2f(b6[5]));
+}
+// Morton keys of the shapes' fat-box centres (dead shapes last) and of the rays' origins, with
+// identity values for the sort
+__global__ void k_query_keys(DevState d, const uint4* qinfo, const uint32_t* bounds6, uint32_t* skey, uint32_t* sidx,
+                             const float4* rays, uint32_t n, uint32_t* rkey, uint32_t* ridx) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    v3 lo, hi;
+    load_bounds(bounds6, lo, hi);
+    if (i < d.ns) {
+        const bool live = (qinfo[i].z & QF_LIVE) != 0u;
+        const v3 c = (ld3(d.s_fmin[i]) + ld3(d.s_fmax[i])) * 0.5f;
+        skey[i] = live ? morton(c, lo, hi) : 0x7FFFFFFFu;
+        sidx[i] = i;
+    }
+    if (i < n) {
+        // where the ray crosses the scene: the midpoint of its segment inside the centres' box
+        // (grown by 1 m); rays that miss the box go last
+        const float4 a = rays[2 * i], b = rays[2 * i + 1];
+        QRay r;
+        r.o = ld3(a);
+        r.maxd = a.w;
+        r.d = ld3(b);
+        const v3 glo = lo - splat(1.0f), ghi = hi + splat(1.0f);
+        const v3 inv = mk(1.0f / (r.d.x + 1e-9f), 1.0f / (r.d.y + 1e-9f), 1.0f / (r.d.z + 1e-9f));
+        const v3 t0 = vmul(glo - r.o, inv), t1 = vmul(ghi - r.o, inv);
+        const v3 tmin = vmin(t0, t1), tmax = vmax(t0, t1);
+        const float tn = max4(tmin.x, tmin.y, tmin.z, 0.0f), tf = min4(tmax.x, tmax.y, tmax.z, r.maxd);
+        rkey[i] = tf < tn ? 0x3FFFFFFFu : morton(r.o + r.d * (0.5f * (tn + tf)), lo, hi);
+        ridx[i] = i;
+    }
+}
+// the sorted shape records (fmin.w = the shape's slot) and each tile's box
+__global__ void k_query_gather(DevState d, const float4* geom, const uint4* qinfo, const uint32_t* order, QShape* sorted) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= d.ns) return;
+    const uint32_t j = order[p];
+    QShape s;
+    s.fmin = d.s_fmin[j];
+    s.fmin.w = __uint_as_float(j);
+    s.fmax = d.s_fmax[j];
+    s.g0 = geom[3 * j];
+    s.g1 = geom[3 * j + 1];
+    s.g2 = geom[3 * j + 2];
+    s.info = qinfo[j];
+    sorted[p] = s;
+}
+__global__ void __launch_bounds__(kQueryTile) k_tile_bounds(DevState d, const QShape* sorted, float4* tile_box) {
+    __shared__ float red[6][kQueryTile];
+    const uint32_t p = blockIdx.x * kQueryTile + threadIdx.x;
+    float v[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
+    if (p < d.ns && (sorted[p].info.z & QF_LIVE)) {
+        const float4 a = sorted[p].fmin, b = sorted[p].fmax;
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = b.x; v[4] = b.y; v[5] = b.z;
+    }
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (uint32_t h = kQueryTile / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int k = 0; k < 6; ++k) {
+                const float o = red[k][threadIdx.x + h];
+                red[k][threadIdx.x] = k < 3 ? gmin(red[k][threadIdx.x], o) : gmax(red[k][threadIdx.x], o);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        tile_box[2 * blockIdx.x] = make_float4(red[0][0], red[1][0], red[2][0], 0.0f);
+        tile_box[2 * blockIdx.x + 1] = make_float4(red[3][0], red[4][0], red[5][0], 0.0f);
+    }
 }
 
 __device__ __forceinline__ QShape load_qshape(const DevState& d, const float4* geom, const uint4* qinfo, uint32_t j) {
@@ -225,19 +333,21 @@ __device__ __forceinline__ QShape load_qshape(const DevState& d, const float4* g
 }
 
 // rays: 2 float4 per ray (origin, max distance; direction, -); hits: 3 float4 per ray (point,
-// distance; normal, hit flag bits; body, shape, -, - as bits). any != 0: stop at the first hit.
-__global__ void __launch_bounds__(kQueryTile) k_raycast(DevState d, const float4* geom, const uint4* qinfo,
-                                                        const float4* rays, uint32_t n, QueryFilterDev f, uint32_t any,
-                                                        float4* hits) {
+// distance; normal, hit flag bits; body, shape, -, - as bits). Thread i of the grid takes ray
+// ray_order[i]; any != 0: stop at the first hit.
+__global__ void __launch_bounds__(kQueryTile) k_raycast(DevState d, const QShape* sorted, const float4* tile_box,
+                                                        const float4* rays, const uint32_t* ray_order, uint32_t n,
+                                                        QueryFilterDev f, uint32_t any, float4* hits) {
     __shared__ QShape tile[kQueryTile];
     const uint32_t i = blockIdx.x * kQueryTile + threadIdx.x;
+    const uint32_t ri = i < n ? ray_order[i] : 0u;
     QRay r;
     r.o = mk(0.0f, 0.0f, 0.0f);
     r.d = r.o;
     r.maxd = 0.0f;
     bool active = i < n;
     if (active) {
-        const float4 a = rays[2 * i], b = rays[2 * i + 1];
+        const float4 a = rays[2 * ri], b = rays[2 * ri + 1];
         r.o = ld3(a);
         r.maxd = a.w;
         r.d = ld3(b);
@@ -245,37 +355,43 @@ __global__ void __launch_bounds__(kQueryTile) k_raycast(DevState d, const float4
     bool found = false;
     float best_t = 0.0f;
     v3 best_n = mk(0.0f, 0.0f, 0.0f);
-    uint32_t best_s = 0u;
-    for (uint32_t base = 0; base < d.ns; base += kQueryTile) {
-        __syncthreads();
-        if (base + threadIdx.x < d.ns) tile[threadIdx.x] = load_qshape(d, geom, qinfo, base + threadIdx.x);
+    uint32_t best_s = 0u, best_owner = 0u;
+    const uint32_t ntiles = (d.ns + kQueryTile - 1u) / kQueryTile;
+    for (uint32_t tix = 0; tix < ntiles; ++tix) {
+        const bool need = active && ray_aabb(r, ld3(tile_box[2 * tix]), ld3(tile_box[2 * tix + 1]));
+        if (!__syncthreads_or(need ? 1 : 0)) continue;
+        const uint32_t base = tix * kQueryTile;
+        if (base + threadIdx.x < d.ns) tile[threadIdx.x] = sorted[base + threadIdx.x];
         __syncthreads();
         const uint32_t m = d.ns - base < kQueryTile ? d.ns - base : kQueryTile;
-        for (uint32_t k = 0; active && k < m; ++k) {
+        for (uint32_t k = 0; need && active && k < m; ++k) {
             const QShape& s = tile[k];
             if (!accepts(s.info, f)) continue;
             if (!ray_aabb(r, ld3(s.fmin), ld3(s.fmax))) continue;
             float t;
             v3 nn;
             if (!shape_ray(r, s, t, nn)) continue;
-            if (!found || t < best_t) {
+            const uint32_t slot = __float_as_uint(s.fmin.w);
+            if (!found || t < best_t || (t == best_t && slot < best_s)) {
                 found = true;
                 best_t = t;
                 best_n = nn;
-                best_s = base + k;
+                best_s = slot;
+                best_owner = s.info.x;
                 if (any) active = false;
             }
         }
+        __syncthreads();
     }
     if (i >= n) return;
     if (!found) {
-        hits[3 * i] = hits[3 * i + 1] = hits[3 * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        hits[3 * ri] = hits[3 * ri + 1] = hits[3 * ri + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
     const v3 p = r.o + r.d * best_t;
-    hits[3 * i] = st3(p, best_t);
-    hits[3 * i + 1] = st3(best_n, __uint_as_float(1u));
-    hits[3 * i + 2] = make_float4(__uint_as_float(qinfo[best_s].x), __uint_as_float(best_s), 0.0f, 0.0f);
+    hits[3 * ri] = st3(p, best_t);
+    hits[3 * ri + 1] = st3(best_n, __uint_as_float(1u));
+    hits[3 * ri + 2] = make_float4(__uint_as_float(best_owner), __uint_as_float(best_s), 0.0f, 0.0f);
 }
 
 // every hit of one ray, one thread per shape, in arbitrary order (the host sorts them)
@@ -304,14 +420,39 @@ __global__ void k_raycast_all(DevState d, const float4* geom, const uint4* qinfo
 
 }  // namespace
 
-void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, hipStream_t st) {
-    if (d.ns) hipLaunchKernelGGL(k_query_prep, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, geom, qinfo);
+size_t query_sort_bytes(uint32_t cap) {
+    size_t bytes = 0;
+    uint32_t* none = nullptr;
+    if (rocprim::radix_sort_pairs(nullptr, bytes, none, none, none, none, cap, 0u, 30u) != hipSuccess) return 0;
+    return bytes;
 }
-void launch_raycast(const DevState& d, const float4* geom, const uint4* qinfo, const float4* rays, uint32_t n,
-                    const QueryFilterDev& f, uint32_t any, float4* hits, hipStream_t st) {
-    if (n)
-        hipLaunchKernelGGL(k_raycast, dim3((n + kQueryTile - 1u) / kQueryTile), dim3(kQueryTile), 0, st, d, geom, qinfo,
-                           rays, n, f, any, hits);
+size_t query_shape_record_bytes() { return sizeof(QShape); }
+void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st) {
+    if (!d.ns) return;
+    hipLaunchKernelGGL(k_query_prep, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, geom, qinfo, bounds6);
+}
+hipError_t launch_raycast(const DevState& d, const QueryScratch& q, const float4* rays, uint32_t n, const QueryFilterDev& f,
+                          uint32_t any, float4* hits, hipStream_t st) {
+    if (!n) return hipSuccess;
+    const uint32_t m = d.ns > n ? d.ns : n;
+    hipLaunchKernelGGL(k_query_keys, dim3((m + 255u) / 256u), dim3(256), 0, st, d, q.info, q.bounds6, q.skey, q.sidx, rays,
+                       n, q.rkey, q.ridx);
+    size_t bytes = q.sort_bytes;
+    hipError_t e = hipSuccess;
+    if (d.ns) {
+        e = rocprim::radix_sort_pairs(q.sort_tmp, bytes, q.skey, q.skey2, q.sidx, q.sidx2, d.ns, 0u, 30u, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_query_gather, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, q.geom, q.info, q.sidx2,
+                           static_cast<QShape*>(q.sorted));
+        hipLaunchKernelGGL(k_tile_bounds, dim3((d.ns + kQueryTile - 1u) / kQueryTile), dim3(kQueryTile), 0, st, d,
+                           static_cast<const QShape*>(q.sorted), q.tile_box);
+    }
+    bytes = q.sort_bytes;
+    e = rocprim::radix_sort_pairs(q.sort_tmp, bytes, q.rkey, q.rkey2, q.ridx, q.ridx2, n, 0u, 30u, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_raycast, dim3((n + kQueryTile - 1u) / kQueryTile), dim3(kQueryTile), 0, st, d,
+                       static_cast<const QShape*>(q.sorted), q.tile_box, rays, q.ridx2, n, f, any, hits);
+    return hipSuccess;
 }
 void launch_raycast_all(const DevState& d, const float4* geom, const uint4* qinfo, const float4* ray,
                         const QueryFilterDev& f, float4* hits, uint32_t cap, uint32_t* count, hipStream_t st) {

@@ -225,15 +225,31 @@ void launch_init_bodies(const DevState& d, hipStream_t st);
 // manifolds (at most n_pairs) into the snapshot ev_snap[cur], compares it with last step's (at
 // most prev_bound entries) and appends the events to the queue
 size_t events_sort_bytes(uint32_t cap_pairs);
-// scene raycasts (pxg_query.hip): per-shape world geometry (3 float4 per shape) and query info
-// (uint4) from the current poses, then the closest (or any) hit of n rays (2 float4 each; hits 3
-// float4 each), or every hit of one ray (unordered, at most cap, count in *count)
+// scene raycasts (pxg_query.hip). Per batch: per-shape world geometry (3 float4 per shape),
+// query info (uint4) and the bounds of the live fat-box centres (6 order-encoded floats, reset
+// to empty before the prep) from the current poses; then shapes and rays sorted by Morton code,
+// the sorted shape records and their tiles' boxes, and the closest (or any) hit of n rays (2
+// float4 each; hits 3 float4 each). raycast_all: every hit of one ray (unordered, at most cap,
+// count in *count) from the unsorted records.
 struct QueryFilterDev {
     uint32_t layer_mask, include_triggers, ignore_body;
 };
-void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, hipStream_t st);
-void launch_raycast(const DevState& d, const float4* geom, const uint4* qinfo, const float4* rays, uint32_t n,
-                    const QueryFilterDev& f, uint32_t any, float4* hits, hipStream_t st);
+struct QueryScratch {
+    float4* geom;       // 3 per shape slot
+    uint4* info;        // per shape slot
+    uint32_t* bounds6;
+    uint32_t *skey, *skey2, *sidx, *sidx2;  // per shape slot
+    uint32_t *rkey, *rkey2, *ridx, *ridx2;  // per ray
+    void* sorted;       // per shape slot: query_shape_record_bytes() each
+    float4* tile_box;   // 2 per tile of 256 shapes
+    void* sort_tmp;
+    size_t sort_bytes;
+};
+size_t query_sort_bytes(uint32_t cap);
+size_t query_shape_record_bytes();
+void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);
+hipError_t launch_raycast(const DevState& d, const QueryScratch& q, const float4* rays, uint32_t n, const QueryFilterDev& f,
+                          uint32_t any, float4* hits, hipStream_t st);
 void launch_raycast_all(const DevState& d, const float4* geom, const uint4* qinfo, const float4* ray,
                         const QueryFilterDev& f, float4* hits, uint32_t cap, uint32_t* count, hipStream_t st);
 hipError_t launch_events(const DevState& d, uint32_t n_pairs, uint32_t prev_bound, int cur, hipStream_t st);

@@ -77,9 +77,8 @@ struct pxg_world {
     int ev_cur = 0;
     uint32_t ev_prev_n = 0;
     uint32_t ev_cap_req = 0;
-    // scene queries: per-shape world geometry and info, rays and hits (grown on demand)
-    float4* q_geom = nullptr;
-    uint4* q_info = nullptr;
+    // scene queries: per-shape records and sort scratch, rays and hits (grown on demand)
+    QueryScratch q{};
     float4* q_rays = nullptr;
     float4* q_hits = nullptr;
     uint32_t* q_count = nullptr;
@@ -932,24 +931,39 @@ static QueryFilterDev query_filter(const pxg_query_filter* f) {
 // query buffers for n rays (n hits), and the per-shape records filled from the current poses
 static int query_setup(pxg_world* w, uint32_t n) {
     if (!w->shapes_ready) return fail(PXG_ERR_STATE, "scene query: upload bodies and shapes first");
-    if (!w->q_geom) {
-        DALLOC(w->q_geom, 3 * size_t(w->cap_s));
-        DALLOC(w->q_info, w->cap_s);
+    QueryScratch& q = w->q;
+    const size_t S = w->cap_s;
+    if (!q.geom) {
+        DALLOC(q.geom, 3 * S); DALLOC(q.info, S); DALLOC(q.bounds6, 6);
+        DALLOC(q.skey, S); DALLOC(q.skey2, S); DALLOC(q.sidx, S); DALLOC(q.sidx2, S);
+        uint8_t* rec = nullptr;
+        DALLOC(rec, S * query_shape_record_bytes());
+        q.sorted = rec;
+        DALLOC(q.tile_box, 2 * ((S + 255) / 256) + 2);
         DALLOC(w->q_count, 1);
     }
-    if (n > w->q_cap) {
+    if (n > w->q_cap || !q.sort_tmp) {
         HIPCHK(hipStreamSynchronize(w->stream));
-        dfree(w, w->q_rays);
-        dfree(w, w->q_hits);
-        w->q_rays = nullptr;
-        w->q_hits = nullptr;
+        void* old[8] = {w->q_rays, w->q_hits, q.rkey, q.rkey2, q.ridx, q.ridx2, q.sort_tmp, nullptr};
+        for (void* p : old) dfree(w, p);
+        w->q_rays = w->q_hits = nullptr;
+        q.rkey = q.rkey2 = q.ridx = q.ridx2 = nullptr;
+        q.sort_tmp = nullptr;
         w->q_cap = 0;
-        uint32_t cap = std::max<uint32_t>(n, 1024u);
+        const uint32_t cap = std::max<uint32_t>(std::max(n, w->q_cap), 1024u);
         DALLOC(w->q_rays, 2 * size_t(cap));
         DALLOC(w->q_hits, 3 * size_t(cap));
+        DALLOC(q.rkey, cap); DALLOC(q.rkey2, cap); DALLOC(q.ridx, cap); DALLOC(q.ridx2, cap);
+        q.sort_bytes = query_sort_bytes(std::max<uint32_t>(cap, w->cap_s));
+        if (!q.sort_bytes) return fail(PXG_ERR_HIP, "scene query: sizing the sort failed");
+        uint8_t* tmp = nullptr;
+        DALLOC(tmp, q.sort_bytes);
+        q.sort_tmp = tmp;
         w->q_cap = cap;
     }
-    launch_query_prep(w->d, w->q_geom, w->q_info, w->stream);
+    launch_fill_u32(q.bounds6, 0xFFFFFFFFu, 3, w->stream);
+    launch_fill_u32(q.bounds6 + 3, 0u, 3, w->stream);
+    launch_query_prep(w->d, q.geom, q.info, q.bounds6, w->stream);
     return PXG_OK;
 }
 static_assert(sizeof(pxg_ray) == 2 * sizeof(float4) && sizeof(pxg_raycast_hit) == 3 * sizeof(float4),
@@ -960,7 +974,7 @@ static int raycast_batch(pxg_world* w, const pxg_ray* rays, uint32_t n, const px
     int r = query_setup(w, n);
     if (r != PXG_OK) return r;
     HIPCHK(hipMemcpyAsync(w->q_rays, rays, size_t(n) * sizeof(pxg_ray), hipMemcpyHostToDevice, w->stream));
-    launch_raycast(w->d, w->q_geom, w->q_info, w->q_rays, n, query_filter(filter), any, w->q_hits, w->stream);
+    HIPCHK(launch_raycast(w->d, w->q, w->q_rays, n, query_filter(filter), any, w->q_hits, w->stream));
     (void)what;
     return PXG_OK;
 }
@@ -973,6 +987,17 @@ extern "C" int pxg_raycast(pxg_world* w, const pxg_ray* rays, uint32_t n, const 
     if (r != PXG_OK) return r;
     HIPCHK(hipMemcpyAsync(hits, w->q_hits, size_t(n) * sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_raycast_device(pxg_world* w, const pxg_ray* d_rays, uint32_t n, const pxg_query_filter* filter,
+                                  pxg_raycast_hit* d_hits) {
+    if (!w || (n && (!d_rays || !d_hits))) return fail(PXG_ERR_INVALID, "pxg_raycast_device: null argument");
+    if (!n) return PXG_OK;
+    int r = query_setup(w, n);  // the ray keys and order need room for n rays
+    if (r != PXG_OK) return r;
+    HIPCHK(launch_raycast(w->d, w->q, reinterpret_cast<const float4*>(d_rays), n, query_filter(filter), 0u,
+                          reinterpret_cast<float4*>(d_hits), w->stream));
     return PXG_OK;
 }
 
@@ -998,7 +1023,7 @@ extern "C" int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query
     if (r != PXG_OK) return r;
     HIPCHK(hipMemcpyAsync(w->q_rays, ray, sizeof(pxg_ray), hipMemcpyHostToDevice, w->stream));
     launch_fill_u32(w->q_count, 0u, 1, w->stream);
-    launch_raycast_all(w->d, w->q_geom, w->q_info, w->q_rays, query_filter(filter), w->q_hits, w->q_cap, w->q_count,
+    launch_raycast_all(w->d, w->q.geom, w->q.info, w->q_rays, query_filter(filter), w->q_hits, w->q_cap, w->q_count,
                        w->stream);
     uint32_t count = 0;
     HIPCHK(hipMemcpyAsync(&count, w->q_count, sizeof(count), hipMemcpyDeviceToHost, w->stream));

@@ -640,6 +640,41 @@ void bouncing_sphere_enters_and_exits_and_step_clears_the_queue() {
     CHECK(most <= 2);  // one substep per step(kH): at most this step's events of the one pair
 }
 
+// queries_test.cpp:22-37, 73-97, 270-293: exact distance, point and normal; every hit nearest
+// first; any hit; the ignore filter
+void raycasts_hit_boxes_with_exact_distances_and_filters() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId b9 = world.add_body(body(PXG_STATIC, 9.0f, 0, 0), box_shape(0.5f, 0.5f, 0.5f));
+    pxg::BodyId b3 = world.add_body(body(PXG_STATIC, 3.0f, 0, 0), box_shape(0.5f, 0.5f, 0.5f));
+    pxg::BodyId b6 = world.add_body(body(PXG_STATIC, 6.0f, 0, 0), box_shape(0.5f, 0.5f, 0.5f));
+    (void)b9;
+    pxg::Ray ray;
+    ray.origin = pxg::Vec3{0.0f, 0.0f, 0.0f};
+    ray.direction = pxg::Vec3{1.0f, 0.0f, 0.0f};
+    ray.max_distance = 100.0f;
+    pxg::RaycastHit hit;
+    CHECK(world.raycast(ray, hit));
+    CHECK_NEAR(hit.distance, 2.5f, 1e-3f);
+    CHECK_NEAR(hit.point.x, 2.5f, 1e-3f);
+    CHECK_NEAR(hit.normal.x, -1.0f, 1e-3f);
+    CHECK(hit.body == b3);
+    std::vector<pxg::RaycastHit> all = world.raycast_all(ray);
+    CHECK(all.size() == 3);
+    CHECK_NEAR(all[0].distance, 2.5f, 1e-3f);
+    CHECK_NEAR(all[1].distance, 5.5f, 1e-3f);
+    CHECK_NEAR(all[2].distance, 8.5f, 1e-3f);
+    CHECK(world.raycast_any(ray));
+    pxg::Ray back = ray;
+    back.direction = pxg::Vec3{-1.0f, 0.0f, 0.0f};
+    CHECK(!world.raycast_any(back));
+    pxg::QueryFilter ignore_near;
+    ignore_near.ignore = b3;
+    CHECK(world.raycast(ray, hit, ignore_near));
+    CHECK(hit.body == b6);
+    CHECK_NEAR(hit.distance, 5.5f, 1e-4f);
+    CHECK(world.raycast_all(ray, ignore_near).size() == 2);
+}
+
 int main() {
     struct T {
         const char* name;
@@ -670,6 +705,7 @@ int main() {
         {"trigger_fires_enter_and_exit_once_with_stays_between", trigger_fires_enter_and_exit_once_with_stays_between},
         {"bouncing_sphere_enters_and_exits_and_step_clears_the_queue",
          bouncing_sphere_enters_and_exits_and_step_clears_the_queue},
+        {"raycasts_hit_boxes_with_exact_distances_and_filters", raycasts_hit_boxes_with_exact_distances_and_filters},
     };
     for (const T& t : tests) {
         int before = g_failures;
