@@ -263,6 +263,16 @@ typedef struct pxg_raycast_hit {
     uint32_t reserved[2];
 } pxg_raycast_hit;
 
+/* The query shape of overlap_sphere / overlap_box / overlap_capsule (world.h:1002-1060). */
+typedef struct pxg_overlap_query {
+    uint32_t type;          /* PXG_SPHERE, PXG_BOX or PXG_CAPSULE */
+    float radius;           /* sphere, capsule */
+    float center[3];        /* sphere, box */
+    float half_extents[3];  /* box */
+    float orientation[4];   /* box: x, y, z, w */
+    float a[3], b[3];       /* capsule end points */
+} pxg_overlap_query;
+
 typedef struct pxg_world pxg_world;
 
 /* --- configuration and lifetime --- */
@@ -348,6 +358,13 @@ int pxg_raycast_device(pxg_world* w, const pxg_ray* d_rays, uint32_t n, const px
                        pxg_raycast_hit* d_hits);
 int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query_filter* filter, pxg_raycast_hit* hits,
                     uint32_t cap, uint32_t* n);
+/* Every body with a shape overlapping the query shape (overlap_sphere / overlap_box /
+ * overlap_capsule, world.h:1002-1060): the query's AABB against the fat AABBs (inclusive), the
+ * filter, then the exact narrowphase test (query/queries.h:139-258, sphere, box and capsule
+ * shapes). Bodies come once per overlapping shape, in shape-slot order; n receives the count,
+ * min(cap, n) are copied. */
+int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_filter* filter, uint32_t* bodies,
+                uint32_t cap, uint32_t* n);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);
 

@@ -120,6 +120,9 @@ struct Ray {
     Vec3 direction{0.0f, 0.0f, -1.0f};  // unit length
     float max_distance = 3.402823466e+38f;
 };
+struct Quat {
+    float x = 0.0f, y = 0.0f, z = 0.0f, w = 1.0f;
+};
 struct RaycastHit {
     Vec3 point;
     Vec3 normal{0.0f, 0.0f, 1.0f};
@@ -514,6 +517,33 @@ public:
         for (uint32_t i = 0; i < n && i < raw.size(); ++i) out.push_back(from_hit(raw[i]));
         return out;
     }
+    // Bodies whose shape overlaps a world-space sphere / OBB / capsule (world.h:1002-1060), once
+    // per overlapping shape, in shape-slot order.
+    std::vector<BodyId> overlap_sphere(Vec3 center, float radius, const QueryFilter& filter = QueryFilter()) {
+        pxg_overlap_query q = overlap_base(PXG_SPHERE);
+        set3(q.center, center);
+        q.radius = radius;
+        return overlap(q, filter);
+    }
+    std::vector<BodyId> overlap_box(Vec3 center, Vec3 half_extents, Quat orientation,
+                                    const QueryFilter& filter = QueryFilter()) {
+        pxg_overlap_query q = overlap_base(PXG_BOX);
+        set3(q.center, center);
+        set3(q.half_extents, half_extents);
+        q.orientation[0] = orientation.x;
+        q.orientation[1] = orientation.y;
+        q.orientation[2] = orientation.z;
+        q.orientation[3] = orientation.w;
+        return overlap(q, filter);
+    }
+    std::vector<BodyId> overlap_capsule(Vec3 a, Vec3 b, float radius, const QueryFilter& filter = QueryFilter()) {
+        pxg_overlap_query q = overlap_base(PXG_CAPSULE);
+        set3(q.a, a);
+        set3(q.b, b);
+        q.radius = radius;
+        return overlap(q, filter);
+    }
+
     // The closest hits of many rays in one GPU pass (no counterpart in the reference, which casts
     // one ray per call): hits[i] is valid where found[i] != 0.
     void raycast_batch(const std::vector<Ray>& rays, std::vector<RaycastHit>& hits, std::vector<uint8_t>& found,
@@ -659,6 +689,25 @@ private:
         return bodies_[i].alive != 0;
     }
 
+    static pxg_overlap_query overlap_base(uint32_t type) {
+        pxg_overlap_query q;
+        std::memset(&q, 0, sizeof(q));
+        q.type = type;
+        q.orientation[3] = 1.0f;
+        return q;
+    }
+    std::vector<BodyId> overlap(const pxg_overlap_query& q, const QueryFilter& filter) {
+        push();
+        pxg_query_filter f = to_filter(filter);
+        uint32_t n = 0;
+        check(pxg_overlap(w_, &q, &f, nullptr, 0, &n), "overlap");
+        std::vector<uint32_t> raw(n);
+        if (n) check(pxg_overlap(w_, &q, &f, raw.data(), n, &n), "overlap");
+        std::vector<BodyId> out;
+        for (uint32_t i = 0; i < n && i < raw.size(); ++i)
+            out.push_back(BodyId{raw[i], raw[i] < generations_.size() ? generations_[raw[i]] : 0u});
+        return out;
+    }
     static pxg_ray to_ray(const Ray& r) {
         pxg_ray o;
         std::memset(&o, 0, sizeof(o));

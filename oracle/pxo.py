@@ -54,7 +54,7 @@ def lib():
             "pxo_atan2": ([f32, f32, u32], f32),
             "pxo_get_events": ([vp, vp, u32], u32), "pxo_clear_events": ([vp], None),
             "pxo_raycast": ([vp, vp, u32, vp, vp], None), "pxo_raycast_all": ([vp, vp, vp, vp, u32], u32),
-            "pxo_ray_shape": ([vp, vp, vp, vp, vp, vp], i32), "pxo_ray_aabb": ([vp, vp, vp], i32),
+            "pxo_ray_shape": ([vp, vp, vp, vp, vp, vp], i32), "pxo_overlap": ([vp, vp, vp, vp, u32], u32), "pxo_ray_aabb": ([vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -177,6 +177,15 @@ class OracleWorld:
         f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter))
         lib().pxo_raycast(self.h, ptr(r), len(r), f, ptr(out))
         return out[:len(r)]
+
+    def overlap(self, query, query_filter=None):
+        """Bodies with a shape overlapping the query shape, in shape-slot order (world.h:1002-1060)."""
+        q = np.ascontiguousarray(query).reshape(1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter))
+        n = lib().pxo_overlap(self.h, ptr(q), f, None, 0)
+        out = np.zeros(max(n, 1), np.uint32)
+        lib().pxo_overlap(self.h, ptr(q), f, ptr(out), n)
+        return out[:n]
 
     def raycast_all(self, ray, query_filter=None):
         """Every hit of one ray, nearest first (world.h:825-847)."""

@@ -132,6 +132,14 @@ inline QueryFilter filter_from(const pxg_query_filter* f) {
     if (f) { q.layer_mask = f->layer_mask; q.include_triggers = f->include_triggers != 0; q.ignore = f->ignore_body; }
     return q;
 }
+template <class T> OverlapQuery<T> overlap_from(const pxg_overlap_query& in) {
+    OverlapQuery<T> q;
+    q.type = int(in.type);
+    q.sph.c = v3<T>(in.center); q.sph.r = T(in.radius);
+    q.box.c = v3<T>(in.center); q.box.he = v3<T>(in.half_extents); q.box.q = q4<T>(in.orientation);
+    q.cap.a = v3<T>(in.a); q.cap.b = v3<T>(in.b); q.cap.r = T(in.radius);
+    return q;
+}
 template <class T> void hit_to(const RayHit<T>& h, pxg_raycast_hit& o) {
     std::memset(&o, 0, sizeof(o));
     put3(o.point, h.point); o.distance = float(h.distance); put3(o.normal, h.normal);
@@ -285,6 +293,19 @@ uint32_t pxo_raycast_all(void* p, const pxg_ray* ray, const pxg_query_filter* f,
         auto hits = world_raycast(w, ray_from<T>(*ray), filter_from(f), false);
         n = uint32_t(hits.size());
         for (uint32_t i = 0; i < n && i < cap; ++i) hit_to(hits[i], out[i]);
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+    return n;
+}
+// overlap_sphere / overlap_box / overlap_capsule: returns the count, copies min(cap, count) bodies
+uint32_t pxo_overlap(void* p, const pxg_overlap_query* q, const pxg_query_filter* f, uint32_t* bodies, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    uint32_t n = 0;
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        auto out = world_overlap(w, overlap_from<T>(*q), filter_from(f));
+        n = uint32_t(out.size());
+        for (uint32_t i = 0; i < n && i < cap; ++i) bodies[i] = out[i];
     };
     if (h->dbl) go(h->d); else go(h->f);
     return n;

@@ -198,4 +198,63 @@ template <class T> std::vector<RayHit<T>> world_raycast(World<T>& w, const Ray<T
     return out;
 }
 
+// overlap_sphere / overlap_box / overlap_capsule (world.h:1002-1060): the query AABB against the
+// fat AABBs (AABB::overlaps, inclusive, geometry/aabb.h:35-39; the trees' query), the filter,
+// then shape_overlaps_sphere / _obb / _capsule (query/queries.h:139-258). Bodies in slot order
+// (the reference lists them in tree order).
+template <class T> struct OverlapQuery {
+    int type = PXG_SPHERE;
+    Sph<T> sph;
+    OBB<T> box;
+    Cap<T> cap;
+};
+template <class T> bool shape_overlaps(const Shape<T>& s, const V3<T>& pos, const Q<T>& rot, const OverlapQuery<T>& q) {
+    V3<T> n, p;
+    T pen;
+    Manifold<T> m;
+    if (q.type == PXG_SPHERE) {
+        if (s.type == PXG_SPHERE) return sphere_vs_sphere(world_sphere(s, pos, rot), q.sph, n, pen, p);
+        if (s.type == PXG_BOX) return sphere_vs_box(q.sph, world_obb(s, pos, rot), n, pen, p);
+        return capsule_vs_sphere(world_capsule(s, pos, rot), q.sph, n, pen, p);
+    }
+    if (q.type == PXG_BOX) {
+        if (s.type == PXG_SPHERE) return sphere_vs_box(world_sphere(s, pos, rot), q.box, n, pen, p);
+        if (s.type == PXG_BOX) return box_box(world_obb(s, pos, rot), q.box, m, false);
+        return capsule_vs_box(world_capsule(s, pos, rot), q.box, m);
+    }
+    if (s.type == PXG_SPHERE) return capsule_vs_sphere(q.cap, world_sphere(s, pos, rot), n, pen, p);
+    if (s.type == PXG_BOX) return capsule_vs_box(q.cap, world_obb(s, pos, rot), m);
+    return capsule_vs_capsule(q.cap, world_capsule(s, pos, rot), n, pen, p);
+}
+template <class T> AABB<T> overlap_bounds(const OverlapQuery<T>& q) {
+    AABB<T> b;
+    if (q.type == PXG_SPHERE) {
+        V3<T> r(q.sph.r, q.sph.r, q.sph.r);
+        b.mn = q.sph.c - r;
+        b.mx = q.sph.c + r;
+    } else if (q.type == PXG_BOX) {
+        b = obb_bounds(q.box);
+    } else {
+        V3<T> r(q.cap.r, q.cap.r, q.cap.r);
+        b.mn = vmin(q.cap.a, q.cap.b) - r;
+        b.mx = vmax(q.cap.a, q.cap.b) + r;
+    }
+    return b;
+}
+template <class T> std::vector<uint32_t> world_overlap(World<T>& w, const OverlapQuery<T>& q, const QueryFilter& f) {
+    std::vector<uint32_t> out;
+    const AABB<T> qb = overlap_bounds(q);
+    for (uint32_t i = 0; i < w.shapes.size(); ++i) {
+        const Shape<T>& s = w.shapes[i];
+        if (i >= w.proxy.size() || w.proxy[i] < 0 || s.owner >= w.bodies.size() || !w.bodies[s.owner].alive) continue;
+        if (!((f.layer_mask >> (s.layer & 31u)) & 1u)) continue;
+        if (!f.include_triggers && s.trigger) continue;
+        if (f.ignore != ~0u && s.owner == f.ignore) continue;
+        if (!w.fat(i).overlaps(qb)) continue;
+        const Body<T>& b = w.bodies[s.owner];
+        if (shape_overlaps(s, b.pos, b.q, q)) out.push_back(s.owner);
+    }
+    return out;
+}
+
 }  // namespace pxo

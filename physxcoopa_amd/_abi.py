@@ -80,6 +80,29 @@ def rays(origins, directions, max_distance=np.finfo(np.float32).max):
     return r
 
 
+OVERLAP_DTYPE = np.dtype([("type", "<u4"), ("radius", "<f4"), ("center", "<f4", 3), ("half_extents", "<f4", 3),
+                          ("orientation", "<f4", 4), ("a", "<f4", 3), ("b", "<f4", 3)])
+assert OVERLAP_DTYPE.itemsize == 72
+
+
+def overlap_sphere(center, radius):
+    q = np.zeros(1, OVERLAP_DTYPE)
+    q["type"], q["center"], q["radius"], q["orientation"] = PXG_SPHERE, center, radius, (0, 0, 0, 1)
+    return q
+
+
+def overlap_box(center, half_extents, orientation=(0, 0, 0, 1)):
+    q = np.zeros(1, OVERLAP_DTYPE)
+    q["type"], q["center"], q["half_extents"], q["orientation"] = PXG_BOX, center, half_extents, orientation
+    return q
+
+
+def overlap_capsule(a, b, radius):
+    q = np.zeros(1, OVERLAP_DTYPE)
+    q["type"], q["a"], q["b"], q["radius"], q["orientation"] = PXG_CAPSULE, a, b, radius, (0, 0, 0, 1)
+    return q
+
+
 def query_filter(layer_mask=0xFFFFFFFF, include_triggers=True, ignore_body=PXG_NO_BODY):
     f = np.zeros(1, FILTER_DTYPE)
     f["layer_mask"] = layer_mask

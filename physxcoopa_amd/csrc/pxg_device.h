@@ -160,6 +160,22 @@ PXD Cap world_capsule(const ShapeGeom& s, v3 pos, quat rot) {
     o.r = s.cr;
     return o;
 }
+// OBB::bounds (geometry/obb.h:39-49): the 8 corners through mat4_cast with the centre as the
+// translation column (aabb.h:83-95)
+PXD void obb_bounds(const OBB& o, v3& mn, v3& mx) {
+    m3 m = mat3_cast(o.q);
+    mn = splat(3.402823466e+38f);
+    mx = splat(-3.402823466e+38f);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float px = (i & 1) ? o.he.x : -o.he.x, py = (i & 2) ? o.he.y : -o.he.y, pz = (i & 4) ? o.he.z : -o.he.z;
+        v3 w = mk((m.c0.x * px + m.c1.x * py) + (m.c2.x * pz + o.c.x * 1.0f),
+                  (m.c0.y * px + m.c1.y * py) + (m.c2.y * pz + o.c.y * 1.0f),
+                  (m.c0.z * px + m.c1.z * py) + (m.c2.z * pz + o.c.z * 1.0f));
+        mn = vmin(mn, w);
+        mx = vmax(mx, w);
+    }
+}
 // Tight world AABB (collision/shape.h:198-228; box via 8 transformed corners, aabb.h:83-95)
 PXD void world_bounds(const ShapeGeom& s, v3 pos, quat rot, v3& mn, v3& mx) {
     if (s.type == SH_SPHERE) {
@@ -167,19 +183,7 @@ PXD void world_bounds(const ShapeGeom& s, v3 pos, quat rot, v3& mn, v3& mx) {
         mn = sp.c - splat(sp.r);
         mx = sp.c + splat(sp.r);
     } else if (s.type == SH_BOX) {
-        OBB o = world_obb(s, pos, rot);
-        m3 m = mat3_cast(o.q);
-        mn = splat(3.402823466e+38f);
-        mx = splat(-3.402823466e+38f);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            float px = (i & 1) ? o.he.x : -o.he.x, py = (i & 2) ? o.he.y : -o.he.y, pz = (i & 4) ? o.he.z : -o.he.z;
-            v3 w = mk((m.c0.x * px + m.c1.x * py) + (m.c2.x * pz + o.c.x * 1.0f),
-                      (m.c0.y * px + m.c1.y * py) + (m.c2.y * pz + o.c.y * 1.0f),
-                      (m.c0.z * px + m.c1.z * py) + (m.c2.z * pz + o.c.z * 1.0f));
-            mn = vmin(mn, w);
-            mx = vmax(mx, w);
-        }
+        obb_bounds(world_obb(s, pos, rot), mn, mx);
     } else {
         Cap c = world_capsule(s, pos, rot);
         mn = vmin(c.a, c.b) - splat(c.r);

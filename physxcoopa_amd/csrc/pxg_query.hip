@@ -418,7 +418,81 @@ __global__ void k_raycast_all(DevState d, const float4* geom, const uint4* qinfo
     hits[3 * slot + 2] = make_float4(__uint_as_float(s.info.x), __uint_as_float(j), 0.0f, 0.0f);
 }
 
+// overlap_sphere / overlap_box / overlap_capsule (world.h:1002-1060, query/queries.h:139-258):
+// one thread per shape; flags[j] = owner + 1 when shape j overlaps the query, else 0
+__device__ __forceinline__ bool shape_overlaps(const QShape& s, const OverlapDev& q) {
+    v3 n, p;
+    float pen;
+    Man m;
+    m.count = 0;
+    const Sph qs = {ld3(q.c), q.r};
+    OBB qb;
+    qb.c = ld3(q.c);
+    qb.he = ld3(q.he);
+    qb.q = ldq(q.q);
+    const Cap qc = {ld3(q.a), ld3(q.b), q.r};
+    Sph ss;
+    ss.c = ld3(s.g0);
+    ss.r = s.g0.w;
+    OBB sb;
+    sb.c = ld3(s.g0);
+    sb.he = ld3(s.g1);
+    sb.q = ldq(s.g2);
+    Cap sc;
+    sc.a = ld3(s.g0);
+    sc.b = ld3(s.g1);
+    sc.r = s.g0.w;
+    const uint32_t t = s.info.y;
+    if (q.type == SH_SPHERE) {
+        if (t == SH_SPHERE) return sphere_sphere(ss, qs, n, pen, p);
+        if (t == SH_BOX) return sphere_box_any(qs, sb, false, n, pen, p) != 0;
+        return capsule_sphere(sc, qs, n, pen, p);
+    }
+    if (q.type == SH_BOX) {
+        if (t == SH_SPHERE) return sphere_box_any(ss, qb, false, n, pen, p) != 0;
+        if (t == SH_BOX) return box_box(sb, qb, m, false);
+        return capsule_box(sc, qb, m);
+    }
+    if (t == SH_SPHERE) return capsule_sphere(qc, ss, n, pen, p);
+    if (t == SH_BOX) return capsule_box(qc, sb, m);
+    return capsule_capsule(qc, sc, n, pen, p);
+}
+__global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, OverlapDev q, QueryFilterDev f,
+                          uint32_t* flags) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.ns) return;
+    v3 qmn, qmx;
+    if (q.type == SH_SPHERE) {
+        qmn = ld3(q.c) - splat(q.r);
+        qmx = ld3(q.c) + splat(q.r);
+    } else if (q.type == SH_BOX) {
+        OBB qb;
+        qb.c = ld3(q.c);
+        qb.he = ld3(q.he);
+        qb.q = ldq(q.q);
+        obb_bounds(qb, qmn, qmx);
+    } else {
+        qmn = vmin(ld3(q.a), ld3(q.b)) - splat(q.r);
+        qmx = vmax(ld3(q.a), ld3(q.b)) + splat(q.r);
+    }
+    const QShape s = load_qshape(d, geom, qinfo, j);
+    uint32_t out = 0u;
+    if (accepts(s.info, f)) {
+        const v3 fmn = ld3(s.fmin), fmx = ld3(s.fmax);
+        // AABB::overlaps (geometry/aabb.h:35-39), inclusive
+        const bool box_hit = fmn.x <= qmx.x && fmx.x >= qmn.x && fmn.y <= qmx.y && fmx.y >= qmn.y && fmn.z <= qmx.z &&
+                             fmx.z >= qmn.z;
+        if (box_hit && shape_overlaps(s, q)) out = s.info.x + 1u;
+    }
+    flags[j] = out;
+}
+
 }  // namespace
+
+void launch_overlap(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev& q, const QueryFilterDev& f,
+                    uint32_t* flags, hipStream_t st) {
+    if (d.ns) hipLaunchKernelGGL(k_overlap, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, geom, qinfo, q, f, flags);
+}
 
 size_t query_sort_bytes(uint32_t cap) {
     size_t bytes = 0;

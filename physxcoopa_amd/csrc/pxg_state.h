@@ -245,6 +245,14 @@ struct QueryScratch {
     void* sort_tmp;
     size_t sort_bytes;
 };
+// overlap query shape: type (SH_*), sphere (c, r), box (c, he, q), capsule (a, b, r)
+struct OverlapDev {
+    uint32_t type;
+    float r;
+    float4 c, he, q, a, b;
+};
+void launch_overlap(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev& q, const QueryFilterDev& f,
+                    uint32_t* flags, hipStream_t st);
 size_t query_sort_bytes(uint32_t cap);
 size_t query_shape_record_bytes();
 void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);

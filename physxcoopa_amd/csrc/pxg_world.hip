@@ -1041,6 +1041,38 @@ extern "C" int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query
     return PXG_OK;
 }
 
+extern "C" int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_filter* filter, uint32_t* bodies,
+                           uint32_t cap, uint32_t* n) {
+    if (!w || !query || !n || (cap && !bodies)) return fail(PXG_ERR_INVALID, "pxg_overlap: null argument");
+    *n = 0;
+    if (query->type > PXG_CAPSULE) return fail(PXG_ERR_INVALID, "pxg_overlap: the query shape is a sphere, box or capsule");
+    // one flag per shape: the ray-hit buffer (one record per shape at least) holds them
+    int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u));
+    if (r != PXG_OK) return r;
+    OverlapDev q;
+    q.type = query->type;
+    q.r = query->radius;
+    q.c = make_float4(query->center[0], query->center[1], query->center[2], 0.0f);
+    q.he = make_float4(query->half_extents[0], query->half_extents[1], query->half_extents[2], 0.0f);
+    q.q = make_float4(query->orientation[0], query->orientation[1], query->orientation[2], query->orientation[3]);
+    q.a = make_float4(query->a[0], query->a[1], query->a[2], 0.0f);
+    q.b = make_float4(query->b[0], query->b[1], query->b[2], 0.0f);
+    uint32_t* flags = reinterpret_cast<uint32_t*>(w->q_hits);
+    launch_overlap(w->d, w->q.geom, w->q.info, q, query_filter(filter), flags, w->stream);
+    std::vector<uint32_t> h(w->n_shapes);
+    if (!h.empty())
+        HIPCHK(hipMemcpyAsync(h.data(), flags, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    uint32_t count = 0;
+    for (uint32_t f : h) {
+        if (!f) continue;
+        if (count < cap) bodies[count] = f - 1u;
+        ++count;
+    }
+    *n = count;
+    return PXG_OK;
+}
+
 extern "C" int pxg_synchronize(pxg_world* w) {
     if (!w) return fail(PXG_ERR_INVALID, "pxg_synchronize: null handle");
     HIPCHK(hipStreamSynchronize(w->stream));

@@ -149,6 +149,17 @@ class World:
         check(lib().pxg_raycast_any(self.h, ptr(r), len(r), f, ptr(out)), "pxg_raycast_any")
         return out[:len(r)].astype(bool)
 
+    def overlap(self, query, query_filter=None):
+        """Bodies with a shape overlapping the query shape (OVERLAP_DTYPE; world.h:1002-1060), once
+        per overlapping shape, in shape-slot order."""
+        q = np.ascontiguousarray(query, dtype=_abi.OVERLAP_DTYPE).reshape(1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        n = ctypes.c_uint32()
+        check(lib().pxg_overlap(self.h, ptr(q), f, None, 0, ctypes.byref(n)), "pxg_overlap")
+        out = np.zeros(max(n.value, 1), np.uint32)
+        check(lib().pxg_overlap(self.h, ptr(q), f, ptr(out), n.value, ctypes.byref(n)), "pxg_overlap")
+        return out[:n.value]
+
     def raycast_all(self, ray, query_filter=None):
         """Every hit of one ray, nearest first (world.h:825-847)."""
         r = np.ascontiguousarray(ray, dtype=_abi.RAY_DTYPE).reshape(1)

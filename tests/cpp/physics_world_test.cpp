@@ -675,6 +675,19 @@ void raycasts_hit_boxes_with_exact_distances_and_filters() {
     CHECK(world.raycast_all(ray, ignore_near).size() == 2);
 }
 
+// queries_test.cpp:100-123: overlap queries find only the touched body
+void overlaps_find_only_touched_bodies() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId near_id = world.add_body(body(PXG_STATIC, 0, 0, 0), box_shape(0.5f, 0.5f, 0.5f));
+    world.add_body(body(PXG_STATIC, 20.0f, 0, 0), box_shape(0.5f, 0.5f, 0.5f));
+    std::vector<pxg::BodyId> s = world.overlap_sphere(pxg::Vec3{0.2f, 0.0f, 0.0f}, 0.5f);
+    CHECK(s.size() == 1 && s[0] == near_id);
+    std::vector<pxg::BodyId> b = world.overlap_box(pxg::Vec3{0, 0, 0}, pxg::Vec3{1, 1, 1}, pxg::Quat{});
+    CHECK(b.size() == 1 && b[0] == near_id);
+    std::vector<pxg::BodyId> c = world.overlap_capsule(pxg::Vec3{0, 0, -0.4f}, pxg::Vec3{0, 0, 0.4f}, 0.3f);
+    CHECK(c.size() == 1 && c[0] == near_id);
+}
+
 int main() {
     struct T {
         const char* name;
@@ -706,6 +719,7 @@ int main() {
         {"bouncing_sphere_enters_and_exits_and_step_clears_the_queue",
          bouncing_sphere_enters_and_exits_and_step_clears_the_queue},
         {"raycasts_hit_boxes_with_exact_distances_and_filters", raycasts_hit_boxes_with_exact_distances_and_filters},
+        {"overlaps_find_only_touched_bodies", overlaps_find_only_touched_bodies},
     };
     for (const T& t : tests) {
         int before = g_failures;

@@ -1,7 +1,8 @@
-"""Scene raycasts on the GPU (pxg_raycast / pxg_raycast_any / pxg_raycast_all) against the CPU
-oracle: the reference's query tests (physxcoopa/tests/queries_test.cpp) through the C ABI, and
-bit-identical hits (distance, point, normal, body, shape) on settled piles, with the oracle given
-the GPU's poses and fat AABBs, up to the full 16k-body pile."""
+"""Scene queries on the GPU (pxg_raycast / pxg_raycast_any / pxg_raycast_all / pxg_overlap)
+against the CPU oracle: the reference's query tests (physxcoopa/tests/queries_test.cpp) through
+the C ABI, bit-identical raycast hits (distance, point, normal, body, shape) on settled piles up
+to the full 16k-body pile, and identical overlap body lists, with the oracle given the GPU's poses
+and fat AABBs."""
 import time
 
 import numpy as np
@@ -105,3 +106,30 @@ def test_raycast_full_pile(World, oracle):
     o = ow.raycast(r[sub])
     assert g[sub].tobytes() == o.tobytes()
     assert g["hit"].mean() > 0.9
+
+
+def test_overlaps_equal_oracle(World, oracle):
+    """overlap_sphere / overlap_box / overlap_capsule on a settled pile: the GPU's body lists equal
+    the oracle's (queries_test.cpp:100-123 semantics, slot order)."""
+    b, s = scenes.mixed_pile_scene(n=300, seed=6)
+    cfg = _abi.default_config()
+    gw = World(cfg, b, s)
+    for _ in range(80):
+        gw.step_fixed()
+    ow = oracle_from(oracle, gw, cfg, s)
+    rng = np.random.default_rng(6)
+    total = 0
+    for k in range(300):
+        c = rng.uniform([-5, -5, 0], [5, 5, 3])
+        if k % 3 == 0:
+            q = _abi.overlap_sphere(c, rng.uniform(0.2, 1.5))
+        elif k % 3 == 1:
+            q = _abi.overlap_box(c, rng.uniform(0.2, 1.2, 3), scenes.random_quaternion(rng))
+        else:
+            q = _abi.overlap_capsule(c, c + rng.uniform(-1, 1, 3), rng.uniform(0.1, 0.6))
+        f = _abi.query_filter(include_triggers=bool(k % 2))
+        g, o = gw.overlap(q, f), ow.overlap(q, f)
+        assert np.array_equal(g, o), (k, g, o)
+        total += len(g)
+    print("overlaps: %d bodies over 300 queries" % total, flush=True)
+    assert total > 300

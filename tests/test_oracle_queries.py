@@ -233,3 +233,52 @@ def test_world_raycast_is_the_nearest_of_all_hits(oracle):
         np.testing.assert_allclose(allh["point"], r[i]["origin"] + allh["distance"][:, None] * r[i]["direction"],
                                    atol=1e-5)
     assert n_hit > 40
+
+
+def test_overlaps_find_only_touched_bodies(oracle):
+    # queries_test.cpp:100-123
+    w, (near, far) = static_box_world(oracle, [(0.0, 0.0, 0.0), (20.0, 0.0, 0.0)])
+    assert list(w.overlap(_abi.overlap_sphere((0.2, 0, 0), 0.5))) == [near]
+    assert list(w.overlap(_abi.overlap_box((0, 0, 0), (1, 1, 1)))) == [near]
+    assert list(w.overlap(_abi.overlap_capsule((0, 0, -0.4), (0, 0, 0.4), 0.3))) == [near]
+    assert list(w.overlap(_abi.overlap_sphere((10.0, 0, 0), 1.0))) == []
+    assert list(w.overlap(_abi.overlap_sphere((0.2, 0, 0), 0.5), _abi.query_filter(ignore_body=near))) == []
+
+
+def test_sphere_overlaps_match_distance_closed_forms(oracle):
+    """A sphere query against spheres and boxes: overlap iff the centre distance to the shape is
+    at most the query radius (float64 closed forms: |c - c_i| <= r + r_i for spheres, the clamped
+    box distance for boxes), away from the boundary."""
+    rng = np.random.default_rng(21)
+    sb = scenes.SceneBuilder()
+    geo = []
+    for k in range(60):
+        c = rng.uniform(-6, 6, 3)
+        q = scenes.random_quaternion(rng)
+        i = sb.add_body(PXG_STATIC, c, q)
+        if k % 2:
+            r = rng.uniform(0.2, 0.8)
+            sb.add_shape(i, PXG_SPHERE, radius=r)
+            geo.append(("s", c, r))
+        else:
+            he = rng.uniform(0.2, 0.8, 3)
+            sb.add_shape(i, PXG_BOX, half_extents=he)
+            geo.append(("b", c, quat_matrix(np.asarray(q, np.float64)), he))
+    b, s = sb.arrays()
+    w = oracle.OracleWorld(_abi.default_config(), b, s)
+    checked = 0
+    for _ in range(200):
+        qc, qr = rng.uniform(-6, 6, 3), rng.uniform(0.3, 2.0)
+        qc32, qr32 = qc.astype(np.float32), np.float32(qr)
+        got = set(int(x) for x in w.overlap(_abi.overlap_sphere(qc32, qr32)))
+        for i, g in enumerate(geo):
+            if g[0] == "s":
+                dist = np.linalg.norm(qc32.astype(np.float64) - g[1]) - g[2]
+            else:
+                lp = (qc32.astype(np.float64) - g[1]) @ g[2]
+                dist = np.linalg.norm(lp - np.clip(lp, -g[3], g[3]))
+            if abs(dist - qr) < 1e-4:
+                continue
+            assert (i in got) == (dist <= qr), (i, dist, qr)
+            checked += 1
+    assert checked > 10000
