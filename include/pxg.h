@@ -252,7 +252,8 @@ typedef struct pxg_query_filter {
     uint32_t ignore_body;      /* PXG_NO_BODY: none */
     uint32_t reserved;
 } pxg_query_filter;
-/* RaycastHit (query/queries.h:56-76): hit = 0 when the ray hit nothing (the rest is then zero). */
+/* RaycastHit (query/queries.h:56-76): hit = 0 when nothing was hit (the rest is then zero);
+ * started_inside: shape casts only, the cast shape already overlapped the target at its start. */
 typedef struct pxg_raycast_hit {
     float point[3];
     float distance;
@@ -260,8 +261,20 @@ typedef struct pxg_raycast_hit {
     uint32_t hit;
     uint32_t body;
     uint32_t shape_index;
-    uint32_t reserved[2];
+    uint32_t started_inside;
+    uint32_t reserved;
 } pxg_raycast_hit;
+/* One shape cast (shape_cast(), world.h:877-909): the shape (sphere or capsule; its local centre,
+ * local rotation, capsule axis and dimensions are used, owner / material / layer are not) posed at
+ * position / orientation, swept along direction (normalised by the library) up to max_distance. */
+typedef struct pxg_cast_query {
+    pxg_shape shape;
+    float position[3];
+    float max_distance;
+    float orientation[4];
+    float direction[3];
+    float reserved;
+} pxg_cast_query;
 
 /* The query shape of overlap_sphere / overlap_box / overlap_capsule (world.h:1002-1060). */
 typedef struct pxg_overlap_query {
@@ -365,6 +378,12 @@ int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query_filter* fi
  * min(cap, n) are copied. */
 int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_filter* filter, uint32_t* bodies,
                 uint32_t cap, uint32_t* n);
+/* The first body a swept sphere or capsule touches (shape_cast() / sphere_cast() / capsule_cast(),
+ * world.h:877-950; query/sweep.h:422-505: conservative advancement on the exact closest distance,
+ * a start overlap reported at distance 0 only when the sweep leads into the target). hit->hit = 0
+ * when nothing is touched. Equal distances go to the lower shape slot. Box casts return
+ * PXG_ERR_INVALID (the swept separating-axis test they need is not built). */
+int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const pxg_query_filter* filter, pxg_raycast_hit* hit);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);
 

@@ -129,6 +129,7 @@ struct RaycastHit {
     float distance = 0.0f;
     BodyId body;
     uint32_t shape_index = 0xFFFFFFFFu;
+    bool started_inside = false;  // shape casts only
 };
 struct QueryFilter {
     uint32_t layer_mask = ~0u;
@@ -544,6 +545,25 @@ public:
         return overlap(q, filter);
     }
 
+    // The first body a swept sphere / capsule touches (sphere_cast(), capsule_cast(),
+    // world.h:911-950). Box casts are not offered.
+    bool sphere_cast(Vec3 origin, float radius, Vec3 dir, float max_distance, RaycastHit& hit,
+                     const QueryFilter& filter = QueryFilter()) {
+        pxg_cast_query c = cast_base(max_distance, origin, Quat{}, dir);
+        c.shape.type = PXG_SPHERE;
+        c.shape.radius = radius;
+        return shape_cast(c, hit, filter);
+    }
+    bool capsule_cast(Vec3 origin, float radius, float half_height, int direction_axis, Quat orientation, Vec3 dir,
+                      float max_distance, RaycastHit& hit, const QueryFilter& filter = QueryFilter()) {
+        pxg_cast_query c = cast_base(max_distance, origin, orientation, dir);
+        c.shape.type = PXG_CAPSULE;
+        c.shape.capsule_radius = radius;
+        c.shape.capsule_half_height = half_height;
+        c.shape.capsule_axis = direction_axis;
+        return shape_cast(c, hit, filter);
+    }
+
     // The closest hits of many rays in one GPU pass (no counterpart in the reference, which casts
     // one ray per call): hits[i] is valid where found[i] != 0.
     void raycast_batch(const std::vector<Ray>& rays, std::vector<RaycastHit>& hits, std::vector<uint8_t>& found,
@@ -689,6 +709,30 @@ private:
         return bodies_[i].alive != 0;
     }
 
+    static pxg_cast_query cast_base(float max_distance, Vec3 origin, Quat orientation, Vec3 dir) {
+        pxg_cast_query c;
+        std::memset(&c, 0, sizeof(c));
+        c.shape = no_shape();
+        c.shape.enabled = 1;
+        set3(c.position, origin);
+        c.orientation[0] = orientation.x;
+        c.orientation[1] = orientation.y;
+        c.orientation[2] = orientation.z;
+        c.orientation[3] = orientation.w;
+        set3(c.direction, dir);
+        c.max_distance = max_distance;
+        return c;
+    }
+    bool shape_cast(const pxg_cast_query& c, RaycastHit& hit, const QueryFilter& filter) {
+        push();
+        pxg_query_filter f = to_filter(filter);
+        pxg_raycast_hit raw;
+        check(pxg_shape_cast(w_, &c, &f, &raw), "shape_cast");
+        if (!raw.hit) return false;
+        hit = from_hit(raw);
+        hit.started_inside = raw.started_inside != 0;
+        return true;
+    }
     static pxg_overlap_query overlap_base(uint32_t type) {
         pxg_overlap_query q;
         std::memset(&q, 0, sizeof(q));

@@ -54,7 +54,8 @@ def lib():
             "pxo_atan2": ([f32, f32, u32], f32),
             "pxo_get_events": ([vp, vp, u32], u32), "pxo_clear_events": ([vp], None),
             "pxo_raycast": ([vp, vp, u32, vp, vp], None), "pxo_raycast_all": ([vp, vp, vp, vp, u32], u32),
-            "pxo_ray_shape": ([vp, vp, vp, vp, vp, vp], i32), "pxo_overlap": ([vp, vp, vp, vp, u32], u32), "pxo_ray_aabb": ([vp, vp, vp], i32),
+            "pxo_ray_shape": ([vp, vp, vp, vp, vp, vp], i32), "pxo_overlap": ([vp, vp, vp, vp, u32], u32),
+            "pxo_shape_cast": ([vp, vp, vp, vp], i32), "pxo_ray_aabb": ([vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -177,6 +178,17 @@ class OracleWorld:
         f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter))
         lib().pxo_raycast(self.h, ptr(r), len(r), f, ptr(out))
         return out[:len(r)]
+
+    def shape_cast(self, cast, query_filter=None):
+        """shape_cast() for a sphere or capsule (CAST_DTYPE in): the hit record (hit = 0: none)."""
+        from physxcoopa_amd._abi import HIT_DTYPE
+        c = np.ascontiguousarray(cast).reshape(1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter))
+        out = np.zeros(1, HIT_DTYPE)
+        r = lib().pxo_shape_cast(self.h, ptr(c), f, ptr(out))
+        if r < 0:
+            raise ValueError("box casts are not supported")
+        return out[0]
 
     def overlap(self, query, query_filter=None):
         """Bodies with a shape overlapping the query shape, in shape-slot order (world.h:1002-1060)."""

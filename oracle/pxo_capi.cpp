@@ -310,6 +310,26 @@ uint32_t pxo_overlap(void* p, const pxg_overlap_query* q, const pxg_query_filter
     if (h->dbl) go(h->d); else go(h->f);
     return n;
 }
+// shape_cast() for a sphere or capsule (returns 1 on a hit; -1 for a box cast)
+int pxo_shape_cast(void* p, const pxg_cast_query* c, const pxg_query_filter* f, pxg_raycast_hit* out) {
+    Handle* h = static_cast<Handle*>(p);
+    std::memset(out, 0, sizeof(*out));
+    if (c->shape.type == PXG_BOX) return -1;
+    int r = 0;
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        Shape<T> s = shape_from<T>(c->shape);
+        RayHit<T> hit;
+        bool started = false;
+        if (!world_shape_cast(w, s, v3<T>(c->position), q4<T>(c->orientation), v3<T>(c->direction), T(c->max_distance),
+                              filter_from(f), hit, started)) return;
+        hit_to(hit, *out);
+        out->started_inside = started ? 1u : 0u;
+        r = 1;
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+    return r;
+}
 // raycast_shape for one shape at a pose (float)
 int pxo_ray_shape(const pxg_shape* s, const float* pos, const float* rot, const pxg_ray* ray, float* t, float* n3) {
     Shape<float> sh = shape_from<float>(*s);

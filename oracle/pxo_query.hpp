@@ -257,4 +257,264 @@ template <class T> std::vector<uint32_t> world_overlap(World<T>& w, const Overla
     return out;
 }
 
+// ---------------- shape casts: sphere and capsule (query/sweep.h, world.h:877-950) ----------------
+constexpr float k_sweep_tolerance = 1e-4f;  // sweep.h:68
+
+template <class T> struct CastGeom {  // sweep.h:86-121: core segment (a == b for a sphere) + radius, or the box
+    int type = PXG_SPHERE;
+    V3<T> a, b;
+    T r = T(0);
+    OBB<T> box;
+};
+template <class T> CastGeom<T> cast_geometry(const Shape<T>& s, const V3<T>& pos, const Q<T>& rot) {
+    CastGeom<T> g;
+    g.type = s.type;
+    if (s.type == PXG_SPHERE) {
+        Sph<T> sp = world_sphere(s, pos, rot);
+        g.a = g.b = sp.c;
+        g.r = sp.r;
+    } else if (s.type == PXG_CAPSULE) {
+        Cap<T> c = world_capsule(s, pos, rot);
+        g.a = c.a; g.b = c.b; g.r = c.r;
+    } else {
+        g.box = world_obb(s, pos, rot);
+    }
+    return g;
+}
+
+template <class T> inline T vc(const V3<T>& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+template <class T> inline void vset(V3<T>& v, int i, T x) { if (i == 0) v.x = x; else if (i == 1) v.y = x; else v.z = x; }
+
+// closest_points_segment_box (collision/distance.h:41-102): exact, over the pieces between the
+// segment's slab-plane crossings; returns the squared distance
+template <class T> T closest_seg_box(const V3<T>& p0, const V3<T>& p1, const OBB<T>& box, V3<T>& on_seg, V3<T>& on_box) {
+    V3<T> ax[3];
+    obb_axes(box, ax);
+    const V3<T> rel0 = p0 - box.c, rel1 = p1 - box.c;
+    V3<T> o, d;
+    for (int i = 0; i < 3; ++i) {
+        vset(o, i, dot(rel0, ax[i]));
+        vset(d, i, dot(rel1, ax[i]) - vc(o, i));
+    }
+    const V3<T>& h = box.he;
+    T breaks[8];
+    int count = 0;
+    breaks[count++] = T(0);
+    breaks[count++] = T(1);
+    for (int i = 0; i < 3; ++i) {
+        if (std::abs(vc(d, i)) < T(1e-12f)) continue;
+        const T planes[2] = {-vc(h, i), vc(h, i)};
+        for (int k = 0; k < 2; ++k) {
+            T sv = (planes[k] - vc(o, i)) / vc(d, i);
+            if (sv > T(0) && sv < T(1)) breaks[count++] = sv;
+        }
+    }
+    std::sort(breaks, breaks + count);
+    const V3<T> nh = -h;
+    auto sq_at = [&](T sv) {
+        V3<T> p = o + d * sv;
+        V3<T> ex = p - V3<T>(gclamp(p.x, nh.x, h.x), gclamp(p.y, nh.y, h.y), gclamp(p.z, nh.z, h.z));
+        return dot(ex, ex);
+    };
+    T best_s = T(0);
+    T best = sq_at(T(0));
+    for (int k = 0; k + 1 < count; ++k) {
+        T s0 = breaks[k], s1 = breaks[k + 1];
+        if (s1 - s0 < T(1e-9f)) continue;
+        V3<T> mid = o + d * (T(0.5f) * (s0 + s1));
+        T a = T(0), b = T(0);
+        for (int i = 0; i < 3; ++i) {
+            T c;
+            if (vc(mid, i) > vc(h, i)) c = vc(h, i);
+            else if (vc(mid, i) < -vc(h, i)) c = -vc(h, i);
+            else continue;
+            a += vc(d, i) * vc(d, i);
+            b += vc(d, i) * (vc(o, i) - c);
+        }
+        T sv = a > T(1e-12f) ? sclamp(-b / a, s0, s1) : s0;
+        T v = sq_at(sv);
+        if (v < best) { best = v; best_s = sv; }
+    }
+    T end_v = sq_at(T(1));
+    if (end_v < best) { best = end_v; best_s = T(1); }
+    V3<T> local = o + d * best_s;
+    V3<T> cl(gclamp(local.x, nh.x, h.x), gclamp(local.y, nh.y, h.y), gclamp(local.z, nh.z, h.z));
+    on_seg = p0 + (p1 - p0) * best_s;
+    on_box = box.c + ax[0] * cl.x + ax[1] * cl.y + ax[2] * cl.z;
+    return best;
+}
+
+// push_axis_, project_obb_, segment_box_mtv_ (sweep.h:215-230, 304-338)
+template <class T> inline void push_axis(V3<T>* axes, int& count, const V3<T>& L) {
+    T len2 = dot(L, L);
+    if (len2 < T(1e-10f)) return;
+    axes[count++] = L / std::sqrt(len2);
+}
+template <class T> inline void project_obb(const OBB<T>& box, const V3<T> bx[3], const V3<T>& L, T& lo, T& hi) {
+    T c = dot(box.c, L);
+    T r = box.he.x * std::abs(dot(bx[0], L)) + box.he.y * std::abs(dot(bx[1], L)) + box.he.z * std::abs(dot(bx[2], L));
+    lo = c - r;
+    hi = c + r;
+}
+template <class T> void segment_box_mtv(const V3<T>& p0, const V3<T>& p1, T radius, const OBB<T>& box, V3<T>& normal,
+                                        T& depth) {
+    V3<T> bx[3];
+    obb_axes(box, bx);
+    V3<T> axes[6];
+    int count = 0;
+    for (int i = 0; i < 3; ++i) push_axis(axes, count, bx[i]);
+    for (int i = 0; i < 3; ++i) push_axis(axes, count, cross(p1 - p0, bx[i]));
+    depth = std::numeric_limits<T>::max();
+    normal = V3<T>(T(0), T(0), T(1));
+    for (int k = 0; k < count; ++k) {
+        const V3<T>& L = axes[k];
+        T s0 = dot(p0, L), s1 = dot(p1, L);
+        T b0, b1;
+        project_obb(box, bx, L, b0, b1);
+        T lo = gmin(s0, s1), hi = gmax(s0, s1);
+        T overlap = gmin(hi - b0, b1 - lo) + radius;
+        if (overlap < depth) {
+            depth = overlap;
+            normal = (lo + hi) >= (b0 + b1) ? L : -L;
+        }
+    }
+}
+
+// convex_penetration for a sphere or capsule probe (sweep.h:339-384)
+template <class T> bool convex_penetration(const Shape<T>& probe, const V3<T>& pos, const Q<T>& rot, const Shape<T>& target,
+                                           const V3<T>& tpos, const Q<T>& trot, V3<T>& normal, T& depth, V3<T>& point) {
+    if (probe.type == PXG_CAPSULE && target.type == PXG_BOX) {
+        CastGeom<T> core = cast_geometry(probe, pos, rot);
+        OBB<T> box = world_obb(target, tpos, trot);
+        V3<T> on_seg, on_box;
+        T dist = std::sqrt(closest_seg_box(core.a, core.b, box, on_seg, on_box));
+        if (dist >= core.r) return false;
+        if (dist > k_epsilon<T>()) {
+            normal = (on_seg - on_box) / dist;
+            depth = core.r - dist;
+        } else {
+            segment_box_mtv(core.a, core.b, core.r, box, normal, depth);
+        }
+        point = on_box;
+        return depth > T(0);
+    }
+    Manifold<T> m;
+    if (!generate_contacts(0u, probe, pos, rot, 0u, target, tpos, trot, m, false)) return false;
+    normal = -m.n;
+    depth = T(0);
+    V3<T> sum(T(0), T(0), T(0));
+    for (int k = 0; k < m.count; ++k) {
+        depth = gmax(depth, m.p[k].pen);
+        sum += m.p[k].pos;
+    }
+    point = m.count > 0 ? sum / T(m.count) : tpos;
+    return depth > T(0);
+}
+
+template <class T> struct SweepHit {
+    T distance = T(0);
+    V3<T> normal{T(0), T(0), T(1)}, point{T(0), T(0), T(0)};
+    bool started_inside = false;
+};
+
+// sweep_convex for a sphere or capsule cast shape against a sphere, box or capsule target
+// (sweep.h:149-168 advance_, 408-416 resolve_at_start_, 422-505)
+template <class T> bool sweep_round(const Shape<T>& shape, const V3<T>& pos, const Q<T>& rot, const V3<T>& dir, T maxd,
+                                    const Shape<T>& target, const V3<T>& tpos, const Q<T>& trot, SweepHit<T>& out) {
+    const CastGeom<T> cast = cast_geometry(shape, pos, rot);
+    T toi = T(0);
+    V3<T> n(T(0), T(0), T(1)), p(T(0), T(0), T(0));
+    auto dist_at = [&](T t, V3<T>& normal, V3<T>& point) {
+        V3<T> a = cast.a + dir * t, b = cast.b + dir * t;
+        V3<T> on_cast, on_target;
+        T target_radius = T(0);
+        if (target.type == PXG_BOX) {
+            closest_seg_box(a, b, world_obb(target, tpos, trot), on_cast, on_target);
+        } else {
+            CastGeom<T> core = cast_geometry(target, tpos, trot);
+            closest_points_seg_seg(a, b, core.a, core.b, on_cast, on_target);
+            target_radius = core.r;
+        }
+        V3<T> diff = on_cast - on_target;
+        T dist = length(diff);
+        if (dist > k_epsilon<T>()) normal = diff / dist;
+        point = on_target + normal * target_radius;
+        return dist - cast.r - target_radius;
+    };
+    // advance_
+    T t = T(0);
+    T dd = dist_at(t, n, p);
+    bool at_start = dd <= T(k_sweep_tolerance);
+    if (!at_start) {
+        bool hit = false;
+        for (int iter = 0; iter < 64; ++iter) {
+            T closing = -dot(dir, n);
+            if (closing <= T(1e-6f)) return false;
+            t += dd / closing;
+            if (t > maxd) return false;
+            dd = dist_at(t, n, p);
+            if (dd <= T(k_sweep_tolerance)) { hit = true; break; }
+        }
+        (void)hit;
+        toi = t;
+        out.distance = toi;
+        out.normal = n;
+        out.point = p;
+        out.started_inside = false;
+        return true;
+    }
+    V3<T> normal = n, point = p, pen_n, pen_p;
+    T depth = T(0), pen_d = T(0);
+    if (convex_penetration(shape, pos, rot, target, tpos, trot, pen_n, pen_d, pen_p)) {
+        normal = pen_n;
+        point = pen_p;
+        depth = pen_d;
+    }
+    if (dot(dir, normal) >= T(-1e-4f)) return false;  // resolve_at_start_
+    out.distance = T(0);
+    out.normal = normal;
+    out.point = point;
+    out.started_inside = depth > T(k_sweep_tolerance);
+    return true;
+}
+
+// shape_cast (world.h:877-909) for a sphere or capsule: the swept AABB against the fat AABBs, the
+// filter, then each candidate's sweep; the nearest wins (lower slot on ties: the reference's tie
+// order is its tree order)
+template <class T> bool world_shape_cast(World<T>& w, const Shape<T>& shape, const V3<T>& pos, const Q<T>& rot,
+                                         const V3<T>& dir, T maxd, const QueryFilter& f, RayHit<T>& hit, bool& started) {
+    if (!(maxd >= T(0))) return false;
+    T dir_len = length(dir);
+    if (dir_len < T(1e-8f)) return false;
+    const V3<T> ud = dir / dir_len;
+    AABB<T> b0 = world_bounds(shape, pos, rot), b1 = world_bounds(shape, pos + ud * maxd, rot);
+    AABB<T> swept;
+    swept.mn = vmin(b0.mn, b1.mn);
+    swept.mx = vmax(b0.mx, b1.mx);
+    bool found = false;
+    T best = maxd;
+    for (uint32_t i = 0; i < w.shapes.size(); ++i) {
+        const Shape<T>& s = w.shapes[i];
+        if (i >= w.proxy.size() || w.proxy[i] < 0 || s.owner >= w.bodies.size() || !w.bodies[s.owner].alive) continue;
+        if (!((f.layer_mask >> (s.layer & 31u)) & 1u)) continue;
+        if (!f.include_triggers && s.trigger) continue;
+        if (f.ignore != ~0u && s.owner == f.ignore) continue;
+        if (!w.fat(i).overlaps(swept)) continue;
+        if (found && best <= T(0)) continue;
+        const Body<T>& b = w.bodies[s.owner];
+        SweepHit<T> r;
+        if (!sweep_round(shape, pos, rot, ud, best, s, b.pos, b.q, r)) continue;
+        if (found && r.distance >= best) continue;
+        best = r.distance;
+        hit.point = r.point;
+        hit.normal = r.normal;
+        hit.distance = r.distance;
+        hit.body = s.owner;
+        hit.shape = i;
+        started = r.started_inside;
+        found = true;
+    }
+    return found;
+}
+
 }  // namespace pxo

@@ -66,7 +66,7 @@ PXG_NO_BODY = 0xFFFFFFFF
 FILTER_DTYPE = np.dtype([("layer_mask", "<u4"), ("include_triggers", "<u4"), ("ignore_body", "<u4"), ("reserved", "<u4")])
 assert FILTER_DTYPE.itemsize == 16
 HIT_DTYPE = np.dtype([("point", "<f4", 3), ("distance", "<f4"), ("normal", "<f4", 3), ("hit", "<u4"), ("body", "<u4"),
-                      ("shape_index", "<u4"), ("reserved", "<u4", 2)])
+                      ("shape_index", "<u4"), ("started_inside", "<u4"), ("reserved", "<u4")])
 assert HIT_DTYPE.itemsize == 48
 
 
@@ -101,6 +101,30 @@ def overlap_capsule(a, b, radius):
     q = np.zeros(1, OVERLAP_DTYPE)
     q["type"], q["a"], q["b"], q["radius"], q["orientation"] = PXG_CAPSULE, a, b, radius, (0, 0, 0, 1)
     return q
+
+
+CAST_DTYPE = np.dtype([("shape", SHAPE_DTYPE), ("position", "<f4", 3), ("max_distance", "<f4"), ("orientation", "<f4", 4),
+                       ("direction", "<f4", 3), ("reserved", "<f4")])
+assert CAST_DTYPE.itemsize == 144
+
+
+def sphere_cast(origin, radius, direction, max_distance):
+    """sphere_cast() (world.h:911-920) as a CAST_DTYPE record."""
+    c = np.zeros(1, CAST_DTYPE)
+    c["shape"]["type"], c["shape"]["enabled"], c["shape"]["radius"] = PXG_SPHERE, 1, radius
+    c["shape"]["local_rotation"] = (0, 0, 0, 1)
+    c["position"], c["orientation"], c["direction"], c["max_distance"] = origin, (0, 0, 0, 1), direction, max_distance
+    return c
+
+
+def capsule_cast(origin, radius, half_height, axis, orientation, direction, max_distance):
+    """capsule_cast() (world.h:938-950) as a CAST_DTYPE record."""
+    c = np.zeros(1, CAST_DTYPE)
+    sh = c["shape"]
+    sh["type"], sh["enabled"], sh["capsule_radius"], sh["capsule_half_height"] = PXG_CAPSULE, 1, radius, half_height
+    sh["capsule_axis"], sh["local_rotation"] = axis, (0, 0, 0, 1)
+    c["position"], c["orientation"], c["direction"], c["max_distance"] = origin, orientation, direction, max_distance
+    return c
 
 
 def query_filter(layer_mask=0xFFFFFFFF, include_triggers=True, ignore_body=PXG_NO_BODY):

@@ -487,7 +487,284 @@ __global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, Ov
     flags[j] = out;
 }
 
+// ---- shape casts: sphere and capsule (query/sweep.h:68-505, world.h:877-909) ----
+constexpr float kSweepTol = 1e-4f;  // sweep.h:68
+
+struct CastCore {  // sweep.h:86-121: core segment (a == b for a sphere) and radius
+    v3 a, b;
+    float r;
+};
+__device__ __forceinline__ CastCore cast_core(const ShapeGeom& s, v3 pos, quat rot) {
+    CastCore g;
+    if (s.type == SH_SPHERE) {
+        Sph sp = world_sphere(s, pos, rot);
+        g.a = g.b = sp.c;
+        g.r = sp.r;
+    } else {
+        Cap c = world_capsule(s, pos, rot);
+        g.a = c.a;
+        g.b = c.b;
+        g.r = c.r;
+    }
+    return g;
+}
+// closest_points_segment_box (collision/distance.h:41-102); returns the squared distance
+__device__ float closest_seg_box(v3 p0, v3 p1, const OBB& box, v3& on_seg, v3& on_box) {
+    v3 ax[3];
+    axes(box, ax);
+    const v3 rel0 = p0 - box.c, rel1 = p1 - box.c;
+    v3 o, d;
+    for (int i = 0; i < 3; ++i) {
+        setc(o, i, dot(rel0, ax[i]));
+        setc(d, i, dot(rel1, ax[i]) - comp(o, i));
+    }
+    const v3 h = box.he, nh = -box.he;
+    float br[8];
+    int count = 0;
+    br[count++] = 0.0f;
+    br[count++] = 1.0f;
+    for (int i = 0; i < 3; ++i) {
+        if (fabsf(comp(d, i)) < 1e-12f) continue;
+        for (int k = 0; k < 2; ++k) {
+            const float plane = k == 0 ? -comp(h, i) : comp(h, i);
+            const float sv = (plane - comp(o, i)) / comp(d, i);
+            if (sv > 0.0f && sv < 1.0f) br[count++] = sv;
+        }
+    }
+    for (int a = 1; a < count; ++a) {  // sort (equal values are interchangeable)
+        const float v = br[a];
+        int b = a;
+        while (b > 0 && br[b - 1] > v) {
+            br[b] = br[b - 1];
+            --b;
+        }
+        br[b] = v;
+    }
+    auto sq_at = [&](float sv) {
+        v3 p = o + d * sv;
+        v3 ex = p - mk(gclamp(p.x, nh.x, h.x), gclamp(p.y, nh.y, h.y), gclamp(p.z, nh.z, h.z));
+        return dot(ex, ex);
+    };
+    float best_s = 0.0f;
+    float best = sq_at(0.0f);
+    for (int k = 0; k + 1 < count; ++k) {
+        const float s0 = br[k], s1 = br[k + 1];
+        if (s1 - s0 < 1e-9f) continue;
+        const v3 mid = o + d * (0.5f * (s0 + s1));
+        float a = 0.0f, b = 0.0f;
+        for (int i = 0; i < 3; ++i) {
+            float c;
+            if (comp(mid, i) > comp(h, i)) c = comp(h, i);
+            else if (comp(mid, i) < -comp(h, i)) c = -comp(h, i);
+            else continue;
+            a += comp(d, i) * comp(d, i);
+            b += comp(d, i) * (comp(o, i) - c);
+        }
+        const float sv = a > 1e-12f ? sclamp(-b / a, s0, s1) : s0;
+        const float v = sq_at(sv);
+        if (v < best) {
+            best = v;
+            best_s = sv;
+        }
+    }
+    const float end_v = sq_at(1.0f);
+    if (end_v < best) {
+        best = end_v;
+        best_s = 1.0f;
+    }
+    const v3 local = o + d * best_s;
+    const v3 cl = mk(gclamp(local.x, nh.x, h.x), gclamp(local.y, nh.y, h.y), gclamp(local.z, nh.z, h.z));
+    on_seg = p0 + (p1 - p0) * best_s;
+    on_box = box.c + ax[0] * cl.x + ax[1] * cl.y + ax[2] * cl.z;
+    return best;
+}
+// push_axis_, project_obb_, segment_box_mtv_ (sweep.h:215-230, 304-338)
+__device__ __forceinline__ void push_axis(v3* axs, int& count, v3 L) {
+    const float len2 = dot(L, L);
+    if (len2 < 1e-10f) return;
+    axs[count++] = L / sqrtf(len2);
+}
+__device__ __forceinline__ void project_obb(const OBB& box, const v3 bx[3], v3 L, float& lo, float& hi) {
+    const float c = dot(box.c, L);
+    const float r = box.he.x * fabsf(dot(bx[0], L)) + box.he.y * fabsf(dot(bx[1], L)) + box.he.z * fabsf(dot(bx[2], L));
+    lo = c - r;
+    hi = c + r;
+}
+__device__ void segment_box_mtv(v3 p0, v3 p1, float radius, const OBB& box, v3& normal, float& depth) {
+    v3 bx[3];
+    axes(box, bx);
+    v3 axs[6];
+    int count = 0;
+    for (int i = 0; i < 3; ++i) push_axis(axs, count, bx[i]);
+    for (int i = 0; i < 3; ++i) push_axis(axs, count, cross(p1 - p0, bx[i]));
+    depth = 3.402823466e+38f;
+    normal = mk(0.0f, 0.0f, 1.0f);
+    for (int k = 0; k < count; ++k) {
+        const v3 L = axs[k];
+        const float s0 = dot(p0, L), s1 = dot(p1, L);
+        float b0, b1;
+        project_obb(box, bx, L, b0, b1);
+        const float lo = gmin(s0, s1), hi = gmax(s0, s1);
+        const float overlap = gmin(hi - b0, b1 - lo) + radius;
+        if (overlap < depth) {
+            depth = overlap;
+            normal = (lo + hi) >= (b0 + b1) ? L : -L;
+        }
+    }
+}
+// convex_penetration for a sphere or capsule probe (sweep.h:339-384)
+__device__ bool convex_penetration(const ShapeGeom& probe, v3 pos, quat rot, const ShapeGeom& target, v3 tpos, quat trot,
+                                   v3& normal, float& depth, v3& point) {
+    if (probe.type == SH_CAPSULE && target.type == SH_BOX) {
+        const CastCore core = cast_core(probe, pos, rot);
+        const OBB box = world_obb(target, tpos, trot);
+        v3 on_seg, on_box;
+        const float dist = sqrtf(closest_seg_box(core.a, core.b, box, on_seg, on_box));
+        if (dist >= core.r) return false;
+        if (dist > 1e-6f) {
+            normal = (on_seg - on_box) / dist;
+            depth = core.r - dist;
+        } else {
+            segment_box_mtv(core.a, core.b, core.r, box, normal, depth);
+        }
+        point = on_box;
+        return depth > 0.0f;
+    }
+    Man m;
+    if (!generate_contacts(probe, pos, rot, target, tpos, trot, false, m)) return false;
+    normal = -m.n;
+    depth = 0.0f;
+    v3 sum = mk(0.0f, 0.0f, 0.0f);
+    for (int k = 0; k < m.count; ++k) {
+        depth = gmax(depth, m.p[k].pen);
+        sum = sum + m.p[k].pos;
+    }
+    point = m.count > 0 ? sum / float(m.count) : tpos;
+    return depth > 0.0f;
+}
+// the gap between the cast core moved by dir * t and the target (sweep.h:467-483)
+__device__ __forceinline__ float cast_gap(const CastCore& cast, v3 dir, float t, const ShapeGeom& target, v3 tpos,
+                                          quat trot, v3& normal, v3& point) {
+    const v3 a = cast.a + dir * t, b = cast.b + dir * t;
+    v3 on_cast, on_target;
+    float target_radius = 0.0f;
+    if (target.type == SH_BOX) {
+        closest_seg_box(a, b, world_obb(target, tpos, trot), on_cast, on_target);
+    } else {
+        const CastCore core = cast_core(target, tpos, trot);
+        seg_seg(a, b, core.a, core.b, on_cast, on_target);
+        target_radius = core.r;
+    }
+    const v3 diff = on_cast - on_target;
+    const float dist = len(diff);
+    if (dist > 1e-6f) normal = diff / dist;
+    point = on_target + normal * target_radius;
+    return dist - cast.r - target_radius;
+}
+// sweep_convex for a sphere or capsule cast (sweep.h:149-168, 408-416, 422-505)
+__device__ bool sweep_round(const ShapeGeom& shape, v3 pos, quat rot, v3 dir, float maxd, const ShapeGeom& target,
+                            v3 tpos, quat trot, float& dist_out, v3& n_out, v3& p_out, bool& started) {
+    const CastCore cast = cast_core(shape, pos, rot);
+    v3 n = mk(0.0f, 0.0f, 1.0f), p = mk(0.0f, 0.0f, 0.0f);
+    float t = 0.0f;
+    float dd = cast_gap(cast, dir, t, target, tpos, trot, n, p);
+    if (!(dd <= kSweepTol)) {
+        for (int iter = 0; iter < 64; ++iter) {
+            const float closing = -dot(dir, n);
+            if (closing <= 1e-6f) return false;
+            t += dd / closing;
+            if (t > maxd) return false;
+            dd = cast_gap(cast, dir, t, target, tpos, trot, n, p);
+            if (dd <= kSweepTol) break;
+        }
+        dist_out = t;
+        n_out = n;
+        p_out = p;
+        started = false;
+        return true;
+    }
+    v3 normal = n, point = p, pen_n, pen_p;
+    float depth = 0.0f, pen_d = 0.0f;
+    if (convex_penetration(shape, pos, rot, target, tpos, trot, pen_n, pen_d, pen_p)) {
+        normal = pen_n;
+        point = pen_p;
+        depth = pen_d;
+    }
+    if (dot(dir, normal) >= -1e-4f) return false;
+    dist_out = 0.0f;
+    n_out = normal;
+    p_out = point;
+    started = depth > kSweepTol;
+    return true;
+}
+__device__ __forceinline__ ShapeGeom decode_shape(const DevState& d, uint32_t i, uint4 info) {
+    ShapeGeom s;
+    s.type = int(info.y & 15u);
+    s.caxis = int((info.y >> 8) & 3u);
+    const float4 lc = d.s_lc[i];
+    s.lc = mk(lc.x, lc.y, lc.z);
+    s.radius = lc.w;
+    s.lr = ldq(d.s_lr[i]);
+    const float4 dim = d.s_dim[i];
+    s.he = mk(dim.x, dim.y, dim.z);
+    s.cr = dim.w;
+    s.chh = d.s_mat[i].x;
+    return s;
+}
+// one thread per target shape: record (point, distance; normal, started; owner, slot) and the
+// nearest (distance, slot) key
+__global__ void k_shape_cast(DevState d, const uint4* qinfo, CastDev c, QueryFilterDev f, float4* rec,
+                             unsigned long long* best) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.ns) return;
+    const float dir_len = len(ld3(c.dir));
+    if (!(c.maxd >= 0.0f) || dir_len < 1e-8f) return;
+    const v3 ud = ld3(c.dir) / dir_len;
+    const v3 pos = ld3(c.pos);
+    const quat rot = ldq(c.rot);
+    v3 mn0, mx0, mn1, mx1;
+    world_bounds(c.shape, pos, rot, mn0, mx0);
+    world_bounds(c.shape, pos + ud * c.maxd, rot, mn1, mx1);
+    const v3 smn = vmin(mn0, mn1), smx = vmax(mx0, mx1);
+    const uint4 qi = qinfo[j];
+    if (!accepts(qi, f)) return;
+    const v3 fmn = ld3(d.s_fmin[j]), fmx = ld3(d.s_fmax[j]);
+    if (!(fmn.x <= smx.x && fmx.x >= smn.x && fmn.y <= smx.y && fmx.y >= smn.y && fmn.z <= smx.z && fmx.z >= smn.z)) return;
+    const uint4 info = d.s_info[j];
+    const ShapeGeom target = decode_shape(d, j, info);
+    const v3 tpos = ld3(d.b_pos[info.x]);
+    const quat trot = ldq(d.b_rot[info.x]);
+    float dist;
+    v3 n, p;
+    bool started;
+    if (!sweep_round(c.shape, pos, rot, ud, c.maxd, target, tpos, trot, dist, n, p, started)) return;
+    rec[3 * j] = st3(p, dist);
+    rec[3 * j + 1] = st3(n, __uint_as_float(started ? 1u : 0u));
+    rec[3 * j + 2] = make_float4(__uint_as_float(info.x), __uint_as_float(j), 0.0f, 0.0f);
+    atomicMin(best, (static_cast<unsigned long long>(f2o(dist)) << 32) | j);
+}
+// the nearest record (lower slot on equal distances) as the hit, or an empty hit
+__global__ void k_cast_pick(const float4* rec, const unsigned long long* best, float4* hit) {
+    const unsigned long long b = *best;
+    if (b == ~0ull) {
+        hit[0] = hit[1] = hit[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const uint32_t j = uint32_t(b & 0xFFFFFFFFull);
+    const float4 r1 = rec[3 * j + 1];
+    hit[0] = rec[3 * j];
+    hit[1] = make_float4(r1.x, r1.y, r1.z, __uint_as_float(1u));
+    const float4 r2 = rec[3 * j + 2];
+    hit[2] = make_float4(r2.x, r2.y, r1.w, 0.0f);
+}
+
 }  // namespace
+
+void launch_shape_cast(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
+                       unsigned long long* best, float4* hit, hipStream_t st) {
+    if (d.ns) hipLaunchKernelGGL(k_shape_cast, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, qinfo, c, f, rec, best);
+    hipLaunchKernelGGL(k_cast_pick, dim3(1), dim3(1), 0, st, rec, best, hit);
+}
 
 void launch_overlap(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev& q, const QueryFilterDev& f,
                     uint32_t* flags, hipStream_t st) {

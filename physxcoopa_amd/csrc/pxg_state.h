@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pxg_device.h"
+
 namespace pxg {
 
 struct MlpDev;
@@ -253,6 +255,15 @@ struct OverlapDev {
 };
 void launch_overlap(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev& q, const QueryFilterDev& f,
                     uint32_t* flags, hipStream_t st);
+// shape cast: the cast shape (sphere or capsule, decoded), its pose, direction and max distance
+struct CastDev {
+    pxd::ShapeGeom shape;
+    float4 pos, rot, dir;
+    float maxd;
+};
+// per-shape records (3 float4 per shape), the nearest key (reset to ~0 before), the hit (3 float4)
+void launch_shape_cast(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
+                       unsigned long long* best, float4* hit, hipStream_t st);
 size_t query_sort_bytes(uint32_t cap);
 size_t query_shape_record_bytes();
 void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);

@@ -83,6 +83,7 @@ struct pxg_world {
     float4* q_hits = nullptr;
     uint32_t* q_count = nullptr;
     uint32_t q_cap = 0;
+    unsigned long long* q_best = nullptr;  // shape casts: nearest (distance, slot) key
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -1070,6 +1071,47 @@ extern "C" int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const p
         ++count;
     }
     *n = count;
+    return PXG_OK;
+}
+
+extern "C" int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const pxg_query_filter* filter,
+                              pxg_raycast_hit* hit) {
+    if (!w || !cast || !hit) return fail(PXG_ERR_INVALID, "pxg_shape_cast: null argument");
+    std::memset(hit, 0, sizeof(*hit));
+    if (cast->shape.type == PXG_BOX)
+        return fail(PXG_ERR_INVALID, "pxg_shape_cast: box casts are not supported (sphere and capsule casts are)");
+    if (cast->shape.type != PXG_SPHERE && cast->shape.type != PXG_CAPSULE)
+        return fail(PXG_ERR_INVALID, "pxg_shape_cast: the cast shape is a sphere or a capsule");
+    // per-shape records in the hit buffer (3 float4 per shape), the hit itself after them
+    int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u) + 1u);
+    if (r != PXG_OK) return r;
+    if (!w->q_best) DALLOC(w->q_best, 1);
+    const pxg_shape& sh = cast->shape;
+    CastDev c;
+    c.shape.type = int(sh.type);
+    c.shape.caxis = sh.capsule_axis;
+    c.shape.lc.x = sh.local_center[0];
+    c.shape.lc.y = sh.local_center[1];
+    c.shape.lc.z = sh.local_center[2];
+    c.shape.radius = sh.radius;
+    c.shape.lr.x = sh.local_rotation[0];
+    c.shape.lr.y = sh.local_rotation[1];
+    c.shape.lr.z = sh.local_rotation[2];
+    c.shape.lr.w = sh.local_rotation[3];
+    c.shape.he.x = sh.half_extents[0];
+    c.shape.he.y = sh.half_extents[1];
+    c.shape.he.z = sh.half_extents[2];
+    c.shape.cr = sh.capsule_radius;
+    c.shape.chh = sh.capsule_half_height;
+    c.pos = make_float4(cast->position[0], cast->position[1], cast->position[2], 0.0f);
+    c.rot = make_float4(cast->orientation[0], cast->orientation[1], cast->orientation[2], cast->orientation[3]);
+    c.dir = make_float4(cast->direction[0], cast->direction[1], cast->direction[2], 0.0f);
+    c.maxd = cast->max_distance;
+    launch_fill_u64(w->q_best, ~0ull, 1, w->stream);
+    float4* out = w->q_hits + 3 * size_t(w->q_cap - 1);
+    launch_shape_cast(w->d, w->q.info, c, query_filter(filter), w->q_hits, w->q_best, out, w->stream);
+    HIPCHK(hipMemcpyAsync(hit, out, sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
     return PXG_OK;
 }
 

@@ -149,6 +149,15 @@ class World:
         check(lib().pxg_raycast_any(self.h, ptr(r), len(r), f, ptr(out)), "pxg_raycast_any")
         return out[:len(r)].astype(bool)
 
+    def shape_cast(self, cast, query_filter=None):
+        """The first body a swept sphere or capsule touches (CAST_DTYPE in; world.h:877-950): a
+        HIT_DTYPE record, hit = 0 when none."""
+        c = np.ascontiguousarray(cast, dtype=_abi.CAST_DTYPE).reshape(1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        out = np.zeros(1, _abi.HIT_DTYPE)
+        check(lib().pxg_shape_cast(self.h, ptr(c), f, ptr(out)), "pxg_shape_cast")
+        return out[0]
+
     def overlap(self, query, query_filter=None):
         """Bodies with a shape overlapping the query shape (OVERLAP_DTYPE; world.h:1002-1060), once
         per overlapping shape, in shape-slot order."""

@@ -688,6 +688,30 @@ void overlaps_find_only_touched_bodies() {
     CHECK(c.size() == 1 && c[0] == near_id);
 }
 
+// queries_test.cpp:132-174: a sphere cast meets a box corner at 4.5 - sqrt(0.07); a capsule
+// cast meets a Y-axis capsule at 2.6
+void sweeps_are_exact_against_box_corner_and_capsule() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId box_id = world.add_body(body(PXG_STATIC, 5.0f, 0, 0), box_shape(0.5f, 0.5f, 0.5f));
+    pxg::RaycastHit hit;
+    CHECK(world.sphere_cast(pxg::Vec3{0.0f, 0.8f, 0.8f}, 0.5f, pxg::Vec3{1, 0, 0}, 100.0f, hit));
+    CHECK_NEAR(hit.distance, 4.5f - std::sqrt(0.07f), 1e-3f);
+    CHECK_NEAR(hit.point.x, 4.5f, 1e-3f);
+    CHECK(hit.body == box_id && !hit.started_inside);
+    pxg::PhysicsWorld capsules;
+    pxg_shape cap = pxg::no_shape();
+    cap.enabled = 1;
+    cap.type = PXG_CAPSULE;
+    cap.capsule_radius = 0.2f;
+    cap.capsule_half_height = 1.0f;
+    cap.capsule_axis = 1;
+    capsules.add_body(body(PXG_STATIC, 3.0f, 0.0f, 0.8f), cap);
+    CHECK(capsules.capsule_cast(pxg::Vec3{0, 0, 0}, 0.3f, 0.5f, 2, pxg::Quat{}, pxg::Vec3{1, 0, 0}, 100.0f, hit));
+    CHECK_NEAR(hit.distance, 2.6f, 1e-3f);
+    CHECK_NEAR(hit.normal.x, -0.8f, 1e-2f);
+    CHECK(!capsules.capsule_cast(pxg::Vec3{0, 0, -0.65f}, 0.3f, 0.5f, 2, pxg::Quat{}, pxg::Vec3{1, 0, 0}, 100.0f, hit));
+}
+
 int main() {
     struct T {
         const char* name;
@@ -720,6 +744,7 @@ int main() {
          bouncing_sphere_enters_and_exits_and_step_clears_the_queue},
         {"raycasts_hit_boxes_with_exact_distances_and_filters", raycasts_hit_boxes_with_exact_distances_and_filters},
         {"overlaps_find_only_touched_bodies", overlaps_find_only_touched_bodies},
+        {"sweeps_are_exact_against_box_corner_and_capsule", sweeps_are_exact_against_box_corner_and_capsule},
     };
     for (const T& t : tests) {
         int before = g_failures;

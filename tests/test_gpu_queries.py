@@ -133,3 +133,51 @@ def test_overlaps_equal_oracle(World, oracle):
         total += len(g)
     print("overlaps: %d bodies over 300 queries" % total, flush=True)
     assert total > 300
+
+
+def test_reference_sweep_tests_on_gpu(World):
+    # queries_test.cpp:132-174, 224-246 (sphere and capsule casts)
+    sb = scenes.SceneBuilder()
+    box = sb.add_body(PXG_STATIC, (5.0, 0, 0))
+    sb.add_shape(box, PXG_BOX, half_extents=(0.5, 0.5, 0.5))
+    b, s = sb.arrays()
+    w = World(_abi.default_config(), b, s)
+    h = w.shape_cast(_abi.sphere_cast((0.0, 0.8, 0.8), 0.5, (1, 0, 0), 100.0))
+    assert h["hit"] == 1 and h["body"] == box and abs(h["distance"] - (4.5 - 0.07 ** 0.5)) < 1e-3
+    np.testing.assert_allclose(h["point"], [4.5, 0.5, 0.5], atol=1e-3)
+    h = w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (1, 0, 0), 2.0))
+    assert h["hit"] == 1 and h["started_inside"] == 1 and abs(h["distance"]) < 1e-6
+    assert w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (-1, 0, 0), 2.0))["hit"] == 0
+    from physxcoopa_amd._lib import PxgError
+    with pytest.raises(PxgError, match="box casts"):
+        c = _abi.sphere_cast((0, 0, 0), 0.5, (1, 0, 0), 10.0)
+        c["shape"]["type"] = PXG_BOX
+        w.shape_cast(c)
+
+
+def test_shape_casts_equal_oracle(World, oracle):
+    """Sphere and capsule casts from above and beside a settled pile: the GPU's hit records equal
+    the oracle's bit for bit (conservative advancement, start-overlap rules, nearest target)."""
+    b, s = scenes.mixed_pile_scene(n=300, seed=8)
+    cfg = _abi.default_config()
+    gw = World(cfg, b, s)
+    for _ in range(80):
+        gw.step_fixed()
+    ow = oracle_from(oracle, gw, cfg, s)
+    rng = np.random.default_rng(8)
+    hits = started = 0
+    for k in range(400):
+        o = rng.uniform([-5, -5, 0.2], [5, 5, 6])
+        d = rng.normal(size=3)
+        d[2] -= 1.0
+        if k % 2 == 0:
+            c = _abi.sphere_cast(o, rng.uniform(0.1, 0.6), d, rng.uniform(1, 12))
+        else:
+            c = _abi.capsule_cast(o, rng.uniform(0.1, 0.4), rng.uniform(0.1, 0.6), int(rng.integers(0, 3)),
+                                  scenes.random_quaternion(rng), d, rng.uniform(1, 12))
+        g, oo = gw.shape_cast(c), ow.shape_cast(c)
+        assert g.tobytes() == oo.tobytes(), (k, g, oo)
+        hits += int(g["hit"])
+        started += int(g["started_inside"])
+    print("casts: %d of 400 hit, %d started inside" % (hits, started), flush=True)
+    assert hits > 150

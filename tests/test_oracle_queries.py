@@ -3,6 +3,8 @@
 an independent method (the first root of the shape's signed distance along the ray, found by
 sampling and bisection in float64), the inside-start rules of geometry/ray.h, the slab test the
 trees cull with, and the QueryFilter fields."""
+import math
+
 import numpy as np
 import pytest
 
@@ -282,3 +284,118 @@ def test_sphere_overlaps_match_distance_closed_forms(oracle):
             assert (i in got) == (dist <= qr), (i, dist, qr)
             checked += 1
     assert checked > 10000
+
+
+# --- shape casts (sphere and capsule) ---
+
+def test_sweeps_are_exact_against_box_corner_and_capsule(oracle):
+    # queries_test.cpp:132-174 (the sphere and capsule parts; box casts are not built)
+    w, (box,) = static_box_world(oracle, [(5.0, 0.0, 0.0)])
+    h = w.shape_cast(_abi.sphere_cast((0.0, 0.8, 0.8), 0.5, (1, 0, 0), 100.0))
+    e = 4.5 - math.sqrt(0.07)
+    assert h["hit"] == 1 and h["body"] == box and h["shape_index"] == box and not h["started_inside"]
+    assert abs(h["distance"] - e) < 1e-3
+    np.testing.assert_allclose(h["point"], [4.5, 0.5, 0.5], atol=1e-3)
+    np.testing.assert_allclose(h["normal"], np.array([-math.sqrt(0.07), 0.3, 0.3]) / 0.5, atol=1e-2)
+    sb = scenes.SceneBuilder()
+    t = sb.add_body(PXG_STATIC, (3.0, 0.0, 0.8))
+    sb.add_shape(t, PXG_CAPSULE, capsule_radius=0.2, capsule_half_height=1.0, capsule_axis=1)
+    b, s = sb.arrays()
+    cw = oracle.OracleWorld(_abi.default_config(), b, s)
+    h = cw.shape_cast(_abi.capsule_cast((0, 0, 0), 0.3, 0.5, 2, (0, 0, 0, 1), (1, 0, 0), 100.0))
+    assert h["hit"] == 1 and abs(h["distance"] - 2.6) < 1e-3
+    np.testing.assert_allclose(h["normal"], [-0.8, 0, -0.6], atol=1e-2)
+    assert cw.shape_cast(_abi.capsule_cast((0, 0, -0.65), 0.3, 0.5, 2, (0, 0, 0, 1), (1, 0, 0), 100.0))["hit"] == 0
+
+
+def test_sweep_reports_started_inside_and_ignores_resting_contact(oracle):
+    # queries_test.cpp:224-246 (sphere and capsule parts)
+    w, (box,) = static_box_world(oracle, [(5.0, 0.0, 0.0)])
+    h = w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (1, 0, 0), 2.0))
+    assert h["hit"] == 1 and h["started_inside"] == 1 and abs(h["distance"]) < 1e-6 and h["body"] == box
+    np.testing.assert_allclose(h["normal"], [-1, 0, 0], atol=1e-3)
+    assert w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (-1, 0, 0), 2.0))["hit"] == 0
+    sb = scenes.SceneBuilder()
+    sb.static_box((0, 0, -0.5), (10, 10, 0.5))
+    b, s = sb.arrays()
+    fw = oracle.OracleWorld(_abi.default_config(), b, s)
+    assert fw.shape_cast(_abi.sphere_cast((0, 0, 0.5), 0.5, (1, 0, 0), 3.0))["hit"] == 0
+    assert fw.shape_cast(_abi.capsule_cast((0, 0, 0.9), 0.4, 0.5, 2, (0, 0, 0, 1), (0, 1, 0), 3.0))["hit"] == 0
+    h = fw.shape_cast(_abi.sphere_cast((0, 0, 0.5), 0.5, (0, 0, -1), 3.0))
+    assert h["hit"] == 1 and abs(h["distance"]) < 1e-3 and h["started_inside"] == 0
+
+
+def cast_first_root(kind, geo, core_a, core_b, r, d, maxd, n=4001):
+    """First t at which the cast core (segment a-b, radius r) moved by d*t touches the shape,
+    by sampling the gap min over the core of sdf - r, then bisection (float64)."""
+    us = np.linspace(0.0, 1.0, 41)
+
+    def gap(t):
+        pts = core_a[None] + us[:, None] * (core_b - core_a)[None] + t * d[None]
+        return np.min(sdf(kind, pts, geo)) - r
+    ts = np.linspace(0.0, maxd, n)
+    g = np.array([gap(t) for t in ts])
+    if g[0] <= 0:
+        return None
+    idx = np.nonzero(g <= 0)[0]
+    if len(idx) == 0:
+        return np.inf
+    lo, hi = ts[idx[0] - 1], ts[idx[0]]
+    for _ in range(50):
+        mid = 0.5 * (lo + hi)
+        if gap(mid) > 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+@pytest.mark.parametrize("kind", [PXG_SPHERE, PXG_BOX, PXG_CAPSULE])
+@pytest.mark.parametrize("cast_kind", [PXG_SPHERE, PXG_CAPSULE])
+def test_cast_distance_equals_first_touch(oracle, kind, cast_kind):
+    """The cast distance of a sphere or capsule against one target is where the swept core first
+    comes within its radius of the target (float64, sampled along the capsule's core); the
+    conservative advancement stops within 1e-4 of contact."""
+    rng = np.random.default_rng(40 + 3 * kind + cast_kind)
+    hits = 0
+    for trial in range(40):
+        sb = scenes.SceneBuilder()
+        pos = rng.uniform(-1, 1, 3)
+        q = scenes.random_quaternion(rng)
+        i = sb.add_body(PXG_STATIC, pos, q)
+        he = rng.uniform(0.3, 0.9, 3)
+        rad, cr, chh = rng.uniform(0.3, 0.9), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.7)
+        sb.add_shape(i, kind, radius=rad, half_extents=he, capsule_radius=cr, capsule_half_height=chh, capsule_axis=2)
+        b, s = sb.arrays()
+        w = oracle.OracleWorld(_abi.default_config(), b, s)
+        R = quat_matrix(np.asarray(q, np.float64))
+        if kind == PXG_SPHERE:
+            geo = (pos, rad)
+        elif kind == PXG_BOX:
+            geo = (pos, R, he)
+        else:
+            geo = (pos - R[:, 2] * chh, pos + R[:, 2] * chh, cr)
+        o = (pos + rng.uniform(-5, 5, 3)).astype(np.float32).astype(np.float64)
+        tgt = pos + rng.uniform(-0.8, 0.8, 3)
+        d = ((tgt - o) / np.linalg.norm(tgt - o)).astype(np.float32).astype(np.float64)
+        cr_cast = 0.25
+        if cast_kind == PXG_SPHERE:
+            c = _abi.sphere_cast(o, cr_cast, d, 12.0)
+            ca = cb = o
+        else:
+            cq = scenes.random_quaternion(rng)
+            Rc = quat_matrix(np.asarray(cq, np.float64))
+            hh = 0.4
+            c = _abi.capsule_cast(o, cr_cast, hh, 2, cq, d, 12.0)
+            ca, cb = o - Rc[:, 2] * hh, o + Rc[:, 2] * hh
+        ref = cast_first_root(kind, geo, ca, cb, cr_cast, d, 12.0)
+        if ref is None:
+            continue
+        h = w.shape_cast(c)
+        if not np.isfinite(ref):
+            assert h["hit"] == 0 or h["distance"] > 11.0
+            continue
+        assert h["hit"] == 1, (kind, cast_kind, ref)
+        assert -2e-3 <= ref - h["distance"] <= 2e-3, (kind, cast_kind, float(h["distance"]), ref)
+        hits += 1
+    assert hits > 8
