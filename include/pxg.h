@@ -264,7 +264,7 @@ typedef struct pxg_raycast_hit {
     uint32_t started_inside;
     uint32_t reserved;
 } pxg_raycast_hit;
-/* One shape cast (shape_cast(), world.h:877-909): the shape (sphere or capsule; its local centre,
+/* One shape cast (shape_cast(), world.h:877-909): the shape (sphere, box or capsule; its local centre,
  * local rotation, capsule axis and dimensions are used, owner / material / layer are not) posed at
  * position / orientation, swept along direction (normalised by the library) up to max_distance. */
 typedef struct pxg_cast_query {
@@ -378,11 +378,11 @@ int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query_filter* fi
  * min(cap, n) are copied. */
 int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_filter* filter, uint32_t* bodies,
                 uint32_t cap, uint32_t* n);
-/* The first body a swept sphere or capsule touches (shape_cast() / sphere_cast() / capsule_cast(),
- * world.h:877-950; query/sweep.h:422-505: conservative advancement on the exact closest distance,
+/* The first body a swept sphere, box or capsule touches (shape_cast() / sphere_cast() /
+ * box_cast() / capsule_cast(), world.h:877-950; query/sweep.h:422-505: conservative advancement on
+ * the exact closest distance for round shapes, a swept separating-axis test for box against box,
  * a start overlap reported at distance 0 only when the sweep leads into the target). hit->hit = 0
- * when nothing is touched. Equal distances go to the lower shape slot. Box casts return
- * PXG_ERR_INVALID (the swept separating-axis test they need is not built). */
+ * when nothing is touched. Equal distances go to the lower shape slot. */
 int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const pxg_query_filter* filter, pxg_raycast_hit* hit);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);

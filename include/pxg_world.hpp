@@ -545,13 +545,20 @@ public:
         return overlap(q, filter);
     }
 
-    // The first body a swept sphere / capsule touches (sphere_cast(), capsule_cast(),
-    // world.h:911-950). Box casts are not offered.
+    // The first body a swept sphere / box / capsule touches (sphere_cast(), box_cast(),
+    // capsule_cast(), world.h:911-950).
     bool sphere_cast(Vec3 origin, float radius, Vec3 dir, float max_distance, RaycastHit& hit,
                      const QueryFilter& filter = QueryFilter()) {
         pxg_cast_query c = cast_base(max_distance, origin, Quat{}, dir);
         c.shape.type = PXG_SPHERE;
         c.shape.radius = radius;
+        return shape_cast(c, hit, filter);
+    }
+    bool box_cast(Vec3 origin, Vec3 half_extents, Quat orientation, Vec3 dir, float max_distance, RaycastHit& hit,
+                  const QueryFilter& filter = QueryFilter()) {
+        pxg_cast_query c = cast_base(max_distance, origin, orientation, dir);
+        c.shape.type = PXG_BOX;
+        set3(c.shape.half_extents, half_extents);
         return shape_cast(c, hit, filter);
     }
     bool capsule_cast(Vec3 origin, float radius, float half_height, int direction_axis, Quat orientation, Vec3 dir,

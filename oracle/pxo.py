@@ -180,14 +180,12 @@ class OracleWorld:
         return out[:len(r)]
 
     def shape_cast(self, cast, query_filter=None):
-        """shape_cast() for a sphere or capsule (CAST_DTYPE in): the hit record (hit = 0: none)."""
+        """shape_cast() (CAST_DTYPE in): the hit record (hit = 0: none)."""
         from physxcoopa_amd._abi import HIT_DTYPE
         c = np.ascontiguousarray(cast).reshape(1)
         f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter))
         out = np.zeros(1, HIT_DTYPE)
-        r = lib().pxo_shape_cast(self.h, ptr(c), f, ptr(out))
-        if r < 0:
-            raise ValueError("box casts are not supported")
+        lib().pxo_shape_cast(self.h, ptr(c), f, ptr(out))
         return out[0]
 
     def overlap(self, query, query_filter=None):

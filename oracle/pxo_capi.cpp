@@ -310,11 +310,10 @@ uint32_t pxo_overlap(void* p, const pxg_overlap_query* q, const pxg_query_filter
     if (h->dbl) go(h->d); else go(h->f);
     return n;
 }
-// shape_cast() for a sphere or capsule (returns 1 on a hit; -1 for a box cast)
+// shape_cast() for a sphere, capsule or box (returns 1 on a hit)
 int pxo_shape_cast(void* p, const pxg_cast_query* c, const pxg_query_filter* f, pxg_raycast_hit* out) {
     Handle* h = static_cast<Handle*>(p);
     std::memset(out, 0, sizeof(*out));
-    if (c->shape.type == PXG_BOX) return -1;
     int r = 0;
     auto go = [&](auto& w) {
         using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;

@@ -257,7 +257,7 @@ template <class T> std::vector<uint32_t> world_overlap(World<T>& w, const Overla
     return out;
 }
 
-// ---------------- shape casts: sphere and capsule (query/sweep.h, world.h:877-950) ----------------
+// ---------------- shape casts (query/sweep.h, world.h:877-950) ----------------
 constexpr float k_sweep_tolerance = 1e-4f;  // sweep.h:68
 
 template <class T> struct CastGeom {  // sweep.h:86-121: core segment (a == b for a sphere) + radius, or the box
@@ -383,9 +383,11 @@ template <class T> void segment_box_mtv(const V3<T>& p0, const V3<T>& p1, T radi
 // convex_penetration for a sphere or capsule probe (sweep.h:339-384)
 template <class T> bool convex_penetration(const Shape<T>& probe, const V3<T>& pos, const Q<T>& rot, const Shape<T>& target,
                                            const V3<T>& tpos, const Q<T>& trot, V3<T>& normal, T& depth, V3<T>& point) {
-    if (probe.type == PXG_CAPSULE && target.type == PXG_BOX) {
-        CastGeom<T> core = cast_geometry(probe, pos, rot);
-        OBB<T> box = world_obb(target, tpos, trot);
+    const bool probe_core_vs_box = probe.type == PXG_CAPSULE && target.type == PXG_BOX;
+    const bool box_vs_target_core = probe.type == PXG_BOX && target.type == PXG_CAPSULE;
+    if (probe_core_vs_box || box_vs_target_core) {
+        CastGeom<T> core = probe_core_vs_box ? cast_geometry(probe, pos, rot) : cast_geometry(target, tpos, trot);
+        OBB<T> box = probe_core_vs_box ? world_obb(target, tpos, trot) : world_obb(probe, pos, rot);
         V3<T> on_seg, on_box;
         T dist = std::sqrt(closest_seg_box(core.a, core.b, box, on_seg, on_box));
         if (dist >= core.r) return false;
@@ -396,6 +398,7 @@ template <class T> bool convex_penetration(const Shape<T>& probe, const V3<T>& p
             segment_box_mtv(core.a, core.b, core.r, box, normal, depth);
         }
         point = on_box;
+        if (!probe_core_vs_box) normal = -normal;  // the probe is the box: push it away from the core
         return depth > T(0);
     }
     Manifold<T> m;
@@ -411,74 +414,171 @@ template <class T> bool convex_penetration(const Shape<T>& probe, const V3<T>& p
     return depth > T(0);
 }
 
+// obb_corners_, swept_sat_, sweep_box_box_, polytope_contact_point_ (sweep.h:128-140, 175-258, 282-295)
+template <class T> void obb_corners(const OBB<T>& box, V3<T> out[8]) {
+    V3<T> ax[3];
+    obb_axes(box, ax);
+    int k = 0;
+    for (int sx = -1; sx <= 1; sx += 2)
+        for (int sy = -1; sy <= 1; sy += 2)
+            for (int sz = -1; sz <= 1; sz += 2)
+                out[k++] = box.c + ax[0] * (T(sx) * box.he.x) + ax[1] * (T(sy) * box.he.y) + ax[2] * (T(sz) * box.he.z);
+}
+enum class Advance { Hit, Miss, AtStart };
+template <class T> Advance sweep_box_box(const OBB<T>& a, const OBB<T>& b, const V3<T>& dir, T maxd, T& toi, V3<T>& axis,
+                                         V3<T>& mtv_axis, T& mtv_depth) {
+    V3<T> ax[3], bx[3];
+    obb_axes(a, ax);
+    obb_axes(b, bx);
+    V3<T> axes[15];
+    int count = 0;
+    for (int i = 0; i < 3; ++i) push_axis(axes, count, ax[i]);
+    for (int i = 0; i < 3; ++i) push_axis(axes, count, bx[i]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) push_axis(axes, count, cross(ax[i], bx[j]));
+    T t_first = -std::numeric_limits<T>::max(), t_last = std::numeric_limits<T>::max();
+    mtv_depth = std::numeric_limits<T>::max();
+    for (int k = 0; k < count; ++k) {
+        const V3<T>& L = axes[k];
+        T a0, a1, b0, b1;
+        project_obb(a, ax, L, a0, a1);
+        project_obb(b, bx, L, b0, b1);
+        T v = dot(dir, L);
+        T overlap = gmin(a1 - b0, b1 - a0);
+        if (overlap < mtv_depth) {
+            mtv_depth = overlap;
+            mtv_axis = (a0 + a1) >= (b0 + b1) ? L : -L;
+        }
+        if (std::abs(v) < T(1e-7f)) {
+            if (overlap <= T(k_sweep_tolerance)) return Advance::Miss;
+            continue;
+        }
+        T enter = v > T(0) ? (b0 - a1) / v : (b1 - a0) / v;
+        T exit = v > T(0) ? (b1 - a0) / v : (b0 - a1) / v;
+        if (enter > t_first) {
+            t_first = enter;
+            axis = v > T(0) ? -L : L;
+        }
+        t_last = gmin(t_last, exit);
+        if (t_first > t_last || t_first > maxd || t_last < T(0)) return Advance::Miss;
+    }
+    if (t_first <= T(k_sweep_tolerance)) return Advance::AtStart;
+    toi = t_first;
+    return Advance::Hit;
+}
+template <class T> V3<T> polytope_contact_point(const OBB<T>& box_at_toi, const V3<T>& n, const OBB<T>& target) {
+    V3<T> corners[8];
+    obb_corners(box_at_toi, corners);
+    T lo = std::numeric_limits<T>::max();
+    for (int k = 0; k < 8; ++k) lo = gmin(lo, dot(corners[k], n));
+    V3<T> sum(T(0), T(0), T(0));
+    int count = 0;
+    T eps = T(1e-3f) * max4(box_at_toi.he.x, box_at_toi.he.y, box_at_toi.he.z, T(1e-3f));
+    for (int k = 0; k < 8; ++k) {
+        if (dot(corners[k], n) <= lo + eps) {
+            sum += corners[k];
+            ++count;
+        }
+    }
+    return closest_point_on_obb(sum / T(count > 1 ? count : 1), target);
+}
+
 template <class T> struct SweepHit {
     T distance = T(0);
     V3<T> normal{T(0), T(0), T(1)}, point{T(0), T(0), T(0)};
     bool started_inside = false;
 };
 
-// sweep_convex for a sphere or capsule cast shape against a sphere, box or capsule target
-// (sweep.h:149-168 advance_, 408-416 resolve_at_start_, 422-505)
-template <class T> bool sweep_round(const Shape<T>& shape, const V3<T>& pos, const Q<T>& rot, const V3<T>& dir, T maxd,
-                                    const Shape<T>& target, const V3<T>& tpos, const Q<T>& trot, SweepHit<T>& out) {
+// sweep_convex (sweep.h:149-168 advance_, 408-416 resolve_at_start_, 422-505) for a sphere,
+// capsule or box cast shape against a sphere, box or capsule target
+template <class T> bool sweep_convex(const Shape<T>& shape, const V3<T>& pos, const Q<T>& rot, const V3<T>& dir, T maxd,
+                                     const Shape<T>& target, const V3<T>& tpos, const Q<T>& trot, SweepHit<T>& out) {
     const CastGeom<T> cast = cast_geometry(shape, pos, rot);
     T toi = T(0);
-    V3<T> n(T(0), T(0), T(1)), p(T(0), T(0), T(0));
-    auto dist_at = [&](T t, V3<T>& normal, V3<T>& point) {
-        V3<T> a = cast.a + dir * t, b = cast.b + dir * t;
-        V3<T> on_cast, on_target;
-        T target_radius = T(0);
-        if (target.type == PXG_BOX) {
-            closest_seg_box(a, b, world_obb(target, tpos, trot), on_cast, on_target);
+    V3<T> n(T(0), T(0), T(1)), p(T(0), T(0), T(0)), mtv_axis(T(0), T(0), T(1));
+    T mtv_depth = T(0);
+    const bool polytopes = shape.type == PXG_BOX && target.type == PXG_BOX;
+    Advance result;
+    if (polytopes) {
+        OBB<T> tb = world_obb(target, tpos, trot);
+        result = sweep_box_box(cast.box, tb, dir, maxd, toi, n, mtv_axis, mtv_depth);
+        if (result == Advance::Hit) {
+            OBB<T> moved = cast.box;
+            moved.c += dir * toi;
+            p = polytope_contact_point(moved, n, tb);
+        } else if (result == Advance::AtStart) {
+            p = closest_point_on_obb(cast.box.c, tb);
+        }
+    } else {
+        const bool box_cast = shape.type == PXG_BOX;
+        auto dist_at = [&](T t, V3<T>& normal, V3<T>& point) {
+            if (box_cast) {  // the moving box against the target's core
+                CastGeom<T> core = cast_geometry(target, tpos, trot);
+                OBB<T> moved = cast.box;
+                moved.c += dir * t;
+                V3<T> on_seg, on_box;
+                T dist = std::sqrt(closest_seg_box(core.a, core.b, moved, on_seg, on_box));
+                if (dist > k_epsilon<T>()) normal = (on_box - on_seg) / dist;
+                point = on_seg + normal * core.r;
+                return dist - core.r;
+            }
+            V3<T> a = cast.a + dir * t, b = cast.b + dir * t;
+            V3<T> on_cast, on_target;
+            T target_radius = T(0);
+            if (target.type == PXG_BOX) {
+                closest_seg_box(a, b, world_obb(target, tpos, trot), on_cast, on_target);
+            } else {
+                CastGeom<T> core = cast_geometry(target, tpos, trot);
+                closest_points_seg_seg(a, b, core.a, core.b, on_cast, on_target);
+                target_radius = core.r;
+            }
+            V3<T> diff = on_cast - on_target;
+            T dist = length(diff);
+            if (dist > k_epsilon<T>()) normal = diff / dist;
+            point = on_target + normal * target_radius;
+            return dist - cast.r - target_radius;
+        };
+        T t = T(0);
+        T dd = dist_at(t, n, p);
+        if (dd <= T(k_sweep_tolerance)) {
+            result = Advance::AtStart;
         } else {
-            CastGeom<T> core = cast_geometry(target, tpos, trot);
-            closest_points_seg_seg(a, b, core.a, core.b, on_cast, on_target);
-            target_radius = core.r;
+            result = Advance::Hit;
+            for (int iter = 0; iter < 64; ++iter) {
+                T closing = -dot(dir, n);
+                if (closing <= T(1e-6f)) { result = Advance::Miss; break; }
+                t += dd / closing;
+                if (t > maxd) { result = Advance::Miss; break; }
+                dd = dist_at(t, n, p);
+                if (dd <= T(k_sweep_tolerance)) break;
+            }
+            toi = t;
         }
-        V3<T> diff = on_cast - on_target;
-        T dist = length(diff);
-        if (dist > k_epsilon<T>()) normal = diff / dist;
-        point = on_target + normal * target_radius;
-        return dist - cast.r - target_radius;
-    };
-    // advance_
-    T t = T(0);
-    T dd = dist_at(t, n, p);
-    bool at_start = dd <= T(k_sweep_tolerance);
-    if (!at_start) {
-        bool hit = false;
-        for (int iter = 0; iter < 64; ++iter) {
-            T closing = -dot(dir, n);
-            if (closing <= T(1e-6f)) return false;
-            t += dd / closing;
-            if (t > maxd) return false;
-            dd = dist_at(t, n, p);
-            if (dd <= T(k_sweep_tolerance)) { hit = true; break; }
+    }
+    if (result == Advance::Miss) return false;
+    if (result == Advance::AtStart) {
+        V3<T> normal = polytopes ? mtv_axis : n, point = p, pen_n, pen_p;
+        T depth = polytopes ? mtv_depth : T(0), pen_d = T(0);
+        if (!polytopes && convex_penetration(shape, pos, rot, target, tpos, trot, pen_n, pen_d, pen_p)) {
+            normal = pen_n;
+            point = pen_p;
+            depth = pen_d;
         }
-        (void)hit;
-        toi = t;
-        out.distance = toi;
-        out.normal = n;
-        out.point = p;
-        out.started_inside = false;
+        if (dot(dir, normal) >= T(-1e-4f)) return false;  // resolve_at_start_
+        out.distance = T(0);
+        out.normal = normal;
+        out.point = point;
+        out.started_inside = depth > T(k_sweep_tolerance);
         return true;
     }
-    V3<T> normal = n, point = p, pen_n, pen_p;
-    T depth = T(0), pen_d = T(0);
-    if (convex_penetration(shape, pos, rot, target, tpos, trot, pen_n, pen_d, pen_p)) {
-        normal = pen_n;
-        point = pen_p;
-        depth = pen_d;
-    }
-    if (dot(dir, normal) >= T(-1e-4f)) return false;  // resolve_at_start_
-    out.distance = T(0);
-    out.normal = normal;
-    out.point = point;
-    out.started_inside = depth > T(k_sweep_tolerance);
+    out.distance = toi;
+    out.normal = n;
+    out.point = p;
+    out.started_inside = false;
     return true;
 }
 
-// shape_cast (world.h:877-909) for a sphere or capsule: the swept AABB against the fat AABBs, the
+// shape_cast (world.h:877-909): the swept AABB against the fat AABBs, the
 // filter, then each candidate's sweep; the nearest wins (lower slot on ties: the reference's tie
 // order is its tree order)
 template <class T> bool world_shape_cast(World<T>& w, const Shape<T>& shape, const V3<T>& pos, const Q<T>& rot,
@@ -503,7 +603,7 @@ template <class T> bool world_shape_cast(World<T>& w, const Shape<T>& shape, con
         if (found && best <= T(0)) continue;
         const Body<T>& b = w.bodies[s.owner];
         SweepHit<T> r;
-        if (!sweep_round(shape, pos, rot, ud, best, s, b.pos, b.q, r)) continue;
+        if (!sweep_convex(shape, pos, rot, ud, best, s, b.pos, b.q, r)) continue;
         if (found && r.distance >= best) continue;
         best = r.distance;
         hit.point = r.point;

@@ -117,6 +117,15 @@ def sphere_cast(origin, radius, direction, max_distance):
     return c
 
 
+def box_cast(origin, half_extents, orientation, direction, max_distance):
+    """box_cast() (world.h:924-933) as a CAST_DTYPE record."""
+    c = np.zeros(1, CAST_DTYPE)
+    c["shape"]["type"], c["shape"]["enabled"], c["shape"]["half_extents"] = PXG_BOX, 1, half_extents
+    c["shape"]["local_rotation"] = (0, 0, 0, 1)
+    c["position"], c["orientation"], c["direction"], c["max_distance"] = origin, orientation, direction, max_distance
+    return c
+
+
 def capsule_cast(origin, radius, half_height, axis, orientation, direction, max_distance):
     """capsule_cast() (world.h:938-950) as a CAST_DTYPE record."""
     c = np.zeros(1, CAST_DTYPE)

@@ -487,7 +487,7 @@ __global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, Ov
     flags[j] = out;
 }
 
-// ---- shape casts: sphere and capsule (query/sweep.h:68-505, world.h:877-909) ----
+// ---- shape casts: sphere, capsule and box (query/sweep.h:68-505, world.h:877-909) ----
 constexpr float kSweepTol = 1e-4f;  // sweep.h:68
 
 struct CastCore {  // sweep.h:86-121: core segment (a == b for a sphere) and radius
@@ -615,9 +615,11 @@ __device__ void segment_box_mtv(v3 p0, v3 p1, float radius, const OBB& box, v3& 
 // convex_penetration for a sphere or capsule probe (sweep.h:339-384)
 __device__ bool convex_penetration(const ShapeGeom& probe, v3 pos, quat rot, const ShapeGeom& target, v3 tpos, quat trot,
                                    v3& normal, float& depth, v3& point) {
-    if (probe.type == SH_CAPSULE && target.type == SH_BOX) {
-        const CastCore core = cast_core(probe, pos, rot);
-        const OBB box = world_obb(target, tpos, trot);
+    const bool probe_core_vs_box = probe.type == SH_CAPSULE && target.type == SH_BOX;
+    const bool box_vs_target_core = probe.type == SH_BOX && target.type == SH_CAPSULE;
+    if (probe_core_vs_box || box_vs_target_core) {
+        const CastCore core = probe_core_vs_box ? cast_core(probe, pos, rot) : cast_core(target, tpos, trot);
+        const OBB box = probe_core_vs_box ? world_obb(target, tpos, trot) : world_obb(probe, pos, rot);
         v3 on_seg, on_box;
         const float dist = sqrtf(closest_seg_box(core.a, core.b, box, on_seg, on_box));
         if (dist >= core.r) return false;
@@ -628,6 +630,7 @@ __device__ bool convex_penetration(const ShapeGeom& probe, v3 pos, quat rot, con
             segment_box_mtv(core.a, core.b, core.r, box, normal, depth);
         }
         point = on_box;
+        if (!probe_core_vs_box) normal = -normal;  // the probe is the box: push it away from the core
         return depth > 0.0f;
     }
     Man m;
@@ -642,9 +645,89 @@ __device__ bool convex_penetration(const ShapeGeom& probe, v3 pos, quat rot, con
     point = m.count > 0 ? sum / float(m.count) : tpos;
     return depth > 0.0f;
 }
-// the gap between the cast core moved by dir * t and the target (sweep.h:467-483)
-__device__ __forceinline__ float cast_gap(const CastCore& cast, v3 dir, float t, const ShapeGeom& target, v3 tpos,
-                                          quat trot, v3& normal, v3& point) {
+// obb_corners_, swept_sat_, sweep_box_box_, polytope_contact_point_ (sweep.h:128-140, 175-258, 282-295)
+__device__ void obb_corners(const OBB& box, v3 out[8]) {
+    v3 ax[3];
+    axes(box, ax);
+    int k = 0;
+    for (int sx = -1; sx <= 1; sx += 2)
+        for (int sy = -1; sy <= 1; sy += 2)
+            for (int sz = -1; sz <= 1; sz += 2)
+                out[k++] = box.c + ax[0] * (float(sx) * box.he.x) + ax[1] * (float(sy) * box.he.y) +
+                           ax[2] * (float(sz) * box.he.z);
+}
+enum { ADV_HIT = 0, ADV_MISS = 1, ADV_AT_START = 2 };
+__device__ int sweep_box_box(const OBB& a, const OBB& b, v3 dir, float maxd, float& toi, v3& axis, v3& mtv_axis,
+                             float& mtv_depth) {
+    v3 ax[3], bx[3];
+    axes(a, ax);
+    axes(b, bx);
+    v3 axs[15];
+    int count = 0;
+    for (int i = 0; i < 3; ++i) push_axis(axs, count, ax[i]);
+    for (int i = 0; i < 3; ++i) push_axis(axs, count, bx[i]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) push_axis(axs, count, cross(ax[i], bx[j]));
+    float t_first = -3.402823466e+38f, t_last = 3.402823466e+38f;
+    mtv_depth = 3.402823466e+38f;
+    for (int k = 0; k < count; ++k) {
+        const v3 L = axs[k];
+        float a0, a1, b0, b1;
+        project_obb(a, ax, L, a0, a1);
+        project_obb(b, bx, L, b0, b1);
+        const float v = dot(dir, L);
+        const float overlap = gmin(a1 - b0, b1 - a0);
+        if (overlap < mtv_depth) {
+            mtv_depth = overlap;
+            mtv_axis = (a0 + a1) >= (b0 + b1) ? L : -L;
+        }
+        if (fabsf(v) < 1e-7f) {
+            if (overlap <= kSweepTol) return ADV_MISS;
+            continue;
+        }
+        const float enter = v > 0.0f ? (b0 - a1) / v : (b1 - a0) / v;
+        const float exit = v > 0.0f ? (b1 - a0) / v : (b0 - a1) / v;
+        if (enter > t_first) {
+            t_first = enter;
+            axis = v > 0.0f ? -L : L;
+        }
+        t_last = gmin(t_last, exit);
+        if (t_first > t_last || t_first > maxd || t_last < 0.0f) return ADV_MISS;
+    }
+    if (t_first <= kSweepTol) return ADV_AT_START;
+    toi = t_first;
+    return ADV_HIT;
+}
+__device__ v3 polytope_contact_point(const OBB& box_at_toi, v3 n, const OBB& target) {
+    v3 corners[8];
+    obb_corners(box_at_toi, corners);
+    float lo = 3.402823466e+38f;
+    for (int k = 0; k < 8; ++k) lo = gmin(lo, dot(corners[k], n));
+    v3 sum = mk(0.0f, 0.0f, 0.0f);
+    int count = 0;
+    const float eps = 1e-3f * max4(box_at_toi.he.x, box_at_toi.he.y, box_at_toi.he.z, 1e-3f);
+    for (int k = 0; k < 8; ++k) {
+        if (dot(corners[k], n) <= lo + eps) {
+            sum = sum + corners[k];
+            ++count;
+        }
+    }
+    return closest_point_on_obb(sum / float(count > 1 ? count : 1), target);
+}
+// the gap between the cast shape moved by dir * t and the target (sweep.h:452-483): a sphere or
+// capsule core against any target, or a box against a round target's core
+__device__ __forceinline__ float cast_gap(bool box_cast, const CastCore& cast, const OBB& cast_box, v3 dir, float t,
+                                          const ShapeGeom& target, v3 tpos, quat trot, v3& normal, v3& point) {
+    if (box_cast) {
+        const CastCore core = cast_core(target, tpos, trot);
+        OBB moved = cast_box;
+        moved.c = moved.c + dir * t;
+        v3 on_seg, on_box;
+        const float dist = sqrtf(closest_seg_box(core.a, core.b, moved, on_seg, on_box));
+        if (dist > 1e-6f) normal = (on_box - on_seg) / dist;
+        point = on_seg + normal * core.r;
+        return dist - core.r;
+    }
     const v3 a = cast.a + dir * t, b = cast.b + dir * t;
     v3 on_cast, on_target;
     float target_radius = 0.0f;
@@ -661,40 +744,70 @@ __device__ __forceinline__ float cast_gap(const CastCore& cast, v3 dir, float t,
     point = on_target + normal * target_radius;
     return dist - cast.r - target_radius;
 }
-// sweep_convex for a sphere or capsule cast (sweep.h:149-168, 408-416, 422-505)
-__device__ bool sweep_round(const ShapeGeom& shape, v3 pos, quat rot, v3 dir, float maxd, const ShapeGeom& target,
-                            v3 tpos, quat trot, float& dist_out, v3& n_out, v3& p_out, bool& started) {
-    const CastCore cast = cast_core(shape, pos, rot);
-    v3 n = mk(0.0f, 0.0f, 1.0f), p = mk(0.0f, 0.0f, 0.0f);
-    float t = 0.0f;
-    float dd = cast_gap(cast, dir, t, target, tpos, trot, n, p);
-    if (!(dd <= kSweepTol)) {
-        for (int iter = 0; iter < 64; ++iter) {
-            const float closing = -dot(dir, n);
-            if (closing <= 1e-6f) return false;
-            t += dd / closing;
-            if (t > maxd) return false;
-            dd = cast_gap(cast, dir, t, target, tpos, trot, n, p);
-            if (dd <= kSweepTol) break;
+// sweep_convex (sweep.h:149-168, 408-416, 422-505)
+__device__ bool sweep_convex(const ShapeGeom& shape, v3 pos, quat rot, v3 dir, float maxd, const ShapeGeom& target,
+                             v3 tpos, quat trot, float& dist_out, v3& n_out, v3& p_out, bool& started) {
+    const bool box_cast = shape.type == SH_BOX;
+    const CastCore cast = box_cast ? CastCore{pos, pos, 0.0f} : cast_core(shape, pos, rot);
+    const OBB cast_box = world_obb(shape, pos, rot);
+    const bool polytopes = box_cast && target.type == SH_BOX;
+    v3 n = mk(0.0f, 0.0f, 1.0f), p = mk(0.0f, 0.0f, 0.0f), mtv_axis = mk(0.0f, 0.0f, 1.0f);
+    float mtv_depth = 0.0f, toi = 0.0f;
+    int result;
+    if (polytopes) {
+        const OBB tb = world_obb(target, tpos, trot);
+        result = sweep_box_box(cast_box, tb, dir, maxd, toi, n, mtv_axis, mtv_depth);
+        if (result == ADV_HIT) {
+            OBB moved = cast_box;
+            moved.c = moved.c + dir * toi;
+            p = polytope_contact_point(moved, n, tb);
+        } else if (result == ADV_AT_START) {
+            p = closest_point_on_obb(cast_box.c, tb);
         }
-        dist_out = t;
-        n_out = n;
-        p_out = p;
-        started = false;
+    } else {
+        float t = 0.0f;
+        float dd = cast_gap(box_cast, cast, cast_box, dir, t, target, tpos, trot, n, p);
+        if (dd <= kSweepTol) {
+            result = ADV_AT_START;
+        } else {
+            result = ADV_HIT;
+            for (int iter = 0; iter < 64; ++iter) {
+                const float closing = -dot(dir, n);
+                if (closing <= 1e-6f) {
+                    result = ADV_MISS;
+                    break;
+                }
+                t += dd / closing;
+                if (t > maxd) {
+                    result = ADV_MISS;
+                    break;
+                }
+                dd = cast_gap(box_cast, cast, cast_box, dir, t, target, tpos, trot, n, p);
+                if (dd <= kSweepTol) break;
+            }
+            toi = t;
+        }
+    }
+    if (result == ADV_MISS) return false;
+    if (result == ADV_AT_START) {
+        v3 normal = polytopes ? mtv_axis : n, point = p, pen_n, pen_p;
+        float depth = polytopes ? mtv_depth : 0.0f, pen_d = 0.0f;
+        if (!polytopes && convex_penetration(shape, pos, rot, target, tpos, trot, pen_n, pen_d, pen_p)) {
+            normal = pen_n;
+            point = pen_p;
+            depth = pen_d;
+        }
+        if (dot(dir, normal) >= -1e-4f) return false;
+        dist_out = 0.0f;
+        n_out = normal;
+        p_out = point;
+        started = depth > kSweepTol;
         return true;
     }
-    v3 normal = n, point = p, pen_n, pen_p;
-    float depth = 0.0f, pen_d = 0.0f;
-    if (convex_penetration(shape, pos, rot, target, tpos, trot, pen_n, pen_d, pen_p)) {
-        normal = pen_n;
-        point = pen_p;
-        depth = pen_d;
-    }
-    if (dot(dir, normal) >= -1e-4f) return false;
-    dist_out = 0.0f;
-    n_out = normal;
-    p_out = point;
-    started = depth > kSweepTol;
+    dist_out = toi;
+    n_out = n;
+    p_out = p;
+    started = false;
     return true;
 }
 __device__ __forceinline__ ShapeGeom decode_shape(const DevState& d, uint32_t i, uint4 info) {
@@ -737,7 +850,7 @@ __global__ void k_shape_cast(DevState d, const uint4* qinfo, CastDev c, QueryFil
     float dist;
     v3 n, p;
     bool started;
-    if (!sweep_round(c.shape, pos, rot, ud, c.maxd, target, tpos, trot, dist, n, p, started)) return;
+    if (!sweep_convex(c.shape, pos, rot, ud, c.maxd, target, tpos, trot, dist, n, p, started)) return;
     rec[3 * j] = st3(p, dist);
     rec[3 * j + 1] = st3(n, __uint_as_float(started ? 1u : 0u));
     rec[3 * j + 2] = make_float4(__uint_as_float(info.x), __uint_as_float(j), 0.0f, 0.0f);

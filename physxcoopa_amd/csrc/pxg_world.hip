@@ -1078,10 +1078,8 @@ extern "C" int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const px
                               pxg_raycast_hit* hit) {
     if (!w || !cast || !hit) return fail(PXG_ERR_INVALID, "pxg_shape_cast: null argument");
     std::memset(hit, 0, sizeof(*hit));
-    if (cast->shape.type == PXG_BOX)
-        return fail(PXG_ERR_INVALID, "pxg_shape_cast: box casts are not supported (sphere and capsule casts are)");
-    if (cast->shape.type != PXG_SPHERE && cast->shape.type != PXG_CAPSULE)
-        return fail(PXG_ERR_INVALID, "pxg_shape_cast: the cast shape is a sphere or a capsule");
+    if (cast->shape.type > PXG_CAPSULE)
+        return fail(PXG_ERR_INVALID, "pxg_shape_cast: the cast shape is a sphere, a box or a capsule");
     // per-shape records in the hit buffer (3 float4 per shape), the hit itself after them
     int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u) + 1u);
     if (r != PXG_OK) return r;

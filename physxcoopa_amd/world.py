@@ -150,7 +150,7 @@ class World:
         return out[:len(r)].astype(bool)
 
     def shape_cast(self, cast, query_filter=None):
-        """The first body a swept sphere or capsule touches (CAST_DTYPE in; world.h:877-950): a
+        """The first body a swept sphere, box or capsule touches (CAST_DTYPE in; world.h:877-950): a
         HIT_DTYPE record, hit = 0 when none."""
         c = np.ascontiguousarray(cast, dtype=_abi.CAST_DTYPE).reshape(1)
         f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))

@@ -698,6 +698,11 @@ void sweeps_are_exact_against_box_corner_and_capsule() {
     CHECK_NEAR(hit.distance, 4.5f - std::sqrt(0.07f), 1e-3f);
     CHECK_NEAR(hit.point.x, 4.5f, 1e-3f);
     CHECK(hit.body == box_id && !hit.started_inside);
+    // a box yawed 45 degrees meets the face x = 4.5 with its leading edge at 4.5 - 0.5 sqrt(2)
+    pxg::Quat yaw{0.0f, 0.0f, std::sin(0.3926991f), std::cos(0.3926991f)};
+    CHECK(world.box_cast(pxg::Vec3{0, 0, 0}, pxg::Vec3{0.5f, 0.5f, 0.5f}, yaw, pxg::Vec3{1, 0, 0}, 100.0f, hit));
+    CHECK_NEAR(hit.distance, 4.5f - 0.5f * std::sqrt(2.0f), 1e-3f);
+    CHECK_NEAR(hit.normal.x, -1.0f, 1e-3f);
     pxg::PhysicsWorld capsules;
     pxg_shape cap = pxg::no_shape();
     cap.enabled = 1;

@@ -136,7 +136,7 @@ def test_overlaps_equal_oracle(World, oracle):
 
 
 def test_reference_sweep_tests_on_gpu(World):
-    # queries_test.cpp:132-174, 224-246 (sphere and capsule casts)
+    # queries_test.cpp:132-174, 224-246
     sb = scenes.SceneBuilder()
     box = sb.add_body(PXG_STATIC, (5.0, 0, 0))
     sb.add_shape(box, PXG_BOX, half_extents=(0.5, 0.5, 0.5))
@@ -148,16 +148,17 @@ def test_reference_sweep_tests_on_gpu(World):
     h = w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (1, 0, 0), 2.0))
     assert h["hit"] == 1 and h["started_inside"] == 1 and abs(h["distance"]) < 1e-6
     assert w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (-1, 0, 0), 2.0))["hit"] == 0
-    from physxcoopa_amd._lib import PxgError
-    with pytest.raises(PxgError, match="box casts"):
-        c = _abi.sphere_cast((0, 0, 0), 0.5, (1, 0, 0), 10.0)
-        c["shape"]["type"] = PXG_BOX
-        w.shape_cast(c)
+    yaw = (0.0, 0.0, float(np.sin(np.pi / 8)), float(np.cos(np.pi / 8)))
+    h = w.shape_cast(_abi.box_cast((0, 0, 0), (0.5, 0.5, 0.5), yaw, (1, 0, 0), 100.0))
+    assert h["hit"] == 1 and abs(h["distance"] - (4.5 - 0.5 * 2 ** 0.5)) < 1e-3
+    h = w.shape_cast(_abi.box_cast((4.1, 0, 0), (0.5, 0.5, 0.5), (0, 0, 0, 1), (1, 0, 0), 2.0))
+    assert h["hit"] == 1 and h["started_inside"] == 1
 
 
 def test_shape_casts_equal_oracle(World, oracle):
-    """Sphere and capsule casts from above and beside a settled pile: the GPU's hit records equal
-    the oracle's bit for bit (conservative advancement, start-overlap rules, nearest target)."""
+    """Sphere, box and capsule casts from above and beside a settled pile: the GPU's hit records
+    equal the oracle's bit for bit (conservative advancement, swept SAT, start-overlap rules,
+    nearest target)."""
     b, s = scenes.mixed_pile_scene(n=300, seed=8)
     cfg = _abi.default_config()
     gw = World(cfg, b, s)
@@ -170,8 +171,10 @@ def test_shape_casts_equal_oracle(World, oracle):
         o = rng.uniform([-5, -5, 0.2], [5, 5, 6])
         d = rng.normal(size=3)
         d[2] -= 1.0
-        if k % 2 == 0:
+        if k % 3 == 0:
             c = _abi.sphere_cast(o, rng.uniform(0.1, 0.6), d, rng.uniform(1, 12))
+        elif k % 3 == 1:
+            c = _abi.box_cast(o, rng.uniform(0.1, 0.5, 3), scenes.random_quaternion(rng), d, rng.uniform(1, 12))
         else:
             c = _abi.capsule_cast(o, rng.uniform(0.1, 0.4), rng.uniform(0.1, 0.6), int(rng.integers(0, 3)),
                                   scenes.random_quaternion(rng), d, rng.uniform(1, 12))
@@ -180,4 +183,4 @@ def test_shape_casts_equal_oracle(World, oracle):
         hits += int(g["hit"])
         started += int(g["started_inside"])
     print("casts: %d of 400 hit, %d started inside" % (hits, started), flush=True)
-    assert hits > 150
+    assert hits > 150 and started > 20

@@ -286,10 +286,10 @@ def test_sphere_overlaps_match_distance_closed_forms(oracle):
     assert checked > 10000
 
 
-# --- shape casts (sphere and capsule) ---
+# --- shape casts ---
 
 def test_sweeps_are_exact_against_box_corner_and_capsule(oracle):
-    # queries_test.cpp:132-174 (the sphere and capsule parts; box casts are not built)
+    # queries_test.cpp:132-174
     w, (box,) = static_box_world(oracle, [(5.0, 0.0, 0.0)])
     h = w.shape_cast(_abi.sphere_cast((0.0, 0.8, 0.8), 0.5, (1, 0, 0), 100.0))
     e = 4.5 - math.sqrt(0.07)
@@ -297,6 +297,12 @@ def test_sweeps_are_exact_against_box_corner_and_capsule(oracle):
     assert abs(h["distance"] - e) < 1e-3
     np.testing.assert_allclose(h["point"], [4.5, 0.5, 0.5], atol=1e-3)
     np.testing.assert_allclose(h["normal"], np.array([-math.sqrt(0.07), 0.3, 0.3]) / 0.5, atol=1e-2)
+    # a box yawed 45 degrees: its leading edge meets the face x = 4.5 at 4.5 - 0.5 sqrt(2)
+    yaw = (0.0, 0.0, math.sin(math.pi / 8), math.cos(math.pi / 8))
+    h = w.shape_cast(_abi.box_cast((0, 0, 0), (0.5, 0.5, 0.5), yaw, (1, 0, 0), 100.0))
+    assert h["hit"] == 1 and abs(h["distance"] - (4.5 - 0.5 * math.sqrt(2.0))) < 1e-3
+    np.testing.assert_allclose(h["normal"], [-1, 0, 0], atol=1e-3)
+    np.testing.assert_allclose(h["point"], [4.5, 0, 0], atol=1e-2)
     sb = scenes.SceneBuilder()
     t = sb.add_body(PXG_STATIC, (3.0, 0.0, 0.8))
     sb.add_shape(t, PXG_CAPSULE, capsule_radius=0.2, capsule_half_height=1.0, capsule_axis=1)
@@ -309,18 +315,23 @@ def test_sweeps_are_exact_against_box_corner_and_capsule(oracle):
 
 
 def test_sweep_reports_started_inside_and_ignores_resting_contact(oracle):
-    # queries_test.cpp:224-246 (sphere and capsule parts)
+    # queries_test.cpp:224-246
     w, (box,) = static_box_world(oracle, [(5.0, 0.0, 0.0)])
     h = w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (1, 0, 0), 2.0))
     assert h["hit"] == 1 and h["started_inside"] == 1 and abs(h["distance"]) < 1e-6 and h["body"] == box
     np.testing.assert_allclose(h["normal"], [-1, 0, 0], atol=1e-3)
     assert w.shape_cast(_abi.sphere_cast((4.8, 0, 0), 0.25, (-1, 0, 0), 2.0))["hit"] == 0
+    h = w.shape_cast(_abi.box_cast((4.1, 0, 0), (0.5, 0.5, 0.5), (0, 0, 0, 1), (1, 0, 0), 2.0))
+    assert h["hit"] == 1 and h["started_inside"] == 1
+    np.testing.assert_allclose(h["normal"], [-1, 0, 0], atol=1e-3)
+    assert w.shape_cast(_abi.box_cast((4.1, 0, 0), (0.5, 0.5, 0.5), (0, 0, 0, 1), (-1, 0, 0), 2.0))["hit"] == 0
     sb = scenes.SceneBuilder()
     sb.static_box((0, 0, -0.5), (10, 10, 0.5))
     b, s = sb.arrays()
     fw = oracle.OracleWorld(_abi.default_config(), b, s)
     assert fw.shape_cast(_abi.sphere_cast((0, 0, 0.5), 0.5, (1, 0, 0), 3.0))["hit"] == 0
     assert fw.shape_cast(_abi.capsule_cast((0, 0, 0.9), 0.4, 0.5, 2, (0, 0, 0, 1), (0, 1, 0), 3.0))["hit"] == 0
+    assert fw.shape_cast(_abi.box_cast((0, 0, 0.25), (0.25, 0.25, 0.25), (0, 0, 0, 1), (1, 0, 0), 3.0))["hit"] == 0
     h = fw.shape_cast(_abi.sphere_cast((0, 0, 0.5), 0.5, (0, 0, -1), 3.0))
     assert h["hit"] == 1 and abs(h["distance"]) < 1e-3 and h["started_inside"] == 0
 
@@ -397,5 +408,79 @@ def test_cast_distance_equals_first_touch(oracle, kind, cast_kind):
             continue
         assert h["hit"] == 1, (kind, cast_kind, ref)
         assert -2e-3 <= ref - h["distance"] <= 2e-3, (kind, cast_kind, float(h["distance"]), ref)
+        hits += 1
+    assert hits > 8
+
+
+def boxes_overlap(ca, Ra, ha, cb, Rb, hb):
+    """Static separating-axis test of two OBBs in float64 (15 axes)."""
+    axes = [Ra[:, i] for i in range(3)] + [Rb[:, i] for i in range(3)]
+    axes += [np.cross(Ra[:, i], Rb[:, j]) for i in range(3) for j in range(3)]
+    d = cb - ca
+    for L in axes:
+        n = np.linalg.norm(L)
+        if n < 1e-9:
+            continue
+        L = L / n
+        ra = np.sum(ha * np.abs(Ra.T @ L))
+        rb = np.sum(hb * np.abs(Rb.T @ L))
+        if abs(d @ L) > ra + rb:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("kind", [PXG_SPHERE, PXG_BOX, PXG_CAPSULE])
+def test_box_cast_distance_equals_first_touch(oracle, kind):
+    """A box cast against one target: where the moving box first touches it (float64: bisection on
+    the static 15-axis SAT for a box target, the signed distance of the moving box at the round
+    target's core for spheres and capsules)."""
+    rng = np.random.default_rng(70 + kind)
+    hits = 0
+    for trial in range(40):
+        sb = scenes.SceneBuilder()
+        pos = rng.uniform(-1, 1, 3)
+        q = scenes.random_quaternion(rng)
+        i = sb.add_body(PXG_STATIC, pos, q)
+        he = rng.uniform(0.3, 0.9, 3)
+        rad, cr, chh = rng.uniform(0.3, 0.9), rng.uniform(0.2, 0.5), rng.uniform(0.2, 0.7)
+        sb.add_shape(i, kind, radius=rad, half_extents=he, capsule_radius=cr, capsule_half_height=chh, capsule_axis=2)
+        b, s = sb.arrays()
+        w = oracle.OracleWorld(_abi.default_config(), b, s)
+        R = quat_matrix(np.asarray(q, np.float64))
+        o = (pos + rng.uniform(-5, 5, 3)).astype(np.float32).astype(np.float64)
+        tgt = pos + rng.uniform(-0.8, 0.8, 3)
+        d = ((tgt - o) / np.linalg.norm(tgt - o)).astype(np.float32).astype(np.float64)
+        cq = scenes.random_quaternion(rng)
+        Rc = quat_matrix(np.asarray(cq, np.float64))
+        chex = rng.uniform(0.15, 0.45, 3).astype(np.float32).astype(np.float64)
+        c = _abi.box_cast(o, chex, cq, d, 12.0)
+        if kind == PXG_BOX:
+            def touching(t):
+                return boxes_overlap(o + d * t, Rc, chex, pos, R, he)
+        else:
+            core = (pos, pos, rad) if kind == PXG_SPHERE else (pos - R[:, 2] * chh, pos + R[:, 2] * chh, cr)
+            us = np.linspace(0.0, 1.0, 41)
+            pts = core[0][None] + us[:, None] * (core[1] - core[0])[None]
+
+            def touching(t):
+                return np.min(sdf(PXG_BOX, pts, (o + d * t, Rc, chex))) <= core[2]
+        if touching(0.0):
+            continue
+        ts = np.linspace(0.0, 12.0, 2401)
+        first = next((t for t in ts if touching(t)), None)
+        h = w.shape_cast(c)
+        if first is None:
+            assert h["hit"] == 0 or h["distance"] > 11.9
+            continue
+        lo, hi = first - ts[1], first
+        for _ in range(40):
+            mid = 0.5 * (lo + hi)
+            if touching(mid):
+                hi = mid
+            else:
+                lo = mid
+        ref = 0.5 * (lo + hi)
+        assert h["hit"] == 1, (kind, ref)
+        assert -2e-3 <= ref - h["distance"] <= 2e-3, (kind, float(h["distance"]), ref)
         hits += 1
     assert hits > 8
