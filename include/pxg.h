@@ -286,6 +286,18 @@ typedef struct pxg_overlap_query {
     float a[3], b[3];       /* capsule end points */
 } pxg_overlap_query;
 
+/* One overlap found by compute_penetration() (query::Penetration, query/queries.h:111-117): moving
+ * the query shape by normal * depth separates it from the target shape; normal is unit and points
+ * from the target toward the query shape; point is a representative contact point on the target. */
+typedef struct pxg_penetration {
+    float normal[3];
+    float depth;
+    float point[3];
+    uint32_t body;
+    uint32_t shape_index;
+    uint32_t reserved;
+} pxg_penetration;
+
 typedef struct pxg_world pxg_world;
 
 /* --- configuration and lifetime --- */
@@ -384,6 +396,14 @@ int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_fi
  * a start overlap reported at distance 0 only when the sweep leads into the target). hit->hit = 0
  * when nothing is touched. Equal distances go to the lower shape slot. */
 int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const pxg_query_filter* filter, pxg_raycast_hit* hit);
+/* Every shape the posed query shape (sphere, box or capsule) penetrates, with the push-out that
+ * separates them (compute_penetration(), world.h:963-995; query/sweep.h:339-384: the solver's
+ * contact generation, with the capsule-against-box pair handled on the exact segment-box distance).
+ * Candidates are live shapes whose fat AABB meets the query shape's bounds and that pass the filter;
+ * only positive depths are reported, in shape-slot order. n receives the count, min(cap, n) are
+ * copied. */
+int pxg_compute_penetration(pxg_world* w, const pxg_shape* shape, const float position[3], const float orientation[4],
+                            const pxg_query_filter* filter, pxg_penetration* out, uint32_t cap, uint32_t* n);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);
 

@@ -131,6 +131,15 @@ struct RaycastHit {
     uint32_t shape_index = 0xFFFFFFFFu;
     bool started_inside = false;  // shape casts only
 };
+// One overlap of compute_penetration() (query::Penetration, query/queries.h:111-117): moving the
+// query shape by normal * depth separates it; normal points from the target toward the query shape.
+struct Penetration {
+    Vec3 normal{0.0f, 0.0f, 1.0f};
+    float depth = 0.0f;
+    Vec3 point;  // a representative contact point on the target
+    BodyId body;
+    uint32_t shape_index = 0xFFFFFFFFu;
+};
 struct QueryFilter {
     uint32_t layer_mask = ~0u;
     bool include_triggers = true;
@@ -569,6 +578,32 @@ public:
         c.shape.capsule_half_height = half_height;
         c.shape.capsule_axis = direction_axis;
         return shape_cast(c, hit, filter);
+    }
+
+    // Every shape that a sphere, box or capsule posed at pos / rot penetrates, with its push-out, in
+    // shape-slot order (compute_penetration(), world.h:963-995). shape.type must be PXG_SPHERE,
+    // PXG_BOX or PXG_CAPSULE; its owner, material and layer are not used.
+    std::vector<Penetration> compute_penetration(const pxg_shape& shape, Vec3 pos, Quat rot,
+                                                 const QueryFilter& filter = QueryFilter()) {
+        push();
+        pxg_query_filter f = to_filter(filter);
+        const float p[3] = {pos.x, pos.y, pos.z}, q[4] = {rot.x, rot.y, rot.z, rot.w};
+        uint32_t n = 0;
+        check(pxg_compute_penetration(w_, &shape, p, q, &f, nullptr, 0, &n), "compute_penetration");
+        std::vector<pxg_penetration> raw(n);
+        if (n) check(pxg_compute_penetration(w_, &shape, p, q, &f, raw.data(), n, &n), "compute_penetration");
+        std::vector<Penetration> out;
+        for (uint32_t i = 0; i < n && i < raw.size(); ++i) {
+            Penetration e;
+            e.normal = Vec3{raw[i].normal[0], raw[i].normal[1], raw[i].normal[2]};
+            e.depth = raw[i].depth;
+            e.point = Vec3{raw[i].point[0], raw[i].point[1], raw[i].point[2]};
+            const uint32_t b = raw[i].body;
+            e.body = BodyId{b, b < generations_.size() ? generations_[b] : 0u};
+            e.shape_index = raw[i].shape_index;
+            out.push_back(e);
+        }
+        return out;
     }
 
     // The closest hits of many rays in one GPU pass (no counterpart in the reference, which casts

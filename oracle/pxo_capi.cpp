@@ -329,6 +329,28 @@ int pxo_shape_cast(void* p, const pxg_cast_query* c, const pxg_query_filter* f, 
     if (h->dbl) go(h->d); else go(h->f);
     return r;
 }
+// compute_penetration(): every penetration in slot order; returns the count, copies min(cap, count)
+int pxo_compute_penetration(void* p, const pxg_shape* shape, const float* pos, const float* rot,
+                            const pxg_query_filter* f, pxg_penetration* out, int cap) {
+    Handle* h = static_cast<Handle*>(p);
+    int count = 0;
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        auto pens = world_penetration(w, shape_from<T>(*shape), v3<T>(pos), q4<T>(rot), filter_from(f));
+        count = int(pens.size());
+        for (int i = 0; i < count && i < cap; ++i) {
+            pxg_penetration& o = out[i];
+            std::memset(&o, 0, sizeof(o));
+            put3(o.normal, pens[i].normal);
+            o.depth = float(pens[i].depth);
+            put3(o.point, pens[i].point);
+            o.body = pens[i].body;
+            o.shape_index = pens[i].shape;
+        }
+    };
+    if (h->dbl) go(h->d); else go(h->f);
+    return count;
+}
 // raycast_shape for one shape at a pose (float)
 int pxo_ray_shape(const pxg_shape* s, const float* pos, const float* rot, const pxg_ray* ray, float* t, float* n3) {
     Shape<float> sh = shape_from<float>(*s);

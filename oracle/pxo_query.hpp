@@ -617,4 +617,33 @@ template <class T> bool world_shape_cast(World<T>& w, const Shape<T>& shape, con
     return found;
 }
 
+// compute_penetration (world.h:963-995) for a sphere, box or capsule query shape: live shapes
+// whose fat AABB meets the query shape's world bounds (query_aabb_, inclusive) and that pass the
+// filter, each through convex_penetration; slot order (the reference's order is its tree order)
+template <class T> struct Penetration {
+    V3<T> normal, point;
+    T depth = T(0);
+    uint32_t body = 0, shape = 0;
+};
+template <class T> std::vector<Penetration<T>> world_penetration(World<T>& w, const Shape<T>& shape, const V3<T>& pos,
+                                                                 const Q<T>& rot, const QueryFilter& f) {
+    std::vector<Penetration<T>> out;
+    const AABB<T> qb = world_bounds(shape, pos, rot);
+    for (uint32_t i = 0; i < w.shapes.size(); ++i) {
+        const Shape<T>& s = w.shapes[i];
+        if (i >= w.proxy.size() || w.proxy[i] < 0 || s.owner >= w.bodies.size() || !w.bodies[s.owner].alive) continue;
+        if (!((f.layer_mask >> (s.layer & 31u)) & 1u)) continue;
+        if (!f.include_triggers && s.trigger) continue;
+        if (f.ignore != ~0u && s.owner == f.ignore) continue;
+        if (!w.fat(i).overlaps(qb)) continue;
+        const Body<T>& b = w.bodies[s.owner];
+        Penetration<T> p;
+        if (!convex_penetration(shape, pos, rot, s, b.pos, b.q, p.normal, p.depth, p.point)) continue;
+        p.body = s.owner;
+        p.shape = i;
+        out.push_back(p);
+    }
+    return out;
+}
+
 }  // namespace pxo

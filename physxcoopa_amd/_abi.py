@@ -107,6 +107,18 @@ CAST_DTYPE = np.dtype([("shape", SHAPE_DTYPE), ("position", "<f4", 3), ("max_dis
                        ("direction", "<f4", 3), ("reserved", "<f4")])
 assert CAST_DTYPE.itemsize == 144
 
+PEN_DTYPE = np.dtype([("normal", "<f4", 3), ("depth", "<f4"), ("point", "<f4", 3), ("body", "<u4"), ("shape_index", "<u4"),
+                      ("reserved", "<u4")])
+assert PEN_DTYPE.itemsize == 40
+
+
+def probe_parts(probe):
+    """(shape, position, orientation) arrays of a CAST_DTYPE record, as compute_penetration takes
+    the posed query shape (its direction and max_distance are not used)."""
+    c = np.ascontiguousarray(probe).reshape(1)
+    return (np.ascontiguousarray(c["shape"]), np.ascontiguousarray(c["position"][0], np.float32),
+            np.ascontiguousarray(c["orientation"][0], np.float32))
+
 
 def sphere_cast(origin, radius, direction, max_distance):
     """sphere_cast() (world.h:911-920) as a CAST_DTYPE record."""

@@ -871,7 +871,37 @@ __global__ void k_cast_pick(const float4* rec, const unsigned long long* best, f
     hit[2] = make_float4(r2.x, r2.y, r1.w, 0.0f);
 }
 
+// compute_penetration (world.h:963-995): one thread per target shape; the probe's world bounds
+// against the fat AABB (AABB::overlaps, inclusive), the filter, then convex_penetration
+__global__ void k_penetration(DevState d, const uint4* qinfo, CastDev c, QueryFilterDev f, float4* rec,
+                              uint32_t* count) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.ns) return;
+    const uint4 qi = qinfo[j];
+    if (!accepts(qi, f)) return;
+    const v3 pos = ld3(c.pos);
+    const quat rot = ldq(c.rot);
+    v3 qmn, qmx;
+    world_bounds(c.shape, pos, rot, qmn, qmx);
+    const v3 fmn = ld3(d.s_fmin[j]), fmx = ld3(d.s_fmax[j]);
+    if (!(fmn.x <= qmx.x && fmx.x >= qmn.x && fmn.y <= qmx.y && fmx.y >= qmn.y && fmn.z <= qmx.z && fmx.z >= qmn.z)) return;
+    const uint4 info = d.s_info[j];
+    const ShapeGeom target = decode_shape(d, j, info);
+    v3 n, p;
+    float depth;
+    if (!convex_penetration(c.shape, pos, rot, target, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]), n, depth, p)) return;
+    const uint32_t k = atomicAdd(count, 1u);
+    rec[3 * k] = st3(n, depth);
+    rec[3 * k + 1] = st3(p, __uint_as_float(j));
+    rec[3 * k + 2] = make_float4(__uint_as_float(info.x), 0.0f, 0.0f, 0.0f);
+}
+
 }  // namespace
+
+void launch_penetration(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
+                        uint32_t* count, hipStream_t st) {
+    if (d.ns) hipLaunchKernelGGL(k_penetration, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, qinfo, c, f, rec, count);
+}
 
 void launch_shape_cast(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
                        unsigned long long* best, float4* hit, hipStream_t st) {

@@ -264,6 +264,11 @@ struct CastDev {
 // per-shape records (3 float4 per shape), the nearest key (reset to ~0 before), the hit (3 float4)
 void launch_shape_cast(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
                        unsigned long long* best, float4* hit, hipStream_t st);
+// compute_penetration: the posed probe in c (dir and maxd unused) against every shape; each
+// penetration appends (normal, depth; point, slot; owner) as 3 float4 at rec[3 * atomicAdd(count, 1)]
+// (count reset to 0 before; room for 3 float4 per shape)
+void launch_penetration(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
+                        uint32_t* count, hipStream_t st);
 size_t query_sort_bytes(uint32_t cap);
 size_t query_shape_record_bytes();
 void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);

@@ -1074,18 +1074,10 @@ extern "C" int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const p
     return PXG_OK;
 }
 
-extern "C" int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const pxg_query_filter* filter,
-                              pxg_raycast_hit* hit) {
-    if (!w || !cast || !hit) return fail(PXG_ERR_INVALID, "pxg_shape_cast: null argument");
-    std::memset(hit, 0, sizeof(*hit));
-    if (cast->shape.type > PXG_CAPSULE)
-        return fail(PXG_ERR_INVALID, "pxg_shape_cast: the cast shape is a sphere, a box or a capsule");
-    // per-shape records in the hit buffer (3 float4 per shape), the hit itself after them
-    int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u) + 1u);
-    if (r != PXG_OK) return r;
-    if (!w->q_best) DALLOC(w->q_best, 1);
-    const pxg_shape& sh = cast->shape;
-    CastDev c;
+// a query shape (sphere, box or capsule; local centre, local rotation, capsule axis and dimensions)
+// posed at position / orientation, as the query kernels take it
+static CastDev posed_probe(const pxg_shape& sh, const float position[3], const float orientation[4]) {
+    CastDev c{};
     c.shape.type = int(sh.type);
     c.shape.caxis = sh.capsule_axis;
     c.shape.lc.x = sh.local_center[0];
@@ -1101,8 +1093,22 @@ extern "C" int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const px
     c.shape.he.z = sh.half_extents[2];
     c.shape.cr = sh.capsule_radius;
     c.shape.chh = sh.capsule_half_height;
-    c.pos = make_float4(cast->position[0], cast->position[1], cast->position[2], 0.0f);
-    c.rot = make_float4(cast->orientation[0], cast->orientation[1], cast->orientation[2], cast->orientation[3]);
+    c.pos = make_float4(position[0], position[1], position[2], 0.0f);
+    c.rot = make_float4(orientation[0], orientation[1], orientation[2], orientation[3]);
+    return c;
+}
+
+extern "C" int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const pxg_query_filter* filter,
+                              pxg_raycast_hit* hit) {
+    if (!w || !cast || !hit) return fail(PXG_ERR_INVALID, "pxg_shape_cast: null argument");
+    std::memset(hit, 0, sizeof(*hit));
+    if (cast->shape.type > PXG_CAPSULE)
+        return fail(PXG_ERR_INVALID, "pxg_shape_cast: the cast shape is a sphere, a box or a capsule");
+    // per-shape records in the hit buffer (3 float4 per shape), the hit itself after them
+    int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u) + 1u);
+    if (r != PXG_OK) return r;
+    if (!w->q_best) DALLOC(w->q_best, 1);
+    CastDev c = posed_probe(cast->shape, cast->position, cast->orientation);
     c.dir = make_float4(cast->direction[0], cast->direction[1], cast->direction[2], 0.0f);
     c.maxd = cast->max_distance;
     launch_fill_u64(w->q_best, ~0ull, 1, w->stream);
@@ -1110,6 +1116,58 @@ extern "C" int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const px
     launch_shape_cast(w->d, w->q.info, c, query_filter(filter), w->q_hits, w->q_best, out, w->stream);
     HIPCHK(hipMemcpyAsync(hit, out, sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_compute_penetration(pxg_world* w, const pxg_shape* shape, const float position[3],
+                                       const float orientation[4], const pxg_query_filter* filter, pxg_penetration* out,
+                                       uint32_t cap, uint32_t* n) {
+    if (!w || !shape || !position || !orientation || !n || (cap && !out))
+        return fail(PXG_ERR_INVALID, "pxg_compute_penetration: null argument");
+    *n = 0;
+    if (shape->type > PXG_CAPSULE)
+        return fail(PXG_ERR_INVALID, "pxg_compute_penetration: the query shape is a sphere, a box or a capsule");
+    // up to one record (3 float4) per shape in the hit buffer; the count in the nearest-key word
+    int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u) + 1u);
+    if (r != PXG_OK) return r;
+    if (!w->q_best) DALLOC(w->q_best, 1);
+    const CastDev c = posed_probe(*shape, position, orientation);
+    uint32_t* count = reinterpret_cast<uint32_t*>(w->q_best);
+    launch_fill_u64(w->q_best, 0ull, 1, w->stream);
+    launch_penetration(w->d, w->q.info, c, query_filter(filter), w->q_hits, count, w->stream);
+    uint32_t k = 0;
+    HIPCHK(hipMemcpyAsync(&k, count, sizeof(k), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    std::vector<float4> rec(3 * size_t(k));
+    if (k) {
+        HIPCHK(hipMemcpyAsync(rec.data(), w->q_hits, rec.size() * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
+        HIPCHK(hipStreamSynchronize(w->stream));
+    }
+    // appended in arrival order: report them in shape-slot order
+    std::vector<uint32_t> order(k);
+    for (uint32_t i = 0; i < k; ++i) order[i] = i;
+    auto slot = [&](uint32_t i) {
+        uint32_t v;
+        std::memcpy(&v, &rec[3 * size_t(i) + 1].w, sizeof(v));
+        return v;
+    };
+    std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return slot(x) < slot(y); });
+    const uint32_t m = std::min(cap, k);
+    for (uint32_t i = 0; i < m; ++i) {
+        const float4 a = rec[3 * size_t(order[i])], b = rec[3 * size_t(order[i]) + 1];
+        pxg_penetration& o = out[i];
+        o.normal[0] = a.x;
+        o.normal[1] = a.y;
+        o.normal[2] = a.z;
+        o.depth = a.w;
+        o.point[0] = b.x;
+        o.point[1] = b.y;
+        o.point[2] = b.z;
+        o.shape_index = slot(order[i]);
+        std::memcpy(&o.body, &rec[3 * size_t(order[i]) + 2].x, sizeof(o.body));
+        o.reserved = 0u;
+    }
+    *n = k;
     return PXG_OK;
 }
 

@@ -158,6 +158,20 @@ class World:
         check(lib().pxg_shape_cast(self.h, ptr(c), f, ptr(out)), "pxg_shape_cast")
         return out[0]
 
+    def compute_penetration(self, probe, query_filter=None):
+        """Every shape the posed query shape penetrates, with its push-out (world.h:963-995): the
+        shape, position and orientation of a CAST_DTYPE record (direction and max_distance unused)
+        in, a PEN_DTYPE array in shape-slot order out."""
+        sh, pos, rot = _abi.probe_parts(np.asarray(probe, dtype=_abi.CAST_DTYPE))
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        n = ctypes.c_uint32()
+        check(lib().pxg_compute_penetration(self.h, ptr(sh), ptr(pos), ptr(rot), f, None, 0, ctypes.byref(n)),
+              "pxg_compute_penetration")
+        out = np.zeros(max(n.value, 1), _abi.PEN_DTYPE)
+        check(lib().pxg_compute_penetration(self.h, ptr(sh), ptr(pos), ptr(rot), f, ptr(out), n.value, ctypes.byref(n)),
+              "pxg_compute_penetration")
+        return out[:n.value]
+
     def overlap(self, query, query_filter=None):
         """Bodies with a shape overlapping the query shape (OVERLAP_DTYPE; world.h:1002-1060), once
         per overlapping shape, in shape-slot order."""

@@ -717,6 +717,45 @@ void sweeps_are_exact_against_box_corner_and_capsule() {
     CHECK(!capsules.capsule_cast(pxg::Vec3{0, 0, -0.65f}, 0.3f, 0.5f, 2, pxg::Quat{}, pxg::Vec3{1, 0, 0}, 100.0f, hit));
 }
 
+// queries_test.cpp:335-360 (the convex part): push-outs from a box floor, exact depths and +Z normals
+void compute_penetration_is_exact_against_box() {
+    pxg::PhysicsWorld world;
+    pxg::BodyId floor_id = world.add_body(body(PXG_STATIC, 0.0f, 0.0f, -0.5f), box_shape(2.0f, 2.0f, 0.5f));
+    pxg_shape sphere = pxg::no_shape();
+    sphere.enabled = 1;
+    sphere.type = PXG_SPHERE;
+    sphere.radius = 0.5f;
+    std::vector<pxg::Penetration> p = world.compute_penetration(sphere, pxg::Vec3{0.0f, 0.0f, 0.4f}, pxg::Quat{});
+    CHECK(p.size() == 1);
+    if (p.size() == 1) {
+        CHECK_NEAR(p[0].depth, 0.1f, 1e-4f);
+        CHECK_NEAR(p[0].normal.z, 1.0f, 1e-4f);
+        CHECK(p[0].body == floor_id && p[0].shape_index == 0u);
+    }
+    p = world.compute_penetration(box_shape(0.25f, 0.25f, 0.25f), pxg::Vec3{0.5f, 0.0f, 0.1f}, pxg::Quat{});
+    CHECK(p.size() == 1);
+    if (p.size() == 1) {
+        CHECK_NEAR(p[0].depth, 0.15f, 1e-4f);
+        CHECK_NEAR(p[0].normal.z, 1.0f, 1e-4f);
+    }
+    pxg_shape cap = pxg::no_shape();
+    cap.enabled = 1;
+    cap.type = PXG_CAPSULE;
+    cap.capsule_radius = 0.2f;
+    cap.capsule_half_height = 0.5f;
+    cap.capsule_axis = 0;
+    p = world.compute_penetration(cap, pxg::Vec3{0.0f, 0.0f, -0.3f}, pxg::Quat{});
+    CHECK(p.size() == 1);
+    if (p.size() == 1) {
+        CHECK_NEAR(p[0].depth, 0.5f, 1e-4f);
+        CHECK_NEAR(p[0].normal.z, 1.0f, 1e-4f);
+    }
+    CHECK(world.compute_penetration(sphere, pxg::Vec3{0.0f, 0.0f, 0.6f}, pxg::Quat{}).empty());
+    pxg::QueryFilter ignore_floor;
+    ignore_floor.ignore = floor_id;
+    CHECK(world.compute_penetration(sphere, pxg::Vec3{0.0f, 0.0f, 0.4f}, pxg::Quat{}, ignore_floor).empty());
+}
+
 int main() {
     struct T {
         const char* name;
@@ -750,6 +789,7 @@ int main() {
         {"raycasts_hit_boxes_with_exact_distances_and_filters", raycasts_hit_boxes_with_exact_distances_and_filters},
         {"overlaps_find_only_touched_bodies", overlaps_find_only_touched_bodies},
         {"sweeps_are_exact_against_box_corner_and_capsule", sweeps_are_exact_against_box_corner_and_capsule},
+        {"compute_penetration_is_exact_against_box", compute_penetration_is_exact_against_box},
     };
     for (const T& t : tests) {
         int before = g_failures;

@@ -184,3 +184,52 @@ def test_shape_casts_equal_oracle(World, oracle):
         started += int(g["started_inside"])
     print("casts: %d of 400 hit, %d started inside" % (hits, started), flush=True)
     assert hits > 150 and started > 20
+
+
+def test_reference_penetration_tests_on_gpu(World):
+    # queries_test.cpp:335-360 (the convex part)
+    sb = scenes.SceneBuilder()
+    floor = sb.add_body(PXG_STATIC, (0.0, 0.0, -0.5))
+    sb.add_shape(floor, PXG_BOX, half_extents=(2.0, 2.0, 0.5))
+    b, s = sb.arrays()
+    w = World(_abi.default_config(), b, s)
+    ident = (0, 0, 0, 1)
+    for probe, depth in ((_abi.sphere_cast((0, 0, 0.4), 0.5, (0, 0, 1), 0.0), 0.1),
+                         (_abi.box_cast((0.5, 0, 0.1), (0.25, 0.25, 0.25), ident, (0, 0, 1), 0.0), 0.15),
+                         (_abi.capsule_cast((0, 0, -0.3), 0.2, 0.5, 0, ident, (0, 0, 1), 0.0), 0.5)):
+        p = w.compute_penetration(probe)
+        assert len(p) == 1 and p[0]["body"] == floor and abs(p[0]["depth"] - depth) < 1e-4
+        np.testing.assert_allclose(p[0]["normal"], [0, 0, 1], atol=1e-4)
+    assert len(w.compute_penetration(_abi.sphere_cast((0, 0, 0.6), 0.5, (0, 0, 1), 0.0))) == 0
+    assert len(w.compute_penetration(_abi.sphere_cast((0, 0, 0.4), 0.5, (0, 0, 1), 0.0),
+                                     _abi.query_filter(ignore_body=floor))) == 0
+
+
+def test_penetrations_equal_oracle(World, oracle):
+    """Sphere, box and capsule probes placed inside a settled pile: the GPU's penetration lists
+    (normal, depth, point, body, shape, slot order) equal the oracle's bit for bit."""
+    b, s = scenes.mixed_pile_scene(n=300, seed=9)
+    s = s.copy()
+    s["is_trigger"][4::19] = 1
+    cfg = _abi.default_config()
+    gw = World(cfg, b, s)
+    for _ in range(80):
+        gw.step_fixed()
+    ow = oracle_from(oracle, gw, cfg, s)
+    rng = np.random.default_rng(9)
+    total = 0
+    for k in range(300):
+        c = rng.uniform([-5, -5, 0], [5, 5, 3])
+        if k % 3 == 0:
+            p = _abi.sphere_cast(c, rng.uniform(0.2, 1.0), (0, 0, 1), 0.0)
+        elif k % 3 == 1:
+            p = _abi.box_cast(c, rng.uniform(0.1, 0.8, 3), scenes.random_quaternion(rng), (0, 0, 1), 0.0)
+        else:
+            p = _abi.capsule_cast(c, rng.uniform(0.1, 0.5), rng.uniform(0.1, 0.7), int(rng.integers(0, 3)),
+                                  scenes.random_quaternion(rng), (0, 0, 1), 0.0)
+        f = _abi.query_filter(include_triggers=bool(k % 2))
+        g, o = gw.compute_penetration(p, f), ow.compute_penetration(p, f)
+        assert g.tobytes() == o.tobytes(), (k, g, o)
+        total += len(g)
+    print("penetrations: %d over 300 probes" % total, flush=True)
+    assert total > 300
