@@ -396,6 +396,11 @@ int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_fi
  * a start overlap reported at distance 0 only when the sweep leads into the target). hit->hit = 0
  * when nothing is touched. Equal distances go to the lower shape slot. */
 int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const pxg_query_filter* filter, pxg_raycast_hit* hit);
+/* n shape casts in one GPU pass (the batched form of pxg_shape_cast, for callers that cast many
+ * shapes per step, as vehicles and character motors do; the reference casts one per call):
+ * hits[i] is exactly what pxg_shape_cast returns for casts[i]. */
+int pxg_shape_cast_batch(pxg_world* w, const pxg_cast_query* casts, uint32_t n, const pxg_query_filter* filter,
+                         pxg_raycast_hit* hits);
 /* Every shape the posed query shape (sphere, box or capsule) penetrates, with the push-out that
  * separates them (compute_penetration(), world.h:963-995; query/sweep.h:339-384: the solver's
  * contact generation, with the capsule-against-box pair handled on the exact segment-box distance).

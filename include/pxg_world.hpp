@@ -606,6 +606,26 @@ public:
         return out;
     }
 
+    // Many shape casts in one GPU pass (no counterpart in the reference, which casts one shape per
+    // call): hits[i] is valid where found[i] != 0 and equals what the single cast returns.
+    void shape_cast_batch(const std::vector<pxg_cast_query>& casts, std::vector<RaycastHit>& hits,
+                          std::vector<uint8_t>& found, const QueryFilter& filter = QueryFilter()) {
+        push();
+        pxg_query_filter f = to_filter(filter);
+        std::vector<pxg_raycast_hit> raw(casts.size());
+        if (!casts.empty())
+            check(pxg_shape_cast_batch(w_, casts.data(), static_cast<uint32_t>(casts.size()), &f, raw.data()),
+                  "shape_cast_batch");
+        hits.assign(casts.size(), RaycastHit());
+        found.assign(casts.size(), 0);
+        for (size_t i = 0; i < raw.size(); ++i) {
+            if (!raw[i].hit) continue;
+            hits[i] = from_hit(raw[i]);
+            hits[i].started_inside = raw[i].started_inside != 0;
+            found[i] = 1;
+        }
+    }
+
     // The closest hits of many rays in one GPU pass (no counterpart in the reference, which casts
     // one ray per call): hits[i] is valid where found[i] != 0.
     void raycast_batch(const std::vector<Ray>& rays, std::vector<RaycastHit>& hits, std::vector<uint8_t>& found,

@@ -871,6 +871,83 @@ __global__ void k_cast_pick(const float4* rec, const unsigned long long* best, f
     hit[2] = make_float4(r2.x, r2.y, r1.w, 0.0f);
 }
 
+// Many casts: one wavefront per cast. The lanes share the cast's swept AABB, skip tiles whose box
+// misses it, stride over the shapes of the others, sweep each candidate as k_shape_cast does, and
+// keep the nearest (distance, slot) key; a wavefront minimum picks the hit, written by the lane
+// that holds it (keys are unique: the slot is in the low bits).
+constexpr uint32_t kCastWaves = 4;
+__global__ void __launch_bounds__(64 * kCastWaves) k_shape_cast_batch(DevState d, const QShape* sorted,
+                                                                       const float4* tile_box, const CastDev* casts,
+                                                                       uint32_t n, QueryFilterDev f, float4* hits) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t ci = blockIdx.x * kCastWaves + (threadIdx.x >> 6);
+    if (ci >= n) return;  // the whole wavefront leaves together; the kernel has no block barrier
+    const CastDev c = casts[ci];
+    const float dir_len = len(ld3(c.dir));
+    unsigned long long best = ~0ull;
+    float bd = 0.0f;
+    v3 bn = mk(0.0f, 0.0f, 0.0f), bp = bn;
+    bool bs = false;
+    uint32_t bowner = 0u;
+    if (c.maxd >= 0.0f && dir_len >= 1e-8f) {
+        const v3 ud = ld3(c.dir) / dir_len;
+        const v3 pos = ld3(c.pos);
+        const quat rot = ldq(c.rot);
+        v3 mn0, mx0, mn1, mx1;
+        world_bounds(c.shape, pos, rot, mn0, mx0);
+        world_bounds(c.shape, pos + ud * c.maxd, rot, mn1, mx1);
+        const v3 smn = vmin(mn0, mn1), smx = vmax(mx0, mx1);
+        const uint32_t ntiles = (d.ns + kQueryTile - 1u) / kQueryTile;
+        for (uint32_t tix = 0; tix < ntiles; ++tix) {
+            const v3 tmn = ld3(tile_box[2 * tix]), tmx = ld3(tile_box[2 * tix + 1]);
+            if (!(tmn.x <= smx.x && tmx.x >= smn.x && tmn.y <= smx.y && tmx.y >= smn.y && tmn.z <= smx.z && tmx.z >= smn.z))
+                continue;
+            const uint32_t base = tix * kQueryTile;
+            const uint32_t m = d.ns - base < kQueryTile ? d.ns - base : kQueryTile;
+            for (uint32_t k = lane; k < m; k += 64u) {
+                const QShape& s = sorted[base + k];
+                if (!accepts(s.info, f)) continue;
+                const v3 fmn = ld3(s.fmin), fmx = ld3(s.fmax);
+                if (!(fmn.x <= smx.x && fmx.x >= smn.x && fmn.y <= smx.y && fmx.y >= smn.y && fmn.z <= smx.z &&
+                      fmx.z >= smn.z))
+                    continue;
+                const uint32_t j = __float_as_uint(s.fmin.w);
+                const uint4 info = d.s_info[j];
+                const ShapeGeom target = decode_shape(d, j, info);
+                float dist;
+                v3 nn, pp;
+                bool started;
+                if (!sweep_convex(c.shape, pos, rot, ud, c.maxd, target, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]), dist,
+                                  nn, pp, started))
+                    continue;
+                const unsigned long long key = (static_cast<unsigned long long>(f2o(dist)) << 32) | j;
+                if (key < best) {
+                    best = key;
+                    bd = dist;
+                    bn = nn;
+                    bp = pp;
+                    bs = started;
+                    bowner = info.x;
+                }
+            }
+        }
+    }
+    unsigned long long wmin = best;
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(wmin, off, 64);
+        wmin = o < wmin ? o : wmin;
+    }
+    if (wmin == ~0ull) {
+        if (lane == 0) hits[3 * ci] = hits[3 * ci + 1] = hits[3 * ci + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    if (best != wmin) return;
+    hits[3 * ci] = st3(bp, bd);
+    hits[3 * ci + 1] = st3(bn, __uint_as_float(1u));
+    hits[3 * ci + 2] = make_float4(__uint_as_float(bowner), __uint_as_float(uint32_t(best & 0xFFFFFFFFull)),
+                                   __uint_as_float(bs ? 1u : 0u), 0.0f);
+}
+
 // compute_penetration (world.h:963-995): one thread per target shape; the probe's world bounds
 // against the fat AABB (AABB::overlaps, inclusive), the filter, then convex_penetration
 __global__ void k_penetration(DevState d, const uint4* qinfo, CastDev c, QueryFilterDev f, float4* rec,
@@ -925,6 +1002,26 @@ void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* 
     if (!d.ns) return;
     hipLaunchKernelGGL(k_query_prep, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, geom, qinfo, bounds6);
 }
+hipError_t launch_shape_cast_batch(const DevState& d, const QueryScratch& q, const CastDev* casts, uint32_t n,
+                                   const QueryFilterDev& f, float4* hits, hipStream_t st) {
+    if (!n) return hipSuccess;
+    if (d.ns) {
+        hipLaunchKernelGGL(k_query_keys, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, q.info, q.bounds6, q.skey, q.sidx,
+                           static_cast<const float4*>(nullptr), 0u, static_cast<uint32_t*>(nullptr),
+                           static_cast<uint32_t*>(nullptr));
+        size_t bytes = q.sort_bytes;
+        hipError_t e = rocprim::radix_sort_pairs(q.sort_tmp, bytes, q.skey, q.skey2, q.sidx, q.sidx2, d.ns, 0u, 30u, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_query_gather, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, q.geom, q.info, q.sidx2,
+                           static_cast<QShape*>(q.sorted));
+        hipLaunchKernelGGL(k_tile_bounds, dim3((d.ns + kQueryTile - 1u) / kQueryTile), dim3(kQueryTile), 0, st, d,
+                           static_cast<const QShape*>(q.sorted), q.tile_box);
+    }
+    hipLaunchKernelGGL(k_shape_cast_batch, dim3((n + kCastWaves - 1u) / kCastWaves), dim3(64 * kCastWaves), 0, st, d,
+                       static_cast<const QShape*>(q.sorted), q.tile_box, casts, n, f, hits);
+    return hipSuccess;
+}
+
 hipError_t launch_raycast(const DevState& d, const QueryScratch& q, const float4* rays, uint32_t n, const QueryFilterDev& f,
                           uint32_t any, float4* hits, hipStream_t st) {
     if (!n) return hipSuccess;

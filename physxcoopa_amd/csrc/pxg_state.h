@@ -269,6 +269,10 @@ void launch_shape_cast(const DevState& d, const uint4* qinfo, const CastDev& c, 
 // (count reset to 0 before; room for 3 float4 per shape)
 void launch_penetration(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
                         uint32_t* count, hipStream_t st);
+// many shape casts at once: the shapes binned into Morton-ordered tiles, then one wavefront per
+// cast over the tiles its swept AABB meets; hits: 3 float4 per cast (the pxg_raycast_hit layout)
+hipError_t launch_shape_cast_batch(const DevState& d, const QueryScratch& q, const CastDev* casts, uint32_t n,
+                                   const QueryFilterDev& f, float4* hits, hipStream_t st);
 size_t query_sort_bytes(uint32_t cap);
 size_t query_shape_record_bytes();
 void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);

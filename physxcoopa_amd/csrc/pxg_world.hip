@@ -84,6 +84,8 @@ struct pxg_world {
     uint32_t* q_count = nullptr;
     uint32_t q_cap = 0;
     unsigned long long* q_best = nullptr;  // shape casts: nearest (distance, slot) key
+    CastDev* q_casts = nullptr;  // batched shape casts, q_casts_cap records
+    uint32_t q_casts_cap = 0;
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -1115,6 +1117,36 @@ extern "C" int pxg_shape_cast(pxg_world* w, const pxg_cast_query* cast, const px
     float4* out = w->q_hits + 3 * size_t(w->q_cap - 1);
     launch_shape_cast(w->d, w->q.info, c, query_filter(filter), w->q_hits, w->q_best, out, w->stream);
     HIPCHK(hipMemcpyAsync(hit, out, sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_shape_cast_batch(pxg_world* w, const pxg_cast_query* casts, uint32_t n, const pxg_query_filter* filter,
+                                    pxg_raycast_hit* hits) {
+    if (!w || (n && (!casts || !hits))) return fail(PXG_ERR_INVALID, "pxg_shape_cast_batch: null argument");
+    if (!n) return PXG_OK;
+    std::vector<CastDev> c(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (casts[i].shape.type > PXG_CAPSULE)
+            return fail(PXG_ERR_INVALID, "pxg_shape_cast_batch: cast " + std::to_string(i) +
+                                             " is not a sphere, a box or a capsule");
+        c[i] = posed_probe(casts[i].shape, casts[i].position, casts[i].orientation);
+        c[i].dir = make_float4(casts[i].direction[0], casts[i].direction[1], casts[i].direction[2], 0.0f);
+        c[i].maxd = casts[i].max_distance;
+    }
+    int r = query_setup(w, n);  // hits: 3 float4 per cast in the hit buffer
+    if (r != PXG_OK) return r;
+    if (n > w->q_casts_cap) {
+        HIPCHK(hipStreamSynchronize(w->stream));
+        dfree(w, w->q_casts);
+        w->q_casts = nullptr;
+        w->q_casts_cap = 0;
+        DALLOC(w->q_casts, n);
+        w->q_casts_cap = n;
+    }
+    HIPCHK(hipMemcpyAsync(w->q_casts, c.data(), size_t(n) * sizeof(CastDev), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(launch_shape_cast_batch(w->d, w->q, w->q_casts, n, query_filter(filter), w->q_hits, w->stream));
+    HIPCHK(hipMemcpyAsync(hits, w->q_hits, size_t(n) * sizeof(pxg_raycast_hit), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     return PXG_OK;
 }

@@ -158,6 +158,16 @@ class World:
         check(lib().pxg_shape_cast(self.h, ptr(c), f, ptr(out)), "pxg_shape_cast")
         return out[0]
 
+    def shape_cast_batch(self, casts, query_filter=None):
+        """Many shape casts in one GPU pass (a CAST_DTYPE array in): a HIT_DTYPE array, entry i the
+        same record shape_cast(casts[i]) returns."""
+        c = np.ascontiguousarray(casts, dtype=_abi.CAST_DTYPE).reshape(-1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        out = np.zeros(len(c), _abi.HIT_DTYPE)
+        if len(c):
+            check(lib().pxg_shape_cast_batch(self.h, ptr(c), len(c), f, ptr(out)), "pxg_shape_cast_batch")
+        return out
+
     def compute_penetration(self, probe, query_filter=None):
         """Every shape the posed query shape penetrates, with its push-out (world.h:963-995): the
         shape, position and orientation of a CAST_DTYPE record (direction and max_distance unused)

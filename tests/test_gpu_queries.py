@@ -233,3 +233,68 @@ def test_penetrations_equal_oracle(World, oracle):
         total += len(g)
     print("penetrations: %d over 300 probes" % total, flush=True)
     assert total > 300
+
+
+def random_casts(rng, n, lo, hi, zlo, zhi, scale=1.0):
+    casts = np.zeros(n, _abi.CAST_DTYPE)
+    for k in range(n):
+        o = rng.uniform([lo, lo, zlo], [hi, hi, zhi])
+        d = rng.normal(size=3)
+        d[2] -= 1.0
+        maxd = rng.uniform(1, 12) * scale
+        if k % 3 == 0:
+            casts[k] = _abi.sphere_cast(o, rng.uniform(0.1, 0.6), d, maxd)[0]
+        elif k % 3 == 1:
+            casts[k] = _abi.box_cast(o, rng.uniform(0.1, 0.5, 3), scenes.random_quaternion(rng), d, maxd)[0]
+        else:
+            casts[k] = _abi.capsule_cast(o, rng.uniform(0.1, 0.4), rng.uniform(0.1, 0.6), int(rng.integers(0, 3)),
+                                         scenes.random_quaternion(rng), d, maxd)[0]
+    casts["max_distance"][7::50] = -1.0  # a negative maximum: no hit
+    casts["direction"][11::50] = 0.0  # no direction: no hit
+    return casts
+
+
+def test_shape_cast_batch_equals_single_casts_and_oracle(World, oracle):
+    """600 sphere, box and capsule casts in one pass over a settled pile: every record equals the
+    single-cast entry point's and the oracle's bit for bit, with and without a filter."""
+    b, s = scenes.mixed_pile_scene(n=300, seed=10)
+    s = s.copy()
+    s["is_trigger"][3::13] = 1
+    cfg = _abi.default_config()
+    gw = World(cfg, b, s)
+    for _ in range(80):
+        gw.step_fixed()
+    ow = oracle_from(oracle, gw, cfg, s)
+    casts = random_casts(np.random.default_rng(10), 600, -5, 5, 0.2, 6)
+    for f in (None, _abi.query_filter(include_triggers=False, ignore_body=int(s["owner"][20]))):
+        g = gw.shape_cast_batch(casts, f)
+        hits = int(g["hit"].sum())
+        print("batched casts: %d of %d hit, %d started inside" % (hits, len(casts), int(g["started_inside"].sum())),
+              flush=True)
+        assert hits > 250
+        for k in range(len(casts)):
+            one = gw.shape_cast(casts[k], f)
+            assert g[k].tobytes() == one.tobytes(), (k, g[k], one)
+            if k % 3 == 0:
+                assert g[k].tobytes() == ow.shape_cast(casts[k], f).tobytes(), k
+
+
+def test_shape_cast_batch_full_pile(World, oracle):
+    """The 16k-body pile after 200 steps: 16,384 casts in one pass, every 64th checked against the
+    oracle bit for bit."""
+    sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
+    cfg = _abi.default_config(**sc.config)
+    gw = World(cfg, sc.bodies, sc.shapes)
+    for _ in range(200):
+        gw.step_fixed()
+    casts = random_casts(np.random.default_rng(12), 16384, -34, 34, 5, 45, scale=3.0)
+    gw.shape_cast_batch(casts[:256])
+    t0 = time.perf_counter()
+    g = gw.shape_cast_batch(casts)
+    dt = time.perf_counter() - t0
+    print("full pile: %d casts in %.2f ms (host to host), %d hit" % (len(casts), 1e3 * dt, int(g["hit"].sum())),
+          flush=True)
+    ow = oracle_from(oracle, gw, cfg, sc.shapes)
+    for k in range(0, len(casts), 64):
+        assert g[k].tobytes() == ow.shape_cast(casts[k]).tobytes(), k
+    assert g["hit"].mean() > 0.15
