@@ -390,6 +390,12 @@ int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query_filter* fi
  * min(cap, n) are copied. */
 int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_filter* filter, uint32_t* bodies,
                 uint32_t cap, uint32_t* n);
+/* n overlap queries in one GPU pass (the batched form of pxg_overlap; the reference answers one
+ * per call): counts[i] receives the number of overlapping shapes of queries[i], and
+ * bodies[i * cap ... i * cap + min(cap, counts[i]) - 1] their owners, in shape-slot order, exactly
+ * as pxg_overlap lists them. bodies holds n * cap entries. */
+int pxg_overlap_batch(pxg_world* w, const pxg_overlap_query* queries, uint32_t n, const pxg_query_filter* filter,
+                      uint32_t* bodies, uint32_t cap, uint32_t* counts);
 /* The first body a swept sphere, box or capsule touches (shape_cast() / sphere_cast() /
  * box_cast() / capsule_cast(), world.h:877-950; query/sweep.h:422-505: conservative advancement on
  * the exact closest distance for round shapes, a swept separating-axis test for box against box,

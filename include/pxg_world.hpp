@@ -606,6 +606,25 @@ public:
         return out;
     }
 
+    // Many overlap queries in one GPU pass (no counterpart in the reference, which answers one per
+    // call): out[i] lists what the single query returns (all of it when it fits in cap entries;
+    // counts[i] is always the full count).
+    void overlap_batch(const std::vector<pxg_overlap_query>& queries, std::vector<std::vector<BodyId>>& out,
+                       std::vector<uint32_t>& counts, uint32_t cap = 64, const QueryFilter& filter = QueryFilter()) {
+        push();
+        pxg_query_filter f = to_filter(filter);
+        const uint32_t n = static_cast<uint32_t>(queries.size());
+        std::vector<uint32_t> raw(size_t(n) * cap);
+        counts.assign(n, 0u);
+        if (n) check(pxg_overlap_batch(w_, queries.data(), n, &f, raw.data(), cap, counts.data()), "overlap_batch");
+        out.assign(n, std::vector<BodyId>());
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t k = 0; k < counts[i] && k < cap; ++k) {
+                const uint32_t b = raw[size_t(i) * cap + k];
+                out[i].push_back(BodyId{b, b < generations_.size() ? generations_[b] : 0u});
+            }
+    }
+
     // Many shape casts in one GPU pass (no counterpart in the reference, which casts one shape per
     // call): hits[i] is valid where found[i] != 0 and equals what the single cast returns.
     void shape_cast_batch(const std::vector<pxg_cast_query>& casts, std::vector<RaycastHit>& hits,

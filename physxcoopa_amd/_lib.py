@@ -46,6 +46,8 @@ _SIGNATURES = {
     "pxg_clear_events": ([ctypes.c_void_p], ctypes.c_int),
     "pxg_raycast": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_shape_cast": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
+    "pxg_overlap_batch": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
     "pxg_shape_cast_batch": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p],
                              ctypes.c_int),
     "pxg_compute_penetration": ([ctypes.c_void_p] * 6 + [ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),

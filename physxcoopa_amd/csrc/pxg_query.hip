@@ -457,11 +457,9 @@ __device__ __forceinline__ bool shape_overlaps(const QShape& s, const OverlapDev
     if (t == SH_BOX) return capsule_box(qc, sb, m);
     return capsule_capsule(qc, sc, n, pen, p);
 }
-__global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, OverlapDev q, QueryFilterDev f,
-                          uint32_t* flags) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.ns) return;
-    v3 qmn, qmx;
+// the query shape's bounds (world.h:1002-1060: sphere centre +- r, OBB::bounds, capsule end
+// points +- r)
+__device__ __forceinline__ void overlap_query_bounds(const OverlapDev& q, v3& qmn, v3& qmx) {
     if (q.type == SH_SPHERE) {
         qmn = ld3(q.c) - splat(q.r);
         qmx = ld3(q.c) + splat(q.r);
@@ -475,6 +473,13 @@ __global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, Ov
         qmn = vmin(ld3(q.a), ld3(q.b)) - splat(q.r);
         qmx = vmax(ld3(q.a), ld3(q.b)) + splat(q.r);
     }
+}
+__global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, OverlapDev q, QueryFilterDev f,
+                          uint32_t* flags) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.ns) return;
+    v3 qmn, qmx;
+    overlap_query_bounds(q, qmn, qmx);
     const QShape s = load_qshape(d, geom, qinfo, j);
     uint32_t out = 0u;
     if (accepts(s.info, f)) {
@@ -485,6 +490,44 @@ __global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, Ov
         if (box_hit && shape_overlaps(s, q)) out = s.info.x + 1u;
     }
     flags[j] = out;
+}
+// Many overlap queries: one wavefront per query scans the shapes in slot order, 64 at a time, and
+// appends the owners of overlapping shapes through a ballot (slot order kept); counts[i] is the
+// total, bodies[i * cap ...] holds the first min(cap, total).
+__global__ void __launch_bounds__(256) k_overlap_batch(DevState d, const float4* geom, const uint4* qinfo,
+                                                       const OverlapDev* queries, uint32_t n, QueryFilterDev f,
+                                                       uint32_t* bodies, uint32_t cap, uint32_t* counts) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t qi = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (qi >= n) return;  // the whole wavefront leaves together; the kernel has no block barrier
+    const OverlapDev q = queries[qi];
+    v3 qmn, qmx;
+    overlap_query_bounds(q, qmn, qmx);
+    uint32_t total = 0u;
+    for (uint32_t base = 0; base < d.ns; base += 64u) {
+        const uint32_t j = base + lane;
+        bool hit = false;
+        uint32_t owner = 0u;
+        if (j < d.ns) {
+            const uint4 info = qinfo[j];
+            if (accepts(info, f)) {
+                const v3 fmn = ld3(d.s_fmin[j]), fmx = ld3(d.s_fmax[j]);
+                if (fmn.x <= qmx.x && fmx.x >= qmn.x && fmn.y <= qmx.y && fmx.y >= qmn.y && fmn.z <= qmx.z &&
+                    fmx.z >= qmn.z) {
+                    const QShape s = load_qshape(d, geom, qinfo, j);
+                    hit = shape_overlaps(s, q);
+                    owner = info.x;
+                }
+            }
+        }
+        const unsigned long long mask = __ballot(hit ? 1 : 0);
+        if (hit) {
+            const uint32_t k = total + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+            if (k < cap) bodies[size_t(qi) * cap + k] = owner;
+        }
+        total += uint32_t(__popcll(mask));
+    }
+    if (lane == 0) counts[qi] = total;
 }
 
 // ---- shape casts: sphere, capsule and box (query/sweep.h:68-505, world.h:877-909) ----
@@ -984,6 +1027,11 @@ void launch_shape_cast(const DevState& d, const uint4* qinfo, const CastDev& c, 
                        unsigned long long* best, float4* hit, hipStream_t st) {
     if (d.ns) hipLaunchKernelGGL(k_shape_cast, dim3((d.ns + 255u) / 256u), dim3(256), 0, st, d, qinfo, c, f, rec, best);
     hipLaunchKernelGGL(k_cast_pick, dim3(1), dim3(1), 0, st, rec, best, hit);
+}
+
+void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev* queries, uint32_t n,
+                          const QueryFilterDev& f, uint32_t* bodies, uint32_t cap, uint32_t* counts, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_overlap_batch, dim3((n + 3u) / 4u), dim3(256), 0, st, d, geom, qinfo, queries, n, f, bodies, cap, counts);
 }
 
 void launch_overlap(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev& q, const QueryFilterDev& f,

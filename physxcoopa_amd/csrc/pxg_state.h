@@ -273,6 +273,10 @@ void launch_penetration(const DevState& d, const uint4* qinfo, const CastDev& c,
 // cast over the tiles its swept AABB meets; hits: 3 float4 per cast (the pxg_raycast_hit layout)
 hipError_t launch_shape_cast_batch(const DevState& d, const QueryScratch& q, const CastDev* casts, uint32_t n,
                                    const QueryFilterDev& f, float4* hits, hipStream_t st);
+// many overlap queries, one wavefront each: counts[i] overlapping shapes, the owners of the first
+// min(cap, counts[i]) in slot order at bodies[i * cap]
+void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev* queries, uint32_t n,
+                          const QueryFilterDev& f, uint32_t* bodies, uint32_t cap, uint32_t* counts, hipStream_t st);
 size_t query_sort_bytes(uint32_t cap);
 size_t query_shape_record_bytes();
 void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);

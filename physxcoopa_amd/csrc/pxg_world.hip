@@ -86,6 +86,11 @@ struct pxg_world {
     unsigned long long* q_best = nullptr;  // shape casts: nearest (distance, slot) key
     CastDev* q_casts = nullptr;  // batched shape casts, q_casts_cap records
     uint32_t q_casts_cap = 0;
+    OverlapDev* q_ov = nullptr;  // batched overlaps: queries and counts (q_ov_cap), bodies (q_ov_out_cap)
+    uint32_t* q_ov_counts = nullptr;
+    uint32_t* q_ov_out = nullptr;
+    uint32_t q_ov_cap = 0;
+    size_t q_ov_out_cap = 0;
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -1044,6 +1049,62 @@ extern "C" int pxg_raycast_all(pxg_world* w, const pxg_ray* ray, const pxg_query
     return PXG_OK;
 }
 
+static OverlapDev overlap_dev(const pxg_overlap_query& query) {
+    OverlapDev q;
+    q.type = query.type;
+    q.r = query.radius;
+    q.c = make_float4(query.center[0], query.center[1], query.center[2], 0.0f);
+    q.he = make_float4(query.half_extents[0], query.half_extents[1], query.half_extents[2], 0.0f);
+    q.q = make_float4(query.orientation[0], query.orientation[1], query.orientation[2], query.orientation[3]);
+    q.a = make_float4(query.a[0], query.a[1], query.a[2], 0.0f);
+    q.b = make_float4(query.b[0], query.b[1], query.b[2], 0.0f);
+    return q;
+}
+
+extern "C" int pxg_overlap_batch(pxg_world* w, const pxg_overlap_query* queries, uint32_t n, const pxg_query_filter* filter,
+                                 uint32_t* bodies, uint32_t cap, uint32_t* counts) {
+    if (!w || (n && (!queries || !counts)) || (n && cap && !bodies))
+        return fail(PXG_ERR_INVALID, "pxg_overlap_batch: null argument");
+    if (!n) return PXG_OK;
+    std::vector<OverlapDev> q(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (queries[i].type > PXG_CAPSULE)
+            return fail(PXG_ERR_INVALID, "pxg_overlap_batch: query " + std::to_string(i) + " is not a sphere, box or capsule");
+        q[i] = overlap_dev(queries[i]);
+    }
+    int r = query_setup(w, 1u);
+    if (r != PXG_OK) return r;
+    const size_t out_n = std::max<size_t>(size_t(n) * cap, 1);
+    if (n > w->q_ov_cap || out_n > w->q_ov_out_cap) {
+        HIPCHK(hipStreamSynchronize(w->stream));
+        if (n > w->q_ov_cap) {
+            dfree(w, w->q_ov);
+            dfree(w, w->q_ov_counts);
+            w->q_ov = nullptr;
+            w->q_ov_counts = nullptr;
+            w->q_ov_cap = 0;
+            DALLOC(w->q_ov, n);
+            DALLOC(w->q_ov_counts, n);
+            w->q_ov_cap = n;
+        }
+        if (out_n > w->q_ov_out_cap) {
+            dfree(w, w->q_ov_out);
+            w->q_ov_out = nullptr;
+            w->q_ov_out_cap = 0;
+            DALLOC(w->q_ov_out, out_n);
+            w->q_ov_out_cap = out_n;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(w->q_ov, q.data(), size_t(n) * sizeof(OverlapDev), hipMemcpyHostToDevice, w->stream));
+    launch_overlap_batch(w->d, w->q.geom, w->q.info, w->q_ov, n, query_filter(filter), w->q_ov_out, cap, w->q_ov_counts,
+                         w->stream);
+    HIPCHK(hipMemcpyAsync(counts, w->q_ov_counts, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream));
+    if (cap)
+        HIPCHK(hipMemcpyAsync(bodies, w->q_ov_out, size_t(n) * cap * sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
 extern "C" int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_filter* filter, uint32_t* bodies,
                            uint32_t cap, uint32_t* n) {
     if (!w || !query || !n || (cap && !bodies)) return fail(PXG_ERR_INVALID, "pxg_overlap: null argument");
@@ -1052,14 +1113,7 @@ extern "C" int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const p
     // one flag per shape: the ray-hit buffer (one record per shape at least) holds them
     int r = query_setup(w, std::max<uint32_t>(w->n_shapes, 1u));
     if (r != PXG_OK) return r;
-    OverlapDev q;
-    q.type = query->type;
-    q.r = query->radius;
-    q.c = make_float4(query->center[0], query->center[1], query->center[2], 0.0f);
-    q.he = make_float4(query->half_extents[0], query->half_extents[1], query->half_extents[2], 0.0f);
-    q.q = make_float4(query->orientation[0], query->orientation[1], query->orientation[2], query->orientation[3]);
-    q.a = make_float4(query->a[0], query->a[1], query->a[2], 0.0f);
-    q.b = make_float4(query->b[0], query->b[1], query->b[2], 0.0f);
+    const OverlapDev q = overlap_dev(*query);
     uint32_t* flags = reinterpret_cast<uint32_t*>(w->q_hits);
     launch_overlap(w->d, w->q.geom, w->q.info, q, query_filter(filter), flags, w->stream);
     std::vector<uint32_t> h(w->n_shapes);

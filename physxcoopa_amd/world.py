@@ -158,6 +158,17 @@ class World:
         check(lib().pxg_shape_cast(self.h, ptr(c), f, ptr(out)), "pxg_shape_cast")
         return out[0]
 
+    def overlap_batch(self, queries, query_filter=None, cap=64):
+        """Many overlap queries in one GPU pass (an OVERLAP_DTYPE array in): (counts, bodies), where
+        bodies[i, :min(cap, counts[i])] lists what overlap(queries[i]) returns."""
+        q = np.ascontiguousarray(queries, dtype=_abi.OVERLAP_DTYPE).reshape(-1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        counts = np.zeros(len(q), np.uint32)
+        bodies = np.zeros((len(q), cap), np.uint32)
+        if len(q):
+            check(lib().pxg_overlap_batch(self.h, ptr(q), len(q), f, ptr(bodies), cap, ptr(counts)), "pxg_overlap_batch")
+        return counts, bodies
+
     def shape_cast_batch(self, casts, query_filter=None):
         """Many shape casts in one GPU pass (a CAST_DTYPE array in): a HIT_DTYPE array, entry i the
         same record shape_cast(casts[i]) returns."""

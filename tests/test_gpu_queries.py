@@ -298,3 +298,64 @@ def test_shape_cast_batch_full_pile(World, oracle):
     for k in range(0, len(casts), 64):
         assert g[k].tobytes() == ow.shape_cast(casts[k]).tobytes(), k
     assert g["hit"].mean() > 0.15
+
+
+def random_overlaps(rng, n, lo, hi, zlo, zhi):
+    q = np.zeros(n, _abi.OVERLAP_DTYPE)
+    for k in range(n):
+        c = rng.uniform([lo, lo, zlo], [hi, hi, zhi])
+        if k % 3 == 0:
+            q[k] = _abi.overlap_sphere(c, rng.uniform(0.2, 1.5))[0]
+        elif k % 3 == 1:
+            q[k] = _abi.overlap_box(c, rng.uniform(0.2, 1.2, 3), scenes.random_quaternion(rng))[0]
+        else:
+            q[k] = _abi.overlap_capsule(c, c + rng.uniform(-1, 1, 3), rng.uniform(0.1, 0.6))[0]
+    return q
+
+
+def test_overlap_batch_equals_single_queries_and_oracle(World, oracle):
+    """500 overlap queries in one pass over a settled pile: each list equals the single-query entry
+    point's (and the oracle's), in slot order; a small capacity truncates the list but not the count."""
+    b, s = scenes.mixed_pile_scene(n=300, seed=11)
+    cfg = _abi.default_config()
+    gw = World(cfg, b, s)
+    for _ in range(80):
+        gw.step_fixed()
+    ow = oracle_from(oracle, gw, cfg, s)
+    q = random_overlaps(np.random.default_rng(11), 500, -5, 5, 0, 3)
+    for f in (None, _abi.query_filter(include_triggers=False, ignore_body=int(s["owner"][30]))):
+        counts, bodies = gw.overlap_batch(q, f, cap=64)
+        small_counts, small = gw.overlap_batch(q, f, cap=3)
+        assert np.array_equal(counts, small_counts)
+        print("batched overlaps: %d bodies over %d queries, max %d" % (int(counts.sum()), len(q), int(counts.max())),
+              flush=True)
+        assert counts.sum() > 500 and counts.max() <= 64
+        for k in range(len(q)):
+            one = gw.overlap(q[k], f)
+            assert counts[k] == len(one) and np.array_equal(bodies[k, :len(one)], one), (k, one, bodies[k])
+            assert np.array_equal(small[k, :min(3, len(one))], one[:3])
+            if k % 5 == 0:
+                assert np.array_equal(ow.overlap(q[k], f), one), k
+
+
+def test_overlap_batch_full_pile(World, oracle):
+    """The 16k-body pile after 200 steps: 16,384 overlap queries in one pass, every 64th checked
+    against the oracle."""
+    sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
+    cfg = _abi.default_config(**sc.config)
+    gw = World(cfg, sc.bodies, sc.shapes)
+    for _ in range(200):
+        gw.step_fixed()
+    q = random_overlaps(np.random.default_rng(13), 16384, -34, 34, 0, 20)
+    gw.overlap_batch(q[:256])
+    t0 = time.perf_counter()
+    counts, bodies = gw.overlap_batch(q, cap=128)
+    dt = time.perf_counter() - t0
+    print("full pile: %d overlap queries in %.2f ms (host to host), %d bodies" % (len(q), 1e3 * dt, int(counts.sum())),
+          flush=True)
+    assert counts.max() <= 128
+    ow = oracle_from(oracle, gw, cfg, sc.shapes)
+    for k in range(0, len(q), 64):
+        o = ow.overlap(q[k])
+        assert counts[k] == len(o) and np.array_equal(bodies[k, :len(o)], o), k
+    assert (counts > 0).mean() > 0.1
