@@ -204,13 +204,15 @@ def bench_learned(args):
 
 
 def bench_queries(args):
-    """Scene raycasts on the 16,384-body pile after the warm-up steps: K batches of 65,536 rays
+    """Scene queries on the 16,384-body pile after the warm-up steps: K batches of 65,536 rays
     (from 45 m above the pile, downward with a normal jitter of 0.3, max distance 200 m), closest
     hit per ray. The timed batches run on rays resident in device memory (pxg_raycast_device: the
     per-shape world geometry pass and the raycast kernel); the host-buffer rate (pxg_raycast, PCIe
     copies and one wait included) is reported beside it. No CPU baseline: the oracle tests every
     shape per ray (the result the reference's tree walk gives), which would understate the
-    reference's own tree-culled CPU cost."""
+    reference's own tree-culled CPU cost. Beside it: K batches of 16,384 mixed sphere / box /
+    capsule casts (from 5-45 m, downward-biased, max distance 3-36 m) and of 16,384 mixed overlap
+    queries over the pile (host buffers both)."""
     import numpy as np
     import torch
     from physxcoopa_amd import scenes, _abi
@@ -245,6 +247,36 @@ def bench_queries(args):
         host_hits = w.raycast(rays)
     el_host = time.perf_counter() - t1
     assert dev_hits.tobytes() == host_hits.tobytes(), "device and host entry points differ"
+    # batched shape casts and overlap queries over the same pile (host buffers, one pass each)
+    nq = 16384
+    casts = np.zeros(nq, _abi.CAST_DTYPE)
+    ovq = np.zeros(nq, _abi.OVERLAP_DTYPE)
+    for k in range(nq):
+        c = rng.uniform([-34, -34, 5], [34, 34, 45])
+        dv = rng.normal(size=3)
+        dv[2] -= 1.0
+        maxd = rng.uniform(3, 36)
+        q = scenes.random_quaternion(rng)
+        if k % 3 == 0:
+            casts[k] = _abi.sphere_cast(c, rng.uniform(0.1, 0.6), dv, maxd)[0]
+            ovq[k] = _abi.overlap_sphere(c * (1, 1, 0.25), rng.uniform(0.2, 1.5))[0]
+        elif k % 3 == 1:
+            casts[k] = _abi.box_cast(c, rng.uniform(0.1, 0.5, 3), q, dv, maxd)[0]
+            ovq[k] = _abi.overlap_box(c * (1, 1, 0.25), rng.uniform(0.2, 1.2, 3), q)[0]
+        else:
+            casts[k] = _abi.capsule_cast(c, rng.uniform(0.1, 0.4), rng.uniform(0.1, 0.6), k % 3, q, dv, maxd)[0]
+            cc = c * (1, 1, 0.25)
+            ovq[k] = _abi.overlap_capsule(cc, cc + rng.uniform(-1, 1, 3), rng.uniform(0.1, 0.6))[0]
+    cast_hits = w.shape_cast_batch(casts)
+    ov_counts, _ = w.overlap_batch(ovq, cap=64)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        w.shape_cast_batch(casts)
+    el_cast = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        w.overlap_batch(ovq, cap=64)
+    el_ov = time.perf_counter() - t3
     pairs = float(n) * len(sc.shapes)
     flop = 26.0 * pairs  # slab test against the fat AABB per (ray, shape) pair
     out = {
@@ -255,6 +287,12 @@ def bench_queries(args):
         "config": {"workload": "raycasts_config2_pile_16384", "rays_per_batch": n, "shapes": int(len(sc.shapes)),
                    "hit_fraction": round(float(dev_hits["hit"].mean()), 4)},
         "host_buffers": {"value": round(n * args.steps / el_host, 1), "ms_per_batch": round(1000.0 * el_host / args.steps, 4)},
+        "batched_casts": {"casts_per_s": round(nq * args.steps / el_cast, 1), "ms_per_batch": round(1000.0 * el_cast / args.steps, 4),
+                          "casts_per_batch": nq, "hit_fraction": round(float(cast_hits["hit"].mean()), 4),
+                          "kernel": "k_shape_cast_batch (host buffers)"},
+        "batched_overlaps": {"queries_per_s": round(nq * args.steps / el_ov, 1), "ms_per_batch": round(1000.0 * el_ov / args.steps, 4),
+                             "queries_per_batch": nq, "bodies_found": int(ov_counts.sum()),
+                             "kernel": "k_overlap_batch (host buffers)"},
         "roofline": {"bound": "fp32", "achieved": round(flop * args.steps / el / 1e12, 3), "peak": 157.3,
                      "unit": "TFLOP/s", "frac": round(flop * args.steps / el / 1e12 / 157.3, 5), "traffic": None,
                      "kernel": "k_raycast (+ k_query_prep)", "flop_per_batch": flop},
