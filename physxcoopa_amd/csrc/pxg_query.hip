@@ -491,12 +491,35 @@ __global__ void k_overlap(DevState d, const float4* geom, const uint4* qinfo, Ov
     }
     flags[j] = out;
 }
-// Many overlap queries: one wavefront per query scans the shapes in slot order, 64 at a time, and
-// appends the owners of overlapping shapes through a ballot (slot order kept); counts[i] is the
-// total, bodies[i * cap ...] holds the first min(cap, total).
+// one wavefront per 64-slot chunk: the box of the chunk's live fat AABBs (an inverted box when it
+// has none), so that a query whose bounds miss it can skip every shape in it
+__global__ void __launch_bounds__(256) k_chunk_bounds(DevState d, const uint4* qinfo, float4* chunk_box) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t chunk = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t j = chunk * 64u + lane;
+    if (chunk * 64u >= d.ns) return;  // whole wavefronts only; no block barrier
+    float v[6] = {3.0e38f, 3.0e38f, 3.0e38f, -3.0e38f, -3.0e38f, -3.0e38f};
+    if (j < d.ns && (qinfo[j].z & QF_LIVE)) {
+        const float4 a = d.s_fmin[j], b = d.s_fmax[j];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = b.x; v[4] = b.y; v[5] = b.z;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int k = 0; k < 6; ++k) {
+            const float o = __shfl_xor(v[k], off, 64);
+            v[k] = k < 3 ? gmin(v[k], o) : gmax(v[k], o);
+        }
+    if (lane == 0) {
+        chunk_box[2 * chunk] = make_float4(v[0], v[1], v[2], 0.0f);
+        chunk_box[2 * chunk + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+    }
+}
+// Many overlap queries: one wavefront per query scans the shapes in slot order, 64 at a time
+// (chunks whose box misses the query's bounds skipped: every live fat AABB of a chunk lies in its
+// box, so none of them could pass), and appends the owners of overlapping shapes through a ballot
+// (slot order kept); counts[i] is the total, bodies[i * cap ...] holds the first min(cap, total).
 __global__ void __launch_bounds__(256) k_overlap_batch(DevState d, const float4* geom, const uint4* qinfo,
-                                                       const OverlapDev* queries, uint32_t n, QueryFilterDev f,
-                                                       uint32_t* bodies, uint32_t cap, uint32_t* counts) {
+                                                       const float4* chunk_box, const OverlapDev* queries, uint32_t n,
+                                                       QueryFilterDev f, uint32_t* bodies, uint32_t cap, uint32_t* counts) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t qi = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (qi >= n) return;  // the whole wavefront leaves together; the kernel has no block barrier
@@ -504,7 +527,19 @@ __global__ void __launch_bounds__(256) k_overlap_batch(DevState d, const float4*
     v3 qmn, qmx;
     overlap_query_bounds(q, qmn, qmx);
     uint32_t total = 0u;
-    for (uint32_t base = 0; base < d.ns; base += 64u) {
+    const uint32_t chunks = (d.ns + 63u) / 64u;
+    // each lane tests one chunk box; the wavefront then visits the chunks of the ballot's set bits
+    // in ascending order (one dependent load per 64 chunks instead of one per chunk)
+    for (uint32_t cbase = 0; cbase < chunks; cbase += 64u) {
+      bool need = false;
+      if (cbase + lane < chunks) {
+          const v3 cmn = ld3(chunk_box[2 * (cbase + lane)]), cmx = ld3(chunk_box[2 * (cbase + lane) + 1]);
+          need = cmn.x <= qmx.x && cmx.x >= qmn.x && cmn.y <= qmx.y && cmx.y >= qmn.y && cmn.z <= qmx.z && cmx.z >= qmn.z;
+      }
+      unsigned long long visit = __ballot(need ? 1 : 0);
+      while (visit) {
+        const uint32_t base = (cbase + uint32_t(__ffsll(visit) - 1)) * 64u;
+        visit &= visit - 1ull;
         const uint32_t j = base + lane;
         bool hit = false;
         uint32_t owner = 0u;
@@ -526,6 +561,7 @@ __global__ void __launch_bounds__(256) k_overlap_batch(DevState d, const float4*
             if (k < cap) bodies[size_t(qi) * cap + k] = owner;
         }
         total += uint32_t(__popcll(mask));
+      }
     }
     if (lane == 0) counts[qi] = total;
 }
@@ -941,10 +977,17 @@ __global__ void __launch_bounds__(64 * kCastWaves) k_shape_cast_batch(DevState d
         world_bounds(c.shape, pos + ud * c.maxd, rot, mn1, mx1);
         const v3 smn = vmin(mn0, mn1), smx = vmax(mx0, mx1);
         const uint32_t ntiles = (d.ns + kQueryTile - 1u) / kQueryTile;
-        for (uint32_t tix = 0; tix < ntiles; ++tix) {
-            const v3 tmn = ld3(tile_box[2 * tix]), tmx = ld3(tile_box[2 * tix + 1]);
-            if (!(tmn.x <= smx.x && tmx.x >= smn.x && tmn.y <= smx.y && tmx.y >= smn.y && tmn.z <= smx.z && tmx.z >= smn.z))
-                continue;
+        // each lane tests one tile box; the wavefront visits the tiles of the ballot's set bits
+        for (uint32_t tbase = 0; tbase < ntiles; tbase += 64u) {
+          bool need = false;
+          if (tbase + lane < ntiles) {
+              const v3 tmn = ld3(tile_box[2 * (tbase + lane)]), tmx = ld3(tile_box[2 * (tbase + lane) + 1]);
+              need = tmn.x <= smx.x && tmx.x >= smn.x && tmn.y <= smx.y && tmx.y >= smn.y && tmn.z <= smx.z && tmx.z >= smn.z;
+          }
+          unsigned long long visit = __ballot(need ? 1 : 0);
+          while (visit) {
+            const uint32_t tix = tbase + uint32_t(__ffsll(visit) - 1);
+            visit &= visit - 1ull;
             const uint32_t base = tix * kQueryTile;
             const uint32_t m = d.ns - base < kQueryTile ? d.ns - base : kQueryTile;
             for (uint32_t k = lane; k < m; k += 64u) {
@@ -973,6 +1016,7 @@ __global__ void __launch_bounds__(64 * kCastWaves) k_shape_cast_batch(DevState d
                     bowner = info.x;
                 }
             }
+          }
         }
     }
     unsigned long long wmin = best;
@@ -1029,9 +1073,14 @@ void launch_shape_cast(const DevState& d, const uint4* qinfo, const CastDev& c, 
     hipLaunchKernelGGL(k_cast_pick, dim3(1), dim3(1), 0, st, rec, best, hit);
 }
 
-void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev* queries, uint32_t n,
-                          const QueryFilterDev& f, uint32_t* bodies, uint32_t cap, uint32_t* counts, hipStream_t st) {
-    if (n) hipLaunchKernelGGL(k_overlap_batch, dim3((n + 3u) / 4u), dim3(256), 0, st, d, geom, qinfo, queries, n, f, bodies, cap, counts);
+void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, float4* chunk_box,
+                          const OverlapDev* queries, uint32_t n, const QueryFilterDev& f, uint32_t* bodies, uint32_t cap,
+                          uint32_t* counts, hipStream_t st) {
+    if (!n) return;
+    const uint32_t chunks = (d.ns + 63u) / 64u;
+    if (chunks) hipLaunchKernelGGL(k_chunk_bounds, dim3((chunks + 3u) / 4u), dim3(256), 0, st, d, qinfo, chunk_box);
+    hipLaunchKernelGGL(k_overlap_batch, dim3((n + 3u) / 4u), dim3(256), 0, st, d, geom, qinfo,
+                       static_cast<const float4*>(chunk_box), queries, n, f, bodies, cap, counts);
 }
 
 void launch_overlap(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev& q, const QueryFilterDev& f,

@@ -243,6 +243,7 @@ struct QueryScratch {
     uint32_t *skey, *skey2, *sidx, *sidx2;  // per shape slot
     uint32_t *rkey, *rkey2, *ridx, *ridx2;  // per ray
     void* sorted;       // per shape slot: query_shape_record_bytes() each
+    float4* chunk_box;  // box of the live fat AABBs of each 64-slot chunk, in slot order: 2 per chunk
     float4* tile_box;   // 2 per tile of 256 shapes
     void* sort_tmp;
     size_t sort_bytes;
@@ -274,9 +275,11 @@ void launch_penetration(const DevState& d, const uint4* qinfo, const CastDev& c,
 hipError_t launch_shape_cast_batch(const DevState& d, const QueryScratch& q, const CastDev* casts, uint32_t n,
                                    const QueryFilterDev& f, float4* hits, hipStream_t st);
 // many overlap queries, one wavefront each: counts[i] overlapping shapes, the owners of the first
-// min(cap, counts[i]) in slot order at bodies[i * cap]
-void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, const OverlapDev* queries, uint32_t n,
-                          const QueryFilterDev& f, uint32_t* bodies, uint32_t cap, uint32_t* counts, hipStream_t st);
+// min(cap, counts[i]) in slot order at bodies[i * cap]; chunk_box (2 float4 per 64-slot chunk) is
+// filled first and lets a wavefront skip the chunks its query's bounds miss
+void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, float4* chunk_box,
+                          const OverlapDev* queries, uint32_t n, const QueryFilterDev& f, uint32_t* bodies, uint32_t cap,
+                          uint32_t* counts, hipStream_t st);
 size_t query_sort_bytes(uint32_t cap);
 size_t query_shape_record_bytes();
 void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);

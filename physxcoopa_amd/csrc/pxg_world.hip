@@ -948,6 +948,7 @@ static int query_setup(pxg_world* w, uint32_t n) {
         DALLOC(rec, S * query_shape_record_bytes());
         q.sorted = rec;
         DALLOC(q.tile_box, 2 * ((S + 255) / 256) + 2);
+        DALLOC(q.chunk_box, 2 * ((S + 63) / 64) + 2);
         DALLOC(w->q_count, 1);
     }
     if (n > w->q_cap || !q.sort_tmp) {
@@ -1096,7 +1097,7 @@ extern "C" int pxg_overlap_batch(pxg_world* w, const pxg_overlap_query* queries,
         }
     }
     HIPCHK(hipMemcpyAsync(w->q_ov, q.data(), size_t(n) * sizeof(OverlapDev), hipMemcpyHostToDevice, w->stream));
-    launch_overlap_batch(w->d, w->q.geom, w->q.info, w->q_ov, n, query_filter(filter), w->q_ov_out, cap, w->q_ov_counts,
+    launch_overlap_batch(w->d, w->q.geom, w->q.info, w->q.chunk_box, w->q_ov, n, query_filter(filter), w->q_ov_out, cap, w->q_ov_counts,
                          w->stream);
     HIPCHK(hipMemcpyAsync(counts, w->q_ov_counts, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream));
     if (cap)
