@@ -531,37 +531,37 @@ __global__ void __launch_bounds__(256) k_overlap_batch(DevState d, const float4*
     // each lane tests one chunk box; the wavefront then visits the chunks of the ballot's set bits
     // in ascending order (one dependent load per 64 chunks instead of one per chunk)
     for (uint32_t cbase = 0; cbase < chunks; cbase += 64u) {
-      bool need = false;
-      if (cbase + lane < chunks) {
-          const v3 cmn = ld3(chunk_box[2 * (cbase + lane)]), cmx = ld3(chunk_box[2 * (cbase + lane) + 1]);
-          need = cmn.x <= qmx.x && cmx.x >= qmn.x && cmn.y <= qmx.y && cmx.y >= qmn.y && cmn.z <= qmx.z && cmx.z >= qmn.z;
-      }
-      unsigned long long visit = __ballot(need ? 1 : 0);
-      while (visit) {
-        const uint32_t base = (cbase + uint32_t(__ffsll(visit) - 1)) * 64u;
-        visit &= visit - 1ull;
-        const uint32_t j = base + lane;
-        bool hit = false;
-        uint32_t owner = 0u;
-        if (j < d.ns) {
-            const uint4 info = qinfo[j];
-            if (accepts(info, f)) {
-                const v3 fmn = ld3(d.s_fmin[j]), fmx = ld3(d.s_fmax[j]);
-                if (fmn.x <= qmx.x && fmx.x >= qmn.x && fmn.y <= qmx.y && fmx.y >= qmn.y && fmn.z <= qmx.z &&
-                    fmx.z >= qmn.z) {
-                    const QShape s = load_qshape(d, geom, qinfo, j);
-                    hit = shape_overlaps(s, q);
-                    owner = info.x;
+        bool need = false;
+        if (cbase + lane < chunks) {
+            const v3 cmn = ld3(chunk_box[2 * (cbase + lane)]), cmx = ld3(chunk_box[2 * (cbase + lane) + 1]);
+            need = cmn.x <= qmx.x && cmx.x >= qmn.x && cmn.y <= qmx.y && cmx.y >= qmn.y && cmn.z <= qmx.z && cmx.z >= qmn.z;
+        }
+        unsigned long long visit = __ballot(need ? 1 : 0);
+        while (visit) {
+            const uint32_t base = (cbase + uint32_t(__ffsll(visit) - 1)) * 64u;
+            visit &= visit - 1ull;
+            const uint32_t j = base + lane;
+            bool hit = false;
+            uint32_t owner = 0u;
+            if (j < d.ns) {
+                const uint4 info = qinfo[j];
+                if (accepts(info, f)) {
+                    const v3 fmn = ld3(d.s_fmin[j]), fmx = ld3(d.s_fmax[j]);
+                    if (fmn.x <= qmx.x && fmx.x >= qmn.x && fmn.y <= qmx.y && fmx.y >= qmn.y && fmn.z <= qmx.z &&
+                        fmx.z >= qmn.z) {
+                        const QShape s = load_qshape(d, geom, qinfo, j);
+                        hit = shape_overlaps(s, q);
+                        owner = info.x;
+                    }
                 }
             }
+            const unsigned long long mask = __ballot(hit ? 1 : 0);
+            if (hit) {
+                const uint32_t k = total + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+                if (k < cap) bodies[size_t(qi) * cap + k] = owner;
+            }
+            total += uint32_t(__popcll(mask));
         }
-        const unsigned long long mask = __ballot(hit ? 1 : 0);
-        if (hit) {
-            const uint32_t k = total + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-            if (k < cap) bodies[size_t(qi) * cap + k] = owner;
-        }
-        total += uint32_t(__popcll(mask));
-      }
     }
     if (lane == 0) counts[qi] = total;
 }
@@ -979,44 +979,45 @@ __global__ void __launch_bounds__(64 * kCastWaves) k_shape_cast_batch(DevState d
         const uint32_t ntiles = (d.ns + kQueryTile - 1u) / kQueryTile;
         // each lane tests one tile box; the wavefront visits the tiles of the ballot's set bits
         for (uint32_t tbase = 0; tbase < ntiles; tbase += 64u) {
-          bool need = false;
-          if (tbase + lane < ntiles) {
-              const v3 tmn = ld3(tile_box[2 * (tbase + lane)]), tmx = ld3(tile_box[2 * (tbase + lane) + 1]);
-              need = tmn.x <= smx.x && tmx.x >= smn.x && tmn.y <= smx.y && tmx.y >= smn.y && tmn.z <= smx.z && tmx.z >= smn.z;
-          }
-          unsigned long long visit = __ballot(need ? 1 : 0);
-          while (visit) {
-            const uint32_t tix = tbase + uint32_t(__ffsll(visit) - 1);
-            visit &= visit - 1ull;
-            const uint32_t base = tix * kQueryTile;
-            const uint32_t m = d.ns - base < kQueryTile ? d.ns - base : kQueryTile;
-            for (uint32_t k = lane; k < m; k += 64u) {
-                const QShape& s = sorted[base + k];
-                if (!accepts(s.info, f)) continue;
-                const v3 fmn = ld3(s.fmin), fmx = ld3(s.fmax);
-                if (!(fmn.x <= smx.x && fmx.x >= smn.x && fmn.y <= smx.y && fmx.y >= smn.y && fmn.z <= smx.z &&
-                      fmx.z >= smn.z))
-                    continue;
-                const uint32_t j = __float_as_uint(s.fmin.w);
-                const uint4 info = d.s_info[j];
-                const ShapeGeom target = decode_shape(d, j, info);
-                float dist;
-                v3 nn, pp;
-                bool started;
-                if (!sweep_convex(c.shape, pos, rot, ud, c.maxd, target, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]), dist,
-                                  nn, pp, started))
-                    continue;
-                const unsigned long long key = (static_cast<unsigned long long>(f2o(dist)) << 32) | j;
-                if (key < best) {
-                    best = key;
-                    bd = dist;
-                    bn = nn;
-                    bp = pp;
-                    bs = started;
-                    bowner = info.x;
+            bool need = false;
+            if (tbase + lane < ntiles) {
+                const v3 tmn = ld3(tile_box[2 * (tbase + lane)]), tmx = ld3(tile_box[2 * (tbase + lane) + 1]);
+                need = tmn.x <= smx.x && tmx.x >= smn.x && tmn.y <= smx.y && tmx.y >= smn.y && tmn.z <= smx.z &&
+                       tmx.z >= smn.z;
+            }
+            unsigned long long visit = __ballot(need ? 1 : 0);
+            while (visit) {
+                const uint32_t tix = tbase + uint32_t(__ffsll(visit) - 1);
+                visit &= visit - 1ull;
+                const uint32_t base = tix * kQueryTile;
+                const uint32_t m = d.ns - base < kQueryTile ? d.ns - base : kQueryTile;
+                for (uint32_t k = lane; k < m; k += 64u) {
+                    const QShape& s = sorted[base + k];
+                    if (!accepts(s.info, f)) continue;
+                    const v3 fmn = ld3(s.fmin), fmx = ld3(s.fmax);
+                    if (!(fmn.x <= smx.x && fmx.x >= smn.x && fmn.y <= smx.y && fmx.y >= smn.y && fmn.z <= smx.z &&
+                          fmx.z >= smn.z))
+                        continue;
+                    const uint32_t j = __float_as_uint(s.fmin.w);
+                    const uint4 info = d.s_info[j];
+                    const ShapeGeom target = decode_shape(d, j, info);
+                    float dist;
+                    v3 nn, pp;
+                    bool started;
+                    if (!sweep_convex(c.shape, pos, rot, ud, c.maxd, target, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]),
+                                      dist, nn, pp, started))
+                        continue;
+                    const unsigned long long key = (static_cast<unsigned long long>(f2o(dist)) << 32) | j;
+                    if (key < best) {
+                        best = key;
+                        bd = dist;
+                        bn = nn;
+                        bp = pp;
+                        bs = started;
+                        bowner = info.x;
+                    }
                 }
             }
-          }
         }
     }
     unsigned long long wmin = best;
