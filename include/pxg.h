@@ -415,6 +415,13 @@ int pxg_shape_cast_batch(pxg_world* w, const pxg_cast_query* casts, uint32_t n, 
  * copied. */
 int pxg_compute_penetration(pxg_world* w, const pxg_shape* shape, const float position[3], const float orientation[4],
                             const pxg_query_filter* filter, pxg_penetration* out, uint32_t cap, uint32_t* n);
+/* n compute_penetration queries in one GPU pass (the batched form of pxg_compute_penetration; the
+ * reference answers one per call): probes[i] gives the shape, position and orientation (its
+ * direction and max_distance are not used); counts[i] receives its number of penetrations and
+ * out[i * cap ... i * cap + min(cap, counts[i]) - 1] the records, exactly as
+ * pxg_compute_penetration returns them. out holds n * cap records. */
+int pxg_compute_penetration_batch(pxg_world* w, const pxg_cast_query* probes, uint32_t n, const pxg_query_filter* filter,
+                                  pxg_penetration* out, uint32_t cap, uint32_t* counts);
 /* FNV-1a over alive bodies' position, orientation, velocities (world_state_hash, world.h:1334). */
 int pxg_state_hash(pxg_world* w, uint64_t* out);
 

@@ -50,6 +50,8 @@ _SIGNATURES = {
                            ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
     "pxg_shape_cast_batch": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p],
                              ctypes.c_int),
+    "pxg_compute_penetration_batch": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
     "pxg_compute_penetration": ([ctypes.c_void_p] * 6 + [ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
     "pxg_overlap": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p],
                     ctypes.c_int),

@@ -1061,7 +1061,74 @@ __global__ void k_penetration(DevState d, const uint4* qinfo, CastDev c, QueryFi
     rec[3 * k + 2] = make_float4(__uint_as_float(info.x), 0.0f, 0.0f, 0.0f);
 }
 
+// Many compute_penetration queries: one wavefront per posed probe, the shapes in slot order with
+// the 64-slot chunk boxes culled as in k_overlap_batch; penetrations are compacted through a
+// ballot into out[(i * cap + k) * 10 ...] as pxg_penetration records (normal, depth, point, body,
+// shape, 0); counts[i] is the full count.
+__global__ void __launch_bounds__(256) k_penetration_batch(DevState d, const uint4* qinfo, const float4* chunk_box,
+                                                           const CastDev* probes, uint32_t n, QueryFilterDev f,
+                                                           float* out, uint32_t cap, uint32_t* counts) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t qi = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (qi >= n) return;  // the whole wavefront leaves together; the kernel has no block barrier
+    const CastDev c = probes[qi];
+    const v3 pos = ld3(c.pos);
+    const quat rot = ldq(c.rot);
+    v3 qmn, qmx;
+    world_bounds(c.shape, pos, rot, qmn, qmx);
+    const uint32_t chunks = (d.ns + 63u) / 64u;
+    uint32_t total = 0u;
+    for (uint32_t cbase = 0; cbase < chunks; cbase += 64u) {
+        bool need = false;
+        if (cbase + lane < chunks) {
+            const v3 cmn = ld3(chunk_box[2 * (cbase + lane)]), cmx = ld3(chunk_box[2 * (cbase + lane) + 1]);
+            need = cmn.x <= qmx.x && cmx.x >= qmn.x && cmn.y <= qmx.y && cmx.y >= qmn.y && cmn.z <= qmx.z && cmx.z >= qmn.z;
+        }
+        unsigned long long visit = __ballot(need ? 1 : 0);
+        while (visit) {
+            const uint32_t j = (cbase + uint32_t(__ffsll(visit) - 1)) * 64u + lane;
+            visit &= visit - 1ull;
+            bool hit = false;
+            v3 n3 = mk(0.0f, 0.0f, 0.0f), p3 = n3;
+            float depth = 0.0f;
+            uint32_t owner = 0u;
+            if (j < d.ns && accepts(qinfo[j], f)) {
+                const v3 fmn = ld3(d.s_fmin[j]), fmx = ld3(d.s_fmax[j]);
+                if (fmn.x <= qmx.x && fmx.x >= qmn.x && fmn.y <= qmx.y && fmx.y >= qmn.y && fmn.z <= qmx.z &&
+                    fmx.z >= qmn.z) {
+                    const uint4 info = d.s_info[j];
+                    const ShapeGeom target = decode_shape(d, j, info);
+                    hit = convex_penetration(c.shape, pos, rot, target, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]), n3,
+                                             depth, p3);
+                    owner = info.x;
+                }
+            }
+            const unsigned long long mask = __ballot(hit ? 1 : 0);
+            if (hit) {
+                const uint32_t k = total + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+                if (k < cap) {
+                    float* o = out + (size_t(qi) * cap + k) * 10u;
+                    o[0] = n3.x; o[1] = n3.y; o[2] = n3.z; o[3] = depth;
+                    o[4] = p3.x; o[5] = p3.y; o[6] = p3.z;
+                    o[7] = __uint_as_float(owner); o[8] = __uint_as_float(j); o[9] = 0.0f;
+                }
+            }
+            total += uint32_t(__popcll(mask));
+        }
+    }
+    if (lane == 0) counts[qi] = total;
+}
+
 }  // namespace
+
+void launch_penetration_batch(const DevState& d, const uint4* qinfo, float4* chunk_box, const CastDev* probes, uint32_t n,
+                              const QueryFilterDev& f, float* out, uint32_t cap, uint32_t* counts, hipStream_t st) {
+    if (!n) return;
+    const uint32_t chunks = (d.ns + 63u) / 64u;
+    if (chunks) hipLaunchKernelGGL(k_chunk_bounds, dim3((chunks + 3u) / 4u), dim3(256), 0, st, d, qinfo, chunk_box);
+    hipLaunchKernelGGL(k_penetration_batch, dim3((n + 3u) / 4u), dim3(256), 0, st, d, qinfo,
+                       static_cast<const float4*>(chunk_box), probes, n, f, out, cap, counts);
+}
 
 void launch_penetration(const DevState& d, const uint4* qinfo, const CastDev& c, const QueryFilterDev& f, float4* rec,
                         uint32_t* count, hipStream_t st) {

@@ -280,6 +280,11 @@ hipError_t launch_shape_cast_batch(const DevState& d, const QueryScratch& q, con
 void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, float4* chunk_box,
                           const OverlapDev* queries, uint32_t n, const QueryFilterDev& f, uint32_t* bodies, uint32_t cap,
                           uint32_t* counts, hipStream_t st);
+// many penetration probes (dir and maxd unused), one wavefront each, chunk-culled like the batched
+// overlaps: counts[i] penetrations, the first min(cap, counts[i]) in slot order as pxg_penetration
+// records (10 words) at out[i * cap * 10]
+void launch_penetration_batch(const DevState& d, const uint4* qinfo, float4* chunk_box, const CastDev* probes, uint32_t n,
+                              const QueryFilterDev& f, float* out, uint32_t cap, uint32_t* counts, hipStream_t st);
 size_t query_sort_bytes(uint32_t cap);
 size_t query_shape_record_bytes();
 void launch_query_prep(const DevState& d, float4* geom, uint4* qinfo, uint32_t* bounds6, hipStream_t st);

@@ -91,6 +91,8 @@ struct pxg_world {
     uint32_t* q_ov_out = nullptr;
     uint32_t q_ov_cap = 0;
     size_t q_ov_out_cap = 0;
+    float* q_pen = nullptr;  // batched penetrations: q_pen_cap words
+    size_t q_pen_cap = 0;
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -1255,6 +1257,60 @@ extern "C" int pxg_compute_penetration(pxg_world* w, const pxg_shape* shape, con
         o.reserved = 0u;
     }
     *n = k;
+    return PXG_OK;
+}
+
+extern "C" int pxg_compute_penetration_batch(pxg_world* w, const pxg_cast_query* probes, uint32_t n,
+                                             const pxg_query_filter* filter, pxg_penetration* out, uint32_t cap,
+                                             uint32_t* counts) {
+    static_assert(sizeof(pxg_penetration) == 10 * sizeof(float), "pxg_penetration is 10 words");
+    if (!w || (n && (!probes || !counts)) || (n && cap && !out))
+        return fail(PXG_ERR_INVALID, "pxg_compute_penetration_batch: null argument");
+    if (!n) return PXG_OK;
+    std::vector<CastDev> c(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (probes[i].shape.type > PXG_CAPSULE)
+            return fail(PXG_ERR_INVALID, "pxg_compute_penetration_batch: probe " + std::to_string(i) +
+                                             " is not a sphere, a box or a capsule");
+        c[i] = posed_probe(probes[i].shape, probes[i].position, probes[i].orientation);
+    }
+    int r = query_setup(w, 1u);
+    if (r != PXG_OK) return r;
+    const size_t out_words = std::max<size_t>(size_t(n) * cap * 10u, 1);
+    if (n > w->q_casts_cap || n > w->q_ov_cap || out_words > w->q_pen_cap) {
+        HIPCHK(hipStreamSynchronize(w->stream));
+        if (n > w->q_casts_cap) {
+            dfree(w, w->q_casts);
+            w->q_casts = nullptr;
+            w->q_casts_cap = 0;
+            DALLOC(w->q_casts, n);
+            w->q_casts_cap = n;
+        }
+        if (n > w->q_ov_cap) {
+            dfree(w, w->q_ov);
+            dfree(w, w->q_ov_counts);
+            w->q_ov = nullptr;
+            w->q_ov_counts = nullptr;
+            w->q_ov_cap = 0;
+            DALLOC(w->q_ov, n);
+            DALLOC(w->q_ov_counts, n);
+            w->q_ov_cap = n;
+        }
+        if (out_words > w->q_pen_cap) {
+            dfree(w, w->q_pen);
+            w->q_pen = nullptr;
+            w->q_pen_cap = 0;
+            DALLOC(w->q_pen, out_words);
+            w->q_pen_cap = out_words;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(w->q_casts, c.data(), size_t(n) * sizeof(CastDev), hipMemcpyHostToDevice, w->stream));
+    launch_penetration_batch(w->d, w->q.info, w->q.chunk_box, w->q_casts, n, query_filter(filter), w->q_pen, cap,
+                             w->q_ov_counts, w->stream);
+    HIPCHK(hipMemcpyAsync(counts, w->q_ov_counts, size_t(n) * sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream));
+    if (cap)
+        HIPCHK(hipMemcpyAsync(out, w->q_pen, size_t(n) * cap * sizeof(pxg_penetration), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
     return PXG_OK;
 }
 

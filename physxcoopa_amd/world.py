@@ -179,6 +179,18 @@ class World:
             check(lib().pxg_shape_cast_batch(self.h, ptr(c), len(c), f, ptr(out)), "pxg_shape_cast_batch")
         return out
 
+    def compute_penetration_batch(self, probes, query_filter=None, cap=16):
+        """Many compute_penetration queries in one GPU pass (a CAST_DTYPE array of posed probes in):
+        (counts, records), where records[i, :min(cap, counts[i])] equals compute_penetration(probes[i])."""
+        c = np.ascontiguousarray(probes, dtype=_abi.CAST_DTYPE).reshape(-1)
+        f = None if query_filter is None else ptr(np.ascontiguousarray(query_filter, dtype=_abi.FILTER_DTYPE))
+        counts = np.zeros(len(c), np.uint32)
+        out = np.zeros((len(c), cap), _abi.PEN_DTYPE)
+        if len(c):
+            check(lib().pxg_compute_penetration_batch(self.h, ptr(c), len(c), f, ptr(out), cap, ptr(counts)),
+                  "pxg_compute_penetration_batch")
+        return counts, out
+
     def compute_penetration(self, probe, query_filter=None):
         """Every shape the posed query shape penetrates, with its push-out (world.h:963-995): the
         shape, position and orientation of a CAST_DTYPE record (direction and max_distance unused)

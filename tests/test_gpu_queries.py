@@ -359,3 +359,28 @@ def test_overlap_batch_full_pile(World, oracle):
         o = ow.overlap(q[k])
         assert counts[k] == len(o) and np.array_equal(bodies[k, :len(o)], o), k
     assert (counts > 0).mean() > 0.1
+
+
+def test_penetration_batch_equals_single_queries(World, oracle):
+    """300 posed probes in one pass over a settled pile: each record list equals the single-query
+    entry point's (and the oracle's for every 4th), in slot order; a capacity of 2 truncates the
+    list but not the count."""
+    b, s = scenes.mixed_pile_scene(n=300, seed=12)
+    cfg = _abi.default_config()
+    gw = World(cfg, b, s)
+    for _ in range(80):
+        gw.step_fixed()
+    ow = oracle_from(oracle, gw, cfg, s)
+    probes = random_casts(np.random.default_rng(12), 300, -5, 5, 0, 3)
+    for f in (None, _abi.query_filter(include_triggers=False, ignore_body=int(s["owner"][40]))):
+        counts, recs = gw.compute_penetration_batch(probes, f, cap=16)
+        small_counts, small = gw.compute_penetration_batch(probes, f, cap=2)
+        assert np.array_equal(counts, small_counts) and counts.max() <= 16
+        print("batched penetrations: %d over %d probes" % (int(counts.sum()), len(probes)), flush=True)
+        assert counts.sum() > 300
+        for k in range(len(probes)):
+            one = gw.compute_penetration(probes[k], f)
+            assert counts[k] == len(one) and recs[k, :len(one)].tobytes() == one.tobytes(), k
+            assert small[k, :min(2, len(one))].tobytes() == one[:2].tobytes(), k
+            if k % 4 == 0:
+                assert ow.compute_penetration(probes[k], f).tobytes() == one.tobytes(), k
