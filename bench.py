@@ -1,11 +1,12 @@
 """Benchmark of the rigid-body contact step on one MI355X.
 
 Default workload (N=1): the 16,384-body mixed pile (40% boxes, 40% spheres, 20% capsules over a
-200x200x1 m ground), dt 1/60. W untimed warm-up steps (8 velocity, 2 relax, 3 position
-iterations) let the pile fall and settle; then each solver configuration runs K timed steps in
-residual mode (velocity passes until the residual <= 1e-4, at most 256), inputs resident in HBM.
-A "step" is one full step_fixed: broadphase, narrowphase, colouring, warm start, velocity,
-relax and position passes, cache rebuild, islands/sleep and integration.
+200x200x1 m ground), dt 1/60. --settle steps (default 540; 8 velocity, 2 relax, 3 position
+iterations) let the pile fall and settle whatever --warmup says; then each solver configuration
+runs W untimed and K timed steps in residual mode (velocity passes until the residual <= 1e-4, at
+most --cap), inputs resident in HBM. A "step" is one full step_fixed: broadphase, narrowphase,
+colouring, warm start, velocity, relax and position passes, cache rebuild, islands/sleep and
+integration.
 
 Two configurations of the GPU solver are measured from the same scene: plain (the reference's
 solve order, plain warm start) and batched + learned (greedy colour batches, learned warm start);
@@ -13,7 +14,9 @@ the headline value is the latter. Prints ONE JSON line: value = dynamic body-ste
 the plain solver's figures beside it, the roofline of the dominant kernel (the persistent solve
 kernel, timed with HIP events on the world's stream; algorithmic bytes 192 per manifold + 40 per
 contact point per velocity pass), and the CPU oracle (the reference algorithm restated in C++,
-one core) timed on a bounded sample of the same settled pile with the same residual policy.
+one core) timed on the first timed steps of the plain world, continued from its state with the
+same residual policy (the same steps with the same pass counts: the reference order is
+bit-identical on both sides).
 
 Other workloads: --workload batch (4096 independent scenes), ragdolls (2048 jointed ragdoll
 scenes, learned warm start on), learned (config-1 piles, passes to the residual with and without
@@ -221,7 +224,7 @@ def bench_queries(args):
     sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
     cfg = _abi.default_config(**sc.config)
     w = World(cfg, sc.bodies, sc.shapes)
-    for _ in range(args.warmup):
+    for _ in range(args.settle + args.warmup):
         w.step_fixed()
     w.synchronize()
     n = 65536
@@ -313,59 +316,75 @@ def pile_world(sc, order, net):
 
 def timed_steps(w, steps, per_step_stats):
     """Times `steps` step_fixed calls (device synchronised on both sides); returns (seconds,
-    velocity passes, contact points, max residual, profile of the solve kernel)."""
+    per-step records or None, statistics of the last step, profile of the solve kernel). With
+    per_step_stats every step is followed by a statistics read (one wait), which gives each
+    step's wall time, velocity passes, contact points and residual."""
     w.set_profiling(True)
     w.profile()  # reset
     w.synchronize()
-    passes = pts = 0
-    resid = 0.0
+    per = [] if per_step_stats else None
     t0 = time.perf_counter()
+    t_prev = t0
     for _ in range(steps):
         w.step_fixed()
         if per_step_stats:
             st = w.stats()
-            passes += st["velocity_passes"]
-            pts += st["points"]
-            resid = max(resid, st["residual"])
+            t = time.perf_counter()
+            per.append({"seconds": t - t_prev, "passes": int(st["velocity_passes"]), "points": int(st["points"]),
+                        "residual": float(st["residual"])})
+            t_prev = t
     w.synchronize()
     elapsed = time.perf_counter() - t0
     prof = w.profile()
     w.set_profiling(False)
-    if not per_step_stats:
-        st = w.stats()
-        passes, pts, resid = st["velocity_passes"] * steps, st["points"] * steps, None
-    return elapsed, passes, pts, resid, prof
+    return elapsed, per, w.stats(), prof
 
 
-def roofline_of(prof, kernel="k_solve_persistent", traffic=None):
+def roofline_of(prof, kernel="k_solve_persistent"):
     vel_s = prof["velocity_ms"] / 1000.0
     achieved = prof["velocity_bytes"] / vel_s / 1e9 if vel_s > 0 else 0.0
     n = max(1, prof["velocity_launches"])
+    # traffic: HBM bytes per launch need rocprofv3 PMC counters, which this run does not collect
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
             "launches": int(prof["velocity_launches"]), "avg_launch_us": round(1e6 * vel_s / n, 3),
             "bytes_per_launch": round(prof["velocity_bytes"] / n, 1)}
 
 
-def measured_traffic(key):
-    """HBM bytes per launch of the solve kernel from the committed rocprofv3 PMC summary of the same
-    workload (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or None."""
+def profiled_traffic(key):
+    """The solve kernel's PMC traffic (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel
+    calibration) from the committed rocprofv3 summary of an earlier profiling run, with the state
+    of the pile it was measured on, or None. Not a measurement of this run."""
     path = os.path.join(ROOT, "profiles", "solve_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f).get(key)
-        return None if t is None else round(float(t["hbm_bytes_per_launch"]), 1)
-    except (OSError, ValueError, KeyError):
+    except (OSError, ValueError):
         return None
+    if not t:
+        return None
+    out = {"source": "profiles/solve_traffic.json", "key": key}
+    out.update(t)
+    return out
+
+
+def pass_summary(per):
+    """Velocity passes and residuals of per-step records: whether each step reached the tolerance."""
+    import numpy as np
+    passes = np.array([r["passes"] for r in per])
+    return {"min": int(passes.min()), "median": float(np.median(passes)), "mean": round(float(passes.mean()), 1),
+            "max": int(passes.max()), "per_step": [int(x) for x in passes]}
 
 
 def bench_pile(args):
     """The 16,384-body pile, two configurations from the same scene: the plain GPU solver (the
     reference's solve order, plain warm start) and the batched + learned one (greedy colour
-    batches, learned warm start). Each is warmed up with the fixed iteration counts (8 velocity,
-    2 relax, 3 position), timed for 60 steps at those counts, then switched to residual mode
-    (velocity passes until the residual <= 1e-4, at most 256) and timed for K steps. The headline
-    value is the batched + learned solver in residual mode."""
+    batches, learned warm start). Each world first settles the pile for --settle steps at the fixed
+    iteration counts (8 velocity, 2 relax, 3 position), is timed for --fixed-steps steps at those
+    counts, then switches to residual mode (velocity passes until the residual <= 1e-4, at most
+    --cap) for W untimed and K timed steps. The headline value is the batched + learned solver in
+    residual mode; every step's pass count and final residual are reported, and the line says
+    whether each timed step reached the tolerance."""
     import numpy as np
     from physxcoopa_amd import scenes, _abi, learned
     t0 = time.time()
@@ -375,55 +394,80 @@ def bench_pile(args):
     net = learned.MlpParams.load(learned.DEFAULT_NET)
     runs = {}
     worlds = {}
+    snap = None
     plan = [("plain", _abi.PXG_ORDER_REFERENCE, None), ("batched_learned", _abi.PXG_ORDER_COLOURED, net)]
     if args.only:
         plan = [p for p in plan if p[0] == args.only]
+    tol = 1e-4
     for name, order, mlp in plan:
         w, cfg = pile_world(sc, order, mlp)
+        worlds[name] = w
         t1 = time.time()
-        for i in range(args.warmup):
+        for i in range(args.settle):
             w.step_fixed()
             if (i + 1) % 180 == 0:
                 w.synchronize()
-                log("%s warm-up %d/%d %.1fs" % (name, i + 1, args.warmup, time.time() - t1))
+                log("%s settle %d/%d %.1fs" % (name, i + 1, args.settle, time.time() - t1))
         w.synchronize()
-        el_f, passes_f, pts_f, _, prof_f = timed_steps(w, args.fixed_steps, False)
+        el_f, _, st_f, prof_f = timed_steps(w, args.fixed_steps, False)
         res_cfg = _abi.default_config(**sc.config)
         res_cfg.solve_order = order
-        res_cfg.residual_mode, res_cfg.residual_tolerance, res_cfg.residual_max_iterations = 1, 1e-4, 256
+        res_cfg.residual_mode, res_cfg.residual_tolerance, res_cfg.residual_max_iterations = 1, tol, args.cap
+        res_cfg.residual_check_interval = args.check_every
         w.set_config(res_cfg)
-        el, passes, pts, resid, prof = timed_steps(w, args.steps, True)
-        st = w.stats()
+        for i in range(args.warmup):
+            w.step_fixed()
+            st = w.stats()
+            log("%s residual-mode warm-up %d/%d: %d passes, residual %.3g" %
+                (name, i + 1, args.warmup, st["velocity_passes"], st["residual"]))
+        if name == "plain":
+            w.synchronize()
+            k, imp = w.warm_cache()
+            snap = (w.bodies(), w.fat_aabbs(), k, imp, res_cfg)
+        el, per, st, prof = timed_steps(w, args.steps, True)
+        reached = [r["residual"] <= tol for r in per]
         runs[name] = {"value": n_dyn * args.steps / el, "ms_per_step": 1000.0 * el / args.steps,
-                      "contacts_per_s": pts / el, "velocity_passes_per_step": passes / args.steps,
-                      "residual_max": resid, "fixed_iterations_value": n_dyn * args.fixed_steps / el_f,
+                      "contacts_per_s": sum(r["points"] for r in per) / el, "per": per,
+                      "passes": pass_summary(per), "residual_max": max(r["residual"] for r in per),
+                      "reached": sum(reached), "fixed_iterations_value": n_dyn * args.fixed_steps / el_f,
                       "fixed_iterations_ms_per_step": 1000.0 * el_f / args.fixed_steps,
                       "manifolds": st["manifolds"], "points": st["points"], "colours": st["colours"],
                       "awake_bodies": st["awake_bodies"], "learned_predictions": st["learned_predictions"],
-                      "prof": prof, "prof_fixed": prof_f, "cfg": res_cfg}
-        worlds[name] = w
-        log("%s: residual mode %.2f ms/step (%.1f passes), fixed %.2f ms/step" %
-            (name, runs[name]["ms_per_step"], runs[name]["velocity_passes_per_step"], runs[name]["fixed_iterations_ms_per_step"]))
+                      "fixed_manifolds": st_f["manifolds"], "prof": prof}
+        log("%s: residual mode %.2f ms/step (passes %s, %d/%d steps reached %g), fixed %.2f ms/step" %
+            (name, runs[name]["ms_per_step"], runs[name]["passes"]["per_step"], runs[name]["reached"], args.steps,
+             tol, runs[name]["fixed_iterations_ms_per_step"]))
     head = runs.get("batched_learned") or runs["plain"]
     cpu = None
-    if not args.no_cpu and "plain" in worlds:
+    if not args.no_cpu and snap is not None:
+        # The oracle continues from the plain GPU world's state at the start of its timed steps.
+        # In the reference order the GPU and the oracle are bit-identical, so the oracle's steps are
+        # the same steps with the same pass counts, and their times compare one to one.
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pxo
-        w = worlds["plain"]
-        ow = pxo.OracleWorld(runs["plain"]["cfg"], w.bodies(), sc.shapes)
-        ow.set_fat_aabbs(w.fat_aabbs())
-        k, imp = w.warm_cache()
+        bodies, fat, k, imp, res_cfg = snap
+        ow = pxo.OracleWorld(res_cfg, bodies, sc.shapes)
+        ow.set_fat_aabbs(fat)
         ow.set_warm_cache(k, imp)
-        n_cpu, t_cpu, p_cpu = 0, 0.0, 0
-        while t_cpu < args.cpu_seconds and n_cpu < args.steps:
+        n_cpu, t_cpu, p_cpu = 0, 0.0, []
+        log("CPU oracle: residual-mode steps from the plain world's state (budget %.0f s, at least one step)" %
+            args.cpu_seconds)
+        while n_cpu < args.steps and (n_cpu == 0 or t_cpu < args.cpu_seconds):
             t_cpu += ow.run(1)
+            p_cpu.append(int(ow.stats()["velocity_passes"]))
             n_cpu += 1
-            p_cpu += int(ow.stats()["velocity_passes"])
+            log("CPU oracle step %d: %d passes, %.1f s total" % (n_cpu, p_cpu[-1], t_cpu))
+        gpu_same = sum(r["seconds"] for r in runs["plain"]["per"][:n_cpu])
+        gpu_passes = [r["passes"] for r in runs["plain"]["per"][:n_cpu]]
         cpu = {"value": round(n_dyn * n_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": 1, "kind": "port",
-               "sample": "%d residual-mode steps (tolerance 1e-4, cap 256; %.1f passes/step) of the settled %d-body "
-                         "pile from the plain GPU world's state (C++ oracle, -O2, one thread)" %
-                         (n_cpu, p_cpu / n_cpu, n_dyn),
-               "ms_per_step": round(1000 * t_cpu / n_cpu, 2)}
+               "sample": "the first %d of the plain GPU world's timed residual-mode steps (tolerance 1e-4, cap %d) of "
+                         "the %d-body pile after %d settling steps, continued from its state (C++ oracle, -O2, one "
+                         "thread); velocity passes per step %s (GPU: %s)" %
+                         (n_cpu, args.cap, n_dyn, args.settle, p_cpu, gpu_passes),
+               "ms_per_step": round(1000 * t_cpu / n_cpu, 2),
+               "plain_gpu_same_steps_ms_per_step": round(1000 * gpu_same / n_cpu, 3),
+               "plain_gpu_vs_cpu_same_steps": round(t_cpu / gpu_same, 2) if gpu_same > 0 else None,
+               "passes_equal_gpu": p_cpu == gpu_passes}
     key = "config2_pile_16384/%s" % ("coloured_learned_residual" if head is runs.get("batched_learned") else "reference_residual")
     out = {
         "metric": "body·steps/s at solver residual <=1e-4 (plain vs batched+learned); HBM GB/s",
@@ -441,23 +485,34 @@ def bench_pile(args):
         "config": {"workload": sc.name, "dynamic_bodies": n_dyn,
                    "solver": "batched+learned GPU (greedy colour batches, learned warm start)" if "batched_learned" in runs
                    else "plain GPU (reference order)",
-                   "residual_tolerance": 1e-4, "residual_max_passes": 256, "warmup_iterations": "8 velocity, 2 relax, 3 position"},
-        "contacts_per_s": round(head["contacts_per_s"], 1),
-        "velocity_passes_per_step": round(head["velocity_passes_per_step"], 2),
+                   "residual_tolerance": tol, "residual_max_passes": args.cap,
+                   "residual_check_interval": args.check_every,
+                   "settle_steps": args.settle, "settle_iterations": "8 velocity, 2 relax, 3 position",
+                   "residual_mode_warmup_steps": args.warmup},
+        "residual_reached": head["reached"] == args.steps,
+        "steps_reaching_tolerance": head["reached"],
         "residual_max": round(float(head["residual_max"]), 7),
+        "velocity_passes": head["passes"],
+        "contacts_per_s": round(head["contacts_per_s"], 1),
         "manifolds": head["manifolds"], "points": head["points"], "colours": head["colours"],
         "awake_bodies": head["awake_bodies"], "learned_predictions_last_step": head["learned_predictions"],
         "fixed_iterations": {k: {"value": round(r["fixed_iterations_value"], 1),
-                                 "ms_per_step": round(r["fixed_iterations_ms_per_step"], 4)} for k, r in runs.items()},
+                                 "ms_per_step": round(r["fixed_iterations_ms_per_step"], 4),
+                                 "manifolds": r["fixed_manifolds"]} for k, r in runs.items()},
         "plain_gpu": None if "plain" not in runs else {
             "value": round(runs["plain"]["value"], 1), "ms_per_step": round(runs["plain"]["ms_per_step"], 4),
-            "velocity_passes_per_step": round(runs["plain"]["velocity_passes_per_step"], 2),
+            "residual_reached": runs["plain"]["reached"] == args.steps,
+            "steps_reaching_tolerance": runs["plain"]["reached"],
+            "velocity_passes": runs["plain"]["passes"],
             "residual_max": round(float(runs["plain"]["residual_max"]), 7)},
         "vs_plain_gpu": round(head["value"] / runs["plain"]["value"], 3) if "plain" in runs and "batched_learned" in runs else None,
         "vs_cpu": round(head["value"] / cpu["value"], 2) if cpu else None,
-        "roofline": roofline_of(head["prof"], traffic=measured_traffic(key)),
+        "roofline": roofline_of(head["prof"]),
+        "traffic_profiled": profiled_traffic(key),
         "cpu_baseline": cpu,
     }
+    for w in worlds.values():
+        w.close()
     print(json.dumps(out), flush=True)
 
 
@@ -465,7 +520,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20, help="timed steps (residual mode for the pile)")
-    ap.add_argument("--warmup", type=int, default=540)
+    ap.add_argument("--warmup", type=int, default=2,
+                    help="untimed steps before the timed ones (pile: residual-mode steps after settling; "
+                         "batch and queries: ordinary steps)")
+    ap.add_argument("--settle", type=int, default=540,
+                    help="pile and queries: steps at the fixed iteration counts that let the pile fall and settle "
+                         "before anything is timed")
+    ap.add_argument("--cap", type=int, default=16384, help="pile: most velocity passes per step in residual mode")
+    ap.add_argument("--check-every", type=int, default=8,
+                    help="pile: residual mode evaluates the residual after every k-th velocity pass (and the last)")
     ap.add_argument("--fixed-steps", type=int, default=60, help="pile: steps timed at the fixed iteration counts")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")

@@ -92,7 +92,11 @@ typedef struct pxg_config {
      * parallel (colours = longest chains of that order). PXG_ORDER_COLOURED: greedy colouring,
      * fewer colours (faster), a different but equally valid sequential order. */
     uint32_t solve_order;             /* PXG_ORDER_REFERENCE */
-    uint32_t reserved[6];
+    /* Residual mode: the residual is evaluated after every residual_check_interval-th velocity
+     * pass (and after the last one the cap allows); the passes between run without the residual
+     * sweep. 0 and 1 both mean every pass. */
+    uint32_t residual_check_interval; /* 1 */
+    uint32_t reserved[5];
 } pxg_config;
 
 enum pxg_solve_order { PXG_ORDER_REFERENCE = 0, PXG_ORDER_COLOURED = 1 };

@@ -39,6 +39,8 @@ def lib():
             "pxo_get_cache": ([vp, vp, vp, u32], u32), "pxo_set_cache": ([vp, vp, vp, u32], None),
             "pxo_state_hash": ([vp], u64), "pxo_stats": ([vp, vp], None),
             "pxo_residual_trace": ([vp, vp, u32], u32),
+            "pxo_want_residual_detail": ([vp, u32], None), "pxo_residual_detail": ([vp, vp, u32], u32),
+            "pxo_eval_residual": ([vp, vp, u32, u32, f32], ctypes.c_double),
             "pxo_generate_contacts": ([vp, vp, vp, vp, vp, vp, i32, vp], i32),
             "pxo_box_box": ([vp, vp, vp, vp, vp, vp, i32, vp], i32),
             "pxo_sat": ([vp, vp, vp, vp, vp, vp, i32, vp, vp, vp, vp], i32),
@@ -259,6 +261,27 @@ class OracleWorld:
         out = np.zeros(max(n, 1), np.float64)
         lib().pxo_residual_trace(self.h, ptr(out), n)
         return out[:n]
+
+    RESIDUAL_DETAIL_FIELDS = ("manifold", "point", "rn", "rt1", "rt2", "lambda_n", "lambda_t1", "lambda_t2", "vn",
+                              "vt1", "vt2", "target", "separation", "a_active", "b_active", "kn")
+
+    def want_residual_detail(self, on=True):
+        """Record every contact point's residual terms after the last velocity pass of each step."""
+        lib().pxo_want_residual_detail(self.h, 1 if on else 0)
+
+    def eval_residual(self, manifolds, use_restitution=False, h=None):
+        """The velocity residual of `manifolds` (MANIFOLD_DTYPE, impulses included) over the current
+        bodies, without solving; the per-point terms are then in residual_detail()."""
+        m = np.ascontiguousarray(manifolds, dtype=MANIFOLD_DTYPE).reshape(-1)
+        return float(lib().pxo_eval_residual(self.h, ptr(m), len(m), 1 if use_restitution else 0,
+                                             np.float32(self._cfg.fixed_dt if h is None else h)))
+
+    def residual_detail(self):
+        """The last step's per-point residual terms: a dict of float64 arrays (RESIDUAL_DETAIL_FIELDS)."""
+        n = lib().pxo_residual_detail(self.h, None, 0)
+        out = np.zeros((max(n, 1), 16), np.float64)
+        lib().pxo_residual_detail(self.h, ptr(out), n)
+        return {k: out[:n, i] for i, k in enumerate(self.RESIDUAL_DETAIL_FIELDS)}
 
 
 # ---- single-function helpers for known-answer tests ----

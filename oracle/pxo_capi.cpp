@@ -20,6 +20,7 @@ template <class T> void cfg_from(Config<T>& c, const pxg_config* in) {
     c.fixed_dt = T(in->fixed_dt); c.max_lin_vel = T(in->max_linear_velocity); c.exact_quat = in->use_exact_quaternion_integration != 0;
     c.gravity = V3<T>(T(in->gravity[0]), T(in->gravity[1]), T(in->gravity[2]));
     c.residual_mode = in->residual_mode != 0; c.residual_tol = T(in->residual_tolerance); c.residual_max = in->residual_max_iterations;
+    c.residual_every = in->residual_check_interval ? in->residual_check_interval : 1u;
 }
 template <class T> V3<T> v3(const float* f) { return V3<T>(T(f[0]), T(f[1]), T(f[2])); }
 template <class T> Q<T> q4(const float* f) { Q<T> q; q.x = T(f[0]); q.y = T(f[1]); q.z = T(f[2]); q.w = T(f[3]); return q; }
@@ -69,6 +70,21 @@ template <class T> void manifold_to(const Manifold<T>& m, pxg_manifold& d) {
         o.normal_impulse = float(c.ni); o.tangent_impulse[0] = float(c.ti[0]); o.tangent_impulse[1] = float(c.ti[1]);
         o.feature_id = c.fid; o.restitution_bias = float(c.rbias);
     }
+}
+
+template <class T> Manifold<T> manifold_from(const pxg_manifold& d) {
+    Manifold<T> m;
+    m.a = d.a; m.b = d.b; m.sa = d.shape_a; m.sb = d.shape_b;
+    m.n = v3<T>(d.normal); m.friction = T(d.friction); m.restitution = T(d.restitution);
+    m.count = int(d.count); m.trigger = d.is_trigger != 0; m.valid = d.valid != 0;
+    for (int i = 0; i < 4; ++i) {
+        const pxg_contact& o = d.points[i];
+        Contact<T>& c = m.p[i];
+        c.pos = v3<T>(o.position); c.pen = T(o.penetration); c.sep = T(o.separation);
+        c.ni = T(o.normal_impulse); c.ti[0] = T(o.tangent_impulse[0]); c.ti[1] = T(o.tangent_impulse[1]);
+        c.fid = o.feature_id; c.rbias = T(o.restitution_bias);
+    }
+    return m;
 }
 
 template <class T> Joint<T> joint_from(const pxg_joint& d) {
@@ -452,6 +468,37 @@ uint32_t pxo_residual_trace(void* p, double* out, uint32_t cap) {
     auto go = [&](auto& w) -> uint32_t {
         for (uint32_t i = 0; i < w.residual_trace.size() && i < cap; ++i) out[i] = double(w.residual_trace[i]);
         return uint32_t(w.residual_trace.size());
+    };
+    return h->dbl ? go(h->d) : go(h->f);
+}
+// The velocity residual of the given manifolds (impulses included) over the world's current
+// bodies, in the given order, without solving anything; the manifolds replace the world's. The
+// per-point terms go to the residual detail.
+double pxo_eval_residual(void* p, const pxg_manifold* ms, uint32_t n, uint32_t use_rest, float hstep) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) -> double {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        w.manifolds.clear();
+        for (uint32_t i = 0; i < n; ++i) w.manifolds.push_back(manifold_from<T>(ms[i]));
+        std::vector<Manifold<T>*> order;
+        for (auto& m : w.manifolds) order.push_back(&m);
+        return double(w.residual(order, use_rest != 0, T(hstep), &w.residual_detail));
+    };
+    return h->dbl ? go(h->d) : go(h->f);
+}
+
+// Per-point residual terms after the last velocity pass of each following step (16 doubles per
+// point, see World::residual_detail); on = 0 stops recording. Returns the point count.
+void pxo_want_residual_detail(void* p, uint32_t on) {
+    Handle* h = static_cast<Handle*>(p);
+    h->f.want_residual_detail = on != 0; h->d.want_residual_detail = on != 0;
+}
+uint32_t pxo_residual_detail(void* p, double* out, uint32_t cap) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) -> uint32_t {
+        for (uint32_t i = 0; i < w.residual_detail.size() && i < cap; ++i)
+            for (int k = 0; k < 16; ++k) out[16 * i + k] = w.residual_detail[i][k];
+        return uint32_t(w.residual_detail.size());
     };
     return h->dbl ? go(h->d) : go(h->f);
 }

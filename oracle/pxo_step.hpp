@@ -7,6 +7,7 @@
 #include "pxo_joints.hpp"
 
 #include <algorithm>
+#include <array>
 #include <iterator>
 #include <set>
 
@@ -53,6 +54,7 @@ template <class T> struct Config {
     bool residual_mode = false;
     T residual_tol = T(1e-4f);
     uint32_t residual_max = 256;
+    uint32_t residual_every = 1;  // residual evaluated after every k-th pass and the last
 };
 
 template <class T> struct World {
@@ -88,6 +90,11 @@ template <class T> struct World {
     uint32_t vel_passes = 0;
     T last_residual = 0;
     std::vector<T> residual_trace;
+    // per contact point after the last velocity pass (before the relax passes), when asked for:
+    // {manifold in solve order, point, rn, rt1, rt2, lambda_n, lambda_t1, lambda_t2, v_n, v_t1,
+    //  v_t2, target, separation, a active, b active, k_n}
+    bool want_residual_detail = false;
+    std::vector<std::array<double, 16>> residual_detail;
     uint64_t steps = 0;
     T accumulator = 0;
 
@@ -335,10 +342,11 @@ template <class T> struct World {
     // Velocity residual over active contact rows (m/s): for the normal row the complementarity
     // violation of (lambda_n, v_n - target), for each friction row the violation of the Coulomb
     // cone's stick/slip conditions. Evaluated on the current velocities and impulses.
-    T residual(std::vector<Manifold<T>*>& ms, bool use_rest, T h) {
+    T residual(std::vector<Manifold<T>*>& ms, bool use_rest, T h, std::vector<std::array<double, 16>>* detail = nullptr) {
         T r = 0;
-        for (Manifold<T>* mp : ms) {
-            Manifold<T>& m = *mp;
+        if (detail) detail->clear();
+        for (size_t mi = 0; mi < ms.size(); ++mi) {
+            Manifold<T>& m = *ms[mi];
             if (!m.valid || m.trigger) continue;
             Body<T>& a = bodies[m.a];
             Body<T>& b = bodies[m.b];
@@ -356,6 +364,7 @@ template <class T> struct World {
                 r = gmax(r, rn);
                 T maxf = m.friction * p.ni;
                 const V3<T>* tg[2] = {&t1, &t2};
+                T rts[2], vts[2];
                 for (int t = 0; t < 2; ++t) {
                     T vt = dot(dv, *tg[t]);
                     // stick: |lambda_t| < mu*lambda_n needs v_t = 0; slip: the friction impulse
@@ -363,6 +372,18 @@ template <class T> struct World {
                     T rt = T(0);
                     if (maxf > T(0)) rt = std::abs(p.ti[t]) < maxf ? std::abs(vt) : gmax(T(0), (p.ti[t] < T(0) ? -vt : vt));
                     r = gmax(r, rt);
+                    rts[t] = rt; vts[t] = vt;
+                }
+                if (detail) {
+                    bool ad = active(a), bd = active(b);
+                    T ima = ad ? a.inv_mass : T(0), imb = bd ? b.inv_mass : T(0);
+                    M3<T> Ia = ad ? a.iiw : M3<T>(T(0));
+                    M3<T> Ib = bd ? b.iiw : M3<T>(T(0));
+                    V3<T> ran = cross(ra, m.n), rbn = cross(rb, m.n);
+                    T kn = ima + imb + dot(ran, mul(Ia, ran)) + dot(rbn, mul(Ib, rbn));
+                    detail->push_back({double(mi), double(i), double(rn), double(rts[0]), double(rts[1]), double(p.ni),
+                                       double(p.ti[0]), double(p.ti[1]), double(vn), double(vts[0]), double(vts[1]),
+                                       double(target), double(p.sep), double(ad), double(bd), double(kn)});
                 }
             }
         }
@@ -506,24 +527,29 @@ template <class T> struct World {
         js.warm_start();
         residual_trace.clear();
         if (cfg.residual_mode) {
+            // at least one pass; the residual after every k-th pass and after the cap's last
             uint32_t i = 0;
+            const uint32_t every = std::max(1u, cfg.residual_every);
             T r = residual(ms, false, h);
             while (i < cfg.residual_max) {
-                if (i >= 1 && r <= cfg.residual_tol) break;
                 velocity_pass(ms, false, i, h);
                 js.velocity_pass(h, (i & 1u) != 0);
                 ++i;
+                if (i % every != 0u && i != cfg.residual_max) continue;
                 r = residual(ms, false, h);
                 residual_trace.push_back(r);
+                if (r <= cfg.residual_tol) break;
             }
             vel_passes = i;
             last_residual = r;
+            if (want_residual_detail) residual(ms, false, h, &residual_detail);
         } else {
             for (uint32_t i = 0; i < cfg.vel_iters; ++i) {
                 velocity_pass(ms, false, i, h);
                 js.velocity_pass(h, (i & 1u) != 0);
             }
             vel_passes = cfg.vel_iters;
+            if (want_residual_detail) residual(ms, false, h, &residual_detail);
         }
         for (uint32_t i = 0; i < cfg.relax_iters; ++i) {
             velocity_pass(ms, true, i, h);

@@ -195,7 +195,8 @@ class Config(ctypes.Structure):
         ("residual_max_iterations", ctypes.c_uint32), ("large_extent", ctypes.c_float),
         ("max_bodies", ctypes.c_uint32), ("max_shapes", ctypes.c_uint32), ("max_pairs", ctypes.c_uint32),
         ("max_manifolds", ctypes.c_uint32), ("learned_warm_start", ctypes.c_uint32),
-        ("solve_order", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6),
+        ("solve_order", ctypes.c_uint32), ("residual_check_interval", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 5),
     ]
 
 
@@ -237,6 +238,7 @@ def default_config(**overrides):
     c.residual_mode = 0
     c.residual_tolerance = 1e-4
     c.residual_max_iterations = 256
+    c.residual_check_interval = 1
     c.large_extent = 16.0
     c.learned_warm_start = 0
     c.solve_order = PXG_ORDER_REFERENCE

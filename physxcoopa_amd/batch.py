@@ -4,6 +4,7 @@ Each scene behaves like its own PhysicsWorld stepped with step_fixed(h); one `st
 advances every scene k times in one persistent, graph-captured launch.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -35,6 +36,11 @@ class Batch:
             self.h = None
 
     def __del__(self):
+        # At interpreter exit the HIP runtime may already be torn down (and a profiler's exit
+        # hooks may be running): destroying the handle then would call into a dead runtime.
+        # The process is ending and the driver reclaims the device memory, so skip it.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

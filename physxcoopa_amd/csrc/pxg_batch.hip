@@ -108,6 +108,7 @@ struct Cfg {
     uint32_t vel_iters, relax_iters, pos_iters, residual_mode, residual_max;
     float residual_tol;
     uint32_t steps;
+    uint32_t check_every;  // residual mode: the residual after every k-th pass and the last
 };
 
 __device__ __forceinline__ uint32_t hash32(uint32_t x) {
@@ -709,7 +710,7 @@ __device__ void scene_step(const BatchDev& bd, const Cfg& cfg, Lds& L, const Sce
     for (uint32_t it = 0; it < max_pass; ++it) {
         vpass(0, it);
         ++passes;
-        if (cfg.residual_mode) {
+        if (cfg.residual_mode && (passes % cfg.check_every == 0u || passes == max_pass)) {
             if (t == 0) L.residual = 0.0f;
             __syncthreads();
             float r = 0.0f;
@@ -1594,6 +1595,7 @@ extern "C" int pxg_batch_step(pxg_batch* b, float h, uint32_t steps) {
         c.vel_iters = g.velocity_iterations; c.relax_iters = g.relax_iterations; c.pos_iters = g.position_iterations;
         c.residual_mode = g.residual_mode; c.residual_max = g.residual_max_iterations ? g.residual_max_iterations : 256u;
         c.residual_tol = g.residual_tolerance; c.steps = steps;
+        c.check_every = g.residual_check_interval ? g.residual_check_interval : 1u;
         BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(pxb::k_batch_reset_queue, dim3(1), dim3(1), 0, b->stream, b->d.queue);
         hipError_t le = hipSuccess;

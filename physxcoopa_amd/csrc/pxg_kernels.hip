@@ -1684,6 +1684,7 @@ struct SolveCfg {
     float residual_tol;
     uint32_t one_xcd;  // launched on every XCD; the blocks of one carry on with near barriers
     uint32_t ready;    // residual mode: positions split by ready_list (residuals during the tail)
+    uint32_t check_every;  // residual mode: the residual after every k-th pass and the last
 };
 
 // A velocity-pass manifold's slot loaded ahead (points in the pass's point order, rows with the
@@ -1876,7 +1877,8 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
         // (final before the tail); the rest follow the tail's barrier. The tail's own block stays
         // out of it: residual gathers on its CU would slow the tail's chain of colours.
         float r = 0.0f;
-        const bool split = cfg.ready != 0u && L == 0u && T < C && g.nblocks > 1u;
+        const bool check = (it + 1u) % cfg.check_every == 0u || it + 1u == max_pass;
+        const bool split = check && cfg.ready != 0u && L == 0u && T < C && g.nblocks > 1u;
         if (T < C) {
             if (tail_wave) {
                 run_tail(co, T, C, true, [&](uint32_t s) { velocity_manifold(d, prm, s, 0, it); });
@@ -1894,6 +1896,12 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
             if (grid_sync(g, err)) return;
         }
         ++passes;
+        if (!check) {
+            // no residual this pass: the pass ended at the last colour's, the tail's or the last
+            // joint layer's barrier; load the next pass's first slot and go on
+            if (T && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, it + 1u, 0, y);
+            continue;
+        }
         r = gmax(r, residual_span(d, prm, split ? d.ready_list : nullptr, split ? d.ready_cnt[0] : 0u, n_active, tid, nthreads));
         for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
         if (lane0()) s_red[threadIdx.x >> 6] = r;
@@ -2391,13 +2399,14 @@ void launch_islands_sleep(const DevState& d, const StepParams& p, uint32_t bound
 }
 // cooperative launches; returns the HIP error of the launch. The barrier count is zeroed by a
 // kernel (not a memset) so the launch sequence may be captured in a graph.
-hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[7], float tol,
+hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[8], float tol,
                                    uint32_t nblocks, hipStream_t st) {
     SolveCfg c;
     c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
     c.residual_mode = cfgv[3]; c.residual_max = cfgv[4]; c.residual_tol = tol;
     c.one_xcd = cfgv[5];
     c.ready = cfgv[6];
+    c.check_every = cfgv[7] ? cfgv[7] : 1u;
     launch_fill_u32(d.bar, 0u, kBarWords, st);
     DevState dd = d;
     StepParams pp = p;
@@ -2406,7 +2415,7 @@ hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks),
                                       dim3(kPersistBlock), args, 0, st);
 }
-void launch_solve_bytes(const DevState& d, const uint32_t cfgv[7], uint32_t bound, hipStream_t st) {
+void launch_solve_bytes(const DevState& d, const uint32_t cfgv[8], uint32_t bound, hipStream_t st) {
     if (bound) hipLaunchKernelGGL(k_solve_bytes, dim3(blocks(bound)), dim3(kBlock), 0, st, d, cfgv[0], cfgv[1], cfgv[2], cfgv[3]);
 }
 hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint32_t nblocks, uint32_t reference,

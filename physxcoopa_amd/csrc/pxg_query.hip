@@ -311,7 +311,9 @@ __global__ void __launch_bounds__(kQueryTile) k_tile_bounds(DevState d, const QS
         if (threadIdx.x < h)
             for (int k = 0; k < 6; ++k) {
                 const float o = red[k][threadIdx.x + h];
-                red[k][threadIdx.x] = k < 3 ? gmin(red[k][threadIdx.x], o) : gmax(red[k][threadIdx.x], o);
+                // fminf / fmaxf drop a NaN operand: a shape with a non-finite fat box (which no
+                // box test accepts) cannot make the tile box NaN and hide the tile's other shapes
+                red[k][threadIdx.x] = k < 3 ? fminf(red[k][threadIdx.x], o) : fmaxf(red[k][threadIdx.x], o);
             }
         __syncthreads();
     }
@@ -506,7 +508,7 @@ __global__ void __launch_bounds__(256) k_chunk_bounds(DevState d, const uint4* q
     for (int off = 32; off > 0; off >>= 1)
         for (int k = 0; k < 6; ++k) {
             const float o = __shfl_xor(v[k], off, 64);
-            v[k] = k < 3 ? gmin(v[k], o) : gmax(v[k], o);
+            v[k] = k < 3 ? fminf(v[k], o) : fmaxf(v[k], o);  // NaN-dropping, as in k_tile_bounds
         }
     if (lane == 0) {
         chunk_box[2 * chunk] = make_float4(v[0], v[1], v[2], 0.0f);

@@ -299,12 +299,13 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* scratch, hipStream_t st);
 // persistent cooperative kernels (kPersistBlock threads per block, grid barrier on d.bar);
-// cfgv = vel, relax, pos iterations, residual mode, residual max passes
+// cfgv = vel, relax, pos iterations, residual mode, residual max passes, one XCD, ready split,
+// residual check interval
 // algorithmic bytes of the last persistent solve into cnt->prof_vel_bytes (profiling)
-void launch_solve_bytes(const DevState& d, const uint32_t cfgv[6], uint32_t bound, hipStream_t st);
+void launch_solve_bytes(const DevState& d, const uint32_t cfgv[8], uint32_t bound, hipStream_t st);
 // splits the active solve positions for the residual computed during the tail colours
 void launch_ready_split(const DevState& d, uint32_t bound, hipStream_t st);
-hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[7], float tol,
+hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const uint32_t cfgv[8], float tol,
                                    uint32_t nblocks, hipStream_t st);
 // colours every active manifold: last step's colours where still valid, then Jones-Plassmann
 // rounds (at most max_rounds) for the rest

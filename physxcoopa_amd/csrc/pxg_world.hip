@@ -114,6 +114,7 @@ extern "C" void pxg_config_default(pxg_config* c) {
     c->gravity[2] = -9.81f;
     c->residual_tolerance = 1e-4f;
     c->residual_max_iterations = 256;
+    c->residual_check_interval = 1;
     c->large_extent = 16.0f;
 }
 
@@ -710,10 +711,12 @@ static int solve_eager(pxg_world* w, const StepParams& p, uint32_t bound, bool s
     for (uint32_t l = 0; l < L; ++l) launch_joint_warm(d, w->jl_fwd[l], w->jl_fwd[l + 1], st);
     if (c.residual_mode) {
         uint32_t maxit = c.residual_max_iterations ? c.residual_max_iterations : 256;
+        const uint32_t every = c.residual_check_interval ? c.residual_check_interval : 1u;
         while (passes < maxit) {
             for (uint32_t k = 0; k < C; ++k) velocity_launch(w, p, co[k], co[k + 1], 0, passes);
             joint_pass(passes);
             ++passes;
+            if (passes % every != 0u && passes != maxit) continue;
             launch_residual(d, p, active, 0, st);
             HIPCHK(hipMemcpyAsync(&w->h_cnt->residual_bits, &d.cnt->residual_bits, 4, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
@@ -791,9 +794,9 @@ static int phase_solve(pxg_world* w, float h) {
         // tail colours
         const bool ready = c.residual_mode && d.nj == 0;
         if (ready) launch_ready_split(d, bound, st);
-        uint32_t cfgv[7] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
+        uint32_t cfgv[8] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
                             c.residual_max_iterations ? c.residual_max_iterations : 256u, one_xcd ? 1u : 0u,
-                            ready ? 1u : 0u};
+                            ready ? 1u : 0u, c.residual_check_interval ? c.residual_check_interval : 1u};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (w->profiling) {
             e0 = take_event(w);

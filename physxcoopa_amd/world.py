@@ -7,6 +7,7 @@ callbacks between the collision and solve phases, and world_state_hash(). All co
 libpxg.so on the GPU; there is no CPU fallback.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -39,6 +40,11 @@ class World:
             self.h = None
 
     def __del__(self):
+        # At interpreter exit the HIP runtime may already be torn down (and a profiler's exit
+        # hooks may be running): destroying the handle then would call into a dead runtime.
+        # The process is ending and the driver reclaims the device memory, so skip it.
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -188,3 +188,71 @@ def test_batch_learned_warm_start_matches_oracle(oracle):
     print("learned batch parity:", rep, "predicted", predicted, flush=True)
     assert predicted > 0
     assert rep["bitexact"] == rep["checks"]
+
+
+def _batch_from(scene_arrays):
+    """(bodies, shapes, body offsets, shape offsets) of a list of per-scene (bodies, shapes)."""
+    bo = np.zeros(len(scene_arrays) + 1, np.uint32)
+    so = np.zeros(len(scene_arrays) + 1, np.uint32)
+    bo[1:] = np.cumsum([len(b) for b, _ in scene_arrays])
+    so[1:] = np.cumsum([len(s) for _, s in scene_arrays])
+    b = np.ascontiguousarray(np.concatenate([b for b, _ in scene_arrays]))
+    s = np.ascontiguousarray(np.concatenate([s for _, s in scene_arrays]))
+    return b, s, bo, so
+
+
+def test_batch_config3_sizes_match_oracle_per_scene(oracle):
+    """Config-3 scenes at their real sizes: 64 scenes, the first eight of 200-256 dynamic bodies
+    (four box pyramids, four random drops: the scenes that fill the workgroup's LDS), the rest
+    N ~ U{8..256} alternating pyramids and drops. Every scene is stepped 150 times in the batch and
+    in its own oracle world fed the batch's solve order, and compared after every step."""
+    from physxcoopa_amd.batch import Batch
+    rng = np.random.default_rng(31)
+    sizes = [200, 205, 220, 230, 240, 248, 256, 256] + [int(x) for x in rng.integers(8, 257, 56)]
+    arrays = []
+    for k, n in enumerate(sizes):
+        arrays.append(scenes.pyramid_scene(n) if k % 2 == 0 else scenes.drop_scene(n, rng))
+    b, s, bo, so = _batch_from(arrays)
+    cfg = _abi.default_config(velocity_iterations=8, position_iterations=3)
+    batch = Batch(cfg, b, s, bo, so)
+    ows = [oracle.OracleWorld(cfg, b[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]]) for k in range(len(sizes))]
+    rep = dict(checks=0, bitexact=0)
+    max_manifolds = 0
+    for step in range(150):
+        batch.step(1)
+        for k in range(len(sizes)):
+            gm = batch.manifolds(k)
+            max_manifolds = max(max_manifolds, len(gm))
+            order, _ = batch.solve_order(k)
+            pairs = np.stack([gm["shape_a"][order], gm["shape_b"][order]], axis=1) if len(order) else np.zeros((0, 2))
+            ows[k].set_order(pairs)
+            ows[k].step_fixed()
+            compare_scene(batch, k, ows[k], rep)
+    st = batch.stats()
+    print("config-3 sizes parity:", rep, "largest manifold count", max_manifolds, "colours max", st["colours"].max(),
+          flush=True)
+    assert np.all(st["error"] == 0)
+    assert rep["bitexact"] == rep["checks"] == 150 * len(sizes)
+    assert max_manifolds > 200
+
+
+def test_batch_config3_full_launch_properties():
+    """The full config-3 batch (4096 scenes of 8-256 bodies) in one persistent launch: no scene
+    reports an error, and the result does not depend on the tuning (block size, queue order)."""
+    from physxcoopa_amd.batch import Batch
+    b, s, bo, so, settings = scenes.config3_scenes(seed=3, n_scenes=4096, n_min=8, n_max=256)
+    cfg = _abi.default_config(**settings)
+    out = []
+    for block, lpt in ((256, True), (128, False)):
+        bt = Batch(cfg, b, s, bo, so)
+        bt.set_tuning(block, lpt)
+        bt.step(30)
+        bt.step(30)
+        st = bt.stats()
+        assert np.all(st["error"] == 0), "scenes with errors: %s" % np.nonzero(st["error"])[0][:10]
+        assert st["manifolds"].sum() > 0
+        out.append(bt.bodies().tobytes())
+        print("tuning", block, lpt, "manifolds", int(st["manifolds"].sum()), "max colours", int(st["colours"].max()),
+              flush=True)
+        bt.close()
+    assert out[0] == out[1]

@@ -113,26 +113,34 @@ def test_parity_eager_launch_path(sides, monkeypatch):
     assert rep["manifolds"] > 0
 
 
-def test_parity_residual_mode(sides):
-    """Residual mode: velocity passes repeat until the residual <= 1e-4 (cap 64 here); the GPU
-    must run the same number of passes as the oracle and reach the same state."""
+@pytest.mark.parametrize("every,eager", [(1, False), (3, False), (3, True)], ids=["every_pass", "every_3rd", "every_3rd_eager"])
+def test_parity_residual_mode(sides, every, eager, monkeypatch):
+    """Residual mode: velocity passes repeat until the residual <= 1e-4 (cap 64 here), the residual
+    evaluated after every `every`-th pass and after the 64th (64 is not a multiple of 3); the GPU
+    must run the same number of passes as the oracle and reach the same state, on the persistent
+    and on the one-launch-per-colour path."""
     World, OracleWorld = sides
+    if eager:
+        monkeypatch.setenv("PXG_EAGER", "1")
     sc = scenes.config1_scene(seed=1)
-    c = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=64, **sc.config)
+    c = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=64,
+                            residual_check_interval=every, **sc.config)
     gw, ow, rep = lockstep(World, OracleWorld, c, sc.bodies, sc.shapes, 80, resync=True)
     print(rep, gw.stats(), ow.stats(), flush=True)
     assert gw.stats()["velocity_passes"] == ow.stats()["velocity_passes"]
     assert abs(gw.stats()["residual"] - ow.stats()["residual"]) <= 1e-6 + 1e-4 * abs(ow.stats()["residual"])
 
 
-@pytest.mark.parametrize("n,lattice,warm,steps", [(2048, (32, 32, 8), 90, 6), (16384, (64, 64, 40), 240, 3)],
-                         ids=["pile_2048", "pile_16384"])
-def test_parity_residual_mode_coloured_piles(sides, n, lattice, warm, steps):
+@pytest.mark.parametrize("n,lattice,warm,steps,every", [(2048, (32, 32, 8), 90, 6, 1), (2048, (32, 32, 8), 90, 4, 8),
+                                                        (16384, (64, 64, 40), 240, 3, 1)],
+                         ids=["pile_2048", "pile_2048_every_8th", "pile_16384"])
+def test_parity_residual_mode_coloured_piles(sides, n, lattice, warm, steps, every):
     """Residual mode on piles large enough for a multi-block solve with a tail of short colours
     (the coloured order): the residual of each manifold is computed during the colour phase after
     the last colour writing its bodies, and the GPU must run the oracle's pass count and reach its
-    state bit for bit. The GPU state after `warm` fixed-iteration steps is handed to the oracle;
-    the oracle then solves in the GPU's order."""
+    state bit for bit (with the residual after every pass, or after every 8th and the last). The
+    GPU state after `warm` fixed-iteration steps is handed to the oracle; the oracle then solves in
+    the GPU's order."""
     World, OracleWorld = sides
     sc = scenes.config2_scene(seed=2, n=n, lattice=lattice)
     c = _abi.default_config(**sc.config)
@@ -140,7 +148,8 @@ def test_parity_residual_mode_coloured_piles(sides, n, lattice, warm, steps):
     gw = World(c, sc.bodies, sc.shapes)
     for _ in range(warm):
         gw.step_fixed()
-    rc = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=256, **sc.config)
+    rc = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=256,
+                             residual_check_interval=every, **sc.config)
     rc.solve_order = _abi.PXG_ORDER_COLOURED
     gw.set_config(rc)
     ow = OracleWorld(rc, gw.bodies(), sc.shapes)
@@ -189,3 +198,20 @@ def test_full_pile_free_running_equals_oracle(sides):
             asleep = int(len(sc.bodies) - gw.bodies()["awake"].sum())
             print("step", i + 1, "asleep", asleep, flush=True)
     assert asleep > 0
+
+
+def test_gpu_state_hash_equals_fnv_over_body_bytes(sides):
+    """pxg_state_hash equals the numpy FNV-1a restatement of world.h:1334-1359 over the bodies
+    downloaded from the GPU (config-1 scene, 30 steps), and a dead body is left out."""
+    from test_oracle_residual import state_hash_numpy
+    World, _ = sides
+    sc = scenes.config1_scene(seed=1)
+    gw = World(_abi.default_config(**sc.config), sc.bodies, sc.shapes)
+    for _ in range(30):
+        gw.step_fixed()
+    b = gw.bodies()
+    assert gw.state_hash() == state_hash_numpy(b)
+    b["alive"][5] = 0
+    gw.set_bodies(b)
+    assert gw.state_hash() == state_hash_numpy(b)
+    gw.close()

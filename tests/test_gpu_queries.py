@@ -384,3 +384,58 @@ def test_penetration_batch_equals_single_queries(World, oracle):
             assert small[k, :min(2, len(one))].tobytes() == one[:2].tobytes(), k
             if k % 4 == 0:
                 assert ow.compute_penetration(probes[k], f).tobytes() == one.tobytes(), k
+
+
+def test_penetration_batch_full_pile(World, oracle):
+    """The 16k-body pile after 200 steps: 16,384 posed probes in one pass (more than 64 chunks of
+    64 shapes, so every probe's wavefront loops over several ballot rounds of chunk boxes). Every
+    64th probe's records equal the single-query entry point's and the oracle's; a capacity of 2
+    keeps every count and truncates the lists to their first two records."""
+    sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
+    cfg = _abi.default_config(**sc.config)
+    gw = World(cfg, sc.bodies, sc.shapes)
+    for _ in range(200):
+        gw.step_fixed()
+    probes = random_casts(np.random.default_rng(17), 16384, -34, 34, 0.0, 2.5)
+    counts, recs = gw.compute_penetration_batch(probes, None, cap=32)
+    small_counts, small = gw.compute_penetration_batch(probes, None, cap=2)
+    print("full pile: %d penetrations over %d probes, max %d per probe" % (int(counts.sum()), len(probes),
+                                                                          int(counts.max())), flush=True)
+    assert np.array_equal(counts, small_counts)
+    assert counts.max() <= 32 and (counts > 0).mean() > 0.2
+    ow = oracle_from(oracle, gw, cfg, sc.shapes)
+    for k in range(0, len(probes), 64):
+        one = gw.compute_penetration(probes[k], None)
+        assert counts[k] == len(one) and recs[k, :len(one)].tobytes() == one.tobytes(), k
+        assert small[k, :min(2, len(one))].tobytes() == one[:2].tobytes(), k
+        assert ow.compute_penetration(probes[k], None).tobytes() == one.tobytes(), k
+
+
+def test_batch_queries_ignore_a_nan_fat_box(World):
+    """A shape whose fat AABB is NaN is accepted by no box test; as the first slot of its 64-slot
+    chunk it must not hide the chunk's other shapes from the batched overlaps and penetrations
+    (the chunk box reduction drops NaN operands). The world is only queried after the boxes are
+    set, never stepped."""
+    b, s = scenes.mixed_pile_scene(n=300, seed=11)
+    cfg = _abi.default_config()
+    gw = World(cfg, b, s)
+    for _ in range(80):
+        gw.step_fixed()
+    fat = gw.fat_aabbs()
+    fat[64] = np.nan
+    fat[128, 0] = np.nan
+    gw.set_fat_aabbs(fat)
+    rng = np.random.default_rng(21)
+    q = random_overlaps(rng, 300, -5, 5, 0, 3)
+    counts, bodies = gw.overlap_batch(q, cap=64)
+    hits_in_chunks = 0
+    for k in range(len(q)):
+        one = gw.overlap(q[k])
+        assert counts[k] == len(one) and np.array_equal(bodies[k, :len(one)], one), k
+        hits_in_chunks += int(np.any((s["owner"][64:192][:, None] == one[None, :])))
+    assert hits_in_chunks > 0
+    probes = random_casts(rng, 200, -5, 5, 0, 3)
+    pc, recs = gw.compute_penetration_batch(probes, None, cap=16)
+    for k in range(len(probes)):
+        one = gw.compute_penetration(probes[k], None)
+        assert pc[k] == len(one) and recs[k, :len(one)].tobytes() == one.tobytes(), k
