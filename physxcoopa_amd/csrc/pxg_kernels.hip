@@ -1859,6 +1859,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     // barrier overlaps the load of this thread's first slot of the next colour (of the next pass
     // after the residual's barrier: loads only, harmless when the loop ends there)
     uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : 0u;
+    uint32_t checks = 0;  // residual evaluations so far: their parity picks the reduction slot
     PreSlot y;
     y.s = kNoSlot;
     if (max_pass && T && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, 0u, 0, y);
@@ -1909,19 +1910,20 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
         if (threadIdx.x == 0) {
             float m = 0.0f;
             for (int w = 0; w < kPersistBlock / 64; ++w) m = fmaxf(m, s_red[w]);
-            if (m > 0.0f) atomicMax(&d.cnt->res_slot[it & 1], __float_as_uint(m));
+            if (m > 0.0f) atomicMax(&d.cnt->res_slot[checks & 1u], __float_as_uint(m));
         }
         grid_arrive(g);
         if (T && co[0] + tid < co[1]) prefetch_slot(d, co[0] + tid, it + 1u, 0, y);
         if (grid_wait(g, err)) return;
         // every block's maximum is in: all threads read the same value. The other slot is
-        // cleared for the next pass: its last readers read it before this pass's first barrier,
-        // and its next writers add to it only after the next pass's colour barriers.
-        float rr = __uint_as_float(__hip_atomic_load(&d.cnt->res_slot[it & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        // cleared for the next evaluation: its last readers read it before this evaluation's
+        // barrier, and its next writers add to it only after the next pass's colour barriers.
+        float rr = __uint_as_float(__hip_atomic_load(&d.cnt->res_slot[checks & 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (tid == 0) {
             d.cnt->residual_bits = __float_as_uint(rr);
-            d.cnt->res_slot[(it + 1) & 1] = 0u;
+            d.cnt->res_slot[(checks + 1u) & 1u] = 0u;
         }
+        ++checks;
         if (rr <= cfg.residual_tol) break;
     }
     if (cfg.residual_mode && tid == 0) d.cnt->passes = passes;
