@@ -400,6 +400,19 @@ int pxg_overlap(pxg_world* w, const pxg_overlap_query* query, const pxg_query_fi
  * as pxg_overlap lists them. bodies holds n * cap entries. */
 int pxg_overlap_batch(pxg_world* w, const pxg_overlap_query* queries, uint32_t n, const pxg_query_filter* filter,
                       uint32_t* bodies, uint32_t cap, uint32_t* counts);
+/* pxg_overlap_batch for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It has no counterpart in the
+ * reference. d_queries (n records), d_bodies (n * cap entries) and d_counts (n entries) are
+ * device pointers; the results are those of pxg_overlap_batch, except that a query whose type is
+ * not a sphere, box or capsule gets the count PXG_NO_BODY. The call only enqueues work on the
+ * world's stream and returns: the results are ready after pxg_synchronize (or any later wait on
+ * the world's stream). It allocates nothing: its scratch is a fixed device array of the library,
+ * one per device (about 4 MB), which limits it to worlds of at most 65536 shape slots
+ * (PXG_ERR_CAPACITY beyond). Every call on one device shares that array: calls on one device that
+ * overlap in time (two worlds' streams, two host threads) race and must be serialised by the
+ * caller. */
+int pxg_overlap_batch_noalloc(pxg_world* w, const pxg_overlap_query* d_queries, uint32_t n,
+                              const pxg_query_filter* filter, uint32_t* d_bodies, uint32_t cap, uint32_t* d_counts);
 /* The first body a swept sphere, box or capsule touches (shape_cast() / sphere_cast() /
  * box_cast() / capsule_cast(), world.h:877-950; query/sweep.h:422-505: conservative advancement on
  * the exact closest distance for round shapes, a swept separating-axis test for box against box,

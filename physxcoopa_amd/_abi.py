@@ -9,6 +9,7 @@ import ctypes
 import numpy as np
 
 PXG_OK = 0
+PXG_ERR_INVALID, PXG_ERR_CAPACITY, PXG_ERR_HIP, PXG_ERR_STATE, PXG_ERR_DEVICE = 1, 2, 3, 4, 5
 PXG_STATIC, PXG_KINEMATIC, PXG_DYNAMIC = 0, 1, 2
 PXG_SPHERE, PXG_BOX, PXG_CAPSULE = 0, 1, 2
 PXG_AVERAGE, PXG_MINIMUM, PXG_MAXIMUM, PXG_MULTIPLY = 0, 1, 2, 3

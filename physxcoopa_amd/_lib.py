@@ -48,6 +48,8 @@ _SIGNATURES = {
     "pxg_shape_cast": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_overlap_batch": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                            ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
+    "pxg_overlap_batch_noalloc": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),
     "pxg_shape_cast_batch": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p],
                              ctypes.c_int),
     "pxg_compute_penetration_batch": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,

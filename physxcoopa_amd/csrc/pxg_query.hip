@@ -13,6 +13,7 @@
 // its tile's box, so nothing the reference tests is skipped). The closest hit keeps the nearest
 // distance, equal distances going to the lower shape slot.
 #include "pxg_state.h"
+#include "../../include/pxg.h"
 #include "pxg_device.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -515,17 +516,41 @@ __global__ void __launch_bounds__(256) k_chunk_bounds(DevState d, const uint4* q
         chunk_box[2 * chunk + 1] = make_float4(v[3], v[4], v[5], 0.0f);
     }
 }
+// A batched overlap query as the kernel takes it: the host's decoded record (type checked on the
+// host), or the ABI record read straight from device memory (false for a type other than sphere,
+// box or capsule).
+__device__ __forceinline__ bool load_overlap(const OverlapDev* qs, uint32_t i, OverlapDev& q) {
+    q = qs[i];
+    return true;
+}
+__device__ __forceinline__ bool load_overlap(const pxg_overlap_query* qs, uint32_t i, OverlapDev& q) {
+    const pxg_overlap_query r = qs[i];
+    q.type = r.type;
+    q.r = r.radius;
+    q.c = make_float4(r.center[0], r.center[1], r.center[2], 0.0f);
+    q.he = make_float4(r.half_extents[0], r.half_extents[1], r.half_extents[2], 0.0f);
+    q.q = make_float4(r.orientation[0], r.orientation[1], r.orientation[2], r.orientation[3]);
+    q.a = make_float4(r.a[0], r.a[1], r.a[2], 0.0f);
+    q.b = make_float4(r.b[0], r.b[1], r.b[2], 0.0f);
+    return r.type <= uint32_t(SH_CAPSULE);
+}
 // Many overlap queries: one wavefront per query scans the shapes in slot order, 64 at a time
 // (chunks whose box misses the query's bounds skipped: every live fat AABB of a chunk lies in its
 // box, so none of them could pass), and appends the owners of overlapping shapes through a ballot
 // (slot order kept); counts[i] is the total, bodies[i * cap ...] holds the first min(cap, total).
+// A query record of an unknown shape type gets the count PXG_NO_BODY and no bodies.
+template <class Q>
 __global__ void __launch_bounds__(256) k_overlap_batch(DevState d, const float4* geom, const uint4* qinfo,
-                                                       const float4* chunk_box, const OverlapDev* queries, uint32_t n,
+                                                       const float4* chunk_box, const Q* queries, uint32_t n,
                                                        QueryFilterDev f, uint32_t* bodies, uint32_t cap, uint32_t* counts) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t qi = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (qi >= n) return;  // the whole wavefront leaves together; the kernel has no block barrier
-    const OverlapDev q = queries[qi];
+    OverlapDev q;
+    if (!load_overlap(queries, qi, q)) {
+        if (lane == 0) counts[qi] = 0xFFFFFFFFu;
+        return;
+    }
     v3 qmn, qmx;
     overlap_query_bounds(q, qmn, qmx);
     uint32_t total = 0u;
@@ -1149,7 +1174,17 @@ void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qi
     if (!n) return;
     const uint32_t chunks = (d.ns + 63u) / 64u;
     if (chunks) hipLaunchKernelGGL(k_chunk_bounds, dim3((chunks + 3u) / 4u), dim3(256), 0, st, d, qinfo, chunk_box);
-    hipLaunchKernelGGL(k_overlap_batch, dim3((n + 3u) / 4u), dim3(256), 0, st, d, geom, qinfo,
+    hipLaunchKernelGGL(k_overlap_batch<OverlapDev>, dim3((n + 3u) / 4u), dim3(256), 0, st, d, geom, qinfo,
+                       static_cast<const float4*>(chunk_box), queries, n, f, bodies, cap, counts);
+}
+
+void launch_overlap_batch_records(const DevState& d, const float4* geom, const uint4* qinfo, float4* chunk_box,
+                                  const pxg_overlap_query* queries, uint32_t n, const QueryFilterDev& f, uint32_t* bodies,
+                                  uint32_t cap, uint32_t* counts, hipStream_t st) {
+    if (!n) return;
+    const uint32_t chunks = (d.ns + 63u) / 64u;
+    if (chunks) hipLaunchKernelGGL(k_chunk_bounds, dim3((chunks + 3u) / 4u), dim3(256), 0, st, d, qinfo, chunk_box);
+    hipLaunchKernelGGL(k_overlap_batch<pxg_overlap_query>, dim3((n + 3u) / 4u), dim3(256), 0, st, d, geom, qinfo,
                        static_cast<const float4*>(chunk_box), queries, n, f, bodies, cap, counts);
 }
 

@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "pxg_device.h"
+#include "../../include/pxg.h"
 
 namespace pxg {
 
@@ -57,7 +58,7 @@ struct Counters {  // device-side counters, read back by the host at the step's 
     float grid[12];         // ox, oy, oz, inv_cell, ext_y, ext_z, nx, ny, nz (as floats)
     unsigned long long prof_vel_bytes;  // algorithmic bytes of profiled velocity passes
     uint32_t passes;        // velocity passes run by the last solve
-    uint32_t res_slot[2];   // residual max-reduction slots (float bits), alternating per pass
+    uint32_t res_slot[2];   // residual max-reduction slots (float bits), alternating per evaluation
     uint32_t ev_n;          // contact events queued (<= the queue's capacity)
     uint32_t ev_dropped;    // events lost to a full queue since it was last cleared
     uint32_t ev_snap_n[2];  // entries of the two pair snapshots
@@ -280,6 +281,10 @@ hipError_t launch_shape_cast_batch(const DevState& d, const QueryScratch& q, con
 void launch_overlap_batch(const DevState& d, const float4* geom, const uint4* qinfo, float4* chunk_box,
                           const OverlapDev* queries, uint32_t n, const QueryFilterDev& f, uint32_t* bodies, uint32_t cap,
                           uint32_t* counts, hipStream_t st);
+// the same over ABI query records in device memory (an unknown shape type: count PXG_NO_BODY)
+void launch_overlap_batch_records(const DevState& d, const float4* geom, const uint4* qinfo, float4* chunk_box,
+                                  const pxg_overlap_query* queries, uint32_t n, const QueryFilterDev& f, uint32_t* bodies,
+                                  uint32_t cap, uint32_t* counts, hipStream_t st);
 // many penetration probes (dir and maxd unused), one wavefront each, chunk-culled like the batched
 // overlaps: counts[i] penetrations, the first min(cap, counts[i]) in slot order as pxg_penetration
 // records (10 words) at out[i * cap * 10]

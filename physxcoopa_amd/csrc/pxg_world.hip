@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,7 @@ struct pxg_world {
     pxg_stats stats{};
     uint32_t colour_round = 0;
     bool shapes_ready = false;
+    void* noalloc_scratch = nullptr;  // this device's address of `scratch` (pxg_overlap_batch_noalloc)
     // profiling: HIP event pairs around every velocity-pass launch
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -196,6 +198,38 @@ static void velocity_launch(pxg_world* w, const StepParams& p, uint32_t b, uint3
     w->ev_pending.push_back({e0, e1});
 }
 
+// ---- the scratch of pxg_overlap_batch_noalloc ----
+// A file-scope device array, so that entry point needs no allocation: the per-shape world geometry
+// (3 float4) and query record (uint4), the scene bounds the geometry pass also writes (6 words) and
+// the 64-slot chunk boxes (2 float4), for at most kNoAllocShapes shape slots (about 4 MB, held by
+// every device the library runs on until the process ends). In host code `scratch` is not a device
+// address: its address on a device comes from hipGetSymbolAddress with that device current.
+constexpr uint32_t kNoAllocShapes = 65536u;
+constexpr size_t kScrGeom = 0;
+constexpr size_t kScrInfo = kScrGeom + size_t(3) * kNoAllocShapes * sizeof(float4);
+constexpr size_t kScrBounds = kScrInfo + size_t(kNoAllocShapes) * sizeof(uint4);
+constexpr size_t kScrChunks = kScrBounds + 256;
+constexpr size_t kScrBytes = kScrChunks + size_t(2) * (kNoAllocShapes / 64u) * sizeof(float4);
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+// the address of `scratch` on `device` (current), looked up once per device
+static void* noalloc_scratch_address(int device) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        addr[device] = p;
+    }
+    return addr[device];
+}
+
 extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
     if (!out) return fail(PXG_ERR_INVALID, "pxg_create: null output");
     *out = nullptr;
@@ -216,6 +250,7 @@ extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
         return fail(PXG_ERR_HIP, "pxg_create: hipHostMalloc failed");
     }
     std::memset(w->h_cnt, 0, sizeof(Counters));
+    w->noalloc_scratch = noalloc_scratch_address(w->device);  // set-up, once per device
     int coop = 0;
     (void)hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, w->device);
     w->coop_blocks = coop ? max_coresident_blocks(w->device) : 0;
@@ -1108,6 +1143,32 @@ extern "C" int pxg_overlap_batch(pxg_world* w, const pxg_overlap_query* queries,
     if (cap)
         HIPCHK(hipMemcpyAsync(bodies, w->q_ov_out, size_t(n) * cap * sizeof(uint32_t), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
+    return PXG_OK;
+}
+
+extern "C" int pxg_overlap_batch_noalloc(pxg_world* w, const pxg_overlap_query* d_queries, uint32_t n,
+                                         const pxg_query_filter* filter, uint32_t* d_bodies, uint32_t cap,
+                                         uint32_t* d_counts) {
+    if (!w || (n && (!d_queries || !d_counts)) || (n && cap && !d_bodies))
+        return fail(PXG_ERR_INVALID, "pxg_overlap_batch_noalloc: null argument");
+    if (!w->shapes_ready) return fail(PXG_ERR_STATE, "pxg_overlap_batch_noalloc: upload bodies and shapes first");
+    if (w->d.ns > kNoAllocShapes)
+        return fail(PXG_ERR_CAPACITY, "pxg_overlap_batch_noalloc: the world has more than 65536 shape slots");
+    if (!w->noalloc_scratch) return fail(PXG_ERR_HIP, "pxg_overlap_batch_noalloc: no device address for the scratch");
+    if (!n) return PXG_OK;
+    unsigned char* base = static_cast<unsigned char*>(w->noalloc_scratch);
+    float4* geom = reinterpret_cast<float4*>(base + kScrGeom);
+    uint4* info = reinterpret_cast<uint4*>(base + kScrInfo);
+    uint32_t* bounds6 = reinterpret_cast<uint32_t*>(base + kScrBounds);
+    float4* chunk_box = reinterpret_cast<float4*>(base + kScrChunks);
+    // everything the kernels read from the scratch is written first in this sequence: the bounds
+    // by fill kernels, the geometry and records of every slot by the geometry pass, every chunk's
+    // box by k_chunk_bounds
+    launch_fill_u32(bounds6, 0xFFFFFFFFu, 3, w->stream);
+    launch_fill_u32(bounds6 + 3, 0u, 3, w->stream);
+    launch_query_prep(w->d, geom, info, bounds6, w->stream);
+    launch_overlap_batch_records(w->d, geom, info, chunk_box, d_queries, n, query_filter(filter), d_bodies, cap, d_counts,
+                                 w->stream);
     return PXG_OK;
 }
 
