@@ -304,6 +304,16 @@ def bench_queries(args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: the cores this process may use, at most 16 (one GPU's
+    share of the host on the benchmark machines)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def pile_world(sc, order, net):
     from physxcoopa_amd import _abi
     from physxcoopa_amd.world import World
@@ -394,7 +404,7 @@ def bench_pile(args):
     net = learned.MlpParams.load(learned.DEFAULT_NET)
     runs = {}
     worlds = {}
-    snap = None
+    snap = snap_fixed = None
     plan = [("plain", _abi.PXG_ORDER_REFERENCE, None), ("batched_learned", _abi.PXG_ORDER_COLOURED, net)]
     if args.only:
         plan = [p for p in plan if p[0] == args.only]
@@ -409,6 +419,9 @@ def bench_pile(args):
                 w.synchronize()
                 log("%s settle %d/%d %.1fs" % (name, i + 1, args.settle, time.time() - t1))
         w.synchronize()
+        if name == "plain":
+            k, imp = w.warm_cache()
+            snap_fixed = (w.bodies(), w.fat_aabbs(), k, imp, cfg)
         el_f, _, st_f, prof_f = timed_steps(w, args.fixed_steps, False)
         res_cfg = _abi.default_config(**sc.config)
         res_cfg.solve_order = order
@@ -438,17 +451,33 @@ def bench_pile(args):
             (name, runs[name]["ms_per_step"], runs[name]["passes"]["per_step"], runs[name]["reached"], args.steps,
              tol, runs[name]["fixed_iterations_ms_per_step"]))
     head = runs.get("batched_learned") or runs["plain"]
-    cpu = None
+    cpu = cpu_fixed = None
     if not args.no_cpu and snap is not None:
-        # The oracle continues from the plain GPU world's state at the start of its timed steps.
-        # In the reference order the GPU and the oracle are bit-identical, so the oracle's steps are
-        # the same steps with the same pass counts, and their times compare one to one.
+        # The oracle continues from the plain GPU world's state: in the reference order the GPU and
+        # the oracle are bit-identical, so the oracle's steps are the same steps with the same pass
+        # counts, and their times compare one to one. Bounds, pair discovery and narrowphase run on
+        # a host thread pool (the stages the reference runs on its job system); the solver is serial.
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pxo
-        bodies, fat, k, imp, res_cfg = snap
-        ow = pxo.OracleWorld(res_cfg, bodies, sc.shapes)
-        ow.set_fat_aabbs(fat)
-        ow.set_warm_cache(k, imp)
+        threads = cpu_threads()
+
+        def oracle_from(sn):
+            bodies, fat, k, imp, c = sn
+            ow = pxo.OracleWorld(c, bodies, sc.shapes)
+            ow.set_fat_aabbs(fat)
+            ow.set_warm_cache(k, imp)
+            ow.set_threads(threads)
+            return ow
+        ow = oracle_from(snap_fixed)
+        n_fix = min(args.fixed_steps, 20)
+        t_fix = ow.run(n_fix)
+        cpu_fixed = {"value": round(n_dyn * n_fix / t_fix, 1), "unit": "body·steps/s", "cores": threads, "kind": "port",
+                     "sample": "%d steps at 8 velocity / 2 relax / 3 position iterations of the %d-body pile after %d "
+                               "settling steps, from the plain GPU world's state (C++ oracle, -O2; bounds, pairs and "
+                               "narrowphase on %d host threads, solver serial)" % (n_fix, n_dyn, args.settle, threads),
+                     "ms_per_step": round(1000 * t_fix / n_fix, 2)}
+        log("CPU oracle, fixed iterations: %.1f ms/step on %d threads" % (cpu_fixed["ms_per_step"], threads))
+        ow = oracle_from(snap)
         n_cpu, t_cpu, p_cpu = 0, 0.0, []
         log("CPU oracle: residual-mode steps from the plain world's state (budget %.0f s, at least one step)" %
             args.cpu_seconds)
@@ -459,11 +488,12 @@ def bench_pile(args):
             log("CPU oracle step %d: %d passes, %.1f s total" % (n_cpu, p_cpu[-1], t_cpu))
         gpu_same = sum(r["seconds"] for r in runs["plain"]["per"][:n_cpu])
         gpu_passes = [r["passes"] for r in runs["plain"]["per"][:n_cpu]]
-        cpu = {"value": round(n_dyn * n_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": 1, "kind": "port",
-               "sample": "the first %d of the plain GPU world's timed residual-mode steps (tolerance 1e-4, cap %d) of "
-                         "the %d-body pile after %d settling steps, continued from its state (C++ oracle, -O2, one "
-                         "thread); velocity passes per step %s (GPU: %s)" %
-                         (n_cpu, args.cap, n_dyn, args.settle, p_cpu, gpu_passes),
+        cpu = {"value": round(n_dyn * n_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": threads, "kind": "port",
+               "sample": "the first %d of the plain GPU world's timed residual-mode steps (tolerance 1e-4, cap %d, "
+                         "residual every %d passes) of the %d-body pile after %d settling steps, continued from its "
+                         "state (C++ oracle, -O2; bounds, pairs and narrowphase on %d host threads, solver serial); "
+                         "velocity passes per step %s (GPU: %s)" %
+                         (n_cpu, args.cap, args.check_every, n_dyn, args.settle, threads, p_cpu, gpu_passes),
                "ms_per_step": round(1000 * t_cpu / n_cpu, 2),
                "plain_gpu_same_steps_ms_per_step": round(1000 * gpu_same / n_cpu, 3),
                "plain_gpu_vs_cpu_same_steps": round(t_cpu / gpu_same, 2) if gpu_same > 0 else None,
@@ -510,6 +540,8 @@ def bench_pile(args):
         "roofline": roofline_of(head["prof"]),
         "traffic_profiled": profiled_traffic(key),
         "cpu_baseline": cpu,
+        "cpu_baseline_fixed_iterations": cpu_fixed,
+        "vs_cpu_fixed_iterations": round(head["fixed_iterations_value"] / cpu_fixed["value"], 2) if cpu_fixed else None,
     }
     for w in worlds.values():
         w.close()

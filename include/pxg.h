@@ -37,6 +37,10 @@ extern "C" {
 #define PXG_ERR_DEVICE 5    /* a kernel reported an internal error (bounded wait expired, ...) */
 
 enum pxg_body_type { PXG_STATIC = 0, PXG_KINEMATIC = 1, PXG_DYNAMIC = 2 };
+/* How a renderer blends a body's previous and current pose between fixed substeps
+ * (dynamics::Interpolation, dynamics/body.h:26-33; Body::interpolation defaults to Interpolate,
+ * body.h:118). Carried with the body; the step does not read it. */
+enum pxg_interpolation { PXG_INTERPOLATE = 0, PXG_INTERPOLATE_NONE = 1 };
 enum pxg_shape_type { PXG_SPHERE = 0, PXG_BOX = 1, PXG_CAPSULE = 2 };
 enum pxg_combine { PXG_AVERAGE = 0, PXG_MINIMUM = 1, PXG_MAXIMUM = 2, PXG_MULTIPLY = 3 };
 enum pxg_phase { PXG_PHASE_FORCES_COLLIDE = 0, PXG_PHASE_SOLVE_INTEGRATE = 1 };
@@ -109,7 +113,7 @@ typedef struct pxg_body {
     uint32_t use_gravity;
     uint32_t constraints;   /* PXG_FREEZE_* bits */
     uint32_t generation;    /* carried, not interpreted */
-    uint32_t reserved0;
+    uint32_t interpolation; /* pxg_interpolation; 0 = PXG_INTERPOLATE, the reference's default */
     uint32_t reserved1;
     float position[3];
     float sleep_timer;
@@ -317,6 +321,25 @@ int pxg_set_gravity(pxg_world* w, const float g[3]);              /* set_gravity
  * recomputed from orientation exactly as add_body() does. */
 int pxg_set_bodies(pxg_world* w, const pxg_body* bodies, uint32_t n);
 int pxg_get_bodies(pxg_world* w, pxg_body* out, uint32_t n);
+/* Writes records[k] into body indices[k] for k < n and leaves every other body as it is: what
+ * edits through get_body() (world.h:347) need to reach the GPU, without the whole table. World
+ * inverse inertia of the written bodies is recomputed as pxg_set_bodies does. Indices must be
+ * distinct and below the body count, and a body's type may not change (PXG_ERR_STATE: use
+ * pxg_set_bodies); allowed between the two step phases like pxg_set_bodies. */
+int pxg_set_bodies_subset(pxg_world* w, const uint32_t* indices, const pxg_body* records, uint32_t n);
+
+/* What a renderer's transform write-back reads of a body (system/physics_system.h:1058-1075:
+ * type, interpolation mode, previous and current pose), 64 bytes per body. */
+typedef struct pxg_pose {
+    float position[3];
+    uint32_t type;            /* pxg_body_type */
+    float orientation[4];     /* x, y, z, w */
+    float prev_position[3];
+    uint32_t interpolation;   /* pxg_interpolation */
+    float prev_orientation[4];
+} pxg_pose;
+/* The poses of bodies [0, n): one download of 64 bytes per body instead of the full records. */
+int pxg_get_poses(pxg_world* w, pxg_pose* out, uint32_t n);
 /* Replaces the shape table and creates each enabled shape's broadphase proxy with
  * fat = tight.expand(margin) at its owner's current pose (create_shape_proxy_, world.h:1369-1382). */
 int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n);

@@ -4,7 +4,10 @@
 //
 // State model: the GPU owns body state between steps. The host keeps a mirror that is pulled
 // on demand (get_body, body_count, ...) and pushed before the next step when it was handed out
-// for writing or the body set changed. Errors follow the reference's conventions: stale handles
+// for writing or the body set changed; a push uploads only the bodies whose records changed.
+// get_body() and pose() may be called from several threads at once between steps (the first
+// call after a step downloads, the others wait for it), as the reference's job-parallel
+// transform write-back does (system/physics_system.h:1058-1075). Errors follow the reference's conventions: stale handles
 // are no-ops or nullptr (world.h:340-364); a failure inside the library throws
 // std::runtime_error("[physxcoopa] ...") (util/error.h:24-26).
 #pragma once
@@ -12,10 +15,12 @@
 #include "pxg.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -281,7 +286,7 @@ public:
     pxg_body* get_body(BodyId id) {
         if (!is_valid(id)) return nullptr;
         pull();
-        bodies_dirty_ = true;
+        bodies_dirty_.store(true, std::memory_order_relaxed);
         return &bodies_[id.index];
     }
     const pxg_body* get_body(BodyId id) const {
@@ -289,6 +294,21 @@ public:
         const_cast<PhysicsWorld*>(this)->pull();
         return &bodies_[id.index];
     }
+    // What a transform write-back reads of `id` (type, interpolation mode, previous and current
+    // pose), or nullptr for a stale handle: the poses of all bodies are downloaded once per step,
+    // 64 bytes per body, by the first call after the step (thread-safe like get_body).
+    const pxg_pose* pose(BodyId id) const {
+        if (!is_valid(id)) return nullptr;
+        const_cast<PhysicsWorld*>(this)->pull_poses();
+        return &poses_[id.index];
+    }
+    // Fractional progress through the next substep after the last step(dt), in [0, 1), for
+    // render interpolation (world.h:1240, 1319-1320).
+    float interpolation_alpha() const { return interpolation_alpha_; }
+    // Body records the last step_fixed() uploaded before stepping (0 when nothing changed on the
+    // host; the full count after a structural change).
+    uint32_t bodies_uploaded_last_step() const { return bodies_uploaded_; }
+
     // The primary shape of `id` (world.h:367-374); edits apply before the next step.
     pxg_shape* get_shape(BodyId id) {
         if (!is_valid(id)) return nullptr;
@@ -431,6 +451,7 @@ public:
             accumulator_ -= h;
             ++n;
         }
+        interpolation_alpha_ = accumulator_ / h;
         return n;
     }
 
@@ -446,19 +467,21 @@ public:
             check(pxg_step_fixed(w_, h), "step_fixed");
         } else {
             check(pxg_step_phase(w_, PXG_PHASE_FORCES_COLLIDE, h), "step_fixed (collide)");
-            state_on_device_ = true;
-            bodies_dirty_ = false;
+            state_on_device_.store(true, std::memory_order_release);
+            poses_valid_.store(false, std::memory_order_release);
+            bodies_dirty_.store(false, std::memory_order_relaxed);
             in_substep_ = true;
             on_substep.emit(*this, h);
             in_substep_ = false;
-            if (bodies_dirty_) {
-                check(pxg_set_bodies(w_, bodies_.data(), static_cast<uint32_t>(bodies_.size())), "on_substep upload");
-                bodies_dirty_ = false;
+            if (bodies_dirty_.load(std::memory_order_relaxed)) {
+                upload_changed_bodies("on_substep upload");
+                bodies_dirty_.store(false, std::memory_order_relaxed);
             }
             push_joints();
             check(pxg_step_phase(w_, PXG_PHASE_SOLVE_INTEGRATE, h), "step_fixed (solve)");
         }
-        state_on_device_ = true;
+        state_on_device_.store(true, std::memory_order_release);
+        poses_valid_.store(false, std::memory_order_release);
         joints_on_device_ = !joints_.empty();
         manifolds_valid_ = false;
         events_valid_ = false;
@@ -859,16 +882,68 @@ private:
         return o;
     }
 
-    // device -> host mirror
+    // device -> host mirror; the first caller downloads, concurrent callers wait for it
     void pull() {
         // between the two phases this is the collide phase's state (forces integrated)
-        if (!state_on_device_ || bodies_.empty()) return;
-        check(pxg_get_bodies(w_, bodies_.data(), static_cast<uint32_t>(bodies_.size())), "get_body");
-        state_on_device_ = false;
+        if (!state_on_device_.load(std::memory_order_acquire)) return;
+        std::lock_guard<std::mutex> lock(pull_mu_);
+        if (!state_on_device_.load(std::memory_order_relaxed)) return;
+        if (!bodies_.empty()) {
+            check(pxg_get_bodies(w_, bodies_.data(), static_cast<uint32_t>(bodies_.size())), "get_body");
+            on_device_ = bodies_;
+        }
+        state_on_device_.store(false, std::memory_order_release);
+    }
+    // the write-back's poses: from the device after a step, else from the host mirror
+    void pull_poses() {
+        if (poses_valid_.load(std::memory_order_acquire)) return;
+        std::lock_guard<std::mutex> lock(pull_mu_);
+        if (poses_valid_.load(std::memory_order_relaxed)) return;
+        poses_.resize(bodies_.size());
+        if (state_on_device_.load(std::memory_order_relaxed)) {
+            if (!poses_.empty()) check(pxg_get_poses(w_, poses_.data(), static_cast<uint32_t>(poses_.size())), "pose");
+        } else {
+            for (size_t i = 0; i < bodies_.size(); ++i) {
+                const pxg_body& b = bodies_[i];
+                pxg_pose& p = poses_[i];
+                std::memcpy(p.position, b.position, sizeof(p.position));
+                std::memcpy(p.orientation, b.orientation, sizeof(p.orientation));
+                std::memcpy(p.prev_position, b.prev_position, sizeof(p.prev_position));
+                std::memcpy(p.prev_orientation, b.prev_orientation, sizeof(p.prev_orientation));
+                p.type = b.type;
+                p.interpolation = b.interpolation;
+            }
+        }
+        poses_valid_.store(true, std::memory_order_release);
+    }
+    // uploads the bodies whose host record differs from what the device holds (all of them when
+    // most changed or a type changed)
+    void upload_changed_bodies(const char* what) {
+        const uint32_t n = static_cast<uint32_t>(bodies_.size());
+        std::vector<uint32_t> idx;
+        std::vector<pxg_body> rec;
+        const bool known = on_device_.size() == bodies_.size();
+        for (uint32_t i = 0; known && i < n; ++i) {
+            if (std::memcmp(&bodies_[i], &on_device_[i], sizeof(pxg_body)) == 0) continue;
+            idx.push_back(i);
+            rec.push_back(bodies_[i]);
+        }
+        int r = PXG_ERR_STATE;
+        if (known && idx.size() * 4 <= n)
+            r = idx.empty() ? PXG_OK : pxg_set_bodies_subset(w_, idx.data(), rec.data(), static_cast<uint32_t>(idx.size()));
+        if (r == PXG_OK) {
+            bodies_uploaded_ += static_cast<uint32_t>(idx.size());
+        } else {
+            check(pxg_set_bodies(w_, bodies_.data(), n), what);
+            bodies_uploaded_ += n;
+        }
+        on_device_ = bodies_;
+        poses_valid_.store(false, std::memory_order_release);
     }
 
     // host -> device before a step
     void push() {
+        bodies_uploaded_ = 0;
         if (std::memcmp(&applied_cfg_, &cfg_, sizeof(cfg_)) != 0) {
             check(pxg_set_config(w_, &cfg_), "config");
             applied_cfg_ = cfg_;
@@ -884,6 +959,9 @@ private:
             if (uploaded_shapes_) check(pxg_get_fat_aabbs(w_, fat.data(), uploaded_shapes_), "fat AABBs");
             if (n > cfg_.max_bodies || n > cfg_.max_shapes) grow(n);
             check(pxg_set_bodies(w_, bodies_.data(), n), "bodies");
+            bodies_uploaded_ = n;
+            on_device_ = bodies_;
+            poses_valid_.store(false, std::memory_order_release);
             check(pxg_set_shapes(w_, shapes_.data(), n), "shapes");
             std::vector<float> now(size_t(n) * 6);
             check(pxg_get_fat_aabbs(w_, now.data(), n), "fat AABBs");
@@ -901,10 +979,10 @@ private:
             uploaded_shapes_ = n;
             new_shapes_.clear();
             structure_dirty_ = false;
-            bodies_dirty_ = false;
-        } else if (bodies_dirty_) {
-            check(pxg_set_bodies(w_, bodies_.data(), n), "bodies");
-            bodies_dirty_ = false;
+            bodies_dirty_.store(false, std::memory_order_relaxed);
+        } else if (bodies_dirty_.load(std::memory_order_relaxed)) {
+            upload_changed_bodies("bodies");
+            bodies_dirty_.store(false, std::memory_order_relaxed);
         }
         push_joints();
     }
@@ -926,7 +1004,13 @@ private:
     std::vector<uint64_t> pending_cache_keys_;
     std::vector<float> pending_cache_imp_;
     uint32_t uploaded_shapes_ = 0;
-    bool structure_dirty_ = false, bodies_dirty_ = false, state_on_device_ = false, in_substep_ = false;
+    bool structure_dirty_ = false, in_substep_ = false;
+    std::atomic<bool> bodies_dirty_{false}, state_on_device_{false}, poses_valid_{false};
+    std::mutex pull_mu_;
+    std::vector<pxg_body> on_device_;  // the records the device holds, as of the last pull or upload
+    std::vector<pxg_pose> poses_;
+    uint32_t bodies_uploaded_ = 0;
+    float interpolation_alpha_ = 0.0f;
     std::vector<pxg_joint> joints_;
     std::vector<uint32_t> joint_generations_, free_joints_;
     std::vector<uint8_t> joint_alive_;

@@ -31,7 +31,7 @@ def lib():
             "pxo_create": ([vp, i32], vp), "pxo_destroy": ([vp], None), "pxo_set_config": ([vp, vp], None),
             "pxo_set_bodies": ([vp, vp, u32], None), "pxo_get_bodies": ([vp, vp, u32], None),
             "pxo_set_shapes": ([vp, vp, u32], None), "pxo_set_layer_masks": ([vp, vp], None),
-            "pxo_set_brute_force": ([vp, u32], None), "pxo_set_order": ([vp, vp, u32], None),
+            "pxo_set_brute_force": ([vp, u32], None), "pxo_set_threads": ([vp, u32], None), "pxo_set_order": ([vp, vp, u32], None),
             "pxo_set_miss_table": ([vp, vp, vp, u32], None), "pxo_step_fixed": ([vp, f32], None),
             "pxo_step_phase": ([vp, u32, f32], None), "pxo_run": ([vp, u32, f32], ctypes.c_double),
             "pxo_get_pairs": ([vp, vp, u32], u32), "pxo_get_manifolds": ([vp, vp, u32], u32),
@@ -100,6 +100,10 @@ class OracleWorld:
                 lib().pxo_destroy(self.h)
         except Exception:
             pass
+
+    def set_threads(self, n):
+        """Host threads for the reference's job-parallel stages (bounds, pairs, narrowphase); 1 = serial."""
+        lib().pxo_set_threads(self.h, int(n))
 
     def set_config(self, config):
         self._cfg = config

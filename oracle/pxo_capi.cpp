@@ -242,6 +242,12 @@ void pxo_set_atan2_mode(void* p, uint32_t mode) {
     h->f.atan2_mode = mode ? Atan2Mode::Rounded : Atan2Mode::Libm;
     h->d.atan2_mode = h->f.atan2_mode;
 }
+// Host threads for bounds, pair discovery and narrowphase (1: serial, the reference's default).
+void pxo_set_threads(void* p, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    if (h->dbl) h->d.set_threads(n);
+    else h->f.set_threads(n);
+}
 void pxo_set_brute_force(void* p, uint32_t on) {
     Handle* h = static_cast<Handle*>(p);
     h->f.brute_force = on; h->d.brute_force = on;

@@ -5,10 +5,12 @@
 #pragma once
 #include "pxo_world.hpp"
 #include "pxo_joints.hpp"
+#include "pxo_pool.hpp"
 
 #include <algorithm>
 #include <array>
 #include <iterator>
+#include <memory>
 #include <set>
 
 namespace pxo {
@@ -97,6 +99,14 @@ template <class T> struct World {
     std::vector<std::array<double, 16>> residual_detail;
     uint64_t steps = 0;
     T accumulator = 0;
+    // host threads for bounds, pair discovery and narrowphase (the reference's job-parallel
+    // stages); none: serial, the reference's default (jobs_ == nullptr, world.h:2161)
+    std::unique_ptr<Pool> pool;
+    void set_threads(unsigned n) { pool.reset(n > 1 ? new Pool(n) : nullptr); }
+    template <class Fn> void for_range(size_t count, Fn&& fn) {
+        if (pool) pool->run(count, fn);
+        else fn(size_t(0), count, 0u);
+    }
 
     World() { for (int i = 0; i < 32; ++i) layer_mask[i] = 0xFFFFFFFFu; }
 
@@ -154,11 +164,13 @@ template <class T> struct World {
     void broadphase(T h) {
         size_t ns = shapes.size();
         std::vector<AABB<T>> tight(ns);
-        for (size_t i = 0; i < ns; ++i) {
-            if (proxy[i] < 0) continue;
-            const Body<T>& b = bodies[shapes[i].owner];
-            tight[i] = world_bounds(shapes[i], b.pos, b.q);
-        }
+        for_range(ns, [&](size_t begin, size_t end, unsigned) {  // world.h:1489-1499
+            for (size_t i = begin; i < end; ++i) {
+                if (proxy[i] < 0) continue;
+                const Body<T>& b = bodies[shapes[i].owner];
+                tight[i] = world_bounds(shapes[i], b.pos, b.q);
+            }
+        });
         for (size_t i = 0; i < ns; ++i) {
             if (proxy[i] < 0 || in_static[i]) continue;
             const Body<T>& b = bodies[shapes[i].owner];
@@ -179,19 +191,25 @@ template <class T> struct World {
             }
             return;
         }
-        for (size_t k = 0; k < ns; ++k) {
-            uint32_t i = uint32_t(k);
-            if (proxy[i] < 0) continue;
-            uint32_t oi = shapes[i].owner;
-            dyn_tree.query(tight[i], [&](int32_t j) {
-                if (oi != shapes[j].owner) pairs.push_back({std::min(i, uint32_t(j)), std::max(i, uint32_t(j))});
-            });
-            if (!in_static[i]) {
-                static_tree.query(tight[i], [&](int32_t j) {
-                    if (oi != shapes[j].owner) pairs.push_back({std::min(i, uint32_t(j)), std::max(i, uint32_t(j))});
+        // per-worker pair buckets (world.h:1545-1577), merged and sorted into the pair cache
+        std::vector<std::vector<std::pair<uint32_t, uint32_t>>> buckets(pool ? pool->workers() : 1u);
+        for_range(ns, [&](size_t begin, size_t end, unsigned worker) {
+            auto& out = buckets[worker];
+            for (size_t k = begin; k < end; ++k) {
+                uint32_t i = uint32_t(k);
+                if (proxy[i] < 0) continue;
+                uint32_t oi = shapes[i].owner;
+                dyn_tree.query(tight[i], [&](int32_t j) {
+                    if (oi != shapes[j].owner) out.push_back({std::min(i, uint32_t(j)), std::max(i, uint32_t(j))});
                 });
+                if (!in_static[i]) {
+                    static_tree.query(tight[i], [&](int32_t j) {
+                        if (oi != shapes[j].owner) out.push_back({std::min(i, uint32_t(j)), std::max(i, uint32_t(j))});
+                    });
+                }
             }
-        }
+        });
+        for (auto& b : buckets) pairs.insert(pairs.end(), b.begin(), b.end());
         std::sort(pairs.begin(), pairs.end());
         pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
         // PairCache::add drops x == y (pair_cache.h:43-46); owners differ so shapes differ too
@@ -210,29 +228,34 @@ template <class T> struct World {
     JointSolver<T, Body<T>> joint_solver() { return JointSolver<T, Body<T>>{bodies, joints, atan2_mode}; }
 
     // world.h:1598-1654
+    // per pair in parallel (world.h:1598-1645), then compacted in pair order (1647-1653)
     void narrowphase(T h) {
         manifolds.clear();
-        for (auto& pr : pairs) {
-            uint32_t i = pr.first, j = pr.second;
-            const Shape<T>& si = shapes[i];
-            const Shape<T>& sj = shapes[j];
-            uint32_t ii = si.owner, ij = sj.owner;
-            const Body<T>& bi = bodies[ii];
-            const Body<T>& bj = bodies[ij];
-            if (!bi.alive || !bj.alive) continue;
-            bool either_dyn = bi.type == PXG_DYNAMIC || bj.type == PXG_DYNAMIC;
-            if (!either_dyn) continue;
-            if (!should_collide(si.layer, sj.layer)) continue;
-            if (joint_blocks(ii, ij)) continue;
-            if (!ignored.empty() && ignored.count(body_pair_key(ii, ij))) continue;
-            T ei = min_extent(si), ej = min_extent(sj);
-            bool spec = (length(bi.v) * h > ei) || (length(bj.v) * h > ej);
-            Manifold<T> m;
-            generate_contacts(ii, si, bi.pos, bi.q, ij, sj, bj.pos, bj.q, m, spec);
-            if (!m.valid) continue;
-            m.sa = i; m.sb = j;
-            manifolds.push_back(m);
-        }
+        std::vector<Manifold<T>> slot(pairs.size());
+        for_range(pairs.size(), [&](size_t begin, size_t end, unsigned) {
+            for (size_t k = begin; k < end; ++k) {
+                uint32_t i = pairs[k].first, j = pairs[k].second;
+                const Shape<T>& si = shapes[i];
+                const Shape<T>& sj = shapes[j];
+                uint32_t ii = si.owner, ij = sj.owner;
+                const Body<T>& bi = bodies[ii];
+                const Body<T>& bj = bodies[ij];
+                if (!bi.alive || !bj.alive) continue;
+                bool either_dyn = bi.type == PXG_DYNAMIC || bj.type == PXG_DYNAMIC;
+                if (!either_dyn) continue;
+                if (!should_collide(si.layer, sj.layer)) continue;
+                if (joint_blocks(ii, ij)) continue;
+                if (!ignored.empty() && ignored.count(body_pair_key(ii, ij))) continue;
+                T ei = min_extent(si), ej = min_extent(sj);
+                bool spec = (length(bi.v) * h > ei) || (length(bj.v) * h > ej);
+                Manifold<T>& m = slot[k];
+                generate_contacts(ii, si, bi.pos, bi.q, ij, sj, bj.pos, bj.q, m, spec);
+                if (!m.valid) continue;
+                m.sa = i; m.sb = j;
+            }
+        });
+        for (auto& m : slot)
+            if (m.valid) manifolds.push_back(m);
     }
 
     // ---------------- solver (dynamics/solver.h) ----------------

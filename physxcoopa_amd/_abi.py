@@ -11,6 +11,7 @@ import numpy as np
 PXG_OK = 0
 PXG_ERR_INVALID, PXG_ERR_CAPACITY, PXG_ERR_HIP, PXG_ERR_STATE, PXG_ERR_DEVICE = 1, 2, 3, 4, 5
 PXG_STATIC, PXG_KINEMATIC, PXG_DYNAMIC = 0, 1, 2
+PXG_INTERPOLATE, PXG_INTERPOLATE_NONE = 0, 1
 PXG_SPHERE, PXG_BOX, PXG_CAPSULE = 0, 1, 2
 PXG_AVERAGE, PXG_MINIMUM, PXG_MAXIMUM, PXG_MULTIPLY = 0, 1, 2, 3
 PXG_PHASE_FORCES_COLLIDE, PXG_PHASE_SOLVE_INTEGRATE = 0, 1
@@ -18,7 +19,7 @@ PXG_ORDER_REFERENCE, PXG_ORDER_COLOURED = 0, 1
 
 BODY_DTYPE = np.dtype([
     ("type", "<u4"), ("alive", "<u4"), ("awake", "<u4"), ("use_gravity", "<u4"),
-    ("constraints", "<u4"), ("generation", "<u4"), ("reserved0", "<u4"), ("reserved1", "<u4"),
+    ("constraints", "<u4"), ("generation", "<u4"), ("interpolation", "<u4"), ("reserved1", "<u4"),
     ("position", "<f4", 3), ("sleep_timer", "<f4"),
     ("orientation", "<f4", 4),
     ("linear_velocity", "<f4", 3), ("mass", "<f4"),
@@ -31,6 +32,12 @@ BODY_DTYPE = np.dtype([
     ("prev_orientation", "<f4", 4),
 ])
 assert BODY_DTYPE.itemsize == 212
+
+POSE_DTYPE = np.dtype([
+    ("position", "<f4", 3), ("type", "<u4"), ("orientation", "<f4", 4),
+    ("prev_position", "<f4", 3), ("interpolation", "<u4"), ("prev_orientation", "<f4", 4),
+])
+assert POSE_DTYPE.itemsize == 64
 
 SHAPE_DTYPE = np.dtype([
     ("owner", "<u4"), ("type", "<u4"), ("enabled", "<u4"), ("is_trigger", "<u4"),

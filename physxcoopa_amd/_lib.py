@@ -16,6 +16,8 @@ _SIGNATURES = {
     "pxg_set_gravity": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_set_bodies": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_bodies": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_set_bodies_subset": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_get_poses": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_shapes": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_layer_masks": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_set_joints": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),

@@ -60,7 +60,7 @@ def build_facade_test(verbose=False):
     src = os.path.join(root, "tests", "cpp", "physics_world_test.cpp")
     if not os.path.exists(src):
         return None
-    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(root, "include"),
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-pthread", "-I", os.path.join(root, "include"),
            src, "-L", HERE, "-lpxg", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", FACADE_TEST]
     if verbose:
         print(" ".join(cmd), flush=True)

@@ -37,7 +37,8 @@ inline bool pack_bodies(const pxg_body* in, size_t n, BodySoA& o) {
         o.ppos[i] = make_float4(b.prev_position[0], b.prev_position[1], b.prev_position[2], 0.0f);
         o.prot[i] = make_float4(b.prev_orientation[0], b.prev_orientation[1], b.prev_orientation[2], b.prev_orientation[3]);
         o.flags[i] = (b.type & BF_TYPE) | (b.alive ? BF_ALIVE : 0u) | (b.awake ? BF_AWAKE : 0u) |
-                     (b.use_gravity ? BF_GRAVITY : 0u) | ((b.constraints & 63u) << BF_CONSTRAINT_SHIFT);
+                     (b.use_gravity ? BF_GRAVITY : 0u) | ((b.constraints & 63u) << BF_CONSTRAINT_SHIFT) |
+                     (b.interpolation == PXG_INTERPOLATE_NONE ? BF_INTERP_NONE : 0u);
     }
     return true;
 }
@@ -52,6 +53,7 @@ inline void unpack_bodies(const BodySoA& s, const std::vector<float4>& iiw, size
         b.awake = (f & BF_AWAKE) ? 1u : 0u;
         b.use_gravity = (f & BF_GRAVITY) ? 1u : 0u;
         b.constraints = (f >> BF_CONSTRAINT_SHIFT) & 63u;
+        b.interpolation = (f & BF_INTERP_NONE) ? PXG_INTERPOLATE_NONE : PXG_INTERPOLATE;
         const float4 &p = s.pos[i], &r = s.rot[i], &v = s.vel[i], &w = s.angv[i], &fo = s.force[i], &to = s.torque[i];
         b.position[0] = p.x; b.position[1] = p.y; b.position[2] = p.z; b.sleep_timer = p.w;
         b.orientation[0] = r.x; b.orientation[1] = r.y; b.orientation[2] = r.z; b.orientation[3] = r.w;

@@ -167,6 +167,33 @@ __global__ void k_init_bodies(DevState d) {
     store_iiw(d.b_iiw, i, m);
 }
 
+// Writes n staged body records (9 float4 fields each, SoA over the n records, then the flag words)
+// into bodies idx[k]; with `inertia` the written bodies' world inverse inertia is recomputed as
+// k_init_bodies does for all of them
+__global__ void k_scatter_bodies(DevState d, const uint32_t* idx, const float4* rec, const uint32_t* flags, uint32_t n,
+                                 int inertia) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t i = idx[k];
+    d.b_pos[i] = rec[k];
+    d.b_rot[i] = rec[n + k];
+    d.b_vel[i] = rec[2 * n + k];
+    d.b_angv[i] = rec[3 * n + k];
+    d.b_force[i] = rec[4 * n + k];
+    d.b_torque[i] = rec[5 * n + k];
+    d.b_iil[i] = rec[6 * n + k];
+    d.b_prev_pos[i] = rec[7 * n + k];
+    d.b_prev_rot[i] = rec[8 * n + k];
+    const uint32_t f = flags[k];
+    d.b_flags[i] = f;
+    if (inertia) {
+        m3 m;
+        if ((f & BF_TYPE) != 2u) { m.c0 = splat(0.0f); m.c1 = splat(0.0f); m.c2 = splat(0.0f); }
+        else m = world_inertia(ldq(rec[n + k]), ld3(rec[6 * n + k]));
+        store_iiw(d.b_iiw, i, m);
+    }
+}
+
 // apply_gravity + integrate_forces (world.h:1283-1295, integrator.h:31-35, 78-94)
 __global__ void k_integrate_forces(DevState d, StepParams p) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2262,6 +2289,10 @@ static size_t scan_scratch_words(size_t n) {
 
 void launch_init_bodies(const DevState& d, hipStream_t st) {
     if (d.nb) hipLaunchKernelGGL(k_init_bodies, dim3(blocks(d.nb)), dim3(kBlock), 0, st, d);
+}
+void launch_scatter_bodies(const DevState& d, const uint32_t* idx, const float4* rec, const uint32_t* flags, uint32_t n,
+                           int inertia, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_scatter_bodies, dim3(blocks(n)), dim3(kBlock), 0, st, d, idx, rec, flags, n, inertia);
 }
 void launch_init_proxies(const DevState& d, const StepParams& p, hipStream_t st) {
     if (d.ns) hipLaunchKernelGGL(k_init_proxies, dim3(blocks(d.ns)), dim3(kBlock), 0, st, d, p);

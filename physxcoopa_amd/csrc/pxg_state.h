@@ -32,6 +32,7 @@ constexpr uint32_t kMaxShapes = (1u << 20) / kPairLanes;  // the pair-count scan
 
 // body flag bits
 constexpr uint32_t BF_TYPE = 3u, BF_ALIVE = 1u << 2, BF_AWAKE = 1u << 3, BF_GRAVITY = 1u << 4;
+constexpr uint32_t BF_INTERP_NONE = 1u << 5;  // render interpolation off (carried, never read by the step)
 constexpr int BF_CONSTRAINT_SHIFT = 8;
 // shape info bits (s_info.y)
 constexpr uint32_t SF_ENABLED = 1u << 4, SF_TRIGGER = 1u << 5, SF_STATIC = 1u << 6;
@@ -224,6 +225,10 @@ void launch_joint_velocity(const DevState& d, const StepParams& p, int rev, uint
 void launch_joint_position(const DevState& d, const StepParams& p, uint32_t begin, uint32_t end, hipStream_t st);
 void launch_joints_post(const DevState& d, const StepParams& p, uint32_t pos_iters, hipStream_t st);
 void launch_init_bodies(const DevState& d, hipStream_t st);
+// staged records (9 float4 fields, SoA over n) and flags into bodies idx[0..n); `inertia`: recompute
+// their world inverse inertia
+void launch_scatter_bodies(const DevState& d, const uint32_t* idx, const float4* rec, const uint32_t* flags, uint32_t n,
+                           int inertia, hipStream_t st);
 // contact events of this step (world.h:1964-2018, pxg_events.hip): sorts this step's valid
 // manifolds (at most n_pairs) into the snapshot ev_snap[cur], compares it with last step's (at
 // most prev_bound entries) and appends the events to the queue

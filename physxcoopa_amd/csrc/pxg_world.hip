@@ -57,6 +57,8 @@ struct pxg_world {
     uint32_t colour_round = 0;
     bool shapes_ready = false;
     void* noalloc_scratch = nullptr;  // this device's address of `scratch` (pxg_overlap_batch_noalloc)
+    uint8_t* body_stage = nullptr;    // staging of pxg_set_bodies_subset (grown on demand)
+    size_t body_stage_bytes = 0;
     // profiling: HIP event pairs around every velocity-pass launch
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -453,6 +455,73 @@ extern "C" int pxg_get_bodies(pxg_world* w, pxg_body* out, uint32_t n) {
     HIPCHK(hipMemcpyAsync(soa.flags.data(), d.b_flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     unpack_bodies(soa, iiw, n, out);
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_bodies_subset(pxg_world* w, const uint32_t* indices, const pxg_body* records, uint32_t n) {
+    if (!w || (n && (!indices || !records))) return fail(PXG_ERR_INVALID, "pxg_set_bodies_subset: null argument");
+    if (!n) return PXG_OK;
+    std::vector<uint8_t> seen(w->n_bodies, 0);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t i = indices[k];
+        if (i >= w->n_bodies) return fail(PXG_ERR_INVALID, "pxg_set_bodies_subset: index beyond the body count");
+        if (seen[i]) return fail(PXG_ERR_INVALID, "pxg_set_bodies_subset: an index appears twice");
+        seen[i] = 1;
+        if (records[k].type != w->body_types[i])
+            return fail(PXG_ERR_STATE, "pxg_set_bodies_subset: a body's type changes (upload the table with pxg_set_bodies)");
+    }
+    BodySoA soa;
+    if (!pack_bodies(records, n, soa)) return fail(PXG_ERR_INVALID, "pxg_set_bodies_subset: bad body type");
+    // one staging upload: the 9 float4 fields (SoA over the n records), then the flags and indices
+    const size_t f4 = size_t(9) * n * sizeof(float4), words = size_t(2) * n * sizeof(uint32_t);
+    if (f4 + words > w->body_stage_bytes) {
+        HIPCHK(hipStreamSynchronize(w->stream));
+        dfree(w, w->body_stage);
+        w->body_stage = nullptr;
+        w->body_stage_bytes = 0;
+        DALLOC(w->body_stage, f4 + words);
+        w->body_stage_bytes = f4 + words;
+    }
+    std::vector<uint8_t> host(f4 + words);
+    const std::vector<float4>* fields[9] = {&soa.pos, &soa.rot, &soa.vel, &soa.angv, &soa.force, &soa.torque,
+                                            &soa.iil, &soa.ppos, &soa.prot};
+    for (int f = 0; f < 9; ++f) std::memcpy(host.data() + size_t(f) * n * sizeof(float4), fields[f]->data(), n * sizeof(float4));
+    std::memcpy(host.data() + f4, soa.flags.data(), n * sizeof(uint32_t));
+    std::memcpy(host.data() + f4 + n * sizeof(uint32_t), indices, n * sizeof(uint32_t));
+    HIPCHK(hipMemcpyAsync(w->body_stage, host.data(), host.size(), hipMemcpyHostToDevice, w->stream));
+    const float4* rec = reinterpret_cast<const float4*>(w->body_stage);
+    const uint32_t* flags = reinterpret_cast<const uint32_t*>(w->body_stage + f4);
+    const uint32_t* idx = flags + n;
+    // as pxg_set_bodies: world inverse inertia recomputed outside the step, kept between its phases
+    launch_scatter_bodies(w->d, idx, rec, flags, n, w->phase == 0 ? 1 : 0, w->stream);
+    HIPCHK(hipStreamSynchronize(w->stream));  // the host staging buffer goes out of scope
+    return PXG_OK;
+}
+
+extern "C" int pxg_get_poses(pxg_world* w, pxg_pose* out, uint32_t n) {
+    if (!w || (!out && n)) return fail(PXG_ERR_INVALID, "pxg_get_poses: null argument");
+    if (n > w->n_bodies) return fail(PXG_ERR_INVALID, "pxg_get_poses: n exceeds the body count");
+    if (!n) return PXG_OK;
+    std::vector<float4> pos(n), rot(n), ppos(n), prot(n);
+    std::vector<uint32_t> flags(n);
+    const DevState& d = w->d;
+    hipStream_t s = w->stream;
+    HIPCHK(hipMemcpyAsync(pos.data(), d.b_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(rot.data(), d.b_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(ppos.data(), d.b_prev_pos, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(prot.data(), d.b_prev_rot, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(flags.data(), d.b_flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (uint32_t i = 0; i < n; ++i) {
+        pxg_pose& o = out[i];
+        o.position[0] = pos[i].x; o.position[1] = pos[i].y; o.position[2] = pos[i].z;
+        o.type = flags[i] & BF_TYPE;
+        o.orientation[0] = rot[i].x; o.orientation[1] = rot[i].y; o.orientation[2] = rot[i].z; o.orientation[3] = rot[i].w;
+        o.prev_position[0] = ppos[i].x; o.prev_position[1] = ppos[i].y; o.prev_position[2] = ppos[i].z;
+        o.interpolation = (flags[i] & BF_INTERP_NONE) ? PXG_INTERPOLATE_NONE : PXG_INTERPOLATE;
+        o.prev_orientation[0] = prot[i].x; o.prev_orientation[1] = prot[i].y; o.prev_orientation[2] = prot[i].z;
+        o.prev_orientation[3] = prot[i].w;
+    }
     return PXG_OK;
 }
 

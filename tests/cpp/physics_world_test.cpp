@@ -10,6 +10,7 @@
 #include <cstring>
 #include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -756,6 +757,223 @@ void compute_penetration_is_exact_against_box() {
     CHECK(world.compute_penetration(sphere, pxg::Vec3{0.0f, 0.0f, 0.4f}, pxg::Quat{}, ignore_floor).empty());
 }
 
+// The write-back of system/physics_system.h:1058-1075: each primary dynamic body's render pose,
+// glm::mix of the previous and current position and glm::slerp of the orientations at
+// interpolation_alpha() (current pose when the body's interpolation is None).
+struct RenderPose {
+    float p[3], q[4];
+};
+RenderPose slerp_pose(const pxg_pose& b, float alpha) {
+    RenderPose r;
+    if (b.interpolation == PXG_INTERPOLATE_NONE) {
+        std::memcpy(r.p, b.position, sizeof(r.p));
+        std::memcpy(r.q, b.orientation, sizeof(r.q));
+        return r;
+    }
+    for (int i = 0; i < 3; ++i) r.p[i] = b.prev_position[i] * (1.0f - alpha) + b.position[i] * alpha;
+    const float* a = b.prev_orientation;
+    float c[4] = {b.orientation[0], b.orientation[1], b.orientation[2], b.orientation[3]};
+    float cosv = a[0] * c[0] + a[1] * c[1] + a[2] * c[2] + a[3] * c[3];
+    if (cosv < 0.0f) {  // the shorter arc, as glm::slerp takes it
+        for (float& x : c) x = -x;
+        cosv = -cosv;
+    }
+    if (cosv > 1.0f - 1e-6f) {
+        for (int i = 0; i < 4; ++i) r.q[i] = a[i] * (1.0f - alpha) + c[i] * alpha;
+    } else {
+        const float angle = std::acos(cosv);
+        const float wa = std::sin((1.0f - alpha) * angle) / std::sin(angle), wc = std::sin(alpha * angle) / std::sin(angle);
+        for (int i = 0; i < 4; ++i) r.q[i] = a[i] * wa + c[i] * wc;
+    }
+    return r;
+}
+pxg_pose pose_of(const pxg_body& b) {
+    pxg_pose p;
+    std::memcpy(p.position, b.position, sizeof(p.position));
+    std::memcpy(p.orientation, b.orientation, sizeof(p.orientation));
+    std::memcpy(p.prev_position, b.prev_position, sizeof(p.prev_position));
+    std::memcpy(p.prev_orientation, b.prev_orientation, sizeof(p.prev_orientation));
+    p.type = b.type;
+    p.interpolation = b.interpolation;
+    return p;
+}
+
+// The transform write-back run as the reference runs it, from job workers: four threads each
+// take a range of bodies and read them through get_body() (the non-const overload, as
+// PhysicsSystem does) and through pose(); both give the serial pass's render poses bit for bit.
+// A read-only pass uploads nothing before the next step; a velocity edited through get_body() in a
+// worker uploads that one body, and the world then steps exactly as one whose whole body table
+// was re-uploaded with the same edit.
+void write_back_from_worker_threads_matches_a_serial_pass() {
+    auto build = [](pxg::PhysicsWorld& w, std::vector<pxg::BodyId>& ids) {
+        add_static_ground(w);
+        for (int i = 0; i < 96; ++i) {
+            pxg_body b = body(PXG_DYNAMIC, -2.5f + 0.55f * float(i % 10), -2.5f + 0.55f * float((i / 10) % 10),
+                              0.6f + 0.7f * float(i / 100) + 0.01f * float(i % 7));
+            b.angular_velocity[2] = 0.3f * float(i % 5);
+            if (i % 9 == 0) b.interpolation = PXG_INTERPOLATE_NONE;
+            ids.push_back(w.add_body(b, box_shape(0.25f, 0.25f, 0.25f)));
+        }
+    };
+    pxg::PhysicsWorld world, twin;
+    std::vector<pxg::BodyId> ids, twin_ids;
+    build(world, ids);
+    build(twin, twin_ids);
+    for (int i = 0; i < 20; ++i) {
+        world.step_fixed(kH);
+        twin.step_fixed(kH);
+    }
+    CHECK(world.step(2.5f * kH) == 2u);
+    CHECK_NEAR(world.interpolation_alpha(), 0.5f, 1e-4f);
+    twin.step_fixed(kH);
+    twin.step_fixed(kH);
+    const float alpha = world.interpolation_alpha();
+    CHECK(world.pose(ids[0])->interpolation == PXG_INTERPOLATE_NONE && world.pose(ids[1])->interpolation == PXG_INTERPOLATE);
+    // a new step: the workers' first calls download the state, the others wait for it
+    world.step_fixed(kH);
+    twin.step_fixed(kH);
+    std::vector<RenderPose> by_body(ids.size()), by_pose(ids.size());
+    auto worker = [&](size_t begin, size_t end) {
+        for (size_t k = begin; k < end; ++k) {
+            pxg_body* b = world.get_body(ids[k]);
+            if (!b || b->type != PXG_DYNAMIC) continue;
+            by_body[k] = slerp_pose(pose_of(*b), alpha);
+            by_pose[k] = slerp_pose(*world.pose(ids[k]), alpha);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < 4; ++t) pool.emplace_back(worker, ids.size() * t / 4, ids.size() * (t + 1) / 4);
+    for (auto& th : pool) th.join();
+    for (size_t k = 0; k < ids.size(); ++k) {
+        const RenderPose serial = slerp_pose(pose_of(*world.get_body(ids[k])), alpha);
+        CHECK(std::memcmp(&by_body[k], &serial, sizeof(RenderPose)) == 0);
+        CHECK(std::memcmp(&by_pose[k], &serial, sizeof(RenderPose)) == 0);
+    }
+    world.step_fixed(kH);
+    twin.step_fixed(kH);
+    CHECK(world.bodies_uploaded_last_step() == 0u);
+    // one worker edits one body's velocity; the others only read
+    std::vector<std::thread> edit;
+    for (int t = 0; t < 4; ++t)
+        edit.emplace_back([&, t]() {
+            for (size_t k = size_t(t); k < ids.size(); k += 4) {
+                pxg_body* b = world.get_body(ids[k]);
+                if (k == 37) b->linear_velocity[0] += 1.5f;
+            }
+        });
+    for (auto& th : edit) th.join();
+    {   // the twin gets the same edit through a full re-upload of its table
+        pxg_world* h = twin.handle();
+        const uint32_t n = twin.body_count();  // the ground and the 96 boxes
+        std::vector<pxg_body> all(n);
+        CHECK(pxg_get_bodies(h, all.data(), n) == PXG_OK);
+        all[twin_ids[37].index].linear_velocity[0] += 1.5f;
+        CHECK(pxg_set_bodies(h, all.data(), n) == PXG_OK);
+    }
+    world.step_fixed(kH);
+    CHECK(world.bodies_uploaded_last_step() == 1u);
+    for (int i = 0; i < 30; ++i) {
+        if (i) world.step_fixed(kH);
+        pxg_world* h = twin.handle();
+        CHECK(pxg_step_fixed(h, kH) == PXG_OK);
+        uint64_t ha = 0, hb = 0;
+        CHECK(pxg_state_hash(world.handle(), &ha) == PXG_OK && pxg_state_hash(h, &hb) == PXG_OK);
+        CHECK(ha == hb);
+    }
+}
+
+// The batched overlap and cast methods against the single-query ones on a small scene whose body
+// slots carry later generations (a removed and re-added body): the same BodyIds in the same order,
+// lists cut at the capacity while the counts stay whole, and the same cast hits.
+void batched_overlaps_and_casts_equal_single_queries() {
+    pxg::PhysicsWorld world;
+    add_static_ground(world);
+    std::vector<pxg::BodyId> ids;
+    for (int i = 0; i < 40; ++i) {
+        pxg_body b = body(PXG_DYNAMIC, -1.5f + 0.5f * float(i % 7), -1.5f + 0.5f * float(i / 7), 0.25f);
+        ids.push_back(world.add_body(b, i % 2 ? sphere_shape(0.22f) : box_shape(0.2f, 0.2f, 0.2f)));
+    }
+    world.remove_body(ids[5]);
+    world.remove_body(ids[12]);
+    world.step_fixed(kH);
+    ids[5] = world.add_body(body(PXG_DYNAMIC, 0.1f, 0.2f, 0.3f), sphere_shape(0.3f));
+    ids[12] = world.add_body(body(PXG_DYNAMIC, -0.4f, 0.7f, 0.3f), box_shape(0.3f, 0.2f, 0.25f));
+    CHECK(ids[5].generation == 1u && ids[12].generation == 1u);
+    for (int i = 0; i < 10; ++i) world.step_fixed(kH);
+    std::vector<pxg_overlap_query> qs;
+    for (int i = 0; i < 24; ++i) {
+        pxg_overlap_query q;
+        std::memset(&q, 0, sizeof(q));
+        const float x = -1.4f + 0.13f * float(i), y = 1.0f - 0.09f * float(i);
+        q.type = uint32_t(i % 3);
+        q.center[0] = x; q.center[1] = y; q.center[2] = 0.3f;
+        q.radius = 0.3f + 0.05f * float(i % 5);
+        q.half_extents[0] = 0.4f; q.half_extents[1] = 0.3f; q.half_extents[2] = 0.2f;
+        q.orientation[2] = std::sin(0.1f * float(i)); q.orientation[3] = std::cos(0.1f * float(i));
+        q.a[0] = x - 0.5f; q.a[1] = y; q.a[2] = 0.2f;
+        q.b[0] = x + 0.5f; q.b[1] = y + 0.3f; q.b[2] = 0.3f;
+        qs.push_back(q);
+    }
+    for (uint32_t cap : {64u, 2u}) {
+        std::vector<std::vector<pxg::BodyId>> lists;
+        std::vector<uint32_t> counts;
+        world.overlap_batch(qs, lists, counts, cap);
+        bool truncated = false;
+        for (size_t i = 0; i < qs.size(); ++i) {
+            const pxg_overlap_query& q = qs[i];
+            std::vector<pxg::BodyId> one;
+            if (q.type == PXG_SPHERE) one = world.overlap_sphere(pxg::Vec3{q.center[0], q.center[1], q.center[2]}, q.radius);
+            else if (q.type == PXG_BOX)
+                one = world.overlap_box(pxg::Vec3{q.center[0], q.center[1], q.center[2]},
+                                        pxg::Vec3{q.half_extents[0], q.half_extents[1], q.half_extents[2]},
+                                        pxg::Quat{q.orientation[0], q.orientation[1], q.orientation[2], q.orientation[3]});
+            else one = world.overlap_capsule(pxg::Vec3{q.a[0], q.a[1], q.a[2]}, pxg::Vec3{q.b[0], q.b[1], q.b[2]}, q.radius);
+            CHECK(counts[i] == one.size());
+            CHECK(lists[i].size() == std::min<size_t>(cap, one.size()));
+            for (size_t k = 0; k < lists[i].size(); ++k) CHECK(lists[i][k] == one[k] && world.is_valid(lists[i][k]));
+            truncated = truncated || one.size() > cap;
+        }
+        if (cap == 2u) CHECK(truncated);
+    }
+    std::vector<pxg_cast_query> casts;
+    for (int i = 0; i < 18; ++i) {
+        pxg_cast_query c;
+        std::memset(&c, 0, sizeof(c));
+        c.shape = pxg::no_shape();
+        c.shape.enabled = 1;
+        c.shape.type = uint32_t(i % 3);
+        c.shape.radius = 0.15f;
+        c.shape.half_extents[0] = 0.1f; c.shape.half_extents[1] = 0.15f; c.shape.half_extents[2] = 0.1f;
+        c.shape.capsule_radius = 0.1f;
+        c.shape.capsule_half_height = 0.2f;
+        c.shape.capsule_axis = 0;
+        c.position[0] = -1.6f + 0.18f * float(i); c.position[1] = -1.2f + 0.1f * float(i); c.position[2] = 2.0f;
+        c.orientation[3] = 1.0f;
+        c.direction[2] = -1.0f;
+        c.max_distance = 5.0f;
+        casts.push_back(c);
+    }
+    std::vector<pxg::RaycastHit> hits;
+    std::vector<uint8_t> found;
+    world.shape_cast_batch(casts, hits, found);
+    int n_found = 0;
+    for (size_t i = 0; i < casts.size(); ++i) {
+        const pxg_cast_query& c = casts[i];
+        const pxg::Vec3 o{c.position[0], c.position[1], c.position[2]}, d{0.0f, 0.0f, -1.0f};
+        pxg::RaycastHit one;
+        bool hit;
+        if (c.shape.type == PXG_SPHERE) hit = world.sphere_cast(o, 0.15f, d, 5.0f, one);
+        else if (c.shape.type == PXG_BOX) hit = world.box_cast(o, pxg::Vec3{0.1f, 0.15f, 0.1f}, pxg::Quat{}, d, 5.0f, one);
+        else hit = world.capsule_cast(o, 0.1f, 0.2f, 0, pxg::Quat{}, d, 5.0f, one);
+        CHECK(hit == (found[i] != 0));
+        if (!hit) continue;
+        ++n_found;
+        CHECK(hits[i].body == one.body && hits[i].shape_index == one.shape_index && hits[i].distance == one.distance);
+        CHECK(hits[i].point.x == one.point.x && hits[i].normal.z == one.normal.z);
+    }
+    CHECK(n_found > 9);
+}
+
 int main() {
     struct T {
         const char* name;
@@ -790,6 +1008,8 @@ int main() {
         {"overlaps_find_only_touched_bodies", overlaps_find_only_touched_bodies},
         {"sweeps_are_exact_against_box_corner_and_capsule", sweeps_are_exact_against_box_corner_and_capsule},
         {"compute_penetration_is_exact_against_box", compute_penetration_is_exact_against_box},
+        {"write_back_from_worker_threads_matches_a_serial_pass", write_back_from_worker_threads_matches_a_serial_pass},
+        {"batched_overlaps_and_casts_equal_single_queries", batched_overlaps_and_casts_equal_single_queries},
     };
     for (const T& t : tests) {
         int before = g_failures;
