@@ -52,11 +52,27 @@ def calibrate_copy():
     print(json.dumps({"copy_bytes_read_per_launch": n, "copy_bytes_written_per_launch": n, "launches": 10}), flush=True)
 
 
+def batch_step_bytes(st, cfg):
+    """Equivalent-HBM algorithmic bytes of one step of every scene from their counts (the canonical
+    per-body, per-pair, per-manifold and per-point figures: 392 B per dynamic body; 128 B per
+    candidate pair; per manifold with n points 32 + 24n written, (192 + 40n) per velocity or relax
+    pass, 92 + 4n per position pass and 248 + 44n for warm start, cache rebuild and restitution)."""
+    M = float(st["manifolds"].sum())
+    C = float(st["points"].sum())
+    P = float(st["pairs"].sum())
+    vel = float(st["velocity_passes"].sum()) / max(1, len(st))  # passes per step (fixed count or residual mode)
+    per_m = (32 + 192 * (vel + cfg.relax_iterations) + 92 * cfg.position_iterations + 248) * M
+    per_p = (24 + 40 * (vel + cfg.relax_iterations) + 4 * cfg.position_iterations + 44) * C
+    return 128.0 * P + per_m + per_p
+
+
 def bench_batch(args):
     """Batched scenes stepped K at a time in one graph-captured persistent launch. batch: 4096
     independent scenes (half box pyramids, half random drops of the pile's shape mix) of 8-256
-    bodies; ragdolls: 2048 scenes of one 15-bone ragdoll over 32 boxes (joints)."""
+    bodies; ragdolls: 2048 scenes of one 15-bone ragdoll over 32 boxes (joints). The launch
+    parameters come from the autotuner (physxcoopa_amd/autotune.py) run on the batch itself."""
     import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
     from physxcoopa_amd import scenes, _abi, autotune, learned
     from physxcoopa_amd.batch import Batch
     t0 = time.time()
@@ -78,11 +94,14 @@ def bench_batch(args):
             batch.set_ignored_pairs(io, ig)
         if net is not None:
             batch.set_mlp(net)
-    (block, lpt), timings = autotune.tune(cfg, b, s, bo, so, steps=8, use_cache=False, log=log, setup=setup)
+    t_tune = time.time()
+    choice, timings = autotune.tune(cfg, b, s, bo, so, steps=30, use_cache=False, log=log, setup=setup)
+    t_tune = time.time() - t_tune
+    log("autotuner chose %s in %.1f s" % (autotune.label(choice), t_tune))
     bt = Batch(cfg, b, s, bo, so)
     setup(bt)
-    bt.set_tuning(block, lpt)
-    chunk = 30
+    autotune.apply(bt, choice)
+    chunk = choice["steps_per_launch"]
     done = 0
     while done < args.warmup:
         k = min(chunk, args.warmup - done)
@@ -90,53 +109,76 @@ def bench_batch(args):
         done += k
     bt.synchronize()
     log("warm-up %d steps in %.1fs" % (args.warmup, time.time() - t0))
-    t_start = time.perf_counter()
-    done = 0
-    while done < args.steps:
-        k = min(chunk, args.steps - done)
-        bt.step(k)
-        done += k
-    bt.synchronize()
-    elapsed = time.perf_counter() - t_start
-    st = bt.stats()
+    # timed: whole launches of `chunk` steps; the statistics after each launch give the counts
+    # the algorithmic bytes are taken from (one download per launch, outside the timed region)
+    launches = max(1, (args.steps + chunk - 1) // chunk)
+    elapsed, step_bytes = 0.0, 0.0
+    for _ in range(launches):
+        t1 = time.perf_counter()
+        bt.step(chunk)
+        bt.synchronize()
+        elapsed += time.perf_counter() - t1
+        st = bt.stats()
+        step_bytes += chunk * batch_step_bytes(st, cfg)
+    steps = launches * chunk
     assert np.all(st["error"] == 0), "batch reported errors"
-    value = n_dyn * args.steps / elapsed
+    value = n_dyn * steps / elapsed
     cpu = None
     if not args.no_cpu:
+        # scenes stepped concurrently on host threads (the oracle's ctypes calls release the
+        # interpreter lock), each scene an oracle world of its own from the GPU state
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pxo
         bodies = bt.bodies()
         rng = np.random.default_rng(0)
-        t_cpu, bs_cpu = 0.0, 0
         gj = bt.joints() if jt is not None else None
-        for k in rng.permutation(len(bo) - 1):
+        threads = cpu_threads()
+        order = rng.permutation(len(bo) - 1)
+
+        def run_scene(k):
             sb, ss = bodies[bo[k]:bo[k + 1]], s[so[k]:so[k + 1]]
             ow = pxo.OracleWorld(cfg, sb, ss)
             if gj is not None:
                 ow.set_joints(gj[jo[k]:jo[k + 1]])
                 ow.set_ignored_pairs(ig[io[k]:io[k + 1]])
-            t_cpu += ow.run(10)
-            bs_cpu += int(np.sum(sb["type"] == 2)) * 10
-            if t_cpu > args.cpu_seconds:
-                break
-        cpu = {"value": round(bs_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": 1, "kind": "port",
-               "sample": "randomly chosen scenes of the batch, 10 steps each from the GPU state, %d body-steps "
-                         "(C++ oracle, -O2, one thread)" % bs_cpu}
+            ow.run(10)
+            return int(np.sum(sb["type"] == 2)) * 10
+        bs_cpu, n_sc = 0, 0
+        t_cpu0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            for start in range(0, len(order), 4 * threads):
+                bs_cpu += sum(ex.map(run_scene, order[start:start + 4 * threads]))
+                n_sc += len(order[start:start + 4 * threads])
+                if time.perf_counter() - t_cpu0 > args.cpu_seconds:
+                    break
+        t_cpu = time.perf_counter() - t_cpu0
+        cpu = {"value": round(bs_cpu / t_cpu, 1), "unit": "body·steps/s", "cores": threads, "kind": "port",
+               "sample": "%d randomly chosen scenes of the batch, 10 steps each from the GPU state, %d body-steps "
+                         "(C++ oracle, -O2, one scene per host thread, %d threads)" % (n_sc, bs_cpu, threads)}
+    st = bt.stats()
+    ach = step_bytes / elapsed / 1e9
     out = {
         "metric": "body·steps/s at solver residual <=1e-4 (plain vs batched+learned); HBM GB/s",
-        "value": round(value, 1), "unit": "body·steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * elapsed / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "value": round(value, 1), "unit": "body·steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / steps, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": name, "scenes": len(bo) - 1, "dynamic_bodies": n_dyn, "joints": 0 if jt is None else len(jt),
                    "velocity_iterations": int(cfg.velocity_iterations), "position_iterations": int(cfg.position_iterations),
-                   "block": block, "longest_first": lpt, "steps_per_launch": chunk},
+                   "tuning": choice},
         "learned_warm_start": net is not None,
         "learned_predictions": int(st["learned_predictions"].sum()) if net is not None else 0,
-        "autotune_seconds": timings,
+        "autotune": {"seconds": round(t_tune, 1), "ms_per_step": {k: round(1e3 * v, 4) for k, v in timings.items()}},
         "manifolds": int(st["manifolds"].sum()), "points": int(st["points"].sum()),
-        "roofline": None,
+        "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_batch",
+                     "bytes_per_launch": round(step_bytes / launches, 1),
+                     "avg_launch_us": round(1e6 * elapsed / launches, 3),
+                     "note": "equivalent-HBM algorithmic bytes (the scenes live in LDS during a launch); launch time "
+                             "from the host clock around each synchronised graph launch"},
         "cpu_baseline": cpu,
+        "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
+    bt.close()
     print(json.dumps(out), flush=True)
 
 
@@ -302,6 +344,88 @@ def bench_queries(args):
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
+
+
+def state_aggregates(bodies, manifolds):
+    """Kinetic energy (J; linear plus rotational, the body-frame inertia from inv_inertia_local),
+    the largest contact penetration (m) and the number of sleeping dynamic bodies of one state."""
+    import numpy as np
+    b = bodies[(bodies["type"] == 2) & (bodies["alive"] != 0)]
+    v = b["linear_velocity"].astype(np.float64)
+    w = b["angular_velocity"].astype(np.float64)
+    q = b["orientation"].astype(np.float64)
+    # the angular velocity in the body frame: rotate by the conjugate quaternion
+    u, s = -q[:, :3], q[:, 3:4]
+    t = 2.0 * np.cross(u, w)
+    wb = w + s * t + np.cross(u, t)
+    iil = b["inv_inertia_local"].astype(np.float64)
+    rot = np.where(iil > 0, wb ** 2 / np.where(iil > 0, iil, 1.0), 0.0).sum(1)
+    ke = float(0.5 * (b["mass"].astype(np.float64) * (v ** 2).sum(1)).sum() + 0.5 * rot.sum())
+    pen = 0.0
+    for m in manifolds:
+        if m["valid"] and not m["is_trigger"] and m["count"]:
+            pen = max(pen, float(m["points"]["penetration"][:m["count"]].max()))
+    return {"kinetic_energy": ke, "max_penetration": pen, "asleep": int((b["awake"] == 0).sum())}
+
+
+def bench_aggregates(args):
+    """The FP32 GPU step against the FP32 and FP64 oracles over 300 steps of config 1 (64 boxes) and
+    config 2 (the 16,384-body pile) at the fixed iteration counts: the GPU's state hash is compared
+    with the FP32 oracle's every step, and the 300-step aggregates (kinetic energy, maximum
+    penetration, sleeping bodies) and body positions of FP32 runs are compared with the FP64 run."""
+    import numpy as np
+    from physxcoopa_amd import scenes, _abi
+    from physxcoopa_amd.world import World
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pxo
+    threads = cpu_threads()
+    out = {}
+    t_all = time.time()
+    for name, sc in (("config1", scenes.config1_scene(seed=1)),
+                     ("config2", scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40)))):
+        cfg = _abi.default_config(**sc.config)
+        gw = World(cfg, sc.bodies, sc.shapes)
+        of = pxo.OracleWorld(cfg, sc.bodies, sc.shapes)
+        od = pxo.OracleWorld(cfg, sc.bodies, sc.shapes, dbl=True)
+        of.set_threads(threads)
+        od.set_threads(threads)
+        equal = 0
+        trace = {"gpu": [], "fp32": [], "fp64": []}
+        pos_err = {}
+        steps = 300
+        for i in range(steps):
+            gw.step_fixed()
+            of.step_fixed()
+            od.step_fixed()
+            equal += int(gw.state_hash() == of.state_hash())
+            gb, fb, db = gw.bodies(), of.bodies(), od.bodies()
+            trace["gpu"].append(state_aggregates(gb, gw.manifolds()))
+            trace["fp32"].append(state_aggregates(fb, of.manifolds()))
+            trace["fp64"].append(state_aggregates(db, od.manifolds()))
+            if i + 1 in (60, 150, 300):
+                d = np.abs(gb["position"].astype(np.float64) - db["position"].astype(np.float64)).max(axis=1)
+                pos_err[str(i + 1)] = {"max": round(float(d.max()), 7), "median": round(float(np.median(d)), 9)}
+            if (i + 1) % 50 == 0:
+                log("%s: %d/%d steps, %.0f s" % (name, i + 1, steps, time.time() - t_all))
+        def dev(key):
+            a = np.array([t[key] for t in trace["gpu"]], np.float64)
+            b = np.array([t[key] for t in trace["fp64"]], np.float64)
+            return {"max_abs": round(float(np.abs(a - b).max()), 7),
+                    "max_rel": round(float((np.abs(a - b) / np.maximum(np.abs(b), 1e-12)).max()), 5)}
+        out[name] = {
+            "dynamic_bodies": sc.n_dynamic, "steps": steps,
+            "gpu_hash_equals_oracle_fp32_steps": equal,
+            "final": {k: {kk: (round(vv, 7) if isinstance(vv, float) else vv) for kk, vv in trace[k][-1].items()}
+                      for k in trace},
+            "gpu_vs_fp64_over_steps": {"kinetic_energy": dev("kinetic_energy"), "max_penetration": dev("max_penetration"),
+                                       "asleep_max_abs": int(max(abs(a["asleep"] - b["asleep"])
+                                                                 for a, b in zip(trace["gpu"], trace["fp64"])))},
+            "position_gpu_fp32_vs_fp64_m": pos_err,
+        }
+        gw.close()
+    print(json.dumps({"metric": "FP32 GPU step against the FP32 and FP64 CPU oracles (300-step aggregates)",
+                      "value": None, "unit": None, "n_gpus": 1, "data": "synthetic", "cpu_threads": threads,
+                      "results": out}), flush=True)
 
 
 def cpu_threads():
@@ -566,7 +690,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["plain", "batched_learned"], default=None,
                     help="pile: run one of the two configurations only (for profiling)")
-    ap.add_argument("--workload", choices=["pile", "batch", "ragdolls", "learned", "queries"], default="pile",
+    ap.add_argument("--workload", choices=["pile", "batch", "ragdolls", "learned", "queries", "aggregates"], default="pile",
                     help="pile: the 16k-body pile (one world); batch: 4096 independent scenes of 8-256 bodies; "
                          "ragdolls: 2048 scenes of a jointed ragdoll over 32 boxes; learned: config-1 piles, "
                          "passes and time to the residual with and without the learned warm start; queries: "
@@ -588,6 +712,8 @@ def main():
         bench_learned(args)
     elif args.workload == "queries":
         bench_queries(args)
+    elif args.workload == "aggregates":
+        bench_aggregates(args)
     else:
         bench_pile(args)
 

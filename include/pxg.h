@@ -41,7 +41,7 @@ enum pxg_body_type { PXG_STATIC = 0, PXG_KINEMATIC = 1, PXG_DYNAMIC = 2 };
  * (dynamics::Interpolation, dynamics/body.h:26-33; Body::interpolation defaults to Interpolate,
  * body.h:118). Carried with the body; the step does not read it. */
 enum pxg_interpolation { PXG_INTERPOLATE = 0, PXG_INTERPOLATE_NONE = 1 };
-enum pxg_shape_type { PXG_SPHERE = 0, PXG_BOX = 1, PXG_CAPSULE = 2 };
+enum pxg_shape_type { PXG_SPHERE = 0, PXG_BOX = 1, PXG_CAPSULE = 2, PXG_CONVEX_HULL = 3 };
 enum pxg_combine { PXG_AVERAGE = 0, PXG_MINIMUM = 1, PXG_MAXIMUM = 2, PXG_MULTIPLY = 3 };
 enum pxg_phase { PXG_PHASE_FORCES_COLLIDE = 0, PXG_PHASE_SOLVE_INTEGRATE = 1 };
 
@@ -142,11 +142,11 @@ typedef struct pxg_shape {
     uint32_t enabled;        /* 0 = no broadphase proxy (body without collider) */
     uint32_t is_trigger;
     uint32_t layer;          /* 0..31 */
-    int32_t capsule_axis;    /* 0 = X, 1 = Y, 2 = Z */
+    int32_t capsule_axis;    /* 0 = X, 1 = Y, 2 = Z; PXG_CONVEX_HULL: the hull's index in pxg_set_hulls' table */
     uint32_t friction_combine;
     uint32_t restitution_combine;
     float local_center[3];
-    float radius;            /* sphere */
+    float radius;            /* sphere; PXG_CONVEX_HULL: the uniform scale (Shape::hull_scale, shape.h:98) */
     float local_rotation[4]; /* x, y, z, w */
     float half_extents[3];   /* box */
     float capsule_radius;
@@ -316,6 +316,23 @@ int pxg_get_config(pxg_world* w, pxg_config* out);                /* config(), w
 int pxg_set_config(pxg_world* w, const pxg_config* cfg);          /* capacities must not change */
 int pxg_set_gravity(pxg_world* w, const float g[3]);              /* set_gravity(), world.h:1144 */
 
+/* A built convex hull (geometry::ConvexHull, geometry/convex_hull.h:38-94), as the reference's
+ * build_convex_hull() leaves it: vertices, merged polygon faces (outward unit normal and plane
+ * offset d, vertex lists counter-clockwise about the normal), unique edges (vertex index pairs,
+ * lower first) and the local bounds. Caller-owned arrays, copied by pxg_set_hulls. */
+typedef struct pxg_hull {
+    const float* vertices;         /* 3 per vertex */
+    uint32_t n_vertices;           /* at most 64 (the builder's default cap) */
+    uint32_t n_faces;
+    const float* face_planes;      /* 4 per face: normal xyz, d */
+    const uint32_t* face_starts;   /* n_faces + 1: face f spans [face_starts[f], face_starts[f + 1]) */
+    const uint32_t* face_vertices; /* at most 8 per face on the GPU */
+    const uint32_t* edges;         /* 2 per edge */
+    uint32_t n_edges;
+    float local_min[3];
+    float local_max[3];
+} pxg_hull;
+
 /* --- state upload / download (caller-owned host buffers are copied) --- */
 /* Replaces the body table (add_body/remove_body, world.h:95-143). World inverse inertia is
  * recomputed from orientation exactly as add_body() does. */
@@ -343,6 +360,12 @@ int pxg_get_poses(pxg_world* w, pxg_pose* out, uint32_t n);
 /* Replaces the shape table and creates each enabled shape's broadphase proxy with
  * fat = tight.expand(margin) at its owner's current pose (create_shape_proxy_, world.h:1369-1382). */
 int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n);
+/* Replaces the table of convex hulls that PXG_CONVEX_HULL shapes index (the hulls a
+ * ConvexCollider's Shape::hull points at, shape.h:93-98); upload before the shapes that use them.
+ * Hull narrowphase: collision/sat_hull.h (hull against hull, box, sphere and capsule; SAT, no
+ * speculative contacts). A hull with more than 64 vertices or a face of more than 8 vertices is
+ * refused (PXG_ERR_INVALID). */
+int pxg_set_hulls(pxg_world* w, const pxg_hull* hulls, uint32_t n);
 /* 32 x 32 layer matrix rows (broadphase/layer_matrix.h:21-47). */
 int pxg_set_layer_masks(pxg_world* w, const uint32_t masks[32]);
 /* Replaces the joint table: the joints_ array behind add_hinge_joint / add_ball_joint /
@@ -545,6 +568,10 @@ int pxg_batch_create(const pxg_config* cfg, uint32_t n_scenes, const uint32_t* b
 int pxg_batch_destroy(pxg_batch* b);
 /* block_size 64/128/256/512 threads per scene; longest_first = take scenes by decreasing size */
 int pxg_batch_set_tuning(pxg_batch* b, uint32_t block_size, uint32_t longest_first);
+/* Workgroups (one scene each at a time, scenes from a queue) launched per compute unit: 0 = as
+ * many as the kernel's LDS and registers allow; larger values are clamped to that. Tunings change
+ * only the speed, never the results. */
+int pxg_batch_set_occupancy(pxg_batch* b, uint32_t workgroups_per_cu);
 int pxg_batch_set_layer_masks(pxg_batch* b, const uint32_t masks[32]);
 int pxg_batch_step(pxg_batch* b, float h, uint32_t steps);
 int pxg_batch_synchronize(pxg_batch* b);

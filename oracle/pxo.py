@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from physxcoopa_amd._abi import BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, ptr
+from physxcoopa_amd._abi import BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, Hull, hull_table, ptr
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libpxo.so")
@@ -31,7 +31,10 @@ def lib():
             "pxo_create": ([vp, i32], vp), "pxo_destroy": ([vp], None), "pxo_set_config": ([vp, vp], None),
             "pxo_set_bodies": ([vp, vp, u32], None), "pxo_get_bodies": ([vp, vp, u32], None),
             "pxo_set_shapes": ([vp, vp, u32], None), "pxo_set_layer_masks": ([vp, vp], None),
-            "pxo_set_brute_force": ([vp, u32], None), "pxo_set_threads": ([vp, u32], None), "pxo_set_order": ([vp, vp, u32], None),
+            "pxo_set_brute_force": ([vp, u32], None), "pxo_set_threads": ([vp, u32], None),
+            "pxo_set_hulls": ([vp, vp, u32], None), "pxo_set_kat_hulls": ([vp, u32], None),
+            "pxo_shape_bounds": ([vp, vp, vp, vp, vp], None),
+            "pxo_build_hull": ([vp, u32, f32, u32, vp, vp, vp, vp, vp, vp, vp, vp], i32), "pxo_set_order": ([vp, vp, u32], None),
             "pxo_set_miss_table": ([vp, vp, vp, u32], None), "pxo_step_fixed": ([vp, f32], None),
             "pxo_step_phase": ([vp, u32, f32], None), "pxo_run": ([vp, u32, f32], ctypes.c_double),
             "pxo_get_pairs": ([vp, vp, u32], u32), "pxo_get_manifolds": ([vp, vp, u32], u32),
@@ -78,7 +81,7 @@ def f32(x, n=None):
 class OracleWorld:
     """The CPU reference step over pxg ABI arrays. dbl=True runs the same algorithm in float64."""
 
-    def __init__(self, config, bodies, shapes, dbl=False, brute_force=False, layer_masks=None):
+    def __init__(self, config, bodies, shapes, dbl=False, brute_force=False, layer_masks=None, hulls=None):
         L = lib()
         self._cfg = config
         self.h = L.pxo_create(ctypes.byref(config), 1 if dbl else 0)
@@ -90,6 +93,9 @@ class OracleWorld:
         if layer_masks is not None:
             m = np.ascontiguousarray(np.asarray(layer_masks, np.uint32))
             L.pxo_set_layer_masks(self.h, ptr(m))
+        if hulls:
+            tab = hull_table(hulls)
+            L.pxo_set_hulls(self.h, ctypes.byref(tab), len(hulls))
         L.pxo_set_shapes(self.h, ptr(s), self.n_shapes)
         if brute_force:
             L.pxo_set_brute_force(self.h, 1)
@@ -373,3 +379,41 @@ def ray_aabb(ray, mn, mx):
 
 def pack_key(a, b, fid):
     return int(lib().pxo_pack_key(a, b, fid))
+
+
+def build_hull(points, tolerance=1e-5, max_vertices=64):
+    """build_convex_hull (geometry/convex_hull.h:188-449) restated: a Hull, or None for a
+    degenerate cloud."""
+    p = np.ascontiguousarray(np.asarray(points, np.float32).reshape(-1, 3))
+    sizes = np.array([256, 512, 2048, 1024], np.uint32)
+    v = np.zeros((256, 3), np.float32)
+    pl = np.zeros((512, 4), np.float32)
+    fs = np.zeros(513, np.uint32)
+    fv = np.zeros(2048, np.uint32)
+    ed = np.zeros((1024, 2), np.uint32)
+    b6 = np.zeros(6, np.float32)
+    vc = np.zeros(4, np.float32)
+    ok = lib().pxo_build_hull(ptr(p), len(p), np.float32(tolerance), max_vertices, ptr(sizes), ptr(v), ptr(pl), ptr(fs),
+                              ptr(fv), ptr(ed), ptr(b6), ptr(vc))
+    if not ok:
+        return None
+    nv, nf, nfv, ne = (int(x) for x in sizes)
+    return Hull(v[:nv], pl[:nf], fs[:nf + 1], fv[:nfv], ed[:ne], b6, volume=float(vc[0]), centroid=vc[1:4].copy())
+
+
+def set_kat_hulls(hulls):
+    """The hull table the single-pair helpers (generate_contacts, shape_bounds) resolve
+    PXG_CONVEX_HULL shapes against (the shape's capsule_axis is the index)."""
+    tab = hull_table(hulls)
+    lib().pxo_set_kat_hulls(ctypes.byref(tab), len(hulls))
+
+
+def shape_bounds(shape, pos, rot):
+    """(world AABB as 6 floats, CCD min extent) of a posed shape (shape.h:198-228, world.h:1411-1424)."""
+    s = np.ascontiguousarray(shape).reshape(1)
+    out = np.zeros(6, np.float32)
+    ext = np.zeros(1, np.float32)
+    keep = [f32(pos, 3), f32(rot, 4)]
+    lib().pxo_shape_bounds(ptr(s), ptr(keep[0]), ptr(keep[1]), ptr(out), ptr(ext))
+    return out, float(ext[0])
+

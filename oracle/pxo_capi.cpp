@@ -56,7 +56,27 @@ template <class T> Shape<T> shape_from(const pxg_shape& d) {
     s.layer = d.layer; s.caxis = d.capsule_axis; s.fcomb = int(d.friction_combine); s.rcomb = int(d.restitution_combine);
     s.lc = v3<T>(d.local_center); s.radius = T(d.radius); s.lr = q4<T>(d.local_rotation); s.he = v3<T>(d.half_extents);
     s.cr = T(d.capsule_radius); s.chh = T(d.capsule_half_height); s.friction = T(d.dynamic_friction); s.restitution = T(d.restitution);
+    if (s.type == PXG_CONVEX_HULL) s.hull_scale = T(d.radius);
     return s;
+}
+// a hull shape's pointer into the world's hull table (nullptr for a bad index: no contacts)
+template <class T> void resolve_hull(Shape<T>& s, const std::vector<Hull<T>>& hulls, int32_t index) {
+    if (s.type != PXG_CONVEX_HULL) return;
+    s.hull = (index >= 0 && size_t(index) < hulls.size()) ? &hulls[size_t(index)] : nullptr;
+}
+inline Hull<float> hull_from(const pxg_hull& d) {
+    Hull<float> h;
+    for (uint32_t i = 0; i < d.n_vertices; ++i) h.v.push_back(V3<float>(d.vertices[3 * i], d.vertices[3 * i + 1], d.vertices[3 * i + 2]));
+    for (uint32_t f = 0; f < d.n_faces; ++f) {
+        h.pn.push_back(V3<float>(d.face_planes[4 * f], d.face_planes[4 * f + 1], d.face_planes[4 * f + 2]));
+        h.pd.push_back(d.face_planes[4 * f + 3]);
+    }
+    h.fs.assign(d.face_starts, d.face_starts + d.n_faces + 1);
+    h.fv.assign(d.face_vertices, d.face_vertices + d.face_starts[d.n_faces]);
+    for (uint32_t e = 0; e < d.n_edges; ++e) h.e.push_back({d.edges[2 * e], d.edges[2 * e + 1]});
+    h.lb.mn = V3<float>(d.local_min[0], d.local_min[1], d.local_min[2]);
+    h.lb.mx = V3<float>(d.local_max[0], d.local_max[1], d.local_max[2]);
+    return h;
 }
 template <class T> void manifold_to(const Manifold<T>& m, pxg_manifold& d) {
     std::memset(&d, 0, sizeof(d));
@@ -202,7 +222,10 @@ void pxo_set_shapes(void* p, const pxg_shape* in, uint32_t n) {
     auto go = [&](auto& w) {
         using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
         w.shapes.resize(n);
-        for (uint32_t i = 0; i < n; ++i) w.shapes[i] = shape_from<T>(in[i]);
+        for (uint32_t i = 0; i < n; ++i) {
+            w.shapes[i] = shape_from<T>(in[i]);
+            resolve_hull(w.shapes[i], w.hulls, in[i].capsule_axis);
+        }
         w.create_proxies();
     };
     if (h->dbl) go(h->d); else go(h->f);
@@ -241,6 +264,45 @@ void pxo_set_atan2_mode(void* p, uint32_t mode) {
     Handle* h = static_cast<Handle*>(p);
     h->f.atan2_mode = mode ? Atan2Mode::Rounded : Atan2Mode::Libm;
     h->d.atan2_mode = h->f.atan2_mode;
+}
+// The hull table PXG_CONVEX_HULL shapes index (upload before the shapes).
+void pxo_set_hulls(void* p, const pxg_hull* in, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    h->f.hulls.clear();
+    h->d.hulls.clear();
+    for (uint32_t i = 0; i < n; ++i) {
+        Hull<float> hf = hull_from(in[i]);
+        h->f.hulls.push_back(hf);
+        h->d.hulls.push_back(hf.as<double>());
+    }
+}
+// build_convex_hull (convex_hull.h:188-449) of n points (xyz). Returns 1 when a hull was built;
+// sizes[0..3] = vertices, faces, face vertex entries, edges; the arrays receive at most the caps
+// given in sizes[0..3] on entry. volume_centroid: volume, centroid xyz.
+int pxo_build_hull(const float* pts, uint32_t n, float tol, uint32_t max_vertices, uint32_t* sizes, float* vertices,
+                   float* planes, uint32_t* face_starts, uint32_t* face_vertices, uint32_t* edges, float* bounds6,
+                   float* volume_centroid) {
+    std::vector<V3<float>> cloud(n);
+    for (uint32_t i = 0; i < n; ++i) cloud[i] = V3<float>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]);
+    Hull<float> hl;
+    bool ok = build_convex_hull(cloud.data(), cloud.size(), hl, tol, max_vertices);
+    const uint32_t cap[4] = {sizes[0], sizes[1], sizes[2], sizes[3]};
+    sizes[0] = uint32_t(hl.v.size()); sizes[1] = uint32_t(hl.faces()); sizes[2] = uint32_t(hl.fv.size());
+    sizes[3] = uint32_t(hl.e.size());
+    if (!ok) return 0;
+    if (sizes[0] > cap[0] || sizes[1] > cap[1] || sizes[2] > cap[2] || sizes[3] > cap[3]) return 1;
+    for (size_t i = 0; i < hl.v.size(); ++i) { vertices[3 * i] = hl.v[i].x; vertices[3 * i + 1] = hl.v[i].y; vertices[3 * i + 2] = hl.v[i].z; }
+    for (size_t f = 0; f < hl.faces(); ++f) {
+        planes[4 * f] = hl.pn[f].x; planes[4 * f + 1] = hl.pn[f].y; planes[4 * f + 2] = hl.pn[f].z; planes[4 * f + 3] = hl.pd[f];
+    }
+    for (size_t f = 0; f <= hl.faces(); ++f) face_starts[f] = hl.fs[f];
+    for (size_t k = 0; k < hl.fv.size(); ++k) face_vertices[k] = hl.fv[k];
+    for (size_t e = 0; e < hl.e.size(); ++e) { edges[2 * e] = hl.e[e].first; edges[2 * e + 1] = hl.e[e].second; }
+    bounds6[0] = hl.lb.mn.x; bounds6[1] = hl.lb.mn.y; bounds6[2] = hl.lb.mn.z;
+    bounds6[3] = hl.lb.mx.x; bounds6[4] = hl.lb.mx.y; bounds6[5] = hl.lb.mx.z;
+    volume_centroid[0] = hl.volume;
+    volume_centroid[1] = hl.centroid.x; volume_centroid[2] = hl.centroid.y; volume_centroid[3] = hl.centroid.z;
+    return 1;
 }
 // Host threads for bounds, pair discovery and narrowphase (1: serial, the reference's default).
 void pxo_set_threads(void* p, uint32_t n) {
@@ -510,13 +572,28 @@ uint32_t pxo_residual_detail(void* p, double* out, uint32_t cap) {
 }
 
 // ---- single-function entry points for known-answer tests (FP32) ----
+static std::vector<Hull<float>> g_kat_hulls;  // the hulls the single-pair entry points' shapes index
+void pxo_set_kat_hulls(const pxg_hull* in, uint32_t n) {
+    g_kat_hulls.clear();
+    for (uint32_t i = 0; i < n; ++i) g_kat_hulls.push_back(hull_from(in[i]));
+}
 int pxo_generate_contacts(const pxg_shape* sa, const float* pa, const float* ra, const pxg_shape* sb, const float* pb,
                           const float* rb, int spec, pxg_manifold* out) {
     Manifold<float> m;
-    bool hit = generate_contacts(0u, shape_from<float>(*sa), v3<float>(pa), q4<float>(ra), 1u, shape_from<float>(*sb),
-                                 v3<float>(pb), q4<float>(rb), m, spec != 0);
+    Shape<float> A = shape_from<float>(*sa), B = shape_from<float>(*sb);
+    resolve_hull(A, g_kat_hulls, sa->capsule_axis);
+    resolve_hull(B, g_kat_hulls, sb->capsule_axis);
+    bool hit = generate_contacts(0u, A, v3<float>(pa), q4<float>(ra), 1u, B, v3<float>(pb), q4<float>(rb), m, spec != 0);
     manifold_to(m, *out);
     return hit ? 1 : 0;
+}
+// the world bounds and CCD extent of a shape (hulls from the single-pair table)
+void pxo_shape_bounds(const pxg_shape* s, const float* pos, const float* rot, float* out6, float* min_extent) {
+    Shape<float> S = shape_from<float>(*s);
+    resolve_hull(S, g_kat_hulls, s->capsule_axis);
+    AABB<float> b = world_bounds(S, v3<float>(pos), q4<float>(rot));
+    out6[0] = b.mn.x; out6[1] = b.mn.y; out6[2] = b.mn.z; out6[3] = b.mx.x; out6[4] = b.mx.y; out6[5] = b.mx.z;
+    *min_extent = World<float>::min_extent(S);
 }
 int pxo_box_box(const float* ca, const float* hea, const float* qa, const float* cb, const float* heb, const float* qb, int spec,
                 pxg_manifold* out) {

@@ -63,6 +63,7 @@ template <class T> struct World {
     Config<T> cfg;
     std::vector<Body<T>> bodies;
     std::vector<Shape<T>> shapes;
+    std::vector<Hull<T>> hulls;  // the hulls PXG_CONVEX_HULL shapes point at
     std::vector<int32_t> proxy;      // per shape: tree node, -1 = none
     std::vector<char> in_static;     // per shape
     Tree<T> static_tree, dyn_tree;
@@ -155,6 +156,7 @@ template <class T> struct World {
     }
 
     static T min_extent(const Shape<T>& s) {  // world.h:1411-1424
+        if (s.type == PXG_CONVEX_HULL) return hull_min_extent(s);
         if (s.type == PXG_SPHERE) return s.radius;
         if (s.type == PXG_BOX) return std::min(std::min(s.he.x, s.he.y), s.he.z);
         return std::min(s.cr, s.chh);

@@ -12,7 +12,7 @@ PXG_OK = 0
 PXG_ERR_INVALID, PXG_ERR_CAPACITY, PXG_ERR_HIP, PXG_ERR_STATE, PXG_ERR_DEVICE = 1, 2, 3, 4, 5
 PXG_STATIC, PXG_KINEMATIC, PXG_DYNAMIC = 0, 1, 2
 PXG_INTERPOLATE, PXG_INTERPOLATE_NONE = 0, 1
-PXG_SPHERE, PXG_BOX, PXG_CAPSULE = 0, 1, 2
+PXG_SPHERE, PXG_BOX, PXG_CAPSULE, PXG_CONVEX_HULL = 0, 1, 2, 3
 PXG_AVERAGE, PXG_MINIMUM, PXG_MAXIMUM, PXG_MULTIPLY = 0, 1, 2, 3
 PXG_PHASE_FORCES_COLLIDE, PXG_PHASE_SOLVE_INTEGRATE = 0, 1
 PXG_ORDER_REFERENCE, PXG_ORDER_COLOURED = 0, 1
@@ -263,3 +263,59 @@ def ptr(a):
     """ctypes void pointer to a contiguous numpy array."""
     assert a.flags["C_CONTIGUOUS"]
     return ctypes.c_void_p(a.ctypes.data)
+
+
+class HullDesc(ctypes.Structure):
+    """pxg_hull (include/pxg.h): a built convex hull's arrays (pointers) and sizes."""
+    _fields_ = [("vertices", ctypes.c_void_p), ("n_vertices", ctypes.c_uint32), ("n_faces", ctypes.c_uint32),
+                ("face_planes", ctypes.c_void_p), ("face_starts", ctypes.c_void_p), ("face_vertices", ctypes.c_void_p),
+                ("edges", ctypes.c_void_p), ("n_edges", ctypes.c_uint32), ("local_min", ctypes.c_float * 3),
+                ("local_max", ctypes.c_float * 3)]
+
+
+class Hull:
+    """A built convex hull (geometry::ConvexHull's arrays): vertices (n, 3), face planes (f, 4:
+    outward unit normal and offset), face starts (f + 1), face vertices, edges (e, 2), local bounds
+    (6) and, when known, volume and centroid. desc() gives the pxg_hull record over these arrays
+    (keep this object alive while the record is used)."""
+
+    def __init__(self, vertices, planes, face_starts, face_vertices, edges, bounds6, volume=None, centroid=None):
+        self.vertices = np.ascontiguousarray(np.asarray(vertices, np.float32).reshape(-1, 3))
+        self.planes = np.ascontiguousarray(np.asarray(planes, np.float32).reshape(-1, 4))
+        self.face_starts = np.ascontiguousarray(np.asarray(face_starts, np.uint32))
+        self.face_vertices = np.ascontiguousarray(np.asarray(face_vertices, np.uint32))
+        self.edges = np.ascontiguousarray(np.asarray(edges, np.uint32).reshape(-1, 2))
+        self.bounds6 = np.asarray(bounds6, np.float32)
+        self.volume = volume
+        self.centroid = centroid
+
+    @property
+    def n_faces(self):
+        return len(self.planes)
+
+    def face(self, f):
+        return self.face_vertices[self.face_starts[f]:self.face_starts[f + 1]]
+
+    def desc(self):
+        d = HullDesc()
+        d.vertices = self.vertices.ctypes.data
+        d.n_vertices = len(self.vertices)
+        d.n_faces = len(self.planes)
+        d.face_planes = self.planes.ctypes.data
+        d.face_starts = self.face_starts.ctypes.data
+        d.face_vertices = self.face_vertices.ctypes.data if len(self.face_vertices) else None
+        d.edges = self.edges.ctypes.data if len(self.edges) else None
+        d.n_edges = len(self.edges)
+        for i in range(3):
+            d.local_min[i] = float(self.bounds6[i])
+            d.local_max[i] = float(self.bounds6[3 + i])
+        return d
+
+
+def hull_table(hulls):
+    """A ctypes array of pxg_hull records for a list of Hull objects."""
+    arr = (HullDesc * max(1, len(hulls)))()
+    for i, h in enumerate(hulls):
+        arr[i] = h.desc()
+    return arr
+

@@ -16,6 +16,7 @@ _SIGNATURES = {
     "pxg_set_gravity": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_set_bodies": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_bodies": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_set_hulls": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_bodies_subset": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_poses": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_shapes": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
@@ -83,6 +84,7 @@ _SIGNATURES = {
                           ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_batch_destroy": ([ctypes.c_void_p], ctypes.c_int),
     "pxg_batch_set_tuning": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32], ctypes.c_int),
+    "pxg_batch_set_occupancy": ([ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_batch_set_layer_masks": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_batch_step": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_uint32], ctypes.c_int),
     "pxg_batch_synchronize": ([ctypes.c_void_p], ctypes.c_int),

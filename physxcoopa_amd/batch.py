@@ -81,6 +81,11 @@ class Batch:
         check_batch(lib().pxg_batch_set_tuning(self.h, block_size, 1 if longest_first else 0), "pxg_batch_set_tuning")
         self.tuning = (block_size, bool(longest_first))
 
+    def set_occupancy(self, workgroups_per_cu):
+        """Workgroups launched per compute unit (0: as many as fit)."""
+        check_batch(lib().pxg_batch_set_occupancy(self.h, int(workgroups_per_cu)), "pxg_batch_set_occupancy")
+        self.occupancy = int(workgroups_per_cu)
+
     def step(self, steps=1, h=None):
         check_batch(lib().pxg_batch_step(self.h, np.float32(self._cfg.fixed_dt if h is None else h), steps),
                     "pxg_batch_step")

@@ -1153,9 +1153,10 @@ struct pxg_batch {
     bool longest_first = true;
     int blocks_per_cu[4] = {0, 0, 0, 0};  // TB = 64, 128, 256, 512
     int cus = 0;
+    uint32_t wg_per_cu = 0;                // workgroups launched per CU (0: as many as fit)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    uint32_t graph_steps = 0, graph_block = 0, graph_lpt = 2;
+    uint32_t graph_steps = 0, graph_block = 0, graph_lpt = 2, graph_wpc = 0;
     float graph_h = 0.0f;
     pxg_config graph_cfg{};
     // joints and ignored pairs (scene-local body indices), body types for the joint layering
@@ -1546,6 +1547,12 @@ extern "C" int pxg_batch_set_tuning(pxg_batch* b, uint32_t block_size, uint32_t 
     return PXG_OK;
 }
 
+extern "C" int pxg_batch_set_occupancy(pxg_batch* b, uint32_t workgroups_per_cu) {
+    if (!b) return bfail(PXG_ERR_INVALID, "pxg_batch_set_occupancy: null handle");
+    b->wg_per_cu = workgroups_per_cu;
+    return PXG_OK;
+}
+
 extern "C" int pxg_batch_set_layer_masks(pxg_batch* b, const uint32_t masks[32]) {
     if (!b || !masks) return bfail(PXG_ERR_INVALID, "pxg_batch_set_layer_masks: null argument");
     BCHK(hipMemcpyAsync(b->layer, masks, 32 * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
@@ -1557,7 +1564,9 @@ template <int TB>
 static hipError_t batch_launch(pxg_batch* b, const pxb::Cfg& c, int bpc) {
     pxb::BatchDev d = b->d;
     if (!b->longest_first) d.queue_order = b->d.queue_order + b->n_scenes;
-    uint32_t grid = std::max(1u, std::min<uint32_t>(b->n_scenes, uint32_t(std::max(1, bpc) * std::max(1, b->cus))));
+    const uint32_t fit = uint32_t(std::max(1, bpc));
+    const uint32_t per_cu = b->wg_per_cu ? std::min(b->wg_per_cu, fit) : fit;
+    uint32_t grid = std::max(1u, std::min<uint32_t>(b->n_scenes, per_cu * uint32_t(std::max(1, b->cus))));
     hipLaunchKernelGGL(pxb::k_batch<TB>, dim3(grid), dim3(TB), 0, b->stream, d, c);
     return hipGetLastError();
 }
@@ -1580,7 +1589,7 @@ extern "C" int pxg_batch_step(pxg_batch* b, float h, uint32_t steps) {
     }
     const pxg_config& g = b->cfg;
     bool rebuild = !b->exec || b->graph_steps != steps || b->graph_block != uint32_t(b->block) ||
-                   b->graph_lpt != uint32_t(b->longest_first) || b->graph_h != h ||
+                   b->graph_lpt != uint32_t(b->longest_first) || b->graph_wpc != b->wg_per_cu || b->graph_h != h ||
                    std::memcmp(&b->graph_cfg, &g, sizeof(g)) != 0;
     if (rebuild) {
         if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -1617,6 +1626,7 @@ extern "C" int pxg_batch_step(pxg_batch* b, float h, uint32_t steps) {
         b->graph_steps = steps;
         b->graph_block = uint32_t(b->block);
         b->graph_lpt = uint32_t(b->longest_first);
+        b->graph_wpc = b->wg_per_cu;
         b->graph_h = h;
         b->graph_cfg = g;
     }

@@ -131,14 +131,15 @@ PXD m3 world_inertia(quat q, v3 d) {
 // ---------------------------------------------------------------------------------------------
 // shapes and world instancing (collision/shape.h:160-228, geometry/obb.h)
 // ---------------------------------------------------------------------------------------------
-enum { SH_SPHERE = 0, SH_BOX = 1, SH_CAPSULE = 2 };
+enum { SH_SPHERE = 0, SH_BOX = 1, SH_CAPSULE = 2 };  // convex hulls (3): pxg_hull.h
 
 struct ShapeGeom {  // one shape's local description, decoded from the packed device record
     int type, caxis;
     v3 lc;
     quat lr;
-    float radius, cr, chh;
+    float radius, cr, chh;  // a convex hull's uniform scale is in radius
     v3 he;
+    uint32_t hull;          // a convex hull's index in the hull table
 };
 struct OBB { v3 c, he; quat q; };
 struct Sph { v3 c; float r; };

@@ -77,19 +77,26 @@ struct ShapeSoA {
 // static_owner(i) tells whether shape i's owner is a static body. Returns an error string or null.
 template <class StaticFn>
 inline const char* pack_shapes(const pxg_shape* in, size_t n, uint32_t n_owner_bodies, StaticFn static_owner,
-                               ShapeSoA& o) {
+                               ShapeSoA& o, uint32_t n_hulls = 0) {
     o.info.resize(n); o.lc.resize(n); o.lr.resize(n); o.dim.resize(n); o.mat.resize(n);
     for (size_t i = 0; i < n; ++i) {
         const pxg_shape& s = in[i];
         if (s.owner >= n_owner_bodies) return "shape owner out of range";
-        if (s.type > 2) return "only sphere, box and capsule shapes are supported";
-        if (s.capsule_axis < 0 || s.capsule_axis > 2) return "bad capsule axis";
+        const bool hull = s.type == PXG_CONVEX_HULL;
+        if (s.type > PXG_CONVEX_HULL) return "only sphere, box, capsule and convex hull shapes are supported";
+        if (hull && (s.capsule_axis < 0 || uint32_t(s.capsule_axis) >= n_hulls))
+            return "a convex hull shape's index (capsule_axis) is not in the uploaded hull table";
+        if (!hull && (s.capsule_axis < 0 || s.capsule_axis > 2)) return "bad capsule axis";
         uint32_t flags = s.type | (s.enabled ? SF_ENABLED : 0u) | (s.is_trigger ? SF_TRIGGER : 0u) |
-                         (static_owner(i) ? SF_STATIC : 0u) | (uint32_t(s.capsule_axis) << 8);
+                         (static_owner(i) ? SF_STATIC : 0u) | (hull ? 0u : uint32_t(s.capsule_axis) << 8);
         o.info[i] = make_uint4(s.owner, flags, s.layer & 31u, (s.friction_combine & 255u) | ((s.restitution_combine & 255u) << 8));
         o.lc[i] = make_float4(s.local_center[0], s.local_center[1], s.local_center[2], s.radius);
         o.lr[i] = make_float4(s.local_rotation[0], s.local_rotation[1], s.local_rotation[2], s.local_rotation[3]);
         o.dim[i] = make_float4(s.half_extents[0], s.half_extents[1], s.half_extents[2], s.capsule_radius);
+        if (hull) {  // the hull index goes where a box keeps its first half extent
+            uint32_t k = uint32_t(s.capsule_axis);
+            std::memcpy(&o.dim[i].x, &k, sizeof(k));
+        }
         o.mat[i] = make_float4(s.capsule_half_height, s.dynamic_friction, s.restitution, s.static_friction);
     }
     return nullptr;

@@ -5,6 +5,7 @@
 // work item per thread, grid-stride where the item count lives on the device.
 #include "pxg_state.h"
 #include "pxg_device.h"
+#include "pxg_hull.h"
 #include "pxg_joints.h"
 #include "pxg_mlp.h"
 
@@ -48,7 +49,22 @@ __device__ __forceinline__ ShapeGeom load_shape(const DevState& d, uint32_t i, u
     s.he = mk(dim.x, dim.y, dim.z);
     s.cr = dim.w;
     s.chh = d.s_mat[i].x;
+    s.hull = __float_as_uint(dim.x);
     return s;
+}
+__device__ __forceinline__ HullTab hull_tab(const DevState& d) {
+    HullTab t;
+    t.meta = d.h_meta; t.meta2 = d.h_meta2; t.lb = d.h_lb; t.v = d.h_v; t.pl = d.h_pl;
+    t.face = d.h_face; t.fv = d.h_fv; t.e = d.h_e;
+    return t;
+}
+// tight world AABB and CCD extent of any shape type
+__device__ __forceinline__ void shape_bounds(const DevState& d, const ShapeGeom& s, v3 pos, quat rot, v3& mn, v3& mx) {
+    if (s.type == SH_HULL) hull_bounds(hull_tab(d), s, pos, rot, mn, mx);
+    else world_bounds(s, pos, rot, mn, mx);
+}
+__device__ __forceinline__ float shape_min_extent(const DevState& d, const ShapeGeom& s) {
+    return s.type == SH_HULL ? hull_min_extent(hull_tab(d), s) : min_extent(s);
 }
 __device__ __forceinline__ bool shape_live(const DevState& d, uint4 info) {
     return (info.y & SF_ENABLED) && (d.b_flags[info.x] & BF_ALIVE);
@@ -252,7 +268,7 @@ __global__ void k_init_proxies(DevState d, StepParams p) {
     }
     ShapeGeom s = load_shape(d, i, info);
     v3 mn, mx;
-    world_bounds(s, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]), mn, mx);
+    shape_bounds(d, s, ld3(d.b_pos[info.x]), ldq(d.b_rot[info.x]), mn, mx);
     d.s_fmin[i] = st3(mn - splat(p.margin), 0.0f);
     d.s_fmax[i] = st3(mx + splat(p.margin), 0.0f);
 }
@@ -271,7 +287,7 @@ __global__ void k_bounds(DevState d, StepParams p) {
             ShapeGeom s = load_shape(d, i, info);
             v3 mn, mx;
             uint32_t ob = info.x;
-            world_bounds(s, ld3(d.b_pos[ob]), ldq(d.b_rot[ob]), mn, mx);
+            shape_bounds(d, s, ld3(d.b_pos[ob]), ldq(d.b_rot[ob]), mn, mx);
             v3 fmn = ld3(d.s_fmin[i]), fmx = ld3(d.s_fmax[i]);
             if (!(info.y & SF_STATIC)) {
                 v3 disp = ld3(d.b_vel[ob]) * p.h;
@@ -534,9 +550,13 @@ __global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
         (d.layer_mask[ii.z & 31u] & (1u << (ij.z & 31u))) && !contacts_off(d, bi, bj)) {
         ShapeGeom si = load_shape(d, i, ii), sj = load_shape(d, j, ij);
         float4 vi = d.b_vel[bi], vj = d.b_vel[bj];
-        bool spec = (len(ld3(vi)) * prm.h > min_extent(si)) || (len(ld3(vj)) * prm.h > min_extent(sj));
-        if (generate_contacts(si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), spec, m))
+        bool spec = (len(ld3(vi)) * prm.h > shape_min_extent(d, si)) || (len(ld3(vj)) * prm.h > shape_min_extent(d, sj));
+        if (si.type == SH_HULL || sj.type == SH_HULL) {
+            if (hull_contacts(hull_tab(d), si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), m))
+                valid = 1u;
+        } else if (generate_contacts(si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), spec, m)) {
             valid = 1u;
+        }
     }
     d.pm_valid[k] = valid;
     if (!valid) return;

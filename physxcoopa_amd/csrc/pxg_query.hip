@@ -219,7 +219,8 @@ __global__ void k_query_prep(DevState d, float4* geom, uint4* qinfo, uint32_t* b
     s.cr = dim.w;
     s.chh = d.s_mat[i].x;
     const uint32_t owner = info.x;
-    const bool live = (info.y & SF_ENABLED) && owner < d.nb && (d.b_flags[owner] & BF_ALIVE);
+    // scene queries test sphere, box and capsule shapes; convex hulls are not query targets here
+    const bool live = (info.y & SF_ENABLED) && owner < d.nb && (d.b_flags[owner] & BF_ALIVE) && s.type <= SH_CAPSULE;
     float4 g0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g1 = g0, g2 = g0;
     if (live) {
         const v3 pos = ld3(d.b_pos[owner]);

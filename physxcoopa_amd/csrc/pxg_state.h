@@ -85,6 +85,12 @@ struct DevState {
     float4 *s_lr;       // local rotation
     float4 *s_dim;      // box half extents, capsule radius
     float4 *s_mat;      // capsule half height, dynamic friction, restitution, static friction
+    // convex hulls (pxg_set_hulls): the table PXG_CONVEX_HULL shapes index (pxg_hull.h's HullTab)
+    uint4 *h_meta, *h_meta2;
+    float4 *h_lb, *h_v, *h_pl;
+    uint2 *h_face, *h_e;
+    uint32_t *h_fv;
+    uint32_t n_hulls;
     float4 *s_fmin, *s_fmax;   // fat AABB (persistent broadphase state)
     float4 *s_tmin, *s_tmax;   // tight AABB of this step; w of s_tmin: 1 = large shape
     uint32_t *layer_mask;      // 32 rows

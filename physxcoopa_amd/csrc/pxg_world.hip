@@ -525,6 +525,69 @@ extern "C" int pxg_get_poses(pxg_world* w, pxg_pose* out, uint32_t n) {
     return PXG_OK;
 }
 
+extern "C" int pxg_set_hulls(pxg_world* w, const pxg_hull* hulls, uint32_t n) {
+    if (!w || (n && !hulls)) return fail(PXG_ERR_INVALID, "pxg_set_hulls: null argument");
+    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_hulls: not allowed between the two step phases");
+    std::vector<uint4> meta(n), meta2(n);
+    std::vector<float4> lb(2 * size_t(n)), v, pl;
+    std::vector<uint2> face, edge;
+    std::vector<uint32_t> fv;
+    for (uint32_t k = 0; k < n; ++k) {
+        const pxg_hull& h = hulls[k];
+        const std::string at = "pxg_set_hulls: hull " + std::to_string(k) + ": ";
+        if (!h.vertices || !h.face_planes || !h.face_starts || !h.face_vertices || (h.n_edges && !h.edges))
+            return fail(PXG_ERR_INVALID, at + "null array");
+        if (h.n_vertices < 4 || h.n_vertices > 64) return fail(PXG_ERR_INVALID, at + "4 to 64 vertices");
+        if (h.n_faces < 4) return fail(PXG_ERR_INVALID, at + "at least 4 faces");
+        meta[k] = make_uint4(uint32_t(v.size()), h.n_vertices, uint32_t(pl.size()), h.n_faces);
+        meta2[k] = make_uint4(uint32_t(edge.size()), h.n_edges, 0u, 0u);
+        lb[2 * k] = make_float4(h.local_min[0], h.local_min[1], h.local_min[2], 0.0f);
+        lb[2 * k + 1] = make_float4(h.local_max[0], h.local_max[1], h.local_max[2], 0.0f);
+        for (uint32_t i = 0; i < h.n_vertices; ++i)
+            v.push_back(make_float4(h.vertices[3 * i], h.vertices[3 * i + 1], h.vertices[3 * i + 2], 0.0f));
+        for (uint32_t f = 0; f < h.n_faces; ++f) {
+            const uint32_t s = h.face_starts[f], e = h.face_starts[f + 1];
+            if (e < s + 3 || e - s > 8) return fail(PXG_ERR_INVALID, at + "a face has fewer than 3 or more than 8 vertices");
+            pl.push_back(make_float4(h.face_planes[4 * f], h.face_planes[4 * f + 1], h.face_planes[4 * f + 2], h.face_planes[4 * f + 3]));
+            face.push_back(make_uint2(uint32_t(fv.size()), e - s));
+            for (uint32_t i = s; i < e; ++i) {
+                if (h.face_vertices[i] >= h.n_vertices) return fail(PXG_ERR_INVALID, at + "face vertex index out of range");
+                fv.push_back(h.face_vertices[i]);
+            }
+        }
+        for (uint32_t i = 0; i < h.n_edges; ++i) {
+            if (h.edges[2 * i] >= h.n_vertices || h.edges[2 * i + 1] >= h.n_vertices)
+                return fail(PXG_ERR_INVALID, at + "edge vertex index out of range");
+            edge.push_back(make_uint2(h.edges[2 * i], h.edges[2 * i + 1]));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(w->stream));
+    DevState& d = w->d;
+    void* old[8] = {d.h_meta, d.h_meta2, d.h_lb, d.h_v, d.h_pl, d.h_face, d.h_e, d.h_fv};
+    for (void* p : old) dfree(w, p);
+    d.h_meta = d.h_meta2 = nullptr;
+    d.h_lb = d.h_v = d.h_pl = nullptr;
+    d.h_face = d.h_e = nullptr;
+    d.h_fv = nullptr;
+    d.n_hulls = 0;
+    if (!n) return PXG_OK;
+    DALLOC(d.h_meta, n); DALLOC(d.h_meta2, n); DALLOC(d.h_lb, 2 * size_t(n));
+    DALLOC(d.h_v, v.size()); DALLOC(d.h_pl, pl.size()); DALLOC(d.h_face, face.size()); DALLOC(d.h_fv, fv.size());
+    DALLOC(d.h_e, std::max<size_t>(edge.size(), 1));
+    hipStream_t st = w->stream;
+    HIPCHK(hipMemcpyAsync(d.h_meta, meta.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.h_meta2, meta2.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.h_lb, lb.data(), lb.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.h_v, v.data(), v.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.h_pl, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.h_face, face.data(), face.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.h_fv, fv.data(), fv.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (!edge.empty()) HIPCHK(hipMemcpyAsync(d.h_e, edge.data(), edge.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    d.n_hulls = n;
+    return PXG_OK;
+}
+
 extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n) {
     if (!w || (!shapes && n)) return fail(PXG_ERR_INVALID, "pxg_set_shapes: null argument");
     if (!w->cap_b) return fail(PXG_ERR_STATE, "pxg_set_shapes: upload bodies first");
@@ -532,7 +595,8 @@ extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n)
     int r = alloc_shapes(w, n);
     if (r != PXG_OK) return r;
     ShapeSoA soa;
-    const char* err = pack_shapes(shapes, n, w->n_bodies, [&](size_t i) { return w->body_types[shapes[i].owner] == PXG_STATIC; }, soa);
+    const char* err = pack_shapes(shapes, n, w->n_bodies, [&](size_t i) { return w->body_types[shapes[i].owner] == PXG_STATIC; }, soa,
+                                  w->d.n_hulls);
     if (err) return fail(PXG_ERR_INVALID, std::string("pxg_set_shapes: ") + err);
     DevState& d = w->d;
     hipStream_t st = w->stream;

@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from ._abi import (BODY_DTYPE, SHAPE_DTYPE, PXG_STATIC, PXG_DYNAMIC, PXG_SPHERE, PXG_BOX, PXG_CAPSULE,
+from ._abi import (BODY_DTYPE, SHAPE_DTYPE, PXG_STATIC, PXG_DYNAMIC, PXG_SPHERE, PXG_BOX, PXG_CAPSULE, PXG_CONVEX_HULL,
                    PXG_AVERAGE)
 
 F = np.float32
@@ -328,6 +328,31 @@ def drop_scene(n_bodies, rng):
         quat = random_quaternion(rng)
         pos = ((ix - 3.5) * 1.25, (iy - 3.5) * 1.25, 1.2 + iz * 1.25)
         _add_pile_body(sb, kind, dims, pos, quat, 0.5, 0.0)
+    return sb.arrays()
+
+
+def hull_pile_scene(hulls, n=40, seed=4, spread=1.5, others=True):
+    """Convex-hull bodies (cycling through `hulls`, a list of _abi.Hull; 2 kg, the inertia of their
+    bounding box) dropped with random orientations over a 20x20x1 m ground box, with spheres,
+    boxes and capsules among them when `others`."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    sb.static_box((0, 0, -0.5), (10, 10, 0.5), friction=0.6)
+    for i in range(n):
+        pos = (rng.uniform(-spread, spread), rng.uniform(-spread, spread), 0.7 + 0.9 * i)
+        q = random_quaternion(rng)
+        kind = i % 5 if others else 0
+        if kind in (0, 1, 2):
+            k = i % len(hulls)
+            h = hulls[k]
+            ext = (np.asarray(h.bounds6[3:], np.float64) - np.asarray(h.bounds6[:3], np.float64)) * 0.5
+            b = sb.add_body(PXG_DYNAMIC, position=pos, orientation=q, mass=2.0,
+                            inv_inertia=box_inverse_inertia(ext, 2.0))
+            sb.add_shape(b, PXG_CONVEX_HULL, radius=1.0, capsule_axis=k, friction=0.6)
+        elif kind == 3:
+            sb.dynamic_sphere(pos, rng.uniform(0.2, 0.4), mass=1.5, friction=0.5)
+        else:
+            sb.dynamic_capsule(pos, 0.15, 0.3, axis=int(rng.integers(0, 3)), orientation=q, mass=1.5, friction=0.5)
     return sb.arrays()
 
 

@@ -19,7 +19,7 @@ from ._lib import lib, check
 class World:
     """One GPU world handle built from ABI arrays (bodies, shapes)."""
 
-    def __init__(self, config, bodies, shapes, layer_masks=None):
+    def __init__(self, config, bodies, shapes, layer_masks=None, hulls=None):
         self._cfg = config
         h = ctypes.c_void_p()
         check(lib().pxg_create(ctypes.byref(config), ctypes.byref(h)), "pxg_create")
@@ -32,6 +32,8 @@ class World:
         if layer_masks is not None:
             m = np.ascontiguousarray(np.asarray(layer_masks, np.uint32))
             check(lib().pxg_set_layer_masks(self.h, ptr(m)), "pxg_set_layer_masks")
+        if hulls:
+            self.set_hulls(hulls)
         check(lib().pxg_set_shapes(self.h, ptr(s), self.n_shapes), "pxg_set_shapes")
 
     def close(self):
@@ -61,6 +63,12 @@ class World:
     def set_layer_masks(self, masks):
         m = np.ascontiguousarray(np.asarray(masks, np.uint32))
         check(lib().pxg_set_layer_masks(self.h, ptr(m)), "pxg_set_layer_masks")
+
+    def set_hulls(self, hulls):
+        """Uploads the convex hulls (a list of _abi.Hull) that PXG_CONVEX_HULL shapes index."""
+        tab = _abi.hull_table(hulls)
+        self._hulls = list(hulls)  # the arrays the records point at stay alive through the call
+        check(lib().pxg_set_hulls(self.h, ctypes.byref(tab), len(hulls)), "pxg_set_hulls")
 
     def set_joints(self, joints):
         """Replaces the joint table (pxg_joint records, see physxcoopa_amd.joints)."""

@@ -256,3 +256,32 @@ def test_batch_config3_full_launch_properties():
               flush=True)
         bt.close()
     assert out[0] == out[1]
+
+
+def test_autotuner_search_and_result_invariance(small_batch, tmp_path, monkeypatch):
+    """The autotuner times the batch over its search space (threads per scene, workgroups per CU,
+    queue order, steps per launch), caches its choice, and the chosen tuning (and one workgroup per
+    CU) steps the scenes to exactly the default tuning's state."""
+    from physxcoopa_amd import autotune
+    from physxcoopa_amd.batch import Batch
+    b, s, bo, so, settings = small_batch
+    cfg = _abi.default_config(**settings)
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tuning.json"))
+    choice, timings = autotune.tune(cfg, b, s, bo, so, steps=10, rounds=1)
+    print("autotuner:", autotune.label(choice), len(timings), "candidates timed", flush=True)
+    assert set(choice) == set(autotune.DEFAULT)
+    for k, v in choice.items():
+        assert v in autotune.SPACE[k]
+    assert len(timings) >= sum(len(v) for v in autotune.SPACE.values()) - len(autotune.SPACE) + 1
+    again, _ = autotune.tune(cfg, b, s, bo, so, steps=10, rounds=1)
+    assert again == choice
+    ref = Batch(cfg, b, s, bo, so)
+    ref.step(30)
+    for tuning in (choice, dict(autotune.DEFAULT, workgroups_per_cu=1, steps_per_launch=10)):
+        t = Batch(cfg, b, s, bo, so)
+        autotune.apply(t, tuning)
+        for _ in range(30 // tuning["steps_per_launch"]):
+            t.step(tuning["steps_per_launch"])
+        assert ref.bodies().tobytes() == t.bodies().tobytes(), tuning
+        t.close()
+    ref.close()
