@@ -41,7 +41,7 @@ enum pxg_body_type { PXG_STATIC = 0, PXG_KINEMATIC = 1, PXG_DYNAMIC = 2 };
  * (dynamics::Interpolation, dynamics/body.h:26-33; Body::interpolation defaults to Interpolate,
  * body.h:118). Carried with the body; the step does not read it. */
 enum pxg_interpolation { PXG_INTERPOLATE = 0, PXG_INTERPOLATE_NONE = 1 };
-enum pxg_shape_type { PXG_SPHERE = 0, PXG_BOX = 1, PXG_CAPSULE = 2, PXG_CONVEX_HULL = 3 };
+enum pxg_shape_type { PXG_SPHERE = 0, PXG_BOX = 1, PXG_CAPSULE = 2, PXG_CONVEX_HULL = 3, PXG_TRIANGLE_MESH = 4 };
 enum pxg_combine { PXG_AVERAGE = 0, PXG_MINIMUM = 1, PXG_MAXIMUM = 2, PXG_MULTIPLY = 3 };
 enum pxg_phase { PXG_PHASE_FORCES_COLLIDE = 0, PXG_PHASE_SOLVE_INTEGRATE = 1 };
 
@@ -142,11 +142,13 @@ typedef struct pxg_shape {
     uint32_t enabled;        /* 0 = no broadphase proxy (body without collider) */
     uint32_t is_trigger;
     uint32_t layer;          /* 0..31 */
-    int32_t capsule_axis;    /* 0 = X, 1 = Y, 2 = Z; PXG_CONVEX_HULL: the hull's index in pxg_set_hulls' table */
+    int32_t capsule_axis;    /* 0 = X, 1 = Y, 2 = Z; PXG_CONVEX_HULL: the hull's index in pxg_set_hulls' table;
+                                PXG_TRIANGLE_MESH: the mesh's index in pxg_set_meshes' table */
     uint32_t friction_combine;
     uint32_t restitution_combine;
     float local_center[3];
-    float radius;            /* sphere; PXG_CONVEX_HULL: the uniform scale (Shape::hull_scale, shape.h:98) */
+    float radius;            /* sphere; PXG_CONVEX_HULL: the uniform scale (Shape::hull_scale, shape.h:98);
+                                PXG_TRIANGLE_MESH: the uniform scale (Shape::mesh_scale, shape.h:91) */
     float local_rotation[4]; /* x, y, z, w */
     float half_extents[3];   /* box */
     float capsule_radius;
@@ -331,7 +333,30 @@ typedef struct pxg_hull {
     uint32_t n_edges;
     float local_min[3];
     float local_max[3];
+    float centroid[3];             /* the solid's centroid (what a hull resting on a mesh is measured from) */
+    float volume;
 } pxg_hull;
+
+/* A triangle soup, as TriangleMesh::from_triangles (geometry/triangle_mesh.h:75-170) takes it:
+ * positions and index triples. Cooking welds positions closer than 1e-4 of the bounds' diagonal,
+ * drops the triangles the weld collapses or leaves without area, and computes unit face normals,
+ * edge adjacency and the triangle tree (geometry/mesh_bvh.h). Caller-owned arrays. */
+typedef struct pxg_mesh_desc {
+    const float* positions;        /* 3 per position */
+    uint32_t n_positions;
+    uint32_t n_indices;            /* 3 per triangle; a trailing partial triangle is ignored */
+    const uint32_t* indices;       /* each < n_positions */
+} pxg_mesh_desc;
+
+/* Sizes and bounds of a cooked mesh. */
+typedef struct pxg_mesh_info {
+    uint32_t n_vertices;           /* after welding (at most n_positions) */
+    uint32_t n_triangles;          /* after dropping degenerate triangles (at most n_indices / 3) */
+    uint32_t n_nodes;              /* triangle tree nodes */
+    uint32_t non_manifold_edges;   /* edges shared by more than two triangles (left unlinked) */
+    float local_min[3];            /* bounds of every welded vertex */
+    float local_max[3];
+} pxg_mesh_info;
 
 /* --- state upload / download (caller-owned host buffers are copied) --- */
 /* Replaces the body table (add_body/remove_body, world.h:95-143). World inverse inertia is
@@ -366,6 +391,21 @@ int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n);
  * speculative contacts). A hull with more than 64 vertices or a face of more than 8 vertices is
  * refused (PXG_ERR_INVALID). */
 int pxg_set_hulls(pxg_world* w, const pxg_hull* hulls, uint32_t n);
+/* Cooks one mesh on the host (no device use) as pxg_set_meshes does: info receives the sizes;
+ * each non-null array receives the cooked data -- vertices (3 floats per vertex; 3 * n_positions
+ * floats always suffice), indices (3 per triangle; n_indices suffice), normals (3 floats per
+ * triangle; n_indices suffice) and adjacency (per triangle the neighbour across edges v0-v1,
+ * v1-v2, v2-v0, 0xFFFFFFFF for none; n_indices suffice). PXG_ERR_INVALID for a null array with
+ * n > 0 or an index out of range. TriangleMesh::from_triangles, triangle_mesh.h:75-170. */
+int pxg_cook_mesh(const pxg_mesh_desc* mesh, pxg_mesh_info* info, float* vertices, uint32_t* indices, float* normals,
+                  uint32_t* adjacency);
+/* Replaces the table of triangle meshes that PXG_TRIANGLE_MESH shapes index (the meshes a
+ * MeshCollider's Shape::mesh points at, shape.h:84-91), cooking each as pxg_cook_mesh does;
+ * upload before the shapes that use them. A mesh shape belongs to a static or kinematic body.
+ * Mesh narrowphase: collision/mesh_contact.h (sphere, capsule, box and convex hull against the
+ * mesh's triangles, internal-edge correction, embedded recovery; no mesh against mesh, no
+ * speculative contacts). A mesh that cooks to no triangle is refused (PXG_ERR_INVALID). */
+int pxg_set_meshes(pxg_world* w, const pxg_mesh_desc* meshes, uint32_t n);
 /* 32 x 32 layer matrix rows (broadphase/layer_matrix.h:21-47). */
 int pxg_set_layer_masks(pxg_world* w, const uint32_t masks[32]);
 /* Replaces the joint table: the joints_ array behind add_hinge_joint / add_ball_joint /

@@ -9,7 +9,7 @@ import subprocess
 
 import numpy as np
 
-from physxcoopa_amd._abi import BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, Hull, hull_table, ptr
+from physxcoopa_amd._abi import BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, Hull, MeshInfo, hull_table, mesh_table, ptr
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libpxo.so")
@@ -62,6 +62,11 @@ def lib():
             "pxo_ray_shape": ([vp, vp, vp, vp, vp, vp], i32), "pxo_overlap": ([vp, vp, vp, vp, u32], u32),
             "pxo_shape_cast": ([vp, vp, vp, vp], i32), "pxo_ray_aabb": ([vp, vp, vp], i32),
             "pxo_compute_penetration": ([vp, vp, vp, vp, vp, vp, i32], i32),
+            "pxo_set_meshes": ([vp, vp, u32], i32), "pxo_set_kat_meshes": ([vp, u32], i32),
+            "pxo_cook_mesh": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+            "pxo_mesh_contacts": ([vp, vp, vp, vp, vp, vp, i32, vp], i32),
+            "pxo_mesh_candidates": ([vp, vp, vp, vp, vp, vp, vp, u32], u32),
+            "pxo_mat4_inverse": ([vp, vp], None), "pxo_closest_on_triangle": ([vp, vp, vp, vp, vp, vp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -81,7 +86,7 @@ def f32(x, n=None):
 class OracleWorld:
     """The CPU reference step over pxg ABI arrays. dbl=True runs the same algorithm in float64."""
 
-    def __init__(self, config, bodies, shapes, dbl=False, brute_force=False, layer_masks=None, hulls=None):
+    def __init__(self, config, bodies, shapes, dbl=False, brute_force=False, layer_masks=None, hulls=None, meshes=None):
         L = lib()
         self._cfg = config
         self.h = L.pxo_create(ctypes.byref(config), 1 if dbl else 0)
@@ -96,6 +101,9 @@ class OracleWorld:
         if hulls:
             tab = hull_table(hulls)
             L.pxo_set_hulls(self.h, ctypes.byref(tab), len(hulls))
+        if meshes:
+            tab = mesh_table(meshes)
+            assert L.pxo_set_meshes(self.h, ctypes.byref(tab), len(meshes)) == 1, "mesh index out of range"
         L.pxo_set_shapes(self.h, ptr(s), self.n_shapes)
         if brute_force:
             L.pxo_set_brute_force(self.h, 1)
@@ -417,3 +425,85 @@ def shape_bounds(shape, pos, rot):
     lib().pxo_shape_bounds(ptr(s), ptr(keep[0]), ptr(keep[1]), ptr(out), ptr(ext))
     return out, float(ext[0])
 
+
+
+# ---- triangle meshes (geometry/triangle_mesh.h, geometry/mesh_bvh.h, collision/mesh_contact.h) ----
+
+class CookedMesh:
+    """from_triangles' output: welded vertices (n, 3), indices (t, 3), unit face normals (t, 3),
+    adjacency (t, 3; 0xFFFFFFFF for none), bounds (6), non-manifold edge count, and the triangle
+    tree: node bounds (k, 6), node links (k, 4: left, right, first, count) and triangle order."""
+
+
+def cook_mesh(mesh):
+    """TriangleMesh::from_triangles (triangle_mesh.h:75-170) and MeshBVH::build (mesh_bvh.h:31-51)
+    of a physxcoopa_amd._abi.TriangleMesh; None for an index out of range."""
+    d = mesh.desc()
+    np_, ni = len(mesh.positions), len(mesh.indices)
+    info = MeshInfo()
+    v = np.zeros((max(np_, 1), 3), np.float32)
+    idx = np.zeros(max(ni, 3), np.uint32)
+    nrm = np.zeros(max(ni, 3), np.float32)
+    adj = np.zeros(max(ni, 3), np.uint32)
+    nmax = max(1, 2 * (ni // 3) + 1)
+    nb = np.zeros((nmax, 6), np.float32)
+    nl = np.zeros((nmax, 4), np.int32)
+    order = np.zeros(max(ni // 3, 1), np.uint32)
+    if not lib().pxo_cook_mesh(ctypes.byref(d), ctypes.byref(info), ptr(v), ptr(idx), ptr(nrm), ptr(adj), ptr(nb), ptr(nl),
+                               ptr(order)):
+        return None
+    c = CookedMesh()
+    nv, nt, nn = info.n_vertices, info.n_triangles, info.n_nodes
+    c.vertices = v[:nv].copy()
+    c.indices = idx[:3 * nt].reshape(-1, 3).copy()
+    c.normals = nrm[:3 * nt].reshape(-1, 3).copy()
+    c.adjacency = adj[:3 * nt].reshape(-1, 3).copy()
+    c.bounds = np.array(list(info.local_min) + list(info.local_max), np.float32)
+    c.non_manifold_edges = info.non_manifold_edges
+    c.node_bounds = nb[:nn].copy()
+    c.node_links = nl[:nn].copy()
+    c.order = order[:nt].copy()
+    return c
+
+
+def set_kat_meshes(meshes):
+    """The mesh table the single-pair helpers (generate_contacts, shape_bounds, mesh_contacts,
+    mesh_candidates) resolve PXG_TRIANGLE_MESH shapes against (capsule_axis is the index)."""
+    tab = mesh_table(meshes)
+    assert lib().pxo_set_kat_meshes(ctypes.byref(tab), len(meshes)) == 1, "mesh index out of range"
+
+
+def mesh_contacts(shape, pos, rot, mesh_shape, mesh_pos, mesh_rot, use_bvh=True):
+    """generate_mesh_contacts (mesh_contact.h:838-884): (hit, manifold), normal from the mesh
+    toward the shape; use_bvh=False scans every triangle."""
+    out = np.zeros(1, MANIFOLD_DTYPE)
+    s, m = np.ascontiguousarray(shape).reshape(1), np.ascontiguousarray(mesh_shape).reshape(1)
+    keep = [f32(pos, 3), f32(rot, 4), f32(mesh_pos, 3), f32(mesh_rot, 4)]
+    hit = lib().pxo_mesh_contacts(ptr(s), ptr(keep[0]), ptr(keep[1]), ptr(m), ptr(keep[2]), ptr(keep[3]),
+                                  1 if use_bvh else 0, ptr(out))
+    return bool(hit), out[0]
+
+
+def mesh_candidates(shape, pos, rot, mesh_shape, mesh_pos, mesh_rot, cap=1 << 16):
+    """mesh_contact_candidates (mesh_contact.h:330-340): the ascending triangle indices."""
+    out = np.zeros(cap, np.uint32)
+    s, m = np.ascontiguousarray(shape).reshape(1), np.ascontiguousarray(mesh_shape).reshape(1)
+    keep = [f32(pos, 3), f32(rot, 4), f32(mesh_pos, 3), f32(mesh_rot, 4)]
+    n = lib().pxo_mesh_candidates(ptr(s), ptr(keep[0]), ptr(keep[1]), ptr(m), ptr(keep[2]), ptr(keep[3]), ptr(out), cap)
+    return out[:min(n, cap)].copy()
+
+
+def mat4_inverse(m):
+    """glm::inverse of a 4x4 given as (4, 4) columns (m[col][row])."""
+    a = f32(np.asarray(m, np.float32).reshape(16), 16)
+    out = np.zeros(16, np.float32)
+    lib().pxo_mat4_inverse(ptr(a), ptr(out))
+    return out.reshape(4, 4)
+
+
+def closest_on_triangle(p, a, b, c):
+    """(closest_point_on_triangle, barycentric_ of p) for triangle abc (mesh_contact.h:61-98, 247-258)."""
+    k = [f32(p, 3), f32(a, 3), f32(b, 3), f32(c, 3)]
+    out, bary = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().pxo_closest_on_triangle(ptr(k[0]), ptr(k[1]), ptr(k[2]), ptr(k[3]), ptr(out), ptr(bary))
+    return out, bary

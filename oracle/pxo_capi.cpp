@@ -57,7 +57,23 @@ template <class T> Shape<T> shape_from(const pxg_shape& d) {
     s.lc = v3<T>(d.local_center); s.radius = T(d.radius); s.lr = q4<T>(d.local_rotation); s.he = v3<T>(d.half_extents);
     s.cr = T(d.capsule_radius); s.chh = T(d.capsule_half_height); s.friction = T(d.dynamic_friction); s.restitution = T(d.restitution);
     if (s.type == PXG_CONVEX_HULL) s.hull_scale = T(d.radius);
+    if (s.type == PXG_TRIANGLE_MESH) s.mesh_scale = T(d.radius);
     return s;
+}
+// a mesh shape's pointer into the world's mesh table (nullptr for a bad index: no contacts)
+template <class T> void resolve_mesh(Shape<T>& s, const std::vector<Mesh>& meshes, int32_t index) {
+    if (s.type != PXG_TRIANGLE_MESH) return;
+    s.mesh = (index >= 0 && size_t(index) < meshes.size()) ? &meshes[size_t(index)] : nullptr;
+}
+// from_triangles of a pxg_mesh_desc (false for an index out of range)
+inline bool mesh_from(const pxg_mesh_desc& d, Mesh& out) {
+    std::vector<V3<float>> pos(d.n_positions);
+    for (uint32_t i = 0; i < d.n_positions; ++i) pos[i] = V3<float>(d.positions[3 * i], d.positions[3 * i + 1], d.positions[3 * i + 2]);
+    std::vector<uint32_t> ind(d.indices, d.indices + d.n_indices);
+    for (uint32_t x : ind)
+        if (x >= d.n_positions) return false;
+    out = mesh_from_triangles(pos, ind);
+    return true;
 }
 // a hull shape's pointer into the world's hull table (nullptr for a bad index: no contacts)
 template <class T> void resolve_hull(Shape<T>& s, const std::vector<Hull<T>>& hulls, int32_t index) {
@@ -76,6 +92,8 @@ inline Hull<float> hull_from(const pxg_hull& d) {
     for (uint32_t e = 0; e < d.n_edges; ++e) h.e.push_back({d.edges[2 * e], d.edges[2 * e + 1]});
     h.lb.mn = V3<float>(d.local_min[0], d.local_min[1], d.local_min[2]);
     h.lb.mx = V3<float>(d.local_max[0], d.local_max[1], d.local_max[2]);
+    h.centroid = V3<float>(d.centroid[0], d.centroid[1], d.centroid[2]);
+    h.volume = d.volume;
     return h;
 }
 template <class T> void manifold_to(const Manifold<T>& m, pxg_manifold& d) {
@@ -225,6 +243,7 @@ void pxo_set_shapes(void* p, const pxg_shape* in, uint32_t n) {
         for (uint32_t i = 0; i < n; ++i) {
             w.shapes[i] = shape_from<T>(in[i]);
             resolve_hull(w.shapes[i], w.hulls, in[i].capsule_axis);
+            resolve_mesh(w.shapes[i], w.meshes, in[i].capsule_axis);
         }
         w.create_proxies();
     };
@@ -275,6 +294,44 @@ void pxo_set_hulls(void* p, const pxg_hull* in, uint32_t n) {
         h->f.hulls.push_back(hf);
         h->d.hulls.push_back(hf.as<double>());
     }
+}
+// The mesh table PXG_TRIANGLE_MESH shapes index (upload before the shapes); 0 for a bad index.
+int pxo_set_meshes(void* p, const pxg_mesh_desc* in, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    std::vector<Mesh> ms(n);
+    for (uint32_t i = 0; i < n; ++i)
+        if (!mesh_from(in[i], ms[i])) return 0;
+    h->f.meshes = ms;
+    h->d.meshes = ms;
+    return 1;
+}
+// from_triangles into caller arrays (pxg_cook_mesh's layout) plus the tree: nodes as
+// (min xyz, max xyz) floats and (left, right, start, count) ints, and the triangle order.
+int pxo_cook_mesh(const pxg_mesh_desc* in, pxg_mesh_info* info, float* vertices, uint32_t* indices, float* normals,
+                  uint32_t* adjacency, float* node_bounds, int32_t* node_links, uint32_t* order) {
+    Mesh m;
+    if (!mesh_from(*in, m)) return 0;
+    info->n_vertices = uint32_t(m.v.size());
+    info->n_triangles = uint32_t(m.tris());
+    info->n_nodes = uint32_t(m.nodes.size());
+    info->non_manifold_edges = uint32_t(m.non_manifold);
+    put3(info->local_min, m.bounds.mn);
+    put3(info->local_max, m.bounds.mx);
+    for (size_t i = 0; vertices && i < m.v.size(); ++i) put3(vertices + 3 * i, m.v[i]);
+    for (size_t i = 0; indices && i < m.idx.size(); ++i) indices[i] = m.idx[i];
+    for (size_t t = 0; normals && t < m.tris(); ++t) put3(normals + 3 * t, m.n[t]);
+    for (size_t t = 0; adjacency && t < m.tris(); ++t)
+        for (int e = 0; e < 3; ++e) adjacency[3 * t + size_t(e)] = m.adj[t][size_t(e)];
+    for (size_t k = 0; node_bounds && k < m.nodes.size(); ++k) {
+        put3(node_bounds + 6 * k, m.nodes[k].b.mn);
+        put3(node_bounds + 6 * k + 3, m.nodes[k].b.mx);
+    }
+    for (size_t k = 0; node_links && k < m.nodes.size(); ++k) {
+        node_links[4 * k] = m.nodes[k].left; node_links[4 * k + 1] = m.nodes[k].right;
+        node_links[4 * k + 2] = int32_t(m.nodes[k].start); node_links[4 * k + 3] = int32_t(m.nodes[k].count);
+    }
+    for (size_t t = 0; order && t < m.order.size(); ++t) order[t] = m.order[t];
+    return 1;
 }
 // build_convex_hull (convex_hull.h:188-449) of n points (xyz). Returns 1 when a hull was built;
 // sizes[0..3] = vertices, faces, face vertex entries, edges; the arrays receive at most the caps
@@ -577,12 +634,60 @@ void pxo_set_kat_hulls(const pxg_hull* in, uint32_t n) {
     g_kat_hulls.clear();
     for (uint32_t i = 0; i < n; ++i) g_kat_hulls.push_back(hull_from(in[i]));
 }
+static std::vector<Mesh> g_kat_meshes;  // the meshes the single-pair entry points' shapes index
+int pxo_set_kat_meshes(const pxg_mesh_desc* in, uint32_t n) {
+    g_kat_meshes.assign(n, Mesh{});
+    for (uint32_t i = 0; i < n; ++i)
+        if (!mesh_from(in[i], g_kat_meshes[i])) return 0;
+    return 1;
+}
+// generate_mesh_contacts (mesh_contact.h:838-884) with the BVH (use_bvh 1) or every triangle (0):
+// normal from the mesh toward the shape
+int pxo_mesh_contacts(const pxg_shape* s, const float* pos, const float* rot, const pxg_shape* ms, const float* mpos,
+                      const float* mrot, int use_bvh, pxg_manifold* out) {
+    Shape<float> S = shape_from<float>(*s), M = shape_from<float>(*ms);
+    resolve_hull(S, g_kat_hulls, s->capsule_axis);
+    resolve_mesh(M, g_kat_meshes, ms->capsule_axis);
+    Manifold<float> m;
+    bool hit = mesh_contacts(S, v3<float>(pos), q4<float>(rot), M, v3<float>(mpos), q4<float>(mrot), m, use_bvh != 0);
+    manifold_to(m, *out);
+    return hit ? 1 : 0;
+}
+// the ascending candidate triangles of mesh_contact_candidates (mesh_contact.h:330-340)
+uint32_t pxo_mesh_candidates(const pxg_shape* s, const float* pos, const float* rot, const pxg_shape* ms,
+                             const float* mpos, const float* mrot, uint32_t* out, uint32_t cap) {
+    Shape<float> S = shape_from<float>(*s), M = shape_from<float>(*ms);
+    resolve_hull(S, g_kat_hulls, s->capsule_axis);
+    resolve_mesh(M, g_kat_meshes, ms->capsule_axis);
+    if (!M.mesh) return 0;
+    std::vector<uint32_t> c;
+    mesh_candidates(S, v3<float>(pos), q4<float>(rot), *static_cast<const Mesh*>(M.mesh),
+                    mesh_world_transform(M, v3<float>(mpos), q4<float>(mrot)), c);
+    for (size_t i = 0; i < c.size() && i < cap; ++i) out[i] = c[i];
+    return uint32_t(c.size());
+}
+// glm::inverse of a 4x4 (column-major, 16 floats)
+void pxo_mat4_inverse(const float* in, float* out) {
+    M4<float> m;
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) m.m[c][r] = in[4 * c + r];
+    M4<float> i = m4_inverse(m);
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) out[4 * c + r] = i.m[c][r];
+}
+// closest_point_on_triangle and barycentric_ (mesh_contact.h:61-98, 247-258)
+void pxo_closest_on_triangle(const float* p, const float* a, const float* b, const float* c, float* out, float* bary) {
+    put3(out, closest_on_triangle(v3<float>(p), v3<float>(a), v3<float>(b), v3<float>(c)));
+    put3(bary, barycentric(v3<float>(p), v3<float>(a), v3<float>(b), v3<float>(c)));
+}
 int pxo_generate_contacts(const pxg_shape* sa, const float* pa, const float* ra, const pxg_shape* sb, const float* pb,
                           const float* rb, int spec, pxg_manifold* out) {
     Manifold<float> m;
     Shape<float> A = shape_from<float>(*sa), B = shape_from<float>(*sb);
     resolve_hull(A, g_kat_hulls, sa->capsule_axis);
     resolve_hull(B, g_kat_hulls, sb->capsule_axis);
+    resolve_mesh(A, g_kat_meshes, sa->capsule_axis);
+    resolve_mesh(B, g_kat_meshes, sb->capsule_axis);
     bool hit = generate_contacts(0u, A, v3<float>(pa), q4<float>(ra), 1u, B, v3<float>(pb), q4<float>(rb), m, spec != 0);
     manifold_to(m, *out);
     return hit ? 1 : 0;
@@ -591,6 +696,7 @@ int pxo_generate_contacts(const pxg_shape* sa, const float* pa, const float* ra,
 void pxo_shape_bounds(const pxg_shape* s, const float* pos, const float* rot, float* out6, float* min_extent) {
     Shape<float> S = shape_from<float>(*s);
     resolve_hull(S, g_kat_hulls, s->capsule_axis);
+    resolve_mesh(S, g_kat_meshes, s->capsule_axis);
     AABB<float> b = world_bounds(S, v3<float>(pos), q4<float>(rot));
     out6[0] = b.mn.x; out6[1] = b.mn.y; out6[2] = b.mn.z; out6[3] = b.mx.x; out6[4] = b.mx.y; out6[5] = b.mx.z;
     *min_extent = World<float>::min_extent(S);

@@ -64,6 +64,7 @@ template <class T> struct World {
     std::vector<Body<T>> bodies;
     std::vector<Shape<T>> shapes;
     std::vector<Hull<T>> hulls;  // the hulls PXG_CONVEX_HULL shapes point at
+    std::vector<Mesh> meshes;    // the cooked meshes PXG_TRIANGLE_MESH shapes point at
     std::vector<int32_t> proxy;      // per shape: tree node, -1 = none
     std::vector<char> in_static;     // per shape
     Tree<T> static_tree, dyn_tree;
@@ -157,6 +158,7 @@ template <class T> struct World {
 
     static T min_extent(const Shape<T>& s) {  // world.h:1411-1424
         if (s.type == PXG_CONVEX_HULL) return hull_min_extent(s);
+        if (s.type == PXG_TRIANGLE_MESH) return std::numeric_limits<float>::max();
         if (s.type == PXG_SPHERE) return s.radius;
         if (s.type == PXG_BOX) return std::min(std::min(s.he.x, s.he.y), s.he.z);
         return std::min(s.cr, s.chh);

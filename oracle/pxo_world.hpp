@@ -66,6 +66,8 @@ template <class T> struct Shape {
     int fcomb = 0, rcomb = 0;
     const void* hull = nullptr;  // PXG_CONVEX_HULL: the world's Hull<T> (shape.h:97)
     T hull_scale = T(1);
+    const void* mesh = nullptr;  // PXG_TRIANGLE_MESH: the world's Mesh (shape.h:85)
+    T mesh_scale = T(1);
 };
 
 template <class T> struct Contact {
@@ -179,12 +181,16 @@ template <class T> inline Cap<T> world_capsule(const Shape<T>& s, const V3<T>& p
     return Cap<T>{c - axw * s.chh, c + axw * s.chh, s.cr};
 }
 template <class T> AABB<T> hull_world_bounds(const Shape<T>& s, const V3<T>& pos, const Q<T>& rot);  // pxo_hull.hpp
+template <class T> AABB<T> mesh_world_bounds(const Shape<T>& s, const V3<T>& pos, const Q<T>& rot);  // pxo_mesh.hpp
+template <class T> bool mesh_pair_contacts(const Shape<T>& sa, const V3<T>& pa, const Q<T>& ra, const Shape<T>& sb,
+                                           const V3<T>& pb, const Q<T>& rb, Manifold<T>& out, bool& hit);
 template <class T> T hull_min_extent(const Shape<T>& s);
 template <class T> bool hull_pair_contacts(const Shape<T>& sa, const V3<T>& pa, const Q<T>& ra, const Shape<T>& sb,
                                            const V3<T>& pb, const Q<T>& rb, Manifold<T>& out, bool& hit);
 template <class T> inline AABB<T> world_bounds(const Shape<T>& s, const V3<T>& pos, const Q<T>& rot) {  // shape.h:198-228
     AABB<T> out;
     if (s.type == PXG_CONVEX_HULL) return hull_world_bounds(s, pos, rot);
+    if (s.type == PXG_TRIANGLE_MESH) return mesh_world_bounds(s, pos, rot);
     if (s.type == PXG_SPHERE) {
         Sph<T> sp = world_sphere(s, pos, rot);
         out.mn = sp.c - V3<T>(sp.r);
@@ -581,6 +587,8 @@ template <class T> bool generate_contacts(uint32_t ida, const Shape<T>& sa, cons
         if (hit) out.n = -out.n;
     } else if (ta == PXG_BOX && tb == PXG_CAPSULE) {
         hit = capsule_vs_box(world_capsule(sb, pb, rb), world_obb(sa, pa, ra), out);
+    } else if (ta == PXG_TRIANGLE_MESH || tb == PXG_TRIANGLE_MESH) {
+        mesh_pair_contacts(sa, pa, ra, sb, pb, rb, out, hit);
     } else if (!hull_pair_contacts(sa, pa, ra, sb, pb, rb, out, hit)) {
         return false;
     }
@@ -703,3 +711,4 @@ private:
 }  // namespace pxo
 
 #include "pxo_hull.hpp"
+#include "pxo_mesh.hpp"

@@ -12,7 +12,7 @@ PXG_OK = 0
 PXG_ERR_INVALID, PXG_ERR_CAPACITY, PXG_ERR_HIP, PXG_ERR_STATE, PXG_ERR_DEVICE = 1, 2, 3, 4, 5
 PXG_STATIC, PXG_KINEMATIC, PXG_DYNAMIC = 0, 1, 2
 PXG_INTERPOLATE, PXG_INTERPOLATE_NONE = 0, 1
-PXG_SPHERE, PXG_BOX, PXG_CAPSULE, PXG_CONVEX_HULL = 0, 1, 2, 3
+PXG_SPHERE, PXG_BOX, PXG_CAPSULE, PXG_CONVEX_HULL, PXG_TRIANGLE_MESH = 0, 1, 2, 3, 4
 PXG_AVERAGE, PXG_MINIMUM, PXG_MAXIMUM, PXG_MULTIPLY = 0, 1, 2, 3
 PXG_PHASE_FORCES_COLLIDE, PXG_PHASE_SOLVE_INTEGRATE = 0, 1
 PXG_ORDER_REFERENCE, PXG_ORDER_COLOURED = 0, 1
@@ -270,7 +270,7 @@ class HullDesc(ctypes.Structure):
     _fields_ = [("vertices", ctypes.c_void_p), ("n_vertices", ctypes.c_uint32), ("n_faces", ctypes.c_uint32),
                 ("face_planes", ctypes.c_void_p), ("face_starts", ctypes.c_void_p), ("face_vertices", ctypes.c_void_p),
                 ("edges", ctypes.c_void_p), ("n_edges", ctypes.c_uint32), ("local_min", ctypes.c_float * 3),
-                ("local_max", ctypes.c_float * 3)]
+                ("local_max", ctypes.c_float * 3), ("centroid", ctypes.c_float * 3), ("volume", ctypes.c_float)]
 
 
 class Hull:
@@ -309,7 +309,31 @@ class Hull:
         for i in range(3):
             d.local_min[i] = float(self.bounds6[i])
             d.local_max[i] = float(self.bounds6[3 + i])
+        if self.centroid is None:
+            self.volume, self.centroid = solid_volume_centroid(self.vertices, self.face_starts, self.face_vertices)
+        for i in range(3):
+            d.centroid[i] = float(self.centroid[i])
+        d.volume = float(self.volume or 0.0)
         return d
+
+
+def solid_volume_centroid(vertices, face_starts, face_vertices):
+    """Volume and centroid of a closed convex polyhedron (faces counter-clockwise about their
+    outward normals): signed tetrahedra from the vertex mean to each face's fan triangles, in
+    float64. For hulls given without the builder's own centroid."""
+    v = np.asarray(vertices, np.float64)
+    o = v.mean(axis=0)
+    vol = 0.0
+    acc = np.zeros(3)
+    for f in range(len(face_starts) - 1):
+        idx = face_vertices[face_starts[f]:face_starts[f + 1]]
+        for k in range(1, len(idx) - 1):
+            a, b, c = v[idx[0]] - o, v[idx[k]] - o, v[idx[k + 1]] - o
+            t = np.dot(a, np.cross(b, c)) / 6.0
+            vol += t
+            acc += t * (a + b + c) / 4.0
+    cen = o + (acc / vol if vol > 0 else 0.0)
+    return vol, cen.astype(np.float32)
 
 
 def hull_table(hulls):
@@ -319,3 +343,43 @@ def hull_table(hulls):
         arr[i] = h.desc()
     return arr
 
+
+
+class MeshDesc(ctypes.Structure):
+    """pxg_mesh_desc (include/pxg.h): a triangle soup's positions and index triples."""
+    _fields_ = [("positions", ctypes.c_void_p), ("n_positions", ctypes.c_uint32), ("n_indices", ctypes.c_uint32),
+                ("indices", ctypes.c_void_p)]
+
+
+class MeshInfo(ctypes.Structure):
+    """pxg_mesh_info (include/pxg.h): sizes and local bounds of a cooked mesh."""
+    _fields_ = [("n_vertices", ctypes.c_uint32), ("n_triangles", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32),
+                ("non_manifold_edges", ctypes.c_uint32), ("local_min", ctypes.c_float * 3),
+                ("local_max", ctypes.c_float * 3)]
+
+
+class TriangleMesh:
+    """A triangle soup for PXG_TRIANGLE_MESH shapes: positions (n, 3) and indices (3 per triangle),
+    cooked by the library as TriangleMesh::from_triangles (geometry/triangle_mesh.h:75-170) does.
+    desc() gives the pxg_mesh_desc record over these arrays (keep this object alive while the
+    record is used)."""
+
+    def __init__(self, positions, indices):
+        self.positions = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1, 3))
+        self.indices = np.ascontiguousarray(np.asarray(indices, np.uint32).reshape(-1))
+
+    def desc(self):
+        d = MeshDesc()
+        d.positions = self.positions.ctypes.data if len(self.positions) else None
+        d.n_positions = len(self.positions)
+        d.indices = self.indices.ctypes.data if len(self.indices) else None
+        d.n_indices = len(self.indices)
+        return d
+
+
+def mesh_table(meshes):
+    """A ctypes array of pxg_mesh_desc records for a list of TriangleMesh objects."""
+    arr = (MeshDesc * max(1, len(meshes)))()
+    for i, m in enumerate(meshes):
+        arr[i] = m.desc()
+    return arr

@@ -17,6 +17,8 @@ _SIGNATURES = {
     "pxg_set_bodies": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_bodies": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_hulls": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_set_meshes": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
+    "pxg_cook_mesh": ([ctypes.c_void_p] * 6, ctypes.c_int),
     "pxg_set_bodies_subset": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_get_poses": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),
     "pxg_set_shapes": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32], ctypes.c_int),

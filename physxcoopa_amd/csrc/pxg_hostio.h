@@ -74,30 +74,42 @@ struct ShapeSoA {
     std::vector<uint4> info;
     std::vector<float4> lc, lr, dim, mat;
 };
-// static_owner(i) tells whether shape i's owner is a static body. Returns an error string or null.
+// static_owner(i) tells whether shape i's owner is a static body. n_hulls / n_meshes: the table
+// sizes convex hull and triangle mesh shapes index (0: such shapes are refused); refs, if given,
+// receives 1 + the largest hull and mesh index used. Returns an error string or null.
 template <class StaticFn>
 inline const char* pack_shapes(const pxg_shape* in, size_t n, uint32_t n_owner_bodies, StaticFn static_owner,
-                               ShapeSoA& o, uint32_t n_hulls = 0) {
+                               ShapeSoA& o, uint32_t n_hulls = 0, uint32_t n_meshes = 0, uint32_t* refs = nullptr) {
     o.info.resize(n); o.lc.resize(n); o.lr.resize(n); o.dim.resize(n); o.mat.resize(n);
+    uint32_t hull_refs = 0, mesh_refs = 0;
     for (size_t i = 0; i < n; ++i) {
         const pxg_shape& s = in[i];
         if (s.owner >= n_owner_bodies) return "shape owner out of range";
-        const bool hull = s.type == PXG_CONVEX_HULL;
-        if (s.type > PXG_CONVEX_HULL) return "only sphere, box, capsule and convex hull shapes are supported";
+        const bool hull = s.type == PXG_CONVEX_HULL, mesh = s.type == PXG_TRIANGLE_MESH;
+        if (s.type > PXG_TRIANGLE_MESH) return "only sphere, box, capsule, convex hull and triangle mesh shapes are supported";
         if (hull && (s.capsule_axis < 0 || uint32_t(s.capsule_axis) >= n_hulls))
             return "a convex hull shape's index (capsule_axis) is not in the uploaded hull table";
-        if (!hull && (s.capsule_axis < 0 || s.capsule_axis > 2)) return "bad capsule axis";
+        if (mesh && (s.capsule_axis < 0 || uint32_t(s.capsule_axis) >= n_meshes))
+            return "a triangle mesh shape's index (capsule_axis) is not in the uploaded mesh table";
+        if (!hull && !mesh && (s.capsule_axis < 0 || s.capsule_axis > 2)) return "bad capsule axis";
+        if (hull) hull_refs = std::max(hull_refs, uint32_t(s.capsule_axis) + 1);
+        if (mesh) mesh_refs = std::max(mesh_refs, uint32_t(s.capsule_axis) + 1);
+        const bool indexed = hull || mesh;
         uint32_t flags = s.type | (s.enabled ? SF_ENABLED : 0u) | (s.is_trigger ? SF_TRIGGER : 0u) |
-                         (static_owner(i) ? SF_STATIC : 0u) | (hull ? 0u : uint32_t(s.capsule_axis) << 8);
+                         (static_owner(i) ? SF_STATIC : 0u) | (indexed ? 0u : uint32_t(s.capsule_axis) << 8);
         o.info[i] = make_uint4(s.owner, flags, s.layer & 31u, (s.friction_combine & 255u) | ((s.restitution_combine & 255u) << 8));
         o.lc[i] = make_float4(s.local_center[0], s.local_center[1], s.local_center[2], s.radius);
         o.lr[i] = make_float4(s.local_rotation[0], s.local_rotation[1], s.local_rotation[2], s.local_rotation[3]);
         o.dim[i] = make_float4(s.half_extents[0], s.half_extents[1], s.half_extents[2], s.capsule_radius);
-        if (hull) {  // the hull index goes where a box keeps its first half extent
+        if (indexed) {  // the hull or mesh index goes where a box keeps its first half extent
             uint32_t k = uint32_t(s.capsule_axis);
             std::memcpy(&o.dim[i].x, &k, sizeof(k));
         }
         o.mat[i] = make_float4(s.capsule_half_height, s.dynamic_friction, s.restitution, s.static_friction);
+    }
+    if (refs) {
+        refs[0] = hull_refs;
+        refs[1] = mesh_refs;
     }
     return nullptr;
 }

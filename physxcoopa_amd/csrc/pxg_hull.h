@@ -21,6 +21,7 @@ struct HullTab {
     const uint4* meta;    // per hull: vertex offset, vertex count, face offset, face count
     const uint4* meta2;   // per hull: edge offset, edge count
     const float4* lb;     // per hull: local bounds min, max
+    const float4* cen;    // per hull: centroid, volume
     const float4* v;      // vertices
     const float4* pl;     // face planes: outward unit normal, offset
     const uint2* face;    // per face: first entry in fv, vertex count

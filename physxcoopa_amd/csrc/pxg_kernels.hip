@@ -5,7 +5,7 @@
 // work item per thread, grid-stride where the item count lives on the device.
 #include "pxg_state.h"
 #include "pxg_device.h"
-#include "pxg_hull.h"
+#include "pxg_mesh.h"
 #include "pxg_joints.h"
 #include "pxg_mlp.h"
 
@@ -54,16 +54,24 @@ __device__ __forceinline__ ShapeGeom load_shape(const DevState& d, uint32_t i, u
 }
 __device__ __forceinline__ HullTab hull_tab(const DevState& d) {
     HullTab t;
-    t.meta = d.h_meta; t.meta2 = d.h_meta2; t.lb = d.h_lb; t.v = d.h_v; t.pl = d.h_pl;
+    t.meta = d.h_meta; t.meta2 = d.h_meta2; t.lb = d.h_lb; t.cen = d.h_cen; t.v = d.h_v; t.pl = d.h_pl;
     t.face = d.h_face; t.fv = d.h_fv; t.e = d.h_e;
+    return t;
+}
+__device__ __forceinline__ MeshTab mesh_tab(const DevState& d) {
+    MeshTab t;
+    t.meta = d.me_meta; t.lb = d.me_lb; t.v = d.me_v; t.tri = d.me_tri; t.n = d.me_n; t.adj = d.me_adj;
+    t.node_lo = d.me_node_lo; t.node_hi = d.me_node_hi; t.node_tri = d.me_node_tri; t.order = d.me_order;
     return t;
 }
 // tight world AABB and CCD extent of any shape type
 __device__ __forceinline__ void shape_bounds(const DevState& d, const ShapeGeom& s, v3 pos, quat rot, v3& mn, v3& mx) {
     if (s.type == SH_HULL) hull_bounds(hull_tab(d), s, pos, rot, mn, mx);
+    else if (s.type == SH_MESH) mesh_bounds(mesh_tab(d), s, pos, rot, mn, mx);
     else world_bounds(s, pos, rot, mn, mx);
 }
 __device__ __forceinline__ float shape_min_extent(const DevState& d, const ShapeGeom& s) {
+    if (s.type == SH_MESH) return 3.402823466e+38f;  // world.h:1416: a mesh never gates speculation
     return s.type == SH_HULL ? hull_min_extent(hull_tab(d), s) : min_extent(s);
 }
 __device__ __forceinline__ bool shape_live(const DevState& d, uint4 info) {
@@ -534,36 +542,13 @@ __device__ __forceinline__ bool contacts_off(const DevState& d, uint32_t a, uint
     }
     return lo < d.n_nc && d.nc_keys[lo] == key;
 }
-__global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_pairs) return;
-    uint2 pr = d.pairs[k];
-    uint32_t i = pr.x, j = pr.y;
-    uint4 ii = d.s_info[i], ij = d.s_info[j];
-    uint32_t bi = ii.x, bj = ij.x;
-    uint32_t fi = d.b_flags[bi], fj = d.b_flags[bj];
-    uint32_t valid = 0;
-    Man m;
-    m.count = 0;
-    float4 mi = d.s_mat[i], mj = d.s_mat[j];
-    if ((fi & BF_ALIVE) && (fj & BF_ALIVE) && ((fi & BF_TYPE) == 2u || (fj & BF_TYPE) == 2u) &&
-        (d.layer_mask[ii.z & 31u] & (1u << (ij.z & 31u))) && !contacts_off(d, bi, bj)) {
-        ShapeGeom si = load_shape(d, i, ii), sj = load_shape(d, j, ij);
-        float4 vi = d.b_vel[bi], vj = d.b_vel[bj];
-        bool spec = (len(ld3(vi)) * prm.h > shape_min_extent(d, si)) || (len(ld3(vj)) * prm.h > shape_min_extent(d, sj));
-        if (si.type == SH_HULL || sj.type == SH_HULL) {
-            if (hull_contacts(hull_tab(d), si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), m))
-                valid = 1u;
-        } else if (generate_contacts(si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), spec, m)) {
-            valid = 1u;
-        }
-    }
-    d.pm_valid[k] = valid;
-    if (!valid) return;
-    float fr = combine(mi.y, mj.y, int(ii.w & 255u), int(ij.w & 255u));
-    float re = combine(mi.z, mj.z, int(ii.w >> 8), int(ij.w >> 8));
-    uint32_t trig = ((ii.y | ij.y) & SF_TRIGGER) ? 1u : 0u;
-    d.pm_ab[k] = make_uint2(bi, bj);
+__device__ __forceinline__ void store_pair_manifold(const DevState& d, uint32_t k, uint32_t i, uint32_t j, uint4 ii,
+                                                    uint4 ij, const Man& m) {
+    const float4 mi = d.s_mat[i], mj = d.s_mat[j];
+    const float fr = combine(mi.y, mj.y, int(ii.w & 255u), int(ij.w & 255u));
+    const float re = combine(mi.z, mj.z, int(ii.w >> 8), int(ij.w >> 8));
+    const uint32_t trig = ((ii.y | ij.y) & SF_TRIGGER) ? 1u : 0u;
+    d.pm_ab[k] = make_uint2(ii.x, ij.x);
     d.pm_nf[k] = st3(m.n, fr);
     d.pm_meta[k] = make_uint4(uint32_t(m.count) | (trig << 8), __float_as_uint(re), 0u, 0u);
     for (int t = 0; t < 4; ++t) {
@@ -575,6 +560,74 @@ __global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
             d.pm_q[4 * k + t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     }
+}
+// the pair filter of world.h:1545-1577: both alive, one dynamic, layers, not switched off
+__device__ __forceinline__ bool pair_collides(const DevState& d, uint4 ii, uint4 ij) {
+    const uint32_t fi = d.b_flags[ii.x], fj = d.b_flags[ij.x];
+    return (fi & BF_ALIVE) && (fj & BF_ALIVE) && ((fi & BF_TYPE) == 2u || (fj & BF_TYPE) == 2u) &&
+           (d.layer_mask[ii.z & 31u] & (1u << (ij.z & 31u))) && !contacts_off(d, ii.x, ij.x);
+}
+__device__ __forceinline__ bool mesh_pair(uint4 ii, uint4 ij) {
+    return int(ii.y & 15u) == SH_MESH || int(ij.y & 15u) == SH_MESH;
+}
+
+__global__ void k_narrowphase(DevState d, StepParams prm, uint32_t n_pairs) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_pairs) return;
+    uint2 pr = d.pairs[k];
+    uint32_t i = pr.x, j = pr.y;
+    uint4 ii = d.s_info[i], ij = d.s_info[j];
+    if (mesh_pair(ii, ij)) return;  // k_narrowphase_mesh's
+    uint32_t bi = ii.x, bj = ij.x;
+    uint32_t valid = 0;
+    Man m;
+    m.count = 0;
+    if (pair_collides(d, ii, ij)) {
+        ShapeGeom si = load_shape(d, i, ii), sj = load_shape(d, j, ij);
+        float4 vi = d.b_vel[bi], vj = d.b_vel[bj];
+        bool spec = (len(ld3(vi)) * prm.h > shape_min_extent(d, si)) || (len(ld3(vj)) * prm.h > shape_min_extent(d, sj));
+        if (si.type == SH_HULL || sj.type == SH_HULL) {
+            if (hull_contacts(hull_tab(d), si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), m))
+                valid = 1u;
+        } else if (generate_contacts(si, ld3(d.b_pos[bi]), ldq(d.b_rot[bi]), sj, ld3(d.b_pos[bj]), ldq(d.b_rot[bj]), spec, m)) {
+            valid = 1u;
+        }
+    }
+    d.pm_valid[k] = valid;
+    if (valid) store_pair_manifold(d, k, i, j, ii, ij, m);
+}
+
+// The pairs with a mesh (narrowphase.h:313-320): generate_mesh_contacts with the normal turned
+// from a to b; no mesh against mesh, no speculative contacts. Launched only when meshes exist.
+__global__ void k_narrowphase_mesh(DevState d, uint32_t n_pairs) {
+    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_pairs) return;
+    uint2 pr = d.pairs[k];
+    uint32_t i = pr.x, j = pr.y;
+    uint4 ii = d.s_info[i], ij = d.s_info[j];
+    if (!mesh_pair(ii, ij)) return;
+    uint32_t valid = 0;
+    Man m;
+    m.count = 0;
+    if (pair_collides(d, ii, ij)) {
+        const ShapeGeom si = load_shape(d, i, ii), sj = load_shape(d, j, ij);
+        const v3 pi = ld3(d.b_pos[ii.x]), pj = ld3(d.b_pos[ij.x]);
+        const quat ri = ldq(d.b_rot[ii.x]), rj = ldq(d.b_rot[ij.x]);
+        bool overflow = false;
+        if (si.type == SH_MESH && sj.type == SH_MESH) {
+            valid = 0u;
+        } else if (sj.type == SH_MESH) {
+            if (mesh_contacts(mesh_tab(d), hull_tab(d), si, pi, ri, sj, pj, rj, m, overflow)) {
+                m.n = -m.n;
+                valid = 1u;
+            }
+        } else if (mesh_contacts(mesh_tab(d), hull_tab(d), sj, pj, rj, si, pi, ri, m, overflow)) {
+            valid = 1u;
+        }
+        if (overflow) atomicOr(&d.cnt->err, ERR_MESH_VERTICES);
+    }
+    d.pm_valid[k] = valid;
+    if (valid) store_pair_manifold(d, k, i, j, ii, ij, m);
 }
 
 // compaction in pair order (world.h:1647-1653)
@@ -2343,6 +2396,7 @@ void launch_broadphase_write(const DevState& d, const StepParams& p, hipStream_t
 void launch_narrowphase(const DevState& d, const StepParams& p, uint32_t n_pairs, hipStream_t st) {
     if (!n_pairs) return;
     hipLaunchKernelGGL(k_narrowphase, dim3(blocks(n_pairs)), dim3(kBlock), 0, st, d, p, n_pairs);
+    if (d.n_meshes) hipLaunchKernelGGL(k_narrowphase_mesh, dim3(blocks(n_pairs)), dim3(kBlock), 0, st, d, n_pairs);
     launch_scan(d.pm_valid, d.pm_off, n_pairs, nullptr, d.scan_tmp, st);
     hipLaunchKernelGGL(k_compact, dim3(blocks(n_pairs)), dim3(kBlock), 0, st, d, n_pairs);
 }

@@ -39,7 +39,7 @@ constexpr uint32_t SF_ENABLED = 1u << 4, SF_TRIGGER = 1u << 5, SF_STATIC = 1u <<
 
 // error bits reported through DevState::err
 constexpr uint32_t ERR_PAIRS = 1u, ERR_LARGE = 2u, ERR_COLOURS = 4u, ERR_MANIFOLDS = 8u, ERR_CACHE = 16u,
-                   ERR_CELLS = 32u;
+                   ERR_CELLS = 32u, ERR_MESH_VERTICES = 64u;
 
 struct Counters {  // device-side counters, read back by the host at the step's sync points
     uint32_t pairs;
@@ -87,10 +87,16 @@ struct DevState {
     float4 *s_mat;      // capsule half height, dynamic friction, restitution, static friction
     // convex hulls (pxg_set_hulls): the table PXG_CONVEX_HULL shapes index (pxg_hull.h's HullTab)
     uint4 *h_meta, *h_meta2;
-    float4 *h_lb, *h_v, *h_pl;
+    float4 *h_lb, *h_v, *h_pl, *h_cen;
     uint2 *h_face, *h_e;
     uint32_t *h_fv;
     uint32_t n_hulls;
+    // triangle meshes (pxg_set_meshes): the table PXG_TRIANGLE_MESH shapes index (pxg_mesh.h's MeshTab)
+    uint4 *me_meta, *me_tri, *me_adj;
+    float4 *me_lb, *me_v, *me_n, *me_node_lo, *me_node_hi;
+    uint2 *me_node_tri;
+    uint32_t *me_order;
+    uint32_t n_meshes;
     float4 *s_fmin, *s_fmax;   // fat AABB (persistent broadphase state)
     float4 *s_tmin, *s_tmax;   // tight AABB of this step; w of s_tmin: 1 = large shape
     uint32_t *layer_mask;      // 32 rows

@@ -5,6 +5,7 @@
 #include "../../include/pxg.h"
 #include "pxg_state.h"
 #include "pxg_hostio.h"
+#include "pxg_mesh_cook.h"
 #include "pxg_mlp.h"
 
 #include <hip/hip_runtime.h>
@@ -58,6 +59,7 @@ struct pxg_world {
     bool shapes_ready = false;
     void* noalloc_scratch = nullptr;  // this device's address of `scratch` (pxg_overlap_batch_noalloc)
     uint8_t* body_stage = nullptr;    // staging of pxg_set_bodies_subset (grown on demand)
+    uint32_t hull_refs = 0, mesh_refs = 0;  // 1 + the largest hull / mesh index the shapes use
     size_t body_stage_bytes = 0;
     // profiling: HIP event pairs around every velocity-pass launch
     bool profiling = false;
@@ -528,8 +530,9 @@ extern "C" int pxg_get_poses(pxg_world* w, pxg_pose* out, uint32_t n) {
 extern "C" int pxg_set_hulls(pxg_world* w, const pxg_hull* hulls, uint32_t n) {
     if (!w || (n && !hulls)) return fail(PXG_ERR_INVALID, "pxg_set_hulls: null argument");
     if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_hulls: not allowed between the two step phases");
+    if (n < w->hull_refs) return fail(PXG_ERR_STATE, "pxg_set_hulls: the shapes use hull " + std::to_string(w->hull_refs - 1));
     std::vector<uint4> meta(n), meta2(n);
-    std::vector<float4> lb(2 * size_t(n)), v, pl;
+    std::vector<float4> lb(2 * size_t(n)), cen(n), v, pl;
     std::vector<uint2> face, edge;
     std::vector<uint32_t> fv;
     for (uint32_t k = 0; k < n; ++k) {
@@ -543,6 +546,7 @@ extern "C" int pxg_set_hulls(pxg_world* w, const pxg_hull* hulls, uint32_t n) {
         meta2[k] = make_uint4(uint32_t(edge.size()), h.n_edges, 0u, 0u);
         lb[2 * k] = make_float4(h.local_min[0], h.local_min[1], h.local_min[2], 0.0f);
         lb[2 * k + 1] = make_float4(h.local_max[0], h.local_max[1], h.local_max[2], 0.0f);
+        cen[k] = make_float4(h.centroid[0], h.centroid[1], h.centroid[2], h.volume);
         for (uint32_t i = 0; i < h.n_vertices; ++i)
             v.push_back(make_float4(h.vertices[3 * i], h.vertices[3 * i + 1], h.vertices[3 * i + 2], 0.0f));
         for (uint32_t f = 0; f < h.n_faces; ++f) {
@@ -563,21 +567,22 @@ extern "C" int pxg_set_hulls(pxg_world* w, const pxg_hull* hulls, uint32_t n) {
     }
     HIPCHK(hipStreamSynchronize(w->stream));
     DevState& d = w->d;
-    void* old[8] = {d.h_meta, d.h_meta2, d.h_lb, d.h_v, d.h_pl, d.h_face, d.h_e, d.h_fv};
+    void* old[9] = {d.h_meta, d.h_meta2, d.h_lb, d.h_cen, d.h_v, d.h_pl, d.h_face, d.h_e, d.h_fv};
     for (void* p : old) dfree(w, p);
     d.h_meta = d.h_meta2 = nullptr;
-    d.h_lb = d.h_v = d.h_pl = nullptr;
+    d.h_lb = d.h_cen = d.h_v = d.h_pl = nullptr;
     d.h_face = d.h_e = nullptr;
     d.h_fv = nullptr;
     d.n_hulls = 0;
     if (!n) return PXG_OK;
-    DALLOC(d.h_meta, n); DALLOC(d.h_meta2, n); DALLOC(d.h_lb, 2 * size_t(n));
+    DALLOC(d.h_meta, n); DALLOC(d.h_meta2, n); DALLOC(d.h_lb, 2 * size_t(n)); DALLOC(d.h_cen, n);
     DALLOC(d.h_v, v.size()); DALLOC(d.h_pl, pl.size()); DALLOC(d.h_face, face.size()); DALLOC(d.h_fv, fv.size());
     DALLOC(d.h_e, std::max<size_t>(edge.size(), 1));
     hipStream_t st = w->stream;
     HIPCHK(hipMemcpyAsync(d.h_meta, meta.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d.h_meta2, meta2.data(), n * sizeof(uint4), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d.h_lb, lb.data(), lb.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.h_cen, cen.data(), cen.size() * sizeof(float4), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d.h_v, v.data(), v.size() * sizeof(float4), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d.h_pl, pl.data(), pl.size() * sizeof(float4), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(d.h_face, face.data(), face.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
@@ -588,6 +593,104 @@ extern "C" int pxg_set_hulls(pxg_world* w, const pxg_hull* hulls, uint32_t n) {
     return PXG_OK;
 }
 
+static int check_mesh_desc(const pxg_mesh_desc* m, const std::string& at) {
+    if (!m) return fail(PXG_ERR_INVALID, at + "null mesh");
+    if ((m->n_positions && !m->positions) || (m->n_indices && !m->indices)) return fail(PXG_ERR_INVALID, at + "null array");
+    return PXG_OK;
+}
+
+extern "C" int pxg_cook_mesh(const pxg_mesh_desc* m, pxg_mesh_info* info, float* vertices, uint32_t* indices,
+                             float* normals, uint32_t* adjacency) {
+    int r = check_mesh_desc(m, "pxg_cook_mesh: ");
+    if (r != PXG_OK) return r;
+    pxh::CookedMesh c;
+    if (!pxh::cook_mesh(m->positions, m->n_positions, m->indices, m->n_indices, c))
+        return fail(PXG_ERR_INVALID, "pxg_cook_mesh: an index is out of range");
+    if (info) {
+        info->n_vertices = uint32_t(c.vertices.size());
+        info->n_triangles = c.n_tris();
+        info->n_nodes = uint32_t(c.nodes.size());
+        info->non_manifold_edges = c.non_manifold;
+        const float lo[3] = {c.bounds.lo.x, c.bounds.lo.y, c.bounds.lo.z}, hi[3] = {c.bounds.hi.x, c.bounds.hi.y, c.bounds.hi.z};
+        std::memcpy(info->local_min, lo, sizeof(lo));
+        std::memcpy(info->local_max, hi, sizeof(hi));
+    }
+    for (size_t i = 0; vertices && i < c.vertices.size(); ++i) {
+        vertices[3 * i] = c.vertices[i].x; vertices[3 * i + 1] = c.vertices[i].y; vertices[3 * i + 2] = c.vertices[i].z;
+    }
+    if (indices) std::copy(c.tris.begin(), c.tris.end(), indices);
+    for (size_t t = 0; normals && t < c.normals.size(); ++t) {
+        normals[3 * t] = c.normals[t].x; normals[3 * t + 1] = c.normals[t].y; normals[3 * t + 2] = c.normals[t].z;
+    }
+    if (adjacency) std::copy(c.neighbours.begin(), c.neighbours.end(), adjacency);
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_meshes(pxg_world* w, const pxg_mesh_desc* meshes, uint32_t n) {
+    if (!w || (n && !meshes)) return fail(PXG_ERR_INVALID, "pxg_set_meshes: null argument");
+    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_meshes: not allowed between the two step phases");
+    if (n < w->mesh_refs) return fail(PXG_ERR_STATE, "pxg_set_meshes: the shapes use mesh " + std::to_string(w->mesh_refs - 1));
+    std::vector<uint4> meta(n), tri, adj;
+    std::vector<float4> lb(2 * size_t(n)), v, nrm, node_lo, node_hi;
+    std::vector<uint2> node_tri;
+    std::vector<uint32_t> order;
+    for (uint32_t k = 0; k < n; ++k) {
+        const std::string at = "pxg_set_meshes: mesh " + std::to_string(k) + ": ";
+        int r = check_mesh_desc(&meshes[k], at);
+        if (r != PXG_OK) return r;
+        pxh::CookedMesh c;
+        if (!pxh::cook_mesh(meshes[k].positions, meshes[k].n_positions, meshes[k].indices, meshes[k].n_indices, c))
+            return fail(PXG_ERR_INVALID, at + "an index is out of range");
+        if (c.n_tris() == 0) return fail(PXG_ERR_INVALID, at + "no triangle left after welding");
+        if (c.depth > pxh::kTreeDepthMax) return fail(PXG_ERR_INVALID, at + "triangle tree too deep");
+        meta[k] = make_uint4(uint32_t(v.size()), uint32_t(tri.size()), uint32_t(node_lo.size()), c.n_tris());
+        lb[2 * k] = make_float4(c.bounds.lo.x, c.bounds.lo.y, c.bounds.lo.z, 0.0f);
+        lb[2 * k + 1] = make_float4(c.bounds.hi.x, c.bounds.hi.y, c.bounds.hi.z, 0.0f);
+        for (const pxh::P3& p : c.vertices) v.push_back(make_float4(p.x, p.y, p.z, 0.0f));
+        for (uint32_t t = 0; t < c.n_tris(); ++t) {
+            tri.push_back(make_uint4(c.tris[3 * t], c.tris[3 * t + 1], c.tris[3 * t + 2], 0u));
+            adj.push_back(make_uint4(c.neighbours[3 * t], c.neighbours[3 * t + 1], c.neighbours[3 * t + 2], 0u));
+            nrm.push_back(make_float4(c.normals[t].x, c.normals[t].y, c.normals[t].z, 0.0f));
+        }
+        for (const pxh::MeshNodeH& nd : c.nodes) {
+            float l, rgt;
+            std::memcpy(&l, &nd.left, sizeof(l));
+            std::memcpy(&rgt, &nd.right, sizeof(rgt));
+            node_lo.push_back(make_float4(nd.box.lo.x, nd.box.lo.y, nd.box.lo.z, l));
+            node_hi.push_back(make_float4(nd.box.hi.x, nd.box.hi.y, nd.box.hi.z, rgt));
+            node_tri.push_back(make_uint2(nd.first, nd.count));
+        }
+        order.insert(order.end(), c.order.begin(), c.order.end());
+    }
+    HIPCHK(hipStreamSynchronize(w->stream));
+    DevState& d = w->d;
+    void* old[10] = {d.me_meta, d.me_lb, d.me_v, d.me_tri, d.me_n, d.me_adj, d.me_node_lo, d.me_node_hi, d.me_node_tri, d.me_order};
+    for (void* p : old) dfree(w, p);
+    d.me_meta = d.me_tri = d.me_adj = nullptr;
+    d.me_lb = d.me_v = d.me_n = d.me_node_lo = d.me_node_hi = nullptr;
+    d.me_node_tri = nullptr;
+    d.me_order = nullptr;
+    d.n_meshes = 0;
+    if (!n) return PXG_OK;
+    DALLOC(d.me_meta, n); DALLOC(d.me_lb, 2 * size_t(n)); DALLOC(d.me_v, v.size()); DALLOC(d.me_tri, tri.size());
+    DALLOC(d.me_n, nrm.size()); DALLOC(d.me_adj, adj.size()); DALLOC(d.me_node_lo, node_lo.size());
+    DALLOC(d.me_node_hi, node_hi.size()); DALLOC(d.me_node_tri, node_tri.size()); DALLOC(d.me_order, order.size());
+    hipStream_t st = w->stream;
+    HIPCHK(hipMemcpyAsync(d.me_meta, meta.data(), meta.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_lb, lb.data(), lb.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_v, v.data(), v.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_tri, tri.data(), tri.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_n, nrm.data(), nrm.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_adj, adj.data(), adj.size() * sizeof(uint4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_node_lo, node_lo.data(), node_lo.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_node_hi, node_hi.data(), node_hi.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_node_tri, node_tri.data(), node_tri.size() * sizeof(uint2), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d.me_order, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    d.n_meshes = n;
+    return PXG_OK;
+}
+
 extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n) {
     if (!w || (!shapes && n)) return fail(PXG_ERR_INVALID, "pxg_set_shapes: null argument");
     if (!w->cap_b) return fail(PXG_ERR_STATE, "pxg_set_shapes: upload bodies first");
@@ -595,9 +698,12 @@ extern "C" int pxg_set_shapes(pxg_world* w, const pxg_shape* shapes, uint32_t n)
     int r = alloc_shapes(w, n);
     if (r != PXG_OK) return r;
     ShapeSoA soa;
+    uint32_t refs[2] = {0, 0};
     const char* err = pack_shapes(shapes, n, w->n_bodies, [&](size_t i) { return w->body_types[shapes[i].owner] == PXG_STATIC; }, soa,
-                                  w->d.n_hulls);
+                                  w->d.n_hulls, w->d.n_meshes, refs);
     if (err) return fail(PXG_ERR_INVALID, std::string("pxg_set_shapes: ") + err);
+    w->hull_refs = refs[0];
+    w->mesh_refs = refs[1];
     DevState& d = w->d;
     hipStream_t st = w->stream;
     if (n) {
@@ -753,6 +859,7 @@ static int read_counters(pxg_world* w) {
         if (err & ERR_COLOURS) what += " more than 64 colours;";
         if (err & ERR_MANIFOLDS) what += " manifold buffer full;";
         if (err & ERR_CACHE) what += " warm-start table full;";
+        if (err & ERR_MESH_VERTICES) what += " more than 32 mesh vertices inside one box or hull;";
         if (err & ERR_BARRIER_BIT) {
             what += " grid barrier timed out;";
             HIPCHK(hipMemset(w->d.bar, 0, kBarWords * sizeof(uint32_t)));

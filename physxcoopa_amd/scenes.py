@@ -14,7 +14,7 @@ import math
 
 import numpy as np
 
-from ._abi import (BODY_DTYPE, SHAPE_DTYPE, PXG_STATIC, PXG_DYNAMIC, PXG_SPHERE, PXG_BOX, PXG_CAPSULE, PXG_CONVEX_HULL,
+from ._abi import (BODY_DTYPE, SHAPE_DTYPE, PXG_STATIC, PXG_DYNAMIC, PXG_SPHERE, PXG_BOX, PXG_CAPSULE, PXG_CONVEX_HULL, PXG_TRIANGLE_MESH, TriangleMesh,
                    PXG_AVERAGE)
 
 F = np.float32
@@ -353,6 +353,88 @@ def hull_pile_scene(hulls, n=40, seed=4, spread=1.5, others=True):
             sb.dynamic_sphere(pos, rng.uniform(0.2, 0.4), mass=1.5, friction=0.5)
         else:
             sb.dynamic_capsule(pos, 0.15, 0.3, axis=int(rng.integers(0, 3)), orientation=q, mass=1.5, friction=0.5)
+    return sb.arrays()
+
+
+# ---- triangle meshes (the meshes of physxcoopa/tests/narrowphase_test.cpp:200-490) ----
+
+def tiled_floor(n):
+    """n x n quads over [-2, 2]^2 at z = 0, two triangles each (narrowphase_test.cpp:314-327)."""
+    step = np.float32(4.0) / np.float32(n)
+    v = [(np.float32(-2.0) + np.float32(i) * step, np.float32(-2.0) + np.float32(j) * step, 0.0)
+         for j in range(n + 1) for i in range(n + 1)]
+    idx = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            idx += [a, a + 1, a + n + 2, a, a + n + 2, a + n + 1]
+    return TriangleMesh(v, idx)
+
+
+def two_triangle_floor():
+    """A 4 x 4 m floor of two triangles sharing the diagonal x == y."""
+    return TriangleMesh([(-2, -2, 0), (2, -2, 0), (2, 2, 0), (-2, 2, 0)], [0, 1, 2, 0, 2, 3])
+
+
+def ledge():
+    """A flat top (x in [-2, 0], z = 0, facing +z) and the wall under its edge (x = 0, facing +x),
+    wall triangles first (narrowphase_test.cpp:307-312)."""
+    return TriangleMesh([(-2, -2, 0), (0, -2, 0), (0, 2, 0), (-2, 2, 0), (0, -2, -2), (0, 2, -2)],
+                        [1, 4, 5, 1, 5, 2, 0, 1, 2, 0, 2, 3])
+
+
+def bumpy_terrain(cells=16, cell=0.5, amplitude=0.4):
+    """A cells x cells heightfield of z = amplitude sin(1.3 x) cos(0.9 y) in float32
+    (narrowphase_test.cpp:200-216), as a triangle soup."""
+    v = []
+    for j in range(cells + 1):
+        for i in range(cells + 1):
+            x, y = np.float32(i * cell), np.float32(j * cell)
+            v.append((x, y, np.float32(amplitude) * np.float32(math.sin(x * 1.3)) * np.float32(math.cos(y * 0.9))))
+    idx = []
+    for j in range(cells):
+        for i in range(cells):
+            a = j * (cells + 1) + i
+            b, c = a + 1, a + cells + 1
+            idx += [a, b, c + 1, a, c + 1, c]
+    return TriangleMesh(v, idx)
+
+
+def mesh_shape_on(sb, mesh_index, position=(0, 0, 0), orientation=(0, 0, 0, 1), scale=1.0, btype=PXG_STATIC,
+                  friction=0.6, local_center=(0, 0, 0), local_rotation=(0, 0, 0, 1)):
+    """A static (or kinematic) body carrying a PXG_TRIANGLE_MESH shape of table entry mesh_index."""
+    b = sb.add_body(btype, position, orientation, mass=1.0)
+    sb.add_shape(b, PXG_TRIANGLE_MESH, radius=scale, capsule_axis=mesh_index, friction=friction,
+                 local_center=local_center, local_rotation=local_rotation)
+    return b
+
+
+def terrain_pile_scene(hulls=None, n=48, seed=11, mesh_position=(-4.0, -4.0, 0.0), mesh_orientation=(0, 0, 0, 1)):
+    """Spheres, boxes, capsules and (given `hulls`) convex hulls dropped with random orientations
+    onto the bumpy terrain mesh (table entry 0; 8 x 8 m placed at mesh_position), for lockstep
+    tests of the mesh narrowphase."""
+    rng = np.random.default_rng(seed)
+    sb = SceneBuilder()
+    mesh_shape_on(sb, 0, position=mesh_position, orientation=mesh_orientation)
+    kinds = 4 if hulls else 3
+    for i in range(n):
+        pos = (rng.uniform(-2.5, 2.5), rng.uniform(-2.5, 2.5), 1.0 + 0.35 * i)
+        q = random_quaternion(rng)
+        kind = i % kinds
+        if kind == 0:
+            sb.dynamic_sphere(pos, rng.uniform(0.15, 0.35), mass=1.0, friction=0.5)
+        elif kind == 1:
+            he = tuple(rng.uniform(0.12, 0.35, 3))
+            sb.dynamic_box(pos, he, orientation=q, mass=1.5, friction=0.6)
+        elif kind == 2:
+            sb.dynamic_capsule(pos, rng.uniform(0.1, 0.2), rng.uniform(0.15, 0.35), axis=int(rng.integers(0, 3)),
+                               orientation=q, mass=1.2, friction=0.5)
+        else:
+            k = i % len(hulls)
+            h = hulls[k]
+            ext = (np.asarray(h.bounds6[3:], np.float64) - np.asarray(h.bounds6[:3], np.float64)) * 0.5
+            b = sb.add_body(PXG_DYNAMIC, position=pos, orientation=q, mass=2.0, inv_inertia=box_inverse_inertia(ext, 2.0))
+            sb.add_shape(b, PXG_CONVEX_HULL, radius=1.0, capsule_axis=k, friction=0.6)
     return sb.arrays()
 
 

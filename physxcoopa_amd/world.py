@@ -19,7 +19,7 @@ from ._lib import lib, check
 class World:
     """One GPU world handle built from ABI arrays (bodies, shapes)."""
 
-    def __init__(self, config, bodies, shapes, layer_masks=None, hulls=None):
+    def __init__(self, config, bodies, shapes, layer_masks=None, hulls=None, meshes=None):
         self._cfg = config
         h = ctypes.c_void_p()
         check(lib().pxg_create(ctypes.byref(config), ctypes.byref(h)), "pxg_create")
@@ -34,6 +34,8 @@ class World:
             check(lib().pxg_set_layer_masks(self.h, ptr(m)), "pxg_set_layer_masks")
         if hulls:
             self.set_hulls(hulls)
+        if meshes:
+            self.set_meshes(meshes)
         check(lib().pxg_set_shapes(self.h, ptr(s), self.n_shapes), "pxg_set_shapes")
 
     def close(self):
@@ -69,6 +71,13 @@ class World:
         tab = _abi.hull_table(hulls)
         self._hulls = list(hulls)  # the arrays the records point at stay alive through the call
         check(lib().pxg_set_hulls(self.h, ctypes.byref(tab), len(hulls)), "pxg_set_hulls")
+
+    def set_meshes(self, meshes):
+        """Cooks and uploads the triangle meshes (a list of _abi.TriangleMesh) that
+        PXG_TRIANGLE_MESH shapes index."""
+        tab = _abi.mesh_table(meshes)
+        self._meshes = list(meshes)
+        check(lib().pxg_set_meshes(self.h, ctypes.byref(tab), len(meshes)), "pxg_set_meshes")
 
     def set_joints(self, joints):
         """Replaces the joint table (pxg_joint records, see physxcoopa_amd.joints)."""
@@ -499,3 +508,22 @@ class PhysicsWorld:
 
     def stats(self):
         return self._world.stats() if self._world is not None else {}
+
+
+def cook_mesh(mesh):
+    """pxg_cook_mesh of an _abi.TriangleMesh on the host: a dict of the welded vertices (n, 3),
+    indices (t, 3), unit face normals (t, 3), adjacency (t, 3; 0xFFFFFFFF for none), bounds (6)
+    and the non-manifold edge count (TriangleMesh::from_triangles, triangle_mesh.h:75-170)."""
+    d = mesh.desc()
+    ni = len(mesh.indices)
+    info = _abi.MeshInfo()
+    v = np.zeros((max(len(mesh.positions), 1), 3), np.float32)
+    idx = np.zeros(max(ni, 3), np.uint32)
+    nrm = np.zeros(max(ni, 3), np.float32)
+    adj = np.zeros(max(ni, 3), np.uint32)
+    check(lib().pxg_cook_mesh(ctypes.byref(d), ctypes.byref(info), ptr(v), ptr(idx), ptr(nrm), ptr(adj)), "pxg_cook_mesh")
+    nv, nt = info.n_vertices, info.n_triangles
+    return {"vertices": v[:nv].copy(), "indices": idx[:3 * nt].reshape(-1, 3).copy(),
+            "normals": nrm[:3 * nt].reshape(-1, 3).copy(), "adjacency": adj[:3 * nt].reshape(-1, 3).copy(),
+            "bounds": np.array(list(info.local_min) + list(info.local_max), np.float32),
+            "non_manifold_edges": info.non_manifold_edges, "n_nodes": info.n_nodes}

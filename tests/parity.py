@@ -93,13 +93,18 @@ def compare_step(gw, ow, pre_bodies, report, check_events=True):
 
 
 def lockstep(World, OracleWorld, cfg, bodies, shapes, steps, resync=True, feed_order=True, joints=None, ignored=None,
-             rounded_atan2=True, hulls=None):
+             rounded_atan2=True, hulls=None, meshes=None):
     """feed_order: give the oracle the GPU's solve order each step; without it the oracle solves in
     its own (the reference's) manifold order. joints / ignored: a joint table and ignored body
     pairs for both sides. rounded_atan2: the oracle evaluates atan2 in double and rounds once, as
     the GPU does (False: the reference's float std::atan2, one ulp apart at most)."""
-    gw = World(cfg, bodies, shapes, hulls=hulls) if hulls else World(cfg, bodies, shapes)
-    ow = OracleWorld(cfg, bodies, shapes, hulls=hulls) if hulls else OracleWorld(cfg, bodies, shapes)
+    extra = {}
+    if hulls:
+        extra["hulls"] = hulls
+    if meshes:
+        extra["meshes"] = meshes
+    gw = World(cfg, bodies, shapes, **extra)
+    ow = OracleWorld(cfg, bodies, shapes, **extra)
     ow.set_atan2_mode(rounded_atan2)
     if joints is not None:
         gw.set_joints(joints)
