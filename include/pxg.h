@@ -100,7 +100,11 @@ typedef struct pxg_config {
      * pass (and after the last one the cap allows); the passes between run without the residual
      * sweep. 0 and 1 both mean every pass. */
     uint32_t residual_check_interval; /* 1 */
-    uint32_t reserved[5];
+    /* Cloth and hair (util/config.h:139-144): XPBD substeps per 1/60 s and constraint iterations
+     * per substep, unless a cloth or groom overrides them. */
+    uint32_t cloth_substeps;          /* 4 */
+    uint32_t cloth_iterations;        /* 1 */
+    uint32_t reserved[3];
 } pxg_config;
 
 enum pxg_solve_order { PXG_ORDER_REFERENCE = 0, PXG_ORDER_COLOURED = 1 };
@@ -358,6 +362,146 @@ typedef struct pxg_mesh_info {
     float local_max[3];
 } pxg_mesh_info;
 
+/* --- cloth and hair (XPBD, stepped once per frame by pxg_step: world.h:1241-1256) --- */
+/* cloth::ClothParticle (cloth/cloth.h:52-57), also a strand particle. */
+typedef struct pxg_particle {
+    float position[3];
+    float inv_mass;                /* 0 = pinned */
+    float prev_position[3];
+    float reserved0;
+    float velocity[3];
+    float reserved1;
+} pxg_particle;
+
+/* cloth::ClothConstraint (cloth/cloth.h:61-66): an XPBD distance constraint. */
+typedef struct pxg_distance_constraint {
+    uint32_t a, b;
+    float rest_length;
+    float lambda;                  /* the last substep's multiplier (reset every substep) */
+} pxg_distance_constraint;
+
+/* cloth::ClothTether (cloth/cloth.h:75-79). */
+typedef struct pxg_cloth_tether {
+    uint32_t particle, anchor;
+    float max_length;
+    uint32_t reserved;
+} pxg_cloth_tether;
+
+/* cloth::ClothAnchor (cloth/cloth.h:89-106): a pinned particle following a body, or a static pin. */
+typedef struct pxg_cloth_anchor {
+    uint32_t particle;
+    uint32_t has_body;             /* 0: a static pin at world_position */
+    uint32_t body, generation;     /* the body followed (dynamics::BodyId); stale = static pin */
+    float local_position[3];
+    float world_position[3];
+    float prev_world_position[3];
+    float reserved;
+} pxg_cloth_anchor;
+
+/* cloth::ClothParams (cloth/cloth.h:120-226). */
+typedef struct pxg_cloth_params {
+    float stretch_compliance;      /* 0 */
+    float bend_compliance;         /* 5e-5 */
+    float damping;                 /* 0.05 */
+    float thickness;               /* 0.03 */
+    float friction;                /* 0.5 */
+    float gravity_scale;           /* 1 */
+    float max_velocity;            /* 100 */
+    float external_acceleration[3];
+    float wind[3];
+    float wind_turbulence;         /* 0 */
+    float air_drag;                /* 0.02 */
+    float air_lift;                /* 0.02 */
+    uint32_t self_collision;       /* 0 */
+    float self_distance;           /* 0: the builder's 0.6 x the particle spacing */
+    float tether_scale;            /* 1.02 */
+    uint32_t substeps;             /* 0: the world's */
+    uint32_t iterations;           /* 0: the world's */
+    float sleep_threshold;         /* 0.05 */
+    float sleep_time;              /* 0.5 */
+    uint32_t layer;
+} pxg_cloth_params;
+
+/* cloth::Cloth (cloth/cloth.h:249-315) over caller-owned arrays: particles, stretch and bend
+ * constraints with their particle-disjoint batches ([begin, end) pairs), tethers, anchors, the
+ * triangles aerodynamics uses, the grid size (columns 0: not a grid; self-collision skips grid
+ * neighbours by it) and the sleep state. alive 0 is an empty slot. */
+typedef struct pxg_cloth {
+    pxg_particle* particles;
+    uint32_t n_particles;
+    uint32_t columns, rows;
+    pxg_distance_constraint* stretch;
+    uint32_t n_stretch;
+    pxg_distance_constraint* bend;
+    uint32_t n_bend;
+    const uint32_t* stretch_batches;
+    uint32_t n_stretch_batches;
+    const uint32_t* bend_batches;
+    uint32_t n_bend_batches;
+    const pxg_cloth_tether* tethers;
+    uint32_t n_tethers;
+    pxg_cloth_anchor* anchors;
+    uint32_t n_anchors;
+    const uint32_t* triangles;
+    uint32_t n_triangle_indices;
+    pxg_cloth_params params;
+    uint32_t alive, enabled, awake;
+    float sleep_timer, time;
+    float bounds_min[3], bounds_max[3];
+} pxg_cloth;
+
+/* hair::Strand (hair/strand.h:40-72): a chain of particles from a driven root. */
+typedef struct pxg_strand {
+    uint32_t first, count;
+    float root_position[3];
+    float root_orientation[4];     /* x, y, z, w */
+    float prev_root_position[3];
+    float prev_root_orientation[4];
+} pxg_strand;
+
+/* hair::StrandParams (hair/strand.h:81-180). */
+typedef struct pxg_strand_params {
+    float stretch_compliance;      /* 0 */
+    float global_shape;            /* 0.22 */
+    float global_shape_falloff;    /* 1.6 */
+    float local_shape;             /* 0.35 */
+    float damping;                 /* 0.08 */
+    float thickness;               /* 0.01 */
+    float friction;                /* 0.4 */
+    float gravity_scale;           /* 1 */
+    float max_velocity;            /* 50 */
+    float external_acceleration[3];
+    float wind[3];
+    float wind_turbulence;         /* 0.1 */
+    float air_drag;                /* 0.6 */
+    uint32_t substeps;             /* 0: the world's */
+    uint32_t iterations;           /* 0: the world's */
+    float sleep_threshold;         /* 0.02 */
+    float sleep_time;              /* 0.6 */
+    uint32_t layer;
+} pxg_strand_params;
+
+/* hair::StrandSet (hair/strand.h:187-240) over caller-owned arrays. collide_bodies: (index,
+ * generation) pairs; when given, the only bodies the groom collides with. alive 0 is an empty slot. */
+typedef struct pxg_strand_set {
+    pxg_particle* particles;
+    uint32_t n_particles;
+    pxg_distance_constraint* stretch;
+    uint32_t n_stretch;
+    const uint32_t* stretch_batches;
+    uint32_t n_stretch_batches;
+    const float* rest_local;       /* 3 per particle: rest position in its strand's root frame */
+    const float* rest_offset;      /* 3 per particle: rest offset from the previous particle */
+    pxg_strand* strands;
+    uint32_t n_strands;
+    const uint32_t* collide_bodies;
+    uint32_t n_collide_bodies;
+    pxg_strand_params params;
+    uint32_t alive, enabled, awake;
+    float sleep_timer, time;
+    float bounds_min[3], bounds_max[3];
+} pxg_strand_set;
+
 /* --- state upload / download (caller-owned host buffers are copied) --- */
 /* Replaces the body table (add_body/remove_body, world.h:95-143). World inverse inertia is
  * recomputed from orientation exactly as add_body() does. */
@@ -406,6 +550,24 @@ int pxg_cook_mesh(const pxg_mesh_desc* mesh, pxg_mesh_info* info, float* vertice
  * mesh's triangles, internal-edge correction, embedded recovery; no mesh against mesh, no
  * speculative contacts). A mesh that cooks to no triangle is refused (PXG_ERR_INVALID). */
 int pxg_set_meshes(pxg_world* w, const pxg_mesh_desc* meshes, uint32_t n);
+/* Replaces the cloth table (add_cloth / remove_cloth / get_cloth, world.h:552-620): slot i is
+ * cloths[i]; a slot with alive 0 or no particle is empty. Cloth is advanced by pxg_step once per
+ * frame after the rigid substeps (step_cloths_, world.h:1749-1847), collides with the world's
+ * shapes one-way and is part of pxg_state_hash. Batches must be particle-disjoint ranges of their
+ * constraint arrays and every index in range (PXG_ERR_INVALID otherwise); at most 8192 particles
+ * per cloth. */
+int pxg_set_cloths(pxg_world* w, const pxg_cloth* cloths, uint32_t n);
+/* Reads cloth slot `index` back into out's arrays (which must have the uploaded sizes; any may
+ * be null): particles, constraints' multipliers, anchors' world positions, and the sleep state,
+ * time and bounds. */
+int pxg_get_cloth(pxg_world* w, uint32_t index, pxg_cloth* out);
+/* Replaces the hair table (add_strands / remove_strands / get_strands, world.h:623-700), stepped
+ * by pxg_step after the cloths (step_strands_, world.h:1861-1957). Not hashed (world.h:1334-1359). */
+int pxg_set_strand_sets(pxg_world* w, const pxg_strand_set* sets, uint32_t n);
+int pxg_get_strand_set(pxg_world* w, uint32_t index, pxg_strand_set* out);
+/* hair::set_strand_roots (hair/strand_builder.h:143-153) on set `index`: the first n strands'
+ * roots move to positions (3 floats each) and, if given, orientations (x, y, z, w). */
+int pxg_set_strand_roots(pxg_world* w, uint32_t index, const float* positions, const float* orientations, uint32_t n);
 /* 32 x 32 layer matrix rows (broadphase/layer_matrix.h:21-47). */
 int pxg_set_layer_masks(pxg_world* w, const uint32_t masks[32]);
 /* Replaces the joint table: the joints_ array behind add_hinge_joint / add_ball_joint /

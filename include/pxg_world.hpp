@@ -720,6 +720,33 @@ public:
         return s;
     }
 
+    // The convex hulls and triangle meshes that shapes index (Shape::hull / Shape::mesh,
+    // collision/shape.h:84-98): PXG_CONVEX_HULL / PXG_TRIANGLE_MESH shapes name a table entry in
+    // capsule_axis and their scale in radius. The arrays are copied; the tables go up before the
+    // next shape upload (and again whenever the world is re-created to grow).
+    void set_hulls(const pxg_hull* hulls, uint32_t n) {
+        hull_arrays_.assign(n, HullArrays{});
+        for (uint32_t k = 0; k < n; ++k) {
+            const pxg_hull& h = hulls[k];
+            HullArrays& a = hull_arrays_[k];
+            a.desc = h;
+            a.v.assign(h.vertices, h.vertices + 3 * size_t(h.n_vertices));
+            a.planes.assign(h.face_planes, h.face_planes + 4 * size_t(h.n_faces));
+            a.starts.assign(h.face_starts, h.face_starts + h.n_faces + 1);
+            a.fv.assign(h.face_vertices, h.face_vertices + (h.n_faces ? h.face_starts[h.n_faces] : 0u));
+            if (h.n_edges) a.edges.assign(h.edges, h.edges + 2 * size_t(h.n_edges));
+        }
+        geometry_dirty_ = structure_dirty_ = true;
+    }
+    void set_meshes(const pxg_mesh_desc* meshes, uint32_t n) {
+        mesh_arrays_.assign(n, MeshArrays{});
+        for (uint32_t k = 0; k < n; ++k) {
+            mesh_arrays_[k].positions.assign(meshes[k].positions, meshes[k].positions + 3 * size_t(meshes[k].n_positions));
+            mesh_arrays_[k].indices.assign(meshes[k].indices, meshes[k].indices + meshes[k].n_indices);
+        }
+        geometry_dirty_ = structure_dirty_ = true;
+    }
+
     pxg_world* handle() { return w_; }
 
 private:
@@ -742,6 +769,7 @@ private:
         cfg_ = c;
         applied_cfg_ = c;
         check(pxg_set_layer_masks(w_, layers_.rows()), "layers");
+        geometry_dirty_ = !hull_arrays_.empty() || !mesh_arrays_.empty();
         joints_dirty_ = !joints_.empty();
         ignored_dirty_ = !ignored_.empty();
         pending_cache_keys_ = std::move(keys);
@@ -962,6 +990,11 @@ private:
             bodies_uploaded_ = n;
             on_device_ = bodies_;
             poses_valid_.store(false, std::memory_order_release);
+            if (geometry_dirty_) {
+                // the old shapes may index past the new tables: clear them, then the tables, then the shapes
+                check(pxg_set_shapes(w_, shapes_.data(), 0), "shapes");
+                push_geometry();
+            }
             check(pxg_set_shapes(w_, shapes_.data(), n), "shapes");
             std::vector<float> now(size_t(n) * 6);
             check(pxg_get_fat_aabbs(w_, now.data(), n), "fat AABBs");
@@ -1016,6 +1049,34 @@ private:
     std::vector<uint8_t> joint_alive_;
     std::vector<std::pair<uint32_t, uint32_t>> ignored_;
     bool joints_dirty_ = false, ignored_dirty_ = false, joints_on_device_ = false;
+    struct HullArrays {
+        pxg_hull desc;
+        std::vector<float> v, planes;
+        std::vector<uint32_t> starts, fv, edges;
+    };
+    struct MeshArrays {
+        std::vector<float> positions;
+        std::vector<uint32_t> indices;
+    };
+    std::vector<HullArrays> hull_arrays_;
+    std::vector<MeshArrays> mesh_arrays_;
+    bool geometry_dirty_ = false;
+    void push_geometry() {
+        std::vector<pxg_hull> hs;
+        for (HullArrays& a : hull_arrays_) {
+            pxg_hull h = a.desc;
+            h.vertices = a.v.data(); h.face_planes = a.planes.data(); h.face_starts = a.starts.data();
+            h.face_vertices = a.fv.data(); h.edges = a.edges.empty() ? nullptr : a.edges.data();
+            hs.push_back(h);
+        }
+        std::vector<pxg_mesh_desc> ms;
+        for (MeshArrays& a : mesh_arrays_)
+            ms.push_back(pxg_mesh_desc{a.positions.data(), static_cast<uint32_t>(a.positions.size() / 3),
+                                       static_cast<uint32_t>(a.indices.size()), a.indices.data()});
+        check(pxg_set_hulls(w_, hs.data(), static_cast<uint32_t>(hs.size())), "hulls");
+        check(pxg_set_meshes(w_, ms.data(), static_cast<uint32_t>(ms.size())), "meshes");
+        geometry_dirty_ = false;
+    }
     float accumulator_ = 0.0f;
     mutable std::vector<pxg_manifold> manifolds_;
     mutable bool manifolds_valid_ = false;

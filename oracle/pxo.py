@@ -10,6 +10,7 @@ import subprocess
 import numpy as np
 
 from physxcoopa_amd._abi import BODY_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, Hull, MeshInfo, hull_table, mesh_table, ptr
+from physxcoopa_amd import _abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "_build", "libpxo.so")
@@ -63,6 +64,15 @@ def lib():
             "pxo_shape_cast": ([vp, vp, vp, vp], i32), "pxo_ray_aabb": ([vp, vp, vp], i32),
             "pxo_compute_penetration": ([vp, vp, vp, vp, vp, vp, i32], i32),
             "pxo_set_meshes": ([vp, vp, u32], i32), "pxo_set_kat_meshes": ([vp, u32], i32),
+            "pxo_set_cloths": ([vp, vp, u32], None), "pxo_get_cloth": ([vp, u32, vp], None),
+            "pxo_set_strand_sets": ([vp, vp, u32], None), "pxo_get_strand_set": ([vp, u32, vp], None),
+            "pxo_set_strand_roots": ([vp, u32, vp, vp, u32], None), "pxo_step_frame": ([vp, f32], u32),
+            "pxo_interpolation_alpha": ([vp], f32), "pxo_set_collider_order": ([vp, u32], None),
+            "pxo_cloth_grid": ([vp, vp, vp], vp), "pxo_cloth_pin": ([vp, vp, f32, vp, vp], u32),
+            "pxo_cloth_tethers": ([vp], None), "pxo_cloth_sizes": ([vp, vp], None),
+            "pxo_cloth_export": ([vp, vp, vp], None), "pxo_cloth_free": ([vp], None),
+            "pxo_strands_make": ([vp, vp, u32, vp, vp, vp], vp), "pxo_strands_sizes": ([vp, vp], None),
+            "pxo_strands_export": ([vp, vp], None), "pxo_strands_free": ([vp], None),
             "pxo_cook_mesh": ([vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
             "pxo_mesh_contacts": ([vp, vp, vp, vp, vp, vp, i32, vp], i32),
             "pxo_mesh_candidates": ([vp, vp, vp, vp, vp, vp, vp, u32], u32),
@@ -86,7 +96,8 @@ def f32(x, n=None):
 class OracleWorld:
     """The CPU reference step over pxg ABI arrays. dbl=True runs the same algorithm in float64."""
 
-    def __init__(self, config, bodies, shapes, dbl=False, brute_force=False, layer_masks=None, hulls=None, meshes=None):
+    def __init__(self, config, bodies, shapes, dbl=False, brute_force=False, layer_masks=None, hulls=None, meshes=None,
+                 cloths=None, strands=None):
         L = lib()
         self._cfg = config
         self.h = L.pxo_create(ctypes.byref(config), 1 if dbl else 0)
@@ -105,6 +116,10 @@ class OracleWorld:
             tab = mesh_table(meshes)
             assert L.pxo_set_meshes(self.h, ctypes.byref(tab), len(meshes)) == 1, "mesh index out of range"
         L.pxo_set_shapes(self.h, ptr(s), self.n_shapes)
+        if cloths:
+            self.set_cloths(cloths)
+        if strands:
+            self.set_strand_sets(strands)
         if brute_force:
             L.pxo_set_brute_force(self.h, 1)
 
@@ -114,6 +129,48 @@ class OracleWorld:
                 lib().pxo_destroy(self.h)
         except Exception:
             pass
+
+    def set_cloths(self, cloths):
+        """Replaces the cloth table (a list of _abi.Cloth)."""
+        self._cloths = [c.copy() for c in cloths]
+        tab = _abi.cloth_table(self._cloths)
+        lib().pxo_set_cloths(self.h, ctypes.byref(tab), len(cloths))
+
+    def cloth(self, index):
+        """Cloth slot `index` as an _abi.Cloth (particles, multipliers, anchors, sleep state)."""
+        c = self._cloths[index].copy()
+        d = c.desc()
+        lib().pxo_get_cloth(self.h, index, ctypes.byref(d))
+        c.take_state(d)
+        return c
+
+    def set_strand_sets(self, sets):
+        self._strands = [x.copy() for x in sets]
+        tab = _abi.strand_table(self._strands)
+        lib().pxo_set_strand_sets(self.h, ctypes.byref(tab), len(sets))
+
+    def strand_set(self, index):
+        x = self._strands[index].copy()
+        d = x.desc()
+        lib().pxo_get_strand_set(self.h, index, ctypes.byref(d))
+        x.take_state(d)
+        return x
+
+    def set_strand_roots(self, index, positions, orientations=None):
+        p = f32(np.asarray(positions, np.float32).reshape(-1))
+        q = f32(np.asarray(orientations, np.float32).reshape(-1)) if orientations is not None else None
+        lib().pxo_set_strand_roots(self.h, index, ptr(p), ptr(q) if q is not None else None, len(p) // 3)
+
+    def step(self, dt):
+        """step(dt) (world.h:1229-1257): fixed substeps, then cloth and hair once. Returns the substeps."""
+        return lib().pxo_step_frame(self.h, np.float32(dt))
+
+    def interpolation_alpha(self):
+        return lib().pxo_interpolation_alpha(self.h)
+
+    def set_collider_order(self, tree):
+        """True: cloth and hair meet colliders in the reference's tree order; False: in slot order."""
+        lib().pxo_set_collider_order(self.h, 1 if tree else 0)
 
     def set_threads(self, n):
         """Host threads for the reference's job-parallel stages (bounds, pairs, narrowphase); 1 = serial."""
@@ -507,3 +564,66 @@ def closest_on_triangle(p, a, b, c):
     out, bary = np.zeros(3, np.float32), np.zeros(3, np.float32)
     lib().pxo_closest_on_triangle(ptr(k[0]), ptr(k[1]), ptr(k[2]), ptr(k[3]), ptr(out), ptr(bary))
     return out, bary
+
+
+# ---- cloth and hair builders (cloth/cloth_builder.h, hair/strand_builder.h) ----
+
+def grid_cloth(columns=21, rows=21, width=2.0, height=2.0, center=(0, 0, 0), orientation=(0, 0, 0, 1),
+               total_mass=1.0, shear=True, params=None, pins=(), tethers=False):
+    """make_grid_cloth (cloth_builder.h:103-239), then each pin in order -- (point, radius) for
+    pin_static, (point, radius, (body, generation), (position, orientation)) for pin_to_body --
+    then build_tethers when asked. An _abi.Cloth."""
+    params = params if params is not None else _abi.ClothParams()
+    fd = f32([width, height] + list(center) + list(orientation) + [total_mass])
+    ui = np.array([columns, rows, 1 if shear else 0], np.uint32)
+    b = lib().pxo_cloth_grid(ptr(fd), ptr(ui), ctypes.byref(params))
+    try:
+        for pin in pins:
+            pt = f32(pin[0], 3)
+            if len(pin) == 2:
+                lib().pxo_cloth_pin(b, ptr(pt), np.float32(pin[1]), None, None)
+            else:
+                body = np.array(pin[2], np.uint32)
+                pose = f32(list(pin[3][0]) + list(pin[3][1]), 7)
+                lib().pxo_cloth_pin(b, ptr(pt), np.float32(pin[1]), ptr(body), ptr(pose))
+        if tethers:
+            lib().pxo_cloth_tethers(b)
+        sz = np.zeros(8, np.uint32)
+        lib().pxo_cloth_sizes(b, ptr(sz))
+        c = _abi.Cloth(np.zeros(sz[0], _abi.PARTICLE_DTYPE), np.zeros(sz[1], _abi.CONSTRAINT_DTYPE),
+                       np.zeros(sz[2], _abi.CONSTRAINT_DTYPE), np.zeros((sz[3], 2), np.uint32),
+                       np.zeros((sz[4], 2), np.uint32), np.zeros(sz[5], _abi.TETHER_DTYPE),
+                       np.zeros(sz[6], _abi.ANCHOR_DTYPE), np.zeros(sz[7], np.uint32))
+        batches = np.zeros(2 * (int(sz[3]) + int(sz[4])) + 2, np.uint32)
+        d = c.desc()
+        lib().pxo_cloth_export(b, ctypes.byref(d), ptr(batches))
+        c.stretch_batches[:] = batches[:2 * sz[3]].reshape(-1, 2)
+        c.bend_batches[:] = batches[2 * sz[3]:2 * (sz[3] + sz[4])].reshape(-1, 2)
+        c.columns, c.rows, c.params = d.columns, d.rows, d.params
+        c.bounds = np.array(list(d.bounds_min) + list(d.bounds_max), np.float32)
+        return c
+    finally:
+        lib().pxo_cloth_free(b)
+
+
+def make_strands(strands, params=None, masses=None, root_orientations=None):
+    """make_strands (strand_builder.h:66-130) of a list of point lists. An _abi.StrandSet."""
+    params = params if params is not None else _abi.StrandParams()
+    counts = np.array([len(s) for s in strands], np.uint32)
+    pts = f32(np.concatenate([np.asarray(s, np.float32).reshape(-1, 3) for s in strands]).reshape(-1))
+    rq = f32(np.asarray(root_orientations if root_orientations is not None else [(0, 0, 0, 1)] * len(strands),
+                        np.float32).reshape(-1))
+    ms = f32(np.asarray(masses if masses is not None else [0.001] * len(strands), np.float32))
+    b = lib().pxo_strands_make(ptr(pts), ptr(counts), len(strands), ptr(rq), ptr(ms), ctypes.byref(params))
+    try:
+        sz = np.zeros(4, np.uint32)
+        lib().pxo_strands_sizes(b, ptr(sz))
+        x = _abi.StrandSet(np.zeros(sz[0], _abi.PARTICLE_DTYPE), np.zeros(sz[1], _abi.CONSTRAINT_DTYPE),
+                           np.zeros((sz[2], 2), np.uint32), np.zeros((sz[0], 3), np.float32),
+                           np.zeros((sz[0], 3), np.float32), np.zeros(sz[3], _abi.STRAND_DTYPE), params=params)
+        d = x.desc()
+        lib().pxo_strands_export(b, ctypes.byref(d))
+        x.bounds = np.array(list(d.bounds_min) + list(d.bounds_max), np.float32)
+        return x
+    finally:
+        lib().pxo_strands_free(b)

@@ -21,6 +21,7 @@ template <class T> void cfg_from(Config<T>& c, const pxg_config* in) {
     c.gravity = V3<T>(T(in->gravity[0]), T(in->gravity[1]), T(in->gravity[2]));
     c.residual_mode = in->residual_mode != 0; c.residual_tol = T(in->residual_tolerance); c.residual_max = in->residual_max_iterations;
     c.residual_every = in->residual_check_interval ? in->residual_check_interval : 1u;
+    c.cloth_substeps = in->cloth_substeps; c.cloth_iterations = in->cloth_iterations;
 }
 template <class T> V3<T> v3(const float* f) { return V3<T>(T(f[0]), T(f[1]), T(f[2])); }
 template <class T> Q<T> q4(const float* f) { Q<T> q; q.x = T(f[0]); q.y = T(f[1]); q.z = T(f[2]); q.w = T(f[3]); return q; }
@@ -770,5 +771,282 @@ float pxo_cone_swing(const float* ta, const float* tb, float* axis, uint32_t mod
     return a;
 }
 float pxo_atan2(float y, float x, uint32_t mode) { return atan2_of(y, x, mode ? Atan2Mode::Rounded : Atan2Mode::Libm); }
+
+
+// ---- cloth and hair ----
+}  // extern "C"
+namespace {
+template <class T> CParticle<T> particle_from(const pxg_particle& d) {
+    CParticle<T> p;
+    p.pos = v3<T>(d.position); p.prev = v3<T>(d.prev_position); p.vel = v3<T>(d.velocity); p.inv_mass = T(d.inv_mass);
+    return p;
+}
+template <class T> void particle_to(const CParticle<T>& p, pxg_particle& d) {
+    std::memset(&d, 0, sizeof(d));
+    put3(d.position, p.pos); put3(d.prev_position, p.prev); put3(d.velocity, p.vel); d.inv_mass = float(p.inv_mass);
+}
+template <class T> std::vector<CConstraint<T>> constraints_from(const pxg_distance_constraint* c, uint32_t n) {
+    std::vector<CConstraint<T>> out(n);
+    for (uint32_t i = 0; i < n; ++i) { out[i].a = c[i].a; out[i].b = c[i].b; out[i].rest = T(c[i].rest_length); out[i].lambda = T(c[i].lambda); }
+    return out;
+}
+std::vector<CBatch> batches_from(const uint32_t* b, uint32_t n) {
+    std::vector<CBatch> out(n);
+    for (uint32_t i = 0; i < n; ++i) out[i] = CBatch{b[2 * i], b[2 * i + 1]};
+    return out;
+}
+template <class T> ClothParams<T> cloth_params_from(const pxg_cloth_params& p) {
+    ClothParams<T> o;
+    o.stretch_compliance = T(p.stretch_compliance); o.bend_compliance = T(p.bend_compliance); o.damping = T(p.damping);
+    o.thickness = T(p.thickness); o.friction = T(p.friction); o.gravity_scale = T(p.gravity_scale);
+    o.max_velocity = T(p.max_velocity); o.external_acceleration = v3<T>(p.external_acceleration); o.wind = v3<T>(p.wind);
+    o.wind_turbulence = T(p.wind_turbulence); o.air_drag = T(p.air_drag); o.air_lift = T(p.air_lift);
+    o.self_collision = p.self_collision != 0; o.self_distance = T(p.self_distance); o.tether_scale = T(p.tether_scale);
+    o.substeps = p.substeps; o.iterations = p.iterations; o.sleep_threshold = T(p.sleep_threshold);
+    o.sleep_time = T(p.sleep_time); o.layer = p.layer;
+    return o;
+}
+template <class T> void cloth_params_to(const ClothParams<T>& o, pxg_cloth_params& p) {
+    p.stretch_compliance = float(o.stretch_compliance); p.bend_compliance = float(o.bend_compliance); p.damping = float(o.damping);
+    p.thickness = float(o.thickness); p.friction = float(o.friction); p.gravity_scale = float(o.gravity_scale);
+    p.max_velocity = float(o.max_velocity); put3(p.external_acceleration, o.external_acceleration); put3(p.wind, o.wind);
+    p.wind_turbulence = float(o.wind_turbulence); p.air_drag = float(o.air_drag); p.air_lift = float(o.air_lift);
+    p.self_collision = o.self_collision; p.self_distance = float(o.self_distance); p.tether_scale = float(o.tether_scale);
+    p.substeps = o.substeps; p.iterations = o.iterations; p.sleep_threshold = float(o.sleep_threshold);
+    p.sleep_time = float(o.sleep_time); p.layer = o.layer;
+}
+template <class T> Cloth<T> cloth_from(const pxg_cloth& d) {
+    Cloth<T> c;
+    for (uint32_t i = 0; i < d.n_particles; ++i) c.particles.push_back(particle_from<T>(d.particles[i]));
+    c.columns = d.columns; c.rows = d.rows;
+    c.stretch = constraints_from<T>(d.stretch, d.n_stretch);
+    c.bend = constraints_from<T>(d.bend, d.n_bend);
+    c.stretch_batches = batches_from(d.stretch_batches, d.n_stretch_batches);
+    c.bend_batches = batches_from(d.bend_batches, d.n_bend_batches);
+    for (uint32_t i = 0; i < d.n_tethers; ++i) c.tethers.push_back(CTether<T>{d.tethers[i].particle, d.tethers[i].anchor, T(d.tethers[i].max_length)});
+    for (uint32_t i = 0; i < d.n_anchors; ++i) {
+        const pxg_cloth_anchor& a = d.anchors[i];
+        CAnchor<T> o;
+        o.particle = a.particle; o.has_body = a.has_body != 0; o.body = a.body; o.generation = a.generation;
+        o.local = v3<T>(a.local_position); o.world = v3<T>(a.world_position); o.prev_world = v3<T>(a.prev_world_position);
+        c.anchors.push_back(o);
+    }
+    c.triangles.assign(d.triangles, d.triangles + d.n_triangle_indices);
+    c.params = cloth_params_from<T>(d.params);
+    c.enabled = d.enabled != 0; c.awake = d.awake != 0; c.sleep_timer = T(d.sleep_timer); c.time = T(d.time);
+    c.bounds.mn = v3<T>(d.bounds_min); c.bounds.mx = v3<T>(d.bounds_max);
+    return c;
+}
+// writes into the arrays of `d` (sizes must match what was uploaded)
+template <class T> void cloth_to(const Cloth<T>& c, pxg_cloth& d) {
+    for (size_t i = 0; d.particles && i < c.particles.size() && i < d.n_particles; ++i) particle_to(c.particles[i], d.particles[i]);
+    for (size_t i = 0; d.stretch && i < c.stretch.size() && i < d.n_stretch; ++i) d.stretch[i].lambda = float(c.stretch[i].lambda);
+    for (size_t i = 0; d.bend && i < c.bend.size() && i < d.n_bend; ++i) d.bend[i].lambda = float(c.bend[i].lambda);
+    for (size_t i = 0; d.anchors && i < c.anchors.size() && i < d.n_anchors; ++i) {
+        put3(d.anchors[i].world_position, c.anchors[i].world);
+        put3(d.anchors[i].prev_world_position, c.anchors[i].prev_world);
+    }
+    d.awake = c.awake; d.enabled = c.enabled; d.sleep_timer = float(c.sleep_timer); d.time = float(c.time);
+    put3(d.bounds_min, c.bounds.mn); put3(d.bounds_max, c.bounds.mx);
+}
+template <class T> StrandParams<T> strand_params_from(const pxg_strand_params& p) {
+    StrandParams<T> o;
+    o.stretch_compliance = T(p.stretch_compliance); o.global_shape = T(p.global_shape);
+    o.global_shape_falloff = T(p.global_shape_falloff); o.local_shape = T(p.local_shape); o.damping = T(p.damping);
+    o.thickness = T(p.thickness); o.friction = T(p.friction); o.gravity_scale = T(p.gravity_scale);
+    o.max_velocity = T(p.max_velocity); o.external_acceleration = v3<T>(p.external_acceleration); o.wind = v3<T>(p.wind);
+    o.wind_turbulence = T(p.wind_turbulence); o.air_drag = T(p.air_drag); o.substeps = p.substeps;
+    o.iterations = p.iterations; o.sleep_threshold = T(p.sleep_threshold); o.sleep_time = T(p.sleep_time); o.layer = p.layer;
+    return o;
+}
+template <class T> StrandSet<T> strands_from(const pxg_strand_set& d) {
+    StrandSet<T> s;
+    for (uint32_t i = 0; i < d.n_particles; ++i) {
+        s.particles.push_back(particle_from<T>(d.particles[i]));
+        s.rest_local.push_back(v3<T>(d.rest_local + 3 * i));
+        s.rest_offset.push_back(v3<T>(d.rest_offset + 3 * i));
+    }
+    s.stretch = constraints_from<T>(d.stretch, d.n_stretch);
+    s.stretch_batches = batches_from(d.stretch_batches, d.n_stretch_batches);
+    for (uint32_t i = 0; i < d.n_strands; ++i) {
+        const pxg_strand& x = d.strands[i];
+        Strand<T> st;
+        st.first = x.first; st.count = x.count; st.root = v3<T>(x.root_position); st.root_q = q4<T>(x.root_orientation);
+        st.prev_root = v3<T>(x.prev_root_position); st.prev_root_q = q4<T>(x.prev_root_orientation);
+        s.strands.push_back(st);
+    }
+    for (uint32_t i = 0; i < d.n_collide_bodies; ++i) s.collide_bodies.push_back({d.collide_bodies[2 * i], d.collide_bodies[2 * i + 1]});
+    s.params = strand_params_from<T>(d.params);
+    s.enabled = d.enabled != 0; s.awake = d.awake != 0; s.sleep_timer = T(d.sleep_timer); s.time = T(d.time);
+    s.bounds.mn = v3<T>(d.bounds_min); s.bounds.mx = v3<T>(d.bounds_max);
+    return s;
+}
+template <class T> void strands_to(const StrandSet<T>& s, pxg_strand_set& d) {
+    for (size_t i = 0; d.particles && i < s.particles.size() && i < d.n_particles; ++i) particle_to(s.particles[i], d.particles[i]);
+    for (size_t i = 0; d.stretch && i < s.stretch.size() && i < d.n_stretch; ++i) d.stretch[i].lambda = float(s.stretch[i].lambda);
+    for (size_t i = 0; d.strands && i < s.strands.size() && i < d.n_strands; ++i) {
+        put3(d.strands[i].root_position, s.strands[i].root); put4(d.strands[i].root_orientation, s.strands[i].root_q);
+        put3(d.strands[i].prev_root_position, s.strands[i].prev_root); put4(d.strands[i].prev_root_orientation, s.strands[i].prev_root_q);
+    }
+    d.awake = s.awake; d.enabled = s.enabled; d.sleep_timer = float(s.sleep_timer); d.time = float(s.time);
+    put3(d.bounds_min, s.bounds.mn); put3(d.bounds_max, s.bounds.mx);
+}
+}  // namespace
+extern "C" {
+
+void pxo_set_cloths(void* p, const pxg_cloth* in, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        w.cloths.clear(); w.cloth_alive.clear();
+        for (uint32_t i = 0; i < n; ++i) {
+            w.cloths.push_back(cloth_from<T>(in[i]));
+            w.cloths.back().bounds = particle_bounds(w.cloths.back().particles);  // add_cloth, world.h:555
+            w.cloth_alive.push_back(in[i].alive && in[i].n_particles ? 1 : 0);
+        }
+    };
+    go(h->f); go(h->d);
+}
+void pxo_get_cloth(void* p, uint32_t index, pxg_cloth* out) {
+    Handle* h = static_cast<Handle*>(p);
+    if (h->dbl) { if (index < h->d.cloths.size()) cloth_to(h->d.cloths[index], *out); }
+    else if (index < h->f.cloths.size()) cloth_to(h->f.cloths[index], *out);
+}
+void pxo_set_strand_sets(void* p, const pxg_strand_set* in, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        w.strand_sets.clear(); w.strand_alive.clear();
+        for (uint32_t i = 0; i < n; ++i) {
+            w.strand_sets.push_back(strands_from<T>(in[i]));
+            w.strand_sets.back().bounds = particle_bounds(w.strand_sets.back().particles);  // add_strands, world.h:638
+            w.strand_alive.push_back(in[i].alive && in[i].n_particles && in[i].n_strands ? 1 : 0);
+        }
+    };
+    go(h->f); go(h->d);
+}
+void pxo_get_strand_set(void* p, uint32_t index, pxg_strand_set* out) {
+    Handle* h = static_cast<Handle*>(p);
+    if (h->dbl) { if (index < h->d.strand_sets.size()) strands_to(h->d.strand_sets[index], *out); }
+    else if (index < h->f.strand_sets.size()) strands_to(h->f.strand_sets[index], *out);
+}
+void pxo_set_strand_roots(void* p, uint32_t index, const float* pos, const float* rot, uint32_t n) {
+    Handle* h = static_cast<Handle*>(p);
+    auto go = [&](auto& w) {
+        using T = std::remove_reference_t<decltype(w.bodies[0].mass)>;
+        if (index >= w.strand_sets.size()) return;
+        std::vector<V3<T>> ps(n);
+        std::vector<Q<T>> qs(rot ? n : 0);
+        for (uint32_t i = 0; i < n; ++i) ps[i] = v3<T>(pos + 3 * i);
+        for (uint32_t i = 0; rot && i < n; ++i) qs[i] = q4<T>(rot + 4 * i);
+        set_strand_roots(w.strand_sets[index], ps, qs);
+    };
+    go(h->f); go(h->d);
+}
+// step() (world.h:1229-1257): returns the fixed substeps run
+uint32_t pxo_step_frame(void* p, float dt) {
+    Handle* h = static_cast<Handle*>(p);
+    return h->dbl ? h->d.step(double(dt)) : h->f.step(dt);
+}
+float pxo_interpolation_alpha(void* p) {
+    Handle* h = static_cast<Handle*>(p);
+    return h->dbl ? float(h->d.interpolation_alpha) : h->f.interpolation_alpha;
+}
+// 1: colliders in the reference's tree order; 0: in shape slot order within each tree
+void pxo_set_collider_order(void* p, uint32_t tree) {
+    Handle* h = static_cast<Handle*>(p);
+    h->f.collider_tree_order = tree != 0;
+    h->d.collider_tree_order = tree != 0;
+}
+
+// ---- builders (test fixtures): grid cloth with pins and tethers, strands ----
+struct ClothBuild { Cloth<float> c; };
+// fd: width, height, center xyz, orientation xyzw, total_mass; ui: columns, rows, shear
+void* pxo_cloth_grid(const float* fd, const uint32_t* ui, const pxg_cloth_params* params) {
+    GridClothDesc<float> d;
+    d.width = fd[0]; d.height = fd[1]; d.center = v3<float>(fd + 2); d.orientation = q4<float>(fd + 5); d.total_mass = fd[9];
+    d.columns = ui[0]; d.rows = ui[1]; d.shear = ui[2] != 0;
+    d.params = cloth_params_from<float>(*params);
+    return new ClothBuild{make_grid_cloth(d)};
+}
+// pin_static (body == nullptr) or pin_to_body; body_pose: position xyz, orientation xyzw
+uint32_t pxo_cloth_pin(void* b, const float* point, float radius, const uint32_t* body, const float* body_pose) {
+    ClothBuild* cb = static_cast<ClothBuild*>(b);
+    if (!body) return pin_particles(cb->c, v3<float>(point), radius, false, 0u, 0u, V3<float>(), Q<float>());
+    return pin_particles(cb->c, v3<float>(point), radius, true, body[0], body[1], v3<float>(body_pose), q4<float>(body_pose + 3));
+}
+void pxo_cloth_tethers(void* b) { build_tethers(static_cast<ClothBuild*>(b)->c); }
+// sizes: particles, stretch, bend, stretch batches, bend batches, tethers, anchors, triangle indices
+void pxo_cloth_sizes(void* b, uint32_t* s) {
+    const Cloth<float>& c = static_cast<ClothBuild*>(b)->c;
+    s[0] = uint32_t(c.particles.size()); s[1] = uint32_t(c.stretch.size()); s[2] = uint32_t(c.bend.size());
+    s[3] = uint32_t(c.stretch_batches.size()); s[4] = uint32_t(c.bend_batches.size()); s[5] = uint32_t(c.tethers.size());
+    s[6] = uint32_t(c.anchors.size()); s[7] = uint32_t(c.triangles.size());
+}
+void pxo_cloth_export(void* b, pxg_cloth* d, uint32_t* batches_out) {
+    const Cloth<float>& c = static_cast<ClothBuild*>(b)->c;
+    for (size_t i = 0; i < c.particles.size(); ++i) particle_to(c.particles[i], d->particles[i]);
+    for (size_t i = 0; i < c.stretch.size(); ++i) d->stretch[i] = pxg_distance_constraint{c.stretch[i].a, c.stretch[i].b, c.stretch[i].rest, 0.0f};
+    for (size_t i = 0; i < c.bend.size(); ++i) d->bend[i] = pxg_distance_constraint{c.bend[i].a, c.bend[i].b, c.bend[i].rest, 0.0f};
+    uint32_t k = 0;
+    for (const CBatch& x : c.stretch_batches) { batches_out[k++] = x.begin; batches_out[k++] = x.end; }
+    for (const CBatch& x : c.bend_batches) { batches_out[k++] = x.begin; batches_out[k++] = x.end; }
+    pxg_cloth_tether* t = const_cast<pxg_cloth_tether*>(d->tethers);
+    for (size_t i = 0; i < c.tethers.size(); ++i) t[i] = pxg_cloth_tether{c.tethers[i].particle, c.tethers[i].anchor, c.tethers[i].max_len, 0u};
+    for (size_t i = 0; i < c.anchors.size(); ++i) {
+        pxg_cloth_anchor& a = d->anchors[i];
+        std::memset(&a, 0, sizeof(a));
+        a.particle = c.anchors[i].particle; a.has_body = c.anchors[i].has_body; a.body = c.anchors[i].body;
+        a.generation = c.anchors[i].generation; put3(a.local_position, c.anchors[i].local);
+        put3(a.world_position, c.anchors[i].world); put3(a.prev_world_position, c.anchors[i].prev_world);
+    }
+    uint32_t* tri = const_cast<uint32_t*>(d->triangles);
+    for (size_t i = 0; i < c.triangles.size(); ++i) tri[i] = c.triangles[i];
+    d->columns = c.columns; d->rows = c.rows;
+    cloth_params_to(c.params, d->params);
+    put3(d->bounds_min, c.bounds.mn); put3(d->bounds_max, c.bounds.mx);
+}
+void pxo_cloth_free(void* b) { delete static_cast<ClothBuild*>(b); }
+
+// make_strands: counts[i] points per strand (xyz in pts), root orientations (xyzw) and masses per strand
+struct StrandBuild { StrandSet<float> s; };
+void* pxo_strands_make(const float* pts, const uint32_t* counts, uint32_t n_strands, const float* root_q,
+                       const float* masses, const pxg_strand_params* params) {
+    std::vector<StrandDesc<float>> ds(n_strands);
+    size_t k = 0;
+    for (uint32_t i = 0; i < n_strands; ++i) {
+        for (uint32_t j = 0; j < counts[i]; ++j, ++k) ds[i].points.push_back(v3<float>(pts + 3 * k));
+        ds[i].root_q = q4<float>(root_q + 4 * i);
+        ds[i].particle_mass = masses[i];
+    }
+    return new StrandBuild{make_strands(ds, strand_params_from<float>(*params))};
+}
+// sizes: particles, stretch, batches, strands
+void pxo_strands_sizes(void* b, uint32_t* s) {
+    const StrandSet<float>& x = static_cast<StrandBuild*>(b)->s;
+    s[0] = uint32_t(x.particles.size()); s[1] = uint32_t(x.stretch.size()); s[2] = uint32_t(x.stretch_batches.size());
+    s[3] = uint32_t(x.strands.size());
+}
+void pxo_strands_export(void* b, pxg_strand_set* d) {
+    const StrandSet<float>& x = static_cast<StrandBuild*>(b)->s;
+    float* rl = const_cast<float*>(d->rest_local);
+    float* ro = const_cast<float*>(d->rest_offset);
+    for (size_t i = 0; i < x.particles.size(); ++i) {
+        particle_to(x.particles[i], d->particles[i]);
+        put3(rl + 3 * i, x.rest_local[i]);
+        put3(ro + 3 * i, x.rest_offset[i]);
+    }
+    for (size_t i = 0; i < x.stretch.size(); ++i) d->stretch[i] = pxg_distance_constraint{x.stretch[i].a, x.stretch[i].b, x.stretch[i].rest, 0.0f};
+    uint32_t* bt = const_cast<uint32_t*>(d->stretch_batches);
+    for (size_t i = 0; i < x.stretch_batches.size(); ++i) { bt[2 * i] = x.stretch_batches[i].begin; bt[2 * i + 1] = x.stretch_batches[i].end; }
+    for (size_t i = 0; i < x.strands.size(); ++i) {
+        pxg_strand& st = d->strands[i];
+        st.first = x.strands[i].first; st.count = x.strands[i].count;
+        put3(st.root_position, x.strands[i].root); put4(st.root_orientation, x.strands[i].root_q);
+        put3(st.prev_root_position, x.strands[i].prev_root); put4(st.prev_root_orientation, x.strands[i].prev_root_q);
+    }
+    put3(d->bounds_min, x.bounds.mn); put3(d->bounds_max, x.bounds.mx);
+}
+void pxo_strands_free(void* b) { delete static_cast<StrandBuild*>(b); }
 
 }  // extern "C"

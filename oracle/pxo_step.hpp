@@ -6,6 +6,7 @@
 #include "pxo_world.hpp"
 #include "pxo_joints.hpp"
 #include "pxo_pool.hpp"
+#include "pxo_soft.hpp"
 
 #include <algorithm>
 #include <array>
@@ -57,6 +58,7 @@ template <class T> struct Config {
     T residual_tol = T(1e-4f);
     uint32_t residual_max = 256;
     uint32_t residual_every = 1;  // residual evaluated after every k-th pass and the last
+    uint32_t cloth_substeps = 4, cloth_iterations = 1;  // util/config.h:139-144
 };
 
 template <class T> struct World {
@@ -85,6 +87,17 @@ template <class T> struct World {
     std::vector<Joint<T>> joints;
     std::set<uint64_t> ignored;
     Atan2Mode atan2_mode = Atan2Mode::Libm;
+    // cloths and hair (world.h:538-700): slot tables, a dead slot is not stepped or hashed
+    std::vector<Cloth<T>> cloths;
+    std::vector<char> cloth_alive;
+    std::vector<StrandSet<T>> strand_sets;
+    std::vector<char> strand_alive;
+    // candidate colliders of a cloth or groom: true visits them in the reference's order (static
+    // tree, then dynamic tree, each in its traversal order); false in shape slot order within each
+    // tree (the order the GPU uses). The two differ only for a particle that meets two colliders of
+    // one tree in one substep.
+    bool collider_tree_order = true;
+    T interpolation_alpha = T(0);
     // contact events (collision/contact_event.h:29-36): last step's confirmed shape pairs,
     // [0] solid and [1] trigger (world.h:2112-2113), and the queue callers drain (world.h:744)
     struct Event { uint32_t a, b, type, trigger; };
@@ -164,8 +177,8 @@ template <class T> struct World {
         return std::min(s.cr, s.chh);
     }
 
-    // world.h:1511-1518 + 1545-1577 + pair_cache.h:49-52
-    void broadphase(T h) {
+    // sync_broadphase_ (world.h:1511-1518): tight bounds, and the dynamic tree's proxies moved
+    std::vector<AABB<T>> sync_bounds(T h) {
         size_t ns = shapes.size();
         std::vector<AABB<T>> tight(ns);
         for_range(ns, [&](size_t begin, size_t end, unsigned) {  // world.h:1489-1499
@@ -180,6 +193,12 @@ template <class T> struct World {
             const Body<T>& b = bodies[shapes[i].owner];
             dyn_tree.move(proxy[i], tight[i], b.v * h);
         }
+        return tight;
+    }
+    // world.h:1511-1518 + 1545-1577 + pair_cache.h:49-52
+    void broadphase(T h) {
+        size_t ns = shapes.size();
+        std::vector<AABB<T>> tight = sync_bounds(h);
         pairs.clear();
         if (brute_force) {
             // the same rule, by brute force: (i, j) iff owners differ, not both static, and
@@ -682,6 +701,159 @@ template <class T> struct World {
     }
     void step_fixed(T h) { step_phase_collide(h); step_phase_solve(h); }  // world.h:1278-1317
 
+    // step() (world.h:1229-1257): fixed substeps from the accumulator, then cloth and hair once
+    // per frame (the broadphase synced first when no substep ran). Returns the substeps run.
+    uint32_t step(T dt) {
+        events.clear();
+        const T frame = std::min(dt, T(0.25f));
+        accumulator += frame;
+        const T h = cfg.fixed_dt;
+        uint32_t n = 0;
+        while (accumulator >= h && n < cfg.max_substeps) {
+            step_fixed(h);
+            accumulator -= h;
+            ++n;
+        }
+        interpolation_alpha = accumulator / h;
+        const bool soft = !cloths.empty() || !strand_sets.empty();
+        if (soft && frame > T(0)) {
+            if (n == 0) sync_bounds(frame);
+            if (!cloths.empty()) step_cloths(frame);
+            if (!strand_sets.empty()) step_strands(frame);
+        }
+        return n;
+    }
+    bool body_valid(bool has, uint32_t index, uint32_t generation) const {
+        return has && index < bodies.size() && bodies[index].alive && bodies[index].generation == generation;
+    }
+    // the colliders a cloth or groom meets (world.h:1778-1810, 1880-1912): enabled, non-trigger
+    // shapes of live bodies that the layer matrix lets collide with `layer` and whose world
+    // bounds overlap `query`; `moving` is set when one belongs to a moving non-static body
+    void add_collider(uint32_t slot, uint32_t layer, const AABB<T>& query, std::vector<Collider<T>>& out, bool& moving) {
+        if (slot >= shapes.size()) return;
+        const Shape<T>& sh = shapes[slot];
+        if (!sh.enabled || sh.trigger) return;
+        if (!should_collide(layer, sh.layer)) return;
+        const Body<T>& b = bodies[sh.owner];
+        if (!b.alive) return;
+        Collider<T> c;
+        c.shape = &sh;
+        c.pos = b.pos;
+        c.rot = b.q;
+        c.bounds = world_bounds(sh, b.pos, b.q);
+        if (!c.bounds.overlaps(query)) return;
+        out.push_back(c);
+        if (b.type != PXG_STATIC && (dot(b.v, b.v) > k_epsilon<T>() || dot(b.w, b.w) > k_epsilon<T>())) moving = true;
+    }
+    void gather_colliders(uint32_t layer, const AABB<T>& query, std::vector<Collider<T>>& out, bool& moving) {
+        out.clear();
+        if (collider_tree_order) {
+            static_tree.query(query, [&](int32_t j) { add_collider(uint32_t(j), layer, query, out, moving); });
+            dyn_tree.query(query, [&](int32_t j) { add_collider(uint32_t(j), layer, query, out, moving); });
+            return;
+        }
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t j = 0; j < shapes.size(); ++j) {
+                if (proxy[j] < 0 || bool(in_static[j]) != (pass == 0)) continue;
+                if (!fat(j).overlaps(query)) continue;
+                add_collider(uint32_t(j), layer, query, out, moving);
+            }
+    }
+    uint32_t soft_substeps(T dt) const {  // world.h:1753-1757
+        const T internal = std::max(cfg.fixed_dt / T(std::max(1u, cfg.cloth_substeps)), k_epsilon<T>());
+        const long r = std::lround(dt / internal);
+        return uint32_t(std::clamp<long>(r, 1, 16));
+    }
+    // step_cloths_ (world.h:1749-1847)
+    void step_cloths(T dt) {
+        const uint32_t fs = soft_substeps(dt);
+        std::vector<Collider<T>> cols;
+        for (size_t ci = 0; ci < cloths.size(); ++ci) {
+            if (!cloth_alive[ci]) continue;
+            Cloth<T>& c = cloths[ci];
+            if (!c.enabled || c.particles.empty()) continue;
+            bool anchor_moved = false;
+            for (CAnchor<T>& a : c.anchors) {
+                const bool live = body_valid(a.has_body, a.body, a.generation);
+                const V3<T> target = live ? bodies[a.body].pos + qrot(bodies[a.body].q, a.local) : a.world;
+                const V3<T> d = target - a.world;
+                if (dot(d, d) > k_epsilon<T>()) anchor_moved = true;
+                a.world = target;
+            }
+            const T travel = max_particle_speed(c.particles) * dt;
+            const AABB<T> query = c.bounds.expand(c.params.thickness + travel + cfg.margin);
+            bool moving = false;
+            gather_colliders(c.params.layer, query, cols, moving);
+            if (anchor_moved || moving) {
+                c.awake = true;
+                c.sleep_timer = T(0);
+            }
+            if (!c.awake) {
+                for (CAnchor<T>& a : c.anchors) a.prev_world = a.world;
+                continue;
+            }
+            solve_cloth(c, cols, cfg.gravity, fs, cfg.cloth_iterations, dt, atan2_mode);
+            if (max_particle_speed(c.particles) < c.params.sleep_threshold) {
+                c.sleep_timer += dt;
+                if (c.sleep_timer >= c.params.sleep_time) {
+                    c.awake = false;
+                    for (CParticle<T>& p : c.particles) p.vel = V3<T>(T(0));
+                }
+            } else {
+                c.sleep_timer = T(0);
+            }
+        }
+    }
+    // step_strands_ (world.h:1861-1957)
+    void step_strands(T dt) {
+        const uint32_t fs = soft_substeps(dt);
+        std::vector<Collider<T>> cols;
+        for (size_t si = 0; si < strand_sets.size(); ++si) {
+            if (!strand_alive[si]) continue;
+            StrandSet<T>& set = strand_sets[si];
+            if (!set.enabled || set.empty()) continue;
+            bool root_moved = false;
+            for (const Strand<T>& st : set.strands) {
+                const V3<T> d = st.root - st.prev_root;
+                if (dot(d, d) > k_epsilon<T>()) { root_moved = true; break; }
+            }
+            const T travel = max_particle_speed(set.particles) * dt;
+            const AABB<T> query = set.bounds.expand(set.params.thickness + travel + cfg.margin);
+            bool moving = false;
+            if (set.collide_bodies.empty()) {
+                gather_colliders(set.params.layer, query, cols, moving);
+            } else {
+                cols.clear();
+                for (const auto& id : set.collide_bodies) {
+                    if (!body_valid(true, id.first, id.second)) continue;
+                    for (size_t j = 0; j < shapes.size(); ++j)
+                        if (shapes[j].owner == id.first) add_collider(uint32_t(j), set.params.layer, query, cols, moving);
+                }
+            }
+            if (root_moved || moving) {
+                set.awake = true;
+                set.sleep_timer = T(0);
+            }
+            if (!set.awake) {
+                for (Strand<T>& st : set.strands) {
+                    st.prev_root = st.root;
+                    st.prev_root_q = st.root_q;
+                }
+                continue;
+            }
+            solve_strands(set, cols, cfg.gravity, fs, cfg.cloth_iterations, dt, atan2_mode);
+            if (max_particle_speed(set.particles) < set.params.sleep_threshold) {
+                set.sleep_timer += dt;
+                if (set.sleep_timer >= set.params.sleep_time) {
+                    set.awake = false;
+                    for (CParticle<T>& p : set.particles) p.vel = V3<T>(T(0));
+                }
+            } else {
+                set.sleep_timer = T(0);
+            }
+        }
+    }
+
     uint64_t state_hash() const {  // world.h:1334-1359 (FP32 layout)
         uint64_t hsh = 1469598103934665603ull;
         auto mix = [&](const void* p, size_t n) {
@@ -695,6 +867,14 @@ template <class T> struct World {
             float v[3] = {float(b.v.x), float(b.v.y), float(b.v.z)};
             float w[3] = {float(b.w.x), float(b.w.y), float(b.w.z)};
             mix(p, 12); mix(q, 16); mix(v, 12); mix(w, 12);
+        }
+        for (size_t ci = 0; ci < cloths.size(); ++ci) {  // world.h:1352-1357
+            if (!cloth_alive[ci]) continue;
+            for (const CParticle<T>& pt : cloths[ci].particles) {
+                float p[3] = {float(pt.pos.x), float(pt.pos.y), float(pt.pos.z)};
+                float v[3] = {float(pt.vel.x), float(pt.vel.y), float(pt.vel.z)};
+                mix(p, 12); mix(v, 12);
+            }
         }
         return hsh;
     }

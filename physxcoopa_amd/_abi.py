@@ -204,7 +204,8 @@ class Config(ctypes.Structure):
         ("max_bodies", ctypes.c_uint32), ("max_shapes", ctypes.c_uint32), ("max_pairs", ctypes.c_uint32),
         ("max_manifolds", ctypes.c_uint32), ("learned_warm_start", ctypes.c_uint32),
         ("solve_order", ctypes.c_uint32), ("residual_check_interval", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 5),
+        ("cloth_substeps", ctypes.c_uint32), ("cloth_iterations", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 3),
     ]
 
 
@@ -250,6 +251,8 @@ def default_config(**overrides):
     c.large_extent = 16.0
     c.learned_warm_start = 0
     c.solve_order = PXG_ORDER_REFERENCE
+    c.cloth_substeps = 4
+    c.cloth_iterations = 1
     for k, v in overrides.items():
         if k == "gravity":
             for i in range(3):
@@ -382,4 +385,207 @@ def mesh_table(meshes):
     arr = (MeshDesc * max(1, len(meshes)))()
     for i, m in enumerate(meshes):
         arr[i] = m.desc()
+    return arr
+
+
+# ---- cloth and hair (include/pxg.h: pxg_cloth, pxg_strand_set) ----
+
+PARTICLE_DTYPE = np.dtype([("position", "<f4", 3), ("inv_mass", "<f4"), ("prev_position", "<f4", 3),
+                           ("reserved0", "<f4"), ("velocity", "<f4", 3), ("reserved1", "<f4")])
+CONSTRAINT_DTYPE = np.dtype([("a", "<u4"), ("b", "<u4"), ("rest_length", "<f4"), ("lambda", "<f4")])
+TETHER_DTYPE = np.dtype([("particle", "<u4"), ("anchor", "<u4"), ("max_length", "<f4"), ("reserved", "<u4")])
+ANCHOR_DTYPE = np.dtype([("particle", "<u4"), ("has_body", "<u4"), ("body", "<u4"), ("generation", "<u4"),
+                         ("local_position", "<f4", 3), ("world_position", "<f4", 3),
+                         ("prev_world_position", "<f4", 3), ("reserved", "<f4")])
+STRAND_DTYPE = np.dtype([("first", "<u4"), ("count", "<u4"), ("root_position", "<f4", 3),
+                         ("root_orientation", "<f4", 4), ("prev_root_position", "<f4", 3),
+                         ("prev_root_orientation", "<f4", 4)])
+assert PARTICLE_DTYPE.itemsize == 48 and ANCHOR_DTYPE.itemsize == 56 and STRAND_DTYPE.itemsize == 64
+
+
+class ClothParams(ctypes.Structure):
+    """pxg_cloth_params (cloth::ClothParams, cloth/cloth.h:120-226) with the reference defaults."""
+    _fields_ = [("stretch_compliance", ctypes.c_float), ("bend_compliance", ctypes.c_float),
+                ("damping", ctypes.c_float), ("thickness", ctypes.c_float), ("friction", ctypes.c_float),
+                ("gravity_scale", ctypes.c_float), ("max_velocity", ctypes.c_float),
+                ("external_acceleration", ctypes.c_float * 3), ("wind", ctypes.c_float * 3),
+                ("wind_turbulence", ctypes.c_float), ("air_drag", ctypes.c_float), ("air_lift", ctypes.c_float),
+                ("self_collision", ctypes.c_uint32), ("self_distance", ctypes.c_float), ("tether_scale", ctypes.c_float),
+                ("substeps", ctypes.c_uint32), ("iterations", ctypes.c_uint32), ("sleep_threshold", ctypes.c_float),
+                ("sleep_time", ctypes.c_float), ("layer", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        self.bend_compliance, self.damping, self.thickness, self.friction = 5e-5, 0.05, 0.03, 0.5
+        self.gravity_scale, self.max_velocity, self.air_drag, self.air_lift = 1.0, 100.0, 0.02, 0.02
+        self.tether_scale, self.sleep_threshold, self.sleep_time = 1.02, 0.05, 0.5
+        for k, v in kw.items():
+            if k in ("wind", "external_acceleration"):
+                for i in range(3):
+                    getattr(self, k)[i] = v[i]
+            else:
+                setattr(self, k, v)
+
+
+class ClothDesc(ctypes.Structure):
+    """pxg_cloth (cloth::Cloth, cloth/cloth.h:249-315) over caller-owned arrays."""
+    _fields_ = [("particles", ctypes.c_void_p), ("n_particles", ctypes.c_uint32), ("columns", ctypes.c_uint32),
+                ("rows", ctypes.c_uint32), ("stretch", ctypes.c_void_p), ("n_stretch", ctypes.c_uint32),
+                ("bend", ctypes.c_void_p), ("n_bend", ctypes.c_uint32), ("stretch_batches", ctypes.c_void_p),
+                ("n_stretch_batches", ctypes.c_uint32), ("bend_batches", ctypes.c_void_p),
+                ("n_bend_batches", ctypes.c_uint32), ("tethers", ctypes.c_void_p), ("n_tethers", ctypes.c_uint32),
+                ("anchors", ctypes.c_void_p), ("n_anchors", ctypes.c_uint32), ("triangles", ctypes.c_void_p),
+                ("n_triangle_indices", ctypes.c_uint32), ("params", ClothParams), ("alive", ctypes.c_uint32),
+                ("enabled", ctypes.c_uint32), ("awake", ctypes.c_uint32), ("sleep_timer", ctypes.c_float),
+                ("time", ctypes.c_float), ("bounds_min", ctypes.c_float * 3), ("bounds_max", ctypes.c_float * 3)]
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and len(a) else None
+
+
+class Cloth:
+    """A cloth's arrays (particles, stretch and bend constraints, batches as (begin, end) rows,
+    tethers, anchors, triangles) with its grid size, parameters and sleep state. desc() gives the
+    pxg_cloth record over them (keep the object alive while the record is used); read() fills them
+    back from one."""
+
+    def __init__(self, particles, stretch, bend, stretch_batches, bend_batches, tethers, anchors, triangles,
+                 columns=0, rows=0, params=None, enabled=True, awake=True, sleep_timer=0.0, time=0.0, alive=True):
+        self.particles = np.ascontiguousarray(particles, PARTICLE_DTYPE)
+        self.stretch = np.ascontiguousarray(stretch, CONSTRAINT_DTYPE)
+        self.bend = np.ascontiguousarray(bend, CONSTRAINT_DTYPE)
+        self.stretch_batches = np.ascontiguousarray(np.asarray(stretch_batches, np.uint32).reshape(-1, 2))
+        self.bend_batches = np.ascontiguousarray(np.asarray(bend_batches, np.uint32).reshape(-1, 2))
+        self.tethers = np.ascontiguousarray(tethers, TETHER_DTYPE)
+        self.anchors = np.ascontiguousarray(anchors, ANCHOR_DTYPE)
+        self.triangles = np.ascontiguousarray(np.asarray(triangles, np.uint32).reshape(-1))
+        self.columns, self.rows = columns, rows
+        self.params = params if params is not None else ClothParams()
+        self.enabled, self.awake, self.sleep_timer, self.time, self.alive = enabled, awake, sleep_timer, time, alive
+        self.bounds = np.zeros(6, np.float32)
+
+    def copy(self):
+        c = Cloth(self.particles.copy(), self.stretch.copy(), self.bend.copy(), self.stretch_batches.copy(),
+                  self.bend_batches.copy(), self.tethers.copy(), self.anchors.copy(), self.triangles.copy(),
+                  self.columns, self.rows, ClothParams.from_buffer_copy(self.params), self.enabled, self.awake,
+                  self.sleep_timer, self.time, self.alive)
+        c.bounds = self.bounds.copy()
+        return c
+
+    def desc(self):
+        d = ClothDesc()
+        d.particles, d.n_particles = _p(self.particles), len(self.particles)
+        d.columns, d.rows = self.columns, self.rows
+        d.stretch, d.n_stretch = _p(self.stretch), len(self.stretch)
+        d.bend, d.n_bend = _p(self.bend), len(self.bend)
+        d.stretch_batches, d.n_stretch_batches = _p(self.stretch_batches), len(self.stretch_batches)
+        d.bend_batches, d.n_bend_batches = _p(self.bend_batches), len(self.bend_batches)
+        d.tethers, d.n_tethers = _p(self.tethers), len(self.tethers)
+        d.anchors, d.n_anchors = _p(self.anchors), len(self.anchors)
+        d.triangles, d.n_triangle_indices = _p(self.triangles), len(self.triangles)
+        d.params = self.params
+        d.alive, d.enabled, d.awake = int(self.alive), int(self.enabled), int(self.awake)
+        d.sleep_timer, d.time = self.sleep_timer, self.time
+        for i in range(3):
+            d.bounds_min[i], d.bounds_max[i] = float(self.bounds[i]), float(self.bounds[3 + i])
+        return d
+
+    def take_state(self, d):
+        """The sleep state, time and bounds of a pxg_cloth record filled by a read-back."""
+        self.awake, self.sleep_timer, self.time = bool(d.awake), float(d.sleep_timer), float(d.time)
+        self.bounds = np.array(list(d.bounds_min) + list(d.bounds_max), np.float32)
+
+
+def cloth_table(cloths):
+    arr = (ClothDesc * max(1, len(cloths)))()
+    for i, c in enumerate(cloths):
+        arr[i] = c.desc()
+    return arr
+
+
+class StrandParams(ctypes.Structure):
+    """pxg_strand_params (hair::StrandParams, hair/strand.h:81-180) with the reference defaults."""
+    _fields_ = [("stretch_compliance", ctypes.c_float), ("global_shape", ctypes.c_float),
+                ("global_shape_falloff", ctypes.c_float), ("local_shape", ctypes.c_float), ("damping", ctypes.c_float),
+                ("thickness", ctypes.c_float), ("friction", ctypes.c_float), ("gravity_scale", ctypes.c_float),
+                ("max_velocity", ctypes.c_float), ("external_acceleration", ctypes.c_float * 3),
+                ("wind", ctypes.c_float * 3), ("wind_turbulence", ctypes.c_float), ("air_drag", ctypes.c_float),
+                ("substeps", ctypes.c_uint32), ("iterations", ctypes.c_uint32), ("sleep_threshold", ctypes.c_float),
+                ("sleep_time", ctypes.c_float), ("layer", ctypes.c_uint32)]
+
+    def __init__(self, **kw):
+        super().__init__()
+        self.global_shape, self.global_shape_falloff, self.local_shape, self.damping = 0.22, 1.6, 0.35, 0.08
+        self.thickness, self.friction, self.gravity_scale, self.max_velocity = 0.01, 0.4, 1.0, 50.0
+        self.wind_turbulence, self.air_drag, self.sleep_threshold, self.sleep_time = 0.1, 0.6, 0.02, 0.6
+        for k, v in kw.items():
+            if k in ("wind", "external_acceleration"):
+                for i in range(3):
+                    getattr(self, k)[i] = v[i]
+            else:
+                setattr(self, k, v)
+
+
+class StrandSetDesc(ctypes.Structure):
+    """pxg_strand_set (hair::StrandSet, hair/strand.h:187-240) over caller-owned arrays."""
+    _fields_ = [("particles", ctypes.c_void_p), ("n_particles", ctypes.c_uint32), ("stretch", ctypes.c_void_p),
+                ("n_stretch", ctypes.c_uint32), ("stretch_batches", ctypes.c_void_p),
+                ("n_stretch_batches", ctypes.c_uint32), ("rest_local", ctypes.c_void_p),
+                ("rest_offset", ctypes.c_void_p), ("strands", ctypes.c_void_p), ("n_strands", ctypes.c_uint32),
+                ("collide_bodies", ctypes.c_void_p), ("n_collide_bodies", ctypes.c_uint32),
+                ("params", StrandParams), ("alive", ctypes.c_uint32), ("enabled", ctypes.c_uint32),
+                ("awake", ctypes.c_uint32), ("sleep_timer", ctypes.c_float), ("time", ctypes.c_float),
+                ("bounds_min", ctypes.c_float * 3), ("bounds_max", ctypes.c_float * 3)]
+
+
+class StrandSet:
+    """A groom's arrays (particles, stretch constraints and batches, rest poses, strands, collide
+    bodies as (index, generation) rows) with its parameters and sleep state."""
+
+    def __init__(self, particles, stretch, stretch_batches, rest_local, rest_offset, strands, collide_bodies=(),
+                 params=None, enabled=True, awake=True, sleep_timer=0.0, time=0.0, alive=True):
+        self.particles = np.ascontiguousarray(particles, PARTICLE_DTYPE)
+        self.stretch = np.ascontiguousarray(stretch, CONSTRAINT_DTYPE)
+        self.stretch_batches = np.ascontiguousarray(np.asarray(stretch_batches, np.uint32).reshape(-1, 2))
+        self.rest_local = np.ascontiguousarray(np.asarray(rest_local, np.float32).reshape(-1, 3))
+        self.rest_offset = np.ascontiguousarray(np.asarray(rest_offset, np.float32).reshape(-1, 3))
+        self.strands = np.ascontiguousarray(strands, STRAND_DTYPE)
+        self.collide_bodies = np.ascontiguousarray(np.asarray(collide_bodies, np.uint32).reshape(-1, 2))
+        self.params = params if params is not None else StrandParams()
+        self.enabled, self.awake, self.sleep_timer, self.time, self.alive = enabled, awake, sleep_timer, time, alive
+        self.bounds = np.zeros(6, np.float32)
+
+    def copy(self):
+        s = StrandSet(self.particles.copy(), self.stretch.copy(), self.stretch_batches.copy(), self.rest_local.copy(),
+                      self.rest_offset.copy(), self.strands.copy(), self.collide_bodies.copy(),
+                      StrandParams.from_buffer_copy(self.params), self.enabled, self.awake, self.sleep_timer,
+                      self.time, self.alive)
+        s.bounds = self.bounds.copy()
+        return s
+
+    def desc(self):
+        d = StrandSetDesc()
+        d.particles, d.n_particles = _p(self.particles), len(self.particles)
+        d.stretch, d.n_stretch = _p(self.stretch), len(self.stretch)
+        d.stretch_batches, d.n_stretch_batches = _p(self.stretch_batches), len(self.stretch_batches)
+        d.rest_local, d.rest_offset = _p(self.rest_local), _p(self.rest_offset)
+        d.strands, d.n_strands = _p(self.strands), len(self.strands)
+        d.collide_bodies, d.n_collide_bodies = _p(self.collide_bodies), len(self.collide_bodies)
+        d.params = self.params
+        d.alive, d.enabled, d.awake = int(self.alive), int(self.enabled), int(self.awake)
+        d.sleep_timer, d.time = self.sleep_timer, self.time
+        for i in range(3):
+            d.bounds_min[i], d.bounds_max[i] = float(self.bounds[i]), float(self.bounds[3 + i])
+        return d
+
+    def take_state(self, d):
+        self.awake, self.sleep_timer, self.time = bool(d.awake), float(d.sleep_timer), float(d.time)
+        self.bounds = np.array(list(d.bounds_min) + list(d.bounds_max), np.float32)
+
+
+def strand_table(sets):
+    arr = (StrandSetDesc * max(1, len(sets)))()
+    for i, s in enumerate(sets):
+        arr[i] = s.desc()
     return arr

@@ -210,6 +210,86 @@ PXD int mesh_window(const MeshTab& t, const uint4& meta, v3 qlo, v3 qhi, bool fi
     }
     return n;
 }
+// Depth-first walks of the tree, left subtree first (MeshBVH::query / raycast, mesh_bvh.h:122-145):
+// leaf(tri) for every triangle of every leaf whose bounds overlap [qlo, qhi] / the ray's slab test
+// passes, in the reference's visiting order.
+template <class Leaf>
+PXD void mesh_walk_box(const MeshTab& t, const uint4& meta, v3 qlo, v3 qhi, Leaf&& leaf) {
+    int32_t stack[kMeshStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const int32_t k = stack[--sp];
+        const float4 lo = t.node_lo[meta.z + uint32_t(k)], hi = t.node_hi[meta.z + uint32_t(k)];
+        if (!(lo.x <= qhi.x && hi.x >= qlo.x && lo.y <= qhi.y && hi.y >= qlo.y && lo.z <= qhi.z && hi.z >= qlo.z)) continue;
+        const int32_t left = __float_as_int(lo.w);
+        if (left >= 0) {
+            stack[sp++] = __float_as_int(hi.w);
+            stack[sp++] = left;
+            continue;
+        }
+        const uint2 r = t.node_tri[meta.z + uint32_t(k)];
+        for (uint32_t i = 0; i < r.y; ++i) leaf(t.order[meta.y + r.x + i]);
+    }
+}
+// Ray::intersect (geometry/ray.h:31-42): slab test with 1e-9 added to every direction component
+PXD bool ray_slab(v3 o, v3 dir, float maxd, v3 lo, v3 hi, float& t) {
+    const v3 nd = dir + splat(1e-9f);
+    const v3 inv = mk(1.0f / nd.x, 1.0f / nd.y, 1.0f / nd.z);
+    const v3 t0 = mk((lo.x - o.x) * inv.x, (lo.y - o.y) * inv.y, (lo.z - o.z) * inv.z);
+    const v3 t1 = mk((hi.x - o.x) * inv.x, (hi.y - o.y) * inv.y, (hi.z - o.z) * inv.z);
+    const v3 a = vmin(t0, t1), b = vmax(t0, t1);
+    float tn = a.x;  // std::max({...}) / std::min({...}): the first extreme
+    if (tn < a.y) tn = a.y;
+    if (tn < a.z) tn = a.z;
+    if (tn < 0.0f) tn = 0.0f;
+    float tf = b.x;
+    if (b.y < tf) tf = b.y;
+    if (b.z < tf) tf = b.z;
+    if (maxd < tf) tf = maxd;
+    if (tf < tn) return false;
+    t = tn;
+    return true;
+}
+template <class Leaf>
+PXD void mesh_walk_ray(const MeshTab& t, const uint4& meta, v3 o, v3 dir, float maxd, Leaf&& leaf) {
+    int32_t stack[kMeshStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const int32_t k = stack[--sp];
+        const float4 lo = t.node_lo[meta.z + uint32_t(k)], hi = t.node_hi[meta.z + uint32_t(k)];
+        float th;
+        if (!ray_slab(o, dir, maxd, ld3(lo), ld3(hi), th)) continue;
+        const int32_t left = __float_as_int(lo.w);
+        if (left >= 0) {
+            stack[sp++] = __float_as_int(hi.w);
+            stack[sp++] = left;
+            continue;
+        }
+        const uint2 r = t.node_tri[meta.z + uint32_t(k)];
+        for (uint32_t i = 0; i < r.y; ++i) leaf(t.order[meta.y + r.x + i]);
+    }
+}
+// ray_vs_triangle (geometry/ray.h:205-226): Moller-Trumbore
+PXD bool ray_triangle(v3 o, v3 dir, float maxd, v3 a, v3 b, v3 c, float& t) {
+    const v3 e1 = b - a, e2 = c - a;
+    const v3 pv = cross(dir, e2);
+    const float det = dot(e1, pv);
+    if (det > -1e-6f && det < 1e-6f) return false;
+    const float inv = 1.0f / det;
+    const v3 tv = o - a;
+    const float u = dot(tv, pv) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    const v3 qv = cross(tv, e1);
+    const float v = dot(dir, qv) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    const float cand = dot(e2, qv) * inv;
+    if (cand < 0.0f || cand > maxd) return false;
+    t = cand;
+    return true;
+}
+
 // Calls visit(tri) for the candidate triangles in ascending order until it returns false.
 template <class Visit>
 PXD void mesh_candidates(const MeshTab& t, const uint4& meta, v3 qlo, v3 qhi, Visit&& visit) {

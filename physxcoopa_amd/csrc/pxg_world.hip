@@ -7,6 +7,7 @@
 #include "pxg_hostio.h"
 #include "pxg_mesh_cook.h"
 #include "pxg_mlp.h"
+#include "pxg_soft.h"
 
 #include <hip/hip_runtime.h>
 
@@ -99,6 +100,22 @@ struct pxg_world {
     size_t q_ov_out_cap = 0;
     float* q_pen = nullptr;  // batched penetrations: q_pen_cap words
     size_t q_pen_cap = 0;
+    // bodies' liveness and generations as last uploaded (the device keeps neither generation nor
+    // history): anchors and collide bodies resolve against them
+    std::vector<uint32_t> body_gen;
+    std::vector<uint8_t> body_alive;
+    // cloth and hair: device tables (replaced by each upload), host copies of the records, the
+    // anchors' and collide bodies' (index, generation) ids, largest particle counts
+    SoftDev sd{};
+    std::vector<void*> cloth_bufs, strand_bufs;
+    std::vector<ClothRec> cl_rec;
+    std::vector<StrandRec> st_rec;
+    std::vector<uint32_t> cl_anchor_ids;   // 3 per anchor: has_body, body, generation
+    std::vector<uint32_t> st_collide_ids;  // 2 per collide body: index, generation
+    std::vector<uint32_t> cl_anchor_res, st_collide_res;
+    uint32_t cl_max_np = 0, st_max_np = 0;
+    bool soft_resolve = false;  // bodies changed since the ids were last resolved
+    size_t soft_lds = 0;        // dynamic LDS the soft kernels are allowed
 };
 
 extern "C" void pxg_config_default(pxg_config* c) {
@@ -122,6 +139,8 @@ extern "C" void pxg_config_default(pxg_config* c) {
     c->residual_max_iterations = 256;
     c->residual_check_interval = 1;
     c->large_extent = 16.0f;
+    c->cloth_substeps = 4;
+    c->cloth_iterations = 1;
 }
 
 extern "C" const char* pxg_last_error(void) { return g_last_error.c_str(); }
@@ -156,6 +175,7 @@ static void dfree(pxg_world* w, void* p) {
 }
 
 static int rebuild_joints(pxg_world* w);
+static int step_soft(pxg_world* w, float frame, uint32_t substeps_run);
 
 static StepParams params(const pxg_world* w, float h) {
     StepParams p;
@@ -430,6 +450,13 @@ extern "C" int pxg_set_bodies(pxg_world* w, const pxg_body* bodies, uint32_t n) 
         w->n_bodies = n;
         launch_init_bodies(d, s);
     }
+    w->body_gen.resize(n);
+    w->body_alive.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        w->body_gen[i] = bodies[i].generation;
+        w->body_alive[i] = bodies[i].alive ? 1 : 0;
+    }
+    w->soft_resolve = true;
     HIPCHK(hipStreamSynchronize(s));  // host staging buffers go out of scope
     if (!seam && (!w->joints.empty() || !w->ignored.empty())) return rebuild_joints(w);  // body types order the joint layers
     return PXG_OK;
@@ -457,6 +484,7 @@ extern "C" int pxg_get_bodies(pxg_world* w, pxg_body* out, uint32_t n) {
     HIPCHK(hipMemcpyAsync(soa.flags.data(), d.b_flags, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     unpack_bodies(soa, iiw, n, out);
+    for (uint32_t i = 0; i < n && i < w->body_gen.size(); ++i) out[i].generation = w->body_gen[i];
     return PXG_OK;
 }
 
@@ -496,6 +524,11 @@ extern "C" int pxg_set_bodies_subset(pxg_world* w, const uint32_t* indices, cons
     const uint32_t* idx = flags + n;
     // as pxg_set_bodies: world inverse inertia recomputed outside the step, kept between its phases
     launch_scatter_bodies(w->d, idx, rec, flags, n, w->phase == 0 ? 1 : 0, w->stream);
+    for (uint32_t k = 0; k < n; ++k) {
+        w->body_gen[indices[k]] = records[k].generation;
+        w->body_alive[indices[k]] = records[k].alive ? 1 : 0;
+    }
+    w->soft_resolve = true;
     HIPCHK(hipStreamSynchronize(w->stream));  // the host staging buffer goes out of scope
     return PXG_OK;
 }
@@ -1151,7 +1184,7 @@ extern "C" int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run) {
         ++n;
     }
     if (substeps_run) *substeps_run = n;
-    return PXG_OK;
+    return step_soft(w, frame, n);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1810,6 +1843,584 @@ extern "C" int pxg_set_warm_cache(pxg_world* w, const uint64_t* keys, const floa
 }
 
 // FNV-1a over alive bodies' position, orientation and velocities (world.h:1334-1359)
+// ---------------------------------------------------------------------------------------------
+// cloth and hair (world.h:552-700 tables, 1749-1957 stepping)
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+// util::compute_bounds over particle positions (cloth.h:320-327): glm::min / max in order
+void particle_bounds(const pxg_particle* ps, uint32_t n, float mn[3], float mx[3]) {
+    for (int k = 0; k < 3; ++k) {
+        mn[k] = 3.402823466e+38f;
+        mx[k] = -3.402823466e+38f;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float v = ps[i].position[k];
+            if (v < mn[k]) mn[k] = v;
+            if (mx[k] < v) mx[k] = v;
+        }
+}
+// ClothSolver's bucket count (cloth_solver.h:96-100)
+uint32_t cloth_buckets(size_t n) {
+    uint32_t b = 16;
+    while (b < n * 2u && b < (1u << 22)) b <<= 1;
+    return b;
+}
+// [begin, end) pairs within n_con, each batch's constraints touching disjoint particles below np
+const char* check_batches(const uint32_t* b, uint32_t nb, const pxg_distance_constraint* con, uint32_t n_con, uint32_t np,
+                          std::vector<uint32_t>& stamp, uint32_t& tick) {
+    for (uint32_t k = 0; k < nb; ++k) {
+        const uint32_t begin = b[2 * k], end = b[2 * k + 1];
+        if (begin > end || end > n_con) return "a batch is not a range of its constraints";
+        ++tick;
+        for (uint32_t i = begin; i < end; ++i) {
+            const uint32_t a = con[i].a, c = con[i].b;
+            if (a >= np || c >= np) return "a constraint's particle is out of range";
+            if (stamp[a] == tick || (c != a && stamp[c] == tick)) return "a batch's constraints share a particle";
+            stamp[a] = stamp[c] = tick;
+        }
+    }
+    return nullptr;
+}
+inline float4 f4(const float* v, float w) { return make_float4(v[0], v[1], v[2], w); }
+inline float4 f4q(const float* q) { return make_float4(q[0], q[1], q[2], q[3]); }
+inline float u2f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+inline uint32_t f2u(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+void put_particle(const float4& p, const float4& q, const float4& v, pxg_particle& o) {
+    o.position[0] = p.x; o.position[1] = p.y; o.position[2] = p.z; o.inv_mass = p.w;
+    o.prev_position[0] = q.x; o.prev_position[1] = q.y; o.prev_position[2] = q.z;
+    o.velocity[0] = v.x; o.velocity[1] = v.y; o.velocity[2] = v.z;
+}
+
+}  // namespace
+
+template <class T>
+static int soft_alloc(pxg_world* w, std::vector<void*>& bufs, T*& dst, const std::vector<T>& src) {
+    const size_t n = std::max<size_t>(src.size(), 1);
+    T* p = nullptr;
+    int r = dalloc(w, p, n);
+    if (r != PXG_OK) return r;
+    bufs.push_back(p);
+    dst = p;
+    if (!src.empty()) HIPCHK(hipMemcpyAsync(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, w->stream));
+    return PXG_OK;
+}
+template <class T>
+static int soft_upload(pxg_world* w, std::vector<void*>& bufs, const T*& dst, const std::vector<T>& src) {
+    T* p = nullptr;
+    int r = soft_alloc(w, bufs, p, src);
+    dst = p;
+    return r;
+}
+#define SOFT_ALLOC(bufs, dst, src)                      \
+    do {                                                \
+        int r_ = soft_alloc(w, (bufs), (dst), (src));   \
+        if (r_ != PXG_OK) return r_;                    \
+    } while (0)
+#define SOFT_UPLOAD(bufs, dst, src)                     \
+    do {                                                \
+        int r_ = soft_upload(w, (bufs), (dst), (src));  \
+        if (r_ != PXG_OK) return r_;                    \
+    } while (0)
+
+static int soft_lds(pxg_world* w, uint32_t np) {
+    const size_t bytes = size_t(np) * sizeof(float4);
+    if (bytes <= w->soft_lds) return PXG_OK;
+    HIPCHK(soft_lds_limit(bytes));
+    w->soft_lds = bytes;
+    return PXG_OK;
+}
+
+// anchors' and collide bodies' (index, generation) against the bodies as last uploaded
+// (BodyId validity, world.h:1764-1772 and 1893-1899)
+static int soft_resolve(pxg_world* w) {
+    auto live = [w](uint32_t index, uint32_t gen) {
+        return index < w->body_gen.size() && w->body_alive[index] && w->body_gen[index] == gen;
+    };
+    if (!w->cl_anchor_res.empty()) {
+        for (size_t a = 0; a < w->cl_anchor_res.size(); ++a) {
+            const uint32_t* id = &w->cl_anchor_ids[3 * a];
+            w->cl_anchor_res[a] = id[0] && live(id[1], id[2]) ? id[1] : kNoBody;
+        }
+        HIPCHK(hipMemcpyAsync(const_cast<uint32_t*>(w->sd.cl_anchor_body), w->cl_anchor_res.data(),
+                              w->cl_anchor_res.size() * sizeof(uint32_t), hipMemcpyHostToDevice, w->stream));
+    }
+    if (!w->st_collide_res.empty()) {
+        for (size_t k = 0; k < w->st_collide_res.size(); ++k) {
+            const uint32_t* id = &w->st_collide_ids[2 * k];
+            w->st_collide_res[k] = live(id[0], id[1]) ? id[0] : kNoBody;
+        }
+        HIPCHK(hipMemcpyAsync(const_cast<uint32_t*>(w->sd.st_collide), w->st_collide_res.data(),
+                              w->st_collide_res.size() * sizeof(uint32_t), hipMemcpyHostToDevice, w->stream));
+    }
+    HIPCHK(hipStreamSynchronize(w->stream));  // the host arrays may change on the next call
+    w->soft_resolve = false;
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_cloths(pxg_world* w, const pxg_cloth* cloths, uint32_t n) {
+    if (!w || (n && !cloths)) return fail(PXG_ERR_INVALID, "pxg_set_cloths: null argument");
+    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_cloths: not allowed between the two step phases");
+    std::vector<ClothRec> rec(n);
+    std::vector<SoftState> state(n);
+    std::vector<float4> pos, prev, vel, anchor;
+    std::vector<uint4> con, tether, tri;
+    std::vector<uint2> batch;
+    std::vector<uint32_t> ptri_off, ptri, ids;
+    size_t n_bins = 0, n_tri = 0;
+    std::vector<uint32_t> stamp;
+    uint32_t tick = 0, max_np = 0;
+    for (uint32_t ci = 0; ci < n; ++ci) {
+        const pxg_cloth& c = cloths[ci];
+        ClothRec& r = rec[ci];
+        std::memset(&r, 0, sizeof(r));
+        SoftState& st = state[ci];
+        std::memset(&st, 0, sizeof(st));
+        const std::string at = "pxg_set_cloths: cloth " + std::to_string(ci) + ": ";
+        r.alive = c.alive && c.n_particles ? 1u : 0u;
+        if (!r.alive) continue;
+        const uint32_t np = c.n_particles;
+        if (!c.particles) return fail(PXG_ERR_INVALID, at + "null particles");
+        if (np > kSoftMaxParticles) return fail(PXG_ERR_CAPACITY, at + "more than 8192 particles");
+        if ((c.n_stretch && !c.stretch) || (c.n_bend && !c.bend) || (c.n_stretch_batches && !c.stretch_batches) ||
+            (c.n_bend_batches && !c.bend_batches) || (c.n_tethers && !c.tethers) || (c.n_anchors && !c.anchors) ||
+            (c.n_triangle_indices && !c.triangles))
+            return fail(PXG_ERR_INVALID, at + "null array");
+        stamp.assign(np, 0u);
+        tick = 0;
+        if (const char* e = check_batches(c.stretch_batches, c.n_stretch_batches, c.stretch, c.n_stretch, np, stamp, tick))
+            return fail(PXG_ERR_INVALID, at + e);
+        if (const char* e = check_batches(c.bend_batches, c.n_bend_batches, c.bend, c.n_bend, np, stamp, tick))
+            return fail(PXG_ERR_INVALID, at + e);
+        max_np = std::max(max_np, np);
+        r.p0 = uint32_t(pos.size());
+        r.np = np;
+        r.c0 = uint32_t(con.size());
+        r.n_stretch = c.n_stretch;
+        r.n_bend = c.n_bend;
+        r.b0 = uint32_t(batch.size());
+        r.n_sb = c.n_stretch_batches;
+        r.n_bb = c.n_bend_batches;
+        r.t0 = uint32_t(tether.size());
+        r.n_tether = c.n_tethers;
+        r.a0 = uint32_t(anchor.size() / 3);
+        r.n_anchor = c.n_anchors;
+        r.tr0 = uint32_t(n_tri);
+        r.n_tri = c.n_triangle_indices / 3;
+        r.pt0 = uint32_t(ptri_off.size());
+        r.columns = c.columns;
+        r.enabled = c.enabled ? 1u : 0u;
+        const pxg_cloth_params& q = c.params;
+        r.self_collision = q.self_collision ? 1u : 0u;
+        r.substeps = q.substeps;
+        r.iterations = q.iterations;
+        r.layer = q.layer;
+        r.stretch_compliance = q.stretch_compliance; r.bend_compliance = q.bend_compliance; r.damping = q.damping;
+        r.thickness = q.thickness; r.friction = q.friction; r.gravity_scale = q.gravity_scale;
+        r.max_velocity = q.max_velocity; r.wind_turbulence = q.wind_turbulence; r.air_drag = q.air_drag;
+        r.air_lift = q.air_lift; r.self_distance = q.self_distance; r.sleep_threshold = q.sleep_threshold;
+        r.sleep_time = q.sleep_time;
+        for (int k = 0; k < 3; ++k) {
+            r.ext[k] = q.external_acceleration[k];
+            r.wind[k] = q.wind[k];
+        }
+        if (r.self_collision && r.self_distance > 1e-6f) {
+            r.buckets = cloth_buckets(np);
+            r.bins0 = uint32_t(n_bins);
+            n_bins += 2 * size_t(np) + r.buckets + 1;
+        }
+        for (uint32_t i = 0; i < np; ++i) {
+            const pxg_particle& p = c.particles[i];
+            pos.push_back(f4(p.position, p.inv_mass));
+            prev.push_back(f4(p.prev_position, p.inv_mass));
+            vel.push_back(f4(p.velocity, 0.0f));
+        }
+        for (uint32_t i = 0; i < c.n_stretch; ++i)
+            con.push_back(make_uint4(c.stretch[i].a, c.stretch[i].b, f2u(c.stretch[i].rest_length), f2u(c.stretch[i].lambda)));
+        for (uint32_t i = 0; i < c.n_bend; ++i)
+            con.push_back(make_uint4(c.bend[i].a, c.bend[i].b, f2u(c.bend[i].rest_length), f2u(c.bend[i].lambda)));
+        for (uint32_t k = 0; k < c.n_stretch_batches; ++k)
+            batch.push_back(make_uint2(c.stretch_batches[2 * k], c.stretch_batches[2 * k + 1]));
+        for (uint32_t k = 0; k < c.n_bend_batches; ++k)  // bend constraints follow the stretch ones
+            batch.push_back(make_uint2(c.n_stretch + c.bend_batches[2 * k], c.n_stretch + c.bend_batches[2 * k + 1]));
+        // tethers run in parallel unless one's anchor is moved by another tether, or two move one particle
+        ++tick;
+        bool serial = false;
+        for (uint32_t t = 0; t < c.n_tethers; ++t) {
+            const pxg_cloth_tether& te = c.tethers[t];
+            if (te.particle >= np || te.anchor >= np) return fail(PXG_ERR_INVALID, at + "a tether's particle is out of range");
+            if (stamp[te.particle] == tick) serial = true;
+            stamp[te.particle] = tick;
+            tether.push_back(make_uint4(te.particle, te.anchor, f2u(te.max_length), 0u));
+        }
+        for (uint32_t t = 0; t < c.n_tethers && !serial; ++t)
+            if (stamp[c.tethers[t].anchor] == tick) serial = true;
+        r.tether_serial = serial ? 1u : 0u;
+        for (uint32_t a = 0; a < c.n_anchors; ++a) {
+            const pxg_cloth_anchor& x = c.anchors[a];
+            anchor.push_back(f4(x.world_position, u2f(x.particle)));
+            anchor.push_back(f4(x.prev_world_position, 0.0f));
+            anchor.push_back(f4(x.local_position, 0.0f));
+            ids.push_back(x.has_body ? 1u : 0u);
+            ids.push_back(x.body);
+            ids.push_back(x.generation);
+        }
+        // per particle, its triangles in ascending order, once per corner it is
+        std::vector<uint32_t> count(np + 1, 0u);
+        for (uint32_t t = 0; t < r.n_tri; ++t)
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t i = c.triangles[3 * t + k];
+                if (i >= np) return fail(PXG_ERR_INVALID, at + "a triangle's particle is out of range");
+                ++count[i];
+            }
+        const uint32_t base = uint32_t(ptri.size());
+        std::vector<uint32_t> fill(np, 0u);
+        uint32_t run = base;
+        for (uint32_t i = 0; i < np; ++i) {
+            ptri_off.push_back(run);
+            fill[i] = run;
+            run += count[i];
+        }
+        ptri_off.push_back(run);
+        ptri.resize(run);
+        for (uint32_t t = 0; t < r.n_tri; ++t) {
+            const uint32_t a = c.triangles[3 * t], b = c.triangles[3 * t + 1], cc = c.triangles[3 * t + 2];
+            tri.push_back(make_uint4(a, b, cc, 0u));
+            ptri[fill[a]++] = t;
+            ptri[fill[b]++] = t;
+            ptri[fill[cc]++] = t;
+        }
+        n_tri += r.n_tri;
+        st.awake = c.awake ? 1u : 0u;
+        st.sleep_timer = c.sleep_timer;
+        st.time = c.time;
+        particle_bounds(c.particles, np, st.bmin, st.bmax);  // add_cloth recomputes them (world.h:555)
+    }
+    int rr = soft_lds(w, max_np);
+    if (rr != PXG_OK) return rr;
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (void* p : w->cloth_bufs) dfree(w, p);
+    w->cloth_bufs.clear();
+    SoftDev& s = w->sd;
+    s.n_cloths = 0;
+    w->cl_rec.clear();
+    w->cl_anchor_ids.clear();
+    w->cl_anchor_res.clear();
+    w->cl_max_np = 0;
+    if (!n) return PXG_OK;
+    std::vector<float4> share(std::max<size_t>(n_tri, 1));
+    std::vector<uint32_t> bins(n_bins), cols(size_t(n) * kSoftMaxColliders), res(ids.size() / 3, kNoBody);
+    SoftState* dstate = nullptr;
+    SOFT_UPLOAD(w->cloth_bufs, s.cl, rec);
+    SOFT_ALLOC(w->cloth_bufs, dstate, state);
+    s.cl_state = dstate;
+    SOFT_ALLOC(w->cloth_bufs, s.cl_pos, pos);
+    SOFT_ALLOC(w->cloth_bufs, s.cl_prev, prev);
+    SOFT_ALLOC(w->cloth_bufs, s.cl_vel, vel);
+    SOFT_ALLOC(w->cloth_bufs, s.cl_con, con);
+    SOFT_UPLOAD(w->cloth_bufs, s.cl_batch, batch);
+    SOFT_UPLOAD(w->cloth_bufs, s.cl_tether, tether);
+    SOFT_ALLOC(w->cloth_bufs, s.cl_anchor, anchor);
+    SOFT_UPLOAD(w->cloth_bufs, s.cl_anchor_body, res);
+    SOFT_UPLOAD(w->cloth_bufs, s.cl_tri, tri);
+    SOFT_UPLOAD(w->cloth_bufs, s.cl_ptri_off, ptri_off);
+    SOFT_UPLOAD(w->cloth_bufs, s.cl_ptri, ptri);
+    SOFT_ALLOC(w->cloth_bufs, s.cl_share, share);
+    SOFT_ALLOC(w->cloth_bufs, s.cl_bins, bins);
+    SOFT_ALLOC(w->cloth_bufs, s.cl_cols, cols);
+    s.n_cloths = n;
+    w->cl_rec = rec;
+    w->cl_anchor_ids = ids;
+    w->cl_anchor_res = res;
+    w->cl_max_np = max_np;
+    return soft_resolve(w);
+}
+
+extern "C" int pxg_get_cloth(pxg_world* w, uint32_t index, pxg_cloth* out) {
+    if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_get_cloth: null argument");
+    if (index >= w->cl_rec.size()) return fail(PXG_ERR_INVALID, "pxg_get_cloth: no such cloth slot");
+    const ClothRec& r = w->cl_rec[index];
+    const SoftDev& s = w->sd;
+    const std::string at = "pxg_get_cloth: ";
+    if ((out->particles && out->n_particles != r.np) || (out->stretch && out->n_stretch != r.n_stretch) ||
+        (out->bend && out->n_bend != r.n_bend) || (out->anchors && out->n_anchors != r.n_anchor))
+        return fail(PXG_ERR_INVALID, at + "array sizes differ from the uploaded cloth");
+    hipStream_t st = w->stream;
+    SoftState state;
+    HIPCHK(hipMemcpyAsync(&state, s.cl_state + index, sizeof(state), hipMemcpyDeviceToHost, st));
+    std::vector<float4> pos(r.np), prev(r.np), vel(r.np), anc(3 * size_t(r.n_anchor));
+    std::vector<uint4> con(size_t(r.n_stretch) + r.n_bend);
+    if (r.alive && out->particles && r.np) {
+        HIPCHK(hipMemcpyAsync(pos.data(), s.cl_pos + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(prev.data(), s.cl_prev + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(vel.data(), s.cl_vel + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, st));
+    }
+    if (r.alive && !con.empty() && (out->stretch || out->bend))
+        HIPCHK(hipMemcpyAsync(con.data(), s.cl_con + r.c0, con.size() * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    if (r.alive && out->anchors && r.n_anchor)
+        HIPCHK(hipMemcpyAsync(anc.data(), s.cl_anchor + 3 * size_t(r.a0), anc.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!r.alive) return PXG_OK;
+    for (uint32_t i = 0; out->particles && i < r.np; ++i) put_particle(pos[i], prev[i], vel[i], out->particles[i]);
+    for (uint32_t i = 0; out->stretch && i < r.n_stretch; ++i) out->stretch[i].lambda = u2f(con[i].w);
+    for (uint32_t i = 0; out->bend && i < r.n_bend; ++i) out->bend[i].lambda = u2f(con[r.n_stretch + i].w);
+    for (uint32_t a = 0; out->anchors && a < r.n_anchor; ++a) {
+        const float4 x = anc[3 * a], y = anc[3 * a + 1];
+        pxg_cloth_anchor& o = out->anchors[a];
+        o.world_position[0] = x.x; o.world_position[1] = x.y; o.world_position[2] = x.z;
+        o.prev_world_position[0] = y.x; o.prev_world_position[1] = y.y; o.prev_world_position[2] = y.z;
+    }
+    out->awake = state.awake;
+    out->enabled = r.enabled;
+    out->sleep_timer = state.sleep_timer;
+    out->time = state.time;
+    for (int k = 0; k < 3; ++k) {
+        out->bounds_min[k] = state.bmin[k];
+        out->bounds_max[k] = state.bmax[k];
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_strand_sets(pxg_world* w, const pxg_strand_set* sets, uint32_t n) {
+    if (!w || (n && !sets)) return fail(PXG_ERR_INVALID, "pxg_set_strand_sets: null argument");
+    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_set_strand_sets: not allowed between the two step phases");
+    std::vector<StrandRec> rec(n);
+    std::vector<SoftState> state(n);
+    std::vector<float4> pos, prev, vel, rest_local, rest_offset, strand;
+    std::vector<uint4> con;
+    std::vector<uint2> batch;
+    std::vector<uint32_t> ids, stamp;
+    uint32_t tick = 0, max_np = 0;
+    for (uint32_t si = 0; si < n; ++si) {
+        const pxg_strand_set& g = sets[si];
+        StrandRec& r = rec[si];
+        std::memset(&r, 0, sizeof(r));
+        SoftState& st = state[si];
+        std::memset(&st, 0, sizeof(st));
+        const std::string at = "pxg_set_strand_sets: set " + std::to_string(si) + ": ";
+        r.alive = g.alive && g.n_particles && g.n_strands ? 1u : 0u;
+        if (!r.alive) continue;
+        const uint32_t np = g.n_particles;
+        if (!g.particles || !g.rest_local || !g.rest_offset || !g.strands) return fail(PXG_ERR_INVALID, at + "null array");
+        if ((g.n_stretch && !g.stretch) || (g.n_stretch_batches && !g.stretch_batches) ||
+            (g.n_collide_bodies && !g.collide_bodies))
+            return fail(PXG_ERR_INVALID, at + "null array");
+        if (np > kSoftMaxParticles) return fail(PXG_ERR_CAPACITY, at + "more than 8192 particles");
+        stamp.assign(np, 0u);
+        tick = 0;
+        if (const char* e = check_batches(g.stretch_batches, g.n_stretch_batches, g.stretch, g.n_stretch, np, stamp, tick))
+            return fail(PXG_ERR_INVALID, at + e);
+        ++tick;
+        for (uint32_t k = 0; k < g.n_strands; ++k) {  // strands are disjoint ranges of the particles
+            const pxg_strand& x = g.strands[k];
+            if (!x.count) continue;
+            if (x.first >= np || x.count > np - x.first) return fail(PXG_ERR_INVALID, at + "a strand's range is out of range");
+            for (uint32_t i = x.first; i < x.first + x.count; ++i) {
+                if (stamp[i] == tick) return fail(PXG_ERR_INVALID, at + "two strands share a particle");
+                stamp[i] = tick;
+            }
+        }
+        max_np = std::max(max_np, np);
+        r.p0 = uint32_t(pos.size());
+        r.np = np;
+        r.c0 = uint32_t(con.size());
+        r.n_stretch = g.n_stretch;
+        r.b0 = uint32_t(batch.size());
+        r.n_b = g.n_stretch_batches;
+        r.s0 = uint32_t(strand.size() / 4);
+        r.n_strands = g.n_strands;
+        r.cb0 = uint32_t(ids.size() / 2);
+        r.n_cb = g.n_collide_bodies;
+        r.enabled = g.enabled ? 1u : 0u;
+        const pxg_strand_params& q = g.params;
+        r.substeps = q.substeps;
+        r.iterations = q.iterations;
+        r.layer = q.layer;
+        r.stretch_compliance = q.stretch_compliance; r.global_shape = q.global_shape;
+        r.global_shape_falloff = q.global_shape_falloff; r.local_shape = q.local_shape; r.damping = q.damping;
+        r.thickness = q.thickness; r.friction = q.friction; r.gravity_scale = q.gravity_scale;
+        r.max_velocity = q.max_velocity; r.wind_turbulence = q.wind_turbulence; r.air_drag = q.air_drag;
+        r.sleep_threshold = q.sleep_threshold; r.sleep_time = q.sleep_time;
+        for (int k = 0; k < 3; ++k) {
+            r.ext[k] = q.external_acceleration[k];
+            r.wind[k] = q.wind[k];
+        }
+        for (uint32_t i = 0; i < np; ++i) {
+            const pxg_particle& p = g.particles[i];
+            pos.push_back(f4(p.position, p.inv_mass));
+            prev.push_back(f4(p.prev_position, p.inv_mass));
+            vel.push_back(f4(p.velocity, 0.0f));
+            rest_local.push_back(f4(g.rest_local + 3 * size_t(i), 0.0f));
+            rest_offset.push_back(f4(g.rest_offset + 3 * size_t(i), 0.0f));
+        }
+        for (uint32_t i = 0; i < g.n_stretch; ++i)
+            con.push_back(make_uint4(g.stretch[i].a, g.stretch[i].b, f2u(g.stretch[i].rest_length), f2u(g.stretch[i].lambda)));
+        for (uint32_t k = 0; k < g.n_stretch_batches; ++k)
+            batch.push_back(make_uint2(g.stretch_batches[2 * k], g.stretch_batches[2 * k + 1]));
+        for (uint32_t k = 0; k < g.n_strands; ++k) {
+            const pxg_strand& x = g.strands[k];
+            strand.push_back(f4(x.root_position, u2f(x.first)));
+            strand.push_back(f4q(x.root_orientation));
+            strand.push_back(f4(x.prev_root_position, u2f(x.count)));
+            strand.push_back(f4q(x.prev_root_orientation));
+        }
+        for (uint32_t k = 0; k < g.n_collide_bodies; ++k) {
+            ids.push_back(g.collide_bodies[2 * k]);
+            ids.push_back(g.collide_bodies[2 * k + 1]);
+        }
+        st.awake = g.awake ? 1u : 0u;
+        st.sleep_timer = g.sleep_timer;
+        st.time = g.time;
+        particle_bounds(g.particles, np, st.bmin, st.bmax);  // add_strands recomputes them (world.h:638)
+    }
+    int rr = soft_lds(w, max_np);
+    if (rr != PXG_OK) return rr;
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (void* p : w->strand_bufs) dfree(w, p);
+    w->strand_bufs.clear();
+    SoftDev& s = w->sd;
+    s.n_sets = 0;
+    w->st_rec.clear();
+    w->st_collide_ids.clear();
+    w->st_collide_res.clear();
+    w->st_max_np = 0;
+    if (!n) return PXG_OK;
+    std::vector<uint32_t> cols(size_t(n) * kSoftMaxColliders), res(ids.size() / 2, kNoBody);
+    SoftState* dstate = nullptr;
+    SOFT_UPLOAD(w->strand_bufs, s.st, rec);
+    SOFT_ALLOC(w->strand_bufs, dstate, state);
+    s.st_state = dstate;
+    SOFT_ALLOC(w->strand_bufs, s.st_pos, pos);
+    SOFT_ALLOC(w->strand_bufs, s.st_prev, prev);
+    SOFT_ALLOC(w->strand_bufs, s.st_vel, vel);
+    SOFT_ALLOC(w->strand_bufs, s.st_con, con);
+    SOFT_UPLOAD(w->strand_bufs, s.st_batch, batch);
+    SOFT_UPLOAD(w->strand_bufs, s.st_rest_local, rest_local);
+    SOFT_UPLOAD(w->strand_bufs, s.st_rest_offset, rest_offset);
+    SOFT_ALLOC(w->strand_bufs, s.st_strand, strand);
+    SOFT_UPLOAD(w->strand_bufs, s.st_collide, res);
+    SOFT_ALLOC(w->strand_bufs, s.st_cols, cols);
+    s.n_sets = n;
+    w->st_rec = rec;
+    w->st_collide_ids = ids;
+    w->st_collide_res = res;
+    w->st_max_np = max_np;
+    return soft_resolve(w);
+}
+
+extern "C" int pxg_get_strand_set(pxg_world* w, uint32_t index, pxg_strand_set* out) {
+    if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_get_strand_set: null argument");
+    if (index >= w->st_rec.size()) return fail(PXG_ERR_INVALID, "pxg_get_strand_set: no such set slot");
+    const StrandRec& r = w->st_rec[index];
+    const SoftDev& s = w->sd;
+    if ((out->particles && out->n_particles != r.np) || (out->stretch && out->n_stretch != r.n_stretch) ||
+        (out->strands && out->n_strands != r.n_strands))
+        return fail(PXG_ERR_INVALID, "pxg_get_strand_set: array sizes differ from the uploaded set");
+    hipStream_t st = w->stream;
+    SoftState state;
+    HIPCHK(hipMemcpyAsync(&state, s.st_state + index, sizeof(state), hipMemcpyDeviceToHost, st));
+    std::vector<float4> pos(r.np), prev(r.np), vel(r.np), strand(4 * size_t(r.n_strands));
+    std::vector<uint4> con(r.n_stretch);
+    if (r.alive && out->particles) {
+        HIPCHK(hipMemcpyAsync(pos.data(), s.st_pos + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(prev.data(), s.st_prev + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(vel.data(), s.st_vel + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, st));
+    }
+    if (r.alive && out->stretch && !con.empty())
+        HIPCHK(hipMemcpyAsync(con.data(), s.st_con + r.c0, con.size() * sizeof(uint4), hipMemcpyDeviceToHost, st));
+    if (r.alive && out->strands)
+        HIPCHK(hipMemcpyAsync(strand.data(), s.st_strand + 4 * size_t(r.s0), strand.size() * sizeof(float4),
+                              hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!r.alive) return PXG_OK;
+    for (uint32_t i = 0; out->particles && i < r.np; ++i) put_particle(pos[i], prev[i], vel[i], out->particles[i]);
+    for (uint32_t i = 0; out->stretch && i < r.n_stretch; ++i) out->stretch[i].lambda = u2f(con[i].w);
+    for (uint32_t k = 0; out->strands && k < r.n_strands; ++k) {
+        pxg_strand& o = out->strands[k];
+        const float4 a = strand[4 * k], q = strand[4 * k + 1], b = strand[4 * k + 2], pq = strand[4 * k + 3];
+        o.root_position[0] = a.x; o.root_position[1] = a.y; o.root_position[2] = a.z;
+        o.root_orientation[0] = q.x; o.root_orientation[1] = q.y; o.root_orientation[2] = q.z; o.root_orientation[3] = q.w;
+        o.prev_root_position[0] = b.x; o.prev_root_position[1] = b.y; o.prev_root_position[2] = b.z;
+        o.prev_root_orientation[0] = pq.x; o.prev_root_orientation[1] = pq.y; o.prev_root_orientation[2] = pq.z;
+        o.prev_root_orientation[3] = pq.w;
+    }
+    out->awake = state.awake;
+    out->enabled = r.enabled;
+    out->sleep_timer = state.sleep_timer;
+    out->time = state.time;
+    for (int k = 0; k < 3; ++k) {
+        out->bounds_min[k] = state.bmin[k];
+        out->bounds_max[k] = state.bmax[k];
+    }
+    return PXG_OK;
+}
+
+extern "C" int pxg_set_strand_roots(pxg_world* w, uint32_t index, const float* positions, const float* orientations,
+                                    uint32_t n) {
+    if (!w || (n && !positions)) return fail(PXG_ERR_INVALID, "pxg_set_strand_roots: null argument");
+    if (index >= w->st_rec.size()) return fail(PXG_ERR_INVALID, "pxg_set_strand_roots: no such set slot");
+    const StrandRec& r = w->st_rec[index];
+    const uint32_t m = std::min(n, r.n_strands);
+    if (!r.alive || !m) return PXG_OK;
+    float4* dev = w->sd.st_strand + 4 * size_t(r.s0);
+    std::vector<float4> strand(4 * size_t(m));
+    hipStream_t st = w->stream;
+    HIPCHK(hipMemcpyAsync(strand.data(), dev, strand.size() * sizeof(float4), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (uint32_t k = 0; k < m; ++k) {  // prev = current; current = the new root
+        float4* x = &strand[4 * k];
+        x[2] = make_float4(x[0].x, x[0].y, x[0].z, x[2].w);
+        x[3] = x[1];
+        x[0] = make_float4(positions[3 * k], positions[3 * k + 1], positions[3 * k + 2], x[0].w);
+        if (orientations) x[1] = f4q(orientations + 4 * size_t(k));
+    }
+    HIPCHK(hipMemcpyAsync(dev, strand.data(), strand.size() * sizeof(float4), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PXG_OK;
+}
+
+// the once-per-frame cloth and hair pass of step() (world.h:1241-1256)
+static int step_soft(pxg_world* w, float frame, uint32_t substeps_run) {
+    SoftDev& s = w->sd;
+    if ((!s.n_cloths && !s.n_sets) || !(frame > 0.0f)) return PXG_OK;
+    const pxg_config& c = w->cfg;
+    if (w->soft_resolve) {
+        int r = soft_resolve(w);
+        if (r != PXG_OK) return r;
+    }
+    hipStream_t st = w->stream;
+    if (substeps_run == 0 && w->shapes_ready) launch_refit(w->d, frame, c.aabb_margin, st);  // sync_broadphase_
+    // world.h:1753-1757: substeps of fixed_dt / cloth_substeps in this frame, 1 to 16
+    const float internal = std::max(c.fixed_dt / float(std::max(1u, c.cloth_substeps)), 1e-6f);
+    const uint32_t fs = uint32_t(std::clamp<long>(std::lround(frame / internal), 1, 16));
+    pxd::v3 g;
+    g.x = c.gravity[0];
+    g.y = c.gravity[1];
+    g.z = c.gravity[2];
+    launch_cloths(w->d, s, w->cl_max_np, frame, fs, c.cloth_iterations, g, c.aabb_margin, st);
+    launch_strands(w->d, s, w->st_max_np, frame, fs, c.cloth_iterations, g, c.aabb_margin, st);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(PXG_ERR_HIP, std::string("cloth / hair launch: ") + hipGetErrorString(e));
+    std::vector<SoftState> cs(s.n_cloths), ss(s.n_sets);
+    if (s.n_cloths) HIPCHK(hipMemcpyAsync(cs.data(), s.cl_state, cs.size() * sizeof(SoftState), hipMemcpyDeviceToHost, st));
+    if (s.n_sets) HIPCHK(hipMemcpyAsync(ss.data(), s.st_state, ss.size() * sizeof(SoftState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (const SoftState& x : cs)
+        if (x.error) return fail(PXG_ERR_CAPACITY, "pxg_step: a cloth met more than 256 colliders in a frame");
+    for (const SoftState& x : ss)
+        if (x.error) return fail(PXG_ERR_CAPACITY, "pxg_step: a hair groom met more than 256 colliders in a frame");
+    return PXG_OK;
+}
+
 extern "C" int pxg_state_hash(pxg_world* w, uint64_t* out) {
     if (!w || !out) return fail(PXG_ERR_INVALID, "pxg_state_hash: null argument");
     uint32_t n = w->n_bodies;
@@ -1827,6 +2438,17 @@ extern "C" int pxg_state_hash(pxg_world* w, uint64_t* out) {
         mix(b[i].orientation, 16);
         mix(b[i].linear_velocity, 12);
         mix(b[i].angular_velocity, 12);
+    }
+    for (const ClothRec& r : w->cl_rec) {  // alive cloths' particles (world.h:1352-1357)
+        if (!r.alive) continue;
+        std::vector<float4> pos(r.np), vel(r.np);
+        HIPCHK(hipMemcpyAsync(pos.data(), w->sd.cl_pos + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
+        HIPCHK(hipMemcpyAsync(vel.data(), w->sd.cl_vel + r.p0, r.np * sizeof(float4), hipMemcpyDeviceToHost, w->stream));
+        HIPCHK(hipStreamSynchronize(w->stream));
+        for (uint32_t i = 0; i < r.np; ++i) {
+            mix(&pos[i], 12);
+            mix(&vel[i], 12);
+        }
     }
     *out = h;
     return PXG_OK;

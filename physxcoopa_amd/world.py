@@ -19,7 +19,8 @@ from ._lib import lib, check
 class World:
     """One GPU world handle built from ABI arrays (bodies, shapes)."""
 
-    def __init__(self, config, bodies, shapes, layer_masks=None, hulls=None, meshes=None):
+    def __init__(self, config, bodies, shapes, layer_masks=None, hulls=None, meshes=None, cloths=None,
+                 strands=None):
         self._cfg = config
         h = ctypes.c_void_p()
         check(lib().pxg_create(ctypes.byref(config), ctypes.byref(h)), "pxg_create")
@@ -37,6 +38,10 @@ class World:
         if meshes:
             self.set_meshes(meshes)
         check(lib().pxg_set_shapes(self.h, ptr(s), self.n_shapes), "pxg_set_shapes")
+        if cloths:
+            self.set_cloths(cloths)
+        if strands:
+            self.set_strand_sets(strands)
 
     def close(self):
         if getattr(self, "h", None):
@@ -78,6 +83,41 @@ class World:
         tab = _abi.mesh_table(meshes)
         self._meshes = list(meshes)
         check(lib().pxg_set_meshes(self.h, ctypes.byref(tab), len(meshes)), "pxg_set_meshes")
+
+    def set_cloths(self, cloths):
+        """Replaces the cloth table (a list of _abi.Cloth; slot i is cloths[i])."""
+        tab = _abi.cloth_table(cloths)
+        self._cloths = [c.copy() for c in cloths]  # the shapes read back into
+        check(lib().pxg_set_cloths(self.h, ctypes.byref(tab), len(cloths)), "pxg_set_cloths")
+
+    def cloth(self, index):
+        """Cloth slot `index` as uploaded, with its particles, multipliers, anchors and sleep state
+        read back."""
+        c = self._cloths[index].copy()
+        d = c.desc()
+        check(lib().pxg_get_cloth(self.h, index, ctypes.byref(d)), "pxg_get_cloth")
+        c.take_state(d)
+        return c
+
+    def set_strand_sets(self, sets):
+        """Replaces the hair table (a list of _abi.StrandSet)."""
+        tab = _abi.strand_table(sets)
+        self._strands = [x.copy() for x in sets]
+        check(lib().pxg_set_strand_sets(self.h, ctypes.byref(tab), len(sets)), "pxg_set_strand_sets")
+
+    def strand_set(self, index):
+        s = self._strands[index].copy()
+        d = s.desc()
+        check(lib().pxg_get_strand_set(self.h, index, ctypes.byref(d)), "pxg_get_strand_set")
+        s.take_state(d)
+        return s
+
+    def set_strand_roots(self, index, positions, orientations=None):
+        """hair::set_strand_roots on set `index` (positions n x 3, orientations n x 4 xyzw)."""
+        p = np.ascontiguousarray(np.asarray(positions, np.float32).reshape(-1, 3))
+        q = None if orientations is None else np.ascontiguousarray(np.asarray(orientations, np.float32).reshape(-1, 4))
+        check(lib().pxg_set_strand_roots(self.h, index, ptr(p), None if q is None else ptr(q), len(p)),
+              "pxg_set_strand_roots")
 
     def set_joints(self, joints):
         """Replaces the joint table (pxg_joint records, see physxcoopa_amd.joints)."""

@@ -974,6 +974,92 @@ void batched_overlaps_and_casts_equal_single_queries() {
     CHECK(n_found > 9);
 }
 
+// A floor of n x n quads over [-2, 2]^2 at z = 0 (narrowphase_test.cpp:314-327), as a soup.
+struct FloorSoup {
+    std::vector<float> positions;
+    std::vector<uint32_t> indices;
+    explicit FloorSoup(int n) {
+        const float step = 4.0f / float(n);
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) positions.insert(positions.end(), {-2.0f + float(i) * step, -2.0f + float(j) * step, 0.0f});
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const uint32_t a = uint32_t(j * (n + 1) + i), b = a + 1, c = a + uint32_t(n) + 2, d = a + uint32_t(n) + 1;
+                indices.insert(indices.end(), {a, b, c, a, c, d});
+            }
+    }
+    pxg_mesh_desc desc() const {
+        return pxg_mesh_desc{positions.data(), uint32_t(positions.size() / 3), uint32_t(indices.size()), indices.data()};
+    }
+};
+pxg_shape mesh_shape(int32_t index) {
+    pxg_shape s = box_shape(0, 0, 0);
+    s.type = PXG_TRIANGLE_MESH;
+    s.capsule_axis = index;
+    s.radius = 1.0f;  // scale
+    return s;
+}
+
+// narrowphase_test.cpp:432-450: a box whose centre has sunk under a mesh floor is lifted back out
+void embedded_box_climbs_back_onto_a_mesh_floor() {
+    pxg::PhysicsWorld world;
+    FloorSoup floor(8);
+    const pxg_mesh_desc d = floor.desc();
+    world.set_meshes(&d, 1);
+    world.add_body(body(PXG_STATIC, 0, 0, 0), mesh_shape(0));
+    pxg::BodyId box = add_dynamic_box(world, 0.3f, -0.4f, -0.1f, 0.11f, 0.11f, 0.11f, 1.2f);
+    for (int i = 0; i < 240; ++i) world.step_fixed(kH);
+    const pxg_body* b = world.get_body(box);
+    CHECK_NEAR(b->position[2], 0.11, 0.02);
+    CHECK(vlen(b->linear_velocity) < 0.05f);
+}
+
+// A cube hull built by hand (geometry::ConvexHull's arrays) dropped on the mesh floor rests at its
+// half height; the tables survive the world being re-created to grow past its body capacity.
+void cube_hull_rests_on_a_mesh_floor_across_world_growth() {
+    pxg_config cfg = pxg::default_config();
+    cfg.max_bodies = cfg.max_shapes = 8;
+    pxg::PhysicsWorld world(cfg);
+    const float h = 0.3f;
+    std::vector<float> v;
+    for (int k = 0; k < 8; ++k) v.insert(v.end(), {(k & 1) ? h : -h, (k & 2) ? h : -h, (k & 4) ? h : -h});
+    // faces counter-clockwise about their outward normals: -x, +x, -y, +y, -z, +z
+    const std::vector<uint32_t> fv = {0, 4, 6, 2, 1, 3, 7, 5, 0, 1, 5, 4, 2, 6, 7, 3, 0, 2, 3, 1, 4, 5, 7, 6};
+    const std::vector<uint32_t> starts = {0, 4, 8, 12, 16, 20, 24};
+    const std::vector<float> planes = {-1, 0, 0, h, 1, 0, 0, h, 0, -1, 0, h, 0, 1, 0, h, 0, 0, -1, h, 0, 0, 1, h};
+    std::vector<uint32_t> edges;
+    for (uint32_t a = 0; a < 8; ++a)
+        for (uint32_t bit = 1; bit < 8; bit <<= 1)
+            if (!(a & bit)) edges.insert(edges.end(), {a, a | bit});
+    pxg_hull hull;
+    std::memset(&hull, 0, sizeof(hull));
+    hull.vertices = v.data(); hull.n_vertices = 8; hull.n_faces = 6;
+    hull.face_planes = planes.data(); hull.face_starts = starts.data(); hull.face_vertices = fv.data();
+    hull.edges = edges.data(); hull.n_edges = uint32_t(edges.size() / 2);
+    for (int i = 0; i < 3; ++i) { hull.local_min[i] = -h; hull.local_max[i] = h; }
+    hull.volume = 8.0f * h * h * h;
+    world.set_hulls(&hull, 1);
+    FloorSoup floor(16);
+    const pxg_mesh_desc d = floor.desc();
+    world.set_meshes(&d, 1);
+    world.add_body(body(PXG_STATIC, 0, 0, 0), mesh_shape(0));
+    pxg_body hb = body(PXG_DYNAMIC, 0.2f, 0.1f, 1.0f);
+    const float inv = 1.0f / (1.0f / 3.0f * (h * h + h * h));
+    hb.inv_inertia_local[0] = hb.inv_inertia_local[1] = hb.inv_inertia_local[2] = inv;
+    pxg_shape hs = box_shape(0, 0, 0);
+    hs.type = PXG_CONVEX_HULL;
+    hs.capsule_axis = 0;
+    hs.radius = 1.0f;
+    pxg::BodyId cube = world.add_body(hb, hs);
+    for (int i = 0; i < 90; ++i) world.step_fixed(kH);
+    for (int i = 0; i < 12; ++i) add_dynamic_sphere(world, -1.5f + 0.25f * float(i), 1.5f, 0.5f, 0.1f);  // grows the world
+    for (int i = 0; i < 150; ++i) world.step_fixed(kH);
+    const pxg_body* b = world.get_body(cube);
+    CHECK_NEAR(b->position[2], h, 0.02);
+    CHECK(best_up_alignment(b->orientation) > 0.999f);
+    CHECK(vlen(b->linear_velocity) < 0.05f);
+}
+
 int main() {
     struct T {
         const char* name;
@@ -1008,6 +1094,8 @@ int main() {
         {"overlaps_find_only_touched_bodies", overlaps_find_only_touched_bodies},
         {"sweeps_are_exact_against_box_corner_and_capsule", sweeps_are_exact_against_box_corner_and_capsule},
         {"compute_penetration_is_exact_against_box", compute_penetration_is_exact_against_box},
+        {"embedded_box_climbs_back_onto_a_mesh_floor", embedded_box_climbs_back_onto_a_mesh_floor},
+        {"cube_hull_rests_on_a_mesh_floor_across_world_growth", cube_hull_rests_on_a_mesh_floor_across_world_growth},
         {"write_back_from_worker_threads_matches_a_serial_pass", write_back_from_worker_threads_matches_a_serial_pass},
         {"batched_overlaps_and_casts_equal_single_queries", batched_overlaps_and_casts_equal_single_queries},
     };

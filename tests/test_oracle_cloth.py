@@ -1,0 +1,432 @@
+"""The oracle's XPBD cloth and hair: the grid builder, pins and tethers (cloth/cloth_builder.h),
+the particle projection against every shape type (cloth/cloth_collision.h), the cloth and strand
+substep loops (cloth/cloth_solver.h, hair/strand_solver.h) and the world's once-per-frame drivers
+(world.h:1229-1257, 1749-1957), pinned by the reference's cloth_solver_test.cpp,
+cloth_collision_test.cpp and hair_solver_test.cpp cases, by closed forms (free fall under
+semi-implicit Euler, a distance constraint's exact projection, the projection distances), and by
+the collider order of the trees agreeing with slot order when no particle meets two colliders of
+one tree."""
+import math
+
+import numpy as np
+import pytest
+
+from physxcoopa_amd import scenes, _abi
+
+H = np.float32(1.0 / 60.0)
+
+
+def world(oracle, sb=None, cloths=None, strands=None, meshes=None, **cfg):
+    bodies, shapes = (sb or scenes.SceneBuilder()).arrays()
+    return oracle.OracleWorld(_abi.default_config(**cfg), bodies, shapes, cloths=cloths, strands=strands, meshes=meshes)
+
+
+def sheet(oracle, res, center, params=None, **kw):  # tests/support/cloth_fixtures.h:18-28
+    return oracle.grid_cloth(res, res, 4.0, 4.0, center, params=params, **kw)
+
+
+def still_air(**kw):
+    return _abi.ClothParams(damping=0.0, air_drag=0.0, air_lift=0.0, **kw)
+
+
+# ---- builder (cloth_solver_test.cpp:35-77) ----
+
+def test_grid_builder_topology_and_disjoint_batches(oracle):
+    res = 9
+    c = sheet(oracle, res, (0, 0, 2))
+    assert len(c.particles) == res * res and c.columns == res and c.rows == res
+    assert len(c.stretch) == 2 * (res - 1) * res + 2 * (res - 1) * (res - 1)
+    assert len(c.bend) == 2 * (res - 2) * res
+    assert len(c.triangles) == (res - 1) * (res - 1) * 6
+    assert abs(c.stretch[0]["rest_length"] - 4.0 / (res - 1)) < 1e-5
+    for cons, batches in ((c.stretch, c.stretch_batches), (c.bend, c.bend_batches)):
+        covered = 0
+        for b, e in batches:
+            idx = np.concatenate([cons["a"][b:e], cons["b"][b:e]])
+            assert len(np.unique(idx)) == len(idx)
+            covered += e - b
+        assert covered == len(cons)
+    assert abs(c.params.self_distance - 0.6 * 0.5) < 1e-6
+
+
+def test_tethers_are_geodesic_distances(oracle):
+    """build_tethers: a free particle's max length is tether_scale x its shortest path to the
+    nearest pinned particle along stretch edges (checked against a brute-force Dijkstra)."""
+    c = sheet(oracle, 7, (0, 0, 1), pins=[((-2, -2, 1), 0.1), ((2, 2, 1), 0.1)], tethers=True)
+    pinned = {int(a["particle"]) for a in c.anchors}
+    assert pinned == {0, 48}
+    n = len(c.particles)
+    pos = c.particles["position"].astype(np.float64)
+    dist = np.full(n, np.inf)
+    for p in pinned:
+        dist[p] = 0
+    for _ in range(n):  # Bellman-Ford relaxations over the stretch graph
+        for k in c.stretch:
+            a, b = int(k["a"]), int(k["b"])
+            e = np.linalg.norm(pos[a] - pos[b])
+            dist[b] = min(dist[b], dist[a] + e)
+            dist[a] = min(dist[a], dist[b] + e)
+    assert len(c.tethers) == n - 2
+    for t in c.tethers:
+        assert abs(t["max_length"] - 1.02 * dist[t["particle"]]) < 1e-4
+        assert t["anchor"] in pinned
+
+
+# ---- the world's cloth (cloth_solver_test.cpp:81-371) ----
+
+def test_free_fall_matches_analytic(oracle):
+    c = sheet(oracle, 5, (0, 0, 10), still_air())
+    start = c.particles["position"][0].copy()
+    ow = world(oracle, cloths=[c])
+    for _ in range(60):
+        ow.step(H)
+    p = ow.cloth(0).particles["position"][0]
+    t, hs = 60 * float(H), float(H) / 4
+    assert abs((start[2] - p[2]) - (0.5 * 9.81 * t * t + 0.5 * 9.81 * t * hs)) < 0.05
+    assert abs(p[0] - start[0]) < 1e-4 and abs(p[1] - start[1]) < 1e-4
+
+
+def test_advances_when_no_rigid_substep_runs(oracle):
+    c = sheet(oracle, 5, (0, 0, 10), still_air())
+    z0 = c.particles["position"][0][2]
+    ow = world(oracle, cloths=[c])
+    half = np.float32(H * np.float32(0.5))
+    assert ow.step(half) == 0
+    z1 = ow.cloth(0).particles["position"][0][2]
+    assert z0 - z1 > 1e-4
+    ow.step(half)
+    z2 = ow.cloth(0).particles["position"][0][2]
+    assert z1 - z2 > 1e-4
+    t, hs = float(H), float(H) / 4
+    assert abs((z0 - z2) - (0.5 * 9.81 * t * t + 0.5 * 9.81 * t * hs)) < 1e-5
+
+
+def test_static_anchors_hold_and_tethers_cap_distance(oracle):
+    p = _abi.ClothParams(tether_scale=1.02, external_acceleration=(200.0, 0.0, 0.0))
+    c = sheet(oracle, 11, (0, 0, 5), p, pins=[((0, 0, 5), 0.5)], tethers=True)
+    assert len(c.tethers) > 0
+    ow = world(oracle, cloths=[c])
+    for _ in range(180):
+        ow.step(H)
+    sim = ow.cloth(0)
+    pos = sim.particles["position"]
+    for t in sim.tethers:
+        assert np.linalg.norm(pos[t["particle"]] - pos[t["anchor"]]) <= t["max_length"] + 1e-3
+    for a in sim.anchors:
+        np.testing.assert_allclose(pos[a["particle"]], a["world_position"], atol=1e-5)
+        assert sim.particles["inv_mass"][a["particle"]] == 0.0
+
+
+def test_anchor_tracks_kinematic_body_across_uneven_frames(oracle):
+    sb = scenes.SceneBuilder()
+    ball = sb.add_body(_abi.PXG_KINEMATIC, (0, 0, 3), mass=1.0, inv_mass=0.0, use_gravity=False)
+    sb.add_shape(ball, _abi.PXG_SPHERE, radius=1.0)
+    c = sheet(oracle, 15, (0, 0, 4.05), pins=[((0, 0, 4.0), 0.4, (ball, 0), ((0, 0, 3), (0, 0, 0, 1)))], tethers=True)
+    assert len(c.anchors) > 0
+    ow = world(oracle, sb, cloths=[c])
+    for _ in range(120):
+        ow.step(H)
+    sim = ow.cloth(0)
+    bp = ow.bodies()["position"][ball]
+    rest = [sim.particles["position"][a["particle"]] - bp for a in sim.anchors]
+    for frame in range(240):
+        dt = np.float32(H * np.float32(0.9)) if frame % 2 == 0 else np.float32(H * np.float32(1.1))
+        b = ow.bodies()
+        b["position"][ball][0] += np.float32(4.0) * dt
+        b["linear_velocity"][ball] = (4.0, 0.0, 0.0)
+        ow.set_bodies(b)
+        ow.step(dt)
+        live = ow.cloth(0)
+        bp = ow.bodies()["position"][ball]
+        for a, r in zip(live.anchors, rest):
+            np.testing.assert_allclose(live.particles["position"][a["particle"]] - bp, r, atol=1e-3)
+
+
+def collapsed_sheet(oracle, params):
+    c = sheet(oracle, 9, (0, 0, 5), params)
+    c.stretch = c.stretch[:0]
+    c.bend = c.bend[:0]
+    c.stretch_batches = c.stretch_batches[:0]
+    c.bend_batches = c.bend_batches[:0]
+    for i in range(len(c.particles)):
+        f = np.float32(i) * np.float32(0.013)
+        c.particles["position"][i] = (np.sin(f) * np.float32(0.01), np.cos(f) * np.float32(0.01),
+                                      np.float32(5.0) + np.sin(f * np.float32(2)) * np.float32(0.01))
+    c.particles["prev_position"] = c.particles["position"]
+    c.particles["velocity"] = 0
+    return c
+
+
+@pytest.mark.parametrize("on", [True, False])
+def test_self_collision_separates_non_neighbour_particles(oracle, on):
+    p = _abi.ClothParams(self_collision=1 if on else 0, gravity_scale=0.0, air_drag=0.0, air_lift=0.0)
+    c = collapsed_sheet(oracle, p)
+    dmin, cols = c.params.self_distance, c.columns
+    ow = world(oracle, cloths=[c])
+    for _ in range(60):
+        ow.step(H)
+    pos = ow.cloth(0).particles["position"]
+    viol = 0
+    for i in range(len(pos)):
+        for j in range(i + 1, len(pos)):
+            if abs(i % cols - j % cols) <= 1 and abs(i // cols - j // cols) <= 1:
+                continue
+            if np.linalg.norm(pos[i] - pos[j]) < dmin * 0.8:
+                viol += 1
+    assert (viol == 0) if on else (viol > 0)
+
+
+def test_sleeps_when_settled_and_wakes_on_anchor_motion(oracle):
+    sb = scenes.SceneBuilder()
+    post = sb.add_body(_abi.PXG_KINEMATIC, (0, 0, 5), mass=1.0, inv_mass=0.0, use_gravity=False)
+    p = _abi.ClothParams(damping=4.0, sleep_threshold=0.25, sleep_time=0.25)
+    c = sheet(oracle, 9, (0, 0, 5), p, pins=[((0, 0, 5), 3.0, (post, 0), ((0, 0, 5), (0, 0, 0, 1)))])
+    ow = world(oracle, sb, cloths=[c])
+    for _ in range(600):
+        ow.step(H)
+    assert not ow.cloth(0).awake
+    before = ow.cloth(0).particles["position"][-1].copy()
+    for _ in range(60):
+        ow.step(H)
+    np.testing.assert_allclose(ow.cloth(0).particles["position"][-1], before, atol=1e-6)
+    b = ow.bodies()
+    b["position"][post][0] += 0.5
+    ow.set_bodies(b)
+    ow.step(H)
+    assert ow.cloth(0).awake
+
+
+def hashed_drape(oracle):
+    sb = scenes.SceneBuilder()
+    b = sb.add_body(_abi.PXG_STATIC, (0, 0, 3), mass=1.0)
+    sb.add_shape(b, _abi.PXG_SPHERE, radius=1.0)
+    p = _abi.ClothParams(self_collision=1, wind=(0.4, 0.0, 0.0), wind_turbulence=0.6)
+    c = sheet(oracle, 13, (0, 0, 4.05), p, pins=[((0, 0, 4.05), 0.4)], tethers=True)
+    return sb, c
+
+
+def test_stepping_is_deterministic_and_hashes_the_cloth(oracle):
+    def run():
+        sb, c = hashed_drape(oracle)
+        ow = world(oracle, sb, cloths=[c])
+        for _ in range(180):
+            ow.step(H)
+        return ow.state_hash()
+    h1 = run()
+    assert run() == h1
+    assert world(oracle, hashed_drape(oracle)[0]).state_hash() != h1
+
+
+# ---- collisions (cloth_collision_test.cpp) ----
+
+def test_sheet_drapes_over_sphere_without_penetrating(oracle):
+    sb = scenes.SceneBuilder()
+    b = sb.add_body(_abi.PXG_STATIC, (0, 0, 3), mass=1.0)
+    sb.add_shape(b, _abi.PXG_SPHERE, radius=1.0)
+    c = sheet(oracle, 21, (0, 0, 4.05), _abi.ClothParams(thickness=0.03), pins=[((0, 0, 4.05), 0.4)], tethers=True)
+    ow = world(oracle, sb, cloths=[c])
+    for _ in range(300):
+        ow.step(H)
+    sim = ow.cloth(0)
+    pos = sim.particles["position"]
+    assert np.all(np.linalg.norm(pos - np.array([0, 0, 3.0], np.float32), axis=1) >= 1.0)
+    st = sim.stretch
+    assert np.max(np.linalg.norm(pos[st["a"]] - pos[st["b"]], axis=1) / st["rest_length"]) < 1.25
+    assert sim.bounds[2] < 3.0
+
+
+def test_self_collision_cannot_push_a_particle_into_a_collider(oracle):
+    sb = scenes.SceneBuilder()
+    b = sb.add_body(_abi.PXG_STATIC, (0, 0, 0), mass=1.0)
+    sb.add_shape(b, _abi.PXG_SPHERE, radius=1.0)
+    p = _abi.ClothParams(self_collision=1, self_distance=0.2, gravity_scale=0.0, air_drag=0.0, air_lift=0.0,
+                         thickness=0.03, substeps=1)
+    c = sheet(oracle, 9, (0, 0, 50), p)
+    c.stretch, c.bend = c.stretch[:0], c.bend[:0]
+    c.stretch_batches, c.bend_batches = c.stretch_batches[:0], c.bend_batches[:0]
+    pa, pb = 0, 5 * c.columns + 5
+    for i, x in ((pa, 1.06), (pb, 1.04)):
+        c.particles["position"][i] = (x, 0, 0)
+        c.particles["prev_position"][i] = (x, 0, 0)
+        c.particles["velocity"][i] = 0
+    ow = world(oracle, sb, cloths=[c])
+    ow.step(H)
+    pos = ow.cloth(0).particles["position"]
+    assert np.linalg.norm(pos[pa] - pos[pb]) > 0.05
+    assert np.linalg.norm(pos[pa]) >= 1.03 - 1e-4 and np.linalg.norm(pos[pb]) >= 1.03 - 1e-4
+
+
+def test_mesh_collider_stops_tunnelling_through_the_solver(oracle):
+    sb = scenes.SceneBuilder()
+    scenes.mesh_shape_on(sb, 0)
+    c = oracle.grid_cloth(5, 5, 1.0, 1.0, (0, 0, 1), params=_abi.ClothParams(thickness=0.03, air_drag=0.0, air_lift=0.0))
+    c.particles["velocity"] = (0.0, 0.0, -60.0)
+    ow = world(oracle, sb, cloths=[c], meshes=[scenes.two_triangle_floor()])
+    for _ in range(120):
+        ow.step(H)
+    assert np.all(ow.cloth(0).particles["position"][:, 2] > 0.0)
+
+
+def test_sheet_settles_on_mesh_floor_at_thickness(oracle):
+    sb = scenes.SceneBuilder()
+    scenes.mesh_shape_on(sb, 0)
+    c = oracle.grid_cloth(9, 9, 2.0, 2.0, (0, 0, 0.6), params=_abi.ClothParams(thickness=0.03))
+    ow = world(oracle, sb, cloths=[c], meshes=[scenes.two_triangle_floor()])
+    for _ in range(300):
+        ow.step(H)
+    low = ow.cloth(0).particles["position"][:, 2].min()
+    assert low >= 0.0 and abs(low - 0.03) < 5e-3
+
+
+@pytest.mark.parametrize("kind", ["box", "capsule", "hull"])
+def test_sheet_rests_on_each_shape_at_thickness(oracle, kind):
+    """A sheet dropped on a box top, a lying capsule and a cube hull rests with its lowest particle
+    over the shape at the thickness standoff (closed form: the top at z = 1)."""
+    sb = scenes.SceneBuilder()
+    hulls = None
+    b = sb.add_body(_abi.PXG_STATIC, (0, 0, 0), mass=1.0)
+    if kind == "box":
+        sb.add_shape(b, _abi.PXG_BOX, half_extents=(1.5, 1.5, 1.0))
+    elif kind == "capsule":
+        sb.add_shape(b, _abi.PXG_CAPSULE, capsule_radius=1.0, capsule_half_height=1.0, capsule_axis=0)
+    else:
+        hulls = [oracle.build_hull([(sx * 1.5, sy * 1.5, sz * 1.0) for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])]
+        sb.add_shape(b, _abi.PXG_CONVEX_HULL, capsule_axis=0, radius=1.0)
+    c = oracle.grid_cloth(7, 7, 0.6, 0.6, (0, 0, 1.3), params=_abi.ClothParams(thickness=0.03))
+    bodies, shapes = sb.arrays()
+    ow = oracle.OracleWorld(_abi.default_config(), bodies, shapes, cloths=[c], hulls=hulls)
+    for _ in range(240):
+        ow.step(H)
+    pos = ow.cloth(0).particles["position"]
+    centre = pos[len(pos) // 2]
+    assert abs(centre[2] - 1.03) < 4e-3
+    assert np.all(pos[:, 2] > 0.9)
+
+
+def test_tree_order_equals_slot_order_when_colliders_do_not_meet(oracle):
+    """Colliders visited in the trees' order or in slot order give identical cloth when no
+    particle meets two colliders of one tree (here: one static sphere and one static box apart)."""
+    def run(tree):
+        sb = scenes.SceneBuilder()
+        b = sb.add_body(_abi.PXG_STATIC, (0, 0, 3), mass=1.0)
+        sb.add_shape(b, _abi.PXG_SPHERE, radius=0.6)
+        b2 = sb.add_body(_abi.PXG_STATIC, (2.5, 0, 2.5), mass=1.0)
+        sb.add_shape(b2, _abi.PXG_BOX, half_extents=(0.4, 0.4, 0.4))
+        c = sheet(oracle, 11, (0.8, 0, 4), pins=[((0.8, 0, 4), 0.3)], tethers=True)
+        ow = world(oracle, sb, cloths=[c])
+        ow.set_collider_order(tree)
+        for _ in range(120):
+            ow.step(H)
+        return ow.state_hash()
+    assert run(True) == run(False)
+
+
+# ---- hair (hair_solver_test.cpp) ----
+
+def straight(root, count=8, spacing=0.05):
+    return [(root[0] + spacing * k, root[1], root[2]) for k in range(count)]
+
+
+def bent(root, spacing=0.05):
+    return [(root[0] + spacing * k, root[1], root[2]) for k in range(4)] + \
+           [(root[0] + spacing * 3, root[1] + spacing * k, root[2]) for k in range(1, 5)]
+
+
+def groom_world(oracle, strands, params, sb=None, iterations=4):
+    s = oracle.make_strands(strands, params)
+    return world(oracle, sb, strands=[s], cloth_iterations=iterations)
+
+
+def run_frames(ow, n):
+    for _ in range(n):
+        ow.step(H)
+    return ow.strand_set(0)
+
+
+def strand_len(s, i=0):
+    st = s.strands[i]
+    p = s.particles["position"][st["first"]:st["first"] + st["count"]].astype(np.float64)
+    return float(np.sum(np.linalg.norm(np.diff(p, axis=0), axis=1)))
+
+
+def test_roots_are_pinned_and_the_rest_is_not(oracle):
+    ow = groom_world(oracle, [straight((0, 0, 0))], _abi.StrandParams())
+    s0 = ow.strand_set(0)
+    assert len(s0.particles) == 8 and s0.particles["inv_mass"][0] == 0 and s0.particles["inv_mass"][1] > 0
+    assert len(s0.stretch) == 7 and len(s0.stretch_batches) == 2
+    s = run_frames(ow, 60)
+    assert np.linalg.norm(s.particles["position"][0]) < 1e-5
+    assert s.particles["position"][7][2] < -0.005
+
+
+def test_a_hanging_strand_does_not_stretch(oracle):
+    ow = groom_world(oracle, [straight((0, 0, 0), 10)], _abi.StrandParams(global_shape=0.0, local_shape=0.0))
+    rest = strand_len(ow.strand_set(0))
+    assert abs(rest - 0.45) < 1e-4
+    s = run_frames(ow, 180)
+    assert abs(strand_len(s) - rest) < 0.01 * rest and s.particles["position"][9][2] < -0.3
+
+
+def test_the_global_shape_constraint_holds_the_groom_up(oracle):
+    def tip(g):
+        return run_frames(groom_world(oracle, [straight((0, 0, 0))], _abi.StrandParams(global_shape=g, local_shape=0.0)),
+                          180).particles["position"][7][2]
+    limp, stiff = tip(0.0), tip(0.6)
+    assert limp < -0.1 and stiff > limp + 0.1
+
+
+def test_the_global_constraint_falls_off_toward_the_tip(oracle):
+    ow = groom_world(oracle, [straight((0, 0, 0), 9)],
+                     _abi.StrandParams(global_shape=0.5, global_shape_falloff=2.0, local_shape=0.0))
+    start = ow.strand_set(0).particles["position"].copy()
+    p = run_frames(ow, 180).particles["position"]
+    assert np.linalg.norm(p[8] - start[8]) > 2 * np.linalg.norm(p[2] - start[2])
+
+
+def test_the_local_shape_constraint_holds_a_curl(oracle):
+    def corner(l):
+        return abs(run_frames(groom_world(oracle, [bent((0, 0, 0))], _abi.StrandParams(global_shape=0.0, local_shape=l)),
+                              240).particles["position"][-1][1])
+    assert corner(0.9) > corner(0.0) + 0.02
+
+
+def test_moving_the_root_carries_the_strand(oracle):
+    ow = groom_world(oracle, [straight((0, 0, 0))], _abi.StrandParams(global_shape=0.4), iterations=1)
+    for i in range(120):
+        ow.set_strand_roots(0, [(np.float32(0.01) * np.float32(i), 0.0, 0.0)])
+        ow.step(H)
+    s = ow.strand_set(0)
+    p = s.particles["position"]
+    assert abs(p[0][0] - 1.19) < 1e-3 and p[7][0] > 0.6
+    assert abs(strand_len(s) - 0.35) < 0.01 and np.linalg.norm(p[7] - p[0]) <= 0.355
+
+
+def test_a_strand_stays_out_of_a_collider(oracle):
+    sb = scenes.SceneBuilder()
+    b = sb.add_body(_abi.PXG_STATIC, (0, 0, 0), mass=1.0)
+    sb.add_shape(b, _abi.PXG_SPHERE, radius=0.3)
+    ow = groom_world(oracle, [straight((0, 0, 0.6), 12)], _abi.StrandParams(global_shape=0.0, local_shape=0.0, thickness=0.02),
+                     sb)
+    p = run_frames(ow, 300).particles["position"]
+    assert np.min(np.linalg.norm(p, axis=1) - 0.3) > -0.005
+
+
+def test_a_settled_groom_goes_to_sleep_and_degenerate_input_builds_nothing(oracle):
+    ow = groom_world(oracle, [straight((0, 0, 0))], _abi.StrandParams(sleep_time=0.2, sleep_threshold=0.05), iterations=1)
+    assert not run_frames(ow, 400).awake
+    assert len(oracle.make_strands([[(0, 0, 0)]]).particles) == 0
+
+
+def test_hair_trig_modes_agree_to_rounding(oracle):
+    """The groom's sin, pow and acos as the reference's float libm calls, or in double rounded
+    once (the GPU's): the two runs stay within a few micrometres over 120 frames."""
+    def run(rounded):
+        ow = groom_world(oracle, [bent((0, 0, 0)), straight((0.1, 0, 0))],
+                         _abi.StrandParams(wind=(1.0, 0.3, 0.0), wind_turbulence=0.5))
+        ow.set_atan2_mode(rounded)
+        for i in range(120):
+            ow.set_strand_roots(0, [(0.002 * i, 0, 0), (0.1, 0, 0.001 * i)],
+                                [(0, 0, math.sin(0.01 * i), math.cos(0.01 * i))] * 2)
+            ow.step(H)
+        return ow.strand_set(0).particles["position"].copy()
+    np.testing.assert_allclose(run(True), run(False), atol=5e-5)
