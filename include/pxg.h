@@ -116,7 +116,8 @@ typedef struct pxg_body {
     uint32_t awake;
     uint32_t use_gravity;
     uint32_t constraints;   /* PXG_FREEZE_* bits */
-    uint32_t generation;    /* carried, not interpreted */
+    uint32_t generation;    /* BodyId generation: cloth anchors and groom collide bodies name a body
+                             * by (index, generation) and follow it only while both match */
     uint32_t interpolation; /* pxg_interpolation; 0 = PXG_INTERPOLATE, the reference's default */
     uint32_t reserved1;
     float position[3];
@@ -588,6 +589,10 @@ int pxg_step_fixed(pxg_world* w, float h);                        /* step_fixed(
  * Bodies may be re-uploaded between the two. */
 int pxg_step_phase(pxg_world* w, uint32_t phase, float h);
 int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run);     /* step(), world.h:1229 */
+/* The once-per-frame cloth and hair pass that ends step() (world.h:1241-1256), for callers that
+ * run the fixed steps themselves: frame is the clamped frame time, substeps_run the fixed steps
+ * the frame ran (0: the broadphase boxes are synced over the frame first). pxg_step calls it. */
+int pxg_step_soft(pxg_world* w, float frame, uint32_t substeps_run);
 int pxg_synchronize(pxg_world* w);
 
 /* --- results --- */

@@ -13,6 +13,7 @@
 #pragma once
 
 #include "pxg.h"
+#include "pxg_soft.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -235,6 +236,7 @@ public:
         pxg_body& b = bodies_[index];
         b = initial;
         b.alive = 1;
+        b.generation = generations_[index];  // cloth anchors and groom collide lists name bodies by BodyId
         std::memcpy(b.prev_position, b.position, sizeof(b.position));
         std::memcpy(b.prev_orientation, b.orientation, sizeof(b.orientation));
         pxg_shape s = shape;
@@ -452,6 +454,11 @@ public:
             ++n;
         }
         interpolation_alpha_ = accumulator_ / h;
+        if (frame > 0.0f && (live_count(cloth_alive_) || live_count(strand_alive_))) {  // world.h:1241-1256
+            push();
+            check(pxg_step_soft(w_, frame, n), "step (cloth and hair)");
+            soft_on_device_ = true;
+        }
         return n;
     }
 
@@ -747,12 +754,104 @@ public:
         geometry_dirty_ = structure_dirty_ = true;
     }
 
+    // --- cloth and hair (world.h:552-700) ---
+    // The tables live on the GPU between frames: the first get_cloth / get_strands /
+    // for_each_* after a step downloads the alive entries; handing them out for writing sends the
+    // tables up again before the next step. Anchors and collide lists name bodies by BodyId
+    // (index, generation); a stale one pins in place / is skipped, as in the reference.
+    ClothId add_cloth(Cloth&& c) {
+        if (c.particles.empty()) return ClothId{};
+        pull_soft();
+        compute_bounds(c.particles, c.bounds_min, c.bounds_max);
+        const uint32_t index = take_slot(cloths_, cloth_generations_, cloth_alive_, free_cloths_);
+        cloths_[index] = std::move(c);
+        cloths_dirty_ = true;
+        return ClothId{index, cloth_generations_[index]};
+    }
+    bool is_valid(ClothId id) const {
+        return id.is_valid() && id.index < cloths_.size() && cloth_alive_[id.index] &&
+               cloth_generations_[id.index] == id.generation;
+    }
+    void remove_cloth(ClothId id) {
+        if (!is_valid(id)) return;
+        pull_soft();
+        cloths_[id.index] = Cloth{};
+        cloth_alive_[id.index] = 0;
+        ++cloth_generations_[id.index];
+        free_cloths_.push_back(id.index);
+        cloths_dirty_ = true;
+    }
+    Cloth* get_cloth(ClothId id) {
+        if (!is_valid(id)) return nullptr;
+        pull_soft();
+        cloths_dirty_ = true;
+        return &cloths_[id.index];
+    }
+    const Cloth* get_cloth(ClothId id) const {
+        if (!is_valid(id)) return nullptr;
+        const_cast<PhysicsWorld*>(this)->pull_soft();
+        return &cloths_[id.index];
+    }
+    std::size_t cloth_count() const { return live_count(cloth_alive_); }
+    template <typename Fn>
+    void for_each_cloth(Fn&& fn) {
+        pull_soft();
+        cloths_dirty_ = true;
+        for (uint32_t i = 0; i < cloths_.size(); ++i)
+            if (cloth_alive_[i]) fn(ClothId{i, cloth_generations_[i]}, cloths_[i]);
+    }
+
+    StrandSetId add_strands(StrandSet&& set) {
+        if (set.particles.empty() || set.strands.empty()) return StrandSetId{};
+        pull_soft();
+        compute_bounds(set.particles, set.bounds_min, set.bounds_max);
+        const uint32_t index = take_slot(strand_sets_, strand_generations_, strand_alive_, free_strands_);
+        strand_sets_[index] = std::move(set);
+        strands_dirty_ = true;
+        return StrandSetId{index, strand_generations_[index]};
+    }
+    bool is_valid(StrandSetId id) const {
+        return id.is_valid() && id.index < strand_sets_.size() && strand_alive_[id.index] &&
+               strand_generations_[id.index] == id.generation;
+    }
+    void remove_strands(StrandSetId id) {
+        if (!is_valid(id)) return;
+        pull_soft();
+        strand_sets_[id.index] = StrandSet{};
+        strand_alive_[id.index] = 0;
+        ++strand_generations_[id.index];
+        free_strands_.push_back(id.index);
+        strands_dirty_ = true;
+    }
+    StrandSet* get_strands(StrandSetId id) {
+        if (!is_valid(id)) return nullptr;
+        pull_soft();
+        strands_dirty_ = true;
+        return &strand_sets_[id.index];
+    }
+    const StrandSet* get_strands(StrandSetId id) const {
+        if (!is_valid(id)) return nullptr;
+        const_cast<PhysicsWorld*>(this)->pull_soft();
+        return &strand_sets_[id.index];
+    }
+    std::size_t strand_set_count() const { return live_count(strand_alive_); }
+    template <typename Fn>
+    void for_each_strands(Fn&& fn) {
+        pull_soft();
+        strands_dirty_ = true;
+        for (uint32_t i = 0; i < strand_sets_.size(); ++i)
+            if (strand_alive_[i]) fn(StrandSetId{i, strand_generations_[i]}, strand_sets_[i]);
+    }
+    // A body's BodyId as a groom's collide-list entry or a cloth anchor's (index, generation).
+    std::pair<uint32_t, uint32_t> body_ref(BodyId id) const { return {id.index, id.generation}; }
+
     pxg_world* handle() { return w_; }
 
 private:
     // A new handle with room for at least n bodies; the warm-start cache moves over (the caller
     // restores the fat AABBs).
     void grow(uint32_t n) {
+        pull_soft();  // the cloth and hair state on the old handle
         uint32_t cap = cfg_.max_bodies;
         while (cap < n) cap *= 2;
         uint32_t nk = 0;
@@ -772,6 +871,8 @@ private:
         geometry_dirty_ = !hull_arrays_.empty() || !mesh_arrays_.empty();
         joints_dirty_ = !joints_.empty();
         ignored_dirty_ = !ignored_.empty();
+        cloths_dirty_ = !cloths_.empty();
+        strands_dirty_ = !strand_sets_.empty();
         pending_cache_keys_ = std::move(keys);
         pending_cache_imp_ = std::move(imp);
     }
@@ -1018,6 +1119,135 @@ private:
             bodies_dirty_.store(false, std::memory_order_relaxed);
         }
         push_joints();
+        push_soft();
+    }
+
+    // ---- cloth and hair tables ----
+    template <class T>
+    static uint32_t take_slot(std::vector<T>& items, std::vector<uint32_t>& gens, std::vector<uint8_t>& alive,
+                              std::vector<uint32_t>& free_list) {
+        uint32_t index;
+        if (!free_list.empty()) {
+            index = free_list.back();
+            free_list.pop_back();
+        } else {
+            index = static_cast<uint32_t>(items.size());
+            items.emplace_back();
+            gens.push_back(0);
+            alive.push_back(0);
+        }
+        alive[index] = 1;
+        return index;
+    }
+    static std::size_t live_count(const std::vector<uint8_t>& alive) {
+        std::size_t n = 0;
+        for (uint8_t a : alive) n += a ? 1 : 0;
+        return n;
+    }
+    static pxg_cloth cloth_record(Cloth& c, bool alive) {
+        pxg_cloth d;
+        std::memset(&d, 0, sizeof(d));
+        d.particles = c.particles.data();
+        d.n_particles = static_cast<uint32_t>(c.particles.size());
+        d.columns = c.columns;
+        d.rows = c.rows;
+        d.stretch = c.stretch.data();
+        d.n_stretch = static_cast<uint32_t>(c.stretch.size());
+        d.bend = c.bend.data();
+        d.n_bend = static_cast<uint32_t>(c.bend.size());
+        d.stretch_batches = c.stretch_batches.data();
+        d.n_stretch_batches = static_cast<uint32_t>(c.stretch_batches.size() / 2);
+        d.bend_batches = c.bend_batches.data();
+        d.n_bend_batches = static_cast<uint32_t>(c.bend_batches.size() / 2);
+        d.tethers = c.tethers.data();
+        d.n_tethers = static_cast<uint32_t>(c.tethers.size());
+        d.anchors = c.anchors.data();
+        d.n_anchors = static_cast<uint32_t>(c.anchors.size());
+        d.triangles = c.triangles.data();
+        d.n_triangle_indices = static_cast<uint32_t>(c.triangles.size());
+        d.params = c.params;
+        d.alive = alive ? 1u : 0u;
+        d.enabled = c.enabled ? 1u : 0u;
+        d.awake = c.awake ? 1u : 0u;
+        d.sleep_timer = c.sleep_timer;
+        d.time = c.time;
+        std::memcpy(d.bounds_min, c.bounds_min, sizeof(d.bounds_min));
+        std::memcpy(d.bounds_max, c.bounds_max, sizeof(d.bounds_max));
+        return d;
+    }
+    static pxg_strand_set strand_record(StrandSet& s, std::vector<uint32_t>& ids, bool alive) {
+        pxg_strand_set d;
+        std::memset(&d, 0, sizeof(d));
+        d.particles = s.particles.data();
+        d.n_particles = static_cast<uint32_t>(s.particles.size());
+        d.stretch = s.stretch.data();
+        d.n_stretch = static_cast<uint32_t>(s.stretch.size());
+        d.stretch_batches = s.stretch_batches.data();
+        d.n_stretch_batches = static_cast<uint32_t>(s.stretch_batches.size() / 2);
+        d.rest_local = s.rest_local.data();
+        d.rest_offset = s.rest_offset.data();
+        d.strands = s.strands.data();
+        d.n_strands = static_cast<uint32_t>(s.strands.size());
+        ids.clear();
+        for (const auto& b : s.collide_bodies) {
+            ids.push_back(b.first);
+            ids.push_back(b.second);
+        }
+        d.collide_bodies = ids.data();
+        d.n_collide_bodies = static_cast<uint32_t>(s.collide_bodies.size());
+        d.params = s.params;
+        d.alive = alive ? 1u : 0u;
+        d.enabled = s.enabled ? 1u : 0u;
+        d.awake = s.awake ? 1u : 0u;
+        d.sleep_timer = s.sleep_timer;
+        d.time = s.time;
+        std::memcpy(d.bounds_min, s.bounds_min, sizeof(d.bounds_min));
+        std::memcpy(d.bounds_max, s.bounds_max, sizeof(d.bounds_max));
+        return d;
+    }
+    void push_soft() {
+        if (cloths_dirty_) {
+            std::vector<pxg_cloth> recs;
+            for (size_t i = 0; i < cloths_.size(); ++i) recs.push_back(cloth_record(cloths_[i], cloth_alive_[i] != 0));
+            check(pxg_set_cloths(w_, recs.data(), static_cast<uint32_t>(recs.size())), "cloths");
+            cloths_dirty_ = false;
+        }
+        if (strands_dirty_) {
+            std::vector<std::vector<uint32_t>> ids(strand_sets_.size());
+            std::vector<pxg_strand_set> recs;
+            for (size_t i = 0; i < strand_sets_.size(); ++i)
+                recs.push_back(strand_record(strand_sets_[i], ids[i], strand_alive_[i] != 0));
+            check(pxg_set_strand_sets(w_, recs.data(), static_cast<uint32_t>(recs.size())), "strand sets");
+            strands_dirty_ = false;
+        }
+    }
+    // device -> host: the alive cloths' and grooms' state after a frame
+    void pull_soft() {
+        if (!soft_on_device_) return;
+        std::vector<uint32_t> ids;
+        for (uint32_t i = 0; i < cloths_.size(); ++i) {
+            if (!cloth_alive_[i]) continue;
+            Cloth& c = cloths_[i];
+            pxg_cloth d = cloth_record(c, true);
+            check(pxg_get_cloth(w_, i, &d), "get_cloth");
+            c.awake = d.awake != 0;
+            c.sleep_timer = d.sleep_timer;
+            c.time = d.time;
+            std::memcpy(c.bounds_min, d.bounds_min, sizeof(c.bounds_min));
+            std::memcpy(c.bounds_max, d.bounds_max, sizeof(c.bounds_max));
+        }
+        for (uint32_t i = 0; i < strand_sets_.size(); ++i) {
+            if (!strand_alive_[i]) continue;
+            StrandSet& s = strand_sets_[i];
+            pxg_strand_set d = strand_record(s, ids, true);
+            check(pxg_get_strand_set(w_, i, &d), "get_strands");
+            s.awake = d.awake != 0;
+            s.sleep_timer = d.sleep_timer;
+            s.time = d.time;
+            std::memcpy(s.bounds_min, d.bounds_min, sizeof(s.bounds_min));
+            std::memcpy(s.bounds_max, d.bounds_max, sizeof(s.bounds_max));
+        }
+        soft_on_device_ = false;
     }
 
     struct Deferred {
@@ -1078,6 +1308,11 @@ private:
         geometry_dirty_ = false;
     }
     float accumulator_ = 0.0f;
+    std::vector<Cloth> cloths_;
+    std::vector<StrandSet> strand_sets_;
+    std::vector<uint32_t> cloth_generations_, free_cloths_, strand_generations_, free_strands_;
+    std::vector<uint8_t> cloth_alive_, strand_alive_;
+    bool cloths_dirty_ = false, strands_dirty_ = false, soft_on_device_ = false;
     mutable std::vector<pxg_manifold> manifolds_;
     mutable bool manifolds_valid_ = false;
     mutable std::vector<ContactEvent> events_;

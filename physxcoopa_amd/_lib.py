@@ -41,6 +41,7 @@ _SIGNATURES = {
     "pxg_step_fixed": ([ctypes.c_void_p, ctypes.c_float], ctypes.c_int),
     "pxg_step_phase": ([ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float], ctypes.c_int),
     "pxg_step": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p], ctypes.c_int),
+    "pxg_step_soft": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_uint32], ctypes.c_int),
     "pxg_synchronize": ([ctypes.c_void_p], ctypes.c_int),
     "pxg_get_stats": ([ctypes.c_void_p, ctypes.c_void_p], ctypes.c_int),
     "pxg_get_pairs": ([ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p], ctypes.c_int),

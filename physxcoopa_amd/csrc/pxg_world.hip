@@ -1187,6 +1187,12 @@ extern "C" int pxg_step(pxg_world* w, float dt, uint32_t* substeps_run) {
     return step_soft(w, frame, n);
 }
 
+extern "C" int pxg_step_soft(pxg_world* w, float frame, uint32_t substeps_run) {
+    if (!w) return fail(PXG_ERR_INVALID, "pxg_step_soft: null handle");
+    if (w->phase != 0) return fail(PXG_ERR_STATE, "pxg_step_soft: not allowed between the two step phases");
+    return step_soft(w, std::min(frame, 0.25f), substeps_run);
+}
+
 // ---------------------------------------------------------------------------------------------
 // contact events (world.h:744-748, 1998-2018)
 // ---------------------------------------------------------------------------------------------

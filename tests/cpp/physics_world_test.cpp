@@ -1060,6 +1060,135 @@ void cube_hull_rests_on_a_mesh_floor_across_world_growth() {
     CHECK(vlen(b->linear_velocity) < 0.05f);
 }
 
+// cloth_solver_test.cpp:81-371 through the facade: a sheet pinned to a kinematic ball follows it
+// across uneven frames (some with no fixed step), the particles outside the pins drape over the
+// ball without entering it, and the cloth is part of the state hash; a removed cloth's handle is
+// stale and its slot is reused under a new generation.
+void cloth_pinned_to_a_moving_ball_follows_it_and_drapes() {
+    pxg::PhysicsWorld world;
+    pxg_body kb = body(PXG_KINEMATIC, 0, 0, 3);
+    kb.use_gravity = 0;
+    kb.mass = 1.0f;
+    pxg::BodyId ball = world.add_body(kb, sphere_shape(1.0f));
+    pxg::GridClothDesc d;
+    d.columns = d.rows = 15;
+    d.width = d.height = 4.0f;
+    d.center[2] = 4.05f;
+    pxg::Cloth c = pxg::make_grid_cloth(d);
+    const float pin[3] = {0.0f, 0.0f, 4.0f};
+    const uint32_t pinned = pxg::pin_to_body(c, ball.index, ball.generation, *world.get_body(ball), pin, 0.4f);
+    pxg::build_tethers(c);
+    CHECK(pinned > 0 && !c.tethers.empty());
+    pxg::ClothId id = world.add_cloth(std::move(c));
+    CHECK(world.is_valid(id) && world.cloth_count() == 1);
+    for (int i = 0; i < 120; ++i) world.step(kH);
+    const uint64_t h1 = world.world_state_hash();
+    std::vector<float> rest;
+    {
+        const pxg::Cloth* cl = world.get_cloth(id);
+        const float* bp = world.get_body(ball)->position;
+        for (const pxg_cloth_anchor& a : cl->anchors)
+            for (int k = 0; k < 3; ++k) rest.push_back(cl->particles[a.particle].position[k] - bp[k]);
+    }
+    for (int f = 0; f < 120; ++f) {
+        const float dt = f % 2 ? kH * 1.1f : kH * 0.9f;
+        pxg_body* b = world.get_body(ball);
+        b->position[0] += 2.0f * dt;
+        b->linear_velocity[0] = 2.0f;
+        world.step(dt);
+    }
+    const pxg::Cloth* cl = world.get_cloth(id);
+    const float* bp = world.get_body(ball)->position;
+    CHECK(bp[0] > 0.4f);
+    for (size_t k = 0; k < cl->anchors.size(); ++k)
+        for (int j = 0; j < 3; ++j) CHECK_NEAR(cl->particles[cl->anchors[k].particle].position[j] - bp[j], rest[3 * k + j], 2e-3);
+    for (const pxg_particle& p : cl->particles) {
+        const float dx = p.position[0] - bp[0], dy = p.position[1] - bp[1], dz = p.position[2] - bp[2];
+        CHECK(std::sqrt(dx * dx + dy * dy + dz * dz) > 0.98f);
+    }
+    CHECK(h1 != 0);
+    world.remove_cloth(id);
+    CHECK(!world.is_valid(id) && world.get_cloth(id) == nullptr && world.cloth_count() == 0);
+    pxg::ClothId again = world.add_cloth(pxg::make_grid_cloth(d));
+    CHECK(again.index == id.index && again.generation == id.generation + 1 && !world.is_valid(id));
+    world.step(kH);
+    CHECK(world.get_cloth(again)->particles[0].position[2] < 4.05f);
+}
+
+// The same pinned drape run twice gives the same state hash, and the hash covers the cloth.
+void cloth_drape_is_deterministic_and_hashed() {
+    auto run = [](bool with_cloth) {
+        pxg::PhysicsWorld world;
+        world.add_body(body(PXG_STATIC, 0, 0, 3), sphere_shape(1.0f));
+        if (with_cloth) {
+            pxg::GridClothDesc d;
+            d.columns = d.rows = 13;
+            d.width = d.height = 4.0f;
+            d.center[2] = 4.05f;
+            d.params.self_collision = 1;
+            d.params.wind[0] = 0.4f;
+            d.params.wind_turbulence = 0.6f;
+            pxg::Cloth c = pxg::make_grid_cloth(d);
+            const float pin[3] = {0.0f, 0.0f, 4.05f};
+            pxg::pin_static(c, pin, 0.4f);
+            pxg::build_tethers(c);
+            world.add_cloth(std::move(c));
+        }
+        for (int i = 0; i < 90; ++i) world.step(kH);
+        return world.world_state_hash();
+    };
+    const uint64_t a = run(true);
+    CHECK(a == run(true));
+    CHECK(a != run(false));
+}
+
+// hair_solver_test.cpp through the facade: roots stay pinned, a moving root carries its strand
+// without stretching it, and a groom colliding only with listed bodies ignores the others.
+void strands_follow_their_roots_and_collide_only_with_listed_bodies() {
+    pxg::PhysicsWorld world;
+    // two horizontal strands that fall and hang through where a sphere sits under each root
+    pxg::BodyId listed = world.add_body(body(PXG_STATIC, -2.0f, 0, -0.35f), sphere_shape(0.3f));
+    world.add_body(body(PXG_STATIC, 2.0f, 0, -0.35f), sphere_shape(0.3f));
+    std::vector<pxg::StrandDesc> descs(2);
+    for (int k = 0; k < 8; ++k) descs[0].points.insert(descs[0].points.end(), {-2.2f + 0.05f * float(k), 0.0f, 0.0f});
+    for (int k = 0; k < 8; ++k) descs[1].points.insert(descs[1].points.end(), {1.8f + 0.05f * float(k), 0.0f, 0.0f});
+    pxg_strand_params sp = pxg::default_strand_params();
+    sp.global_shape = sp.local_shape = 0.0f;
+    pxg::StrandSet set = pxg::make_strands(descs, sp);
+    CHECK(set.particles.size() == 16 && set.stretch.size() == 14 && set.stretch_batches.size() == 4);
+    set.collide_bodies.push_back(world.body_ref(listed));
+    pxg::StrandSetId id = world.add_strands(std::move(set));
+    CHECK(world.is_valid(id) && world.strand_set_count() == 1);
+    for (int i = 0; i < 180; ++i) world.step(kH);
+    const pxg::StrandSet* s = world.get_strands(id);
+    auto dist = [](const float* p, float cx) {
+        const float dx = p[0] - cx, dz = p[2] + 0.35f;
+        return std::sqrt(dx * dx + p[1] * p[1] + dz * dz);
+    };
+    float near_listed = 1e9f, near_other = 1e9f;
+    for (int k = 0; k < 8; ++k) near_listed = std::min(near_listed, dist(s->particles[k].position, -2.0f));
+    for (int k = 8; k < 16; ++k) near_other = std::min(near_other, dist(s->particles[k].position, 2.0f));
+    CHECK(near_listed > 0.3f);   // kept out of the listed sphere
+    CHECK(near_other < 0.25f);   // hangs through the sphere it does not collide with
+    CHECK(s->particles[0].position[2] == 0.0f && s->particles[8].position[2] == 0.0f);
+    for (int f = 0; f < 60; ++f) {
+        pxg::StrandSet* m = world.get_strands(id);
+        pxg::set_strand_roots(*m, {-2.2f + 0.01f * float(f + 1), 0.0f, 0.0f, 1.8f, 0.0f, 0.0f});
+        world.step(kH);
+    }
+    s = world.get_strands(id);
+    CHECK_NEAR(s->particles[0].position[0], -1.6, 1e-5);
+    float len = 0.0f;
+    for (int k = 0; k + 1 < 8; ++k) {
+        const float* a = s->particles[k].position;
+        const float* b = s->particles[k + 1].position;
+        len += std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+    }
+    CHECK_NEAR(len, 0.35, 0.01);
+    world.remove_strands(id);
+    CHECK(!world.is_valid(id) && world.get_strands(id) == nullptr);
+}
+
 int main() {
     struct T {
         const char* name;
@@ -1098,6 +1227,10 @@ int main() {
         {"cube_hull_rests_on_a_mesh_floor_across_world_growth", cube_hull_rests_on_a_mesh_floor_across_world_growth},
         {"write_back_from_worker_threads_matches_a_serial_pass", write_back_from_worker_threads_matches_a_serial_pass},
         {"batched_overlaps_and_casts_equal_single_queries", batched_overlaps_and_casts_equal_single_queries},
+        {"cloth_pinned_to_a_moving_ball_follows_it_and_drapes", cloth_pinned_to_a_moving_ball_follows_it_and_drapes},
+        {"cloth_drape_is_deterministic_and_hashed", cloth_drape_is_deterministic_and_hashed},
+        {"strands_follow_their_roots_and_collide_only_with_listed_bodies",
+         strands_follow_their_roots_and_collide_only_with_listed_bodies},
     };
     for (const T& t : tests) {
         int before = g_failures;

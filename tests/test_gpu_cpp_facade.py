@@ -15,4 +15,4 @@ def test_cpp_physics_world_suite():
     out = r.stdout.decode()
     print(out)
     assert r.returncode == 0, out
-    assert out.count("PASS ") >= 12
+    assert out.count("PASS ") >= 33 and "FAIL " not in out

@@ -430,3 +430,23 @@ def test_hair_trig_modes_agree_to_rounding(oracle):
             ow.step(H)
         return ow.strand_set(0).particles["position"].copy()
     np.testing.assert_allclose(run(True), run(False), atol=5e-5)
+
+
+def test_state_hash_is_fnv_over_bodies_then_alive_cloth_particles(oracle):
+    """world.h:1334-1359: after the bodies, each alive cloth's particles in order, position then
+    velocity (3 x f32 each); a dead cloth slot is left out. Checked against the numpy FNV-1a that
+    the published test vectors pin (tests/test_oracle_residual.py)."""
+    from test_oracle_residual import fnv1a64, state_hash_numpy
+    sb, c = hashed_drape(oracle)
+    dead = c.copy()
+    dead.alive = False
+    ow = world(oracle, sb, cloths=[c, dead, c])
+    for _ in range(30):
+        ow.step(H)
+    h = np.uint64(state_hash_numpy(ow.bodies()))
+    for k in (0, 2):
+        p = ow.cloth(k).particles
+        data = b"".join(p[i]["position"].tobytes() + p[i]["velocity"].tobytes() for i in range(len(p)))
+        h = np.uint64(fnv1a64(data, h))
+    assert ow.state_hash() == int(h)
+    assert int(h) != state_hash_numpy(ow.bodies())
