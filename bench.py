@@ -529,7 +529,8 @@ def bench_pile(args):
     runs = {}
     worlds = {}
     snap = snap_fixed = None
-    plan = [("plain", _abi.PXG_ORDER_REFERENCE, None), ("batched_learned", _abi.PXG_ORDER_COLOURED, net)]
+    batched_order = _abi.PXG_ORDER_TILED if args.batched_order == "tiled" else _abi.PXG_ORDER_COLOURED
+    plan = [("plain", _abi.PXG_ORDER_REFERENCE, None), ("batched_learned", batched_order, net)]
     if args.only:
         plan = [p for p in plan if p[0] == args.only]
     tol = 1e-4
@@ -571,9 +572,16 @@ def bench_pile(args):
                       "manifolds": st["manifolds"], "points": st["points"], "colours": st["colours"],
                       "awake_bodies": st["awake_bodies"], "learned_predictions": st["learned_predictions"],
                       "fixed_manifolds": st_f["manifolds"], "prof": prof}
-        log("%s: residual mode %.2f ms/step (passes %s, %d/%d steps reached %g), fixed %.2f ms/step" %
+        log("%s: residual mode %.2f ms/step (passes %s, %d/%d steps reached %g), fixed %.2f ms/step, colours %d, "
+            "tiled workgroups %d (in LDS %d, largest tile %d KiB)" %
             (name, runs[name]["ms_per_step"], runs[name]["passes"]["per_step"], runs[name]["reached"], args.steps,
-             tol, runs[name]["fixed_iterations_ms_per_step"]))
+             tol, runs[name]["fixed_iterations_ms_per_step"], st["colours"], st["tiled_workgroups"] & 0xFF,
+             (st["tiled_workgroups"] >> 8) & 0xFF, st["tiled_workgroups"] >> 16))
+        tt = prof.get("tiled_ticks") or {}
+        n_pass = sum(r["passes"] for r in per)
+        if tt and tt.get("colours") and n_pass:
+            runs[name]["tiled_us_per_pass"] = {k: round(v * 0.01 / n_pass, 3) for k, v in tt.items()}
+            log("%s: tiled solve, first tile's workgroup, us per velocity pass: %s" % (name, runs[name]["tiled_us_per_pass"]))
     head = runs.get("batched_learned") or runs["plain"]
     cpu = cpu_fixed = None
     if not args.no_cpu and snap is not None:
@@ -661,7 +669,8 @@ def bench_pile(args):
             "residual_max": round(float(runs["plain"]["residual_max"]), 7)},
         "vs_plain_gpu": round(head["value"] / runs["plain"]["value"], 3) if "plain" in runs and "batched_learned" in runs else None,
         "vs_cpu": round(head["value"] / cpu["value"], 2) if cpu else None,
-        "roofline": roofline_of(head["prof"]),
+        "roofline": roofline_of(head["prof"], "k_solve_tiled" if head is runs.get("batched_learned")
+                                and args.batched_order == "tiled" else "k_solve_persistent"),
         "traffic_profiled": profiled_traffic(key),
         "cpu_baseline": cpu,
         "cpu_baseline_fixed_iterations": cpu_fixed,
@@ -688,6 +697,9 @@ def main():
     ap.add_argument("--fixed-steps", type=int, default=60, help="pile: steps timed at the fixed iteration counts")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batched-order", choices=["tiled", "coloured"], default="tiled",
+                    help="pile: the batched solver's order (colour batches within spatial tiles, each tile solved "
+                         "in one workgroup's LDS; or greedy colour batches over the whole pile)")
     ap.add_argument("--only", choices=["plain", "batched_learned"], default=None,
                     help="pile: run one of the two configurations only (for profiling)")
     ap.add_argument("--workload", choices=["pile", "batch", "ragdolls", "learned", "queries", "aggregates"], default="pile",

@@ -94,7 +94,10 @@ typedef struct pxg_config {
      * the reference's manifold order (pair order, broadphase/pair_cache.h:23-52), so the result
      * is the reference's sequential Gauss-Seidel result; manifolds that share no such body run in
      * parallel (colours = longest chains of that order). PXG_ORDER_COLOURED: greedy colouring,
-     * fewer colours (faster), a different but equally valid sequential order. */
+     * fewer colours (faster), a different but equally valid sequential order. PXG_ORDER_TILED:
+     * greedy colouring within spatial tiles of the awake bodies (manifolds crossing tiles in two
+     * later phases), each tile solved by one workgroup in LDS: the fastest order for large piles;
+     * worlds with joints use PXG_ORDER_COLOURED instead. */
     uint32_t solve_order;             /* PXG_ORDER_REFERENCE */
     /* Residual mode: the residual is evaluated after every residual_check_interval-th velocity
      * pass (and after the last one the cap allows); the passes between run without the residual
@@ -107,7 +110,7 @@ typedef struct pxg_config {
     uint32_t reserved[3];
 } pxg_config;
 
-enum pxg_solve_order { PXG_ORDER_REFERENCE = 0, PXG_ORDER_COLOURED = 1 };
+enum pxg_solve_order { PXG_ORDER_REFERENCE = 0, PXG_ORDER_COLOURED = 1, PXG_ORDER_TILED = 2 };
 
 /* Full dynamic state of one body (physxcoopa/dynamics/body.h:85-216). */
 typedef struct pxg_body {
@@ -199,7 +202,9 @@ typedef struct pxg_stats {
     uint32_t awake_bodies;
     uint32_t learned_predictions;/* warm-start misses filled by the learned predictor */
     float residual;              /* velocity residual after the last velocity pass (m/s) */
-    float reserved0;
+    uint32_t tiled_workgroups;   /* PXG_ORDER_TILED, the last solve: its workgroups (bits 0-7), how
+                                    many held their tile in LDS (bits 8-15), and the most LDS a
+                                    tile needed, in KiB (bits 16-31) */
     uint64_t steps;              /* fixed steps run since creation */
 } pxg_stats;
 
@@ -746,7 +751,11 @@ typedef struct pxg_profile {
     double velocity_ms;          /* summed event time of velocity-pass launches */
     double velocity_bytes;       /* summed algorithmic bytes of those launches */
     uint64_t velocity_launches;
-    uint64_t reserved[5];
+    /* PXG_ORDER_TILED solves, the workgroup of the first tile, summed over the launches, in 10 ns
+     * ticks: running its colours, staging its bodies in and out of LDS, waiting at grid barriers,
+     * computing and reducing residuals */
+    uint64_t tiled_colour_ticks, tiled_stage_ticks, tiled_barrier_ticks, tiled_residual_ticks;
+    uint64_t reserved[1];
 } pxg_profile;
 int pxg_set_profiling(pxg_world* w, uint32_t on);
 int pxg_get_profile(pxg_world* w, pxg_profile* out);

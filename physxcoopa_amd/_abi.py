@@ -15,7 +15,7 @@ PXG_INTERPOLATE, PXG_INTERPOLATE_NONE = 0, 1
 PXG_SPHERE, PXG_BOX, PXG_CAPSULE, PXG_CONVEX_HULL, PXG_TRIANGLE_MESH = 0, 1, 2, 3, 4
 PXG_AVERAGE, PXG_MINIMUM, PXG_MAXIMUM, PXG_MULTIPLY = 0, 1, 2, 3
 PXG_PHASE_FORCES_COLLIDE, PXG_PHASE_SOLVE_INTEGRATE = 0, 1
-PXG_ORDER_REFERENCE, PXG_ORDER_COLOURED = 0, 1
+PXG_ORDER_REFERENCE, PXG_ORDER_COLOURED, PXG_ORDER_TILED = 0, 1, 2
 
 BODY_DTYPE = np.dtype([
     ("type", "<u4"), ("alive", "<u4"), ("awake", "<u4"), ("use_gravity", "<u4"),
@@ -215,14 +215,16 @@ class Stats(ctypes.Structure):
         ("pairs", ctypes.c_uint32), ("manifolds", ctypes.c_uint32), ("points", ctypes.c_uint32),
         ("colours", ctypes.c_uint32), ("active_manifolds", ctypes.c_uint32), ("velocity_passes", ctypes.c_uint32),
         ("awake_bodies", ctypes.c_uint32), ("learned_predictions", ctypes.c_uint32),
-        ("residual", ctypes.c_float), ("reserved0", ctypes.c_float), ("steps", ctypes.c_uint64),
+        ("residual", ctypes.c_float), ("tiled_workgroups", ctypes.c_uint32), ("steps", ctypes.c_uint64),
     ]
 
 
 class Profile(ctypes.Structure):
     """pxg_profile (include/pxg.h)."""
     _fields_ = [("velocity_ms", ctypes.c_double), ("velocity_bytes", ctypes.c_double),
-                ("velocity_launches", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 5)]
+                ("velocity_launches", ctypes.c_uint64), ("tiled_colour_ticks", ctypes.c_uint64),
+                ("tiled_stage_ticks", ctypes.c_uint64), ("tiled_barrier_ticks", ctypes.c_uint64),
+                ("tiled_residual_ticks", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 1)]
 
 
 def default_config(**overrides):

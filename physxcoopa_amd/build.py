@@ -14,7 +14,7 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpxg.so")
-SOURCES = ["pxg_kernels.hip", "pxg_world.hip", "pxg_batch.hip", "pxg_mlp.hip", "pxg_events.hip", "pxg_query.hip",
+SOURCES = ["pxg_kernels.hip", "pxg_world.hip", "pxg_batch.hip", "pxg_mlp.hip", "pxg_events.hip", "pxg_query.hip", "pxg_tiles.hip",
            "pxg_soft.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PXG_ARCH", "gfx950")

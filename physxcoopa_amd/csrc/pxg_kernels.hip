@@ -1745,6 +1745,7 @@ struct SolveCfg {
     uint32_t one_xcd;  // launched on every XCD; the blocks of one carry on with near barriers
     uint32_t ready;    // residual mode: positions split by ready_list (residuals during the tail)
     uint32_t check_every;  // residual mode: the residual after every k-th pass and the last
+    uint32_t tiled_lds;    // tiled solve: 1 = groups in LDS where they fit, 0 = always on global memory
 };
 
 // A velocity-pass manifold's slot loaded ahead (points in the pass's point order, rows with the
@@ -2031,6 +2032,492 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     }
 }
 
+// ---- tiled solve (PXG_ORDER_TILED, set up by pxg_tiles.hip) ----
+// One workgroup per tile. A phase's groups share no awake dynamic body, so a workgroup runs its
+// group's colours back to back with workgroup barriers, every group of the phase at once; a grid
+// barrier separates the three phases. With one group per workgroup and room in LDS, the group's
+// bodies (velocities staged in at the phase's start and written out at its end, inertia and
+// position once) and the impulses of all the workgroup's solve positions live in LDS; otherwise
+// (LDS too small, or fewer workgroups than tiles) the workgroup runs its groups on global memory.
+// The arithmetic per manifold is velocity_manifold's / warm_manifold's / residual_manifold's.
+constexpr uint32_t kTileLds = 156u * 1024u;  // dynamic LDS per workgroup (one workgroup per CU)
+
+// exclusive scan over the kPersistBlock threads of a workgroup
+__device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t wsum[kPersistBlock / 64];
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= uint32_t(o)) x += y;
+    }
+    if (lane == 63u) wsum[wid] = x;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kPersistBlock / 64; ++w) {
+        if (w < wid) off += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    total = tot;
+    return off + x - v;
+}
+
+// LDS of one workgroup in the LDS form: per local solve position (the workgroup's colours' ranges
+// one after the other) its normal and friction (hn) and (both bodies' indices in the group's table
+// as a | b << 16, first point << 3 | point count) (hm); per point (p0) position and the velocity
+// passes' normal target, (p1) the three effective masses and the relax passes' normal target, and
+// the impulses; per phase the group's table: every body its manifolds touch, with velocity (V: v,
+// inverse mass), angular velocity (W: w, 1 if awake dynamic else 0), sb_I's three records (I:
+// effective inverse inertia and position) and the body's index (bi). Nothing of a colour's work
+// goes to global memory.
+struct TLds {
+    float4* hn;
+    uint2* hm;
+    float4 *p0, *p1, *imp;
+};
+// load_sbody's unpacking of (v, w, sb_I records)
+__device__ __forceinline__ SBody tl_body(float4 v, float4 w, float4 a, float4 c, float4 e) {
+    SBody x;
+    x.v = ld3(v); x.im = v.w;
+    x.w = ld3(w); x.act = w.w != 0.0f;
+    x.I.c0 = mk(a.x, a.y, a.z); x.I.c1 = mk(a.w, c.x, c.y); x.I.c2 = mk(c.z, c.w, e.x);
+    x.x = mk(e.y, e.z, e.w);
+    return x;
+}
+__device__ __forceinline__ SBody tl_entry(const float4* V, const float4* W, const float4* I, uint32_t l) {
+    return tl_body(V[l], W[l], I[3u * l], I[3u * l + 1u], I[3u * l + 2u]);
+}
+// one solve position in LDS: OP 0 = warm_manifold, 1 = velocity_manifold, 2 = velocity_manifold in a
+// relax pass (same arithmetic; the targets were selected when the points were staged: the
+// velocity passes' is the point row's own, which row_for_pass keeps there)
+template <int OP>
+__device__ __forceinline__ void tl_run(const TLds& t, float4* V, float4* W, const float4* I, uint32_t L, uint32_t offset) {
+    const float4 nf = t.hn[L];
+    const uint2 hm = t.hm[L];
+    const uint32_t la = hm.x & 0xFFFFu, lb = hm.x >> 16;
+    const uint32_t cnt = hm.y & 7u, pt = hm.y >> 3;
+    SBody A = tl_entry(V, W, I, la), B = tl_entry(V, W, I, lb);
+    v3 n = ld3(nf);
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+    if (OP == 0) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            if (i >= cnt) break;
+            const float4 imp = t.imp[pt + i];
+            v3 J = n * imp.x + t1 * imp.y + t2 * imp.z;
+            if (dot(J, J) > 0.0f) apply_pair(A, B, ld3(t.p0[pt + i]), J);
+        }
+    } else {
+        // point k of the pass is (k + offset % cnt) % cnt, with cnt <= 4 taken without a division
+        const uint32_t r = cnt == 3u ? offset - 3u * (__umulhi(offset, 0xAAAAAAABu) >> 1)
+                                     : (cnt ? offset & (cnt - 1u) : 0u);
+        uint32_t idx[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) idx[k] = k < cnt ? (k + r >= cnt ? k + r - cnt : k + r) : k;
+        float4 pp[4], row[4], imp[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            pp[k] = row[k] = imp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (k < cnt) {
+                pp[k] = t.p0[pt + idx[k]];
+                const float4 q = t.p1[pt + idx[k]];
+                row[k] = make_float4(q.x, q.y, q.z, OP == 2 ? q.w : pp[k].w);
+                imp[k] = t.imp[pt + idx[k]];
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            if (k >= cnt) break;
+            imp[k] = velocity_point(A, B, n, t1, t2, nf.w, ld3(pp[k]), row[k], imp[k]);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k)
+            if (k < cnt) t.imp[pt + idx[k]] = imp[k];
+    }
+    if (A.act) { V[la] = st3(A.v, A.im); W[la] = st3(A.w, 1.0f); }
+    if (B.act) { V[lb] = st3(B.v, B.im); W[lb] = st3(B.w, 1.0f); }
+}
+__device__ __forceinline__ void tl_op(int op, const TLds& t, float4* V, float4* W, const float4* I, uint32_t L,
+                                      uint32_t offset) {
+    if (op == 0) tl_run<0>(t, V, W, I, L, offset);
+    else if (op == 1) tl_run<1>(t, V, W, I, L, offset);
+    else tl_run<2>(t, V, W, I, L, offset);
+}
+// residual_manifold (velocity passes) of a local solve position, all from LDS (the group's
+// velocities freshly staged after the pass). The normal target -sep / h of a separated point is
+// the velocity passes' target staged in p0.w.
+__device__ __forceinline__ float tl_residual(const TLds& t, const float4* V, const float4* W, const float4* I, uint32_t L) {
+    float r = 0.0f;
+    const float4 nf = t.hn[L];
+    const uint2 hm = t.hm[L];
+    SBody A = tl_entry(V, W, I, hm.x & 0xFFFFu), B = tl_entry(V, W, I, hm.x >> 16);
+    v3 n = ld3(nf);
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+    const uint32_t cnt = hm.y & 7u, pt = hm.y >> 3;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        if (i >= cnt) break;
+        const float4 pp = t.p0[pt + i], imp = t.imp[pt + i];
+        v3 pos = ld3(pp);
+        v3 ra = pos - A.x, rb = pos - B.x;
+        v3 dv = (B.v + cross(B.w, rb)) - (A.v + cross(A.w, ra));
+        float vn = dot(dv, n);
+        float target = pp.w;
+        float rn = imp.x > 0.0f ? fabsf(vn - target) : gmax(0.0f, target - vn);
+        r = gmax(r, rn);
+        float maxf = nf.w * imp.x;
+        for (int q = 0; q < 2; ++q) {
+            float vt = dot(dv, q == 0 ? t1 : t2);
+            float lt = q == 0 ? imp.y : imp.z;
+            float rt = 0.0f;
+            if (maxf > 0.0f) rt = fabsf(lt) < maxf ? fabsf(vt) : gmax(0.0f, lt < 0.0f ? -vt : vt);
+            r = gmax(r, rt);
+        }
+    }
+    return r;
+}
+
+// per-workgroup state of the tiled solve (shared)
+struct TileBlock {
+    uint32_t cols[3], base[3];       // colours per phase, first global colour of each
+    uint32_t lds;                    // 1: the LDS form
+    uint32_t nslots, npts;           // local solve positions and their points
+    uint32_t lo[3 * kTileColours], hi[3 * kTileColours], lb[3 * kTileColours];  // per (phase, colour)
+    uint32_t nbody[3], body0[3];     // the group's body table
+    uint32_t p_off, v_off[3];        // LDS float4 offsets: points, each phase's table
+};
+// the (phase, colour) entry holding local solve position L: the last with lb <= L
+__device__ __forceinline__ uint32_t tl_find(const TileBlock& tb, uint32_t L) {
+    uint32_t lo = 0, hi = 3u * kTileColours;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tb.lb[mid] <= L) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kPersistBlock) k_solve_tiled(DevState d, StepParams prm, SolveCfg cfg, uint32_t* bar) {
+    extern __shared__ float4 t_lds[];
+    __shared__ TileBlock tb;
+    __shared__ float s_red[kPersistBlock / 64];
+    GridBarrier g = grid_barrier_init(bar);
+    uint32_t* err = &d.cnt->err;
+    uint32_t rank = blockIdx.x;
+    if (cfg.one_xcd) {
+        if (!near_blocks(bar, err, rank, g.nblocks)) return;
+        g.near = 1;
+    }
+    const uint32_t nblk = g.nblocks, tid = threadIdx.x, nth = kPersistBlock;
+    const uint32_t gtid = rank * nth + tid, gnth = nblk * nth;
+    // ---- set-up: this workgroup's groups, colour ranges, and whether they fit in LDS ----
+    const bool one_group = nblk == kTiles;
+    if (tid == 0) {
+        // at most kTileColours in all (more is a colouring error, already reported: the excess
+        // colours are left out rather than read past t_off)
+        const uint32_t* c = d.cnt->tile_cols;
+        const uint32_t c0 = c[0] < kTileColours ? c[0] : kTileColours;
+        const uint32_t c1 = c[1] < kTileColours - c0 ? c[1] : kTileColours - c0;
+        const uint32_t c2 = c[2] < kTileColours - c0 - c1 ? c[2] : kTileColours - c0 - c1;
+        tb.cols[0] = c0; tb.cols[1] = c1; tb.cols[2] = c2;
+        tb.base[0] = 0u; tb.base[1] = c0; tb.base[2] = c0 + c1;
+        tb.nslots = 0u;
+    }
+    __syncthreads();
+    // ranges of this workgroup's colours (LDS form: group = rank in phases 0 and 1, group 0 of
+    // phase 2 on workgroup 0)
+    if (one_group) {
+        for (uint32_t k = tid; k < 3u * kTileColours; k += nth) {
+            const uint32_t ph = k / kTileColours, c = k - ph * kTileColours;
+            const bool own = ph < 2u || rank == 0u;
+            uint32_t lo = 0u, hi = 0u;
+            if (own && c < tb.cols[ph]) {
+                const uint32_t b = (tb.base[ph] + c) * kTiles + (ph < 2u ? rank : 0u);
+                lo = d.t_off[b];
+                hi = d.t_off[b + 1u];
+            }
+            tb.lo[k] = lo;
+            tb.hi[k] = hi;
+        }
+        if (tid < 3u) {
+            const uint32_t ph = tid;
+            const bool own = ph < 2u || rank == 0u;
+            const uint32_t key = ph * kTiles + (ph < 2u ? rank : 0u);
+            tb.body0[ph] = own ? d.tb_off[key] : 0u;
+            tb.nbody[ph] = own ? d.tb_off[key + 1u] - d.tb_off[key] : 0u;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t n = 0;
+            for (uint32_t k = 0; k < 3u * kTileColours; ++k) {
+                tb.lb[k] = n;
+                n += tb.hi[k] - tb.lo[k];
+            }
+            tb.nslots = n;
+        }
+        __syncthreads();
+    }
+    TLds t;
+    const uint32_t hm_f4 = (tb.nslots + 1u) / 2u;  // float4s of the hm array
+    t.hn = t_lds;
+    t.hm = reinterpret_cast<uint2*>(t_lds + tb.nslots);
+    bool lds = one_group && cfg.tiled_lds && (tb.nslots + hm_f4) * 16u <= kTileLds;
+    if (lds) {
+        // headers, with each position's first point from a scan of the point counts
+        uint32_t carry = 0;
+        for (uint32_t base = 0; base < tb.nslots; base += nth) {
+            const uint32_t L = base + tid;
+            uint32_t cnt = 0, s = 0;
+            if (L < tb.nslots) {
+                const uint32_t k = tl_find(tb, L);
+                s = tb.lo[k] + (L - tb.lb[k]);
+                cnt = d.sm_cnt[s];
+            }
+            uint32_t tot;
+            const uint32_t ex = tile_scan(cnt, tot);
+            if (L < tb.nslots) {
+                t.hn[L] = d.sm_nf[s];
+                t.hm[L] = make_uint2(d.sm_loc[s], ((carry + ex) << 3) | cnt);
+            }
+            carry += tot;
+        }
+        if (tid == 0) {
+            tb.npts = carry;
+            uint32_t off = tb.nslots + hm_f4;
+            tb.p_off = off;
+            off += 3u * carry;
+            for (uint32_t ph = 0; ph < 3u; ++ph) {
+                tb.v_off[ph] = off;
+                off += 5u * tb.nbody[ph] + (tb.nbody[ph] + 3u) / 4u;  // V, W, 3 I, then the indices
+            }
+            tb.lds = off * 16u <= kTileLds ? 1u : 0u;
+            atomicMax(&d.cnt->tile_need, off * 16u);
+        }
+        __syncthreads();
+        lds = tb.lds != 0u;
+    }
+    if (tid == 0) atomicAdd(&d.cnt->tile_wg, 1u | (lds ? 0x100u : 0u));
+    if (lds) {
+        t.p0 = t_lds + tb.p_off;
+        t.p1 = t.p0 + tb.npts;
+        t.imp = t.p1 + tb.npts;
+        // the points: positions, targets, effective masses, impulses
+        for (uint32_t L = tid; L < tb.nslots; L += nth) {
+            const uint32_t k = tl_find(tb, L), s = tb.lo[k] + (L - tb.lb[k]);
+            const uint2 hm = t.hm[L];
+            const uint32_t cnt = hm.y & 7u, pt = hm.y >> 3;
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const float4 pp = d.sp_pos[4u * s + q], row = d.sp_row[4u * s + q];
+                const float4 rr = row_for_pass(row, d.sp_aux[4u * s + q], 1);
+                t.p0[pt + q] = make_float4(pp.x, pp.y, pp.z, row.w);
+                t.p1[pt + q] = make_float4(row.x, row.y, row.z, rr.w);
+                t.imp[pt + q] = d.sp_imp[4u * s + q];
+            }
+        }
+        // the tables' constants: inertia, position, body index
+        for (uint32_t ph = 0; ph < 3u; ++ph) {
+            const uint32_t n = tb.nbody[ph];
+            float4* I = t_lds + tb.v_off[ph] + 2u * n;
+            uint32_t* bi = reinterpret_cast<uint32_t*>(I + 3u * n);
+            for (uint32_t i = tid; i < n; i += nth) {
+                const uint32_t b = d.tb_list[tb.body0[ph] + i];
+                bi[i] = b;
+                I[3u * i] = d.sb_I[3u * b];
+                I[3u * i + 1u] = d.sb_I[3u * b + 1u];
+                I[3u * i + 2u] = d.sb_I[3u * b + 2u];
+            }
+        }
+        __syncthreads();
+    }
+    // stages a phase's table velocities in from global memory (every entry: those of bodies that
+    // are not awake dynamic never change, but a table is small)
+    auto stage_in = [&](uint32_t ph) __attribute__((always_inline)) {
+        const uint32_t n = tb.nbody[ph];
+        float4* V = t_lds + tb.v_off[ph];
+        const uint32_t* bi = reinterpret_cast<const uint32_t*>(V + 5u * n);
+        for (uint32_t i = tid; i < n; i += nth) {
+            const uint32_t b = bi[i];
+            V[i] = ld_dev(d.sb_v, b);
+            V[n + i] = ld_dev(d.sb_w, b);
+        }
+    };
+    // profiling: workgroup 0's time in its colours, staging, grid barriers and residuals (10 ns
+    // ticks of the real-time counter), added to Counters::prof_stage at the end
+    const bool prof = prm.profile != 0u && gtid == 0u;
+    unsigned long long tk_col = 0, tk_stage = 0, tk_bar = 0, tk_res = 0;
+    auto tick = [&]() __attribute__((always_inline)) -> unsigned long long {
+        return prof ? static_cast<unsigned long long>(wall_clock64()) : 0ull;
+    };
+    // ---- one phase of a pass: op 0 warm start, 1 velocity, 2 relax ----
+    auto phase = [&](uint32_t ph, int op, uint32_t offset) __attribute__((always_inline)) {
+        if (lds) {
+            if (ph == 2u && rank != 0u) return;
+            const unsigned long long t0 = tick();
+            const uint32_t n = tb.nbody[ph], C = tb.cols[ph], k0 = ph * kTileColours;
+            float4* V = t_lds + tb.v_off[ph];
+            float4* W = V + n;
+            const float4* I = W + n;
+            const uint32_t* bi = reinterpret_cast<const uint32_t*>(I + 3u * n);
+            stage_in(ph);
+            __syncthreads();
+            const unsigned long long t1 = tick();
+            tk_stage += t1 - t0;
+            for (uint32_t c = 0; c < C; ++c) {
+                const uint32_t k = k0 + c, lo = tb.lo[k], hi = tb.hi[k], lb = tb.lb[k];
+                for (uint32_t j = tid; lo + j < hi; j += nth) tl_op(op, t, V, W, I, lb + j, offset);
+                __syncthreads();
+            }
+            const unsigned long long t2 = tick();
+            tk_col += t2 - t1;
+            for (uint32_t i = tid; i < n; i += nth) {
+                const float4 w = W[i];
+                if (w.w != 0.0f) {
+                    const uint32_t b = bi[i];
+                    d.sb_v[b] = V[i];
+                    d.sb_w[b] = w;
+                }
+            }
+            tk_stage += tick() - t2;
+            return;
+        }
+        const uint32_t g0 = ph < 2u ? rank : (rank == 0u ? 0u : 1u), gend = ph < 2u ? kTiles : 1u;
+        for (uint32_t gg = g0; gg < gend; gg += nblk) {
+            for (uint32_t c = 0; c < tb.cols[ph]; ++c) {
+                const uint32_t b = (tb.base[ph] + c) * kTiles + gg;
+                const uint32_t lo = d.t_off[b], hi = d.t_off[b + 1u];
+                for (uint32_t s = lo + tid; s < hi; s += nth) {
+                    if (op == 0) warm_manifold(d, s);
+                    else velocity_manifold(d, prm, s, op == 2 ? 1 : 0, offset);
+                }
+                wait_stores();
+                __syncthreads();
+            }
+        }
+    };
+    // a whole pass: the three phases, a grid barrier after each that has colours
+    auto pass = [&](int op, uint32_t offset) __attribute__((always_inline)) -> bool {
+        for (uint32_t ph = 0; ph < 3u; ++ph) {
+            if (!tb.cols[ph]) continue;
+            phase(ph, op, offset);
+            const unsigned long long t0 = tick();
+            if (grid_sync(g, err)) return true;
+            tk_bar += tick() - t0;
+        }
+        return false;
+    };
+    // maximum residual over this workgroup's solve positions (after the pass's last grid barrier)
+    auto block_residual = [&]() __attribute__((always_inline)) -> float {
+        float r = 0.0f;
+        if (lds) {
+            for (uint32_t ph = 0; ph < 3u; ++ph) stage_in(ph);
+            __syncthreads();
+            for (uint32_t ph = 0; ph < 3u; ++ph) {
+                const uint32_t n = tb.nbody[ph];
+                const float4* V = t_lds + tb.v_off[ph];
+                const float4* W = V + n;
+                const float4* I = W + n;
+                const uint32_t k0 = ph * kTileColours;
+                const uint32_t L0 = tb.lb[k0], L1 = ph < 2u ? tb.lb[k0 + kTileColours] : tb.nslots;
+                for (uint32_t L = L0 + tid; L < L1; L += nth) r = gmax(r, tl_residual(t, V, W, I, L));
+            }
+        } else {
+            for (uint32_t ph = 0; ph < 3u; ++ph) {
+                const uint32_t g0 = ph < 2u ? rank : (rank == 0u ? 0u : 1u), gend = ph < 2u ? kTiles : 1u;
+                for (uint32_t gg = g0; gg < gend; gg += nblk)
+                    for (uint32_t c = 0; c < tb.cols[ph]; ++c) {
+                        const uint32_t b = (tb.base[ph] + c) * kTiles + gg;
+                        const uint32_t lo = d.t_off[b], hi = d.t_off[b + 1u];
+                        for (uint32_t s = lo + tid; s < hi; s += nth) r = gmax(r, residual_manifold(d, prm, s, 0));
+                    }
+            }
+        }
+        return r;
+    };
+    // ---- the solve (dynamics/solver.h:914-958 after the restitution capture) ----
+    if (pass(0, 0u)) return;
+    uint32_t passes = 0;
+    if (!cfg.residual_mode) {
+        for (uint32_t it = 0; it < cfg.vel_iters; ++it)
+            if (pass(1, it)) return;
+        passes = cfg.vel_iters;
+    }
+    const uint32_t max_pass = cfg.residual_mode ? cfg.residual_max : 0u;
+    uint32_t checks = 0;
+    for (uint32_t it = 0; it < max_pass; ++it) {
+        if (pass(1, it)) return;
+        ++passes;
+        const bool check = (it + 1u) % cfg.check_every == 0u || it + 1u == max_pass;
+        if (!check) continue;
+        const unsigned long long tr = tick();
+        float r = block_residual();
+        for (int o = 32; o > 0; o >>= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+        if (lane0()) s_red[tid >> 6] = r;
+        __syncthreads();
+        if (tid == 0) {
+            float m = 0.0f;
+            for (uint32_t w = 0; w < kPersistBlock / 64; ++w) m = fmaxf(m, s_red[w]);
+            if (m > 0.0f) atomicMax(&d.cnt->res_slot[checks & 1u], __float_as_uint(m));
+        }
+        if (grid_sync(g, err)) return;
+        // every workgroup's maximum is in; the other slot is cleared for the next evaluation (its
+        // last readers read it before this evaluation's barrier, its next writers add to it after
+        // the next pass's barriers)
+        const float rr = __uint_as_float(__hip_atomic_load(&d.cnt->res_slot[checks & 1u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (gtid == 0) {
+            d.cnt->residual_bits = __float_as_uint(rr);
+            d.cnt->res_slot[(checks + 1u) & 1u] = 0u;
+        }
+        ++checks;
+        tk_res += tick() - tr;
+        if (rr <= cfg.residual_tol) break;
+    }
+    if (gtid == 0) d.cnt->passes = passes;
+    for (uint32_t it = 0; it < cfg.relax_iters; ++it)
+        if (pass(2, it)) return;
+    if (prof) {
+        atomicAdd(&d.cnt->prof_stage[0], tk_col);
+        atomicAdd(&d.cnt->prof_stage[1], tk_stage);
+        atomicAdd(&d.cnt->prof_stage[2], tk_bar);
+        atomicAdd(&d.cnt->prof_stage[3], tk_res);
+    }
+    if (lds) {
+        // the impulses back for the warm-start cache
+        for (uint32_t L = tid; L < tb.nslots; L += nth) {
+            const uint32_t k = tl_find(tb, L), s = tb.lo[k] + (L - tb.lb[k]);
+            const uint2 hm = t.hm[L];
+            const uint32_t cnt = hm.y & 7u, pt = hm.y >> 3;
+            for (uint32_t q = 0; q < cnt; ++q) d.sp_imp[4u * s + q] = t.imp[pt + q];
+        }
+    }
+    // solve_position: snapshot, then position passes, on global memory
+    for (uint32_t i = gtid; i < d.nb; i += gnth) {
+        const float4 p = d.b_pos[i];
+        d.sb_x[i] = make_float4(p.x, p.y, p.z, ld_dev(d.sb_v, i).w);
+        d.sb_ref[i] = p;
+    }
+    if (grid_sync(g, err)) return;
+    for (uint32_t it = 0; it < cfg.pos_iters; ++it) {
+        for (uint32_t ph = 0; ph < 3u; ++ph) {
+            if (!tb.cols[ph]) continue;
+            const uint32_t g0 = ph < 2u ? rank : (rank == 0u ? 0u : 1u), gend = ph < 2u ? kTiles : 1u;
+            for (uint32_t gg = g0; gg < gend; gg += nblk)
+                for (uint32_t c = 0; c < tb.cols[ph]; ++c) {
+                    const uint32_t b = (tb.base[ph] + c) * kTiles + gg;
+                    const uint32_t lo = d.t_off[b], hi = d.t_off[b + 1u];
+                    for (uint32_t s = lo + tid; s < hi; s += nth) position_manifold(d, prm, s);
+                    wait_stores();
+                    __syncthreads();
+                }
+            if (grid_sync(g, err)) return;
+        }
+    }
+}
+
 // ---- reference solve order ----
 // The reference solves manifolds one after the other in pair order (broadphase/pair_cache.h:23-52,
 // dynamics/solver.h:212-281). Two manifolds that write no common body commute, so any order that
@@ -2161,7 +2648,9 @@ __device__ void block_scan_range(const uint32_t* in, uint32_t* out, uint32_t n) 
 // at the same awake dynamic body, in which case every claimant drops it; the outcome does not
 // depend on timing. Jones-Plassmann rounds (first-fit over the per-body masks) colour the rest.
 // reference: 0 = coloured order starting from last step's colours, 1 = the reference order's
-// chain layers, 2 = coloured order from scratch (Jones-Plassmann rounds only).
+// chain layers, 2 = coloured order from scratch (Jones-Plassmann rounds only), 3 = the tiled
+// order's colours: Jones-Plassmann rounds with one mask word per (phase, body) (col_best and
+// col_mask then hold 3 * nb words, phase-major), each phase's colours counted in tile_cols.
 __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d, uint32_t max_rounds, uint32_t reference,
                                                                    uint32_t* bar) {
     GridBarrier g = grid_barrier_init(bar);
@@ -2240,27 +2729,31 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
     }
     if (grid_sync(g, err)) return;
     // C: Jones-Plassmann rounds for the rest
+    const bool tiled = reference == 3u;
     for (uint32_t round = 1; round <= max_rounds; ++round) {
         if (__hip_atomic_load(&d.cnt->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) break;
         for (uint32_t m = tid; m < n_man; m += nthreads) {
             if (d.m_colour[m] != kUncoloured) continue;
             uint2 ab = d.m_ab[m];
+            const uint32_t po = tiled ? (d.m_tile[m] >> 8) * d.nb : 0u;
             unsigned long long key = colour_key(round, m, ab, d.m_shapes[m], d.m_meta[m].x & 255u);
-            if (body_active(d.b_flags[ab.x])) atomicMax(&d.col_best[ab.x], key);
-            if (body_active(d.b_flags[ab.y])) atomicMax(&d.col_best[ab.y], key);
+            if (body_active(d.b_flags[ab.x])) atomicMax(&d.col_best[po + ab.x], key);
+            if (body_active(d.b_flags[ab.y])) atomicMax(&d.col_best[po + ab.y], key);
         }
         if (grid_sync(g, err)) return;
         uint32_t base = tid - (tid & 63u);
         for (uint32_t m0 = base; m0 < n_man; m0 += nthreads) {
             uint32_t m = m0 + (tid & 63u);
-            uint32_t took = 0, top = 0;
+            uint32_t took = 0, top = 0, ph = 0;
             if (m < n_man && d.m_colour[m] == kUncoloured) {
                 uint2 ab = d.m_ab[m];
+                ph = tiled ? d.m_tile[m] >> 8 : 0u;
+                const uint32_t po = ph * d.nb;
                 unsigned long long key = colour_key(round, m, ab, d.m_shapes[m], d.m_meta[m].x & 255u);
                 bool aa = body_active(d.b_flags[ab.x]), ba = body_active(d.b_flags[ab.y]);
-                bool mine = (!aa || d.col_best[ab.x] == key) && (!ba || d.col_best[ab.y] == key);
+                bool mine = (!aa || d.col_best[po + ab.x] == key) && (!ba || d.col_best[po + ab.y] == key);
                 if (mine) {
-                    unsigned long long used = (aa ? d.col_mask[ab.x] : 0ull) | (ba ? d.col_mask[ab.y] : 0ull);
+                    unsigned long long used = (aa ? d.col_mask[po + ab.x] : 0ull) | (ba ? d.col_mask[po + ab.y] : 0ull);
                     unsigned long long freeb = ~used;
                     if (freeb == 0ull) {
                         atomicOr(err, ERR_COLOURS);
@@ -2268,11 +2761,17 @@ __global__ void __launch_bounds__(kPersistBlock) k_colour_persistent(DevState d,
                         uint32_t c = uint32_t(__ffsll(static_cast<long long>(freeb)) - 1);
                         d.m_colour[m] = c;
                         unsigned long long bit = 1ull << c;
-                        if (aa) atomicOr(&d.col_mask[ab.x], bit);
-                        if (ba) atomicOr(&d.col_mask[ab.y], bit);
+                        if (aa) atomicOr(&d.col_mask[po + ab.x], bit);
+                        if (ba) atomicOr(&d.col_mask[po + ab.y], bit);
                         took = 1u;
                         top = c + 1u;
                     }
+                }
+            }
+            if (tiled) {
+                for (uint32_t q = 0; q < 3u; ++q) {
+                    const uint32_t tq = wave_max(ph == q ? top : 0u);
+                    if (lane0() && tq) atomicMax(&d.cnt->tile_cols[q], tq);
                 }
             }
             took = wave_sum(took);
@@ -2474,6 +2973,7 @@ hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const
     c.one_xcd = cfgv[5];
     c.ready = cfgv[6];
     c.check_every = cfgv[7] ? cfgv[7] : 1u;
+    c.tiled_lds = 0u;
     launch_fill_u32(d.bar, 0u, kBarWords, st);
     DevState dd = d;
     StepParams pp = p;
@@ -2481,6 +2981,30 @@ hipError_t launch_solve_persistent(const DevState& d, const StepParams& p, const
     void* args[] = {&dd, &pp, &c, &bar};
     return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_persistent), dim3(nblocks),
                                       dim3(kPersistBlock), args, 0, st);
+}
+hipError_t launch_solve_tiled(const DevState& d, const StepParams& p, const uint32_t cfgv[8], float tol, uint32_t nblocks,
+                              hipStream_t st) {
+    static bool attr = false;  // the dynamic LDS beyond the default 64 KB, set once per process
+    if (!attr) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_solve_tiled),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(kTileLds));
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    SolveCfg c;
+    c.vel_iters = cfgv[0]; c.relax_iters = cfgv[1]; c.pos_iters = cfgv[2];
+    c.residual_mode = cfgv[3]; c.residual_max = cfgv[4]; c.residual_tol = tol;
+    c.one_xcd = cfgv[5];
+    c.ready = 0u;
+    c.check_every = cfgv[7] ? cfgv[7] : 1u;
+    c.tiled_lds = cfgv[6];
+    launch_fill_u32(d.bar, 0u, kBarWords, st);
+    DevState dd = d;
+    StepParams pp = p;
+    uint32_t* bar = d.bar;
+    void* args[] = {&dd, &pp, &c, &bar};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_tiled), dim3(nblocks), dim3(kPersistBlock),
+                                      args, kTileLds, st);
 }
 void launch_solve_bytes(const DevState& d, const uint32_t cfgv[8], uint32_t bound, hipStream_t st) {
     if (bound) hipLaunchKernelGGL(k_solve_bytes, dim3(blocks(bound)), dim3(kBlock), 0, st, d, cfgv[0], cfgv[1], cfgv[2], cfgv[3]);

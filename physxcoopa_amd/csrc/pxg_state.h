@@ -29,6 +29,11 @@ constexpr int kMaxLarge = 256;       // shapes on the brute-force broadphase pat
 constexpr uint32_t kMaxCells = 1u << 18;
 constexpr int kPairLanes = 9;        // broadphase threads per emitter shape
 constexpr uint32_t kMaxShapes = (1u << 20) / kPairLanes;  // the pair-count scan covers 2^20 entries
+// tiled solve order (pxg_tiles.hip): tiles per tiling (one per CU of one XCD), as x strips times
+// y tiles; colours over the three phases; the sort bucket of the manifolds the solver skips
+constexpr uint32_t kTiles = 32, kTileNx = 8, kTileNy = 4;
+constexpr uint32_t kTileColours = 64;
+constexpr uint32_t kTileSkipBucket = kTileColours * kTiles;
 
 // body flag bits
 constexpr uint32_t BF_TYPE = 3u, BF_ALIVE = 1u << 2, BF_AWAKE = 1u << 3, BF_GRAVITY = 1u << 4;
@@ -58,11 +63,18 @@ struct Counters {  // device-side counters, read back by the host at the step's 
     uint32_t colour_off[kBuckets + 1];
     float grid[12];         // ox, oy, oz, inv_cell, ext_y, ext_z, nx, ny, nz (as floats)
     unsigned long long prof_vel_bytes;  // algorithmic bytes of profiled velocity passes
+    unsigned long long prof_stage[4];   // profiled tiled solves, workgroup 0 (10 ns ticks): colours,
+                                        // staging bodies in and out, grid barriers, residuals
     uint32_t passes;        // velocity passes run by the last solve
     uint32_t res_slot[2];   // residual max-reduction slots (float bits), alternating per evaluation
     uint32_t ev_n;          // contact events queued (<= the queue's capacity)
     uint32_t ev_dropped;    // events lost to a full queue since it was last cleared
     uint32_t ev_snap_n[2];  // entries of the two pair snapshots
+    uint32_t tile_n;        // tiled order: awake dynamic bodies
+    uint32_t tile_cols[3];  // tiled order: colours of each phase
+    uint32_t tile_wg;       // tiled solve: workgroups | workgroups that held their tile in LDS << 8
+    uint32_t tile_need;     // tiled solve: the most LDS bytes a workgroup's tile needed
+    uint32_t tile_nu;       // tiled order: entries of all group tables
 };
 
 struct DevState {
@@ -152,6 +164,21 @@ struct DevState {
     float4 *sb_I;       // 3 per body: effective world inverse inertia (9) + position (3)
     float4 *sb_x;       // position pass: position, effective inverse mass
     float4 *sb_ref;     // position pass reference positions
+    // tiled solve order (pxg_tiles.hip; allocated on the first tiled step)
+    uint32_t *t_body;     // 2 * nb: each awake dynamic body's tile in the first, then the second tiling
+    uint32_t *m_tile;     // per manifold: phase << 8 | group
+    uint32_t *t_off;      // kTileSkipBucket + 2: first solve position of each (global colour * kTiles
+                          // + group); then of the skipped group; then the end of the order
+    uint32_t *ts_a;       // 2 per manifold: its bodies' table keys (phase * kTiles + group) << 20 | body
+    uint32_t *tf, *tu;    // the sorted keys' first-occurrence flags and their scan
+    uint32_t *tb_key, *tb_list;  // per (phase, group), ascending: the bodies its manifolds touch (key, body)
+    uint32_t *tb_off;     // 3 * kTiles + 1 offsets into tb_list
+    uint32_t *sm_loc;     // per solve position: body a's | body b's index << 16 in the group's table
+    unsigned long long *t_col_best, *t_col_mask;  // 3 * nb: the colouring's per-phase body words
+    uint32_t *tk_a, *tk_b, *tv_a, *tv_b;         // sort keys and values
+    unsigned long long *tk64_a, *tk64_b;
+    void *t_sort_tmp;
+    size_t t_sort_bytes;
     // warm-start cache: two open-addressing tables, swapped every step
     unsigned long long *ck_keys[2];
     unsigned long long *ck_ord[2];  // last writer in the reference's manifold order
@@ -321,8 +348,8 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t st);
 void launch_fill_u64(unsigned long long* p, unsigned long long v, size_t n, hipStream_t st);
 void launch_scan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, uint32_t* scratch, hipStream_t st);
 // persistent cooperative kernels (kPersistBlock threads per block, grid barrier on d.bar);
-// cfgv = vel, relax, pos iterations, residual mode, residual max passes, one XCD, ready split,
-// residual check interval
+// cfgv = vel, relax, pos iterations, residual mode, residual max passes, one XCD, ready split (the
+// tiled solve: tiles in LDS allowed), residual check interval
 // algorithmic bytes of the last persistent solve into cnt->prof_vel_bytes (profiling)
 void launch_solve_bytes(const DevState& d, const uint32_t cfgv[8], uint32_t bound, hipStream_t st);
 // splits the active solve positions for the residual computed during the tail colours
@@ -337,6 +364,17 @@ hipError_t launch_colour_persistent(const DevState& d, uint32_t max_rounds, uint
 void launch_reference_chains(const DevState& d, uint32_t n_manifolds, hipStream_t st);
 void launch_reference_layers(const DevState& d, uint32_t n_manifolds, uint32_t rounds, hipStream_t st);
 int max_coresident_blocks(int device);
+// tiled solve order (pxg_tiles.hip): sort scratch bytes; the tiles and each active manifold's
+// phase and group (after launch_colour_init, before the colouring); the order, colour table and
+// body tables (after the colouring); each solve position's body indices (after the solver set-up)
+size_t tile_sort_bytes(uint32_t cap_bodies, uint32_t cap_items);
+hipError_t launch_tile_setup(const DevState& d, uint32_t n_bound, hipStream_t st);
+hipError_t launch_tile_order(const DevState& d, uint32_t n_bound, hipStream_t st);
+void launch_tile_slots(const DevState& d, uint32_t n_bound, hipStream_t st);
+// the tiled solve: warm start, velocity (fixed or residual mode), relax and position passes of
+// the tiled order, each tile's colours in one workgroup (its bodies and impulses in LDS)
+hipError_t launch_solve_tiled(const DevState& d, const StepParams& p, const uint32_t cfgv[8], float tol, uint32_t nblocks,
+                              hipStream_t st);
 constexpr uint32_t ERR_BARRIER_BIT = 64u;
 
 }  // namespace pxg

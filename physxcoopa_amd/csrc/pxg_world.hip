@@ -51,6 +51,7 @@ struct pxg_world {
     bool results_pending = false;  // the last step's counters are still on the device
     uint32_t colour_blocks = 64, solve_blocks = 32;  // caps on the persistent kernels' grids
     bool solve_one_xcd = true;  // the solve runs on the blocks of one XCD with near barriers
+    bool tiled_lds = true;      // tiled order: tiles in LDS where they fit (PXG_TILED_LDS=0: global memory)
     // coloured order: colour every step afresh (-1: in residual mode only, where the solve dominates
     // the step) instead of starting from last step's colours
     int colour_fresh = -1;
@@ -284,6 +285,7 @@ extern "C" int pxg_create(const pxg_config* cfg, pxg_world** out) {
     if (const char* v = std::getenv("PXG_SOLVE_BLOCKS")) w->solve_blocks = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("PXG_COLOUR_BLOCKS")) w->colour_blocks = std::max(1, std::atoi(v));
     if (const char* v = std::getenv("PXG_SOLVE_ONE_XCD")) w->solve_one_xcd = v[0] != '0';
+    if (const char* v = std::getenv("PXG_TILED_LDS")) w->tiled_lds = v[0] != '0';
     if (const char* v = std::getenv("PXG_COLOUR_FRESH")) w->colour_fresh = v[0] == '1' ? 1 : 0;
     *out = w;
     return PXG_OK;
@@ -346,6 +348,29 @@ static int alloc_bodies(pxg_world* w, uint32_t n) {
     DALLOC(d.bar, kBarWords);
     HIPCHK(hipMemsetAsync(d.bar, 0, kBarWords * sizeof(uint32_t), w->stream));
     HIPCHK(hipMemsetAsync(d.cnt, 0, sizeof(Counters), w->stream));
+    return PXG_OK;
+}
+
+// the tiled solve order's buffers, on the first step that uses it
+static int ensure_tiled(pxg_world* w) {
+    DevState& d = w->d;
+    if (d.t_body) return PXG_OK;
+    const size_t B = w->cap_b, M = w->cap_m, P = w->cap_p;
+    if (B >= (1u << 20) || M > (1u << 20) || 2 * P > (1u << 22))
+        return fail(PXG_ERR_CAPACITY, "tiled solve order: at most 2^20 - 1 bodies, 2^20 manifolds and 2^21 pairs");
+    DALLOC(d.t_body, 2 * B); DALLOC(d.m_tile, M);
+    DALLOC(d.t_off, size_t(kTileSkipBucket) + 16); DALLOC(d.tb_off, 3 * kTiles + 16);
+    DALLOC(d.ts_a, 2 * P); DALLOC(d.tf, 2 * P); DALLOC(d.tu, 2 * P); DALLOC(d.tb_key, 2 * P); DALLOC(d.tb_list, 2 * P);
+    DALLOC(d.sm_loc, M);
+    DALLOC(d.t_col_best, 3 * B); DALLOC(d.t_col_mask, 3 * B);
+    const size_t items = std::max(2 * P, 3 * B);
+    DALLOC(d.tk_a, items); DALLOC(d.tk_b, items); DALLOC(d.tv_a, 2 * B); DALLOC(d.tv_b, 2 * B);
+    DALLOC(d.tk64_a, 2 * B); DALLOC(d.tk64_b, 2 * B);
+    d.t_sort_bytes = tile_sort_bytes(uint32_t(B), uint32_t(items));
+    if (!d.t_sort_bytes) return fail(PXG_ERR_HIP, "tiled solve order: sizing the sorts failed");
+    uint8_t* tmp = nullptr;
+    DALLOC(tmp, d.t_sort_bytes);
+    d.t_sort_tmp = tmp;
     return PXG_OK;
 }
 
@@ -952,6 +977,9 @@ static int finish_results(pxg_world* w) {
     float residual = 0.0f;
     std::memcpy(&residual, &hc.residual_bits, 4);
     sst.residual = residual;
+    sst.tiled_workgroups = w->cfg.solve_order == PXG_ORDER_TILED
+                               ? (hc.tile_wg & 0xFFFFu) | (std::min((hc.tile_need + 1023u) >> 10, 0xFFFFu) << 16)
+                               : 0u;
     return PXG_OK;
 }
 
@@ -1066,6 +1094,13 @@ static int phase_solve(pxg_world* w, float h) {
     w->stats.pairs = P;
     w->stats.steps += 1;
     bool persistent = w->coop_blocks > 0 && !w->eager;
+    // the tiled order: contact solves of worlds without joints (joint layers run between the
+    // colours of one grid-wide order)
+    bool tiled = persistent && c.solve_order == PXG_ORDER_TILED && d.nj == 0 && d.nb > 0;
+    if (tiled) {
+        int r = ensure_tiled(w);
+        if (r != PXG_OK) return r;
+    }
     if (persistent) {
         uint32_t nb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kPersistBlock - 1) / kPersistBlock));
         nb = std::min(nb, w->colour_blocks);
@@ -1073,12 +1108,24 @@ static int phase_solve(pxg_world* w, float h) {
         // each later colour then holds only one-point manifolds and its phase is shorter, which
         // outweighs the extra colouring rounds when the solve runs hundreds of passes.
         const bool fresh = w->colour_fresh < 0 ? c.residual_mode != 0 : w->colour_fresh > 0;
-        const uint32_t mode = c.solve_order == PXG_ORDER_REFERENCE ? 1u : (fresh ? 2u : 0u);
-        hipError_t e = launch_colour_persistent(d, 1024, nb, mode, st);
+        const uint32_t mode = c.solve_order == PXG_ORDER_REFERENCE ? 1u : (tiled ? 3u : (fresh ? 2u : 0u));
+        hipError_t e = hipSuccess;
+        DevState dc = d;
+        if (tiled) {
+            e = launch_tile_setup(d, bound, st);
+            if (e != hipSuccess) return fail(PXG_ERR_HIP, std::string("tiled order: ") + hipGetErrorString(e));
+            dc.col_best = d.t_col_best;  // one mask word per (phase, body)
+            dc.col_mask = d.t_col_mask;
+        }
+        e = launch_colour_persistent(dc, 1024, nb, mode, st);
         if (e != hipSuccess) {
             if (!cooperative_refused(e)) return fail(PXG_ERR_HIP, std::string("colouring launch: ") + hipGetErrorString(e));
             drop_persistent(w, e);
             persistent = false;
+            tiled = false;
+        } else if (tiled) {
+            e = launch_tile_order(d, bound, st);
+            if (e != hipSuccess) return fail(PXG_ERR_HIP, std::string("tiled order: ") + hipGetErrorString(e));
         } else {
             launch_build_order(d, bound, st);
         }
@@ -1090,28 +1137,33 @@ static int phase_solve(pxg_world* w, float h) {
     bool solved = false;
     if (persistent) {
         launch_solver_setup(d, p, bound, st);
+        if (tiled) launch_tile_slots(d, bound, st);
         if (d.mlp && !d.record) launch_apply_predictions(d, 4 * bound, st);
         // the widest colour is at most the manifold count; threads loop when a colour is wider
         uint32_t snb = std::max(1u, std::min<uint32_t>(w->coop_blocks, (bound + kPersistBlock - 1) / kPersistBlock));
         snb = std::min(snb, w->solve_blocks);
         // one XCD: launch on all of them (blocks go to XCDs in turn); the blocks that land on one
         // carry on, the rest leave at once
+        // the tiled solve: one workgroup per tile (on one XCD when every XCD can take them)
+        if (tiled) snb = kTiles;
         const bool one_xcd = w->solve_one_xcd && static_cast<uint32_t>(w->coop_blocks) >= kXcds * snb;
         if (one_xcd) snb *= kXcds;
         // residual mode without joints: split the manifolds for the residual computed during the
         // tail colours
-        const bool ready = c.residual_mode && d.nj == 0;
+        const bool ready = c.residual_mode && d.nj == 0 && !tiled;
         if (ready) launch_ready_split(d, bound, st);
         uint32_t cfgv[8] = {c.velocity_iterations, c.relax_iterations, c.position_iterations, c.residual_mode,
                             c.residual_max_iterations ? c.residual_max_iterations : 256u, one_xcd ? 1u : 0u,
-                            ready ? 1u : 0u, c.residual_check_interval ? c.residual_check_interval : 1u};
+                            tiled ? (w->tiled_lds ? 1u : 0u) : (ready ? 1u : 0u),
+                            c.residual_check_interval ? c.residual_check_interval : 1u};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (w->profiling) {
             e0 = take_event(w);
             e1 = take_event(w);
             (void)hipEventRecord(e0, st);
         }
-        hipError_t e = launch_solve_persistent(d, p, cfgv, c.residual_tolerance, snb, st);
+        hipError_t e = tiled ? launch_solve_tiled(d, p, cfgv, c.residual_tolerance, snb, st)
+                             : launch_solve_persistent(d, p, cfgv, c.residual_tolerance, snb, st);
         if (e == hipSuccess) {
             solved = true;
             if (w->profiling) {
@@ -2603,9 +2655,14 @@ extern "C" int pxg_get_profile(pxg_world* w, pxg_profile* out) {
     out->velocity_ms = w->prof_vel_ms;
     out->velocity_launches = w->prof_vel_launches;
     out->velocity_bytes = double(w->h_cnt->prof_vel_bytes);
+    out->tiled_colour_ticks = w->h_cnt->prof_stage[0];
+    out->tiled_stage_ticks = w->h_cnt->prof_stage[1];
+    out->tiled_barrier_ticks = w->h_cnt->prof_stage[2];
+    out->tiled_residual_ticks = w->h_cnt->prof_stage[3];
     w->prof_vel_ms = 0.0;
     w->prof_vel_launches = 0;
-    unsigned long long zero = 0;
-    HIPCHK(hipMemcpy(&w->d.cnt->prof_vel_bytes, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    unsigned long long zero[5] = {0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpy(&w->d.cnt->prof_vel_bytes, &zero[0], sizeof(zero[0]), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(&w->d.cnt->prof_stage[0], &zero[1], 4 * sizeof(zero[0]), hipMemcpyHostToDevice));
     return PXG_OK;
 }

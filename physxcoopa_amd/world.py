@@ -362,7 +362,9 @@ class World:
     def profile(self):
         p = Profile()
         check(lib().pxg_get_profile(self.h, ctypes.byref(p)), "pxg_get_profile")
-        return dict(velocity_ms=p.velocity_ms, velocity_bytes=p.velocity_bytes, velocity_launches=p.velocity_launches)
+        return dict(velocity_ms=p.velocity_ms, velocity_bytes=p.velocity_bytes, velocity_launches=p.velocity_launches,
+                    tiled_ticks=dict(colours=p.tiled_colour_ticks, staging=p.tiled_stage_ticks,
+                                     barriers=p.tiled_barrier_ticks, residuals=p.tiled_residual_ticks))
 
 
 class BodyId:
