@@ -630,7 +630,10 @@ def bench_pile(args):
                "plain_gpu_same_steps_ms_per_step": round(1000 * gpu_same / n_cpu, 3),
                "plain_gpu_vs_cpu_same_steps": round(t_cpu / gpu_same, 2) if gpu_same > 0 else None,
                "passes_equal_gpu": p_cpu == gpu_passes}
-    key = "config2_pile_16384/%s" % ("coloured_learned_residual" if head is runs.get("batched_learned") else "reference_residual")
+    if head is runs.get("batched_learned"):
+        key = "config2_pile_16384/%s_learned_residual" % ("tiled" if args.batched_order == "tiled" else "coloured")
+    else:
+        key = "config2_pile_16384/reference_residual"
     out = {
         "metric": "body·steps/s at solver residual <=1e-4 (plain vs batched+learned); HBM GB/s",
         "value": round(head["value"], 1),
@@ -645,7 +648,7 @@ def bench_pile(args):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": sc.name, "dynamic_bodies": n_dyn,
-                   "solver": "batched+learned GPU (greedy colour batches, learned warm start)" if "batched_learned" in runs
+                   "solver": ("batched+learned GPU (%s, learned warm start)" % ("colour batches within spatial tiles, each tile in one workgroup's LDS" if args.batched_order == "tiled" else "greedy colour batches")) if "batched_learned" in runs
                    else "plain GPU (reference order)",
                    "residual_tolerance": tol, "residual_max_passes": args.cap,
                    "residual_check_interval": args.check_every,
