@@ -9,18 +9,20 @@ colouring, warm start, velocity, relax and position passes, cache rebuild, islan
 integration.
 
 Two configurations of the GPU solver are measured from the same scene: plain (the reference's
-solve order, plain warm start) and batched + learned (greedy colour batches, learned warm start);
-the headline value is the latter. Prints ONE JSON line: value = dynamic body-steps per second,
-the plain solver's figures beside it, the roofline of the dominant kernel (the persistent solve
-kernel, timed with HIP events on the world's stream; algorithmic bytes 192 per manifold + 40 per
-contact point per velocity pass), and the CPU oracle (the reference algorithm restated in C++,
-one core) timed on the first timed steps of the plain world, continued from its state with the
-same residual policy (the same steps with the same pass counts: the reference order is
-bit-identical on both sides).
+solve order, plain warm start) and batched + learned (colour batches within 32 spatial tiles, each
+tile solved in one workgroup's LDS, learned warm start; --batched-order coloured for greedy colour
+batches over the whole pile); the headline value is the latter. Prints ONE JSON line: value =
+dynamic body-steps per second, the plain solver's figures beside it, the roofline of the batched
+solver's kernel (timed with HIP events on the world's stream; equivalent-HBM algorithmic bytes 192
+per manifold + 40 per contact point per velocity pass), and the CPU oracle (the reference algorithm
+restated in C++; bounds, pairs and narrowphase on the host's threads, the solver serial) timed on
+the first timed steps of the plain world, continued from its state with the same residual policy
+(the same steps with the same pass counts: the reference order is bit-identical on both sides).
 
 Other workloads: --workload batch (4096 independent scenes), ragdolls (2048 jointed ragdoll
 scenes, learned warm start on), learned (config-1 piles, passes to the residual with and without
-the learned warm start).
+the learned warm start; and the 16k pile's drop phase from identical states), queries, aggregates
+(FP32 against FP64).
 """
 import argparse
 import json
@@ -216,6 +218,7 @@ def bench_learned(args):
                      "residual_mean_last": float(st["residual"].mean())}
         bt.close()
         log("%s: %.2f s, %d velocity passes" % (name, el, res[name]["passes"]))
+    drop = learned_drop_phase(net)
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -244,8 +247,64 @@ def bench_learned(args):
         "vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "roofline": None,
         "cpu_baseline": cpu,
+        "drop_phase_16k": drop,
     }
     print(json.dumps(out), flush=True)
+
+
+def learned_drop_phase(net, every=20, first=40, last=300):
+    """The learned warm start while the 16,384-body pile falls and lands, where warm-start misses are
+    common: from the same states (snapshots of one plain run in the tiled order, every `every`-th
+    step), one residual-mode step (tolerance 1e-4, residual after every pass, at most 4096 passes)
+    with the plain warm start, with the learned one, and with the misses seeded with the impulses
+    the plain step converged to (what a perfect predictor of misses would give)."""
+    import numpy as np
+    from physxcoopa_amd import scenes, _abi
+    from physxcoopa_amd.world import World
+    sc = scenes.config2_scene(seed=2, n=16384, lattice=(64, 64, 40))
+    cfg = _abi.default_config(**sc.config)
+    cfg.solve_order = _abi.PXG_ORDER_TILED
+    w0 = World(cfg, sc.bodies, sc.shapes)
+    rc = _abi.default_config(residual_mode=1, residual_tolerance=1e-4, residual_max_iterations=4096,
+                             residual_check_interval=1, **sc.config)
+    rc.solve_order = _abi.PXG_ORDER_TILED
+    w = World(rc, sc.bodies, sc.shapes)
+    rows = []
+    for step in range(1, last + 1):
+        w0.step_fixed()
+        if step < first or step % every:
+            continue
+        b, f, (k0, i0) = w0.bodies(), w0.fat_aabbs(), w0.warm_cache()
+
+        def run(keys, imp, mlp):
+            w.set_mlp(mlp)
+            w.set_bodies(b)
+            w.set_fat_aabbs(f)
+            w.set_warm_cache(keys, imp)
+            w.synchronize()
+            t0 = time.perf_counter()
+            w.step_fixed()
+            st = w.stats()
+            return st, time.perf_counter() - t0
+        st, tp = run(k0, i0, None)
+        kc, ic = w.warm_cache()
+        known = set(k0.tolist())
+        miss = np.array([x not in known for x in kc.tolist()], bool)
+        stl, tl = run(k0, i0, net)
+        ste, _ = run(np.concatenate([k0, kc[miss]]), np.concatenate([i0, ic[miss]]), None)
+        rows.append({"step": step, "contacts": len(kc), "misses": int(miss.sum()),
+                     "passes": {"plain": st["velocity_passes"], "learned": stl["velocity_passes"],
+                                "exact_misses": ste["velocity_passes"]},
+                     "ms": {"plain": round(1000 * tp, 2), "learned": round(1000 * tl, 2)}})
+        log("drop phase step %d: %d misses of %d contacts; passes plain %d, learned %d, exact misses %d" %
+            (step, rows[-1]["misses"], rows[-1]["contacts"], st["velocity_passes"], stl["velocity_passes"],
+             ste["velocity_passes"]))
+    w.close()
+    w0.close()
+    tot = {k: sum(r["passes"][k] for r in rows) for k in ("plain", "learned", "exact_misses")}
+    return {"states": len(rows), "residual_tolerance": 1e-4, "residual_max_passes": 4096,
+            "velocity_passes": tot, "ms": {k: round(sum(r["ms"][k] for r in rows), 1) for k in ("plain", "learned")},
+            "per_state": rows}
 
 
 def bench_queries(args):

@@ -54,7 +54,8 @@ __device__ __forceinline__ void contact_features(const FeatBody& A, const FeatBo
 }
 
 // ---- inference on the matrix cores ----
-typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef short mlp_i16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint16_t mlp_f2bf(float f) {  // round to nearest even
     uint32_t u = __float_as_uint(f);
@@ -78,64 +79,60 @@ __device__ __forceinline__ void mlp_wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-// One wave64 evaluates rows row0 .. row0+15 of `feats` (kMlpIn floats per row, n rows valid) with
-// v_mfma_f32_16x16x16_bf16: lane = 16 g + r holds row r's K values 4g..4g+3 of each 16-wide K step
-// (A operand) and weight row r of the 16-wide N tile (B operand); each accumulator lane holds 4
-// rows x 1 column, re-laid out through the wave's LDS buffer h between layers. Writes 3 outputs
-// per valid row to out. All 64 lanes must call it.
+// eight consecutive BF16 weights (a row of W, K values k .. k+7) as an MFMA operand
+__device__ __forceinline__ mlp_bf16x8 mlp_w8(const uint16_t* p) {
+    mlp_i16x8 v;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v[q] = short(p[q]);
+    return __builtin_bit_cast(mlp_bf16x8, v);
+}
+// One 16x16 output tile over K = 32 with gfx950's v_mfma_f32_16x16x32_bf16: lane 16 g + r gives
+// row r's K values 8g .. 8g+7 (A) and weight row (column r of the tile)'s same K values (B); the
+// accumulator lane holds rows 4g .. 4g+3 of column r.
+__device__ __forceinline__ mlp_f32x4 mlp_mfma(mlp_i16x8 a, mlp_bf16x8 b, mlp_f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mlp_bf16x8, a), b, acc, 0, 0, 0);
+}
+// One wave64 evaluates rows row0 .. row0+15 of `feats` (kMlpIn floats per row, n rows valid): each
+// layer one K = 32 step per 16-wide output tile (layer 1: the 24 inputs padded to 32) or two (the
+// 64-wide hidden layers), activations re-laid out through the wave's LDS buffer h between layers.
+// Writes 3 outputs per valid row to out. All 64 lanes must call it.
 __device__ __forceinline__ void mlp_rows16(const MlpView& W, const float* feats, uint32_t row0, uint32_t n,
                                            uint16_t (*h)[kMlpH], float* out) {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
     const uint32_t row = row0 + uint32_t(r);
-    mlp_bf16x4 a1[2];
-    for (int ks = 0; ks < 2; ++ks) {
-        for (int q = 0; q < 4; ++q) {
-            int k = ks * 16 + 4 * g + q;
-            float x = 0.0f;
-            if (row < n && k < kMlpIn) x = (feats[size_t(row) * kMlpIn + k] - W.shift[k]) * W.scale[k];
-            a1[ks][q] = short(mlp_f2bf(x));
-        }
+    mlp_i16x8 a1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int k = 8 * g + q;
+        float x = 0.0f;
+        if (row < n && k < kMlpIn) x = (feats[size_t(row) * kMlpIn + k] - W.shift[k]) * W.scale[k];
+        a1[q] = short(mlp_f2bf(x));
     }
     for (int nt = 0; nt < 4; ++nt) {
         mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int ks = 0; ks < 2; ++ks) {
-            mlp_bf16x4 b;
-            const uint16_t* wp = &W.w1[(nt * 16 + r) * kMlpInPad + ks * 16 + 4 * g];
-            b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[ks], b, acc, 0, 0, 0);
-        }
+        acc = mlp_mfma(a1, mlp_w8(&W.w1[(nt * 16 + r) * kMlpInPad + 8 * g]), acc);
         int col = nt * 16 + r;
         for (int i = 0; i < 4; ++i) h[4 * g + i][col] = mlp_f2bf(fmaxf(acc[i] + W.b1[col], 0.0f));
     }
     mlp_wave_sync();
-    mlp_bf16x4 a2[4];
-    for (int ks = 0; ks < 4; ++ks)
-        for (int q = 0; q < 4; ++q) a2[ks][q] = short(h[r][ks * 16 + 4 * g + q]);
+    mlp_i16x8 a2[2];
+    for (int ks = 0; ks < 2; ++ks)
+        for (int q = 0; q < 8; ++q) a2[ks][q] = short(h[r][ks * 32 + 8 * g + q]);
     mlp_wave_sync();
     for (int nt = 0; nt < 4; ++nt) {
         mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int ks = 0; ks < 4; ++ks) {
-            mlp_bf16x4 b;
-            const uint16_t* wp = &W.w2[(nt * 16 + r) * kMlpH + ks * 16 + 4 * g];
-            b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[ks], b, acc, 0, 0, 0);
-        }
+        for (int ks = 0; ks < 2; ++ks) acc = mlp_mfma(a2[ks], mlp_w8(&W.w2[(nt * 16 + r) * kMlpH + ks * 32 + 8 * g]), acc);
         int col = nt * 16 + r;
         for (int i = 0; i < 4; ++i) h[4 * g + i][col] = mlp_f2bf(fmaxf(acc[i] + W.b2[col], 0.0f));
     }
     mlp_wave_sync();
-    mlp_bf16x4 a3[4];
-    for (int ks = 0; ks < 4; ++ks)
-        for (int q = 0; q < 4; ++q) a3[ks][q] = short(h[r][ks * 16 + 4 * g + q]);
+    mlp_i16x8 a3[2];
+    for (int ks = 0; ks < 2; ++ks)
+        for (int q = 0; q < 8; ++q) a3[ks][q] = short(h[r][ks * 32 + 8 * g + q]);
     mlp_wave_sync();
     mlp_f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int ks = 0; ks < 4; ++ks) {
-        mlp_bf16x4 b;
-        const uint16_t* wp = &W.w3[r * kMlpH + ks * 16 + 4 * g];
-        b[0] = short(wp[0]); b[1] = short(wp[1]); b[2] = short(wp[2]); b[3] = short(wp[3]);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[ks], b, acc, 0, 0, 0);
-    }
+    for (int ks = 0; ks < 2; ++ks) acc = mlp_mfma(a3[ks], mlp_w8(&W.w3[r * kMlpH + ks * 32 + 8 * g]), acc);
     if (r < kMlpOut) {
         for (int i = 0; i < 4; ++i) {
             uint32_t orow = row0 + uint32_t(4 * g + i);
