@@ -1,9 +1,9 @@
 // The learned warm start's network: 24 -> 64 -> 64 -> 3, ReLU.
 //
 // Inference runs on the matrix cores: each wave64 evaluates 16 contact rows with
-// v_mfma_f32_16x16x16_bf16 (BF16 inputs and weights, FP32 accumulation); the hidden activations
+// v_mfma_f32_16x16x32_bf16 (BF16 inputs and weights, FP32 accumulation); the hidden activations
 // are re-laid out through LDS between layers (the MFMA output layout holds 4 rows x 1 column per
-// lane, the input layout 1 row x 4 K values per lane). Training runs in FP32: per block a tile of
+// lane, the input layout 1 row x 8 K values per lane). Training runs in FP32: per block a tile of
 // 32 samples goes forward and backward in LDS, each block writes its partial gradients, and one
 // reduction kernel sums them in block order (deterministic) and applies Adam.
 #include "../../include/pxg.h"
