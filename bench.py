@@ -689,6 +689,16 @@ def bench_pile(args):
                "plain_gpu_same_steps_ms_per_step": round(1000 * gpu_same / n_cpu, 3),
                "plain_gpu_vs_cpu_same_steps": round(t_cpu / gpu_same, 2) if gpu_same > 0 else None,
                "passes_equal_gpu": p_cpu == gpu_passes}
+        # The sample is one or a few steps whose pass counts vary (4k-16k); for the whole timed
+        # workload the oracle would run the plain GPU's passes (bit-identical), at the sample's
+        # time per pass: the CPU rate on the same steps, scaled from the measured sample.
+        all_passes = sum(r["passes"] for r in runs["plain"]["per"])
+        t_cpu_all = t_cpu / max(1, sum(p_cpu)) * all_passes
+        cpu["same_steps_scaled"] = {
+            "value": round(n_dyn * args.steps / t_cpu_all, 1), "unit": "body·steps/s",
+            "method": "the sample's %.3f ms per velocity pass x the %d velocity passes the plain GPU world (and so the "
+                      "oracle, bit-identical) runs over the %d timed steps" %
+                      (1000 * t_cpu / max(1, sum(p_cpu)), all_passes, args.steps)}
     if head is runs.get("batched_learned"):
         key = "config2_pile_16384/%s_learned_residual" % ("tiled" if args.batched_order == "tiled" else "coloured")
     else:
@@ -731,6 +741,7 @@ def bench_pile(args):
             "residual_max": round(float(runs["plain"]["residual_max"]), 7)},
         "vs_plain_gpu": round(head["value"] / runs["plain"]["value"], 3) if "plain" in runs and "batched_learned" in runs else None,
         "vs_cpu": round(head["value"] / cpu["value"], 2) if cpu else None,
+        "vs_cpu_same_steps_scaled": round(head["value"] / cpu["same_steps_scaled"]["value"], 2) if cpu else None,
         "roofline": roofline_of(head["prof"], "k_solve_tiled" if head is runs.get("batched_learned")
                                 and args.batched_order == "tiled" else "k_solve_persistent"),
         "traffic_profiled": profiled_traffic(key),
