@@ -2147,6 +2147,123 @@ __device__ __forceinline__ void tl_op(int op, const TLds& t, float4* V, float4* 
     else if (op == 1) tl_run<1>(t, V, W, I, L, offset);
     else tl_run<2>(t, V, W, I, L, offset);
 }
+// ---- two lanes per solve position ----
+// A colour lasts as long as its longest manifold's instruction stream on one lane (a wave issues
+// one instruction every ~4 cycles whatever the dependences, while further waves on a SIMD cost
+// nothing), and half of a contact row is the two bodies' updates, independent of each other. So
+// a colour narrow enough takes two adjacent lanes per solve position: lane 0 of the pair owns
+// body A, lane 1 body B; both compute the row's shared values from both point velocities (read
+// across the pair with a DPP broadcast, bit for bit the same on both lanes), and each applies
+// the impulse to its own body with the reference's sign (A: v - J m, B: v + J m, written as
+// v + (J m) * -1 / * 1, which IEEE arithmetic makes identical to the subtraction / addition).
+__device__ __forceinline__ float pair_lane(float x, int side) {
+    const int v = __float_as_int(x);
+    // quad_perm [0,0,2,2] or [1,1,3,3]: both lanes of a pair read the pair's lane `side`
+    const int r = side ? __builtin_amdgcn_mov_dpp(v, 0xF5, 0xF, 0xF, false) : __builtin_amdgcn_mov_dpp(v, 0xA0, 0xF, 0xF, false);
+    return __int_as_float(r);
+}
+__device__ __forceinline__ v3 pair_lane3(v3 a, int side) {
+    return mk(pair_lane(a.x, side), pair_lane(a.y, side), pair_lane(a.z, side));
+}
+// apply_pair for the lane's own body: sg = -1 for A, +1 for B
+__device__ __forceinline__ void apply_own(SBody& s, v3 r, v3 J, float sg) {
+    if (!s.act) return;
+    s.v = s.v + (J * s.im) * sg;
+    s.w = s.w + mul(s.I, cross(r, J)) * sg;
+}
+// velocity_point with the lane's own body (side 0: A, 1: B)
+__device__ __forceinline__ float4 velocity_point2(SBody& S, float sg, v3 n, v3 t1, v3 t2, float mu, v3 pos, float4 row,
+                                                  float4 imp) {
+    v3 r = pos - S.x;
+    {
+        const float kn = row.x;
+        if (kn > kEps) {
+            v3 vme = S.v + cross(S.w, r);
+            v3 va = pair_lane3(vme, 0), vb = pair_lane3(vme, 1);
+            float vn = dot(vb - va, n);
+            float target = row.w;
+            float lambda = -(vn - target) / kn;
+            float ni = gmax(imp.x + lambda, 0.0f);
+            float delta = ni - imp.x;
+            imp.x = ni;
+            apply_own(S, r, n * delta, sg);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        v3 dir = t == 0 ? t1 : t2;
+        const float kt = t == 0 ? row.y : row.z;
+        if (kt <= kEps) continue;
+        v3 vme = S.v + cross(S.w, r);
+        v3 va = pair_lane3(vme, 0), vb = pair_lane3(vme, 1);
+        float vt = dot(vb - va, dir);
+        float lt = -vt / kt;
+        float maxf = mu * imp.x;
+        float old = t == 0 ? imp.y : imp.z;
+        float nt = sclamp(old + lt, -maxf, maxf);
+        float delta = nt - old;
+        if (t == 0) imp.y = nt; else imp.z = nt;
+        apply_own(S, r, dir * delta, sg);
+    }
+    return imp;
+}
+// tl_run with two lanes per solve position (both lanes of the pair call it with the same L)
+template <int OP>
+__device__ __forceinline__ void tl_run2(const TLds& t, float4* V, float4* W, const float4* I, uint32_t L, uint32_t offset,
+                                        int side) {
+    const float4 nf = t.hn[L];
+    const uint2 hm = t.hm[L];
+    const uint32_t l = side ? hm.x >> 16 : hm.x & 0xFFFFu;
+    const uint32_t cnt = hm.y & 7u, pt = hm.y >> 3;
+    const float sg = side ? 1.0f : -1.0f;
+    SBody S = tl_entry(V, W, I, l);
+    v3 n = ld3(nf);
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+    if (OP == 0) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i) {
+            if (i >= cnt) break;
+            const float4 imp = t.imp[pt + i];
+            v3 J = n * imp.x + t1 * imp.y + t2 * imp.z;
+            if (dot(J, J) > 0.0f) apply_own(S, ld3(t.p0[pt + i]) - S.x, J, sg);
+        }
+    } else {
+        const uint32_t r = cnt == 3u ? offset - 3u * (__umulhi(offset, 0xAAAAAAABu) >> 1)
+                                     : (cnt ? offset & (cnt - 1u) : 0u);
+        uint32_t idx[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) idx[k] = k < cnt ? (k + r >= cnt ? k + r - cnt : k + r) : k;
+        float4 pp[4], row[4], imp[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            pp[k] = row[k] = imp[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (k < cnt) {
+                pp[k] = t.p0[pt + idx[k]];
+                const float4 q = t.p1[pt + idx[k]];
+                row[k] = make_float4(q.x, q.y, q.z, OP == 2 ? q.w : pp[k].w);
+                imp[k] = t.imp[pt + idx[k]];
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            if (k >= cnt) break;
+            imp[k] = velocity_point2(S, sg, n, t1, t2, nf.w, ld3(pp[k]), row[k], imp[k]);
+        }
+        if (side == 0) {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k)
+                if (k < cnt) t.imp[pt + idx[k]] = imp[k];
+        }
+    }
+    if (S.act) { V[l] = st3(S.v, S.im); W[l] = st3(S.w, 1.0f); }
+}
+__device__ __forceinline__ void tl_op2(int op, const TLds& t, float4* V, float4* W, const float4* I, uint32_t L,
+                                       uint32_t offset, int side) {
+    if (op == 0) tl_run2<0>(t, V, W, I, L, offset, side);
+    else if (op == 1) tl_run2<1>(t, V, W, I, L, offset, side);
+    else tl_run2<2>(t, V, W, I, L, offset, side);
+}
 // residual_manifold (velocity passes) of a local solve position, all from LDS (the group's
 // velocities freshly staged after the pass). The normal target -sep / h of a separated point is
 // the velocity passes' target staged in p0.w.
@@ -2369,7 +2486,12 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_tiled(DevState d, StepP
             tk_stage += t1 - t0;
             for (uint32_t c = 0; c < C; ++c) {
                 const uint32_t k = k0 + c, lo = tb.lo[k], hi = tb.hi[k], lb = tb.lb[k];
-                for (uint32_t j = tid; lo + j < hi; j += nth) tl_op(op, t, V, W, I, lb + j, offset);
+                if (hi - lo <= nth / 2u) {
+                    // two lanes per solve position (the whole colour in one round)
+                    if (lo + (tid >> 1) < hi) tl_op2(op, t, V, W, I, lb + (tid >> 1), offset, int(tid & 1u));
+                } else {
+                    for (uint32_t j = tid; lo + j < hi; j += nth) tl_op(op, t, V, W, I, lb + j, offset);
+                }
                 __syncthreads();
             }
             const unsigned long long t2 = tick();
