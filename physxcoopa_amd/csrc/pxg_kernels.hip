@@ -2041,10 +2041,13 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
 // (LDS too small, or fewer workgroups than tiles) the workgroup runs its groups on global memory.
 // The arithmetic per manifold is velocity_manifold's / warm_manifold's / residual_manifold's.
 constexpr uint32_t kTileLds = 156u * 1024u;  // dynamic LDS per workgroup (one workgroup per CU)
+// threads per workgroup: 8 waves (2 per SIMD); 768 threads, which would let the widest colours of
+// a tile run two lanes per solve position too, measured no faster (611 against 601 ms per step)
+constexpr uint32_t kTileBlock = 512u;
 
-// exclusive scan over the kPersistBlock threads of a workgroup
+// exclusive scan over the kTileBlock threads of a workgroup
 __device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t& total) {
-    __shared__ uint32_t wsum[kPersistBlock / 64];
+    __shared__ uint32_t wsum[kTileBlock / 64];
     const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -2056,7 +2059,7 @@ __device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t& total) {
     __syncthreads();
     uint32_t off = 0, tot = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kPersistBlock / 64; ++w) {
+    for (uint32_t w = 0; w < kTileBlock / 64; ++w) {
         if (w < wid) off += wsum[w];
         tot += wsum[w];
     }
@@ -2319,10 +2322,10 @@ __device__ __forceinline__ uint32_t tl_find(const TileBlock& tb, uint32_t L) {
     return lo;
 }
 
-__global__ void __launch_bounds__(kPersistBlock) k_solve_tiled(DevState d, StepParams prm, SolveCfg cfg, uint32_t* bar) {
+__global__ void __launch_bounds__(kTileBlock) k_solve_tiled(DevState d, StepParams prm, SolveCfg cfg, uint32_t* bar) {
     extern __shared__ float4 t_lds[];
     __shared__ TileBlock tb;
-    __shared__ float s_red[kPersistBlock / 64];
+    __shared__ float s_red[kTileBlock / 64];
     GridBarrier g = grid_barrier_init(bar);
     uint32_t* err = &d.cnt->err;
     uint32_t rank = blockIdx.x;
@@ -2330,7 +2333,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_tiled(DevState d, StepP
         if (!near_blocks(bar, err, rank, g.nblocks)) return;
         g.near = 1;
     }
-    const uint32_t nblk = g.nblocks, tid = threadIdx.x, nth = kPersistBlock;
+    const uint32_t nblk = g.nblocks, tid = threadIdx.x, nth = kTileBlock;
     const uint32_t gtid = rank * nth + tid, gnth = nblk * nth;
     // ---- set-up: this workgroup's groups, colour ranges, and whether they fit in LDS ----
     const bool one_group = nblk == kTiles;
@@ -2582,7 +2585,7 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_tiled(DevState d, StepP
         __syncthreads();
         if (tid == 0) {
             float m = 0.0f;
-            for (uint32_t w = 0; w < kPersistBlock / 64; ++w) m = fmaxf(m, s_red[w]);
+            for (uint32_t w = 0; w < kTileBlock / 64; ++w) m = fmaxf(m, s_red[w]);
             if (m > 0.0f) atomicMax(&d.cnt->res_slot[checks & 1u], __float_as_uint(m));
         }
         if (grid_sync(g, err)) return;
@@ -3125,7 +3128,7 @@ hipError_t launch_solve_tiled(const DevState& d, const StepParams& p, const uint
     StepParams pp = p;
     uint32_t* bar = d.bar;
     void* args[] = {&dd, &pp, &c, &bar};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_tiled), dim3(nblocks), dim3(kPersistBlock),
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&k_solve_tiled), dim3(nblocks), dim3(kTileBlock),
                                       args, kTileLds, st);
 }
 void launch_solve_bytes(const DevState& d, const uint32_t cfgv[8], uint32_t bound, hipStream_t st) {
