@@ -95,8 +95,9 @@ typedef struct pxg_config {
      * is the reference's sequential Gauss-Seidel result; manifolds that share no such body run in
      * parallel (colours = longest chains of that order). PXG_ORDER_COLOURED: greedy colouring,
      * fewer colours (faster), a different but equally valid sequential order. PXG_ORDER_TILED:
-     * greedy colouring within spatial tiles of the awake bodies (manifolds crossing tiles in two
-     * later phases), each tile solved by one workgroup in LDS: the fastest order for large piles;
+     * greedy colouring within spatial tiles of the awake bodies (manifolds crossing tiles in a
+     * later phase, grouped by connected components), each tile solved by one workgroup in LDS:
+     * the fastest order for large piles;
      * worlds with joints use PXG_ORDER_COLOURED instead. */
     uint32_t solve_order;             /* PXG_ORDER_REFERENCE */
     /* Residual mode: the residual is evaluated after every residual_check_interval-th velocity

@@ -29,8 +29,8 @@ constexpr int kMaxLarge = 256;       // shapes on the brute-force broadphase pat
 constexpr uint32_t kMaxCells = 1u << 18;
 constexpr int kPairLanes = 9;        // broadphase threads per emitter shape
 constexpr uint32_t kMaxShapes = (1u << 20) / kPairLanes;  // the pair-count scan covers 2^20 entries
-// tiled solve order (pxg_tiles.hip): tiles per tiling (one per CU of one XCD), as x strips times
-// y tiles; colours over the three phases; the sort bucket of the manifolds the solver skips
+// tiled solve order (pxg_tiles.hip): tiles (one per CU of one XCD), as x strips times y tiles;
+// colours over the phases; the sort bucket of the manifolds the solver skips
 constexpr uint32_t kTiles = 32, kTileNx = 8, kTileNy = 4;
 constexpr uint32_t kTileColours = 64;
 constexpr uint32_t kTileSkipBucket = kTileColours * kTiles;
@@ -75,6 +75,7 @@ struct Counters {  // device-side counters, read back by the host at the step's 
     uint32_t tile_wg;       // tiled solve: workgroups | workgroups that held their tile in LDS << 8
     uint32_t tile_need;     // tiled solve: the most LDS bytes a workgroup's tile needed
     uint32_t tile_nu;       // tiled order: entries of all group tables
+    uint32_t tile_ncomp;    // tiled order: components of the crossing manifolds
 };
 
 struct DevState {
@@ -165,7 +166,9 @@ struct DevState {
     float4 *sb_x;       // position pass: position, effective inverse mass
     float4 *sb_ref;     // position pass reference positions
     // tiled solve order (pxg_tiles.hip; allocated on the first tiled step)
-    uint32_t *t_body;     // 2 * nb: each awake dynamic body's tile in the first, then the second tiling
+    uint32_t *t_body;     // 2 * nb: each awake dynamic body's tile, then each body's crossing-component root
+    uint32_t *t_par;      // nb: union-find parents of the crossing components
+    uint32_t *t_csz;      // nb: crossing manifolds per component root, then each root's group
     uint32_t *m_tile;     // per manifold: phase << 8 | group
     uint32_t *t_off;      // kTileSkipBucket + 2: first solve position of each (global colour * kTiles
                           // + group); then of the skipped group; then the end of the order

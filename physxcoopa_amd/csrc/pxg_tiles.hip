@@ -1,11 +1,13 @@
 // Tiled solve order (PXG_ORDER_TILED) for large piles: set-up kernels of one step.
 //
 // The awake dynamic bodies are split into kTiles spatial tiles of equal count (kTileNx strips by x
-// rank, each cut into kTileNy tiles by y rank), and into a second tiling shifted by half a tile in
-// both ranks. A manifold whose awake dynamic bodies all lie in one tile of the first tiling is in
-// phase 0 (group = that tile); else, if they share a tile of the second tiling, phase 1; else
-// phase 2 (one group). Groups of one phase share no awake dynamic body, so each group's colours
-// can run in one workgroup with workgroup barriers only, all groups of a phase at once.
+// rank, each cut into kTileNy tiles by y rank). A manifold whose awake dynamic bodies lie in one
+// tile is in phase 0 (group = that tile). The others cross a tile boundary: their bodies are joined
+// into connected components (union-find over the crossing manifolds), and each component goes
+// whole to one of kTiles groups of phase 1 (components dealt by size, in snake order). Groups of one phase share no
+// awake dynamic body, so each group's colours can run in one workgroup with workgroup barriers
+// only, all groups of a phase at once. (Phase 2, one group, is kept by the solve for generality;
+// this set-up leaves it empty.)
 //
 // The colouring (Jones-Plassmann, per-phase masks) gives each manifold a colour within its phase;
 // the solve order is sorted by (global colour = colours of the earlier phases + colour, group,
@@ -16,9 +18,10 @@
 // ground, can be in several groups' tables), and each solve position's two indices in its group's
 // table (sm_loc), let the solve stage a group's bodies in LDS.
 //
-// Every step: 4 radix sorts (bodies by x; bodies by (strip, y) for both tilings; manifolds by
-// key; manifold sides by (phase, group, body)), each over at most max(3 bodies, 2 pairs)
-// 32/64-bit keys, and one scan (the distinct sides).
+// Every step: 5 radix sorts (bodies by x; bodies by (strip, y); crossing components by size;
+// manifolds by key; manifold sides by (phase, group, body)), each over at most max(bodies,
+// 2 pairs) 32/64-bit keys, one union-find over the crossing manifolds and one scan (the distinct
+// sides).
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "pxg_state.h"
@@ -65,42 +68,96 @@ __global__ void k_tile_xkeys(DevState d) {
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
     if ((threadIdx.x & 63u) == 0u && c) atomicAdd(&d.cnt->tile_n, c);
 }
-// from the x rank r of each awake dynamic body: its strip in both tilings (the second shifted by
-// half a strip, the two half strips at the ends forming one strip), and the (tiling, strip, y)
-// sort keys of the second sort
+// from the x rank r of each awake dynamic body: its strip, and the (strip, y) keys of the second
+// sort
 __global__ void k_tile_strips(DevState d) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= d.nb) return;
     const uint32_t n = d.cnt->tile_n;
     const uint32_t b = d.tv_b[r];
-    unsigned long long ka = ~0ull, kb = ~0ull;
+    unsigned long long k = ~0ull;
     if (r < n) {
-        const unsigned long long sa = (static_cast<unsigned long long>(r) * kTileNx) / n;
-        const unsigned long long sb = ((static_cast<unsigned long long>(r) * kTileNx + n / 2u) / n) % kTileNx;
-        const unsigned long long y = ord_bits(d.b_pos[b].y);
-        ka = (sa << 32) | y;
-        kb = (1ull << 35) | (sb << 32) | y;
+        const unsigned long long s = (static_cast<unsigned long long>(r) * kTileNx) / n;
+        k = (s << 32) | ord_bits(d.b_pos[b].y);
     }
-    d.tk64_a[2u * r] = ka;
-    d.tk64_a[2u * r + 1u] = kb;
-    d.tv_a[2u * r] = b;
-    d.tv_a[2u * r + 1u] = b;
+    d.tk64_a[r] = k;
+    d.tv_a[r] = b;
 }
-// y rank within the strip -> tile of each awake dynamic body in both tilings
+// y rank within the strip -> tile of each awake dynamic body
 __global__ void k_tile_ranks(DevState d) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n2 = 2u * d.cnt->tile_n;
-    if (p >= n2) return;
-    const unsigned long long key = d.tk64_b[p];
-    const unsigned long long pre = key >> 32;  // tiling << 3 | strip
-    const uint32_t lo = lower_bound(d.tk64_b, n2, pre << 32);
-    const uint32_t hi = lower_bound(d.tk64_b, n2, (pre + 1ull) << 32);
+    const uint32_t n = d.cnt->tile_n;
+    if (p >= n) return;
+    const unsigned long long strip = d.tk64_b[p] >> 32;
+    const uint32_t lo = lower_bound(d.tk64_b, n, strip << 32);
+    const uint32_t hi = lower_bound(d.tk64_b, n, (strip + 1ull) << 32);
     const uint32_t rank = p - lo, cnt = hi - lo;
-    const uint32_t tiling = uint32_t(pre >> 3), strip = uint32_t(pre & 7u);
-    // the second tiling's cuts sit half a tile further along (its end halves form one tile)
-    const unsigned long long half = tiling ? cnt / 2u : 0u;
-    const uint32_t ty = uint32_t(((static_cast<unsigned long long>(rank) * kTileNy + half) / cnt) % kTileNy);
-    d.t_body[tiling * d.nb + d.tv_b[p]] = strip * kTileNy + ty;
+    const uint32_t ty = uint32_t((static_cast<unsigned long long>(rank) * kTileNy) / cnt);
+    d.t_body[d.tv_b[p]] = uint32_t(strip) * kTileNy + ty;
+}
+// union-find over the manifolds that cross a tile boundary (the larger root is hooked under the
+// smaller, so every component's root is its smallest body whatever the order of the unions)
+__device__ __forceinline__ uint32_t tile_find(uint32_t* parent, uint32_t x) {
+    while (true) {
+        const uint32_t p = parent[x];
+        if (p == x) return x;
+        const uint32_t g = parent[p];
+        if (g != p) parent[x] = g;  // path halving (a benign race: g is an ancestor of x)
+        x = p;
+    }
+}
+__global__ void k_tile_uf_init(DevState d) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < d.nb) d.t_par[b] = b;
+}
+__global__ void k_tile_cross(DevState d, uint32_t n_bound) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_bound || m >= d.cnt->manifolds || d.m_colour[m] != kColUncoloured) return;
+    const uint2 ab = d.m_ab[m];
+    if (!awake_dynamic(d.b_flags[ab.x]) || !awake_dynamic(d.b_flags[ab.y])) return;
+    if (d.t_body[ab.x] == d.t_body[ab.y]) return;
+    uint32_t a = tile_find(d.t_par, ab.x), b = tile_find(d.t_par, ab.y);
+    for (int guard = 0; guard < (1 << 24) && a != b; ++guard) {
+        if (a > b) { const uint32_t t = a; a = b; b = t; }
+        const uint32_t old = atomicCAS(&d.t_par[b], b, a);
+        if (old == b) return;
+        b = tile_find(d.t_par, old);
+        a = tile_find(d.t_par, a);
+    }
+}
+// each body's component root (read-only over the parent array), into t_body's second half
+__global__ void k_tile_uf_roots(DevState d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.nb) return;
+    uint32_t x = i;
+    for (uint32_t p = d.t_par[x]; p != x; p = d.t_par[x]) x = p;
+    d.t_body[d.nb + i] = x;
+}
+// crossing manifolds per component (counted at the root)
+__global__ void k_tile_csize(DevState d, uint32_t n_bound) {
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_bound || m >= d.cnt->manifolds || d.m_colour[m] != kColUncoloured) return;
+    const uint2 ab = d.m_ab[m];
+    if (!awake_dynamic(d.b_flags[ab.x]) || !awake_dynamic(d.b_flags[ab.y])) return;
+    if (d.t_body[ab.x] == d.t_body[ab.y]) return;
+    atomicAdd(&d.t_csz[d.t_body[d.nb + ab.x]], 1u);
+}
+// sort keys of the components: larger first, then by root; other bodies last
+__global__ void k_tile_ckeys(DevState d) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= d.nb) return;
+    const uint32_t c = d.t_csz[b];
+    d.tk64_a[b] = c ? ((static_cast<unsigned long long>(~c) << 32) | b) : ~0ull;
+    d.tv_a[b] = b;
+    if (c) atomicAdd(&d.cnt->tile_ncomp, 1u);
+}
+// the components dealt to the groups in snake order of size (0, 1, .., 31, 31, 30, .., 0, 0, 1, ..),
+// which keeps the groups' manifold counts close
+__global__ void k_tile_cgroup(DevState d) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.cnt->tile_ncomp) return;
+    const uint32_t round = k / kTiles, pos = k % kTiles;
+    d.t_csz[d.tv_b[k]] = (round & 1u) ? kTiles - 1u - pos : pos;
 }
 // phase and group of each active (still uncoloured) manifold, and the table keys of its two bodies
 // (phase * kTiles + group) << 20 | body (none for the other manifolds)
@@ -117,12 +174,10 @@ __global__ void k_tile_phase(DevState d, uint32_t n_bound) {
         if (!aa || !ba || ta == tb) {
             ph = 0u;
             g = aa ? ta : tb;
-        } else if (d.t_body[nb + ab.x] == d.t_body[nb + ab.y]) {
-            ph = 1u;
-            g = d.t_body[nb + ab.x];
         } else {
-            ph = 2u;
-            g = 0u;
+            // crossing: its component's group (both bodies have the same root)
+            ph = 1u;
+            g = d.t_csz[d.t_body[nb + ab.x]];
         }
         d.m_tile[m] = (ph << 8) | g;
         const uint32_t grp = (ph * kTiles + g) << 20;
@@ -208,6 +263,7 @@ __global__ void k_tile_slots(DevState d, uint32_t n_bound) {
 __global__ void k_tile_reset(DevState d) {
     if (threadIdx.x == 0) {
         d.cnt->tile_n = 0u;
+        d.cnt->tile_ncomp = 0u;
         d.cnt->tile_wg = 0u;
         d.cnt->tile_need = 0u;
         d.cnt->tile_cols[0] = d.cnt->tile_cols[1] = d.cnt->tile_cols[2] = 0u;
@@ -229,7 +285,7 @@ size_t tile_sort_bytes(uint32_t cap_bodies, uint32_t cap_items) {
     uint32_t* k32 = nullptr;
     unsigned long long* k64 = nullptr;
     if (rocprim::radix_sort_pairs(nullptr, a, k32, k32, k32, k32, cap_bodies, 0u, 32u) != hipSuccess) return 0;
-    if (rocprim::radix_sort_pairs(nullptr, b, k64, k64, k32, k32, 2u * cap_bodies, 0u, 36u) != hipSuccess) return 0;
+    if (rocprim::radix_sort_pairs(nullptr, b, k64, k64, k32, k32, cap_bodies, 0u, 64u) != hipSuccess) return 0;
     if (rocprim::radix_sort_keys(nullptr, c, k32, k32, cap_items, 0u, 32u) != hipSuccess) return 0;
     return std::max(a, std::max(b, c));
 }
@@ -250,9 +306,20 @@ hipError_t launch_tile_setup(const DevState& d, uint32_t n_bound, hipStream_t st
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tile_strips, dim3(nblocks(nb)), dim3(kBlock), 0, st, d);
     bytes = d.t_sort_bytes;
-    e = rocprim::radix_sort_pairs(d.t_sort_tmp, bytes, d.tk64_a, d.tk64_b, d.tv_a, d.tv_b, 2u * nb, 0u, 36u, st);
+    e = rocprim::radix_sort_pairs(d.t_sort_tmp, bytes, d.tk64_a, d.tk64_b, d.tv_a, d.tv_b, nb, 0u, 35u, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_tile_ranks, dim3(nblocks(2u * size_t(nb))), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_tile_ranks, dim3(nblocks(nb)), dim3(kBlock), 0, st, d);
+    hipLaunchKernelGGL(k_tile_uf_init, dim3(nblocks(nb)), dim3(kBlock), 0, st, d);
+    if (n_bound) hipLaunchKernelGGL(k_tile_cross, dim3(nblocks(n_bound)), dim3(kBlock), 0, st, d, n_bound);
+    hipLaunchKernelGGL(k_tile_uf_roots, dim3(nblocks(nb)), dim3(kBlock), 0, st, d);
+    // components sized, sorted by size, dealt to the groups (t_csz then holds each root's group)
+    hipLaunchKernelGGL(k_tile_fill, dim3(nblocks(nb)), dim3(kBlock), 0, st, d.t_csz, 0u, size_t(nb));
+    if (n_bound) hipLaunchKernelGGL(k_tile_csize, dim3(nblocks(n_bound)), dim3(kBlock), 0, st, d, n_bound);
+    hipLaunchKernelGGL(k_tile_ckeys, dim3(nblocks(nb)), dim3(kBlock), 0, st, d);
+    bytes = d.t_sort_bytes;
+    e = rocprim::radix_sort_pairs(d.t_sort_tmp, bytes, d.tk64_a, d.tk64_b, d.tv_a, d.tv_b, nb, 0u, 64u, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tile_cgroup, dim3(nblocks(nb)), dim3(kBlock), 0, st, d);
     if (n_bound) hipLaunchKernelGGL(k_tile_phase, dim3(nblocks(n_bound)), dim3(kBlock), 0, st, d, n_bound);
     return hipSuccess;
 }

@@ -358,7 +358,7 @@ static int ensure_tiled(pxg_world* w) {
     const size_t B = w->cap_b, M = w->cap_m, P = w->cap_p;
     if (B >= (1u << 20) || M > (1u << 20) || 2 * P > (1u << 22))
         return fail(PXG_ERR_CAPACITY, "tiled solve order: at most 2^20 - 1 bodies, 2^20 manifolds and 2^21 pairs");
-    DALLOC(d.t_body, 2 * B); DALLOC(d.m_tile, M);
+    DALLOC(d.t_body, 2 * B); DALLOC(d.t_par, B); DALLOC(d.t_csz, B); DALLOC(d.m_tile, M);
     DALLOC(d.t_off, size_t(kTileSkipBucket) + 16); DALLOC(d.tb_off, 3 * kTiles + 16);
     DALLOC(d.ts_a, 2 * P); DALLOC(d.tf, 2 * P); DALLOC(d.tu, 2 * P); DALLOC(d.tb_key, 2 * P); DALLOC(d.tb_list, 2 * P);
     DALLOC(d.sm_loc, M);
