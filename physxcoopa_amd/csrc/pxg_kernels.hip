@@ -2168,11 +2168,13 @@ __device__ __forceinline__ float pair_lane(float x, int side) {
 __device__ __forceinline__ v3 pair_lane3(v3 a, int side) {
     return mk(pair_lane(a.x, side), pair_lane(a.y, side), pair_lane(a.z, side));
 }
-// apply_pair for the lane's own body: sg = -1 for A, +1 for B
+// apply_pair for the lane's own body: sg = -1 for A, +1 for B. x * sg is exact, so the fused
+// v + x * sg rounds once, as v - x and v + x do.
 __device__ __forceinline__ void apply_own(SBody& s, v3 r, v3 J, float sg) {
     if (!s.act) return;
-    s.v = s.v + (J * s.im) * sg;
-    s.w = s.w + mul(s.I, cross(r, J)) * sg;
+    const v3 lv = J * s.im, aw = mul(s.I, cross(r, J));
+    s.v = mk(__builtin_fmaf(lv.x, sg, s.v.x), __builtin_fmaf(lv.y, sg, s.v.y), __builtin_fmaf(lv.z, sg, s.v.z));
+    s.w = mk(__builtin_fmaf(aw.x, sg, s.w.x), __builtin_fmaf(aw.y, sg, s.w.y), __builtin_fmaf(aw.z, sg, s.w.z));
 }
 // velocity_point with the lane's own body (side 0: A, 1: B)
 __device__ __forceinline__ float4 velocity_point2(SBody& S, float sg, v3 n, v3 t1, v3 t2, float mu, v3 pos, float4 row,
@@ -2267,6 +2269,127 @@ __device__ __forceinline__ void tl_op2(int op, const TLds& t, float4* V, float4*
     else if (op == 1) tl_run2<1>(t, V, W, I, L, offset, side);
     else tl_run2<2>(t, V, W, I, L, offset, side);
 }
+// ---- eight lanes per solve position ----
+// A manifold with several points is a long chain of rows; eight lanes cut each row to about half
+// the instructions of the two-lane form. Lanes 0-3 of the octet hold body A, lanes 4-7 body B;
+// lane c of a quad holds component c of its body's velocities (x, y, z; lane 3 repeats lane 0's
+// work and writes nothing). Each lane keeps the other two components of the point's lever arm, the
+// row direction and the body's inverse inertia row it needs as constants, so a row's arithmetic is
+// per component, with the reference's operations and order:
+//   cross(w, r)_c = w_{c+1} r_{c+2} - r_{c+1} w_{c+2}        (the w's from the quad)
+//   (vb - va)_c    from the other body's quad                (row_shl / row_shr by 4)
+//   dot(d, n)      = (d_x n_x + d_y n_y) + d_z n_z            (the products from the quad)
+//   mul(I, c)_c    = (I_c0 c_x + I_c1 c_y) + I_c2 c_z
+// so every lane of the octet computes the same bits as velocity_point.
+template <int CTRL>
+__device__ __forceinline__ float qperm(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
+}
+constexpr int kQpNext = 1 | (2 << 2) | (0 << 4) | (1 << 6);  // lane c reads lane c + 1 (mod 3)
+constexpr int kQpPrev = 2 | (0 << 2) | (1 << 4) | (2 << 6);  // lane c reads lane c + 2 (mod 3)
+constexpr int kQpX = 0x00, kQpY = 0x55, kQpZ = 0xAA;
+// the row's shared value from the quad: (p_x + p_y) + p_z on every lane
+__device__ __forceinline__ float quad_dot(float p) {
+    return (qperm<kQpX>(p) + qperm<kQpY>(p)) + qperm<kQpZ>(p);
+}
+// (vb - va) component: body A's quad reads B's lanes (row_shl 4), B's quad A's (row_shr 4)
+__device__ __forceinline__ float octet_rel(float u) {
+    const int ui = __float_as_int(u);
+    const float ua = __int_as_float(__builtin_amdgcn_update_dpp(ui, ui, 0x114, 0xF, 0xA, false));
+    const float ub = __int_as_float(__builtin_amdgcn_update_dpp(ui, ui, 0x104, 0xF, 0x5, false));
+    return ub - ua;
+}
+__device__ __forceinline__ float rot3(v3 a, uint32_t c) { return c == 0u ? a.x : (c == 1u ? a.y : a.z); }
+// a row on the octet takes the row direction rotated to the lane (D0 = dir_c, D1 = dir_{c+1},
+// D2 = dir_{c+2}), the lever arm's other two components (R1 = r_{c+1}, R2 = r_{c+2}) and the
+// body's inverse inertia row c (M0, M1, M2); OctRow is the lane's velocity components
+struct OctRow {
+    float v, w;
+};
+__device__ __forceinline__ float octet_vrel(const OctRow& s, float D0, float R1, float R2) {
+    const float u = s.v + (qperm<kQpNext>(s.w) * R2 - R1 * qperm<kQpPrev>(s.w));
+    return quad_dot(octet_rel(u) * D0);
+}
+__device__ __forceinline__ void octet_apply(OctRow& s, bool act, float im, float sg, float D0, float D1, float D2,
+                                            float R1, float R2, float M0, float M1, float M2, float delta) {
+    if (!act) return;
+    const float J0 = D0 * delta, J1 = D1 * delta, J2 = D2 * delta;
+    const float cr = R1 * J2 - J1 * R2;
+    const float aw = (M0 * qperm<kQpX>(cr) + M1 * qperm<kQpY>(cr)) + M2 * qperm<kQpZ>(cr);
+    s.v = __builtin_fmaf(J0 * im, sg, s.v);
+    s.w = __builtin_fmaf(aw, sg, s.w);
+}
+// velocity_manifold (a velocity pass, or a relax pass when relax) of local solve position L on
+// the octet; lane = the lane's index in the octet
+__device__ __forceinline__ void tl_run8(const TLds& t, float4* V, float4* W, const float4* I, uint32_t L,
+                                        uint32_t offset, uint32_t lane, bool relax) {
+    const float4 nf = t.hn[L];
+    const uint2 hm = t.hm[L];
+    const uint32_t side = lane >> 2, c = lane & 3u, cc = c == 3u ? 0u : c;
+    const uint32_t c1 = cc == 2u ? 0u : cc + 1u, c2 = cc == 0u ? 2u : cc - 1u;
+    const uint32_t l = side ? hm.x >> 16 : hm.x & 0xFFFFu;
+    const uint32_t cnt = hm.y & 7u, pt = hm.y >> 3;
+    const float sg = side ? 1.0f : -1.0f;
+    const float* vb = reinterpret_cast<const float*>(V + l);
+    const float* wb = reinterpret_cast<const float*>(W + l);
+    const float* ib = reinterpret_cast<const float*>(I + 3u * l);
+    OctRow s;
+    s.v = vb[cc];
+    s.w = wb[cc];
+    const float im = vb[3];
+    const bool act = wb[3] != 0.0f;
+    const float M0 = ib[cc], M1 = ib[3u + cc], M2 = ib[6u + cc];
+    const float X1 = ib[9u + c1], X2 = ib[9u + c2];
+    const v3 n = ld3(nf);
+    v3 t1, t2;
+    orthonormal_basis(n, t1, t2);
+    const float N0 = rot3(n, cc), N1 = rot3(n, c1), N2 = rot3(n, c2);
+    const float A0 = rot3(t1, cc), A1 = rot3(t1, c1), A2 = rot3(t1, c2);
+    const float B0 = rot3(t2, cc), B1 = rot3(t2, c1), B2 = rot3(t2, c2);
+    const uint32_t r = cnt == 3u ? offset - 3u * (__umulhi(offset, 0xAAAAAAABu) >> 1)
+                                 : (cnt ? offset & (cnt - 1u) : 0u);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        if (k >= cnt) break;
+        const uint32_t q = pt + (k + r >= cnt ? k + r - cnt : k + r);
+        const float* pp = reinterpret_cast<const float*>(t.p0 + q);
+        const float4 row = t.p1[q];
+        float4 imp = t.imp[q];
+        const float R1 = pp[c1] - X1, R2 = pp[c2] - X2;
+        if (row.x > kEps) {
+            const float target = relax ? row.w : pp[3];
+            const float vn = octet_vrel(s, N0, R1, R2);
+            const float lambda = -(vn - target) / row.x;
+            const float ni = gmax(imp.x + lambda, 0.0f);
+            const float delta = ni - imp.x;
+            imp.x = ni;
+            octet_apply(s, act, im, sg, N0, N1, N2, R1, R2, M0, M1, M2, delta);
+        }
+        if (!(row.y <= kEps)) {
+            const float vt = octet_vrel(s, A0, R1, R2);
+            const float lt = -vt / row.y;
+            const float maxf = nf.w * imp.x;
+            const float nt = sclamp(imp.y + lt, -maxf, maxf);
+            const float delta = nt - imp.y;
+            imp.y = nt;
+            octet_apply(s, act, im, sg, A0, A1, A2, R1, R2, M0, M1, M2, delta);
+        }
+        if (!(row.z <= kEps)) {
+            const float vt = octet_vrel(s, B0, R1, R2);
+            const float lt = -vt / row.z;
+            const float maxf = nf.w * imp.x;
+            const float nt = sclamp(imp.z + lt, -maxf, maxf);
+            const float delta = nt - imp.z;
+            imp.z = nt;
+            octet_apply(s, act, im, sg, B0, B1, B2, R1, R2, M0, M1, M2, delta);
+        }
+        if (lane == 0u) t.imp[q] = imp;
+    }
+    if (act && c < 3u) {
+        reinterpret_cast<float*>(V + l)[c] = s.v;
+        reinterpret_cast<float*>(W + l)[c] = s.w;
+    }
+}
 // residual_manifold (velocity passes) of a local solve position, all from LDS (the group's
 // velocities freshly staged after the pass). The normal target -sep / h of a separated point is
 // the velocity passes' target staged in p0.w.
@@ -2310,7 +2433,25 @@ struct TileBlock {
     uint32_t lo[3 * kTileColours], hi[3 * kTileColours], lb[3 * kTileColours];  // per (phase, colour)
     uint32_t nbody[3], body0[3];     // the group's body table
     uint32_t p_off, v_off[3];        // LDS float4 offsets: points, each phase's table
+    uint16_t plan[3 * kTileColours]; // per (phase, colour): octets and the lanes of the rest
 };
+// How a colour's velocity and relax passes spread over the workgroup's lanes (its solve positions
+// are ordered by point count, most first): the first H on eight lanes each, the rest on two lanes
+// or one each, all in one round; or, when that does not fit, every position on two lanes (at most
+// half as many positions as lanes) or on one lane in rounds. H is every position when eight lanes
+// each fit, else the positions with two or more points rounded up to a multiple of 8 (whole
+// waves). plan = H | mode << 8.
+constexpr uint32_t kPlanOct = 0u, kPlanMix2 = 1u, kPlanMix1 = 2u, kPlanOld = 3u;
+__device__ __forceinline__ uint32_t tile_plan(uint32_t n, uint32_t heavy, uint32_t nth) {
+    if (n * 8u <= nth) return n | (kPlanOct << 8);
+    const uint32_t H = (heavy + 7u) & ~7u;
+    if (H * 8u <= nth && H <= n) {
+        const uint32_t lanes = nth - H * 8u, rest = n - H;
+        if (rest * 2u <= lanes) return H | (kPlanMix2 << 8);
+        if (rest <= lanes) return H | (kPlanMix1 << 8);
+    }
+    return kPlanOld << 8;
+}
 // the (phase, colour) entry holding local solve position L: the last with lb <= L
 __device__ __forceinline__ uint32_t tl_find(const TileBlock& tb, uint32_t L) {
     uint32_t lo = 0, hi = 3u * kTileColours;
@@ -2423,6 +2564,17 @@ __global__ void __launch_bounds__(kTileBlock) k_solve_tiled(DevState d, StepPara
     }
     if (tid == 0) atomicAdd(&d.cnt->tile_wg, 1u | (lds ? 0x100u : 0u));
     if (lds) {
+        // each colour's lane plan from its positions' point counts (more points first)
+        for (uint32_t k = tid; k < 3u * kTileColours; k += nth) {
+            const uint32_t n = tb.hi[k] - tb.lo[k], L0 = tb.lb[k];
+            uint32_t a = 0, b = n;
+            while (a < b) {
+                const uint32_t mid = (a + b) >> 1;
+                if ((t.hm[L0 + mid].y & 7u) >= 2u) a = mid + 1u;
+                else b = mid;
+            }
+            tb.plan[k] = uint16_t(tile_plan(n, a, nth));
+        }
         t.p0 = t_lds + tb.p_off;
         t.p1 = t.p0 + tb.npts;
         t.imp = t.p1 + tb.npts;
@@ -2488,12 +2640,24 @@ __global__ void __launch_bounds__(kTileBlock) k_solve_tiled(DevState d, StepPara
             const unsigned long long t1 = tick();
             tk_stage += t1 - t0;
             for (uint32_t c = 0; c < C; ++c) {
-                const uint32_t k = k0 + c, lo = tb.lo[k], hi = tb.hi[k], lb = tb.lb[k];
-                if (hi - lo <= nth / 2u) {
-                    // two lanes per solve position (the whole colour in one round)
-                    if (lo + (tid >> 1) < hi) tl_op2(op, t, V, W, I, lb + (tid >> 1), offset, int(tid & 1u));
-                } else {
-                    for (uint32_t j = tid; lo + j < hi; j += nth) tl_op(op, t, V, W, I, lb + j, offset);
+                const uint32_t k = k0 + c, lo = tb.lo[k], hi = tb.hi[k], lb = tb.lb[k], n = hi - lo;
+                const uint32_t plan = tb.plan[k];
+                uint32_t H = plan & 0xFFu, mode = plan >> 8;
+                // the warm start and colours without a plan: two lanes per solve position when the
+                // whole colour fits in one round, else one lane in rounds
+                if (op == 0 || mode == kPlanOld) {
+                    H = 0u;
+                    mode = n <= nth / 2u ? kPlanMix2 : kPlanMix1;
+                }
+                // eight lanes for the first H positions, two or one for the rest (whole waves
+                // each: H is all positions or a multiple of 8)
+                if (tid < 8u * H) {
+                    tl_run8(t, V, W, I, lb + (tid >> 3), offset, tid & 7u, op == 2);
+                } else if (mode == kPlanMix2) {
+                    const uint32_t j = H + ((tid - 8u * H) >> 1);
+                    if (j < n) tl_op2(op, t, V, W, I, lb + j, offset, int(tid & 1u));
+                } else if (mode == kPlanMix1) {
+                    for (uint32_t j = H + (tid - 8u * H); j < n; j += nth) tl_op(op, t, V, W, I, lb + j, offset);
                 }
                 __syncthreads();
             }

@@ -11,16 +11,17 @@
 //
 // The colouring (Jones-Plassmann, per-phase masks) gives each manifold a colour within its phase;
 // the solve order is sorted by (global colour = colours of the earlier phases + colour, group,
-// manifold), so it is colour-major and every colour is a valid colour of the whole world (its
-// manifolds share no awake dynamic body): the sequential solve of that order is what the tiled
-// solve computes. t_off gives every (global colour, group) range; per (phase, group) a table of
+// more contact points first, manifold), so it is colour-major and every colour is a valid colour
+// of the whole world (its manifolds share no awake dynamic body): the sequential solve of that
+// order is what the tiled solve computes. Within a (colour, group) range the manifolds with more
+// points come first, so the solve can give them more lanes each. t_off gives every (global colour, group) range; per (phase, group) a table of
 // every body its manifolds touch (tb_list, ascending; a body that is not awake dynamic, such as the
 // ground, can be in several groups' tables), and each solve position's two indices in its group's
 // table (sm_loc), let the solve stage a group's bodies in LDS.
 //
 // Every step: 5 radix sorts (bodies by x; bodies by (strip, y); crossing components by size;
-// manifolds by key; manifold sides by (phase, group, body)), each over at most max(bodies,
-// 2 pairs) 32/64-bit keys, one union-find over the crossing manifolds and one scan (the distinct
+// manifolds by (colour, group, points); manifold sides by (phase, group, body)), each over at most
+// max(bodies, 2 pairs) 32/64-bit keys, one union-find over the crossing manifolds and one scan (the distinct
 // sides).
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -189,34 +190,41 @@ __global__ void k_tile_phase(DevState d, uint32_t n_bound) {
 }
 // solve-order keys: (global colour * kTiles + group) << 20 | manifold; the skipped group after
 // every colour; triggers, uncoloured and unused entries last
+// order keys: bucket (global colour * kTiles + group; kTileSkipBucket for skipped manifolds, one
+// more for none) << 2 | (4 - points); the value is the manifold, and the stable sort keeps
+// manifold order within a key
+constexpr uint32_t kOrderKeyBits = 14u;
 __global__ void k_tile_keys(DevState d, uint32_t n_bound) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n_bound) return;
-    uint32_t key = kNone;
+    uint32_t key = (kTileSkipBucket + 1u) << 2;
     if (m < d.cnt->manifolds) {
         const uint32_t c = d.m_colour[m];
+        const uint32_t cnt = d.m_meta[m].x & 255u;
+        const uint32_t fewer = cnt >= 1u && cnt <= 4u ? 4u - cnt : 3u;
         if (c == kColSkipped) {
-            key = (kTileSkipBucket << 20) | m;
+            key = (kTileSkipBucket << 2) | fewer;
         } else if (c < uint32_t(kMaskColours)) {
             const uint32_t t = d.m_tile[m], ph = t >> 8, g = t & 255u;
             const uint32_t* cols = d.cnt->tile_cols;
             const uint32_t base = ph == 0u ? 0u : (ph == 1u ? cols[0] : cols[0] + cols[1]);
             const uint32_t gc = base + c;
             if (gc >= kTileColours) atomicOr(&d.cnt->err, ERR_COLOURS);
-            else key = ((gc * kTiles + g) << 20) | m;
+            else key = ((gc * kTiles + g) << 2) | fewer;
         }
     }
     d.tk_a[m] = key;
+    d.tv_a[m] = m;
 }
 // bucket offsets (t_off), the colour table, the order and the solver counts
 __global__ void k_tile_offsets(DevState d, uint32_t n_bound) {
     for (uint32_t b = threadIdx.x; b <= kTileSkipBucket + 1u; b += blockDim.x)
-        d.t_off[b] = lower_bound(d.tk_b, n_bound, b << 20);
+        d.t_off[b] = lower_bound(d.tk_b, n_bound, b << 2);
 }
 __global__ void k_tile_order(DevState d, uint32_t n_bound) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t active = d.t_off[kTileSkipBucket], ordered = d.t_off[kTileSkipBucket + 1u];
-    if (i < ordered && i < n_bound) d.order[i] = d.tk_b[i] & 0xFFFFFu;
+    if (i < ordered && i < n_bound) d.order[i] = d.tv_b[i];
     if (i <= uint32_t(kBuckets)) {
         const uint32_t c = i;
         d.cnt->colour_off[c] = c == uint32_t(kBuckets) ? ordered
@@ -281,13 +289,14 @@ inline uint32_t nblocks(size_t n) { return uint32_t(std::min<size_t>((n + kBlock
 }  // namespace
 
 size_t tile_sort_bytes(uint32_t cap_bodies, uint32_t cap_items) {
-    size_t a = 0, b = 0, c = 0;
+    size_t a = 0, b = 0, c = 0, e = 0;
     uint32_t* k32 = nullptr;
     unsigned long long* k64 = nullptr;
     if (rocprim::radix_sort_pairs(nullptr, a, k32, k32, k32, k32, cap_bodies, 0u, 32u) != hipSuccess) return 0;
     if (rocprim::radix_sort_pairs(nullptr, b, k64, k64, k32, k32, cap_bodies, 0u, 64u) != hipSuccess) return 0;
     if (rocprim::radix_sort_keys(nullptr, c, k32, k32, cap_items, 0u, 32u) != hipSuccess) return 0;
-    return std::max(a, std::max(b, c));
+    if (rocprim::radix_sort_pairs(nullptr, e, k32, k32, k32, k32, cap_items, 0u, kOrderKeyBits) != hipSuccess) return 0;
+    return std::max(std::max(a, e), std::max(b, c));
 }
 
 hipError_t launch_tile_setup(const DevState& d, uint32_t n_bound, hipStream_t st) {
@@ -329,7 +338,7 @@ hipError_t launch_tile_order(const DevState& d, uint32_t n_bound, hipStream_t st
     const uint32_t n = n_bound ? n_bound : 1u;
     hipLaunchKernelGGL(k_tile_keys, dim3(nblocks(n)), dim3(kBlock), 0, st, d, n);
     size_t bytes = d.t_sort_bytes;
-    hipError_t e = rocprim::radix_sort_keys(d.t_sort_tmp, bytes, d.tk_a, d.tk_b, n, 0u, 32u, st);
+    hipError_t e = rocprim::radix_sort_pairs(d.t_sort_tmp, bytes, d.tk_a, d.tk_b, d.tv_a, d.tv_b, n, 0u, kOrderKeyBits, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(1024), 0, st, d, n);
     hipLaunchKernelGGL(k_tile_order, dim3(nblocks(std::max<uint32_t>(n, kBuckets + 1))), dim3(kBlock), 0, st, d, n);

@@ -364,7 +364,8 @@ static int ensure_tiled(pxg_world* w) {
     DALLOC(d.sm_loc, M);
     DALLOC(d.t_col_best, 3 * B); DALLOC(d.t_col_mask, 3 * B);
     const size_t items = std::max(2 * P, 3 * B);
-    DALLOC(d.tk_a, items); DALLOC(d.tk_b, items); DALLOC(d.tv_a, 2 * B); DALLOC(d.tv_b, 2 * B);
+    DALLOC(d.tk_a, items); DALLOC(d.tk_b, items);
+    DALLOC(d.tv_a, std::max(2 * B, M)); DALLOC(d.tv_b, std::max(2 * B, M));
     DALLOC(d.tk64_a, 2 * B); DALLOC(d.tk64_b, 2 * B);
     d.t_sort_bytes = tile_sort_bytes(uint32_t(B), uint32_t(items));
     if (!d.t_sort_bytes) return fail(PXG_ERR_HIP, "tiled solve order: sizing the sorts failed");
