@@ -2041,9 +2041,10 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
 // (LDS too small, or fewer workgroups than tiles) the workgroup runs its groups on global memory.
 // The arithmetic per manifold is velocity_manifold's / warm_manifold's / residual_manifold's.
 constexpr uint32_t kTileLds = 156u * 1024u;  // dynamic LDS per workgroup (one workgroup per CU)
-// threads per workgroup: 8 waves (2 per SIMD); 768 threads, which would let the widest colours of
-// a tile run two lanes per solve position too, measured no faster (611 against 601 ms per step)
-constexpr uint32_t kTileBlock = 512u;
+// threads per workgroup: 12 waves (3 per SIMD, within the kernel's register budget), so that more
+// colours fit the eight-lane and two-lane plans (431 against 438 ms per residual-mode step on the
+// settled 16k pile with 512)
+constexpr uint32_t kTileBlock = 768u;
 
 // exclusive scan over the kTileBlock threads of a workgroup
 __device__ __forceinline__ uint32_t tile_scan(uint32_t v, uint32_t& total) {
