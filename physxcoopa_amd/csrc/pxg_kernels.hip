@@ -2279,7 +2279,7 @@ __device__ __forceinline__ void tl_op2(int op, const TLds& t, float4* V, float4*
 // per component, with the reference's operations and order:
 //   cross(w, r)_c = w_{c+1} r_{c+2} - r_{c+1} w_{c+2}        (the w's from the quad)
 //   (vb - va)_c    from the other body's quad                (row_shl / row_shr by 4)
-//   dot(d, n)      = (d_x n_x + d_y n_y) + d_z n_z            (the products from the quad)
+//   dot(d, n)      = (d_x n_x + d_y n_y) + d_z n_z            (d's components from the quad)
 //   mul(I, c)_c    = (I_c0 c_x + I_c1 c_y) + I_c2 c_z
 // so every lane of the octet computes the same bits as velocity_point.
 template <int CTRL>
@@ -2289,16 +2289,18 @@ __device__ __forceinline__ float qperm(float x) {
 constexpr int kQpNext = 1 | (2 << 2) | (0 << 4) | (1 << 6);  // lane c reads lane c + 1 (mod 3)
 constexpr int kQpPrev = 2 | (0 << 2) | (1 << 4) | (2 << 6);  // lane c reads lane c + 2 (mod 3)
 constexpr int kQpX = 0x00, kQpY = 0x55, kQpZ = 0xAA;
-// the row's shared value from the quad: (p_x + p_y) + p_z on every lane
-__device__ __forceinline__ float quad_dot(float p) {
-    return (qperm<kQpX>(p) + qperm<kQpY>(p)) + qperm<kQpZ>(p);
-}
-// (vb - va) component: body A's quad reads B's lanes (row_shl 4), B's quad A's (row_shr 4)
+// (vb - va) component c on both quads: body A's lanes take B's value (row_shl 4) minus their own,
+// B's lanes their own minus A's (row_shr 4), each instruction writing only its quads (bank mask).
+// The leading wait covers the DPP read of u written just before (two wait states after a VALU
+// write); the compiler adds the one before the DPP reads of the result.
 __device__ __forceinline__ float octet_rel(float u) {
-    const int ui = __float_as_int(u);
-    const float ua = __int_as_float(__builtin_amdgcn_update_dpp(ui, ui, 0x114, 0xF, 0xA, false));
-    const float ub = __int_as_float(__builtin_amdgcn_update_dpp(ui, ui, 0x104, 0xF, 0x5, false));
-    return ub - ua;
+    float d;
+    asm("s_nop 1\n\t"
+        "v_sub_f32_dpp %0, %1, %1 row_shl:4 row_mask:0xf bank_mask:0x5\n\t"
+        "v_subrev_f32_dpp %0, %1, %1 row_shr:4 row_mask:0xf bank_mask:0xa"
+        : "=&v"(d)
+        : "v"(u));
+    return d;
 }
 __device__ __forceinline__ float rot3(v3 a, uint32_t c) { return c == 0u ? a.x : (c == 1u ? a.y : a.z); }
 // a row on the octet takes the row direction rotated to the lane (D0 = dir_c, D1 = dir_{c+1},
@@ -2307,18 +2309,22 @@ __device__ __forceinline__ float rot3(v3 a, uint32_t c) { return c == 0u ? a.x :
 struct OctRow {
     float v, w;
 };
-__device__ __forceinline__ float octet_vrel(const OctRow& s, float D0, float R1, float R2) {
+// the relative velocity along dir (full vector, the same on every lane): dot(vb - va, dir) with
+// the three components of vb - va read from the quad
+__device__ __forceinline__ float octet_vrel(const OctRow& s, v3 dir, float R1, float R2) {
     const float u = s.v + (qperm<kQpNext>(s.w) * R2 - R1 * qperm<kQpPrev>(s.w));
-    return quad_dot(octet_rel(u) * D0);
+    const float d = octet_rel(u);
+    return (qperm<kQpX>(d) * dir.x + qperm<kQpY>(d) * dir.y) + qperm<kQpZ>(d) * dir.z;
 }
-__device__ __forceinline__ void octet_apply(OctRow& s, bool act, float im, float sg, float D0, float D1, float D2,
+// (computed on every lane, kept where `on`: no branch in the row's chain)
+__device__ __forceinline__ void octet_apply(OctRow& s, bool on, float im, float sg, float D0, float D1, float D2,
                                             float R1, float R2, float M0, float M1, float M2, float delta) {
-    if (!act) return;
     const float J0 = D0 * delta, J1 = D1 * delta, J2 = D2 * delta;
     const float cr = R1 * J2 - J1 * R2;
     const float aw = (M0 * qperm<kQpX>(cr) + M1 * qperm<kQpY>(cr)) + M2 * qperm<kQpZ>(cr);
-    s.v = __builtin_fmaf(J0 * im, sg, s.v);
-    s.w = __builtin_fmaf(aw, sg, s.w);
+    const float nv = __builtin_fmaf(J0 * im, sg, s.v), nw = __builtin_fmaf(aw, sg, s.w);
+    s.v = on ? nv : s.v;
+    s.w = on ? nw : s.w;
 }
 // velocity_manifold (a velocity pass, or a relax pass when relax) of local solve position L on
 // the octet; lane = the lane's index in the octet
@@ -2357,32 +2363,37 @@ __device__ __forceinline__ void tl_run8(const TLds& t, float4* V, float4* W, con
         const float4 row = t.p1[q];
         float4 imp = t.imp[q];
         const float R1 = pp[c1] - X1, R2 = pp[c2] - X2;
-        if (row.x > kEps) {
+        // the three rows without branches: a row whose effective mass is not above kEps (the
+        // reference skips it) is computed and its results dropped
+        {
+            const bool on = row.x > kEps;
             const float target = relax ? row.w : pp[3];
-            const float vn = octet_vrel(s, N0, R1, R2);
+            const float vn = octet_vrel(s, n, R1, R2);
             const float lambda = -(vn - target) / row.x;
             const float ni = gmax(imp.x + lambda, 0.0f);
             const float delta = ni - imp.x;
-            imp.x = ni;
-            octet_apply(s, act, im, sg, N0, N1, N2, R1, R2, M0, M1, M2, delta);
+            imp.x = on ? ni : imp.x;
+            octet_apply(s, act && on, im, sg, N0, N1, N2, R1, R2, M0, M1, M2, delta);
         }
-        if (!(row.y <= kEps)) {
-            const float vt = octet_vrel(s, A0, R1, R2);
+        {
+            const bool on = !(row.y <= kEps);
+            const float vt = octet_vrel(s, t1, R1, R2);
             const float lt = -vt / row.y;
             const float maxf = nf.w * imp.x;
             const float nt = sclamp(imp.y + lt, -maxf, maxf);
             const float delta = nt - imp.y;
-            imp.y = nt;
-            octet_apply(s, act, im, sg, A0, A1, A2, R1, R2, M0, M1, M2, delta);
+            imp.y = on ? nt : imp.y;
+            octet_apply(s, act && on, im, sg, A0, A1, A2, R1, R2, M0, M1, M2, delta);
         }
-        if (!(row.z <= kEps)) {
-            const float vt = octet_vrel(s, B0, R1, R2);
+        {
+            const bool on = !(row.z <= kEps);
+            const float vt = octet_vrel(s, t2, R1, R2);
             const float lt = -vt / row.z;
             const float maxf = nf.w * imp.x;
             const float nt = sclamp(imp.z + lt, -maxf, maxf);
             const float delta = nt - imp.z;
-            imp.z = nt;
-            octet_apply(s, act, im, sg, B0, B1, B2, R1, R2, M0, M1, M2, delta);
+            imp.z = on ? nt : imp.z;
+            octet_apply(s, act && on, im, sg, B0, B1, B2, R1, R2, M0, M1, M2, delta);
         }
         if (lane == 0u) t.imp[q] = imp;
     }
