@@ -5,7 +5,10 @@ manifolds crossing tiles in two later phases; the order is colour-major and ever
 valid colour of the whole world, so the oracle's sequential solve of it must give the GPU's
 state bit for bit. Covered: fixed iterations and residual mode on piles of 2,048 and 16,384 bodies
 (tiles of 64 and 512 bodies), a small pile where most manifolds cross tiles (the later phases
-carry the solve), and the global-memory form of the same kernel (PXG_TILED_LDS=0)."""
+carry the solve), and the global-memory form of the same kernel (PXG_TILED_LDS=0). Stacks of
+16,384 boxes (four-point face contacts in nearly every manifold: colours with more multi-point
+manifolds than eight lanes each can hold) and a pile of spheres (one-point manifolds only) take
+the lane plans the mixed pile uses little."""
 import numpy as np
 import pytest
 
@@ -21,11 +24,43 @@ def sides(oracle):
     return World, oracle.OracleWorld
 
 
-def _pile_lockstep(sides, n, lattice, warm, steps, residual=False, every=1, cap=256):
+def _box_stacks(nx, ny, nz, seed=12):
+    """nx * ny columns of nz unit boxes (1 kg, friction 0.6), each turned by a small random yaw,
+    1.05 m apart, on a static ground."""
+    rng = np.random.default_rng(seed)
+    sb = scenes.SceneBuilder()
+    sb.static_box((0, 0, -0.5), (100, 100, 0.5), friction=0.6)
+    for ix in range(nx):
+        for iy in range(ny):
+            for iz in range(nz):
+                q = scenes.quat_axis_angle((0.0, 0.0, 1.0), float(rng.uniform(-0.05, 0.05)))
+                sb.dynamic_box(((ix - nx / 2) * 1.05, (iy - ny / 2) * 1.05, 0.5 + iz * 1.001), (0.5, 0.5, 0.5), q,
+                               mass=1.0, friction=0.6)
+    b, s = sb.arrays()
+    return scenes.Scene("box_stacks", b, s, dict(velocity_iterations=8, position_iterations=3))
+
+
+def _sphere_pile(nx, ny, nz, seed=13):
+    """nx * ny * nz spheres (radius 0.35-0.5 m, 1 kg) on a jittered 1 m lattice over a ground."""
+    rng = np.random.default_rng(seed)
+    sb = scenes.SceneBuilder()
+    sb.static_box((0, 0, -0.5), (100, 100, 0.5), friction=0.5)
+    for ix in range(nx):
+        for iy in range(ny):
+            for iz in range(nz):
+                pos = (ix - nx / 2 + float(rng.uniform(-0.05, 0.05)), iy - ny / 2 + float(rng.uniform(-0.05, 0.05)),
+                       1.0 + iz)
+                sb.dynamic_sphere(pos, float(rng.uniform(0.35, 0.5)), mass=1.0, friction=0.5)
+    b, s = sb.arrays()
+    return scenes.Scene("sphere_pile", b, s, dict(velocity_iterations=8, position_iterations=3))
+
+
+def _pile_lockstep(sides, n, lattice, warm, steps, residual=False, every=1, cap=256, sc=None):
     """`warm` fixed-iteration steps on the GPU in the tiled order, then `steps` steps in lockstep:
     each step the oracle gets the GPU's input state and the GPU's solve order."""
     World, OracleWorld = sides
-    sc = scenes.config2_scene(seed=2, n=n, lattice=lattice)
+    if sc is None:
+        sc = scenes.config2_scene(seed=2, n=n, lattice=lattice)
     c = _abi.default_config(**sc.config)
     c.solve_order = _abi.PXG_ORDER_TILED
     gw = World(c, sc.bodies, sc.shapes)
@@ -80,6 +115,17 @@ def test_tiled_order_residual_every_8th_pile_2048(sides):
 
 def test_tiled_order_residual_mode_pile_16384(sides):
     _pile_lockstep(sides, 16384, (64, 64, 40), 240, 2, residual=True, every=8, cap=512)
+
+
+def test_tiled_order_box_stacks_16384(sides):
+    gw, rep = _pile_lockstep(sides, 0, None, 20, 2, sc=_box_stacks(32, 32, 16))
+    gm = gw.manifolds()
+    assert (gm["count"] == 4).sum() > len(gm) // 2
+
+
+def test_tiled_order_sphere_pile_16384(sides):
+    gw, rep = _pile_lockstep(sides, 0, None, 60, 2, sc=_sphere_pile(32, 32, 16))
+    assert gw.manifolds()["count"].max() == 1
 
 
 def test_tiled_order_small_pile_crossing_tiles(sides):
