@@ -1632,8 +1632,10 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 struct GridBarrier {
     uint32_t* count;
     uint32_t nblocks;
-    uint32_t epoch;  // barriers passed (thread 0's copy)
+    uint32_t epoch;  // barriers passed (thread 0's copy; every thread's in the flag form)
     uint32_t near;
+    uint32_t* flags;  // flag form (near only): block r's epoch at flags[16 r]; null: counting form
+    uint32_t rank;
 };
 // Waits until this wave's global stores have completed (s_waitcnt vmcnt(0); the expcnt and
 // lgkmcnt fields of the GFX9 encoding at their maxima wait for nothing). The signal fences keep
@@ -1662,8 +1664,45 @@ __device__ __forceinline__ void barrier_add(GridBarrier& g) {
     if (g.near) __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Flag form of the near barrier: each block, once its stores have completed at the L2, stores
+// the number of barriers it has reached in its own 64-byte line, and wave 0 of every block reads
+// all blocks' lines (one device-scope load per lane) until none is behind. No atomic
+// read-modify-write, so the blocks' arrivals are not serialized on one line. The wait is bounded
+// as barrier_poll's.
+__device__ __forceinline__ bool flag_sync(GridBarrier& g, uint32_t* err) {
+    __shared__ uint32_t s_abort_f;
+    wait_stores();
+    __syncthreads();
+    g.epoch += 1;
+    if (threadIdx.x == 0) __hip_atomic_store(g.flags + 16u * g.rank, g.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 64u) {
+        const uint32_t lane = threadIdx.x;
+        uint32_t spins = 0, abort = 0;
+        while (true) {
+            const uint32_t v = lane < g.nblocks
+                                   ? __hip_atomic_load(g.flags + 16u * lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                   : g.epoch;
+            if (__ballot(v < g.epoch) == 0ull) break;
+            if (++spins > kSpinLimit) {
+                if (lane == 0u) atomicOr(err, ERR_BARRIER);
+                abort = 1u;
+                break;
+            }
+            if ((spins & 255u) == 0u &&
+                (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_BARRIER)) {
+                abort = 1u;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (lane == 0u) s_abort_f = abort;
+    }
+    __syncthreads();
+    return s_abort_f != 0u;
+}
 __device__ __forceinline__ bool grid_sync(GridBarrier& g, uint32_t* err) {
     __shared__ uint32_t s_abort;
+    if (g.flags) return flag_sync(g, err);
     if (g.near) wait_stores();
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1699,6 +1738,8 @@ __device__ __forceinline__ GridBarrier grid_barrier_init(uint32_t* bar) {
     g.nblocks = gridDim.x;
     g.epoch = 0;
     g.near = 0;
+    g.flags = nullptr;
+    g.rank = 0;
     return g;
 }
 
@@ -2485,6 +2526,10 @@ __global__ void __launch_bounds__(kTileBlock) k_solve_tiled(DevState d, StepPara
     if (cfg.one_xcd) {
         if (!near_blocks(bar, err, rank, g.nblocks)) return;
         g.near = 1;
+        if (g.nblocks <= kBarFlagBlocks) {
+            g.flags = bar + kBarFlags;
+            g.rank = rank;
+        }
     }
     const uint32_t nblk = g.nblocks, tid = threadIdx.x, nth = kTileBlock;
     const uint32_t gtid = rank * nth + tid, gnth = nblk * nth;

@@ -15,7 +15,10 @@ namespace pxg {
 struct MlpDev;
 constexpr int kBlock = 256;
 // grid barrier words: [0] arrivals, [1] set-up arrivals, [2 + x] blocks on XCD x (near barriers)
-constexpr uint32_t kBarWords = 18;
+// grid barrier words: the counting barrier (0), the XCD pick (1-17), and from kBarFlags one 64-byte
+// line per block of a one-XCD launch for the flag form of the barrier (at most kBarFlagBlocks)
+constexpr uint32_t kBarFlags = 32, kBarFlagBlocks = 64;
+constexpr uint32_t kBarWords = kBarFlags + 16 * kBarFlagBlocks;
 // XCDs of an MI355X: a one-XCD solve launches this many blocks per block that carries on
 constexpr uint32_t kXcds = 8;
 constexpr int kPersistBlock = 512;  // persistent cooperative kernels: 8 waves per block (up to 256 VGPRs each)
