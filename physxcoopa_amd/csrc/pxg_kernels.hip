@@ -1614,7 +1614,7 @@ __global__ void k_update_inertia(DevState d) {
 // between colours. Every wait is bounded: a barrier that does not complete within the spin
 // budget raises ERR_BARRIER and releases all waiters, so the kernel always terminates.
 // ------------------------------------------------------------------------------------------
-constexpr uint32_t ERR_BARRIER = 64u;
+constexpr uint32_t ERR_BARRIER = ERR_BARRIER_BIT;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
 // Counting barrier: bar[0] is zeroed before each launch and counts arrivals; barrier k of the

@@ -381,6 +381,7 @@ void launch_tile_slots(const DevState& d, uint32_t n_bound, hipStream_t st);
 // the tiled order, each tile's colours in one workgroup (its bodies and impulses in LDS)
 hipError_t launch_solve_tiled(const DevState& d, const StepParams& p, const uint32_t cfgv[8], float tol, uint32_t nblocks,
                               hipStream_t st);
-constexpr uint32_t ERR_BARRIER_BIT = 64u;
+// a grid barrier's bounded wait expired (its own bit: the others above report capacity limits)
+constexpr uint32_t ERR_BARRIER_BIT = 128u;
 
 }  // namespace pxg

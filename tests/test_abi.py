@@ -99,3 +99,14 @@ def test_cpp_facade_header_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
                         "-I", os.path.join(ROOT, "include"), str(src)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     assert r.returncode == 0, r.stdout.decode()
+
+
+def test_device_error_bits_are_distinct():
+    """The device's error word (Counters::err) packs one bit per condition: the capacity limits
+    and the grid barrier's expired wait. A shared bit would make a capacity error abort every
+    later barrier wait (the barriers poll the word for the timeout bit) and report both."""
+    text = open(os.path.join(ROOT, "physxcoopa_amd", "csrc", "pxg_state.h")).read()
+    bits = {name: int(val) for name, val in re.findall(r"\b(ERR_[A-Z_]+)\s*=\s*(\d+)u", text)}
+    assert {"ERR_PAIRS", "ERR_MESH_VERTICES", "ERR_BARRIER_BIT"} <= set(bits)
+    assert all(v and v & (v - 1) == 0 for v in bits.values())
+    assert len(set(bits.values())) == len(bits)
