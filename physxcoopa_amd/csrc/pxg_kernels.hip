@@ -2322,7 +2322,10 @@ __device__ __forceinline__ void tl_op2(int op, const TLds& t, float4* V, float4*
 //   (vb - va)_c    from the other body's quad                (row_shl / row_shr by 4)
 //   dot(d, n)      = (d_x n_x + d_y n_y) + d_z n_z            (d's components from the quad)
 //   mul(I, c)_c    = (I_c0 c_x + I_c1 c_y) + I_c2 c_z
-// so every lane of the octet computes the same bits as velocity_point.
+// so every lane of the octet computes the same bits as velocity_point. The quad permutes are
+// issued where all four lanes of each quad are active (a quad is one body: its lanes share every
+// branch), so bound_ctrl (a disabled source reads 0) never applies; it lets the compiler fold
+// each permute into the instruction that uses it.
 template <int CTRL>
 __device__ __forceinline__ float qperm(float x) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, true));
