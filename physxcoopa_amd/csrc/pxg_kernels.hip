@@ -1669,12 +1669,14 @@ __device__ __forceinline__ void barrier_add(GridBarrier& g) {
 // all blocks' lines (one device-scope load per lane) until none is behind. No atomic
 // read-modify-write, so the blocks' arrivals are not serialized on one line. The wait is bounded
 // as barrier_poll's.
-__device__ __forceinline__ bool flag_sync(GridBarrier& g, uint32_t* err) {
-    __shared__ uint32_t s_abort_f;
+__device__ __forceinline__ void flag_arrive(GridBarrier& g) {
     wait_stores();
     __syncthreads();
     g.epoch += 1;
     if (threadIdx.x == 0) __hip_atomic_store(g.flags + 16u * g.rank, g.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool flag_wait(GridBarrier& g, uint32_t* err) {
+    __shared__ uint32_t s_abort_f;
     if (threadIdx.x < 64u) {
         const uint32_t lane = threadIdx.x;
         uint32_t spins = 0, abort = 0;
@@ -1700,6 +1702,10 @@ __device__ __forceinline__ bool flag_sync(GridBarrier& g, uint32_t* err) {
     __syncthreads();
     return s_abort_f != 0u;
 }
+__device__ __forceinline__ bool flag_sync(GridBarrier& g, uint32_t* err) {
+    flag_arrive(g);
+    return flag_wait(g, err);
+}
 __device__ __forceinline__ bool grid_sync(GridBarrier& g, uint32_t* err) {
     __shared__ uint32_t s_abort;
     if (g.flags) return flag_sync(g, err);
@@ -1718,11 +1724,16 @@ __device__ __forceinline__ bool grid_sync(GridBarrier& g, uint32_t* err) {
 // grid_wait blocks until every block has arrived. Work between the two (loads of data no other
 // block writes in this phase) overlaps the barrier's latency.
 __device__ __forceinline__ void grid_arrive(GridBarrier& g) {
+    if (g.flags) {
+        flag_arrive(g);
+        return;
+    }
     if (g.near) wait_stores();
     __syncthreads();
     if (threadIdx.x == 0) barrier_add(g);
 }
 __device__ __forceinline__ bool grid_wait(GridBarrier& g, uint32_t* err) {
+    if (g.flags) return flag_wait(g, err);
     __shared__ uint32_t s_abort_w;
     if (threadIdx.x == 0) {
         const bool abort = barrier_poll(g, err);
@@ -1892,6 +1903,10 @@ __global__ void __launch_bounds__(kPersistBlock) k_solve_persistent(DevState d, 
     if (cfg.one_xcd) {
         if (!near_blocks(bar, err, rank, g.nblocks)) return;
         g.near = 1;
+        if (g.nblocks <= kBarFlagBlocks) {
+            g.flags = bar + kBarFlags;
+            g.rank = rank;
+        }
     }
     const uint32_t C = d.cnt->colours;
     const uint32_t* co = d.cnt->colour_off;
